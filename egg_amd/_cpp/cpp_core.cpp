@@ -1,0 +1,422 @@
+// pybind11 module egg_amd._cpp.cpp_core: the reference's egg._cpp.cpp_core
+// Python surface (reference src/bindings.cpp:1541-2130, stub
+// egg/_cpp/cpp_core.pyi:27-68) over the eggmi C ABI (include/eggmi.h).
+// Dict unpacking mirrors unpack_context (src/bindings.cpp:334-490); errors
+// map EGGMI_INVALID_ARGUMENT -> ValueError, everything else -> RuntimeError.
+#include "eggmi.h"
+
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <algorithm>
+#include <array>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace py = pybind11;
+
+namespace
+{
+
+using IArr = py::array_t<int32_t, py::array::c_style | py::array::forcecast>;
+using DArr = py::array_t<double, py::array::c_style | py::array::forcecast>;
+
+void check(eggmi_status st)
+{
+    if (st == EGGMI_OK) return;
+    const std::string msg = eggmi_last_error();
+    if (st == EGGMI_INVALID_ARGUMENT) throw py::value_error(msg);
+    throw std::runtime_error(msg);
+}
+
+int phase_id(const std::string& phase)
+{
+    if (phase == "untangle") return EGGMI_PHASE_UNTANGLE;
+    if (phase == "shape_size") return EGGMI_PHASE_SHAPE_SIZE;
+    return EGGMI_PHASE_BARRIER;  // "barrier" and any other name select the shape barrier
+}
+
+int device_id(const std::string& device)
+{
+    if (device == "auto") return 0;
+    if (device == "gpu") return 1;
+    if (device == "cpu") return 2;
+    if (device == "tables") return 3;  // host-side index maps only (inspection)
+    throw py::value_error("device must be 'auto', 'cpu', 'gpu' or 'tables'; got '" + device + "'");
+}
+
+int require_dim(int d)
+{
+    if (d != 2 && d != 3) throw py::value_error("dim must be 2 or 3; got " + std::to_string(d));
+    return d;
+}
+
+size_t checked_num_nodes(const DArr& X, int d)
+{
+    if (X.ndim() != 1)
+        throw py::value_error("X must be a flat 1-D array of length d*N; got ndim=" + std::to_string(X.ndim()) +
+                              " (did you pass an (N, d) array? ravel() it first).");
+    if (X.shape(0) % d != 0)
+        throw py::value_error("X length must be a multiple of d (" + std::to_string(d) +
+                              " coordinates per node); got " + std::to_string(X.shape(0)) + ".");
+    return (size_t)X.shape(0) / d;
+}
+
+IArr get_i(const py::dict& d, const char* key)
+{
+    if (!d.contains(key)) throw py::value_error(std::string("missing wire field '") + key + "'");
+    return d[key].cast<IArr>();
+}
+DArr get_d(const py::dict& d, const char* key)
+{
+    if (!d.contains(key)) throw py::value_error(std::string("missing wire field '") + key + "'");
+    return d[key].cast<DArr>();
+}
+
+// Keeps the converted arrays alive while the C structs point into them.
+struct Wire {
+    int dim = 2;
+    std::vector<IArr> iarrs;
+    std::vector<DArr> darrs;
+    std::vector<eggmi_group> groups;
+    eggmi_stencil es {};
+    eggmi_structured st {};
+    std::vector<IArr> bgd_arrs;
+    std::vector<const int32_t*> bgd_ptrs;
+    std::vector<int32_t> shapes;
+    IArr hsb, hsp, hdb, hdp, sb, sl;
+
+    const int32_t* keep(IArr a)
+    {
+        iarrs.push_back(a);
+        return iarrs.back().data();
+    }
+    const double* keep(DArr a)
+    {
+        darrs.push_back(a);
+        return darrs.back().data();
+    }
+
+    Wire(const py::dict& ctx, const py::dict& structured, int d) : dim(d)
+    {
+        const int dd = d * d;
+        py::list gl = ctx["groups"].cast<py::list>();
+        for (auto item : gl) {
+            py::dict g = item.cast<py::dict>();
+            eggmi_group eg {};
+            eg.ndof = g["D"].cast<int64_t>();
+            IArr dof = get_i(g, "dof_idx"), P = get_i(g, "P_of"), role = get_i(g, "role"), gc = get_i(g, "gc");
+            if ((int64_t)dof.size() != eg.ndof)
+                throw py::value_error("validate_context: group dof_idx size " + std::to_string(dof.size()) +
+                                      " != " + std::to_string(eg.ndof));
+            if ((int64_t)P.size() != eg.ndof)
+                throw py::value_error("validate_context: group P_of size " + std::to_string(P.size()) +
+                                      " != " + std::to_string(eg.ndof));
+            eg.dof_idx = keep(dof);
+            eg.P_of = keep(P);
+            eg.total_samples = (int64_t)gc.size();
+            if ((int64_t)role.size() != eg.total_samples)
+                throw py::value_error("validate_context: group role size " + std::to_string(role.size()) +
+                                      " != " + std::to_string(eg.total_samples));
+            eg.role = keep(role);
+            eg.gc = keep(gc);
+            for (int k = 0; k < d; ++k) {
+                const std::string gk = "gn" + std::to_string(k), sk = "s" + std::to_string(k);
+                IArr gn = get_i(g, gk.c_str());
+                DArr s = get_d(g, sk.c_str());
+                if ((int64_t)gn.size() != eg.total_samples || (int64_t)s.size() != eg.total_samples)
+                    throw py::value_error("validate_context: group " + gk + "/" + sk + " size != " +
+                                          std::to_string(eg.total_samples));
+                eg.gn[k] = keep(gn);
+                eg.s[k] = keep(s);
+            }
+            DArr W = get_d(g, "W_inv");
+            if (W.size() % dd != 0) throw py::value_error("validate_context: group W_inv size not a multiple of d*d");
+            eg.w_rows = (int64_t)W.size() / dd;
+            eg.W_inv = keep(W);
+            if (g.contains("interior_block")) {
+                IArr ib = get_i(g, "interior_block"), il = get_i(g, "interior_logical");
+                if ((int64_t)ib.size() != eg.ndof || (int64_t)il.size() != eg.ndof * d)
+                    throw py::value_error("validate_context: interior_block/interior_logical size mismatch");
+                eg.interior_block = keep(ib);
+                eg.interior_logical = keep(il);
+            } else {
+                std::vector<int32_t> neg(eg.ndof, -1), zero(eg.ndof * d, 0);
+                eg.interior_block = keep(IArr((py::ssize_t)neg.size(), neg.data()));
+                eg.interior_logical = keep(IArr((py::ssize_t)zero.size(), zero.data()));
+            }
+            if (g.contains("weight") || g.contains("curv_nodes") || g.contains("dir_nodes"))
+                throw py::value_error("wire: per-sample weights and curvature / directional terms are not "
+                                      "supported by this compute core");
+            groups.push_back(eg);
+        }
+        py::dict e = ctx["energy_stencil"].cast<py::dict>();
+        es.num_samples = e["num_samples"].cast<int64_t>();
+        IArr egc = get_i(e, "gc");
+        if ((int64_t)egc.size() != es.num_samples)
+            throw py::value_error("validate_context: energy_stencil gc size " + std::to_string(egc.size()) +
+                                  " != " + std::to_string(es.num_samples));
+        es.gc = keep(egc);
+        for (int k = 0; k < d; ++k) {
+            const std::string gk = "gn" + std::to_string(k), sk = "s" + std::to_string(k);
+            IArr gn = get_i(e, gk.c_str());
+            DArr s = get_d(e, sk.c_str());
+            if ((int64_t)gn.size() != es.num_samples || (int64_t)s.size() != es.num_samples)
+                throw py::value_error("validate_context: energy_stencil " + gk + " size != " +
+                                      std::to_string(es.num_samples));
+            es.gn[k] = keep(gn);
+            es.s[k] = keep(s);
+        }
+        DArr eW = get_d(e, "W_inv");
+        if (eW.size() % dd != 0) throw py::value_error("validate_context: energy_stencil W_inv size");
+        es.w_rows = (int64_t)eW.size() / dd;
+        es.W_inv = keep(eW);
+        if (e.contains("weight") || e.contains("dir_nodes"))
+            throw py::value_error("wire: energy-stencil weights / directional terms are not supported");
+
+        // structured tables
+        py::list shl = structured["interior_shapes"].cast<py::list>();
+        py::list bgl = structured["block_global_dof"].cast<py::list>();
+        if (py::len(bgl) != py::len(shl))
+            throw py::value_error("structured: block_global_dof count != interior_shapes count");
+        for (auto sh : shl) {
+            IArr a = sh.cast<IArr>();
+            if ((int)a.size() != d) throw py::value_error("structured: each interior shape must have D entries");
+            for (int k = 0; k < d; ++k) shapes.push_back(a.data()[k]);
+        }
+        size_t b = 0;
+        for (auto item : bgl) {
+            IArr a = item.cast<IArr>();
+            int64_t n = 1;
+            for (int k = 0; k < d; ++k) n *= shapes[b * d + k];
+            if ((int64_t)a.size() != n) throw py::value_error("structured: block_global_dof length != prod(shape)");
+            bgd_arrs.push_back(a);
+            ++b;
+        }
+        for (auto& a : bgd_arrs) bgd_ptrs.push_back(a.data());
+        hsb = get_i(structured, "halo_src_block");
+        hsp = get_i(structured, "halo_src_padded");
+        hdb = get_i(structured, "halo_dst_block");
+        hdp = get_i(structured, "halo_dst_padded");
+        sb = get_i(structured, "sing_block");
+        sl = get_i(structured, "sing_logical");
+        const int64_t nh = (int64_t)hsb.size();
+        if ((int64_t)hdb.size() != nh || (int64_t)hsp.size() != nh * d || (int64_t)hdp.size() != nh * d)
+            throw py::value_error("structured: halo table sizes disagree");
+        if ((int64_t)sl.size() != (int64_t)sb.size() * d) throw py::value_error("structured: sing table sizes disagree");
+        st.num_blocks = (int32_t)bgd_arrs.size();
+        st.interior_shapes = shapes.data();
+        st.block_global_dof = bgd_ptrs.data();
+        st.n_halo = nh;
+        st.halo_src_block = hsb.data();
+        st.halo_src_padded = hsp.data();
+        st.halo_dst_block = hdb.data();
+        st.halo_dst_padded = hdp.data();
+        st.n_sing = (int64_t)sb.size();
+        st.sing_block = sb.data();
+        st.sing_logical = sl.data();
+    }
+};
+
+py::array_t<double> to_array(const std::vector<double>& v)
+{
+    return py::array_t<double>((py::ssize_t)v.size(), v.data());
+}
+
+class Session
+{
+  public:
+    Session(const py::dict& ctx, const py::dict& structured, const DArr& X, const std::string& device, int dim,
+            const py::object& control) :
+        dim_(require_dim(dim))
+    {
+        if (!control.is_none())
+            throw py::value_error("control-net smoothing (run_control) is not provided by this compute core");
+        n_ = checked_num_nodes(X, dim_);
+        Wire w(ctx, structured, dim_);
+        check(eggmi_session_create(dim_, w.groups.data(), (int)w.groups.size(), &w.es, &w.st, X.data(),
+                                   (int64_t)n_, device_id(device), &s_));
+    }
+    ~Session()
+    {
+        if (s_) eggmi_session_destroy(s_);
+    }
+    Session(const Session&) = delete;
+    Session& operator=(const Session&) = delete;
+
+    std::tuple<py::array_t<double>, py::array_t<double>> run(int n_sweeps, const std::string& phase, double delta,
+                                                             double omega, int report_every)
+    {
+        const int k = report_every <= 0 ? n_sweeps : report_every;
+        const int64_t cap = n_sweeps <= 0 ? 1 : (n_sweeps + k - 1) / k;
+        std::vector<double> e(cap), m(cap);
+        int64_t n = 0;
+        check(eggmi_run(s_, n_sweeps, phase_id(phase), delta, omega, report_every, e.data(), m.data(), cap, &n));
+        e.resize(n);
+        m.resize(n);
+        return {to_array(e), to_array(m)};
+    }
+
+    std::tuple<py::array_t<double>, py::array_t<double>> run_fas(int n_cycles, int nu_pre, int nu_post,
+                                                                 int nu_coarse, double omega, int max_levels,
+                                                                 const std::string& phase)
+    {
+        if (phase == "untangle")
+            throw py::value_error("run_fas: FAS smooths the barrier phases only; use run(phase=\"untangle\") to "
+                                  "untangle first");
+        const int64_t cap = std::max(1, n_cycles) + 1;
+        std::vector<double> e(cap), m(cap);
+        int64_t n = 0;
+        check(eggmi_run_fas(s_, n_cycles, nu_pre, nu_post, nu_coarse, omega, max_levels, phase_id(phase), e.data(),
+                            m.data(), cap, &n));
+        e.resize(n);
+        m.resize(n);
+        return {to_array(e), to_array(m)};
+    }
+
+    py::list mg_levels(int max_levels)
+    {
+        int32_t nl = 0, nb = 0;
+        check(eggmi_mg_levels(s_, max_levels, nullptr, 0, &nl, &nb));
+        std::vector<int32_t> shp((size_t)nl * nb * dim_ + 1);
+        check(eggmi_mg_levels(s_, max_levels, shp.data(), (int64_t)shp.size(), &nl, &nb));
+        py::list levels;
+        for (int l = 0; l < nl; ++l) {
+            py::list blocks;
+            for (int b = 0; b < nb; ++b) {
+                py::tuple t(dim_);
+                for (int k = 0; k < dim_; ++k) t[k] = shp[((size_t)l * nb + b) * dim_ + k];
+                blocks.append(t);
+            }
+            levels.append(blocks);
+        }
+        return levels;
+    }
+
+    py::array_t<double> get_X()
+    {
+        py::array_t<double> out((py::ssize_t)(n_ * dim_));
+        check(eggmi_get_X(s_, out.mutable_data(), (int64_t)(n_ * dim_)));
+        return out;
+    }
+
+    void set_X(const DArr& X) { check(eggmi_set_X(s_, X.data(), (int64_t)X.size())); }
+
+    py::dict info()
+    {
+        eggmi_info inf {};
+        check(eggmi_session_info(s_, &inf));
+        py::dict d;
+        d["n_slots"] = inf.n_slots;
+        d["n_overflow"] = inf.n_overflow;
+        d["n_halo"] = inf.n_halo;
+        d["n_fan"] = inf.n_fan;
+        d["n_share"] = inf.n_share;
+        d["n_tiles"] = inf.n_tiles;
+        d["n_stored"] = inf.n_stored;
+        d["n_synth"] = inf.n_synth;
+        d["structured_energy"] = inf.structured_energy;
+        return d;
+    }
+
+    py::dict tables()
+    {
+        eggmi_info inf {};
+        check(eggmi_session_info(s_, &inf));
+        const int64_t nc = inf.n_halo + inf.n_fan + inf.n_share;
+        py::array_t<int32_t> src(nc), dst(nc), g2s((py::ssize_t)n_);
+        int32_t nl = 0, nb = 0;
+        check(eggmi_mg_levels(s_, 1, nullptr, 0, &nl, &nb));
+        py::array_t<int64_t> base(nb);
+        check(eggmi_session_tables(s_, src.mutable_data(), dst.mutable_data(), g2s.mutable_data(),
+                                   base.mutable_data()));
+        py::dict d;
+        d["copy_src"] = src;
+        d["copy_dst"] = dst;
+        d["g2s"] = g2s;
+        d["block_base"] = base;
+        return d;
+    }
+
+    py::array_t<float> packed()
+    {
+        eggmi_info inf {};
+        check(eggmi_session_info(s_, &inf));
+        py::array_t<float> out(inf.n_slots * dim_);
+        check(eggmi_get_packed(s_, out.mutable_data(), inf.n_slots * dim_));
+        return out.attr("reshape")(dim_, inf.n_slots);
+    }
+
+    void halo_exchange() { check(eggmi_halo_exchange(s_)); }
+
+    int set_tile_variant(int v)
+    {
+        int nv = 0;
+        check(eggmi_set_tile_variant(s_, v, &nv));
+        return nv;
+    }
+
+    py::tuple last_timing()
+    {
+        double ms = 0, kms = 0;
+        int64_t n = 0;
+        check(eggmi_last_timing(s_, &ms, &n, &kms));
+        return py::make_tuple(ms, n, kms);
+    }
+
+    void set_kernel_timing(bool on) { check(eggmi_set_kernel_timing(s_, on ? 1 : 0)); }
+
+  private:
+    int dim_;
+    size_t n_ = 0;
+    eggmi_session* s_ = nullptr;
+};
+
+}  // namespace
+
+PYBIND11_MODULE(cpp_core, m)
+{
+    m.doc() = "egg compute core for AMD Instinct MI355X (HIP, gfx950) over the eggmi C ABI.";
+    m.def("ping", [] { return eggmi_ping(); }, "Liveness check; returns 0.");
+    // FP32 bulk arithmetic (FP64 reductions); tolerance floors follow the
+    // reference's float build.
+    m.attr("REAL_IS_FLOAT") = true;
+
+    py::class_<Session>(m, "CppStructuredSweepSession",
+                        "Persistent device-resident structured smoothing session on the MI355X.")
+      .def(py::init<const py::dict&, const py::dict&, const DArr&, const std::string&, int, const py::object&>(),
+           py::arg("ctx_arrays"), py::arg("structured"), py::arg("X"), py::kw_only(), py::arg("device") = "auto",
+           py::arg("dim") = 2, py::arg("control") = py::none())
+      .def("run", &Session::run, py::arg("n_sweeps"), py::kw_only(), py::arg("phase") = "barrier",
+           py::arg("delta") = 0.0, py::arg("omega") = 1.0, py::arg("report_every") = 0,
+           "Run n_sweeps of block-Jacobi on the resident store; returns (energies, mindets).")
+      .def("run_fas", &Session::run_fas, py::arg("n_cycles"), py::kw_only(), py::arg("nu_pre") = 2,
+           py::arg("nu_post") = 2, py::arg("nu_coarse") = 32, py::arg("omega") = 1.0, py::arg("max_levels") = 32,
+           py::arg("phase") = "barrier", "Run n_cycles FAS V-cycles; returns per-cycle (energies, mindets).")
+      .def("mg_levels", &Session::mg_levels, py::arg("max_levels") = 32,
+           "Coarse-hierarchy shape: one list per coarse level of per-block interior node-count tuples.")
+      .def("get_X", &Session::get_X, "Download X gathered back to global node order, shape (N*dim,).")
+      .def("set_X", &Session::set_X, py::arg("X"), "Upload X (global order, flat) into the resident store.")
+      .def("info", &Session::info)
+      .def("tables", &Session::tables)
+      .def("packed", &Session::packed)
+      .def("halo_exchange", &Session::halo_exchange)
+      .def("set_tile_variant", &Session::set_tile_variant, py::arg("variant"))
+      .def("last_timing", &Session::last_timing)
+      .def("set_kernel_timing", &Session::set_kernel_timing, py::arg("on"));
+
+    m.def(
+      "cpp_structured_sweep",
+      [](const py::dict& ctx, const py::dict& structured, const DArr& X, int n_sweeps, const std::string& device,
+         const std::string& phase, double delta, int dim, double omega, int report_every) {
+          Session s(ctx, structured, X, device, dim, py::none());
+          auto [e, m] = s.run(n_sweeps, phase, delta, omega, report_every);
+          return py::make_tuple(s.get_X(), e, m);
+      },
+      py::arg("ctx_arrays"), py::arg("structured"), py::arg("X"), py::arg("n_sweeps"), py::kw_only(),
+      py::arg("device") = "auto", py::arg("phase") = "barrier", py::arg("delta") = 0.0, py::arg("dim") = 2,
+      py::arg("omega") = 1.0, py::arg("report_every") = 0,
+      "Run n_sweeps of block-Jacobi over a halo-padded structured store; returns (X_out, energies, mindets).");
+}

@@ -1,0 +1,250 @@
+// C ABI of the MI355X structured smoother (include/eggmi.h). Converts
+// exceptions to status codes and keeps the calling thread's last message.
+#include "eggmi.h"
+#include "session.hpp"
+
+#include <cstring>
+#include <new>
+#include <string>
+
+struct eggmi_session {
+    eggmi::Session s;
+};
+
+namespace
+{
+thread_local std::string g_last_error;
+
+template <class F> eggmi_status guarded(F&& f)
+{
+    try {
+        f();
+        g_last_error.clear();
+        return EGGMI_OK;
+    } catch (const eggmi::InvalidArgument& e) {
+        g_last_error = e.what();
+        return EGGMI_INVALID_ARGUMENT;
+    } catch (const eggmi::NoDevice& e) {
+        g_last_error = e.what();
+        return EGGMI_NO_DEVICE;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "out of host memory";
+        return EGGMI_INTERNAL;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return EGGMI_INTERNAL;
+    }
+}
+
+void need(const void* p, const char* what)
+{
+    if (!p) throw eggmi::InvalidArgument(std::string(what) + " must not be null");
+}
+
+int phase_kind(int phase)
+{
+    if (phase < 0 || phase > 2) throw eggmi::InvalidArgument("unknown phase " + std::to_string(phase));
+    return phase;  // eggmi_phase values equal the kernel Kind values
+}
+}  // namespace
+
+extern "C" {
+
+int eggmi_ping(void) { return 0; }
+
+const char* eggmi_last_error(void) { return g_last_error.c_str(); }
+
+eggmi_status eggmi_session_create(int dim, const eggmi_group* groups, int n_groups, const eggmi_stencil* stencil,
+                                  const eggmi_structured* structured, const double* X, int64_t num_nodes,
+                                  int device, eggmi_session** out)
+{
+    return guarded([&] {
+        need(out, "out");
+        *out = nullptr;
+        need(stencil, "stencil");
+        need(structured, "structured");
+        need(X, "X");
+        if (n_groups < 0) throw eggmi::InvalidArgument("n_groups must be >= 0");
+        if (n_groups > 0) need(groups, "groups");
+        auto* s = new eggmi_session;
+        try {
+            s->s.create(dim, groups, n_groups, stencil, structured, X, num_nodes, device);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
+
+eggmi_status eggmi_session_destroy(eggmi_session* s)
+{
+    return guarded([&] { delete s; });
+}
+
+eggmi_status eggmi_run(eggmi_session* s, int n_sweeps, int phase, double delta, double omega, int report_every,
+                       double* energies, double* mindets, int64_t capacity, int64_t* n_reports)
+{
+    return guarded([&] {
+        need(s, "session");
+        need(n_reports, "n_reports");
+        const int kind = phase_kind(phase);
+        const int k = report_every <= 0 ? n_sweeps : report_every;
+        const int64_t want = n_sweeps <= 0 ? 0 : (n_sweeps + k - 1) / k;
+        if (want > capacity) throw eggmi::InvalidArgument("report capacity too small");
+        auto [e, m] = s->s.run_jacobi(n_sweeps, kind, delta, omega, report_every);
+        for (size_t i = 0; i < e.size(); ++i) {
+            energies[i] = e[i];
+            mindets[i] = m[i];
+        }
+        *n_reports = (int64_t)e.size();
+    });
+}
+
+eggmi_status eggmi_run_fas(eggmi_session* s, int n_cycles, int nu_pre, int nu_post, int nu_coarse, double omega,
+                           int max_levels, int phase, double* energies, double* mindets, int64_t capacity,
+                           int64_t* n_reports)
+{
+    return guarded([&] {
+        need(s, "session");
+        need(n_reports, "n_reports");
+        const int kind = phase_kind(phase);
+        if (kind == 1)
+            throw eggmi::InvalidArgument("run_fas: FAS smooths the barrier phases only; use run(phase=\"untangle\") "
+                                         "to untangle first");
+        auto [e, m] = s->s.run_fas(n_cycles, nu_pre, nu_post, nu_coarse, omega, max_levels, kind);
+        if ((int64_t)e.size() > capacity) throw eggmi::InvalidArgument("report capacity too small");
+        for (size_t i = 0; i < e.size(); ++i) {
+            energies[i] = e[i];
+            mindets[i] = m[i];
+        }
+        *n_reports = (int64_t)e.size();
+    });
+}
+
+eggmi_status eggmi_mg_levels(eggmi_session* s, int max_levels, int32_t* shapes_out, int64_t capacity,
+                             int32_t* n_levels, int32_t* n_blocks)
+{
+    return guarded([&] {
+        need(s, "session");
+        auto lv = s->s.mg_level_shapes(max_levels);
+        const int d = s->s.dim;
+        const int64_t need_n = (int64_t)lv.size() * (int64_t)s->s.fine.lay.shape.size() * d;
+        if (n_levels) *n_levels = (int32_t)lv.size();
+        if (n_blocks) *n_blocks = (int32_t)s->s.fine.lay.shape.size();
+        if (!shapes_out) return;
+        if (need_n > capacity) throw eggmi::InvalidArgument("shape capacity too small");
+        int64_t q = 0;
+        for (auto& level : lv)
+            for (auto& sh : level)
+                for (int k = 0; k < d; ++k) shapes_out[q++] = sh[k];
+    });
+}
+
+eggmi_status eggmi_get_X(eggmi_session* s, double* X_out, int64_t n)
+{
+    return guarded([&] {
+        need(s, "session");
+        need(X_out, "X_out");
+        if (n < s->s.M * s->s.dim) throw eggmi::InvalidArgument("X_out too small");
+        s->s.get_X(X_out);
+    });
+}
+
+eggmi_status eggmi_set_X(eggmi_session* s, const double* X, int64_t n)
+{
+    return guarded([&] {
+        need(s, "session");
+        need(X, "X");
+        if (n != s->s.M * s->s.dim) throw eggmi::InvalidArgument("X length must be d*N");
+        s->s.set_X(X);
+    });
+}
+
+eggmi_status eggmi_session_info(eggmi_session* s, eggmi_info* info)
+{
+    return guarded([&] {
+        need(s, "session");
+        need(info, "info");
+        const auto& L = s->s.fine;
+        info->n_slots = L.lay.n_slots();
+        info->n_overflow = L.lay.overflow;
+        info->n_halo = L.n_halo;
+        info->n_fan = L.n_fan;
+        info->n_share = L.n_share;
+        info->n_tiles = (int64_t)L.tiles.size();
+        info->n_stored = (int64_t)L.st_dof.size();
+        info->n_synth = L.n_synth;
+        info->structured_energy = L.structured_energy ? 1 : 0;
+    });
+}
+
+eggmi_status eggmi_session_tables(eggmi_session* s, int32_t* copy_src, int32_t* copy_dst, int32_t* g2s,
+                                  int64_t* block_base)
+{
+    return guarded([&] {
+        need(s, "session");
+        const auto& L = s->s.fine;
+        if (copy_src) std::memcpy(copy_src, L.copy_src.data(), L.copy_src.size() * sizeof(int32_t));
+        if (copy_dst) std::memcpy(copy_dst, L.copy_dst.data(), L.copy_dst.size() * sizeof(int32_t));
+        if (g2s) std::memcpy(g2s, s->s.g2s.data(), s->s.g2s.size() * sizeof(int32_t));
+        if (block_base)
+            for (size_t b = 0; b < L.lay.base.size(); ++b) block_base[b] = L.lay.base[b];
+    });
+}
+
+eggmi_status eggmi_get_packed(eggmi_session* s, float* out, int64_t n)
+{
+    return guarded([&] {
+        need(s, "session");
+        need(out, "out");
+        auto v = s->s.packed();
+        if ((int64_t)v.size() > n) throw eggmi::InvalidArgument("out too small");
+        std::memcpy(out, v.data(), v.size() * sizeof(float));
+    });
+}
+
+eggmi_status eggmi_halo_exchange(eggmi_session* s)
+{
+    return guarded([&] {
+        need(s, "session");
+        s->s.need_device();
+        auto& L = s->s.fine;
+        s->s.halo(L, L.x[L.cur].p);
+        EGGMI_HIP(hipStreamSynchronize(s->s.stream));
+    });
+}
+
+eggmi_status eggmi_set_tile_variant(eggmi_session* s, int variant, int* n_variants)
+{
+    return guarded([&] {
+        need(s, "session");
+        const int nv = eggmi::num_tile_variants(s->s.dim);
+        if (n_variants) *n_variants = nv;
+        if (variant < 0) return;
+        if (variant >= nv) throw eggmi::InvalidArgument("tile variant out of range");
+        s->s.tile_variant = variant;
+        s->s.fine.make_tiles(variant);
+        for (auto& c : s->s.coarse) c->make_tiles(variant);
+    });
+}
+
+eggmi_status eggmi_last_timing(eggmi_session* s, double* ms, int64_t* sweep_launches, double* sweep_kernel_ms)
+{
+    return guarded([&] {
+        need(s, "session");
+        if (ms) *ms = s->s.last_ms;
+        if (sweep_launches) *sweep_launches = s->s.last_launches;
+        if (sweep_kernel_ms) *sweep_kernel_ms = s->s.last_kernel_ms;
+    });
+}
+
+eggmi_status eggmi_set_kernel_timing(eggmi_session* s, int on)
+{
+    return guarded([&] {
+        need(s, "session");
+        s->s.kernel_timing = on != 0;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,103 @@
+// Device-side data descriptors and kernel launchers of the structured TMOP
+// smoother. Host code (session.cpp) fills these and calls the launchers; the
+// kernels live in sweep_kernels.hip.
+//
+// Store layout (HBM): the halo-padded per-block node slots of the reference
+// (src/structured.hpp:47-163 BlockLayout, row-major padded axes, blocks back
+// to back, overflow slots last) with the coordinate axis split into planes:
+// coordinate k of slot n is x[k * plane + n]. Node slot numbers are therefore
+// identical to the reference's real-offset / D, so every index table (halo,
+// share, fan, patch stencils, g2s) is the reference's table in node units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace eggmi
+{
+
+struct BlockDesc {
+    long long base;  // node slot of padded index (0, ..., 0)
+    int n[3];        // interior node counts per axis
+    int pad;
+    long long st[3];  // node strides of the padded axes (last axis = 1)
+};
+
+struct TileDesc {
+    int blk;
+    int i0, j0, k0;  // first logical index of the tile on each axis
+};
+
+// Stored-stencil DOFs (block faces, interfaces, singular corners): one
+// thread per DOF walks its occurrences [off[d], off[d+1]).
+struct StoredView {
+    int n = 0;
+    const int* dof = nullptr;  // [n] slot written
+    const int* off = nullptr;  // [n+1]
+    long long total = 0;       // occurrences
+    const int* gc = nullptr;   // [total] corner slot
+    const int* gn = nullptr;   // [dim][total] neighbour slots
+    const uint8_t* meta = nullptr;  // [total] bits 0-1: role+1, bits 2..4: sign of axis k (1 = -1)
+    const float* W = nullptr;       // [total][dim*dim] per-occurrence W_inv rows, or null (uniform)
+};
+
+struct SweepArgs {
+    const float* xin = nullptr;
+    float* xout = nullptr;
+    long long plane = 0;
+    const TileDesc* tiles = nullptr;
+    int n_tiles = 0;
+    const BlockDesc* blocks = nullptr;
+    const uint8_t* mask = nullptr;  // [slots] 1 = synthesized interior DOF
+    StoredView st;
+    float omega = 1.0f;
+    float delta = 0.0f;
+    const float* tau = nullptr;  // FAS linear shift (plane layout), null = none
+    float Wu[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};  // uniform W_inv for stored occurrences
+};
+
+// Energy / min-det reduction over the structured cell stencil.
+struct CellReduceArgs {
+    const float* x = nullptr;
+    long long plane = 0;
+    const BlockDesc* blocks = nullptr;
+    const long long* cell_off = nullptr;  // [nblocks+1] exclusive prefix of cells per block
+    int nblocks = 0;
+    long long ncells = 0;
+    double delta = 0.0;
+    const float* tau = nullptr;  // optional: subtract sum tau.x over owner slots (FAS)
+};
+
+// Energy / min-det reduction over an explicit sample table (fallback when the
+// wire's energy stencil is not the plain structured cell stencil).
+struct SampleReduceArgs {
+    const float* x = nullptr;
+    long long plane = 0;
+    long long ns = 0;
+    const int* gc = nullptr;
+    const int* gn = nullptr;  // [dim][ns]
+    const float* s = nullptr;  // [dim][ns]
+    const float* W = nullptr;  // [ns][dd] or null (uniform)
+    double Wu[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    double delta = 0.0;
+};
+
+constexpr int kReduceBlocks = 1024;
+constexpr int kReduceThreads = 256;
+
+// Launchers (stream-ordered, asynchronous). kind: 0 shape, 1 untangle, 2 shape_size.
+void launch_sweep(int dim, int kind, int tile_variant, const SweepArgs& a, hipStream_t s);
+void launch_copy(float* x, long long plane, int dim, const int* src, const int* dst, long long n, hipStream_t s);
+void launch_reduce_cells(int dim, int kind, const CellReduceArgs& a, double* partials, hipStream_t s);
+void launch_reduce_samples(int dim, int kind, const SampleReduceArgs& a, double* partials, hipStream_t s);
+void launch_reduce_finish(const double* partials, int nparts, double* out_e, double* out_m, hipStream_t s);
+
+// Tile shapes the sweep kernels are instantiated for (the autotuner picks one).
+struct TileShape {
+    int tx, ty, tz;
+};
+int num_tile_variants(int dim);
+TileShape tile_variant_shape(int dim, int v);
+
+}  // namespace eggmi

@@ -1,0 +1,732 @@
+// Host side of the MI355X structured smoother (see session.hpp).
+#include "session.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <unordered_map>
+#include <utility>
+
+namespace eggmi
+{
+
+void hip_check(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) throw Internal(std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
+}
+
+// ------------------------------------------------------------------ layout
+
+void Layout::build(int d, const std::vector<std::array<int, 3>>& shapes)
+{
+    dim = d;
+    shape = shapes;
+    base.assign(shapes.size(), 0);
+    stride.assign(shapes.size(), {0, 0, 0});
+    long long acc = 0;
+    for (size_t b = 0; b < shapes.size(); ++b) {
+        base[b] = acc;
+        long long st = 1;
+        for (int k = d - 1; k >= 0; --k) {
+            stride[b][k] = st;
+            st *= shapes[b][k] + 2;
+        }
+        acc += st;
+    }
+    padded_nodes = acc;
+    overflow = 0;
+}
+
+long long Layout::padded_slot(int b, const int* padded) const
+{
+    long long s = base[b];
+    for (int k = 0; k < dim; ++k) s += (long long)padded[k] * stride[b][k];
+    return s;
+}
+
+long long Layout::interior_slot(int b, const int* logical) const
+{
+    long long s = base[b];
+    for (int k = 0; k < dim; ++k) s += (long long)(logical[k] + 1) * stride[b][k];
+    return s;
+}
+
+long long Layout::block_padded_count(int b) const
+{
+    long long n = 1;
+    for (int k = 0; k < dim; ++k) n *= shape[b][k] + 2;
+    return n;
+}
+
+// ------------------------------------------------------------------ level
+
+void Level::make_tiles(int variant)
+{
+    const TileShape ts = tile_variant_shape(lay.dim, variant);
+    // axis 0 <- tz (3D) / ty (2D), last axis <- tx
+    const int step[3] = {lay.dim == 2 ? ts.ty : ts.tz, lay.dim == 2 ? ts.tx : ts.ty, ts.tx};
+    tiles.clear();
+    for (size_t b = 0; b < lay.shape.size(); ++b) {
+        const auto& n = lay.shape[b];
+        bool any = true;
+        for (int k = 0; k < lay.dim; ++k) any = any && n[k] >= 3;
+        if (!any) continue;
+        // only blocks that hold at least one synthesized DOF
+        bool has = false;
+        {
+            const long long s0 = lay.base[b], s1 = s0 + lay.block_padded_count((int)b);
+            for (long long s = s0; s < s1 && !has; ++s) has = mask[s] != 0;
+        }
+        if (!has) continue;
+        const int e0 = n[0] - 2, e1 = n[1] - 2, e2 = lay.dim == 3 ? n[2] - 2 : 1;
+        for (int i = 0; i < e0; i += step[0])
+            for (int j = 0; j < e1; j += step[1])
+                for (int k = 0; k < e2; k += (lay.dim == 3 ? step[2] : 1)) {
+                    TileDesc t;
+                    t.blk = (int)b;
+                    t.i0 = 1 + i;
+                    t.j0 = 1 + j;
+                    t.k0 = 1 + k;
+                    tiles.push_back(t);
+                }
+    }
+    d_tiles.upload(tiles);
+}
+
+void Level::upload_tables()
+{
+    std::vector<BlockDesc> bd(lay.shape.size());
+    for (size_t b = 0; b < bd.size(); ++b) {
+        bd[b].base = lay.base[b];
+        for (int k = 0; k < 3; ++k) {
+            bd[b].n[k] = k < lay.dim ? lay.shape[b][k] : 1;
+            bd[b].st[k] = k < lay.dim ? lay.stride[b][k] : 0;
+        }
+        bd[b].pad = 0;
+    }
+    blocks.upload(bd);
+    d_copy_src.upload(copy_src);
+    d_copy_dst.upload(copy_dst);
+    d_mask.upload(mask);
+    d_st_dof.upload(st_dof);
+    d_st_off.upload(st_off);
+    d_st_gc.upload(st_gc);
+    d_st_gn.upload(st_gn);
+    d_st_meta.upload(st_meta);
+    d_st_W.upload(st_W);
+    // structured energy cells
+    cell_off.assign(lay.shape.size() + 1, 0);
+    for (size_t b = 0; b < lay.shape.size(); ++b) {
+        long long c = 1;
+        for (int k = 0; k < lay.dim; ++k) c *= std::max(0, lay.shape[b][k] - 1);
+        cell_off[b + 1] = cell_off[b] + c;
+    }
+    ncells = cell_off.back();
+    d_cell_off.upload(cell_off);
+}
+
+SweepArgs Level::sweep_args(const float* xin, float* xout, float omega, float delta, const float* tau) const
+{
+    SweepArgs a;
+    a.xin = xin;
+    a.xout = xout;
+    a.plane = plane;
+    a.tiles = d_tiles.p;
+    a.n_tiles = (int)tiles.size();
+    a.blocks = blocks.p;
+    a.mask = d_mask.p;
+    a.st.n = (int)st_dof.size();
+    a.st.dof = d_st_dof.p;
+    a.st.off = d_st_off.p;
+    a.st.total = (long long)st_gc.size();
+    a.st.gc = d_st_gc.p;
+    a.st.gn = d_st_gn.p;
+    a.st.meta = d_st_meta.p;
+    a.st.W = st_W.empty() ? nullptr : d_st_W.p;
+    a.omega = omega;
+    a.delta = delta;
+    a.tau = tau;
+    for (int i = 0; i < 9; ++i) a.Wu[i] = Wu[i];
+    return a;
+}
+
+// ------------------------------------------------------------------ validation
+
+namespace
+{
+
+void check_index_range(const int32_t* v, long long n, long long N, const std::string& grp, const char* name)
+{
+    for (long long i = 0; i < n; ++i) {
+        if (v[i] < 0 || v[i] >= N) {
+            throw InvalidArgument("validate_context: " + grp + " " + name + " index " + std::to_string(v[i]) +
+                                  " out of range [0, " + std::to_string(N) + ")");
+        }
+    }
+}
+
+// Mirrors the checks of the reference's validate_context (src/bindings.cpp:98-253)
+// for the fields this path consumes.
+void validate(int dim, const eggmi_group* groups, int n_groups, const eggmi_stencil* es, long long N)
+{
+    const int dd = dim * dim;
+    for (int gi = 0; gi < n_groups; ++gi) {
+        const eggmi_group& g = groups[gi];
+        const std::string grp = "group[" + std::to_string(gi) + "]";
+        if (g.ndof < 0 || g.total_samples < 0) throw InvalidArgument("validate_context: " + grp + " negative size");
+        if (g.w_rows != 1 && g.w_rows != g.total_samples)
+            throw InvalidArgument("validate_context: " + grp + " W_inv size " + std::to_string(g.w_rows * dd) +
+                                  " != " + std::to_string(g.total_samples * dd));
+        for (long long p = 0; p < g.total_samples; ++p) {
+            if (g.role[p] < -1 || g.role[p] > dim)
+                throw InvalidArgument("validate_context: " + grp + " role " + std::to_string(g.role[p]) +
+                                      " out of range [-1, " + std::to_string(dim) + "]");
+        }
+        long long sumP = 0;
+        for (long long d = 0; d < g.ndof; ++d) {
+            if (g.P_of[d] < 0) throw InvalidArgument("validate_context: " + grp + " negative P_of");
+            sumP += g.P_of[d];
+        }
+        if (sumP != g.total_samples)
+            throw InvalidArgument("validate_context: " + grp + " sum(P_of) " + std::to_string(sumP) +
+                                  " != total_samples " + std::to_string(g.total_samples));
+        check_index_range(g.gc, g.total_samples, N, grp, "gc");
+        for (int k = 0; k < dim; ++k) {
+            const std::string nm = "gn" + std::to_string(k);
+            check_index_range(g.gn[k], g.total_samples, N, grp, nm.c_str());
+        }
+        check_index_range(g.dof_idx, g.ndof, N, grp, "dof_idx");
+        for (int k = 0; k < dim; ++k)
+            for (long long p = 0; p < g.total_samples; ++p)
+                if (g.s[k][p] != 1.0 && g.s[k][p] != -1.0)
+                    throw InvalidArgument("validate_context: " + grp + " s" + std::to_string(k) +
+                                          " must be +-1 per sample");
+    }
+    if (es->num_samples < 0) throw InvalidArgument("validate_context: energy_stencil negative size");
+    if (es->w_rows != 1 && es->w_rows != es->num_samples)
+        throw InvalidArgument("validate_context: energy_stencil W_inv size " + std::to_string(es->w_rows * dd) +
+                              " != " + std::to_string(es->num_samples * dd));
+    check_index_range(es->gc, es->num_samples, N, "energy_stencil", "gc");
+    for (int k = 0; k < dim; ++k) {
+        const std::string nm = "gn" + std::to_string(k);
+        check_index_range(es->gn[k], es->num_samples, N, "energy_stencil", nm.c_str());
+    }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ session
+
+void Session::ensure_timing_events(size_t n)
+{
+    while (tev.size() < n) {
+        hipEvent_t e;
+        EGGMI_HIP(hipEventCreate(&e));
+        tev.push_back(e);
+    }
+}
+
+Session::~Session()
+{
+    for (auto e : tev) (void)hipEventDestroy(e);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+void Session::create(int d, const eggmi_group* groups, int n_groups, const eggmi_stencil* es,
+                     const eggmi_structured* st, const double* X, long long num_nodes, int device)
+{
+    if (d != 2 && d != 3) throw InvalidArgument("dim must be 2 or 3; got " + std::to_string(d));
+    if (device < 0 || device > 3) throw InvalidArgument("device must be 0 (auto), 1 (gpu), 2 (cpu) or 3 (tables only)");
+    dim = d;
+    M = num_nodes;
+    validate(d, groups, n_groups, es, num_nodes);
+
+    // ---- structured remap (reference src/bindings.cpp:758-1101) ----
+    const int nb = st->num_blocks;
+    std::vector<std::array<int, 3>> shapes(nb, {1, 1, 1});
+    for (int b = 0; b < nb; ++b)
+        for (int k = 0; k < d; ++k) {
+            shapes[b][k] = st->interior_shapes[b * d + k];
+            if (shapes[b][k] < 1) throw InvalidArgument("structured: interior shape entries must be >= 1");
+        }
+    Layout& lay = fine.lay;
+    lay.build(d, shapes);
+    bgd.assign(nb, {});
+    for (int b = 0; b < nb; ++b) {
+        long long n = 1;
+        for (int k = 0; k < d; ++k) n *= shapes[b][k];
+        bgd[b].assign(st->block_global_dof[b], st->block_global_dof[b] + n);
+        for (int v : bgd[b])
+            if (v < 0 || v >= M)
+                throw InvalidArgument("structured: block_global_dof entry " + std::to_string(v) + " out of range");
+    }
+    auto unravel = [&](int b, long long f, int* logical) {
+        for (int k = d - 1; k >= 0; --k) {
+            logical[k] = (int)(f % shapes[b][k]);
+            f /= shapes[b][k];
+        }
+    };
+    auto ravel = [&](int b, const int* logical) {
+        long long f = 0;
+        for (int k = 0; k < d; ++k) f = f * shapes[b][k] + logical[k];
+        return f;
+    };
+
+    // pass 1: interior membership (gid -> (block, slot) list, CSR), owners,
+    // initial scatter of X into every copy
+    std::vector<int> owner_block(M, -1);
+    std::vector<long long> owner_slot(M, -1);
+    std::vector<int> mcount(M + 1, 0);
+    for (int b = 0; b < nb; ++b)
+        for (int g : bgd[b]) ++mcount[g + 1];
+    for (long long g = 0; g < M; ++g) mcount[g + 1] += mcount[g];
+    std::vector<int> mfill(mcount.begin(), mcount.end() - 1);
+    std::vector<int> mblock(mcount[M]);
+    std::vector<long long> mslot(mcount[M]);
+    const long long nslots_blocks = lay.padded_nodes;
+    std::vector<double> Xp((size_t)nslots_blocks * d, 0.0);
+    for (int b = 0; b < nb; ++b) {
+        int lg[3];
+        for (long long f = 0; f < (long long)bgd[b].size(); ++f) {
+            unravel(b, f, lg);
+            const int gid = bgd[b][f];
+            const long long sidx = lay.interior_slot(b, lg);
+            mblock[mfill[gid]] = b;
+            mslot[mfill[gid]++] = sidx;
+            if (owner_block[gid] < 0) {
+                owner_block[gid] = b;
+                owner_slot[gid] = sidx;
+            }
+            for (int k = 0; k < d; ++k) Xp[(size_t)sidx * d + k] = X[(size_t)gid * d + k];
+        }
+    }
+    // singular host override
+    for (long long s = 0; s < st->n_sing; ++s) {
+        const int ob = st->sing_block[s];
+        if (ob < 0 || ob >= nb) throw InvalidArgument("structured: sing_block out of range");
+        int lg[3];
+        for (int k = 0; k < d; ++k) {
+            lg[k] = st->sing_logical[s * d + k];
+            if (lg[k] < 0 || lg[k] >= shapes[ob][k]) throw InvalidArgument("structured: sing_logical out of range");
+        }
+        const int gid = bgd[ob][ravel(ob, lg)];
+        owner_block[gid] = ob;
+        owner_slot[gid] = lay.interior_slot(ob, lg);
+    }
+    // pass 2: owner -> copy broadcasts
+    std::vector<int> share_src, share_dst;
+    for (int b = 0; b < nb; ++b) {
+        int lg[3];
+        for (long long f = 0; f < (long long)bgd[b].size(); ++f) {
+            const int gid = bgd[b][f];
+            if (owner_block[gid] == b) continue;
+            unravel(b, f, lg);
+            share_src.push_back((int)owner_slot[gid]);
+            share_dst.push_back((int)lay.interior_slot(b, lg));
+        }
+    }
+    // halo entries: ghost maps per block (these override interior entries)
+    std::vector<std::unordered_map<int, long long>> ghost_map(nb);
+    std::vector<int> halo_src, halo_dst;
+    std::vector<std::vector<char>> used_ghost(nb);
+    for (int b = 0; b < nb; ++b) used_ghost[b].assign(lay.block_padded_count(b), 0);
+    for (long long e = 0; e < st->n_halo; ++e) {
+        const int sb = st->halo_src_block[e], db = st->halo_dst_block[e];
+        if (sb < 0 || sb >= nb || db < 0 || db >= nb) throw InvalidArgument("structured: halo block out of range");
+        int sp[3], dp[3], sl[3];
+        for (int k = 0; k < d; ++k) {
+            sp[k] = st->halo_src_padded[e * d + k];
+            dp[k] = st->halo_dst_padded[e * d + k];
+            sl[k] = sp[k] - 1;
+            if (sl[k] < 0 || sl[k] >= shapes[sb][k]) throw InvalidArgument("structured: halo_src_padded not interior");
+            if (dp[k] < 0 || dp[k] > shapes[db][k] + 1) throw InvalidArgument("structured: halo_dst_padded out of range");
+        }
+        const int gid = bgd[sb][ravel(sb, sl)];
+        const long long ds = lay.padded_slot(db, dp);
+        ghost_map[db][gid] = ds;
+        used_ghost[db][ds - lay.base[db]] = 1;
+        halo_src.push_back((int)lay.padded_slot(sb, sp));
+        halo_dst.push_back((int)ds);
+    }
+    // block-local lookup: ghost entries first (they override), then interior copies
+    auto find_in_block = [&](int b, int gid, long long& slot) -> bool {
+        auto it = ghost_map[b].find(gid);
+        if (it != ghost_map[b].end()) {
+            slot = it->second;
+            return true;
+        }
+        for (int m = mcount[gid]; m < mcount[gid + 1]; ++m)
+            if (mblock[m] == b) {
+                slot = mslot[m];
+                return true;
+            }
+        return false;
+    };
+    g2s.assign(M, -1);
+    for (long long g = 0; g < M; ++g) {
+        const int ob = owner_block[g];
+        if (ob < 0) continue;
+        long long slot;
+        if (!find_in_block(ob, (int)g, slot))
+            throw InvalidArgument("structured: owner_block names a block that does not contain its global node");
+        g2s[g] = (int)slot;
+    }
+    // spare ghost-ring slots per block, in padded row-major order
+    std::vector<std::vector<long long>> free_ghosts(nb);
+    std::vector<size_t> cursor(nb, 0);
+    for (int b = 0; b < nb; ++b) {
+        const long long tot = lay.block_padded_count(b);
+        for (long long f = 0; f < tot; ++f) {
+            int pd[3];
+            long long rem = f;
+            bool ring = false;
+            for (int k = d - 1; k >= 0; --k) {
+                pd[k] = (int)(rem % (shapes[b][k] + 2));
+                rem /= shapes[b][k] + 2;
+            }
+            for (int k = 0; k < d; ++k) ring = ring || pd[k] == 0 || pd[k] == shapes[b][k] + 1;
+            if (ring && !used_ghost[b][f]) free_ghosts[b].push_back(lay.base[b] + f);
+        }
+    }
+    std::vector<int> fan_src, fan_dst;
+    long long n_overflow = 0;
+    auto lookup = [&](int b, int gid, const char* what) -> long long {
+        long long slot;
+        if (find_in_block(b, gid, slot)) return slot;
+        const int src = g2s[gid];
+        if (src < 0)
+            throw InvalidArgument(std::string("structured: ") + what + " references global node " +
+                                  std::to_string(gid) + " absent from every block's interior map");
+        if (cursor[b] < free_ghosts[b].size()) {
+            slot = free_ghosts[b][cursor[b]++];
+        } else {
+            slot = (long long)(Xp.size() / d);
+            for (int k = 0; k < d; ++k) Xp.push_back(Xp[(size_t)src * d + k]);
+            ++n_overflow;
+        }
+        ghost_map[b][gid] = slot;
+        fan_src.push_back(src);
+        fan_dst.push_back((int)slot);
+        return slot;
+    };
+
+    // DOF tables: synthesized interior DOFs get a mask bit; stored ones their
+    // remapped occurrences (relative to the owner block)
+    const int dd = d * d;
+    std::vector<int> synth_slots;
+    std::vector<int> gn_rows;  // [total][dim] stored neighbour slots
+    Level& L = fine;
+    L.st_off.assign(1, 0);
+    bool w_uniform_all = true;
+    std::vector<double> wu_ref;
+    for (int gi = 0; gi < n_groups; ++gi)
+        if (groups[gi].w_rows != 1) w_uniform_all = false;
+    if (w_uniform_all && n_groups > 0) {
+        // all groups uniform: they must agree to share one uniform row
+        for (int gi = 0; gi < n_groups; ++gi) {
+            std::vector<double> w(groups[gi].W_inv, groups[gi].W_inv + dd);
+            if (wu_ref.empty())
+                wu_ref = w;
+            else if (w != wu_ref)
+                w_uniform_all = false;
+        }
+    }
+    for (int gi = 0; gi < n_groups; ++gi) {
+        const eggmi_group& g = groups[gi];
+        long long off = 0;
+        for (long long di = 0; di < g.ndof; ++di) {
+            const int gid = g.dof_idx[di];
+            const int ob = owner_block[gid];
+            if (ob < 0) throw InvalidArgument("structured: dof_idx " + std::to_string(gid) + " is in no block");
+            const long long P = g.P_of[di];
+            std::vector<long long> occ_slots;
+            for (long long p = off; p < off + P; ++p) {
+                occ_slots.push_back(lookup(ob, g.gc[p], "gc"));
+                for (int k = 0; k < d; ++k) occ_slots.push_back(lookup(ob, g.gn[k][p], "gn"));
+            }
+            const long long dslot = lookup(ob, gid, "dof_idx");
+            const int ib = g.interior_block[di];
+            if (ib >= 0) {
+                if (P != 0) throw InvalidArgument("structured: interior DOF with stored samples");
+                if (ib >= nb) throw InvalidArgument("structured: interior_block out of range");
+                int lg[3];
+                for (int k = 0; k < d; ++k) {
+                    lg[k] = g.interior_logical[di * d + k];
+                    if (lg[k] < 1 || lg[k] > shapes[ib][k] - 2)
+                        throw InvalidArgument("structured: interior_logical outside [1, n-2]");
+                }
+                if (lay.interior_slot(ib, lg) != dslot)
+                    throw InvalidArgument("structured: interior DOF does not live at its block slot");
+                synth_slots.push_back((int)dslot);
+            } else {
+                L.st_dof.push_back((int)dslot);
+                for (long long p = off, q = 0; p < off + P; ++p, ++q) {
+                    L.st_gc.push_back((int)occ_slots[q * (d + 1)]);
+                    for (int k = 0; k < d; ++k) gn_rows.push_back((int)occ_slots[q * (d + 1) + 1 + k]);
+                    uint8_t meta = (uint8_t)((g.role[p] + 1) & 3);
+                    for (int k = 0; k < d; ++k)
+                        if (g.s[k][p] < 0) meta |= (uint8_t)(4 << k);
+                    L.st_meta.push_back(meta);
+                    if (!w_uniform_all) {
+                        const double* w = g.w_rows == 1 ? g.W_inv : g.W_inv + p * dd;
+                        for (int i = 0; i < dd; ++i) L.st_W.push_back((float)w[i]);
+                    }
+                }
+                L.st_off.push_back((int)L.st_gc.size());
+            }
+            off += P;
+        }
+    }
+    // gn stored [dim][total]
+    {
+        const long long total = (long long)L.st_gc.size();
+        L.st_gn.assign((size_t)total * d, 0);
+        for (long long q = 0; q < total; ++q)
+            for (int k = 0; k < d; ++k) L.st_gn[(size_t)k * total + q] = gn_rows[(size_t)q * d + k];
+    }
+    if (w_uniform_all && !wu_ref.empty())
+        for (int i = 0; i < dd; ++i) L.Wu[i] = (float)wu_ref[i];
+    lay.overflow = n_overflow;
+    const long long nslots = lay.n_slots();
+    if ((long long)Xp.size() != nslots * d) throw Internal("structured: packed X size disagrees with the layout");
+    if (nslots >= (1ll << 31)) throw InvalidArgument("structured: more than 2^31 node slots");
+    L.mask.assign(nslots, 0);
+    for (int s : synth_slots) L.mask[s] = 1;
+    L.n_synth = (long long)synth_slots.size();
+    // copy table: halo | fan | share
+    L.copy_src = halo_src;
+    L.copy_dst = halo_dst;
+    L.copy_src.insert(L.copy_src.end(), fan_src.begin(), fan_src.end());
+    L.copy_dst.insert(L.copy_dst.end(), fan_dst.begin(), fan_dst.end());
+    L.copy_src.insert(L.copy_src.end(), share_src.begin(), share_src.end());
+    L.copy_dst.insert(L.copy_dst.end(), share_dst.begin(), share_dst.end());
+    L.n_halo = (long long)halo_src.size();
+    L.n_fan = (long long)fan_src.size();
+    L.n_share = (long long)share_src.size();
+
+    // ---- energy stencil: structured cells when the wire is the plain cell stencil ----
+    {
+        bool canon = es->w_rows == 1;
+        if (canon)
+            for (int i = 0; i < d; ++i)
+                for (int j = 0; j < d; ++j)
+                    if (es->W_inv[i * d + j] != (i == j ? 1.0 : 0.0)) canon = false;
+        const int nc = 1 << d;
+        long long q = 0;
+        for (int b = 0; b < nb && canon; ++b) {
+            bool ok = true;
+            for (int k = 0; k < d; ++k) ok = ok && shapes[b][k] >= 2;
+            if (!ok) continue;
+            int cdim[3] = {1, 1, 1};
+            for (int k = 0; k < d; ++k) cdim[k] = shapes[b][k] - 1;
+            const long long ncell = (long long)cdim[0] * cdim[1] * (d == 3 ? cdim[2] : 1);
+            for (long long c = 0; c < ncell && canon; ++c) {
+                int cl[3] = {0, 0, 0};
+                long long rem = c;
+                for (int k = d - 1; k >= 0; --k) {
+                    cl[k] = (int)(rem % cdim[k]);
+                    rem /= cdim[k];
+                }
+                for (int bb = 0; bb < nc && canon; ++bb, ++q) {
+                    if (q >= es->num_samples) {
+                        canon = false;
+                        break;
+                    }
+                    int corner[3];
+                    for (int k = 0; k < d; ++k) corner[k] = cl[k] + ((bb >> (d - 1 - k)) & 1);
+                    if (es->gc[q] != bgd[b][ravel(b, corner)]) canon = false;
+                    for (int k = 0; k < d && canon; ++k) {
+                        int nbr[3] = {corner[0], corner[1], corner[2]};
+                        const int bit = (bb >> (d - 1 - k)) & 1;
+                        nbr[k] = cl[k] + (1 - bit);
+                        if (es->gn[k][q] != bgd[b][ravel(b, nbr)]) canon = false;
+                        if (es->s[k][q] != (bit ? -1.0 : 1.0)) canon = false;
+                    }
+                }
+            }
+        }
+        if (q != es->num_samples) canon = false;
+        L.structured_energy = canon;
+        if (!canon) {
+            const long long ns = es->num_samples;
+            std::vector<int> gc(ns), gn((size_t)ns * d);
+            std::vector<float> sv((size_t)ns * d), W;
+            for (long long i = 0; i < ns; ++i) {
+                gc[i] = g2s[es->gc[i]];
+                for (int k = 0; k < d; ++k) {
+                    gn[(size_t)k * ns + i] = g2s[es->gn[k][i]];
+                    sv[(size_t)k * ns + i] = (float)es->s[k][i];
+                }
+                if (gc[i] < 0)
+                    throw InvalidArgument("structured: an energy-stencil index references a global node absent "
+                                          "from every block's interior map");
+            }
+            if (es->w_rows == 1) {
+                for (int i = 0; i < dd; ++i) L.es_Wu[i] = es->W_inv[i];
+            } else {
+                W.assign(es->W_inv, es->W_inv + (size_t)ns * dd);
+            }
+            L.es_n = ns;
+            L.d_es_gc.upload(gc);
+            L.d_es_gn.upload(gn);
+            L.d_es_s.upload(sv);
+            L.d_es_W.upload(W);
+        }
+    }
+
+    // ---- device residency (host validation and remap above touch no device) ----
+    if (device == 3) {  // tables only: index maps for inspection, no device state
+        host_only = true;
+        return;
+    }
+    if (device == 2)
+        throw NoDevice("select_queue: no device available for device='cpu' (this build runs on the MI355X "
+                       "GPU only). Use device='gpu' or 'auto'.");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        throw NoDevice("select_queue: no HIP device available for device='" +
+                       std::string(device == 1 ? "gpu" : "auto") + "'.");
+    EGGMI_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    EGGMI_HIP(hipEventCreate(&ev0));
+    EGGMI_HIP(hipEventCreate(&ev1));
+    L.plane = (nslots + 63) / 64 * 64;
+    L.upload_tables();
+    L.make_tiles(tile_variant);
+    std::vector<float> xs((size_t)L.plane * d, 0.0f);
+    for (long long s = 0; s < nslots; ++s)
+        for (int k = 0; k < d; ++k) xs[(size_t)k * L.plane + s] = (float)Xp[(size_t)s * d + k];
+    L.x[0].upload(xs);
+    L.x[1].alloc(xs.size());
+    L.cur = 0;
+    partials.alloc(2 * kReduceBlocks);
+}
+
+void Session::halo(Level& L, float* x)
+{
+    launch_copy(x, L.plane, dim, L.d_copy_src.p, L.d_copy_dst.p, (long long)L.copy_src.size(), stream);
+}
+
+void Session::reduce(Level& L, const float* x, int kind, double delta, double* out_e, double* out_m)
+{
+    if (L.structured_energy) {
+        CellReduceArgs a;
+        a.x = x;
+        a.plane = L.plane;
+        a.blocks = L.blocks.p;
+        a.cell_off = L.d_cell_off.p;
+        a.nblocks = (int)L.lay.shape.size();
+        a.ncells = L.ncells;
+        a.delta = delta;
+        launch_reduce_cells(dim, kind, a, partials.p, stream);
+    } else {
+        SampleReduceArgs a;
+        a.x = x;
+        a.plane = L.plane;
+        a.ns = L.es_n;
+        a.gc = L.d_es_gc.p;
+        a.gn = L.d_es_gn.p;
+        a.s = L.d_es_s.p;
+        a.W = L.d_es_W.n ? L.d_es_W.p : nullptr;
+        for (int i = 0; i < 9; ++i) a.Wu[i] = L.es_Wu[i];
+        a.delta = delta;
+        launch_reduce_samples(dim, kind, a, partials.p, stream);
+    }
+    launch_reduce_finish(partials.p, kReduceBlocks, out_e, out_m, stream);
+}
+
+void Session::need_device() const
+{
+    if (host_only) throw InvalidArgument("session was built with device tables only (no device state)");
+}
+
+std::pair<std::vector<double>, std::vector<double>> Session::run_jacobi(int n_sweeps, int kind, double delta,
+                                                                        double omega, int report_every)
+{
+    need_device();
+    if (n_sweeps <= 0) return {{}, {}};
+    const int k = report_every <= 0 ? n_sweeps : report_every;
+    const int nrep = (n_sweeps + k - 1) / k;
+    if ((int)d_reports.n < 2 * nrep) d_reports.alloc(2 * nrep);
+    Level& L = fine;
+    float* xin = L.x[L.cur].p;
+    float* xout = L.x[1 - L.cur].p;
+    const size_t bytes = (size_t)L.plane * dim * sizeof(float);
+    EGGMI_HIP(hipEventRecord(ev0, stream));
+    halo(L, xin);
+    EGGMI_HIP(hipMemcpyAsync(xout, xin, bytes, hipMemcpyDeviceToDevice, stream));
+    int rep = 0;
+    if (kernel_timing) ensure_timing_events(2 * n_sweeps);
+    for (int s = 0; s < n_sweeps; ++s) {
+        const SweepArgs a = L.sweep_args(xin, xout, (float)omega, (float)delta, nullptr);
+        if (kernel_timing) EGGMI_HIP(hipEventRecord(tev[2 * s], stream));
+        launch_sweep(dim, kind, tile_variant, a, stream);
+        if (kernel_timing) EGGMI_HIP(hipEventRecord(tev[2 * s + 1], stream));
+        halo(L, xout);
+        if ((s + 1) % k == 0 || s == n_sweeps - 1) {
+            reduce(L, xout, kind, delta, d_reports.p + rep, d_reports.p + nrep + rep);
+            ++rep;
+        }
+        std::swap(xin, xout);
+    }
+    EGGMI_HIP(hipEventRecord(ev1, stream));
+    L.cur = (xin == L.x[0].p) ? 0 : 1;
+    std::vector<double> hr(2 * nrep);
+    EGGMI_HIP(hipMemcpyAsync(hr.data(), d_reports.p, sizeof(double) * 2 * nrep, hipMemcpyDeviceToHost, stream));
+    EGGMI_HIP(hipStreamSynchronize(stream));
+    float ms = 0;
+    EGGMI_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+    last_ms = ms;
+    last_launches = n_sweeps;
+    last_kernel_ms = 0.0;
+    if (kernel_timing)
+        for (int s = 0; s < n_sweeps; ++s) {
+            float k = 0;
+            EGGMI_HIP(hipEventElapsedTime(&k, tev[2 * s], tev[2 * s + 1]));
+            last_kernel_ms += k;
+        }
+    return {std::vector<double>(hr.begin(), hr.begin() + nrep), std::vector<double>(hr.begin() + nrep, hr.end())};
+}
+
+void Session::get_X(double* out) const
+{
+    need_device();
+    const Level& L = fine;
+    std::vector<float> xs((size_t)L.plane * dim);
+    EGGMI_HIP(hipMemcpy(xs.data(), L.x[L.cur].p, xs.size() * sizeof(float), hipMemcpyDeviceToHost));
+    for (long long g = 0; g < M; ++g) {
+        const int s = g2s[g];
+        for (int k = 0; k < dim; ++k) out[(size_t)g * dim + k] = s < 0 ? 0.0 : (double)xs[(size_t)k * L.plane + s];
+    }
+}
+
+void Session::set_X(const double* X)
+{
+    need_device();
+    Level& L = fine;
+    std::vector<float> xs((size_t)L.plane * dim);
+    EGGMI_HIP(hipMemcpy(xs.data(), L.x[L.cur].p, xs.size() * sizeof(float), hipMemcpyDeviceToHost));
+    for (long long g = 0; g < M; ++g) {
+        const int s = g2s[g];
+        if (s < 0) continue;
+        for (int k = 0; k < dim; ++k) xs[(size_t)k * L.plane + s] = (float)X[(size_t)g * dim + k];
+    }
+    EGGMI_HIP(hipMemcpy(L.x[L.cur].p, xs.data(), xs.size() * sizeof(float), hipMemcpyHostToDevice));
+    halo(L, L.x[L.cur].p);
+    EGGMI_HIP(hipStreamSynchronize(stream));
+}
+
+std::vector<float> Session::packed() const
+{
+    need_device();
+    const Level& L = fine;
+    const long long ns = L.lay.n_slots();
+    std::vector<float> xs((size_t)L.plane * dim), out((size_t)ns * dim);
+    EGGMI_HIP(hipMemcpy(xs.data(), L.x[L.cur].p, xs.size() * sizeof(float), hipMemcpyDeviceToHost));
+    for (int k = 0; k < dim; ++k)
+        std::memcpy(out.data() + (size_t)k * ns, xs.data() + (size_t)k * L.plane, ns * sizeof(float));
+    return out;
+}
+
+}  // namespace eggmi

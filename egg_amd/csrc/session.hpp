@@ -1,0 +1,177 @@
+// Host side of the MI355X structured smoother: wire validation, the
+// structured remap onto the halo-padded store, device residency and the
+// sweep / FAS drivers. Plain C++ over the HIP runtime; exposed through the C
+// ABI in capi.cpp (include/eggmi.h).
+#pragma once
+
+#include "eggmi.h"
+#include "kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace eggmi
+{
+
+struct InvalidArgument : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct NoDevice : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct Internal : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void hip_check(hipError_t e, const char* what);
+#define EGGMI_HIP(expr) ::eggmi::hip_check((expr), #expr)
+
+// Owning device array.
+template <class T> struct DArr {
+    T* p = nullptr;
+    size_t n = 0;
+    DArr() = default;
+    DArr(const DArr&) = delete;
+    DArr& operator=(const DArr&) = delete;
+    DArr(DArr&& o) noexcept : p(o.p), n(o.n)
+    {
+        o.p = nullptr;
+        o.n = 0;
+    }
+    DArr& operator=(DArr&& o) noexcept
+    {
+        if (this != &o) {
+            release();
+            p = o.p;
+            n = o.n;
+            o.p = nullptr;
+            o.n = 0;
+        }
+        return *this;
+    }
+    ~DArr() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count)
+    {
+        release();
+        n = count;
+        if (count) EGGMI_HIP(hipMalloc(&p, count * sizeof(T)));
+    }
+    void upload(const std::vector<T>& h)
+    {
+        alloc(h.size());
+        if (n) EGGMI_HIP(hipMemcpy(p, h.data(), n * sizeof(T), hipMemcpyHostToDevice));
+    }
+};
+
+// Per-block geometry of one level of the halo-padded store
+// (reference src/structured.hpp:47-163 BlockLayout<D>, in node units).
+struct Layout {
+    int dim = 2;
+    std::vector<std::array<int, 3>> shape;  // interior node counts
+    std::vector<long long> base;            // node slot of padded (0,..,0)
+    std::vector<std::array<long long, 3>> stride;
+    long long padded_nodes = 0;  // sum of padded block sizes
+    long long overflow = 0;      // extra slots past the last block
+
+    void build(int d, const std::vector<std::array<int, 3>>& shapes);
+    long long n_slots() const { return padded_nodes + overflow; }
+    long long padded_slot(int b, const int* padded) const;
+    long long interior_slot(int b, const int* logical) const;
+    long long block_padded_count(int b) const;
+};
+
+// Everything one level of the hierarchy needs on the device.
+struct Level {
+    Layout lay;
+    long long plane = 0;  // coordinate plane stride (slots rounded up)
+    DArr<BlockDesc> blocks;
+    // fused copy table: halo [0, n_halo), fan [n_halo, n_halo+n_fan), share after
+    std::vector<int> copy_src, copy_dst;
+    long long n_halo = 0, n_fan = 0, n_share = 0;
+    DArr<int> d_copy_src, d_copy_dst;
+    // synthesized interior DOFs
+    std::vector<uint8_t> mask;  // [n_slots]
+    DArr<uint8_t> d_mask;
+    long long n_synth = 0;
+    std::vector<TileDesc> tiles;
+    DArr<TileDesc> d_tiles;
+    // stored-stencil DOFs
+    std::vector<int> st_dof, st_off, st_gc, st_gn;
+    std::vector<uint8_t> st_meta;
+    std::vector<float> st_W;  // empty = uniform
+    float Wu[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    DArr<int> d_st_dof, d_st_off, d_st_gc, d_st_gn;
+    DArr<uint8_t> d_st_meta;
+    DArr<float> d_st_W;
+    // energy reduction: structured cells, or an explicit sample table
+    bool structured_energy = true;
+    std::vector<long long> cell_off;
+    DArr<long long> d_cell_off;
+    long long ncells = 0;
+    long long es_n = 0;
+    DArr<int> d_es_gc, d_es_gn;
+    DArr<float> d_es_s, d_es_W;
+    double es_Wu[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    // device state
+    DArr<float> x[2];
+    int cur = 0;  // which x holds the canonical state
+
+    void upload_tables();
+    void make_tiles(int variant);
+    SweepArgs sweep_args(const float* xin, float* xout, float omega, float delta, const float* tau) const;
+};
+
+struct Session {
+    int dim = 2;
+    long long M = 0;  // global node count
+    std::vector<int> g2s;
+    std::vector<std::vector<int>> bgd;  // per-block global ids (row-major)
+    Level fine;
+    int tile_variant = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    DArr<double> partials;
+    DArr<double> d_reports;
+    double last_ms = 0.0;
+    long long last_launches = 0;
+    bool kernel_timing = false;     // bracket every sweep launch with events
+    double last_kernel_ms = 0.0;    // summed sweep-kernel time of the last run
+    std::vector<hipEvent_t> tev;
+    void ensure_timing_events(size_t n);
+    // FAS hierarchy (built lazily)
+    std::vector<std::unique_ptr<Level>> coarse;
+    int built_levels = -1;
+    bool host_only = false;  // created with device = 3: index tables only
+
+    Session() = default;
+    ~Session();
+
+    void create(int dim, const eggmi_group* groups, int n_groups, const eggmi_stencil* es,
+                const eggmi_structured* st, const double* X, long long num_nodes, int device);
+    void halo(Level& L, float* x);
+    void reduce(Level& L, const float* x, int kind, double delta, double* out_e, double* out_m);
+    std::pair<std::vector<double>, std::vector<double>> run_jacobi(int n_sweeps, int kind, double delta,
+                                                                   double omega, int report_every);
+    std::pair<std::vector<double>, std::vector<double>> run_fas(int n_cycles, int nu_pre, int nu_post,
+                                                                int nu_coarse, double omega, int max_levels,
+                                                                int kind);
+    std::vector<std::vector<std::array<int, 3>>> mg_level_shapes(int max_levels);
+    void get_X(double* out) const;
+    void set_X(const double* X);
+    std::vector<float> packed() const;
+    void need_device() const;
+};
+
+}  // namespace eggmi

@@ -1,0 +1,206 @@
+"""Synthetic multi-block structured grids for the benchmark workloads.
+
+Every generator returns a :class:`Grid`: per-block global-id arrays (the
+``block_global_dof`` of the structured tables), positions ``X`` (N, d) in
+global order and the moving-DOF mask. Blocks are oriented so that every
+cell's corner Jacobian has positive determinant before any perturbation.
+
+Workloads (seeds are arguments; the benchmark fixes them):
+
+* :func:`single_block_2d` — one n x n block on the unit square, interior
+  nodes moved by U(-amp h, amp h) per coordinate, re-drawn until every
+  corner sample keeps det A > 0; boundary fixed.
+* :func:`ogrid_2d` — 5-block O-grid: a central n x n block whose four edges
+  are 90-degree arcs of the circle r = 0.5 (built by transfinite
+  interpolation) inside four n x m ring blocks reaching the square
+  [-2, 2]^2. The circle is an internal interface of free nodes; the outer
+  square is fixed; the four central-block corners are valence-3 nodes.
+* :func:`tangle` — displace a seeded fraction of the free nodes by
+  ``factor`` times their local edge length in a seeded direction.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+
+@dataclass
+class Grid:
+    d: int
+    blocks: list  # per-block int32 global-id arrays
+    X: np.ndarray  # (N, d)
+    free: np.ndarray  # (N,) bool
+
+    @property
+    def n_nodes(self) -> int:
+        return int(self.X.shape[0])
+
+    @property
+    def n_free(self) -> int:
+        return int(self.free.sum())
+
+
+def corner_dets(blocks, X: np.ndarray, d: int) -> np.ndarray:
+    """det A of every (cell, corner) sample of the blocks."""
+    out = []
+    for ids in blocks:
+        ids = np.asarray(ids)
+        if any(s < 2 for s in ids.shape):
+            continue
+        nc = 1 << d
+        corners = []
+        for b in range(nc):
+            sl = tuple(slice((b >> (d - 1 - k)) & 1, ((b >> (d - 1 - k)) & 1) + ids.shape[k] - 1) for k in range(d))
+            corners.append(ids[sl].reshape(-1))
+        C = np.stack(corners, 1)
+        for b in range(nc):
+            cols = []
+            for k in range(d):
+                bit = 1 << (d - 1 - k)
+                s = -1.0 if b & bit else 1.0
+                cols.append(s * (X[C[:, b ^ bit]] - X[C[:, b]]))
+            out.append(np.linalg.det(np.stack(cols, axis=2)))
+    return np.concatenate(out) if out else np.zeros(0)
+
+
+def min_det(grid: Grid) -> float:
+    return float(corner_dets(grid.blocks, grid.X, grid.d).min())
+
+
+def _merge(block_coords: list, d: int, scale: float = 1e8):
+    """Global ids by merging coincident block nodes (rounded coordinates)."""
+    keys = {}
+    blocks = []
+    pts = []
+    for P in block_coords:
+        shp = P.shape[:-1]
+        flat = P.reshape(-1, d)
+        ids = np.empty(flat.shape[0], np.int32)
+        rk = np.round(flat * scale).astype(np.int64)
+        for n in range(flat.shape[0]):
+            key = tuple(rk[n])
+            g = keys.get(key)
+            if g is None:
+                g = len(pts)
+                keys[key] = g
+                pts.append(flat[n])
+            ids[n] = g
+        blocks.append(ids.reshape(shp))
+    return blocks, np.asarray(pts, np.float64)
+
+
+def single_block_2d(n: int = 129, *, seed: int = 0, amp: float = 0.3) -> Grid:
+    """n x n nodes on [0, 1]^2, node (i, j) at (i h, j h); interior perturbed
+    by U(-amp h, amp h) per coordinate (re-drawn until min det A > 0)."""
+    h = 1.0 / (n - 1)
+    ids = np.arange(n * n, dtype=np.int32).reshape(n, n)
+    i, j = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    X0 = np.stack([i * h, j * h], -1).reshape(-1, 2)
+    free = np.zeros((n, n), bool)
+    free[1:-1, 1:-1] = True
+    free = free.reshape(-1)
+    rng = np.random.default_rng(seed)
+    for _ in range(100):
+        X = X0.copy()
+        X[free] += rng.uniform(-amp * h, amp * h, size=(int(free.sum()), 2))
+        if corner_dets([ids], X, 2).min() > 0:
+            return Grid(2, [ids], X, free)
+    raise RuntimeError("could not draw a valid perturbation")
+
+
+def single_block_3d(n: int = 17, *, seed: int = 0, amp: float = 0.2) -> Grid:
+    """n^3 nodes on [0, 1]^3, interior perturbed by U(-amp h, amp h)."""
+    h = 1.0 / (n - 1)
+    ids = np.arange(n**3, dtype=np.int32).reshape(n, n, n)
+    g = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    X0 = np.stack([c * h for c in g], -1).reshape(-1, 3)
+    free = np.zeros((n, n, n), bool)
+    free[1:-1, 1:-1, 1:-1] = True
+    free = free.reshape(-1)
+    rng = np.random.default_rng(seed)
+    for _ in range(100):
+        X = X0.copy()
+        X[free] += rng.uniform(-amp * h, amp * h, size=(int(free.sum()), 3))
+        if corner_dets([ids], X, 3).min() > 0:
+            return Grid(3, [ids], X, free)
+    raise RuntimeError("could not draw a valid perturbation")
+
+
+def _ang(deg):
+    return np.deg2rad(deg)
+
+
+def ogrid_2d(n: int = 129, m: int = 65, *, r: float = 0.5, half: float = 2.0) -> Grid:
+    """5-block O-grid: central n x n disc block + four n x m ring blocks."""
+    u = np.linspace(0.0, 1.0, n)
+
+    def circ(theta):
+        return np.stack([r * np.cos(theta), r * np.sin(theta)], -1)
+
+    # central block, node (i, j) -> (u_i, v_j); edges are arcs of the circle
+    U, V = np.meshgrid(u, u, indexing="ij")
+    Bu = circ(_ang(225.0 + 90.0 * u))  # v = 0
+    Tu = circ(_ang(135.0 - 90.0 * u))  # v = 1
+    Lv = circ(_ang(225.0 - 90.0 * u))  # u = 0
+    Rv = circ(_ang(-45.0 + 90.0 * u))  # u = 1
+    P00, P10, P01, P11 = Bu[0], Bu[-1], Tu[0], Tu[-1]
+    C = ((1 - V)[..., None] * Bu[:, None, :] + V[..., None] * Tu[:, None, :]
+         + (1 - U)[..., None] * Lv[None, :, :] + U[..., None] * Rv[None, :, :]
+         - ((1 - U) * (1 - V))[..., None] * P00 - (U * (1 - V))[..., None] * P10
+         - ((1 - U) * V)[..., None] * P01 - (U * V)[..., None] * P11)
+    # exact boundary positions
+    C[:, 0] = Bu
+    C[:, -1] = Tu
+    C[0, :] = Lv
+    C[-1, :] = Rv
+    blocks = [C]
+    # rings: axis 0 tangential (clockwise), axis 1 radial outward
+    w = np.linspace(0.0, 1.0, m)
+    sides = [
+        (315.0, lambda t: np.stack([half - 2 * half * t, -half * np.ones_like(t)], -1)),  # bottom
+        (225.0, lambda t: np.stack([-half * np.ones_like(t), -half + 2 * half * t], -1)),  # left
+        (135.0, lambda t: np.stack([-half + 2 * half * t, half * np.ones_like(t)], -1)),  # top
+        (45.0, lambda t: np.stack([half * np.ones_like(t), half - 2 * half * t], -1)),  # right
+    ]
+    for a0, side in sides:
+        inner = circ(_ang(a0 - 90.0 * u))
+        outer = side(u)
+        P = (1 - w)[None, :, None] * inner[:, None, :] + w[None, :, None] * outer[:, None, :]
+        blocks.append(P)
+    ids, X = _merge(blocks, 2)
+    free = np.ones(X.shape[0], bool)
+    free[np.isclose(np.abs(X[:, 0]), half) | np.isclose(np.abs(X[:, 1]), half)] = False
+    return Grid(2, ids, X, free)
+
+
+def local_edge_length(grid: Grid) -> np.ndarray:
+    """Shortest incident block edge per node."""
+    h = np.full(grid.n_nodes, np.inf)
+    for ids in grid.blocks:
+        ids = np.asarray(ids)
+        for ax in range(grid.d):
+            a = np.take(ids, np.arange(ids.shape[ax] - 1), axis=ax).reshape(-1)
+            b = np.take(ids, np.arange(1, ids.shape[ax]), axis=ax).reshape(-1)
+            L = np.linalg.norm(grid.X[a] - grid.X[b], axis=1)
+            np.minimum.at(h, a, L)
+            np.minimum.at(h, b, L)
+    return h
+
+
+def tangle(grid: Grid, *, frac: float = 0.02, factor: float = 1.5, seed: int = 0) -> Grid:
+    """Displace ``frac`` of the free nodes by ``factor`` x local edge length in
+    a seeded random direction (re-drawn until some det A <= 0)."""
+    rng = np.random.default_rng(seed)
+    free_ids = np.flatnonzero(grid.free)
+    h = local_edge_length(grid)
+    for _ in range(100):
+        pick = rng.choice(free_ids, size=max(1, int(round(frac * free_ids.size))), replace=False)
+        dirs = rng.normal(size=(pick.size, grid.d))
+        dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+        X = grid.X.copy()
+        X[pick] += factor * h[pick, None] * dirs
+        if corner_dets(grid.blocks, X, grid.d).min() <= 0:
+            return Grid(grid.d, grid.blocks, X, grid.free.copy())
+    raise RuntimeError("tangling did not invert any cell")

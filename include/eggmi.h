@@ -1,0 +1,164 @@
+/*
+ * eggmi — C ABI of the MI355X structured TMOP smoother / untangler.
+ *
+ * Drop-in compute core for the reference's pybind module egg._cpp.cpp_core
+ * (reference src/bindings.cpp:1541-2130, stub egg/_cpp/cpp_core.pyi:27-68).
+ * Every pointer is borrowed for the duration of the call and owned by the
+ * caller; the library copies what it keeps. Functions return an eggmi_status;
+ * on failure eggmi_last_error() returns the message of the calling thread's
+ * last error. A session is not thread-safe (the reference holds the GIL for
+ * every call, src/bindings.cpp has no gil_scoped_release).
+ */
+#ifndef EGGMI_H
+#define EGGMI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    EGGMI_OK = 0,
+    EGGMI_INVALID_ARGUMENT = 1, /* reference: std::invalid_argument -> ValueError */
+    EGGMI_NO_DEVICE = 2,        /* reference: select_queue runtime_error -> RuntimeError */
+    EGGMI_INTERNAL = 3          /* reference: std::logic_error / device failure */
+} eggmi_status;
+
+typedef enum { EGGMI_PHASE_BARRIER = 0, EGGMI_PHASE_UNTANGLE = 1, EGGMI_PHASE_SHAPE_SIZE = 2 } eggmi_phase;
+
+typedef struct eggmi_session eggmi_session;
+
+/* One sweep group of the flat wire (reference egg/smoothing/flat_context.py:
+ * 283-300, unpacked at src/bindings.cpp:334-490). Global node ids. */
+typedef struct {
+    int64_t ndof;                     /* "D" */
+    const int32_t* dof_idx;           /* [ndof] */
+    const int32_t* P_of;              /* [ndof] stored occurrences per DOF */
+    const int32_t* interior_block;    /* [ndof] block of a synthesized interior DOF, -1 = stored */
+    const int32_t* interior_logical;  /* [ndof*dim] */
+    int64_t total_samples;            /* sum(P_of) */
+    const int32_t* role;              /* [total] -1 absent, 0 corner, 1..dim neighbour */
+    const int32_t* gc;                /* [total] */
+    const int32_t* gn[3];             /* [total] each, gn[0..dim-1] */
+    const double* s[3];               /* [total] each, +-1 */
+    const double* W_inv;              /* [w_rows*dim*dim] */
+    int64_t w_rows;                   /* 1 (uniform target) or total */
+} eggmi_group;
+
+/* The energy stencil (every (cell, corner) sample). */
+typedef struct {
+    int64_t num_samples;
+    const int32_t* gc;
+    const int32_t* gn[3];
+    const double* s[3];
+    const double* W_inv;
+    int64_t w_rows; /* 1 or num_samples */
+} eggmi_stencil;
+
+/* Halo-padded structured tables (reference egg/smoothing/cpp_backend.py:
+ * 679-691 structured_arrays). */
+typedef struct {
+    int32_t num_blocks;
+    const int32_t* interior_shapes;          /* [num_blocks*dim] node counts */
+    const int32_t* const* block_global_dof;  /* [num_blocks] -> [prod(shape)] row-major */
+    int64_t n_halo;
+    const int32_t* halo_src_block;   /* [n_halo] */
+    const int32_t* halo_src_padded;  /* [n_halo*dim] */
+    const int32_t* halo_dst_block;
+    const int32_t* halo_dst_padded;
+    int64_t n_sing;
+    const int32_t* sing_block;   /* [n_sing] */
+    const int32_t* sing_logical; /* [n_sing*dim] */
+} eggmi_structured;
+
+/* Build a device-resident session: validate the wire
+ * (src/bindings.cpp:98-253), re-home it onto the halo-padded store
+ * (src/bindings.cpp:758-1101) and upload once.
+ * device: 0 = "auto", 1 = "gpu", 2 = "cpu" (no CPU backend: EGGMI_NO_DEVICE),
+ * 3 = tables only (validation + remap on the host, no device state: only
+ * eggmi_session_info / eggmi_session_tables / eggmi_mg_levels may follow).
+ * Replaces make_structured_session (src/bindings.cpp:1153-1259). */
+eggmi_status eggmi_session_create(int dim, const eggmi_group* groups, int n_groups, const eggmi_stencil* stencil,
+                                  const eggmi_structured* structured, const double* X, int64_t num_nodes,
+                                  int device, eggmi_session** out);
+
+eggmi_status eggmi_session_destroy(eggmi_session* s);
+
+/* n_sweeps of block-Jacobi; writes ceil(n/k) (k = report_every <= 0 ? n : k)
+ * (energy, min det) reports. Replaces CppStructuredSweepSession::run
+ * (src/bindings.cpp:1293-1307) -> StructuredExecutorT::run_jacobi
+ * (src/structured_sweep.hpp:87-104). */
+eggmi_status eggmi_run(eggmi_session* s, int n_sweeps, int phase, double delta, double omega, int report_every,
+                       double* energies, double* mindets, int64_t capacity, int64_t* n_reports);
+
+/* FAS V-cycles. Replaces CppStructuredSweepSession::run_fas
+ * (src/bindings.cpp:1311-1335) -> StructuredExecutorT::run_fas
+ * (src/structured_sweep.hpp:114-149) -> egg::run_fas (src/fas.hpp:398-602). */
+eggmi_status eggmi_run_fas(eggmi_session* s, int n_cycles, int nu_pre, int nu_post, int nu_coarse, double omega,
+                           int max_levels, int phase, double* energies, double* mindets, int64_t capacity,
+                           int64_t* n_reports);
+
+/* Coarse hierarchy shapes (src/bindings.cpp:1340-1356): shapes_out receives
+ * level-major, block-major dim-tuples for levels 1..n (fine level
+ * excluded); *n_levels the number of coarse levels, *n_blocks per level. */
+eggmi_status eggmi_mg_levels(eggmi_session* s, int max_levels, int32_t* shapes_out, int64_t capacity,
+                             int32_t* n_levels, int32_t* n_blocks);
+
+/* X gathered back to global node order (src/bindings.cpp:1359-1376). */
+eggmi_status eggmi_get_X(eggmi_session* s, double* X_out, int64_t n);
+
+/* Replace the resident positions (global order) — checkpoint resume. */
+eggmi_status eggmi_set_X(eggmi_session* s, const double* X, int64_t n);
+
+/* Session introspection for index-map parity (no reference counterpart;
+ * the reference keeps these tables internal to StructuredRemap):
+ * slot counts and the copy tables in node units (reference real offsets / dim). */
+typedef struct {
+    int64_t n_slots;      /* padded nodes + overflow */
+    int64_t n_overflow;
+    int64_t n_halo;       /* face-ghost entries */
+    int64_t n_fan;        /* spare-ghost / overflow mirrors */
+    int64_t n_share;      /* owner -> copy broadcasts */
+    int64_t n_tiles;
+    int64_t n_stored;     /* stored-stencil DOFs */
+    int64_t n_synth;      /* synthesized interior DOFs */
+    int32_t structured_energy; /* 1: energy stencil reduced from the block cells */
+} eggmi_info;
+
+eggmi_status eggmi_session_info(eggmi_session* s, eggmi_info* info);
+
+/* copy tables (halo | fan | share order), g2s, block node bases */
+eggmi_status eggmi_session_tables(eggmi_session* s, int32_t* copy_src, int32_t* copy_dst, int32_t* g2s,
+                                  int64_t* block_base);
+
+/* The packed device store, coordinate-major ([dim][n_slots], FP32). */
+eggmi_status eggmi_get_packed(eggmi_session* s, float* out, int64_t n);
+
+/* One fused halo + fan + share copy on the resident store
+ * (src/structured_halo.hpp:219-237 fused_halo_broadcast). */
+eggmi_status eggmi_halo_exchange(eggmi_session* s);
+
+/* Select the sweep tile shape (autotuner hook); returns the variant count
+ * in *n_variants when variant < 0. */
+eggmi_status eggmi_set_tile_variant(eggmi_session* s, int variant, int* n_variants);
+
+/* Device time of the last eggmi_run / eggmi_run_fas call in ms (HIP events
+ * on the session stream), the number of sweep-kernel launches it made and,
+ * when kernel timing is on, the summed duration of those launches. */
+eggmi_status eggmi_last_timing(eggmi_session* s, double* ms, int64_t* sweep_launches, double* sweep_kernel_ms);
+
+/* Bracket every sweep launch with HIP events (measurement only). */
+eggmi_status eggmi_set_kernel_timing(eggmi_session* s, int on);
+
+const char* eggmi_last_error(void);
+
+/* Library liveness (reference cpp_core.ping, src/bindings.cpp:1544). */
+int eggmi_ping(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EGGMI_H */

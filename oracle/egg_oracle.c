@@ -1,0 +1,559 @@
+/*
+ * CPU oracle for the structured TMOP block-Jacobi smoother.
+ *
+ * TEST INFRASTRUCTURE ONLY. This file is the checker the HIP path is compared
+ * against; nothing in the shipped library links, loads or calls it. Only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it.
+ *
+ * Plain C restatement of the reference algorithm, compiled twice: REAL=double
+ * (symbols *_f64, the reference's default CPU build) and REAL=float (*_f32).
+ * Everything works in GLOBAL node-id space: halo, share and fan copies of the
+ * reference are pure data movement (every copy holds its owner's value when a
+ * sweep reads it), so the semantics of one sweep are "every free DOF reads the
+ * previous snapshot of the global X and writes its own node".
+ *
+ * Functions and the reference code they restate (paths relative to the
+ * reference repository root):
+ *   o_metric2d    src/metric.hpp:38-102 (mu2), :133-142 (untangle), :150-197
+ *                 (shape_size); NumPy twin egg/smoothing/batch.py:62-148
+ *   o_metric3d    src/metric.hpp:203-243 (mu_cond3, cond3_size, untangle3)
+ *   o_patch_eval  src/patch.hpp:84-117 (assemble_vecT), :211-229 (role_Jb),
+ *                 :237-320 (accumulate_sample), :325-346 (patch_eval)
+ *   o_trial       src/patch.hpp:176-202 (trial_energy_mindet)
+ *   o_solve       src/solve.hpp:32-77 (solveNxN<2>, <3>)
+ *   o_energy_mindet  src/sweep.hpp:1881-1922 (reduce_energy_mindet); the sum
+ *                 and min are accumulated in double (FP64 reductions)
+ *   o_jacobi      src/sweep.hpp:1425-1650 (metric + interior update kernels),
+ *                 :1938-2099 (jacobi_sweep_once / run_block_jacobi)
+ *
+ * Pinning: golden vectors from tests/cpp/golden_metric.hpp,
+ * golden_metric_3d.hpp and golden_patch.hpp (tests/golden/*.json), finite
+ * differences, the delta->0 limit of the untangle surrogate, the analytic
+ * minimiser of a Cartesian block, and brute force on tiny inputs (tests/).
+ */
+#include <math.h>
+#include <stddef.h>
+#include <stdint.h>
+#include <string.h>
+
+#ifndef REAL
+#define REAL double
+#endif
+#ifndef SFX
+#define SFX _f64
+#endif
+#define CAT2(a, b) a##b
+#define CAT(a, b) CAT2(a, b)
+#define FN(name) CAT(name, SFX)
+
+#undef real
+#define real REAL
+
+/* Per-precision tolerances (reference src/core.hpp:37-51). */
+#define IS_F32 (sizeof(real) == 4)
+#define TOL_ENERGY ((real)(IS_F32 ? 1e-6 : 1e-12))
+#define TOL_ZNORM ((real)(IS_F32 ? 1e-7 : 1e-15))
+#define MAX_BACKTRACKS 10 /* reference src/sweep.hpp:43 */
+
+#ifndef EGG_ORACLE_KINDS
+#define EGG_ORACLE_KINDS
+enum { KIND_SHAPE = 0, KIND_UNTANGLE = 1, KIND_SHAPE_SIZE = 2 };
+#endif
+
+/* ------------------------------------------------------------------ 2D metric */
+
+/* d vec(cof)/d vec(T), cof = [d, -c, -b, a]. */
+static const real FN(kP2)[16] = {0, 0, 0, 1, 0, 0, -1, 0, 0, -1, 0, 0, 1, 0, 0, 0};
+
+static real FN(untangle_dh)(real D, real delta, real* dh1, real* dh2)
+{
+    /* Dh = (D + sqrt(D^2 + 4 delta^2)) / 2; Dh' = Dh / r; Dh'' = 2 delta^2 / r^3 */
+    const real r = sqrt(D * D + (real)4 * delta * delta);
+    const real Dh = (real)0.5 * (D + r);
+    if (dh1) *dh1 = Dh / r;
+    if (dh2) *dh2 = (real)2 * delta * delta / (r * r * r);
+    return Dh;
+}
+
+/* value, grad (4) and Hessian (16, row-major) of the 2D metric at vec(T). */
+void FN(o_metric2d)(int kind, const real* t, real delta, real* val, real* g, real* H)
+{
+    const real a = t[0], b = t[1], c = t[2], d = t[3];
+    const real D = a * d - b * c;
+    const real s = a * a + b * b + c * c + d * d;
+    const real cof[4] = {d, -c, -b, a};
+    if (kind == KIND_SHAPE_SIZE) {
+        /* mu = (s/2D)^2 - 1 + (D-1)^2 */
+        const real cond = s / ((real)2 * D);
+        if (val) *val = cond * cond - (real)1 + (D - (real)1) * (D - (real)1);
+        /* cond' = t/D - s cof/(2 D^2) */
+        real cg[4];
+        for (int i = 0; i < 4; ++i) cg[i] = t[i] / D - s * cof[i] / ((real)2 * D * D);
+        if (g)
+            for (int i = 0; i < 4; ++i) g[i] = (real)2 * cond * cg[i] + (real)2 * (D - (real)1) * cof[i];
+        if (H) {
+            for (int i = 0; i < 4; ++i)
+                for (int j = 0; j < 4; ++j) {
+                    const real eye = (i == j) ? (real)1 : (real)0;
+                    const real ch = eye / D - (t[i] * cof[j] + cof[i] * t[j]) / (D * D) +
+                                    s * cof[i] * cof[j] / (D * D * D) -
+                                    s * FN(kP2)[i * 4 + j] / ((real)2 * D * D);
+                    H[i * 4 + j] = (real)2 * cg[i] * cg[j] + (real)2 * cond * ch +
+                                   (real)2 * cof[i] * cof[j] +
+                                   (real)2 * (D - (real)1) * FN(kP2)[i * 4 + j];
+                }
+        }
+        return;
+    }
+    /* shape (phi = D) and untangle (phi = Dh(D)): mu = s / (2 phi) - 1 */
+    real p = D, p1 = (real)1, p2 = (real)0;
+    if (kind == KIND_UNTANGLE) p = FN(untangle_dh)(D, delta, &p1, &p2);
+    if (val) *val = s / ((real)2 * p) - (real)1;
+    if (g)
+        for (int i = 0; i < 4; ++i) g[i] = t[i] / p - s * p1 * cof[i] / ((real)2 * p * p);
+    if (H) {
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) {
+                const real eye = (i == j) ? (real)1 : (real)0;
+                H[i * 4 + j] = eye / p - p1 * (t[i] * cof[j] + cof[i] * t[j]) / (p * p) +
+                               s * p1 * p1 * cof[i] * cof[j] / (p * p * p) -
+                               s / ((real)2 * p * p) * (p2 * cof[i] * cof[j] + p1 * FN(kP2)[i * 4 + j]);
+            }
+    }
+}
+
+/* ------------------------------------------------------------------ 3D metric */
+
+static void FN(cof3)(const real* t, real* c)
+{
+    c[0] = t[4] * t[8] - t[5] * t[7];
+    c[1] = t[5] * t[6] - t[3] * t[8];
+    c[2] = t[3] * t[7] - t[4] * t[6];
+    c[3] = t[2] * t[7] - t[1] * t[8];
+    c[4] = t[0] * t[8] - t[2] * t[6];
+    c[5] = t[1] * t[6] - t[0] * t[7];
+    c[6] = t[1] * t[5] - t[2] * t[4];
+    c[7] = t[2] * t[3] - t[0] * t[5];
+    c[8] = t[0] * t[4] - t[1] * t[3];
+}
+
+/* Symmetric bilinear cofactor form: cof(T) = Q(T,T), Q(A,B)_pq =
+ * 1/2 eps_pkl eps_qmn (A_km B_ln + B_km A_ln) / 2 * 2 -> written out as the
+ * mixed 2x2 minors. */
+static void FN(cofQ)(const real* A, const real* B, real* Q)
+{
+    static const int r1[3] = {1, 2, 0}, r2[3] = {2, 0, 1};
+    for (int p = 0; p < 3; ++p)
+        for (int q = 0; q < 3; ++q) {
+            const int k = r1[p], l = r2[p], m = r1[q], n = r2[q];
+            Q[p * 3 + q] = (real)0.5 * (A[k * 3 + m] * B[l * 3 + n] - A[k * 3 + n] * B[l * 3 + m] +
+                                        B[k * 3 + m] * A[l * 3 + n] - B[k * 3 + n] * A[l * 3 + m]);
+        }
+}
+
+static real FN(dot9)(const real* a, const real* b)
+{
+    real s = 0;
+    for (int i = 0; i < 9; ++i) s += a[i] * b[i];
+    return s;
+}
+
+/* value, grad (9) and Hessian (81) of mu_cond3 (kind 0), its untangle
+ * surrogate (kind 1) and cond3 + (det-1)^2 (kind 2). Derivatives are taken
+ * as second directional derivatives along unit directions e_a, e_b:
+ *   f = s q / (9 phi(D)^2), s = |T|^2, q = |cof T|^2, D = det T. */
+void FN(o_metric3d)(int kind, const real* t, real delta, real* val, real* g, real* H)
+{
+    real cof[9];
+    FN(cof3)(t, cof);
+    const real s = FN(dot9)(t, t);
+    const real q = FN(dot9)(cof, cof);
+    const real D = t[0] * cof[0] + t[1] * cof[1] + t[2] * cof[2];
+    real p = D, p1 = (real)1, p2 = (real)0;
+    if (kind == KIND_UNTANGLE) p = FN(untangle_dh)(D, delta, &p1, &p2);
+    const real h = p * p;
+    const real gq = s * q;
+    if (val) {
+        *val = gq / ((real)9 * h) - (real)1;
+        if (kind == KIND_SHAPE_SIZE) *val += (D - (real)1) * (D - (real)1);
+    }
+    if (!g && !H) return;
+    /* first derivatives along e_a */
+    real QT[9][9]; /* Q(T, e_a) */
+    real s1[9], D1[9], q1[9], h1[9], g1[9];
+    for (int a = 0; a < 9; ++a) {
+        real E[9] = {0};
+        E[a] = 1;
+        FN(cofQ)(t, E, QT[a]);
+        s1[a] = (real)2 * t[a];
+        D1[a] = cof[a];
+        q1[a] = (real)4 * FN(dot9)(cof, QT[a]);
+        h1[a] = (real)2 * p * p1 * D1[a];
+        g1[a] = s1[a] * q + s * q1[a];
+    }
+    if (g)
+        for (int a = 0; a < 9; ++a) {
+            g[a] = (g1[a] * h - gq * h1[a]) / ((real)9 * h * h);
+            if (kind == KIND_SHAPE_SIZE) g[a] += (real)2 * (D - (real)1) * D1[a];
+        }
+    if (H) {
+        for (int a = 0; a < 9; ++a)
+            for (int b = 0; b < 9; ++b) {
+                real Ea[9] = {0}, Eb[9] = {0}, Qab[9];
+                Ea[a] = 1;
+                Eb[b] = 1;
+                FN(cofQ)(Ea, Eb, Qab);
+                const real s2 = (a == b) ? (real)2 : (real)0;
+                const real D2 = (real)2 * FN(dot9)(QT[a], Eb);
+                const real q2 = (real)8 * FN(dot9)(QT[a], QT[b]) + (real)4 * FN(dot9)(cof, Qab);
+                const real h2 = (real)2 * (p1 * p1 + p * p2) * D1[a] * D1[b] + (real)2 * p * p1 * D2;
+                const real g2 = s2 * q + s1[a] * q1[b] + s1[b] * q1[a] + s * q2;
+                real v = (g2 / h - (g1[a] * h1[b] + g1[b] * h1[a]) / (h * h) - gq * h2 / (h * h) +
+                          (real)2 * gq * h1[a] * h1[b] / (h * h * h)) /
+                         (real)9;
+                if (kind == KIND_SHAPE_SIZE) v += (real)2 * D1[a] * D1[b] + (real)2 * (D - (real)1) * D2;
+                H[a * 9 + b] = v;
+            }
+    }
+}
+
+static void FN(metric_any)(int dim, int kind, const real* t, real delta, real* val, real* g, real* H)
+{
+    if (dim == 2)
+        FN(o_metric2d)(kind, t, delta, val, g, H);
+    else
+        FN(o_metric3d)(kind, t, delta, val, g, H);
+}
+
+/* ------------------------------------------------------------------ patch */
+
+static real FN(det_n)(int dim, const real* A)
+{
+    if (dim == 2) return A[0] * A[3] - A[1] * A[2];
+    return A[0] * (A[4] * A[8] - A[5] * A[7]) - A[1] * (A[3] * A[8] - A[5] * A[6]) +
+           A[2] * (A[3] * A[7] - A[4] * A[6]);
+}
+
+/* A[:,k] = s_k (nbr_k - corner); T = A W (reference src/patch.hpp:95-117). */
+static void FN(assemble)(int dim, const real* X, int gc, const int* gn, const real* s, const real* W,
+                     real* T, real* detA)
+{
+    real A[9];
+    for (int i = 0; i < dim; ++i)
+        for (int k = 0; k < dim; ++k)
+            A[i * dim + k] = s[k] * (X[(size_t)gn[k] * dim + i] - X[(size_t)gc * dim + i]);
+    *detA = FN(det_n)(dim, A);
+    for (int i = 0; i < dim; ++i)
+        for (int k = 0; k < dim; ++k) {
+            real acc = 0;
+            for (int j = 0; j < dim; ++j) acc += A[i * dim + j] * W[j * dim + k];
+            T[i * dim + k] = acc;
+        }
+}
+
+/* Occurrence arrays for one DOF patch: P occurrences, each with corner gc,
+ * neighbours gn[k*P + p], signs s[k*P + p], role, and a W_inv row (w_stride 0
+ * = one uniform row). */
+typedef struct {
+    int P;
+    const int* gc;
+    const int* gn; /* [dim][P] */
+    const real* s; /* [dim][P] */
+    const int* role;
+    const real* W;
+    int w_stride;
+} FN(patch_t);
+
+static void FN(occ_fetch)(int dim, const FN(patch_t)* pv, int p, int* gn, real* s)
+{
+    for (int k = 0; k < dim; ++k) {
+        gn[k] = pv->gn[(size_t)k * pv->P + p];
+        s[k] = pv->s[(size_t)k * pv->P + p];
+    }
+}
+
+/* Full patch evaluation: gradient, contracted Hessian, energy and min det A
+ * over the patch (reference src/patch.hpp:237-346). */
+static void FN(patch_eval_impl)(int dim, int kind, real delta, const FN(patch_t)* pv, const real* X, real* grad,
+                            real* hess, real* energy, real* mindet)
+{
+    const int VT = dim * dim;
+    for (int i = 0; i < dim; ++i) grad[i] = 0;
+    for (int i = 0; i < dim * dim; ++i) hess[i] = 0;
+    *energy = 0;
+    *mindet = INFINITY;
+    for (int p = 0; p < pv->P; ++p) {
+        int gn[3];
+        real s[3], T[9], detA, val, g[9], H[81];
+        FN(occ_fetch)(dim, pv, p, gn, s);
+        const real* W = pv->W + (size_t)pv->w_stride * p;
+        FN(assemble)(dim, X, pv->gc[p], gn, s, W, T, &detA);
+        if (detA < *mindet) *mindet = detA;
+        const int role = pv->role[p];
+        FN(metric_any)(dim, kind, T, delta, &val, g, role >= 0 ? H : NULL);
+        *energy += val;
+        /* dmu/dA[i][k] = sum_j g[i,j] W[k,j] */
+        real dA[3][3];
+        for (int i = 0; i < dim; ++i)
+            for (int k = 0; k < dim; ++k) {
+                real acc = 0;
+                for (int j = 0; j < dim; ++j) acc += g[i * dim + j] * W[k * dim + j];
+                dA[i][k] = acc;
+            }
+        if (role == 0) {
+            for (int i = 0; i < dim; ++i) {
+                real acc = 0;
+                for (int k = 0; k < dim; ++k) acc += s[k] * dA[i][k];
+                grad[i] += -acc;
+            }
+        } else if (role >= 1) {
+            const int ax = role - 1;
+            for (int i = 0; i < dim; ++i) grad[i] += s[ax] * dA[i][ax];
+        }
+        if (role >= 0) {
+            /* Jb (VT x dim): row a=(i,c) has its only nonzero in column i */
+            real u[3];
+            for (int c = 0; c < dim; ++c) {
+                if (role == 0) {
+                    real acc = 0;
+                    for (int k = 0; k < dim; ++k) acc += s[k] * W[k * dim + c];
+                    u[c] = -acc;
+                } else {
+                    const int ax = role - 1;
+                    u[c] = s[ax] * W[ax * dim + c];
+                }
+            }
+            real Jb[81];
+            for (int e = 0; e < VT * dim; ++e) Jb[e] = 0;
+            for (int a = 0; a < VT; ++a) Jb[a * dim + a / dim] = u[a % dim];
+            for (int i = 0; i < dim; ++i)
+                for (int j = 0; j < dim; ++j) {
+                    real acc = 0;
+                    for (int a = 0; a < VT; ++a) {
+                        real hj = 0;
+                        for (int b = 0; b < VT; ++b) hj += H[a * VT + b] * Jb[b * dim + j];
+                        acc += Jb[a * dim + i] * hj;
+                    }
+                    hess[i * dim + j] += acc;
+                }
+        }
+    }
+}
+
+void FN(o_patch_eval)(int dim, int kind, real delta, int P, const int* gc, const int* gn, const real* s,
+                      const int* role, const real* W, int w_stride, const real* X, real* grad, real* hess,
+                      real* energy, real* mindet)
+{
+    FN(patch_t) pv = {P, gc, gn, s, role, W, w_stride};
+    FN(patch_eval_impl)(dim, kind, delta, &pv, X, grad, hess, energy, mindet);
+}
+
+/* Trial energy + min det with node `dof` moved to `trial`
+ * (reference src/patch.hpp:176-202). */
+static void FN(trial_impl)(int dim, int kind, real delta, const FN(patch_t)* pv, const real* X, int dof,
+                       const real* trial, real* e_new, real* mdet)
+{
+    *e_new = 0;
+    *mdet = INFINITY;
+    for (int p = 0; p < pv->P; ++p) {
+        int gn[3];
+        real s[3];
+        FN(occ_fetch)(dim, pv, p, gn, s);
+        const real* W = pv->W + (size_t)pv->w_stride * p;
+        int ids[4] = {pv->gc[p], gn[0], gn[1], dim == 3 ? gn[2] : 0};
+        real pts[4][3];
+        for (int n = 0; n <= dim; ++n)
+            for (int i = 0; i < dim; ++i)
+                pts[n][i] = (ids[n] == dof) ? trial[i] : X[(size_t)ids[n] * dim + i];
+        real A[9], T[9];
+        for (int i = 0; i < dim; ++i)
+            for (int k = 0; k < dim; ++k) A[i * dim + k] = s[k] * (pts[k + 1][i] - pts[0][i]);
+        const real detA = FN(det_n)(dim, A);
+        for (int i = 0; i < dim; ++i)
+            for (int k = 0; k < dim; ++k) {
+                real acc = 0;
+                for (int j = 0; j < dim; ++j) acc += A[i * dim + j] * W[j * dim + k];
+                T[i * dim + k] = acc;
+            }
+        real val;
+        FN(metric_any)(dim, kind, T, delta, &val, NULL, NULL);
+        *e_new += val;
+        if (detA < *mdet) *mdet = detA;
+    }
+}
+
+/* Newton step x = -H^{-1} g with the fallbacks of reference src/solve.hpp:32-77:
+ * |g| < znorm -> 0; non-finite -> -0.1 g/|g|. */
+void FN(o_solve)(int dim, const real* H, const real* g, real* x)
+{
+    if (dim == 2) {
+        const real gnorm = sqrt(g[0] * g[0] + g[1] * g[1]);
+        if (gnorm < TOL_ZNORM) {
+            x[0] = x[1] = 0;
+            return;
+        }
+        const real det = H[0] * H[3] - H[1] * H[2];
+        const real inv = (real)1 / det;
+        x[0] = -inv * (H[3] * g[0] - H[1] * g[1]);
+        x[1] = -inv * (-H[2] * g[0] + H[0] * g[1]);
+        if (!isfinite(x[0]) || !isfinite(x[1])) {
+            const real c = (real)-0.1 / gnorm;
+            x[0] = c * g[0];
+            x[1] = c * g[1];
+        }
+        return;
+    }
+    const real gnorm = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+    if (gnorm < TOL_ZNORM) {
+        x[0] = x[1] = x[2] = 0;
+        return;
+    }
+    const real c00 = H[4] * H[8] - H[5] * H[7];
+    const real c01 = -(H[3] * H[8] - H[5] * H[6]);
+    const real c02 = H[3] * H[7] - H[4] * H[6];
+    const real det = H[0] * c00 + H[1] * c01 + H[2] * c02;
+    const real inv = (real)1 / det;
+    const real c10 = -(H[1] * H[8] - H[2] * H[7]);
+    const real c11 = H[0] * H[8] - H[2] * H[6];
+    const real c12 = -(H[0] * H[7] - H[1] * H[6]);
+    const real c20 = H[1] * H[5] - H[2] * H[4];
+    const real c21 = -(H[0] * H[5] - H[2] * H[3]);
+    const real c22 = H[0] * H[4] - H[1] * H[3];
+    x[0] = -inv * (c00 * g[0] + c10 * g[1] + c20 * g[2]);
+    x[1] = -inv * (c01 * g[0] + c11 * g[1] + c21 * g[2]);
+    x[2] = -inv * (c02 * g[0] + c12 * g[1] + c22 * g[2]);
+    if (!isfinite(x[0]) || !isfinite(x[1]) || !isfinite(x[2])) {
+        const real c = (real)-0.1 / gnorm;
+        for (int i = 0; i < 3; ++i) x[i] = c * g[i];
+    }
+}
+
+/* ------------------------------------------------------------ reductions */
+
+/* Sum of mu and min det A over every energy-stencil sample
+ * (reference src/sweep.hpp:1881-1922). mu and det are evaluated in double
+ * from the REAL positions and accumulated in double. */
+void FN(o_energy_mindet)(int dim, int kind, double delta, long long ns, const int* gc, const int* gn,
+                         const real* s, const real* W, int w_stride, const real* X, double* E, double* md)
+{
+    double e = 0.0, m = INFINITY;
+    for (long long i = 0; i < ns; ++i) {
+        double A[9], T[9];
+        const real* w = W + (size_t)w_stride * i;
+        for (int r = 0; r < dim; ++r)
+            for (int k = 0; k < dim; ++k) {
+                const int nk = gn[(size_t)k * ns + i];
+                A[r * dim + k] = (double)s[(size_t)k * ns + i] *
+                                 ((double)X[(size_t)nk * dim + r] - (double)X[(size_t)gc[i] * dim + r]);
+            }
+        for (int r = 0; r < dim; ++r)
+            for (int k = 0; k < dim; ++k) {
+                double acc = 0;
+                for (int j = 0; j < dim; ++j) acc += A[r * dim + j] * (double)w[j * dim + k];
+                T[r * dim + k] = acc;
+            }
+        double val, det;
+        if (dim == 2) {
+            det = A[0] * A[3] - A[1] * A[2];
+            o_metric2d_f64(kind, T, delta, &val, NULL, NULL);
+        } else {
+            det = A[0] * (A[4] * A[8] - A[5] * A[7]) - A[1] * (A[3] * A[8] - A[5] * A[6]) +
+                  A[2] * (A[3] * A[7] - A[4] * A[6]);
+            o_metric3d_f64(kind, T, delta, &val, NULL, NULL);
+        }
+        e += val;
+        if (det < m) m = det;
+    }
+    *E = e;
+    *md = m;
+}
+
+/* ------------------------------------------------------------ Jacobi sweep */
+
+/* One DOF's block-Jacobi update: metric pass, Newton, <=10 halvings, omega
+ * post-blend (reference src/sweep.hpp:1534-1650). X is the frozen snapshot. */
+static void FN(update_dof)(int dim, int kind, real delta, real omega, const FN(patch_t)* pv, const real* X, int dof,
+                       const real* tau, real* out)
+{
+    real grad[3], hess[9], e0, md;
+    FN(patch_eval_impl)(dim, kind, delta, pv, X, grad, hess, &e0, &md);
+    real pos[3];
+    for (int k = 0; k < dim; ++k) pos[k] = X[(size_t)dof * dim + k];
+    if (tau) {
+        for (int k = 0; k < dim; ++k) {
+            grad[k] -= tau[(size_t)dof * dim + k];
+            e0 -= tau[(size_t)dof * dim + k] * pos[k];
+        }
+    }
+    real dlt[3];
+    FN(o_solve)(dim, hess, grad, dlt);
+    real alpha = 1, cur[3];
+    for (int k = 0; k < dim; ++k) cur[k] = pos[k];
+    for (int it = 0; it < MAX_BACKTRACKS; ++it) {
+        real trial[3], e_new, mdet;
+        for (int k = 0; k < dim; ++k) trial[k] = pos[k] + alpha * dlt[k];
+        FN(trial_impl)(dim, kind, delta, pv, X, dof, trial, &e_new, &mdet);
+        if (tau)
+            for (int k = 0; k < dim; ++k) e_new -= tau[(size_t)dof * dim + k] * trial[k];
+        const int ok = isfinite(e_new) && (e_new <= e0 + TOL_ENERGY) && (kind == KIND_UNTANGLE || mdet > 0);
+        if (ok) {
+            for (int k = 0; k < dim; ++k) cur[k] = trial[k];
+            break;
+        }
+        alpha *= (real)0.5;
+    }
+    if (omega != (real)1)
+        for (int k = 0; k < dim; ++k) cur[k] = pos[k] + omega * (cur[k] - pos[k]);
+    for (int k = 0; k < dim; ++k) out[k] = cur[k];
+}
+
+/*
+ * n_sweeps of double-buffered block-Jacobi over the free DOFs `dof[ndof]`
+ * (global ids). DOF d owns occurrences [poff[d], poff[d+1]) of the patch
+ * arrays (pgn/ps are [dim][total]). X (n_nodes*dim) is updated in place with
+ * the final state. Reports (E, min det) on sweeps (s+1) % k == 0 and the
+ * last one, with k = report_every <= 0 ? n_sweeps : report_every.
+ * `tau` (nullable, n_nodes*dim) is the FAS linear shift. Returns the number
+ * of reports written.
+ */
+int FN(o_jacobi)(int dim, int kind, double delta, double omega, int n_sweeps, int report_every, int ndof,
+                 const int* dof, const long long* poff, const int* pgc, const int* pgn, const real* ps,
+                 const int* prole, const real* W, int w_stride, int n_nodes, real* X, const real* tau,
+                 long long ns, const int* egc, const int* egn, const real* es, const real* eW, int ew_stride,
+                 real* work, double* energies, double* mindets)
+{
+    const long long total = poff[ndof];
+    const int k = report_every <= 0 ? n_sweeps : report_every;
+    int nrep = 0;
+    real* xin = X;
+    real* xout = work; /* n_nodes*dim */
+    memcpy(xout, xin, sizeof(real) * (size_t)n_nodes * dim);
+    for (int sw = 0; sw < n_sweeps; ++sw) {
+#pragma omp parallel for schedule(static)
+        for (int d = 0; d < ndof; ++d) {
+            const long long o = poff[d];
+            const int P = (int)(poff[d + 1] - o);
+            /* per-DOF view into the [dim][total] arrays: gn/s rows have stride total */
+            int gn_l[3 * 128];
+            real s_l[3 * 128];
+            for (int kk = 0; kk < dim; ++kk)
+                for (int p = 0; p < P; ++p) {
+                    gn_l[kk * P + p] = pgn[(size_t)kk * total + o + p];
+                    s_l[kk * P + p] = ps[(size_t)kk * total + o + p];
+                }
+            FN(patch_t) pv = {P, pgc + o, gn_l, s_l, prole + o, W + (size_t)w_stride * o, w_stride};
+            FN(update_dof)(dim, kind, (real)delta, (real)omega, &pv, xin, dof[d], tau,
+                       xout + (size_t)dof[d] * dim);
+        }
+        if ((sw + 1) % k == 0 || sw == n_sweeps - 1) {
+            FN(o_energy_mindet)(dim, kind, delta, ns, egc, egn, es, eW, ew_stride, xout, &energies[nrep],
+                                &mindets[nrep]);
+            ++nrep;
+        }
+        real* t = xin;
+        xin = xout;
+        xout = t;
+    }
+    if (xin != X) memcpy(X, xin, sizeof(real) * (size_t)n_nodes * dim);
+    return nrep;
+}

@@ -1,0 +1,250 @@
+"""ctypes front-end of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Loads ``oracle/liboracle.so`` (built from ``oracle/egg_oracle.c`` by
+``oracle/Makefile``). Only tests/, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` import this module; the product path in
+``egg_amd`` never does.
+
+Everything is in GLOBAL node-id space (see the header of egg_oracle.c). The
+per-DOF patch of an interior DOF is materialised here in the reference's
+occurrence order (src/structured_patch.hpp:89-130: cell bits outer, corner
+bits inner, axis 0 most significant), so the C sweep only has a stored path.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from itertools import product
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+_LIB = None
+
+KIND = {"barrier": 0, "shape": 0, "untangle": 1, "shape_size": 2}
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+
+
+def build() -> Path:
+    """Compile the oracle library with make (gcc)."""
+    subprocess.run(["make", "-s", "-C", str(_HERE)], check=True)
+    return _HERE / "liboracle.so"
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        so = _HERE / "liboracle.so"
+        if not so.exists():
+            build()
+        L = C.CDLL(str(so))
+        for sfx, rp in (("_f64", _dp), ("_f32", _fp)):
+            rt = C.c_double if sfx == "_f64" else C.c_float
+            f = getattr(L, "o_metric2d" + sfx)
+            f.argtypes = [C.c_int, rp, rt, rp, rp, rp]
+            f.restype = None
+            f = getattr(L, "o_metric3d" + sfx)
+            f.argtypes = [C.c_int, rp, rt, rp, rp, rp]
+            f.restype = None
+            f = getattr(L, "o_patch_eval" + sfx)
+            f.argtypes = [C.c_int, C.c_int, rt, C.c_int, _ip, _ip, rp, _ip, rp, C.c_int, rp, rp, rp, rp, rp]
+            f.restype = None
+            f = getattr(L, "o_solve" + sfx)
+            f.argtypes = [C.c_int, rp, rp, rp]
+            f.restype = None
+            f = getattr(L, "o_energy_mindet" + sfx)
+            f.argtypes = [C.c_int, C.c_int, C.c_double, C.c_longlong, _ip, _ip, rp, rp, C.c_int, rp, _dp, _dp]
+            f.restype = None
+            f = getattr(L, "o_jacobi" + sfx)
+            f.argtypes = [
+                C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                _ip, _lp, _ip, _ip, rp, _ip, rp, C.c_int, C.c_int, rp, C.c_void_p,
+                C.c_longlong, _ip, _ip, rp, rp, C.c_int, rp, _dp, _dp,
+            ]
+            f.restype = C.c_int
+        L.o_set_threads.argtypes = [C.c_int]
+        L.o_set_threads.restype = None
+        _LIB = L
+    return _LIB
+
+
+def _dt(prec: str):
+    return np.float64 if prec == "f64" else np.float32
+
+
+def metric(dim: int, kind: str | int, t, delta: float = 0.0, prec: str = "f64"):
+    """(value, grad, Hessian) of the per-sample metric at vec(T)."""
+    dt = _dt(prec)
+    vt = dim * dim
+    k = KIND[kind] if isinstance(kind, str) else kind
+    t = np.ascontiguousarray(t, dtype=dt)
+    val = np.zeros(1, dt)
+    g = np.zeros(vt, dt)
+    H = np.zeros(vt * vt, dt)
+    fn = getattr(lib(), f"o_metric{dim}d_{prec}")
+    fn(k, t, dt(delta), val, g, H)
+    return float(val[0]), g, H.reshape(vt, vt)
+
+
+def patch_eval(dim, kind, delta, gc, gn, s, role, W, X, prec="f64"):
+    """(grad, hess, energy, mindet) of one patch. gn, s: (dim, P);
+    W: (P, dim*dim) per-occurrence rows or (dim*dim,) uniform."""
+    dt = _dt(prec)
+    P = len(gc)
+    W = np.ascontiguousarray(W, dtype=dt)
+    ws = 0 if W.ndim == 1 else dim * dim
+    grad = np.zeros(dim, dt)
+    hess = np.zeros(dim * dim, dt)
+    e = np.zeros(1, dt)
+    m = np.zeros(1, dt)
+    fn = getattr(lib(), f"o_patch_eval_{prec}")
+    fn(
+        dim, KIND[kind], dt(delta), P,
+        np.ascontiguousarray(gc, np.int32), np.ascontiguousarray(gn, np.int32).reshape(-1),
+        np.ascontiguousarray(s, dt).reshape(-1), np.ascontiguousarray(role, np.int32),
+        W.reshape(-1), ws, np.ascontiguousarray(X, dt).reshape(-1), grad, hess, e, m,
+    )
+    return grad, hess.reshape(dim, dim), float(e[0]), float(m[0])
+
+
+def solve(dim, H, g, prec="f64"):
+    dt = _dt(prec)
+    x = np.zeros(dim, dt)
+    getattr(lib(), f"o_solve_{prec}")(
+        dim, np.ascontiguousarray(H, dt).reshape(-1), np.ascontiguousarray(g, dt), x
+    )
+    return x
+
+
+def _stencil_arrays(es: dict, dim: int, dt):
+    ns = int(es["num_samples"])
+    gc = np.ascontiguousarray(es["gc"], np.int32)
+    gn = np.ascontiguousarray(np.stack([np.asarray(es[f"gn{k}"]) for k in range(dim)]), np.int32)
+    s = np.ascontiguousarray(np.stack([np.asarray(es[f"s{k}"]) for k in range(dim)]), dt)
+    W = np.ascontiguousarray(es["W_inv"], dt).reshape(-1)
+    ws = 0 if W.size == dim * dim else dim * dim
+    return ns, gc, gn.reshape(-1), s.reshape(-1), W, ws
+
+
+def energy_mindet(dim, kind, delta, es, X, prec="f64"):
+    """(sum mu, min det A) over the energy stencil, accumulated in double."""
+    dt = _dt(prec)
+    ns, gc, gn, s, W, ws = _stencil_arrays(es, dim, dt)
+    E = np.zeros(1)
+    m = np.zeros(1)
+    getattr(lib(), f"o_energy_mindet_{prec}")(
+        dim, KIND[kind], float(delta), ns, gc, gn, s, W, ws,
+        np.ascontiguousarray(X, dt).reshape(-1), E, m,
+    )
+    return float(E[0]), float(m[0])
+
+
+def interior_occurrences(dim: int, logical, idx):
+    """Occurrences of an interior DOF's patch in the reference order
+    (src/structured_patch.hpp:89-130). ``idx`` maps a logical tuple to a
+    node id. Returns lists gc, gn[dim], s[dim], role."""
+    nc = 1 << dim
+    gcs, gns, ss, roles = [], [[] for _ in range(dim)], [[] for _ in range(dim)], []
+    for occ in range(nc * nc):
+        cell, o = divmod(occ, nc)
+        obit = [(o >> (dim - 1 - k)) & 1 for k in range(dim)]
+        corner = [logical[k] - 1 + ((cell >> (dim - 1 - k)) & 1) + obit[k] for k in range(dim)]
+        gcs.append(idx(tuple(corner)))
+        role = 0 if list(corner) == list(logical) else -1
+        for k in range(dim):
+            nbr = list(corner)
+            nbr[k] += 1 if obit[k] == 0 else -1
+            gns[k].append(idx(tuple(nbr)))
+            ss[k].append(1.0 if obit[k] == 0 else -1.0)
+            if role < 0 and nbr == list(logical):
+                role = k + 1
+        roles.append(role)
+    return gcs, gns, ss, roles
+
+
+class Patches:
+    """Per-DOF patches in global id space (CSR over DOFs)."""
+
+    def __init__(self, dim, dof, poff, gc, gn, s, role, W):
+        self.dim = dim
+        self.dof = np.ascontiguousarray(dof, np.int32)
+        self.poff = np.ascontiguousarray(poff, np.int64)
+        self.gc = np.ascontiguousarray(gc, np.int32)
+        self.gn = np.ascontiguousarray(gn, np.int32)  # (dim, total)
+        self.s = np.ascontiguousarray(s, np.float64)  # (dim, total)
+        self.role = np.ascontiguousarray(role, np.int32)
+        self.W = np.ascontiguousarray(W, np.float64)  # (dd,) uniform or (total, dd)
+
+
+def materialize_patches(wire: dict, block_global_dof: list[np.ndarray], dim: int) -> Patches:
+    """Expand the wire's single group into explicit per-DOF patches: stored
+    occurrences for DOFs with P_of > 0, synthesized ones (reference order)
+    for interior DOFs (interior_block >= 0). Uniform W_inv only for the
+    synthesized case, as in the reference (identity)."""
+    g = wire["groups"][0]
+    dof = np.asarray(g["dof_idx"], np.int32)
+    P_of = np.asarray(g["P_of"], np.int64)
+    iblk = np.asarray(g["interior_block"], np.int32)
+    ilog = np.asarray(g["interior_logical"], np.int32).reshape(len(dof), dim)
+    soff = np.concatenate([[0], np.cumsum(P_of)])
+    gc_all, gn_all, s_all, role_all, poff = [], [[] for _ in range(dim)], [[] for _ in range(dim)], [], [0]
+    W = np.asarray(g["W_inv"], np.float64)
+    uniform = W.size == dim * dim
+    Wrows = []
+    for d in range(len(dof)):
+        if iblk[d] >= 0:
+            ids = block_global_dof[iblk[d]]
+            gcs, gns, ss, roles = interior_occurrences(dim, tuple(ilog[d]), lambda l: int(ids[l]))
+            if not uniform:
+                raise ValueError("interior synthesis requires a uniform W_inv")
+        else:
+            a, b = soff[d], soff[d + 1]
+            gcs = list(np.asarray(g["gc"])[a:b])
+            gns = [list(np.asarray(g[f"gn{k}"])[a:b]) for k in range(dim)]
+            ss = [list(np.asarray(g[f"s{k}"])[a:b]) for k in range(dim)]
+            roles = list(np.asarray(g["role"])[a:b])
+            if not uniform:
+                Wrows.append(W.reshape(-1, dim * dim)[a:b])
+        gc_all += gcs
+        for k in range(dim):
+            gn_all[k] += gns[k]
+            s_all[k] += ss[k]
+        role_all += roles
+        poff.append(poff[-1] + len(gcs))
+    Wp = W.reshape(-1) if uniform else np.concatenate(Wrows)
+    return Patches(dim, dof, poff, gc_all, gn_all, s_all, role_all, Wp)
+
+
+def jacobi(dim, kind, delta, omega, n_sweeps, X, patches: Patches, es: dict, *, report_every=0,
+           prec="f64", tau=None, threads=None):
+    """Run n_sweeps of block-Jacobi in global id space. Returns
+    (X_out, energies, mindets)."""
+    dt = _dt(prec)
+    if threads is not None:
+        lib().o_set_threads(int(threads))
+    Xw = np.ascontiguousarray(X, dt).reshape(-1).copy()
+    n_nodes = Xw.size // dim
+    work = np.zeros_like(Xw)
+    k = n_sweeps if report_every <= 0 else report_every
+    nrep = (n_sweeps + k - 1) // k
+    en = np.zeros(max(nrep, 1))
+    md = np.zeros(max(nrep, 1))
+    ns, egc, egn, es_, eW, ews = _stencil_arrays(es, dim, dt)
+    W = np.ascontiguousarray(patches.W, dt).reshape(-1)
+    ws = 0 if W.size == dim * dim else dim * dim
+    tau_arr = None if tau is None else np.ascontiguousarray(tau, dt).reshape(-1)
+    got = getattr(lib(), f"o_jacobi_{prec}")(
+        dim, KIND[kind], float(delta), float(omega), int(n_sweeps), int(report_every),
+        len(patches.dof), patches.dof, patches.poff, patches.gc, patches.gn.reshape(-1),
+        np.ascontiguousarray(patches.s, dt).reshape(-1), patches.role, W, ws, n_nodes, Xw,
+        None if tau_arr is None else tau_arr.ctypes.data_as(C.c_void_p),
+        ns, egc, egn, es_, eW, ews, work, en, md,
+    )
+    return Xw.reshape(-1, dim), en[:got], md[:got]

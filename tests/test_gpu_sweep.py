@@ -1,0 +1,196 @@
+"""HIP sweep parity on the MI355X through the C ABI (pybind cpp_core ->
+libeggmi.so), checked against the CPU oracle on the same inputs.
+
+Bar (FP32 device arithmetic, FP64 reductions): node positions within 1e-5
+of the domain extent of the FP64 oracle on valid meshes; reported energies
+and min det equal the FP64 reduction of the returned positions to 1e-9
+relative; halo copies and index maps bit-exact.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from tests.cases import Case, cube, lr_pair, ogrid, single
+
+pytestmark = pytest.mark.gpu
+
+POS_TOL = 1e-5  # x domain extent
+
+
+def _session(case, **kw):
+    from egg_amd.smoothing import CppStructuredSweepSession
+
+    return CppStructuredSweepSession(case.ctx, case.bsc, case.X, **kw)
+
+
+def _extent(case):
+    X = case.X
+    return float((X.max(0) - X.min(0)).max())
+
+
+def _run_both(oracle, case, n, *, phase="barrier", delta=0.0, omega=0.8, report_every=1, prec="f64"):
+    s = _session(case)
+    e, m = s.run(n, phase=phase, delta=delta, omega=omega, report_every=report_every)
+    Xg = s.get_X()
+    Xo, eo, mo = oracle.jacobi(case.d, phase, delta, omega, n, case.X, case.patches(oracle),
+                               case.ctx.energy_stencil, report_every=report_every, prec=prec)
+    return s, Xg, np.asarray(e), np.asarray(m), Xo, eo, mo
+
+
+@pytest.mark.parametrize("n,sweeps", [(33, 40), (129, 40)])
+def test_single_block_parity(oracle, n, sweeps):
+    """Config-1 shape (one block, U(+-0.3h) interior, omega 0.8): positions
+    per the bar against the FP64 oracle, energies per report."""
+    c = single(n=n, seed=1)
+    _, Xg, e, m, Xo, eo, mo = _run_both(oracle, c, sweeps)
+    err = np.abs(Xg - Xo).max() / _extent(c)
+    print(f"n={n} max|dX|/extent={err:.2e} E_gpu={e[-1]:.9g} E_f64={eo[-1]:.9g}")
+    assert err < POS_TOL
+    np.testing.assert_allclose(e[5:], eo[5:], rtol=1e-5)
+    np.testing.assert_allclose(m[5:], mo[5:], rtol=1e-4)
+
+
+def test_single_block_parity_fp32_oracle(oracle):
+    """Same algorithm, same precision: the FP32 oracle agrees even closer."""
+    c = single(n=33, seed=2)
+    _, Xg, e, m, Xo, eo, mo = _run_both(oracle, c, 40, prec="f32")
+    assert np.abs(Xg - Xo).max() / _extent(c) < POS_TOL
+    np.testing.assert_allclose(e[-1], eo[-1], rtol=1e-5)
+
+
+def test_reports_equal_fp64_reduction_of_returned_x(oracle):
+    """Reported (E, min det) are the FP64 reduction of the returned FP32 state."""
+    c = ogrid(33, 17)
+    s = _session(c)
+    e, m = s.run(7, omega=0.8, report_every=3)
+    assert len(e) == 3  # sweeps 3, 6 and the last one
+    X = s.get_X().astype(np.float32)
+    E, M = oracle.energy_mindet(2, "barrier", 0.0, c.ctx.energy_stencil, X, prec="f32")
+    assert e[-1] == pytest.approx(E, rel=1e-9) and m[-1] == pytest.approx(M, rel=1e-9)
+
+
+@pytest.mark.parametrize("make", [lambda: ogrid(33, 17), lambda: lr_pair(17)])
+def test_multiblock_parity(oracle, make):
+    """Multi-block (halo ghosts, owner broadcast, stored-stencil interface and
+    valence-3 DOFs) against the FP64 oracle in global id space."""
+    c = make()
+    _, Xg, e, m, Xo, eo, mo = _run_both(oracle, c, 30)
+    err = np.abs(Xg - Xo).max() / _extent(c)
+    print(f"multi-block max|dX|/extent={err:.2e}")
+    assert err < POS_TOL
+    np.testing.assert_allclose(e, eo, rtol=1e-5)
+
+
+def test_stored_path_and_sample_reduction(oracle):
+    """A non-isotropic uniform target ships every DOF with a stored patch and
+    forces the explicit-sample energy reduction."""
+    from egg_amd import grids
+
+    W = np.array([[1.0, 0.15], [0.0, 0.9]])
+    c = Case(grids.single_block_2d(17, seed=5, amp=0.2), w_inv=W)
+    s = _session(c)
+    info = s.core.info()
+    assert info["n_synth"] == 0 and info["n_stored"] == c.grid.n_free and info["structured_energy"] == 0
+    e, m = s.run(20, omega=0.8, report_every=1)
+    Xo, eo, mo = oracle.jacobi(2, "barrier", 0.0, 0.8, 20, c.X, c.patches(oracle), c.ctx.energy_stencil,
+                               report_every=1)
+    assert np.abs(s.get_X() - Xo).max() / _extent(c) < POS_TOL
+    np.testing.assert_allclose(e, eo, rtol=1e-5)
+
+
+def test_halo_exchange_bit_exact():
+    """After the fused copy every ghost / copy / mirror slot holds its source's
+    coordinates bit for bit, and every interior slot its node's FP32 value."""
+    c = ogrid(33, 17)
+    s = _session(c)
+    core = s.core
+    core.halo_exchange()
+    P = core.packed()
+    tab = core.tables()
+    src, dst = tab["copy_src"], tab["copy_dst"]
+    assert np.array_equal(P[:, dst].view(np.uint32), P[:, src].view(np.uint32))
+    g2s = tab["g2s"]
+    np.testing.assert_array_equal(P[:, g2s].T, c.X.astype(np.float32))
+    # after sweeps the invariant still holds (the copy runs after every sweep)
+    s.run(5, omega=0.8)
+    P = core.packed()
+    assert np.array_equal(P[:, dst].view(np.uint32), P[:, src].view(np.uint32))
+
+
+def test_tile_variants_agree():
+    """Every LDS tile shape computes the same nodes the same way."""
+    c = single(n=65, seed=3)
+    out = []
+    nv = _session(c).core.set_tile_variant(-1)
+    for v in range(nv):
+        s = _session(c)
+        s.core.set_tile_variant(v)
+        s.run(5, omega=0.8)
+        out.append(s.get_X())
+    for X in out[1:]:
+        np.testing.assert_allclose(X, out[0], rtol=0, atol=1e-6)
+
+
+def test_analytic_minimiser():
+    """Perturbed Cartesian block -> the uniform grid (reference
+    tests/smoothing/test_structured_sweep.py:159-174), at FP32 resolution."""
+    c = single(n=9, seed=11, amp=0.25)
+    uniform = np.stack(np.meshgrid(np.arange(9), np.arange(9), indexing="ij"), -1).reshape(-1, 2) / 8.0
+    s = _session(c)
+    _, m = s.run(3000, omega=1.0)
+    assert m[-1] > 0
+    np.testing.assert_allclose(s.get_X(), uniform, atol=2e-5)
+
+
+def test_untangle_ogrid(oracle):
+    """delta-continuation untangle clears a 2%-tangled O-grid; the reported
+    min det is the FP64 reduction of the returned state."""
+    from egg_amd.smoothing import cpp_untangle
+
+    c = ogrid(33, 17, tangle_seed=4)
+    md0 = oracle.energy_mindet(2, "barrier", 0.0, c.ctx.energy_stencil, c.X)[1]
+    assert md0 < 0
+    X, md, outer, delta = cpp_untangle(c.ctx, c.bsc, c.X)
+    assert md > 1e-9 and outer >= 1
+    E, M = oracle.energy_mindet(2, "barrier", 0.0, c.ctx.energy_stencil, X.astype(np.float32), prec="f32")
+    assert M == pytest.approx(md, rel=1e-9)
+
+
+def test_untangle_sweep_parity(oracle):
+    """Untangle-phase sweeps on a mildly folded block track the FP64 oracle."""
+    from egg_amd import grids
+
+    g = grids.single_block_2d(17, seed=6, amp=0.2)
+    g = grids.tangle(g, frac=0.02, factor=1.2, seed=2)
+    c = Case(g)
+    md = grids.min_det(g)
+    _, Xg, e, m, Xo, eo, mo = _run_both(oracle, c, 10, phase="untangle", delta=2 * abs(md), omega=0.5)
+    assert np.abs(Xg - Xo).max() / _extent(c) < 1e-4
+    np.testing.assert_allclose(e, eo, rtol=1e-4)
+
+
+def test_shape_size_parity(oracle):
+    c = single(n=17, seed=8, amp=0.2)
+    _, Xg, e, m, Xo, eo, mo = _run_both(oracle, c, 10, phase="shape_size")
+    assert np.abs(Xg - Xo).max() / _extent(c) < POS_TOL
+    np.testing.assert_allclose(e, eo, rtol=1e-5)
+
+
+def test_3d_block_parity(oracle):
+    """3D hex block (64-occurrence patches, mu_cond3) against the FP64 oracle."""
+    c = cube(n=9, seed=1)
+    _, Xg, e, m, Xo, eo, mo = _run_both(oracle, c, 10)
+    err = np.abs(Xg - Xo).max() / _extent(c)
+    print(f"3d max|dX|/extent={err:.2e}")
+    assert err < POS_TOL
+    np.testing.assert_allclose(e, eo, rtol=1e-5)
+
+
+def test_set_get_x_roundtrip():
+    c = ogrid(17, 9)
+    s = _session(c)
+    X2 = c.X + 1e-3
+    s.set_X(X2)
+    np.testing.assert_allclose(s.get_X(), X2.astype(np.float32), rtol=0, atol=0)

@@ -1,0 +1,149 @@
+"""Host-side logic without a GPU: the C ABI library, wire builders, the
+structured remap's index maps and validation errors."""
+
+from __future__ import annotations
+
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from tests.cases import lr_pair, ogrid, single
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def _core():
+    from egg_amd._cpp import cpp_core
+
+    return cpp_core
+
+
+def test_capi_exports_every_declared_symbol():
+    """libeggmi.so loads and exports every function include/eggmi.h declares."""
+    lib = ctypes.CDLL(str(ROOT / "egg_amd" / "_lib" / "libeggmi.so"))
+    decl = (ROOT / "include" / "eggmi.h").read_text()
+    names = set(re.findall(r"\b(eggmi_[A-Za-z_]+)\s*\(", decl))
+    assert {"eggmi_session_create", "eggmi_run", "eggmi_run_fas", "eggmi_get_X"} <= names
+    for n in sorted(names):
+        assert hasattr(lib, n), n
+    lib.eggmi_ping.restype = ctypes.c_int
+    assert lib.eggmi_ping() == 0
+
+
+def test_flat_context_counts():
+    c = single(n=9)
+    w = c.ctx.wire
+    g = w["groups"][0]
+    assert w["energy_stencil"]["num_samples"] == 4 * 8 * 8
+    assert g["D"] == 49 and int(np.sum(g["P_of"])) == len(g["gc"])
+    assert np.all(g["interior_block"] == 0)  # every free DOF of one block is interior
+    o = ogrid(17, 9)
+    g = o.ctx.wire["groups"][0]
+    stored = np.asarray(g["interior_block"]) < 0
+    P = np.asarray(g["P_of"])[stored]
+    # interface DOFs see 4 cells (16 samples), the 4 valence-3 corners 3 cells (12)
+    assert np.count_nonzero(P == 12) == 4
+    assert np.all((P == 12) | (P == 16))
+
+
+def _tables(case, device="tables"):
+    core = _core()
+    from egg_amd.smoothing import structured_arrays
+
+    s = core.CppStructuredSweepSession(case.ctx.wire, structured_arrays(case.bsc),
+                                       np.ascontiguousarray(case.X).ravel(), device=device, dim=case.d)
+    return s.info(), s.tables()
+
+
+@pytest.mark.parametrize("make", [lambda: single(n=9), lambda: lr_pair(9), lambda: ogrid(17, 9)])
+def test_remap_tables_match_oracle(make):
+    """Copy tables (halo | fan | share), g2s and block bases equal the
+    restatement of the reference's apply_structured_remap, entry by entry."""
+    from oracle.remap import remap
+
+    c = make()
+    info, tab = _tables(c)
+    b = c.bsc
+    ref = remap(c.d, b.interior_shapes, b.block_global_dof, b.halo_src_block, b.halo_src_padded,
+                b.halo_dst_block, b.halo_dst_padded, b.sing_block, b.sing_logical,
+                groups=c.ctx.wire["groups"], n_nodes=c.grid.n_nodes)
+    assert info["n_halo"] == ref["n_halo"] and info["n_fan"] == ref["n_fan"] and info["n_share"] == ref["n_share"]
+    assert info["n_overflow"] == ref["n_overflow"] and info["n_slots"] == ref["n_slots"]
+    np.testing.assert_array_equal(tab["copy_src"], ref["copy_src"])
+    np.testing.assert_array_equal(tab["copy_dst"], ref["copy_dst"])
+    np.testing.assert_array_equal(tab["g2s"], ref["g2s"])
+    np.testing.assert_array_equal(tab["block_base"], ref["base"])
+    assert info["n_stored"] == len(ref["stored"])
+
+
+def test_ogrid_halo_tables_shape():
+    """O-grid: shared face nodes get ghost sources and non-owner copies a
+    broadcast; the valence-3 corner patches resolve through face ghosts (the
+    radial node behind a corner is the twin of both ring copies), so no
+    spare-ghost mirrors are needed."""
+    c = ogrid(17, 9)
+    info, _ = _tables(c)
+    assert info["n_halo"] == c.bsc.num_halo_entries > 0
+    # ring copies of the circle (4 x 17) and of the four connector lines (4 x 8)
+    assert info["n_share"] == 4 * 17 + 4 * 8
+    assert info["n_fan"] == 0 and info["n_overflow"] == 0
+    assert info["structured_energy"] == 1
+
+
+@pytest.mark.parametrize(
+    "mutate,msg",
+    [
+        (lambda g, e: g.__setitem__("dof_idx", np.full_like(g["dof_idx"], 10**7)), "dof_idx index"),
+        (lambda g, e: g.__setitem__("role", np.full_like(g["role"], 7)), "role 7 out of range"),
+        (lambda g, e: g.__setitem__("P_of", np.asarray(g["P_of"]) + 1), "sum(P_of)"),
+        (lambda g, e: e.__setitem__("gc", np.full_like(e["gc"], -1)), "energy_stencil gc index"),
+        (lambda g, e: g.__setitem__("s0", np.asarray(g["s0"]) * 0.5), "must be +-1"),
+    ],
+)
+def test_validation_errors(mutate, msg):
+    """Malformed wires raise ValueError before any device work
+    (reference tests/test_cpp_validation.py:78-143)."""
+    import copy
+
+    from egg_amd.smoothing import structured_arrays
+
+    c = ogrid(9, 5)
+    w = copy.deepcopy(c.ctx.wire)
+    mutate(w["groups"][0], w["energy_stencil"])
+    with pytest.raises(ValueError, match=re.escape(msg)):
+        _core().CppStructuredSweepSession(w, structured_arrays(c.bsc), c.X.ravel(), device="tables")
+
+
+def test_x_shape_errors():
+    from egg_amd.smoothing import structured_arrays
+
+    c = single(n=5)
+    core = _core()
+    with pytest.raises(ValueError, match="flat 1-D"):
+        core.CppStructuredSweepSession(c.ctx.wire, structured_arrays(c.bsc), c.X, device="tables")
+    with pytest.raises(ValueError, match="multiple of d"):
+        core.CppStructuredSweepSession(c.ctx.wire, structured_arrays(c.bsc), c.X.ravel()[:-1], device="tables")
+    with pytest.raises(ValueError, match="dim must be 2 or 3"):
+        core.CppStructuredSweepSession(c.ctx.wire, structured_arrays(c.bsc), c.X.ravel(), device="tables", dim=4)
+
+
+def test_cpu_device_is_refused():
+    """There is no CPU fallback: device='cpu' is a RuntimeError."""
+    from egg_amd.smoothing import structured_arrays
+
+    c = single(n=5)
+    with pytest.raises(RuntimeError, match="device='cpu'"):
+        _core().CppStructuredSweepSession(c.ctx.wire, structured_arrays(c.bsc), c.X.ravel(), device="cpu")
+
+
+def test_grids_valid():
+    from egg_amd import grids
+
+    g = grids.ogrid_2d(33, 17)
+    assert grids.min_det(g) > 0
+    t = grids.tangle(g, seed=1)
+    assert grids.min_det(t) < 0
+    assert np.array_equal(t.free, g.free)
