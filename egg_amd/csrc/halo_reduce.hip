@@ -1,0 +1,395 @@
+// HIP kernels around the sweep: the fused halo / fan / share copy and the
+// FP64 energy + min-det reductions, plus the dimension dispatch of the
+// sweep launchers.
+#include "mg.hpp"
+#include "sweep_common.hpp"
+
+namespace eggmi
+{
+
+void launch_sweep2d(int kind, int v, const SweepArgs& a, hipStream_t s);
+void launch_grad2d(int kind, int v, const SweepArgs& a, hipStream_t s);
+void launch_sweep3d(int kind, int v, const SweepArgs& a, hipStream_t s);
+void launch_grad3d(int kind, int v, const SweepArgs& a, hipStream_t s);
+
+// node tiles (tx = last axis); every node is worked on by 2^D lanes
+static constexpr TileShape kTiles2[] = {{32, 2, 1}, {16, 4, 1}, {64, 1, 1}, {32, 4, 1}};
+static constexpr TileShape kTiles3[] = {{8, 4, 2}, {4, 4, 4}};
+
+int num_tile_variants(int dim) { return dim == 2 ? 4 : 2; }
+TileShape tile_variant_shape(int dim, int v) { return dim == 2 ? kTiles2[v] : kTiles3[v]; }
+
+void launch_sweep(int dim, int kind, int v, const SweepArgs& a, hipStream_t s)
+{
+    if (dim == 2)
+        launch_sweep2d(kind, v, a, s);
+    else
+        launch_sweep3d(kind, v, a, s);
+}
+
+void launch_grad(int dim, int kind, int v, const SweepArgs& a, hipStream_t s)
+{
+    if (dim == 2)
+        launch_grad2d(kind, v, a, s);
+    else
+        launch_grad3d(kind, v, a, s);
+}
+
+// ------------------------------------------------------------------ copies
+
+// Fused ghost-halo + fan-mirror + shared-node copy (reference
+// src/structured_halo.hpp:198-237): x[dst] = x[src] for every coordinate.
+__global__ __launch_bounds__(256) void copy_nodes(float* x, long long plane, int dim, const int* __restrict__ src,
+                                                  const int* __restrict__ dst, long long n)
+{
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int s = src[e], d = dst[e];
+    for (int k = 0; k < dim; ++k) x[k * plane + d] = x[k * plane + s];
+}
+
+// ------------------------------------------------------------------ reductions
+
+__device__ __forceinline__ void block_reduce_store(double e, double m, double* out)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        e += __shfl_xor(e, off, 64);
+        m = fmin(m, __shfl_xor(m, off, 64));
+    }
+    __shared__ double se[kReduceThreads / 64], sm[kReduceThreads / 64];
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) {
+        se[w] = e;
+        sm[w] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double te = 0.0, tm = __builtin_huge_val();
+        for (int i = 0; i < kReduceThreads / 64; ++i) {
+            te += se[i];
+            tm = fmin(tm, sm[i]);
+        }
+        out[0] = te;
+        out[1] = tm;
+    }
+}
+
+template <int KIND> __device__ __forceinline__ double mu2_d(double a, double b, double c, double d, double delta)
+{
+    return mu2_value<KIND, double>(a, b, c, d, delta);
+}
+
+// Sum of mu and min det A over every (cell, corner) sample of the structured
+// blocks, evaluated and accumulated in FP64 from the FP32 store
+// (reference src/sweep.hpp:1881-1922; energy stencil of
+// egg/smoothing/flat_context.py:64-133).
+template <int DIM, int KIND>
+__global__ __launch_bounds__(kReduceThreads) void reduce_cells(const CellReduceArgs a, double* partials)
+{
+    double esum = 0.0, emin = __builtin_huge_val();
+    const long long stride = (long long)gridDim.x * kReduceThreads;
+    for (long long cid = (long long)blockIdx.x * kReduceThreads + threadIdx.x; cid < a.ncells; cid += stride) {
+        int lo = 0, hi = a.nblocks - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (a.cell_off[mid] <= cid)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const BlockDesc bd = a.blocks[lo];
+        long long r = cid - a.cell_off[lo];
+        if constexpr (DIM == 2) {
+            const int w1 = bd.n[1] - 1;
+            const int ci = int(r / w1), cj = int(r - (long long)ci * w1);
+            double px[2][2], py[2][2];
+#pragma unroll
+            for (int o0 = 0; o0 < 2; ++o0)
+#pragma unroll
+                for (int o1 = 0; o1 < 2; ++o1) {
+                    const long long node = bd.base + (long long)(ci + o0 + 1) * bd.st[0] + (cj + o1 + 1);
+                    px[o0][o1] = a.x[node];
+                    py[o0][o1] = a.x[a.plane + node];
+                }
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int o0 = b >> 1, o1 = b & 1;
+                const double s0 = o0 ? -1.0 : 1.0, s1 = o1 ? -1.0 : 1.0;
+                const double A00 = s0 * (px[1 - o0][o1] - px[o0][o1]);
+                const double A01 = s1 * (px[o0][1 - o1] - px[o0][o1]);
+                const double A10 = s0 * (py[1 - o0][o1] - py[o0][o1]);
+                const double A11 = s1 * (py[o0][1 - o1] - py[o0][o1]);
+                esum += mu2_d<KIND>(A00, A01, A10, A11, a.delta);
+                emin = fmin(emin, A00 * A11 - A01 * A10);
+            }
+        } else {
+            const int w2 = bd.n[2] - 1, w1 = bd.n[1] - 1;
+            const int ci = int(r / ((long long)w1 * w2));
+            r -= (long long)ci * w1 * w2;
+            const int cj = int(r / w2), ck = int(r - (long long)cj * w2);
+            double p[3][8];
+#pragma unroll
+            for (int o = 0; o < 8; ++o) {
+                const int o0 = o >> 2, o1 = (o >> 1) & 1, o2 = o & 1;
+                const long long node =
+                  bd.base + (long long)(ci + o0 + 1) * bd.st[0] + (long long)(cj + o1 + 1) * bd.st[1] + (ck + o2 + 1);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) p[c][o] = a.x[c * a.plane + node];
+            }
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                double A[9];
+#pragma unroll
+                for (int kk = 0; kk < 3; ++kk) {
+                    const int bit = 1 << (2 - kk);
+                    const double s = (b & bit) ? -1.0 : 1.0;
+#pragma unroll
+                    for (int ii = 0; ii < 3; ++ii) A[ii * 3 + kk] = s * (p[ii][b ^ bit] - p[ii][b]);
+                }
+                esum += mu3_value<KIND, double>(A, a.delta);
+                emin = fmin(emin, det3(A));
+            }
+        }
+    }
+    block_reduce_store(esum, emin, partials + 2 * blockIdx.x);
+}
+
+template <int DIM, int KIND>
+__global__ __launch_bounds__(kReduceThreads) void reduce_samples(const SampleReduceArgs a, double* partials)
+{
+    double esum = 0.0, emin = __builtin_huge_val();
+    const long long stride = (long long)gridDim.x * kReduceThreads;
+    for (long long i = (long long)blockIdx.x * kReduceThreads + threadIdx.x; i < a.ns; i += stride) {
+        const int ic = a.gc[i];
+        double A[9], T[9];
+#pragma unroll
+        for (int kk = 0; kk < DIM; ++kk) {
+            const int in = a.gn[kk * a.ns + i];
+            const double s = a.s[kk * a.ns + i];
+#pragma unroll
+            for (int ii = 0; ii < DIM; ++ii)
+                A[ii * DIM + kk] = s * ((double)a.x[ii * a.plane + in] - (double)a.x[ii * a.plane + ic]);
+        }
+#pragma unroll
+        for (int ii = 0; ii < DIM; ++ii)
+#pragma unroll
+            for (int kk = 0; kk < DIM; ++kk) {
+                double acc = 0.0;
+#pragma unroll
+                for (int jj = 0; jj < DIM; ++jj)
+                    acc += A[ii * DIM + jj] * (a.W ? (double)a.W[i * DIM * DIM + jj * DIM + kk] : a.Wu[jj * DIM + kk]);
+                T[ii * DIM + kk] = acc;
+            }
+        if constexpr (DIM == 2) {
+            esum += mu2_d<KIND>(T[0], T[1], T[2], T[3], a.delta);
+            emin = fmin(emin, A[0] * A[3] - A[1] * A[2]);
+        } else {
+            esum += mu3_value<KIND, double>(T, a.delta);
+            emin = fmin(emin, det3(A));
+        }
+    }
+    block_reduce_store(esum, emin, partials + 2 * blockIdx.x);
+}
+
+__global__ __launch_bounds__(kReduceThreads) void reduce_finish(const double* partials, int nparts, double* out_e,
+                                                                double* out_m)
+{
+    double e = 0.0, m = __builtin_huge_val();
+    for (int i = threadIdx.x; i < nparts; i += kReduceThreads) {
+        e += partials[2 * i];
+        m = fmin(m, partials[2 * i + 1]);
+    }
+    __shared__ double tmp[2];
+    block_reduce_store(e, m, tmp);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        *out_e = tmp[0];
+        *out_m = tmp[1];
+    }
+}
+
+// Two line-search trials in one pass (reference src/fas.hpp:174-234): every
+// cell sample is evaluated at x + a0 c and x + a1 c (positions formed in
+// FP32 exactly as masked_axpy forms them), accumulated in FP64.
+template <int DIM, int KIND>
+__global__ __launch_bounds__(kReduceThreads) void linesearch_pair(const CellReduceArgs a, const float* corr, float a0,
+                                                                  float a1, double* partials)
+{
+    double e0 = 0.0, e1 = 0.0, m0 = __builtin_huge_val(), m1 = __builtin_huge_val();
+    const long long stride = (long long)gridDim.x * kReduceThreads;
+    for (long long cid = (long long)blockIdx.x * kReduceThreads + threadIdx.x; cid < a.ncells; cid += stride) {
+        int lo = 0, hi = a.nblocks - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (a.cell_off[mid] <= cid)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const BlockDesc bd = a.blocks[lo];
+        long long r = cid - a.cell_off[lo];
+        constexpr int NC = 1 << DIM;
+        long long nodes[NC];
+        if constexpr (DIM == 2) {
+            const int w1 = bd.n[1] - 1;
+            const int ci = int(r / w1), cj = int(r - (long long)ci * w1);
+#pragma unroll
+            for (int o = 0; o < 4; ++o)
+                nodes[o] = bd.base + (long long)(ci + (o >> 1) + 1) * bd.st[0] + (cj + (o & 1) + 1);
+        } else {
+            const int w2 = bd.n[2] - 1, w1 = bd.n[1] - 1;
+            const int ci = int(r / ((long long)w1 * w2));
+            r -= (long long)ci * w1 * w2;
+            const int cj = int(r / w2), ck = int(r - (long long)cj * w2);
+#pragma unroll
+            for (int o = 0; o < 8; ++o)
+                nodes[o] = bd.base + (long long)(ci + (o >> 2) + 1) * bd.st[0] +
+                           (long long)(cj + ((o >> 1) & 1) + 1) * bd.st[1] + (ck + (o & 1) + 1);
+        }
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const float al = t == 0 ? a0 : a1;
+            double p[DIM][NC];
+#pragma unroll
+            for (int o = 0; o < NC; ++o)
+#pragma unroll
+                for (int c = 0; c < DIM; ++c) {
+                    const long long q = c * a.plane + nodes[o];
+                    p[c][o] = (double)(a.x[q] + al * corr[q]);
+                }
+            double es = 0.0, ms = __builtin_huge_val();
+#pragma unroll
+            for (int b = 0; b < NC; ++b) {
+                double A[DIM * DIM];
+#pragma unroll
+                for (int kk = 0; kk < DIM; ++kk) {
+                    const int bit = 1 << (DIM - 1 - kk);
+                    const double sg = (b & bit) ? -1.0 : 1.0;
+#pragma unroll
+                    for (int ii = 0; ii < DIM; ++ii) A[ii * DIM + kk] = sg * (p[ii][b ^ bit] - p[ii][b]);
+                }
+                if constexpr (DIM == 2) {
+                    es += mu2_value<KIND, double>(A[0], A[1], A[2], A[3], a.delta);
+                    ms = fmin(ms, A[0] * A[3] - A[1] * A[2]);
+                } else {
+                    es += mu3_value<KIND, double>(A, a.delta);
+                    ms = fmin(ms, det3(A));
+                }
+            }
+            if (t == 0) {
+                e0 += es;
+                m0 = fmin(m0, ms);
+            } else {
+                e1 += es;
+                m1 = fmin(m1, ms);
+            }
+        }
+    }
+    block_reduce_store(e0, m0, partials + 4 * blockIdx.x);
+    __syncthreads();
+    block_reduce_store(e1, m1, partials + 4 * blockIdx.x + 2);
+}
+
+__global__ __launch_bounds__(kReduceThreads) void pair_finish(const double* partials, int nparts, double* out)
+{
+    double e0 = 0.0, e1 = 0.0, m0 = __builtin_huge_val(), m1 = __builtin_huge_val();
+    for (int i = threadIdx.x; i < nparts; i += kReduceThreads) {
+        e0 += partials[4 * i];
+        m0 = fmin(m0, partials[4 * i + 1]);
+        e1 += partials[4 * i + 2];
+        m1 = fmin(m1, partials[4 * i + 3]);
+    }
+    __shared__ double tmp[4];
+    block_reduce_store(e0, m0, tmp);
+    __syncthreads();
+    block_reduce_store(e1, m1, tmp + 2);
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int i = 0; i < 4; ++i) out[i] = tmp[i];
+}
+
+void launch_linesearch_pair(int dim, int kind, const CellReduceArgs& a, const float* corr, float a0, float a1,
+                            double* partials, double* out, hipStream_t s)
+{
+    const int nb = kReduceBlocks / 2;  // 4 partials per block fit the 2*kReduceBlocks buffer
+#define EGGMI_LP(D, K) linesearch_pair<D, K><<<nb, kReduceThreads, 0, s>>>(a, corr, a0, a1, partials)
+    if (dim == 2) {
+        if (kind == kShapeSize)
+            EGGMI_LP(2, kShapeSize);
+        else
+            EGGMI_LP(2, kShape);
+    } else {
+        if (kind == kShapeSize)
+            EGGMI_LP(3, kShapeSize);
+        else
+            EGGMI_LP(3, kShape);
+    }
+#undef EGGMI_LP
+    pair_finish<<<1, kReduceThreads, 0, s>>>(partials, nb, out);
+}
+
+void launch_copy(float* x, long long plane, int dim, const int* src, const int* dst, long long n, hipStream_t s)
+{
+    if (n <= 0) return;
+    copy_nodes<<<dim3((unsigned)((n + 255) / 256)), 256, 0, s>>>(x, plane, dim, src, dst, n);
+}
+
+static int reduce_grid(long long n)
+{
+    long long b = (n + kReduceThreads - 1) / kReduceThreads;
+    if (b > kReduceBlocks) b = kReduceBlocks;
+    return b < 1 ? 1 : int(b);
+}
+
+void launch_reduce_cells(int dim, int kind, const CellReduceArgs& a, double* partials, hipStream_t s)
+{
+    const int nb = kReduceBlocks;  // fixed partial count; finish reads all of them
+#define EGGMI_RC(D, K) reduce_cells<D, K><<<nb, kReduceThreads, 0, s>>>(a, partials)
+    if (dim == 2) {
+        if (kind == kUntangle)
+            EGGMI_RC(2, kUntangle);
+        else if (kind == kShapeSize)
+            EGGMI_RC(2, kShapeSize);
+        else
+            EGGMI_RC(2, kShape);
+    } else {
+        if (kind == kUntangle)
+            EGGMI_RC(3, kUntangle);
+        else if (kind == kShapeSize)
+            EGGMI_RC(3, kShapeSize);
+        else
+            EGGMI_RC(3, kShape);
+    }
+#undef EGGMI_RC
+    (void)reduce_grid;
+}
+
+void launch_reduce_samples(int dim, int kind, const SampleReduceArgs& a, double* partials, hipStream_t s)
+{
+    const int nb = kReduceBlocks;
+#define EGGMI_RS(D, K) reduce_samples<D, K><<<nb, kReduceThreads, 0, s>>>(a, partials)
+    if (dim == 2) {
+        if (kind == kUntangle)
+            EGGMI_RS(2, kUntangle);
+        else if (kind == kShapeSize)
+            EGGMI_RS(2, kShapeSize);
+        else
+            EGGMI_RS(2, kShape);
+    } else {
+        if (kind == kUntangle)
+            EGGMI_RS(3, kUntangle);
+        else if (kind == kShapeSize)
+            EGGMI_RS(3, kShapeSize);
+        else
+            EGGMI_RS(3, kShape);
+    }
+#undef EGGMI_RS
+}
+
+void launch_reduce_finish(const double* partials, int nparts, double* out_e, double* out_m, hipStream_t s)
+{
+    reduce_finish<<<1, kReduceThreads, 0, s>>>(partials, nparts, out_e, out_m);
+}
+
+}  // namespace eggmi

@@ -1,0 +1,210 @@
+// Inter-grid transfer kernels of the FAS multigrid (gfx950), one launch per
+// operation over all blocks of a level (the block of a thread's node is
+// found by binary search over the per-block node-count prefix):
+//   restrict_inject      coarse(I) = fine(I * f) (exact copy)
+//   restrict_full_weight tensor (1/4, 1/2, 1/4) stencil on coarsened axes of
+//                        the ghost-inclusive fine gradient, at free coarse
+//                        nodes (zero elsewhere)
+//   prolong_correction   multilinear interpolation of the coarse correction
+//   masked_axpy          x += a c at free slots
+//   zero_unmasked        c = 0 at non-free slots
+//   vec_sub              out = a - b
+//   linesearch_pair      fine energy / min det at x + a0 c and x + a1 c
+// (reference src/mg_transfer.hpp:116-341 and src/fas.hpp:174-234).
+#include "mg.hpp"
+
+namespace eggmi
+{
+
+__device__ __forceinline__ int find_block(const long long* off, int nb, long long i)
+{
+    int lo = 0, hi = nb - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= i)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// logical index of the r-th node of an n[0] x n[1] (x n[2]) block, row-major
+template <int D> __device__ __forceinline__ void unravel(long long r, const int* n, int* I)
+{
+    if constexpr (D == 2) {
+        I[0] = int(r / n[1]);
+        I[1] = int(r - (long long)I[0] * n[1]);
+    } else {
+        const long long s12 = (long long)n[1] * n[2];
+        I[0] = int(r / s12);
+        r -= (long long)I[0] * s12;
+        I[1] = int(r / n[2]);
+        I[2] = int(r - (long long)I[1] * n[2]);
+    }
+}
+
+template <int D> __device__ __forceinline__ long long padded_slot(const BlockDesc& b, const int* P)
+{
+    long long s = b.base;
+#pragma unroll
+    for (int k = 0; k < D; ++k) s += (long long)P[k] * b.st[k];
+    return s;
+}
+
+// Threads enumerate the coarse interior nodes of every block.
+template <int D> __global__ __launch_bounds__(256) void restrict_kernel(const TransferArgs a, const float* fx,
+                                                                       float* cx, const float* fgrad,
+                                                                       float* ctau, const uint8_t* cfree)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= a.n) return;
+    const int b = find_block(a.off, a.nb, i);
+    const BlockDesc fb = a.fine[b], cb = a.coarse[b];
+    int I[3], Pc[3], Pf[3];
+    unravel<D>(i - a.off[b], cb.n, I);
+    int f[3];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        f[k] = a.factor[b * 3 + k];
+        Pc[k] = I[k] + 1;
+        Pf[k] = I[k] * f[k] + 1;
+    }
+    const long long cs = padded_slot<D>(cb, Pc);
+    const long long fs = padded_slot<D>(fb, Pf);
+    if (cx) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) cx[k * a.cplane + cs] = fx[k * a.fplane + fs];
+    }
+    if (ctau) {
+        float acc[D] = {};
+        if (cfree[cs]) {
+            constexpr int NO = (D == 2) ? 9 : 27;
+            for (int e = 0; e < NO; ++e) {
+                int rem = e, P[3];
+                float w = 1.0f;
+                bool ok = true;
+#pragma unroll
+                for (int k = 0; k < D; ++k) {
+                    const int o = rem % 3 - 1;
+                    rem /= 3;
+                    if (f[k] == 1) {
+                        if (o != 0) ok = false;
+                        P[k] = Pf[k];
+                    } else {
+                        w *= (o == 0) ? 0.5f : 0.25f;
+                        P[k] = Pf[k] + o;
+                    }
+                }
+                if (!ok) continue;
+                const long long s = padded_slot<D>(fb, P);
+#pragma unroll
+                for (int k = 0; k < D; ++k) acc[k] += w * fgrad[k * a.fplane + s];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < D; ++k) ctau[k * a.cplane + cs] = acc[k];
+    }
+}
+
+// Threads enumerate the fine interior nodes of every block.
+template <int D> __global__ __launch_bounds__(256) void prolong_kernel(const TransferArgs a, const float* ccorr,
+                                                                      float* fcorr)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= a.n) return;
+    const int b = find_block(a.off, a.nb, i);
+    const BlockDesc fb = a.fine[b], cb = a.coarse[b];
+    int J[3], c[3], r[3], P[3];
+    unravel<D>(i - a.off[b], fb.n, J);
+    float w0 = 1.0f;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        const int f = a.factor[b * 3 + k];
+        c[k] = J[k] / f;
+        r[k] = J[k] - c[k] * f;
+        if (r[k] != 0) w0 *= 0.5f;
+        P[k] = J[k] + 1;
+    }
+    float acc[D] = {};
+    for (int e = 0; e < (1 << D); ++e) {
+        int C[3];
+        bool ok = true;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            C[k] = c[k] + 1;
+            if ((e >> k) & 1) {
+                if (r[k] == 0) ok = false;
+                C[k] += 1;
+            }
+        }
+        if (!ok) continue;
+        const long long s = padded_slot<D>(cb, C);
+#pragma unroll
+        for (int k = 0; k < D; ++k) acc[k] += w0 * ccorr[k * a.cplane + s];
+    }
+    const long long fs = padded_slot<D>(fb, P);
+#pragma unroll
+    for (int k = 0; k < D; ++k) fcorr[k * a.fplane + fs] = acc[k];
+}
+
+__global__ __launch_bounds__(256) void masked_axpy_kernel(float* x, const float* c, float alpha, const uint8_t* free,
+                                                          long long n, long long plane, int dim)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n || !free[i]) return;
+    for (int k = 0; k < dim; ++k) x[k * plane + i] = x[k * plane + i] + alpha * c[k * plane + i];
+}
+
+__global__ __launch_bounds__(256) void zero_unmasked_kernel(float* c, const uint8_t* free, long long n,
+                                                            long long plane, int dim)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n || free[i]) return;
+    for (int k = 0; k < dim; ++k) c[k * plane + i] = 0.0f;
+}
+
+__global__ __launch_bounds__(256) void vec_sub_kernel(float* out, const float* a, const float* b, long long n)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] = a[i] - b[i];
+}
+
+static unsigned nblk(long long n) { return (unsigned)((n + 255) / 256); }
+
+void launch_restrict(int dim, const TransferArgs& a, const float* fx, float* cx, const float* fgrad, float* ctau,
+                     const uint8_t* cfree, hipStream_t s)
+{
+    if (a.n <= 0) return;
+    if (dim == 2)
+        restrict_kernel<2><<<nblk(a.n), 256, 0, s>>>(a, fx, cx, fgrad, ctau, cfree);
+    else
+        restrict_kernel<3><<<nblk(a.n), 256, 0, s>>>(a, fx, cx, fgrad, ctau, cfree);
+}
+
+void launch_prolong(int dim, const TransferArgs& a, const float* ccorr, float* fcorr, hipStream_t s)
+{
+    if (a.n <= 0) return;
+    if (dim == 2)
+        prolong_kernel<2><<<nblk(a.n), 256, 0, s>>>(a, ccorr, fcorr);
+    else
+        prolong_kernel<3><<<nblk(a.n), 256, 0, s>>>(a, ccorr, fcorr);
+}
+
+void launch_masked_axpy(float* x, const float* c, float alpha, const uint8_t* free, long long n, long long plane,
+                        int dim, hipStream_t s)
+{
+    if (n > 0) masked_axpy_kernel<<<nblk(n), 256, 0, s>>>(x, c, alpha, free, n, plane, dim);
+}
+
+void launch_zero_unmasked(float* c, const uint8_t* free, long long n, long long plane, int dim, hipStream_t s)
+{
+    if (n > 0) zero_unmasked_kernel<<<nblk(n), 256, 0, s>>>(c, free, n, plane, dim);
+}
+
+void launch_vec_sub(float* out, const float* a, const float* b, long long n, hipStream_t s)
+{
+    if (n > 0) vec_sub_kernel<<<nblk(n), 256, 0, s>>>(out, a, b, n);
+}
+
+}  // namespace eggmi

@@ -64,33 +64,62 @@ void Level::make_tiles(int variant)
 {
     const TileShape ts = tile_variant_shape(lay.dim, variant);
     // axis 0 <- tz (3D) / ty (2D), last axis <- tx
-    const int step[3] = {lay.dim == 2 ? ts.ty : ts.tz, lay.dim == 2 ? ts.tx : ts.ty, ts.tx};
+    const int step[3] = {lay.dim == 2 ? ts.ty : ts.tz, lay.dim == 2 ? ts.tx : ts.ty, lay.dim == 2 ? 1 : ts.tx};
     tiles.clear();
     for (size_t b = 0; b < lay.shape.size(); ++b) {
         const auto& n = lay.shape[b];
-        bool any = true;
-        for (int k = 0; k < lay.dim; ++k) any = any && n[k] >= 3;
-        if (!any) continue;
-        // only blocks that hold at least one synthesized DOF
-        bool has = false;
-        {
-            const long long s0 = lay.base[b], s1 = s0 + lay.block_padded_count((int)b);
-            for (long long s = s0; s < s1 && !has; ++s) has = mask[s] != 0;
-        }
-        if (!has) continue;
-        const int e0 = n[0] - 2, e1 = n[1] - 2, e2 = lay.dim == 3 ? n[2] - 2 : 1;
-        for (int i = 0; i < e0; i += step[0])
-            for (int j = 0; j < e1; j += step[1])
-                for (int k = 0; k < e2; k += (lay.dim == 3 ? step[2] : 1)) {
+        // bounding box of the block's synthesized DOFs (logical, interior slots only)
+        int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-1, -1, -1};
+        const int n2 = lay.dim == 3 ? n[2] : 1;
+        for (int i = 0; i < n[0]; ++i)
+            for (int j = 0; j < n[1]; ++j)
+                for (int k = 0; k < n2; ++k) {
+                    const int lg[3] = {i, j, k};
+                    if (!mask[lay.interior_slot((int)b, lg)]) continue;
+                    for (int a = 0; a < 3; ++a) {
+                        lo[a] = std::min(lo[a], lg[a]);
+                        hi[a] = std::max(hi[a], lg[a]);
+                    }
+                }
+        if (hi[0] < 0) continue;
+        for (int i = lo[0]; i <= hi[0]; i += step[0])
+            for (int j = lo[1]; j <= hi[1]; j += step[1])
+                for (int k = lo[2]; k <= hi[2]; k += step[2]) {
                     TileDesc t;
                     t.blk = (int)b;
-                    t.i0 = 1 + i;
-                    t.j0 = 1 + j;
-                    t.k0 = 1 + k;
+                    t.i0 = i;
+                    t.j0 = j;
+                    t.k0 = k;
                     tiles.push_back(t);
                 }
     }
     d_tiles.upload(tiles);
+}
+
+void Level::build_push()
+{
+    // root source of a copy: follow owner -> copy broadcasts backwards (a
+    // table entry whose source is itself a broadcast destination)
+    const long long ns = lay.n_slots();
+    std::vector<int> share_from(ns, -1);
+    for (size_t q = copy_src.size() - n_share; q < copy_src.size(); ++q) share_from[copy_dst[q]] = copy_src[q];
+    std::vector<uint8_t> is_dof(mask.begin(), mask.end());
+    for (int d : st_dof) is_dof[d] = 1;
+    std::vector<std::vector<int>> lists;
+    std::vector<int> count(ns + 1, 0);
+    std::vector<std::pair<int, int>> pushes;
+    for (size_t q = 0; q < copy_src.size(); ++q) {
+        int r = copy_src[q];
+        for (int guard = 0; share_from[r] >= 0 && guard < 8; ++guard) r = share_from[r];
+        if (is_dof[r]) pushes.emplace_back(r, copy_dst[q]);
+    }
+    for (auto& pr : pushes) ++count[pr.first + 1];
+    for (long long i = 0; i < ns; ++i) count[i + 1] += count[i];
+    cp_off.assign(count.begin(), count.end());
+    cp_dst.assign(pushes.size(), 0);
+    std::vector<int> fill(count.begin(), count.end() - 1);
+    for (auto& pr : pushes) cp_dst[fill[pr.first]++] = pr.second;
+    n_push = (long long)pushes.size();
 }
 
 void Level::upload_tables()
@@ -107,6 +136,8 @@ void Level::upload_tables()
     blocks.upload(bd);
     d_copy_src.upload(copy_src);
     d_copy_dst.upload(copy_dst);
+    d_cp_off.upload(cp_off);
+    d_cp_dst.upload(cp_dst);
     d_mask.upload(mask);
     d_st_dof.upload(st_dof);
     d_st_off.upload(st_off);
@@ -146,6 +177,8 @@ SweepArgs Level::sweep_args(const float* xin, float* xout, float omega, float de
     a.omega = omega;
     a.delta = delta;
     a.tau = tau;
+    a.cp_off = n_push ? d_cp_off.p : nullptr;
+    a.cp_dst = n_push ? d_cp_dst.p : nullptr;
     for (int i = 0; i < 9; ++i) a.Wu[i] = Wu[i];
     return a;
 }
@@ -345,6 +378,10 @@ void Session::create(int d, const eggmi_group* groups, int n_groups, const eggmi
         }
         const int gid = bgd[sb][ravel(sb, sl)];
         const long long ds = lay.padded_slot(db, dp);
+        fine_halo.sb.push_back(sb);
+        fine_halo.db.push_back(db);
+        fine_halo.sp.push_back({sp[0], d > 1 ? sp[1] : 0, d > 2 ? sp[2] : 0});
+        fine_halo.dp.push_back({dp[0], d > 1 ? dp[1] : 0, d > 2 ? dp[2] : 0});
         ghost_map[db][gid] = ds;
         used_ghost[db][ds - lay.base[db]] = 1;
         halo_src.push_back((int)lay.padded_slot(sb, sp));
@@ -494,6 +531,10 @@ void Session::create(int d, const eggmi_group* groups, int n_groups, const eggmi
     if (nslots >= (1ll << 31)) throw InvalidArgument("structured: more than 2^31 node slots");
     L.mask.assign(nslots, 0);
     for (int s : synth_slots) L.mask[s] = 1;
+    fine_free = L.mask;
+    for (int s : L.st_dof) fine_free[s] = 1;
+    fine_halo.share_src.assign(share_src.begin(), share_src.end());
+    fine_halo.share_dst.assign(share_dst.begin(), share_dst.end());
     L.n_synth = (long long)synth_slots.size();
     // copy table: halo | fan | share
     L.copy_src = halo_src;
@@ -505,6 +546,7 @@ void Session::create(int d, const eggmi_group* groups, int n_groups, const eggmi
     L.n_halo = (long long)halo_src.size();
     L.n_fan = (long long)fan_src.size();
     L.n_share = (long long)share_src.size();
+    L.build_push();
 
     // ---- energy stencil: structured cells when the wire is the plain cell stencil ----
     {
@@ -663,7 +705,6 @@ std::pair<std::vector<double>, std::vector<double>> Session::run_jacobi(int n_sw
         if (kernel_timing) EGGMI_HIP(hipEventRecord(tev[2 * s], stream));
         launch_sweep(dim, kind, tile_variant, a, stream);
         if (kernel_timing) EGGMI_HIP(hipEventRecord(tev[2 * s + 1], stream));
-        halo(L, xout);
         if ((s + 1) % k == 0 || s == n_sweeps - 1) {
             reduce(L, xout, kind, delta, d_reports.p + rep, d_reports.p + nrep + rep);
             ++rep;

@@ -6,6 +6,7 @@
 
 #include "eggmi.h"
 #include "kernels.hpp"
+#include "mg.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -92,6 +93,20 @@ struct Layout {
     long long block_padded_count(int b) const;
 };
 
+// Cross-block copy tables of one level in block / padded-index form (face
+// ghost fills) plus owner -> copy broadcasts (slots): the input of the coarse
+// interface table derivation.
+struct HaloHost {
+    std::vector<int> sb, db;
+    std::vector<std::array<int, 3>> sp, dp;
+    std::vector<long long> share_src, share_dst;
+};
+
+struct FasParams {
+    int nu_pre = 2, nu_post = 2, nu_coarse = 32;
+    float omega = 1.0f;
+};
+
 // Everything one level of the hierarchy needs on the device.
 struct Level {
     Layout lay;
@@ -101,6 +116,11 @@ struct Level {
     std::vector<int> copy_src, copy_dst;
     long long n_halo = 0, n_fan = 0, n_share = 0;
     DArr<int> d_copy_src, d_copy_dst;
+    // push form of the copies whose (root) source is a relaxed DOF: the
+    // sweep kernel writes a DOF's new value into every replica slot too
+    std::vector<int> cp_off, cp_dst;
+    DArr<int> d_cp_off, d_cp_dst;
+    long long n_push = 0;
     // synthesized interior DOFs
     std::vector<uint8_t> mask;  // [n_slots]
     DArr<uint8_t> d_mask;
@@ -127,9 +147,18 @@ struct Level {
     // device state
     DArr<float> x[2];
     int cur = 0;  // which x holds the canonical state
+    // FAS state (coarse levels; the fine level uses grad / corr / d_free)
+    std::vector<std::array<int, 3>> factor;  // per block, vs the parent level
+    std::vector<uint8_t> free_mask, free_interior;
+    DArr<uint8_t> d_free;
+    DArr<int> d_factor;
+    DArr<long long> d_coarse_off, d_fine_off;  // node-count prefixes for the transfers
+    long long n_coarse_nodes = 0, n_fine_nodes = 0;
+    DArr<float> rx, tau, grad, corr;
 
     void upload_tables();
     void make_tiles(int variant);
+    void build_push();
     SweepArgs sweep_args(const float* xin, float* xout, float omega, float delta, const float* tau) const;
 };
 
@@ -153,6 +182,9 @@ struct Session {
     // FAS hierarchy (built lazily)
     std::vector<std::unique_ptr<Level>> coarse;
     int built_levels = -1;
+    std::vector<uint8_t> fine_free;  // relaxed fine DOF slots
+    HaloHost fine_halo;
+    DArr<double> d_ls;
     bool host_only = false;  // created with device = 3: index tables only
 
     Session() = default;
@@ -172,6 +204,14 @@ struct Session {
     void set_X(const double* X);
     std::vector<float> packed() const;
     void need_device() const;
+    // FAS internals
+    void build_hierarchy();
+    void upload_level(Level& L);
+    TransferArgs transfer(const Level& parent, const Level& child, bool prolong) const;
+    void smooth(Level& L, int sweeps, int kind, float omega, const float* tau);
+    void grad(Level& L, const float* x, float* out, const float* tau, int kind);
+    void restrict_to(Level& parent, const float* x_parent, Level& child, int kind);
+    void coarse_vcycle(size_t li, size_t n_levels, int kind, const FasParams& o);
 };
 
 }  // namespace eggmi

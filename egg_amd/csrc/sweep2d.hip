@@ -1,0 +1,51 @@
+// 2D instantiations of the structured sweep kernel (sweep_impl.hpp): quad
+// patches of 4 cells x 4 corner samples, 4 lanes per node, shape / untangle /
+// shape_size metrics; tile shapes in nodes (last axis x first axis).
+#include "sweep_impl.hpp"
+
+namespace eggmi
+{
+
+template <int KIND, bool TAU, bool GRAD> static void sweep2d_v(int v, const SweepArgs& a, hipStream_t s)
+{
+    switch (v) {
+    case 0: launch_variant<2, KIND, 32, 2, 1, TAU, GRAD>(a, s); break;
+    case 1: launch_variant<2, KIND, 16, 4, 1, TAU, GRAD>(a, s); break;
+    case 2: launch_variant<2, KIND, 64, 1, 1, TAU, GRAD>(a, s); break;
+    default: launch_variant<2, KIND, 32, 4, 1, TAU, GRAD>(a, s); break;
+    }
+}
+
+void launch_sweep2d(int kind, int v, const SweepArgs& a, hipStream_t s)
+{
+    if (a.tau) {
+        if (kind == kShapeSize)
+            sweep2d_v<kShapeSize, true, false>(v, a, s);
+        else
+            sweep2d_v<kShape, true, false>(v, a, s);  // FAS levels: barrier phases only
+        return;
+    }
+    if (kind == kUntangle)
+        sweep2d_v<kUntangle, false, false>(v, a, s);
+    else if (kind == kShapeSize)
+        sweep2d_v<kShapeSize, false, false>(v, a, s);
+    else
+        sweep2d_v<kShape, false, false>(v, a, s);
+}
+
+void launch_grad2d(int kind, int v, const SweepArgs& a, hipStream_t s)
+{
+    if (a.tau) {
+        if (kind == kShapeSize)
+            sweep2d_v<kShapeSize, true, true>(v, a, s);
+        else
+            sweep2d_v<kShape, true, true>(v, a, s);
+    } else {
+        if (kind == kShapeSize)
+            sweep2d_v<kShapeSize, false, true>(v, a, s);
+        else
+            sweep2d_v<kShape, false, true>(v, a, s);
+    }
+}
+
+}  // namespace eggmi

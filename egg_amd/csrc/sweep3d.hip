@@ -1,0 +1,49 @@
+// 3D instantiations of the structured sweep kernel (sweep_impl.hpp): hex
+// patches of 8 cells x 8 corner samples, 8 lanes per node, mu_cond3 family
+// metrics; tile shapes in nodes (last axis x middle x first axis).
+#include "sweep_impl.hpp"
+
+namespace eggmi
+{
+
+template <int KIND, bool TAU, bool GRAD> static void sweep3d_v(int v, const SweepArgs& a, hipStream_t s)
+{
+    switch (v) {
+    case 0: launch_variant<3, KIND, 8, 4, 2, TAU, GRAD>(a, s); break;
+    default: launch_variant<3, KIND, 4, 4, 4, TAU, GRAD>(a, s); break;
+    }
+}
+
+void launch_sweep3d(int kind, int v, const SweepArgs& a, hipStream_t s)
+{
+    if (a.tau) {
+        if (kind == kShapeSize)
+            sweep3d_v<kShapeSize, true, false>(v, a, s);
+        else
+            sweep3d_v<kShape, true, false>(v, a, s);  // FAS levels: barrier phases only
+        return;
+    }
+    if (kind == kUntangle)
+        sweep3d_v<kUntangle, false, false>(v, a, s);
+    else if (kind == kShapeSize)
+        sweep3d_v<kShapeSize, false, false>(v, a, s);
+    else
+        sweep3d_v<kShape, false, false>(v, a, s);
+}
+
+void launch_grad3d(int kind, int v, const SweepArgs& a, hipStream_t s)
+{
+    if (a.tau) {
+        if (kind == kShapeSize)
+            sweep3d_v<kShapeSize, true, true>(v, a, s);
+        else
+            sweep3d_v<kShape, true, true>(v, a, s);
+    } else {
+        if (kind == kShapeSize)
+            sweep3d_v<kShapeSize, false, true>(v, a, s);
+        else
+            sweep3d_v<kShape, false, true>(v, a, s);
+    }
+}
+
+}  // namespace eggmi

@@ -1,0 +1,438 @@
+// Dimension-generic HIP sweep kernel of the structured TMOP smoother (gfx950).
+//
+// Work decomposition. One sweep is one launch. Workgroups [0, n_tiles) each
+// own a tile of nodes of one block and stage the tile plus a one-node halo of
+// every coordinate plane in LDS (coalesced row loads; the halo reaches into
+// the ghost layer of the padded block, so face DOFs of coarse levels work as
+// well as interior ones). Each node is worked on by 2^D lanes, one per cell
+// of its patch: a lane evaluates the 2^D corner samples of its cell, and the
+// node's gradient, contracted Hessian, patch energy and min det are reduced
+// across the lane group with wavefront shuffles. Newton step, <= 10
+// halvings and the omega post-blend then run redundantly on every lane of
+// the group (identical values, so the accept decision is uniform), and lane
+// 0 writes the node and pushes it to every ghost / copy / mirror slot that
+// replicates it (the reference's per-sweep halo exchange, done by the
+// writer). The remaining workgroups give each stored-stencil DOF (block
+// faces, interfaces, singular corners) a group of G lanes, one per
+// occurrence, with the same shuffle reductions.
+//
+// Semantics per DOF follow the reference's metric + interior update
+// kernels (src/sweep.hpp:1425-1650): frozen snapshot `xin`, accept iff the
+// trial energy is finite, <= e0 + 1e-6 and (barrier phases) min det > 0.
+// Sample order inside the reductions differs from the reference's
+// occurrence order (floating-point summation order only).
+#pragma once
+
+#include "sweep_common.hpp"
+
+namespace eggmi
+{
+
+template <int D> struct Vec {
+    float v[D];
+};
+
+// Butterfly reductions inside aligned groups of W lanes (W a power of 2).
+// xor-butterflies give every lane of the group the bitwise-same result.
+template <int W> __device__ __forceinline__ float group_sum(float v)
+{
+#pragma unroll
+    for (int m = 1; m < W; m <<= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+template <int W> __device__ __forceinline__ float group_min(float v)
+{
+#pragma unroll
+    for (int m = 1; m < W; m <<= 1) v = fminf(v, __shfl_xor(v, m, 64));
+    return v;
+}
+
+// Metric value, gradient and contracted Hessian (upper triangle, row-major)
+// of one sample with matrix T along u; returns mu.
+template <int D, int KIND>
+__device__ __forceinline__ float sample_grad_hess(const float* T, const float* u, float delta, float* g, float* h)
+{
+    if constexpr (D == 2) return mu2_grad_hess<KIND, float>(T[0], T[1], T[2], T[3], u[0], u[1], delta, g[0], g[1], h[0], h[1], h[2]);
+    else return mu3_grad_hess<KIND, float>(T, u, delta, g, h);
+}
+template <int D, int KIND> __device__ __forceinline__ float sample_value(const float* T, float delta)
+{
+    if constexpr (D == 2) return mu2_value<KIND, float>(T[0], T[1], T[2], T[3], delta);
+    else return mu3_value<KIND, float>(T, delta);
+}
+template <int D> __device__ __forceinline__ float detD(const float* A)
+{
+    if constexpr (D == 2) return A[0] * A[3] - A[1] * A[2];
+    else return det3(A);
+}
+
+template <int D> constexpr int kNH = D * (D + 1) / 2;
+
+// Newton step with the reference fallbacks from the reduced gradient/Hessian.
+template <int D> __device__ __forceinline__ void newton(const float* g, const float* h, float* dl)
+{
+    if constexpr (D == 2) {
+        solve2<float>(h[0], h[1], h[1], h[2], g[0], g[1], dl[0], dl[1]);
+    } else {
+        const float H[9] = {h[0], h[1], h[2], h[1], h[3], h[4], h[2], h[4], h[5]};
+        solve3<float>(H, g, dl);
+    }
+}
+
+// Write the accepted node and push it to its replicas.
+template <int D>
+__device__ __forceinline__ void store_node(const SweepArgs& a, long long node, const float* cur)
+{
+#pragma unroll
+    for (int k = 0; k < D; ++k) a.xout[k * a.plane + node] = cur[k];
+    if (a.cp_off) {
+        const int q0 = a.cp_off[node], q1 = a.cp_off[node + 1];
+        for (int q = q0; q < q1; ++q) {
+            const int dst = a.cp_dst[q];
+#pragma unroll
+            for (int k = 0; k < D; ++k) a.xout[k * a.plane + dst] = cur[k];
+        }
+    }
+}
+
+// ------------------------------------------------------------------ stored
+
+// Stored-stencil DOF d handled by a group of G lanes (lane l owns
+// occurrences l, l+G, ... of the DOF's patch).
+template <int D, int KIND, bool TAU, bool GRAD, int G>
+__device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
+{
+    const StoredView& st = a.st;
+    const float* X = a.xin;
+    const long long PL = a.plane;
+    int dof = 0, p0 = 0, p1 = 0;
+    if (valid) {
+        dof = st.dof[d];
+        p0 = st.off[d];
+        p1 = st.off[d + 1];
+    }
+    float pos[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) pos[k] = valid ? X[k * PL + dof] : 0.0f;
+
+    // builds T (and det A) of occurrence p, with the DOF moved to `sub` when given
+    auto build = [&](int p, const float* sub, float* T, float& det, float* s) {
+        const int m = st.meta[p];
+        const int ic = st.gc[p];
+        float cn[D], nb[D][D];
+#pragma unroll
+        for (int i = 0; i < D; ++i) cn[i] = (sub && ic == dof) ? sub[i] : X[i * PL + ic];
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const int in = st.gn[k * st.total + p];
+            s[k] = (m & (4 << k)) ? -1.0f : 1.0f;
+#pragma unroll
+            for (int i = 0; i < D; ++i) nb[k][i] = (sub && in == dof) ? sub[i] : X[i * PL + in];
+        }
+        float A[D * D];
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+#pragma unroll
+            for (int k = 0; k < D; ++k) A[i * D + k] = s[k] * (nb[k][i] - cn[i]);
+        det = detD<D>(A);
+        const float* w = st.W ? st.W + (long long)(D * D) * p : a.Wu;
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                float acc = 0.0f;
+#pragma unroll
+                for (int j = 0; j < D; ++j) acc += A[i * D + j] * w[j * D + k];
+                T[i * D + k] = acc;
+            }
+    };
+
+    float g[D] = {}, h[kNH<D>] = {}, e = 0.0f, md = kInf;
+    for (int p = p0 + l; p < p1; p += G) {
+        float T[D * D], det, s[D];
+        build(p, nullptr, T, det, s);
+        md = fminf(md, det);
+        const int role = (st.meta[p] & 3) - 1;
+        if (role >= 0) {
+            const float* w = st.W ? st.W + (long long)(D * D) * p : a.Wu;
+            float u[D];
+#pragma unroll
+            for (int c = 0; c < D; ++c) {
+                if (role == 0) {
+                    float acc = 0.0f;
+#pragma unroll
+                    for (int k = 0; k < D; ++k) acc += s[k] * w[k * D + c];
+                    u[c] = -acc;
+                } else {
+                    u[c] = s[role - 1] * w[(role - 1) * D + c];
+                }
+            }
+            e += sample_grad_hess<D, KIND>(T, u, a.delta, g, h);
+        } else {
+            e += sample_value<D, KIND>(T, a.delta);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) g[k] = group_sum<G>(g[k]);
+    float e0 = group_sum<G>(e);
+    if (!valid) return;
+    float tau[D] = {};
+    if constexpr (TAU) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            tau[k] = a.tau[k * PL + dof];
+            g[k] -= tau[k];
+            e0 -= tau[k] * pos[k];
+        }
+    }
+    if constexpr (GRAD) {
+        if (l == 0)
+#pragma unroll
+            for (int k = 0; k < D; ++k) a.gout[k * PL + dof] = g[k];
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < kNH<D>; ++k) h[k] = group_sum<G>(h[k]);
+    float dl[D];
+    newton<D>(g, h, dl);
+    float alpha = 1.0f, cur[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) cur[k] = pos[k];
+    for (int it = 0; it < 10; ++it) {
+        float tr[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) tr[k] = pos[k] + alpha * dl[k];
+        float et = 0.0f, mt = kInf;
+        for (int p = p0 + l; p < p1; p += G) {
+            float T[D * D], det, s[D];
+            build(p, tr, T, det, s);
+            et += sample_value<D, KIND>(T, a.delta);
+            mt = fminf(mt, det);
+        }
+        et = group_sum<G>(et);
+        mt = group_min<G>(mt);
+        if constexpr (TAU) {
+#pragma unroll
+            for (int k = 0; k < D; ++k) et -= tau[k] * tr[k];
+        }
+        if (accept<KIND>(et, e0, mt)) {
+#pragma unroll
+            for (int k = 0; k < D; ++k) cur[k] = tr[k];
+            break;
+        }
+        alpha *= 0.5f;
+    }
+    if (a.omega != 1.0f)
+#pragma unroll
+        for (int k = 0; k < D; ++k) cur[k] = pos[k] + a.omega * (cur[k] - pos[k]);
+    if (l == 0) store_node<D>(a, dof, cur);
+}
+
+// ------------------------------------------------------------------ tiles
+
+// Tile of TX (last axis) x TY x TZ nodes; 2^D lanes per node.
+template <int D, int KIND, int TX, int TY, int TZ, bool TAU, bool GRAD>
+__device__ void tile_sweep(const SweepArgs& a)
+{
+    constexpr int NC = 1 << D;
+    constexpr int NT = NC * TX * TY * TZ;
+    constexpr int LX = TX + 2, LY = TY + 2, LZ = (D == 3) ? TZ + 2 : 1;
+    __shared__ float sh[D][LZ * LY * LX];
+    const TileDesc td = a.tiles[blockIdx.x];
+    const BlockDesc bd = a.blocks[td.blk];
+    const long long PL = a.plane;
+    const int tid = threadIdx.x;
+
+    // ---- stage the tile + one-node halo (logical -1 .. n: padded ghosts included)
+    for (int e = tid; e < LZ * LY * LX; e += NT) {
+        const int c = e % LX, q = (e / LX) % LY, r = e / (LX * LY);
+        long long node;
+        if constexpr (D == 2) {
+            const int li = min(td.i0 - 1 + q, bd.n[0]);
+            const int lj = min(td.j0 - 1 + c, bd.n[1]);
+            node = bd.base + (long long)(li + 1) * bd.st[0] + (lj + 1);
+        } else {
+            const int l0 = min(td.i0 - 1 + r, bd.n[0]);
+            const int l1 = min(td.j0 - 1 + q, bd.n[1]);
+            const int l2 = min(td.k0 - 1 + c, bd.n[2]);
+            node = bd.base + (long long)(l0 + 1) * bd.st[0] + (long long)(l1 + 1) * bd.st[1] + (l2 + 1);
+        }
+#pragma unroll
+        for (int k = 0; k < D; ++k) sh[k][e] = a.xin[k * PL + node];
+    }
+    __syncthreads();
+
+    const int cell = tid & (NC - 1);
+    const int nid = tid / NC;
+    const int tx = nid % TX, ty = (nid / TX) % TY, tz = nid / (TX * TY);
+    int lg[3];
+    long long node;
+    if constexpr (D == 2) {
+        lg[0] = td.i0 + ty;
+        lg[1] = td.j0 + tx;
+        lg[2] = 0;
+        if (lg[0] > bd.n[0] - 1 || lg[1] > bd.n[1] - 1) return;
+        node = bd.base + (long long)(lg[0] + 1) * bd.st[0] + (lg[1] + 1);
+    } else {
+        lg[0] = td.i0 + tz;
+        lg[1] = td.j0 + ty;
+        lg[2] = td.k0 + tx;
+        if (lg[0] > bd.n[0] - 1 || lg[1] > bd.n[1] - 1 || lg[2] > bd.n[2] - 1) return;
+        node = bd.base + (long long)(lg[0] + 1) * bd.st[0] + (long long)(lg[1] + 1) * bd.st[1] + (lg[2] + 1);
+    }
+    if (!a.mask[node]) return;  // whole lane group leaves together
+
+    // ---- the lane's cell: corners permuted so that corner 0 is the DOF
+    int cb[3] = {0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < D; ++k) cb[k] = (cell >> (D - 1 - k)) & 1;
+    float rho[D];  // axis sign flip of the permuted frame
+#pragma unroll
+    for (int k = 0; k < D; ++k) rho[k] = cb[k] ? 1.0f : -1.0f;  // r_k = 1 - cb_k; rho = r_k ? -1 : 1
+    float P[NC][D];
+#pragma unroll
+    for (int op = 0; op < NC; ++op) {
+        int loc[3] = {0, 0, 0};
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const int obit = (op >> (D - 1 - k)) & 1;
+            const int o = obit ^ (1 - cb[k]);  // actual corner bit
+            loc[k] = cb[k] + o;
+        }
+        int li;
+        if constexpr (D == 2)
+            li = (ty + loc[0]) * LX + (tx + loc[1]);
+        else
+            li = ((tz + loc[0]) * LY + (ty + loc[1])) * LX + (tx + loc[2]);
+#pragma unroll
+        for (int k = 0; k < D; ++k) P[op][k] = sh[k][li];
+    }
+
+    const float delta = a.delta;
+    // A of permuted sample op with corner point `cp` and the DOF at `x0`
+    auto build = [&](auto OP, const float* x0, float* A) {
+        constexpr int op = decltype(OP)::value;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const int nb = op ^ (1 << (D - 1 - k));
+            const float sk = (((op >> (D - 1 - k)) & 1) ? -1.0f : 1.0f) * rho[k];
+#pragma unroll
+            for (int i = 0; i < D; ++i) {
+                const float pn = (nb == 0 && x0) ? x0[i] : P[nb][i];
+                const float pc = (op == 0 && x0) ? x0[i] : P[op][i];
+                A[i * D + k] = sk * (pn - pc);
+            }
+        }
+    };
+
+    float g[D] = {}, h[kNH<D>] = {}, em = 0.0f, ec = 0.0f, mc = kInf, md = kInf;
+    static_for<NC>([&](auto OP) {
+        constexpr int op = decltype(OP)::value;
+        constexpr int pc = __builtin_popcount(op);
+        float A[D * D];
+        build(OP, nullptr, A);
+        const float det = detD<D>(A);
+        md = fminf(md, det);
+        if constexpr (pc <= 1) {  // the DOF is the corner (op = 0) or an axis neighbour
+            float u[D];
+#pragma unroll
+            for (int c = 0; c < D; ++c) {
+                const float sc = (((op >> (D - 1 - c)) & 1) ? -1.0f : 1.0f) * rho[c];
+                if constexpr (op == 0)
+                    u[c] = -sc;
+                else
+                    u[c] = (op == (1 << (D - 1 - c))) ? sc : 0.0f;
+            }
+            em += sample_grad_hess<D, KIND>(A, u, delta, g, h);
+        } else {
+            ec += sample_value<D, KIND>(A, delta);
+            mc = fminf(mc, det);
+        }
+    });
+#pragma unroll
+    for (int k = 0; k < D; ++k) g[k] = group_sum<NC>(g[k]);
+    float e0 = group_sum<NC>(ec + em);
+    float pos[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) pos[k] = P[0][k];
+    float tau[D] = {};
+    if constexpr (TAU) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            tau[k] = a.tau[k * PL + node];
+            g[k] -= tau[k];
+            e0 -= tau[k] * pos[k];
+        }
+    }
+    if constexpr (GRAD) {
+        if (cell == 0)
+#pragma unroll
+            for (int k = 0; k < D; ++k) a.gout[k * PL + node] = g[k];
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < kNH<D>; ++k) h[k] = group_sum<NC>(h[k]);
+    float dl[D];
+    newton<D>(g, h, dl);
+    float alpha = 1.0f, cur[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) cur[k] = pos[k];
+    for (int it = 0; it < 10; ++it) {
+        float tr[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) tr[k] = pos[k] + alpha * dl[k];
+        float et = 0.0f, mt = mc;
+        static_for<NC>([&](auto OP) {
+            constexpr int op = decltype(OP)::value;
+            if constexpr (__builtin_popcount(op) <= 1) {
+                float A[D * D];
+                build(OP, tr, A);
+                et += sample_value<D, KIND>(A, delta);
+                mt = fminf(mt, detD<D>(A));
+            }
+        });
+        et = group_sum<NC>(ec + et);
+        mt = group_min<NC>(mt);
+        if constexpr (TAU) {
+#pragma unroll
+            for (int k = 0; k < D; ++k) et -= tau[k] * tr[k];
+        }
+        if (accept<KIND>(et, e0, mt)) {
+#pragma unroll
+            for (int k = 0; k < D; ++k) cur[k] = tr[k];
+            break;
+        }
+        alpha *= 0.5f;
+    }
+    if (a.omega != 1.0f)
+#pragma unroll
+        for (int k = 0; k < D; ++k) cur[k] = pos[k] + a.omega * (cur[k] - pos[k]);
+    if (cell == 0) store_node<D>(a, node, cur);
+}
+
+template <int D, int KIND, int TX, int TY, int TZ, bool TAU, bool GRAD>
+__global__ __launch_bounds__((1 << D) * TX * TY * TZ) void sweep_kernel(const SweepArgs a)
+{
+    constexpr int NT = (1 << D) * TX * TY * TZ;
+    constexpr int G = (D == 2) ? 16 : 64;
+    if (int(blockIdx.x) >= a.n_tiles) {
+        const int gid = (int(blockIdx.x) - a.n_tiles) * (NT / G) + int(threadIdx.x) / G;
+        stored_group<D, KIND, TAU, GRAD, G>(a, gid, int(threadIdx.x) % G, gid < a.st.n);
+        return;
+    }
+    tile_sweep<D, KIND, TX, TY, TZ, TAU, GRAD>(a);
+}
+
+template <int D, int KIND, int TX, int TY, int TZ, bool TAU, bool GRAD>
+void launch_variant(const SweepArgs& a, hipStream_t s)
+{
+    constexpr int NT = (1 << D) * TX * TY * TZ;
+    constexpr int G = (D == 2) ? 16 : 64;
+    const int per_wg = NT / G;
+    const int nst = (a.st.n + per_wg - 1) / per_wg;
+    const unsigned grid = (unsigned)(a.n_tiles + nst);
+    if (grid == 0) return;
+    sweep_kernel<D, KIND, TX, TY, TZ, TAU, GRAD><<<dim3(grid), dim3(NT), 0, s>>>(a);
+}
+
+}  // namespace eggmi

@@ -67,11 +67,11 @@ template <int KIND, class R> __host__ __device__ __forceinline__ R mu2_value(R a
     const R D = a * d - b * c;
     const R s = a * a + b * b + c * c + d * d;
     if constexpr (KIND == kShapeSize) {
-        const R cond = s * rcp(R(2) * D);
+        const R cond = R(0.5) * s * rcp(D);
         return cond * cond - R(1) + (D - R(1)) * (D - R(1));
     } else {
         const Phi<KIND, R> ph(D, delta);
-        return s * rcp(R(2) * ph.p) - R(1);
+        return R(0.5) * s * rcp(ph.p) - R(1);  // same operation order as mu2_grad_hess
     }
 }
 
@@ -163,8 +163,9 @@ template <int KIND, class R> __host__ __device__ __forceinline__ R mu3_value(con
     }
     const R D = t[0] * c[0] + t[1] * c[1] + t[2] * c[2];
     const Phi<KIND, R> ph(D, delta);
-    const R ip = rcp(ph.p);
-    R v = s * q * (R(1) / R(9)) * ip * ip - R(1);
+    const R gq = s * q;
+    const R ih = rcp(ph.p * ph.p);
+    R v = gq * (R(1) / R(9)) * ih - R(1);  // same operation order as mu3_grad_hess
     if constexpr (KIND == kShapeSize) v += (D - R(1)) * (D - R(1));
     return v;
 }

@@ -557,3 +557,30 @@ int FN(o_jacobi)(int dim, int kind, double delta, double omega, int n_sweeps, in
     if (xin != X) memcpy(X, xin, sizeof(real) * (size_t)n_nodes * dim);
     return nrep;
 }
+
+/* Per-DOF patch gradient (minus tau when given) into gout[dof*dim + k]
+ * (the FAS restriction input; reference src/sweep.hpp:1425-1514 metric_kernel
+ * with HasTau). */
+void FN(o_patch_grads)(int dim, int kind, double delta, int ndof, const int* dof, const long long* poff,
+                       const int* pgc, const int* pgn, const real* ps, const int* prole, const real* W,
+                       int w_stride, const real* X, const real* tau, real* gout)
+{
+    const long long total = poff[ndof];
+#pragma omp parallel for schedule(static)
+    for (int d = 0; d < ndof; ++d) {
+        const long long o = poff[d];
+        const int P = (int)(poff[d + 1] - o);
+        int gn_l[3 * 128];
+        real s_l[3 * 128];
+        for (int kk = 0; kk < dim; ++kk)
+            for (int p = 0; p < P; ++p) {
+                gn_l[kk * P + p] = pgn[(size_t)kk * total + o + p];
+                s_l[kk * P + p] = ps[(size_t)kk * total + o + p];
+            }
+        FN(patch_t) pv = {P, pgc + o, gn_l, s_l, prole + o, W + (size_t)w_stride * o, w_stride};
+        real grad[3], hess[9], e0, md;
+        FN(patch_eval_impl)(dim, kind, (real)delta, &pv, X, grad, hess, &e0, &md);
+        for (int k = 0; k < dim; ++k)
+            gout[(size_t)dof[d] * dim + k] = grad[k] - (tau ? tau[(size_t)dof[d] * dim + k] : (real)0);
+    }
+}

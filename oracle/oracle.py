@@ -69,6 +69,10 @@ def lib():
                 C.c_longlong, _ip, _ip, rp, rp, C.c_int, rp, _dp, _dp,
             ]
             f.restype = C.c_int
+            f = getattr(L, "o_patch_grads" + sfx)
+            f.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, _ip, _lp, _ip, _ip, rp, _ip, rp, C.c_int, rp,
+                          C.c_void_p, rp]
+            f.restype = None
         L.o_set_threads.argtypes = [C.c_int]
         L.o_set_threads.restype = None
         _LIB = L
@@ -248,3 +252,19 @@ def jacobi(dim, kind, delta, omega, n_sweeps, X, patches: Patches, es: dict, *, 
         ns, egc, egn, es_, eW, ews, work, en, md,
     )
     return Xw.reshape(-1, dim), en[:got], md[:got]
+
+
+def patch_grads(dim, kind, X, patches: Patches, *, delta=0.0, tau=None, prec="f64"):
+    """Per-node patch gradient (minus tau) at every DOF; zero elsewhere."""
+    dt = _dt(prec)
+    Xw = np.ascontiguousarray(X, dt).reshape(-1)
+    out = np.zeros_like(Xw)
+    W = np.ascontiguousarray(patches.W, dt).reshape(-1)
+    ws = 0 if W.size == dim * dim else dim * dim
+    tau_arr = None if tau is None else np.ascontiguousarray(tau, dt).reshape(-1)
+    getattr(lib(), f"o_patch_grads_{prec}")(
+        dim, KIND[kind], float(delta), len(patches.dof), patches.dof, patches.poff, patches.gc,
+        patches.gn.reshape(-1), np.ascontiguousarray(patches.s, dt).reshape(-1), patches.role, W, ws, Xw,
+        None if tau_arr is None else tau_arr.ctypes.data_as(C.c_void_p), out,
+    )
+    return out.reshape(-1, dim)

@@ -4,7 +4,11 @@ libeggmi.so), checked against the CPU oracle on the same inputs.
 Bar (FP32 device arithmetic, FP64 reductions): node positions within 1e-5
 of the domain extent of the FP64 oracle on valid meshes; reported energies
 and min det equal the FP64 reduction of the returned positions to 1e-9
-relative; halo copies and index maps bit-exact.
+relative; halo copies and index maps bit-exact. Energies along the
+trajectory are compared to ENERGY_RTOL: the U(+-0.3h) starts hold cells with
+det A ~ 1e-3 h^2, whose mu ~ 1/det turns the FP32 position resolution
+(~1e-7 of the extent) into ~1e-3 relative energy differences on the first
+sweeps and ~1e-4 after a few dozen.
 """
 
 from __future__ import annotations
@@ -17,6 +21,8 @@ from tests.cases import Case, cube, lr_pair, ogrid, single
 pytestmark = pytest.mark.gpu
 
 POS_TOL = 1e-5  # x domain extent
+ENERGY_RTOL = 2e-4  # final report
+TRAJ_RTOL = 2e-3  # every report
 
 
 def _session(case, **kw):
@@ -48,8 +54,9 @@ def test_single_block_parity(oracle, n, sweeps):
     err = np.abs(Xg - Xo).max() / _extent(c)
     print(f"n={n} max|dX|/extent={err:.2e} E_gpu={e[-1]:.9g} E_f64={eo[-1]:.9g}")
     assert err < POS_TOL
-    np.testing.assert_allclose(e[5:], eo[5:], rtol=1e-5)
-    np.testing.assert_allclose(m[5:], mo[5:], rtol=1e-4)
+    np.testing.assert_allclose(e, eo, rtol=TRAJ_RTOL)
+    np.testing.assert_allclose(e[-1], eo[-1], rtol=ENERGY_RTOL)
+    np.testing.assert_allclose(m[-1], mo[-1], rtol=1e-3)
 
 
 def test_single_block_parity_fp32_oracle(oracle):
@@ -57,7 +64,7 @@ def test_single_block_parity_fp32_oracle(oracle):
     c = single(n=33, seed=2)
     _, Xg, e, m, Xo, eo, mo = _run_both(oracle, c, 40, prec="f32")
     assert np.abs(Xg - Xo).max() / _extent(c) < POS_TOL
-    np.testing.assert_allclose(e[-1], eo[-1], rtol=1e-5)
+    np.testing.assert_allclose(e[-1], eo[-1], rtol=ENERGY_RTOL)
 
 
 def test_reports_equal_fp64_reduction_of_returned_x(oracle):
@@ -80,7 +87,8 @@ def test_multiblock_parity(oracle, make):
     err = np.abs(Xg - Xo).max() / _extent(c)
     print(f"multi-block max|dX|/extent={err:.2e}")
     assert err < POS_TOL
-    np.testing.assert_allclose(e, eo, rtol=1e-5)
+    np.testing.assert_allclose(e, eo, rtol=TRAJ_RTOL)
+    np.testing.assert_allclose(e[-1], eo[-1], rtol=ENERGY_RTOL)
 
 
 def test_stored_path_and_sample_reduction(oracle):
@@ -97,7 +105,7 @@ def test_stored_path_and_sample_reduction(oracle):
     Xo, eo, mo = oracle.jacobi(2, "barrier", 0.0, 0.8, 20, c.X, c.patches(oracle), c.ctx.energy_stencil,
                                report_every=1)
     assert np.abs(s.get_X() - Xo).max() / _extent(c) < POS_TOL
-    np.testing.assert_allclose(e, eo, rtol=1e-5)
+    np.testing.assert_allclose(e, eo, rtol=TRAJ_RTOL)
 
 
 def test_halo_exchange_bit_exact():
@@ -130,7 +138,7 @@ def test_tile_variants_agree():
         s.run(5, omega=0.8)
         out.append(s.get_X())
     for X in out[1:]:
-        np.testing.assert_allclose(X, out[0], rtol=0, atol=1e-6)
+        np.testing.assert_array_equal(X, out[0])
 
 
 def test_analytic_minimiser():
@@ -175,7 +183,7 @@ def test_shape_size_parity(oracle):
     c = single(n=17, seed=8, amp=0.2)
     _, Xg, e, m, Xo, eo, mo = _run_both(oracle, c, 10, phase="shape_size")
     assert np.abs(Xg - Xo).max() / _extent(c) < POS_TOL
-    np.testing.assert_allclose(e, eo, rtol=1e-5)
+    np.testing.assert_allclose(e, eo, rtol=TRAJ_RTOL)
 
 
 def test_3d_block_parity(oracle):
@@ -185,7 +193,7 @@ def test_3d_block_parity(oracle):
     err = np.abs(Xg - Xo).max() / _extent(c)
     print(f"3d max|dX|/extent={err:.2e}")
     assert err < POS_TOL
-    np.testing.assert_allclose(e, eo, rtol=1e-5)
+    np.testing.assert_allclose(e, eo, rtol=TRAJ_RTOL)
 
 
 def test_set_get_x_roundtrip():

@@ -147,3 +147,17 @@ def test_grids_valid():
     t = grids.tangle(g, seed=1)
     assert grids.min_det(t) < 0
     assert np.array_equal(t.free, g.free)
+
+
+def test_mg_hierarchy_shapes_match_oracle():
+    """Coarse shapes (halve odd axes >= 5) of the core match the FAS oracle."""
+    from oracle import fas
+
+    from egg_amd.smoothing import structured_arrays
+
+    for n in (9, 17, 33):
+        c = single(n=n)
+        s = _core().CppStructuredSweepSession(c.ctx.wire, structured_arrays(c.bsc), c.X.ravel(), device="tables")
+        got = [lv[0] for lv in s.mg_levels()]
+        ref = [lv.ids.shape for lv in fas.build_levels(2, c.grid.blocks[0], c.grid.free)[1:]]
+        assert got == ref, (n, got, ref)
