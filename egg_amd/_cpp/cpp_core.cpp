@@ -408,6 +408,25 @@ PYBIND11_MODULE(cpp_core, m)
       .def("set_kernel_timing", &Session::set_kernel_timing, py::arg("on"));
 
     m.def(
+      "pcg_csr",
+      [](const py::array_t<int64_t, py::array::c_style | py::array::forcecast>& rowptr, const IArr& col,
+         const DArr& val, const DArr& b, double rtol, int max_iter) {
+          const int64_t n = (int64_t)b.size();
+          if ((int64_t)rowptr.size() != n + 1) throw py::value_error("pcg: rowptr must have n+1 entries");
+          if (col.size() != val.size()) throw py::value_error("pcg: col and val sizes differ");
+          if (n > 0 && rowptr.data()[n] != (int64_t)col.size()) throw py::value_error("pcg: rowptr[n] != nnz");
+          py::array_t<double> x(n);
+          int iters = 0, status = 0;
+          double relres = 0.0;
+          check(eggmi_pcg_csr(n, rowptr.data(), col.data(), val.data(), b.data(), x.mutable_data(), rtol, max_iter,
+                              &iters, &relres, &status));
+          return py::make_tuple(x, iters, relres, status);
+      },
+      py::arg("rowptr"), py::arg("col"), py::arg("val"), py::arg("b"), py::kw_only(), py::arg("rtol") = 1e-10,
+      py::arg("max_iter") = 200,
+      "Jacobi-preconditioned CG (FP64, on the GPU) for a CSR operator; returns (x, iters, relres, status).");
+
+    m.def(
       "cpp_structured_sweep",
       [](const py::dict& ctx, const py::dict& structured, const DArr& X, int n_sweeps, const std::string& device,
          const std::string& phase, double delta, int dim, double omega, int report_every) {

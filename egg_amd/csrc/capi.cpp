@@ -1,9 +1,12 @@
 // C ABI of the MI355X structured smoother (include/eggmi.h). Converts
 // exceptions to status codes and keeps the calling thread's last message.
 #include "eggmi.h"
+#include "pcg.hpp"
 #include "session.hpp"
 
+#include <algorithm>
 #include <cstring>
+#include <vector>
 #include <new>
 #include <string>
 
@@ -244,6 +247,64 @@ eggmi_status eggmi_set_kernel_timing(eggmi_session* s, int on)
     return guarded([&] {
         need(s, "session");
         s->s.kernel_timing = on != 0;
+    });
+}
+
+eggmi_status eggmi_pcg_csr(int64_t n, const int64_t* rowptr, const int32_t* col, const double* val,
+                           const double* b, double* x, double rtol, int max_iter, int* iters, double* relres,
+                           int* status)
+{
+    return guarded([&] {
+        if (n < 0) throw eggmi::InvalidArgument("pcg: n must be >= 0");
+        need(rowptr, "rowptr");
+        need(b, "b");
+        need(x, "x");
+        if (max_iter < 0) throw eggmi::InvalidArgument("pcg: max_iter must be >= 0");
+        if (!(rtol >= 0.0)) throw eggmi::InvalidArgument("pcg: rtol must be >= 0");
+        const int64_t nnz = n ? rowptr[n] : 0;
+        if (rowptr[0] != 0 || nnz < 0) throw eggmi::InvalidArgument("pcg: rowptr must start at 0");
+        for (int64_t i = 0; i < n; ++i)
+            if (rowptr[i + 1] < rowptr[i]) throw eggmi::InvalidArgument("pcg: rowptr must be non-decreasing");
+        bool has_diag_all = true;
+        for (int64_t i = 0; i < n && has_diag_all; ++i) {
+            bool has = false;
+            for (int64_t q = rowptr[i]; q < rowptr[i + 1]; ++q) {
+                if (col[q] < 0 || col[q] >= n) throw eggmi::InvalidArgument("pcg: column index out of range");
+                if (col[q] == i && val[q] != 0.0) has = true;
+            }
+            has_diag_all = has;
+        }
+        if (!has_diag_all) throw eggmi::InvalidArgument("pcg: every row needs a nonzero diagonal (Jacobi)");
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw eggmi::NoDevice("pcg: no HIP device");
+        hipStream_t s;
+        EGGMI_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        eggmi::DArr<long long> d_rp;
+        eggmi::DArr<int> d_col;
+        eggmi::DArr<double> d_val, d_b, d_x, d_r, d_z, d_p, d_ap, d_dinv, d_part;
+        eggmi::DArr<char> d_state;
+        try {
+            d_rp.upload(std::vector<long long>(rowptr, rowptr + n + 1));
+            d_col.upload(std::vector<int>(col, col + nnz));
+            d_val.upload(std::vector<double>(val, val + nnz));
+            d_b.upload(std::vector<double>(b, b + n));
+            for (auto* a : {&d_x, &d_r, &d_z, &d_p, &d_ap, &d_dinv}) a->alloc(std::max<int64_t>(n, 1));
+            const int64_t grid = (n + 255) / 256;
+            d_part.alloc(2 * std::max<int64_t>(grid, 1));
+            d_state.alloc(eggmi::pcg_state_bytes());
+            eggmi::CsrView A {n, d_rp.p, d_col.p, d_val.p};
+            eggmi::PcgWork w {d_r.p, d_z.p, d_p.p, d_ap.p, d_dinv.p, d_part.p, d_state.p};
+            eggmi::PcgResult res {};
+            if (n > 0) res = eggmi::pcg_solve_device(A, d_b.p, d_x.p, rtol, max_iter, w, s);
+            if (n > 0) EGGMI_HIP(hipMemcpy(x, d_x.p, n * sizeof(double), hipMemcpyDeviceToHost));
+            if (iters) *iters = res.iters;
+            if (relres) *relres = res.relres;
+            if (status) *status = res.status;
+        } catch (...) {
+            (void)hipStreamDestroy(s);
+            throw;
+        }
+        EGGMI_HIP(hipStreamDestroy(s));
     });
 }
 

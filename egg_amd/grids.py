@@ -204,3 +204,104 @@ def tangle(grid: Grid, *, frac: float = 0.02, factor: float = 1.5, seed: int = 0
         if corner_dets(grid.blocks, X, grid.d).min() <= 0:
             return Grid(grid.d, grid.blocks, X, grid.free.copy())
     raise RuntimeError("tangling did not invert any cell")
+
+
+def cubed_sphere_3d(n: int = 65, *, r: float = 1.0, half: float = 2.0) -> Grid:
+    """7-block hex grid: a central n^3 block filling the ball of radius r
+    (the cube [-1, 1]^3 scaled radially onto the ball, so its boundary is
+    the sphere) inside six n^3 shell blocks from the sphere to the cube
+    [-half, half]^3. The sphere is an internal interface; the outer cube is
+    fixed; the central block's 12 edges and 8 corners are singular."""
+    u = np.linspace(-1.0, 1.0, n)
+    A, B, C = np.meshgrid(u, u, u, indexing="ij")
+    xi = np.stack([A, B, C], -1)
+    inf = np.abs(xi).max(-1, keepdims=True)
+    l2 = np.linalg.norm(xi, axis=-1, keepdims=True)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        core = np.where(l2 > 0, r * inf * xi / l2, 0.0)
+    blocks = [core]
+    w = np.linspace(0.0, 1.0, n)
+    for axis in range(3):
+        for sgn in (-1.0, 1.0):
+            a, b = np.meshgrid(u, u, indexing="ij")
+            face = np.zeros(a.shape + (3,))
+            others = [k for k in range(3) if k != axis]
+            face[..., axis] = sgn
+            face[..., others[0]] = a
+            face[..., others[1]] = b
+            inner = r * face / np.linalg.norm(face, axis=-1, keepdims=True)
+            outer = half * face
+            P = (1 - w)[None, None, :, None] * inner[:, :, None, :] + w[None, None, :, None] * outer[:, :, None, :]
+            # orient (tangential a, tangential b, radial) to a positive Jacobian
+            J = np.linalg.det(np.stack([P[1, 0, 0] - P[0, 0, 0], P[0, 1, 0] - P[0, 0, 0], P[0, 0, 1] - P[0, 0, 0]], 1))
+            if J < 0:
+                P = P[::-1]
+            blocks.append(P)
+    ids, X = _merge_fast(blocks, 3)
+    free = np.ones(X.shape[0], bool)
+    free[(np.abs(X) >= half - 1e-9).any(axis=1)] = False
+    return Grid(3, ids, X, free)
+
+
+def _merge_fast(block_coords: list, d: int, scale: float = 1e8):
+    """Vectorized coincident-node merge (rounded coordinates)."""
+    flat = [P.reshape(-1, d) for P in block_coords]
+    allp = np.concatenate(flat)
+    keys = np.round(allp * scale).astype(np.int64)
+    _, first, inv = np.unique(keys, axis=0, return_index=True, return_inverse=True)
+    # number global ids in order of first appearance
+    order = np.argsort(first, kind="stable")
+    rank = np.empty_like(order)
+    rank[order] = np.arange(order.size)
+    gid = rank[inv.reshape(-1)].astype(np.int32)
+    X = allp[np.sort(first)]
+    ids = []
+    o = 0
+    for P in block_coords:
+        m = int(np.prod(P.shape[:-1]))
+        ids.append(gid[o:o + m].reshape(P.shape[:-1]))
+        o += m
+    return ids, X
+
+
+def strip_2d(sides: list[int], ny: int, *, x0: float = 0.0, y0: float = 0.0, h: float | None = None) -> Grid:
+    """Conforming chain of 2D blocks side by side along x: block k has
+    sides[k] x ny nodes (axis 0 = x), consecutive blocks share a whole face.
+    Uniform spacing h (default 1/(ny-1)); the outer boundary is fixed."""
+    h = 1.0 / (ny - 1) if h is None else h
+    blocks, pts = [], []
+    xs = x0
+    for n in sides:
+        i, j = np.meshgrid(np.arange(n), np.arange(ny), indexing="ij")
+        blocks.append(np.stack([xs + i * h, y0 + j * h], -1))
+        xs += (n - 1) * h
+    ids, X = _merge_fast(blocks, 2)
+    x_lo, x_hi = x0, xs
+    y_lo, y_hi = y0, y0 + (ny - 1) * h
+    free = ~(np.isclose(X[:, 0], x_lo) | np.isclose(X[:, 0], x_hi) | np.isclose(X[:, 1], y_lo)
+             | np.isclose(X[:, 1], y_hi))
+    return Grid(2, ids, X, free)
+
+
+def batched_2d(n_grids: int = 64, *, seed: int = 0, min_blocks: int = 3, max_blocks: int = 9,
+               sides=tuple(range(33, 514, 8)), frac: float = 0.02, factor: float = 1.5) -> tuple[Grid, list]:
+    """Many independent multi-block grids in one Grid (disjoint global ids):
+    grid g is a conforming strip of 3..9 blocks whose sides are drawn from
+    {33, 41, ..., 513} (n = 1 mod 8, so three multigrid levels exist), each
+    2 % tangled by 1.5 x the local edge length. Returns the union grid and
+    the per-grid (first block, block count, first node, node count)."""
+    rng = np.random.default_rng(seed)
+    blocks, Xs, frees, index = [], [], [], []
+    node0 = 0
+    for gi in range(n_grids):
+        nb = int(rng.integers(min_blocks, max_blocks + 1))
+        ny = int(rng.choice(sides))
+        nx = [int(rng.choice(sides)) for _ in range(nb)]
+        g = strip_2d(nx, ny, x0=0.0, y0=0.0)
+        g = tangle(g, frac=frac, factor=factor, seed=int(rng.integers(1 << 30)))
+        index.append((len(blocks), nb, node0, g.n_nodes))
+        blocks += [b + node0 for b in g.blocks]
+        Xs.append(g.X)
+        frees.append(g.free)
+        node0 += g.n_nodes
+    return Grid(2, [b.astype(np.int32) for b in blocks], np.concatenate(Xs), np.concatenate(frees)), index

@@ -152,6 +152,17 @@ eggmi_status eggmi_last_timing(eggmi_session* s, double* ms, int64_t* sweep_laun
 /* Bracket every sweep launch with HIP events (measurement only). */
 eggmi_status eggmi_set_kernel_timing(eggmi_session* s, int on);
 
+/* Jacobi-preconditioned conjugate gradients in FP64 on a CSR operator
+ * (rowptr[n+1], col/val[nnz]; SPD expected): the PCG skeleton of the
+ * reference's control-net Gauss-Newton step, pcg_solve
+ * (src/control_solve.hpp:761-818): x0 = 0, stop when |r| <= rtol |b|,
+ * stop on non-positive curvature. x receives the solution; *status is 1
+ * converged, 2 non-positive curvature, 3 zero right-hand side, 0 max_iter
+ * reached. */
+eggmi_status eggmi_pcg_csr(int64_t n, const int64_t* rowptr, const int32_t* col, const double* val,
+                           const double* b, double* x, double rtol, int max_iter, int* iters, double* relres,
+                           int* status);
+
 const char* eggmi_last_error(void);
 
 /* Library liveness (reference cpp_core.ping, src/bindings.cpp:1544). */

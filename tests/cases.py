@@ -59,3 +59,48 @@ def lr_pair(n=9, seed=3):
     rng = np.random.default_rng(seed)
     X[free] += rng.uniform(-0.2 * h, 0.2 * h, size=(int(free.sum()), 2))
     return Case(grids.Grid(2, [a, b], X, free))
+
+
+def block_grid(blocks_spec, fixed_outer=True):
+    """Blocks given as (x0, y0, nx, ny, h) rectangles laid out conformingly on
+    a common lattice; coincident nodes merge; the outer boundary is fixed."""
+    from egg_amd.grids import Grid, _merge_fast
+
+    coords = []
+    for x0, y0, nx, ny, h in blocks_spec:
+        i, j = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+        coords.append(np.stack([x0 + i * h, y0 + j * h], -1))
+    ids, X = _merge_fast(coords, 2)
+    # a node is free when at least three cells surround it (interior and
+    # conforming-interface nodes); block faces on the domain boundary are fixed
+    cells = np.zeros(X.shape[0], np.int64)
+    for b in ids:
+        for sl in ((slice(0, -1), slice(0, -1)), (slice(1, None), slice(0, -1)),
+                   (slice(0, -1), slice(1, None)), (slice(1, None), slice(1, None))):
+            np.add.at(cells, b[sl].reshape(-1), 1)
+    return Grid(2, ids, X, cells >= 3)
+
+
+def bump(grid, amp):
+    """Smooth low-frequency displacement vanishing on the boundary (the error
+    component plain Jacobi damps slowest; reference tests/smoothing/
+    test_fas.py:51-61)."""
+    X = grid.X.copy()
+    lo = X.min(0)
+    span = X.max(0) - lo
+    s = np.sin(np.pi * (X - lo) / span)
+    b = s[:, 0] * s[:, 1]
+    X[:, 0] += amp * b
+    X[:, 1] += 0.7 * amp * b
+    from egg_amd.grids import Grid
+
+    return Grid(2, grid.blocks, X, grid.free)
+
+
+def fas_single(res=16, amp=0.08):
+    return Case(bump(block_grid([(0.0, 0.0, res + 1, res + 1, 4.0 / res)]), amp))
+
+
+def fas_two_blocks(res=16, amp=0.15):
+    h = 4.0 / res
+    return Case(bump(block_grid([(0.0, 0.0, res + 1, res + 1, h), (4.0, 0.0, res + 1, res + 1, h)]), amp))
