@@ -38,7 +38,7 @@ struct StoredView {
     long long total = 0;       // occurrences
     const int* gc = nullptr;   // [total] corner slot
     const int* gn = nullptr;   // [dim][total] neighbour slots
-    const uint8_t* meta = nullptr;  // [total] bits 0-1: role+1, bits 2..4: sign of axis k (1 = -1)
+    const uint8_t* meta = nullptr;  // [total] bits 0-2: role+1 (0..D), bits 3..5: sign of axis k (1 = -1)
     const float* W = nullptr;       // [total][dim*dim] per-occurrence W_inv rows, or null (uniform)
 };
 

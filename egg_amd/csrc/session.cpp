@@ -502,9 +502,12 @@ void Session::create(int d, const eggmi_group* groups, int n_groups, const eggmi
                 for (long long p = off, q = 0; p < off + P; ++p, ++q) {
                     L.st_gc.push_back((int)occ_slots[q * (d + 1)]);
                     for (int k = 0; k < d; ++k) gn_rows.push_back((int)occ_slots[q * (d + 1) + 1 + k]);
-                    uint8_t meta = (uint8_t)((g.role[p] + 1) & 3);
+                    if (g.role[p] < -1 || g.role[p] > d)
+                        throw InvalidArgument("structured: role " + std::to_string(g.role[p]) + " outside [-1, " +
+                                              std::to_string(d) + "]");
+                    uint8_t meta = (uint8_t)(g.role[p] + 1);  // 0..D: 3 bits
                     for (int k = 0; k < d; ++k)
-                        if (g.s[k][p] < 0) meta |= (uint8_t)(4 << k);
+                        if (g.s[k][p] < 0) meta |= (uint8_t)(8 << k);
                     L.st_meta.push_back(meta);
                     if (!w_uniform_all) {
                         const double* w = g.w_rows == 1 ? g.W_inv : g.W_inv + p * dd;

@@ -125,7 +125,7 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
 #pragma unroll
         for (int k = 0; k < D; ++k) {
             const int in = st.gn[k * st.total + p];
-            s[k] = (m & (4 << k)) ? -1.0f : 1.0f;
+            s[k] = (m & (8 << k)) ? -1.0f : 1.0f;
 #pragma unroll
             for (int i = 0; i < D; ++i) nb[k][i] = (sub && in == dof) ? sub[i] : X[i * PL + in];
         }
@@ -152,7 +152,7 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
         float T[D * D], det, s[D];
         build(p, nullptr, T, det, s);
         md = fminf(md, det);
-        const int role = (st.meta[p] & 3) - 1;
+        const int role = (st.meta[p] & 7) - 1;
         if (role >= 0) {
             const float* w = st.W ? st.W + (long long)(D * D) * p : a.Wu;
             float u[D];
