@@ -367,6 +367,13 @@ class Session
     }
 
     void set_kernel_timing(bool on) { check(eggmi_set_kernel_timing(s_, on ? 1 : 0)); }
+    void set_counters(bool on) { check(eggmi_set_counters(s_, on ? 1 : 0)); }
+    py::array_t<int64_t> last_counters(int n_levels)
+    {
+        py::array_t<int64_t> out({n_levels, 2});
+        check(eggmi_last_counters(s_, out.mutable_data(), n_levels));
+        return out;
+    }
 
   private:
     int dim_;
@@ -405,7 +412,9 @@ PYBIND11_MODULE(cpp_core, m)
       .def("halo_exchange", &Session::halo_exchange)
       .def("set_tile_variant", &Session::set_tile_variant, py::arg("variant"))
       .def("last_timing", &Session::last_timing)
-      .def("set_kernel_timing", &Session::set_kernel_timing, py::arg("on"));
+      .def("set_kernel_timing", &Session::set_kernel_timing, py::arg("on"))
+      .def("set_counters", &Session::set_counters, py::arg("on"))
+      .def("last_counters", &Session::last_counters, py::arg("n_levels") = 1);
 
     m.def(
       "pcg_csr",

@@ -250,6 +250,25 @@ eggmi_status eggmi_set_kernel_timing(eggmi_session* s, int on)
     });
 }
 
+eggmi_status eggmi_set_counters(eggmi_session* s, int on)
+{
+    return guarded([&] {
+        need(s, "session");
+        s->s.counting = on != 0;
+    });
+}
+
+eggmi_status eggmi_last_counters(eggmi_session* s, int64_t* out, int n_levels)
+{
+    return guarded([&] {
+        need(s, "session");
+        need(out, "out");
+        if (n_levels < 0) throw eggmi::InvalidArgument("n_levels must be >= 0");
+        const auto& c = s->s.last_counters;
+        for (int i = 0; i < 2 * n_levels; ++i) out[i] = i < (int)c.size() ? (int64_t)c[i] : 0;
+    });
+}
+
 eggmi_status eggmi_pcg_csr(int64_t n, const int64_t* rowptr, const int32_t* col, const double* val,
                            const double* b, double* x, double rtol, int max_iter, int* iters, double* relres,
                            int* status)

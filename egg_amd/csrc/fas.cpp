@@ -433,7 +433,9 @@ void Session::smooth(Level& L, int sweeps, int kind, float omega, const float* t
     for (int s = 0; s < sweeps; ++s) {
         float* xin = L.x[L.cur].p;
         float* xout = L.x[1 - L.cur].p;
-        launch_sweep(dim, kind, tile_variant, L.sweep_args(xin, xout, omega, 0.0f, tau), stream);
+        SweepArgs a = L.sweep_args(xin, xout, omega, 0.0f, tau);
+        a.counters = counters_for(L);
+        launch_sweep(dim, kind, tile_variant, a, stream);
         L.cur = 1 - L.cur;
         ++last_launches;
     }
@@ -529,6 +531,7 @@ std::pair<std::vector<double>, std::vector<double>> Session::run_fas(int n_cycle
     if ((int)d_reports.n < 2 * n_cycles) d_reports.alloc(2 * n_cycles);
     if (!d_ls.n) d_ls.alloc(4);
     last_launches = 0;
+    counters_begin();
     EGGMI_HIP(hipEventRecord(ev0, stream));
     // consistent halo on the resident state, both fine Jacobi buffers equal
     halo(F, F.x[F.cur].p);
@@ -593,6 +596,7 @@ std::pair<std::vector<double>, std::vector<double>> Session::run_fas(int n_cycle
     std::vector<double> hr(2 * n_cycles);
     EGGMI_HIP(hipMemcpyAsync(hr.data(), d_reports.p, sizeof(double) * 2 * n_cycles, hipMemcpyDeviceToHost, stream));
     EGGMI_HIP(hipStreamSynchronize(stream));
+    counters_end();
     float ms = 0;
     EGGMI_HIP(hipEventElapsedTime(&ms, ev0, ev1));
     last_ms = ms;

@@ -58,6 +58,8 @@ struct SweepArgs {
     const int* cp_off = nullptr;  // [slots+1] replicas of each slot (push copies), null = none
     const int* cp_dst = nullptr;
     float Wu[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};  // uniform W_inv for stored occurrences
+    // optional counters (null = off): [0] line-search trials evaluated, [1] DOF updates
+    unsigned long long* counters = nullptr;
 };
 
 // Energy / min-det reduction over the structured cell stencil.

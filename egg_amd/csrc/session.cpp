@@ -703,8 +703,10 @@ std::pair<std::vector<double>, std::vector<double>> Session::run_jacobi(int n_sw
     EGGMI_HIP(hipMemcpyAsync(xout, xin, bytes, hipMemcpyDeviceToDevice, stream));
     int rep = 0;
     if (kernel_timing) ensure_timing_events(2 * n_sweeps);
+    counters_begin();
     for (int s = 0; s < n_sweeps; ++s) {
-        const SweepArgs a = L.sweep_args(xin, xout, (float)omega, (float)delta, nullptr);
+        SweepArgs a = L.sweep_args(xin, xout, (float)omega, (float)delta, nullptr);
+        a.counters = counters_for(L);
         if (kernel_timing) EGGMI_HIP(hipEventRecord(tev[2 * s], stream));
         launch_sweep(dim, kind, tile_variant, a, stream);
         if (kernel_timing) EGGMI_HIP(hipEventRecord(tev[2 * s + 1], stream));
@@ -719,6 +721,7 @@ std::pair<std::vector<double>, std::vector<double>> Session::run_jacobi(int n_sw
     std::vector<double> hr(2 * nrep);
     EGGMI_HIP(hipMemcpyAsync(hr.data(), d_reports.p, sizeof(double) * 2 * nrep, hipMemcpyDeviceToHost, stream));
     EGGMI_HIP(hipStreamSynchronize(stream));
+    counters_end();
     float ms = 0;
     EGGMI_HIP(hipEventElapsedTime(&ms, ev0, ev1));
     last_ms = ms;
@@ -731,6 +734,34 @@ std::pair<std::vector<double>, std::vector<double>> Session::run_jacobi(int n_sw
             last_kernel_ms += k;
         }
     return {std::vector<double>(hr.begin(), hr.begin() + nrep), std::vector<double>(hr.begin() + nrep, hr.end())};
+}
+
+unsigned long long* Session::counters_for(const Level& L)
+{
+    if (!counting) return nullptr;
+    if (&L == &fine) return d_counters.p;
+    for (size_t i = 0; i < coarse.size() && i + 1 < (size_t)kMaxCounterLevels; ++i)
+        if (coarse[i].get() == &L) return d_counters.p + 2 * (i + 1);
+    return nullptr;
+}
+
+void Session::counters_begin()
+{
+    if (!counting) return;
+    if (d_counters.n == 0) d_counters.alloc(2 * kMaxCounterLevels);
+    EGGMI_HIP(hipMemsetAsync(d_counters.p, 0, 2 * kMaxCounterLevels * sizeof(unsigned long long), stream));
+}
+
+// Called after the stream has been synchronised.
+void Session::counters_end()
+{
+    if (!counting) {
+        last_counters.clear();
+        return;
+    }
+    std::vector<unsigned long long> h(2 * kMaxCounterLevels);
+    EGGMI_HIP(hipMemcpy(h.data(), d_counters.p, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    last_counters.assign(h.begin(), h.end());
 }
 
 void Session::get_X(double* out) const

@@ -179,6 +179,14 @@ struct Session {
     double last_kernel_ms = 0.0;    // summed sweep-kernel time of the last run
     std::vector<hipEvent_t> tev;
     void ensure_timing_events(size_t n);
+    // diagnostic counters per level: [2*level] trials evaluated, [2*level+1] DOF updates
+    static constexpr int kMaxCounterLevels = 32;
+    bool counting = false;
+    DArr<unsigned long long> d_counters;
+    std::vector<long long> last_counters;
+    unsigned long long* counters_for(const Level& L);
+    void counters_begin();
+    void counters_end();
     // FAS hierarchy (built lazily)
     std::vector<std::unique_ptr<Level>> coarse;
     int built_levels = -1;

@@ -95,6 +95,15 @@ __device__ __forceinline__ void store_node(const SweepArgs& a, long long node, c
     }
 }
 
+// Diagnostic counters: trials evaluated and DOF updates (off when null).
+__device__ __forceinline__ void count_update(const SweepArgs& a, int ntr)
+{
+    if (a.counters) {
+        atomicAdd(a.counters, (unsigned long long)ntr);
+        atomicAdd(a.counters + 1, 1ull);
+    }
+}
+
 // ------------------------------------------------------------------ stored
 
 // Stored-stencil DOF d handled by a group of G lanes (lane l owns
@@ -196,6 +205,7 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
     float dl[D];
     newton<D>(g, h, dl);
     float alpha = 1.0f, cur[D];
+    int ntr = 10;  // trials evaluated (10 = all rejected)
 #pragma unroll
     for (int k = 0; k < D; ++k) cur[k] = pos[k];
     for (int it = 0; it < 10; ++it) {
@@ -218,6 +228,7 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
         if (accept<KIND>(et, e0, mt)) {
 #pragma unroll
             for (int k = 0; k < D; ++k) cur[k] = tr[k];
+            ntr = it + 1;
             break;
         }
         alpha *= 0.5f;
@@ -225,7 +236,10 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
     if (a.omega != 1.0f)
 #pragma unroll
         for (int k = 0; k < D; ++k) cur[k] = pos[k] + a.omega * (cur[k] - pos[k]);
-    if (l == 0) store_node<D>(a, dof, cur);
+    if (l == 0) {
+        store_node<D>(a, dof, cur);
+        count_update(a, ntr);
+    }
 }
 
 // ------------------------------------------------------------------ tiles
@@ -375,6 +389,7 @@ __device__ void tile_sweep(const SweepArgs& a)
     float dl[D];
     newton<D>(g, h, dl);
     float alpha = 1.0f, cur[D];
+    int ntr = 10;  // trials evaluated (10 = all rejected)
 #pragma unroll
     for (int k = 0; k < D; ++k) cur[k] = pos[k];
     for (int it = 0; it < 10; ++it) {
@@ -400,6 +415,7 @@ __device__ void tile_sweep(const SweepArgs& a)
         if (accept<KIND>(et, e0, mt)) {
 #pragma unroll
             for (int k = 0; k < D; ++k) cur[k] = tr[k];
+            ntr = it + 1;
             break;
         }
         alpha *= 0.5f;
@@ -407,7 +423,10 @@ __device__ void tile_sweep(const SweepArgs& a)
     if (a.omega != 1.0f)
 #pragma unroll
         for (int k = 0; k < D; ++k) cur[k] = pos[k] + a.omega * (cur[k] - pos[k]);
-    if (cell == 0) store_node<D>(a, node, cur);
+    if (cell == 0) {
+        store_node<D>(a, node, cur);
+        count_update(a, ntr);
+    }
 }
 
 template <int D, int KIND, int TX, int TY, int TZ, bool TAU, bool GRAD>

@@ -152,6 +152,15 @@ eggmi_status eggmi_last_timing(eggmi_session* s, double* ms, int64_t* sweep_laun
 /* Bracket every sweep launch with HIP events (measurement only). */
 eggmi_status eggmi_set_kernel_timing(eggmi_session* s, int on);
 
+/* Diagnostic counters (measurement only; no reference counterpart): when on,
+ * every sweep launch counts the line-search trials it evaluated and the DOF
+ * updates it made, per multigrid level. */
+eggmi_status eggmi_set_counters(eggmi_session* s, int on);
+
+/* Counters of the last run / run_fas: out[2*l] trials and out[2*l+1] DOF
+ * updates of level l (0 = fine), for l < n_levels; zeros when counting was off. */
+eggmi_status eggmi_last_counters(eggmi_session* s, int64_t* out, int n_levels);
+
 /* Jacobi-preconditioned conjugate gradients in FP64 on a CSR operator
  * (rowptr[n+1], col/val[nnz]; SPD expected): the PCG skeleton of the
  * reference's control-net Gauss-Newton step, pcg_solve

@@ -1,0 +1,97 @@
+"""Pin E*, the FP64 plain-method converged energy of a benchmark workload
+(TEST INFRASTRUCTURE ONLY: writes a fixture under tests/golden/).
+
+The workload is rebuilt from its generator and seed, untangled with the
+delta-continuation schedule of the reference's cpp_untangle
+(egg/smoothing/cpp_backend.py:773-848: delta_0 = 2 |min det|, delta *= 0.8,
+20 untangle sweeps per delta at omega 0.5, stop once min det > 1e-9) and
+then smoothed by FP64 block-Jacobi sweeps (omega 0.8) of the CPU oracle
+until the energy changes by less than ``--stop`` (relative) over a window
+of ``--window`` sweeps. E* is the energy at that point; the fixture also
+records how it was reached.
+
+  python -m oracle.pin_energy --workload ogrid129 [--threads 8]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def workload(name):
+    from egg_amd import grids
+
+    if name == "ogrid129":
+        return grids.tangle(grids.ogrid_2d(129, 65), frac=0.02, factor=1.5, seed=2026), 2026
+    if name == "ogrid33":
+        return grids.tangle(grids.ogrid_2d(33, 17), frac=0.02, factor=1.5, seed=7), 7
+    raise ValueError(name)
+
+
+def main():
+    from egg_amd.smoothing import build_sweep_context
+
+    from . import oracle as O
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="ogrid129")
+    ap.add_argument("--threads", type=int, default=8)
+    ap.add_argument("--window", type=int, default=2000)
+    ap.add_argument("--stop", type=float, default=1e-10)
+    ap.add_argument("--max-sweeps", type=int, default=400000)
+    args = ap.parse_args()
+
+    g, seed = workload(args.workload)
+    d = g.d
+    ctx = build_sweep_context(g.blocks, g.free, d)
+    P = O.materialize_patches(ctx.wire, g.blocks, d)
+    es = ctx.energy_stencil
+    X = g.X.astype(np.float64)
+    t0 = time.time()
+    _, md = O.energy_mindet(d, "barrier", 0.0, es, X)
+    delta = 2.0 * max(abs(md), 1e-12)
+    outer = 0
+    while md <= 1e-9 and outer < 60:
+        X, _, ms = O.jacobi(d, "untangle", delta, 0.5, 20, X, P, es, report_every=20, threads=args.threads)
+        md = float(ms[-1])
+        outer += 1
+        delta *= 0.8
+    print(f"untangled: outer={outer} min det={md:.3e} ({time.time() - t0:.1f}s)", flush=True)
+    E_prev = None
+    sweeps = 0
+    hist = []
+    while sweeps < args.max_sweeps:
+        X, es_, ms_ = O.jacobi(d, "barrier", 0.0, 0.8, args.window, X, P, es, report_every=args.window,
+                               threads=args.threads)
+        sweeps += args.window
+        E = float(es_[-1])
+        hist.append((sweeps, E))
+        print(f"sweeps={sweeps} E={E:.15g} md={float(ms_[-1]):.3e} ({time.time() - t0:.0f}s)", flush=True)
+        if E_prev is not None and abs(E_prev - E) <= args.stop * abs(E):
+            break
+        E_prev = E
+    out = {
+        "workload": args.workload,
+        "seed": seed,
+        "E_star": E,
+        "min_det": float(ms_[-1]),
+        "untangle_outer": outer,
+        "sweeps": sweeps,
+        "stop": f"|dE|/E <= {args.stop} over {args.window} sweeps",
+        "history": hist[-10:],
+        "how": "FP64 CPU oracle: delta-continuation untangle, then block-Jacobi sweeps omega 0.8",
+    }
+    dst = ROOT / "tests" / "golden" / f"estar_{args.workload}.json"
+    dst.write_text(json.dumps(out, indent=1))
+    print("wrote", dst)
+
+
+if __name__ == "__main__":
+    main()

@@ -104,3 +104,33 @@ def fas_single(res=16, amp=0.08):
 def fas_two_blocks(res=16, amp=0.15):
     h = 4.0 / res
     return Case(bump(block_grid([(0.0, 0.0, res + 1, res + 1, h), (4.0, 0.0, res + 1, res + 1, h)]), amp))
+
+
+def halo_pair(n0=4, n1=3):
+    """Two n0 x n1 blocks sharing a face along axis 0, seeded with the
+    continuous field node (i, j) -> (x0 + i, j): block 0 has x0 = 0, block 1
+    x0 = n0 - 1 (its row i = 0 is block 0's row n0 - 1). The known-answer
+    layout of the reference's tests/cpp/test_halo_device.cpp:28-76."""
+    a = np.arange(n0 * n1, dtype=np.int32).reshape(n0, n1)
+    b = np.concatenate([a[-1:, :], np.arange(n0 * n1, n0 * n1 + (n0 - 1) * n1, dtype=np.int32).reshape(n0 - 1, n1)])
+    N = n0 * n1 + (n0 - 1) * n1
+    X = np.zeros((N, 2))
+    for blk, x0 in ((a, 0.0), (b, float(n0 - 1))):
+        i, j = np.meshgrid(np.arange(n0), np.arange(n1), indexing="ij")
+        X[blk.reshape(-1)] = np.stack([x0 + i, j], -1).reshape(-1, 2)
+    free = np.zeros(N, bool)
+    free[a[1:, 1:-1].reshape(-1)] = True
+    free[b[1:-1, 1:-1].reshape(-1)] = True
+    return Case(grids.Grid(2, [a, b], X, free))
+
+
+def halo_pair_expected(n0=4, n1=3):
+    """(block, padded index) -> expected coordinate after one exchange: the
+    ghost past block 0's high face holds block 1's first interior layer
+    (x = n0, the extrapolation of block 0's own field), the ghost before block
+    1's low face block 0's layer n0 - 2 (x = n0 - 2)."""
+    out = {}
+    for j in range(n1):
+        out[(0, (n0 + 1, j + 1))] = (float(n0), float(j))
+        out[(1, (0, j + 1))] = (float(n0 - 2), float(j))
+    return out

@@ -108,6 +108,27 @@ def test_stored_path_and_sample_reduction(oracle):
     np.testing.assert_allclose(e, eo, rtol=TRAJ_RTOL)
 
 
+def test_halo_pair_known_answer():
+    """The reference's device exchange test (tests/cpp/test_halo_device.cpp:
+    82-136) on the MI355X: after one exchange every face ghost holds the
+    extrapolated coordinate exactly, and the shared face nodes are unchanged."""
+    from tests.cases import halo_pair, halo_pair_expected
+
+    c = halo_pair()
+    s = _session(c)
+    core = s.core
+    core.halo_exchange()
+    P = core.packed()
+    base = core.tables()["block_base"]
+    stride = 3 + 2
+    for (b, (p0, p1)), want in halo_pair_expected().items():
+        slot = int(base[b]) + p0 * stride + p1
+        assert (float(P[0, slot]), float(P[1, slot])) == want, (b, p0, p1)
+    for b, i in ((0, 3), (1, 0)):
+        for j in range(3):
+            assert float(P[0, int(base[b]) + (i + 1) * stride + (j + 1)]) == 3.0
+
+
 def test_halo_exchange_bit_exact():
     """After the fused copy every ghost / copy / mirror slot holds its source's
     coordinates bit for bit, and every interior slot its node's FP32 value."""

@@ -10,7 +10,7 @@ from pathlib import Path
 import numpy as np
 import pytest
 
-from tests.cases import lr_pair, ogrid, single
+from tests.cases import halo_pair, halo_pair_expected, lr_pair, ogrid, single
 
 ROOT = Path(__file__).resolve().parent.parent
 
@@ -77,6 +77,30 @@ def test_remap_tables_match_oracle(make):
     np.testing.assert_array_equal(tab["g2s"], ref["g2s"])
     np.testing.assert_array_equal(tab["block_base"], ref["base"])
     assert info["n_stored"] == len(ref["stored"])
+
+
+def test_halo_pair_known_answer_tables():
+    """The reference's two-block exchange (tests/cpp/test_halo_device.cpp:
+    82-136): each face ghost is sourced from the neighbour block's first
+    interior layer, whose global coordinate is the block's own field
+    extrapolated one node past its face."""
+    c = halo_pair()
+    info, tab = _tables(c)
+    n1 = 3
+    # the 2 x n1 face ghosts, plus the twin-matching builder's entries for the
+    # face corners seen from the side faces (cpp_backend.py:200-236 resolves
+    # them through the same twin; no patch reads them)
+    assert info["n_halo"] == 2 * n1 + 4
+    stride = n1 + 2
+    slot2node = {}
+    for b, ids in enumerate(c.grid.blocks):
+        for (i, j), g in np.ndenumerate(ids):
+            slot2node[int(tab["block_base"][b]) + (i + 1) * stride + (j + 1)] = int(g)
+    src_of = dict(zip(tab["copy_dst"][: info["n_halo"]].tolist(), tab["copy_src"][: info["n_halo"]].tolist()))
+    for (b, (p0, p1)), want in halo_pair_expected().items():
+        dst = int(tab["block_base"][b]) + p0 * stride + p1
+        assert dst in src_of, (b, p0, p1)
+        np.testing.assert_array_equal(c.X[slot2node[src_of[dst]]], want)
 
 
 def test_ogrid_halo_tables_shape():
