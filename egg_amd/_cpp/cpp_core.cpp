@@ -225,6 +225,98 @@ py::array_t<double> to_array(const std::vector<double>& v)
     return py::array_t<double>((py::ssize_t)v.size(), v.data());
 }
 
+using FArr = py::array_t<float, py::array::c_style | py::array::forcecast>;
+
+// Learned proposal network (eggmi_mlp_*).
+class Mlp
+{
+  public:
+    Mlp(int dim, uint64_t seed, int64_t capacity)
+    {
+        check(eggmi_mlp_create(require_dim(dim), seed, capacity, &m_));
+        check(eggmi_mlp_shape(m_, shape_, &np_, nullptr, nullptr));
+    }
+    ~Mlp()
+    {
+        if (m_) eggmi_mlp_destroy(m_);
+    }
+    Mlp(const Mlp&) = delete;
+    Mlp& operator=(const Mlp&) = delete;
+    eggmi_mlp* handle() const { return m_; }
+    int dim() const { return shape_[0]; }
+    py::dict shape() const
+    {
+        int64_t nd = 0, cap = 0;
+        check(eggmi_mlp_shape(m_, nullptr, nullptr, &nd, &cap));
+        py::dict d;
+        d["dim"] = shape_[0];
+        d["n_in"] = shape_[1];
+        d["n_in_padded"] = shape_[2];
+        d["hidden"] = shape_[3];
+        d["n_out"] = shape_[4];
+        d["n_params"] = np_;
+        d["n_data"] = nd;
+        d["capacity"] = cap;
+        return d;
+    }
+    py::array_t<float> get_params() const
+    {
+        py::array_t<float> out((py::ssize_t)np_);
+        check(eggmi_mlp_get_params(m_, out.mutable_data(), np_));
+        return out;
+    }
+    void set_params(const FArr& p)
+    {
+        if (p.ndim() != 1) throw py::value_error("params must be 1-D");
+        check(eggmi_mlp_set_params(m_, p.data(), (int64_t)p.size()));
+    }
+    int64_t rows_of(const FArr& X, int width, const char* what) const
+    {
+        if (X.ndim() != 2 || X.shape(1) != width)
+            throw py::value_error(std::string(what) + " must have shape (rows, " + std::to_string(width) + ")");
+        return (int64_t)X.shape(0);
+    }
+    py::array_t<float> forward(const FArr& X) const
+    {
+        const int64_t r = rows_of(X, shape_[2], "X");
+        py::array_t<float> Y({(py::ssize_t)r, (py::ssize_t)shape_[4]});
+        check(eggmi_mlp_forward(m_, X.data(), r, Y.mutable_data()));
+        return Y;
+    }
+    void data_add(const FArr& X, const FArr& T)
+    {
+        const int64_t r = rows_of(X, shape_[2], "X");
+        if (rows_of(T, shape_[4], "T") != r) throw py::value_error("X and T row counts differ");
+        check(eggmi_mlp_data_add(m_, X.data(), T.data(), r));
+    }
+    py::tuple data_get(int64_t offset, int64_t rows) const
+    {
+        py::array_t<float> X({(py::ssize_t)rows, (py::ssize_t)shape_[2]});
+        py::array_t<float> T({(py::ssize_t)rows, (py::ssize_t)shape_[4]});
+        check(eggmi_mlp_data_get(m_, X.mutable_data(), T.mutable_data(), offset, rows));
+        return py::make_tuple(X, T);
+    }
+    void data_clear() { check(eggmi_mlp_data_clear(m_)); }
+    py::tuple gradient(const IArr& rows)
+    {
+        py::array_t<float> g((py::ssize_t)np_);
+        double loss = 0.0;
+        check(eggmi_mlp_gradient(m_, rows.data(), (int64_t)rows.size(), g.mutable_data(), &loss));
+        return py::make_tuple(g, loss);
+    }
+    py::array_t<double> train(int epochs, int64_t batch, float lr, uint64_t seed)
+    {
+        py::array_t<double> h((py::ssize_t)std::max(0, epochs));
+        check(eggmi_mlp_train(m_, epochs, batch, lr, seed, h.mutable_data()));
+        return h;
+    }
+
+  private:
+    eggmi_mlp* m_ = nullptr;
+    int32_t shape_[5] = {};
+    int64_t np_ = 0;
+};
+
 class Session
 {
   public:
@@ -370,15 +462,40 @@ class Session
     void set_counters(bool on) { check(eggmi_set_counters(s_, on ? 1 : 0)); }
     py::array_t<int64_t> last_counters(int n_levels)
     {
-        py::array_t<int64_t> out({n_levels, 2});
+        py::array_t<int64_t> out({n_levels, 3});
         check(eggmi_last_counters(s_, out.mutable_data(), n_levels));
         return out;
+    }
+    int64_t harvest(Mlp& m, int n_snap, int every, int horizon, double omega)
+    {
+        int64_t added = 0;
+        check(eggmi_session_harvest(s_, m.handle(), n_snap, every, horizon, omega, &added));
+        return added;
+    }
+    void set_proposal(const py::object& m, const std::string& mode, float scale)
+    {
+        int md;
+        if (mode == "replace")
+            md = 0;
+        else if (mode == "predict_correct")
+            md = 1;
+        else
+            throw py::value_error("proposal mode must be 'replace' or 'predict_correct'; got '" + mode + "'");
+        if (m.is_none()) {
+            check(eggmi_session_set_proposal(s_, nullptr, md, scale));
+            proposal_ = py::none();
+            return;
+        }
+        Mlp& net = m.cast<Mlp&>();
+        check(eggmi_session_set_proposal(s_, net.handle(), md, scale));
+        proposal_ = m;  // the session borrows the network: keep it alive
     }
 
   private:
     int dim_;
     size_t n_ = 0;
     eggmi_session* s_ = nullptr;
+    py::object proposal_ = py::none();
 };
 
 }  // namespace
@@ -414,7 +531,31 @@ PYBIND11_MODULE(cpp_core, m)
       .def("last_timing", &Session::last_timing)
       .def("set_kernel_timing", &Session::set_kernel_timing, py::arg("on"))
       .def("set_counters", &Session::set_counters, py::arg("on"))
-      .def("last_counters", &Session::last_counters, py::arg("n_levels") = 1);
+      .def("last_counters", &Session::last_counters, py::arg("n_levels") = 1,
+           "Counters of the last run (counting on): rows per level of (line-search trials, DOF updates, "
+           "accepted learned proposals).")
+      .def("harvest", &Session::harvest, py::arg("mlp"), py::arg("n_snap"), py::kw_only(), py::arg("every") = 8,
+           py::arg("horizon") = 8, py::arg("omega") = 0.8,
+           "Append (features, displacement over `horizon` plain sweeps) pairs of every interior DOF to the "
+           "network's training set; returns the number of pairs added.")
+      .def("set_proposal", &Session::set_proposal, py::arg("mlp"), py::kw_only(), py::arg("mode") = "replace",
+           py::arg("scale") = 1.0f,
+           "Use a trained network (or None) as the learned proposal of subsequent run() calls.");
+
+    py::class_<Mlp>(m, "ProposalMLP", "Learned Newton-step proposal network (BF16 matrix cores, FP32 master).")
+      .def(py::init<int, uint64_t, int64_t>(), py::arg("dim") = 2, py::arg("seed") = 0,
+           py::arg("capacity") = 0)
+      .def_property_readonly("dim", &Mlp::dim)
+      .def("shape", &Mlp::shape)
+      .def("get_params", &Mlp::get_params)
+      .def("set_params", &Mlp::set_params, py::arg("params"))
+      .def("forward", &Mlp::forward, py::arg("X"))
+      .def("data_add", &Mlp::data_add, py::arg("X"), py::arg("T"))
+      .def("data_get", &Mlp::data_get, py::arg("offset"), py::arg("rows"))
+      .def("data_clear", &Mlp::data_clear)
+      .def("gradient", &Mlp::gradient, py::arg("rows"))
+      .def("train", &Mlp::train, py::arg("epochs"), py::kw_only(), py::arg("batch") = 4096,
+           py::arg("lr") = 1e-3, py::arg("seed") = 0);
 
     m.def(
       "pcg_csr",

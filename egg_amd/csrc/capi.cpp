@@ -1,10 +1,13 @@
 // C ABI of the MI355X structured smoother (include/eggmi.h). Converts
 // exceptions to status codes and keeps the calling thread's last message.
 #include "eggmi.h"
+#include "mlp_host.hpp"
 #include "pcg.hpp"
 #include "session.hpp"
 
 #include <algorithm>
+#include <cmath>
+#include <memory>
 #include <cstring>
 #include <vector>
 #include <new>
@@ -12,6 +15,10 @@
 
 struct eggmi_session {
     eggmi::Session s;
+};
+
+struct eggmi_mlp {
+    std::unique_ptr<eggmi::Mlp> m;
 };
 
 namespace
@@ -265,7 +272,152 @@ eggmi_status eggmi_last_counters(eggmi_session* s, int64_t* out, int n_levels)
         need(out, "out");
         if (n_levels < 0) throw eggmi::InvalidArgument("n_levels must be >= 0");
         const auto& c = s->s.last_counters;
-        for (int i = 0; i < 2 * n_levels; ++i) out[i] = i < (int)c.size() ? (int64_t)c[i] : 0;
+        constexpr int K = eggmi::Session::kCountersPerLevel;
+        for (int l = 0; l < n_levels; ++l)
+            for (int j = 0; j < 3; ++j) {
+                const size_t i = (size_t)l * K + j;
+                out[3 * l + j] = i < c.size() ? (int64_t)c[i] : 0;
+            }
+    });
+}
+
+eggmi_status eggmi_mlp_create(int dim, uint64_t seed, int64_t capacity, eggmi_mlp** out)
+{
+    return guarded([&] {
+        need(out, "out");
+        *out = nullptr;
+        auto h = std::make_unique<eggmi_mlp>();
+        h->m = std::make_unique<eggmi::Mlp>(dim, seed, capacity);
+        *out = h.release();
+    });
+}
+
+eggmi_status eggmi_mlp_destroy(eggmi_mlp* m)
+{
+    return guarded([&] { delete m; });
+}
+
+eggmi_status eggmi_mlp_shape(eggmi_mlp* m, int32_t* shape, int64_t* n_params, int64_t* n_data, int64_t* capacity)
+{
+    return guarded([&] {
+        need(m, "mlp");
+        const auto& n = *m->m;
+        if (shape) {
+            shape[0] = n.dim;
+            shape[1] = n.in;
+            shape[2] = n.inp;
+            shape[3] = n.hidden;
+            shape[4] = n.out;
+        }
+        if (n_params) *n_params = n.n_params;
+        if (n_data) *n_data = n.n;
+        if (capacity) *capacity = n.cap;
+    });
+}
+
+eggmi_status eggmi_mlp_get_params(eggmi_mlp* m, float* out, int64_t n)
+{
+    return guarded([&] {
+        need(m, "mlp");
+        need(out, "out");
+        if (n != m->m->n_params) throw eggmi::InvalidArgument("params length must be " + std::to_string(m->m->n_params));
+        m->m->get_params(out);
+    });
+}
+
+eggmi_status eggmi_mlp_set_params(eggmi_mlp* m, const float* in, int64_t n)
+{
+    return guarded([&] {
+        need(m, "mlp");
+        need(in, "params");
+        if (n != m->m->n_params) throw eggmi::InvalidArgument("params length must be " + std::to_string(m->m->n_params));
+        m->m->set_params(in);
+    });
+}
+
+eggmi_status eggmi_mlp_forward(eggmi_mlp* m, const float* X, int64_t rows, float* Y)
+{
+    return guarded([&] {
+        need(m, "mlp");
+        if (rows > 0) {
+            need(X, "X");
+            need(Y, "Y");
+        }
+        m->m->forward(X, rows, Y);
+    });
+}
+
+eggmi_status eggmi_mlp_data_add(eggmi_mlp* m, const float* X, const float* T, int64_t rows)
+{
+    return guarded([&] {
+        need(m, "mlp");
+        if (rows > 0) {
+            need(X, "X");
+            need(T, "T");
+        }
+        m->m->data_add(X, T, rows);
+    });
+}
+
+eggmi_status eggmi_mlp_data_get(eggmi_mlp* m, float* X, float* T, int64_t offset, int64_t rows)
+{
+    return guarded([&] {
+        need(m, "mlp");
+        m->m->data_get(X, T, offset, rows);
+    });
+}
+
+eggmi_status eggmi_mlp_data_clear(eggmi_mlp* m)
+{
+    return guarded([&] {
+        need(m, "mlp");
+        m->m->data_clear();
+    });
+}
+
+eggmi_status eggmi_mlp_gradient(eggmi_mlp* m, const int32_t* rows_idx, int64_t batch, float* grad, double* loss)
+{
+    return guarded([&] {
+        need(m, "mlp");
+        need(rows_idx, "rows_idx");
+        need(grad, "grad");
+        const double l = m->m->gradient(rows_idx, batch, grad);
+        if (loss) *loss = l;
+    });
+}
+
+eggmi_status eggmi_mlp_train(eggmi_mlp* m, int epochs, int64_t batch, float lr, uint64_t seed, double* epoch_loss)
+{
+    return guarded([&] {
+        need(m, "mlp");
+        if (epochs < 0) throw eggmi::InvalidArgument("epochs must be >= 0");
+        const auto h = m->m->train(epochs, batch, lr, seed);
+        if (epoch_loss) std::copy(h.begin(), h.end(), epoch_loss);
+    });
+}
+
+eggmi_status eggmi_session_harvest(eggmi_session* s, eggmi_mlp* m, int n_snap, int every, int horizon,
+                                   double omega, int64_t* added)
+{
+    return guarded([&] {
+        need(s, "session");
+        need(m, "mlp");
+        const long long a = eggmi::harvest(s->s, *m->m, n_snap, every, horizon, omega);
+        if (added) *added = a;
+    });
+}
+
+eggmi_status eggmi_session_set_proposal(eggmi_session* s, eggmi_mlp* m, int mode, float scale)
+{
+    return guarded([&] {
+        need(s, "session");
+        s->s.need_device();
+        if (m && m->m->dim != s->s.dim) throw eggmi::InvalidArgument("proposal network dimension differs from the session's");
+        if (mode != 0 && mode != 1) throw eggmi::InvalidArgument("proposal mode must be 0 (replace) or 1 (predict-correct)");
+        if (!(scale > 0.0f) || !std::isfinite(scale)) throw eggmi::InvalidArgument("proposal scale must be positive");
+        s->s.proposal = m ? m->m.get() : nullptr;
+        s->s.proposal_mode = mode;
+        s->s.proposal_scale = scale;
     });
 }
 

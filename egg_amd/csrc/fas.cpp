@@ -537,6 +537,7 @@ std::pair<std::vector<double>, std::vector<double>> Session::run_fas(int n_cycle
     halo(F, F.x[F.cur].p);
     EGGMI_HIP(hipMemcpyAsync(F.x[1 - F.cur].p, F.x[F.cur].p, (size_t)F.plane * dim * sizeof(float),
                              hipMemcpyDeviceToDevice, stream));
+    pair_valid = true;
     Level& C0 = *coarse[0];
     CellReduceArgs ra;
     ra.plane = F.plane;

@@ -14,6 +14,8 @@
 
 #include <cstdint>
 
+#include "mlp.hpp"
+
 namespace eggmi
 {
 
@@ -42,6 +44,8 @@ struct StoredView {
     const float* W = nullptr;       // [total][dim*dim] per-occurrence W_inv rows, or null (uniform)
 };
 
+constexpr int kCounterSlots = 64, kCounterStride = 16;
+
 struct SweepArgs {
     const float* xin = nullptr;
     float* xout = nullptr;
@@ -58,8 +62,17 @@ struct SweepArgs {
     const int* cp_off = nullptr;  // [slots+1] replicas of each slot (push copies), null = none
     const int* cp_dst = nullptr;
     float Wu[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};  // uniform W_inv for stored occurrences
-    // optional counters (null = off): [0] line-search trials evaluated, [1] DOF updates
+    // optional counters (null = off): kCounterSlots rows of kCounterStride
+    // words, [0] line-search trials evaluated, [1] DOF updates, [2] accepted
+    // learned proposals; the host sums the rows
     unsigned long long* counters = nullptr;
+    // learned proposal for synthesized interior DOFs (fwd == null: off);
+    // mode 0: an accepted proposal replaces the Newton step, 1: the Newton
+    // step continues from it; the network's displacement is scaled by
+    // prop_scale
+    MlpView mlp;
+    int prop_mode = 0;
+    float prop_scale = 1.0f;
 };
 
 // Energy / min-det reduction over the structured cell stencil.

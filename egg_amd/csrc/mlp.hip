@@ -1,0 +1,501 @@
+// BF16 matrix-core kernels of the learned proposal: weight packing, batched
+// inference, and one training step (forward + backward, weight gradients,
+// Adam). FP32 accumulation everywhere; FP32 master weights.
+//
+// Training step over a batch of B samples (B a multiple of 16):
+//   1. fwd_bwd: one wave per 16 samples. Forward as in inference, keeping the
+//      hidden pre-activations in registers; dY = 2 (y - t) / B; dH2 = W3 dY
+//      (16x16x16), dZ2 = dH2 * gelu'(Z2), dH1 = W2 dZ2 (16x16x32, chained),
+//      dZ1 = dH1 * gelu'(Z1). The layer inputs and the dZ's are written in
+//      natural [sample][feature] BF16 layout (each lane stores its 4
+//      consecutive rows of a D tile with one 8-byte store).
+//   2. wgrad: dW = Act^T G as 16x16 output tiles, one wave per (tile, chunk of
+//      256 samples), K = samples (16x16x32); bias gradients as column sums of
+//      G over the chunk. Partials per chunk.
+//   3. adam: one thread per parameter folds the chunk partials in chunk order
+//      (deterministic), then the Adam update of the FP32 master.
+//   4. pack: master -> BF16 fragments for the next step.
+#include "kernels.hpp"
+#include "mlp.hpp"
+
+#include <cmath>
+#include <stdexcept>
+#include <string>
+
+namespace eggmi
+{
+
+void hip_check(hipError_t e, const char* what);
+
+namespace
+{
+
+constexpr int kChunk = 256;  // samples per weight-gradient partial
+
+template <int D> struct Act {
+    // per-batch activation buffers (BF16, natural layout), offsets in elements
+    using L = MlpLayout<D>;
+    __host__ __device__ static long long x(long long) { return 0; }
+    __host__ __device__ static long long h1(long long B) { return B * L::INP; }
+    __host__ __device__ static long long dz1(long long B) { return h1(B) + B * L::H; }
+    __host__ __device__ static long long h2(long long B) { return dz1(B) + B * L::H; }
+    __host__ __device__ static long long dz2(long long B) { return h2(B) + B * L::H; }
+    __host__ __device__ static long long dy(long long B) { return dz2(B) + B * L::H; }
+    __host__ __device__ static long long total(long long B) { return dy(B) + B * 16; }
+};
+
+// Weight-gradient tile table: tiles of dW1 [INP x H], dW2 [H x H], dW3 [H x 16]
+// (16x16 each), then bias column-sum tiles of dZ1 [H], dZ2 [H], dY [16].
+template <int D> struct Tiles {
+    using L = MlpLayout<D>;
+    static constexpr int t1 = (L::INP / 16) * (L::H / 16), t2 = (L::H / 16) * (L::H / 16), t3 = L::H / 16;
+    static constexpr int b1 = L::H / 16, b2 = L::H / 16, b3 = 1;
+    static constexpr int n = t1 + t2 + t3 + b1 + b2 + b3;
+};
+
+__device__ __forceinline__ short bf_bits(float x)
+{
+    const __bf16 b = (__bf16)x;
+    return __builtin_bit_cast(short, b);
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void pack_kernel(const float* __restrict__ P, bf16x8* __restrict__ fwd,
+                                                   char* __restrict__ bw)
+{
+    using L = MlpLayout<D>;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long nf = (long long)L::n_fwd_frags * 64 * 8;
+    const long long nb3 = (long long)L::n_bw3_frags * 64 * 4;
+    const long long nb2 = (long long)L::n_bw2_frags * 64 * 8;
+    if (t < nf) {
+        const int f = (int)(t / 512), lane = (int)((t / 8) % 64), i = (int)(t % 8);
+        const int g = lane >> 4, c = lane & 15;
+        float v = 0.f;
+        if (f < L::F2) {
+            const int mt = f / L::KS1, ks = f % L::KS1;
+            const int m = mt * 16 + c, feat = ks * 32 + 8 * g + i;
+            if (feat < L::IN) v = P[L::oW1 + (long long)feat * L::H + m];
+        } else if (f < L::F3) {
+            const int q = f - L::F2, mt = q / L::KS2, ks = q % L::KS2;
+            const int m = mt * 16 + c, feat = chain_feature(ks * 32 + 8 * g + i);
+            v = P[L::oW2 + (long long)feat * L::H + m];
+        } else {
+            const int ks = f - L::F3;
+            const int m = c, feat = chain_feature(ks * 32 + 8 * g + i);
+            if (m < L::OUT) v = P[L::oW3 + (long long)feat * L::OUT + m];
+        }
+        reinterpret_cast<__bf16*>(fwd)[t] = (__bf16)v;
+        return;
+    }
+    long long u = t - nf;
+    if (u < nb3) {  // dH2 = W3 . dY: A[m = h][k = o]
+        const int mt = (int)(u / 256), lane = (int)((u / 4) % 64), i = (int)(u % 4);
+        const int h = mt * 16 + (lane & 15), o = 4 * (lane >> 4) + i;
+        const float v = o < L::OUT ? P[L::oW3 + (long long)h * L::OUT + o] : 0.f;
+        reinterpret_cast<__bf16*>(bw)[u] = (__bf16)v;
+        return;
+    }
+    u -= nb3;
+    if (u < nb2) {  // dH1 = W2 . dZ2: A[m = h1][slot -> h2]
+        const int f = (int)(u / 512), lane = (int)((u / 8) % 64), i = (int)(u % 8);
+        const int mt = f / L::KS2, ks = f % L::KS2;
+        const int h1 = mt * 16 + (lane & 15), h2 = chain_feature(ks * 32 + 8 * (lane >> 4) + i);
+        reinterpret_cast<__bf16*>(bw + nb3 * 2)[u] = (__bf16)P[L::oW2 + (long long)h1 * L::H + h2];
+    }
+}
+
+// Lane's identity-order B fragments of rows X[r0 + l%16], or zeros past n.
+template <int D>
+__device__ __forceinline__ void load_x(const __bf16* X, long long row, bool valid, bf16x8* xin)
+{
+    using L = MlpLayout<D>;
+    const int g = (threadIdx.x & 63) >> 4;
+#pragma unroll
+    for (int ks = 0; ks < L::KS1; ++ks) {
+        if (valid)
+            xin[ks] = *reinterpret_cast<const bf16x8*>(X + row * L::INP + ks * 32 + 8 * g);
+        else
+            for (int i = 0; i < 8; ++i) xin[ks][i] = (__bf16)0.0f;
+    }
+}
+
+template <int D>
+__global__ __launch_bounds__(64) void forward_kernel(MlpView w, const __bf16* __restrict__ X, long long n,
+                                                     float* __restrict__ Y)
+{
+    using L = MlpLayout<D>;
+    const int lane = threadIdx.x;
+    const long long row = (long long)blockIdx.x * 16 + (lane & 15);
+    bf16x8 xin[L::KS1];
+    load_x<D>(X, row, row < n, xin);
+    const f32x4 y = mlp_forward<D, false>(w, xin);
+    if (lane < 16 && row < n)
+#pragma unroll
+        for (int i = 0; i < L::OUT; ++i) Y[row * L::OUT + i] = y[i];
+}
+
+// D-layout tile (rows 16*mt + 4g + i, column = sample) -> natural buf[sample][width]
+__device__ __forceinline__ void store_tile(__bf16* buf, long long sample, int width, int mt, const f32x4& v)
+{
+    const int g = (threadIdx.x & 63) >> 4;
+    s16x4 s;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) s[i] = bf_bits(v[i]);
+    *reinterpret_cast<s16x4*>(buf + sample * width + mt * 16 + 4 * g) = s;
+}
+
+template <int D>
+__global__ __launch_bounds__(64) void fwd_bwd_kernel(MlpView w, const s16x4* __restrict__ bw3,
+                                                     const bf16x8* __restrict__ bw2, const __bf16* __restrict__ X,
+                                                     const float* __restrict__ T, const int* __restrict__ idx,
+                                                     long long B, __bf16* __restrict__ act,
+                                                     double* __restrict__ loss)
+{
+    using L = MlpLayout<D>;
+    using A = Act<D>;
+    const int lane = threadIdx.x, g = lane >> 4;
+    const long long s = (long long)blockIdx.x * 16 + (lane & 15);  // batch row
+    const long long src = idx[s];
+    bf16x8 xin[L::KS1];
+    load_x<D>(X, src, true, xin);
+    // the batch copy of the inputs (for dW1)
+#pragma unroll
+    for (int ks = 0; ks < L::KS1; ++ks)
+        *reinterpret_cast<bf16x8*>(act + A::x(B) + s * L::INP + ks * 32 + 8 * g) = xin[ks];
+    f32x4 z1[L::MT1], z2[L::MT2];
+    const f32x4 y = mlp_forward<D, true>(w, xin, z1, z2);
+    // dY (rows = outputs) and the loss
+    f32x4 dy;
+    float se = 0.f;
+    const float sc = 2.0f / (float)B;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int o = 4 * g + i;
+        const float r = o < L::OUT ? y[i] - T[src * 4 + o] : 0.f;
+        se += r * r;
+        dy[i] = sc * r;
+    }
+    store_tile(act + A::dy(B), s, 16, 0, dy);
+    // hidden activations as the forward saw them
+#pragma unroll
+    for (int mt = 0; mt < L::MT1; ++mt) {
+        f32x4 h;
+        for (int i = 0; i < 4; ++i) h[i] = gelu(z1[mt][i]);
+        store_tile(act + A::h1(B), s, L::H, mt, h);
+    }
+#pragma unroll
+    for (int mt = 0; mt < L::MT2; ++mt) {
+        f32x4 h;
+        for (int i = 0; i < 4; ++i) h[i] = gelu(z2[mt][i]);
+        store_tile(act + A::h2(B), s, L::H, mt, h);
+    }
+    // dH2 = W3 . dY, dZ2
+    s16x4 bdy;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) bdy[i] = bf_bits(dy[i]);
+    f32x4 dz2[L::MT2];
+#pragma unroll
+    for (int mt = 0; mt < L::MT2; ++mt) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bw3[mt * 64 + lane], bdy, acc, 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[i] *= gelu_grad(z2[mt][i]);
+        dz2[mt] = acc;
+        store_tile(act + A::dz2(B), s, L::H, mt, acc);
+    }
+    // dH1 = W2 . dZ2, dZ1
+#pragma unroll
+    for (int mt = 0; mt < L::MT1; ++mt) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < L::KS2; ++ks) acc = mfma32(bw2[(mt * L::KS2 + ks) * 64 + lane], chain(dz2[2 * ks], dz2[2 * ks + 1]), acc);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[i] *= gelu_grad(z1[mt][i]);
+        store_tile(act + A::dz1(B), s, L::H, mt, acc);
+    }
+    // per-tile squared error (deterministic: lane 0 sums in lane order)
+    __shared__ float red[64];
+    red[lane] = se;
+    __syncthreads();
+    if (lane == 0) {
+        double t = 0.0;
+        for (int i = 0; i < 64; ++i) t += red[i];
+        loss[blockIdx.x] = t;
+    }
+}
+
+// One wave per (tile, chunk): 16x16 tile of Act^T G over kChunk samples, or
+// a bias column-sum tile.
+template <int D>
+__global__ __launch_bounds__(64) void wgrad_kernel(const __bf16* __restrict__ act, long long B,
+                                                   float* __restrict__ part)
+{
+    using L = MlpLayout<D>;
+    using A = Act<D>;
+    using TT = Tiles<D>;
+    const int tile = blockIdx.x, chunk = blockIdx.y;
+    const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
+    const long long b0 = (long long)chunk * kChunk;
+    float* out = part + ((long long)chunk * TT::n + tile) * 256;
+    int t = tile;
+    const __bf16 *a = nullptr, *gm = nullptr;
+    int wa = 0, wg = 0, m0 = 0, n0 = 0;
+    if (t < TT::t1) {
+        a = act + A::x(B), wa = L::INP, gm = act + A::dz1(B), wg = L::H;
+        m0 = (t / (L::H / 16)) * 16, n0 = (t % (L::H / 16)) * 16;
+    } else if ((t -= TT::t1) < TT::t2) {
+        a = act + A::h1(B), wa = L::H, gm = act + A::dz2(B), wg = L::H;
+        m0 = (t / (L::H / 16)) * 16, n0 = (t % (L::H / 16)) * 16;
+    } else if ((t -= TT::t2) < TT::t3) {
+        a = act + A::h2(B), wa = L::H, gm = act + A::dy(B), wg = 16;
+        m0 = t * 16, n0 = 0;
+    } else {
+        // bias: column sums of dZ1 / dZ2 / dY over the chunk (4 row groups of
+        // 16 lanes, folded in group order)
+        t -= TT::t3;
+        const __bf16* col;
+        int w;
+        if (t < TT::b1) {
+            col = act + A::dz1(B) + t * 16, w = L::H;
+        } else if ((t -= TT::b1) < TT::b2) {
+            col = act + A::dz2(B) + t * 16, w = L::H;
+        } else {
+            col = act + A::dy(B), w = 16;
+        }
+        float sacc = 0.f;
+        for (int r = g; r < kChunk; r += 4) sacc += (float)col[(b0 + r) * w + c];
+        __shared__ float red[64];
+        red[lane] = sacc;
+        __syncthreads();
+        if (lane < 16) out[lane] = (red[lane] + red[16 + lane]) + (red[32 + lane] + red[48 + lane]);
+        return;
+    }
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < kChunk; k0 += 32) {
+        bf16x8 fa, fb;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const long long r = b0 + k0 + 8 * g + i;
+            fa[i] = a[r * wa + m0 + c];
+            fb[i] = gm[r * wg + n0 + c];
+        }
+        acc = mfma32(fa, fb, acc);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) out[(4 * g + i) * 16 + c] = acc[i];
+}
+
+// Fold the chunk partials into the gradient (fixed chunk order) and take an
+// Adam step on the FP32 master. lr = 0 only folds.
+template <int D>
+__global__ __launch_bounds__(256) void adam_kernel(float* __restrict__ P, float* __restrict__ m,
+                                                   float* __restrict__ v, float* __restrict__ grad,
+                                                   const float* __restrict__ part, int n_chunks, float lr,
+                                                   float bc1, float bc2)
+{
+    using L = MlpLayout<D>;
+    using TT = Tiles<D>;
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= L::n_params) return;
+    int tile, pos;
+    if (p < L::ob1) {  // W1[feat][h]
+        const int f = (int)(p / L::H), h = (int)(p % L::H);
+        tile = (f / 16) * (L::H / 16) + h / 16, pos = (f % 16) * 16 + h % 16;
+    } else if (p < L::oW2) {
+        const int h = (int)(p - L::ob1);
+        tile = TT::t1 + TT::t2 + TT::t3 + h / 16, pos = h % 16;
+    } else if (p < L::ob2) {
+        const int q = (int)(p - L::oW2), f = q / L::H, h = q % L::H;
+        tile = TT::t1 + (f / 16) * (L::H / 16) + h / 16, pos = (f % 16) * 16 + h % 16;
+    } else if (p < L::oW3) {
+        const int h = (int)(p - L::ob2);
+        tile = TT::t1 + TT::t2 + TT::t3 + TT::b1 + h / 16, pos = h % 16;
+    } else if (p < L::ob3) {
+        const int q = (int)(p - L::oW3), f = q / L::OUT, o = q % L::OUT;
+        tile = TT::t1 + TT::t2 + f / 16, pos = (f % 16) * 16 + o;
+    } else {
+        const int o = (int)(p - L::ob3);
+        tile = TT::n - 1, pos = o;
+    }
+    float gsum = 0.f;
+    for (int c = 0; c < n_chunks; ++c) gsum += part[((long long)c * TT::n + tile) * 256 + pos];
+    grad[p] = gsum;
+    if (lr == 0.f) return;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float mm = b1 * m[p] + (1.f - b1) * gsum;
+    const float vv = b2 * v[p] + (1.f - b2) * gsum * gsum;
+    m[p] = mm;
+    v[p] = vv;
+    P[p] -= lr * (mm / bc1) / (sqrtf(vv / bc2) + eps);
+}
+
+template <int D> constexpr long long bw_bytes()
+{
+    using L = MlpLayout<D>;
+    return (long long)L::n_bw3_frags * 64 * 4 * 2 + (long long)L::n_bw2_frags * 64 * 8 * 2;
+}
+
+template <int D> void pack_impl(const float* P, bf16x8* fwd, void* bw, hipStream_t s)
+{
+    using L = MlpLayout<D>;
+    const long long n = (long long)L::n_fwd_frags * 512 + (long long)L::n_bw3_frags * 256 + (long long)L::n_bw2_frags * 512;
+    pack_kernel<D><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(P, fwd, static_cast<char*>(bw));
+}
+
+template <int D> void step_impl(const MlpTrainArgs& a, float lr, long long step, hipStream_t s)
+{
+    using L = MlpLayout<D>;
+    if (a.batch <= 0 || a.batch % kChunk != 0)
+        throw std::invalid_argument("mlp: batch must be a positive multiple of " + std::to_string(kChunk));
+    MlpView w;
+    w.fwd = a.fwd;
+    w.params = a.params;
+    const auto* bw3 = static_cast<const s16x4*>(a.bw);
+    const auto* bw2 = reinterpret_cast<const bf16x8*>(static_cast<const char*>(a.bw) + (long long)L::n_bw3_frags * 64 * 8);
+    fwd_bwd_kernel<D><<<(unsigned)(a.batch / 16), 64, 0, s>>>(w, bw3, bw2, a.X, a.T, a.idx, a.batch, a.act, a.loss);
+    const int n_chunks = (int)(a.batch / kChunk);
+    wgrad_kernel<D><<<dim3(Tiles<D>::n, n_chunks), 64, 0, s>>>(a.act, a.batch, a.partials);
+    float bc1 = 1.f, bc2 = 1.f;
+    if (lr != 0.f) {
+        bc1 = 1.f - std::pow(0.9f, (float)step);
+        bc2 = 1.f - std::pow(0.999f, (float)step);
+    }
+    adam_kernel<D><<<(unsigned)((L::n_params + 255) / 256), 256, 0, s>>>(a.params, a.adam_m, a.adam_v, a.grad,
+                                                                           a.partials, n_chunks, lr, bc1, bc2);
+    if (lr != 0.f) pack_impl<D>(a.params, a.fwd, a.bw, s);
+}
+
+}  // namespace
+
+int mlp_inp(int dim) { return dim == 2 ? MlpDims<2>::INP : MlpDims<3>::INP; }
+long long mlp_n_params(int dim) { return dim == 2 ? MlpLayout<2>::n_params : MlpLayout<3>::n_params; }
+long long mlp_fwd_frag_count(int dim) { return dim == 2 ? MlpLayout<2>::n_fwd_frags : MlpLayout<3>::n_fwd_frags; }
+long long mlp_bw_bytes(int dim) { return dim == 2 ? bw_bytes<2>() : bw_bytes<3>(); }
+long long mlp_act_elems(int dim, long long B) { return dim == 2 ? Act<2>::total(B) : Act<3>::total(B); }
+long long mlp_partial_elems(int dim, long long B)
+{
+    const int nt = dim == 2 ? Tiles<2>::n : Tiles<3>::n;
+    return (B / kChunk) * nt * 256;
+}
+
+void mlp_pack(int dim, const float* params, bf16x8* fwd, void* bw, hipStream_t s)
+{
+    if (dim == 2)
+        pack_impl<2>(params, fwd, bw, s);
+    else
+        pack_impl<3>(params, fwd, bw, s);
+}
+
+void mlp_forward_batch(int dim, const bf16x8* fwd, const float* params, const __bf16* X, long long n, float* Y,
+                       hipStream_t s)
+{
+    if (n <= 0) return;
+    MlpView w;
+    w.fwd = fwd;
+    w.params = params;
+    const unsigned grid = (unsigned)((n + 15) / 16);
+    if (dim == 2)
+        forward_kernel<2><<<grid, 64, 0, s>>>(w, X, n, Y);
+    else
+        forward_kernel<3><<<grid, 64, 0, s>>>(w, X, n, Y);
+}
+
+// ---- harvesting: features of synthesized interior DOFs from the store, and
+// their displacement some sweeps later in the same local frame
+
+template <int D>
+__global__ __launch_bounds__(256) void features_kernel(const float* __restrict__ x, const float* __restrict__ xprev,
+                                                       long long plane, const int* __restrict__ slots,
+                                                       const int* __restrict__ blks,
+                                                       const BlockDesc* __restrict__ blocks, long long n,
+                                                       __bf16* __restrict__ Xo, float* __restrict__ frame,
+                                                       float* __restrict__ xt)
+{
+    using L = MlpLayout<D>;
+    const long long d = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= n) return;
+    const long long slot = slots[d];
+    const BlockDesc bd = blocks[blks[d]];
+    auto ring = [&](int r, int k) -> float {
+        long long o;
+        if constexpr (D == 2)
+            o = (long long)(r / 3 - 1) * bd.st[0] + (r % 3 - 1);
+        else
+            o = (long long)(r / 9 - 1) * bd.st[0] + (long long)((r / 3) % 3 - 1) * bd.st[1] + (r % 3 - 1);
+        return x[k * plane + slot + o];
+    };
+    const Frame f = local_frame<D>(ring);
+    float dprev[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) dprev[k] = x[k * plane + slot] - xprev[k * plane + slot];
+#pragma unroll
+    for (int j = 0; j < L::INP; ++j) Xo[d * L::INP + j] = (__bf16)feature<D>(ring, f, dprev, j);
+    float* fr = frame + d * (D * D + 1);
+#pragma unroll
+    for (int c = 0; c < D; ++c)
+#pragma unroll
+        for (int k = 0; k < D; ++k) fr[c * D + k] = f.e[c][k];
+    fr[D * D] = f.ok ? f.h : 0.f;
+#pragma unroll
+    for (int k = 0; k < D; ++k) xt[d * D + k] = x[k * plane + slot];
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void targets_kernel(const float* __restrict__ x, long long plane,
+                                                      const int* __restrict__ slots, long long n,
+                                                      const float* __restrict__ frame, const float* __restrict__ xt,
+                                                      float* __restrict__ T)
+{
+    const long long d = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= n) return;
+    const long long slot = slots[d];
+    const float* fr = frame + d * (D * D + 1);
+    const float h = fr[D * D];
+    const float rh = h > 0.f ? 1.0f / h : 0.f;
+    float dx[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) dx[k] = x[k * plane + slot] - xt[d * D + k];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        float v = 0.f;
+        if (c < D)
+#pragma unroll
+            for (int k = 0; k < D; ++k) v += dx[k] * fr[c * D + k];
+        T[d * 4 + c] = v * rh;
+    }
+}
+
+void mlp_features(int dim, const float* x, const float* xprev, long long plane, const int* slots, const int* blks,
+                  const BlockDesc* blocks, long long n, void* Xo, float* frame, float* xt, hipStream_t s)
+{
+    if (n <= 0) return;
+    const unsigned g = (unsigned)((n + 255) / 256);
+    if (dim == 2)
+        features_kernel<2><<<g, 256, 0, s>>>(x, xprev, plane, slots, blks, blocks, n, static_cast<__bf16*>(Xo), frame, xt);
+    else
+        features_kernel<3><<<g, 256, 0, s>>>(x, xprev, plane, slots, blks, blocks, n, static_cast<__bf16*>(Xo), frame, xt);
+}
+
+void mlp_targets(int dim, const float* x, long long plane, const int* slots, long long n, const float* frame,
+                 const float* xt, float* T, hipStream_t s)
+{
+    if (n <= 0) return;
+    const unsigned g = (unsigned)((n + 255) / 256);
+    if (dim == 2)
+        targets_kernel<2><<<g, 256, 0, s>>>(x, plane, slots, n, frame, xt, T);
+    else
+        targets_kernel<3><<<g, 256, 0, s>>>(x, plane, slots, n, frame, xt, T);
+}
+
+void mlp_step(const MlpTrainArgs& a, float lr, long long step, hipStream_t s)
+{
+    if (a.dim == 2)
+        step_impl<2>(a, lr, step, s);
+    else
+        step_impl<3>(a, lr, step, s);
+}
+
+long long mlp_chunk() { return kChunk; }
+
+}  // namespace eggmi

@@ -1,5 +1,6 @@
 // Host side of the MI355X structured smoother (see session.hpp).
 #include "session.hpp"
+#include "mlp_host.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -700,13 +701,23 @@ std::pair<std::vector<double>, std::vector<double>> Session::run_jacobi(int n_sw
     const size_t bytes = (size_t)L.plane * dim * sizeof(float);
     EGGMI_HIP(hipEventRecord(ev0, stream));
     halo(L, xin);
-    EGGMI_HIP(hipMemcpyAsync(xout, xin, bytes, hipMemcpyDeviceToDevice, stream));
+    // the write buffer must hold every slot no sweep writes (fixed nodes);
+    // after a sweep it does, and it keeps the previous state of every DOF
+    // (the learned proposal's previous-displacement input)
+    if (!pair_valid) EGGMI_HIP(hipMemcpyAsync(xout, xin, bytes, hipMemcpyDeviceToDevice, stream));
+    pair_valid = true;
+    const bool propose = proposal != nullptr && kind != 1;
     int rep = 0;
     if (kernel_timing) ensure_timing_events(2 * n_sweeps);
     counters_begin();
     for (int s = 0; s < n_sweeps; ++s) {
         SweepArgs a = L.sweep_args(xin, xout, (float)omega, (float)delta, nullptr);
         a.counters = counters_for(L);
+        if (propose) {
+            a.mlp = proposal->view();
+            a.prop_mode = proposal_mode;
+            a.prop_scale = proposal_scale;
+        }
         if (kernel_timing) EGGMI_HIP(hipEventRecord(tev[2 * s], stream));
         launch_sweep(dim, kind, tile_variant, a, stream);
         if (kernel_timing) EGGMI_HIP(hipEventRecord(tev[2 * s + 1], stream));
@@ -736,32 +747,54 @@ std::pair<std::vector<double>, std::vector<double>> Session::run_jacobi(int n_sw
     return {std::vector<double>(hr.begin(), hr.begin() + nrep), std::vector<double>(hr.begin() + nrep, hr.end())};
 }
 
+void Level::ensure_synth_list()
+{
+    if (!syn_slot.empty() || n_synth == 0) return;
+    for (size_t b = 0; b < lay.shape.size(); ++b) {
+        const long long b0 = lay.base[b], b1 = b0 + lay.block_padded_count((int)b);
+        for (long long sl = b0; sl < b1; ++sl)
+            if (mask[sl]) {
+                syn_slot.push_back((int)sl);
+                syn_blk.push_back((int)b);
+            }
+    }
+    d_syn_slot.upload(syn_slot);
+    d_syn_blk.upload(syn_blk);
+}
+
 unsigned long long* Session::counters_for(const Level& L)
 {
     if (!counting) return nullptr;
+    constexpr long long per = (long long)kCounterSlots * kCounterStride;
     if (&L == &fine) return d_counters.p;
     for (size_t i = 0; i < coarse.size() && i + 1 < (size_t)kMaxCounterLevels; ++i)
-        if (coarse[i].get() == &L) return d_counters.p + 2 * (i + 1);
+        if (coarse[i].get() == &L) return d_counters.p + per * (i + 1);
     return nullptr;
 }
 
 void Session::counters_begin()
 {
     if (!counting) return;
-    if (d_counters.n == 0) d_counters.alloc(2 * kMaxCounterLevels);
-    EGGMI_HIP(hipMemsetAsync(d_counters.p, 0, 2 * kMaxCounterLevels * sizeof(unsigned long long), stream));
+    const size_t n = (size_t)kCounterSlots * kCounterStride * kMaxCounterLevels;
+    if (d_counters.n == 0) d_counters.alloc(n);
+    EGGMI_HIP(hipMemsetAsync(d_counters.p, 0, n * sizeof(unsigned long long), stream));
 }
 
-// Called after the stream has been synchronised.
+// Called after the stream has been synchronised: sums the slot rows.
 void Session::counters_end()
 {
     if (!counting) {
         last_counters.clear();
         return;
     }
-    std::vector<unsigned long long> h(2 * kMaxCounterLevels);
+    const size_t per = (size_t)kCounterSlots * kCounterStride;
+    std::vector<unsigned long long> h(per * kMaxCounterLevels);
     EGGMI_HIP(hipMemcpy(h.data(), d_counters.p, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
-    last_counters.assign(h.begin(), h.end());
+    last_counters.assign((size_t)kCountersPerLevel * kMaxCounterLevels, 0);
+    for (int l = 0; l < kMaxCounterLevels; ++l)
+        for (int r = 0; r < kCounterSlots; ++r)
+            for (int j = 0; j < 3; ++j)
+                last_counters[(size_t)l * kCountersPerLevel + j] += h[l * per + (size_t)r * kCounterStride + j];
 }
 
 void Session::get_X(double* out) const
@@ -790,6 +823,7 @@ void Session::set_X(const double* X)
     EGGMI_HIP(hipMemcpy(L.x[L.cur].p, xs.data(), xs.size() * sizeof(float), hipMemcpyHostToDevice));
     halo(L, L.x[L.cur].p);
     EGGMI_HIP(hipStreamSynchronize(stream));
+    pair_valid = false;
 }
 
 std::vector<float> Session::packed() const

@@ -127,6 +127,9 @@ struct Level {
     long long n_synth = 0;
     std::vector<TileDesc> tiles;
     DArr<TileDesc> d_tiles;
+    std::vector<int> syn_slot, syn_blk;  // synthesized DOFs as a list (lazily, for harvesting)
+    DArr<int> d_syn_slot, d_syn_blk;
+    void ensure_synth_list();
     // stored-stencil DOFs
     std::vector<int> st_dof, st_off, st_gc, st_gn;
     std::vector<uint8_t> st_meta;
@@ -162,6 +165,8 @@ struct Level {
     SweepArgs sweep_args(const float* xin, float* xout, float omega, float delta, const float* tau) const;
 };
 
+struct Mlp;
+
 struct Session {
     int dim = 2;
     long long M = 0;  // global node count
@@ -179,8 +184,10 @@ struct Session {
     double last_kernel_ms = 0.0;    // summed sweep-kernel time of the last run
     std::vector<hipEvent_t> tev;
     void ensure_timing_events(size_t n);
-    // diagnostic counters per level: [2*level] trials evaluated, [2*level+1] DOF updates
+    // diagnostic counters per level: [4*level] trials evaluated, [4*level+1]
+    // DOF updates, [4*level+2] accepted learned proposals
     static constexpr int kMaxCounterLevels = 32;
+    static constexpr int kCountersPerLevel = 4;
     bool counting = false;
     DArr<unsigned long long> d_counters;
     std::vector<long long> last_counters;
@@ -194,6 +201,10 @@ struct Session {
     HaloHost fine_halo;
     DArr<double> d_ls;
     bool host_only = false;  // created with device = 3: index tables only
+    const Mlp* proposal = nullptr;  // learned proposal for run_jacobi (not owned)
+    int proposal_mode = 0;
+    float proposal_scale = 1.0f;
+    bool pair_valid = false;        // the second Jacobi buffer holds the previous sweep's state
 
     Session() = default;
     ~Session();

@@ -6,13 +6,13 @@
 namespace eggmi
 {
 
-template <int KIND, bool TAU, bool GRAD> static void sweep2d_v(int v, const SweepArgs& a, hipStream_t s)
+template <int KIND, bool TAU, bool GRAD, bool PROP = false> static void sweep2d_v(int v, const SweepArgs& a, hipStream_t s)
 {
     switch (v) {
-    case 0: launch_variant<2, KIND, 32, 2, 1, TAU, GRAD>(a, s); break;
-    case 1: launch_variant<2, KIND, 16, 4, 1, TAU, GRAD>(a, s); break;
-    case 2: launch_variant<2, KIND, 64, 1, 1, TAU, GRAD>(a, s); break;
-    default: launch_variant<2, KIND, 32, 4, 1, TAU, GRAD>(a, s); break;
+    case 0: launch_variant<2, KIND, 32, 2, 1, TAU, GRAD, PROP>(a, s); break;
+    case 1: launch_variant<2, KIND, 16, 4, 1, TAU, GRAD, PROP>(a, s); break;
+    case 2: launch_variant<2, KIND, 64, 1, 1, TAU, GRAD, PROP>(a, s); break;
+    default: launch_variant<2, KIND, 32, 4, 1, TAU, GRAD, PROP>(a, s); break;
     }
 }
 
@@ -23,6 +23,13 @@ void launch_sweep2d(int kind, int v, const SweepArgs& a, hipStream_t s)
             sweep2d_v<kShapeSize, true, false>(v, a, s);
         else
             sweep2d_v<kShape, true, false>(v, a, s);  // FAS levels: barrier phases only
+        return;
+    }
+    if (a.mlp.fwd && kind != kUntangle) {  // learned proposal (barrier phases, no FAS shift)
+        if (kind == kShapeSize)
+            sweep2d_v<kShapeSize, false, false, true>(v, a, s);
+        else
+            sweep2d_v<kShape, false, false, true>(v, a, s);
         return;
     }
     if (kind == kUntangle)

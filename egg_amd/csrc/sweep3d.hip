@@ -6,11 +6,11 @@
 namespace eggmi
 {
 
-template <int KIND, bool TAU, bool GRAD> static void sweep3d_v(int v, const SweepArgs& a, hipStream_t s)
+template <int KIND, bool TAU, bool GRAD, bool PROP = false> static void sweep3d_v(int v, const SweepArgs& a, hipStream_t s)
 {
     switch (v) {
-    case 0: launch_variant<3, KIND, 8, 4, 2, TAU, GRAD>(a, s); break;
-    default: launch_variant<3, KIND, 4, 4, 4, TAU, GRAD>(a, s); break;
+    case 0: launch_variant<3, KIND, 8, 4, 2, TAU, GRAD, PROP>(a, s); break;
+    default: launch_variant<3, KIND, 4, 4, 4, TAU, GRAD, PROP>(a, s); break;
     }
 }
 
@@ -21,6 +21,13 @@ void launch_sweep3d(int kind, int v, const SweepArgs& a, hipStream_t s)
             sweep3d_v<kShapeSize, true, false>(v, a, s);
         else
             sweep3d_v<kShape, true, false>(v, a, s);  // FAS levels: barrier phases only
+        return;
+    }
+    if (a.mlp.fwd && kind != kUntangle) {  // learned proposal (barrier phases, no FAS shift)
+        if (kind == kShapeSize)
+            sweep3d_v<kShapeSize, false, false, true>(v, a, s);
+        else
+            sweep3d_v<kShape, false, false, true>(v, a, s);
         return;
     }
     if (kind == kUntangle)

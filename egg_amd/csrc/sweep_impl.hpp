@@ -23,6 +23,7 @@
 // occurrence order (floating-point summation order only).
 #pragma once
 
+#include "mlp.hpp"
 #include "sweep_common.hpp"
 
 namespace eggmi
@@ -95,12 +96,19 @@ __device__ __forceinline__ void store_node(const SweepArgs& a, long long node, c
     }
 }
 
-// Diagnostic counters: trials evaluated and DOF updates (off when null).
-__device__ __forceinline__ void count_update(const SweepArgs& a, int ntr)
+// Diagnostic counters (off when null): spread over kCounterSlots cache lines
+// by workgroup so that the atomics do not serialise on one address.
+__device__ __forceinline__ unsigned long long* counter_slot(const SweepArgs& a)
+{
+    return a.counters + (blockIdx.x & (kCounterSlots - 1)) * kCounterStride;
+}
+__device__ __forceinline__ void count_update(const SweepArgs& a, int ntr, bool proposal = false)
 {
     if (a.counters) {
-        atomicAdd(a.counters, (unsigned long long)ntr);
-        atomicAdd(a.counters + 1, 1ull);
+        unsigned long long* c = counter_slot(a);
+        atomicAdd(c, (unsigned long long)ntr);
+        atomicAdd(c + 1, 1ull);
+        if (proposal) atomicAdd(c + 2, 1ull);
     }
 }
 
@@ -245,7 +253,7 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
 // ------------------------------------------------------------------ tiles
 
 // Tile of TX (last axis) x TY x TZ nodes; 2^D lanes per node.
-template <int D, int KIND, int TX, int TY, int TZ, bool TAU, bool GRAD>
+template <int D, int KIND, int TX, int TY, int TZ, bool TAU, bool GRAD, bool PROP>
 __device__ void tile_sweep(const SweepArgs& a)
 {
     constexpr int NC = 1 << D;
@@ -281,20 +289,27 @@ __device__ void tile_sweep(const SweepArgs& a)
     const int tx = nid % TX, ty = (nid / TX) % TY, tz = nid / (TX * TY);
     int lg[3];
     long long node;
+    bool inside;
     if constexpr (D == 2) {
         lg[0] = td.i0 + ty;
         lg[1] = td.j0 + tx;
         lg[2] = 0;
-        if (lg[0] > bd.n[0] - 1 || lg[1] > bd.n[1] - 1) return;
-        node = bd.base + (long long)(lg[0] + 1) * bd.st[0] + (lg[1] + 1);
+        inside = !(lg[0] > bd.n[0] - 1 || lg[1] > bd.n[1] - 1);
+        node = bd.base + (long long)(min(lg[0], bd.n[0] - 1) + 1) * bd.st[0] + (min(lg[1], bd.n[1] - 1) + 1);
     } else {
         lg[0] = td.i0 + tz;
         lg[1] = td.j0 + ty;
         lg[2] = td.k0 + tx;
-        if (lg[0] > bd.n[0] - 1 || lg[1] > bd.n[1] - 1 || lg[2] > bd.n[2] - 1) return;
-        node = bd.base + (long long)(lg[0] + 1) * bd.st[0] + (long long)(lg[1] + 1) * bd.st[1] + (lg[2] + 1);
+        inside = !(lg[0] > bd.n[0] - 1 || lg[1] > bd.n[1] - 1 || lg[2] > bd.n[2] - 1);
+        node = bd.base + (long long)(min(lg[0], bd.n[0] - 1) + 1) * bd.st[0] +
+               (long long)(min(lg[1], bd.n[1] - 1) + 1) * bd.st[1] + (min(lg[2], bd.n[2] - 1) + 1);
     }
-    if (!a.mask[node]) return;  // whole lane group leaves together
+    const bool active = inside && a.mask[node];
+    // whole lane groups leave together; the proposal variant keeps every lane
+    // until its matrix-core forward pass is done (converged wave)
+    if constexpr (!PROP) {
+        if (!active) return;
+    }
 
     // ---- the lane's cell: corners permuted so that corner 0 is the DOF
     int cb[3] = {0, 0, 0};
@@ -338,6 +353,88 @@ __device__ void tile_sweep(const SweepArgs& a)
             }
         }
     };
+
+    if constexpr (PROP) {
+        // ---- learned proposal: features of the node's one-ring (LDS), the
+        // wave's matrix-core forward pass over its 16 (2D) / 8 (3D) nodes,
+        // accepted iff the patch energy drops and every det stays > 0
+        using ML = MlpLayout<D>;
+        constexpr int NODES = TX * TY * TZ;
+        constexpr int PER_LANE = ML::INP / NC;
+        __shared__ __bf16 feat[NODES][ML::INP];
+        auto ring = [&](int r, int k) -> float {
+            int li;
+            if constexpr (D == 2)
+                li = (ty + r / 3) * LX + (tx + r % 3);
+            else
+                li = ((tz + r / 9) * LY + (ty + (r / 3) % 3)) * LX + (tx + r % 3);
+            return sh[k][li];
+        };
+        const Frame fr = local_frame<D>(ring);
+        float dprev[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) dprev[k] = P[0][k] - a.xout[k * PL + node];
+        // compile-time feature index (no dynamic register indexing); each
+        // lane of the group stores its PER_LANE-wide slice
+#pragma unroll
+        for (int j = 0; j < ML::INP; ++j)
+            if (j / PER_LANE == cell) feat[nid][j] = (__bf16)feature<D>(ring, fr, dprev, j);
+        __syncthreads();
+        constexpr int WN = 64 / NC;  // nodes per wave
+        const int lane = tid & 63, wave_node0 = (tid >> 6) * WN;
+        const int col_node = wave_node0 + ((lane & 15) % WN);
+        bf16x8 xin[ML::KS1];
+#pragma unroll
+        for (int ks = 0; ks < ML::KS1; ++ks)
+#pragma unroll
+            for (int i = 0; i < 8; ++i) xin[ks][i] = feat[col_node][ks * 32 + 8 * (lane >> 4) + i];
+        const f32x4 y = mlp_forward<D, false>(a.mlp, xin);
+        if (!active) return;
+        float pl[D], pw[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) pl[k] = __shfl(y[k], nid - wave_node0, 64);
+        to_world<D>(fr, pl, pw);
+        float tr[D];
+        bool finite_p = fr.ok;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            tr[k] = P[0][k] + a.prop_scale * pw[k];
+            finite_p = finite_p && isfinite(tr[k]);
+        }
+        float e_now = 0.0f, e_try = 0.0f, m_try = kInf;
+        static_for<NC>([&](auto OP) {
+            float A[D * D];
+            build(OP, nullptr, A);
+            e_now += sample_value<D, KIND>(A, delta);
+            if constexpr (__builtin_popcount(decltype(OP)::value) <= 1) {
+                if (finite_p) build(OP, tr, A);
+            }
+            e_try += sample_value<D, KIND>(A, delta);
+            m_try = fminf(m_try, detD<D>(A));
+        });
+        e_now = group_sum<NC>(e_now);
+        e_try = group_sum<NC>(e_try);
+        m_try = group_min<NC>(m_try);
+        // "drops": by more than the FP32 energy slack plus the rounding noise
+        // of a 2^D x 2^D-sample sum (32 ulp), so that a (near-)zero proposal
+        // never counts as progress and never displaces the Newton step
+        const float noise = Tol<float>::energy + 32.0f * 1.1920929e-7f * fabsf(e_now);
+        const bool ok = finite_p && isfinite(e_try) && e_try < e_now - noise && m_try > 0.0f;
+        if (ok) {
+            if (a.prop_mode == 0) {  // replace: the proposal is the update
+                if (cell == 0) {
+                    store_node<D>(a, node, tr);
+                    count_update(a, 0, true);
+                }
+                return;
+            }
+            // predict-correct: the plain Newton update continues from the
+            // accepted proposal (corner 0 of every lane's cell is the DOF)
+            if (cell == 0 && a.counters) atomicAdd(counter_slot(a) + 2, 1ull);
+#pragma unroll
+            for (int k = 0; k < D; ++k) P[0][k] = tr[k];
+        }
+    }
 
     float g[D] = {}, h[kNH<D>] = {}, em = 0.0f, ec = 0.0f, mc = kInf, md = kInf;
     static_for<NC>([&](auto OP) {
@@ -429,7 +526,7 @@ __device__ void tile_sweep(const SweepArgs& a)
     }
 }
 
-template <int D, int KIND, int TX, int TY, int TZ, bool TAU, bool GRAD>
+template <int D, int KIND, int TX, int TY, int TZ, bool TAU, bool GRAD, bool PROP>
 __global__ __launch_bounds__((1 << D) * TX * TY * TZ) void sweep_kernel(const SweepArgs a)
 {
     constexpr int NT = (1 << D) * TX * TY * TZ;
@@ -439,10 +536,10 @@ __global__ __launch_bounds__((1 << D) * TX * TY * TZ) void sweep_kernel(const Sw
         stored_group<D, KIND, TAU, GRAD, G>(a, gid, int(threadIdx.x) % G, gid < a.st.n);
         return;
     }
-    tile_sweep<D, KIND, TX, TY, TZ, TAU, GRAD>(a);
+    tile_sweep<D, KIND, TX, TY, TZ, TAU, GRAD, PROP>(a);
 }
 
-template <int D, int KIND, int TX, int TY, int TZ, bool TAU, bool GRAD>
+template <int D, int KIND, int TX, int TY, int TZ, bool TAU, bool GRAD, bool PROP = false>
 void launch_variant(const SweepArgs& a, hipStream_t s)
 {
     constexpr int NT = (1 << D) * TX * TY * TZ;
@@ -451,7 +548,7 @@ void launch_variant(const SweepArgs& a, hipStream_t s)
     const int nst = (a.st.n + per_wg - 1) / per_wg;
     const unsigned grid = (unsigned)(a.n_tiles + nst);
     if (grid == 0) return;
-    sweep_kernel<D, KIND, TX, TY, TZ, TAU, GRAD><<<dim3(grid), dim3(NT), 0, s>>>(a);
+    sweep_kernel<D, KIND, TX, TY, TZ, TAU, GRAD, PROP><<<dim3(grid), dim3(NT), 0, s>>>(a);
 }
 
 }  // namespace eggmi

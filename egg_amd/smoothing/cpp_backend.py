@@ -214,6 +214,19 @@ class CppStructuredSweepSession:
     def set_X(self, X: np.ndarray) -> None:
         self._session.set_X(np.ascontiguousarray(X, np.float64).ravel())
 
+    def set_proposal(self, mlp, *, mode: str = "replace", scale: float = 1.0) -> None:
+        """Use a trained ``cpp_core.ProposalMLP`` (or None) as the learned
+        Newton-step proposal of subsequent :meth:`run` calls. ``mode``
+        "replace": an accepted proposal is the update; "predict_correct":
+        the Newton step continues from it."""
+        self._session.set_proposal(mlp, mode=mode, scale=scale)
+
+    def harvest(self, mlp, n_snap: int, *, every: int = 8, horizon: int = 8, omega: float = 0.8) -> int:
+        """Append (features, displacement over ``horizon`` plain sweeps) pairs
+        of every interior DOF, at ``n_snap`` snapshots ``every`` sweeps apart,
+        to the network's training set."""
+        return self._session.harvest(mlp, n_snap, every=every, horizon=horizon, omega=omega)
+
     @property
     def core(self):
         """The underlying ``cpp_core.CppStructuredSweepSession``."""

@@ -157,9 +157,58 @@ eggmi_status eggmi_set_kernel_timing(eggmi_session* s, int on);
  * updates it made, per multigrid level. */
 eggmi_status eggmi_set_counters(eggmi_session* s, int on);
 
-/* Counters of the last run / run_fas: out[2*l] trials and out[2*l+1] DOF
- * updates of level l (0 = fine), for l < n_levels; zeros when counting was off. */
+/* Counters of the last run / run_fas: out[3*l] line-search trials, out[3*l+1]
+ * DOF updates and out[3*l+2] accepted learned proposals of level l (0 = fine),
+ * for l < n_levels; zeros when counting was off. */
 eggmi_status eggmi_last_counters(eggmi_session* s, int64_t* out, int n_levels);
+
+/* ---- Learned Newton-step proposal (no reference counterpart). A small MLP
+ * maps a synthesized interior DOF's one-ring (local frame, scaled by the
+ * local edge length) and its previous displacement to a proposed
+ * displacement; eggmi_run accepts a proposal only if the DOF's patch energy
+ * drops and every incident det stays > 0, and runs the plain Newton step
+ * otherwise. Inference and training use the BF16 matrix cores with FP32
+ * accumulation; the FP32 master parameters are laid out
+ * W1[n_in][hidden], b1[hidden], W2[hidden][hidden], b2[hidden],
+ * W3[hidden][n_out], b3[n_out]. Features are rows of n_in_padded floats. */
+typedef struct eggmi_mlp eggmi_mlp;
+
+/* A network for dim 2 (22 -> 64 -> 64 -> 2) or 3 (85 -> 128 -> 128 -> 3),
+ * Glorot-initialised from `seed`, with room for `capacity` training pairs
+ * on the device. */
+eggmi_status eggmi_mlp_create(int dim, uint64_t seed, int64_t capacity, eggmi_mlp** out);
+eggmi_status eggmi_mlp_destroy(eggmi_mlp* m);
+/* shape[5] = {dim, n_in, n_in_padded, hidden, n_out}; any output may be null. */
+eggmi_status eggmi_mlp_shape(eggmi_mlp* m, int32_t* shape, int64_t* n_params, int64_t* n_data, int64_t* capacity);
+eggmi_status eggmi_mlp_get_params(eggmi_mlp* m, float* out, int64_t n);
+/* Replace the parameters (resets the optimiser state). */
+eggmi_status eggmi_mlp_set_params(eggmi_mlp* m, const float* in, int64_t n);
+/* Y[rows][n_out] for features X[rows][n_in_padded] (rounded to BF16). */
+eggmi_status eggmi_mlp_forward(eggmi_mlp* m, const float* X, int64_t rows, float* Y);
+/* Append training pairs X[rows][n_in_padded], T[rows][n_out]. */
+eggmi_status eggmi_mlp_data_add(eggmi_mlp* m, const float* X, const float* T, int64_t rows);
+/* Copy rows [offset, offset+rows) of the training set out (X as stored, BF16). */
+eggmi_status eggmi_mlp_data_get(eggmi_mlp* m, float* X, float* T, int64_t offset, int64_t rows);
+eggmi_status eggmi_mlp_data_clear(eggmi_mlp* m);
+/* Gradient of the mean squared error over the training rows rows_idx[batch]
+ * (batch a positive multiple of 256); *loss = that mean squared error. */
+eggmi_status eggmi_mlp_gradient(eggmi_mlp* m, const int32_t* rows_idx, int64_t batch, float* grad, double* loss);
+/* Adam (lr, betas 0.9 / 0.999) over shuffled batches of the training set;
+ * epoch_loss[epochs] receives each epoch's mean squared error. */
+eggmi_status eggmi_mlp_train(eggmi_mlp* m, int epochs, int64_t batch, float lr, uint64_t seed, double* epoch_loss);
+/* Run n_snap x (every + horizon) plain sweeps (omega) on the session and
+ * append, at each snapshot, one pair per synthesized interior DOF: its
+ * features and its displacement over the next `horizon` sweeps (local frame,
+ * over h). *added = pairs appended (stops early when the set is full). */
+eggmi_status eggmi_session_harvest(eggmi_session* s, eggmi_mlp* m, int n_snap, int every, int horizon,
+                                   double omega, int64_t* added);
+/* Use `m` (null: none) as the proposal of subsequent eggmi_run calls in the
+ * barrier and shape_size phases, its displacement scaled by `scale`. mode 0:
+ * an accepted proposal is the DOF's update; mode 1 (predict-correct): the
+ * plain Newton update (halvings, omega blend) continues from an accepted
+ * proposal. The session borrows m: keep it alive (or unset it) until the
+ * session is destroyed. */
+eggmi_status eggmi_session_set_proposal(eggmi_session* s, eggmi_mlp* m, int mode, float scale);
 
 /* Jacobi-preconditioned conjugate gradients in FP64 on a CSR operator
  * (rowptr[n+1], col/val[nnz]; SPD expected): the PCG skeleton of the

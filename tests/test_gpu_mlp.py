@@ -1,0 +1,150 @@
+"""The learned proposal on the MI355X: BF16 matrix-core forward and backward
+passes against the NumPy oracle (oracle/mlp.py), training, harvested pairs
+against features and displacements recomputed from the session's states,
+and the accept-or-fallback sweep."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import mlp as M
+from tests.cases import ogrid, single
+
+pytestmark = pytest.mark.gpu
+
+
+def _net(dim, capacity=0, seed=0):
+    from egg_amd._cpp import cpp_core
+
+    return cpp_core.ProposalMLP(dim, seed, capacity)
+
+
+def _session(c):
+    from egg_amd.smoothing import CppStructuredSweepSession
+
+    return CppStructuredSweepSession(c.ctx, c.bsc, c.X)
+
+
+def _data(dim, n, rng):
+    d = M.DIMS[dim]
+    X = np.zeros((n, d["n_in_padded"]), np.float32)
+    X[:, :d["n_in"]] = rng.standard_normal((n, d["n_in"]))
+    T = (0.5 * np.tanh(X[:, :d["n_out"]] - 0.3 * X[:, d["n_out"]:2 * d["n_out"]])).astype(np.float32)
+    return X, T
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_forward_matches_oracle(dim):
+    rng = np.random.default_rng(10 + dim)
+    net = _net(dim)
+    p = (rng.standard_normal(M.n_params(dim)) * 0.2).astype(np.float32)
+    net.set_params(p)
+    np.testing.assert_array_equal(net.get_params(), p)
+    X, _ = _data(dim, 1000, rng)
+    Y = net.forward(X)
+    Yo = M.forward(p, X, dim)
+    # same BF16 roundings; FP32 summation order and exp differ, which can flip
+    # the rounding of an activation (2^-8 relative) now and then
+    assert np.linalg.norm(Y - Yo) <= 2e-3 * np.linalg.norm(Yo)
+    assert np.abs(Y - Yo).max() <= 2e-2 * np.abs(Yo).max()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_gradient_matches_oracle(dim):
+    rng = np.random.default_rng(20 + dim)
+    net = _net(dim, capacity=1024)
+    p = (rng.standard_normal(M.n_params(dim)) * 0.2).astype(np.float32)
+    net.set_params(p)
+    X, T = _data(dim, 1024, rng)
+    net.data_add(X, T)
+    Xs, Ts = net.data_get(0, 1024)
+    np.testing.assert_array_equal(Xs, M.bf16(X))
+    np.testing.assert_array_equal(Ts, T)
+    rows = rng.permutation(1024)[:512].astype(np.int32)
+    g, loss = net.gradient(rows)
+    lo, go = M.loss_grad(p, X[rows], T[rows], dim)
+    assert loss == pytest.approx(lo, rel=1e-3)
+    assert np.linalg.norm(g - go) <= 1e-2 * np.linalg.norm(go)
+    # and the exact float64 gradient is close (BF16 operands)
+    _, g64 = M.loss_grad_f64(p, X[rows], T[rows], dim)
+    assert np.linalg.norm(g - g64) <= 5e-2 * np.linalg.norm(g64)
+
+
+def test_training_reduces_loss():
+    rng = np.random.default_rng(30)
+    net = _net(2, capacity=16384, seed=3)
+    X, T = _data(2, 16384, rng)
+    net.data_add(X, T)
+    _, l0 = net.gradient(np.arange(4096, dtype=np.int32))
+    hist = net.train(30, batch=1024, lr=3e-3, seed=1)
+    assert np.all(np.isfinite(hist))
+    assert hist[-1] < 0.1 * l0, (l0, hist)
+    # the trained network's own forward agrees with its training loss
+    Y = net.forward(X[:4096])
+    assert np.mean(np.sum((Y - T[:4096]) ** 2, 1)) < 0.15 * l0
+
+
+def test_harvested_pairs_match_states():
+    """One snapshot after 1 sweep, displacement over the next 2 sweeps: the
+    stored features and targets equal the oracle's features of the states
+    the same sweeps produce."""
+    c = single(n=17, seed=4)
+    s = _session(c)
+    net = _net(2, capacity=4096)
+    added = s.harvest(net, 1, every=1, horizon=2, omega=0.8)
+    n = 15 * 15
+    assert added == n
+    r = _session(c)
+    X0 = c.X
+    r.run(1, omega=0.8)
+    X1 = r.get_X().reshape(-1, 2)
+    r.run(2, omega=0.8)
+    X3 = r.get_X().reshape(-1, 2)
+    Xf, Tf = net.data_get(0, n)
+    ids = c.grid.blocks[0]
+    q = 0
+    for i in range(1, 16):
+        for j in range(1, 16):
+            ring = X1[ids[i - 1:i + 2, j - 1:j + 2].reshape(-1)]
+            E, h, f = M.frame_features(ring, X1[ids[i, j]] - X0[ids[i, j]], 2)
+            np.testing.assert_allclose(Xf[q], M.bf16(f.astype(np.float32)), rtol=2**-7, atol=1e-6)
+            np.testing.assert_allclose(Tf[q], E @ (X3[ids[i, j]] - X1[ids[i, j]]) / h, rtol=1e-3, atol=1e-5)
+            q += 1
+
+
+def test_zero_network_falls_back_to_plain():
+    """A network that proposes no motion is never accepted (the patch energy
+    must drop by more than the FP32 slack), so the sweep is the plain method."""
+    c = ogrid(33, 17)
+    net = _net(2)
+    net.set_params(np.zeros(M.n_params(2), np.float32))
+    a = _session(c)
+    b = _session(c)
+    b.set_proposal(net)
+    b.core.set_counters(True)
+    ea, ma = a.run(20, omega=0.8, report_every=5)
+    eb, mb = b.run(20, omega=0.8, report_every=5)
+    # the fallback is the plain kernel's code in another instantiation:
+    # equal up to the compiler's choice of fused multiply-adds
+    np.testing.assert_allclose(eb, ea, rtol=1e-6)
+    np.testing.assert_allclose(b.get_X(), a.get_X(), rtol=0, atol=1e-5)
+    cnt = b.core.last_counters(1)[0]
+    assert cnt[2] == 0 and cnt[1] == c.grid.n_free * 20
+
+
+def test_wild_network_keeps_mesh_valid():
+    """Large random proposals are mostly rejected; the accepted ones never
+    fold a cell and never raise a DOF's patch energy, so the energy stays
+    finite and every reported min det positive."""
+    c = ogrid(33, 17)
+    rng = np.random.default_rng(5)
+    net = _net(2)
+    net.set_params((rng.standard_normal(M.n_params(2)) * 0.5).astype(np.float32))
+    s = _session(c)
+    s.set_proposal(net)
+    s.core.set_counters(True)
+    e, m = s.run(30, omega=0.8, report_every=1)
+    assert np.all(np.isfinite(e)) and np.all(m > 0)
+    trials, updates, accepted = (int(v) for v in s.core.last_counters(1)[0])
+    assert updates == c.grid.n_free * 30 and 0 <= accepted < updates
