@@ -1,21 +1,31 @@
 """Benchmark: structured TMOP block-Jacobi sweeps on one MI355X.
 
-Workload (the metric's configuration): the 5-block 2D O-grid — a central
-129 x 129 block bounded by arcs of the circle r = 0.5 inside four 129 x 65
-ring blocks reaching the square [-2, 2]^2 (49 409 nodes, 48 897 free) — with
-2 % of the free nodes displaced by 1.5 x their local edge length (seed
-2026). It is untangled with the delta-continuation schedule (20 sweeps per
-delta, omega 0.5), then smoothed by block-Jacobi TMOP sweeps (omega 0.8,
-halo exchange every sweep). A step is one sweep over all free DOFs; the
-metric is node-updates/s = free DOFs x steps / wall time, with the state
-resident in HBM.
+Default workload (the metric's configuration): the 5-block 2D O-grid — a
+central 129 x 129 block bounded by arcs of the circle r = 0.5 inside four
+129 x 65 ring blocks reaching the square [-2, 2]^2 (49 409 nodes, 48 897
+free) — with 2 % of the free nodes displaced by 1.5 x their local edge
+length (seed 2026). It is untangled with the delta-continuation schedule (20
+sweeps per delta, omega 0.5), then smoothed by block-Jacobi TMOP sweeps
+(omega 0.8, halo exchange every sweep). A step is one sweep over all free
+DOFs; the metric is node-updates/s = free DOFs x steps / wall time, with the
+state resident in HBM.
 
-Also reported: the untangle wall time, parity against the FP64 CPU oracle
-on a short prefix, the sweep kernel's roofline (FP32 flops per launch from
-the op-counted constants / measured launch time) and the CPU oracle's
-throughput on the same workload.
+Also reported on that line: time to untangled + converged (relative energy
+within 1e-6 of the pinned FP64 plain-method energy E*, min det > 0) for the
+plain method, the plain method with the learned proposal (network trained
+on held-out seeds first) and FAS V(4,4); the sweep kernel's roofline (FP32
+flops per launch from the op-counted constants and kernel-counted line-search
+trials / measured launch time); and the CPU oracle on a bounded sample of the
+same workload, whose FP64 result is also the parity reference.
 
-  python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+Other workloads (one JSON line each, not the default):
+  --workload batched       64 independent multi-block grids (3-9 blocks, sides
+                           33..513), one session, autotuned tiles, FAS V(4,4)
+                           on 3 levels
+  --workload cubed_sphere  7-block 65^3 cubed-sphere hex grid, FAS 3 levels,
+                           plus the 33^3 Laplacian PCG
+
+  python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--workload W]
 """
 
 from __future__ import annotations
@@ -32,16 +42,301 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-# Op-counted flops per free-DOF update of the 2D interior kernel
-# (oracle/flopcount.cpp over egg_amd/csrc/tmop_math.hpp; shape metric):
-F_METRIC_2D = 1244  # 16-sample metric pass + Newton solve
-F_TRIAL_2D = 272  # one line-search trial
+# Op-counted flops per free-DOF update (oracle/flopcount.cpp over
+# egg_amd/csrc/tmop_math.hpp; shape metric): patch metric pass + Newton
+# solve, and one line-search trial.
+F_METRIC = {2: 1244, 3: 17465}
+F_TRIAL = {2: 272, 3: 3206}
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector peak
+PEAK_HBM_GBS = 8000.0
 SEED = 2026
+CONV_RTOL = 1e-6
+METRIC = "node-updates/s; time-to-untangled+converged, plain vs MLP-proposal, 1 GPU"
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def _threads():
+    n = len(os.sched_getaffinity(0))
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        n = min(n, int(os.environ["OMP_NUM_THREADS"]))
+    return n
+
+
+def _build(g):
+    from egg_amd.smoothing import (CppStructuredSweepSession, build_structured_context_from_block_maps,
+                                   build_sweep_context)
+
+    ctx = build_sweep_context(g.blocks, g.free, g.d)
+    bsc = build_structured_context_from_block_maps(g.d, g.blocks)
+    return ctx, bsc, CppStructuredSweepSession(ctx, bsc, g.X)
+
+
+def _roofline(s, dim, n_free, steps, omega):
+    """Average sweep-kernel launch time (HIP events on the session stream
+    around every launch of a replay) and the flops per launch from the
+    kernel-counted trials of that replay."""
+    core = s.core
+    X0 = s.get_X()
+    core.set_kernel_timing(True)
+    s.run(steps, omega=omega)
+    core.set_kernel_timing(False)
+    _, n_launch, k_ms = core.last_timing()
+    s.set_X(X0)
+    core.set_counters(True)  # separate replay: the counting atomics stay out of the timing
+    s.run(steps, omega=omega)
+    core.set_counters(False)
+    trials, updates, _ = (int(v) for v in core.last_counters(1)[0])
+    s.set_X(X0)
+    assert updates == n_free * steps, (updates, n_free * steps)
+    k_avg = k_ms / n_launch * 1e-3
+    tpu = trials / updates
+    flops = n_free * (F_METRIC[dim] + tpu * F_TRIAL[dim])
+    ach = flops / k_avg / 1e12
+    return {"bound": "valu", "achieved": ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "frac": ach / PEAK_FP32_TFLOPS, "traffic": None, "kernel": f"sweep_kernel<{dim}D, shape>",
+            "kernel_avg_us": k_avg * 1e6, "flops_per_launch": flops, "trials_per_update": tpu,
+            "flops_per_update": f"{F_METRIC[dim]} + trials x {F_TRIAL[dim]}",
+            "algorithmic_bytes_per_launch": n_free * 4 * dim * 2}
+
+
+def _converge(step, e_star, budget_s):
+    """Call step() (returns (energies, mindets, units)) until the last report
+    is within CONV_RTOL of e_star with min det > 0; returns (units, seconds)."""
+    done = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        e, m, u = step()
+        done += u
+        if abs(e[-1] - e_star) <= CONV_RTOL * abs(e_star) and m[-1] > 0:
+            return done, time.perf_counter() - t0, float(e[-1])
+    return None, time.perf_counter() - t0, float(e[-1])
+
+
+def _train_proposal(make_grid, seeds, horizon=2, snaps=40, every=50, epochs=10):
+    from egg_amd._cpp import cpp_core
+
+    net = cpp_core.ProposalMLP(2, 0, 6_000_000)
+    t0 = time.perf_counter()
+    from egg_amd.smoothing import cpp_untangle
+
+    for sd in seeds:
+        g = make_grid(sd)
+        ctx, bsc, sh = _build(g)
+        cpp_untangle(ctx, bsc, g.X, session=sh)
+        sh.harvest(net, snaps, every=every, horizon=horizon, omega=0.8)
+    t_h = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    hist = net.train(epochs, batch=4096, lr=1e-3, seed=1)
+    return net, {"pairs": int(net.shape()["n_data"]), "harvest_s": t_h, "train_s": time.perf_counter() - t0,
+                 "loss_first_last": [float(hist[0]), float(hist[-1])], "held_out_seeds": list(seeds),
+                 "horizon": horizon}
+
+
+def bench_ogrid(args):
+    from egg_amd import grids
+    from egg_amd.smoothing import cpp_untangle
+
+    make = lambda sd: grids.tangle(grids.ogrid_2d(129, 65), frac=0.02, factor=1.5, seed=sd)  # noqa: E731
+    t0 = time.perf_counter()
+    g = make(SEED)
+    ctx, bsc, s = _build(g)
+    n_free = g.n_free
+    log(f"setup {time.perf_counter() - t0:.1f}s  nodes={g.n_nodes} free={n_free} md0={grids.min_det(g):.3e}")
+
+    t0 = time.perf_counter()
+    Xu, md_u, outer, delta = cpp_untangle(ctx, bsc, g.X, session=s)
+    t_untangle = time.perf_counter() - t0
+    log(f"untangle {t_untangle:.3f}s outer={outer} min_det={md_u:.3e}")
+
+    s.run(args.warmup, omega=0.8)
+    roof = _roofline(s, 2, n_free, args.steps, 0.8)
+    t0 = time.perf_counter()
+    e, m = s.run(args.steps, omega=0.8)
+    t_run = time.perf_counter() - t0
+    value = n_free * args.steps / t_run
+    log(f"timed {args.steps} sweeps {t_run * 1e3:.2f} ms -> {value:.4g} node-updates/s  E={e[-1]:.9g}")
+    log(f"sweep kernel {roof['kernel_avg_us']:.2f} us/launch, {roof['trials_per_update']:.3f} trials/update "
+        f"-> {roof['achieved']:.3f} TFLOP/s ({100 * roof['frac']:.1f}% of FP32 peak)")
+
+    # time to untangled + converged, from the same tangled start
+    ttc = None
+    fix = ROOT / "tests" / "golden" / "estar_ogrid129.json"
+    if fix.exists() and not args.no_converge:
+        e_star = json.loads(fix.read_text())["E_star"]
+        ttc = {"E_star": e_star, "rtol": CONV_RTOL, "untangle_s": t_untangle}
+
+        def plain_step(chunk=1000):
+            e, m = s.run(chunk, omega=0.8, report_every=chunk)
+            return e, m, chunk
+
+        s.set_X(Xu)
+        n, t, E = _converge(plain_step, e_star, 60.0)
+        ttc["plain"] = {"sweeps": n, "s": t + t_untangle if n else None, "E_end": E}
+        log(f"plain: {n} sweeps, {t:.3f}s to converge")
+
+        s.set_X(Xu)
+        def fas_step(chunk=5):
+            e, m = s.run_fas(chunk, nu_pre=4, nu_post=4, omega=0.8)
+            return e, m, chunk
+        n, t, E = _converge(fas_step, e_star, 60.0)
+        ttc["fas_v44"] = {"cycles": n, "s": t + t_untangle if n else None, "E_end": E}
+        log(f"FAS V(4,4): {n} cycles, {t:.3f}s to converge")
+
+        net, info = _train_proposal(make, (11, 12, 13))
+        s.set_X(Xu)
+        s.set_proposal(net, mode="predict_correct", scale=0.5)
+        n, t, E = _converge(plain_step, e_star, 60.0)
+        s.set_proposal(None)
+        ttc["mlp_proposal"] = {"sweeps": n, "s": t + t_untangle if n else None, "E_end": E,
+                               "mode": "predict_correct", "scale": 0.5, **info}
+        log(f"MLP proposal: {n} sweeps, {t:.3f}s to converge ({info})")
+
+    cpu = parity = None
+    if not args.no_cpu:
+        from oracle import oracle as O  # the checker: CPU baseline and parity reference
+
+        threads = _threads()
+        n_cpu = args.cpu_sweeps
+        patches = O.materialize_patches(ctx.wire, g.blocks, 2)
+        rates, outs = {}, {}
+        for prec in ("f64", "f32"):
+            t0 = time.perf_counter()
+            outs[prec] = O.jacobi(2, "barrier", 0.0, 0.8, n_cpu, Xu, patches, ctx.energy_stencil, prec=prec,
+                                  threads=threads)
+            rates[prec] = n_free * n_cpu / (time.perf_counter() - t0)
+        best = max(rates, key=rates.get)
+        cpu = {"value": rates[best], "unit": "node-updates/s", "cores": threads, "kind": "port",
+               "sample": f"{n_cpu} sweeps of the same O-grid from the untangled state, OpenMP C oracle "
+                         f"{best.upper()} (f64 {rates['f64']:.4g}, f32 {rates['f32']:.4g} node-updates/s)"}
+        log(f"cpu {cpu}")
+        s.set_X(Xu)
+        ep, mp = s.run(n_cpu, omega=0.8)
+        Xg = s.get_X()
+        Xo, eo, mo = outs["f64"]
+        extent = float((g.X.max(0) - g.X.min(0)).max())
+        parity = {"sweeps": n_cpu, "max_abs_pos_err": float(np.abs(Xg - Xo).max()),
+                  "max_abs_pos_err_rel_extent": float(np.abs(Xg - Xo).max() / extent),
+                  "rel_energy_err": float(abs(ep[-1] - eo[-1]) / abs(eo[-1])),
+                  "min_det_err": float(abs(mp[-1] - mo[-1])), "oracle": "FP64 CPU oracle (oracle/egg_oracle.c)"}
+        log(f"parity {parity}")
+
+    return {
+        "metric": METRIC, "value": value, "unit": "node-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": t_run / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "2D 5-block O-grid (129x129 disc + 4x 129x65 rings, r=0.5 in [-2,2]^2), "
+                               "2% tangled by 1.5h, untangle then block-Jacobi TMOP sweeps, omega 0.8",
+                   "free_dofs": n_free, "seed": SEED, "precision": "FP32 sweeps, FP64 reductions"},
+        "time_to_untangled_s": t_untangle, "untangle_outer": outer,
+        "time_to_untangled_converged": ttc,
+        "final_energy": float(e[-1]), "final_min_det": float(m[-1]),
+        "parity": parity, "roofline": roof, "cpu_baseline": cpu,
+    }
+
+
+def _fas_levels_counts(s, nu_pre, nu_post, max_levels):
+    """DOF updates per level of one counted V-cycle (replayed from, and
+    restored to, the current state)."""
+    X0 = s.get_X()
+    s.core.set_counters(True)
+    s.run_fas(1, nu_pre=nu_pre, nu_post=nu_post, omega=0.8, max_levels=max_levels)
+    s.core.set_counters(False)
+    c = s.core.last_counters(max_levels)
+    s.set_X(X0)
+    return [int(v) for v in c[:, 1]]
+
+
+def bench_batched(args):
+    from egg_amd import grids
+    from egg_amd.smoothing import cpp_untangle
+
+    t0 = time.perf_counter()
+    g, index = grids.batched_2d(64, seed=SEED)
+    ctx, bsc, s = _build(g)
+    n_free = g.n_free
+    log(f"setup {time.perf_counter() - t0:.1f}s grids=64 blocks={len(g.blocks)} nodes={g.n_nodes} free={n_free}")
+    t0 = time.perf_counter()
+    Xu, md_u, outer, _ = cpp_untangle(ctx, bsc, g.X, session=s)
+    t_untangle = time.perf_counter() - t0
+    log(f"untangle {t_untangle:.3f}s outer={outer} min_det={md_u:.3e}")
+    t0 = time.perf_counter()
+    best, ms = s.core.autotune(5, 0.8)
+    log(f"autotune {time.perf_counter() - t0:.2f}s best={best} ms/5 sweeps={ms}")
+    roof = _roofline(s, 2, n_free, 20, 0.8)
+    t0 = time.perf_counter()
+    s.run(20, omega=0.8)
+    t_plain = time.perf_counter() - t0
+    counts = _fas_levels_counts(s, 4, 4, 3)
+    K = max(1, args.steps // 100)
+    t0 = time.perf_counter()
+    e, m = s.run_fas(K, nu_pre=4, nu_post=4, omega=0.8, max_levels=3)
+    t_fas = time.perf_counter() - t0
+    fine = counts[0] * K / t_fas
+    allv = sum(counts) * K / t_fas
+    log(f"FAS V(4,4) x{K}: {t_fas * 1e3:.1f} ms, fine {fine:.4g} / all-level {allv:.4g} node-updates/s, "
+        f"E {e[0]:.6g} -> {e[-1]:.6g}")
+    return {
+        "metric": METRIC, "value": fine, "unit": "node-updates/s", "n_gpus": 1, "steps": K, "warmup": 0,
+        "ms_per_step": t_fas / K * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "batched: 64 independent 2D strips of 3-9 blocks, sides from {33,41,..,513}, "
+                               "2% tangled by 1.5h; one session; FAS V(4,4) on 3 levels, autotuned tiles",
+                   "free_dofs": n_free, "blocks": len(g.blocks), "seed": SEED},
+        "value_is": "fine-level node-updates/s of FAS V(4,4) cycles",
+        "all_level_node_updates_per_s": allv, "updates_per_cycle_by_level": counts,
+        "plain_sweep_node_updates_per_s": n_free * 20 / t_plain,
+        "autotune": {"best_variant": int(best), "ms_per_variant": [float(v) for v in ms]},
+        "time_to_untangled_s": t_untangle, "untangle_outer": outer,
+        "energy_first_last": [float(e[0]), float(e[-1])], "final_min_det": float(m[-1]),
+        "roofline": roof, "cpu_baseline": None, "parity": "tests/test_gpu_batched.py (reduced batch)",
+    }
+
+
+def bench_cubed(args):
+    from egg_amd import grids, pcg
+    from egg_amd._cpp import cpp_core
+    from egg_amd.smoothing import cpp_untangle
+
+    t0 = time.perf_counter()
+    g = grids.tangle(grids.cubed_sphere_3d(65), frac=0.01, factor=1.5, seed=SEED)
+    ctx, bsc, s = _build(g)
+    n_free = g.n_free
+    log(f"setup {time.perf_counter() - t0:.1f}s nodes={g.n_nodes} free={n_free} md0={grids.min_det(g):.3e}")
+    t0 = time.perf_counter()
+    Xu, md_u, outer, _ = cpp_untangle(ctx, bsc, g.X, session=s)
+    t_untangle = time.perf_counter() - t0
+    log(f"untangle {t_untangle:.3f}s outer={outer} min_det={md_u:.3e}")
+    roof = _roofline(s, 3, n_free, 10, 0.8)
+    t0 = time.perf_counter()
+    e, m = s.run(args.steps // 20 or 1, omega=0.8)
+    t_plain = time.perf_counter() - t0
+    value = n_free * (args.steps // 20 or 1) / t_plain
+    counts = _fas_levels_counts(s, 4, 4, 3)
+    K = 5
+    t0 = time.perf_counter()
+    ef, mf = s.run_fas(K, nu_pre=4, nu_post=4, omega=0.8, max_levels=3)
+    t_fas = time.perf_counter() - t0
+    log(f"plain {value:.4g} node-updates/s; FAS x{K} {t_fas * 1e3:.1f} ms E {ef[0]:.6g}->{ef[-1]:.6g}")
+    rp, col, val = pcg.lattice_laplacian(33, 3)
+    b = np.random.default_rng(SEED).standard_normal(33**3)
+    x, iters, relres, status = pcg.pcg_solve(rp, col, val, b, rtol=1e-8, max_iter=1000)
+    pcg_ms = cpp_core.last_pcg_ms()
+    log(f"PCG 33^3: {iters} iterations, relres {relres:.2e}, {pcg_ms:.3f} ms")
+    return {
+        "metric": METRIC, "value": value, "unit": "node-updates/s", "n_gpus": 1, "steps": args.steps // 20 or 1,
+        "warmup": 0, "ms_per_step": t_plain / (args.steps // 20 or 1) * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "3D 7-block cubed sphere (65^3 blocks, r=1 in [-2,2]^3), 1% tangled by 1.5h",
+                   "free_dofs": n_free, "seed": SEED},
+        "fas_v44_3_levels": {"cycles": K, "ms_per_cycle": t_fas / K * 1e3, "updates_per_cycle_by_level": counts,
+                             "energy_first_last": [float(ef[0]), float(ef[-1])]},
+        "pcg_33cubed": {"iterations": int(iters), "relres": float(relres), "status": int(status),
+                        "solve_ms": pcg_ms},
+        "time_to_untangled_s": t_untangle, "untangle_outer": outer, "roofline": roof, "cpu_baseline": None,
+    }
 
 
 def main() -> int:
@@ -49,134 +344,18 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline and parity leg")
+    ap.add_argument("--no-converge", action="store_true", help="skip the time-to-converged runs")
     ap.add_argument("--cpu-sweeps", type=int, default=150)
+    ap.add_argument("--workload", default="ogrid", choices=("ogrid", "batched", "cubed_sphere"))
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the path shards by blocks across GPUs by design, see DESIGN.md)", file=sys.stderr)
+              "(the path would shard by blocks, one halo exchange per sweep, but only the one-GPU path is built)",
+              file=sys.stderr)
         return 2
-
-    from egg_amd import grids
-    from egg_amd.smoothing import (CppStructuredSweepSession, build_structured_context_from_block_maps,
-                                   build_sweep_context, cpp_untangle)
-
-    t0 = time.perf_counter()
-    g = grids.tangle(grids.ogrid_2d(129, 65), frac=0.02, factor=1.5, seed=SEED)
-    ctx = build_sweep_context(g.blocks, g.free, 2)
-    bsc = build_structured_context_from_block_maps(2, g.blocks)
-    s = CppStructuredSweepSession(ctx, bsc, g.X)
-    n_free = g.n_free
-    log(f"setup {time.perf_counter() - t0:.1f}s  nodes={g.n_nodes} free={n_free} md0={grids.min_det(g):.3e}")
-
-    # untangle (delta continuation), timed end to end
-    t0 = time.perf_counter()
-    Xu, md_u, outer, delta = cpp_untangle(ctx, bsc, g.X, session=s)
-    t_untangle = time.perf_counter() - t0
-    log(f"untangle {t_untangle:.3f}s outer={outer} min_det={md_u:.3e}")
-
-    # warmup + timed plain sweeps
-    s.run(args.warmup, omega=0.8)
-    X_start = s.get_X()
-    t0 = time.perf_counter()
-    e, m = s.run(args.steps, omega=0.8)
-    t_run = time.perf_counter() - t0
-    ms_dev, n_launch, _ = s.core.last_timing()
-    value = n_free * args.steps / t_run
-    log(f"timed {args.steps} sweeps {t_run * 1e3:.2f} ms -> {value:.4g} node-updates/s  E={e[-1]:.9g} md={m[-1]:.3e}")
-
-    # sweep-kernel time for the roofline: HIP events on the session stream
-    # around the timed region, which holds back-to-back sweep launches only
-    # (replicas are pushed by the sweep kernel; one halo refresh + copy at the
-    # start and one reduction at the end)
-    core = s.core
-    ms_dev, n_launch, _ = core.last_timing()
-    k_avg_s = ms_dev / n_launch * 1e-3
-    # line-search trials of the same trajectory, counted by the kernel in an
-    # untimed replay from the same state (the sweep is deterministic)
-    X_end = s.get_X()
-    s.set_X(X_start)
-    core.set_counters(True)
-    e_rep, _ = s.run(args.steps, omega=0.8)
-    core.set_counters(False)
-    trials, updates = (int(v) for v in core.last_counters(1)[0])
-    assert updates == n_free * args.steps, (updates, n_free * args.steps)
-    assert e_rep[-1] == e[-1] and np.array_equal(s.get_X(), X_end), "replay diverged"
-    s.set_X(X_end)
-    trials_per_update = trials / updates
-    flops = n_free * (F_METRIC_2D + trials_per_update * F_TRIAL_2D)
-    achieved = flops / k_avg_s / 1e12
-    log(f"sweep kernel avg {k_avg_s * 1e6:.2f} us over {n_launch} launches (events), "
-        f"{trials_per_update:.3f} trials/update -> {achieved:.3f} TFLOP/s")
-
-    # parity on a short prefix from the untangled state vs the FP64 oracle
-    from oracle import oracle as O
-
-    pre = 20
-    sp = CppStructuredSweepSession(ctx, bsc, Xu)
-    ep, mp = sp.run(pre, omega=0.8)
-    Xg = sp.get_X()
-    patches = O.materialize_patches(ctx.wire, g.blocks, 2)
-    t0 = time.perf_counter()
-    Xo, eo, mo = O.jacobi(2, "barrier", 0.0, 0.8, pre, Xu, patches, ctx.energy_stencil, prec="f64")
-    t_cpu64 = time.perf_counter() - t0
-    extent = float((g.X.max(0) - g.X.min(0)).max())
-    parity = {
-        "sweeps": pre,
-        "max_abs_pos_err": float(np.abs(Xg - Xo).max()),
-        "max_abs_pos_err_rel_extent": float(np.abs(Xg - Xo).max() / extent),
-        "rel_energy_err": float(abs(ep[-1] - eo[-1]) / abs(eo[-1])),
-        "min_det_err": float(abs(mp[-1] - mo[-1])),
-        "oracle": "FP64 CPU oracle (oracle/egg_oracle.c)",
-    }
-    log(f"parity {parity}")
-
-    cpu = None
-    if not args.no_cpu:
-        threads = len(os.sched_getaffinity(0))
-        if os.environ.get("OMP_NUM_THREADS", "").isdigit():
-            threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
-        n_cpu = args.cpu_sweeps
-        rates = {}
-        for prec in ("f64", "f32"):
-            t0 = time.perf_counter()
-            O.jacobi(2, "barrier", 0.0, 0.8, n_cpu, Xu, patches, ctx.energy_stencil, prec=prec, threads=threads)
-            rates[prec] = n_free * n_cpu / (time.perf_counter() - t0)
-        best = max(rates, key=rates.get)
-        cpu = {"value": rates[best], "unit": "node-updates/s", "cores": threads, "kind": "port",
-               "sample": f"{n_cpu} sweeps of the same O-grid from the untangled state, OpenMP C oracle "
-                         f"{best.upper()} (f64 {rates['f64']:.4g}, f32 {rates['f32']:.4g} node-updates/s)"}
-        log(f"cpu {cpu}")
-
-    out = {
-        "metric": "node-updates/s; time-to-untangled+converged, plain vs MLP-proposal, 1 GPU",
-        "value": value,
-        "unit": "node-updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": t_run / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": "2D 5-block O-grid (129x129 disc + 4x 129x65 rings, r=0.5 in [-2,2]^2), "
-                               "2% tangled by 1.5h, untangle then block-Jacobi TMOP sweeps, omega 0.8",
-                   "free_dofs": n_free, "seed": SEED, "precision": "FP32 sweeps, FP64 reductions"},
-        "time_to_untangled_s": t_untangle,
-        "untangle_outer": outer,
-        "final_energy": float(e[-1]),
-        "final_min_det": float(m[-1]),
-        "parity": parity,
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "kernel": "sweep2d (interior tiles + stored DOFs)", "kernel_avg_us": k_avg_s * 1e6,
-                     "flops_per_launch": flops, "trials_per_update": trials_per_update,
-                     "flops_per_update": f"{F_METRIC_2D} + trials x {F_TRIAL_2D}"},
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(out), flush=True)
+    fn = {"ogrid": bench_ogrid, "batched": bench_batched, "cubed_sphere": bench_cubed}[args.workload]
+    print(json.dumps(fn(args)), flush=True)
     return 0
 
 

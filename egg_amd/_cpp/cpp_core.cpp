@@ -466,6 +466,15 @@ class Session
         check(eggmi_last_counters(s_, out.mutable_data(), n_levels));
         return out;
     }
+    py::tuple autotune(int n_sweeps, double omega)
+    {
+        int nv = 0;
+        check(eggmi_set_tile_variant(s_, -1, &nv));
+        std::vector<double> ms(nv);
+        int best = 0;
+        check(eggmi_autotune(s_, n_sweeps, omega, ms.data(), nv, &best));
+        return py::make_tuple(best, to_array(ms));
+    }
     int64_t harvest(Mlp& m, int n_snap, int every, int horizon, double omega)
     {
         int64_t added = 0;
@@ -499,6 +508,12 @@ class Session
 };
 
 }  // namespace
+
+double& last_pcg_ms()
+{
+    static double ms = 0.0;
+    return ms;
+}
 
 PYBIND11_MODULE(cpp_core, m)
 {
@@ -534,6 +549,8 @@ PYBIND11_MODULE(cpp_core, m)
       .def("last_counters", &Session::last_counters, py::arg("n_levels") = 1,
            "Counters of the last run (counting on): rows per level of (line-search trials, DOF updates, "
            "accepted learned proposals).")
+      .def("autotune", &Session::autotune, py::arg("n_sweeps") = 20, py::arg("omega") = 0.8,
+           "Time every tile variant on the current state (restored), keep the fastest; returns (best, ms).")
       .def("harvest", &Session::harvest, py::arg("mlp"), py::arg("n_snap"), py::kw_only(), py::arg("every") = 8,
            py::arg("horizon") = 8, py::arg("omega") = 0.8,
            "Append (features, displacement over `horizon` plain sweeps) pairs of every interior DOF to the "
@@ -567,14 +584,16 @@ PYBIND11_MODULE(cpp_core, m)
           if (n > 0 && rowptr.data()[n] != (int64_t)col.size()) throw py::value_error("pcg: rowptr[n] != nnz");
           py::array_t<double> x(n);
           int iters = 0, status = 0;
-          double relres = 0.0;
+          double relres = 0.0, ms = 0.0;
           check(eggmi_pcg_csr(n, rowptr.data(), col.data(), val.data(), b.data(), x.mutable_data(), rtol, max_iter,
-                              &iters, &relres, &status));
+                              &iters, &relres, &status, &ms));
+          last_pcg_ms() = ms;
           return py::make_tuple(x, iters, relres, status);
       },
       py::arg("rowptr"), py::arg("col"), py::arg("val"), py::arg("b"), py::kw_only(), py::arg("rtol") = 1e-10,
       py::arg("max_iter") = 200,
       "Jacobi-preconditioned CG (FP64, on the GPU) for a CSR operator; returns (x, iters, relres, status).");
+    m.def("last_pcg_ms", [] { return last_pcg_ms(); }, "Device time (ms) of the last pcg_csr solve, uploads excluded.");
 
     m.def(
       "cpp_structured_sweep",

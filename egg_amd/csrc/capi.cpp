@@ -232,10 +232,18 @@ eggmi_status eggmi_set_tile_variant(eggmi_session* s, int variant, int* n_varian
         const int nv = eggmi::num_tile_variants(s->s.dim);
         if (n_variants) *n_variants = nv;
         if (variant < 0) return;
-        if (variant >= nv) throw eggmi::InvalidArgument("tile variant out of range");
-        s->s.tile_variant = variant;
-        s->s.fine.make_tiles(variant);
-        for (auto& c : s->s.coarse) c->make_tiles(variant);
+        s->s.set_tile_variant(variant);
+    });
+}
+
+eggmi_status eggmi_autotune(eggmi_session* s, int n_sweeps, double omega, double* ms, int capacity, int* best)
+{
+    return guarded([&] {
+        need(s, "session");
+        const auto t = s->s.autotune(n_sweeps, omega);
+        if (ms)
+            for (int v = 0; v < capacity && v < (int)t.size(); ++v) ms[v] = t[v];
+        if (best) *best = s->s.tile_variant;
     });
 }
 
@@ -423,7 +431,7 @@ eggmi_status eggmi_session_set_proposal(eggmi_session* s, eggmi_mlp* m, int mode
 
 eggmi_status eggmi_pcg_csr(int64_t n, const int64_t* rowptr, const int32_t* col, const double* val,
                            const double* b, double* x, double rtol, int max_iter, int* iters, double* relres,
-                           int* status)
+                           int* status, double* solve_ms)
 {
     return guarded([&] {
         if (n < 0) throw eggmi::InvalidArgument("pcg: n must be >= 0");
@@ -466,7 +474,18 @@ eggmi_status eggmi_pcg_csr(int64_t n, const int64_t* rowptr, const int32_t* col,
             eggmi::CsrView A {n, d_rp.p, d_col.p, d_val.p};
             eggmi::PcgWork w {d_r.p, d_z.p, d_p.p, d_ap.p, d_dinv.p, d_part.p, d_state.p};
             eggmi::PcgResult res {};
+            hipEvent_t e0, e1;
+            EGGMI_HIP(hipEventCreate(&e0));
+            EGGMI_HIP(hipEventCreate(&e1));
+            EGGMI_HIP(hipEventRecord(e0, s));
             if (n > 0) res = eggmi::pcg_solve_device(A, d_b.p, d_x.p, rtol, max_iter, w, s);
+            EGGMI_HIP(hipEventRecord(e1, s));
+            EGGMI_HIP(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            EGGMI_HIP(hipEventElapsedTime(&ms, e0, e1));
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            if (solve_ms) *solve_ms = ms;
             if (n > 0) EGGMI_HIP(hipMemcpy(x, d_x.p, n * sizeof(double), hipMemcpyDeviceToHost));
             if (iters) *iters = res.iters;
             if (relres) *relres = res.relres;

@@ -797,6 +797,64 @@ void Session::counters_end()
                 last_counters[(size_t)l * kCountersPerLevel + j] += h[l * per + (size_t)r * kCounterStride + j];
 }
 
+void Session::set_tile_variant(int v)
+{
+    if (v < 0 || v >= num_tile_variants(dim)) throw InvalidArgument("tile variant out of range");
+    tile_variant = v;
+    if (host_only) return;
+    fine.make_tiles(v);
+    for (auto& c : coarse) c->make_tiles(v);
+}
+
+std::vector<double> Session::autotune(int n_sweeps, double omega)
+{
+    need_device();
+    if (n_sweeps < 1) throw InvalidArgument("autotune: n_sweeps must be >= 1");
+    Level& L = fine;
+    const size_t n = (size_t)L.plane * dim;
+    DArr<float> save[2];
+    for (int b = 0; b < 2; ++b) {
+        save[b].alloc(n);
+        EGGMI_HIP(hipMemcpyAsync(save[b].p, L.x[b].p, n * sizeof(float), hipMemcpyDeviceToDevice, stream));
+    }
+    const int cur0 = L.cur;
+    const bool pair0 = pair_valid;
+    const bool count0 = counting;
+    const Mlp* prop0 = proposal;
+    counting = false;
+    proposal = nullptr;
+    const int nv = num_tile_variants(dim);
+    std::vector<double> ms(nv, 0.0);
+    try {
+        for (int v = 0; v < nv; ++v) {
+            set_tile_variant(v);
+            run_jacobi(1, 0, 0.0, omega, 1);  // warm the variant's code
+            for (int b = 0; b < 2; ++b)
+                EGGMI_HIP(hipMemcpyAsync(L.x[b].p, save[b].p, n * sizeof(float), hipMemcpyDeviceToDevice, stream));
+            L.cur = cur0;
+            pair_valid = pair0;
+            run_jacobi(n_sweeps, 0, 0.0, omega, n_sweeps);
+            ms[v] = last_ms;
+            for (int b = 0; b < 2; ++b)
+                EGGMI_HIP(hipMemcpyAsync(L.x[b].p, save[b].p, n * sizeof(float), hipMemcpyDeviceToDevice, stream));
+            L.cur = cur0;
+            pair_valid = pair0;
+        }
+    } catch (...) {
+        counting = count0;
+        proposal = prop0;
+        throw;
+    }
+    int best = 0;
+    for (int v = 1; v < nv; ++v)
+        if (ms[v] < ms[best]) best = v;
+    set_tile_variant(best);
+    EGGMI_HIP(hipStreamSynchronize(stream));
+    counting = count0;
+    proposal = prop0;
+    return ms;
+}
+
 void Session::get_X(double* out) const
 {
     need_device();

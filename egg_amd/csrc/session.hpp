@@ -223,6 +223,10 @@ struct Session {
     void set_X(const double* X);
     std::vector<float> packed() const;
     void need_device() const;
+    void set_tile_variant(int v);
+    // time n_sweeps plain sweeps per tile variant from the current state
+    // (restored after each trial), keep the fastest; returns ms per variant
+    std::vector<double> autotune(int n_sweeps, double omega);
     // FAS internals
     void build_hierarchy();
     void upload_level(Level& L);

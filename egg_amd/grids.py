@@ -284,12 +284,15 @@ def strip_2d(sides: list[int], ny: int, *, x0: float = 0.0, y0: float = 0.0, h: 
 
 
 def batched_2d(n_grids: int = 64, *, seed: int = 0, min_blocks: int = 3, max_blocks: int = 9,
-               sides=tuple(range(33, 514, 8)), frac: float = 0.02, factor: float = 1.5) -> tuple[Grid, list]:
+               sides=tuple(range(33, 514, 8)), frac: float = 0.02, factor: float = 1.5,
+               jitter: float = 0.0) -> tuple[Grid, list]:
     """Many independent multi-block grids in one Grid (disjoint global ids):
     grid g is a conforming strip of 3..9 blocks whose sides are drawn from
     {33, 41, ..., 513} (n = 1 mod 8, so three multigrid levels exist), each
-    2 % tangled by 1.5 x the local edge length. Returns the union grid and
-    the per-grid (first block, block count, first node, node count)."""
+    2 % tangled by 1.5 x the local edge length (frac = 0: not tangled; free
+    nodes then move by U(+-jitter h) instead, which folds no cell for
+    jitter < 0.25). Returns the union grid and the per-grid (first block,
+    block count, first node, node count)."""
     rng = np.random.default_rng(seed)
     blocks, Xs, frees, index = [], [], [], []
     node0 = 0
@@ -298,7 +301,13 @@ def batched_2d(n_grids: int = 64, *, seed: int = 0, min_blocks: int = 3, max_blo
         ny = int(rng.choice(sides))
         nx = [int(rng.choice(sides)) for _ in range(nb)]
         g = strip_2d(nx, ny, x0=0.0, y0=0.0)
-        g = tangle(g, frac=frac, factor=factor, seed=int(rng.integers(1 << 30)))
+        if frac > 0:
+            g = tangle(g, frac=frac, factor=factor, seed=int(rng.integers(1 << 30)))
+        elif jitter > 0:
+            h = 1.0 / (ny - 1)
+            X = g.X.copy()
+            X[g.free] += rng.uniform(-jitter * h, jitter * h, size=(int(g.free.sum()), 2))
+            g = Grid(2, g.blocks, X, g.free)
         index.append((len(blocks), nb, node0, g.n_nodes))
         blocks += [b + node0 for b in g.blocks]
         Xs.append(g.X)

@@ -144,6 +144,13 @@ eggmi_status eggmi_halo_exchange(eggmi_session* s);
  * in *n_variants when variant < 0. */
 eggmi_status eggmi_set_tile_variant(eggmi_session* s, int variant, int* n_variants);
 
+/* Autotuner (no reference counterpart): time n_sweeps plain sweeps (omega)
+ * with every tile variant from the session's current state, which is
+ * restored after each trial (the variants give bitwise-identical results),
+ * and keep the fastest. ms[v] (v < capacity) receives each variant's time,
+ * *best the variant selected. */
+eggmi_status eggmi_autotune(eggmi_session* s, int n_sweeps, double omega, double* ms, int capacity, int* best);
+
 /* Device time of the last eggmi_run / eggmi_run_fas call in ms (HIP events
  * on the session stream), the number of sweep-kernel launches it made and,
  * when kernel timing is on, the summed duration of those launches. */
@@ -216,10 +223,11 @@ eggmi_status eggmi_session_set_proposal(eggmi_session* s, eggmi_mlp* m, int mode
  * (src/control_solve.hpp:761-818): x0 = 0, stop when |r| <= rtol |b|,
  * stop on non-positive curvature. x receives the solution; *status is 1
  * converged, 2 non-positive curvature, 3 zero right-hand side, 0 max_iter
- * reached. */
+ * reached; *solve_ms (may be null) the device time of the solve, uploads
+ * excluded. */
 eggmi_status eggmi_pcg_csr(int64_t n, const int64_t* rowptr, const int32_t* col, const double* val,
                            const double* b, double* x, double rtol, int max_iter, int* iters, double* relres,
-                           int* status);
+                           int* status, double* solve_ms);
 
 const char* eggmi_last_error(void);
 
