@@ -74,24 +74,20 @@ def _build(g):
     return ctx, bsc, CppStructuredSweepSession(ctx, bsc, g.X)
 
 
-def _roofline(s, dim, n_free, steps, omega):
-    """Average sweep-kernel launch time (HIP events on the session stream
-    around every launch of a replay) and the flops per launch from the
-    kernel-counted trials of that replay."""
+def _roofline(s, dim, n_free, steps, omega, k_avg):
+    """Roofline of the sweep kernel: k_avg = its average launch time (HIP
+    events on the session stream around the timed region, which holds only
+    back-to-back sweep launches); flops per launch from the op-counted
+    constants and the line-search trials the kernel counts in an untimed
+    replay of the same trajectory (the sweep is deterministic)."""
     core = s.core
     X0 = s.get_X()
-    core.set_kernel_timing(True)
-    s.run(steps, omega=omega)
-    core.set_kernel_timing(False)
-    _, n_launch, k_ms = core.last_timing()
-    s.set_X(X0)
-    core.set_counters(True)  # separate replay: the counting atomics stay out of the timing
+    core.set_counters(True)
     s.run(steps, omega=omega)
     core.set_counters(False)
     trials, updates, _ = (int(v) for v in core.last_counters(1)[0])
     s.set_X(X0)
     assert updates == n_free * steps, (updates, n_free * steps)
-    k_avg = k_ms / n_launch * 1e-3
     tpu = trials / updates
     flops = n_free * (F_METRIC[dim] + tpu * F_TRIAL[dim])
     ach = flops / k_avg / 1e12
@@ -100,6 +96,16 @@ def _roofline(s, dim, n_free, steps, omega):
             "kernel_avg_us": k_avg * 1e6, "flops_per_launch": flops, "trials_per_update": tpu,
             "flops_per_update": f"{F_METRIC[dim]} + trials x {F_TRIAL[dim]}",
             "algorithmic_bytes_per_launch": n_free * 4 * dim * 2}
+
+
+def _timed(s, n, omega):
+    """n back-to-back sweeps: (energies, mindets, wall s, kernel avg s)."""
+    X0 = s.get_X()
+    t0 = time.perf_counter()
+    e, m = s.run(n, omega=omega)
+    wall = time.perf_counter() - t0
+    ms, launches, _ = s.core.last_timing()
+    return e, m, wall, ms * 1e-3 / launches, X0
 
 
 def _converge(step, e_star, budget_s):
@@ -152,11 +158,12 @@ def bench_ogrid(args):
     log(f"untangle {t_untangle:.3f}s outer={outer} min_det={md_u:.3e}")
 
     s.run(args.warmup, omega=0.8)
-    roof = _roofline(s, 2, n_free, args.steps, 0.8)
-    t0 = time.perf_counter()
-    e, m = s.run(args.steps, omega=0.8)
-    t_run = time.perf_counter() - t0
+    e, m, t_run, k_avg, X_start = _timed(s, args.steps, 0.8)
     value = n_free * args.steps / t_run
+    X_end = s.get_X()
+    s.set_X(X_start)
+    roof = _roofline(s, 2, n_free, args.steps, 0.8, k_avg)
+    s.set_X(X_end)
     log(f"timed {args.steps} sweeps {t_run * 1e3:.2f} ms -> {value:.4g} node-updates/s  E={e[-1]:.9g}")
     log(f"sweep kernel {roof['kernel_avg_us']:.2f} us/launch, {roof['trials_per_update']:.3f} trials/update "
         f"-> {roof['achieved']:.3f} TFLOP/s ({100 * roof['frac']:.1f}% of FP32 peak)")
@@ -265,10 +272,9 @@ def bench_batched(args):
     t0 = time.perf_counter()
     best, ms = s.core.autotune(5, 0.8)
     log(f"autotune {time.perf_counter() - t0:.2f}s best={best} ms/5 sweeps={ms}")
-    roof = _roofline(s, 2, n_free, 20, 0.8)
-    t0 = time.perf_counter()
-    s.run(20, omega=0.8)
-    t_plain = time.perf_counter() - t0
+    _, _, t_plain, k_avg, X_start = _timed(s, 20, 0.8)
+    s.set_X(X_start)
+    roof = _roofline(s, 2, n_free, 20, 0.8, k_avg)
     counts = _fas_levels_counts(s, 4, 4, 3)
     K = max(1, args.steps // 100)
     t0 = time.perf_counter()
@@ -309,11 +315,13 @@ def bench_cubed(args):
     Xu, md_u, outer, _ = cpp_untangle(ctx, bsc, g.X, session=s)
     t_untangle = time.perf_counter() - t0
     log(f"untangle {t_untangle:.3f}s outer={outer} min_det={md_u:.3e}")
-    roof = _roofline(s, 3, n_free, 10, 0.8)
-    t0 = time.perf_counter()
-    e, m = s.run(args.steps // 20 or 1, omega=0.8)
-    t_plain = time.perf_counter() - t0
-    value = n_free * (args.steps // 20 or 1) / t_plain
+    n3 = args.steps // 20 or 1
+    e, m, t_plain, k_avg, X_start = _timed(s, n3, 0.8)
+    value = n_free * n3 / t_plain
+    X_end = s.get_X()
+    s.set_X(X_start)
+    roof = _roofline(s, 3, n_free, n3, 0.8, k_avg)
+    s.set_X(X_end)
     counts = _fas_levels_counts(s, 4, 4, 3)
     K = 5
     t0 = time.perf_counter()
@@ -326,8 +334,8 @@ def bench_cubed(args):
     pcg_ms = cpp_core.last_pcg_ms()
     log(f"PCG 33^3: {iters} iterations, relres {relres:.2e}, {pcg_ms:.3f} ms")
     return {
-        "metric": METRIC, "value": value, "unit": "node-updates/s", "n_gpus": 1, "steps": args.steps // 20 or 1,
-        "warmup": 0, "ms_per_step": t_plain / (args.steps // 20 or 1) * 1e3, "higher_is_better": True,
+        "metric": METRIC, "value": value, "unit": "node-updates/s", "n_gpus": 1, "steps": n3,
+        "warmup": 0, "ms_per_step": t_plain / n3 * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "3D 7-block cubed sphere (65^3 blocks, r=1 in [-2,2]^3), 1% tangled by 1.5h",
                    "free_dofs": n_free, "seed": SEED},

@@ -466,6 +466,12 @@ class Session
         check(eggmi_last_counters(s_, out.mutable_data(), n_levels));
         return out;
     }
+    py::tuple energy(const std::string& phase, double delta)
+    {
+        double e = 0.0, m = 0.0;
+        check(eggmi_energy(s_, phase_id(phase), delta, &e, &m));
+        return py::make_tuple(e, m);
+    }
     py::tuple autotune(int n_sweeps, double omega)
     {
         int nv = 0;
@@ -549,6 +555,8 @@ PYBIND11_MODULE(cpp_core, m)
       .def("last_counters", &Session::last_counters, py::arg("n_levels") = 1,
            "Counters of the last run (counting on): rows per level of (line-search trials, DOF updates, "
            "accepted learned proposals).")
+      .def("energy", &Session::energy, py::kw_only(), py::arg("phase") = "barrier", py::arg("delta") = 0.0,
+           "(energy, min det) of the resident state, reduced on the device in FP64.")
       .def("autotune", &Session::autotune, py::arg("n_sweeps") = 20, py::arg("omega") = 0.8,
            "Time every tile variant on the current state (restored), keep the fastest; returns (best, ms).")
       .def("harvest", &Session::harvest, py::arg("mlp"), py::arg("n_snap"), py::kw_only(), py::arg("every") = 8,

@@ -236,6 +236,16 @@ eggmi_status eggmi_set_tile_variant(eggmi_session* s, int variant, int* n_varian
     });
 }
 
+eggmi_status eggmi_energy(eggmi_session* s, int phase, double delta, double* energy, double* mindet)
+{
+    return guarded([&] {
+        need(s, "session");
+        const auto r = s->s.energy(phase_kind(phase), delta);
+        if (energy) *energy = r.first;
+        if (mindet) *mindet = r.second;
+    });
+}
+
 eggmi_status eggmi_autotune(eggmi_session* s, int n_sweeps, double omega, double* ms, int capacity, int* best)
 {
     return guarded([&] {

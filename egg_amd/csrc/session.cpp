@@ -797,6 +797,17 @@ void Session::counters_end()
                 last_counters[(size_t)l * kCountersPerLevel + j] += h[l * per + (size_t)r * kCounterStride + j];
 }
 
+std::pair<double, double> Session::energy(int kind, double delta)
+{
+    need_device();
+    if ((int)d_reports.n < 2) d_reports.alloc(2);
+    reduce(fine, fine.x[fine.cur].p, kind, delta, d_reports.p, d_reports.p + 1);
+    double h[2];
+    EGGMI_HIP(hipMemcpyAsync(h, d_reports.p, sizeof(h), hipMemcpyDeviceToHost, stream));
+    EGGMI_HIP(hipStreamSynchronize(stream));
+    return {h[0], h[1]};
+}
+
 void Session::set_tile_variant(int v)
 {
     if (v < 0 || v >= num_tile_variants(dim)) throw InvalidArgument("tile variant out of range");

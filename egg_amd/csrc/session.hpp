@@ -224,6 +224,8 @@ struct Session {
     std::vector<float> packed() const;
     void need_device() const;
     void set_tile_variant(int v);
+    // FP64 energy and min det of the resident state (one reduction)
+    std::pair<double, double> energy(int kind, double delta);
     // time n_sweeps plain sweeps per tile variant from the current state
     // (restored after each trial), keep the fastest; returns ms per variant
     std::vector<double> autotune(int n_sweeps, double omega);

@@ -33,19 +33,44 @@ template <int D> struct Vec {
     float v[D];
 };
 
-// Butterfly reductions inside aligned groups of W lanes (W a power of 2).
-// xor-butterflies give every lane of the group the bitwise-same result.
+// Reductions inside aligned groups of W lanes (W = 4, 8, 16 or 64) with DPP
+// lane moves (a VALU modifier, no LDS round trip). Every step pairs lanes
+// symmetrically (lane a takes b's value exactly when b takes a's), so by
+// commutativity every lane of the group ends with the bitwise-same result:
+//   W >= 16: row_ror 8 (= xor 8), then row_ror 4 (= xor 4 once the values
+//            repeat with period 8); then quad_perm xor 1 and xor 2;
+//   W == 8:  the two quad sums swapped by row_half_mirror;
+//   W == 64: the four 16-lane row sums combined by xor 16 and xor 32.
+template <int CTRL> __device__ __forceinline__ float dpp_mov(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
+constexpr int kDppRor8 = 0x128, kDppRor4 = 0x124, kDppQuadXor1 = 0xB1, kDppQuadXor2 = 0x4E,
+              kDppHalfMirror = 0x141;
+
+template <int W, class Op> __device__ __forceinline__ float group_reduce(float v, Op op)
+{
+    static_assert(W == 4 || W == 8 || W == 16 || W == 64, "group width");
+    if constexpr (W >= 16) {
+        v = op(v, dpp_mov<kDppRor8>(v));
+        v = op(v, dpp_mov<kDppRor4>(v));
+    }
+    v = op(v, dpp_mov<kDppQuadXor1>(v));
+    v = op(v, dpp_mov<kDppQuadXor2>(v));
+    if constexpr (W == 8) v = op(v, dpp_mov<kDppHalfMirror>(v));
+    if constexpr (W == 64) {
+        v = op(v, __shfl_xor(v, 16, 64));
+        v = op(v, __shfl_xor(v, 32, 64));
+    }
+    return v;
+}
 template <int W> __device__ __forceinline__ float group_sum(float v)
 {
-#pragma unroll
-    for (int m = 1; m < W; m <<= 1) v += __shfl_xor(v, m, 64);
-    return v;
+    return group_reduce<W>(v, [](float x, float y) { return x + y; });
 }
 template <int W> __device__ __forceinline__ float group_min(float v)
 {
-#pragma unroll
-    for (int m = 1; m < W; m <<= 1) v = fminf(v, __shfl_xor(v, m, 64));
-    return v;
+    return group_reduce<W>(v, [](float x, float y) { return fminf(x, y); });
 }
 
 // Metric value, gradient and contracted Hessian (upper triangle, row-major)
@@ -254,13 +279,13 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
 
 // Tile of TX (last axis) x TY x TZ nodes; 2^D lanes per node.
 template <int D, int KIND, int TX, int TY, int TZ, bool TAU, bool GRAD, bool PROP>
-__device__ void tile_sweep(const SweepArgs& a)
+__device__ void tile_sweep(const SweepArgs& a, int tile)
 {
     constexpr int NC = 1 << D;
     constexpr int NT = NC * TX * TY * TZ;
     constexpr int LX = TX + 2, LY = TY + 2, LZ = (D == 3) ? TZ + 2 : 1;
     __shared__ float sh[D][LZ * LY * LX];
-    const TileDesc td = a.tiles[blockIdx.x];
+    const TileDesc td = a.tiles[tile];
     const BlockDesc bd = a.blocks[td.blk];
     const long long PL = a.plane;
     const int tid = threadIdx.x;
@@ -531,12 +556,15 @@ __global__ __launch_bounds__((1 << D) * TX * TY * TZ) void sweep_kernel(const Sw
 {
     constexpr int NT = (1 << D) * TX * TY * TZ;
     constexpr int G = (D == 2) ? 16 : 64;
-    if (int(blockIdx.x) >= a.n_tiles) {
-        const int gid = (int(blockIdx.x) - a.n_tiles) * (NT / G) + int(threadIdx.x) / G;
+    // stored-DOF workgroups first: their gather chains are the longest, so
+    // they start while the tiles fill the machine
+    const int n_st = int(gridDim.x) - a.n_tiles;
+    if (int(blockIdx.x) < n_st) {
+        const int gid = int(blockIdx.x) * (NT / G) + int(threadIdx.x) / G;
         stored_group<D, KIND, TAU, GRAD, G>(a, gid, int(threadIdx.x) % G, gid < a.st.n);
         return;
     }
-    tile_sweep<D, KIND, TX, TY, TZ, TAU, GRAD, PROP>(a);
+    tile_sweep<D, KIND, TX, TY, TZ, TAU, GRAD, PROP>(a, int(blockIdx.x) - n_st);
 }
 
 template <int D, int KIND, int TX, int TY, int TZ, bool TAU, bool GRAD, bool PROP = false>

@@ -270,11 +270,11 @@ __host__ __device__ __forceinline__ void solve2(R h00, R h01, R h10, R h11, R g0
         x0 = x1 = R(0);
         return;
     }
-    const R inv = R(1) / (h00 * h11 - h01 * h10);
+    const R inv = rcp(h00 * h11 - h01 * h10);
     x0 = -inv * (h11 * g0 - h01 * g1);
     x1 = -inv * (-h10 * g0 + h00 * g1);
     if (!isfinite(x0) || !isfinite(x1)) {
-        const R c = R(-0.1) / gn;
+        const R c = R(-0.1) * rcp(gn);
         x0 = c * g0;
         x1 = c * g1;
     }
@@ -290,7 +290,7 @@ template <class R> __host__ __device__ __forceinline__ void solve3(const R* H, c
     const R c00 = H[4] * H[8] - H[5] * H[7];
     const R c01 = -(H[3] * H[8] - H[5] * H[6]);
     const R c02 = H[3] * H[7] - H[4] * H[6];
-    const R inv = R(1) / (H[0] * c00 + H[1] * c01 + H[2] * c02);
+    const R inv = rcp(H[0] * c00 + H[1] * c01 + H[2] * c02);
     const R c10 = -(H[1] * H[8] - H[2] * H[7]);
     const R c11 = H[0] * H[8] - H[2] * H[6];
     const R c12 = -(H[0] * H[7] - H[1] * H[6]);
@@ -301,7 +301,7 @@ template <class R> __host__ __device__ __forceinline__ void solve3(const R* H, c
     x[1] = -inv * (c01 * g[0] + c11 * g[1] + c21 * g[2]);
     x[2] = -inv * (c02 * g[0] + c12 * g[1] + c22 * g[2]);
     if (!isfinite(x[0]) || !isfinite(x[1]) || !isfinite(x[2])) {
-        const R c = R(-0.1) / gn;
+        const R c = R(-0.1) * rcp(gn);
         x[0] = c * g[0];
         x[1] = c * g[1];
         x[2] = c * g[2];

@@ -254,12 +254,18 @@ def cpp_untangle(ctx, grid, X: np.ndarray, *, device: str = "auto", sweeps_per_d
     untangle sweeps per step on ``delta_k = delta_0 * shrink**k`` from
     ``delta0_factor * |min det A|`` until ``min det A > margin``.
     Returns ``(X_out, mindet, outer_iters, delta_final)``; pass ``session``
-    to keep working on a resident store."""
-    md = _grid_mindet(X, ctx.energy_stencil, ctx.d)
-    if md > margin:
-        return X, md, 0, 0.0
+    (whose resident state is X) to keep working on a resident store; its
+    min det is then reduced on the device instead of the host."""
     if session is None:
+        md = _grid_mindet(X, ctx.energy_stencil, ctx.d)
+        if md > margin:
+            return X, md, 0, 0.0
         session = CppStructuredSweepSession(ctx, _as_bsc(grid, ctx.d), X, device=device)
+    else:
+        # the session holds X: reduce its min det on the device
+        md = float(session.core.energy(phase="barrier")[1])
+        if md > margin:
+            return X, md, 0, 0.0
     delta = delta0_factor * max(abs(md), 1e-12)
     outer = 0
     for _ in range(max_outer):

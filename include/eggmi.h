@@ -144,6 +144,10 @@ eggmi_status eggmi_halo_exchange(eggmi_session* s);
  * in *n_variants when variant < 0. */
 eggmi_status eggmi_set_tile_variant(eggmi_session* s, int variant, int* n_variants);
 
+/* Energy and min det A of the resident state (the report reduction of
+ * eggmi_run, src/sweep.hpp:1881-1922, FP64 accumulation) without sweeping. */
+eggmi_status eggmi_energy(eggmi_session* s, int phase, double delta, double* energy, double* mindet);
+
 /* Autotuner (no reference counterpart): time n_sweeps plain sweeps (omega)
  * with every tile variant from the session's current state, which is
  * restored after each trial (the variants give bitwise-identical results),
