@@ -157,6 +157,8 @@ def bench_ogrid(args):
     t_untangle = time.perf_counter() - t0
     log(f"untangle {t_untangle:.3f}s outer={outer} min_det={md_u:.3e}")
 
+    tune_best, tune_ms = s.core.autotune(20, 0.8)
+    log(f"autotune: variant {tune_best}, ms per 20 sweeps {np.round(tune_ms, 4).tolist()}")
     s.run(args.warmup, omega=0.8)
     e, m, t_run, k_avg, X_start = _timed(s, args.steps, 0.8)
     value = n_free * args.steps / t_run
@@ -237,6 +239,7 @@ def bench_ogrid(args):
         "config": {"workload": "2D 5-block O-grid (129x129 disc + 4x 129x65 rings, r=0.5 in [-2,2]^2), "
                                "2% tangled by 1.5h, untangle then block-Jacobi TMOP sweeps, omega 0.8",
                    "free_dofs": n_free, "seed": SEED, "precision": "FP32 sweeps, FP64 reductions"},
+        "autotune": {"best_variant": int(tune_best), "ms_per_20_sweeps": [float(v) for v in tune_ms]},
         "time_to_untangled_s": t_untangle, "untangle_outer": outer,
         "time_to_untangled_converged": ttc,
         "final_energy": float(e[-1]), "final_min_det": float(m[-1]),

@@ -119,5 +119,8 @@ struct TileShape {
 };
 int num_tile_variants(int dim);
 TileShape tile_variant_shape(int dim, int v);
+// whether the variant has a learned-proposal kernel (the 2D one-lane-per-
+// sample variants do not)
+bool tile_variant_has_proposal(int dim, int v);
 
 }  // namespace eggmi

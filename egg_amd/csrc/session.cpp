@@ -2,6 +2,8 @@
 #include "session.hpp"
 #include "mlp_host.hpp"
 
+#include <limits>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -811,6 +813,9 @@ std::pair<double, double> Session::energy(int kind, double delta)
 void Session::set_tile_variant(int v)
 {
     if (v < 0 || v >= num_tile_variants(dim)) throw InvalidArgument("tile variant out of range");
+    if (proposal && !tile_variant_has_proposal(dim, v))
+        throw InvalidArgument("tile variant " + std::to_string(v) + " has no learned-proposal kernel; "
+                              "unset the proposal first");
     tile_variant = v;
     if (host_only) return;
     fine.make_tiles(v);
@@ -838,6 +843,10 @@ std::vector<double> Session::autotune(int n_sweeps, double omega)
     std::vector<double> ms(nv, 0.0);
     try {
         for (int v = 0; v < nv; ++v) {
+            if (prop0 && !tile_variant_has_proposal(dim, v)) {
+                ms[v] = std::numeric_limits<double>::infinity();  // not usable with the proposal set
+                continue;
+            }
             set_tile_variant(v);
             run_jacobi(1, 0, 0.0, omega, 1);  // warm the variant's code
             for (int b = 0; b < 2; ++b)
@@ -859,6 +868,7 @@ std::vector<double> Session::autotune(int n_sweeps, double omega)
     int best = 0;
     for (int v = 1; v < nv; ++v)
         if (ms[v] < ms[best]) best = v;
+    proposal = prop0;
     set_tile_variant(best);
     EGGMI_HIP(hipStreamSynchronize(stream));
     counting = count0;

@@ -551,6 +551,172 @@ __device__ void tile_sweep(const SweepArgs& a, int tile)
     }
 }
 
+// ------------------------------------------------------------------ 2D samples
+
+// 2D tile of TX x TY nodes with one lane per (cell, corner) sample: 16 lanes
+// per node, so a wave holds 4 nodes and each lane's dependency chain is one
+// sample long (the cell-per-lane tile runs four samples per lane). Same
+// semantics and sample set as tile_sweep: lane q = 4 * cell + op evaluates
+// the permuted corner op of its cell (corner 0 = the DOF); samples touching
+// the DOF give value, gradient and contracted Hessian, the fourth is frozen.
+template <int KIND, int TX, int TY, bool TAU, bool GRAD>
+__device__ void sample_sweep(const SweepArgs& a, int tile)
+{
+    constexpr int D = 2, NL = 16;
+    constexpr int NT = NL * TX * TY;
+    constexpr int LX = TX + 2, LY = TY + 2;
+    __shared__ float sh[D][LY * LX];
+    const TileDesc td = a.tiles[tile];
+    const BlockDesc bd = a.blocks[td.blk];
+    const long long PL = a.plane;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < LY * LX; e += NT) {
+        const int c = e % LX, q = e / LX;
+        const int li = min(td.i0 - 1 + q, bd.n[0]);
+        const int lj = min(td.j0 - 1 + c, bd.n[1]);
+        const long long nd = bd.base + (long long)(li + 1) * bd.st[0] + (lj + 1);
+#pragma unroll
+        for (int k = 0; k < D; ++k) sh[k][e] = a.xin[k * PL + nd];
+    }
+    __syncthreads();
+
+    const int q = tid & (NL - 1);
+    const int nid = tid / NL;
+    const int tx = nid % TX, ty = nid / TX;
+    const int lg0 = td.i0 + ty, lg1 = td.j0 + tx;
+    if (lg0 > bd.n[0] - 1 || lg1 > bd.n[1] - 1) return;
+    const long long node = bd.base + (long long)(lg0 + 1) * bd.st[0] + (lg1 + 1);
+    if (!a.mask[node]) return;  // whole lane group leaves together
+
+    const int cell = q >> 2, op = q & 3;
+    const int cb0 = (cell >> 1) & 1, cb1 = cell & 1;
+    const float rho0 = cb0 ? 1.0f : -1.0f, rho1 = cb1 ? 1.0f : -1.0f;
+    auto lds_of = [&](int o) {
+        const int o0 = ((o >> 1) & 1) ^ (1 - cb0), o1 = (o & 1) ^ (1 - cb1);
+        return (ty + cb0 + o0) * LX + (tx + cb1 + o1);
+    };
+    const int ic = lds_of(op), i0 = lds_of(op ^ 2), i1 = lds_of(op ^ 1);
+    const int inode = (ty + 1) * LX + (tx + 1);
+    float pc[D], pn0[D], pn1[D], pos[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        pc[k] = sh[k][ic];
+        pn0[k] = sh[k][i0];
+        pn1[k] = sh[k][i1];
+        pos[k] = sh[k][inode];
+    }
+    const float s0 = ((op >> 1) & 1 ? -1.0f : 1.0f) * rho0;
+    const float s1 = ((op & 1) ? -1.0f : 1.0f) * rho1;
+    const bool moves = op != 3;  // the sample has the DOF as corner or axis neighbour
+    const float delta = a.delta;
+    // A with the DOF moved to x0 (null: as is)
+    auto build = [&](const float* x0, float* A) {
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            const float c = (x0 && op == 0) ? x0[i] : pc[i];
+            const float n0 = (x0 && op == 2) ? x0[i] : pn0[i];
+            const float n1 = (x0 && op == 1) ? x0[i] : pn1[i];
+            A[i * D + 0] = s0 * (n0 - c);
+            A[i * D + 1] = s1 * (n1 - c);
+        }
+    };
+    float A[D * D];
+    build(nullptr, A);
+    float g[D] = {}, h[kNH<D>] = {}, e_l, frozen_e = 0.0f, frozen_m = kInf;
+    if (moves) {
+        float u[D];
+        u[0] = op == 0 ? -s0 : (op == 2 ? s0 : 0.0f);
+        u[1] = op == 0 ? -s1 : (op == 1 ? s1 : 0.0f);
+        e_l = sample_grad_hess<D, KIND>(A, u, delta, g, h);
+    } else {
+        e_l = sample_value<D, KIND>(A, delta);
+        frozen_e = e_l;
+        frozen_m = detD<D>(A);
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) g[k] = group_sum<NL>(g[k]);
+    float e0 = group_sum<NL>(e_l);
+    float tau[D] = {};
+    if constexpr (TAU) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            tau[k] = a.tau[k * PL + node];
+            g[k] -= tau[k];
+            e0 -= tau[k] * pos[k];
+        }
+    }
+    if constexpr (GRAD) {
+        if (q == 0)
+#pragma unroll
+            for (int k = 0; k < D; ++k) a.gout[k * PL + node] = g[k];
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < kNH<D>; ++k) h[k] = group_sum<NL>(h[k]);
+    float dl[D];
+    newton<D>(g, h, dl);
+    float alpha = 1.0f, cur[D];
+    int ntr = 10;
+#pragma unroll
+    for (int k = 0; k < D; ++k) cur[k] = pos[k];
+    for (int it = 0; it < 10; ++it) {
+        float tr[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) tr[k] = pos[k] + alpha * dl[k];
+        float et = frozen_e, mt = frozen_m;
+        if (moves) {
+            float At[D * D];
+            build(tr, At);
+            et = sample_value<D, KIND>(At, delta);
+            mt = detD<D>(At);
+        }
+        et = group_sum<NL>(et);
+        mt = group_min<NL>(mt);
+        if constexpr (TAU) {
+#pragma unroll
+            for (int k = 0; k < D; ++k) et -= tau[k] * tr[k];
+        }
+        if (accept<KIND>(et, e0, mt)) {
+#pragma unroll
+            for (int k = 0; k < D; ++k) cur[k] = tr[k];
+            ntr = it + 1;
+            break;
+        }
+        alpha *= 0.5f;
+    }
+    if (a.omega != 1.0f)
+#pragma unroll
+        for (int k = 0; k < D; ++k) cur[k] = pos[k] + a.omega * (cur[k] - pos[k]);
+    if (q == 0) {
+        store_node<D>(a, node, cur);
+        count_update(a, ntr);
+    }
+}
+
+template <int KIND, int TX, int TY, bool TAU, bool GRAD>
+__global__ __launch_bounds__(16 * TX * TY) void sample_kernel(const SweepArgs a)
+{
+    constexpr int NT = 16 * TX * TY, G = 16;
+    const int n_st = int(gridDim.x) - a.n_tiles;
+    if (int(blockIdx.x) < n_st) {
+        const int gid = int(blockIdx.x) * (NT / G) + int(threadIdx.x) / G;
+        stored_group<2, KIND, TAU, GRAD, G>(a, gid, int(threadIdx.x) % G, gid < a.st.n);
+        return;
+    }
+    sample_sweep<KIND, TX, TY, TAU, GRAD>(a, int(blockIdx.x) - n_st);
+}
+
+template <int KIND, int TX, int TY, bool TAU, bool GRAD>
+void launch_sample_variant(const SweepArgs& a, hipStream_t s)
+{
+    constexpr int NT = 16 * TX * TY;
+    const int per_wg = NT / 16;
+    const int nst = (a.st.n + per_wg - 1) / per_wg;
+    const unsigned grid = (unsigned)(a.n_tiles + nst);
+    if (grid == 0) return;
+    sample_kernel<KIND, TX, TY, TAU, GRAD><<<dim3(grid), dim3(NT), 0, s>>>(a);
+}
+
 template <int D, int KIND, int TX, int TY, int TZ, bool TAU, bool GRAD, bool PROP>
 __global__ __launch_bounds__((1 << D) * TX * TY * TZ) void sweep_kernel(const SweepArgs a)
 {

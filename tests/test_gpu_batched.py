@@ -82,8 +82,9 @@ def test_autotuner_keeps_state_and_picks_a_variant():
     assert 0 <= best < len(ms) and np.all(ms > 0)
     assert ms[best] == ms.min()
     np.testing.assert_array_equal(s.get_X(), X0)
-    # tuned session continues exactly like an untuned one
+    # the tuned session continues exactly like one set to the same variant
     r = _session(c)
+    r.core.set_tile_variant(int(best))
     r.run(3, omega=0.8)
     e1, _ = s.run(4, omega=0.8)
     e2, _ = r.run(4, omega=0.8)

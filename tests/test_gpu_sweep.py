@@ -148,18 +148,46 @@ def test_halo_exchange_bit_exact():
     assert np.array_equal(P[:, dst].view(np.uint32), P[:, src].view(np.uint32))
 
 
-def test_tile_variants_agree():
-    """Every LDS tile shape computes the same nodes the same way."""
-    c = single(n=65, seed=3)
+def test_tile_variants_agree(oracle):
+    """Every tile shape of one lane layout computes the same nodes the same
+    way (bitwise); the one-lane-per-cell (0-3) and one-lane-per-sample (4-5)
+    layouts sum a patch in different orders and both match the oracle."""
+    c = ogrid(33, 17)
     out = []
     nv = _session(c).core.set_tile_variant(-1)
+    assert nv == 6
     for v in range(nv):
         s = _session(c)
         s.core.set_tile_variant(v)
         s.run(5, omega=0.8)
         out.append(s.get_X())
-    for X in out[1:]:
+    for X in out[1:4]:
         np.testing.assert_array_equal(X, out[0])
+    np.testing.assert_array_equal(out[5], out[4])
+    Xo, _, _ = oracle.jacobi(2, "barrier", 0.0, 0.8, 5, c.X, c.patches(oracle), c.ctx.energy_stencil)
+    for X in (out[0], out[4]):
+        assert np.abs(X - Xo).max() / _extent(c) < POS_TOL
+
+
+def test_sample_variant_fas_and_untangle(oracle):
+    """The one-lane-per-sample kernels also run the untangle phase and the
+    FAS gradient / tau-shifted sweeps."""
+    c = ogrid(33, 17, tangle_seed=2)
+    s = _session(c)
+    s.core.set_tile_variant(4)
+    delta = 0.01
+    e, m = s.run(10, phase="untangle", delta=delta, omega=0.5, report_every=5)
+    Xo, eo, _ = oracle.jacobi(2, "untangle", delta, 0.5, 10, c.X, c.patches(oracle), c.ctx.energy_stencil,
+                              report_every=5)
+    assert np.abs(s.get_X() - Xo).max() / _extent(c) < POS_TOL
+    np.testing.assert_allclose(e, eo, rtol=TRAJ_RTOL)
+    u = ogrid(33, 17)
+    a = _session(u)
+    b = _session(u)
+    b.core.set_tile_variant(5)
+    ea, _ = a.run_fas(4, nu_pre=2, nu_post=2, omega=0.8)
+    eb, _ = b.run_fas(4, nu_pre=2, nu_post=2, omega=0.8)
+    np.testing.assert_allclose(eb, ea, rtol=1e-5)
 
 
 def test_analytic_minimiser():
