@@ -76,6 +76,36 @@ __device__ __forceinline__ void block_reduce_store(double e, double m, double* o
     }
 }
 
+// Cells of the structured blocks (ids in block order, prefix a.cell_off) are
+// cut into one contiguous chunk per workgroup, walked by its threads with
+// stride kReduceThreads (coalesced rows); a thread finds its first cell's
+// block once and then only steps forward as its ids grow.
+__device__ __forceinline__ void cell_chunk(long long ncells, long long& c0, long long& c1)
+{
+    const long long g = gridDim.x;
+    long long per = (ncells + g - 1) / g;
+    per = (per + kReduceThreads - 1) / kReduceThreads * kReduceThreads;
+    c0 = (long long)blockIdx.x * per;
+    c1 = c0 + per < ncells ? c0 + per : ncells;
+}
+__device__ __forceinline__ int cell_block(const long long* off, int nblocks, long long cid)
+{
+    int lo = 0, hi = nblocks - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= cid)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+__device__ __forceinline__ int cell_block_step(const long long* off, int nblocks, int lo, long long cid)
+{
+    while (lo + 1 < nblocks && off[lo + 1] <= cid) ++lo;
+    return lo;
+}
+
 template <int KIND> __device__ __forceinline__ double mu2_d(double a, double b, double c, double d, double delta)
 {
     return mu2_value<KIND, double>(a, b, c, d, delta);
@@ -89,21 +119,16 @@ template <int DIM, int KIND>
 __global__ __launch_bounds__(kReduceThreads) void reduce_cells(const CellReduceArgs a, double* partials)
 {
     double esum = 0.0, emin = __builtin_huge_val();
-    const long long stride = (long long)gridDim.x * kReduceThreads;
-    for (long long cid = (long long)blockIdx.x * kReduceThreads + threadIdx.x; cid < a.ncells; cid += stride) {
-        int lo = 0, hi = a.nblocks - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (a.cell_off[mid] <= cid)
-                lo = mid;
-            else
-                hi = mid - 1;
-        }
+    long long c0, c1;
+    cell_chunk(a.ncells, c0, c1);
+    int lo = c0 + threadIdx.x < c1 ? cell_block(a.cell_off, a.nblocks, c0 + threadIdx.x) : 0;
+    for (long long cid = c0 + threadIdx.x; cid < c1; cid += kReduceThreads) {
+        lo = cell_block_step(a.cell_off, a.nblocks, lo, cid);
         const BlockDesc bd = a.blocks[lo];
-        long long r = cid - a.cell_off[lo];
+        int r = int(cid - a.cell_off[lo]);  // cells of one block < 2^31
         if constexpr (DIM == 2) {
             const int w1 = bd.n[1] - 1;
-            const int ci = int(r / w1), cj = int(r - (long long)ci * w1);
+            const int ci = r / w1, cj = r - ci * w1;
             double px[2][2], py[2][2];
 #pragma unroll
             for (int o0 = 0; o0 < 2; ++o0)
@@ -126,9 +151,9 @@ __global__ __launch_bounds__(kReduceThreads) void reduce_cells(const CellReduceA
             }
         } else {
             const int w2 = bd.n[2] - 1, w1 = bd.n[1] - 1;
-            const int ci = int(r / ((long long)w1 * w2));
-            r -= (long long)ci * w1 * w2;
-            const int cj = int(r / w2), ck = int(r - (long long)cj * w2);
+            const int ci = r / (w1 * w2);
+            r -= ci * w1 * w2;
+            const int cj = r / w2, ck = r - cj * w2;
             double p[3][8];
 #pragma unroll
             for (int o = 0; o < 8; ++o) {
@@ -218,31 +243,26 @@ __global__ __launch_bounds__(kReduceThreads) void linesearch_pair(const CellRedu
                                                                   float a1, double* partials)
 {
     double e0 = 0.0, e1 = 0.0, m0 = __builtin_huge_val(), m1 = __builtin_huge_val();
-    const long long stride = (long long)gridDim.x * kReduceThreads;
-    for (long long cid = (long long)blockIdx.x * kReduceThreads + threadIdx.x; cid < a.ncells; cid += stride) {
-        int lo = 0, hi = a.nblocks - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (a.cell_off[mid] <= cid)
-                lo = mid;
-            else
-                hi = mid - 1;
-        }
+    long long c0, c1;
+    cell_chunk(a.ncells, c0, c1);
+    int lo = c0 + threadIdx.x < c1 ? cell_block(a.cell_off, a.nblocks, c0 + threadIdx.x) : 0;
+    for (long long cid = c0 + threadIdx.x; cid < c1; cid += kReduceThreads) {
+        lo = cell_block_step(a.cell_off, a.nblocks, lo, cid);
         const BlockDesc bd = a.blocks[lo];
-        long long r = cid - a.cell_off[lo];
+        int r = int(cid - a.cell_off[lo]);  // cells of one block < 2^31
         constexpr int NC = 1 << DIM;
         long long nodes[NC];
         if constexpr (DIM == 2) {
             const int w1 = bd.n[1] - 1;
-            const int ci = int(r / w1), cj = int(r - (long long)ci * w1);
+            const int ci = r / w1, cj = r - ci * w1;
 #pragma unroll
             for (int o = 0; o < 4; ++o)
                 nodes[o] = bd.base + (long long)(ci + (o >> 1) + 1) * bd.st[0] + (cj + (o & 1) + 1);
         } else {
             const int w2 = bd.n[2] - 1, w1 = bd.n[1] - 1;
-            const int ci = int(r / ((long long)w1 * w2));
-            r -= (long long)ci * w1 * w2;
-            const int cj = int(r / w2), ck = int(r - (long long)cj * w2);
+            const int ci = r / (w1 * w2);
+            r -= ci * w1 * w2;
+            const int cj = r / w2, ck = r - cj * w2;
 #pragma unroll
             for (int o = 0; o < 8; ++o)
                 nodes[o] = bd.base + (long long)(ci + (o >> 2) + 1) * bd.st[0] +

@@ -5,11 +5,13 @@
 // rz / pAp (stop on non-positive curvature), x += alpha p, r -= alpha Ap,
 // stop when |r|^2 <= rtol^2 |b|^2, beta = rz'/rz, p = z + beta p.
 //
-// The scalars never leave the device: each reduction ends in the last
-// workgroup to finish (a device-scope ticket), which folds the per-workgroup
-// partials in workgroup order (deterministic) and updates alpha / beta and
-// the stop flag. Three launches per iteration, no host round trip; the host
-// polls the flag every few iterations.
+// The scalars never leave the device: each reduction writes one FP64
+// partial per workgroup and a one-workgroup kernel folds them in workgroup
+// order (deterministic) and updates alpha / beta and the stop flag. A kernel
+// boundary, not an in-kernel device-scope fence, makes the partials of all
+// XCDs visible (a device-scope release per workgroup writes back its XCD's
+// L2 and costs far more than a launch). Five launches per iteration, no host
+// round trip; the host polls the flag every few iterations.
 #include "pcg.hpp"
 
 #include <cmath>
@@ -23,7 +25,6 @@ struct PcgState {
     double b2, stop2, rz, rr, pap, alpha, beta;
     int done;  // 1 converged, 2 non-positive curvature, 3 zero right-hand side
     int iters;
-    unsigned ticket;
 };
 
 __device__ __forceinline__ double wg_sum(double v)
@@ -41,31 +42,17 @@ __device__ __forceinline__ double wg_sum(double v)
     return t;  // valid in thread 0
 }
 
-// Publish this workgroup's partial(s); true in every thread of the last workgroup.
-__device__ __forceinline__ bool last_block(PcgState* st)
-{
-    __shared__ bool last;
-    __threadfence();
-    if (threadIdx.x == 0) {
-        const unsigned t = __hip_atomic_fetch_add(&st->ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    return last;
-}
-
+// fixed-order fold of n partials by one workgroup (valid in thread 0)
 __device__ __forceinline__ double fold(const double* part, int n)
 {
     double v = 0.0;
     for (int i = threadIdx.x; i < n; i += kPcgThreads) v += part[i];
-    // fixed-order fold: strided partial sums, then the workgroup reduction
     return wg_sum(v);
 }
 
 __global__ __launch_bounds__(kPcgThreads) void pcg_init(const CsrView A, const double* __restrict__ dinv,
                                                          const double* __restrict__ b, double* x, double* r,
-                                                         double* z, double* p, double* part, PcgState* st,
-                                                         double rtol2)
+                                                         double* z, double* p, double* part)
 {
     const long long i = (long long)blockIdx.x * kPcgThreads + threadIdx.x;
     double rr = 0.0, rz = 0.0;
@@ -84,9 +71,13 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_init(const CsrView A, const d
         part[blockIdx.x] = s_rr;
         part[gridDim.x + blockIdx.x] = s_rz;
     }
-    if (!last_block(st)) return;
-    const double b2 = fold(part, gridDim.x);
-    const double rz0 = fold(part + gridDim.x, gridDim.x);
+}
+
+__global__ __launch_bounds__(kPcgThreads) void pcg_init_finish(const double* part, int nparts, PcgState* st,
+                                                                double rtol2)
+{
+    const double b2 = fold(part, nparts);
+    const double rz0 = fold(part + nparts, nparts);
     if (threadIdx.x == 0) {
         st->b2 = b2;
         st->rr = b2;
@@ -94,12 +85,11 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_init(const CsrView A, const d
         st->stop2 = rtol2 * b2;
         st->done = (b2 == 0.0) ? 3 : 0;
         st->iters = 0;
-        st->ticket = 0;
     }
 }
 
 __global__ __launch_bounds__(kPcgThreads) void pcg_matvec(const CsrView A, const double* __restrict__ p, double* ap,
-                                                          double* part, PcgState* st)
+                                                          double* part, const PcgState* st)
 {
     if (st->done) return;
     const long long i = (long long)blockIdx.x * kPcgThreads + threadIdx.x;
@@ -112,21 +102,24 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_matvec(const CsrView A, const
     }
     const double s = wg_sum(d);
     if (threadIdx.x == 0) part[blockIdx.x] = s;
-    if (!last_block(st)) return;
-    const double pap = fold(part, gridDim.x);
+}
+
+__global__ __launch_bounds__(kPcgThreads) void pcg_alpha(const double* part, int nparts, PcgState* st)
+{
+    if (st->done) return;
+    const double pap = fold(part, nparts);
     if (threadIdx.x == 0) {
         st->pap = pap;
         if (!(pap > 0.0))
             st->done = 2;
         else
             st->alpha = st->rz / pap;
-        st->ticket = 0;
     }
 }
 
 __global__ __launch_bounds__(kPcgThreads) void pcg_update(const CsrView A, const double* __restrict__ dinv,
                                                           double* x, double* r, double* z, const double* p,
-                                                          const double* ap, double* part, PcgState* st)
+                                                          const double* ap, double* part, const PcgState* st)
 {
     if (st->done) return;
     const double a = st->alpha;
@@ -146,9 +139,13 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_update(const CsrView A, const
         part[blockIdx.x] = s_rr;
         part[gridDim.x + blockIdx.x] = s_rz;
     }
-    if (!last_block(st)) return;
-    const double rr_all = fold(part, gridDim.x);
-    const double rz_new = fold(part + gridDim.x, gridDim.x);
+}
+
+__global__ __launch_bounds__(kPcgThreads) void pcg_beta(const double* part, int nparts, PcgState* st)
+{
+    if (st->done) return;
+    const double rr_all = fold(part, nparts);
+    const double rz_new = fold(part + nparts, nparts);
     if (threadIdx.x == 0) {
         st->rr = rr_all;
         st->iters += 1;
@@ -158,7 +155,6 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_update(const CsrView A, const
             st->beta = rz_new / st->rz;
             st->rz = rz_new;
         }
-        st->ticket = 0;
     }
 }
 
@@ -185,15 +181,18 @@ PcgResult pcg_solve_device(const CsrView& A, const double* b, double* x, double 
                            PcgWork& w, hipStream_t s)
 {
     const unsigned grid = (unsigned)((A.n + kPcgThreads - 1) / kPcgThreads);
+    const int np = (int)grid;
     jacobi_diag<<<grid, kPcgThreads, 0, s>>>(A, w.dinv);
     PcgState* st = reinterpret_cast<PcgState*>(w.state);
-    hip_check(hipMemsetAsync(st, 0, sizeof(PcgState), s), "pcg state");
-    pcg_init<<<grid, kPcgThreads, 0, s>>>(A, w.dinv, b, x, w.r, w.z, w.p, w.part, st, rtol * rtol);
+    pcg_init<<<grid, kPcgThreads, 0, s>>>(A, w.dinv, b, x, w.r, w.z, w.p, w.part);
+    pcg_init_finish<<<1, kPcgThreads, 0, s>>>(w.part, np, st, rtol * rtol);
     PcgState h {};
     constexpr int kPoll = 16;
     for (int it = 0; it < max_iter; ++it) {
         pcg_matvec<<<grid, kPcgThreads, 0, s>>>(A, w.p, w.ap, w.part, st);
+        pcg_alpha<<<1, kPcgThreads, 0, s>>>(w.part, np, st);
         pcg_update<<<grid, kPcgThreads, 0, s>>>(A, w.dinv, x, w.r, w.z, w.p, w.ap, w.part, st);
+        pcg_beta<<<1, kPcgThreads, 0, s>>>(w.part, np, st);
         pcg_direction<<<grid, kPcgThreads, 0, s>>>(A.n, w.z, w.p, st);
         if ((it + 1) % kPoll == 0 || it + 1 == max_iter) {
             hip_check(hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s), "pcg poll");

@@ -155,6 +155,8 @@ void Level::upload_tables()
         for (int k = 0; k < lay.dim; ++k) c *= std::max(0, lay.shape[b][k] - 1);
         cell_off[b + 1] = cell_off[b] + c;
     }
+    for (size_t b = 0; b + 1 < cell_off.size(); ++b)
+        if (cell_off[b + 1] - cell_off[b] >= (1ll << 31)) throw InvalidArgument("a block has 2^31 or more cells");
     ncells = cell_off.back();
     d_cell_off.upload(cell_off);
 }

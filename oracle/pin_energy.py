@@ -11,6 +11,12 @@ of ``--window`` sweeps. E* is the energy at that point; the fixture also
 records how it was reached.
 
   python -m oracle.pin_energy --workload ogrid129 [--threads 8]
+
+For the 129-node O-grid the plain method needs ~10^5 sweeps, hours of FP64
+CPU time from the untangled state; the pinned fixture was made from a warm
+start instead (same basin, the FP64 sweeps then settle the last digits):
+  python -m oracle.pin_energy --workload ogrid129 --gpu-warm-start ws.npy   (GPU)
+  python -m oracle.pin_energy --workload ogrid129 --start ws.npy --window 5000
 """
 
 from __future__ import annotations
@@ -49,7 +55,24 @@ def main():
     ap.add_argument("--start", default=None,
                     help=".npy of a near-converged state (global order) to start the FP64 sweeps from, "
                          "instead of untangling from the generated grid")
+    ap.add_argument("--gpu-warm-start", default=None, metavar="NPY",
+                    help="(on a GPU machine) write a warm start to NPY and stop: the device untangle, "
+                         "300 FAS V(4,4) cycles and 20000 plain sweeps at omega 0.8; then rerun with --start NPY")
     args = ap.parse_args()
+    if args.gpu_warm_start:
+        from egg_amd.smoothing import (CppStructuredSweepSession, build_structured_context_from_block_maps,
+                                       cpp_untangle)
+
+        g, _ = workload(args.workload)
+        ctx = build_sweep_context(g.blocks, g.free, g.d)
+        bsc = build_structured_context_from_block_maps(g.d, g.blocks)
+        s = CppStructuredSweepSession(ctx, bsc, g.X)
+        cpp_untangle(ctx, bsc, g.X, session=s)
+        s.run_fas(300, nu_pre=4, nu_post=4, omega=0.8)
+        s.run(20000, omega=0.8, report_every=20000)
+        np.save(args.gpu_warm_start, s.get_X())
+        print("wrote", args.gpu_warm_start)
+        return
 
     g, seed = workload(args.workload)
     d = g.d
