@@ -51,6 +51,24 @@ PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector peak
 PEAK_HBM_GBS = 8000.0
 SEED = 2026
 CONV_RTOL = 1e-6
+# HBM bytes per sweep launch by workload and tile variant, from two rocprofv3
+# --pmc runs of this file (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch,
+# median; profiles/sweep2d_ogrid129.md, batched64_fas.md,
+# cubed_sphere65_fas_pcg.md). A device copy of N bytes reads FETCH_SIZE =
+# N/2 and WRITE_SIZE = N on this part, so FETCH is scaled by 2 and WRITE by 1.
+PMC_KIB = {
+    ("ogrid", 0): (1318.75, 548.44), ("ogrid", 1): (1472.38, 616.38), ("ogrid", 2): (1224.0, 517.69),
+    ("ogrid", 3): (1067.06, 538.5),
+    ("batched", 0): (591180.7, 277104.7), ("batched", 1): (762148.9, 325449.7),
+    ("batched", 2): (526885.0, 252865.4), ("batched", 3): (508806.1, 276764.8),
+    ("cubed_sphere", 0): (137175.0, 49892.4),
+}
+FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
+
+
+def _traffic(workload, variant):
+    kib = PMC_KIB.get((workload, int(variant)))
+    return None if kib is None else (kib[0] * FETCH_SCALE + kib[1] * WRITE_SCALE) * 1024.0
 METRIC = "node-updates/s; time-to-untangled+converged, plain vs MLP-proposal, 1 GPU"
 
 
@@ -74,7 +92,7 @@ def _build(g):
     return ctx, bsc, CppStructuredSweepSession(ctx, bsc, g.X)
 
 
-def _roofline(s, dim, n_free, steps, omega, k_avg):
+def _roofline(s, dim, n_free, steps, omega, k_avg, traffic=None):
     """Roofline of the sweep kernel: k_avg = its average launch time (HIP
     events on the session stream around the timed region, which holds only
     back-to-back sweep launches); flops per launch from the op-counted
@@ -92,7 +110,7 @@ def _roofline(s, dim, n_free, steps, omega, k_avg):
     flops = n_free * (F_METRIC[dim] + tpu * F_TRIAL[dim])
     ach = flops / k_avg / 1e12
     return {"bound": "valu", "achieved": ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-            "frac": ach / PEAK_FP32_TFLOPS, "traffic": None, "kernel": f"sweep_kernel<{dim}D, shape>",
+            "frac": ach / PEAK_FP32_TFLOPS, "traffic": traffic, "kernel": f"sweep_kernel<{dim}D, shape>",
             "kernel_avg_us": k_avg * 1e6, "flops_per_launch": flops, "trials_per_update": tpu,
             "flops_per_update": f"{F_METRIC[dim]} + trials x {F_TRIAL[dim]}",
             "algorithmic_bytes_per_launch": n_free * 4 * dim * 2}
@@ -164,7 +182,7 @@ def bench_ogrid(args):
     value = n_free * args.steps / t_run
     X_end = s.get_X()
     s.set_X(X_start)
-    roof = _roofline(s, 2, n_free, args.steps, 0.8, k_avg)
+    roof = _roofline(s, 2, n_free, args.steps, 0.8, k_avg, _traffic("ogrid", tune_best))
     s.set_X(X_end)
     log(f"timed {args.steps} sweeps {t_run * 1e3:.2f} ms -> {value:.4g} node-updates/s  E={e[-1]:.9g}")
     log(f"sweep kernel {roof['kernel_avg_us']:.2f} us/launch, {roof['trials_per_update']:.3f} trials/update "
@@ -277,7 +295,7 @@ def bench_batched(args):
     log(f"autotune {time.perf_counter() - t0:.2f}s best={best} ms/5 sweeps={ms}")
     _, _, t_plain, k_avg, X_start = _timed(s, 20, 0.8)
     s.set_X(X_start)
-    roof = _roofline(s, 2, n_free, 20, 0.8, k_avg)
+    roof = _roofline(s, 2, n_free, 20, 0.8, k_avg, _traffic("batched", best))
     counts = _fas_levels_counts(s, 4, 4, 3)
     K = max(1, args.steps // 100)
     t0 = time.perf_counter()
@@ -323,7 +341,7 @@ def bench_cubed(args):
     value = n_free * n3 / t_plain
     X_end = s.get_X()
     s.set_X(X_start)
-    roof = _roofline(s, 3, n_free, n3, 0.8, k_avg)
+    roof = _roofline(s, 3, n_free, n3, 0.8, k_avg, _traffic("cubed_sphere", 0))
     s.set_X(X_end)
     counts = _fas_levels_counts(s, 4, 4, 3)
     K = 5
