@@ -13,11 +13,13 @@ void launch_sweep3d(int kind, int v, const SweepArgs& a, hipStream_t s);
 void launch_grad3d(int kind, int v, const SweepArgs& a, hipStream_t s);
 
 // node tiles (tx = last axis); every node is worked on by 2^D lanes
-static constexpr TileShape kTiles2[] = {{32, 2, 1}, {16, 4, 1}, {64, 1, 1}, {32, 4, 1}, {16, 1, 1}, {8, 2, 1}};
-static constexpr TileShape kTiles3[] = {{8, 4, 2}, {4, 4, 4}};
+static constexpr TileShape kTiles2[] = {{32, 2, 1}, {16, 4, 1}, {64, 1, 1}, {32, 4, 1},
+                                        {16, 1, 1}, {8, 2, 1},  // one lane per sample
+                                        {64, 1, 1}, {32, 2, 1}};  // no LDS staging
+static constexpr TileShape kTiles3[] = {{8, 4, 2}, {4, 4, 4}, {8, 4, 2} /* no LDS staging */};
 
-int num_tile_variants(int dim) { return dim == 2 ? 6 : 2; }
-bool tile_variant_has_proposal(int dim, int v) { return dim == 3 || v < 4; }
+int num_tile_variants(int dim) { return dim == 2 ? 8 : 3; }
+bool tile_variant_has_proposal(int dim, int v) { return dim == 2 ? v < 4 : v < 2; }
 TileShape tile_variant_shape(int dim, int v) { return dim == 2 ? kTiles2[v] : kTiles3[v]; }
 
 void launch_sweep(int dim, int kind, int v, const SweepArgs& a, hipStream_t s)

@@ -64,16 +64,20 @@ __host__ __device__ inline int chain_feature(int slot)
     return c + (i < 4 ? 4 * g + i : 16 + 4 * g + (i - 4));
 }
 
+// tanh(u) = 1 - 2 / (1 + e^{2u}) with the hardware reciprocal (saturates
+// correctly: e^{2u} = inf gives 1, e^{2u} = 0 gives -1)
+__device__ __forceinline__ float tanh_fast(float u)
+{
+    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + __expf(2.0f * u));
+}
 __device__ __forceinline__ float gelu(float x)
 {
-    const float u = 0.7978845608f * (x + 0.044715f * x * x * x);
-    const float t = 1.0f - 2.0f / (1.0f + __expf(2.0f * u));
+    const float t = tanh_fast(0.7978845608f * (x + 0.044715f * x * x * x));
     return 0.5f * x * (1.0f + t);
 }
 __device__ __forceinline__ float gelu_grad(float x)
 {
-    const float u = 0.7978845608f * (x + 0.044715f * x * x * x);
-    const float t = 1.0f - 2.0f / (1.0f + __expf(2.0f * u));
+    const float t = tanh_fast(0.7978845608f * (x + 0.044715f * x * x * x));
     return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * 0.7978845608f * (1.0f + 0.134145f * x * x);
 }
 
@@ -235,6 +239,41 @@ __device__ __forceinline__ float feature(const Ring& ring, const Frame& f, const
         float s = 0.f;
 #pragma unroll
         for (int k = 0; k < D; ++k) s += dprev[k] * f.e[j][k];
+        return s * rh;
+    }
+    return 0.f;
+}
+
+// The same feature for a run-time index j, without indexing a register
+// array (frame rows are selected, ring values come from the accessor), so a
+// lane can compute just its own slice of the features; rh = 1/h (0 if the
+// frame is degenerate).
+template <int D, class Ring>
+__device__ __forceinline__ float feature_rt(const Ring& ring, const Frame& f, const float* dprev, float rh, int j)
+{
+    constexpr int NR = D == 2 ? 9 : 27;
+    constexpr int C = D == 2 ? 4 : 13;
+    constexpr int NCONST = D == 2 ? 2 : 1;
+    auto row = [&](int c, int k) {
+        float v = f.e[0][k];
+        if (c == 1) v = f.e[1][k];
+        if (D == 3 && c == 2) v = f.e[2][k];
+        return v;
+    };
+    if (j < NR * D) {
+        const int a = j / D, c = j - (j / D) * D;
+        float s = 0.f;
+#pragma unroll
+        for (int k = 0; k < D; ++k) s += (ring(a, k) - ring(C, k)) * row(c, k);
+        return s * rh;
+    }
+    j -= NR * D;
+    if (j < NCONST) return j == 0 ? 1.0f : 0.0f;
+    j -= NCONST;
+    if (j < D) {
+        float s = 0.f;
+#pragma unroll
+        for (int k = 0; k < D; ++k) s += dprev[k] * row(j, k);
         return s * rh;
     }
     return 0.f;

@@ -1,7 +1,8 @@
 // 2D instantiations of the structured sweep kernel (sweep_impl.hpp): quad
 // patches of 4 cells x 4 corner samples, shape / untangle / shape_size
 // metrics; tile shapes in nodes (last axis x first axis). Variants 0-3: one
-// lane per cell (4 per node); 4-5: one lane per sample (16 per node).
+// lane per cell (4 per node); 4-5: one lane per sample (16 per node); 6-7:
+// one lane per cell without LDS staging.
 #include "sweep_impl.hpp"
 
 #include <stdexcept>
@@ -12,9 +13,11 @@ namespace eggmi
 
 template <int KIND, bool TAU, bool GRAD, bool PROP = false> static void sweep2d_v(int v, const SweepArgs& a, hipStream_t s)
 {
-    if constexpr (!PROP) {  // one lane per sample (16 per node); no proposal variant
-        if (v == 4) return launch_sample_variant<KIND, 16, 1, TAU, GRAD>(a, s);
+    if constexpr (!PROP) {  // no proposal kernels for these
+        if (v == 4) return launch_sample_variant<KIND, 16, 1, TAU, GRAD>(a, s);  // one lane per sample
         if (v == 5) return launch_sample_variant<KIND, 8, 2, TAU, GRAD>(a, s);
+        if (v == 6) return launch_variant<2, KIND, 64, 1, 1, TAU, GRAD, false, false>(a, s);  // no LDS staging
+        if (v == 7) return launch_variant<2, KIND, 32, 2, 1, TAU, GRAD, false, false>(a, s);
     }
     switch (v) {
     case 0: launch_variant<2, KIND, 32, 2, 1, TAU, GRAD, PROP>(a, s); break;

@@ -8,6 +8,9 @@ namespace eggmi
 
 template <int KIND, bool TAU, bool GRAD, bool PROP = false> static void sweep3d_v(int v, const SweepArgs& a, hipStream_t s)
 {
+    if constexpr (!PROP) {
+        if (v == 2) return launch_variant<3, KIND, 8, 4, 2, TAU, GRAD, false, false>(a, s);  // no LDS staging
+    }
     switch (v) {
     case 0: launch_variant<3, KIND, 8, 4, 2, TAU, GRAD, PROP>(a, s); break;
     default: launch_variant<3, KIND, 4, 4, 4, TAU, GRAD, PROP>(a, s); break;

@@ -278,19 +278,24 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
 // ------------------------------------------------------------------ tiles
 
 // Tile of TX (last axis) x TY x TZ nodes; 2^D lanes per node.
-template <int D, int KIND, int TX, int TY, int TZ, bool TAU, bool GRAD, bool PROP>
+// LDS = false: no staging; each lane reads its cell's corners straight from
+// the store (cached), with no workgroup barrier (an autotuner candidate for
+// latency-bound grids).
+template <int D, int KIND, int TX, int TY, int TZ, bool TAU, bool GRAD, bool PROP, bool LDS>
 __device__ void tile_sweep(const SweepArgs& a, int tile)
 {
+    static_assert(LDS || !PROP, "the proposal reads the one-ring from LDS");
     constexpr int NC = 1 << D;
     constexpr int NT = NC * TX * TY * TZ;
     constexpr int LX = TX + 2, LY = TY + 2, LZ = (D == 3) ? TZ + 2 : 1;
-    __shared__ float sh[D][LZ * LY * LX];
+    __shared__ float sh[D][LDS ? LZ * LY * LX : 1];
     const TileDesc td = a.tiles[tile];
     const BlockDesc bd = a.blocks[td.blk];
     const long long PL = a.plane;
     const int tid = threadIdx.x;
 
     // ---- stage the tile + one-node halo (logical -1 .. n: padded ghosts included)
+    if constexpr (LDS)
     for (int e = tid; e < LZ * LY * LX; e += NT) {
         const int c = e % LX, q = (e / LX) % LY, r = e / (LX * LY);
         long long node;
@@ -307,7 +312,7 @@ __device__ void tile_sweep(const SweepArgs& a, int tile)
 #pragma unroll
         for (int k = 0; k < D; ++k) sh[k][e] = a.xin[k * PL + node];
     }
-    __syncthreads();
+    if constexpr (LDS) __syncthreads();
 
     const int cell = tid & (NC - 1);
     const int nid = tid / NC;
@@ -353,13 +358,21 @@ __device__ void tile_sweep(const SweepArgs& a, int tile)
             const int o = obit ^ (1 - cb[k]);  // actual corner bit
             loc[k] = cb[k] + o;
         }
-        int li;
-        if constexpr (D == 2)
-            li = (ty + loc[0]) * LX + (tx + loc[1]);
-        else
-            li = ((tz + loc[0]) * LY + (ty + loc[1])) * LX + (tx + loc[2]);
+        if constexpr (LDS) {
+            int li;
+            if constexpr (D == 2)
+                li = (ty + loc[0]) * LX + (tx + loc[1]);
+            else
+                li = ((tz + loc[0]) * LY + (ty + loc[1])) * LX + (tx + loc[2]);
 #pragma unroll
-        for (int k = 0; k < D; ++k) P[op][k] = sh[k][li];
+            for (int k = 0; k < D; ++k) P[op][k] = sh[k][li];
+        } else {
+            long long g = node;
+#pragma unroll
+            for (int k = 0; k < D; ++k) g += (long long)(loc[k] - 1) * (k == D - 1 ? 1 : bd.st[k]);
+#pragma unroll
+            for (int k = 0; k < D; ++k) P[op][k] = a.xin[k * PL + g];
+        }
     }
 
     const float delta = a.delta;
@@ -399,11 +412,13 @@ __device__ void tile_sweep(const SweepArgs& a, int tile)
         float dprev[D];
 #pragma unroll
         for (int k = 0; k < D; ++k) dprev[k] = P[0][k] - a.xout[k * PL + node];
-        // compile-time feature index (no dynamic register indexing); each
-        // lane of the group stores its PER_LANE-wide slice
+        // each lane of the group computes and stores its PER_LANE-wide slice
+        const float rh = fr.ok ? __builtin_amdgcn_rcpf(fr.h) : 0.0f;
 #pragma unroll
-        for (int j = 0; j < ML::INP; ++j)
-            if (j / PER_LANE == cell) feat[nid][j] = (__bf16)feature<D>(ring, fr, dprev, j);
+        for (int q = 0; q < PER_LANE; ++q) {
+            const int j = cell * PER_LANE + q;
+            feat[nid][j] = (__bf16)feature_rt<D>(ring, fr, dprev, rh, j);
+        }
         __syncthreads();
         constexpr int WN = 64 / NC;  // nodes per wave
         const int lane = tid & 63, wave_node0 = (tid >> 6) * WN;
@@ -717,7 +732,7 @@ void launch_sample_variant(const SweepArgs& a, hipStream_t s)
     sample_kernel<KIND, TX, TY, TAU, GRAD><<<dim3(grid), dim3(NT), 0, s>>>(a);
 }
 
-template <int D, int KIND, int TX, int TY, int TZ, bool TAU, bool GRAD, bool PROP>
+template <int D, int KIND, int TX, int TY, int TZ, bool TAU, bool GRAD, bool PROP, bool LDS>
 __global__ __launch_bounds__((1 << D) * TX * TY * TZ) void sweep_kernel(const SweepArgs a)
 {
     constexpr int NT = (1 << D) * TX * TY * TZ;
@@ -730,10 +745,10 @@ __global__ __launch_bounds__((1 << D) * TX * TY * TZ) void sweep_kernel(const Sw
         stored_group<D, KIND, TAU, GRAD, G>(a, gid, int(threadIdx.x) % G, gid < a.st.n);
         return;
     }
-    tile_sweep<D, KIND, TX, TY, TZ, TAU, GRAD, PROP>(a, int(blockIdx.x) - n_st);
+    tile_sweep<D, KIND, TX, TY, TZ, TAU, GRAD, PROP, LDS>(a, int(blockIdx.x) - n_st);
 }
 
-template <int D, int KIND, int TX, int TY, int TZ, bool TAU, bool GRAD, bool PROP = false>
+template <int D, int KIND, int TX, int TY, int TZ, bool TAU, bool GRAD, bool PROP = false, bool LDS = true>
 void launch_variant(const SweepArgs& a, hipStream_t s)
 {
     constexpr int NT = (1 << D) * TX * TY * TZ;
@@ -742,7 +757,7 @@ void launch_variant(const SweepArgs& a, hipStream_t s)
     const int nst = (a.st.n + per_wg - 1) / per_wg;
     const unsigned grid = (unsigned)(a.n_tiles + nst);
     if (grid == 0) return;
-    sweep_kernel<D, KIND, TX, TY, TZ, TAU, GRAD, PROP><<<dim3(grid), dim3(NT), 0, s>>>(a);
+    sweep_kernel<D, KIND, TX, TY, TZ, TAU, GRAD, PROP, LDS><<<dim3(grid), dim3(NT), 0, s>>>(a);
 }
 
 }  // namespace eggmi

@@ -143,8 +143,10 @@ eggmi_status eggmi_halo_exchange(eggmi_session* s);
 /* Select the sweep tile shape (autotuner hook); returns the variant count
  * in *n_variants when variant < 0. 2D: variants 0-3 put one lane on each of
  * a node's 4 cells (tiles of 32x2, 16x4, 64x1, 32x4 nodes), 4-5 one lane on
- * each of its 16 samples (16x1, 8x2 nodes; no learned-proposal kernel). 3D:
- * one lane per cell, tiles of 8x4x2 and 4x4x4 nodes. */
+ * each of its 16 samples (16x1, 8x2 nodes), 6-7 one lane per cell without
+ * LDS staging (64x1, 32x2). 3D: one lane per cell, tiles of 8x4x2 and 4x4x4
+ * nodes, 2 = 8x4x2 without LDS staging. Only variants 0-3 (2D) and 0-1 (3D)
+ * have a learned-proposal kernel. */
 eggmi_status eggmi_set_tile_variant(eggmi_session* s, int variant, int* n_variants);
 
 /* Energy and min det A of the resident state (the report reduction of

@@ -150,19 +150,20 @@ def test_halo_exchange_bit_exact():
 
 def test_tile_variants_agree(oracle):
     """Every tile shape of one lane layout computes the same nodes the same
-    way (bitwise); the one-lane-per-cell (0-3) and one-lane-per-sample (4-5)
-    layouts sum a patch in different orders and both match the oracle."""
+    way (bitwise); the one-lane-per-cell (0-3, and 6-7 without LDS staging)
+    and one-lane-per-sample (4-5) layouts sum a patch in different orders and
+    both match the oracle."""
     c = ogrid(33, 17)
     out = []
     nv = _session(c).core.set_tile_variant(-1)
-    assert nv == 6
+    assert nv == 8
     for v in range(nv):
         s = _session(c)
         s.core.set_tile_variant(v)
         s.run(5, omega=0.8)
         out.append(s.get_X())
-    for X in out[1:4]:
-        np.testing.assert_array_equal(X, out[0])
+    for v in (1, 2, 3, 6, 7):  # one lane per cell, with or without LDS staging
+        np.testing.assert_array_equal(out[v], out[0])
     np.testing.assert_array_equal(out[5], out[4])
     Xo, _, _ = oracle.jacobi(2, "barrier", 0.0, 0.8, 5, c.X, c.patches(oracle), c.ctx.energy_stencil)
     for X in (out[0], out[4]):
@@ -251,3 +252,21 @@ def test_set_get_x_roundtrip():
     X2 = c.X + 1e-3
     s.set_X(X2)
     np.testing.assert_allclose(s.get_X(), X2.astype(np.float32), rtol=0, atol=0)
+
+
+def test_3d_tile_variants_agree():
+    """3D tile shapes (8x4x2, 4x4x4, and 8x4x2 without LDS staging) are
+    bitwise identical on the cubed sphere (stored DOFs included)."""
+    from egg_amd import grids
+
+    c = Case(grids.cubed_sphere_3d(9))
+    nv = _session(c).core.set_tile_variant(-1)
+    assert nv == 3
+    out = []
+    for v in range(nv):
+        s = _session(c)
+        s.core.set_tile_variant(v)
+        s.run(3, omega=0.8)
+        out.append(s.get_X())
+    for X in out[1:]:
+        np.testing.assert_array_equal(X, out[0])
