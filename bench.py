@@ -204,13 +204,18 @@ def bench_ogrid(args):
         ttc["plain"] = {"sweeps": n, "s": t + t_untangle if n else None, "E_end": E}
         log(f"plain: {n} sweeps, {t:.3f}s to converge")
 
-        s.set_X(Xu)
-        def fas_step(chunk=5):
-            e, m = s.run_fas(chunk, nu_pre=4, nu_post=4, omega=0.8)
-            return e, m, chunk
-        n, t, E = _converge(fas_step, e_star, 60.0)
-        ttc["fas_v44"] = {"cycles": n, "s": t + t_untangle if n else None, "E_end": E}
-        log(f"FAS V(4,4): {n} cycles, {t:.3f}s to converge")
+        # FAS: V(4,4) on 3 levels (the batched workload's cycle), on every
+        # level, and V(8,8) on 4 levels (the fastest found on this grid)
+        for name, nu, lev in (("fas_v44_3_levels", 4, 3), ("fas_v44_all_levels", 4, 32), ("fas_v88_4_levels", 8, 4)):
+            s.set_X(Xu)
+
+            def fas_step(chunk=5, nu=nu, lev=lev):
+                e, m = s.run_fas(chunk, nu_pre=nu, nu_post=nu, omega=0.8, max_levels=lev)
+                return e, m, chunk
+
+            n, t, E = _converge(fas_step, e_star, 60.0)
+            ttc[name] = {"cycles": n, "s": t + t_untangle if n else None, "E_end": E}
+            log(f"FAS {name}: {n} cycles, {t:.3f}s to converge")
 
         net, info = _train_proposal(make, (11, 12, 13))
         s.set_X(Xu)

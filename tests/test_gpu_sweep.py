@@ -246,6 +246,21 @@ def test_3d_block_parity(oracle):
     np.testing.assert_allclose(e, eo, rtol=TRAJ_RTOL)
 
 
+@pytest.mark.parametrize("phase,delta", [("shape_size", 0.0), ("untangle", 0.05)])
+def test_3d_phases_parity(oracle, phase, delta):
+    """3D shape_size (cond3 + size term) and untangle (delta surrogate)
+    sweeps against the FP64 oracle, including stored DOFs of the cubed
+    sphere for the untangle phase."""
+    from egg_amd import grids
+
+    c = cube(n=9, seed=3) if phase == "shape_size" else Case(grids.tangle(grids.cubed_sphere_3d(9), frac=0.01,
+                                                                           factor=1.5, seed=5))
+    _, Xg, e, m, Xo, eo, mo = _run_both(oracle, c, 6, phase=phase, delta=delta,
+                                        omega=0.8 if phase == "shape_size" else 0.5)
+    assert np.abs(Xg - Xo).max() / _extent(c) < POS_TOL
+    np.testing.assert_allclose(e, eo, rtol=TRAJ_RTOL)
+
+
 def test_set_get_x_roundtrip():
     c = ogrid(17, 9)
     s = _session(c)
