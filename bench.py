@@ -51,6 +51,7 @@ PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector peak
 PEAK_HBM_GBS = 8000.0
 SEED = 2026
 CONV_RTOL = 1e-6
+GRAPH_CHUNK = 256  # sweeps per captured hipGraph (5-8 % per sweep on the small grids)
 # HBM bytes per sweep launch by workload and tile variant, from two rocprofv3
 # --pmc runs of this file (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch,
 # median; profiles/sweep2d_ogrid129.md, batched64_fas.md,
@@ -177,6 +178,7 @@ def bench_ogrid(args):
 
     tune_best, tune_ms = s.core.autotune(20, 0.8)
     log(f"autotune: variant {tune_best}, ms per 20 sweeps {np.round(tune_ms, 4).tolist()}")
+    s.core.set_graph_chunk(GRAPH_CHUNK)  # replay sweeps as captured hipGraph chunks
     s.run(args.warmup, omega=0.8)
     e, m, t_run, k_avg, X_start = _timed(s, args.steps, 0.8)
     value = n_free * args.steps / t_run
@@ -263,6 +265,7 @@ def bench_ogrid(args):
                                "2% tangled by 1.5h, untangle then block-Jacobi TMOP sweeps, omega 0.8",
                    "free_dofs": n_free, "seed": SEED, "precision": "FP32 sweeps, FP64 reductions"},
         "autotune": {"best_variant": int(tune_best), "ms_per_20_sweeps": [float(v) for v in tune_ms]},
+        "graph_chunk": GRAPH_CHUNK,
         "time_to_untangled_s": t_untangle, "untangle_outer": outer,
         "time_to_untangled_converged": ttc,
         "final_energy": float(e[-1]), "final_min_det": float(m[-1]),

@@ -460,6 +460,7 @@ class Session
 
     void set_kernel_timing(bool on) { check(eggmi_set_kernel_timing(s_, on ? 1 : 0)); }
     void set_counters(bool on) { check(eggmi_set_counters(s_, on ? 1 : 0)); }
+    void set_graph_chunk(int chunk) { check(eggmi_set_graph_chunk(s_, chunk)); }
     py::array_t<int64_t> last_counters(int n_levels)
     {
         py::array_t<int64_t> out({n_levels, 3});
@@ -552,6 +553,8 @@ PYBIND11_MODULE(cpp_core, m)
       .def("last_timing", &Session::last_timing)
       .def("set_kernel_timing", &Session::set_kernel_timing, py::arg("on"))
       .def("set_counters", &Session::set_counters, py::arg("on"))
+      .def("set_graph_chunk", &Session::set_graph_chunk, py::arg("chunk"),
+           "Replay plain sweeps as captured hipGraph chunks of `chunk` sweeps (even; 0 = off).")
       .def("last_counters", &Session::last_counters, py::arg("n_levels") = 1,
            "Counters of the last run (counting on): rows per level of (line-search trials, DOF updates, "
            "accepted learned proposals).")

@@ -275,6 +275,17 @@ eggmi_status eggmi_set_kernel_timing(eggmi_session* s, int on)
     });
 }
 
+eggmi_status eggmi_set_graph_chunk(eggmi_session* s, int chunk)
+{
+    return guarded([&] {
+        need(s, "session");
+        s->s.need_device();
+        if (chunk < 0 || chunk % 2 != 0) throw eggmi::InvalidArgument("graph chunk must be an even number >= 0");
+        s->s.release_graphs();
+        s->s.graph_chunk = chunk;
+    });
+}
+
 eggmi_status eggmi_set_counters(eggmi_session* s, int on)
 {
     return guarded([&] {

@@ -26,9 +26,12 @@ struct BlockDesc {
     long long st[3];  // node strides of the padded axes (last axis = 1)
 };
 
-struct TileDesc {
-    int blk;
+// A tile carries its block's geometry (one 64-byte load instead of two
+// dependent ones at the head of the latency-bound small-grid launches).
+struct __align__(16) TileDesc {
+    BlockDesc b;
     int i0, j0, k0;  // first logical index of the tile on each axis
+    int blk;
 };
 
 // Stored-stencil DOFs (block faces, interfaces, singular corners): one

@@ -88,7 +88,8 @@ void Level::make_tiles(int variant)
         for (int i = lo[0]; i <= hi[0]; i += step[0])
             for (int j = lo[1]; j <= hi[1]; j += step[1])
                 for (int k = lo[2]; k <= hi[2]; k += step[2]) {
-                    TileDesc t;
+                    TileDesc t {};
+                    t.b = block_desc(b);
                     t.blk = (int)b;
                     t.i0 = i;
                     t.j0 = j;
@@ -125,17 +126,22 @@ void Level::build_push()
     n_push = (long long)pushes.size();
 }
 
+BlockDesc Level::block_desc(size_t b) const
+{
+    BlockDesc d {};
+    d.base = lay.base[b];
+    for (int k = 0; k < 3; ++k) {
+        d.n[k] = k < lay.dim ? lay.shape[b][k] : 1;
+        d.st[k] = k < lay.dim ? lay.stride[b][k] : 0;
+    }
+    d.pad = 0;
+    return d;
+}
+
 void Level::upload_tables()
 {
     std::vector<BlockDesc> bd(lay.shape.size());
-    for (size_t b = 0; b < bd.size(); ++b) {
-        bd[b].base = lay.base[b];
-        for (int k = 0; k < 3; ++k) {
-            bd[b].n[k] = k < lay.dim ? lay.shape[b][k] : 1;
-            bd[b].st[k] = k < lay.dim ? lay.stride[b][k] : 0;
-        }
-        bd[b].pad = 0;
-    }
+    for (size_t b = 0; b < bd.size(); ++b) bd[b] = block_desc(b);
     blocks.upload(bd);
     d_copy_src.upload(copy_src);
     d_copy_dst.upload(copy_dst);
@@ -266,6 +272,7 @@ void Session::ensure_timing_events(size_t n)
 
 Session::~Session()
 {
+    release_graphs();
     for (auto e : tev) (void)hipEventDestroy(e);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -714,22 +721,37 @@ std::pair<std::vector<double>, std::vector<double>> Session::run_jacobi(int n_sw
     int rep = 0;
     if (kernel_timing) ensure_timing_events(2 * n_sweeps);
     counters_begin();
-    for (int s = 0; s < n_sweeps; ++s) {
-        SweepArgs a = L.sweep_args(xin, xout, (float)omega, (float)delta, nullptr);
-        a.counters = counters_for(L);
-        if (propose) {
-            a.mlp = proposal->view();
-            a.prop_mode = proposal_mode;
-            a.prop_scale = proposal_scale;
+    // graphs carry no per-launch events or counters: plain replays only
+    const bool graphs_on = graph_chunk > 0 && !kernel_timing && !counting;
+    long long launches = 0;
+    int s = 0;
+    while (s < n_sweeps) {
+        const int next_rep = std::min(n_sweeps, (s / k + 1) * k);
+        if (graphs_on && next_rep - s >= graph_chunk) {
+            // an even chunk ends in the buffer it started from
+            EGGMI_HIP(hipGraphLaunch(sweep_graph(xin == L.x[0].p ? 0 : 1, kind, (float)omega, (float)delta, propose),
+                                     stream));
+            s += graph_chunk;
+            launches += graph_chunk;
+        } else {
+            SweepArgs a = L.sweep_args(xin, xout, (float)omega, (float)delta, nullptr);
+            a.counters = counters_for(L);
+            if (propose) {
+                a.mlp = proposal->view();
+                a.prop_mode = proposal_mode;
+                a.prop_scale = proposal_scale;
+            }
+            if (kernel_timing) EGGMI_HIP(hipEventRecord(tev[2 * s], stream));
+            launch_sweep(dim, kind, tile_variant, a, stream);
+            if (kernel_timing) EGGMI_HIP(hipEventRecord(tev[2 * s + 1], stream));
+            std::swap(xin, xout);
+            ++s;
+            ++launches;
         }
-        if (kernel_timing) EGGMI_HIP(hipEventRecord(tev[2 * s], stream));
-        launch_sweep(dim, kind, tile_variant, a, stream);
-        if (kernel_timing) EGGMI_HIP(hipEventRecord(tev[2 * s + 1], stream));
-        if ((s + 1) % k == 0 || s == n_sweeps - 1) {
-            reduce(L, xout, kind, delta, d_reports.p + rep, d_reports.p + nrep + rep);
+        if (s == next_rep) {
+            reduce(L, xin, kind, delta, d_reports.p + rep, d_reports.p + nrep + rep);
             ++rep;
         }
-        std::swap(xin, xout);
     }
     EGGMI_HIP(hipEventRecord(ev1, stream));
     L.cur = (xin == L.x[0].p) ? 0 : 1;
@@ -740,7 +762,7 @@ std::pair<std::vector<double>, std::vector<double>> Session::run_jacobi(int n_sw
     float ms = 0;
     EGGMI_HIP(hipEventElapsedTime(&ms, ev0, ev1));
     last_ms = ms;
-    last_launches = n_sweeps;
+    last_launches = launches;
     last_kernel_ms = 0.0;
     if (kernel_timing)
         for (int s = 0; s < n_sweeps; ++s) {
@@ -749,6 +771,58 @@ std::pair<std::vector<double>, std::vector<double>> Session::run_jacobi(int n_sw
             last_kernel_ms += k;
         }
     return {std::vector<double>(hr.begin(), hr.begin() + nrep), std::vector<double>(hr.begin() + nrep, hr.end())};
+}
+
+void Session::release_graphs()
+{
+    for (auto& g : graphs) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        g = SweepGraph {};
+    }
+}
+
+hipGraphExec_t Session::sweep_graph(int parity, int kind, float omega, float delta, bool propose)
+{
+    SweepGraph& g = graphs[parity];
+    const void* mlp = propose ? (const void*)proposal->view().fwd : nullptr;
+    if (g.exec && g.kind == kind && g.variant == tile_variant && g.chunk == graph_chunk && g.omega == omega &&
+        g.delta == delta && g.mlp == mlp && (!propose || (g.mode == proposal_mode && g.scale == proposal_scale)))
+        return g.exec;
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    g = SweepGraph {};
+    Level& L = fine;
+    float* b0 = L.x[parity].p;
+    float* b1 = L.x[1 - parity].p;
+    hipGraph_t graph = nullptr;
+    EGGMI_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    try {
+        for (int i = 0; i < graph_chunk; ++i) {
+            SweepArgs a = L.sweep_args(i % 2 == 0 ? b0 : b1, i % 2 == 0 ? b1 : b0, omega, delta, nullptr);
+            if (propose) {
+                a.mlp = proposal->view();
+                a.prop_mode = proposal_mode;
+                a.prop_scale = proposal_scale;
+            }
+            launch_sweep(dim, kind, tile_variant, a, stream);
+        }
+    } catch (...) {
+        (void)hipStreamEndCapture(stream, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+    }
+    EGGMI_HIP(hipStreamEndCapture(stream, &graph));
+    const hipError_t ie = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    EGGMI_HIP(ie);
+    g.kind = kind;
+    g.variant = tile_variant;
+    g.chunk = graph_chunk;
+    g.omega = omega;
+    g.delta = delta;
+    g.mlp = mlp;
+    g.mode = proposal_mode;
+    g.scale = proposal_scale;
+    return g.exec;
 }
 
 void Level::ensure_synth_list()
@@ -820,6 +894,7 @@ void Session::set_tile_variant(int v)
                               "unset the proposal first");
     tile_variant = v;
     if (host_only) return;
+    release_graphs();  // they hold the tile table being replaced
     fine.make_tiles(v);
     for (auto& c : coarse) c->make_tiles(v);
 }

@@ -159,6 +159,7 @@ struct Level {
     long long n_coarse_nodes = 0, n_fine_nodes = 0;
     DArr<float> rx, tau, grad, corr;
 
+    BlockDesc block_desc(size_t b) const;
     void upload_tables();
     void make_tiles(int variant);
     void build_push();
@@ -181,6 +182,19 @@ struct Session {
     double last_ms = 0.0;
     long long last_launches = 0;
     bool kernel_timing = false;     // bracket every sweep launch with events
+    // hipGraph replay of sweep chunks (graph_chunk sweeps, even; 0 = off):
+    // one captured graph per starting buffer, re-captured when a parameter
+    // baked into it changes
+    struct SweepGraph {
+        hipGraphExec_t exec = nullptr;
+        int kind = -1, variant = -1, mode = 0, chunk = 0;
+        float omega = 0.0f, delta = 0.0f, scale = 0.0f;
+        const void* mlp = nullptr;
+    };
+    int graph_chunk = 0;
+    SweepGraph graphs[2];
+    void release_graphs();
+    hipGraphExec_t sweep_graph(int parity, int kind, float omega, float delta, bool propose);
     double last_kernel_ms = 0.0;    // summed sweep-kernel time of the last run
     std::vector<hipEvent_t> tev;
     void ensure_timing_events(size_t n);

@@ -290,7 +290,7 @@ __device__ void tile_sweep(const SweepArgs& a, int tile)
     constexpr int LX = TX + 2, LY = TY + 2, LZ = (D == 3) ? TZ + 2 : 1;
     __shared__ float sh[D][LDS ? LZ * LY * LX : 1];
     const TileDesc td = a.tiles[tile];
-    const BlockDesc bd = a.blocks[td.blk];
+    const BlockDesc& bd = td.b;
     const long long PL = a.plane;
     const int tid = threadIdx.x;
 
@@ -582,7 +582,7 @@ __device__ void sample_sweep(const SweepArgs& a, int tile)
     constexpr int LX = TX + 2, LY = TY + 2;
     __shared__ float sh[D][LY * LX];
     const TileDesc td = a.tiles[tile];
-    const BlockDesc bd = a.blocks[td.blk];
+    const BlockDesc& bd = td.b;
     const long long PL = a.plane;
     const int tid = threadIdx.x;
     for (int e = tid; e < LY * LX; e += NT) {

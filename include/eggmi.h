@@ -168,6 +168,12 @@ eggmi_status eggmi_last_timing(eggmi_session* s, double* ms, int64_t* sweep_laun
 /* Bracket every sweep launch with HIP events (measurement only). */
 eggmi_status eggmi_set_kernel_timing(eggmi_session* s, int on);
 
+/* Replay plain sweeps in chunks of `chunk` (even; 0 = off, the default)
+ * captured once as a hipGraph per starting buffer, instead of one launch
+ * each (no reference counterpart). Reports still come at the requested
+ * cadence; per-launch timing and counters fall back to single launches. */
+eggmi_status eggmi_set_graph_chunk(eggmi_session* s, int chunk);
+
 /* Diagnostic counters (measurement only; no reference counterpart): when on,
  * every sweep launch counts the line-search trials it evaluated and the DOF
  * updates it made, per multigrid level. */

@@ -285,3 +285,28 @@ def test_3d_tile_variants_agree():
         out.append(s.get_X())
     for X in out[1:]:
         np.testing.assert_array_equal(X, out[0])
+
+
+def test_graph_chunks_match_launches():
+    """Sweeps replayed as captured hipGraph chunks give bitwise the same
+    states and reports as single launches, for report cadences that do and
+    do not divide into chunks, and after a tile-variant change (which
+    re-captures)."""
+    c = ogrid(33, 17)
+    a = _session(c)
+    b = _session(c)
+    b.core.set_graph_chunk(8)
+    for n, rep in ((50, 7), (64, 0), (13, 5), (40, 16)):
+        ea, ma = a.run(n, omega=0.8, report_every=rep)
+        eb, mb = b.run(n, omega=0.8, report_every=rep)
+        np.testing.assert_array_equal(ea, eb)
+        np.testing.assert_array_equal(ma, mb)
+        np.testing.assert_array_equal(a.get_X(), b.get_X())
+    a.core.set_tile_variant(2)
+    b.core.set_tile_variant(2)
+    ea, _ = a.run(33, omega=0.8, report_every=11)
+    eb, _ = b.run(33, omega=0.8, report_every=11)
+    np.testing.assert_array_equal(ea, eb)
+    np.testing.assert_array_equal(a.get_X(), b.get_X())
+    with pytest.raises(ValueError):
+        b.core.set_graph_chunk(7)
