@@ -178,7 +178,7 @@ def bench_ogrid(args):
 
     tune_best, tune_ms = s.core.autotune(20, 0.8)
     log(f"autotune: variant {tune_best}, ms per 20 sweeps {np.round(tune_ms, 4).tolist()}")
-    s.core.set_graph_chunk(GRAPH_CHUNK)  # replay sweeps as captured hipGraph chunks
+    s.core.set_graph_chunk(args.graph_chunk)  # replay sweeps as captured hipGraph chunks
     s.run(args.warmup, omega=0.8)
     e, m, t_run, k_avg, X_start = _timed(s, args.steps, 0.8)
     value = n_free * args.steps / t_run
@@ -207,12 +207,14 @@ def bench_ogrid(args):
         log(f"plain: {n} sweeps, {t:.3f}s to converge")
 
         # FAS: V(4,4) on 3 levels (the batched workload's cycle), on every
-        # level, and V(8,8) on 4 levels (the fastest found on this grid)
-        for name, nu, lev in (("fas_v44_3_levels", 4, 3), ("fas_v44_all_levels", 4, 32), ("fas_v88_4_levels", 8, 4)):
+        # level, and V(8,8) on 4 levels with 16 coarsest sweeps (the fastest
+        # found on this grid)
+        for name, nu, lev, nuc in (("fas_v44_3_levels", 4, 3, 32), ("fas_v44_all_levels", 4, 32, 32),
+                                   ("fas_v88_4_levels_coarse16", 8, 4, 16)):
             s.set_X(Xu)
 
-            def fas_step(chunk=5, nu=nu, lev=lev):
-                e, m = s.run_fas(chunk, nu_pre=nu, nu_post=nu, omega=0.8, max_levels=lev)
+            def fas_step(chunk=5, nu=nu, lev=lev, nuc=nuc):
+                e, m = s.run_fas(chunk, nu_pre=nu, nu_post=nu, nu_coarse=nuc, omega=0.8, max_levels=lev)
                 return e, m, chunk
 
             n, t, E = _converge(fas_step, e_star, 60.0)
@@ -265,7 +267,7 @@ def bench_ogrid(args):
                                "2% tangled by 1.5h, untangle then block-Jacobi TMOP sweeps, omega 0.8",
                    "free_dofs": n_free, "seed": SEED, "precision": "FP32 sweeps, FP64 reductions"},
         "autotune": {"best_variant": int(tune_best), "ms_per_20_sweeps": [float(v) for v in tune_ms]},
-        "graph_chunk": GRAPH_CHUNK,
+        "graph_chunk": args.graph_chunk,
         "time_to_untangled_s": t_untangle, "untangle_outer": outer,
         "time_to_untangled_converged": ttc,
         "final_energy": float(e[-1]), "final_min_det": float(m[-1]),
@@ -385,6 +387,8 @@ def main() -> int:
     ap.add_argument("--no-converge", action="store_true", help="skip the time-to-converged runs")
     ap.add_argument("--cpu-sweeps", type=int, default=150)
     ap.add_argument("--workload", default="ogrid", choices=("ogrid", "batched", "cubed_sphere"))
+    ap.add_argument("--graph-chunk", type=int, default=GRAPH_CHUNK,
+                    help="sweeps per captured hipGraph replay (even; 0 = single launches, e.g. under a tracer)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "

@@ -185,3 +185,20 @@ def test_mg_hierarchy_shapes_match_oracle():
         got = [lv[0] for lv in s.mg_levels()]
         ref = [lv.ids.shape for lv in fas.build_levels(2, c.grid.blocks[0], c.grid.free)[1:]]
         assert got == ref, (n, got, ref)
+
+
+def test_capi_error_codes_without_device():
+    """The C ABI reports invalid arguments before touching a device, null
+    handles as EGGMI_INVALID_ARGUMENT with a message, and the learned
+    proposal's constructor checks its dimension first."""
+    lib = ctypes.CDLL(str(ROOT / "egg_amd" / "_lib" / "libeggmi.so"))
+    lib.eggmi_last_error.restype = ctypes.c_char_p
+    out = ctypes.c_void_p()
+    assert lib.eggmi_mlp_create(4, ctypes.c_uint64(0), ctypes.c_int64(0), ctypes.byref(out)) == 1
+    assert b"dim" in lib.eggmi_last_error()
+    assert lib.eggmi_set_graph_chunk(None, 8) == 1
+    assert b"session" in lib.eggmi_last_error()
+    assert lib.eggmi_autotune(None, 5, ctypes.c_double(0.8), None, 0, None) == 1
+    assert lib.eggmi_energy(None, 0, ctypes.c_double(0.0), None, None) == 1
+    assert lib.eggmi_mlp_destroy(None) == 0  # deleting nothing is fine
+    assert lib.eggmi_session_destroy(None) == 0
