@@ -378,6 +378,46 @@ def bench_cubed(args):
     }
 
 
+def bench_mlp(args):
+    """The learned proposal's kernels on their own: batched inference (BF16
+    features from HBM) and training steps (forward + backward + Adam), 2D
+    and 3D networks, on synthetic pairs."""
+    from egg_amd._cpp import cpp_core
+
+    out = {}
+    rng = np.random.default_rng(SEED)
+    for dim, (nin, inp, hid, nout) in ((2, (22, 32, 64, 2)), (3, (85, 96, 128, 3))):
+        n = 1 << 20
+        X = np.zeros((n, inp), np.float32)
+        X[:, :nin] = rng.standard_normal((n, nin)).astype(np.float32)
+        T = (0.5 * np.tanh(X[:, :nout])).astype(np.float32)
+        net = cpp_core.ProposalMLP(dim, 0, n)
+        net.data_add(X, T)
+        fwd_flops = 2 * (nin * hid + hid * hid + hid * nout)
+        net.forward(X[:4096])
+        t0 = time.perf_counter()
+        net.forward(X)
+        t_f = time.perf_counter() - t0  # includes the host->device copy of X
+        res = {"forward_samples_per_s_incl_upload": n / t_f}
+        for batch in (4096, 32768):
+            net.train(1, batch=batch, lr=1e-3, seed=0)
+            t0 = time.perf_counter()
+            hist = net.train(2, batch=batch, lr=1e-3, seed=1)
+            t_t = time.perf_counter() - t0
+            rate = 2 * n / t_t
+            res[f"train_batch_{batch}"] = {"samples_per_s": rate, "tflops": rate * 3 * fwd_flops / 1e12,
+                                           "frac_of_bf16_mfma_peak": rate * 3 * fwd_flops / 2.5e15,
+                                           "loss_last": float(hist[-1])}
+        res["flops_per_sample_forward"] = fwd_flops
+        out[f"{dim}d"] = res
+        log(f"MLP {dim}D: {res}")
+    return {"metric": "MLP proposal kernels: training samples/s", "value": out["2d"]["train_batch_32768"]["samples_per_s"],
+            "unit": "samples/s", "n_gpus": 1, "steps": 2, "warmup": 1, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "bf16 (fp32 accumulate)", "data": "synthetic",
+            "config": {"workload": "MLP 22-64-64-2 and 85-128-128-3, 2^20 pairs"}, "mlp": out,
+            "roofline": None, "cpu_baseline": None}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -386,7 +426,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline and parity leg")
     ap.add_argument("--no-converge", action="store_true", help="skip the time-to-converged runs")
     ap.add_argument("--cpu-sweeps", type=int, default=150)
-    ap.add_argument("--workload", default="ogrid", choices=("ogrid", "batched", "cubed_sphere"))
+    ap.add_argument("--workload", default="ogrid", choices=("ogrid", "batched", "cubed_sphere", "mlp"))
     ap.add_argument("--graph-chunk", type=int, default=GRAPH_CHUNK,
                     help="sweeps per captured hipGraph replay (even; 0 = single launches, e.g. under a tracer)")
     args = ap.parse_args()
@@ -395,7 +435,7 @@ def main() -> int:
               "(the path would shard by blocks, one halo exchange per sweep, but only the one-GPU path is built)",
               file=sys.stderr)
         return 2
-    fn = {"ogrid": bench_ogrid, "batched": bench_batched, "cubed_sphere": bench_cubed}[args.workload]
+    fn = {"ogrid": bench_ogrid, "batched": bench_batched, "cubed_sphere": bench_cubed, "mlp": bench_mlp}[args.workload]
     print(json.dumps(fn(args)), flush=True)
     return 0
 

@@ -6,12 +6,13 @@
 //   1. fwd_bwd: one wave per 16 samples. Forward as in inference, keeping the
 //      hidden pre-activations in registers; dY = 2 (y - t) / B; dH2 = W3 dY
 //      (16x16x16), dZ2 = dH2 * gelu'(Z2), dH1 = W2 dZ2 (16x16x32, chained),
-//      dZ1 = dH1 * gelu'(Z1). The layer inputs and the dZ's are written in
-//      natural [sample][feature] BF16 layout (each lane stores its 4
-//      consecutive rows of a D tile with one 8-byte store).
+//      dZ1 = dH1 * gelu'(Z1). The layer inputs and the dZ's are written
+//      feature-major ([feature][sample] BF16), so that the weight-gradient
+//      MFMA fragments below are contiguous.
 //   2. wgrad: dW = Act^T G as 16x16 output tiles, one wave per (tile, chunk of
-//      256 samples), K = samples (16x16x32); bias gradients as column sums of
-//      G over the chunk. Partials per chunk.
+//      256 samples), K = samples (16x16x32: each lane's A and B fragments are
+//      8 consecutive samples of one feature, one 16-byte load each); bias
+//      gradients as row sums of G over the chunk. Partials per chunk.
 //   3. adam: one thread per parameter folds the chunk partials in chunk order
 //      (deterministic), then the Adam update of the FP32 master.
 //   4. pack: master -> BF16 fragments for the next step.
@@ -33,7 +34,7 @@ namespace
 constexpr int kChunk = 256;  // samples per weight-gradient partial
 
 template <int D> struct Act {
-    // per-batch activation buffers (BF16, natural layout), offsets in elements
+    // per-batch activation buffers (BF16, feature-major [feature][B]), offsets in elements
     using L = MlpLayout<D>;
     __host__ __device__ static long long x(long long) { return 0; }
     __host__ __device__ static long long h1(long long B) { return B * L::INP; }
@@ -135,14 +136,14 @@ __global__ __launch_bounds__(64) void forward_kernel(MlpView w, const __bf16* __
         for (int i = 0; i < L::OUT; ++i) Y[row * L::OUT + i] = y[i];
 }
 
-// D-layout tile (rows 16*mt + 4g + i, column = sample) -> natural buf[sample][width]
-__device__ __forceinline__ void store_tile(__bf16* buf, long long sample, int width, int mt, const f32x4& v)
+// D-layout tile (rows 16*mt + 4g + i, column = sample) -> feature-major
+// buf[feature][B] (for each row, the 16 lanes of a group write 16 consecutive
+// samples)
+__device__ __forceinline__ void store_tile(__bf16* buf, long long sample, long long B, int mt, const f32x4& v)
 {
     const int g = (threadIdx.x & 63) >> 4;
-    s16x4 s;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) s[i] = bf_bits(v[i]);
-    *reinterpret_cast<s16x4*>(buf + sample * width + mt * 16 + 4 * g) = s;
+    for (int i = 0; i < 4; ++i) buf[(long long)(mt * 16 + 4 * g + i) * B + sample] = (__bf16)v[i];
 }
 
 template <int D>
@@ -159,10 +160,11 @@ __global__ __launch_bounds__(64) void fwd_bwd_kernel(MlpView w, const s16x4* __r
     const long long src = idx[s];
     bf16x8 xin[L::KS1];
     load_x<D>(X, src, true, xin);
-    // the batch copy of the inputs (for dW1)
+    // the batch copy of the inputs (for dW1), feature-major
 #pragma unroll
     for (int ks = 0; ks < L::KS1; ++ks)
-        *reinterpret_cast<bf16x8*>(act + A::x(B) + s * L::INP + ks * 32 + 8 * g) = xin[ks];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) act[A::x(B) + (long long)(ks * 32 + 8 * g + i) * B + s] = xin[ks][i];
     f32x4 z1[L::MT1], z2[L::MT2];
     const f32x4 y = mlp_forward<D, true>(w, xin, z1, z2);
     // dY (rows = outputs) and the loss
@@ -176,19 +178,19 @@ __global__ __launch_bounds__(64) void fwd_bwd_kernel(MlpView w, const s16x4* __r
         se += r * r;
         dy[i] = sc * r;
     }
-    store_tile(act + A::dy(B), s, 16, 0, dy);
+    store_tile(act + A::dy(B), s, B, 0, dy);
     // hidden activations as the forward saw them
 #pragma unroll
     for (int mt = 0; mt < L::MT1; ++mt) {
         f32x4 h;
         for (int i = 0; i < 4; ++i) h[i] = gelu(z1[mt][i]);
-        store_tile(act + A::h1(B), s, L::H, mt, h);
+        store_tile(act + A::h1(B), s, B, mt, h);
     }
 #pragma unroll
     for (int mt = 0; mt < L::MT2; ++mt) {
         f32x4 h;
         for (int i = 0; i < 4; ++i) h[i] = gelu(z2[mt][i]);
-        store_tile(act + A::h2(B), s, L::H, mt, h);
+        store_tile(act + A::h2(B), s, B, mt, h);
     }
     // dH2 = W3 . dY, dZ2
     s16x4 bdy;
@@ -202,7 +204,7 @@ __global__ __launch_bounds__(64) void fwd_bwd_kernel(MlpView w, const s16x4* __r
 #pragma unroll
         for (int i = 0; i < 4; ++i) acc[i] *= gelu_grad(z2[mt][i]);
         dz2[mt] = acc;
-        store_tile(act + A::dz2(B), s, L::H, mt, acc);
+        store_tile(act + A::dz2(B), s, B, mt, acc);
     }
     // dH1 = W2 . dZ2, dZ1
 #pragma unroll
@@ -212,7 +214,7 @@ __global__ __launch_bounds__(64) void fwd_bwd_kernel(MlpView w, const s16x4* __r
         for (int ks = 0; ks < L::KS2; ++ks) acc = mfma32(bw2[(mt * L::KS2 + ks) * 64 + lane], chain(dz2[2 * ks], dz2[2 * ks + 1]), acc);
 #pragma unroll
         for (int i = 0; i < 4; ++i) acc[i] *= gelu_grad(z1[mt][i]);
-        store_tile(act + A::dz1(B), s, L::H, mt, acc);
+        store_tile(act + A::dz1(B), s, B, mt, acc);
     }
     // per-tile squared error (deterministic: lane 0 sums in lane order)
     __shared__ float red[64];
@@ -240,48 +242,46 @@ __global__ __launch_bounds__(64) void wgrad_kernel(const __bf16* __restrict__ ac
     float* out = part + ((long long)chunk * TT::n + tile) * 256;
     int t = tile;
     const __bf16 *a = nullptr, *gm = nullptr;
-    int wa = 0, wg = 0, m0 = 0, n0 = 0;
+    int m0 = 0, n0 = 0;
     if (t < TT::t1) {
-        a = act + A::x(B), wa = L::INP, gm = act + A::dz1(B), wg = L::H;
+        a = act + A::x(B), gm = act + A::dz1(B);
         m0 = (t / (L::H / 16)) * 16, n0 = (t % (L::H / 16)) * 16;
     } else if ((t -= TT::t1) < TT::t2) {
-        a = act + A::h1(B), wa = L::H, gm = act + A::dz2(B), wg = L::H;
+        a = act + A::h1(B), gm = act + A::dz2(B);
         m0 = (t / (L::H / 16)) * 16, n0 = (t % (L::H / 16)) * 16;
     } else if ((t -= TT::t2) < TT::t3) {
-        a = act + A::h2(B), wa = L::H, gm = act + A::dy(B), wg = 16;
+        a = act + A::h2(B), gm = act + A::dy(B);
         m0 = t * 16, n0 = 0;
     } else {
-        // bias: column sums of dZ1 / dZ2 / dY over the chunk (4 row groups of
-        // 16 lanes, folded in group order)
+        // bias: row sums of dZ1 / dZ2 / dY over the chunk (feature-major:
+        // each group of 16 lanes takes 16 features, the 4 groups split the
+        // samples; folded in group order)
         t -= TT::t3;
-        const __bf16* col;
-        int w;
+        const __bf16* rows;
         if (t < TT::b1) {
-            col = act + A::dz1(B) + t * 16, w = L::H;
+            rows = act + A::dz1(B) + (long long)(t * 16) * B;
         } else if ((t -= TT::b1) < TT::b2) {
-            col = act + A::dz2(B) + t * 16, w = L::H;
+            rows = act + A::dz2(B) + (long long)(t * 16) * B;
         } else {
-            col = act + A::dy(B), w = 16;
+            rows = act + A::dy(B);
         }
         float sacc = 0.f;
-        for (int r = g; r < kChunk; r += 4) sacc += (float)col[(b0 + r) * w + c];
+        const __bf16* row = rows + (long long)c * B + b0;
+        for (int r = g; r < kChunk; r += 4) sacc += (float)row[r];
         __shared__ float red[64];
         red[lane] = sacc;
         __syncthreads();
         if (lane < 16) out[lane] = (red[lane] + red[16 + lane]) + (red[32 + lane] + red[48 + lane]);
         return;
     }
+    // A[m][k] = Act[m0 + m][sample k], B[k][n] = G[n0 + n][sample k]: lane l
+    // holds 8 consecutive samples of feature l % 16 of each
+    const __bf16* ar = a + (long long)(m0 + c) * B + b0 + 8 * g;
+    const __bf16* gr = gm + (long long)(n0 + c) * B + b0 + 8 * g;
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < kChunk; k0 += 32) {
-        bf16x8 fa, fb;
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const long long r = b0 + k0 + 8 * g + i;
-            fa[i] = a[r * wa + m0 + c];
-            fb[i] = gm[r * wg + n0 + c];
-        }
-        acc = mfma32(fa, fb, acc);
-    }
+#pragma unroll 4
+    for (int k0 = 0; k0 < kChunk; k0 += 32)
+        acc = mfma32(*reinterpret_cast<const bf16x8*>(ar + k0), *reinterpret_cast<const bf16x8*>(gr + k0), acc);
 #pragma unroll
     for (int i = 0; i < 4; ++i) out[(4 * g + i) * 16 + c] = acc[i];
 }
