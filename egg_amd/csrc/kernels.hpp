@@ -34,16 +34,21 @@ struct __align__(16) TileDesc {
     int blk;
 };
 
-// Stored-stencil DOFs (block faces, interfaces, singular corners): one
-// thread per DOF walks its occurrences [off[d], off[d+1]).
+// Stored-stencil DOFs (block faces, interfaces, singular corners). A group
+// of stored_lanes(dim) lanes serves one DOF, one occurrence per lane and
+// round: DOF d's occurrence r * G + l sits at padded position
+// (d * rounds + r) * G + l, so a lane finds its occurrence without reading
+// an offset table. Padding entries carry meta bit 7.
+__host__ __device__ constexpr int stored_lanes(int dim) { return dim == 2 ? 16 : 64; }
+constexpr uint8_t kStoredPad = 0x80;
 struct StoredView {
     int n = 0;
-    const int* dof = nullptr;  // [n] slot written
-    const int* off = nullptr;  // [n+1]
-    long long total = 0;       // occurrences
-    const int* gc = nullptr;   // [total] corner slot
-    const int* gn = nullptr;   // [dim][total] neighbour slots
-    const uint8_t* meta = nullptr;  // [total] bits 0-2: role+1 (0..D), bits 3..5: sign of axis k (1 = -1)
+    int rounds = 1;             // ceil(max occurrences of a DOF / lanes)
+    const int* dof = nullptr;   // [n] slot written
+    long long total = 0;        // padded occurrences: n * rounds * lanes
+    const int* gc = nullptr;    // [total] corner slot
+    const int* gn = nullptr;    // [dim][total] neighbour slots
+    const uint8_t* meta = nullptr;  // [total] bits 0-2: role+1 (0..D), bits 3..5: sign of axis k (1 = -1), bit 7: padding
     const float* W = nullptr;       // [total][dim*dim] per-occurrence W_inv rows, or null (uniform)
 };
 

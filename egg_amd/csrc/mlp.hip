@@ -138,99 +138,214 @@ __global__ __launch_bounds__(64) void forward_kernel(MlpView w, const __bf16* __
 
 // D-layout tile (rows 16*mt + 4g + i, column = sample) -> feature-major
 // buf[feature][B] (for each row, the 16 lanes of a group write 16 consecutive
-// samples)
-__device__ __forceinline__ void store_tile(__bf16* buf, long long sample, long long B, int mt, const f32x4& v)
+// samples). `lane_off` = byte offset of (row 4g, this lane's sample) from
+// the buffer; the row base is uniform, so the stores take a scalar base and
+// a 32-bit lane offset (the activation buffer is < 2 GiB).
+__device__ __forceinline__ void store_tile(__bf16* buf, unsigned lane_off, long long B, int mt, const f32x4& v)
 {
-    const int g = (threadIdx.x & 63) >> 4;
+    char* row = reinterpret_cast<char*>(buf + (long long)(mt * 16) * B);
+    const unsigned rs = (unsigned)(B * 2);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) buf[(long long)(mt * 16 + 4 * g + i) * B + sample] = (__bf16)v[i];
+    for (int i = 0; i < 4; ++i) *reinterpret_cast<__bf16*>(row + (lane_off + i * rs)) = (__bf16)v[i];
 }
 
-template <int D>
-__global__ __launch_bounds__(64) void fwd_bwd_kernel(MlpView w, const s16x4* __restrict__ bw3,
-                                                     const bf16x8* __restrict__ bw2, const __bf16* __restrict__ X,
-                                                     const float* __restrict__ T, const int* __restrict__ idx,
-                                                     long long B, __bf16* __restrict__ act,
-                                                     double* __restrict__ loss)
+// Training forward/backward: a workgroup of WAVES waves stages every weight
+// fragment (forward, W3^T, W2^T) in LDS once (all loads in flight together);
+// each wave then takes NT tiles of 16 samples, so each fragment read feeds
+// NT independent MFMA chains. Small batches use one wave and one tile per
+// workgroup (enough workgroups to spread over the chip), large batches four
+// waves sharing one staged copy.
+template <int D> constexpr int kFragBytes = MlpLayout<D>::n_fwd_frags * 1024 + MlpLayout<D>::n_bw3_frags * 512 +
+                                            MlpLayout<D>::n_bw2_frags * 1024;
+
+template <int D, int WAVES, int NT>
+__global__ __launch_bounds__(64 * WAVES) void fwd_bwd_kernel(MlpView w, const void* __restrict__ bw,
+                                                                   const __bf16* __restrict__ X,
+                                                                   const float* __restrict__ T,
+                                                                   const int* __restrict__ idx, long long B,
+                                                                   __bf16* __restrict__ act,
+                                                                   double* __restrict__ loss)
 {
     using L = MlpLayout<D>;
     using A = Act<D>;
-    const int lane = threadIdx.x, g = lane >> 4;
-    const long long s = (long long)blockIdx.x * 16 + (lane & 15);  // batch row
-    const long long src = idx[s];
-    bf16x8 xin[L::KS1];
-    load_x<D>(X, src, true, xin);
+    // one wave reads the fragments from L2 as it goes (a staged copy would
+    // serialise 24-96 KB of loads in front of its work)
+    constexpr bool kStage = WAVES > 1;
+    __shared__ uint4 frag[kStage ? kFragBytes<D> / 16 : 1];
+    const uint4* fsrc = reinterpret_cast<const uint4*>(w.fwd);
+    const uint4* bsrc = static_cast<const uint4*>(bw);
+    if constexpr (kStage) {
+        constexpr int nf = L::n_fwd_frags * 64, nb = (kFragBytes<D> - L::n_fwd_frags * 1024) / 16;
+        for (int e = threadIdx.x; e < nf; e += 64 * WAVES) frag[e] = fsrc[e];
+        for (int e = threadIdx.x; e < nb; e += 64 * WAVES) frag[nf + e] = bsrc[e];
+        fsrc = frag;
+        bsrc = frag + nf;
+    }
+    const bf16x8* wf1 = reinterpret_cast<const bf16x8*>(fsrc);
+    const s16x4* bw3 = reinterpret_cast<const s16x4*>(bsrc);
+    const bf16x8* bw2 = reinterpret_cast<const bf16x8*>(bsrc + L::n_bw3_frags * 32);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, rg = 4 * g;
+    long long s[NT], src[NT];
+    unsigned off[NT], xoff[NT];  // byte offsets of (row 4g / 8g, sample s) in a [feature][B] buffer
+    bf16x8 xin[NT][L::KS1];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        s[t] = (((long long)blockIdx.x * WAVES + wave) * NT + t) * 16 + (lane & 15);  // batch row
+        off[t] = (unsigned)(((long long)rg * B + s[t]) * 2);
+        xoff[t] = (unsigned)(((long long)(8 * g) * B + s[t]) * 2);
+        src[t] = idx[s[t]];
+        load_x<D>(X, src[t], true, xin[t]);
+    }
     // the batch copy of the inputs (for dW1), feature-major
 #pragma unroll
-    for (int ks = 0; ks < L::KS1; ++ks)
+    for (int ks = 0; ks < L::KS1; ++ks) {
+        char* row = reinterpret_cast<char*>(act + A::x(B) + (long long)(ks * 32) * B);
+        const unsigned rs = (unsigned)(B * 2);
 #pragma unroll
-        for (int i = 0; i < 8; ++i) act[A::x(B) + (long long)(ks * 32 + 8 * g + i) * B + s] = xin[ks][i];
-    f32x4 z1[L::MT1], z2[L::MT2];
-    const f32x4 y = mlp_forward<D, true>(w, xin, z1, z2);
-    // dY (rows = outputs) and the loss
-    f32x4 dy;
-    float se = 0.f;
-    const float sc = 2.0f / (float)B;
+        for (int t = 0; t < NT; ++t)
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int o = 4 * g + i;
-        const float r = o < L::OUT ? y[i] - T[src * 4 + o] : 0.f;
-        se += r * r;
-        dy[i] = sc * r;
+            for (int i = 0; i < 8; ++i)
+                *reinterpret_cast<__bf16*>(row + (xoff[t] + i * rs)) = xin[t][ks][i];
     }
-    store_tile(act + A::dy(B), s, B, 0, dy);
-    // hidden activations as the forward saw them
+    if constexpr (kStage) __syncthreads();
+    // layer 1: Z1 = W1 X + b1, H1 = gelu(Z1) (gelu'(Z1) kept)
+    f32x4 dz1[NT][L::MT1];  // gelu'(Z1)
+    bf16x8 hb[NT][L::KS2];
 #pragma unroll
     for (int mt = 0; mt < L::MT1; ++mt) {
-        f32x4 h;
-        for (int i = 0; i < 4; ++i) h[i] = gelu(z1[mt][i]);
-        store_tile(act + A::h1(B), s, B, mt, h);
+        f32x4 acc[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[t] = f32x4 {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < L::KS1; ++ks) {
+            const bf16x8 wf = wf1[(L::F1 + mt * L::KS1 + ks) * 64 + lane];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) acc[t] = mfma32(wf, xin[t][ks], acc[t]);
+        }
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            f32x4 h;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float dg;
+                h[i] = gelu_and_grad(acc[t][i] + w.params[L::ob1 + mt * 16 + rg + i], dg);
+                dz1[t][mt][i] = dg;
+                hb[t][mt >> 1][(mt & 1) * 4 + i] = (__bf16)h[i];
+            }
+            store_tile(act + A::h1(B), off[t], B, mt, h);
+        }
     }
+    // layer 2
+    f32x4 dz2[NT][L::MT2];  // gelu'(Z2)
+    bf16x8 hb2[NT][L::KS3];
 #pragma unroll
     for (int mt = 0; mt < L::MT2; ++mt) {
-        f32x4 h;
-        for (int i = 0; i < 4; ++i) h[i] = gelu(z2[mt][i]);
-        store_tile(act + A::h2(B), s, B, mt, h);
-    }
-    // dH2 = W3 . dY, dZ2
-    s16x4 bdy;
+        f32x4 acc[NT];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) bdy[i] = bf_bits(dy[i]);
-    f32x4 dz2[L::MT2];
+        for (int t = 0; t < NT; ++t) acc[t] = f32x4 {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < L::KS2; ++ks) {
+            const bf16x8 wf = wf1[(L::F2 + mt * L::KS2 + ks) * 64 + lane];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) acc[t] = mfma32(wf, hb[t][ks], acc[t]);
+        }
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            f32x4 h;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float dg;
+                h[i] = gelu_and_grad(acc[t][i] + w.params[L::ob2 + mt * 16 + rg + i], dg);
+                dz2[t][mt][i] = dg;
+                hb2[t][mt >> 1][(mt & 1) * 4 + i] = (__bf16)h[i];
+            }
+            store_tile(act + A::h2(B), off[t], B, mt, h);
+        }
+    }
+    // layer 3, dY (rows = outputs) and the per-tile squared error
+    f32x4 dy[NT];
+    float se[NT];
+    {
+        f32x4 y[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) y[t] = f32x4 {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < L::KS3; ++ks) {
+            const bf16x8 wf = wf1[(L::F3 + ks) * 64 + lane];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) y[t] = mfma32(wf, hb2[t][ks], y[t]);
+        }
+        const float sc = 2.0f / (float)B;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            se[t] = 0.f;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int o = rg + i;
+                const float r = o < L::OUT ? y[t][i] + w.params[L::ob3 + o] - T[src[t] * 4 + o] : 0.f;
+                se[t] += r * r;
+                dy[t][i] = sc * r;
+            }
+            store_tile(act + A::dy(B), off[t], B, 0, dy[t]);
+        }
+    }
+    // dH2 = W3 . dY, dZ2 = dH2 * gelu'(Z2)
+    s16x4 bdy[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) bdy[t][i] = bf_bits(dy[t][i]);
+    bf16x8 bdz2[NT][L::KS2];
 #pragma unroll
     for (int mt = 0; mt < L::MT2; ++mt) {
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bw3[mt * 64 + lane], bdy, acc, 0, 0, 0);
+        const s16x4 wf = bw3[mt * 64 + lane];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) acc[i] *= gelu_grad(z2[mt][i]);
-        dz2[mt] = acc;
-        store_tile(act + A::dz2(B), s, B, mt, acc);
+        for (int t = 0; t < NT; ++t) {
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wf, bdy[t], acc, 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                acc[i] *= dz2[t][mt][i];
+                bdz2[t][mt >> 1][(mt & 1) * 4 + i] = (__bf16)acc[i];
+            }
+            store_tile(act + A::dz2(B), off[t], B, mt, acc);
+        }
     }
-    // dH1 = W2 . dZ2, dZ1
+    // dH1 = W2 . dZ2, dZ1 = dH1 * gelu'(Z1)
 #pragma unroll
     for (int mt = 0; mt < L::MT1; ++mt) {
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        f32x4 acc[NT];
 #pragma unroll
-        for (int ks = 0; ks < L::KS2; ++ks) acc = mfma32(bw2[(mt * L::KS2 + ks) * 64 + lane], chain(dz2[2 * ks], dz2[2 * ks + 1]), acc);
+        for (int t = 0; t < NT; ++t) acc[t] = f32x4 {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int i = 0; i < 4; ++i) acc[i] *= gelu_grad(z1[mt][i]);
-        store_tile(act + A::dz1(B), s, B, mt, acc);
+        for (int ks = 0; ks < L::KS2; ++ks) {
+            const bf16x8 wf = bw2[(mt * L::KS2 + ks) * 64 + lane];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) acc[t] = mfma32(wf, bdz2[t][ks], acc[t]);
+        }
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) acc[t][i] *= dz1[t][mt][i];
+            store_tile(act + A::dz1(B), off[t], B, mt, acc[t]);
+        }
     }
-    // per-tile squared error (deterministic: lane 0 sums in lane order)
-    __shared__ float red[64];
-    red[lane] = se;
+    // squared error of each 16-sample tile (deterministic: one lane sums in lane order)
+    __shared__ float red[WAVES][NT][64];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) red[wave][t][lane] = se[t];
     __syncthreads();
-    if (lane == 0) {
-        double t = 0.0;
-        for (int i = 0; i < 64; ++i) t += red[i];
-        loss[blockIdx.x] = t;
+    if (lane < NT) {
+        double sum = 0.0;
+        for (int i = 0; i < 64; ++i) sum += red[wave][lane][i];
+        loss[((long long)blockIdx.x * WAVES + wave) * NT + lane] = sum;
     }
 }
 
 // One wave per (tile, chunk): 16x16 tile of Act^T G over kChunk samples, or
 // a bias column-sum tile.
 template <int D>
-__global__ __launch_bounds__(64) void wgrad_kernel(const __bf16* __restrict__ act, long long B,
+__global__ __launch_bounds__(64) void wgrad_kernel(const __bf16* __restrict__ act, long long B, int cpw,
                                                    float* __restrict__ part)
 {
     using L = MlpLayout<D>;
@@ -238,7 +353,8 @@ __global__ __launch_bounds__(64) void wgrad_kernel(const __bf16* __restrict__ ac
     using TT = Tiles<D>;
     const int tile = blockIdx.x, chunk = blockIdx.y;
     const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
-    const long long b0 = (long long)chunk * kChunk;
+    const long long b0 = (long long)chunk * cpw * kChunk;
+    const int ns = cpw * kChunk;  // samples of this part
     float* out = part + ((long long)chunk * TT::n + tile) * 256;
     int t = tile;
     const __bf16 *a = nullptr, *gm = nullptr;
@@ -267,7 +383,7 @@ __global__ __launch_bounds__(64) void wgrad_kernel(const __bf16* __restrict__ ac
         }
         float sacc = 0.f;
         const __bf16* row = rows + (long long)c * B + b0;
-        for (int r = g; r < kChunk; r += 4) sacc += (float)row[r];
+        for (int r = g; r < ns; r += 4) sacc += (float)row[r];
         __shared__ float red[64];
         red[lane] = sacc;
         __syncthreads();
@@ -278,21 +394,60 @@ __global__ __launch_bounds__(64) void wgrad_kernel(const __bf16* __restrict__ ac
     // holds 8 consecutive samples of feature l % 16 of each
     const __bf16* ar = a + (long long)(m0 + c) * B + b0 + 8 * g;
     const __bf16* gr = gm + (long long)(n0 + c) * B + b0 + 8 * g;
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll 4
-    for (int k0 = 0; k0 < kChunk; k0 += 32)
-        acc = mfma32(*reinterpret_cast<const bf16x8*>(ar + k0), *reinterpret_cast<const bf16x8*>(gr + k0), acc);
+    // two independent accumulation chains (even / odd 32-sample steps)
+    f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+    for (int k0 = 0; k0 < ns; k0 += 64) {
+        acc0 = mfma32(*reinterpret_cast<const bf16x8*>(ar + k0), *reinterpret_cast<const bf16x8*>(gr + k0), acc0);
+        acc1 = mfma32(*reinterpret_cast<const bf16x8*>(ar + k0 + 32), *reinterpret_cast<const bf16x8*>(gr + k0 + 32),
+                      acc1);
+    }
 #pragma unroll
-    for (int i = 0; i < 4; ++i) out[(4 * g + i) * 16 + c] = acc[i];
+    for (int i = 0; i < 4; ++i) out[(4 * g + i) * 16 + c] = acc0[i] + acc1[i];
 }
 
-// Fold the chunk partials into the gradient (fixed chunk order) and take an
-// Adam step on the FP32 master. lr = 0 only folds.
+// Slot of feature f in a chained (D-layout) 32-chunk: the inverse of
+// chain_feature.
+__device__ __forceinline__ int chain_slot(int f)
+{
+    const int c = f & ~31, q = f & 31;
+    return q < 16 ? c + 8 * (q >> 2) + (q & 3) : c + 8 * ((q - 16) >> 2) + 4 + ((q - 16) & 3);
+}
+
+// Write parameter p's BF16 copies into the packed fragments (the entries
+// pack_kernel derives from it; padding entries stay zero).
+template <int D>
+__device__ __forceinline__ void scatter_param(long long p, float v, __bf16* fwd, __bf16* bw3, __bf16* bw2)
+{
+    using L = MlpLayout<D>;
+    const __bf16 b = (__bf16)v;
+    auto frag = [](int f, int lane, int i) { return (long long)f * 512 + lane * 8 + i; };
+    if (p < L::ob1) {  // W1[feat][m]
+        const int feat = (int)(p / L::H), m = (int)(p % L::H);
+        fwd[frag(L::F1 + (m / 16) * L::KS1 + feat / 32, ((feat % 32) / 8) * 16 + m % 16, feat % 8)] = b;
+    } else if (p >= L::oW2 && p < L::ob2) {  // W2[feat][m]
+        const int q = (int)(p - L::oW2), feat = q / L::H, m = q % L::H;
+        const int sl = chain_slot(feat);
+        fwd[frag(L::F2 + (m / 16) * L::KS2 + sl / 32, ((sl % 32) / 8) * 16 + m % 16, sl % 8)] = b;
+        const int sm = chain_slot(m);  // W2^T: A[h1 = feat][slot(h2 = m)]
+        bw2[frag((feat / 16) * L::KS2 + sm / 32, ((sm % 32) / 8) * 16 + feat % 16, sm % 8)] = b;
+    } else if (p >= L::oW3 && p < L::ob3) {  // W3[feat][o]
+        const int q = (int)(p - L::oW3), feat = q / L::OUT, o = q % L::OUT;
+        const int sl = chain_slot(feat);
+        fwd[frag(L::F3 + sl / 32, ((sl % 32) / 8) * 16 + o, sl % 8)] = b;
+        bw3[(long long)(feat / 16) * 256 + ((o / 4) * 16 + feat % 16) * 4 + o % 4] = b;  // A[h][o]
+    }
+}
+
+// Fold the weight-gradient partials into the gradient (fixed part order),
+// take an Adam step on the FP32 master and refresh its BF16 fragment
+// copies. lr = 0 only folds.
 template <int D>
 __global__ __launch_bounds__(256) void adam_kernel(float* __restrict__ P, float* __restrict__ m,
                                                    float* __restrict__ v, float* __restrict__ grad,
-                                                   const float* __restrict__ part, int n_chunks, float lr,
-                                                   float bc1, float bc2)
+                                                   const float* __restrict__ part, int n_parts, float lr,
+                                                   float bc1, float bc2, __bf16* __restrict__ fwd,
+                                                   __bf16* __restrict__ bw)
 {
     using L = MlpLayout<D>;
     using TT = Tiles<D>;
@@ -318,8 +473,19 @@ __global__ __launch_bounds__(256) void adam_kernel(float* __restrict__ P, float*
         const int o = (int)(p - L::ob3);
         tile = TT::n - 1, pos = o;
     }
+    // eight loads in flight, summed in part order
+    const float* src = part + (long long)tile * 256 + pos;
+    const long long stride = (long long)TT::n * 256;
     float gsum = 0.f;
-    for (int c = 0; c < n_chunks; ++c) gsum += part[((long long)c * TT::n + tile) * 256 + pos];
+    int c = 0;
+    for (; c + 8 <= n_parts; c += 8) {
+        float x[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) x[i] = src[(c + i) * stride];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) gsum += x[i];
+    }
+    for (; c < n_parts; ++c) gsum += src[c * stride];
     grad[p] = gsum;
     if (lr == 0.f) return;
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
@@ -327,7 +493,9 @@ __global__ __launch_bounds__(256) void adam_kernel(float* __restrict__ P, float*
     const float vv = b2 * v[p] + (1.f - b2) * gsum * gsum;
     m[p] = mm;
     v[p] = vv;
-    P[p] -= lr * (mm / bc1) / (sqrtf(vv / bc2) + eps);
+    const float np = P[p] - lr * (mm / bc1) / (sqrtf(vv / bc2) + eps);
+    P[p] = np;
+    scatter_param<D>(p, np, fwd, bw, bw + (long long)L::n_bw3_frags * 256);
 }
 
 template <int D> constexpr long long bw_bytes()
@@ -343,27 +511,47 @@ template <int D> void pack_impl(const float* P, bf16x8* fwd, void* bw, hipStream
     pack_kernel<D><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(P, fwd, static_cast<char*>(bw));
 }
 
+// Weight-gradient parts per step: each wave of wgrad_kernel sums cpw
+// consecutive chunks; cpw is the largest power of two dividing the chunk
+// count that still leaves 4096 waves (four per SIMD, to hide the operand
+// loads), so that the fold in adam_kernel reads fewer parts.
+template <int D> int wgrad_parts(long long batch)
+{
+    const int n_chunks = (int)(batch / kChunk);
+    int cpw = 1;
+    while (n_chunks % (2 * cpw) == 0 && (long long)Tiles<D>::n * (n_chunks / (2 * cpw)) >= 4096) cpw *= 2;
+    return n_chunks / cpw;
+}
+
 template <int D> void step_impl(const MlpTrainArgs& a, float lr, long long step, hipStream_t s)
 {
     using L = MlpLayout<D>;
     if (a.batch <= 0 || a.batch % kChunk != 0)
         throw std::invalid_argument("mlp: batch must be a positive multiple of " + std::to_string(kChunk));
+    if (Act<D>::total(a.batch) * 2 >= (1LL << 31))
+        throw std::invalid_argument("mlp: batch too large for one step (activations must stay below 2 GiB)");
     MlpView w;
     w.fwd = a.fwd;
     w.params = a.params;
-    const auto* bw3 = static_cast<const s16x4*>(a.bw);
-    const auto* bw2 = reinterpret_cast<const bf16x8*>(static_cast<const char*>(a.bw) + (long long)L::n_bw3_frags * 64 * 8);
-    fwd_bwd_kernel<D><<<(unsigned)(a.batch / 16), 64, 0, s>>>(w, bw3, bw2, a.X, a.T, a.idx, a.batch, a.act, a.loss);
-    const int n_chunks = (int)(a.batch / kChunk);
-    wgrad_kernel<D><<<dim3(Tiles<D>::n, n_chunks), 64, 0, s>>>(a.act, a.batch, a.partials);
+    if (a.batch >= 16384) {
+        constexpr int NT = D == 2 ? 2 : 1;
+        fwd_bwd_kernel<D, 4, NT><<<(unsigned)(a.batch / (64 * NT)), 256, 0, s>>>(w, a.bw, a.X, a.T, a.idx, a.batch,
+                                                                                  a.act, a.loss);
+    } else {
+        fwd_bwd_kernel<D, 1, 1><<<(unsigned)(a.batch / 16), 64, 0, s>>>(w, a.bw, a.X, a.T, a.idx, a.batch, a.act,
+                                                                         a.loss);
+    }
+    const int n_parts = wgrad_parts<D>(a.batch);
+    const int cpw = (int)(a.batch / kChunk) / n_parts;
+    wgrad_kernel<D><<<dim3(Tiles<D>::n, n_parts), 64, 0, s>>>(a.act, a.batch, cpw, a.partials);
     float bc1 = 1.f, bc2 = 1.f;
     if (lr != 0.f) {
         bc1 = 1.f - std::pow(0.9f, (float)step);
         bc2 = 1.f - std::pow(0.999f, (float)step);
     }
-    adam_kernel<D><<<(unsigned)((L::n_params + 255) / 256), 256, 0, s>>>(a.params, a.adam_m, a.adam_v, a.grad,
-                                                                           a.partials, n_chunks, lr, bc1, bc2);
-    if (lr != 0.f) pack_impl<D>(a.params, a.fwd, a.bw, s);
+    adam_kernel<D><<<(unsigned)((L::n_params + 255) / 256), 256, 0, s>>>(
+        a.params, a.adam_m, a.adam_v, a.grad, a.partials, n_parts, lr, bc1, bc2,
+        reinterpret_cast<__bf16*>(a.fwd), static_cast<__bf16*>(a.bw));
 }
 
 }  // namespace

@@ -81,6 +81,15 @@ __device__ __forceinline__ float gelu_grad(float x)
     return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * 0.7978845608f * (1.0f + 0.134145f * x * x);
 }
 
+// gelu(x) and gelu'(x) from one tanh (the training forward keeps only the
+// derivative for the backward pass)
+__device__ __forceinline__ float gelu_and_grad(float x, float& dgelu)
+{
+    const float t = tanh_fast(0.7978845608f * (x + 0.044715f * x * x * x));
+    dgelu = 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * 0.7978845608f * (1.0f + 0.134145f * x * x);
+    return 0.5f * x * (1.0f + t);
+}
+
 __device__ __forceinline__ f32x4 mfma32(const bf16x8& a, const bf16x8& b, const f32x4& c)
 {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);

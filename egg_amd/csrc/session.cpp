@@ -149,11 +149,30 @@ void Level::upload_tables()
     d_cp_dst.upload(cp_dst);
     d_mask.upload(mask);
     d_st_dof.upload(st_dof);
-    d_st_off.upload(st_off);
-    d_st_gc.upload(st_gc);
-    d_st_gn.upload(st_gn);
-    d_st_meta.upload(st_meta);
-    d_st_W.upload(st_W);
+    {
+        // stored occurrences in the padded lane layout of StoredView
+        const int G = stored_lanes(lay.dim), D = lay.dim;
+        const long long n = (long long)st_dof.size(), total = (long long)st_gc.size();
+        int rounds = 1;
+        for (long long d = 0; d < n; ++d) rounds = std::max(rounds, (st_off[d + 1] - st_off[d] + G - 1) / G);
+        st_rounds = rounds;
+        st_padded = n * rounds * G;
+        std::vector<int> gc(st_padded, 0), gn((size_t)D * st_padded, 0);
+        std::vector<uint8_t> meta(st_padded, kStoredPad);
+        std::vector<float> W(st_W.empty() ? 0 : (size_t)D * D * st_padded, 0.0f);
+        for (long long d = 0; d < n; ++d)
+            for (int p = st_off[d]; p < st_off[d + 1]; ++p) {
+                const long long q = d * rounds * G + (p - st_off[d]);
+                gc[q] = st_gc[p];
+                meta[q] = st_meta[p];
+                for (int k = 0; k < D; ++k) gn[(size_t)k * st_padded + q] = st_gn[(size_t)k * total + p];
+                for (int e = 0; e < D * D && !W.empty(); ++e) W[(size_t)D * D * q + e] = st_W[(size_t)D * D * p + e];
+            }
+        d_st_gc.upload(gc);
+        d_st_gn.upload(gn);
+        d_st_meta.upload(meta);
+        d_st_W.upload(W);
+    }
     // structured energy cells
     cell_off.assign(lay.shape.size() + 1, 0);
     for (size_t b = 0; b < lay.shape.size(); ++b) {
@@ -178,9 +197,9 @@ SweepArgs Level::sweep_args(const float* xin, float* xout, float omega, float de
     a.blocks = blocks.p;
     a.mask = d_mask.p;
     a.st.n = (int)st_dof.size();
+    a.st.rounds = st_rounds;
     a.st.dof = d_st_dof.p;
-    a.st.off = d_st_off.p;
-    a.st.total = (long long)st_gc.size();
+    a.st.total = st_padded;
     a.st.gc = d_st_gc.p;
     a.st.gn = d_st_gn.p;
     a.st.meta = d_st_meta.p;

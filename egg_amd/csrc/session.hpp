@@ -135,7 +135,9 @@ struct Level {
     std::vector<uint8_t> st_meta;
     std::vector<float> st_W;  // empty = uniform
     float Wu[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
-    DArr<int> d_st_dof, d_st_off, d_st_gc, d_st_gn;
+    DArr<int> d_st_dof, d_st_gc, d_st_gn;  // device copies in the padded lane layout (see StoredView)
+    int st_rounds = 1;
+    long long st_padded = 0;
     DArr<uint8_t> d_st_meta;
     DArr<float> d_st_W;
     // energy reduction: structured cells, or an explicit sample table

@@ -144,40 +144,61 @@ __device__ __forceinline__ void count_update(const SweepArgs& a, int ntr, bool p
 template <int D, int KIND, bool TAU, bool GRAD, int G>
 __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
 {
+    static_assert(G == stored_lanes(D), "stored lane group size");
     const StoredView& st = a.st;
     const float* X = a.xin;
     const long long PL = a.plane;
-    int dof = 0, p0 = 0, p1 = 0;
-    if (valid) {
-        dof = st.dof[d];
-        p0 = st.off[d];
-        p1 = st.off[d + 1];
-    }
+    const int R = st.rounds;
+    const int dof = valid ? st.dof[d] : 0;
     float pos[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) pos[k] = valid ? X[k * PL + dof] : 0.0f;
 
-    // builds T (and det A) of occurrence p, with the DOF moved to `sub` when given
-    auto build = [&](int p, const float* sub, float* T, float& det, float* s) {
-        const int m = st.meta[p];
-        const int ic = st.gc[p];
-        float cn[D], nb[D][D];
+    // One occurrence: its corner and neighbour positions and sign flips,
+    // which of them is the DOF itself, role and W row. Round 0 is loaded
+    // once and kept in registers for the line-search trials.
+    struct Occ {
+        float cn[D], nb[D][D], s[D];
+        bool cd, nd[D], live;
+        int role;
+        const float* w;
+    };
+    auto load = [&](int r, Occ& o) {
+        const long long q = ((long long)d * R + r) * G + l;
+        const int m = valid ? st.meta[q] : kStoredPad;
+        o.live = !(m & kStoredPad);
+        o.role = (m & 7) - 1;
+        o.w = st.W ? st.W + (long long)(D * D) * q : a.Wu;
+        if (!o.live) return;
+        const int ic = st.gc[q];
+        o.cd = ic == dof;
 #pragma unroll
-        for (int i = 0; i < D; ++i) cn[i] = (sub && ic == dof) ? sub[i] : X[i * PL + ic];
+        for (int i = 0; i < D; ++i) o.cn[i] = X[i * PL + ic];
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-            const int in = st.gn[k * st.total + p];
-            s[k] = (m & (8 << k)) ? -1.0f : 1.0f;
+            const int in = st.gn[k * st.total + q];
+            o.nd[k] = in == dof;
+            o.s[k] = (m & (8 << k)) ? -1.0f : 1.0f;
 #pragma unroll
-            for (int i = 0; i < D; ++i) nb[k][i] = (sub && in == dof) ? sub[i] : X[i * PL + in];
+            for (int i = 0; i < D; ++i) o.nb[k][i] = X[i * PL + in];
         }
+    };
+    // builds T (and det A) of an occurrence, with the DOF moved to `sub` when given
+    auto build = [&](const Occ& o, const float* sub, float* T, float& det) {
+        float cn[D], nb[D][D];
+#pragma unroll
+        for (int i = 0; i < D; ++i) cn[i] = (sub && o.cd) ? sub[i] : o.cn[i];
+#pragma unroll
+        for (int k = 0; k < D; ++k)
+#pragma unroll
+            for (int i = 0; i < D; ++i) nb[k][i] = (sub && o.nd[k]) ? sub[i] : o.nb[k][i];
         float A[D * D];
 #pragma unroll
         for (int i = 0; i < D; ++i)
 #pragma unroll
-            for (int k = 0; k < D; ++k) A[i * D + k] = s[k] * (nb[k][i] - cn[i]);
+            for (int k = 0; k < D; ++k) A[i * D + k] = o.s[k] * (nb[k][i] - cn[i]);
         det = detD<D>(A);
-        const float* w = st.W ? st.W + (long long)(D * D) * p : a.Wu;
+        const float* w = o.w;
 #pragma unroll
         for (int i = 0; i < D; ++i)
 #pragma unroll
@@ -189,24 +210,33 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
             }
     };
 
+    Occ o0;
+    load(0, o0);
     float g[D] = {}, h[kNH<D>] = {}, e = 0.0f, md = kInf;
-    for (int p = p0 + l; p < p1; p += G) {
-        float T[D * D], det, s[D];
-        build(p, nullptr, T, det, s);
+    for (int r = 0; r < R; ++r) {
+        Occ o = o0;
+        if (r > 0) load(r, o);
+        if (!o.live) break;  // padding (a DOF's occurrences come first)
+        float T[D * D], det;
+        build(o, nullptr, T, det);
         md = fminf(md, det);
-        const int role = (st.meta[p] & 7) - 1;
+        const int role = o.role;
         if (role >= 0) {
-            const float* w = st.W ? st.W + (long long)(D * D) * p : a.Wu;
+            const float* w = o.w;
+            float sr = o.s[0];  // sign of the role's axis (selected, not indexed)
+#pragma unroll
+            for (int k = 1; k < D; ++k)
+                if (role - 1 == k) sr = o.s[k];
             float u[D];
 #pragma unroll
             for (int c = 0; c < D; ++c) {
                 if (role == 0) {
                     float acc = 0.0f;
 #pragma unroll
-                    for (int k = 0; k < D; ++k) acc += s[k] * w[k * D + c];
+                    for (int k = 0; k < D; ++k) acc += o.s[k] * w[k * D + c];
                     u[c] = -acc;
                 } else {
-                    u[c] = s[role - 1] * w[(role - 1) * D + c];
+                    u[c] = sr * w[(role - 1) * D + c];
                 }
             }
             e += sample_grad_hess<D, KIND>(T, u, a.delta, g, h);
@@ -246,9 +276,12 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
 #pragma unroll
         for (int k = 0; k < D; ++k) tr[k] = pos[k] + alpha * dl[k];
         float et = 0.0f, mt = kInf;
-        for (int p = p0 + l; p < p1; p += G) {
-            float T[D * D], det, s[D];
-            build(p, tr, T, det, s);
+        for (int r = 0; r < R; ++r) {
+            Occ o = o0;
+            if (r > 0) load(r, o);
+            if (!o.live) break;
+            float T[D * D], det;
+            build(o, tr, T, det);
             et += sample_value<D, KIND>(T, a.delta);
             mt = fminf(mt, det);
         }
@@ -294,26 +327,8 @@ __device__ void tile_sweep(const SweepArgs& a, int tile)
     const long long PL = a.plane;
     const int tid = threadIdx.x;
 
-    // ---- stage the tile + one-node halo (logical -1 .. n: padded ghosts included)
-    if constexpr (LDS)
-    for (int e = tid; e < LZ * LY * LX; e += NT) {
-        const int c = e % LX, q = (e / LX) % LY, r = e / (LX * LY);
-        long long node;
-        if constexpr (D == 2) {
-            const int li = min(td.i0 - 1 + q, bd.n[0]);
-            const int lj = min(td.j0 - 1 + c, bd.n[1]);
-            node = bd.base + (long long)(li + 1) * bd.st[0] + (lj + 1);
-        } else {
-            const int l0 = min(td.i0 - 1 + r, bd.n[0]);
-            const int l1 = min(td.j0 - 1 + q, bd.n[1]);
-            const int l2 = min(td.k0 - 1 + c, bd.n[2]);
-            node = bd.base + (long long)(l0 + 1) * bd.st[0] + (long long)(l1 + 1) * bd.st[1] + (l2 + 1);
-        }
-#pragma unroll
-        for (int k = 0; k < D; ++k) sh[k][e] = a.xin[k * PL + node];
-    }
-    if constexpr (LDS) __syncthreads();
-
+    // this lane's node and its mask first: the mask load then overlaps the
+    // tile staging instead of following the barrier
     const int cell = tid & (NC - 1);
     const int nid = tid / NC;
     const int tx = nid % TX, ty = (nid / TX) % TY, tz = nid / (TX * TY);
@@ -335,6 +350,27 @@ __device__ void tile_sweep(const SweepArgs& a, int tile)
                (long long)(min(lg[1], bd.n[1] - 1) + 1) * bd.st[1] + (min(lg[2], bd.n[2] - 1) + 1);
     }
     const bool active = inside && a.mask[node];
+
+    // ---- stage the tile + one-node halo (logical -1 .. n: padded ghosts included)
+    if constexpr (LDS)
+    for (int e = tid; e < LZ * LY * LX; e += NT) {
+        const int c = e % LX, q = (e / LX) % LY, r = e / (LX * LY);
+        long long node;
+        if constexpr (D == 2) {
+            const int li = min(td.i0 - 1 + q, bd.n[0]);
+            const int lj = min(td.j0 - 1 + c, bd.n[1]);
+            node = bd.base + (long long)(li + 1) * bd.st[0] + (lj + 1);
+        } else {
+            const int l0 = min(td.i0 - 1 + r, bd.n[0]);
+            const int l1 = min(td.j0 - 1 + q, bd.n[1]);
+            const int l2 = min(td.k0 - 1 + c, bd.n[2]);
+            node = bd.base + (long long)(l0 + 1) * bd.st[0] + (long long)(l1 + 1) * bd.st[1] + (l2 + 1);
+        }
+#pragma unroll
+        for (int k = 0; k < D; ++k) sh[k][e] = a.xin[k * PL + node];
+    }
+    if constexpr (LDS) __syncthreads();
+
     // whole lane groups leave together; the proposal variant keeps every lane
     // until its matrix-core forward pass is done (converged wave)
     if constexpr (!PROP) {

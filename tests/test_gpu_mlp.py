@@ -71,6 +71,38 @@ def test_gradient_matches_oracle(dim):
     assert np.linalg.norm(g - g64) <= 5e-2 * np.linalg.norm(g64)
 
 
+@pytest.mark.parametrize("dim,n", [(2, 32768), (3, 16384)])
+def test_large_batch_gradient_matches_oracle(dim, n):
+    """Batches whose weight-gradient waves sum several 256-sample chunks."""
+    rng = np.random.default_rng(40 + dim)
+    net = _net(dim, capacity=n)
+    p = (rng.standard_normal(M.n_params(dim)) * 0.2).astype(np.float32)
+    net.set_params(p)
+    X, T = _data(dim, n, rng)
+    net.data_add(X, T)
+    rows = rng.permutation(n).astype(np.int32)
+    g, loss = net.gradient(rows)
+    lo, go = M.loss_grad(p, X[rows], T[rows], dim)
+    assert loss == pytest.approx(lo, rel=1e-3)
+    assert np.linalg.norm(g - go) <= 1e-2 * np.linalg.norm(go)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_training_refreshes_fragments(dim):
+    """The Adam step rewrites the BF16 fragments of every parameter it moves:
+    the trained network's forward equals the forward after repacking its
+    FP32 parameters from scratch."""
+    rng = np.random.default_rng(50 + dim)
+    net = _net(dim, capacity=4096, seed=5)
+    X, T = _data(dim, 4096, rng)
+    net.data_add(X, T)
+    net.train(2, batch=1024, lr=1e-2, seed=2)
+    Y1 = net.forward(X[:2048])
+    net.set_params(net.get_params())
+    Y2 = net.forward(X[:2048])
+    np.testing.assert_array_equal(Y1, Y2)
+
+
 def test_training_reduces_loss():
     rng = np.random.default_rng(30)
     net = _net(2, capacity=16384, seed=3)
