@@ -140,6 +140,93 @@ def _converge(step, e_star, budget_s):
     return None, time.perf_counter() - t0, float(e[-1])
 
 
+def _parity(Xg, Xo, eg, eo, mg, mo, extent, **extra):
+    return {"max_abs_pos_err": float(np.abs(Xg - Xo).max()),
+            "max_abs_pos_err_rel_extent": float(np.abs(Xg - Xo).max() / extent),
+            "rel_energy_err": float(abs(eg - eo) / abs(eo)), "min_det_err": float(abs(mg - mo)),
+            "oracle": "FP64 CPU oracle (oracle/egg_oracle.c)", **extra}
+
+
+def _batched_sample(s, g, index, Xu, n_sweeps, target=300_000):
+    """Parity and CPU baseline of the batched workload on one complete grid
+    of the batch (the one closest to `target` free DOFs): the grids are
+    independent, so the batch's sweeps restricted to that grid are that
+    grid's own sweeps. The GPU runs the whole batch from Xu."""
+    from egg_amd import grids
+    from egg_amd.smoothing import build_sweep_context
+    from oracle import oracle as O  # the checker: CPU baseline and parity reference
+
+    frees = [int(g.free[n0:n0 + nn].sum()) for (_, _, n0, nn) in index]
+    gi = int(np.argmin([abs(f - target) for f in frees]))
+    b0, nb, n0, nn = index[gi]
+    sub = grids.Grid(2, [b - n0 for b in g.blocks[b0:b0 + nb]], Xu[n0:n0 + nn].copy(), g.free[n0:n0 + nn].copy())
+    ctx = build_sweep_context(sub.blocks, sub.free, 2)
+    patches = O.materialize_patches(ctx.wire, sub.blocks, 2)
+    threads = _threads()
+    rates, outs = {}, {}
+    for prec in ("f64", "f32"):
+        t0 = time.perf_counter()
+        outs[prec] = O.jacobi(2, "barrier", 0.0, 0.8, n_sweeps, sub.X, patches, ctx.energy_stencil, prec=prec,
+                              threads=threads)
+        rates[prec] = frees[gi] * n_sweeps / (time.perf_counter() - t0)
+    best = max(rates, key=rates.get)
+    cpu = {"value": rates[best], "unit": "node-updates/s", "cores": threads, "kind": "port",
+           "sample": f"{n_sweeps} sweeps of grid {gi} of the batch ({nb} blocks, {frees[gi]} free DOFs) from the "
+                     f"untangled state, OpenMP C oracle {best.upper()} (f64 {rates['f64']:.4g}, "
+                     f"f32 {rates['f32']:.4g} node-updates/s)"}
+    X0 = s.get_X()
+    s.set_X(Xu)
+    s.run(n_sweeps, omega=0.8)
+    Xg = s.get_X()[n0:n0 + nn]
+    s.set_X(X0)
+    Xo, eo, mo = outs["f64"]
+    eg, mg = O.energy_mindet(2, "barrier", 0.0, ctx.energy_stencil, Xg)  # the GPU state, FP64 on the CPU
+    extent = float((sub.X.max(0) - sub.X.min(0)).max())
+    par = _parity(Xg, Xo, eg, eo[-1], mg, mo[-1], extent, sweeps=n_sweeps, grid=gi, free_dofs=frees[gi],
+                  energy_of_gpu_state="FP64 on the CPU over that grid's cells")
+    return cpu, par
+
+
+def _cubed_sample(s, g, ctx, Xu, every=40):
+    """Parity and CPU baseline of the 3D workload on a DOF sample (every
+    `every`-th DOF of the wire): one Jacobi sweep moves each sampled DOF
+    exactly as the full sweep does (frozen neighbours), so one GPU sweep of
+    the whole grid is compared on the sample; the energy and min det of the
+    GPU state are recomputed in FP64 on the CPU over the whole grid."""
+    from oracle import oracle as O  # the checker: CPU baseline and parity reference
+
+    nd = len(ctx.wire["groups"][0]["dof_idx"])
+    sample = np.arange(0, nd, every)
+    patches = O.materialize_patches(ctx.wire, g.blocks, 3, dofs=sample)
+    threads = _threads()
+    # an empty energy stencil: the sweep's own report would sum the whole grid
+    no_report = {"num_samples": 0, "gc": np.zeros(0, np.int32), "W_inv": np.eye(3),
+                 **{f"gn{k}": np.zeros(0, np.int32) for k in range(3)}, **{f"s{k}": np.zeros(0) for k in range(3)}}
+    rates, outs = {}, {}
+    for prec in ("f64", "f32"):
+        t0 = time.perf_counter()
+        outs[prec] = O.jacobi(3, "barrier", 0.0, 0.8, 1, Xu, patches, no_report, prec=prec, threads=threads)
+        rates[prec] = len(sample) / (time.perf_counter() - t0)
+    best = max(rates, key=rates.get)
+    cpu = {"value": rates[best], "unit": "node-updates/s", "cores": threads, "kind": "port",
+           "sample": f"1 sweep of {len(sample)} DOFs (every {every}th) of the same grid from the untangled state, "
+                     f"OpenMP C oracle {best.upper()} (f64 {rates['f64']:.4g}, f32 {rates['f32']:.4g} "
+                     "node-updates/s)"}
+    X0 = s.get_X()
+    s.set_X(Xu)
+    eg_gpu, mg_gpu = s.run(1, omega=0.8)
+    Xg = s.get_X()
+    s.set_X(X0)
+    dofs = np.asarray(ctx.wire["groups"][0]["dof_idx"])[sample]
+    Xo = outs["f64"][0]
+    ec, mc = O.energy_mindet(3, "barrier", 0.0, ctx.energy_stencil, Xg)
+    extent = float((g.X.max(0) - g.X.min(0)).max())
+    par = _parity(Xg[dofs], Xo[dofs], float(eg_gpu[-1]), ec, float(mg_gpu[-1]), mc, extent, sweeps=1,
+                  sampled_dofs=int(len(sample)),
+                  energy="GPU FP64 reduction vs the FP64 CPU recomputation of the same state")
+    return cpu, par
+
+
 def _train_proposal(make_grid, seeds, horizon=2, snaps=40, every=50, epochs=10):
     from egg_amd._cpp import cpp_core
 
@@ -315,6 +402,11 @@ def bench_batched(args):
     allv = sum(counts) * K / t_fas
     log(f"FAS V(4,4) x{K}: {t_fas * 1e3:.1f} ms, fine {fine:.4g} / all-level {allv:.4g} node-updates/s, "
         f"E {e[0]:.6g} -> {e[-1]:.6g}")
+    cpu = parity = None
+    if not args.no_cpu:
+        cpu, parity = _batched_sample(s, g, index, Xu, 50)
+        log(f"cpu {cpu}")
+        log(f"parity {parity}")
     return {
         "metric": METRIC, "value": fine, "unit": "node-updates/s", "n_gpus": 1, "steps": K, "warmup": 0,
         "ms_per_step": t_fas / K * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -328,7 +420,7 @@ def bench_batched(args):
         "autotune": {"best_variant": int(best), "ms_per_variant": [float(v) for v in ms]},
         "time_to_untangled_s": t_untangle, "untangle_outer": outer,
         "energy_first_last": [float(e[0]), float(e[-1])], "final_min_det": float(m[-1]),
-        "roofline": roof, "cpu_baseline": None, "parity": "tests/test_gpu_batched.py (reduced batch)",
+        "roofline": roof, "cpu_baseline": cpu, "parity": parity,
     }
 
 
@@ -364,6 +456,11 @@ def bench_cubed(args):
     x, iters, relres, status = pcg.pcg_solve(rp, col, val, b, rtol=1e-8, max_iter=1000)
     pcg_ms = cpp_core.last_pcg_ms()
     log(f"PCG 33^3: {iters} iterations, relres {relres:.2e}, {pcg_ms:.3f} ms")
+    cpu = parity = None
+    if not args.no_cpu:
+        cpu, parity = _cubed_sample(s, g, ctx, Xu)
+        log(f"cpu {cpu}")
+        log(f"parity {parity}")
     return {
         "metric": METRIC, "value": value, "unit": "node-updates/s", "n_gpus": 1, "steps": n3,
         "warmup": 0, "ms_per_step": t_plain / n3 * 1e3, "higher_is_better": True,
@@ -374,7 +471,8 @@ def bench_cubed(args):
                              "energy_first_last": [float(ef[0]), float(ef[-1])]},
         "pcg_33cubed": {"iterations": int(iters), "relres": float(relres), "status": int(status),
                         "solve_ms": pcg_ms},
-        "time_to_untangled_s": t_untangle, "untangle_outer": outer, "roofline": roof, "cpu_baseline": None,
+        "time_to_untangled_s": t_untangle, "untangle_outer": outer, "roofline": roof, "cpu_baseline": cpu,
+        "parity": parity,
     }
 
 

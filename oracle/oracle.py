@@ -187,11 +187,78 @@ class Patches:
         self.W = np.ascontiguousarray(W, np.float64)  # (dd,) uniform or (total, dd)
 
 
-def materialize_patches(wire: dict, block_global_dof: list[np.ndarray], dim: int) -> Patches:
+def materialize_patches(wire: dict, block_global_dof: list[np.ndarray], dim: int, dofs=None) -> Patches:
     """Expand the wire's single group into explicit per-DOF patches: stored
-    occurrences for DOFs with P_of > 0, synthesized ones (reference order)
-    for interior DOFs (interior_block >= 0). Uniform W_inv only for the
-    synthesized case, as in the reference (identity)."""
+    occurrences for DOFs with P_of > 0, synthesized ones (reference order,
+    ``interior_occurrences``) for interior DOFs (interior_block >= 0).
+    Uniform W_inv only for the synthesized case, as in the reference
+    (identity). The interior expansion is the per-DOF rule of
+    ``interior_occurrences`` evaluated for all DOFs of a block at once
+    (``tests/test_oracle.py`` checks the two agree). ``dofs`` (positions in
+    the group's DOF list, ascending) restricts the expansion to a sample: a
+    Jacobi sweep of the sample moves exactly those DOFs as the full sweep
+    would."""
+    g = wire["groups"][0]
+    P_all = np.asarray(g["P_of"], np.int64)
+    soff_all = np.concatenate([[0], np.cumsum(P_all)])
+    sel = np.arange(len(P_all)) if dofs is None else np.asarray(dofs, np.int64)
+    dof = np.asarray(g["dof_idx"], np.int32)[sel]
+    n = len(dof)
+    P_of = P_all[sel]
+    iblk = np.asarray(g["interior_block"], np.int32)[sel]
+    ilog = np.asarray(g["interior_logical"], np.int64).reshape(-1, dim)[sel]
+    W = np.asarray(g["W_inv"], np.float64)
+    uniform = W.size == dim * dim
+    nc = 1 << dim
+    interior = iblk >= 0
+    if interior.any() and not uniform:
+        raise ValueError("interior synthesis requires a uniform W_inv")
+    cnt = np.where(interior, nc * nc, P_of)
+    poff = np.concatenate([[0], np.cumsum(cnt)]).astype(np.int64)
+    total = int(poff[-1])
+    gc = np.zeros(total, np.int64)
+    gn = np.zeros((dim, total), np.int64)
+    sg = np.zeros((dim, total), np.float64)
+    role = np.zeros(total, np.int64)
+    # stored DOFs: their wire occurrences in order
+    st = np.nonzero(~interior)[0]
+    soff = soff_all[sel]
+    if len(st):
+        src = np.concatenate([np.arange(soff[d], soff[d] + P_of[d]) for d in st]).astype(np.int64)
+        dst = np.concatenate([np.arange(poff[d], poff[d + 1]) for d in st]).astype(np.int64)
+        gc[dst] = np.asarray(g["gc"])[src]
+        role[dst] = np.asarray(g["role"])[src]
+        for k in range(dim):
+            gn[k, dst] = np.asarray(g[f"gn{k}"])[src]
+            sg[k, dst] = np.asarray(g[f"s{k}"])[src]
+    # interior DOFs, block by block, occurrence by occurrence
+    for b in np.unique(iblk[interior]):
+        ds = np.nonzero(iblk == b)[0]
+        ids = np.asarray(block_global_dof[b])
+        lg = ilog[ds]
+        for occ in range(nc * nc):
+            cell, o = divmod(occ, nc)
+            obit = [(o >> (dim - 1 - k)) & 1 for k in range(dim)]
+            corner = lg - 1 + np.array([((cell >> (dim - 1 - k)) & 1) + obit[k] for k in range(dim)])
+            pos = poff[ds] + occ
+            gc[pos] = ids[tuple(corner.T)]
+            r = np.where(np.all(corner == lg, axis=1), 0, -1)
+            for k in range(dim):
+                nbr = corner.copy()
+                nbr[:, k] += 1 if obit[k] == 0 else -1
+                gn[k, pos] = ids[tuple(nbr.T)]
+                sg[k, pos] = 1.0 if obit[k] == 0 else -1.0
+                r = np.where((r < 0) & np.all(nbr == lg, axis=1), k + 1, r)
+            role[pos] = r
+    if uniform:
+        Wp = W.reshape(-1)
+    else:
+        Wp = np.concatenate([W.reshape(-1, dim * dim)[soff[d]:soff[d] + P_of[d]] for d in range(n)])
+    return Patches(dim, dof, poff, gc, gn, sg, role, Wp)
+
+
+def materialize_patches_loop(wire: dict, block_global_dof: list[np.ndarray], dim: int) -> Patches:
+    """DOF-by-DOF form of materialize_patches (small grids; the check on it)."""
     g = wire["groups"][0]
     dof = np.asarray(g["dof_idx"], np.int32)
     P_of = np.asarray(g["P_of"], np.int64)

@@ -164,6 +164,20 @@ def test_synthesized_patch_order(oracle):
     np.testing.assert_array_equal(np.asarray(g["role"]), synth.role)
 
 
+def test_patch_expansion_forms_agree(oracle):
+    """The block-at-once patch expansion equals the DOF-by-DOF one built on
+    interior_occurrences (stored interface DOFs and interior DOFs, 2D and 3D)."""
+    from egg_amd import grids
+    from tests.cases import Case, ogrid
+
+    for c in (ogrid(n=17, m=9), Case(grids.cubed_sphere_3d(9))):
+        a = oracle.materialize_patches(c.ctx.wire, c.grid.blocks, c.d)
+        b = oracle.materialize_patches_loop(c.ctx.wire, c.grid.blocks, c.d)
+        assert np.any(np.diff(a.poff) != (1 << c.d) ** 2)  # stored DOFs present
+        for f in ("dof", "poff", "gc", "gn", "s", "role", "W"):
+            np.testing.assert_array_equal(getattr(a, f), getattr(b, f), err_msg=f)
+
+
 @pytest.mark.parametrize("prec,atol", [("f64", 1e-9), ("f32", 2e-5)])
 def test_jacobi_analytic_minimiser(oracle, prec, atol):
     """A perturbed Cartesian block minimises to the uniform grid (reference
