@@ -149,6 +149,39 @@ __device__ __forceinline__ void store_tile(__bf16* buf, unsigned lane_off, long 
     for (int i = 0; i < 4; ++i) *reinterpret_cast<__bf16*>(row + (lane_off + i * rs)) = (__bf16)v[i];
 }
 
+// Keyed permutation of [0, n): a 4-round balanced Feistel network on
+// 2^bits >= n values (bits even; the round function is a 32-bit integer
+// hash of the half and the round key), applied until the value falls in
+// [0, n) (cycle walking; at most 4 n / n expected rounds of walking since
+// 2^bits < 4 n).
+__host__ __device__ inline unsigned mix32(unsigned x)
+{
+    x ^= x >> 16;
+    x *= 0x7feb352dU;
+    x ^= x >> 15;
+    x *= 0x846ca68bU;
+    x ^= x >> 16;
+    return x;
+}
+__host__ __device__ inline long long feistel_perm(long long i, long long n, int bits, unsigned long long key)
+{
+    const int h = bits / 2;
+    const unsigned mask = (1u << h) - 1u;
+    unsigned v = (unsigned)i;
+    do {
+        unsigned l = v >> h, r = v & mask;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const unsigned f = mix32(r ^ (unsigned)(key >> (16 * k)) ^ (0x9e3779b9U * (unsigned)(k + 1))) & mask;
+            const unsigned t = l ^ f;
+            l = r;
+            r = t;
+        }
+        v = (l << h) | r;
+    } while ((long long)v >= n);
+    return v;
+}
+
 // Training forward/backward: a workgroup of WAVES waves stages every weight
 // fragment (forward, W3^T, W2^T) in LDS once (all loads in flight together);
 // each wave then takes NT tiles of 16 samples, so each fragment read feeds
@@ -162,7 +195,9 @@ template <int D, int WAVES, int NT>
 __global__ __launch_bounds__(64 * WAVES) void fwd_bwd_kernel(MlpView w, const void* __restrict__ bw,
                                                                    const __bf16* __restrict__ X,
                                                                    const float* __restrict__ T,
-                                                                   const int* __restrict__ idx, long long B,
+                                                                   const int* __restrict__ idx, long long pn,
+                                                                   long long pbase, int pbits,
+                                                                   unsigned long long pkey, long long B,
                                                                    __bf16* __restrict__ act,
                                                                    double* __restrict__ loss)
 {
@@ -193,7 +228,7 @@ __global__ __launch_bounds__(64 * WAVES) void fwd_bwd_kernel(MlpView w, const vo
         s[t] = (((long long)blockIdx.x * WAVES + wave) * NT + t) * 16 + (lane & 15);  // batch row
         off[t] = (unsigned)(((long long)rg * B + s[t]) * 2);
         xoff[t] = (unsigned)(((long long)(8 * g) * B + s[t]) * 2);
-        src[t] = idx[s[t]];
+        src[t] = idx ? idx[s[t]] : feistel_perm(pbase + s[t], pn, pbits, pkey);
         load_x<D>(X, src[t], true, xin[t]);
     }
     // the batch copy of the inputs (for dW1), feature-major
@@ -535,10 +570,14 @@ template <int D> void step_impl(const MlpTrainArgs& a, float lr, long long step,
     w.params = a.params;
     if (a.batch >= 16384) {
         constexpr int NT = D == 2 ? 2 : 1;
-        fwd_bwd_kernel<D, 4, NT><<<(unsigned)(a.batch / (64 * NT)), 256, 0, s>>>(w, a.bw, a.X, a.T, a.idx, a.batch,
+        fwd_bwd_kernel<D, 4, NT><<<(unsigned)(a.batch / (64 * NT)), 256, 0, s>>>(w, a.bw, a.X, a.T, a.idx, a.perm_n,
+                                                                                  a.perm_base, a.perm_bits,
+                                                                                  a.perm_key, a.batch,
                                                                                   a.act, a.loss);
     } else {
-        fwd_bwd_kernel<D, 1, 1><<<(unsigned)(a.batch / 16), 64, 0, s>>>(w, a.bw, a.X, a.T, a.idx, a.batch, a.act,
+        fwd_bwd_kernel<D, 1, 1><<<(unsigned)(a.batch / 16), 64, 0, s>>>(w, a.bw, a.X, a.T, a.idx, a.perm_n,
+                                                                         a.perm_base, a.perm_bits, a.perm_key,
+                                                                         a.batch, a.act,
                                                                          a.loss);
     }
     const int n_parts = wgrad_parts<D>(a.batch);

@@ -306,7 +306,13 @@ struct MlpTrainArgs {
     long long batch;            // samples per step (multiple of 16)
     const __bf16* X;            // dataset features [n][INP]
     const float* T;             // dataset targets [n][4]
-    const int* idx;             // [batch] sample ids of this step
+    const int* idx;             // [batch] sample ids of this step, or null: a permutation (below)
+    // idx == null: sample id of batch row r is perm(perm_base + r), a keyed
+    // permutation of [0, perm_n) (a balanced Feistel network on 2^perm_bits
+    // >= perm_n values, cycle-walked into range), one per epoch
+    long long perm_n = 0, perm_base = 0;
+    int perm_bits = 0;
+    unsigned long long perm_key = 0;
     float* params;              // FP32 master
     float* adam_m;
     float* adam_v;

@@ -223,19 +223,24 @@ std::vector<double> Mlp::train(int epochs, long long batch, float lr, uint64_t s
     if (!(lr > 0.0f) || !std::isfinite(lr)) throw InvalidArgument("mlp: lr must be positive");
     const long long steps = n / batch;
     if (steps == 0) throw InvalidArgument("mlp: training set smaller than one batch");
+    if (n >= (1LL << 31)) throw InvalidArgument("mlp: training set too large");
     std::vector<double> hist;
-    std::vector<int> perm((size_t)n);
+    // each epoch visits the set in a fresh keyed permutation computed on the
+    // device (no host shuffle or index upload)
+    int bits = 2;
+    while ((1LL << bits) < n) bits += 2;
     std::mt19937_64 gen(seed);
     loss.alloc(steps * (batch / 16));
     std::vector<double> l((size_t)(steps * (batch / 16)));
     for (int ep = 0; ep < epochs; ++ep) {
-        std::iota(perm.begin(), perm.end(), 0);
-        std::shuffle(perm.begin(), perm.end(), gen);
-        EGGMI_HIP(hipStreamSynchronize(stream));
-        idx.upload(perm);
+        const unsigned long long key = gen();
         for (long long st = 0; st < steps; ++st) {
             MlpTrainArgs a = args(batch);
-            a.idx = idx.p + st * batch;
+            a.idx = nullptr;
+            a.perm_n = n;
+            a.perm_base = st * batch;
+            a.perm_bits = bits;
+            a.perm_key = key;
             a.loss = loss.p + st * (batch / 16);
             mlp_step(a, lr, ++adam_step, stream);
         }
