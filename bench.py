@@ -58,11 +58,14 @@ GRAPH_CHUNK = 256  # sweeps per captured hipGraph (5-8 % per sweep on the small 
 # cubed_sphere65_fas_pcg.md). A device copy of N bytes reads FETCH_SIZE =
 # N/2 and WRITE_SIZE = N on this part, so FETCH is scaled by 2 and WRITE by 1.
 PMC_KIB = {
-    ("ogrid", 0): (1318.75, 548.44), ("ogrid", 1): (1472.38, 616.38), ("ogrid", 2): (1224.0, 517.69),
-    ("ogrid", 3): (1067.06, 538.5),
-    ("batched", 0): (591180.7, 277104.7), ("batched", 1): (762148.9, 325449.7),
-    ("batched", 2): (526885.0, 252865.4), ("batched", 3): (508806.1, 276764.8),
-    ("cubed_sphere", 0): (137175.0, 49892.4),
+    ("ogrid", 0): (1313.06, 548.44), ("ogrid", 1): (1465.69, 616.38), ("ogrid", 2): (1217.31, 517.69),
+    ("ogrid", 3): (1072.38, 538.5), ("ogrid", 4): (1845.81, 617.75), ("ogrid", 5): (2064.19, 784.62),
+    ("ogrid", 6): (1213.56, 517.69), ("ogrid", 7): (1305.31, 548.44),
+    ("batched", 0): (590517.16, 277104.75), ("batched", 1): (761483.12, 325449.75),
+    ("batched", 2): (526221.81, 252865.33), ("batched", 3): (508322.66, 276764.88),
+    ("batched", 4): (1073575.91, 325607.78), ("batched", 5): (1284411.69, 422312.41),
+    ("batched", 6): (525394.81, 252865.34), ("batched", 7): (589009.12, 277104.78),
+    ("cubed_sphere", 0): (135326.31, 49891.59),
 }
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 
