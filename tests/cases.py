@@ -134,3 +134,51 @@ def halo_pair_expected(n0=4, n1=3):
         out[(0, (n0 + 1, j + 1))] = (float(n0), float(j))
         out[(1, (0, j + 1))] = (float(n0 - 2), float(j))
     return out
+
+
+def star(k=5, n=9, layers=0, seed=7, amp=0.2):
+    """k quadrilateral blocks around a point (a regular k-gon cut by its
+    centre-to-edge-midpoint spokes; block i spans spokes i and i+1 and holds
+    vertex i+1), n x n nodes each, bilinear. The centre is a singular node
+    of valence k: its patch has 4k occurrences (20 for k = 5, more than one
+    round of a 16-lane group). layers > 0 extrudes it into a hex grid of
+    `layers` + 1 node planes; the centre line then has 2k hexes around each
+    interior node (8 * 2k occurrences: 80 for k = 5, two rounds of 64).
+    Interior and spoke nodes are free (the k-gon's edges and the end planes
+    are fixed) and moved by U(+-amp h)."""
+    from egg_amd.grids import Grid, _merge_fast, corner_dets
+
+    th = 2 * np.pi * np.arange(k + 1) / k
+    V = np.stack([np.cos(th), np.sin(th)], -1)
+    M = 0.5 * (V[:-1] + V[1:])  # M[i]: midpoint of edge (V_i, V_{i+1})
+    M = np.concatenate([M, M[:1]])
+    C = np.zeros(2)
+    a = np.linspace(0.0, 1.0, n)
+    A, B = np.meshgrid(a, a, indexing="ij")
+    coords = []
+    for i in range(k):
+        P = ((1 - A) * (1 - B))[..., None] * C + (A * (1 - B))[..., None] * M[i] \
+            + (A * B)[..., None] * V[i + 1] + ((1 - A) * B)[..., None] * M[i + 1]
+        if layers:
+            z = np.linspace(0.0, 0.5, layers + 1)
+            P = np.concatenate([np.repeat(P[:, :, None, :], layers + 1, 2),
+                                np.broadcast_to(z[None, None, :, None], (n, n, layers + 1, 1))], -1)
+        coords.append(P)
+    d = 3 if layers else 2
+    ids, X = _merge_fast(coords, d)
+    fixed = np.zeros(X.shape[0], bool)
+    for b in ids:
+        fixed[b[-1].reshape(-1)] = True       # a = 1: edge M_i -> V_{i+1}
+        fixed[b[:, -1].reshape(-1)] = True    # b = 1: edge M_{i+1} -> V_{i+1}
+        if layers:
+            fixed[b[:, :, 0].reshape(-1)] = True
+            fixed[b[:, :, -1].reshape(-1)] = True
+    free = ~fixed
+    h = 0.5 / (n - 1)
+    rng = np.random.default_rng(seed)
+    for _ in range(100):
+        Xp = X.copy()
+        Xp[free] += rng.uniform(-amp * h, amp * h, size=(int(free.sum()), d))
+        if corner_dets(ids, Xp, d).min() > 0:
+            return Case(Grid(d, ids, Xp, free))
+    raise RuntimeError("could not draw a valid perturbation")

@@ -16,7 +16,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from tests.cases import Case, cube, lr_pair, ogrid, single
+from tests.cases import Case, cube, lr_pair, ogrid, single, star
 
 pytestmark = pytest.mark.gpu
 
@@ -89,6 +89,26 @@ def test_multiblock_parity(oracle, make):
     assert err < POS_TOL
     np.testing.assert_allclose(e, eo, rtol=TRAJ_RTOL)
     np.testing.assert_allclose(e[-1], eo[-1], rtol=ENERGY_RTOL)
+
+
+@pytest.mark.parametrize("kw,phase", [(dict(k=5, n=9), "barrier"), (dict(k=6, n=9), "barrier"),
+                                      (dict(k=5, n=9), "untangle"), (dict(k=5, n=7, layers=6), "barrier")])
+def test_many_occurrence_stored_dofs(oracle, kw, phase):
+    """Singular DOFs whose patches take more than one round of their lane
+    group: the centre of a valence-5 / 6 star (20 / 24 occurrences for 16
+    lanes) and the centre line of the extruded star (80 occurrences for 64
+    lanes), against the FP64 oracle."""
+    c = star(**kw)
+    info = _session(c).core.info()
+    assert info["n_stored"] > 0
+    delta = 1e-3 if phase == "untangle" else 0.0
+    _, Xg, e, m, Xo, eo, mo = _run_both(oracle, c, 20, phase=phase, delta=delta)
+    err = np.abs(Xg - Xo).max() / _extent(c)
+    print(f"star {kw} {phase}: max|dX|/extent={err:.2e}")
+    assert err < POS_TOL
+    np.testing.assert_allclose(e, eo, rtol=TRAJ_RTOL)
+    np.testing.assert_allclose(e[-1], eo[-1], rtol=ENERGY_RTOL)
+    np.testing.assert_allclose(m, mo, rtol=1e-3)
 
 
 def test_stored_path_and_sample_reduction(oracle):

@@ -10,7 +10,7 @@ from pathlib import Path
 import numpy as np
 import pytest
 
-from tests.cases import halo_pair, halo_pair_expected, lr_pair, ogrid, single
+from tests.cases import halo_pair, halo_pair_expected, lr_pair, ogrid, single, star
 
 ROOT = Path(__file__).resolve().parent.parent
 
@@ -58,7 +58,8 @@ def _tables(case, device="tables"):
     return s.info(), s.tables()
 
 
-@pytest.mark.parametrize("make", [lambda: single(n=9), lambda: lr_pair(9), lambda: ogrid(17, 9)])
+@pytest.mark.parametrize("make", [lambda: single(n=9), lambda: lr_pair(9), lambda: ogrid(17, 9),
+                                  lambda: star(k=5, n=7), lambda: star(k=5, n=5, layers=4)])
 def test_remap_tables_match_oracle(make):
     """Copy tables (halo | fan | share), g2s and block bases equal the
     restatement of the reference's apply_structured_remap, entry by entry."""
