@@ -441,12 +441,15 @@ def bench_cubed(args):
     Xu, md_u, outer, _ = cpp_untangle(ctx, bsc, g.X, session=s)
     t_untangle = time.perf_counter() - t0
     log(f"untangle {t_untangle:.3f}s outer={outer} min_det={md_u:.3e}")
+    t0 = time.perf_counter()
+    best, ms = s.core.autotune(3, 0.8)
+    log(f"autotune {time.perf_counter() - t0:.2f}s best={best} ms/3 sweeps={ms}")
     n3 = args.steps // 20 or 1
     e, m, t_plain, k_avg, X_start = _timed(s, n3, 0.8)
     value = n_free * n3 / t_plain
     X_end = s.get_X()
     s.set_X(X_start)
-    roof = _roofline(s, 3, n_free, n3, 0.8, k_avg, _traffic("cubed_sphere", 0))
+    roof = _roofline(s, 3, n_free, n3, 0.8, k_avg, _traffic("cubed_sphere", best))
     s.set_X(X_end)
     counts = _fas_levels_counts(s, 4, 4, 3)
     K = 5
@@ -474,6 +477,7 @@ def bench_cubed(args):
                              "energy_first_last": [float(ef[0]), float(ef[-1])]},
         "pcg_33cubed": {"iterations": int(iters), "relres": float(relres), "status": int(status),
                         "solve_ms": pcg_ms},
+        "autotune": {"best_variant": int(best), "ms_per_variant": [float(v) for v in ms]},
         "time_to_untangled_s": t_untangle, "untangle_outer": outer, "roofline": roof, "cpu_baseline": cpu,
         "parity": parity,
     }

@@ -1,7 +1,13 @@
 // 3D instantiations of the structured sweep kernel (sweep_impl.hpp): hex
-// patches of 8 cells x 8 corner samples, 8 lanes per node, mu_cond3 family
-// metrics; tile shapes in nodes (last axis x middle x first axis).
+// patches of 8 cells x 8 corner samples, mu_cond3 family metrics; tile
+// shapes in nodes (last axis x middle x first axis). Variants 0-1: one lane
+// per cell (8 per node); 2: the same without LDS staging; 3: two cells per
+// lane (4 lanes per node): same fold order as one per lane, not the same
+// multiply-add fusion.
 #include "sweep_impl.hpp"
+
+#include <stdexcept>
+#include <string>
 
 namespace eggmi
 {
@@ -10,10 +16,12 @@ template <int KIND, bool TAU, bool GRAD, bool PROP = false> static void sweep3d_
 {
     if constexpr (!PROP) {
         if (v == 2) return launch_variant<3, KIND, 8, 4, 2, TAU, GRAD, false, false>(a, s);  // no LDS staging
+        if (v == 3) return launch_variant<3, KIND, 8, 4, 2, TAU, GRAD, false, true, 2>(a, s);  // cells per lane
     }
     switch (v) {
     case 0: launch_variant<3, KIND, 8, 4, 2, TAU, GRAD, PROP>(a, s); break;
-    default: launch_variant<3, KIND, 4, 4, 4, TAU, GRAD, PROP>(a, s); break;
+    case 1: launch_variant<3, KIND, 4, 4, 4, TAU, GRAD, PROP>(a, s); break;
+    default: throw std::invalid_argument("3D tile variant " + std::to_string(v) + " has no proposal kernel");
     }
 }
 

@@ -50,7 +50,9 @@ constexpr int kDppRor8 = 0x128, kDppRor4 = 0x124, kDppQuadXor1 = 0xB1, kDppQuadX
 
 template <int W, class Op> __device__ __forceinline__ float group_reduce(float v, Op op)
 {
-    static_assert(W == 4 || W == 8 || W == 16 || W == 64, "group width");
+    static_assert(W == 1 || W == 2 || W == 4 || W == 8 || W == 16 || W == 64, "group width");
+    if constexpr (W == 1) return v;
+    if constexpr (W == 2) return op(v, dpp_mov<kDppQuadXor1>(v));
     if constexpr (W >= 16) {
         v = op(v, dpp_mov<kDppRor8>(v));
         v = op(v, dpp_mov<kDppRor4>(v));
@@ -314,12 +316,14 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
 // LDS = false: no staging; each lane reads its cell's corners straight from
 // the store (cached), with no workgroup barrier (an autotuner candidate for
 // latency-bound grids).
-template <int D, int KIND, int TX, int TY, int TZ, bool TAU, bool GRAD, bool PROP, bool LDS>
+template <int D, int KIND, int TX, int TY, int TZ, bool TAU, bool GRAD, bool PROP, bool LDS, int CPL>
 __device__ void tile_sweep(const SweepArgs& a, int tile)
 {
     static_assert(LDS || !PROP, "the proposal reads the one-ring from LDS");
+    static_assert(CPL == 1 || !PROP, "the proposal's matrix-core tiles take one lane per cell");
     constexpr int NC = 1 << D;
-    constexpr int NT = NC * TX * TY * TZ;
+    constexpr int NL = NC / CPL;  // lanes per node
+    constexpr int NT = NL * TX * TY * TZ;
     constexpr int LX = TX + 2, LY = TY + 2, LZ = (D == 3) ? TZ + 2 : 1;
     __shared__ float sh[D][LDS ? LZ * LY * LX : 1];
     const TileDesc td = a.tiles[tile];
@@ -329,8 +333,9 @@ __device__ void tile_sweep(const SweepArgs& a, int tile)
 
     // this lane's node and its mask first: the mask load then overlaps the
     // tile staging instead of following the barrier
-    const int cell = tid & (NC - 1);
-    const int nid = tid / NC;
+    const int gl = tid & (NL - 1);  // lane within the node's group: cells gl * CPL + j
+    const int cell = gl;            // (the proposal block runs with CPL == 1)
+    const int nid = tid / NL;
     const int tx = nid % TX, ty = (nid / TX) % TY, tz = nid / (TX * TY);
     int lg[3];
     long long node;
@@ -377,55 +382,69 @@ __device__ void tile_sweep(const SweepArgs& a, int tile)
         if (!active) return;
     }
 
-    // ---- the lane's cell: corners permuted so that corner 0 is the DOF
-    int cb[3] = {0, 0, 0};
+    // ---- the lane's cells: corners permuted so that corner 0 is the DOF
+    float rho[CPL][D];  // axis sign flips of each cell's permuted frame
+    float P[CPL][NC][D];
 #pragma unroll
-    for (int k = 0; k < D; ++k) cb[k] = (cell >> (D - 1 - k)) & 1;
-    float rho[D];  // axis sign flip of the permuted frame
+    for (int j = 0; j < CPL; ++j) {
+        const int cj = gl * CPL + j;
+        int cb[3] = {0, 0, 0};
 #pragma unroll
-    for (int k = 0; k < D; ++k) rho[k] = cb[k] ? 1.0f : -1.0f;  // r_k = 1 - cb_k; rho = r_k ? -1 : 1
-    float P[NC][D];
+        for (int k = 0; k < D; ++k) cb[k] = (cj >> (D - 1 - k)) & 1;
 #pragma unroll
-    for (int op = 0; op < NC; ++op) {
-        int loc[3] = {0, 0, 0};
+        for (int k = 0; k < D; ++k) rho[j][k] = cb[k] ? 1.0f : -1.0f;  // r_k = 1 - cb_k; rho = r_k ? -1 : 1
 #pragma unroll
-        for (int k = 0; k < D; ++k) {
-            const int obit = (op >> (D - 1 - k)) & 1;
-            const int o = obit ^ (1 - cb[k]);  // actual corner bit
-            loc[k] = cb[k] + o;
-        }
-        if constexpr (LDS) {
-            int li;
-            if constexpr (D == 2)
-                li = (ty + loc[0]) * LX + (tx + loc[1]);
-            else
-                li = ((tz + loc[0]) * LY + (ty + loc[1])) * LX + (tx + loc[2]);
+        for (int op = 0; op < NC; ++op) {
+            int loc[3] = {0, 0, 0};
 #pragma unroll
-            for (int k = 0; k < D; ++k) P[op][k] = sh[k][li];
-        } else {
-            long long g = node;
+            for (int k = 0; k < D; ++k) {
+                const int obit = (op >> (D - 1 - k)) & 1;
+                const int o = obit ^ (1 - cb[k]);  // actual corner bit
+                loc[k] = cb[k] + o;
+            }
+            if constexpr (LDS) {
+                int li;
+                if constexpr (D == 2)
+                    li = (ty + loc[0]) * LX + (tx + loc[1]);
+                else
+                    li = ((tz + loc[0]) * LY + (ty + loc[1])) * LX + (tx + loc[2]);
 #pragma unroll
-            for (int k = 0; k < D; ++k) g += (long long)(loc[k] - 1) * (k == D - 1 ? 1 : bd.st[k]);
+                for (int k = 0; k < D; ++k) P[j][op][k] = sh[k][li];
+            } else {
+                long long g = node;
 #pragma unroll
-            for (int k = 0; k < D; ++k) P[op][k] = a.xin[k * PL + g];
+                for (int k = 0; k < D; ++k) g += (long long)(loc[k] - 1) * (k == D - 1 ? 1 : bd.st[k]);
+#pragma unroll
+                for (int k = 0; k < D; ++k) P[j][op][k] = a.xin[k * PL + g];
+            }
         }
     }
 
     const float delta = a.delta;
-    // A of permuted sample op with corner point `cp` and the DOF at `x0`
-    auto build = [&](auto OP, const float* x0, float* A) {
+    // A of permuted sample op of the lane's cell j, with the DOF at `x0`
+    auto build = [&](int j, auto OP, const float* x0, float* A) {
         constexpr int op = decltype(OP)::value;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
             const int nb = op ^ (1 << (D - 1 - k));
-            const float sk = (((op >> (D - 1 - k)) & 1) ? -1.0f : 1.0f) * rho[k];
+            const float sk = (((op >> (D - 1 - k)) & 1) ? -1.0f : 1.0f) * rho[j][k];
 #pragma unroll
             for (int i = 0; i < D; ++i) {
-                const float pn = (nb == 0 && x0) ? x0[i] : P[nb][i];
-                const float pc = (op == 0 && x0) ? x0[i] : P[op][i];
+                const float pn = (nb == 0 && x0) ? x0[i] : P[j][nb][i];
+                const float pc = (op == 0 && x0) ? x0[i] : P[j][op][i];
                 A[i * D + k] = sk * (pn - pc);
             }
         }
+    };
+    // sum over the node's cells in the order of the lane-group reduction
+    // ((c0 + c1) + (c2 + c3)) + ...: a lane folds its CPL cells the same way,
+    // then the group folds the lanes (the fold order of every CPL is the
+    // same; the compiler's multiply-add fusion inside a cell may differ)
+    auto cell_sum = [&](const float* x) -> float {
+        if constexpr (CPL == 1) return group_sum<NL>(x[0]);
+        else if constexpr (CPL == 2) return group_sum<NL>(x[0] + x[1]);
+        else if constexpr (CPL == 4) return group_sum<NL>((x[0] + x[1]) + (x[2] + x[3]));
+        else return group_sum<NL>(((x[0] + x[1]) + (x[2] + x[3])) + ((x[4] + x[5]) + (x[6] + x[7])));
     };
 
     if constexpr (PROP) {
@@ -447,7 +466,7 @@ __device__ void tile_sweep(const SweepArgs& a, int tile)
         const Frame fr = local_frame<D>(ring);
         float dprev[D];
 #pragma unroll
-        for (int k = 0; k < D; ++k) dprev[k] = P[0][k] - a.xout[k * PL + node];
+        for (int k = 0; k < D; ++k) dprev[k] = P[0][0][k] - a.xout[k * PL + node];
         // each lane of the group computes and stores its PER_LANE-wide slice
         const float rh = fr.ok ? __builtin_amdgcn_rcpf(fr.h) : 0.0f;
 #pragma unroll
@@ -474,16 +493,16 @@ __device__ void tile_sweep(const SweepArgs& a, int tile)
         bool finite_p = fr.ok;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-            tr[k] = P[0][k] + a.prop_scale * pw[k];
+            tr[k] = P[0][0][k] + a.prop_scale * pw[k];
             finite_p = finite_p && isfinite(tr[k]);
         }
         float e_now = 0.0f, e_try = 0.0f, m_try = kInf;
         static_for<NC>([&](auto OP) {
             float A[D * D];
-            build(OP, nullptr, A);
+            build(0, OP, nullptr, A);
             e_now += sample_value<D, KIND>(A, delta);
             if constexpr (__builtin_popcount(decltype(OP)::value) <= 1) {
-                if (finite_p) build(OP, tr, A);
+                if (finite_p) build(0, OP, tr, A);
             }
             e_try += sample_value<D, KIND>(A, delta);
             m_try = fminf(m_try, detD<D>(A));
@@ -508,40 +527,58 @@ __device__ void tile_sweep(const SweepArgs& a, int tile)
             // accepted proposal (corner 0 of every lane's cell is the DOF)
             if (cell == 0 && a.counters) atomicAdd(counter_slot(a) + 2, 1ull);
 #pragma unroll
-            for (int k = 0; k < D; ++k) P[0][k] = tr[k];
+            for (int k = 0; k < D; ++k) P[0][0][k] = tr[k];
         }
     }
 
-    float g[D] = {}, h[kNH<D>] = {}, em = 0.0f, ec = 0.0f, mc = kInf, md = kInf;
-    static_for<NC>([&](auto OP) {
-        constexpr int op = decltype(OP)::value;
-        constexpr int pc = __builtin_popcount(op);
-        float A[D * D];
-        build(OP, nullptr, A);
-        const float det = detD<D>(A);
-        md = fminf(md, det);
-        if constexpr (pc <= 1) {  // the DOF is the corner (op = 0) or an axis neighbour
-            float u[D];
+    // per cell: gradient and Hessian of the samples touching the DOF (em),
+    // value of the frozen ones (ec) and their min det (mc)
+    float gs[CPL][D] = {}, hs[CPL][kNH<D>] = {}, em[CPL], ec[CPL], mc = kInf;
 #pragma unroll
-            for (int c = 0; c < D; ++c) {
-                const float sc = (((op >> (D - 1 - c)) & 1) ? -1.0f : 1.0f) * rho[c];
-                if constexpr (op == 0)
-                    u[c] = -sc;
-                else
-                    u[c] = (op == (1 << (D - 1 - c))) ? sc : 0.0f;
+    for (int j = 0; j < CPL; ++j) {
+        em[j] = 0.0f;
+        ec[j] = 0.0f;
+        static_for<NC>([&](auto OP) {
+            constexpr int op = decltype(OP)::value;
+            constexpr int pc = __builtin_popcount(op);
+            float A[D * D];
+            build(j, OP, nullptr, A);
+            const float det = detD<D>(A);
+            if constexpr (pc <= 1) {  // the DOF is the corner (op = 0) or an axis neighbour
+                float u[D];
+#pragma unroll
+                for (int c = 0; c < D; ++c) {
+                    const float sc = (((op >> (D - 1 - c)) & 1) ? -1.0f : 1.0f) * rho[j][c];
+                    if constexpr (op == 0)
+                        u[c] = -sc;
+                    else
+                        u[c] = (op == (1 << (D - 1 - c))) ? sc : 0.0f;
+                }
+                em[j] += sample_grad_hess<D, KIND>(A, u, delta, gs[j], hs[j]);
+            } else {
+                ec[j] += sample_value<D, KIND>(A, delta);
+                mc = fminf(mc, det);
             }
-            em += sample_grad_hess<D, KIND>(A, u, delta, g, h);
-        } else {
-            ec += sample_value<D, KIND>(A, delta);
-            mc = fminf(mc, det);
-        }
-    });
+        });
+    }
+    float g[D], h[kNH<D>];
 #pragma unroll
-    for (int k = 0; k < D; ++k) g[k] = group_sum<NC>(g[k]);
-    float e0 = group_sum<NC>(ec + em);
+    for (int k = 0; k < D; ++k) {
+        float x[CPL];
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) x[j] = gs[j][k];
+        g[k] = cell_sum(x);
+    }
+    float e0;
+    {
+        float x[CPL];
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) x[j] = ec[j] + em[j];
+        e0 = cell_sum(x);
+    }
     float pos[D];
 #pragma unroll
-    for (int k = 0; k < D; ++k) pos[k] = P[0][k];
+    for (int k = 0; k < D; ++k) pos[k] = P[0][0][k];
     float tau[D] = {};
     if constexpr (TAU) {
 #pragma unroll
@@ -552,13 +589,18 @@ __device__ void tile_sweep(const SweepArgs& a, int tile)
         }
     }
     if constexpr (GRAD) {
-        if (cell == 0)
+        if (gl == 0)
 #pragma unroll
             for (int k = 0; k < D; ++k) a.gout[k * PL + node] = g[k];
         return;
     }
 #pragma unroll
-    for (int k = 0; k < kNH<D>; ++k) h[k] = group_sum<NC>(h[k]);
+    for (int k = 0; k < kNH<D>; ++k) {
+        float x[CPL];
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) x[j] = hs[j][k];
+        h[k] = cell_sum(x);
+    }
     float dl[D];
     newton<D>(g, h, dl);
     float alpha = 1.0f, cur[D];
@@ -569,18 +611,23 @@ __device__ void tile_sweep(const SweepArgs& a, int tile)
         float tr[D];
 #pragma unroll
         for (int k = 0; k < D; ++k) tr[k] = pos[k] + alpha * dl[k];
-        float et = 0.0f, mt = mc;
-        static_for<NC>([&](auto OP) {
-            constexpr int op = decltype(OP)::value;
-            if constexpr (__builtin_popcount(op) <= 1) {
-                float A[D * D];
-                build(OP, tr, A);
-                et += sample_value<D, KIND>(A, delta);
-                mt = fminf(mt, detD<D>(A));
-            }
-        });
-        et = group_sum<NC>(ec + et);
-        mt = group_min<NC>(mt);
+        float ets[CPL], mt = mc;
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) {
+            float et = 0.0f;
+            static_for<NC>([&](auto OP) {
+                constexpr int op = decltype(OP)::value;
+                if constexpr (__builtin_popcount(op) <= 1) {
+                    float A[D * D];
+                    build(j, OP, tr, A);
+                    et += sample_value<D, KIND>(A, delta);
+                    mt = fminf(mt, detD<D>(A));
+                }
+            });
+            ets[j] = ec[j] + et;
+        }
+        float et = cell_sum(ets);
+        mt = group_min<NL>(mt);
         if constexpr (TAU) {
 #pragma unroll
             for (int k = 0; k < D; ++k) et -= tau[k] * tr[k];
@@ -596,7 +643,7 @@ __device__ void tile_sweep(const SweepArgs& a, int tile)
     if (a.omega != 1.0f)
 #pragma unroll
         for (int k = 0; k < D; ++k) cur[k] = pos[k] + a.omega * (cur[k] - pos[k]);
-    if (cell == 0) {
+    if (gl == 0) {
         store_node<D>(a, node, cur);
         count_update(a, ntr);
     }
@@ -768,11 +815,15 @@ void launch_sample_variant(const SweepArgs& a, hipStream_t s)
     sample_kernel<KIND, TX, TY, TAU, GRAD><<<dim3(grid), dim3(NT), 0, s>>>(a);
 }
 
-template <int D, int KIND, int TX, int TY, int TZ, bool TAU, bool GRAD, bool PROP, bool LDS>
-__global__ __launch_bounds__((1 << D) * TX * TY * TZ) void sweep_kernel(const SweepArgs a)
+// One launch per sweep: the leading workgroups serve the stored DOFs (one
+// lane group of stored_lanes(D) per DOF), the rest one tile each with
+// 2^D / CPL lanes per node (CPL cells per lane).
+template <int D, int KIND, int TX, int TY, int TZ, bool TAU, bool GRAD, bool PROP, bool LDS, int CPL>
+__device__ __forceinline__ void sweep_body(const SweepArgs& a)
 {
-    constexpr int NT = (1 << D) * TX * TY * TZ;
-    constexpr int G = (D == 2) ? 16 : 64;
+    constexpr int NT = ((1 << D) / CPL) * TX * TY * TZ;
+    constexpr int G = stored_lanes(D);
+    static_assert(NT % G == 0, "a workgroup holds whole stored lane groups");
     // stored-DOF workgroups first: their gather chains are the longest, so
     // they start while the tiles fill the machine
     const int n_st = int(gridDim.x) - a.n_tiles;
@@ -781,19 +832,39 @@ __global__ __launch_bounds__((1 << D) * TX * TY * TZ) void sweep_kernel(const Sw
         stored_group<D, KIND, TAU, GRAD, G>(a, gid, int(threadIdx.x) % G, gid < a.st.n);
         return;
     }
-    tile_sweep<D, KIND, TX, TY, TZ, TAU, GRAD, PROP, LDS>(a, int(blockIdx.x) - n_st);
+    tile_sweep<D, KIND, TX, TY, TZ, TAU, GRAD, PROP, LDS, CPL>(a, int(blockIdx.x) - n_st);
 }
 
-template <int D, int KIND, int TX, int TY, int TZ, bool TAU, bool GRAD, bool PROP = false, bool LDS = true>
+template <int D, int KIND, int TX, int TY, int TZ, bool TAU, bool GRAD, bool PROP, bool LDS>
+__global__ __launch_bounds__((1 << D) * TX * TY * TZ) void sweep_kernel(const SweepArgs a)
+{
+    sweep_body<D, KIND, TX, TY, TZ, TAU, GRAD, PROP, LDS, 1>(a);
+}
+
+// several cells per lane: the compiler interleaves a lane's cells for ILP;
+// MINW waves per SIMD bounds the registers that may take (set where the
+// bound costs no spills)
+template <int D, int KIND, int TX, int TY, int TZ, bool TAU, bool GRAD, int CPL, int MINW>
+__global__ __attribute__((amdgpu_flat_work_group_size(1, ((1 << D) / CPL) * TX * TY * TZ),
+                          amdgpu_waves_per_eu(MINW))) void sweep_kernel_cpl(const SweepArgs a)
+{
+    sweep_body<D, KIND, TX, TY, TZ, TAU, GRAD, false, true, CPL>(a);
+}
+
+template <int D, int KIND, int TX, int TY, int TZ, bool TAU, bool GRAD, bool PROP = false, bool LDS = true,
+          int CPL = 1>
 void launch_variant(const SweepArgs& a, hipStream_t s)
 {
-    constexpr int NT = (1 << D) * TX * TY * TZ;
-    constexpr int G = (D == 2) ? 16 : 64;
+    constexpr int NT = ((1 << D) / CPL) * TX * TY * TZ;
+    constexpr int G = stored_lanes(D);
     const int per_wg = NT / G;
     const int nst = (a.st.n + per_wg - 1) / per_wg;
     const unsigned grid = (unsigned)(a.n_tiles + nst);
     if (grid == 0) return;
-    sweep_kernel<D, KIND, TX, TY, TZ, TAU, GRAD, PROP, LDS><<<dim3(grid), dim3(NT), 0, s>>>(a);
+    if constexpr (CPL == 1)
+        sweep_kernel<D, KIND, TX, TY, TZ, TAU, GRAD, PROP, LDS><<<dim3(grid), dim3(NT), 0, s>>>(a);
+    else
+        sweep_kernel_cpl<D, KIND, TX, TY, TZ, TAU, GRAD, CPL, (D == 2 && CPL == 2 ? 4 : 2)><<<dim3(grid), dim3(NT), 0, s>>>(a);
 }
 
 }  // namespace eggmi

@@ -171,12 +171,14 @@ def test_halo_exchange_bit_exact():
 def test_tile_variants_agree(oracle):
     """Every tile shape of one lane layout computes the same nodes the same
     way (bitwise); the one-lane-per-cell (0-3, and 6-7 without LDS staging)
-    and one-lane-per-sample (4-5) layouts sum a patch in different orders and
-    both match the oracle."""
+    layouts are bitwise equal; the several-cells-per-lane (8: two, 9: four)
+    kernels fold a patch in the same order but fuse multiply-adds
+    differently, and the one-lane-per-sample (4-5) layout sums it in another
+    order; all match the oracle."""
     c = ogrid(33, 17)
     out = []
     nv = _session(c).core.set_tile_variant(-1)
-    assert nv == 8
+    assert nv == 10
     for v in range(nv):
         s = _session(c)
         s.core.set_tile_variant(v)
@@ -186,8 +188,40 @@ def test_tile_variants_agree(oracle):
         np.testing.assert_array_equal(out[v], out[0])
     np.testing.assert_array_equal(out[5], out[4])
     Xo, _, _ = oracle.jacobi(2, "barrier", 0.0, 0.8, 5, c.X, c.patches(oracle), c.ctx.energy_stencil)
-    for X in (out[0], out[4]):
+    for X in (out[0], out[4], out[8], out[9]):
         assert np.abs(X - Xo).max() / _extent(c) < POS_TOL
+
+
+@pytest.mark.parametrize("v", [8, 9])
+def test_cells_per_lane_phases_and_fas(oracle, v):
+    """The several-cells-per-lane kernels in the untangle and shape_size
+    phases against the FP64 oracle, and in FAS (gradient-only and
+    tau-shifted sweeps) against the one-lane-per-cell kernels."""
+    c = ogrid(33, 17, tangle_seed=2)
+    s = _session(c)
+    s.core.set_tile_variant(v)
+    e1, _ = s.run(10, phase="untangle", delta=0.01, omega=0.5, report_every=5)
+    Xo, eo, _ = oracle.jacobi(2, "untangle", 0.01, 0.5, 10, c.X, c.patches(oracle), c.ctx.energy_stencil,
+                              report_every=5)
+    assert np.abs(s.get_X() - Xo).max() / _extent(c) < POS_TOL
+    np.testing.assert_allclose(e1, eo, rtol=TRAJ_RTOL)
+    c2 = ogrid(33, 17)
+    s = _session(c2)
+    s.core.set_tile_variant(v)
+    e2, _ = s.run(5, phase="shape_size", omega=0.8, report_every=5)
+    Xo, eo, _ = oracle.jacobi(2, "shape_size", 0.0, 0.8, 5, c2.X, c2.patches(oracle), c2.ctx.energy_stencil,
+                              report_every=5)
+    assert np.abs(s.get_X() - Xo).max() / _extent(c2) < POS_TOL
+    np.testing.assert_allclose(e2, eo, rtol=TRAJ_RTOL)
+    fas = []
+    for var in (0, v):
+        s = _session(c2)
+        s.core.set_tile_variant(var)
+        e3, m3 = s.run_fas(3, nu_pre=2, nu_post=2, omega=0.8, max_levels=3)
+        fas.append((s.get_X(), e3, m3))
+    assert np.abs(fas[0][0] - fas[1][0]).max() / _extent(c2) < POS_TOL
+    np.testing.assert_allclose(fas[1][1], fas[0][1], rtol=1e-5)
+    assert np.all(fas[1][2] > 0)
 
 
 def test_sample_variant_fas_and_untangle(oracle):
@@ -290,21 +324,23 @@ def test_set_get_x_roundtrip():
 
 
 def test_3d_tile_variants_agree():
-    """3D tile shapes (8x4x2, 4x4x4, and 8x4x2 without LDS staging) are
-    bitwise identical on the cubed sphere (stored DOFs included)."""
+    """3D tile shapes (8x4x2, 4x4x4, 8x4x2 without LDS staging) are bitwise
+    identical on the cubed sphere (stored DOFs included); two cells per lane
+    (variant 3) folds in the same order with other multiply-add fusion."""
     from egg_amd import grids
 
     c = Case(grids.cubed_sphere_3d(9))
     nv = _session(c).core.set_tile_variant(-1)
-    assert nv == 3
+    assert nv == 4
     out = []
     for v in range(nv):
         s = _session(c)
         s.core.set_tile_variant(v)
         s.run(3, omega=0.8)
         out.append(s.get_X())
-    for X in out[1:]:
+    for X in out[1:3]:
         np.testing.assert_array_equal(X, out[0])
+    assert np.abs(out[3] - out[0]).max() / _extent(c) < POS_TOL
 
 
 def test_graph_chunks_match_launches():
