@@ -16,11 +16,12 @@ void launch_grad3d(int kind, int v, const SweepArgs& a, hipStream_t s);
 static constexpr TileShape kTiles2[] = {{32, 2, 1}, {16, 4, 1}, {64, 1, 1}, {32, 4, 1},
                                         {16, 1, 1}, {8, 2, 1},  // one lane per sample
                                         {64, 1, 1}, {32, 2, 1},   // no LDS staging
-                                        {32, 4, 1}, {64, 4, 1}};  // 2 / 4 cells per lane
+                                        {32, 4, 1}, {64, 4, 1},   // 2 / 4 cells per lane
+                                        {32, 4, 1}, {64, 2, 1}};  // 2 cells per lane, without / with LDS
 static constexpr TileShape kTiles3[] = {{8, 4, 2}, {4, 4, 4}, {8, 4, 2} /* no LDS staging */,
                                         {8, 4, 2} /* 2 cells per lane */};
 
-int num_tile_variants(int dim) { return dim == 2 ? 10 : 4; }
+int num_tile_variants(int dim) { return dim == 2 ? 12 : 4; }
 bool tile_variant_has_proposal(int dim, int v) { return dim == 2 ? v < 4 : v < 2; }
 TileShape tile_variant_shape(int dim, int v) { return dim == 2 ? kTiles2[v] : kTiles3[v]; }
 

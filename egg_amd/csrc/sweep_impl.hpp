@@ -844,11 +844,11 @@ __global__ __launch_bounds__((1 << D) * TX * TY * TZ) void sweep_kernel(const Sw
 // several cells per lane: the compiler interleaves a lane's cells for ILP;
 // MINW waves per SIMD bounds the registers that may take (set where the
 // bound costs no spills)
-template <int D, int KIND, int TX, int TY, int TZ, bool TAU, bool GRAD, int CPL, int MINW>
+template <int D, int KIND, int TX, int TY, int TZ, bool TAU, bool GRAD, bool LDS, int CPL, int MINW>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, ((1 << D) / CPL) * TX * TY * TZ),
                           amdgpu_waves_per_eu(MINW))) void sweep_kernel_cpl(const SweepArgs a)
 {
-    sweep_body<D, KIND, TX, TY, TZ, TAU, GRAD, false, true, CPL>(a);
+    sweep_body<D, KIND, TX, TY, TZ, TAU, GRAD, false, LDS, CPL>(a);
 }
 
 template <int D, int KIND, int TX, int TY, int TZ, bool TAU, bool GRAD, bool PROP = false, bool LDS = true,
@@ -864,7 +864,8 @@ void launch_variant(const SweepArgs& a, hipStream_t s)
     if constexpr (CPL == 1)
         sweep_kernel<D, KIND, TX, TY, TZ, TAU, GRAD, PROP, LDS><<<dim3(grid), dim3(NT), 0, s>>>(a);
     else
-        sweep_kernel_cpl<D, KIND, TX, TY, TZ, TAU, GRAD, CPL, (D == 2 && CPL == 2 ? 4 : 2)><<<dim3(grid), dim3(NT), 0, s>>>(a);
+        sweep_kernel_cpl<D, KIND, TX, TY, TZ, TAU, GRAD, LDS, CPL, (D == 2 && CPL == 2 ? 4 : 2)>
+            <<<dim3(grid), dim3(NT), 0, s>>>(a);
 }
 
 }  // namespace eggmi

@@ -171,14 +171,15 @@ def test_halo_exchange_bit_exact():
 def test_tile_variants_agree(oracle):
     """Every tile shape of one lane layout computes the same nodes the same
     way (bitwise); the one-lane-per-cell (0-3, and 6-7 without LDS staging)
-    layouts are bitwise equal; the several-cells-per-lane (8: two, 9: four)
-    kernels fold a patch in the same order but fuse multiply-adds
+    layouts are bitwise equal, and so are the two-cells-per-lane ones (8, 10,
+    11); the several-cells-per-lane kernels (two, 9: four) fold a patch in
+    the one-lane-per-cell order but fuse multiply-adds
     differently, and the one-lane-per-sample (4-5) layout sums it in another
     order; all match the oracle."""
     c = ogrid(33, 17)
     out = []
     nv = _session(c).core.set_tile_variant(-1)
-    assert nv == 10
+    assert nv == 12
     for v in range(nv):
         s = _session(c)
         s.core.set_tile_variant(v)
@@ -188,11 +189,13 @@ def test_tile_variants_agree(oracle):
         np.testing.assert_array_equal(out[v], out[0])
     np.testing.assert_array_equal(out[5], out[4])
     Xo, _, _ = oracle.jacobi(2, "barrier", 0.0, 0.8, 5, c.X, c.patches(oracle), c.ctx.energy_stencil)
+    for v in (10, 11):
+        np.testing.assert_array_equal(out[v], out[8])
     for X in (out[0], out[4], out[8], out[9]):
         assert np.abs(X - Xo).max() / _extent(c) < POS_TOL
 
 
-@pytest.mark.parametrize("v", [8, 9])
+@pytest.mark.parametrize("v", [8, 9, 10])
 def test_cells_per_lane_phases_and_fas(oracle, v):
     """The several-cells-per-lane kernels in the untangle and shape_size
     phases against the FP64 oracle, and in FAS (gradient-only and
