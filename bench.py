@@ -58,14 +58,18 @@ GRAPH_CHUNK = 256  # sweeps per captured hipGraph (5-8 % per sweep on the small 
 # cubed_sphere65_fas_pcg.md). A device copy of N bytes reads FETCH_SIZE =
 # N/2 and WRITE_SIZE = N on this part, so FETCH is scaled by 2 and WRITE by 1.
 PMC_KIB = {
-    ("ogrid", 0): (1313.06, 548.44), ("ogrid", 1): (1465.69, 616.38), ("ogrid", 2): (1217.31, 517.69),
-    ("ogrid", 3): (1072.38, 538.5), ("ogrid", 4): (1845.81, 617.75), ("ogrid", 5): (2064.19, 784.62),
-    ("ogrid", 6): (1213.56, 517.69), ("ogrid", 7): (1305.31, 548.44),
-    ("batched", 0): (590517.16, 277104.75), ("batched", 1): (761483.12, 325449.75),
-    ("batched", 2): (526221.81, 252865.33), ("batched", 3): (508322.66, 276764.88),
-    ("batched", 4): (1073575.91, 325607.78), ("batched", 5): (1284411.69, 422312.41),
-    ("batched", 6): (525394.81, 252865.34), ("batched", 7): (589009.12, 277104.78),
-    ("cubed_sphere", 0): (135326.31, 49891.59),
+    ("ogrid", 0): (1314.06, 548.44), ("ogrid", 1): (1466.69, 616.38), ("ogrid", 2): (1218.31, 517.69),
+    ("ogrid", 3): (1072.38, 538.5), ("ogrid", 4): (1856.91, 617.75), ("ogrid", 5): (2058.19, 784.62),
+    ("ogrid", 6): (1214.56, 517.69), ("ogrid", 7): (1306.81, 548.44), ("ogrid", 8): (1089.25, 549.5),
+    ("ogrid", 9): (773.69, 502.5), ("ogrid", 10): (1085.0, 549.5), ("ogrid", 11): (918.38, 505.38),
+    ("batched", 0): (590518.03, 277104.75), ("batched", 1): (761484.0, 325449.72),
+    ("batched", 2): (526222.62, 252865.31), ("batched", 3): (508322.66, 276764.81),
+    ("batched", 4): (1073595.0, 325607.78), ("batched", 5): (1284491.03, 422312.42),
+    ("batched", 6): (525395.94, 252865.33), ("batched", 7): (589010.25, 277104.77),
+    ("batched", 8): (509931.56, 277069.66), ("batched", 9): (376742.16, 252706.42),
+    ("batched", 10): (508547.75, 277069.69), ("batched", 11): (435552.28, 252758.38),
+    ("cubed_sphere", 0): (135327.31, 49889.66), ("cubed_sphere", 1): (185090.25, 69130.72),
+    ("cubed_sphere", 2): (133390.06, 49889.95), ("cubed_sphere", 3): (140850.5, 51144.95),
 }
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 
