@@ -445,7 +445,7 @@ eggmi_status eggmi_session_set_proposal(eggmi_session* s, eggmi_mlp* m, int mode
         if (mode != 0 && mode != 1) throw eggmi::InvalidArgument("proposal mode must be 0 (replace) or 1 (predict-correct)");
         if (!(scale > 0.0f) || !std::isfinite(scale)) throw eggmi::InvalidArgument("proposal scale must be positive");
         if (m && !eggmi::tile_variant_has_proposal(s->s.dim, s->s.tile_variant))
-            s->s.set_tile_variant(0);  // the one-lane-per-sample variants have no proposal kernel
+            s->s.set_tile_variant(0);  // only the one-lane-per-cell LDS shapes have a proposal kernel
         s->s.proposal = m ? m->m.get() : nullptr;
         s->s.proposal_mode = mode;
         s->s.proposal_scale = scale;

@@ -144,9 +144,12 @@ eggmi_status eggmi_halo_exchange(eggmi_session* s);
  * in *n_variants when variant < 0. 2D: variants 0-3 put one lane on each of
  * a node's 4 cells (tiles of 32x2, 16x4, 64x1, 32x4 nodes), 4-5 one lane on
  * each of its 16 samples (16x1, 8x2 nodes), 6-7 one lane per cell without
- * LDS staging (64x1, 32x2). 3D: one lane per cell, tiles of 8x4x2 and 4x4x4
- * nodes, 2 = 8x4x2 without LDS staging. Only variants 0-3 (2D) and 0-1 (3D)
- * have a learned-proposal kernel. */
+ * LDS staging (64x1, 32x2), 8 and 11 two cells per lane (32x4, 64x2), 10
+ * the same without LDS staging (32x4), 9 one lane per node (64x4). 3D: one
+ * lane per cell, tiles of 8x4x2 and 4x4x4 nodes, 2 = 8x4x2 without LDS
+ * staging, 3 = 8x4x2 with two cells per lane. Only variants 0-3 (2D) and
+ * 0-1 (3D) have a learned-proposal kernel; setting a proposal on a session
+ * with another variant selects variant 0. */
 eggmi_status eggmi_set_tile_variant(eggmi_session* s, int variant, int* n_variants);
 
 /* Energy and min det A of the resident state (the report reduction of
@@ -155,9 +158,10 @@ eggmi_status eggmi_energy(eggmi_session* s, int phase, double delta, double* ene
 
 /* Autotuner (no reference counterpart): time n_sweeps plain sweeps (omega)
  * with every tile variant from the session's current state, which is
- * restored after each trial (the variants give bitwise-identical results),
- * and keep the fastest. ms[v] (v < capacity) receives each variant's time,
- * *best the variant selected. */
+ * restored after each trial, and keep the fastest. The variants sum a patch
+ * in the same order except the one-lane-per-sample ones, and differ at most
+ * by FP32 rounding (multiply-add fusion). ms[v] (v < capacity) receives each
+ * variant's time, *best the variant selected. */
 eggmi_status eggmi_autotune(eggmi_session* s, int n_sweeps, double omega, double* ms, int capacity, int* best);
 
 /* Device time of the last eggmi_run / eggmi_run_fas call in ms (HIP events
