@@ -320,12 +320,20 @@ def bench_ogrid(args):
 
         net, info = _train_proposal(make, (11, 12, 13, 14, 15))
         s.set_X(Xu)
-        s.set_proposal(net, mode="predict_correct", scale=0.5)
+        s.set_proposal(net, mode="predict_correct", scale=0.5)  # (selects a variant with a proposal kernel)
         n, t, E = _converge(plain_step, e_star, 60.0)
-        s.set_proposal(None)
         ttc["mlp_proposal"] = {"sweeps": n, "s": t + t_untangle if n else None, "E_end": E,
                                "mode": "predict_correct", "scale": 0.5, **info}
         log(f"MLP proposal: {n} sweeps, {t:.3f}s to converge ({info})")
+        # the proposal in the fine-level smoothing of the fastest FAS cycle
+        s.set_X(Xu)
+        n, t, E = _converge(lambda: (*s.run_fas(5, nu_pre=8, nu_post=8, nu_coarse=16, omega=0.8, max_levels=4), 5),
+                            e_star, 60.0)
+        s.set_proposal(None)
+        ttc["fas_v88_4_levels_coarse16_mlp_proposal"] = {"cycles": n, "s": t + t_untangle if n else None,
+                                                         "E_end": E, "mode": "predict_correct", "scale": 0.5}
+        log(f"FAS V(8,8) + MLP proposal: {n} cycles, {t:.3f}s to converge")
+        s.core.set_tile_variant(int(tune_best))
 
     cpu = parity = None
     if not args.no_cpu:

@@ -9,6 +9,7 @@
 // the face nodes read the level's ghost plane). Transfers run per level
 // pair in one launch each.
 #include "mg.hpp"
+#include "mlp_host.hpp"
 #include "session.hpp"
 
 #include <algorithm>
@@ -430,11 +431,20 @@ TransferArgs Session::transfer(const Level& parent, const Level& child, bool pro
 
 void Session::smooth(Level& L, int sweeps, int kind, float omega, const float* tau)
 {
+    // the learned proposal, when set, runs in the fine level's (unshifted)
+    // smoothing sweeps as in plain runs; the other Jacobi buffer holds each
+    // DOF's state before the last sweep (or correction), its displacement input
+    const bool propose = proposal != nullptr && &L == &fine && tau == nullptr && kind != 1;
     for (int s = 0; s < sweeps; ++s) {
         float* xin = L.x[L.cur].p;
         float* xout = L.x[1 - L.cur].p;
         SweepArgs a = L.sweep_args(xin, xout, omega, 0.0f, tau);
         a.counters = counters_for(L);
+        if (propose) {
+            a.mlp = proposal->view();
+            a.prop_mode = proposal_mode;
+            a.prop_scale = proposal_scale;
+        }
         launch_sweep(dim, kind, tile_variant, a, stream);
         L.cur = 1 - L.cur;
         ++last_launches;

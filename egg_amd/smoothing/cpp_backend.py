@@ -216,9 +216,10 @@ class CppStructuredSweepSession:
 
     def set_proposal(self, mlp, *, mode: str = "replace", scale: float = 1.0) -> None:
         """Use a trained ``cpp_core.ProposalMLP`` (or None) as the learned
-        Newton-step proposal of subsequent :meth:`run` calls. ``mode``
-        "replace": an accepted proposal is the update; "predict_correct":
-        the Newton step continues from it."""
+        Newton-step proposal of subsequent :meth:`run` calls and of the
+        fine-level smoothing sweeps of :meth:`run_fas`. ``mode`` "replace":
+        an accepted proposal is the update; "predict_correct": the Newton
+        step continues from it."""
         self._session.set_proposal(mlp, mode=mode, scale=scale)
 
     def harvest(self, mlp, n_snap: int, *, every: int = 8, horizon: int = 8, omega: float = 0.8) -> int:

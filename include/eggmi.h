@@ -229,7 +229,8 @@ eggmi_status eggmi_mlp_train(eggmi_mlp* m, int epochs, int64_t batch, float lr, 
 eggmi_status eggmi_session_harvest(eggmi_session* s, eggmi_mlp* m, int n_snap, int every, int horizon,
                                    double omega, int64_t* added);
 /* Use `m` (null: none) as the proposal of subsequent eggmi_run calls in the
- * barrier and shape_size phases, its displacement scaled by `scale`. mode 0:
+ * barrier and shape_size phases and of eggmi_run_fas's fine-level smoothing
+ * sweeps, its displacement scaled by `scale`. mode 0:
  * an accepted proposal is the DOF's update; mode 1 (predict-correct): the
  * plain Newton update (halvings, omega blend) continues from an accepted
  * proposal. The session borrows m: keep it alive (or unset it) until the

@@ -165,6 +165,35 @@ def test_zero_network_falls_back_to_plain():
     assert cnt[2] == 0 and cnt[1] == c.grid.n_free * 20
 
 
+def test_proposal_in_fas_fine_smoothing():
+    """With a proposal set, FAS's fine-level smoothing sweeps run the
+    proposal kernel: a zero network is never accepted (the cycle equals the
+    plain FAS cycle up to fused multiply-adds), and its counted updates are
+    the fine sweeps'."""
+    c = ogrid(33, 17)
+    net = _net(2)
+    net.set_params(np.zeros(M.n_params(2), np.float32))
+    a = _session(c)
+    b = _session(c)
+    b.set_proposal(net)
+    b.core.set_counters(True)
+    ea, ma = a.run_fas(3, nu_pre=2, nu_post=2, omega=0.8, max_levels=3)
+    eb, mb = b.run_fas(3, nu_pre=2, nu_post=2, omega=0.8, max_levels=3)
+    np.testing.assert_allclose(eb, ea, rtol=1e-6)
+    np.testing.assert_allclose(b.get_X(), a.get_X(), rtol=0, atol=1e-5)
+    cnt = b.core.last_counters(3)
+    assert cnt[0][2] == 0 and cnt[0][1] == c.grid.n_free * 3 * 4
+    # a random network's proposals are accepted now and then, on the fine
+    # level only, and the cycles stay valid
+    rng = np.random.default_rng(7)
+    net.set_params((rng.standard_normal(M.n_params(2)) * 0.05).astype(np.float32))
+    b.set_X(c.X)
+    e, m = b.run_fas(3, nu_pre=2, nu_post=2, omega=0.8, max_levels=3)
+    cnt = b.core.last_counters(3)
+    assert cnt[0][2] > 0 and cnt[1][2] == 0 and cnt[2][2] == 0
+    assert np.all(np.isfinite(e)) and np.all(m > 0)
+
+
 def test_wild_network_keeps_mesh_valid():
     """Large random proposals are mostly rejected; the accepted ones never
     fold a cell and never raise a DOF's patch energy, so the energy stays
