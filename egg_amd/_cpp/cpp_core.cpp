@@ -554,7 +554,8 @@ PYBIND11_MODULE(cpp_core, m)
       .def("set_kernel_timing", &Session::set_kernel_timing, py::arg("on"))
       .def("set_counters", &Session::set_counters, py::arg("on"))
       .def("set_graph_chunk", &Session::set_graph_chunk, py::arg("chunk"),
-           "Replay plain sweeps as captured hipGraph chunks of `chunk` sweeps (even; 0 = off).")
+           "Replay plain sweeps as captured hipGraph chunks of `chunk` sweeps (even; 0 = off), and FAS "
+           "cycles as captured graphs (chunk > 0).")
       .def("last_counters", &Session::last_counters, py::arg("n_levels") = 1,
            "Counters of the last run (counting on): rows per level of (line-search trials, DOF updates, "
            "accepted learned proposals).")

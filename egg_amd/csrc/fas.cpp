@@ -467,9 +467,7 @@ void Session::restrict_to(Level& parent, const float* x_parent, Level& child, in
     launch_restrict(dim, t, x_parent, child.x[0].p, parent.grad.p, child.tau.p, child.d_free.p, stream);
     child.cur = 0;
     halo(child, child.x[0].p);
-    const size_t bytes = (size_t)child.plane * dim * sizeof(float);
-    EGGMI_HIP(hipMemcpyAsync(child.rx.p, child.x[0].p, bytes, hipMemcpyDeviceToDevice, stream));
-    EGGMI_HIP(hipMemcpyAsync(child.x[1].p, child.x[0].p, bytes, hipMemcpyDeviceToDevice, stream));
+    launch_copy2(child.rx.p, child.x[1].p, child.x[0].p, (long long)child.plane * dim, stream);
     grad(child, child.rx.p, child.grad.p, nullptr, kind);
     launch_vec_sub(child.tau.p, child.grad.p, child.tau.p, (long long)child.plane * dim, stream);
 }
@@ -538,36 +536,50 @@ std::pair<std::vector<double>, std::vector<double>> Session::run_fas(int n_cycle
             for (int k = 0; k < dim; ++k) ext = std::max(ext, sh[k]);
         o.nu_coarse = std::min(nu_coarse, 2 * ext);
     }
-    if ((int)d_reports.n < 2 * n_cycles) d_reports.alloc(2 * n_cycles);
+    // reports (a buffer of FAS's own, which captured cycles append to):
+    // energies at [0, cap), min dets at [cap, 2 cap)
+    if ((long long)d_fas_reports.n < 2LL * n_cycles) {
+        release_graphs();
+        d_fas_reports.alloc(2 * std::max(n_cycles, 64));
+    }
+    const int cap = (int)(d_fas_reports.n / 2);
     if (!d_ls.n) d_ls.alloc(4);
     last_launches = 0;
     counters_begin();
+    const bool graphs_on = graph_chunk > 0 && !counting;
+    if (graphs_on) {
+        if (!d_ladder_parts.n) {
+            d_ladder_parts.alloc((size_t)2 * (kLadder + 1) * (kReduceBlocks / 2));
+            d_ladder_out.alloc(2 * (kLadder + 1));
+            d_cyc_rep.alloc(2);
+            d_alpha.alloc(1);
+            d_rep_count.alloc(1);
+        }
+        const int zero = 0;
+        EGGMI_HIP(hipMemcpyAsync(d_rep_count.p, &zero, sizeof(int), hipMemcpyHostToDevice, stream));
+        EGGMI_HIP(hipStreamSynchronize(stream));  // the source is a host stack value
+    }
     EGGMI_HIP(hipEventRecord(ev0, stream));
     // consistent halo on the resident state, both fine Jacobi buffers equal
     halo(F, F.x[F.cur].p);
     EGGMI_HIP(hipMemcpyAsync(F.x[1 - F.cur].p, F.x[F.cur].p, (size_t)F.plane * dim * sizeof(float),
                              hipMemcpyDeviceToDevice, stream));
     pair_valid = true;
-    Level& C0 = *coarse[0];
-    CellReduceArgs ra;
-    ra.plane = F.plane;
-    ra.blocks = F.blocks.p;
-    ra.cell_off = F.d_cell_off.p;
-    ra.nblocks = (int)F.lay.shape.size();
-    ra.ncells = F.ncells;
     const float tol = 1e-6f;  // FP32 accept slack (reference src/core.hpp:40)
     for (int cyc = 0; cyc < n_cycles; ++cyc) {
-        smooth(F, nu_pre, kind, o.omega, nullptr);
+        if (graphs_on) {
+            const int parity = F.x[F.cur].p == F.x[0].p ? 0 : 1;
+            EGGMI_HIP(hipGraphLaunch(fas_graph(parity, kind, o, n_coarse), stream));
+            const FasGraph& g = fas_graphs[parity];
+            last_launches += g.launches;
+            if ((o.nu_pre + o.nu_post) % 2) F.cur = 1 - F.cur;
+            continue;
+        }
+        fas_head(kind, o, n_coarse);
         float* x = F.x[F.cur].p;
-        grad(F, x, F.grad.p, nullptr, kind);
-        restrict_to(F, x, C0, kind);
-        coarse_vcycle(0, n_coarse, kind, o);
-        launch_vec_sub(C0.grad.p, C0.x[C0.cur].p, C0.rx.p, (long long)C0.plane * dim, stream);
-        launch_prolong(dim, transfer(F, C0, true), C0.grad.p, F.corr.p, stream);
-        launch_zero_unmasked(F.corr.p, F.d_free.p, F.lay.n_slots(), F.plane, dim, stream);
-        halo(F, F.corr.p);
-        // safeguarded application (reference src/fas.hpp:549-581)
-        ra.x = x;
+        // safeguarded application (reference src/fas.hpp:549-581), decided
+        // on the host one trial pair at a time
+        const CellReduceArgs ra = fine_cells(x);
         double ls[4];
         auto ladder = [&](float a0, float a1) {
             if (F.structured_energy) {
@@ -585,7 +597,7 @@ std::pair<std::vector<double>, std::vector<double>> Session::run_fas(int n_cycle
         ladder(0.0f, 1.0f);
         const double e_pre = ls[0];
         if (ok(e_pre, ls[2], ls[3])) alpha = 1.0f;
-        const int max_bt = 10;
+        const int max_bt = kLadder;
         float a = 0.5f;
         for (int it = 1; alpha == 0.0f && it < max_bt; it += 2, a *= 0.25f) {
             ladder(a, a * 0.5f);
@@ -600,19 +612,121 @@ std::pair<std::vector<double>, std::vector<double>> Session::run_fas(int n_cycle
             launch_masked_axpy(x, F.corr.p, alpha, F.d_free.p, F.lay.n_slots(), F.plane, dim, stream);
             halo(F, x);
         }
-        smooth(F, nu_post, kind, o.omega, nullptr);
-        reduce(F, F.x[F.cur].p, kind, 0.0, d_reports.p + cyc, d_reports.p + n_cycles + cyc);
+        fas_tail(kind, o, d_fas_reports.p + cyc, d_fas_reports.p + cap + cyc);
     }
     EGGMI_HIP(hipEventRecord(ev1, stream));
-    std::vector<double> hr(2 * n_cycles);
-    EGGMI_HIP(hipMemcpyAsync(hr.data(), d_reports.p, sizeof(double) * 2 * n_cycles, hipMemcpyDeviceToHost, stream));
+    std::vector<double> e(n_cycles), m(n_cycles);
+    EGGMI_HIP(hipMemcpyAsync(e.data(), d_fas_reports.p, sizeof(double) * n_cycles, hipMemcpyDeviceToHost, stream));
+    EGGMI_HIP(hipMemcpyAsync(m.data(), d_fas_reports.p + cap, sizeof(double) * n_cycles, hipMemcpyDeviceToHost,
+                             stream));
     EGGMI_HIP(hipStreamSynchronize(stream));
     counters_end();
     float ms = 0;
     EGGMI_HIP(hipEventElapsedTime(&ms, ev0, ev1));
     last_ms = ms;
-    return {std::vector<double>(hr.begin(), hr.begin() + n_cycles),
-            std::vector<double>(hr.begin() + n_cycles, hr.end())};
+    return {e, m};
+}
+
+CellReduceArgs Session::fine_cells(const float* x) const
+{
+    const Level& F = fine;
+    CellReduceArgs ra;
+    ra.x = x;
+    ra.plane = F.plane;
+    ra.blocks = F.blocks.p;
+    ra.cell_off = F.d_cell_off.p;
+    ra.nblocks = (int)F.lay.shape.size();
+    ra.ncells = F.ncells;
+    return ra;
+}
+
+void Session::fas_head(int kind, const FasParams& o, size_t n_coarse)
+{
+    Level& F = fine;
+    Level& C0 = *coarse[0];
+    smooth(F, o.nu_pre, kind, o.omega, nullptr);
+    float* x = F.x[F.cur].p;
+    grad(F, x, F.grad.p, nullptr, kind);
+    restrict_to(F, x, C0, kind);
+    coarse_vcycle(0, n_coarse, kind, o);
+    launch_vec_sub(C0.grad.p, C0.x[C0.cur].p, C0.rx.p, (long long)C0.plane * dim, stream);
+    launch_prolong(dim, transfer(F, C0, true), C0.grad.p, F.corr.p, stream);
+    launch_zero_unmasked(F.corr.p, F.d_free.p, F.lay.n_slots(), F.plane, dim, stream);
+    halo(F, F.corr.p);
+}
+
+void Session::fas_device_ladder(int kind)
+{
+    Level& F = fine;
+    if (!F.structured_energy) throw Internal("run_fas: the line search needs the structured cell energy stencil");
+    float* x = F.x[F.cur].p;
+    launch_linesearch_ladder(dim, kind, fine_cells(x), F.corr.p, d_ladder_parts.p, (double)1e-6f, d_ladder_out.p,
+                             d_alpha.p, stream);
+    launch_masked_axpy_dev(x, F.corr.p, d_alpha.p, F.d_free.p, F.lay.n_slots(), F.plane, dim, stream);
+    halo(F, x);  // a no-op copy when nothing moved (the replicas already equal their owners)
+}
+
+void Session::fas_tail(int kind, const FasParams& o, double* out_e, double* out_m)
+{
+    Level& F = fine;
+    smooth(F, o.nu_post, kind, o.omega, nullptr);
+    reduce(F, F.x[F.cur].p, kind, 0.0, out_e, out_m);
+}
+
+hipGraphExec_t Session::fas_graph(int parity, int kind, const FasParams& o, size_t n_coarse)
+{
+    FasGraph& g = fas_graphs[parity];
+    const bool propose = proposal != nullptr && kind != 1;
+    const void* mlp = propose ? (const void*)proposal->view().fwd : nullptr;
+    if (g.exec && g.kind == kind && g.variant == tile_variant && g.nu_pre == o.nu_pre && g.nu_post == o.nu_post &&
+        g.nu_coarse == o.nu_coarse && g.n_coarse == n_coarse && g.omega == o.omega && g.rep == d_fas_reports.p &&
+        g.mlp == mlp && (!propose || (g.mode == proposal_mode && g.scale == proposal_scale)))
+        return g.exec;
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    g = FasGraph {};
+    Level& F = fine;
+    // the capture enqueues (does not run) one cycle from this parity; the
+    // host-side buffer flips it makes are undone afterwards
+    const int cur0 = F.cur;
+    F.cur = parity;
+    std::vector<int> ccur;
+    for (auto& c : coarse) ccur.push_back(c->cur);
+    const long long launches0 = last_launches;
+    hipGraph_t graph = nullptr;
+    EGGMI_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    try {
+        fas_head(kind, o, n_coarse);
+        fas_device_ladder(kind);
+        fas_tail(kind, o, d_cyc_rep.p, d_cyc_rep.p + 1);
+        launch_append_report(d_cyc_rep.p, d_fas_reports.p, (int)(d_fas_reports.n / 2), d_rep_count.p, stream);
+    } catch (...) {
+        (void)hipStreamEndCapture(stream, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        F.cur = cur0;
+        for (size_t i = 0; i < coarse.size(); ++i) coarse[i]->cur = ccur[i];
+        last_launches = launches0;
+        throw;
+    }
+    EGGMI_HIP(hipStreamEndCapture(stream, &graph));
+    const hipError_t ie = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    g.launches = last_launches - launches0;
+    F.cur = cur0;
+    for (size_t i = 0; i < coarse.size(); ++i) coarse[i]->cur = ccur[i];
+    last_launches = launches0;
+    EGGMI_HIP(ie);
+    g.kind = kind;
+    g.variant = tile_variant;
+    g.nu_pre = o.nu_pre;
+    g.nu_post = o.nu_post;
+    g.nu_coarse = o.nu_coarse;
+    g.n_coarse = n_coarse;
+    g.omega = o.omega;
+    g.rep = d_fas_reports.p;
+    g.mlp = mlp;
+    g.mode = proposal_mode;
+    g.scale = proposal_scale;
+    return g.exec;
 }
 
 }  // namespace eggmi

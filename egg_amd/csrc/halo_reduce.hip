@@ -240,9 +240,66 @@ __global__ __launch_bounds__(kReduceThreads) void reduce_finish(const double* pa
     }
 }
 
+// Corner slots of structured cell `cid` (found from the block walk `lo`).
+template <int DIM>
+__device__ __forceinline__ void cell_nodes(const BlockDesc& bd, int r, long long* nodes)
+{
+    if constexpr (DIM == 2) {
+        const int w1 = bd.n[1] - 1;
+        const int ci = r / w1, cj = r - ci * w1;
+#pragma unroll
+        for (int o = 0; o < 4; ++o) nodes[o] = bd.base + (long long)(ci + (o >> 1) + 1) * bd.st[0] + (cj + (o & 1) + 1);
+    } else {
+        const int w2 = bd.n[2] - 1, w1 = bd.n[1] - 1;
+        const int ci = r / (w1 * w2);
+        r -= ci * w1 * w2;
+        const int cj = r / w2, ck = r - cj * w2;
+#pragma unroll
+        for (int o = 0; o < 8; ++o)
+            nodes[o] = bd.base + (long long)(ci + (o >> 2) + 1) * bd.st[0] + (long long)(cj + ((o >> 1) & 1) + 1) * bd.st[1] +
+                       (ck + (o & 1) + 1);
+    }
+}
+
+// Energy and min det of one cell's 2^D corner samples at x + al c (positions
+// formed in FP32 exactly as masked_axpy forms them), in FP64.
+template <int DIM, int KIND>
+__device__ __forceinline__ void cell_trial(const CellReduceArgs& a, const float* corr, const long long* nodes, float al,
+                                           double& es, double& ms)
+{
+    constexpr int NC = 1 << DIM;
+    double p[DIM][NC];
+#pragma unroll
+    for (int o = 0; o < NC; ++o)
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) {
+            const long long q = c * a.plane + nodes[o];
+            p[c][o] = (double)(a.x[q] + al * corr[q]);
+        }
+    es = 0.0;
+    ms = __builtin_huge_val();
+#pragma unroll
+    for (int b = 0; b < NC; ++b) {
+        double A[DIM * DIM];
+#pragma unroll
+        for (int kk = 0; kk < DIM; ++kk) {
+            const int bit = 1 << (DIM - 1 - kk);
+            const double sg = (b & bit) ? -1.0 : 1.0;
+#pragma unroll
+            for (int ii = 0; ii < DIM; ++ii) A[ii * DIM + kk] = sg * (p[ii][b ^ bit] - p[ii][b]);
+        }
+        if constexpr (DIM == 2) {
+            es += mu2_value<KIND, double>(A[0], A[1], A[2], A[3], a.delta);
+            ms = fmin(ms, A[0] * A[3] - A[1] * A[2]);
+        } else {
+            es += mu3_value<KIND, double>(A, a.delta);
+            ms = fmin(ms, det3(A));
+        }
+    }
+}
+
 // Two line-search trials in one pass (reference src/fas.hpp:174-234): every
-// cell sample is evaluated at x + a0 c and x + a1 c (positions formed in
-// FP32 exactly as masked_axpy forms them), accumulated in FP64.
+// cell sample is evaluated at x + a0 c and x + a1 c, accumulated in FP64.
 template <int DIM, int KIND>
 __global__ __launch_bounds__(kReduceThreads) void linesearch_pair(const CellReduceArgs a, const float* corr, float a0,
                                                                   float a1, double* partials)
@@ -254,67 +311,87 @@ __global__ __launch_bounds__(kReduceThreads) void linesearch_pair(const CellRedu
     for (long long cid = c0 + threadIdx.x; cid < c1; cid += kReduceThreads) {
         lo = cell_block_step(a.cell_off, a.nblocks, lo, cid);
         const BlockDesc bd = a.blocks[lo];
-        int r = int(cid - a.cell_off[lo]);  // cells of one block < 2^31
-        constexpr int NC = 1 << DIM;
-        long long nodes[NC];
-        if constexpr (DIM == 2) {
-            const int w1 = bd.n[1] - 1;
-            const int ci = r / w1, cj = r - ci * w1;
-#pragma unroll
-            for (int o = 0; o < 4; ++o)
-                nodes[o] = bd.base + (long long)(ci + (o >> 1) + 1) * bd.st[0] + (cj + (o & 1) + 1);
-        } else {
-            const int w2 = bd.n[2] - 1, w1 = bd.n[1] - 1;
-            const int ci = r / (w1 * w2);
-            r -= ci * w1 * w2;
-            const int cj = r / w2, ck = r - cj * w2;
-#pragma unroll
-            for (int o = 0; o < 8; ++o)
-                nodes[o] = bd.base + (long long)(ci + (o >> 2) + 1) * bd.st[0] +
-                           (long long)(cj + ((o >> 1) & 1) + 1) * bd.st[1] + (ck + (o & 1) + 1);
-        }
-#pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            const float al = t == 0 ? a0 : a1;
-            double p[DIM][NC];
-#pragma unroll
-            for (int o = 0; o < NC; ++o)
-#pragma unroll
-                for (int c = 0; c < DIM; ++c) {
-                    const long long q = c * a.plane + nodes[o];
-                    p[c][o] = (double)(a.x[q] + al * corr[q]);
-                }
-            double es = 0.0, ms = __builtin_huge_val();
-#pragma unroll
-            for (int b = 0; b < NC; ++b) {
-                double A[DIM * DIM];
-#pragma unroll
-                for (int kk = 0; kk < DIM; ++kk) {
-                    const int bit = 1 << (DIM - 1 - kk);
-                    const double sg = (b & bit) ? -1.0 : 1.0;
-#pragma unroll
-                    for (int ii = 0; ii < DIM; ++ii) A[ii * DIM + kk] = sg * (p[ii][b ^ bit] - p[ii][b]);
-                }
-                if constexpr (DIM == 2) {
-                    es += mu2_value<KIND, double>(A[0], A[1], A[2], A[3], a.delta);
-                    ms = fmin(ms, A[0] * A[3] - A[1] * A[2]);
-                } else {
-                    es += mu3_value<KIND, double>(A, a.delta);
-                    ms = fmin(ms, det3(A));
-                }
-            }
-            if (t == 0) {
-                e0 += es;
-                m0 = fmin(m0, ms);
-            } else {
-                e1 += es;
-                m1 = fmin(m1, ms);
-            }
-        }
+        long long nodes[1 << DIM];
+        cell_nodes<DIM>(bd, int(cid - a.cell_off[lo]), nodes);  // cells of one block < 2^31
+        double es, ms;
+        cell_trial<DIM, KIND>(a, corr, nodes, a0, es, ms);
+        e0 += es;
+        m0 = fmin(m0, ms);
+        cell_trial<DIM, KIND>(a, corr, nodes, a1, es, ms);
+        e1 += es;
+        m1 = fmin(m1, ms);
     }
     block_reduce_store(e0, m0, partials + 4 * blockIdx.x);
     __syncthreads();
     block_reduce_store(e1, m1, partials + 4 * blockIdx.x + 2);
+}
+
+// The whole safeguarded ladder in one pass: slot 0 at alpha = 0 (the energy
+// before the correction), slot k = 1..kLadder at alpha = 2^-(k-1), each
+// accumulated exactly as linesearch_pair accumulates its trials (same cell
+// walk, same per-thread and per-block folds), so a slot equals the pair
+// kernel's value for the same alpha bit for bit.
+template <int DIM, int KIND>
+__global__ __launch_bounds__(kReduceThreads) void linesearch_ladder(const CellReduceArgs a, const float* corr,
+                                                                    double* partials)
+{
+    constexpr int NA = kLadder + 1;
+    double e[NA], m[NA];
+#pragma unroll
+    for (int t = 0; t < NA; ++t) {
+        e[t] = 0.0;
+        m[t] = __builtin_huge_val();
+    }
+    long long c0, c1;
+    cell_chunk(a.ncells, c0, c1);
+    int lo = c0 + threadIdx.x < c1 ? cell_block(a.cell_off, a.nblocks, c0 + threadIdx.x) : 0;
+    for (long long cid = c0 + threadIdx.x; cid < c1; cid += kReduceThreads) {
+        lo = cell_block_step(a.cell_off, a.nblocks, lo, cid);
+        const BlockDesc bd = a.blocks[lo];
+        long long nodes[1 << DIM];
+        cell_nodes<DIM>(bd, int(cid - a.cell_off[lo]), nodes);
+#pragma unroll
+        for (int t = 0; t < NA; ++t) {
+            double es, ms;
+            cell_trial<DIM, KIND>(a, corr, nodes, t == 0 ? 0.0f : ladder_alpha(t), es, ms);
+            e[t] += es;
+            m[t] = fmin(m[t], ms);
+        }
+    }
+    for (int t = 0; t < NA; ++t) {
+        block_reduce_store(e[t], m[t], partials + 2 * NA * blockIdx.x + 2 * t);
+        __syncthreads();
+    }
+}
+
+// Fold the ladder's partials per slot (in pair_finish's order) and pick the
+// step: the first alpha = 1, 1/2, ... whose energy is finite, at most
+// e(0) + tol, with every det > 0 (reference src/fas.hpp:549-581); 0 if none.
+// out[2t], out[2t+1]: slot t's energy and min det.
+__global__ __launch_bounds__(kReduceThreads) void ladder_finish(const double* partials, int nparts, double tol,
+                                                                double* out, float* alpha)
+{
+    constexpr int NA = kLadder + 1;
+    __shared__ double tmp[2 * NA];
+    for (int t = 0; t < NA; ++t) {
+        double e = 0.0, m = __builtin_huge_val();
+        for (int i = threadIdx.x; i < nparts; i += kReduceThreads) {
+            e += partials[2 * NA * i + 2 * t];
+            m = fmin(m, partials[2 * NA * i + 2 * t + 1]);
+        }
+        block_reduce_store(e, m, tmp + 2 * t);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        float al = 0.0f;
+        for (int t = 1; t < NA; ++t)
+            if (isfinite(tmp[2 * t]) && tmp[2 * t] <= tmp[0] + tol && tmp[2 * t + 1] > 0.0) {
+                al = ladder_alpha(t);
+                break;
+            }
+        *alpha = al;
+        for (int i = 0; i < 2 * NA; ++i) out[i] = tmp[i];
+    }
 }
 
 __global__ __launch_bounds__(kReduceThreads) void pair_finish(const double* partials, int nparts, double* out)
@@ -333,6 +410,26 @@ __global__ __launch_bounds__(kReduceThreads) void pair_finish(const double* part
     __syncthreads();
     if (threadIdx.x == 0)
         for (int i = 0; i < 4; ++i) out[i] = tmp[i];
+}
+
+void launch_linesearch_ladder(int dim, int kind, const CellReduceArgs& a, const float* corr, double* partials,
+                              double tol, double* out, float* alpha, hipStream_t s)
+{
+    const int nb = kReduceBlocks / 2;  // the pair kernel's grid: the same cell chunks
+#define EGGMI_LL(D, K) linesearch_ladder<D, K><<<nb, kReduceThreads, 0, s>>>(a, corr, partials)
+    if (dim == 2) {
+        if (kind == kShapeSize)
+            EGGMI_LL(2, kShapeSize);
+        else
+            EGGMI_LL(2, kShape);
+    } else {
+        if (kind == kShapeSize)
+            EGGMI_LL(3, kShapeSize);
+        else
+            EGGMI_LL(3, kShape);
+    }
+#undef EGGMI_LL
+    ladder_finish<<<1, kReduceThreads, 0, s>>>(partials, nb, tol, out, alpha);
 }
 
 void launch_linesearch_pair(int dim, int kind, const CellReduceArgs& a, const float* corr, float a0, float a1,

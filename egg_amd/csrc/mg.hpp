@@ -31,6 +31,22 @@ void launch_vec_sub(float* out, const float* a, const float* b, long long n, hip
 
 // Fine energy and min det of x + a0 c and x + a1 c in one pass over the
 // structured cells (FP64 accumulation): out = {E(a0), md(a0), E(a1), md(a1)}.
+// The FAS correction's step-length ladder: kLadder trials alpha = 2^-(t-1),
+// t = 1..kLadder (reference src/fas.hpp:549-581, ten trials).
+constexpr int kLadder = 10;
+__host__ __device__ inline float ladder_alpha(int t) { return 1.0f / float(1 << (t - 1)); }
+// All trials (and alpha = 0) in one pass plus the pick on the device:
+// partials needs 2 (kLadder + 1) kReduceBlocks / 2 doubles, out 2 (kLadder + 1);
+// *alpha receives the accepted step (0: none).
+void launch_linesearch_ladder(int dim, int kind, const CellReduceArgs& a, const float* corr, double* partials,
+                              double tol, double* out, float* alpha, hipStream_t s);
+// a[i] = b[i] = src[i], i < n (a kernel: no copy-engine node in captured cycles)
+void launch_copy2(float* a, float* b, const float* src, long long n, hipStream_t s);
+// rep[*count] = src[0], rep[cap + *count] = src[1] (when *count < cap); ++*count.
+void launch_append_report(const double* src, double* rep, int cap, int* count, hipStream_t s);
+// x += (*alpha) c at free slots; nothing when *alpha == 0.
+void launch_masked_axpy_dev(float* x, const float* c, const float* alpha, const uint8_t* free, long long n,
+                            long long plane, int dim, hipStream_t s);
 void launch_linesearch_pair(int dim, int kind, const CellReduceArgs& a, const float* corr, float a0, float a1,
                             double* partials, double* out, hipStream_t s);
 

@@ -156,6 +156,39 @@ __global__ __launch_bounds__(256) void masked_axpy_kernel(float* x, const float*
     for (int k = 0; k < dim; ++k) x[k * plane + i] = x[k * plane + i] + alpha * c[k * plane + i];
 }
 
+__global__ __launch_bounds__(256) void masked_axpy_dev_kernel(float* x, const float* c, const float* alpha,
+                                                              const uint8_t* free, long long n, long long plane,
+                                                              int dim)
+{
+    const float al = *alpha;
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (al == 0.0f || i >= n || !free[i]) return;
+    for (int k = 0; k < dim; ++k) x[k * plane + i] = x[k * plane + i] + al * c[k * plane + i];
+}
+
+__global__ __launch_bounds__(256) void copy2_kernel(float* __restrict__ a, float* __restrict__ b,
+                                                    const float* __restrict__ src, long long n)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float v = src[i];
+    a[i] = v;
+    b[i] = v;
+}
+
+// Append one report (energy, min det) at the device counter (captured FAS
+// cycles cannot take the cycle index as an argument).
+__global__ void append_report_kernel(const double* src, double* rep, int cap, int* count)
+{
+    if (threadIdx.x != 0) return;
+    const int i = *count;
+    if (i < cap) {
+        rep[i] = src[0];
+        rep[cap + i] = src[1];
+    }
+    *count = i + 1;
+}
+
 __global__ __launch_bounds__(256) void zero_unmasked_kernel(float* c, const uint8_t* free, long long n,
                                                             long long plane, int dim)
 {
@@ -189,6 +222,22 @@ void launch_prolong(int dim, const TransferArgs& a, const float* ccorr, float* f
         prolong_kernel<2><<<nblk(a.n), 256, 0, s>>>(a, ccorr, fcorr);
     else
         prolong_kernel<3><<<nblk(a.n), 256, 0, s>>>(a, ccorr, fcorr);
+}
+
+void launch_copy2(float* a, float* b, const float* src, long long n, hipStream_t s)
+{
+    if (n > 0) copy2_kernel<<<nblk(n), 256, 0, s>>>(a, b, src, n);
+}
+
+void launch_append_report(const double* src, double* rep, int cap, int* count, hipStream_t s)
+{
+    append_report_kernel<<<1, 64, 0, s>>>(src, rep, cap, count);
+}
+
+void launch_masked_axpy_dev(float* x, const float* c, const float* alpha, const uint8_t* free, long long n,
+                            long long plane, int dim, hipStream_t s)
+{
+    if (n > 0) masked_axpy_dev_kernel<<<nblk(n), 256, 0, s>>>(x, c, alpha, free, n, plane, dim);
 }
 
 void launch_masked_axpy(float* x, const float* c, float alpha, const uint8_t* free, long long n, long long plane,

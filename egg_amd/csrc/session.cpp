@@ -798,6 +798,10 @@ void Session::release_graphs()
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
         g = SweepGraph {};
     }
+    for (auto& g : fas_graphs) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        g = FasGraph {};
+    }
 }
 
 hipGraphExec_t Session::sweep_graph(int parity, int kind, float omega, float delta, bool propose)

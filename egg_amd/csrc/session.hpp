@@ -253,6 +253,30 @@ struct Session {
     void grad(Level& L, const float* x, float* out, const float* tau, int kind);
     void restrict_to(Level& parent, const float* x_parent, Level& child, int kind);
     void coarse_vcycle(size_t li, size_t n_levels, int kind, const FasParams& o);
+    // one FAS cycle in three parts: pre-smoothing to the halo-consistent
+    // fine correction; the device line-search ladder and application; post-
+    // smoothing and the report reduction
+    void fas_head(int kind, const FasParams& o, size_t n_coarse);
+    void fas_device_ladder(int kind);
+    void fas_tail(int kind, const FasParams& o, double* out_e, double* out_m);
+    CellReduceArgs fine_cells(const float* x) const;
+    // a whole cycle captured as a hipGraph, per parity of the fine Jacobi
+    // buffer it starts from (the ladder decided on the device, the report
+    // appended to d_reports at the device counter d_rep_count)
+    struct FasGraph {
+        hipGraphExec_t exec = nullptr;
+        int kind = -1, variant = -1, nu_pre = 0, nu_post = 0, nu_coarse = 0, mode = 0;
+        size_t n_coarse = 0;
+        float omega = 0.0f, scale = 0.0f;
+        const void* mlp = nullptr;
+        const double* rep = nullptr;
+        long long launches = 0;
+    };
+    FasGraph fas_graphs[2];
+    hipGraphExec_t fas_graph(int parity, int kind, const FasParams& o, size_t n_coarse);
+    DArr<double> d_fas_reports, d_ladder_parts, d_ladder_out, d_cyc_rep;
+    DArr<float> d_alpha;
+    DArr<int> d_rep_count;
 };
 
 }  // namespace eggmi

@@ -174,8 +174,11 @@ eggmi_status eggmi_set_kernel_timing(eggmi_session* s, int on);
 
 /* Replay plain sweeps in chunks of `chunk` (even; 0 = off, the default)
  * captured once as a hipGraph per starting buffer, instead of one launch
- * each (no reference counterpart). Reports still come at the requested
- * cadence; per-launch timing and counters fall back to single launches. */
+ * each, and (chunk > 0) replay each FAS cycle as one captured graph with its
+ * line-search ladder evaluated and decided on the device (no reference
+ * counterpart; the same bits as single launches). Reports still come at the
+ * requested cadence; per-launch timing and counters fall back to single
+ * launches. */
 eggmi_status eggmi_set_graph_chunk(eggmi_session* s, int chunk);
 
 /* Diagnostic counters (measurement only; no reference counterpart): when on,

@@ -42,12 +42,16 @@ def test_fas_single_block_parity(oracle):
     np.testing.assert_allclose(s.get_X(), Xo, atol=1e-5)
 
 
-def test_fas_bump_parity(oracle):
-    """Same on the reference's smooth-bump grid (4 x 4 square, 17 x 17)."""
+@pytest.mark.parametrize("graphs", [False, True])
+def test_fas_bump_parity(oracle, graphs):
+    """Same on the reference's smooth-bump grid (4 x 4 square, 17 x 17), with
+    cycles launched one kernel at a time (line search decided on the host)
+    and replayed as captured graphs (decided on the device)."""
     from oracle import fas
 
     c = fas_single()
     s = _session(c)
+    s.core.set_graph_chunk(256 if graphs else 0)
     e, m = s.run_fas(5, nu_pre=2, nu_post=2, omega=0.8)
     Xo, eo, mo = fas.run_fas(2, "barrier", c.grid.blocks[0], c.grid.free, c.X, n_cycles=5, nu_pre=2, nu_post=2,
                              omega=0.8)
@@ -148,3 +152,31 @@ def test_fas_3d_block():
     c = Case(grids.single_block_3d(17, seed=2, amp=0.2))
     e, m = _session(c).run_fas(4, nu_pre=2, nu_post=2, omega=0.8)
     assert np.all(m > 0) and _monotone(e) and e[-1] < 0.1 * e[0]
+
+
+@pytest.mark.parametrize("make,nu", [(lambda: fas_single(amp=0.3), (2, 2)), (lambda: fas_two_blocks(), (1, 2)),
+                                     (lambda: ogrid(33, 17, tangle_seed=None), (3, 3))])
+def test_fas_graph_matches_launches(make, nu):
+    """A FAS cycle replayed as a captured hipGraph (line-search ladder and
+    its pick on the device, report appended at a device counter) gives the
+    bits of the same cycle launched kernel by kernel with the ladder decided
+    on the host: over several calls, an odd sweep count per cycle (the
+    replay alternates between the two captured parities), plain sweeps in
+    between, and a changed cycle shape (recapture)."""
+    c = make()
+    a = _session(c)
+    b = _session(c)
+    a.core.set_graph_chunk(0)
+    b.core.set_graph_chunk(256)
+    for rnd in range(3):
+        ea, ma = a.run_fas(4, nu_pre=nu[0], nu_post=nu[1], omega=0.8, max_levels=3)
+        eb, mb = b.run_fas(4, nu_pre=nu[0], nu_post=nu[1], omega=0.8, max_levels=3)
+        np.testing.assert_array_equal(eb, ea)
+        np.testing.assert_array_equal(mb, ma)
+        np.testing.assert_array_equal(b.get_X(), a.get_X())
+        a.run(3, omega=0.8)
+        b.run(3, omega=0.8)
+    ea, _ = a.run_fas(2, nu_pre=2, nu_post=1, omega=0.7, max_levels=2)
+    eb, _ = b.run_fas(2, nu_pre=2, nu_post=1, omega=0.7, max_levels=2)
+    np.testing.assert_array_equal(eb, ea)
+    np.testing.assert_array_equal(b.get_X(), a.get_X())
