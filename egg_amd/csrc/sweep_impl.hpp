@@ -157,8 +157,9 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
     for (int k = 0; k < D; ++k) pos[k] = valid ? X[k * PL + dof] : 0.0f;
 
     // One occurrence: its corner and neighbour positions and sign flips,
-    // which of them is the DOF itself, role and W row. Round 0 is loaded
-    // once and kept in registers for the line-search trials.
+    // which of them is the DOF itself, role and W row. The lane's first
+    // occurrence (r = 0) is loaded once and kept in registers for the
+    // line-search trials.
     struct Occ {
         float cn[D], nb[D][D], s[D];
         bool cd, nd[D], live;
