@@ -194,6 +194,49 @@ def _batched_sample(s, g, index, Xu, n_sweeps, target=300_000):
     return cpu, par
 
 
+def _batched_proposal(s, X0, K, e_fas, t_fas):
+    """The learned proposal in the batched workload: a network trained on
+    pairs harvested from held-out batches (four grids each, seeds 11 and
+    12, 10 snapshots 50 sweeps apart: ~10 M pairs) proposes in the fine
+    smoothing of the same K FAS V(4,4) cycles; energy and time against the
+    cycles without it."""
+    from egg_amd import grids
+    from egg_amd._cpp import cpp_core
+    from egg_amd.smoothing import cpp_untangle
+
+    net = cpp_core.ProposalMLP(2, 0, 12_000_000)
+    t0 = time.perf_counter()
+    for sd in (11, 12):
+        gh, _ = grids.batched_2d(4, seed=sd)
+        ctx_h, bsc_h, sh = _build(gh)
+        cpp_untangle(ctx_h, bsc_h, gh.X, session=sh)
+        sh.harvest(net, 10, every=50, horizon=2, omega=0.8)
+        del sh
+    t_h = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    hist = net.train(4, batch=32768, lr=1e-3, seed=1)
+    t_t = time.perf_counter() - t0
+    s.set_proposal(net, mode="predict_correct", scale=0.5)
+    s.set_X(X0)
+    s.run_fas(1, nu_pre=4, nu_post=4, omega=0.8, max_levels=3)  # warm-up (and capture)
+    s.set_X(X0)
+    t0 = time.perf_counter()
+    e, m = s.run_fas(K, nu_pre=4, nu_post=4, omega=0.8, max_levels=3)
+    t_p = time.perf_counter() - t0
+    # the fine sweep with and without the proposal (both from X0, timed)
+    s.set_X(X0)
+    _, _, t_sp, _, _ = _timed(s, 10, 0.8)
+    s.set_proposal(None)
+    s.set_X(X0)
+    _, _, t_s0, _, _ = _timed(s, 10, 0.8)
+    return {"pairs": int(net.shape()["n_data"]), "harvest_s": t_h, "train_s": t_t,
+            "loss_first_last": [float(hist[0]), float(hist[-1])], "cycles": K,
+            "energy_after": float(e[-1]), "energy_after_without": float(e_fas), "final_min_det": float(m[-1]),
+            "ms_per_cycle": t_p / K * 1e3, "ms_per_cycle_without": t_fas / K * 1e3,
+            "ms_per_fine_sweep": t_sp / 10 * 1e3, "ms_per_fine_sweep_without": t_s0 / 10 * 1e3,
+            "mode": "predict_correct", "scale": 0.5}
+
+
 def _cubed_sample(s, g, ctx, Xu, every=40):
     """Parity and CPU baseline of the 3D workload on a DOF sample (every
     `every`-th DOF of the wire): one Jacobi sweep moves each sampled DOF
@@ -413,6 +456,10 @@ def bench_batched(args):
     roof = _roofline(s, 2, n_free, 20, 0.8, k_avg, _traffic("batched", best))
     counts = _fas_levels_counts(s, 4, 4, 3)
     K = max(1, args.steps // 100)
+    s.core.set_graph_chunk(args.graph_chunk)  # FAS cycles replayed as captured graphs
+    X_fas = s.get_X()
+    s.run_fas(1, nu_pre=4, nu_post=4, omega=0.8, max_levels=3)  # warm-up (and capture)
+    s.set_X(X_fas)
     t0 = time.perf_counter()
     e, m = s.run_fas(K, nu_pre=4, nu_post=4, omega=0.8, max_levels=3)
     t_fas = time.perf_counter() - t0
@@ -420,6 +467,10 @@ def bench_batched(args):
     allv = sum(counts) * K / t_fas
     log(f"FAS V(4,4) x{K}: {t_fas * 1e3:.1f} ms, fine {fine:.4g} / all-level {allv:.4g} node-updates/s, "
         f"E {e[0]:.6g} -> {e[-1]:.6g}")
+    proposal = None
+    if not args.no_converge:
+        proposal = _batched_proposal(s, X_fas, K, e[-1], t_fas)
+        log(f"learned proposal {proposal}")
     cpu = parity = None
     if not args.no_cpu:
         cpu, parity = _batched_sample(s, g, index, Xu, 50)
@@ -438,6 +489,7 @@ def bench_batched(args):
         "autotune": {"best_variant": int(best), "ms_per_variant": [float(v) for v in ms]},
         "time_to_untangled_s": t_untangle, "untangle_outer": outer,
         "energy_first_last": [float(e[0]), float(e[-1])], "final_min_det": float(m[-1]),
+        "learned_proposal": proposal,
         "roofline": roof, "cpu_baseline": cpu, "parity": parity,
     }
 
