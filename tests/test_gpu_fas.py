@@ -154,8 +154,14 @@ def test_fas_3d_block():
     assert np.all(m > 0) and _monotone(e) and e[-1] < 0.1 * e[0]
 
 
+def _cube_case():
+    from egg_amd import grids
+
+    return Case(grids.single_block_3d(17, seed=3, amp=0.15))
+
+
 @pytest.mark.parametrize("make,nu", [(lambda: fas_single(amp=0.3), (2, 2)), (lambda: fas_two_blocks(), (1, 2)),
-                                     (lambda: ogrid(33, 17, tangle_seed=None), (3, 3))])
+                                     (lambda: ogrid(33, 17, tangle_seed=None), (3, 3)), (_cube_case, (2, 1))])
 def test_fas_graph_matches_launches(make, nu):
     """A FAS cycle replayed as a captured hipGraph (line-search ladder and
     its pick on the device, report appended at a device counter) gives the
@@ -180,3 +186,25 @@ def test_fas_graph_matches_launches(make, nu):
     eb, _ = b.run_fas(2, nu_pre=2, nu_post=1, omega=0.7, max_levels=2)
     np.testing.assert_array_equal(eb, ea)
     np.testing.assert_array_equal(b.get_X(), a.get_X())
+
+
+def test_fas_graph_with_proposal_matches_launches():
+    """The captured cycle also carries the learned proposal of the fine
+    smoothing (a random network, so some proposals are accepted)."""
+    from egg_amd._cpp import cpp_core
+    from oracle import mlp as M
+
+    c = ogrid(33, 17)
+    net = cpp_core.ProposalMLP(2, 0, 0)
+    net.set_params((np.random.default_rng(7).standard_normal(M.n_params(2)) * 0.05).astype(np.float32))
+    a = _session(c)
+    b = _session(c)
+    a.core.set_graph_chunk(0)
+    b.core.set_graph_chunk(256)
+    for s in (a, b):
+        s.set_proposal(net, mode="predict_correct", scale=0.5)
+    for _ in range(2):
+        ea, ma = a.run_fas(3, nu_pre=2, nu_post=2, omega=0.8, max_levels=3)
+        eb, mb = b.run_fas(3, nu_pre=2, nu_post=2, omega=0.8, max_levels=3)
+        np.testing.assert_array_equal(eb, ea)
+        np.testing.assert_array_equal(b.get_X(), a.get_X())
