@@ -318,6 +318,8 @@ def bench_ogrid(args):
 
     tune_best, tune_ms = s.core.autotune(20, 0.8)
     log(f"autotune: variant {tune_best}, ms per 20 sweeps {np.round(tune_ms, 4).tolist()}")
+    lv_best, lv_ms = s.core.autotune_levels(16, 0.8)  # FAS coarse levels, from the untangled state
+    log(f"level autotune: variants {lv_best}")
     s.core.set_graph_chunk(args.graph_chunk)  # replay sweeps as captured hipGraph chunks
     s.run(args.warmup, omega=0.8)
     e, m, t_run, k_avg, X_start = _timed(s, args.steps, 0.8)
@@ -414,7 +416,9 @@ def bench_ogrid(args):
         "config": {"workload": "2D 5-block O-grid (129x129 disc + 4x 129x65 rings, r=0.5 in [-2,2]^2), "
                                "2% tangled by 1.5h, untangle then block-Jacobi TMOP sweeps, omega 0.8",
                    "free_dofs": n_free, "seed": SEED, "precision": "FP32 sweeps, FP64 reductions"},
-        "autotune": {"best_variant": int(tune_best), "ms_per_20_sweeps": [float(v) for v in tune_ms]},
+        "autotune": {"best_variant": int(tune_best), "ms_per_20_sweeps": [float(v) for v in tune_ms],
+                     "fas_level_variants": [int(v) for v in lv_best],
+                     "fas_level_ms_per_16_sweeps": np.round(lv_ms, 4).tolist()},
         "graph_chunk": args.graph_chunk,
         "time_to_untangled_s": t_untangle, "untangle_outer": outer,
         "time_to_untangled_converged": ttc,
@@ -450,7 +454,8 @@ def bench_batched(args):
     log(f"untangle {t_untangle:.3f}s outer={outer} min_det={md_u:.3e}")
     t0 = time.perf_counter()
     best, ms = s.core.autotune(5, 0.8)
-    log(f"autotune {time.perf_counter() - t0:.2f}s best={best} ms/5 sweeps={ms}")
+    lv_best, lv_ms = s.core.autotune_levels(4, 0.8, 3)
+    log(f"autotune {time.perf_counter() - t0:.2f}s best={best} ms/5 sweeps={ms} levels {lv_best}")
     _, _, t_plain, k_avg, X_start = _timed(s, 20, 0.8)
     s.set_X(X_start)
     roof = _roofline(s, 2, n_free, 20, 0.8, k_avg, _traffic("batched", best))
@@ -486,7 +491,8 @@ def bench_batched(args):
         "value_is": "fine-level node-updates/s of FAS V(4,4) cycles",
         "all_level_node_updates_per_s": allv, "updates_per_cycle_by_level": counts,
         "plain_sweep_node_updates_per_s": n_free * 20 / t_plain,
-        "autotune": {"best_variant": int(best), "ms_per_variant": [float(v) for v in ms]},
+        "autotune": {"best_variant": int(best), "ms_per_variant": [float(v) for v in ms],
+                     "fas_level_variants": [int(v) for v in lv_best]},
         "time_to_untangled_s": t_untangle, "untangle_outer": outer,
         "energy_first_last": [float(e[0]), float(e[-1])], "final_min_det": float(m[-1]),
         "learned_proposal": proposal,
@@ -510,7 +516,8 @@ def bench_cubed(args):
     log(f"untangle {t_untangle:.3f}s outer={outer} min_det={md_u:.3e}")
     t0 = time.perf_counter()
     best, ms = s.core.autotune(3, 0.8)
-    log(f"autotune {time.perf_counter() - t0:.2f}s best={best} ms/3 sweeps={ms}")
+    lv_best, lv_ms = s.core.autotune_levels(3, 0.8, 3)
+    log(f"autotune {time.perf_counter() - t0:.2f}s best={best} ms/3 sweeps={ms} levels {lv_best}")
     n3 = args.steps // 20 or 1
     e, m, t_plain, k_avg, X_start = _timed(s, n3, 0.8)
     value = n_free * n3 / t_plain
@@ -544,7 +551,8 @@ def bench_cubed(args):
                              "energy_first_last": [float(ef[0]), float(ef[-1])]},
         "pcg_33cubed": {"iterations": int(iters), "relres": float(relres), "status": int(status),
                         "solve_ms": pcg_ms},
-        "autotune": {"best_variant": int(best), "ms_per_variant": [float(v) for v in ms]},
+        "autotune": {"best_variant": int(best), "ms_per_variant": [float(v) for v in ms],
+                     "fas_level_variants": [int(v) for v in lv_best]},
         "time_to_untangled_s": t_untangle, "untangle_outer": outer, "roofline": roof, "cpu_baseline": cpu,
         "parity": parity,
     }

@@ -482,6 +482,25 @@ class Session
         check(eggmi_autotune(s_, n_sweeps, omega, ms.data(), nv, &best));
         return py::make_tuple(best, to_array(ms));
     }
+    void set_level_variants(const std::vector<int>& v)
+    {
+        check(eggmi_set_level_variants(s_, v.data(), (int)v.size()));
+    }
+    py::tuple autotune_levels(int n_sweeps, double omega, int max_levels, const std::string& phase)
+    {
+        int nv = 0;
+        check(eggmi_set_tile_variant(s_, -1, &nv));
+        constexpr int kCap = 64;  // more coarse levels than any hierarchy has
+        std::vector<double> ms((size_t)kCap * nv);
+        std::vector<int> best(kCap);
+        int nl = 0;
+        check(eggmi_autotune_levels(s_, n_sweeps, omega, max_levels, phase_id(phase), ms.data(),
+                                    (int64_t)ms.size(), best.data(), kCap, &nl));
+        ms.resize((size_t)nl * nv);
+        best.resize(nl);
+        py::array_t<double> t = to_array(ms);
+        return py::make_tuple(py::cast(best), t.attr("reshape")(nl, nv));
+    }
     int64_t harvest(Mlp& m, int n_snap, int every, int horizon, double omega)
     {
         int64_t added = 0;
@@ -563,6 +582,12 @@ PYBIND11_MODULE(cpp_core, m)
            "(energy, min det) of the resident state, reduced on the device in FP64.")
       .def("autotune", &Session::autotune, py::arg("n_sweeps") = 20, py::arg("omega") = 0.8,
            "Time every tile variant on the current state (restored), keep the fastest; returns (best, ms).")
+      .def("set_level_variants", &Session::set_level_variants, py::arg("variants"),
+           "Tile variant of each FAS coarse level (variants[i] for level i + 1; -1 = the session's).")
+      .def("autotune_levels", &Session::autotune_levels, py::arg("n_sweeps") = 16, py::arg("omega") = 0.8,
+           py::arg("max_levels") = 32, py::arg("phase") = "barrier",
+           "Time every tile variant on each FAS coarse level from the restricted resident state and keep each "
+           "level's fastest (the fine state is unchanged); returns (best per level, ms [level, variant]).")
       .def("harvest", &Session::harvest, py::arg("mlp"), py::arg("n_snap"), py::kw_only(), py::arg("every") = 8,
            py::arg("horizon") = 8, py::arg("omega") = 0.8,
            "Append (features, displacement over `horizon` plain sweeps) pairs of every interior DOF to the "

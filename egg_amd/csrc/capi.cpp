@@ -257,6 +257,33 @@ eggmi_status eggmi_autotune(eggmi_session* s, int n_sweeps, double omega, double
     });
 }
 
+eggmi_status eggmi_set_level_variants(eggmi_session* s, const int* variants, int n)
+{
+    return guarded([&] {
+        need(s, "session");
+        if (n < 0) throw eggmi::InvalidArgument("n must be >= 0");
+        if (n > 0) need(variants, "variants");
+        s->s.set_level_variants(std::vector<int>(variants, variants + n));
+    });
+}
+
+eggmi_status eggmi_autotune_levels(eggmi_session* s, int n_sweeps, double omega, int max_levels, int phase,
+                                   double* ms, int64_t capacity, int* best, int best_capacity, int* n_levels)
+{
+    return guarded([&] {
+        need(s, "session");
+        need(n_levels, "n_levels");
+        const auto t = s->s.autotune_levels(n_sweeps, omega, max_levels, phase_kind(phase));
+        const int nv = eggmi::num_tile_variants(s->s.dim);
+        const int nl = (int)(t.size() / nv);
+        if (ms)
+            for (int64_t i = 0; i < capacity && i < (int64_t)t.size(); ++i) ms[i] = t[i];
+        if (best)
+            for (int i = 0; i < best_capacity && i < nl; ++i) best[i] = s->s.coarse_variant((size_t)i);
+        *n_levels = nl;
+    });
+}
+
 eggmi_status eggmi_last_timing(eggmi_session* s, double* ms, int64_t* sweep_launches, double* sweep_kernel_ms)
 {
     return guarded([&] {

@@ -380,13 +380,14 @@ std::vector<std::vector<std::array<int, 3>>> Session::mg_level_shapes(int)
     return out;
 }
 
-// Device residency of a coarse level (lazily, first run_fas that cycles it).
-void Session::upload_level(Level& L)
+// Device residency of coarse level i (lazily, first run_fas that cycles it).
+void Session::upload_level(size_t i)
 {
+    Level& L = *coarse[i];
     if (L.x[0].n) return;
     L.plane = (L.lay.n_slots() + 63) / 64 * 64;
     L.upload_tables();
-    L.make_tiles(tile_variant);
+    L.make_tiles(coarse_variant(i));
     const size_t nd = (size_t)L.plane * dim;
     for (auto* a : {&L.x[0], &L.x[1], &L.rx, &L.tau, &L.grad, &L.corr}) {
         a->alloc(nd);
@@ -445,7 +446,7 @@ void Session::smooth(Level& L, int sweeps, int kind, float omega, const float* t
             a.prop_mode = proposal_mode;
             a.prop_scale = proposal_scale;
         }
-        launch_sweep(dim, kind, tile_variant, a, stream);
+        launch_sweep(dim, kind, L.variant, a, stream);
         L.cur = 1 - L.cur;
         ++last_launches;
     }
@@ -455,7 +456,7 @@ void Session::grad(Level& L, const float* x, float* out, const float* tau, int k
 {
     SweepArgs a = L.sweep_args(x, nullptr, 1.0f, 0.0f, tau);
     a.gout = out;
-    launch_grad(dim, kind, tile_variant, a, stream);
+    launch_grad(dim, kind, L.variant, a, stream);
 }
 
 // Restrict the parent state x_parent into child (x, rx and the second
@@ -497,35 +498,11 @@ std::pair<std::vector<double>, std::vector<double>> Session::run_fas(int n_cycle
 {
     need_device();
     if (n_cycles <= 0) return {{}, {}};
-    build_hierarchy();
     const int per_cycle = nu_pre + nu_post;
-    const size_t n_coarse = max_levels > 1 ? std::min(coarse.size(), (size_t)(max_levels - 1)) : 0;
+    const size_t n_coarse = prepare_fas(max_levels);
     if (n_coarse == 0) return run_jacobi(n_cycles * per_cycle, kind, 0.0, omega, per_cycle);
 
     Level& F = fine;
-    if (!F.grad.n) {
-        const size_t nd = (size_t)F.plane * dim;
-        for (auto* a : {&F.grad, &F.corr}) {
-            a->alloc(nd);
-            EGGMI_HIP(hipMemset(a->p, 0, nd * sizeof(float)));
-        }
-        F.d_free.upload(fine_free);
-    }
-    for (size_t i = 0; i < n_coarse; ++i) upload_level(*coarse[i]);
-    // prolongation enumerates the interior nodes of the parent of each level
-    for (size_t i = 0; i < n_coarse; ++i) {
-        Level& P = i == 0 ? F : *coarse[i - 1];
-        if (P.d_fine_off.n) continue;
-        const size_t nb = P.lay.shape.size();
-        std::vector<long long> off(nb + 1, 0);
-        for (size_t b = 0; b < nb; ++b) {
-            long long c = 1;
-            for (int k = 0; k < dim; ++k) c *= P.lay.shape[b][k];
-            off[b + 1] = off[b] + c;
-        }
-        P.d_fine_off.upload(off);
-        P.n_fine_nodes = off.back();
-    }
     FasParams o;
     o.nu_pre = nu_pre;
     o.nu_post = nu_post;
@@ -625,6 +602,132 @@ std::pair<std::vector<double>, std::vector<double>> Session::run_fas(int n_cycle
     EGGMI_HIP(hipEventElapsedTime(&ms, ev0, ev1));
     last_ms = ms;
     return {e, m};
+}
+
+size_t Session::prepare_fas(int max_levels)
+{
+    build_hierarchy();
+    const size_t n_coarse = max_levels > 1 ? std::min(coarse.size(), (size_t)(max_levels - 1)) : 0;
+    if (n_coarse == 0) return 0;
+    Level& F = fine;
+    if (!F.grad.n) {
+        const size_t nd = (size_t)F.plane * dim;
+        for (auto* a : {&F.grad, &F.corr}) {
+            a->alloc(nd);
+            EGGMI_HIP(hipMemset(a->p, 0, nd * sizeof(float)));
+        }
+        F.d_free.upload(fine_free);
+    }
+    for (size_t i = 0; i < n_coarse; ++i) upload_level(i);
+    // prolongation enumerates the interior nodes of the parent of each level
+    for (size_t i = 0; i < n_coarse; ++i) {
+        Level& P = i == 0 ? F : *coarse[i - 1];
+        if (P.d_fine_off.n) continue;
+        const size_t nb = P.lay.shape.size();
+        std::vector<long long> off(nb + 1, 0);
+        for (size_t b = 0; b < nb; ++b) {
+            long long c = 1;
+            for (int k = 0; k < dim; ++k) c *= P.lay.shape[b][k];
+            off[b + 1] = off[b] + c;
+        }
+        P.d_fine_off.upload(off);
+        P.n_fine_nodes = off.back();
+    }
+    return n_coarse;
+}
+
+// Coarse levels run below one wave per SIMD, so a sweep there lasts about
+// one wave's lifetime and the tile shape that spreads a DOF over the most
+// lanes can win, while the fine level is throughput-bound. Each level is
+// timed from the state a cycle's first restriction gives it (the resident
+// fine state restricted down the chain, no smoothing); coarse-level buffers
+// are rewritten by every cycle's restriction, and the fine state is only
+// read (its replicas refreshed, as run_fas does first).
+std::vector<double> Session::autotune_levels(int n_sweeps, double omega, int max_levels, int kind)
+{
+    need_device();
+    if (n_sweeps < 1) throw InvalidArgument("autotune_levels: n_sweeps must be >= 1");
+    if (kind == 1) throw InvalidArgument("autotune_levels: FAS smooths the barrier phases only");
+    const size_t n_coarse = prepare_fas(max_levels);
+    const int nv = num_tile_variants(dim);
+    std::vector<double> ms(n_coarse * nv, 0.0);
+    if (n_coarse == 0) return ms;
+    const bool count0 = counting;
+    const long long launches0 = last_launches;
+    counting = false;
+    std::vector<int> picks(n_coarse);
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    auto drop = [&] {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    };
+    try {
+        Level& F = fine;
+        const float* x = F.x[F.cur].p;
+        halo(F, F.x[F.cur].p);
+        grad(F, x, F.grad.p, nullptr, kind);
+        restrict_to(F, x, *coarse[0], kind);
+        for (size_t i = 0; i < n_coarse; ++i) {
+            Level& L = *coarse[i];
+            const long long n = (long long)L.plane * dim;
+            auto reset = [&] {  // both Jacobi buffers <- the restricted state
+                launch_copy2(L.x[0].p, L.x[1].p, L.rx.p, n, stream);
+                L.cur = 0;
+            };
+            for (int v = 0; v < nv; ++v) {
+                // timed as the captured graph a cycle replays them in: the
+                // launch gaps are the graph's, not the host's launch rate
+                L.make_tiles(v);
+                reset();
+                EGGMI_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+                try {
+                    smooth(L, n_sweeps, kind, (float)omega, L.tau.p);
+                } catch (...) {
+                    (void)hipStreamEndCapture(stream, &graph);
+                    throw;
+                }
+                EGGMI_HIP(hipStreamEndCapture(stream, &graph));
+                EGGMI_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+                EGGMI_HIP(hipGraphLaunch(exec, stream));  // warm the variant's code
+                reset();
+                EGGMI_HIP(hipEventRecord(ev0, stream));
+                EGGMI_HIP(hipGraphLaunch(exec, stream));
+                EGGMI_HIP(hipEventRecord(ev1, stream));
+                EGGMI_HIP(hipEventSynchronize(ev1));
+                float t = 0;
+                EGGMI_HIP(hipEventElapsedTime(&t, ev0, ev1));
+                ms[i * nv + v] = t;
+                drop();
+                reset();
+            }
+            int best = 0;
+            for (int v = 1; v < nv; ++v)
+                if (ms[i * nv + v] < ms[i * nv + best]) best = v;
+            picks[i] = best;
+            L.make_tiles(best);
+            if (i + 1 < n_coarse) {
+                grad(L, L.x[L.cur].p, L.grad.p, L.tau.p, kind);
+                restrict_to(L, L.x[L.cur].p, *coarse[i + 1], kind);
+            }
+        }
+        EGGMI_HIP(hipStreamSynchronize(stream));
+    } catch (...) {
+        drop();
+        counting = count0;
+        last_launches = launches0;
+        for (size_t i = 0; i < n_coarse; ++i) coarse[i]->make_tiles(coarse_variant(i));
+        throw;
+    }
+    counting = count0;
+    last_launches = launches0;
+    std::vector<int> lv = level_variant;
+    if (lv.size() < n_coarse) lv.resize(n_coarse, -1);
+    for (size_t i = 0; i < n_coarse; ++i) lv[i] = picks[i];
+    set_level_variants(lv);
+    return ms;
 }
 
 CellReduceArgs Session::fine_cells(const float* x) const

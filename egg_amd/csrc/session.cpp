@@ -63,9 +63,10 @@ long long Layout::block_padded_count(int b) const
 
 // ------------------------------------------------------------------ level
 
-void Level::make_tiles(int variant)
+void Level::make_tiles(int v)
 {
-    const TileShape ts = tile_variant_shape(lay.dim, variant);
+    const TileShape ts = tile_variant_shape(lay.dim, v);
+    variant = v;
     // axis 0 <- tz (3D) / ty (2D), last axis <- tx
     const int step[3] = {lay.dim == 2 ? ts.ty : ts.tz, lay.dim == 2 ? ts.tx : ts.ty, lay.dim == 2 ? 1 : ts.tx};
     tiles.clear();
@@ -919,7 +920,24 @@ void Session::set_tile_variant(int v)
     if (host_only) return;
     release_graphs();  // they hold the tile table being replaced
     fine.make_tiles(v);
-    for (auto& c : coarse) c->make_tiles(v);
+    for (size_t i = 0; i < coarse.size(); ++i)
+        if (coarse[i]->x[0].n) coarse[i]->make_tiles(coarse_variant(i));
+}
+
+int Session::coarse_variant(size_t i) const
+{
+    return i < level_variant.size() && level_variant[i] >= 0 ? level_variant[i] : tile_variant;
+}
+
+void Session::set_level_variants(const std::vector<int>& v)
+{
+    for (int x : v)
+        if (x < -1 || x >= num_tile_variants(dim)) throw InvalidArgument("level tile variant out of range");
+    level_variant = v;
+    if (host_only) return;
+    release_graphs();
+    for (size_t i = 0; i < coarse.size(); ++i)
+        if (coarse[i]->x[0].n) coarse[i]->make_tiles(coarse_variant(i));
 }
 
 std::vector<double> Session::autotune(int n_sweeps, double omega)

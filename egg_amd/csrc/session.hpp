@@ -127,6 +127,7 @@ struct Level {
     long long n_synth = 0;
     std::vector<TileDesc> tiles;
     DArr<TileDesc> d_tiles;
+    int variant = 0;  // the tile variant the tile table is laid out for
     std::vector<int> syn_slot, syn_blk;  // synthesized DOFs as a list (lazily, for harvesting)
     DArr<int> d_syn_slot, d_syn_blk;
     void ensure_synth_list();
@@ -163,7 +164,7 @@ struct Level {
 
     BlockDesc block_desc(size_t b) const;
     void upload_tables();
-    void make_tiles(int variant);
+    void make_tiles(int v);
     void build_push();
     SweepArgs sweep_args(const float* xin, float* xout, float omega, float delta, const float* tau) const;
 };
@@ -213,6 +214,9 @@ struct Session {
     // FAS hierarchy (built lazily)
     std::vector<std::unique_ptr<Level>> coarse;
     int built_levels = -1;
+    // tile variant of each coarse level (-1 = the session's tile_variant)
+    std::vector<int> level_variant;
+    int coarse_variant(size_t i) const;
     std::vector<uint8_t> fine_free;  // relaxed fine DOF slots
     HaloHost fine_halo;
     DArr<double> d_ls;
@@ -245,9 +249,16 @@ struct Session {
     // time n_sweeps plain sweeps per tile variant from the current state
     // (restored after each trial), keep the fastest; returns ms per variant
     std::vector<double> autotune(int n_sweeps, double omega);
+    // per-level tile variants of the FAS coarse levels (-1 = tile_variant)
+    void set_level_variants(const std::vector<int>& v);
+    // time n_sweeps FAS-shifted sweeps per tile variant on each of the first
+    // max_levels - 1 coarse levels, from the fine state restricted down the
+    // chain, and keep each level's fastest; returns ms [level][variant]
+    std::vector<double> autotune_levels(int n_sweeps, double omega, int max_levels, int kind);
     // FAS internals
     void build_hierarchy();
-    void upload_level(Level& L);
+    size_t prepare_fas(int max_levels);  // device state of the levels a cycle uses; their count
+    void upload_level(size_t i);
     TransferArgs transfer(const Level& parent, const Level& child, bool prolong) const;
     void smooth(Level& L, int sweeps, int kind, float omega, const float* tau);
     void grad(Level& L, const float* x, float* out, const float* tau, int kind);

@@ -164,6 +164,27 @@ eggmi_status eggmi_energy(eggmi_session* s, int phase, double delta, double* ene
  * variant's time, *best the variant selected. */
 eggmi_status eggmi_autotune(eggmi_session* s, int n_sweeps, double omega, double* ms, int capacity, int* best);
 
+/* Tile variant of each FAS coarse level (no reference counterpart):
+ * variants[i] for coarse level i + 1 (level 0 is the fine grid, which keeps
+ * the session's variant), -1 = the session's variant; levels beyond n
+ * follow the session's variant, and n = 0 clears every choice. Coarse levels
+ * never run the learned proposal, so every variant is allowed there. */
+eggmi_status eggmi_set_level_variants(eggmi_session* s, const int* variants, int n);
+
+/* Level autotuner (no reference counterpart). Coarse levels have far fewer
+ * DOFs than the chip has SIMDs, so a sweep there lasts about one wave's
+ * lifetime and a different tile shape can win than on the fine grid. For
+ * each of the first max_levels - 1 coarse levels, from the resident fine
+ * state restricted down the chain (as a cycle's first restriction, without
+ * smoothing), time n_sweeps FAS-shifted sweeps of `phase` (barrier phases
+ * only), replayed as one captured hipGraph as a cycle replays them, with
+ * every tile variant and keep the fastest. The fine state is
+ * not changed. ms[l * V + v] (< capacity; V = the variant count) receives
+ * level l + 1's time with variant v, best[l] (< best_capacity) its pick,
+ * *n_levels the number of levels tuned. */
+eggmi_status eggmi_autotune_levels(eggmi_session* s, int n_sweeps, double omega, int max_levels, int phase,
+                                   double* ms, int64_t capacity, int* best, int best_capacity, int* n_levels);
+
 /* Device time of the last eggmi_run / eggmi_run_fas call in ms (HIP events
  * on the session stream), the number of sweep-kernel launches it made and,
  * when kernel timing is on, the summed duration of those launches. */

@@ -208,3 +208,74 @@ def test_fas_graph_with_proposal_matches_launches():
         eb, mb = b.run_fas(3, nu_pre=2, nu_post=2, omega=0.8, max_levels=3)
         np.testing.assert_array_equal(eb, ea)
         np.testing.assert_array_equal(b.get_X(), a.get_X())
+
+
+@pytest.mark.parametrize("graphs", [0, 256])
+def test_level_variants_one_lane_per_cell_same_bits(graphs):
+    """Each coarse level can run its own tile shape. The one-lane-per-cell
+    shapes (2D variants 0-3, 6, 7) sum a patch in the same order, so cycles
+    whose coarse levels use several of them give the bits of variant 0
+    throughout; one-lane-per-sample shapes on the coarse levels (a different
+    summation order) change the bits, which shows the choice is applied."""
+    c = ogrid(65, 33)
+    a, b, d = _session(c), _session(c), _session(c)
+    for s in (a, b, d):
+        s.core.set_graph_chunk(graphs)
+        s.core.set_tile_variant(0)
+    b.core.set_level_variants([6, 7, 2])
+    d.core.set_level_variants([4, 5, 4])
+    for _ in range(2):
+        ea, ma = a.run_fas(3, nu_pre=2, nu_post=2, omega=0.8, max_levels=4)
+        eb, mb = b.run_fas(3, nu_pre=2, nu_post=2, omega=0.8, max_levels=4)
+        ed, md = d.run_fas(3, nu_pre=2, nu_post=2, omega=0.8, max_levels=4)
+        np.testing.assert_array_equal(eb, ea)
+        np.testing.assert_array_equal(mb, ma)
+        np.testing.assert_array_equal(b.get_X(), a.get_X())
+        np.testing.assert_allclose(ed, ea, rtol=1e-4)
+    assert not np.array_equal(d.get_X(), a.get_X())
+    np.testing.assert_allclose(d.get_X(), a.get_X(), atol=5e-4)
+
+
+def test_level_variants_fas_parity(oracle):
+    """Coarse levels on one-lane-per-sample and cells-per-lane shapes keep
+    per-cycle parity with the FP64 FAS oracle (levels 17, 9 and 5 here)."""
+    from oracle import fas
+
+    c = single(n=33, seed=1)
+    s = _session(c)
+    s.core.set_level_variants([4, 9, 5])
+    e, m = s.run_fas(6, nu_pre=2, nu_post=2, omega=0.8)
+    Xo, eo, mo = fas.run_fas(2, "barrier", c.grid.blocks[0], c.grid.free, c.X, n_cycles=6, nu_pre=2, nu_post=2,
+                             omega=0.8)
+    assert len(e) == 6 and np.all(m > 0)
+    np.testing.assert_allclose(e, eo, rtol=1e-3, atol=1e-5 * float(eo[0]))
+    np.testing.assert_allclose(s.get_X(), Xo, atol=1e-5)
+
+
+@pytest.mark.parametrize("make", [lambda: ogrid(65, 33), _cube_case])
+def test_autotune_levels(make):
+    """The level autotuner times every variant on each coarse level, leaves
+    the fine state as it was, and the cycles that follow give the bits of a
+    session told the same picks directly; out-of-range picks and the
+    untangle phase are refused."""
+    c = make()
+    s, t = _session(c), _session(c)
+    for x in (s, t):
+        x.run(5, omega=0.8)
+    nv = s.core.set_tile_variant(-1)
+    X0 = s.get_X()
+    best, ms = s.core.autotune_levels(4, 0.8, 4)
+    n_lv = len(s.mg_levels())
+    assert ms.shape == (min(3, n_lv), nv) and np.all(ms > 0)
+    assert len(best) == ms.shape[0] and all(int(np.argmin(ms[i])) == best[i] for i in range(len(best)))
+    np.testing.assert_array_equal(s.get_X(), X0)
+    t.core.set_level_variants(best)
+    for _ in range(2):
+        es, ms_ = s.run_fas(2, nu_pre=2, nu_post=2, omega=0.8, max_levels=4)
+        et, mt = t.run_fas(2, nu_pre=2, nu_post=2, omega=0.8, max_levels=4)
+        np.testing.assert_array_equal(es, et)
+        np.testing.assert_array_equal(s.get_X(), t.get_X())
+    with pytest.raises(ValueError, match="level tile variant"):
+        s.core.set_level_variants([nv])
+    with pytest.raises(ValueError, match="barrier"):
+        s.core.autotune_levels(2, 0.8, 3, "untangle")

@@ -201,5 +201,24 @@ def test_capi_error_codes_without_device():
     assert b"session" in lib.eggmi_last_error()
     assert lib.eggmi_autotune(None, 5, ctypes.c_double(0.8), None, 0, None) == 1
     assert lib.eggmi_energy(None, 0, ctypes.c_double(0.0), None, None) == 1
+    assert lib.eggmi_set_level_variants(None, None, 0) == 1
+    n = ctypes.c_int()
+    assert lib.eggmi_autotune_levels(None, 4, ctypes.c_double(0.8), 3, 0, None, ctypes.c_int64(0), None, 0,
+                                     ctypes.byref(n)) == 1
+    assert b"session" in lib.eggmi_last_error()
     assert lib.eggmi_mlp_destroy(None) == 0  # deleting nothing is fine
     assert lib.eggmi_session_destroy(None) == 0
+
+
+def test_level_variants_validated_without_device():
+    """Per-level tile variants are range-checked (-1 = the session's)."""
+    from egg_amd.smoothing import structured_arrays
+
+    c = single(n=17)
+    s = _core().CppStructuredSweepSession(c.ctx.wire, structured_arrays(c.bsc), c.X.ravel(), device="tables")
+    nv = s.set_tile_variant(-1)
+    s.set_level_variants([-1, nv - 1, 0])
+    s.set_level_variants([])
+    for bad in ([nv], [-2], [0, nv + 3]):
+        with pytest.raises(ValueError, match="level tile variant"):
+            s.set_level_variants(bad)
