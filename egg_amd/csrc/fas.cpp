@@ -389,7 +389,7 @@ void Session::upload_level(size_t i)
     L.upload_tables();
     L.make_tiles(coarse_variant(i));
     const size_t nd = (size_t)L.plane * dim;
-    for (auto* a : {&L.x[0], &L.x[1], &L.rx, &L.tau, &L.grad, &L.corr}) {
+    for (auto* a : {&L.x[0], &L.x[1], &L.rx, &L.tau, &L.grad}) {
         a->alloc(nd);
         EGGMI_HIP(hipMemset(a->p, 0, nd * sizeof(float)));
     }
@@ -452,15 +452,17 @@ void Session::smooth(Level& L, int sweeps, int kind, float omega, const float* t
     }
 }
 
-void Session::grad(Level& L, const float* x, float* out, const float* tau, int kind)
+void Session::grad(Level& L, const float* x, float* out, const float* tau, int kind, bool sub)
 {
     SweepArgs a = L.sweep_args(x, nullptr, 1.0f, 0.0f, tau);
     a.gout = out;
+    a.gsub = sub;
     launch_grad(dim, kind, L.variant, a, stream);
 }
 
 // Restrict the parent state x_parent into child (x, rx and the second
-// Jacobi buffer) and the parent's (shifted) gradient into tau = grad_c(Rx) - R grad.
+// Jacobi buffer) and the parent's (shifted) gradient into tau = grad_c(Rx) -
+// R grad (at the child's DOF slots, the only ones its sweeps read).
 void Session::restrict_to(Level& parent, const float* x_parent, Level& child, int kind)
 {
     halo(parent, parent.grad.p);
@@ -469,8 +471,7 @@ void Session::restrict_to(Level& parent, const float* x_parent, Level& child, in
     child.cur = 0;
     halo(child, child.x[0].p);
     launch_copy2(child.rx.p, child.x[1].p, child.x[0].p, (long long)child.plane * dim, stream);
-    grad(child, child.rx.p, child.grad.p, nullptr, kind);
-    launch_vec_sub(child.tau.p, child.grad.p, child.tau.p, (long long)child.plane * dim, stream);
+    grad(child, child.rx.p, child.tau.p, nullptr, kind, true);  // tau = grad_c(Rx) - R grad at the DOFs
 }
 
 void Session::coarse_vcycle(size_t li, size_t n_levels, int kind, const FasParams& o)
@@ -485,9 +486,8 @@ void Session::coarse_vcycle(size_t li, size_t n_levels, int kind, const FasParam
     grad(L, L.x[L.cur].p, L.grad.p, L.tau.p, kind);
     restrict_to(L, L.x[L.cur].p, N, kind);
     coarse_vcycle(li + 1, n_levels, kind, o);
-    launch_vec_sub(N.grad.p, N.x[N.cur].p, N.rx.p, (long long)N.plane * dim, stream);
-    launch_prolong(dim, transfer(L, N, true), N.grad.p, L.corr.p, stream);
-    launch_masked_axpy(L.x[L.cur].p, L.corr.p, 1.0f, L.d_free.p, L.lay.n_slots(), L.plane, dim, stream);
+    // x += P(x_c - rx_c) at free slots, one launch
+    launch_prolong(dim, transfer(L, N, true), N.x[N.cur].p, N.rx.p, L.x[L.cur].p, L.d_free.p, true, stream);
     halo(L, L.x[L.cur].p);
     smooth(L, o.nu_post, kind, o.omega, L.tau.p);
 }
@@ -752,9 +752,8 @@ void Session::fas_head(int kind, const FasParams& o, size_t n_coarse)
     grad(F, x, F.grad.p, nullptr, kind);
     restrict_to(F, x, C0, kind);
     coarse_vcycle(0, n_coarse, kind, o);
-    launch_vec_sub(C0.grad.p, C0.x[C0.cur].p, C0.rx.p, (long long)C0.plane * dim, stream);
-    launch_prolong(dim, transfer(F, C0, true), C0.grad.p, F.corr.p, stream);
-    launch_zero_unmasked(F.corr.p, F.d_free.p, F.lay.n_slots(), F.plane, dim, stream);
+    // fine correction P(x_c - rx_c), zero where not free; its replicas
+    launch_prolong(dim, transfer(F, C0, true), C0.x[C0.cur].p, C0.rx.p, F.corr.p, F.d_free.p, false, stream);
     halo(F, F.corr.p);
 }
 

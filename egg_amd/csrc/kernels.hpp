@@ -67,6 +67,7 @@ struct SweepArgs {
     float delta = 0.0f;
     const float* tau = nullptr;  // FAS linear shift (plane layout), null = none
     float* gout = nullptr;       // gradient-only launches: per-node gradient (plane layout)
+    bool gsub = false;           // gradient-only: gout = gradient - gout, in place
     const int* cp_off = nullptr;  // [slots+1] replicas of each slot (push copies), null = none
     const int* cp_dst = nullptr;
     float Wu[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};  // uniform W_inv for stored occurrences
@@ -114,7 +115,8 @@ constexpr int kReduceThreads = 256;
 
 // Launchers (stream-ordered, asynchronous). kind: 0 shape, 1 untangle, 2 shape_size.
 void launch_sweep(int dim, int kind, int tile_variant, const SweepArgs& a, hipStream_t s);
-// Metric pass only: per-DOF gradient (minus tau when given) into a.gout.
+// Metric pass only: per-DOF gradient (minus tau when given) into a.gout
+// (a.gsub: a.gout = gradient - a.gout, in place).
 void launch_grad(int dim, int kind, int tile_variant, const SweepArgs& a, hipStream_t s);
 void launch_copy(float* x, long long plane, int dim, const int* src, const int* dst, long long n, hipStream_t s);
 void launch_reduce_cells(int dim, int kind, const CellReduceArgs& a, double* partials, hipStream_t s);

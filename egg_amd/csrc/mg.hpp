@@ -23,11 +23,13 @@ struct TransferArgs {
 
 void launch_restrict(int dim, const TransferArgs& a, const float* fx, float* cx, const float* fgrad, float* ctau,
                      const uint8_t* cfree, hipStream_t s);
-void launch_prolong(int dim, const TransferArgs& a, const float* ccorr, float* fcorr, hipStream_t s);
+// Multilinear prolongation of the coarse change cx - crx over the fine
+// level's real nodes: add = true adds it to fout at free fine slots; add =
+// false writes it to fout there and zero at the other real nodes.
+void launch_prolong(int dim, const TransferArgs& a, const float* cx, const float* crx, float* fout,
+                    const uint8_t* ffree, bool add, hipStream_t s);
 void launch_masked_axpy(float* x, const float* c, float alpha, const uint8_t* free, long long n, long long plane,
                         int dim, hipStream_t s);
-void launch_zero_unmasked(float* c, const uint8_t* free, long long n, long long plane, int dim, hipStream_t s);
-void launch_vec_sub(float* out, const float* a, const float* b, long long n, hipStream_t s);
 
 // Fine energy and min det of x + a0 c and x + a1 c in one pass over the
 // structured cells (FP64 accumulation): out = {E(a0), md(a0), E(a1), md(a1)}.

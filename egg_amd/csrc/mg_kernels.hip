@@ -6,9 +6,9 @@
 //                        the ghost-inclusive fine gradient, at free coarse
 //                        nodes (zero elsewhere)
 //   prolong_correction   multilinear interpolation of the coarse correction
+//                        x_c - rx_c, added at a coarse level's free slots or
+//                        written as the fine correction (zero where not free)
 //   masked_axpy          x += a c at free slots
-//   zero_unmasked        c = 0 at non-free slots
-//   vec_sub              out = a - b
 //   linesearch_pair      fine energy / min det at x + a0 c and x + a1 c
 // (reference src/mg_transfer.hpp:116-341 and src/fas.hpp:174-234).
 #include "mg.hpp"
@@ -107,9 +107,14 @@ template <int D> __global__ __launch_bounds__(256) void restrict_kernel(const Tr
     }
 }
 
-// Threads enumerate the fine interior nodes of every block.
-template <int D> __global__ __launch_bounds__(256) void prolong_kernel(const TransferArgs a, const float* ccorr,
-                                                                      float* fcorr)
+// Threads enumerate the fine interior nodes of every block. The coarse
+// correction is the coarse solve's change cx - crx, interpolated
+// multilinearly. ADD: added in place to the fine state at its free slots (a
+// coarse level's correction); otherwise written as the fine correction, zero
+// at the nodes that are not free (the step the line search scales).
+template <int D, bool ADD>
+__global__ __launch_bounds__(256) void prolong_kernel(const TransferArgs a, const float* cx, const float* crx,
+                                                      float* fout, const uint8_t* ffree)
 {
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     if (i >= a.n) return;
@@ -141,11 +146,18 @@ template <int D> __global__ __launch_bounds__(256) void prolong_kernel(const Tra
         if (!ok) continue;
         const long long s = padded_slot<D>(cb, C);
 #pragma unroll
-        for (int k = 0; k < D; ++k) acc[k] += w0 * ccorr[k * a.cplane + s];
+        for (int k = 0; k < D; ++k) acc[k] += w0 * (cx[k * a.cplane + s] - crx[k * a.cplane + s]);
     }
     const long long fs = padded_slot<D>(fb, P);
+    const bool fr = ffree[fs] != 0;
+    if constexpr (ADD) {
+        if (!fr) return;
 #pragma unroll
-    for (int k = 0; k < D; ++k) fcorr[k * a.fplane + fs] = acc[k];
+        for (int k = 0; k < D; ++k) fout[k * a.fplane + fs] = fout[k * a.fplane + fs] + acc[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < D; ++k) fout[k * a.fplane + fs] = fr ? acc[k] : 0.0f;
+    }
 }
 
 __global__ __launch_bounds__(256) void masked_axpy_kernel(float* x, const float* c, float alpha, const uint8_t* free,
@@ -189,20 +201,6 @@ __global__ void append_report_kernel(const double* src, double* rep, int cap, in
     *count = i + 1;
 }
 
-__global__ __launch_bounds__(256) void zero_unmasked_kernel(float* c, const uint8_t* free, long long n,
-                                                            long long plane, int dim)
-{
-    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n || free[i]) return;
-    for (int k = 0; k < dim; ++k) c[k * plane + i] = 0.0f;
-}
-
-__global__ __launch_bounds__(256) void vec_sub_kernel(float* out, const float* a, const float* b, long long n)
-{
-    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (i < n) out[i] = a[i] - b[i];
-}
-
 static unsigned nblk(long long n) { return (unsigned)((n + 255) / 256); }
 
 void launch_restrict(int dim, const TransferArgs& a, const float* fx, float* cx, const float* fgrad, float* ctau,
@@ -215,13 +213,21 @@ void launch_restrict(int dim, const TransferArgs& a, const float* fx, float* cx,
         restrict_kernel<3><<<nblk(a.n), 256, 0, s>>>(a, fx, cx, fgrad, ctau, cfree);
 }
 
-void launch_prolong(int dim, const TransferArgs& a, const float* ccorr, float* fcorr, hipStream_t s)
+void launch_prolong(int dim, const TransferArgs& a, const float* cx, const float* crx, float* fout,
+                    const uint8_t* ffree, bool add, hipStream_t s)
 {
     if (a.n <= 0) return;
-    if (dim == 2)
-        prolong_kernel<2><<<nblk(a.n), 256, 0, s>>>(a, ccorr, fcorr);
-    else
-        prolong_kernel<3><<<nblk(a.n), 256, 0, s>>>(a, ccorr, fcorr);
+    if (dim == 2) {
+        if (add)
+            prolong_kernel<2, true><<<nblk(a.n), 256, 0, s>>>(a, cx, crx, fout, ffree);
+        else
+            prolong_kernel<2, false><<<nblk(a.n), 256, 0, s>>>(a, cx, crx, fout, ffree);
+    } else {
+        if (add)
+            prolong_kernel<3, true><<<nblk(a.n), 256, 0, s>>>(a, cx, crx, fout, ffree);
+        else
+            prolong_kernel<3, false><<<nblk(a.n), 256, 0, s>>>(a, cx, crx, fout, ffree);
+    }
 }
 
 void launch_copy2(float* a, float* b, const float* src, long long n, hipStream_t s)
@@ -244,16 +250,6 @@ void launch_masked_axpy(float* x, const float* c, float alpha, const uint8_t* fr
                         int dim, hipStream_t s)
 {
     if (n > 0) masked_axpy_kernel<<<nblk(n), 256, 0, s>>>(x, c, alpha, free, n, plane, dim);
-}
-
-void launch_zero_unmasked(float* c, const uint8_t* free, long long n, long long plane, int dim, hipStream_t s)
-{
-    if (n > 0) zero_unmasked_kernel<<<nblk(n), 256, 0, s>>>(c, free, n, plane, dim);
-}
-
-void launch_vec_sub(float* out, const float* a, const float* b, long long n, hipStream_t s)
-{
-    if (n > 0) vec_sub_kernel<<<nblk(n), 256, 0, s>>>(out, a, b, n);
 }
 
 }  // namespace eggmi

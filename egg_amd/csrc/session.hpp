@@ -160,7 +160,7 @@ struct Level {
     DArr<int> d_factor;
     DArr<long long> d_coarse_off, d_fine_off;  // node-count prefixes for the transfers
     long long n_coarse_nodes = 0, n_fine_nodes = 0;
-    DArr<float> rx, tau, grad, corr;
+    DArr<float> rx, tau, grad, corr;  // corr: the fine level's correction only
 
     BlockDesc block_desc(size_t b) const;
     void upload_tables();
@@ -261,7 +261,8 @@ struct Session {
     void upload_level(size_t i);
     TransferArgs transfer(const Level& parent, const Level& child, bool prolong) const;
     void smooth(Level& L, int sweeps, int kind, float omega, const float* tau);
-    void grad(Level& L, const float* x, float* out, const float* tau, int kind);
+    // per-DOF (tau-shifted) gradient into out; sub: out = gradient - out
+    void grad(Level& L, const float* x, float* out, const float* tau, int kind, bool sub = false);
     void restrict_to(Level& parent, const float* x_parent, Level& child, int kind);
     void coarse_vcycle(size_t li, size_t n_levels, int kind, const FasParams& o);
     // one FAS cycle in three parts: pre-smoothing to the halo-consistent

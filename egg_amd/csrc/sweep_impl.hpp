@@ -263,7 +263,10 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
     if constexpr (GRAD) {
         if (l == 0)
 #pragma unroll
-            for (int k = 0; k < D; ++k) a.gout[k * PL + dof] = g[k];
+            for (int k = 0; k < D; ++k) {
+                float& o = a.gout[k * PL + dof];
+                o = a.gsub ? g[k] - o : g[k];
+            }
         return;
     }
 #pragma unroll
@@ -592,7 +595,10 @@ __device__ void tile_sweep(const SweepArgs& a, int tile)
     if constexpr (GRAD) {
         if (gl == 0)
 #pragma unroll
-            for (int k = 0; k < D; ++k) a.gout[k * PL + node] = g[k];
+            for (int k = 0; k < D; ++k) {
+                float& o = a.gout[k * PL + node];
+                o = a.gsub ? g[k] - o : g[k];
+            }
         return;
     }
 #pragma unroll
@@ -747,7 +753,10 @@ __device__ void sample_sweep(const SweepArgs& a, int tile)
     if constexpr (GRAD) {
         if (q == 0)
 #pragma unroll
-            for (int k = 0; k < D; ++k) a.gout[k * PL + node] = g[k];
+            for (int k = 0; k < D; ++k) {
+                float& o = a.gout[k * PL + node];
+                o = a.gsub ? g[k] - o : g[k];
+            }
         return;
     }
 #pragma unroll
