@@ -452,20 +452,21 @@ void Session::smooth(Level& L, int sweeps, int kind, float omega, const float* t
     }
 }
 
-void Session::grad(Level& L, const float* x, float* out, const float* tau, int kind, bool sub)
+void Session::grad(Level& L, const float* x, float* out, const float* tau, int kind, bool sub, bool push)
 {
     SweepArgs a = L.sweep_args(x, nullptr, 1.0f, 0.0f, tau);
     a.gout = out;
     a.gsub = sub;
+    a.gpush = push;
     launch_grad(dim, kind, L.variant, a, stream);
 }
 
 // Restrict the parent state x_parent into child (x, rx and the second
-// Jacobi buffer) and the parent's (shifted) gradient into tau = grad_c(Rx) -
-// R grad (at the child's DOF slots, the only ones its sweeps read).
+// Jacobi buffer) and the parent's (shifted) gradient, which parent.grad holds
+// with its replicas (a pushed gradient launch), into tau = grad_c(Rx) - R grad
+// (at the child's DOF slots, the only ones its sweeps read).
 void Session::restrict_to(Level& parent, const float* x_parent, Level& child, int kind)
 {
-    halo(parent, parent.grad.p);
     const TransferArgs t = transfer(parent, child, false);
     launch_restrict(dim, t, x_parent, child.x[0].p, parent.grad.p, child.tau.p, child.d_free.p, stream);
     child.cur = 0;
@@ -483,12 +484,13 @@ void Session::coarse_vcycle(size_t li, size_t n_levels, int kind, const FasParam
     }
     Level& N = *coarse[li + 1];
     smooth(L, o.nu_pre, kind, o.omega, L.tau.p);
-    grad(L, L.x[L.cur].p, L.grad.p, L.tau.p, kind);
+    grad(L, L.x[L.cur].p, L.grad.p, L.tau.p, kind, false, true);
     restrict_to(L, L.x[L.cur].p, N, kind);
     coarse_vcycle(li + 1, n_levels, kind, o);
-    // x += P(x_c - rx_c) at free slots, one launch
-    launch_prolong(dim, transfer(L, N, true), N.x[N.cur].p, N.rx.p, L.x[L.cur].p, L.d_free.p, true, stream);
-    halo(L, L.x[L.cur].p);
+    // x += P(x_c - rx_c) at free slots and their replicas, one launch (the
+    // replicas of fixed nodes keep the restriction's values)
+    launch_prolong(dim, transfer(L, N, true), N.x[N.cur].p, N.rx.p, L.x[L.cur].p, L.d_free.p, true,
+                   L.n_push ? L.d_cp_off.p : nullptr, L.n_push ? L.d_cp_dst.p : nullptr, stream);
     smooth(L, o.nu_post, kind, o.omega, L.tau.p);
 }
 
@@ -668,7 +670,7 @@ std::vector<double> Session::autotune_levels(int n_sweeps, double omega, int max
         Level& F = fine;
         const float* x = F.x[F.cur].p;
         halo(F, F.x[F.cur].p);
-        grad(F, x, F.grad.p, nullptr, kind);
+        grad(F, x, F.grad.p, nullptr, kind, false, true);
         restrict_to(F, x, *coarse[0], kind);
         for (size_t i = 0; i < n_coarse; ++i) {
             Level& L = *coarse[i];
@@ -709,7 +711,7 @@ std::vector<double> Session::autotune_levels(int n_sweeps, double omega, int max
             picks[i] = best;
             L.make_tiles(best);
             if (i + 1 < n_coarse) {
-                grad(L, L.x[L.cur].p, L.grad.p, L.tau.p, kind);
+                grad(L, L.x[L.cur].p, L.grad.p, L.tau.p, kind, false, true);
                 restrict_to(L, L.x[L.cur].p, *coarse[i + 1], kind);
             }
         }
@@ -749,11 +751,12 @@ void Session::fas_head(int kind, const FasParams& o, size_t n_coarse)
     Level& C0 = *coarse[0];
     smooth(F, o.nu_pre, kind, o.omega, nullptr);
     float* x = F.x[F.cur].p;
-    grad(F, x, F.grad.p, nullptr, kind);
+    grad(F, x, F.grad.p, nullptr, kind, false, true);
     restrict_to(F, x, C0, kind);
     coarse_vcycle(0, n_coarse, kind, o);
     // fine correction P(x_c - rx_c), zero where not free; its replicas
-    launch_prolong(dim, transfer(F, C0, true), C0.x[C0.cur].p, C0.rx.p, F.corr.p, F.d_free.p, false, stream);
+    launch_prolong(dim, transfer(F, C0, true), C0.x[C0.cur].p, C0.rx.p, F.corr.p, F.d_free.p, false, nullptr,
+                   nullptr, stream);
     halo(F, F.corr.p);
 }
 

@@ -68,6 +68,7 @@ struct SweepArgs {
     const float* tau = nullptr;  // FAS linear shift (plane layout), null = none
     float* gout = nullptr;       // gradient-only launches: per-node gradient (plane layout)
     bool gsub = false;           // gradient-only: gout = gradient - gout, in place
+    bool gpush = false;          // gradient-only: also write it into the DOF's replica slots
     const int* cp_off = nullptr;  // [slots+1] replicas of each slot (push copies), null = none
     const int* cp_dst = nullptr;
     float Wu[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};  // uniform W_inv for stored occurrences

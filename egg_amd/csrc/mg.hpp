@@ -24,10 +24,12 @@ struct TransferArgs {
 void launch_restrict(int dim, const TransferArgs& a, const float* fx, float* cx, const float* fgrad, float* ctau,
                      const uint8_t* cfree, hipStream_t s);
 // Multilinear prolongation of the coarse change cx - crx over the fine
-// level's real nodes: add = true adds it to fout at free fine slots; add =
-// false writes it to fout there and zero at the other real nodes.
+// level's real nodes: add = true adds it to fout at free fine slots and
+// writes the sums into their replica slots (push table cp_off / cp_dst of the
+// fine level, null = none); add = false writes it to fout at the free slots
+// and zero at the other real nodes (replicas left to a halo copy).
 void launch_prolong(int dim, const TransferArgs& a, const float* cx, const float* crx, float* fout,
-                    const uint8_t* ffree, bool add, hipStream_t s);
+                    const uint8_t* ffree, bool add, const int* cp_off, const int* cp_dst, hipStream_t s);
 void launch_masked_axpy(float* x, const float* c, float alpha, const uint8_t* free, long long n, long long plane,
                         int dim, hipStream_t s);
 

@@ -114,7 +114,8 @@ template <int D> __global__ __launch_bounds__(256) void restrict_kernel(const Tr
 // at the nodes that are not free (the step the line search scales).
 template <int D, bool ADD>
 __global__ __launch_bounds__(256) void prolong_kernel(const TransferArgs a, const float* cx, const float* crx,
-                                                      float* fout, const uint8_t* ffree)
+                                                      float* fout, const uint8_t* ffree, const int* cp_off,
+                                                      const int* cp_dst)
 {
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     if (i >= a.n) return;
@@ -152,8 +153,18 @@ __global__ __launch_bounds__(256) void prolong_kernel(const TransferArgs a, cons
     const bool fr = ffree[fs] != 0;
     if constexpr (ADD) {
         if (!fr) return;
+        float v[D];
 #pragma unroll
-        for (int k = 0; k < D; ++k) fout[k * a.fplane + fs] = fout[k * a.fplane + fs] + acc[k];
+        for (int k = 0; k < D; ++k) {
+            v[k] = fout[k * a.fplane + fs] + acc[k];
+            fout[k * a.fplane + fs] = v[k];
+        }
+        if (cp_off)  // the DOF's replicas (only free slots are written, so no other thread writes them)
+            for (int q = cp_off[fs]; q < cp_off[fs + 1]; ++q) {
+                const long long d = cp_dst[q];
+#pragma unroll
+                for (int k = 0; k < D; ++k) fout[k * a.fplane + d] = v[k];
+            }
     } else {
 #pragma unroll
         for (int k = 0; k < D; ++k) fout[k * a.fplane + fs] = fr ? acc[k] : 0.0f;
@@ -214,19 +225,19 @@ void launch_restrict(int dim, const TransferArgs& a, const float* fx, float* cx,
 }
 
 void launch_prolong(int dim, const TransferArgs& a, const float* cx, const float* crx, float* fout,
-                    const uint8_t* ffree, bool add, hipStream_t s)
+                    const uint8_t* ffree, bool add, const int* cp_off, const int* cp_dst, hipStream_t s)
 {
     if (a.n <= 0) return;
     if (dim == 2) {
         if (add)
-            prolong_kernel<2, true><<<nblk(a.n), 256, 0, s>>>(a, cx, crx, fout, ffree);
+            prolong_kernel<2, true><<<nblk(a.n), 256, 0, s>>>(a, cx, crx, fout, ffree, cp_off, cp_dst);
         else
-            prolong_kernel<2, false><<<nblk(a.n), 256, 0, s>>>(a, cx, crx, fout, ffree);
+            prolong_kernel<2, false><<<nblk(a.n), 256, 0, s>>>(a, cx, crx, fout, ffree, nullptr, nullptr);
     } else {
         if (add)
-            prolong_kernel<3, true><<<nblk(a.n), 256, 0, s>>>(a, cx, crx, fout, ffree);
+            prolong_kernel<3, true><<<nblk(a.n), 256, 0, s>>>(a, cx, crx, fout, ffree, cp_off, cp_dst);
         else
-            prolong_kernel<3, false><<<nblk(a.n), 256, 0, s>>>(a, cx, crx, fout, ffree);
+            prolong_kernel<3, false><<<nblk(a.n), 256, 0, s>>>(a, cx, crx, fout, ffree, nullptr, nullptr);
     }
 }
 

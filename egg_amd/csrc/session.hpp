@@ -261,8 +261,10 @@ struct Session {
     void upload_level(size_t i);
     TransferArgs transfer(const Level& parent, const Level& child, bool prolong) const;
     void smooth(Level& L, int sweeps, int kind, float omega, const float* tau);
-    // per-DOF (tau-shifted) gradient into out; sub: out = gradient - out
-    void grad(Level& L, const float* x, float* out, const float* tau, int kind, bool sub = false);
+    // per-DOF (tau-shifted) gradient into out; sub: out = gradient - out;
+    // push: also into the DOFs' replica slots
+    void grad(Level& L, const float* x, float* out, const float* tau, int kind, bool sub = false,
+              bool push = false);
     void restrict_to(Level& parent, const float* x_parent, Level& child, int kind);
     void coarse_vcycle(size_t li, size_t n_levels, int kind, const FasParams& o);
     // one FAS cycle in three parts: pre-smoothing to the halo-consistent

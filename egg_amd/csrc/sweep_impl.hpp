@@ -107,20 +107,27 @@ template <int D> __device__ __forceinline__ void newton(const float* g, const fl
     }
 }
 
+// Write v into the replica slots of a node (its ghost, fan and share
+// copies whose root is that node), when the level has a push table.
+template <int D>
+__device__ __forceinline__ void push_replicas(const SweepArgs& a, float* out, long long node, const float* v)
+{
+    if (!a.cp_off) return;
+    const int q0 = a.cp_off[node], q1 = a.cp_off[node + 1];
+    for (int q = q0; q < q1; ++q) {
+        const int dst = a.cp_dst[q];
+#pragma unroll
+        for (int k = 0; k < D; ++k) out[k * a.plane + dst] = v[k];
+    }
+}
+
 // Write the accepted node and push it to its replicas.
 template <int D>
 __device__ __forceinline__ void store_node(const SweepArgs& a, long long node, const float* cur)
 {
 #pragma unroll
     for (int k = 0; k < D; ++k) a.xout[k * a.plane + node] = cur[k];
-    if (a.cp_off) {
-        const int q0 = a.cp_off[node], q1 = a.cp_off[node + 1];
-        for (int q = q0; q < q1; ++q) {
-            const int dst = a.cp_dst[q];
-#pragma unroll
-            for (int k = 0; k < D; ++k) a.xout[k * a.plane + dst] = cur[k];
-        }
-    }
+    push_replicas<D>(a, a.xout, node, cur);
 }
 
 // Diagnostic counters (off when null): spread over kCounterSlots cache lines
@@ -267,6 +274,7 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
                 float& o = a.gout[k * PL + dof];
                 o = a.gsub ? g[k] - o : g[k];
             }
+        if (l == 0 && a.gpush) push_replicas<D>(a, a.gout, dof, g);
         return;
     }
 #pragma unroll
@@ -599,6 +607,7 @@ __device__ void tile_sweep(const SweepArgs& a, int tile)
                 float& o = a.gout[k * PL + node];
                 o = a.gsub ? g[k] - o : g[k];
             }
+        if (gl == 0 && a.gpush) push_replicas<D>(a, a.gout, node, g);
         return;
     }
 #pragma unroll
@@ -757,6 +766,7 @@ __device__ void sample_sweep(const SweepArgs& a, int tile)
                 float& o = a.gout[k * PL + node];
                 o = a.gsub ? g[k] - o : g[k];
             }
+        if (q == 0 && a.gpush) push_replicas<D>(a, a.gout, node, g);
         return;
     }
 #pragma unroll
