@@ -147,9 +147,16 @@ struct Wire {
                 eg.interior_block = keep(IArr((py::ssize_t)neg.size(), neg.data()));
                 eg.interior_logical = keep(IArr((py::ssize_t)zero.size(), zero.data()));
             }
-            if (g.contains("weight") || g.contains("curv_nodes") || g.contains("dir_nodes"))
-                throw py::value_error("wire: per-sample weights and curvature / directional terms are not "
-                                      "supported by this compute core");
+            if (g.contains("weight") && !g["weight"].is_none()) {
+                DArr wt = get_d(g, "weight");
+                if ((int64_t)wt.size() != eg.total_samples)
+                    throw py::value_error("validate_context: group weight size " + std::to_string(wt.size()) +
+                                          " != " + std::to_string(eg.total_samples));
+                eg.weight = keep(wt);
+            }
+            if (g.contains("curv_nodes") || g.contains("dir_nodes"))
+                throw py::value_error("wire: curvature / directional terms are not supported by this compute "
+                                      "core");
             groups.push_back(eg);
         }
         py::dict e = ctx["energy_stencil"].cast<py::dict>();
@@ -173,8 +180,15 @@ struct Wire {
         if (eW.size() % dd != 0) throw py::value_error("validate_context: energy_stencil W_inv size");
         es.w_rows = (int64_t)eW.size() / dd;
         es.W_inv = keep(eW);
-        if (e.contains("weight") || e.contains("dir_nodes"))
-            throw py::value_error("wire: energy-stencil weights / directional terms are not supported");
+        if (e.contains("weight") && !e["weight"].is_none()) {
+            DArr wt = get_d(e, "weight");
+            if ((int64_t)wt.size() != es.num_samples)
+                throw py::value_error("validate_context: energy_stencil weight size " + std::to_string(wt.size()) +
+                                      " != " + std::to_string(es.num_samples));
+            es.weight = keep(wt);
+        }
+        if (e.contains("dir_nodes"))
+            throw py::value_error("wire: energy-stencil directional terms are not supported");
 
         // structured tables
         py::list shl = structured["interior_shapes"].cast<py::list>();

@@ -503,6 +503,12 @@ std::pair<std::vector<double>, std::vector<double>> Session::run_fas(int n_cycle
     const int per_cycle = nu_pre + nu_post;
     const size_t n_coarse = prepare_fas(max_levels);
     if (n_coarse == 0) return run_jacobi(n_cycles * per_cycle, kind, 0.0, omega, per_cycle);
+    if (weighted)
+        throw InvalidArgument("run_fas: per-sample energy weights are not supported by the multigrid path "
+                              "(its coarse levels and line search use the unweighted cell energy)");
+    if (!fine.structured_energy)
+        throw InvalidArgument("run_fas: the line search needs the structured cell energy stencil (identity "
+                              "W_inv on the energy stencil)");
 
     Level& F = fine;
     FasParams o;

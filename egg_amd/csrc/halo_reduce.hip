@@ -212,11 +212,13 @@ __global__ __launch_bounds__(kReduceThreads) void reduce_samples(const SampleRed
                     acc += A[ii * DIM + jj] * (a.W ? (double)a.W[i * DIM * DIM + jj * DIM + kk] : a.Wu[jj * DIM + kk]);
                 T[ii * DIM + kk] = acc;
             }
+        // a sample's weight scales mu, not det (reference src/patch.hpp:162)
+        const double w = a.wt ? (double)a.wt[i] : 1.0;
         if constexpr (DIM == 2) {
-            esum += mu2_d<KIND>(T[0], T[1], T[2], T[3], a.delta);
+            esum += w * mu2_d<KIND>(T[0], T[1], T[2], T[3], a.delta);
             emin = fmin(emin, A[0] * A[3] - A[1] * A[2]);
         } else {
-            esum += mu3_value<KIND, double>(T, a.delta);
+            esum += w * mu3_value<KIND, double>(T, a.delta);
             emin = fmin(emin, det3(A));
         }
     }

@@ -50,6 +50,7 @@ struct StoredView {
     const int* gn = nullptr;    // [dim][total] neighbour slots
     const uint8_t* meta = nullptr;  // [total] bits 0-2: role+1 (0..D), bits 3..5: sign of axis k (1 = -1), bit 7: padding
     const float* W = nullptr;       // [total][dim*dim] per-occurrence W_inv rows, or null (uniform)
+    const float* wt = nullptr;      // [total] per-occurrence energy weight, or null (1)
 };
 
 constexpr int kCounterSlots = 64, kCounterStride = 16;
@@ -107,6 +108,7 @@ struct SampleReduceArgs {
     const int* gn = nullptr;  // [dim][ns]
     const float* s = nullptr;  // [dim][ns]
     const float* W = nullptr;  // [ns][dd] or null (uniform)
+    const float* wt = nullptr;  // [ns] energy weight, or null (1)
     double Wu[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
     double delta = 0.0;
 };

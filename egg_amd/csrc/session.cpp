@@ -161,6 +161,7 @@ void Level::upload_tables()
         std::vector<int> gc(st_padded, 0), gn((size_t)D * st_padded, 0);
         std::vector<uint8_t> meta(st_padded, kStoredPad);
         std::vector<float> W(st_W.empty() ? 0 : (size_t)D * D * st_padded, 0.0f);
+        std::vector<float> wt(st_wt.empty() ? 0 : (size_t)st_padded, 0.0f);
         for (long long d = 0; d < n; ++d)
             for (int p = st_off[d]; p < st_off[d + 1]; ++p) {
                 const long long q = d * rounds * G + (p - st_off[d]);
@@ -168,11 +169,13 @@ void Level::upload_tables()
                 meta[q] = st_meta[p];
                 for (int k = 0; k < D; ++k) gn[(size_t)k * st_padded + q] = st_gn[(size_t)k * total + p];
                 for (int e = 0; e < D * D && !W.empty(); ++e) W[(size_t)D * D * q + e] = st_W[(size_t)D * D * p + e];
+                if (!wt.empty()) wt[q] = st_wt[p];
             }
         d_st_gc.upload(gc);
         d_st_gn.upload(gn);
         d_st_meta.upload(meta);
         d_st_W.upload(W);
+        d_st_wt.upload(wt);
     }
     // structured energy cells
     cell_off.assign(lay.shape.size() + 1, 0);
@@ -205,6 +208,7 @@ SweepArgs Level::sweep_args(const float* xin, float* xout, float omega, float de
     a.st.gn = d_st_gn.p;
     a.st.meta = d_st_meta.p;
     a.st.W = st_W.empty() ? nullptr : d_st_W.p;
+    a.st.wt = st_wt.empty() ? nullptr : d_st_wt.p;
     a.omega = omega;
     a.delta = delta;
     a.tau = tau;
@@ -265,6 +269,10 @@ void validate(int dim, const eggmi_group* groups, int n_groups, const eggmi_sten
                 if (g.s[k][p] != 1.0 && g.s[k][p] != -1.0)
                     throw InvalidArgument("validate_context: " + grp + " s" + std::to_string(k) +
                                           " must be +-1 per sample");
+        if (g.weight)
+            for (long long p = 0; p < g.total_samples; ++p)
+                if (!std::isfinite(g.weight[p]))
+                    throw InvalidArgument("validate_context: " + grp + " weight must be finite");
     }
     if (es->num_samples < 0) throw InvalidArgument("validate_context: energy_stencil negative size");
     if (es->w_rows != 1 && es->w_rows != es->num_samples)
@@ -275,6 +283,19 @@ void validate(int dim, const eggmi_group* groups, int n_groups, const eggmi_sten
         const std::string nm = "gn" + std::to_string(k);
         check_index_range(es->gn[k], es->num_samples, N, "energy_stencil", nm.c_str());
     }
+    if (es->weight)
+        for (long long i = 0; i < es->num_samples; ++i)
+            if (!std::isfinite(es->weight[i]))
+                throw InvalidArgument("validate_context: energy_stencil weight must be finite");
+}
+
+// A weight array that is absent or 1 everywhere leaves the unweighted sums.
+bool unit_weights(const double* w, long long n)
+{
+    if (!w) return true;
+    for (long long i = 0; i < n; ++i)
+        if (w[i] != 1.0) return false;
+    return true;
 }
 
 }  // namespace
@@ -492,6 +513,9 @@ void Session::create(int d, const eggmi_group* groups, int n_groups, const eggmi
     std::vector<double> wu_ref;
     for (int gi = 0; gi < n_groups; ++gi)
         if (groups[gi].w_rows != 1) w_uniform_all = false;
+    bool wt_unit_all = true;  // per-occurrence energy weights all 1 (or absent)
+    for (int gi = 0; gi < n_groups; ++gi)
+        if (!unit_weights(groups[gi].weight, groups[gi].total_samples)) wt_unit_all = false;
     if (w_uniform_all && n_groups > 0) {
         // all groups uniform: they must agree to share one uniform row
         for (int gi = 0; gi < n_groups; ++gi) {
@@ -519,6 +543,9 @@ void Session::create(int d, const eggmi_group* groups, int n_groups, const eggmi
             const int ib = g.interior_block[di];
             if (ib >= 0) {
                 if (P != 0) throw InvalidArgument("structured: interior DOF with stored samples");
+                if (!wt_unit_all)
+                    throw InvalidArgument("structured: a synthesized interior DOF needs unit sample weights "
+                                          "(ship its stored stencil instead)");
                 if (ib >= nb) throw InvalidArgument("structured: interior_block out of range");
                 int lg[3];
                 for (int k = 0; k < d; ++k) {
@@ -545,6 +572,7 @@ void Session::create(int d, const eggmi_group* groups, int n_groups, const eggmi
                         const double* w = g.w_rows == 1 ? g.W_inv : g.W_inv + p * dd;
                         for (int i = 0; i < dd; ++i) L.st_W.push_back((float)w[i]);
                     }
+                    if (!wt_unit_all) L.st_wt.push_back(g.weight ? (float)g.weight[p] : 1.0f);
                 }
                 L.st_off.push_back((int)L.st_gc.size());
             }
@@ -560,6 +588,7 @@ void Session::create(int d, const eggmi_group* groups, int n_groups, const eggmi
     }
     if (w_uniform_all && !wu_ref.empty())
         for (int i = 0; i < dd; ++i) L.Wu[i] = (float)wu_ref[i];
+    weighted = !wt_unit_all || !unit_weights(es->weight, es->num_samples);
     lay.overflow = n_overflow;
     const long long nslots = lay.n_slots();
     if ((long long)Xp.size() != nslots * d) throw Internal("structured: packed X size disagrees with the layout");
@@ -584,8 +613,11 @@ void Session::create(int d, const eggmi_group* groups, int n_groups, const eggmi
     L.build_push();
 
     // ---- energy stencil: structured cells when the wire is the plain cell stencil ----
+    std::vector<int> es_gc_h, es_gn_h;  // the explicit sample table otherwise (uploaded below)
+    std::vector<float> es_s_h, es_W_h, es_wt_h;
     {
-        bool canon = es->w_rows == 1;
+        const bool es_weighted = !unit_weights(es->weight, es->num_samples);
+        bool canon = es->w_rows == 1 && !es_weighted;
         if (canon)
             for (int i = 0; i < d; ++i)
                 for (int j = 0; j < d; ++j)
@@ -628,8 +660,13 @@ void Session::create(int d, const eggmi_group* groups, int n_groups, const eggmi
         L.structured_energy = canon;
         if (!canon) {
             const long long ns = es->num_samples;
-            std::vector<int> gc(ns), gn((size_t)ns * d);
-            std::vector<float> sv((size_t)ns * d), W;
+            std::vector<int>& gc = es_gc_h;
+            std::vector<int>& gn = es_gn_h;
+            std::vector<float>& sv = es_s_h;
+            std::vector<float>& W = es_W_h;
+            gc.resize(ns);
+            gn.resize((size_t)ns * d);
+            sv.resize((size_t)ns * d);
             for (long long i = 0; i < ns; ++i) {
                 gc[i] = g2s[es->gc[i]];
                 for (int k = 0; k < d; ++k) {
@@ -645,11 +682,9 @@ void Session::create(int d, const eggmi_group* groups, int n_groups, const eggmi
             } else {
                 W.assign(es->W_inv, es->W_inv + (size_t)ns * dd);
             }
+            if (es_weighted)
+                for (long long i = 0; i < ns; ++i) es_wt_h.push_back((float)es->weight[i]);
             L.es_n = ns;
-            L.d_es_gc.upload(gc);
-            L.d_es_gn.upload(gn);
-            L.d_es_s.upload(sv);
-            L.d_es_W.upload(W);
         }
     }
 
@@ -671,6 +706,13 @@ void Session::create(int d, const eggmi_group* groups, int n_groups, const eggmi
     L.plane = (nslots + 63) / 64 * 64;
     L.upload_tables();
     L.make_tiles(tile_variant);
+    if (!L.structured_energy) {
+        L.d_es_gc.upload(es_gc_h);
+        L.d_es_gn.upload(es_gn_h);
+        L.d_es_s.upload(es_s_h);
+        L.d_es_W.upload(es_W_h);
+        L.d_es_wt.upload(es_wt_h);
+    }
     std::vector<float> xs((size_t)L.plane * d, 0.0f);
     for (long long s = 0; s < nslots; ++s)
         for (int k = 0; k < d; ++k) xs[(size_t)k * L.plane + s] = (float)Xp[(size_t)s * d + k];
@@ -706,6 +748,7 @@ void Session::reduce(Level& L, const float* x, int kind, double delta, double* o
         a.gn = L.d_es_gn.p;
         a.s = L.d_es_s.p;
         a.W = L.d_es_W.n ? L.d_es_W.p : nullptr;
+        a.wt = L.d_es_wt.n ? L.d_es_wt.p : nullptr;
         for (int i = 0; i < 9; ++i) a.Wu[i] = L.es_Wu[i];
         a.delta = delta;
         launch_reduce_samples(dim, kind, a, partials.p, stream);

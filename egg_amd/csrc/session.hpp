@@ -135,12 +135,13 @@ struct Level {
     std::vector<int> st_dof, st_off, st_gc, st_gn;
     std::vector<uint8_t> st_meta;
     std::vector<float> st_W;  // empty = uniform
+    std::vector<float> st_wt;  // per-occurrence energy weight; empty = 1
     float Wu[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
     DArr<int> d_st_dof, d_st_gc, d_st_gn;  // device copies in the padded lane layout (see StoredView)
     int st_rounds = 1;
     long long st_padded = 0;
     DArr<uint8_t> d_st_meta;
-    DArr<float> d_st_W;
+    DArr<float> d_st_W, d_st_wt;
     // energy reduction: structured cells, or an explicit sample table
     bool structured_energy = true;
     std::vector<long long> cell_off;
@@ -148,7 +149,7 @@ struct Level {
     long long ncells = 0;
     long long es_n = 0;
     DArr<int> d_es_gc, d_es_gn;
-    DArr<float> d_es_s, d_es_W;
+    DArr<float> d_es_s, d_es_W, d_es_wt;  // d_es_wt: per-sample energy weight, empty = 1
     double es_Wu[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
     // device state
     DArr<float> x[2];
@@ -221,6 +222,7 @@ struct Session {
     HaloHost fine_halo;
     DArr<double> d_ls;
     bool host_only = false;  // created with device = 3: index tables only
+    bool weighted = false;   // the wire carries per-sample energy weights other than 1
     const Mlp* proposal = nullptr;  // learned proposal for run_jacobi (not owned)
     int proposal_mode = 0;
     float proposal_scale = 1.0f;

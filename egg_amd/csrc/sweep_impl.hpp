@@ -150,7 +150,9 @@ __device__ __forceinline__ void count_update(const SweepArgs& a, int ntr, bool p
 
 // Stored-stencil DOF d handled by a group of G lanes (lane l owns
 // occurrences l, l+G, ... of the DOF's patch).
-template <int D, int KIND, bool TAU, bool GRAD, int G>
+// WT: per-occurrence energy weights (st.wt); a separate instantiation, so the
+// unweighted sums keep their exact operation order.
+template <int D, int KIND, bool TAU, bool GRAD, int G, bool WT>
 __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
 {
     static_assert(G == stored_lanes(D), "stored lane group size");
@@ -172,6 +174,7 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
         bool cd, nd[D], live;
         int role;
         const float* w;
+        float wt;  // energy weight: scales mu's value, gradient and Hessian, not det
     };
     auto load = [&](int r, Occ& o) {
         const long long q = ((long long)d * R + r) * G + l;
@@ -179,7 +182,9 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
         o.live = !(m & kStoredPad);
         o.role = (m & 7) - 1;
         o.w = st.W ? st.W + (long long)(D * D) * q : a.Wu;
+        o.wt = 1.0f;
         if (!o.live) return;
+        if constexpr (WT) o.wt = st.wt[q];
         const int ic = st.gc[q];
         o.cd = ic == dof;
 #pragma unroll
@@ -249,9 +254,22 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
                     u[c] = sr * w[(role - 1) * D + c];
                 }
             }
-            e += sample_grad_hess<D, KIND>(T, u, a.delta, g, h);
+            if constexpr (WT) {  // weighted occurrences (reference src/patch.hpp:250-292)
+                float gw[D] = {}, hw[kNH<D>] = {};
+                const float v = sample_grad_hess<D, KIND>(T, u, a.delta, gw, hw);
+                e += o.wt * v;
+#pragma unroll
+                for (int k = 0; k < D; ++k) g[k] += o.wt * gw[k];
+#pragma unroll
+                for (int k = 0; k < kNH<D>; ++k) h[k] += o.wt * hw[k];
+            } else {
+                e += sample_grad_hess<D, KIND>(T, u, a.delta, g, h);
+            }
         } else {
-            e += sample_value<D, KIND>(T, a.delta);
+            if constexpr (WT)
+                e += o.wt * sample_value<D, KIND>(T, a.delta);
+            else
+                e += sample_value<D, KIND>(T, a.delta);
         }
     }
 #pragma unroll
@@ -296,7 +314,10 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
             if (!o.live) break;
             float T[D * D], det;
             build(o, tr, T, det);
-            et += sample_value<D, KIND>(T, a.delta);
+            if constexpr (WT)
+                et += o.wt * sample_value<D, KIND>(T, a.delta);
+            else
+                et += sample_value<D, KIND>(T, a.delta);
             mt = fminf(mt, det);
         }
         et = group_sum<G>(et);
@@ -818,7 +839,10 @@ __global__ __launch_bounds__(16 * TX * TY) void sample_kernel(const SweepArgs a)
     const int n_st = int(gridDim.x) - a.n_tiles;
     if (int(blockIdx.x) < n_st) {
         const int gid = int(blockIdx.x) * (NT / G) + int(threadIdx.x) / G;
-        stored_group<2, KIND, TAU, GRAD, G>(a, gid, int(threadIdx.x) % G, gid < a.st.n);
+        if (a.st.wt)
+            stored_group<2, KIND, TAU, GRAD, G, true>(a, gid, int(threadIdx.x) % G, gid < a.st.n);
+        else
+            stored_group<2, KIND, TAU, GRAD, G, false>(a, gid, int(threadIdx.x) % G, gid < a.st.n);
         return;
     }
     sample_sweep<KIND, TX, TY, TAU, GRAD>(a, int(blockIdx.x) - n_st);
@@ -849,7 +873,10 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& a)
     const int n_st = int(gridDim.x) - a.n_tiles;
     if (int(blockIdx.x) < n_st) {
         const int gid = int(blockIdx.x) * (NT / G) + int(threadIdx.x) / G;
-        stored_group<D, KIND, TAU, GRAD, G>(a, gid, int(threadIdx.x) % G, gid < a.st.n);
+        if (a.st.wt)
+            stored_group<D, KIND, TAU, GRAD, G, true>(a, gid, int(threadIdx.x) % G, gid < a.st.n);
+        else
+            stored_group<D, KIND, TAU, GRAD, G, false>(a, gid, int(threadIdx.x) % G, gid < a.st.n);
         return;
     }
     tile_sweep<D, KIND, TX, TY, TZ, TAU, GRAD, PROP, LDS, CPL>(a, int(blockIdx.x) - n_st);

@@ -65,11 +65,15 @@ def cell_stencil(blocks: list[np.ndarray], d: int) -> dict:
 
 
 def build_flat_context(blocks: list[np.ndarray], free_mask: np.ndarray, d: int, *,
-                       w_inv: np.ndarray | None = None) -> dict:
+                       w_inv: np.ndarray | None = None, weight: np.ndarray | None = None) -> dict:
     """Assemble the flat wire for the moving DOFs ``free_mask`` of ``blocks``.
 
     ``w_inv`` is one uniform ``(d, d)`` inverse target (identity when
-    omitted) or ``(ns, d, d)`` per energy-stencil sample.
+    omitted) or ``(ns, d, d)`` per energy-stencil sample. ``weight`` is an
+    optional ``(ns,)`` per-sample energy weight (it scales a sample's mu,
+    gradient and Hessian, not its det); as in the reference
+    (egg/smoothing/flat_context.py:186-218) a weighted wire ships every DOF's
+    real stencil, so interior synthesis is off.
     """
     st = cell_stencil(blocks, d)
     ns = st["ns"]
@@ -79,6 +83,11 @@ def build_flat_context(blocks: list[np.ndarray], free_mask: np.ndarray, d: int, 
     uniform = w_inv.shape == (d, d)
     w0 = w_inv if uniform else (w_inv[0] if len(w_inv) else np.eye(d))
     synth = bool(np.allclose(w0, w0[0, 0] * np.eye(d)) and (uniform or np.allclose(w_inv, w0)))
+    if weight is not None:
+        weight = np.asarray(weight, np.float64).reshape(-1)
+        if weight.shape != (ns,):
+            raise ValueError(f"weight must hold one value per energy-stencil sample ({ns}), got {weight.shape}")
+        synth = False
 
     node_block = np.full(N, -1, np.int32)
     node_logical = np.zeros((N, d), np.int32)
@@ -107,6 +116,8 @@ def build_flat_context(blocks: list[np.ndarray], free_mask: np.ndarray, d: int, 
             out[f"s{k}"] = st["s"][k][sid].astype(np.float64)
         out["W_inv"] = (np.asarray(w_inv, np.float64).reshape(1, dd) if uniform
                         else np.ascontiguousarray(w_inv[sid].reshape(-1, dd)))
+        if weight is not None:
+            out["weight"] = np.ascontiguousarray(weight[sid])
         return out
 
     groups = []

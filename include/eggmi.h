@@ -45,6 +45,9 @@ typedef struct {
     const double* s[3];               /* [total] each, +-1 */
     const double* W_inv;              /* [w_rows*dim*dim] */
     int64_t w_rows;                   /* 1 (uniform target) or total */
+    const double* weight;             /* [total] per-occurrence energy weight (scales mu's value,
+                                         gradient and Hessian, not det), NULL = 1
+                                         (src/patch.hpp:45-77, :250-292) */
 } eggmi_group;
 
 /* The energy stencil (every (cell, corner) sample). */
@@ -55,6 +58,7 @@ typedef struct {
     const double* s[3];
     const double* W_inv;
     int64_t w_rows; /* 1 or num_samples */
+    const double* weight; /* [num_samples] energy weight, NULL = 1 (src/sweep.hpp:122, :301-302) */
 } eggmi_stencil;
 
 /* Halo-padded structured tables (reference egg/smoothing/cpp_backend.py:

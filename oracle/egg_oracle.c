@@ -262,6 +262,7 @@ typedef struct {
     const int* role;
     const real* W;
     int w_stride;
+    const real* wt; /* [P] per-occurrence energy weight, NULL = 1 (reference src/patch.hpp:45-77) */
 } FN(patch_t);
 
 static void FN(occ_fetch)(int dim, const FN(patch_t)* pv, int p, int* gn, real* s)
@@ -290,8 +291,11 @@ static void FN(patch_eval_impl)(int dim, int kind, real delta, const FN(patch_t)
         FN(assemble)(dim, X, pv->gc[p], gn, s, W, T, &detA);
         if (detA < *mindet) *mindet = detA;
         const int role = pv->role[p];
+        /* the weight scales mu's value, gradient and Hessian, not the det barrier
+           (reference src/patch.hpp:250-292 accumulate_sample) */
+        const real w = pv->wt ? pv->wt[p] : (real)1;
         FN(metric_any)(dim, kind, T, delta, &val, g, role >= 0 ? H : NULL);
-        *energy += val;
+        *energy += w * val;
         /* dmu/dA[i][k] = sum_j g[i,j] W[k,j] */
         real dA[3][3];
         for (int i = 0; i < dim; ++i)
@@ -304,11 +308,11 @@ static void FN(patch_eval_impl)(int dim, int kind, real delta, const FN(patch_t)
             for (int i = 0; i < dim; ++i) {
                 real acc = 0;
                 for (int k = 0; k < dim; ++k) acc += s[k] * dA[i][k];
-                grad[i] += -acc;
+                grad[i] += w * -acc;
             }
         } else if (role >= 1) {
             const int ax = role - 1;
-            for (int i = 0; i < dim; ++i) grad[i] += s[ax] * dA[i][ax];
+            for (int i = 0; i < dim; ++i) grad[i] += w * (s[ax] * dA[i][ax]);
         }
         if (role >= 0) {
             /* Jb (VT x dim): row a=(i,c) has its only nonzero in column i */
@@ -334,17 +338,17 @@ static void FN(patch_eval_impl)(int dim, int kind, real delta, const FN(patch_t)
                         for (int b = 0; b < VT; ++b) hj += H[a * VT + b] * Jb[b * dim + j];
                         acc += Jb[a * dim + i] * hj;
                     }
-                    hess[i * dim + j] += acc;
+                    hess[i * dim + j] += w * acc;
                 }
         }
     }
 }
 
 void FN(o_patch_eval)(int dim, int kind, real delta, int P, const int* gc, const int* gn, const real* s,
-                      const int* role, const real* W, int w_stride, const real* X, real* grad, real* hess,
-                      real* energy, real* mindet)
+                      const int* role, const real* W, int w_stride, const real* wt, const real* X, real* grad,
+                      real* hess, real* energy, real* mindet)
 {
-    FN(patch_t) pv = {P, gc, gn, s, role, W, w_stride};
+    FN(patch_t) pv = {P, gc, gn, s, role, W, w_stride, wt};
     FN(patch_eval_impl)(dim, kind, delta, &pv, X, grad, hess, energy, mindet);
 }
 
@@ -377,7 +381,7 @@ static void FN(trial_impl)(int dim, int kind, real delta, const FN(patch_t)* pv,
             }
         real val;
         FN(metric_any)(dim, kind, T, delta, &val, NULL, NULL);
-        *e_new += val;
+        *e_new += (pv->wt ? pv->wt[p] : (real)1) * val;
         if (detA < *mdet) *mdet = detA;
     }
 }
@@ -434,7 +438,8 @@ void FN(o_solve)(int dim, const real* H, const real* g, real* x)
  * (reference src/sweep.hpp:1881-1922). mu and det are evaluated in double
  * from the REAL positions and accumulated in double. */
 void FN(o_energy_mindet)(int dim, int kind, double delta, long long ns, const int* gc, const int* gn,
-                         const real* s, const real* W, int w_stride, const real* X, double* E, double* md)
+                         const real* s, const real* W, int w_stride, const real* wt, const real* X, double* E,
+                         double* md)
 {
     double e = 0.0, m = INFINITY;
     for (long long i = 0; i < ns; ++i) {
@@ -461,7 +466,7 @@ void FN(o_energy_mindet)(int dim, int kind, double delta, long long ns, const in
                   A[2] * (A[3] * A[7] - A[4] * A[6]);
             o_metric3d_f64(kind, T, delta, &val, NULL, NULL);
         }
-        e += val;
+        e += (wt ? (double)wt[i] : 1.0) * val;
         if (det < m) m = det;
     }
     *E = e;
@@ -518,9 +523,9 @@ static void FN(update_dof)(int dim, int kind, real delta, real omega, const FN(p
  */
 int FN(o_jacobi)(int dim, int kind, double delta, double omega, int n_sweeps, int report_every, int ndof,
                  const int* dof, const long long* poff, const int* pgc, const int* pgn, const real* ps,
-                 const int* prole, const real* W, int w_stride, int n_nodes, real* X, const real* tau,
-                 long long ns, const int* egc, const int* egn, const real* es, const real* eW, int ew_stride,
-                 real* work, double* energies, double* mindets)
+                 const int* prole, const real* W, int w_stride, const real* pwt, int n_nodes, real* X,
+                 const real* tau, long long ns, const int* egc, const int* egn, const real* es, const real* eW,
+                 int ew_stride, const real* ewt, real* work, double* energies, double* mindets)
 {
     const long long total = poff[ndof];
     const int k = report_every <= 0 ? n_sweeps : report_every;
@@ -541,12 +546,13 @@ int FN(o_jacobi)(int dim, int kind, double delta, double omega, int n_sweeps, in
                     gn_l[kk * P + p] = pgn[(size_t)kk * total + o + p];
                     s_l[kk * P + p] = ps[(size_t)kk * total + o + p];
                 }
-            FN(patch_t) pv = {P, pgc + o, gn_l, s_l, prole + o, W + (size_t)w_stride * o, w_stride};
+            FN(patch_t) pv = {P, pgc + o, gn_l, s_l, prole + o, W + (size_t)w_stride * o, w_stride,
+                              pwt ? pwt + o : NULL};
             FN(update_dof)(dim, kind, (real)delta, (real)omega, &pv, xin, dof[d], tau,
                        xout + (size_t)dof[d] * dim);
         }
         if ((sw + 1) % k == 0 || sw == n_sweeps - 1) {
-            FN(o_energy_mindet)(dim, kind, delta, ns, egc, egn, es, eW, ew_stride, xout, &energies[nrep],
+            FN(o_energy_mindet)(dim, kind, delta, ns, egc, egn, es, eW, ew_stride, ewt, xout, &energies[nrep],
                                 &mindets[nrep]);
             ++nrep;
         }
@@ -563,7 +569,7 @@ int FN(o_jacobi)(int dim, int kind, double delta, double omega, int n_sweeps, in
  * with HasTau). */
 void FN(o_patch_grads)(int dim, int kind, double delta, int ndof, const int* dof, const long long* poff,
                        const int* pgc, const int* pgn, const real* ps, const int* prole, const real* W,
-                       int w_stride, const real* X, const real* tau, real* gout)
+                       int w_stride, const real* pwt, const real* X, const real* tau, real* gout)
 {
     const long long total = poff[ndof];
 #pragma omp parallel for schedule(static)
@@ -577,7 +583,8 @@ void FN(o_patch_grads)(int dim, int kind, double delta, int ndof, const int* dof
                 gn_l[kk * P + p] = pgn[(size_t)kk * total + o + p];
                 s_l[kk * P + p] = ps[(size_t)kk * total + o + p];
             }
-        FN(patch_t) pv = {P, pgc + o, gn_l, s_l, prole + o, W + (size_t)w_stride * o, w_stride};
+        FN(patch_t) pv = {P, pgc + o, gn_l, s_l, prole + o, W + (size_t)w_stride * o, w_stride,
+                          pwt ? pwt + o : NULL};
         real grad[3], hess[9], e0, md;
         FN(patch_eval_impl)(dim, kind, (real)delta, &pv, X, grad, hess, &e0, &md);
         for (int k = 0; k < dim; ++k)

@@ -54,24 +54,26 @@ def lib():
             f.argtypes = [C.c_int, rp, rt, rp, rp, rp]
             f.restype = None
             f = getattr(L, "o_patch_eval" + sfx)
-            f.argtypes = [C.c_int, C.c_int, rt, C.c_int, _ip, _ip, rp, _ip, rp, C.c_int, rp, rp, rp, rp, rp]
+            f.argtypes = [C.c_int, C.c_int, rt, C.c_int, _ip, _ip, rp, _ip, rp, C.c_int, C.c_void_p, rp, rp, rp, rp,
+                          rp]
             f.restype = None
             f = getattr(L, "o_solve" + sfx)
             f.argtypes = [C.c_int, rp, rp, rp]
             f.restype = None
             f = getattr(L, "o_energy_mindet" + sfx)
-            f.argtypes = [C.c_int, C.c_int, C.c_double, C.c_longlong, _ip, _ip, rp, rp, C.c_int, rp, _dp, _dp]
+            f.argtypes = [C.c_int, C.c_int, C.c_double, C.c_longlong, _ip, _ip, rp, rp, C.c_int, C.c_void_p, rp,
+                          _dp, _dp]
             f.restype = None
             f = getattr(L, "o_jacobi" + sfx)
             f.argtypes = [
                 C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
-                _ip, _lp, _ip, _ip, rp, _ip, rp, C.c_int, C.c_int, rp, C.c_void_p,
-                C.c_longlong, _ip, _ip, rp, rp, C.c_int, rp, _dp, _dp,
+                _ip, _lp, _ip, _ip, rp, _ip, rp, C.c_int, C.c_void_p, C.c_int, rp, C.c_void_p,
+                C.c_longlong, _ip, _ip, rp, rp, C.c_int, C.c_void_p, rp, _dp, _dp,
             ]
             f.restype = C.c_int
             f = getattr(L, "o_patch_grads" + sfx)
-            f.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, _ip, _lp, _ip, _ip, rp, _ip, rp, C.c_int, rp,
-                          C.c_void_p, rp]
+            f.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, _ip, _lp, _ip, _ip, rp, _ip, rp, C.c_int,
+                          C.c_void_p, rp, C.c_void_p, rp]
             f.restype = None
         L.o_set_threads.argtypes = [C.c_int]
         L.o_set_threads.restype = None
@@ -81,6 +83,16 @@ def lib():
 
 def _dt(prec: str):
     return np.float64 if prec == "f64" else np.float32
+
+
+def _weights(w, dt):
+    """Per-sample energy weights as a contiguous array of the oracle's real
+    type, or None (unit weight everywhere)."""
+    return None if w is None else np.ascontiguousarray(w, dt).reshape(-1)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
 def metric(dim: int, kind: str | int, t, delta: float = 0.0, prec: str = "f64"):
@@ -97,12 +109,14 @@ def metric(dim: int, kind: str | int, t, delta: float = 0.0, prec: str = "f64"):
     return float(val[0]), g, H.reshape(vt, vt)
 
 
-def patch_eval(dim, kind, delta, gc, gn, s, role, W, X, prec="f64"):
+def patch_eval(dim, kind, delta, gc, gn, s, role, W, X, prec="f64", weight=None):
     """(grad, hess, energy, mindet) of one patch. gn, s: (dim, P);
-    W: (P, dim*dim) per-occurrence rows or (dim*dim,) uniform."""
+    W: (P, dim*dim) per-occurrence rows or (dim*dim,) uniform; weight: (P,)
+    per-occurrence energy weight or None (unit)."""
     dt = _dt(prec)
     P = len(gc)
     W = np.ascontiguousarray(W, dtype=dt)
+    wt = _weights(weight, dt)
     ws = 0 if W.ndim == 1 else dim * dim
     grad = np.zeros(dim, dt)
     hess = np.zeros(dim * dim, dt)
@@ -113,7 +127,7 @@ def patch_eval(dim, kind, delta, gc, gn, s, role, W, X, prec="f64"):
         dim, KIND[kind], dt(delta), P,
         np.ascontiguousarray(gc, np.int32), np.ascontiguousarray(gn, np.int32).reshape(-1),
         np.ascontiguousarray(s, dt).reshape(-1), np.ascontiguousarray(role, np.int32),
-        W.reshape(-1), ws, np.ascontiguousarray(X, dt).reshape(-1), grad, hess, e, m,
+        W.reshape(-1), ws, _ptr(wt), np.ascontiguousarray(X, dt).reshape(-1), grad, hess, e, m,
     )
     return grad, hess.reshape(dim, dim), float(e[0]), float(m[0])
 
@@ -134,17 +148,18 @@ def _stencil_arrays(es: dict, dim: int, dt):
     s = np.ascontiguousarray(np.stack([np.asarray(es[f"s{k}"]) for k in range(dim)]), dt)
     W = np.ascontiguousarray(es["W_inv"], dt).reshape(-1)
     ws = 0 if W.size == dim * dim else dim * dim
-    return ns, gc, gn.reshape(-1), s.reshape(-1), W, ws
+    return ns, gc, gn.reshape(-1), s.reshape(-1), W, ws, _weights(es.get("weight"), dt)
 
 
 def energy_mindet(dim, kind, delta, es, X, prec="f64"):
-    """(sum mu, min det A) over the energy stencil, accumulated in double."""
+    """(sum of weighted mu, min det A) over the energy stencil, accumulated
+    in double (the stencil's optional "weight" scales mu, not det)."""
     dt = _dt(prec)
-    ns, gc, gn, s, W, ws = _stencil_arrays(es, dim, dt)
+    ns, gc, gn, s, W, ws, wt = _stencil_arrays(es, dim, dt)
     E = np.zeros(1)
     m = np.zeros(1)
     getattr(lib(), f"o_energy_mindet_{prec}")(
-        dim, KIND[kind], float(delta), ns, gc, gn, s, W, ws,
+        dim, KIND[kind], float(delta), ns, gc, gn, s, W, ws, _ptr(wt),
         np.ascontiguousarray(X, dt).reshape(-1), E, m,
     )
     return float(E[0]), float(m[0])
@@ -176,7 +191,7 @@ def interior_occurrences(dim: int, logical, idx):
 class Patches:
     """Per-DOF patches in global id space (CSR over DOFs)."""
 
-    def __init__(self, dim, dof, poff, gc, gn, s, role, W):
+    def __init__(self, dim, dof, poff, gc, gn, s, role, W, wt=None):
         self.dim = dim
         self.dof = np.ascontiguousarray(dof, np.int32)
         self.poff = np.ascontiguousarray(poff, np.int64)
@@ -185,6 +200,7 @@ class Patches:
         self.s = np.ascontiguousarray(s, np.float64)  # (dim, total)
         self.role = np.ascontiguousarray(role, np.int32)
         self.W = np.ascontiguousarray(W, np.float64)  # (dd,) uniform or (total, dd)
+        self.wt = None if wt is None else np.ascontiguousarray(wt, np.float64)  # (total,) or None = unit
 
 
 def materialize_patches(wire: dict, block_global_dof: list[np.ndarray], dim: int, dofs=None) -> Patches:
@@ -209,10 +225,13 @@ def materialize_patches(wire: dict, block_global_dof: list[np.ndarray], dim: int
     ilog = np.asarray(g["interior_logical"], np.int64).reshape(-1, dim)[sel]
     W = np.asarray(g["W_inv"], np.float64)
     uniform = W.size == dim * dim
+    wire_wt = g.get("weight")
     nc = 1 << dim
     interior = iblk >= 0
     if interior.any() and not uniform:
         raise ValueError("interior synthesis requires a uniform W_inv")
+    if interior.any() and wire_wt is not None:
+        raise ValueError("interior synthesis requires unit sample weights")
     cnt = np.where(interior, nc * nc, P_of)
     poff = np.concatenate([[0], np.cumsum(cnt)]).astype(np.int64)
     total = int(poff[-1])
@@ -254,7 +273,11 @@ def materialize_patches(wire: dict, block_global_dof: list[np.ndarray], dim: int
         Wp = W.reshape(-1)
     else:
         Wp = np.concatenate([W.reshape(-1, dim * dim)[soff[d]:soff[d] + P_of[d]] for d in range(n)])
-    return Patches(dim, dof, poff, gc, gn, sg, role, Wp)
+    wt = None
+    if wire_wt is not None:
+        ww = np.asarray(wire_wt, np.float64).reshape(-1)
+        wt = np.concatenate([ww[soff[d]:soff[d] + P_of[d]] for d in range(n)]) if n else ww[:0]
+    return Patches(dim, dof, poff, gc, gn, sg, role, Wp, wt)
 
 
 def materialize_patches_loop(wire: dict, block_global_dof: list[np.ndarray], dim: int) -> Patches:
@@ -268,13 +291,14 @@ def materialize_patches_loop(wire: dict, block_global_dof: list[np.ndarray], dim
     gc_all, gn_all, s_all, role_all, poff = [], [[] for _ in range(dim)], [[] for _ in range(dim)], [], [0]
     W = np.asarray(g["W_inv"], np.float64)
     uniform = W.size == dim * dim
-    Wrows = []
+    wire_wt = g.get("weight")
+    Wrows, wts = [], []
     for d in range(len(dof)):
         if iblk[d] >= 0:
             ids = block_global_dof[iblk[d]]
             gcs, gns, ss, roles = interior_occurrences(dim, tuple(ilog[d]), lambda l: int(ids[l]))
-            if not uniform:
-                raise ValueError("interior synthesis requires a uniform W_inv")
+            if not uniform or wire_wt is not None:
+                raise ValueError("interior synthesis requires a uniform W_inv and unit weights")
         else:
             a, b = soff[d], soff[d + 1]
             gcs = list(np.asarray(g["gc"])[a:b])
@@ -283,6 +307,8 @@ def materialize_patches_loop(wire: dict, block_global_dof: list[np.ndarray], dim
             roles = list(np.asarray(g["role"])[a:b])
             if not uniform:
                 Wrows.append(W.reshape(-1, dim * dim)[a:b])
+            if wire_wt is not None:
+                wts.append(np.asarray(wire_wt, np.float64)[a:b])
         gc_all += gcs
         for k in range(dim):
             gn_all[k] += gns[k]
@@ -290,7 +316,8 @@ def materialize_patches_loop(wire: dict, block_global_dof: list[np.ndarray], dim
         role_all += roles
         poff.append(poff[-1] + len(gcs))
     Wp = W.reshape(-1) if uniform else np.concatenate(Wrows)
-    return Patches(dim, dof, poff, gc_all, gn_all, s_all, role_all, Wp)
+    wt = None if wire_wt is None else (np.concatenate(wts) if wts else np.zeros(0))
+    return Patches(dim, dof, poff, gc_all, gn_all, s_all, role_all, Wp, wt)
 
 
 def jacobi(dim, kind, delta, omega, n_sweeps, X, patches: Patches, es: dict, *, report_every=0,
@@ -307,16 +334,16 @@ def jacobi(dim, kind, delta, omega, n_sweeps, X, patches: Patches, es: dict, *, 
     nrep = (n_sweeps + k - 1) // k
     en = np.zeros(max(nrep, 1))
     md = np.zeros(max(nrep, 1))
-    ns, egc, egn, es_, eW, ews = _stencil_arrays(es, dim, dt)
+    ns, egc, egn, es_, eW, ews, ewt = _stencil_arrays(es, dim, dt)
     W = np.ascontiguousarray(patches.W, dt).reshape(-1)
     ws = 0 if W.size == dim * dim else dim * dim
+    pwt = _weights(patches.wt, dt)
     tau_arr = None if tau is None else np.ascontiguousarray(tau, dt).reshape(-1)
     got = getattr(lib(), f"o_jacobi_{prec}")(
         dim, KIND[kind], float(delta), float(omega), int(n_sweeps), int(report_every),
         len(patches.dof), patches.dof, patches.poff, patches.gc, patches.gn.reshape(-1),
-        np.ascontiguousarray(patches.s, dt).reshape(-1), patches.role, W, ws, n_nodes, Xw,
-        None if tau_arr is None else tau_arr.ctypes.data_as(C.c_void_p),
-        ns, egc, egn, es_, eW, ews, work, en, md,
+        np.ascontiguousarray(patches.s, dt).reshape(-1), patches.role, W, ws, _ptr(pwt), n_nodes, Xw,
+        _ptr(tau_arr), ns, egc, egn, es_, eW, ews, _ptr(ewt), work, en, md,
     )
     return Xw.reshape(-1, dim), en[:got], md[:got]
 
@@ -328,10 +355,11 @@ def patch_grads(dim, kind, X, patches: Patches, *, delta=0.0, tau=None, prec="f6
     out = np.zeros_like(Xw)
     W = np.ascontiguousarray(patches.W, dt).reshape(-1)
     ws = 0 if W.size == dim * dim else dim * dim
+    pwt = _weights(patches.wt, dt)
     tau_arr = None if tau is None else np.ascontiguousarray(tau, dt).reshape(-1)
     getattr(lib(), f"o_patch_grads_{prec}")(
         dim, KIND[kind], float(delta), len(patches.dof), patches.dof, patches.poff, patches.gc,
-        patches.gn.reshape(-1), np.ascontiguousarray(patches.s, dt).reshape(-1), patches.role, W, ws, Xw,
-        None if tau_arr is None else tau_arr.ctypes.data_as(C.c_void_p), out,
+        patches.gn.reshape(-1), np.ascontiguousarray(patches.s, dt).reshape(-1), patches.role, W, ws,
+        _ptr(pwt), Xw, _ptr(tau_arr), out,
     )
     return out.reshape(-1, dim)

@@ -9,9 +9,9 @@ from egg_amd.smoothing import build_structured_context_from_block_maps, build_sw
 
 
 class Case:
-    def __init__(self, grid, w_inv=None):
+    def __init__(self, grid, w_inv=None, weight=None):
         self.grid = grid
-        self.ctx = build_sweep_context(grid.blocks, grid.free, grid.d, w_inv=w_inv)
+        self.ctx = build_sweep_context(grid.blocks, grid.free, grid.d, w_inv=w_inv, weight=weight)
         self.bsc = build_structured_context_from_block_maps(grid.d, grid.blocks)
 
     @property

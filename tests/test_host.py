@@ -222,3 +222,28 @@ def test_level_variants_validated_without_device():
     for bad in ([nv], [-2], [0, nv + 3]):
         with pytest.raises(ValueError, match="level tile variant"):
             s.set_level_variants(bad)
+
+
+def test_weighted_wire_validated_without_device():
+    """A weighted wire is accepted with every DOF stored and the explicit
+    sample reduction; a weight array of the wrong size or with a non-finite
+    entry is refused."""
+    from egg_amd import grids
+    from egg_amd.smoothing import structured_arrays
+    from tests.cases import Case
+
+    g = grids.single_block_2d(9, seed=2, amp=0.2)
+    ns = Case(g).ctx.energy_stencil["num_samples"]
+    c = Case(g, weight=np.linspace(0.5, 1.5, ns))
+
+    def make(wire):
+        return _core().CppStructuredSweepSession(wire, structured_arrays(c.bsc), c.X.ravel(), device="tables")
+
+    info = make(c.ctx.wire).info()
+    assert info["n_synth"] == 0 and info["n_stored"] == g.n_free and info["structured_energy"] == 0
+    grp = dict(c.ctx.wire["groups"][0])
+    for bad, msg in ((grp["weight"][:-1], "weight size"), (np.where(np.arange(len(grp["weight"])) == 3, np.nan,
+                                                                    grp["weight"]), "finite")):
+        wire = {"groups": [dict(grp, weight=bad)], "energy_stencil": c.ctx.energy_stencil}
+        with pytest.raises(ValueError, match=msg):
+            make(wire)

@@ -235,3 +235,102 @@ def test_patch_grads_match_energy_fd(oracle):
             fd[k] = (oracle.energy_mindet(3, "barrier", 0.0, es, Xp)[0]
                      - oracle.energy_mindet(3, "barrier", 0.0, es, Xm)[0]) / (2 * h)
         np.testing.assert_allclose(G[i], fd, rtol=1e-6, atol=1e-6 * np.abs(fd).max())
+
+
+def _weighted_patch(seed=0, P=12):
+    """P well-formed corner samples on jittered unit cells (the reference's
+    tests/smoothing/test_weight.py:17-32 construction), roles drawn from
+    {-1, 0, 1, 2}, with the DOF being node 0 of the first sample."""
+    rng = np.random.default_rng(seed)
+    gc = np.arange(P) * 3
+    corner = rng.normal(size=(P, 2))
+    X = np.empty((3 * P, 2))
+    X[gc] = corner
+    X[gc + 1] = corner + np.array([1.0, 0.0]) + 0.15 * rng.normal(size=(P, 2))
+    X[gc + 2] = corner + np.array([0.0, 1.0]) + 0.15 * rng.normal(size=(P, 2))
+    gn = np.stack([gc + 1, gc + 2])
+    s = np.ones((2, P))
+    W = np.broadcast_to(np.eye(2).reshape(-1), (P, 4)).copy()
+    role = rng.integers(-1, 3, size=P)
+    return X, gc, gn, s, role, W
+
+
+def _stencil(gc, gn, s, W, weight=None):
+    es = {"num_samples": len(gc), "gc": gc, "gn0": gn[0], "gn1": gn[1], "s0": s[0], "s1": s[1], "W_inv": W}
+    if weight is not None:
+        es["weight"] = weight
+    return es
+
+
+@pytest.mark.parametrize("kind", ["shape", "shape_size"])
+def test_weights_unit_and_scalar(oracle, kind):
+    """Unit weights are the identity (same bits); a scalar weight scales the
+    energy, gradient and Hessian but not min det (reference
+    tests/smoothing/test_weight.py:35-68), in the patch evaluation and in the
+    energy-stencil reduction."""
+    X, gc, gn, s, role, W = _weighted_patch(seed=3)
+    P = len(gc)
+    g0, H0, e0, m0 = oracle.patch_eval(2, kind, 0.0, gc, gn, s, role, W, X)
+    g1, H1, e1, m1 = oracle.patch_eval(2, kind, 0.0, gc, gn, s, role, W, X, weight=np.ones(P))
+    assert e1 == e0 and m1 == m0 and np.array_equal(g1, g0) and np.array_equal(H1, H0)
+    gw, Hw, ew, mw = oracle.patch_eval(2, kind, 0.0, gc, gn, s, role, W, X, weight=np.full(P, 2.5))
+    assert ew == pytest.approx(2.5 * e0, rel=1e-14) and mw == m0
+    np.testing.assert_allclose(gw, 2.5 * g0, rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(Hw, 2.5 * H0, rtol=1e-13, atol=1e-15)
+    E0, M0 = oracle.energy_mindet(2, kind, 0.0, _stencil(gc, gn, s, W), X)
+    Ew, Mw = oracle.energy_mindet(2, kind, 0.0, _stencil(gc, gn, s, W, np.full(P, 2.5)), X)
+    assert Ew == pytest.approx(2.5 * E0, rel=1e-14) and Mw == M0
+
+
+def test_weights_patch_matches_reduction_and_zero_drops(oracle):
+    """With varying weights the patch energy and min det equal the weighted
+    energy-stencil reduction over the same samples (reference
+    test_weight.py:71-82); a zero weight removes a sample's mu, gradient and
+    Hessian exactly as dropping the sample does, while its det still bounds
+    min det (the barrier is unweighted)."""
+    X, gc, gn, s, role, W = _weighted_patch(seed=7)
+    P = len(gc)
+    w = np.linspace(0.2, 1.8, P)
+    g, H, e, m = oracle.patch_eval(2, "shape", 0.0, gc, gn, s, role, W, X, weight=w)
+    E, M = oracle.energy_mindet(2, "shape", 0.0, _stencil(gc, gn, s, W, w), X)
+    assert e == pytest.approx(E, rel=1e-14) and m == M
+    z = np.ones(P)
+    drop = [1, 4, 9]
+    z[drop] = 0.0
+    keep = np.setdiff1d(np.arange(P), drop)
+    gz, Hz, ez, mz = oracle.patch_eval(2, "shape", 0.0, gc, gn, s, role, W, X, weight=z)
+    gk, Hk, ek, mk = oracle.patch_eval(2, "shape", 0.0, gc[keep], gn[:, keep], s[:, keep], role[keep], W[keep], X)
+    assert ez == pytest.approx(ek, rel=1e-14)
+    np.testing.assert_allclose(gz, gk, rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(Hz, Hk, rtol=1e-13, atol=1e-15)
+    _, _, _, mall = oracle.patch_eval(2, "shape", 0.0, gc, gn, s, role, W, X)
+    assert mz == mall and mz <= mk
+
+
+def test_weights_on_the_wire(oracle):
+    """A weighted wire ships every DOF's stored stencil with per-occurrence
+    weights taken from its energy-stencil samples; unit weights give the
+    unweighted sweep bit for bit, and a power-of-two weight (exact scaling of
+    energy, gradient and Hessian) the same positions with the energies
+    scaled."""
+    from egg_amd import grids
+    from tests.cases import Case
+
+    g = grids.single_block_2d(9, seed=2, amp=0.2)
+    ns = Case(g).ctx.energy_stencil["num_samples"]
+    one = Case(g, weight=np.ones(ns))
+    two = Case(g, weight=np.full(ns, 2.0))
+    grp = two.ctx.wire["groups"][0]
+    assert np.all(np.asarray(grp["interior_block"]) < 0) and np.all(np.asarray(grp["weight"]) == 2.0)
+    runs = [oracle.jacobi(2, "barrier", 0.0, 0.8, 6, c.X, c.patches(oracle), c.ctx.energy_stencil,
+                          report_every=1) for c in (one, two)]
+    # the same stored wire with the weights taken off
+    bare = {"groups": [{k: v for k, v in grp.items() if k != "weight"} for grp in one.ctx.wire["groups"]],
+            "energy_stencil": {k: v for k, v in one.ctx.energy_stencil.items() if k != "weight"}}
+    Xu, eu, mu = oracle.jacobi(2, "barrier", 0.0, 0.8, 6, one.X, oracle.materialize_patches(bare, g.blocks, 2),
+                               bare["energy_stencil"], report_every=1)
+    np.testing.assert_array_equal(runs[0][0], Xu)
+    np.testing.assert_array_equal(runs[0][1], eu)
+    np.testing.assert_array_equal(runs[1][0], runs[0][0])
+    np.testing.assert_allclose(runs[1][1], 2.0 * runs[0][1], rtol=1e-14)
+    np.testing.assert_array_equal(runs[1][2], runs[0][2])
