@@ -175,6 +175,75 @@ def ogrid_2d(n: int = 129, m: int = 65, *, r: float = 0.5, half: float = 2.0) ->
     return Grid(2, ids, X, free)
 
 
+def _smoothstep(t: float) -> float:
+    t = min(max(t, 0.0), 1.0)
+    return t * t * (3.0 - 2.0 * t)
+
+
+def layer_spacing(k: int, *, first_height: float, growth: float, interior_spacing: float,
+                  n_layers: int = 8) -> float:
+    """Wall-normal target spacing of off-wall layer ``k``: geometric growth
+    from ``first_height``, capped at ``interior_spacing`` and, past
+    ``n_layers``, blended to it over another ``n_layers`` with a Hermite
+    smoothstep (the profile of the reference's
+    BoundaryLayerTarget.normal_spacing, egg/smoothing/targets.py:196-212)."""
+    s = min(first_height * growth**k, interior_spacing)
+    if k > n_layers:
+        w = _smoothstep((k - n_layers) / max(n_layers, 1))
+        s = (1.0 - w) * s + w * interior_spacing
+    return s
+
+
+def boundary_layer_w_inv(grid: Grid, walls: dict, *, first_height: float, growth: float,
+                         tangential_spacing: float, n_layers: int = 8) -> np.ndarray:
+    """Per energy-stencil sample inverse target W^-1, shape (ns, d, d), of
+    wall clustering (reference egg/smoothing/targets.py:59-212,
+    BoundaryLayerTarget): a cell of block b in ``walls`` ({b: (wall_axis,
+    wall_side)}) at off-wall layer k asks for spacing ``layer_spacing(k)``
+    along its wall axis and ``tangential_spacing`` along the others; every
+    other block is isotropic at ``tangential_spacing``. The target is given
+    in the block's logical frame: the reference's W is that diagonal turned
+    by the wall frame (tangent, normal), a proper rotation on the right of
+    A W^-1 that the shape metrics do not see. Samples are ordered as the
+    flat wire's energy stencil (blocks in order, cells row-major, 2^d
+    corners per cell)."""
+    d = grid.d
+    nc = 1 << d
+    rows = []
+    for b, ids in enumerate(grid.blocks):
+        shp = np.asarray(ids).shape
+        if any(n < 2 for n in shp):
+            continue
+        cshape = tuple(n - 1 for n in shp)
+        spacing = np.full(cshape + (d,), float(tangential_spacing))
+        if b in walls:
+            ax, side = walls[b]
+            nk = cshape[ax]
+            prof = np.array([layer_spacing(k, first_height=first_height, growth=growth,
+                                           interior_spacing=tangential_spacing, n_layers=n_layers)
+                             for k in range(nk)])
+            k = np.arange(nk) if side == 0 else nk - 1 - np.arange(nk)
+            shape = [1] * d
+            shape[ax] = nk
+            spacing[..., ax] = np.broadcast_to(prof[k].reshape(shape), cshape)
+        winv = np.zeros(cshape + (d, d))
+        for a in range(d):
+            winv[..., a, a] = 1.0 / spacing[..., a]
+        rows.append(np.repeat(winv.reshape(-1, d, d), nc, axis=0))
+    return np.concatenate(rows) if rows else np.zeros((0, d, d))
+
+
+def ogrid_boundary_layer_w_inv(grid: Grid, n: int, *, r: float = 0.5, first_ratio: float = 0.3,
+                               growth: float = 1.2, n_layers: int = 8) -> np.ndarray:
+    """The boundary-layer target of an ``ogrid_2d(n, m)`` grid: its four ring
+    blocks cluster towards the circle (wall axis 1, low side), first layer
+    ``first_ratio`` of the tangential spacing at the circle (a quarter arc
+    over n - 1 cells)."""
+    s_t = 0.5 * np.pi * r / (n - 1)
+    return boundary_layer_w_inv(grid, {b: (1, 0) for b in range(1, 5)}, first_height=first_ratio * s_t,
+                                growth=growth, tangential_spacing=s_t, n_layers=n_layers)
+
+
 def local_edge_length(grid: Grid) -> np.ndarray:
     """Shortest incident block edge per node."""
     h = np.full(grid.n_nodes, np.inf)

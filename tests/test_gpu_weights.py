@@ -1,5 +1,6 @@
-"""Per-sample energy weights on the MI355X (stored stencils and the explicit
-sample reduction) against the FP64 oracle on the same weighted wire. A
+"""Per-sample energy weights and per-sample (boundary-layer) targets on the
+MI355X (stored stencils and the explicit sample reduction) against the FP64
+oracle on the same wire. A
 weight scales a sample's mu, gradient and Hessian, not its det (reference
 src/patch.hpp:45-77, :250-292); a weighted wire ships every DOF's stored
 stencil (egg/smoothing/flat_context.py:186-218).
@@ -91,3 +92,28 @@ def test_fas_on_a_weighted_wire_is_plain_sweeps():
     ej, mj = b.run(9, omega=0.8, report_every=3)
     np.testing.assert_array_equal(ef, ej)
     np.testing.assert_array_equal(a.get_X(), b.get_X())
+
+
+def test_boundary_layer_target_parity(oracle):
+    """The O-grid's rings under a boundary-layer target (anisotropic W^-1 per
+    sample, reference BoundaryLayerTarget): every DOF on the stored path,
+    parity with the FP64 oracle, and the sweeps pull the first ring layer
+    towards the circle."""
+    g = grids.ogrid_2d(33, 17)
+    W = grids.ogrid_boundary_layer_w_inv(g, 33)
+    c = Case(g, w_inv=W)
+    s = _session(c)
+    assert s.core.info()["n_synth"] == 0
+    e, m = s.run(10, omega=0.8, report_every=5)
+    Xo, eo, mo = oracle.jacobi(2, "barrier", 0.0, 0.8, 10, c.X, c.patches(oracle), c.ctx.energy_stencil,
+                               report_every=5)
+    assert np.abs(s.get_X() - Xo).max() / _extent(c) < POS_TOL
+    np.testing.assert_allclose(e, eo, rtol=TRAJ_RTOL)
+    s.run(400, omega=0.8)
+    ring = np.asarray(g.blocks[1])
+    X0, X1 = c.X.reshape(-1, 2), s.get_X().reshape(-1, 2)
+
+    def first_layer(X):
+        return np.linalg.norm(X[ring[1:-1, 1]] - X[ring[1:-1, 0]], axis=1).mean()
+
+    assert first_layer(X1) < 0.7 * first_layer(X0) and s.run(1, omega=0.8)[1][-1] > 0

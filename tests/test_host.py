@@ -247,3 +247,30 @@ def test_weighted_wire_validated_without_device():
         wire = {"groups": [dict(grp, weight=bad)], "energy_stencil": c.ctx.energy_stencil}
         with pytest.raises(ValueError, match=msg):
             make(wire)
+
+
+def test_boundary_layer_target_profile_and_order():
+    """Wall-normal spacing: first_height at the wall, geometric growth, the
+    cap, and the smoothstep hand-over past n_layers (reference
+    egg/smoothing/targets.py:196-212); W^-1 rows follow the wire's sample
+    order (blocks, cells row-major, 2^d corners) with the layer counted from
+    the wall side."""
+    from egg_amd import grids
+
+    kw = dict(first_height=0.1, growth=2.0, interior_spacing=1.0, n_layers=3)
+    assert [grids.layer_spacing(k, **kw) for k in range(5)] == [0.1, 0.2, 0.4, 0.8, 1.0]  # capped at k = 4
+    kw = dict(first_height=0.1, growth=1.5, interior_spacing=1.0, n_layers=3)
+    w = (1 / 3) ** 2 * (3 - 2 / 3)  # smoothstep((4 - 3) / 3)
+    assert grids.layer_spacing(4, **kw) == pytest.approx(0.1 * 1.5**4 * (1 - w) + w, rel=1e-15)
+    assert grids.layer_spacing(6, **kw) == 1.0
+    g = grids.single_block_2d(5)
+    for side in (0, 1):
+        W = grids.boundary_layer_w_inv(g, {0: (1, side)}, first_height=0.25, growth=2.0, tangential_spacing=1.0,
+                                       n_layers=8)
+        assert W.shape == (4 * 16, 2, 2)
+        for i in range(4):
+            for j in range(4):
+                k = j if side == 0 else 3 - j
+                rows = W[4 * (4 * i + j): 4 * (4 * i + j) + 4]
+                np.testing.assert_array_equal(rows, np.broadcast_to(np.diag([1.0, 1.0 / min(0.25 * 2**k, 1.0)]),
+                                                                    (4, 2, 2)))
