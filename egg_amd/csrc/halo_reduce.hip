@@ -17,13 +17,20 @@ static constexpr TileShape kTiles2[] = {{32, 2, 1}, {16, 4, 1}, {64, 1, 1}, {32,
                                         {16, 1, 1}, {8, 2, 1},  // one lane per sample
                                         {64, 1, 1}, {32, 2, 1},   // no LDS staging
                                         {32, 4, 1}, {64, 4, 1},   // 2 / 4 cells per lane
-                                        {32, 4, 1}, {64, 2, 1}};  // 2 cells per lane, without / with LDS
+                                        {32, 4, 1}, {64, 2, 1},   // 2 cells per lane, without / with LDS
+                                        {32, 2, 1, 4}, {32, 2, 1, 8}};  // 4 / 8 lanes per stored DOF
 static constexpr TileShape kTiles3[] = {{8, 4, 2}, {4, 4, 4}, {8, 4, 2} /* no LDS staging */,
-                                        {8, 4, 2} /* 2 cells per lane */};
+                                        {8, 4, 2} /* 2 cells per lane */,
+                                        {8, 4, 2, 16} /* no LDS staging, 16 lanes per stored DOF */};
 
-int num_tile_variants(int dim) { return dim == 2 ? 12 : 4; }
+int num_tile_variants(int dim) { return dim == 2 ? 14 : 5; }
 bool tile_variant_has_proposal(int dim, int v) { return dim == 2 ? v < 4 : v < 2; }
 TileShape tile_variant_shape(int dim, int v) { return dim == 2 ? kTiles2[v] : kTiles3[v]; }
+int tile_variant_stored_lanes(int dim, int v)
+{
+    const TileShape t = tile_variant_shape(dim, v);
+    return t.sg ? t.sg : stored_lanes(dim);
+}
 
 void launch_sweep(int dim, int kind, int v, const SweepArgs& a, hipStream_t s)
 {

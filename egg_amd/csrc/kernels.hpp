@@ -35,7 +35,8 @@ struct __align__(16) TileDesc {
 };
 
 // Stored-stencil DOFs (block faces, interfaces, singular corners). A group
-// of stored_lanes(dim) lanes serves one DOF, one occurrence per lane and
+// of G lanes serves one DOF (G = stored_lanes(dim) by default; some tile
+// variants take fewer lanes and more rounds), one occurrence per lane and
 // round: DOF d's occurrence r * G + l sits at padded position
 // (d * rounds + r) * G + l, so a lane finds its occurrence without reading
 // an offset table. Padding entries carry meta bit 7.
@@ -129,9 +130,11 @@ void launch_reduce_finish(const double* partials, int nparts, double* out_e, dou
 // Tile shapes the sweep kernels are instantiated for (the autotuner picks one).
 struct TileShape {
     int tx, ty, tz;
+    int sg = 0;  // lanes per stored DOF (0 = stored_lanes(dim))
 };
 int num_tile_variants(int dim);
 TileShape tile_variant_shape(int dim, int v);
+int tile_variant_stored_lanes(int dim, int v);
 // whether the variant has a learned-proposal kernel (the 2D one-lane-per-
 // sample variants do not)
 bool tile_variant_has_proposal(int dim, int v);

@@ -67,6 +67,8 @@ void Level::make_tiles(int v)
 {
     const TileShape ts = tile_variant_shape(lay.dim, v);
     variant = v;
+    const int G = tile_variant_stored_lanes(lay.dim, v);
+    if (G != st_lanes) upload_stored(G);
     // axis 0 <- tz (3D) / ty (2D), last axis <- tx
     const int step[3] = {lay.dim == 2 ? ts.ty : ts.tz, lay.dim == 2 ? ts.tx : ts.ty, lay.dim == 2 ? 1 : ts.tx};
     tiles.clear();
@@ -150,33 +152,7 @@ void Level::upload_tables()
     d_cp_dst.upload(cp_dst);
     d_mask.upload(mask);
     d_st_dof.upload(st_dof);
-    {
-        // stored occurrences in the padded lane layout of StoredView
-        const int G = stored_lanes(lay.dim), D = lay.dim;
-        const long long n = (long long)st_dof.size(), total = (long long)st_gc.size();
-        int rounds = 1;
-        for (long long d = 0; d < n; ++d) rounds = std::max(rounds, (st_off[d + 1] - st_off[d] + G - 1) / G);
-        st_rounds = rounds;
-        st_padded = n * rounds * G;
-        std::vector<int> gc(st_padded, 0), gn((size_t)D * st_padded, 0);
-        std::vector<uint8_t> meta(st_padded, kStoredPad);
-        std::vector<float> W(st_W.empty() ? 0 : (size_t)D * D * st_padded, 0.0f);
-        std::vector<float> wt(st_wt.empty() ? 0 : (size_t)st_padded, 0.0f);
-        for (long long d = 0; d < n; ++d)
-            for (int p = st_off[d]; p < st_off[d + 1]; ++p) {
-                const long long q = d * rounds * G + (p - st_off[d]);
-                gc[q] = st_gc[p];
-                meta[q] = st_meta[p];
-                for (int k = 0; k < D; ++k) gn[(size_t)k * st_padded + q] = st_gn[(size_t)k * total + p];
-                for (int e = 0; e < D * D && !W.empty(); ++e) W[(size_t)D * D * q + e] = st_W[(size_t)D * D * p + e];
-                if (!wt.empty()) wt[q] = st_wt[p];
-            }
-        d_st_gc.upload(gc);
-        d_st_gn.upload(gn);
-        d_st_meta.upload(meta);
-        d_st_W.upload(W);
-        d_st_wt.upload(wt);
-    }
+    st_lanes = 0;  // the stored layout follows the tile variant (make_tiles)
     // structured energy cells
     cell_off.assign(lay.shape.size() + 1, 0);
     for (size_t b = 0; b < lay.shape.size(); ++b) {
@@ -188,6 +164,38 @@ void Level::upload_tables()
         if (cell_off[b + 1] - cell_off[b] >= (1ll << 31)) throw InvalidArgument("a block has 2^31 or more cells");
     ncells = cell_off.back();
     d_cell_off.upload(cell_off);
+}
+
+// Stored occurrences in the padded lane layout of StoredView for groups of G
+// lanes per DOF: occurrence p of DOF d goes to round p / G, lane p % G.
+void Level::upload_stored(int G)
+{
+    const int D = lay.dim;
+    const long long n = (long long)st_dof.size(), total = (long long)st_gc.size();
+    int rounds = 1;
+    for (long long d = 0; d < n; ++d) rounds = std::max(rounds, (st_off[d + 1] - st_off[d] + G - 1) / G);
+    st_rounds = rounds;
+    st_padded = n * rounds * G;
+    std::vector<int> gc(st_padded, 0), gn((size_t)D * st_padded, 0);
+    std::vector<uint8_t> meta(st_padded, kStoredPad);
+    std::vector<float> W(st_W.empty() ? 0 : (size_t)D * D * st_padded, 0.0f);
+    std::vector<float> wt(st_wt.empty() ? 0 : (size_t)st_padded, 0.0f);
+    for (long long d = 0; d < n; ++d)
+        for (int p = st_off[d]; p < st_off[d + 1]; ++p) {
+            const int o = p - st_off[d];
+            const long long q = (d * rounds + o / G) * G + o % G;
+            gc[q] = st_gc[p];
+            meta[q] = st_meta[p];
+            for (int k = 0; k < D; ++k) gn[(size_t)k * st_padded + q] = st_gn[(size_t)k * total + p];
+            for (int e = 0; e < D * D && !W.empty(); ++e) W[(size_t)D * D * q + e] = st_W[(size_t)D * D * p + e];
+            if (!wt.empty()) wt[q] = st_wt[p];
+        }
+    d_st_gc.upload(gc);
+    d_st_gn.upload(gn);
+    d_st_meta.upload(meta);
+    d_st_W.upload(W);
+    d_st_wt.upload(wt);
+    st_lanes = G;
 }
 
 SweepArgs Level::sweep_args(const float* xin, float* xout, float omega, float delta, const float* tau) const

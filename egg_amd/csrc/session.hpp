@@ -140,6 +140,7 @@ struct Level {
     DArr<int> d_st_dof, d_st_gc, d_st_gn;  // device copies in the padded lane layout (see StoredView)
     int st_rounds = 1;
     long long st_padded = 0;
+    int st_lanes = 0;  // lanes per stored DOF of the uploaded layout (0 = none yet)
     DArr<uint8_t> d_st_meta;
     DArr<float> d_st_W, d_st_wt;
     // energy reduction: structured cells, or an explicit sample table
@@ -165,7 +166,8 @@ struct Level {
 
     BlockDesc block_desc(size_t b) const;
     void upload_tables();
-    void make_tiles(int v);
+    void make_tiles(int v);  // also lays the stored DOFs out for the variant's lane groups
+    void upload_stored(int G);
     void build_push();
     SweepArgs sweep_args(const float* xin, float* xout, float omega, float delta, const float* tau) const;
 };

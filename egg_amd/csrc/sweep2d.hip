@@ -4,7 +4,8 @@
 // lane per cell (4 per node); 4-5: one lane per sample (16 per node); 6-7:
 // one lane per cell without LDS staging; 8, 11: two cells per lane (2 lanes
 // per node), 10: the same without LDS staging; 9: one lane per node (its 4
-// cells in turn). The one-lane-per-cell
+// cells in turn); 12, 13: variant 0 with 4 / 8 lanes per stored DOF instead
+// of 16 (stored occurrences in turn). The one-lane-per-cell
 // shapes give the same bits; the cells-per-lane ones fold the cells in the
 // same order but the compiler fuses their multiply-adds differently.
 #include "sweep_impl.hpp"
@@ -26,6 +27,9 @@ template <int KIND, bool TAU, bool GRAD, bool PROP = false> static void sweep2d_
         if (v == 9) return launch_variant<2, KIND, 64, 4, 1, TAU, GRAD, false, true, 4>(a, s);
         if (v == 10) return launch_variant<2, KIND, 32, 4, 1, TAU, GRAD, false, false, 2>(a, s);  // + no LDS
         if (v == 11) return launch_variant<2, KIND, 64, 2, 1, TAU, GRAD, false, true, 2>(a, s);
+        // variant 0's tiles with 4 / 8 lanes per stored DOF (4 / 2 occurrences per lane)
+        if (v == 12) return launch_variant<2, KIND, 32, 2, 1, TAU, GRAD, false, true, 1, 4>(a, s);
+        if (v == 13) return launch_variant<2, KIND, 32, 2, 1, TAU, GRAD, false, true, 1, 8>(a, s);
     }
     switch (v) {
     case 0: launch_variant<2, KIND, 32, 2, 1, TAU, GRAD, PROP>(a, s); break;

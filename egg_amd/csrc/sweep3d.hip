@@ -3,7 +3,8 @@
 // shapes in nodes (last axis x middle x first axis). Variants 0-1: one lane
 // per cell (8 per node); 2: the same without LDS staging; 3: two cells per
 // lane (4 lanes per node): same fold order as one per lane, not the same
-// multiply-add fusion.
+// multiply-add fusion; 4: variant 2 with 16 lanes per stored DOF instead of
+// 64 (4 occurrences per lane in turn).
 #include "sweep_impl.hpp"
 
 #include <stdexcept>
@@ -17,6 +18,8 @@ template <int KIND, bool TAU, bool GRAD, bool PROP = false> static void sweep3d_
     if constexpr (!PROP) {
         if (v == 2) return launch_variant<3, KIND, 8, 4, 2, TAU, GRAD, false, false>(a, s);  // no LDS staging
         if (v == 3) return launch_variant<3, KIND, 8, 4, 2, TAU, GRAD, false, true, 2>(a, s);  // cells per lane
+        if (v == 4)  // no LDS staging, 16 lanes per stored DOF (4 occurrences per lane)
+            return launch_variant<3, KIND, 8, 4, 2, TAU, GRAD, false, false, 1, 16>(a, s);
     }
     switch (v) {
     case 0: launch_variant<3, KIND, 8, 4, 2, TAU, GRAD, PROP>(a, s); break;

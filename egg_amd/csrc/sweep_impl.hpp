@@ -155,7 +155,7 @@ __device__ __forceinline__ void count_update(const SweepArgs& a, int ntr, bool p
 template <int D, int KIND, bool TAU, bool GRAD, int G, bool WT>
 __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
 {
-    static_assert(G == stored_lanes(D), "stored lane group size");
+    static_assert(G >= 4 && G <= stored_lanes(D), "stored lane group size");
     const StoredView& st = a.st;
     const float* X = a.xin;
     const long long PL = a.plane;
@@ -860,13 +860,13 @@ void launch_sample_variant(const SweepArgs& a, hipStream_t s)
 }
 
 // One launch per sweep: the leading workgroups serve the stored DOFs (one
-// lane group of stored_lanes(D) per DOF), the rest one tile each with
-// 2^D / CPL lanes per node (CPL cells per lane).
-template <int D, int KIND, int TX, int TY, int TZ, bool TAU, bool GRAD, bool PROP, bool LDS, int CPL>
+// lane group of SG lanes per DOF), the rest one tile each with 2^D / CPL
+// lanes per node (CPL cells per lane).
+template <int D, int KIND, int TX, int TY, int TZ, bool TAU, bool GRAD, bool PROP, bool LDS, int CPL, int SG>
 __device__ __forceinline__ void sweep_body(const SweepArgs& a)
 {
     constexpr int NT = ((1 << D) / CPL) * TX * TY * TZ;
-    constexpr int G = stored_lanes(D);
+    constexpr int G = SG;
     static_assert(NT % G == 0, "a workgroup holds whole stored lane groups");
     // stored-DOF workgroups first: their gather chains are the longest, so
     // they start while the tiles fill the machine
@@ -882,10 +882,11 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& a)
     tile_sweep<D, KIND, TX, TY, TZ, TAU, GRAD, PROP, LDS, CPL>(a, int(blockIdx.x) - n_st);
 }
 
-template <int D, int KIND, int TX, int TY, int TZ, bool TAU, bool GRAD, bool PROP, bool LDS>
+template <int D, int KIND, int TX, int TY, int TZ, bool TAU, bool GRAD, bool PROP, bool LDS,
+          int SG = stored_lanes(D)>
 __global__ __launch_bounds__((1 << D) * TX * TY * TZ) void sweep_kernel(const SweepArgs a)
 {
-    sweep_body<D, KIND, TX, TY, TZ, TAU, GRAD, PROP, LDS, 1>(a);
+    sweep_body<D, KIND, TX, TY, TZ, TAU, GRAD, PROP, LDS, 1, SG>(a);
 }
 
 // several cells per lane: the compiler interleaves a lane's cells for ILP;
@@ -895,21 +896,22 @@ template <int D, int KIND, int TX, int TY, int TZ, bool TAU, bool GRAD, bool LDS
 __global__ __attribute__((amdgpu_flat_work_group_size(1, ((1 << D) / CPL) * TX * TY * TZ),
                           amdgpu_waves_per_eu(MINW))) void sweep_kernel_cpl(const SweepArgs a)
 {
-    sweep_body<D, KIND, TX, TY, TZ, TAU, GRAD, false, LDS, CPL>(a);
+    sweep_body<D, KIND, TX, TY, TZ, TAU, GRAD, false, LDS, CPL, stored_lanes(D)>(a);
 }
 
 template <int D, int KIND, int TX, int TY, int TZ, bool TAU, bool GRAD, bool PROP = false, bool LDS = true,
-          int CPL = 1>
+          int CPL = 1, int SG = stored_lanes(D)>
 void launch_variant(const SweepArgs& a, hipStream_t s)
 {
+    static_assert(CPL == 1 || SG == stored_lanes(D), "cells-per-lane kernels take the default stored groups");
     constexpr int NT = ((1 << D) / CPL) * TX * TY * TZ;
-    constexpr int G = stored_lanes(D);
+    constexpr int G = SG;
     const int per_wg = NT / G;
     const int nst = (a.st.n + per_wg - 1) / per_wg;
     const unsigned grid = (unsigned)(a.n_tiles + nst);
     if (grid == 0) return;
     if constexpr (CPL == 1)
-        sweep_kernel<D, KIND, TX, TY, TZ, TAU, GRAD, PROP, LDS><<<dim3(grid), dim3(NT), 0, s>>>(a);
+        sweep_kernel<D, KIND, TX, TY, TZ, TAU, GRAD, PROP, LDS, SG><<<dim3(grid), dim3(NT), 0, s>>>(a);
     else
         sweep_kernel_cpl<D, KIND, TX, TY, TZ, TAU, GRAD, LDS, CPL, (D == 2 && CPL == 2 ? 4 : 2)>
             <<<dim3(grid), dim3(NT), 0, s>>>(a);

@@ -36,8 +36,11 @@ def _extent(case):
     return float((X.max(0) - X.min(0)).max())
 
 
-def _run_both(oracle, case, n, *, phase="barrier", delta=0.0, omega=0.8, report_every=1, prec="f64"):
+def _run_both(oracle, case, n, *, phase="barrier", delta=0.0, omega=0.8, report_every=1, prec="f64",
+              variant=None):
     s = _session(case)
+    if variant is not None:
+        s.core.set_tile_variant(variant)
     e, m = s.run(n, phase=phase, delta=delta, omega=omega, report_every=report_every)
     Xg = s.get_X()
     Xo, eo, mo = oracle.jacobi(case.d, phase, delta, omega, n, case.X, case.patches(oracle),
@@ -91,18 +94,22 @@ def test_multiblock_parity(oracle, make):
     np.testing.assert_allclose(e[-1], eo[-1], rtol=ENERGY_RTOL)
 
 
+@pytest.mark.parametrize("small", [False, True])
 @pytest.mark.parametrize("kw,phase", [(dict(k=5, n=9), "barrier"), (dict(k=6, n=9), "barrier"),
                                       (dict(k=5, n=9), "untangle"), (dict(k=5, n=7, layers=6), "barrier")])
-def test_many_occurrence_stored_dofs(oracle, kw, phase):
+def test_many_occurrence_stored_dofs(oracle, kw, phase, small):
     """Singular DOFs whose patches take more than one round of their lane
     group: the centre of a valence-5 / 6 star (20 / 24 occurrences for 16
     lanes) and the centre line of the extruded star (80 occurrences for 64
-    lanes), against the FP64 oracle."""
+    lanes), against the FP64 oracle; `small`: with the tile variant of the
+    fewest lanes per stored DOF (2D: 4, 3D: 16), so every stored DOF takes
+    several rounds."""
     c = star(**kw)
     info = _session(c).core.info()
     assert info["n_stored"] > 0
     delta = 1e-3 if phase == "untangle" else 0.0
-    _, Xg, e, m, Xo, eo, mo = _run_both(oracle, c, 20, phase=phase, delta=delta)
+    variant = (12 if c.d == 2 else 4) if small else None
+    _, Xg, e, m, Xo, eo, mo = _run_both(oracle, c, 20, phase=phase, delta=delta, variant=variant)
     err = np.abs(Xg - Xo).max() / _extent(c)
     print(f"star {kw} {phase}: max|dX|/extent={err:.2e}")
     assert err < POS_TOL
@@ -175,11 +182,12 @@ def test_tile_variants_agree(oracle):
     11); the several-cells-per-lane kernels (two, 9: four) fold a patch in
     the one-lane-per-cell order but fuse multiply-adds
     differently, and the one-lane-per-sample (4-5) layout sums it in another
-    order; all match the oracle."""
+    order; 12 and 13 are variant 0 with 4 / 8 lanes per stored DOF (its
+    occurrences summed in another order); all match the oracle."""
     c = ogrid(33, 17)
     out = []
     nv = _session(c).core.set_tile_variant(-1)
-    assert nv == 12
+    assert nv == 14
     for v in range(nv):
         s = _session(c)
         s.core.set_tile_variant(v)
@@ -191,7 +199,7 @@ def test_tile_variants_agree(oracle):
     Xo, _, _ = oracle.jacobi(2, "barrier", 0.0, 0.8, 5, c.X, c.patches(oracle), c.ctx.energy_stencil)
     for v in (10, 11):
         np.testing.assert_array_equal(out[v], out[8])
-    for X in (out[0], out[4], out[8], out[9]):
+    for X in (out[0], out[4], out[8], out[9], out[12], out[13]):
         assert np.abs(X - Xo).max() / _extent(c) < POS_TOL
 
 
@@ -329,12 +337,14 @@ def test_set_get_x_roundtrip():
 def test_3d_tile_variants_agree():
     """3D tile shapes (8x4x2, 4x4x4, 8x4x2 without LDS staging) are bitwise
     identical on the cubed sphere (stored DOFs included); two cells per lane
-    (variant 3) folds in the same order with other multiply-add fusion."""
+    (variant 3) folds in the same order with other multiply-add fusion, and
+    variant 4 (16 lanes per stored DOF) sums stored patches in another
+    order."""
     from egg_amd import grids
 
     c = Case(grids.cubed_sphere_3d(9))
     nv = _session(c).core.set_tile_variant(-1)
-    assert nv == 4
+    assert nv == 5
     out = []
     for v in range(nv):
         s = _session(c)
@@ -343,7 +353,8 @@ def test_3d_tile_variants_agree():
         out.append(s.get_X())
     for X in out[1:3]:
         np.testing.assert_array_equal(X, out[0])
-    assert np.abs(out[3] - out[0]).max() / _extent(c) < POS_TOL
+    for X in out[3:]:
+        assert np.abs(X - out[0]).max() / _extent(c) < POS_TOL
 
 
 def test_graph_chunks_match_launches():

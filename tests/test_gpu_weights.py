@@ -41,10 +41,14 @@ def _extent(c):
 
 @pytest.mark.parametrize("make", [lambda: grids.single_block_2d(17, seed=5, amp=0.2), lambda: lr_pair().grid,
                                   lambda: ogrid(33, 17).grid, lambda: grids.single_block_3d(7, seed=2, amp=0.2)])
-def test_weighted_parity(oracle, make):
+@pytest.mark.parametrize("small", [False, True])
+def test_weighted_parity(oracle, make, small):
+    """`small`: the tile variant with the fewest lanes per stored DOF."""
     g = make()
     c = _weighted(g)
     s = _session(c)
+    if small:
+        s.core.set_tile_variant(12 if g.d == 2 else 4)
     info = s.core.info()
     assert info["n_synth"] == 0 and info["n_stored"] == g.n_free and info["structured_energy"] == 0
     e, m = s.run(12, omega=0.8, report_every=3)
@@ -94,15 +98,17 @@ def test_fas_on_a_weighted_wire_is_plain_sweeps():
     np.testing.assert_array_equal(a.get_X(), b.get_X())
 
 
-def test_boundary_layer_target_parity(oracle):
+@pytest.mark.parametrize("variant", [0, 12, 13])
+def test_boundary_layer_target_parity(oracle, variant):
     """The O-grid's rings under a boundary-layer target (anisotropic W^-1 per
-    sample, reference BoundaryLayerTarget): every DOF on the stored path,
-    parity with the FP64 oracle, and the sweeps pull the first ring layer
-    towards the circle."""
+    sample, reference BoundaryLayerTarget): every DOF on the stored path
+    (16, 4 or 8 lanes per DOF), parity with the FP64 oracle, and the sweeps
+    pull the first ring layer towards the circle."""
     g = grids.ogrid_2d(33, 17)
     W = grids.ogrid_boundary_layer_w_inv(g, 33)
     c = Case(g, w_inv=W)
     s = _session(c)
+    s.core.set_tile_variant(variant)
     assert s.core.info()["n_synth"] == 0
     e, m = s.run(10, omega=0.8, report_every=5)
     Xo, eo, mo = oracle.jacobi(2, "barrier", 0.0, 0.8, 10, c.X, c.patches(oracle), c.ctx.energy_stencil,
