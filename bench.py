@@ -55,7 +55,7 @@ GRAPH_CHUNK = 256  # sweeps per captured hipGraph (5-8 % per sweep on the small 
 # HBM bytes per sweep launch by workload and tile variant, from two rocprofv3
 # --pmc runs of this file (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch,
 # median; profiles/sweep2d_ogrid129.md, batched64_fas.md,
-# cubed_sphere65_fas_pcg.md). A device copy of N bytes reads FETCH_SIZE =
+# cubed_sphere65_fas_pcg.md, ogrid129_boundary_layer.md). A device copy of N bytes reads FETCH_SIZE =
 # N/2 and WRITE_SIZE = N on this part, so FETCH is scaled by 2 and WRITE by 1.
 PMC_KIB = {
     ("ogrid", 0): (1314.06, 548.44), ("ogrid", 1): (1466.69, 616.38), ("ogrid", 2): (1218.31, 517.69),
@@ -70,6 +70,12 @@ PMC_KIB = {
     ("batched", 10): (508547.75, 277069.69), ("batched", 11): (435552.28, 252758.38),
     ("cubed_sphere", 0): (135327.31, 49889.66), ("cubed_sphere", 1): (185090.25, 69130.72),
     ("cubed_sphere", 2): (133390.06, 49889.95), ("cubed_sphere", 3): (140850.5, 51144.95),
+    ("cubed_sphere", 4): (132029.03, 49050.84),
+    ("ogrid_bl", 0): (12851.0, 613.5), ("ogrid_bl", 1): (12856.0, 613.28), ("ogrid_bl", 2): (12855.0, 613.34),
+    ("ogrid_bl", 3): (12605.38, 555.81), ("ogrid_bl", 4): (12839.0, 613.16), ("ogrid_bl", 5): (12845.0, 613.12),
+    ("ogrid_bl", 6): (12855.5, 613.31), ("ogrid_bl", 7): (12854.5, 613.38), ("ogrid_bl", 8): (12853.0, 613.12),
+    ("ogrid_bl", 9): (12851.5, 617.44), ("ogrid_bl", 10): (12852.5, 613.0), ("ogrid_bl", 11): (12856.0, 613.09),
+    ("ogrid_bl", 12): (12221.75, 522.84), ("ogrid_bl", 13): (12616.06, 555.88),
 }
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 
