@@ -149,9 +149,11 @@ eggmi_status eggmi_halo_exchange(eggmi_session* s);
  * a node's 4 cells (tiles of 32x2, 16x4, 64x1, 32x4 nodes), 4-5 one lane on
  * each of its 16 samples (16x1, 8x2 nodes), 6-7 one lane per cell without
  * LDS staging (64x1, 32x2), 8 and 11 two cells per lane (32x4, 64x2), 10
- * the same without LDS staging (32x4), 9 one lane per node (64x4). 3D: one
- * lane per cell, tiles of 8x4x2 and 4x4x4 nodes, 2 = 8x4x2 without LDS
- * staging, 3 = 8x4x2 with two cells per lane. Only variants 0-3 (2D) and
+ * the same without LDS staging (32x4), 9 one lane per node (64x4), 12 and
+ * 13 variant 0 with 4 / 8 lanes per stored-stencil DOF instead of 16. 3D:
+ * one lane per cell, tiles of 8x4x2 and 4x4x4 nodes, 2 = 8x4x2 without LDS
+ * staging, 3 = 8x4x2 with two cells per lane, 4 = variant 2 with 16 lanes
+ * per stored-stencil DOF instead of 64. Only variants 0-3 (2D) and
  * 0-1 (3D) have a learned-proposal kernel; setting a proposal on a session
  * with another variant selects variant 0. */
 eggmi_status eggmi_set_tile_variant(eggmi_session* s, int variant, int* n_variants);
