@@ -154,6 +154,41 @@ struct Wire {
                                           " != " + std::to_string(eg.total_samples));
                 eg.weight = keep(wt);
             }
+            if (g.contains("entities") && !g["entities"].is_none() && py::len(g["entities"]) > 0) {
+                // the reference's per-type SoA sub-dicts, flattened to one tag and
+                // record per group-local DOF (every DOF covered exactly once)
+                std::vector<int32_t> tag(eg.ndof, -1);
+                std::vector<double> rec(eg.ndof * 4, 0.0);
+                for (auto item : g["entities"].cast<py::dict>()) {
+                    py::dict ed = item.second.cast<py::dict>();
+                    if (!ed.contains("tag"))
+                        throw py::value_error("unpack_context: entities carries a group without a tag");
+                    const int t = ed["tag"].cast<int>(), kf = ed["kFields"].cast<int>();
+                    if (!((t == 0 && kf == 0) || (t == 1 && kf == 4) || (t == 2 && kf == 3)))
+                        throw py::value_error("wire: entity tag " + std::to_string(t) + " (" +
+                                              item.first.cast<std::string>() +
+                                              ") is not supported by this compute core (free, lineseg, circle)");
+                    IArr loc = get_i(ed, "dof_local");
+                    std::vector<double> r;
+                    if (kf) {
+                        DArr rr = get_d(ed, "records");
+                        if ((int64_t)rr.size() != (int64_t)loc.size() * kf)
+                            throw py::value_error("wire: entity records size != count * kFields");
+                        r.assign(rr.data(), rr.data() + rr.size());
+                    }
+                    for (py::ssize_t i = 0; i < loc.size(); ++i) {
+                        const int l = loc.data()[i];
+                        if (l < 0 || l >= eg.ndof || tag[l] >= 0)
+                            throw py::value_error("unpack_context: entity dof_local out of range or repeated");
+                        tag[l] = t;
+                        for (int k = 0; k < kf; ++k) rec[(size_t)l * 4 + k] = r[(size_t)i * kf + k];
+                    }
+                }
+                for (int32_t t : tag)
+                    if (t < 0) throw py::value_error("unpack_context: entities do not cover every DOF of the group");
+                eg.entity_tag = keep(IArr((py::ssize_t)tag.size(), tag.data()));
+                eg.entity_rec = keep(DArr((py::ssize_t)rec.size(), rec.data()));
+            }
             if (g.contains("curv_nodes") || g.contains("dir_nodes"))
                 throw py::value_error("wire: curvature / directional terms are not supported by this compute "
                                       "core");

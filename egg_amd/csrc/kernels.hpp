@@ -68,6 +68,12 @@ struct SweepArgs {
     float omega = 1.0f;
     float delta = 0.0f;
     const float* tau = nullptr;  // FAS linear shift (plane layout), null = none
+    // constrained boundary DOFs (2D): stored DOFs st.n .. st.n + con_n - 1 of
+    // the stored arrays, each sliding on the entity con_tag[i] (1 segment
+    // sx sy ex ey, 2 circle cx cy r) with record con_rec[4 i ..]
+    int con_n = 0;
+    const uint8_t* con_tag = nullptr;
+    const float* con_rec = nullptr;
     float* gout = nullptr;       // gradient-only launches: per-node gradient (plane layout)
     bool gsub = false;           // gradient-only: gout = gradient - gout, in place
     bool gpush = false;          // gradient-only: also write it into the DOF's replica slots

@@ -152,6 +152,8 @@ void Level::upload_tables()
     d_cp_dst.upload(cp_dst);
     d_mask.upload(mask);
     d_st_dof.upload(st_dof);
+    d_con_tag.upload(con_tag);
+    d_con_rec.upload(con_rec);
     st_lanes = 0;  // the stored layout follows the tile variant (make_tiles)
     // structured energy cells
     cell_off.assign(lay.shape.size() + 1, 0);
@@ -208,7 +210,10 @@ SweepArgs Level::sweep_args(const float* xin, float* xout, float omega, float de
     a.n_tiles = (int)tiles.size();
     a.blocks = blocks.p;
     a.mask = d_mask.p;
-    a.st.n = (int)st_dof.size();
+    a.st.n = (int)(st_dof.size() - con_tag.size());  // the constrained DOFs follow (con_*)
+    a.con_n = (int)con_tag.size();
+    a.con_tag = d_con_tag.p;
+    a.con_rec = d_con_rec.p;
     a.st.rounds = st_rounds;
     a.st.dof = d_st_dof.p;
     a.st.total = st_padded;
@@ -281,6 +286,18 @@ void validate(int dim, const eggmi_group* groups, int n_groups, const eggmi_sten
             for (long long p = 0; p < g.total_samples; ++p)
                 if (!std::isfinite(g.weight[p]))
                     throw InvalidArgument("validate_context: " + grp + " weight must be finite");
+        if (g.entity_tag)
+            for (long long d = 0; d < g.ndof; ++d) {
+                const int t = g.entity_tag[d];
+                if (t < 0 || t > 2)
+                    throw InvalidArgument("validate_context: " + grp + " entity tag " + std::to_string(t) +
+                                          " is not supported (0 free, 1 segment, 2 circle)");
+                if (t != 0 && dim != 2)
+                    throw InvalidArgument("validate_context: constrained boundary DOFs are 2D only here");
+                for (int k = 0; k < 4; ++k)
+                    if (t != 0 && !std::isfinite(g.entity_rec[d * 4 + k]))
+                        throw InvalidArgument("validate_context: " + grp + " entity record must be finite");
+            }
     }
     if (es->num_samples < 0) throw InvalidArgument("validate_context: energy_stencil negative size");
     if (es->w_rows != 1 && es->w_rows != es->num_samples)
@@ -534,10 +551,18 @@ void Session::create(int d, const eggmi_group* groups, int n_groups, const eggmi
                 w_uniform_all = false;
         }
     }
+    // constrained boundary DOFs (an entity to slide on) are stored last, so
+    // the sweep's stored view excludes them and a kernel of their own takes them
+    for (int pass = 0; pass < 2; ++pass)
     for (int gi = 0; gi < n_groups; ++gi) {
         const eggmi_group& g = groups[gi];
         long long off = 0;
         for (long long di = 0; di < g.ndof; ++di) {
+            const bool con = g.entity_tag && g.entity_tag[di] != 0;
+            if (con != (pass == 1)) {
+                off += g.P_of[di];
+                continue;
+            }
             const int gid = g.dof_idx[di];
             const int ob = owner_block[gid];
             if (ob < 0) throw InvalidArgument("structured: dof_idx " + std::to_string(gid) + " is in no block");
@@ -549,6 +574,11 @@ void Session::create(int d, const eggmi_group* groups, int n_groups, const eggmi
             }
             const long long dslot = lookup(ob, gid, "dof_idx");
             const int ib = g.interior_block[di];
+            if (con) {
+                if (ib >= 0) throw InvalidArgument("structured: a synthesized interior DOF cannot be constrained");
+                L.con_tag.push_back((uint8_t)g.entity_tag[di]);
+                for (int k = 0; k < 4; ++k) L.con_rec.push_back((float)g.entity_rec[di * 4 + k]);
+            }
             if (ib >= 0) {
                 if (P != 0) throw InvalidArgument("structured: interior DOF with stored samples");
                 if (!wt_unit_all)
@@ -605,6 +635,9 @@ void Session::create(int d, const eggmi_group* groups, int n_groups, const eggmi
     for (int s : synth_slots) L.mask[s] = 1;
     fine_free = L.mask;
     for (int s : L.st_dof) fine_free[s] = 1;
+    // constrained DOFs stay out of the multigrid (its corrections would move
+    // them off their entity): neither free for the hierarchy nor corrected
+    for (size_t i = L.st_dof.size() - L.con_tag.size(); i < L.st_dof.size(); ++i) fine_free[L.st_dof[i]] = 0;
     fine_halo.share_src.assign(share_src.begin(), share_src.end());
     fine_halo.share_dst.assign(share_dst.begin(), share_dst.end());
     L.n_synth = (long long)synth_slots.size();

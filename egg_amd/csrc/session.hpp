@@ -136,6 +136,12 @@ struct Level {
     std::vector<uint8_t> st_meta;
     std::vector<float> st_W;  // empty = uniform
     std::vector<float> st_wt;  // per-occurrence energy weight; empty = 1
+    // the last con_tag.size() stored DOFs are constrained to an entity (2D):
+    // tag 1 segment, 2 circle, and a record of 4 floats each
+    std::vector<uint8_t> con_tag;
+    std::vector<float> con_rec;
+    DArr<uint8_t> d_con_tag;
+    DArr<float> d_con_rec;
     float Wu[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
     DArr<int> d_st_dof, d_st_gc, d_st_gn;  // device copies in the padded lane layout (see StoredView)
     int st_rounds = 1;

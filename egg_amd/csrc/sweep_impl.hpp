@@ -150,11 +150,49 @@ __device__ __forceinline__ void count_update(const SweepArgs& a, int ntr, bool p
 
 // Stored-stencil DOF d handled by a group of G lanes (lane l owns
 // occurrences l, l+G, ... of the DOF's patch).
+// 2D entities a constrained boundary DOF slides on (reference
+// src/geometry.hpp:288-320 LineSeg, :398-431 Circle; FP32 tolerances of
+// src/core.hpp: znorm 1e-7, tiny 1e-20): tag 1 segment (sx, sy, ex, ey),
+// tag 2 circle (cx, cy, r).
+__device__ __forceinline__ void ent_project(int tag, const float* e, const float* p, float* out)
+{
+    if (tag == 1) {
+        const float abx = e[2] - e[0], aby = e[3] - e[1];
+        float t = ((p[0] - e[0]) * abx + (p[1] - e[1]) * aby) / fmaxf(abx * abx + aby * aby, 1e-20f);
+        t = fminf(fmaxf(t, 0.0f), 1.0f);
+        out[0] = e[0] + t * abx;
+        out[1] = e[1] + t * aby;
+    } else {
+        const float dx = p[0] - e[0], dy = p[1] - e[1];
+        const float dist = sqrtf(dx * dx + dy * dy);
+        out[0] = dist < 1e-7f ? e[0] + e[2] : e[0] + e[2] * dx / dist;
+        out[1] = dist < 1e-7f ? e[1] : e[1] + e[2] * dy / dist;
+    }
+}
+__device__ __forceinline__ void ent_tangent(int tag, const float* e, const float* p, float* t)
+{
+    if (tag == 1) {
+        const float abx = e[2] - e[0], aby = e[3] - e[1];
+        const float n = sqrtf(abx * abx + aby * aby);
+        t[0] = n < 1e-7f ? 1.0f : abx / n;
+        t[1] = n < 1e-7f ? 0.0f : aby / n;
+    } else {
+        const float dx = p[0] - e[0], dy = p[1] - e[1];
+        const float rn = sqrtf(dx * dx + dy * dy);
+        t[0] = rn < 1e-7f ? -0.0f : -dy / rn;
+        t[1] = rn < 1e-7f ? 1.0f : dx / rn;
+    }
+}
+
 // WT: per-occurrence energy weights (st.wt); a separate instantiation, so the
-// unweighted sums keep their exact operation order.
-template <int D, int KIND, bool TAU, bool GRAD, int G, bool WT>
+// unweighted sums keep their exact operation order. ENT: d is a constrained
+// boundary DOF (2D; stored index st.n + i slides on entity i): tangent-reduced
+// Newton step damped by omega, trials projected onto the entity, no
+// post-blend (reference src/sweep.hpp:1285-1403, :1848-1871).
+template <int D, int KIND, bool TAU, bool GRAD, int G, bool WT, bool ENT = false>
 __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
 {
+    static_assert(!ENT || (D == 2 && !TAU && !GRAD), "constrained DOFs: 2D plain sweeps");
     static_assert(G >= 4 && G <= stored_lanes(D), "stored lane group size");
     const StoredView& st = a.st;
     const float* X = a.xin;
@@ -298,7 +336,29 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
 #pragma unroll
     for (int k = 0; k < kNH<D>; ++k) h[k] = group_sum<G>(h[k]);
     float dl[D];
-    newton<D>(g, h, dl);
+    int etag = 0;
+    float erec[4] = {};
+    if constexpr (ENT) {
+        const int ci = d - st.n;
+        etag = a.con_tag[ci];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) erec[k] = a.con_rec[4 * ci + k];
+        // y = solve1x1(t.H.t, t.g) (src/solve.hpp:83-90), delta = omega t y
+        float t[2];
+        ent_tangent(etag, erec, pos, t);
+        const float M = t[0] * (h[0] * t[0] + h[1] * t[1]) + t[1] * (h[1] * t[0] + h[2] * t[1]);
+        const float r = t[0] * g[0] + t[1] * g[1];
+        const float rn = fabsf(r);
+        float y = 0.0f;
+        if (rn >= 1e-7f) {
+            y = -r / M;
+            if (!isfinite(y)) y = -0.1f * r / rn;
+        }
+        dl[0] = a.omega * (t[0] * y);
+        dl[1] = a.omega * (t[1] * y);
+    } else {
+        newton<D>(g, h, dl);
+    }
     float alpha = 1.0f, cur[D];
     int ntr = 10;  // trials evaluated (10 = all rejected)
 #pragma unroll
@@ -307,6 +367,10 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
         float tr[D];
 #pragma unroll
         for (int k = 0; k < D; ++k) tr[k] = pos[k] + alpha * dl[k];
+        if constexpr (ENT) {
+            const float raw[2] = {tr[0], tr[1]};
+            ent_project(etag, erec, raw, tr);
+        }
         float et = 0.0f, mt = kInf;
         for (int r = 0; r < R; ++r) {
             Occ o = o0;
@@ -334,13 +398,22 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
         }
         alpha *= 0.5f;
     }
-    if (a.omega != 1.0f)
+    if (!ENT && a.omega != 1.0f)  // constrained DOFs take omega on the step instead
 #pragma unroll
         for (int k = 0; k < D; ++k) cur[k] = pos[k] + a.omega * (cur[k] - pos[k]);
     if (l == 0) {
         store_node<D>(a, dof, cur);
         count_update(a, ntr);
     }
+}
+
+// The constrained boundary DOFs of a plain sweep (2D), in a launch of their
+// own beside the sweep kernel: both read xin and write disjoint DOF slots.
+template <int KIND, int G, bool WT>
+__global__ __launch_bounds__(256) void constrained_kernel(const SweepArgs a)
+{
+    const int gid = int(blockIdx.x) * (256 / G) + int(threadIdx.x) / G;
+    stored_group<2, KIND, false, false, G, WT, true>(a, a.st.n + gid, int(threadIdx.x) % G, gid < a.con_n);
 }
 
 // ------------------------------------------------------------------ tiles

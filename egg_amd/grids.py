@@ -244,6 +244,35 @@ def ogrid_boundary_layer_w_inv(grid: Grid, n: int, *, r: float = 0.5, first_rati
                                 growth=growth, tangential_spacing=s_t, n_layers=n_layers)
 
 
+def annulus_2d(n: int = 65, m: int = 33, *, r: float = 0.5, half: float = 2.0, seed: int = 0,
+               amp: float = 0.2) -> tuple[Grid, dict]:
+    """The four ring blocks of ``ogrid_2d(n, m)`` alone: the circle becomes a
+    wall whose nodes slide on it (returned as ``{node: ("circle", (0, 0, r))}``
+    for the wire's ``entities``), the outer square stays fixed. The circle
+    nodes start rotated by up to ``amp`` of their angular spacing and the
+    ring-interior nodes moved by up to ``amp`` of the local spacing (seeded),
+    so the sweeps have both to even out."""
+    g = ogrid_2d(n, m, r=r, half=half)
+    used = np.unique(np.concatenate([np.asarray(b).reshape(-1) for b in g.blocks[1:]]))
+    relabel = np.full(g.n_nodes, -1, np.int64)
+    relabel[used] = np.arange(used.size)
+    blocks = [relabel[np.asarray(b)].astype(np.int32) for b in g.blocks[1:]]
+    X = g.X[used].copy()
+    rad = np.linalg.norm(X, axis=1)
+    on_circle = np.isclose(rad, r)
+    fixed = np.isclose(np.abs(X[:, 0]), half) | np.isclose(np.abs(X[:, 1]), half)
+    free = ~fixed
+    rng = np.random.default_rng(seed)
+    dth = 0.5 * np.pi / (n - 1)
+    th = np.arctan2(X[on_circle, 1], X[on_circle, 0]) + amp * dth * rng.uniform(-1, 1, int(on_circle.sum()))
+    X[on_circle] = r * np.stack([np.cos(th), np.sin(th)], -1)
+    inner = free & ~on_circle
+    h = local_edge_length(Grid(2, blocks, g.X[used], free))
+    X[inner] += amp * h[inner, None] * rng.uniform(-1, 1, (int(inner.sum()), 2))
+    ents = {int(i): ("circle", (0.0, 0.0, float(r))) for i in np.flatnonzero(on_circle)}
+    return Grid(2, blocks, X, free), ents
+
+
 def local_edge_length(grid: Grid) -> np.ndarray:
     """Shortest incident block edge per node."""
     h = np.full(grid.n_nodes, np.inf)

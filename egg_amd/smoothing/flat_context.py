@@ -64,8 +64,39 @@ def cell_stencil(blocks: list[np.ndarray], d: int) -> dict:
             "nc": nc, "ns": gc.shape[0], "ncell": ncell}
 
 
+# Entity types a constrained boundary DOF can slide on here, in the wire's
+# per-type SoA form (reference egg/geometry/entity_soa.py:109-155: tag,
+# record fields): segment (sx, sy, ex, ey) and circle (cx, cy, r), 2D.
+ENTITY_TYPES = {"lineseg": (1, 4), "circle": (2, 3)}
+
+
+def _entities_wire(dofs: np.ndarray, entities: dict | None) -> dict:
+    """The group's ``entities`` dict (reference egg/geometry/entity_soa.py:
+    598-680 group_entities_by_type): one sub-dict per present type with the
+    group-local DOF indices and packed records; free DOFs under "free"."""
+    if not entities:
+        return {}
+    parts: dict[str, dict] = {}
+    for local, node in enumerate(dofs.tolist()):
+        kind, rec = entities.get(int(node), ("free", ()))
+        if kind == "free":
+            tag, kf = 0, 0
+        elif kind in ENTITY_TYPES:
+            tag, kf = ENTITY_TYPES[kind]
+        else:
+            raise ValueError(f"entity type {kind!r} is not supported (lineseg, circle)")
+        if len(rec) != kf:
+            raise ValueError(f"a {kind} record holds {kf} values, got {len(rec)}")
+        p = parts.setdefault(kind, {"tag": tag, "kFields": kf, "dof_local": [], "records": []})
+        p["dof_local"].append(local)
+        p["records"].extend(float(v) for v in rec)
+    return {k: {"tag": v["tag"], "kFields": v["kFields"], "dof_local": np.asarray(v["dof_local"], np.int32),
+                "records": np.asarray(v["records"], np.float64)} for k, v in parts.items()}
+
+
 def build_flat_context(blocks: list[np.ndarray], free_mask: np.ndarray, d: int, *,
-                       w_inv: np.ndarray | None = None, weight: np.ndarray | None = None) -> dict:
+                       w_inv: np.ndarray | None = None, weight: np.ndarray | None = None,
+                       entities: dict | None = None) -> dict:
     """Assemble the flat wire for the moving DOFs ``free_mask`` of ``blocks``.
 
     ``w_inv`` is one uniform ``(d, d)`` inverse target (identity when
@@ -73,7 +104,10 @@ def build_flat_context(blocks: list[np.ndarray], free_mask: np.ndarray, d: int, 
     optional ``(ns,)`` per-sample energy weight (it scales a sample's mu,
     gradient and Hessian, not its det); as in the reference
     (egg/smoothing/flat_context.py:186-218) a weighted wire ships every DOF's
-    real stencil, so interior synthesis is off.
+    real stencil, so interior synthesis is off. ``entities`` maps a moving
+    node id to the entity it slides on, ``("lineseg", (sx, sy, ex, ey))`` or
+    ``("circle", (cx, cy, r))`` (2D): such a DOF ships its stored stencil and
+    the group's ``entities`` field carries the per-type records.
     """
     st = cell_stencil(blocks, d)
     ns = st["ns"]
@@ -100,6 +134,10 @@ def build_flat_context(blocks: list[np.ndarray], free_mask: np.ndarray, d: int, 
             grids = np.meshgrid(*[np.arange(1, s - 1, dtype=np.int32) for s in ids.shape], indexing="ij")
             node_block[inner] = b
             node_logical[inner] = np.stack([g.reshape(-1) for g in grids], axis=1)
+    if entities:
+        if d != 2:
+            raise ValueError("constrained boundary DOFs are 2D only here")
+        node_block[np.asarray([k for k, v in entities.items() if v[0] != "free"], np.int64)] = -1
     interior = node_block >= 0
 
     m_node = st["m_node"]
@@ -133,7 +171,7 @@ def build_flat_context(blocks: list[np.ndarray], free_mask: np.ndarray, d: int, 
             "P_of": P_of,
             "interior_block": np.ascontiguousarray(node_block[dofs]),
             "interior_logical": np.ascontiguousarray(node_logical[dofs]),
-            "entities": {},
+            "entities": _entities_wire(dofs, entities),
         }
         g.update(fields(m_sid))
         groups.append(g)

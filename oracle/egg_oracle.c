@@ -473,6 +473,94 @@ void FN(o_energy_mindet)(int dim, int kind, double delta, long long ns, const in
     *md = m;
 }
 
+/* ---------------------------------------------- constrained boundary DOFs */
+
+/* 2D closed-form entities a boundary DOF slides on (reference
+ * src/geometry.hpp:288-320 LineSeg, :398-431 Circle): tag 1 = segment
+ * (sx, sy, ex, ey), tag 2 = circle (cx, cy, r). */
+#define ENT_SEGMENT 1
+#define ENT_CIRCLE 2
+#define TOL_TINY ((real)(IS_F32 ? 1e-20 : 1e-30))
+
+static void FN(ent_project)(int tag, const real* e, const real* p, real* out)
+{
+    if (tag == ENT_SEGMENT) {
+        const real abx = e[2] - e[0], aby = e[3] - e[1];
+        const real ab2 = abx * abx + aby * aby;
+        real t = ((p[0] - e[0]) * abx + (p[1] - e[1]) * aby) / (ab2 > TOL_TINY ? ab2 : TOL_TINY);
+        t = t < 0 ? 0 : (t > 1 ? 1 : t);
+        out[0] = e[0] + t * abx;
+        out[1] = e[1] + t * aby;
+    } else {
+        const real dx = p[0] - e[0], dy = p[1] - e[1];
+        const real dist = sqrt(dx * dx + dy * dy);
+        if (dist < TOL_ZNORM) {
+            out[0] = e[0] + e[2];
+            out[1] = e[1];
+        } else {
+            out[0] = e[0] + e[2] * dx / dist;
+            out[1] = e[1] + e[2] * dy / dist;
+        }
+    }
+}
+
+static void FN(ent_tangent)(int tag, const real* e, const real* p, real* t)
+{
+    if (tag == ENT_SEGMENT) {
+        const real abx = e[2] - e[0], aby = e[3] - e[1];
+        const real n = sqrt(abx * abx + aby * aby);
+        t[0] = n < TOL_ZNORM ? 1 : abx / n;
+        t[1] = n < TOL_ZNORM ? 0 : aby / n;
+    } else {
+        const real dx = p[0] - e[0], dy = p[1] - e[1];
+        const real rn = sqrt(dx * dx + dy * dy);
+        const real nx = rn < TOL_ZNORM ? 1 : dx / rn, ny = rn < TOL_ZNORM ? 0 : dy / rn;
+        t[0] = -ny;
+        t[1] = nx;
+    }
+}
+
+/* One constrained DOF's update (reference src/sweep.hpp:1285-1403 and
+ * :1848-1871): tangent-reduced Newton step delta = t * y with
+ * y = solve1x1(t.H.t, t.g) (src/solve.hpp:83-90, src/patch.hpp:348-388),
+ * damped by omega on the step (no post-blend); trials pos + alpha delta are
+ * projected onto the entity; the same accept rule as the free DOFs. */
+static void FN(update_constrained)(int kind, real delta, real omega, const FN(patch_t)* pv, const real* X,
+                                   int dof, int tag, const real* ent, real* out)
+{
+    real grad[3], hess[9], e0, md;
+    FN(patch_eval_impl)(2, kind, delta, pv, X, grad, hess, &e0, &md);
+    const real pos[2] = {X[(size_t)dof * 2], X[(size_t)dof * 2 + 1]};
+    real t[2];
+    FN(ent_tangent)(tag, ent, pos, t);
+    const real Ht0 = hess[0] * t[0] + hess[1] * t[1], Ht1 = hess[2] * t[0] + hess[3] * t[1];
+    const real M = t[0] * Ht0 + t[1] * Ht1;
+    const real r = t[0] * grad[0] + t[1] * grad[1];
+    const real rn = fabs(r);
+    real y = 0;
+    if (rn >= TOL_ZNORM) {
+        y = -r / M;
+        if (!isfinite(y)) y = (real)-0.1 * r / rn;
+    }
+    const real dl[2] = {omega * (t[0] * y), omega * (t[1] * y)};
+    real alpha = 1, cur[2] = {pos[0], pos[1]};
+    for (int it = 0; it < MAX_BACKTRACKS; ++it) {
+        const real raw[2] = {pos[0] + alpha * dl[0], pos[1] + alpha * dl[1]};
+        real trial[2], e_new, mdet;
+        FN(ent_project)(tag, ent, raw, trial);
+        FN(trial_impl)(2, kind, delta, pv, X, dof, trial, &e_new, &mdet);
+        const int ok = isfinite(e_new) && (e_new <= e0 + TOL_ENERGY) && (kind == KIND_UNTANGLE || mdet > 0);
+        if (ok) {
+            cur[0] = trial[0];
+            cur[1] = trial[1];
+            break;
+        }
+        alpha *= (real)0.5;
+    }
+    out[0] = cur[0];
+    out[1] = cur[1];
+}
+
 /* ------------------------------------------------------------ Jacobi sweep */
 
 /* One DOF's block-Jacobi update: metric pass, Newton, <=10 halvings, omega
@@ -523,9 +611,10 @@ static void FN(update_dof)(int dim, int kind, real delta, real omega, const FN(p
  */
 int FN(o_jacobi)(int dim, int kind, double delta, double omega, int n_sweeps, int report_every, int ndof,
                  const int* dof, const long long* poff, const int* pgc, const int* pgn, const real* ps,
-                 const int* prole, const real* W, int w_stride, const real* pwt, int n_nodes, real* X,
-                 const real* tau, long long ns, const int* egc, const int* egn, const real* es, const real* eW,
-                 int ew_stride, const real* ewt, real* work, double* energies, double* mindets)
+                 const int* prole, const real* W, int w_stride, const real* pwt, const int* etag,
+                 const real* erec, int n_nodes, real* X, const real* tau, long long ns, const int* egc,
+                 const int* egn, const real* es, const real* eW, int ew_stride, const real* ewt, real* work,
+                 double* energies, double* mindets)
 {
     const long long total = poff[ndof];
     const int k = report_every <= 0 ? n_sweeps : report_every;
@@ -548,8 +637,12 @@ int FN(o_jacobi)(int dim, int kind, double delta, double omega, int n_sweeps, in
                 }
             FN(patch_t) pv = {P, pgc + o, gn_l, s_l, prole + o, W + (size_t)w_stride * o, w_stride,
                               pwt ? pwt + o : NULL};
-            FN(update_dof)(dim, kind, (real)delta, (real)omega, &pv, xin, dof[d], tau,
-                       xout + (size_t)dof[d] * dim);
+            if (etag && etag[d])  /* constrained boundary DOF (2D) */
+                FN(update_constrained)(kind, (real)delta, (real)omega, &pv, xin, dof[d], etag[d], erec + 4 * d,
+                                       xout + (size_t)dof[d] * dim);
+            else
+                FN(update_dof)(dim, kind, (real)delta, (real)omega, &pv, xin, dof[d], tau,
+                               xout + (size_t)dof[d] * dim);
         }
         if ((sw + 1) % k == 0 || sw == n_sweeps - 1) {
             FN(o_energy_mindet)(dim, kind, delta, ns, egc, egn, es, eW, ew_stride, ewt, xout, &energies[nrep],

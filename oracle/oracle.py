@@ -67,7 +67,8 @@ def lib():
             f = getattr(L, "o_jacobi" + sfx)
             f.argtypes = [
                 C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
-                _ip, _lp, _ip, _ip, rp, _ip, rp, C.c_int, C.c_void_p, C.c_int, rp, C.c_void_p,
+                _ip, _lp, _ip, _ip, rp, _ip, rp, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, rp,
+                C.c_void_p,
                 C.c_longlong, _ip, _ip, rp, rp, C.c_int, C.c_void_p, rp, _dp, _dp,
             ]
             f.restype = C.c_int
@@ -191,7 +192,7 @@ def interior_occurrences(dim: int, logical, idx):
 class Patches:
     """Per-DOF patches in global id space (CSR over DOFs)."""
 
-    def __init__(self, dim, dof, poff, gc, gn, s, role, W, wt=None):
+    def __init__(self, dim, dof, poff, gc, gn, s, role, W, wt=None, etag=None, erec=None):
         self.dim = dim
         self.dof = np.ascontiguousarray(dof, np.int32)
         self.poff = np.ascontiguousarray(poff, np.int64)
@@ -201,6 +202,9 @@ class Patches:
         self.role = np.ascontiguousarray(role, np.int32)
         self.W = np.ascontiguousarray(W, np.float64)  # (dd,) uniform or (total, dd)
         self.wt = None if wt is None else np.ascontiguousarray(wt, np.float64)  # (total,) or None = unit
+        # per-DOF entity a constrained boundary DOF slides on (0 free, 1 segment, 2 circle) and its record
+        self.etag = None if etag is None else np.ascontiguousarray(etag, np.int32)
+        self.erec = None if erec is None else np.ascontiguousarray(erec, np.float64).reshape(-1, 4)
 
 
 def materialize_patches(wire: dict, block_global_dof: list[np.ndarray], dim: int, dofs=None) -> Patches:
@@ -277,7 +281,28 @@ def materialize_patches(wire: dict, block_global_dof: list[np.ndarray], dim: int
     if wire_wt is not None:
         ww = np.asarray(wire_wt, np.float64).reshape(-1)
         wt = np.concatenate([ww[soff[d]:soff[d] + P_of[d]] for d in range(n)]) if n else ww[:0]
-    return Patches(dim, dof, poff, gc, gn, sg, role, Wp, wt)
+    etag, erec = _entity_rows(g, len(P_all))
+    if etag is not None:
+        etag, erec = etag[sel], erec[sel]
+        if np.any(etag[interior] != 0):
+            raise ValueError("a synthesized interior DOF cannot be constrained")
+    return Patches(dim, dof, poff, gc, gn, sg, role, Wp, wt, etag, erec)
+
+
+def _entity_rows(g: dict, ndof: int):
+    """Per group-local DOF entity tag (0 free) and record (4 values) from
+    the wire's per-type ``entities`` dicts, or (None, None) when none is
+    constrained."""
+    ents = g.get("entities") or {}
+    tag = np.zeros(ndof, np.int32)
+    rec = np.zeros((ndof, 4))
+    for part in ents.values():
+        loc = np.asarray(part["dof_local"], np.int64)
+        kf = int(part["kFields"])
+        tag[loc] = int(part["tag"])
+        if kf:
+            rec[loc, :kf] = np.asarray(part["records"], np.float64).reshape(-1, kf)
+    return (tag, rec) if np.any(tag) else (None, None)
 
 
 def materialize_patches_loop(wire: dict, block_global_dof: list[np.ndarray], dim: int) -> Patches:
@@ -338,12 +363,14 @@ def jacobi(dim, kind, delta, omega, n_sweeps, X, patches: Patches, es: dict, *, 
     W = np.ascontiguousarray(patches.W, dt).reshape(-1)
     ws = 0 if W.size == dim * dim else dim * dim
     pwt = _weights(patches.wt, dt)
+    etag = patches.etag
+    erec = None if patches.erec is None else np.ascontiguousarray(patches.erec, dt).reshape(-1)
     tau_arr = None if tau is None else np.ascontiguousarray(tau, dt).reshape(-1)
     got = getattr(lib(), f"o_jacobi_{prec}")(
         dim, KIND[kind], float(delta), float(omega), int(n_sweeps), int(report_every),
         len(patches.dof), patches.dof, patches.poff, patches.gc, patches.gn.reshape(-1),
-        np.ascontiguousarray(patches.s, dt).reshape(-1), patches.role, W, ws, _ptr(pwt), n_nodes, Xw,
-        _ptr(tau_arr), ns, egc, egn, es_, eW, ews, _ptr(ewt), work, en, md,
+        np.ascontiguousarray(patches.s, dt).reshape(-1), patches.role, W, ws, _ptr(pwt), _ptr(etag), _ptr(erec),
+        n_nodes, Xw, _ptr(tau_arr), ns, egc, egn, es_, eW, ews, _ptr(ewt), work, en, md,
     )
     return Xw.reshape(-1, dim), en[:got], md[:got]
 

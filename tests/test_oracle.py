@@ -334,3 +334,63 @@ def test_weights_on_the_wire(oracle):
     np.testing.assert_array_equal(runs[1][0], runs[0][0])
     np.testing.assert_allclose(runs[1][1], 2.0 * runs[0][1], rtol=1e-14)
     np.testing.assert_array_equal(runs[1][2], runs[0][2])
+
+
+def test_constrained_dof_reaches_the_symmetric_point(oracle):
+    """A 3 x 3 block whose bottom-middle node slides on the bottom edge
+    (segment entity) and everything else fixed: by symmetry the energy is
+    least at the middle of the edge, which the constrained sweeps reach, on
+    the segment exactly."""
+    from egg_amd import grids
+    from tests.cases import Case
+
+    g = grids.single_block_2d(3, seed=0, amp=0.0)
+    ids = np.asarray(g.blocks[0])
+    n = int(ids[1, 0])
+    X = g.X.copy()
+    X[n] = (0.3, 0.0)
+    free = np.zeros(g.n_nodes, bool)
+    free[n] = True
+    c = Case(grids.Grid(2, g.blocks, X, free), entities={n: ("lineseg", (0.0, 0.0, 1.0, 0.0))})
+    P = c.patches(oracle)
+    assert P.etag is not None and int(P.etag[0]) == 1
+    Xo, e, m = oracle.jacobi(2, "barrier", 0.0, 0.8, 60, c.X, P, c.ctx.energy_stencil, report_every=10)
+    assert Xo[n, 1] == 0.0 and Xo[n, 0] == pytest.approx(0.5, abs=1e-10)
+    assert np.all(np.diff(e) <= 1e-15)
+
+
+def test_constrained_step_matches_finite_differences(oracle):
+    """On the annulus (circle wall), one undamped sweep moves a circle DOF to
+    the projection of pos + y t with y = -E'(0) / E''(0) along the tangent t
+    (E = its patch energy, derivatives by central differences): the
+    tangent-reduced Newton step."""
+    from egg_amd import grids
+    from tests.cases import Case
+
+    g, ents = grids.annulus_2d(17, 9, seed=3)
+    c = Case(g, entities=ents)
+    P = c.patches(oracle)
+    X1, _, _ = oracle.jacobi(2, "barrier", 0.0, 1.0, 1, c.X, P, c.ctx.energy_stencil)
+    checked = 0
+    for d in np.flatnonzero(P.etag == 2)[::5]:
+        a, b = P.poff[d], P.poff[d + 1]
+        dof = int(P.dof[d])
+        args = (P.gc[a:b], P.gn[:, a:b], P.s[:, a:b], P.role[a:b], P.W)
+        pos = c.X[dof]
+        t = np.array([-pos[1], pos[0]]) / np.linalg.norm(pos)
+
+        def energy(sv):
+            Xs = c.X.copy()
+            Xs[dof] = pos + sv * t
+            return oracle.patch_eval(2, "shape", 0.0, *args, Xs)[2]
+
+        h = 1e-4
+        e0, ep, em = energy(0.0), energy(h), energy(-h)
+        y = -((ep - em) / (2 * h)) / ((ep - 2 * e0 + em) / h**2)
+        raw = pos + y * t
+        want = 0.5 * raw / np.linalg.norm(raw)
+        if np.abs(X1[dof] - pos).max() > 0:  # accepted at the full step
+            # central differences with h = 1e-4 on ~0.05 cells: ~1e-5 relative truncation
+            np.testing.assert_allclose(X1[dof], want, rtol=0, atol=1e-4 * np.abs(y))
+            checked += 1
+    assert checked >= 3
