@@ -164,10 +164,12 @@ struct Wire {
                     if (!ed.contains("tag"))
                         throw py::value_error("unpack_context: entities carries a group without a tag");
                     const int t = ed["tag"].cast<int>(), kf = ed["kFields"].cast<int>();
-                    if (!((t == 0 && kf == 0) || (t == 1 && kf == 4) || (t == 2 && kf == 3)))
+                    if (!((t == 0 && kf == 0) || (t == 1 && kf == 4) || (t == 2 && kf == 3) ||
+                          (t == 3 && kf == 4) || (t == 15 && kf == 4)))
                         throw py::value_error("wire: entity tag " + std::to_string(t) + " (" +
                                               item.first.cast<std::string>() +
-                                              ") is not supported by this compute core (free, lineseg, circle)");
+                                              ") is not supported by this compute core (free, lineseg, circle, "
+                                              "ellipse, linerail)");
                     IArr loc = get_i(ed, "dof_local");
                     std::vector<double> r;
                     if (kf) {

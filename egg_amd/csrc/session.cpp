@@ -289,9 +289,9 @@ void validate(int dim, const eggmi_group* groups, int n_groups, const eggmi_sten
         if (g.entity_tag)
             for (long long d = 0; d < g.ndof; ++d) {
                 const int t = g.entity_tag[d];
-                if (t < 0 || t > 2)
+                if (t < 0 || (t > 3 && t != 15))
                     throw InvalidArgument("validate_context: " + grp + " entity tag " + std::to_string(t) +
-                                          " is not supported (0 free, 1 segment, 2 circle)");
+                                          " is not supported (0 free, 1 segment, 2 circle, 3 ellipse, 15 rail)");
                 if (t != 0 && dim != 2)
                     throw InvalidArgument("validate_context: constrained boundary DOFs are 2D only here");
                 for (int k = 0; k < 4; ++k)

@@ -151,17 +151,29 @@ __device__ __forceinline__ void count_update(const SweepArgs& a, int ntr, bool p
 // Stored-stencil DOF d handled by a group of G lanes (lane l owns
 // occurrences l, l+G, ... of the DOF's patch).
 // 2D entities a constrained boundary DOF slides on (reference
-// src/geometry.hpp:288-320 LineSeg, :398-431 Circle; FP32 tolerances of
-// src/core.hpp: znorm 1e-7, tiny 1e-20): tag 1 segment (sx, sy, ex, ey),
-// tag 2 circle (cx, cy, r).
+// src/geometry.hpp:288-320 LineSeg, :322-357 LineRail, :398-431 Circle,
+// :432-464 Ellipse; FP32 tolerances of src/core.hpp: znorm 1e-7, tiny
+// 1e-20): tag 1 segment (sx, sy, ex, ey), 2 circle (cx, cy, r), 3 ellipse
+// (cx, cy, rx, ry), 15 rail (a segment that releases the node beyond its ends).
+__device__ __forceinline__ float rail_foot(const float* e, const float* p)
+{
+    const float abx = e[2] - e[0], aby = e[3] - e[1];
+    return ((p[0] - e[0]) * abx + (p[1] - e[1]) * aby) / fmaxf(abx * abx + aby * aby, 1e-20f);
+}
 __device__ __forceinline__ void ent_project(int tag, const float* e, const float* p, float* out)
 {
-    if (tag == 1) {
+    if (tag == 1 || tag == 15) {
         const float abx = e[2] - e[0], aby = e[3] - e[1];
-        float t = ((p[0] - e[0]) * abx + (p[1] - e[1]) * aby) / fmaxf(abx * abx + aby * aby, 1e-20f);
+        float t = rail_foot(e, p);
+        const bool released = tag == 15 && (t < 0.0f || t > 1.0f);
         t = fminf(fmaxf(t, 0.0f), 1.0f);
-        out[0] = e[0] + t * abx;
-        out[1] = e[1] + t * aby;
+        out[0] = released ? p[0] : e[0] + t * abx;
+        out[1] = released ? p[1] : e[1] + t * aby;
+    } else if (tag == 3) {
+        const float sx = (p[0] - e[0]) / e[2], sy = (p[1] - e[1]) / e[3];
+        const float dist = sqrtf(sx * sx + sy * sy);
+        out[0] = e[0] + (dist < 1e-7f ? 0.70710678f : sx / dist) * e[2];
+        out[1] = e[1] + (dist < 1e-7f ? 0.70710678f : sy / dist) * e[3];
     } else {
         const float dx = p[0] - e[0], dy = p[1] - e[1];
         const float dist = sqrtf(dx * dx + dy * dy);
@@ -171,7 +183,19 @@ __device__ __forceinline__ void ent_project(int tag, const float* e, const float
 }
 __device__ __forceinline__ void ent_tangent(int tag, const float* e, const float* p, float* t)
 {
-    if (tag == 1) {
+    if (tag == 3) {
+        // (-rx sin a, ry cos a) at the parametric angle a of the radially
+        // scaled point: sin a, cos a as its coordinates over its length
+        const float sx = (p[0] - e[0]) / e[2], sy = (p[1] - e[1]) / e[3];
+        const float dist = sqrtf(sx * sx + sy * sy);
+        const float sa = dist < 1e-7f ? 0.0f : sy / dist, ca = dist < 1e-7f ? 1.0f : sx / dist;
+        const float tx = -e[2] * sa, ty = e[3] * ca;
+        const float n = sqrtf(tx * tx + ty * ty);
+        t[0] = n < 1e-7f ? 1.0f : tx / n;
+        t[1] = n < 1e-7f ? 0.0f : ty / n;
+        return;
+    }
+    if (tag == 1 || tag == 15) {
         const float abx = e[2] - e[0], aby = e[3] - e[1];
         const float n = sqrtf(abx * abx + aby * aby);
         t[0] = n < 1e-7f ? 1.0f : abx / n;
@@ -343,19 +367,28 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
         etag = a.con_tag[ci];
 #pragma unroll
         for (int k = 0; k < 4; ++k) erec[k] = a.con_rec[4 * ci + k];
-        // y = solve1x1(t.H.t, t.g) (src/solve.hpp:83-90), delta = omega t y
-        float t[2];
-        ent_tangent(etag, erec, pos, t);
-        const float M = t[0] * (h[0] * t[0] + h[1] * t[1]) + t[1] * (h[1] * t[0] + h[2] * t[1]);
-        const float r = t[0] * g[0] + t[1] * g[1];
-        const float rn = fabsf(r);
-        float y = 0.0f;
-        if (rn >= 1e-7f) {
-            y = -r / M;
-            if (!isfinite(y)) y = -0.1f * r / rn;
+        const float ft = etag == 15 ? rail_foot(erec, pos) : 0.0f;
+        if (etag == 15 && (ft < 0.0f || ft > 1.0f)) {
+            // a rail beyond its ends releases the node: full-space step
+            // (src/patch.hpp:400-409), still damped on the step
+            newton<D>(g, h, dl);
+            dl[0] *= a.omega;
+            dl[1] *= a.omega;
+        } else {
+            // y = solve1x1(t.H.t, t.g) (src/solve.hpp:83-90), delta = omega t y
+            float t[2];
+            ent_tangent(etag, erec, pos, t);
+            const float M = t[0] * (h[0] * t[0] + h[1] * t[1]) + t[1] * (h[1] * t[0] + h[2] * t[1]);
+            const float r = t[0] * g[0] + t[1] * g[1];
+            const float rn = fabsf(r);
+            float y = 0.0f;
+            if (rn >= 1e-7f) {
+                y = -r / M;
+                if (!isfinite(y)) y = -0.1f * r / rn;
+            }
+            dl[0] = a.omega * (t[0] * y);
+            dl[1] = a.omega * (t[1] * y);
         }
-        dl[0] = a.omega * (t[0] * y);
-        dl[1] = a.omega * (t[1] * y);
     } else {
         newton<D>(g, h, dl);
     }

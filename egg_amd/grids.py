@@ -245,13 +245,14 @@ def ogrid_boundary_layer_w_inv(grid: Grid, n: int, *, r: float = 0.5, first_rati
 
 
 def annulus_2d(n: int = 65, m: int = 33, *, r: float = 0.5, half: float = 2.0, seed: int = 0,
-               amp: float = 0.2) -> tuple[Grid, dict]:
+               amp: float = 0.2, aspect: float = 1.0) -> tuple[Grid, dict]:
     """The four ring blocks of ``ogrid_2d(n, m)`` alone: the circle becomes a
     wall whose nodes slide on it (returned as ``{node: ("circle", (0, 0, r))}``
     for the wire's ``entities``), the outer square stays fixed. The circle
     nodes start rotated by up to ``amp`` of their angular spacing and the
     ring-interior nodes moved by up to ``amp`` of the local spacing (seeded),
-    so the sweeps have both to even out."""
+    so the sweeps have both to even out. ``aspect`` != 1 scales every y
+    coordinate by it: the wall is then the ellipse of radii (r, aspect r)."""
     g = ogrid_2d(n, m, r=r, half=half)
     used = np.unique(np.concatenate([np.asarray(b).reshape(-1) for b in g.blocks[1:]]))
     relabel = np.full(g.n_nodes, -1, np.int64)
@@ -269,7 +270,11 @@ def annulus_2d(n: int = 65, m: int = 33, *, r: float = 0.5, half: float = 2.0, s
     inner = free & ~on_circle
     h = local_edge_length(Grid(2, blocks, g.X[used], free))
     X[inner] += amp * h[inner, None] * rng.uniform(-1, 1, (int(inner.sum()), 2))
-    ents = {int(i): ("circle", (0.0, 0.0, float(r))) for i in np.flatnonzero(on_circle)}
+    if aspect != 1.0:
+        X[:, 1] *= aspect
+        ents = {int(i): ("ellipse", (0.0, 0.0, float(r), float(aspect * r))) for i in np.flatnonzero(on_circle)}
+    else:
+        ents = {int(i): ("circle", (0.0, 0.0, float(r))) for i in np.flatnonzero(on_circle)}
     return Grid(2, blocks, X, free), ents
 
 

@@ -66,8 +66,9 @@ def cell_stencil(blocks: list[np.ndarray], d: int) -> dict:
 
 # Entity types a constrained boundary DOF can slide on here, in the wire's
 # per-type SoA form (reference egg/geometry/entity_soa.py:109-155: tag,
-# record fields): segment (sx, sy, ex, ey) and circle (cx, cy, r), 2D.
-ENTITY_TYPES = {"lineseg": (1, 4), "circle": (2, 3)}
+# record fields), 2D: segment (sx, sy, ex, ey), circle (cx, cy, r), ellipse
+# (cx, cy, rx, ry), rail (sx, sy, ex, ey; releases the node beyond its ends).
+ENTITY_TYPES = {"lineseg": (1, 4), "circle": (2, 3), "ellipse": (3, 4), "linerail": (15, 4)}
 
 
 def _entities_wire(dofs: np.ndarray, entities: dict | None) -> dict:
@@ -84,7 +85,7 @@ def _entities_wire(dofs: np.ndarray, entities: dict | None) -> dict:
         elif kind in ENTITY_TYPES:
             tag, kf = ENTITY_TYPES[kind]
         else:
-            raise ValueError(f"entity type {kind!r} is not supported (lineseg, circle)")
+            raise ValueError(f"entity type {kind!r} is not supported ({', '.join(ENTITY_TYPES)})")
         if len(rec) != kf:
             raise ValueError(f"a {kind} record holds {kf} values, got {len(rec)}")
         p = parts.setdefault(kind, {"tag": tag, "kFields": kf, "dof_local": [], "records": []})

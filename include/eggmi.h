@@ -49,10 +49,11 @@ typedef struct {
                                          gradient and Hessian, not det), NULL = 1
                                          (src/patch.hpp:45-77, :250-292) */
     const int32_t* entity_tag;        /* [ndof] entity a boundary DOF slides on (2D): 0 free,
-                                         1 segment, 2 circle; NULL = all free (the group's
-                                         "entities", src/bindings.cpp:412-446) */
-    const double* entity_rec;         /* [ndof*4] its record: segment (sx, sy, ex, ey), circle
-                                         (cx, cy, r, -) (src/geometry.hpp:288-431) */
+                                         1 segment, 2 circle, 3 ellipse, 15 rail; NULL = all free
+                                         (the group's "entities", src/bindings.cpp:412-446) */
+    const double* entity_rec;         /* [ndof*4] its record: segment / rail (sx, sy, ex, ey),
+                                         circle (cx, cy, r, -), ellipse (cx, cy, rx, ry)
+                                         (src/geometry.hpp:288-464) */
 } eggmi_group;
 
 /* The energy stencil (every (cell, corner) sample). */

@@ -476,21 +476,43 @@ void FN(o_energy_mindet)(int dim, int kind, double delta, long long ns, const in
 /* ---------------------------------------------- constrained boundary DOFs */
 
 /* 2D closed-form entities a boundary DOF slides on (reference
- * src/geometry.hpp:288-320 LineSeg, :398-431 Circle): tag 1 = segment
- * (sx, sy, ex, ey), tag 2 = circle (cx, cy, r). */
+ * src/geometry.hpp:288-320 LineSeg, :322-357 LineRail, :398-431 Circle,
+ * :432-464 Ellipse): tag 1 = segment (sx, sy, ex, ey), 2 = circle (cx, cy,
+ * r), 3 = ellipse (cx, cy, rx, ry), 15 = rail (sx, sy, ex, ey: a segment
+ * that releases the node beyond its ends). */
 #define ENT_SEGMENT 1
 #define ENT_CIRCLE 2
+#define ENT_ELLIPSE 3
+#define ENT_RAIL 15
 #define TOL_TINY ((real)(IS_F32 ? 1e-20 : 1e-30))
+
+static real FN(rail_foot)(const real* e, const real* p)
+{
+    const real abx = e[2] - e[0], aby = e[3] - e[1];
+    const real ab2 = abx * abx + aby * aby;
+    return ((p[0] - e[0]) * abx + (p[1] - e[1]) * aby) / (ab2 > TOL_TINY ? ab2 : TOL_TINY);
+}
 
 static void FN(ent_project)(int tag, const real* e, const real* p, real* out)
 {
-    if (tag == ENT_SEGMENT) {
+    if (tag == ENT_SEGMENT || tag == ENT_RAIL) {
         const real abx = e[2] - e[0], aby = e[3] - e[1];
-        const real ab2 = abx * abx + aby * aby;
-        real t = ((p[0] - e[0]) * abx + (p[1] - e[1]) * aby) / (ab2 > TOL_TINY ? ab2 : TOL_TINY);
+        real t = FN(rail_foot)(e, p);
+        if (tag == ENT_RAIL && (t < 0 || t > 1)) { /* beyond an end: released */
+            out[0] = p[0];
+            out[1] = p[1];
+            return;
+        }
         t = t < 0 ? 0 : (t > 1 ? 1 : t);
         out[0] = e[0] + t * abx;
         out[1] = e[1] + t * aby;
+    } else if (tag == ENT_ELLIPSE) {
+        const real sx = (p[0] - e[0]) / e[2], sy = (p[1] - e[1]) / e[3];
+        const real dist = sqrt(sx * sx + sy * sy);
+        const real ux = dist < TOL_ZNORM ? (real)0.70710678118654752440 : sx / dist;
+        const real uy = dist < TOL_ZNORM ? (real)0.70710678118654752440 : sy / dist;
+        out[0] = e[0] + ux * e[2];
+        out[1] = e[1] + uy * e[3];
     } else {
         const real dx = p[0] - e[0], dy = p[1] - e[1];
         const real dist = sqrt(dx * dx + dy * dy);
@@ -506,7 +528,16 @@ static void FN(ent_project)(int tag, const real* e, const real* p, real* out)
 
 static void FN(ent_tangent)(int tag, const real* e, const real* p, real* t)
 {
-    if (tag == ENT_SEGMENT) {
+    if (tag == ENT_ELLIPSE) {
+        /* parametric angle of the radially scaled point */
+        const real ang = atan2((p[1] - e[1]) / e[3], (p[0] - e[0]) / e[2]);
+        const real tx = -e[2] * sin(ang), ty = e[3] * cos(ang);
+        const real n = sqrt(tx * tx + ty * ty);
+        t[0] = n < TOL_ZNORM ? 1 : tx / n;
+        t[1] = n < TOL_ZNORM ? 0 : ty / n;
+        return;
+    }
+    if (tag == ENT_SEGMENT || tag == ENT_RAIL) {
         const real abx = e[2] - e[0], aby = e[3] - e[1];
         const real n = sqrt(abx * abx + aby * aby);
         t[0] = n < TOL_ZNORM ? 1 : abx / n;
@@ -531,18 +562,29 @@ static void FN(update_constrained)(int kind, real delta, real omega, const FN(pa
     real grad[3], hess[9], e0, md;
     FN(patch_eval_impl)(2, kind, delta, pv, X, grad, hess, &e0, &md);
     const real pos[2] = {X[(size_t)dof * 2], X[(size_t)dof * 2 + 1]};
-    real t[2];
-    FN(ent_tangent)(tag, ent, pos, t);
-    const real Ht0 = hess[0] * t[0] + hess[1] * t[1], Ht1 = hess[2] * t[0] + hess[3] * t[1];
-    const real M = t[0] * Ht0 + t[1] * Ht1;
-    const real r = t[0] * grad[0] + t[1] * grad[1];
-    const real rn = fabs(r);
-    real y = 0;
-    if (rn >= TOL_ZNORM) {
-        y = -r / M;
-        if (!isfinite(y)) y = (real)-0.1 * r / rn;
+    real dl[2];
+    const real ft = tag == ENT_RAIL ? FN(rail_foot)(ent, pos) : 0;
+    if (tag == ENT_RAIL && (ft < 0 || ft > 1)) {
+        /* a rail whose foot lies beyond an end releases the node: full-space
+           step (src/patch.hpp:400-409) */
+        FN(o_solve)(2, hess, grad, dl);
+        dl[0] *= omega;
+        dl[1] *= omega;
+    } else {
+        real t[2];
+        FN(ent_tangent)(tag, ent, pos, t);
+        const real Ht0 = hess[0] * t[0] + hess[1] * t[1], Ht1 = hess[2] * t[0] + hess[3] * t[1];
+        const real M = t[0] * Ht0 + t[1] * Ht1;
+        const real r = t[0] * grad[0] + t[1] * grad[1];
+        const real rn = fabs(r);
+        real y = 0;
+        if (rn >= TOL_ZNORM) {
+            y = -r / M;
+            if (!isfinite(y)) y = (real)-0.1 * r / rn;
+        }
+        dl[0] = omega * (t[0] * y);
+        dl[1] = omega * (t[1] * y);
     }
-    const real dl[2] = {omega * (t[0] * y), omega * (t[1] * y)};
     real alpha = 1, cur[2] = {pos[0], pos[1]};
     for (int it = 0; it < MAX_BACKTRACKS; ++it) {
         const real raw[2] = {pos[0] + alpha * dl[0], pos[1] + alpha * dl[1]};

@@ -1,8 +1,9 @@
-"""Constrained boundary DOFs on the MI355X: nodes that slide on a segment or
-a circle (the group's "entities"; reference src/sweep.hpp:1285-1403 and
-:1717-1876, src/geometry.hpp:288-431) against the FP64 oracle on the same
-wire: a tangent-reduced Newton step damped by omega, trials projected onto
-the entity, the free DOFs' accept rule.
+"""Constrained boundary DOFs on the MI355X: nodes that slide on a segment, a
+circle, an ellipse or a rail (the group's "entities"; reference
+src/sweep.hpp:1285-1403 and :1717-1876, src/geometry.hpp:288-464) against
+the FP64 oracle on the same wire: a tangent-reduced Newton step damped by
+omega, trials projected onto the entity (a rail releases its node beyond its
+ends: full-space step, no projection), the free DOFs' accept rule.
 
 Bar: positions within 1e-5 of the domain extent, energies within 2e-3 per
 report; constrained nodes on their entity to FP32 rounding (segments along
@@ -37,9 +38,10 @@ def _annulus(n=33, m=17, seed=1, **kw):
     return Case(g, entities=ents, **kw), np.array(sorted(ents))
 
 
-def _slab(n=17):
+def _slab(n=17, rail=False):
     """A single block whose bottom and top edges (corners excluded) slide on
-    their segments."""
+    their segments; `rail`: on rails instead, the bottom one covering only
+    x in [0.2, 0.8], so the nodes beyond it are released."""
     g = grids.single_block_2d(n, seed=4, amp=0.25)
     ids = np.asarray(g.blocks[0])
     free = g.free.copy()
@@ -51,14 +53,26 @@ def _slab(n=17):
             node = int(ids[i, j])
             free[node] = True
             X[node, 0] += 0.3 / (n - 1) * rng.uniform(-1, 1)
-            ents[node] = ("lineseg", (0.0, y, 1.0, y))
+            if rail:
+                ents[node] = ("linerail", (0.2, y, 0.8, y) if j == 0 else (0.0, y, 1.0, y))
+            else:
+                ents[node] = ("lineseg", (0.0, y, 1.0, y))
     return Case(grids.Grid(2, g.blocks, X, free), entities=ents), np.array(sorted(ents))
 
 
+def _make(name):
+    if name == "annulus":
+        return _annulus()
+    if name == "ellipse":
+        g, ents = grids.annulus_2d(33, 17, seed=1, aspect=0.6)
+        return Case(g, entities=ents), np.array(sorted(ents))
+    return _slab(rail=name == "rail")
+
+
 @pytest.mark.parametrize("variant", [0, 8, 12, 13])
-@pytest.mark.parametrize("make", ["annulus", "slab"])
+@pytest.mark.parametrize("make", ["annulus", "slab", "ellipse", "rail"])
 def test_constrained_parity(oracle, make, variant):
-    c, con = _annulus() if make == "annulus" else _slab()
+    c, con = _make(make)
     s = _session(c)
     s.core.set_tile_variant(variant)
     e, m = s.run(20, omega=0.8, report_every=5)
@@ -69,9 +83,15 @@ def test_constrained_parity(oracle, make, variant):
     np.testing.assert_allclose(e, eo, rtol=TRAJ_RTOL)
     if make == "annulus":
         np.testing.assert_allclose(np.linalg.norm(X[con], axis=1), 0.5, rtol=1e-6)
-    else:
+    elif make == "ellipse":
+        np.testing.assert_allclose(np.linalg.norm(X[con] / (0.5, 0.3), axis=1), 1.0, rtol=1e-6)
+    elif make == "slab":
         assert np.all(np.isin(X[con, 1], (0.0, 1.0)))
         assert np.all((X[con, 0] >= 0.0) & (X[con, 0] <= 1.0))
+    else:  # rails: on them where the foot is inside, released beyond
+        on = (X[con, 0] >= 0.2) & (X[con, 0] <= 0.8) | (X[con, 1] > 0.5)
+        assert np.all(np.isin(X[con[on], 1], (0.0, 1.0)))
+        assert np.any(~on)
     assert np.abs(X[con] - c.X[con]).max() > 1e-4  # they moved
 
 

@@ -359,25 +359,28 @@ def test_constrained_dof_reaches_the_symmetric_point(oracle):
     assert np.all(np.diff(e) <= 1e-15)
 
 
-def test_constrained_step_matches_finite_differences(oracle):
-    """On the annulus (circle wall), one undamped sweep moves a circle DOF to
-    the projection of pos + y t with y = -E'(0) / E''(0) along the tangent t
-    (E = its patch energy, derivatives by central differences): the
-    tangent-reduced Newton step."""
+@pytest.mark.parametrize("aspect", [1.0, 0.6])
+def test_constrained_step_matches_finite_differences(oracle, aspect):
+    """On the annulus (circle wall; ellipse for aspect 0.6), one undamped
+    sweep moves a wall DOF to the projection of pos + y t with
+    y = -E'(0) / E''(0) along the wall tangent t (E = its patch energy,
+    derivatives by central differences): the tangent-reduced Newton step."""
     from egg_amd import grids
     from tests.cases import Case
 
-    g, ents = grids.annulus_2d(17, 9, seed=3)
+    g, ents = grids.annulus_2d(17, 9, seed=3, aspect=aspect)
     c = Case(g, entities=ents)
     P = c.patches(oracle)
     X1, _, _ = oracle.jacobi(2, "barrier", 0.0, 1.0, 1, c.X, P, c.ctx.energy_stencil)
+    rx, ry = 0.5, 0.5 * aspect
     checked = 0
-    for d in np.flatnonzero(P.etag == 2)[::5]:
+    for d in np.flatnonzero(P.etag != 0)[::5]:
         a, b = P.poff[d], P.poff[d + 1]
         dof = int(P.dof[d])
         args = (P.gc[a:b], P.gn[:, a:b], P.s[:, a:b], P.role[a:b], P.W)
         pos = c.X[dof]
-        t = np.array([-pos[1], pos[0]]) / np.linalg.norm(pos)
+        t = np.array([-rx * pos[1] / ry, ry * pos[0] / rx])  # tangent of the ellipse (x/rx)^2 + (y/ry)^2 = 1
+        t /= np.linalg.norm(t)
 
         def energy(sv):
             Xs = c.X.copy()
@@ -388,7 +391,8 @@ def test_constrained_step_matches_finite_differences(oracle):
         e0, ep, em = energy(0.0), energy(h), energy(-h)
         y = -((ep - em) / (2 * h)) / ((ep - 2 * e0 + em) / h**2)
         raw = pos + y * t
-        want = 0.5 * raw / np.linalg.norm(raw)
+        sc = np.linalg.norm(raw / (rx, ry))
+        want = raw / sc  # radial-scaling projection (exact on the circle)
         if np.abs(X1[dof] - pos).max() > 0:  # accepted at the full step
             # central differences with h = 1e-4 on ~0.05 cells: ~1e-5 relative truncation
             np.testing.assert_allclose(X1[dof], want, rtol=0, atol=1e-4 * np.abs(y))
