@@ -62,12 +62,14 @@ PMC_KIB = {
     ("ogrid", 3): (1072.38, 538.5), ("ogrid", 4): (1856.91, 617.75), ("ogrid", 5): (2058.19, 784.62),
     ("ogrid", 6): (1214.56, 517.69), ("ogrid", 7): (1306.81, 548.44), ("ogrid", 8): (1089.25, 549.5),
     ("ogrid", 9): (773.69, 502.5), ("ogrid", 10): (1085.0, 549.5), ("ogrid", 11): (918.38, 505.38),
+    ("ogrid", 12): (1309.0, 548.06), ("ogrid", 13): (1310.0, 537.75),
     ("batched", 0): (590518.03, 277104.75), ("batched", 1): (761484.0, 325449.72),
     ("batched", 2): (526222.62, 252865.31), ("batched", 3): (508322.66, 276764.81),
     ("batched", 4): (1073595.0, 325607.78), ("batched", 5): (1284491.03, 422312.42),
     ("batched", 6): (525395.94, 252865.33), ("batched", 7): (589010.25, 277104.77),
     ("batched", 8): (509931.56, 277069.66), ("batched", 9): (376742.16, 252706.42),
     ("batched", 10): (508547.75, 277069.69), ("batched", 11): (435552.28, 252758.38),
+    ("batched", 12): (588511.62, 276649.7), ("batched", 13): (588924.25, 276800.11),
     ("cubed_sphere", 0): (135327.31, 49889.66), ("cubed_sphere", 1): (185090.25, 69130.72),
     ("cubed_sphere", 2): (133390.06, 49889.95), ("cubed_sphere", 3): (140850.5, 51144.95),
     ("cubed_sphere", 4): (132029.03, 49050.84),
