@@ -191,9 +191,20 @@ struct Wire {
                 eg.entity_tag = keep(IArr((py::ssize_t)tag.size(), tag.data()));
                 eg.entity_rec = keep(DArr((py::ssize_t)rec.size(), rec.data()));
             }
-            if (g.contains("curv_nodes") || g.contains("dir_nodes"))
-                throw py::value_error("wire: curvature / directional terms are not supported by this compute "
-                                      "core");
+            if (g.contains("curv_k") && !g["curv_k"].is_none() && g["curv_k"].cast<int>() > 0) {
+                const int K = g["curv_k"].cast<int>();
+                IArr cn = get_i(g, "curv_nodes"), cs = get_i(g, "curv_slot");
+                DArr cw = get_d(g, "curv_weight");
+                if ((int64_t)cn.size() != eg.ndof * K * 4 || (int64_t)cs.size() != eg.ndof * K ||
+                    (int64_t)cw.size() != eg.ndof * K)
+                    throw py::value_error("validate_context: curvature table sizes != ndof * curv_k (* 4)");
+                eg.curv_k = K;
+                eg.curv_nodes = keep(cn);
+                eg.curv_slot = keep(cs);
+                eg.curv_weight = keep(cw);
+            }
+            if (g.contains("dir_nodes"))
+                throw py::value_error("wire: directional terms are not supported by this compute core");
             groups.push_back(eg);
         }
         py::dict e = ctx["energy_stencil"].cast<py::dict>();

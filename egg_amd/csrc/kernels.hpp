@@ -72,8 +72,14 @@ struct SweepArgs {
     // the stored arrays, each sliding on the entity con_tag[i] (1 segment
     // sx sy ex ey, 2 circle cx cy r) with record con_rec[4 i ..]
     int con_n = 0;
-    const uint8_t* con_tag = nullptr;
+    const uint8_t* con_tag = nullptr;  // 0: a free DOF (there for its curvature windows)
     const float* con_rec = nullptr;
+    // block-interface C2 curvature windows of those DOFs (2D): curv_k per DOF,
+    // four node slots each, the DOF's place (-1 = padding) and the weight
+    int curv_k = 0;
+    const int* curv_nodes = nullptr;
+    const int8_t* curv_slot = nullptr;
+    const float* curv_w = nullptr;
     float* gout = nullptr;       // gradient-only launches: per-node gradient (plane layout)
     bool gsub = false;           // gradient-only: gout = gradient - gout, in place
     bool gpush = false;          // gradient-only: also write it into the DOF's replica slots

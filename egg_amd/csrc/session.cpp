@@ -154,6 +154,9 @@ void Level::upload_tables()
     d_st_dof.upload(st_dof);
     d_con_tag.upload(con_tag);
     d_con_rec.upload(con_rec);
+    d_curv_nodes.upload(curv_nodes);
+    d_curv_slot.upload(curv_slot);
+    d_curv_w.upload(curv_w);
     st_lanes = 0;  // the stored layout follows the tile variant (make_tiles)
     // structured energy cells
     cell_off.assign(lay.shape.size() + 1, 0);
@@ -214,6 +217,10 @@ SweepArgs Level::sweep_args(const float* xin, float* xout, float omega, float de
     a.con_n = (int)con_tag.size();
     a.con_tag = d_con_tag.p;
     a.con_rec = d_con_rec.p;
+    a.curv_k = con_tag.empty() ? 0 : curv_k;
+    a.curv_nodes = d_curv_nodes.p;
+    a.curv_slot = d_curv_slot.p;
+    a.curv_w = d_curv_w.p;
     a.st.rounds = st_rounds;
     a.st.dof = d_st_dof.p;
     a.st.total = st_padded;
@@ -286,6 +293,20 @@ void validate(int dim, const eggmi_group* groups, int n_groups, const eggmi_sten
             for (long long p = 0; p < g.total_samples; ++p)
                 if (!std::isfinite(g.weight[p]))
                     throw InvalidArgument("validate_context: " + grp + " weight must be finite");
+        if (g.curv_k < 0) throw InvalidArgument("validate_context: " + grp + " negative curv_k");
+        if (g.curv_k > 0) {
+            if (dim != 2) throw InvalidArgument("validate_context: curvature windows are 2D only");
+            if (g.curv_k > 64) throw InvalidArgument("validate_context: more than 64 curvature windows per DOF");
+            for (long long q = 0; q < g.ndof * g.curv_k; ++q) {
+                if (g.curv_slot[q] < -1 || g.curv_slot[q] > 3)
+                    throw InvalidArgument("validate_context: " + grp + " curv_slot outside [-1, 3]");
+                if (g.curv_slot[q] >= 0) {
+                    if (!std::isfinite(g.curv_weight[q]))
+                        throw InvalidArgument("validate_context: " + grp + " curv_weight must be finite");
+                    check_index_range(g.curv_nodes + 4 * q, 4, N, grp, "curv_nodes");
+                }
+            }
+        }
         if (g.entity_tag)
             for (long long d = 0; d < g.ndof; ++d) {
                 const int t = g.entity_tag[d];
@@ -538,6 +559,9 @@ void Session::create(int d, const eggmi_group* groups, int n_groups, const eggmi
     std::vector<double> wu_ref;
     for (int gi = 0; gi < n_groups; ++gi)
         if (groups[gi].w_rows != 1) w_uniform_all = false;
+    int kmax_curv = 0;  // curvature windows per special DOF (the widest group's)
+    for (int gi = 0; gi < n_groups; ++gi) kmax_curv = std::max(kmax_curv, (int)groups[gi].curv_k);
+    L.curv_k = kmax_curv;
     bool wt_unit_all = true;  // per-occurrence energy weights all 1 (or absent)
     for (int gi = 0; gi < n_groups; ++gi)
         if (!unit_weights(groups[gi].weight, groups[gi].total_samples)) wt_unit_all = false;
@@ -559,7 +583,10 @@ void Session::create(int d, const eggmi_group* groups, int n_groups, const eggmi
         long long off = 0;
         for (long long di = 0; di < g.ndof; ++di) {
             const bool con = g.entity_tag && g.entity_tag[di] != 0;
-            if (con != (pass == 1)) {
+            bool curv = false;
+            for (int j = 0; j < g.curv_k; ++j) curv = curv || g.curv_slot[di * g.curv_k + j] >= 0;
+            const bool special = con || curv;
+            if (special != (pass == 1)) {
                 off += g.P_of[di];
                 continue;
             }
@@ -573,11 +600,57 @@ void Session::create(int d, const eggmi_group* groups, int n_groups, const eggmi
                 for (int k = 0; k < d; ++k) occ_slots.push_back(lookup(ob, g.gn[k][p], "gn"));
             }
             const long long dslot = lookup(ob, gid, "dof_idx");
-            const int ib = g.interior_block[di];
-            if (con) {
-                if (ib >= 0) throw InvalidArgument("structured: a synthesized interior DOF cannot be constrained");
-                L.con_tag.push_back((uint8_t)g.entity_tag[di]);
-                for (int k = 0; k < 4; ++k) L.con_rec.push_back((float)g.entity_rec[di * 4 + k]);
+            int ib = g.interior_block[di];
+            if (special) {
+                if (con && ib >= 0)
+                    throw InvalidArgument("structured: a synthesized interior DOF cannot be constrained");
+                L.con_tag.push_back(con ? (uint8_t)g.entity_tag[di] : (uint8_t)0);
+                for (int k = 0; k < 4; ++k) L.con_rec.push_back(con ? (float)g.entity_rec[di * 4 + k] : 0.0f);
+                for (int j = 0; j < kmax_curv; ++j) {
+                    const bool live = j < g.curv_k && g.curv_slot[di * g.curv_k + j] >= 0;
+                    L.curv_slot.push_back(live ? (int8_t)g.curv_slot[di * g.curv_k + j] : (int8_t)-1);
+                    L.curv_w.push_back(live ? (float)g.curv_weight[di * g.curv_k + j] : 0.0f);
+                    for (int m = 0; m < 4; ++m) {
+                        const int nid = live ? g.curv_nodes[(di * g.curv_k + j) * 4 + m] : 0;
+                        if (live && g2s[nid] < 0)
+                            throw InvalidArgument("structured: a curvature window node is in no block");
+                        L.curv_nodes.push_back(live ? g2s[nid] : 0);
+                    }
+                }
+                if (ib >= 0) {
+                    // a synthesized interior DOF with curvature windows takes the
+                    // stored path: its patch expanded here in the reference order
+                    // (cell bits outer, corner bits inner, axis 0 most significant;
+                    // src/structured_patch.hpp:89-130) with the uniform target
+                    if (P != 0) throw InvalidArgument("structured: interior DOF with stored samples");
+                    int lg[3];
+                    for (int k = 0; k < d; ++k) lg[k] = g.interior_logical[di * d + k];
+                    L.st_dof.push_back((int)dslot);
+                    for (int occ = 0; occ < 16; ++occ) {
+                        const int cell = occ >> 2, o = occ & 3;
+                        int corner[2], ob2[2];
+                        for (int k = 0; k < 2; ++k) {
+                            ob2[k] = (o >> (1 - k)) & 1;
+                            corner[k] = lg[k] - 1 + ((cell >> (1 - k)) & 1) + ob2[k];
+                        }
+                        int role = (corner[0] == lg[0] && corner[1] == lg[1]) ? 0 : -1;
+                        L.st_gc.push_back((int)lay.interior_slot(ib, corner));
+                        uint8_t meta = 0;
+                        for (int k = 0; k < 2; ++k) {
+                            int nbr[2] = {corner[0], corner[1]};
+                            nbr[k] += ob2[k] == 0 ? 1 : -1;
+                            gn_rows.push_back((int)lay.interior_slot(ib, nbr));
+                            if (ob2[k]) meta |= (uint8_t)(8 << k);
+                            if (role < 0 && nbr[0] == lg[0] && nbr[1] == lg[1]) role = k + 1;
+                        }
+                        L.st_meta.push_back((uint8_t)(meta | (uint8_t)(role + 1)));
+                        if (!w_uniform_all) throw Internal("structured: synthesized DOF under a per-sample target");
+                        if (!wt_unit_all) L.st_wt.push_back(1.0f);
+                    }
+                    L.st_off.push_back((int)L.st_gc.size());
+                    off += P;
+                    continue;
+                }
             }
             if (ib >= 0) {
                 if (P != 0) throw InvalidArgument("structured: interior DOF with stored samples");
@@ -637,7 +710,8 @@ void Session::create(int d, const eggmi_group* groups, int n_groups, const eggmi
     for (int s : L.st_dof) fine_free[s] = 1;
     // constrained DOFs stay out of the multigrid (its corrections would move
     // them off their entity): neither free for the hierarchy nor corrected
-    for (size_t i = L.st_dof.size() - L.con_tag.size(); i < L.st_dof.size(); ++i) fine_free[L.st_dof[i]] = 0;
+    for (size_t i = 0; i < L.con_tag.size(); ++i)
+        if (L.con_tag[i]) fine_free[L.st_dof[L.st_dof.size() - L.con_tag.size() + i]] = 0;
     fine_halo.share_src.assign(share_src.begin(), share_src.end());
     fine_halo.share_dst.assign(share_dst.begin(), share_dst.end());
     L.n_synth = (long long)synth_slots.size();

@@ -142,6 +142,15 @@ struct Level {
     std::vector<float> con_rec;
     DArr<uint8_t> d_con_tag;
     DArr<float> d_con_rec;
+    // their curvature windows (curv_k per special DOF): owner slots of the
+    // four nodes, the DOF's place in the window (-1 = padding), the weight
+    int curv_k = 0;
+    std::vector<int> curv_nodes;
+    std::vector<int8_t> curv_slot;
+    std::vector<float> curv_w;
+    DArr<int> d_curv_nodes;
+    DArr<int8_t> d_curv_slot;
+    DArr<float> d_curv_w;
     float Wu[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
     DArr<int> d_st_dof, d_st_gc, d_st_gn;  // device copies in the padded lane layout (see StoredView)
     int st_rounds = 1;

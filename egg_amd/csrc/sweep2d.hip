@@ -40,22 +40,23 @@ template <int KIND, bool TAU, bool GRAD, bool PROP = false> static void sweep2d_
     }
 }
 
-template <int KIND, int G> static void constrained_g(const SweepArgs& a, hipStream_t s)
+template <int KIND, int G, bool GRAD> static void constrained_g(const SweepArgs& a, hipStream_t s)
 {
     const unsigned grid = (unsigned)((a.con_n + 256 / G - 1) / (256 / G));
     if (a.st.wt)
-        constrained_kernel<KIND, G, true><<<dim3(grid), dim3(256), 0, s>>>(a);
+        constrained_kernel<KIND, G, true, GRAD><<<dim3(grid), dim3(256), 0, s>>>(a);
     else
-        constrained_kernel<KIND, G, false><<<dim3(grid), dim3(256), 0, s>>>(a);
+        constrained_kernel<KIND, G, false, GRAD><<<dim3(grid), dim3(256), 0, s>>>(a);
 }
 
-// The constrained boundary DOFs, laid out for the variant's stored lane groups.
-template <int KIND> static void launch_constrained(int v, const SweepArgs& a, hipStream_t s)
+// The special stored DOFs (constrained, or carrying curvature windows), laid
+// out for the variant's stored lane groups.
+template <int KIND, bool GRAD = false> static void launch_constrained(int v, const SweepArgs& a, hipStream_t s)
 {
     switch (tile_variant_stored_lanes(2, v)) {
-    case 4: constrained_g<KIND, 4>(a, s); break;
-    case 8: constrained_g<KIND, 8>(a, s); break;
-    default: constrained_g<KIND, 16>(a, s); break;
+    case 4: constrained_g<KIND, 4, GRAD>(a, s); break;
+    case 8: constrained_g<KIND, 8, GRAD>(a, s); break;
+    default: constrained_g<KIND, 16, GRAD>(a, s); break;
     }
 }
 
@@ -98,7 +99,20 @@ static void sweep2d_main(int kind, int v, const SweepArgs& a, hipStream_t s)
         sweep2d_v<kShape, false, false>(v, a, s);
 }
 
+static void grad2d_main(int kind, int v, const SweepArgs& a, hipStream_t s);
+
 void launch_grad2d(int kind, int v, const SweepArgs& a, hipStream_t s)
+{
+    grad2d_main(kind, v, a, s);
+    if (a.con_n > 0 && !a.tau) {  // the fine level's special DOFs (FAS gradient)
+        if (kind == kShapeSize)
+            launch_constrained<kShapeSize, true>(v, a, s);
+        else
+            launch_constrained<kShape, true>(v, a, s);
+    }
+}
+
+static void grad2d_main(int kind, int v, const SweepArgs& a, hipStream_t s)
 {
     if (a.tau) {
         if (kind == kShapeSize)

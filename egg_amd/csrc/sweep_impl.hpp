@@ -208,6 +208,36 @@ __device__ __forceinline__ void ent_tangent(int tag, const float* e, const float
     }
 }
 
+// Block-interface C2 curvature window p0..p3 (reference src/curvature.hpp):
+// residual r = turn(p0, p1, p2) - turn(p1, p2, p3) of the signed turning
+// angles and, with grad, dr/dp_m = perp(e)/|e|^2 combinations per edge.
+__device__ __forceinline__ float curv_turn(const float* a, const float* b, const float* c)
+{
+    const float e1x = b[0] - a[0], e1y = b[1] - a[1], e2x = c[0] - b[0], e2y = c[1] - b[1];
+    return atan2f(e1x * e2y - e1y * e2x, e1x * e2x + e1y * e2y);
+}
+__device__ __forceinline__ float curv_residual(const float (&p)[4][2], float (&dr)[4][2])
+{
+    float gx[3], gy[3];
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        const float ex = p[e + 1][0] - p[e][0], ey = p[e + 1][1] - p[e][1];
+        const float l = ex * ex + ey * ey;
+        const float il = l > 0.0f ? 1.0f / l : 0.0f;
+        gx[e] = -ey * il;
+        gy[e] = ex * il;
+    }
+    dr[0][0] = gx[0];
+    dr[0][1] = gy[0];
+    dr[1][0] = -gx[0] - 2.0f * gx[1];
+    dr[1][1] = -gy[0] - 2.0f * gy[1];
+    dr[2][0] = 2.0f * gx[1] + gx[2];
+    dr[2][1] = 2.0f * gy[1] + gy[2];
+    dr[3][0] = -gx[2];
+    dr[3][1] = -gy[2];
+    return curv_turn(p[0], p[1], p[2]) - curv_turn(p[1], p[2], p[3]);
+}
+
 // WT: per-occurrence energy weights (st.wt); a separate instantiation, so the
 // unweighted sums keep their exact operation order. ENT: d is a constrained
 // boundary DOF (2D; stored index st.n + i slides on entity i): tangent-reduced
@@ -216,7 +246,7 @@ __device__ __forceinline__ void ent_tangent(int tag, const float* e, const float
 template <int D, int KIND, bool TAU, bool GRAD, int G, bool WT, bool ENT = false>
 __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
 {
-    static_assert(!ENT || (D == 2 && !TAU && !GRAD), "constrained DOFs: 2D plain sweeps");
+    static_assert(!ENT || (D == 2 && !TAU), "special DOFs: 2D, fine level");
     static_assert(G >= 4 && G <= stored_lanes(D), "stored lane group size");
     const StoredView& st = a.st;
     const float* X = a.xin;
@@ -287,6 +317,19 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
             }
     };
 
+    // special DOFs (ENT): tag != 0 slides on an entity; tag 0 is free and
+    // carries curvature windows, lane l taking windows l, l + G, ...
+    int etag = 0;
+    float erec[4] = {};
+    const int ci = d - st.n;
+    if constexpr (ENT) {
+        if (valid) {
+            etag = a.con_tag[ci];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) erec[k] = a.con_rec[4 * ci + k];
+        }
+    }
+    const bool curv = ENT && valid && etag == 0 && a.curv_k > 0;
     Occ o0;
     load(0, o0);
     float g[D] = {}, h[kNH<D>] = {}, e = 0.0f, md = kInf;
@@ -334,6 +377,40 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
                 e += sample_value<D, KIND>(T, a.delta);
         }
     }
+    if constexpr (ENT) {
+        if (curv)  // curvature on top of the shape metric (reference src/sweep.hpp:1461-1487)
+            for (int j = l; j < a.curv_k; j += G) {
+                const long long w = (long long)ci * a.curv_k + j;
+                const int sl = a.curv_slot[w];
+                if (sl < 0) continue;
+                float p[4][2], dr[4][2];
+#pragma unroll
+                for (int m = 0; m < 4; ++m)
+#pragma unroll
+                    for (int i = 0; i < 2; ++i) p[m][i] = X[i * PL + a.curv_nodes[4 * w + m]];
+                const float r = curv_residual(p, dr);
+                const float wt = a.curv_w[w], gc = 2.0f * wt * r, hc = 2.0f * wt;
+                float dsx = 0.0f, dsy = 0.0f, off = 0.0f;
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const float nm = sqrtf(dr[m][0] * dr[m][0] + dr[m][1] * dr[m][1]);
+                    if (m == sl) {
+                        dsx = dr[m][0];
+                        dsy = dr[m][1];
+                    } else {
+                        off += nm;
+                    }
+                }
+                // Gauss-Newton Hessian plus the diagonal stabiliser 2 w |dr_s| sum |dr_m|
+                const float stab = hc * sqrtf(dsx * dsx + dsy * dsy) * off;
+                g[0] += gc * dsx;
+                g[1] += gc * dsy;
+                h[0] += hc * dsx * dsx + stab;
+                h[1] += hc * dsx * dsy;
+                h[2] += hc * dsy * dsy + stab;
+                e += wt * r * r;
+            }
+    }
 #pragma unroll
     for (int k = 0; k < D; ++k) g[k] = group_sum<G>(g[k]);
     float e0 = group_sum<G>(e);
@@ -360,13 +437,7 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
 #pragma unroll
     for (int k = 0; k < kNH<D>; ++k) h[k] = group_sum<G>(h[k]);
     float dl[D];
-    int etag = 0;
-    float erec[4] = {};
-    if constexpr (ENT) {
-        const int ci = d - st.n;
-        etag = a.con_tag[ci];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) erec[k] = a.con_rec[4 * ci + k];
+    if (ENT && etag != 0) {
         const float ft = etag == 15 ? rail_foot(erec, pos) : 0.0f;
         if (etag == 15 && (ft < 0.0f || ft > 1.0f)) {
             // a rail beyond its ends releases the node: full-space step
@@ -400,11 +471,26 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
         float tr[D];
 #pragma unroll
         for (int k = 0; k < D; ++k) tr[k] = pos[k] + alpha * dl[k];
-        if constexpr (ENT) {
+        if (ENT && etag != 0) {
             const float raw[2] = {tr[0], tr[1]};
             ent_project(etag, erec, raw, tr);
         }
         float et = 0.0f, mt = kInf;
+        if constexpr (ENT) {
+            if (curv)  // the windows' energy with the DOF moved to the trial
+                for (int j = l; j < a.curv_k; j += G) {
+                    const long long w = (long long)ci * a.curv_k + j;
+                    const int sl = a.curv_slot[w];
+                    if (sl < 0) continue;
+                    float p[4][2];
+#pragma unroll
+                    for (int m = 0; m < 4; ++m)
+#pragma unroll
+                        for (int i = 0; i < 2; ++i) p[m][i] = m == sl ? tr[i] : X[i * PL + a.curv_nodes[4 * w + m]];
+                    const float r = curv_turn(p[0], p[1], p[2]) - curv_turn(p[1], p[2], p[3]);
+                    et += a.curv_w[w] * r * r;
+                }
+        }
         for (int r = 0; r < R; ++r) {
             Occ o = o0;
             if (r > 0) load(r, o);
@@ -431,7 +517,7 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
         }
         alpha *= 0.5f;
     }
-    if (!ENT && a.omega != 1.0f)  // constrained DOFs take omega on the step instead
+    if (etag == 0 && a.omega != 1.0f)  // constrained DOFs take omega on the step instead
 #pragma unroll
         for (int k = 0; k < D; ++k) cur[k] = pos[k] + a.omega * (cur[k] - pos[k]);
     if (l == 0) {
@@ -440,13 +526,14 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
     }
 }
 
-// The constrained boundary DOFs of a plain sweep (2D), in a launch of their
+// The special stored DOFs of a fine-level sweep or gradient pass (2D:
+// constrained ones and those with curvature windows), in a launch of their
 // own beside the sweep kernel: both read xin and write disjoint DOF slots.
-template <int KIND, int G, bool WT>
+template <int KIND, int G, bool WT, bool GRAD>
 __global__ __launch_bounds__(256) void constrained_kernel(const SweepArgs a)
 {
     const int gid = int(blockIdx.x) * (256 / G) + int(threadIdx.x) / G;
-    stored_group<2, KIND, false, false, G, WT, true>(a, a.st.n + gid, int(threadIdx.x) % G, gid < a.con_n);
+    stored_group<2, KIND, false, GRAD, G, WT, true>(a, a.st.n + gid, int(threadIdx.x) % G, gid < a.con_n);
 }
 
 // ------------------------------------------------------------------ tiles

@@ -278,6 +278,36 @@ def annulus_2d(n: int = 65, m: int = 33, *, r: float = 0.5, half: float = 2.0, s
     return Grid(2, blocks, X, free), ents
 
 
+def interface_c2_windows(grid: Grid, weight: float = 1.0) -> tuple[np.ndarray, np.ndarray]:
+    """Curvature windows across the block interfaces of a 2D grid: every node
+    on the faces of exactly two blocks (not a face corner) gets the windows
+    (a2, a1, n, b1) and (a1, n, b1, b2), with a1, a2 and b1, b2 its first two
+    inward neighbours across each block's face: the grid line through the
+    interface should not kink there (the reference's block-interface C2
+    term). Returns (nodes (W, 4) global ids, weight (W,))."""
+    inward: dict[int, list] = {}
+    for ids in grid.blocks:
+        ids = np.asarray(ids)
+        nx, ny = ids.shape
+        if nx < 3 or ny < 3:
+            continue
+        for ax, n in ((0, nx), (1, ny)):
+            other = ny if ax == 0 else nx
+            for side in (0, 1):
+                i0, i1, i2 = (0, 1, 2) if side == 0 else (n - 1, n - 2, n - 3)
+                for t in range(1, other - 1):
+                    idx = (lambda i: (i, t)) if ax == 0 else (lambda i: (t, i))
+                    inward.setdefault(int(ids[idx(i0)]), []).append((int(ids[idx(i1)]), int(ids[idx(i2)])))
+    rows = []
+    for node, lst in inward.items():
+        if len(lst) != 2:
+            continue
+        (a1, a2), (b1, b2) = lst
+        rows += [(a2, a1, node, b1), (a1, node, b1, b2)]
+    nodes = np.asarray(rows, np.int32).reshape(-1, 4)
+    return nodes, np.full(nodes.shape[0], float(weight))
+
+
 def local_edge_length(grid: Grid) -> np.ndarray:
     """Shortest incident block edge per node."""
     h = np.full(grid.n_nodes, np.inf)

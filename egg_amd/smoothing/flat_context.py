@@ -95,9 +95,40 @@ def _entities_wire(dofs: np.ndarray, entities: dict | None) -> dict:
                 "records": np.asarray(v["records"], np.float64)} for k, v in parts.items()}
 
 
+def curvature_table(free_mask: np.ndarray, nodes: np.ndarray, weight: np.ndarray) -> dict:
+    """Per-DOF curvature window table (reference
+    egg/smoothing/interface_c2.py:230-279): every free node of a window
+    (four global node ids, a weight) lists it with its place in the window,
+    padded to a common width ``curv_k`` (place -1 = padding), in the group's
+    DOF order."""
+    free = np.asarray(free_mask, bool)
+    dofs = np.flatnonzero(free)
+    nodes = np.asarray(nodes, np.int64).reshape(-1, 4)
+    weight = np.asarray(weight, np.float64).reshape(-1)
+    pos = np.full(free.shape[0], -1, np.int64)
+    pos[dofs] = np.arange(dofs.size)
+    lists = [[] for _ in range(dofs.size)]
+    for wi in range(nodes.shape[0]):
+        for slot in range(4):
+            n = int(nodes[wi, slot])
+            if free[n]:
+                lists[int(pos[n])].append((nodes[wi], slot, float(weight[wi])))
+    K = max((len(x) for x in lists), default=0)
+    if K == 0:
+        return {}
+    cn = np.full((dofs.size, K, 4), -1, np.int32)
+    cs = np.full((dofs.size, K), -1, np.int32)
+    cw = np.zeros((dofs.size, K))
+    for i, entry in enumerate(lists):
+        for j, (row, slot, w) in enumerate(entry):
+            cn[i, j], cs[i, j], cw[i, j] = row, slot, w
+    return {"curv_k": int(K), "curv_nodes": cn.reshape(-1), "curv_slot": cs.reshape(-1),
+            "curv_weight": cw.reshape(-1)}
+
+
 def build_flat_context(blocks: list[np.ndarray], free_mask: np.ndarray, d: int, *,
                        w_inv: np.ndarray | None = None, weight: np.ndarray | None = None,
-                       entities: dict | None = None) -> dict:
+                       entities: dict | None = None, curvature: tuple | None = None) -> dict:
     """Assemble the flat wire for the moving DOFs ``free_mask`` of ``blocks``.
 
     ``w_inv`` is one uniform ``(d, d)`` inverse target (identity when
@@ -109,6 +140,8 @@ def build_flat_context(blocks: list[np.ndarray], free_mask: np.ndarray, d: int, 
     node id to the entity it slides on, ``("lineseg", (sx, sy, ex, ey))`` or
     ``("circle", (cx, cy, r))`` (2D): such a DOF ships its stored stencil and
     the group's ``entities`` field carries the per-type records.
+    ``curvature`` = (nodes (W, 4), weight (W,)) adds block-interface C2
+    curvature windows (2D; reference egg/smoothing/flat_context.py:293-304).
     """
     st = cell_stencil(blocks, d)
     ns = st["ns"]
@@ -175,6 +208,10 @@ def build_flat_context(blocks: list[np.ndarray], free_mask: np.ndarray, d: int, 
             "entities": _entities_wire(dofs, entities),
         }
         g.update(fields(m_sid))
+        if curvature is not None:
+            if d != 2:
+                raise ValueError("curvature windows are 2D only")
+            g.update(curvature_table(free_mask, *curvature))
         groups.append(g)
     es = {"num_samples": int(ns)}
     es.update(fields(np.arange(ns)))

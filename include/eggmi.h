@@ -54,6 +54,11 @@ typedef struct {
     const double* entity_rec;         /* [ndof*4] its record: segment / rail (sx, sy, ex, ey),
                                          circle (cx, cy, r, -), ellipse (cx, cy, rx, ry)
                                          (src/geometry.hpp:288-464) */
+    int32_t curv_k;                   /* block-interface C2 curvature windows per DOF (2D; 0 =
+                                         none; src/curvature.hpp, src/sweep.hpp:1461-1487) */
+    const int32_t* curv_nodes;        /* [ndof*curv_k*4] global node ids of each window */
+    const int32_t* curv_slot;         /* [ndof*curv_k] the DOF's place (0..3) in it, -1 = pad */
+    const double* curv_weight;        /* [ndof*curv_k] window weight */
 } eggmi_group;
 
 /* The energy stencil (every (cell, corner) sample). */

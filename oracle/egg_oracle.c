@@ -603,15 +603,105 @@ static void FN(update_constrained)(int kind, real delta, real omega, const FN(pa
     out[1] = cur[1];
 }
 
+/* ------------------------------------------- block-interface C2 curvature */
+
+/* A window of four nodes p0..p3 along a grid line: residual r = turn(p0, p1,
+ * p2) - turn(p1, p2, p3) (signed turning angles), energy w r^2; the DOF is
+ * node `slot` of the window (reference src/curvature.hpp). */
+static double FN(curv_turn)(const double* a, const double* b, const double* c)
+{
+    const double e1x = b[0] - a[0], e1y = b[1] - a[1], e2x = c[0] - b[0], e2y = c[1] - b[1];
+    return atan2(e1x * e2y - e1y * e2x, e1x * e2x + e1y * e2y);
+}
+
+/* r and dr/dp_m (m = 0..3) of one window (reference src/curvature.hpp:57-94):
+ * the turning angle's gradient is perp(e)/|e|^2 per edge. */
+static double FN(curv_residual_grad)(const double p[4][2], double dr[4][2])
+{
+    double g[3][2];
+    for (int e = 0; e < 3; ++e) {
+        const double ex = p[e + 1][0] - p[e][0], ey = p[e + 1][1] - p[e][1];
+        const double l = ex * ex + ey * ey;
+        const double il = l > 0 ? 1.0 / l : 0.0;
+        g[e][0] = -ey * il;
+        g[e][1] = ex * il;
+    }
+    for (int i = 0; i < 2; ++i) {
+        dr[0][i] = g[0][i];
+        dr[1][i] = -g[0][i] - 2.0 * g[1][i];
+        dr[2][i] = 2.0 * g[1][i] + g[2][i];
+        dr[3][i] = -g[2][i];
+    }
+    return FN(curv_turn)(p[0], p[1], p[2]) - FN(curv_turn)(p[1], p[2], p[3]);
+}
+
+/* The DOF's windows into grad, hess (2x2) and energy (reference
+ * src/curvature.hpp:116-164): gradient 2 w r dr_s, Gauss-Newton Hessian
+ * 2 w dr_s dr_s^T plus the diagonal stabiliser 2 w |dr_s| sum_{m != s} |dr_m|,
+ * energy w r^2. Windows with slot < 0 are padding. */
+static void FN(curv_accumulate)(int K, const int* nodes, const int* slot, const real* wt, const real* X,
+                                real* grad, real* hess, real* e0)
+{
+    for (int j = 0; j < K; ++j) {
+        const int s = slot[j];
+        if (s < 0) continue;
+        double p[4][2], dr[4][2];
+        for (int m = 0; m < 4; ++m)
+            for (int i = 0; i < 2; ++i) p[m][i] = (double)X[(size_t)nodes[4 * j + m] * 2 + i];
+        const double r = FN(curv_residual_grad)(p, dr);
+        const double w = (double)wt[j];
+        const double gc = 2.0 * w * r, hc = 2.0 * w;
+        grad[0] += (real)(gc * dr[s][0]);
+        grad[1] += (real)(gc * dr[s][1]);
+        double off = 0;
+        for (int m = 0; m < 4; ++m)
+            if (m != s) off += sqrt(dr[m][0] * dr[m][0] + dr[m][1] * dr[m][1]);
+        const double stab = hc * sqrt(dr[s][0] * dr[s][0] + dr[s][1] * dr[s][1]) * off;
+        hess[0] += (real)(hc * dr[s][0] * dr[s][0] + stab);
+        hess[1] += (real)(hc * dr[s][0] * dr[s][1]);
+        hess[2] += (real)(hc * dr[s][1] * dr[s][0]);
+        hess[3] += (real)(hc * dr[s][1] * dr[s][1] + stab);
+        *e0 += (real)(w * r * r);
+    }
+}
+
+/* The windows' energy with the DOF (node `slot` of each) moved to `trial`
+ * (reference src/curvature.hpp:166-183). */
+static real FN(curv_trial)(int K, const int* nodes, const int* slot, const real* wt, const real* X,
+                           const real* trial)
+{
+    double e = 0;
+    for (int j = 0; j < K; ++j) {
+        const int s = slot[j];
+        if (s < 0) continue;
+        double p[4][2];
+        for (int m = 0; m < 4; ++m)
+            for (int i = 0; i < 2; ++i)
+                p[m][i] = m == s ? (double)trial[i] : (double)X[(size_t)nodes[4 * j + m] * 2 + i];
+        const double r = FN(curv_turn)(p[0], p[1], p[2]) - FN(curv_turn)(p[1], p[2], p[3]);
+        e += (double)wt[j] * r * r;
+    }
+    return (real)e;
+}
+
+typedef struct {
+    int K;
+    const int* nodes; /* [K*4] global node ids */
+    const int* slot;  /* [K] the DOF's slot in each window, -1 = padding */
+    const real* wt;   /* [K] window weight */
+} FN(curv_t);
+
 /* ------------------------------------------------------------ Jacobi sweep */
 
 /* One DOF's block-Jacobi update: metric pass, Newton, <=10 halvings, omega
  * post-blend (reference src/sweep.hpp:1534-1650). X is the frozen snapshot. */
 static void FN(update_dof)(int dim, int kind, real delta, real omega, const FN(patch_t)* pv, const real* X, int dof,
-                       const real* tau, real* out)
+                       const real* tau, const FN(curv_t)* cv, real* out)
 {
     real grad[3], hess[9], e0, md;
     FN(patch_eval_impl)(dim, kind, delta, pv, X, grad, hess, &e0, &md);
+    if (cv) /* curvature windows on top of the shape metric (src/sweep.hpp:1461-1487) */
+        FN(curv_accumulate)(cv->K, cv->nodes, cv->slot, cv->wt, X, grad, hess, &e0);
     real pos[3];
     for (int k = 0; k < dim; ++k) pos[k] = X[(size_t)dof * dim + k];
     if (tau) {
@@ -628,6 +718,7 @@ static void FN(update_dof)(int dim, int kind, real delta, real omega, const FN(p
         real trial[3], e_new, mdet;
         for (int k = 0; k < dim; ++k) trial[k] = pos[k] + alpha * dlt[k];
         FN(trial_impl)(dim, kind, delta, pv, X, dof, trial, &e_new, &mdet);
+        if (cv) e_new += FN(curv_trial)(cv->K, cv->nodes, cv->slot, cv->wt, X, trial);
         if (tau)
             for (int k = 0; k < dim; ++k) e_new -= tau[(size_t)dof * dim + k] * trial[k];
         const int ok = isfinite(e_new) && (e_new <= e0 + TOL_ENERGY) && (kind == KIND_UNTANGLE || mdet > 0);
@@ -654,7 +745,8 @@ static void FN(update_dof)(int dim, int kind, real delta, real omega, const FN(p
 int FN(o_jacobi)(int dim, int kind, double delta, double omega, int n_sweeps, int report_every, int ndof,
                  const int* dof, const long long* poff, const int* pgc, const int* pgn, const real* ps,
                  const int* prole, const real* W, int w_stride, const real* pwt, const int* etag,
-                 const real* erec, int n_nodes, real* X, const real* tau, long long ns, const int* egc,
+                 const real* erec, int curv_k, const int* cnodes, const int* cslot, const real* cwt,
+                 int n_nodes, real* X, const real* tau, long long ns, const int* egc,
                  const int* egn, const real* es, const real* eW, int ew_stride, const real* ewt, real* work,
                  double* energies, double* mindets)
 {
@@ -682,9 +774,12 @@ int FN(o_jacobi)(int dim, int kind, double delta, double omega, int n_sweeps, in
             if (etag && etag[d])  /* constrained boundary DOF (2D) */
                 FN(update_constrained)(kind, (real)delta, (real)omega, &pv, xin, dof[d], etag[d], erec + 4 * d,
                                        xout + (size_t)dof[d] * dim);
-            else
+            else {
+                FN(curv_t) cv = {curv_k, cnodes ? cnodes + (size_t)4 * curv_k * d : NULL,
+                                 cslot ? cslot + (size_t)curv_k * d : NULL, cwt ? cwt + (size_t)curv_k * d : NULL};
                 FN(update_dof)(dim, kind, (real)delta, (real)omega, &pv, xin, dof[d], tau,
-                               xout + (size_t)dof[d] * dim);
+                               curv_k > 0 ? &cv : NULL, xout + (size_t)dof[d] * dim);
+            }
         }
         if ((sw + 1) % k == 0 || sw == n_sweeps - 1) {
             FN(o_energy_mindet)(dim, kind, delta, ns, egc, egn, es, eW, ew_stride, ewt, xout, &energies[nrep],
@@ -704,7 +799,8 @@ int FN(o_jacobi)(int dim, int kind, double delta, double omega, int n_sweeps, in
  * with HasTau). */
 void FN(o_patch_grads)(int dim, int kind, double delta, int ndof, const int* dof, const long long* poff,
                        const int* pgc, const int* pgn, const real* ps, const int* prole, const real* W,
-                       int w_stride, const real* pwt, const real* X, const real* tau, real* gout)
+                       int w_stride, const real* pwt, int curv_k, const int* cnodes, const int* cslot,
+                       const real* cwt, const real* X, const real* tau, real* gout)
 {
     const long long total = poff[ndof];
 #pragma omp parallel for schedule(static)
@@ -722,6 +818,9 @@ void FN(o_patch_grads)(int dim, int kind, double delta, int ndof, const int* dof
                           pwt ? pwt + o : NULL};
         real grad[3], hess[9], e0, md;
         FN(patch_eval_impl)(dim, kind, (real)delta, &pv, X, grad, hess, &e0, &md);
+        if (curv_k > 0)
+            FN(curv_accumulate)(curv_k, cnodes + (size_t)4 * curv_k * d, cslot + (size_t)curv_k * d,
+                                cwt + (size_t)curv_k * d, X, grad, hess, &e0);
         for (int k = 0; k < dim; ++k)
             gout[(size_t)dof[d] * dim + k] = grad[k] - (tau ? tau[(size_t)dof[d] * dim + k] : (real)0);
     }

@@ -67,14 +67,14 @@ def lib():
             f = getattr(L, "o_jacobi" + sfx)
             f.argtypes = [
                 C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
-                _ip, _lp, _ip, _ip, rp, _ip, rp, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, rp,
-                C.c_void_p,
+                _ip, _lp, _ip, _ip, rp, _ip, rp, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, rp, C.c_void_p,
                 C.c_longlong, _ip, _ip, rp, rp, C.c_int, C.c_void_p, rp, _dp, _dp,
             ]
             f.restype = C.c_int
             f = getattr(L, "o_patch_grads" + sfx)
             f.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, _ip, _lp, _ip, _ip, rp, _ip, rp, C.c_int,
-                          C.c_void_p, rp, C.c_void_p, rp]
+                          C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, rp, C.c_void_p, rp]
             f.restype = None
         L.o_set_threads.argtypes = [C.c_int]
         L.o_set_threads.restype = None
@@ -192,7 +192,7 @@ def interior_occurrences(dim: int, logical, idx):
 class Patches:
     """Per-DOF patches in global id space (CSR over DOFs)."""
 
-    def __init__(self, dim, dof, poff, gc, gn, s, role, W, wt=None, etag=None, erec=None):
+    def __init__(self, dim, dof, poff, gc, gn, s, role, W, wt=None, etag=None, erec=None, curv=None):
         self.dim = dim
         self.dof = np.ascontiguousarray(dof, np.int32)
         self.poff = np.ascontiguousarray(poff, np.int64)
@@ -205,6 +205,8 @@ class Patches:
         # per-DOF entity a constrained boundary DOF slides on (0 free, 1 segment, 2 circle) and its record
         self.etag = None if etag is None else np.ascontiguousarray(etag, np.int32)
         self.erec = None if erec is None else np.ascontiguousarray(erec, np.float64).reshape(-1, 4)
+        # per-DOF curvature windows (K, nodes (ndof, K, 4), slot (ndof, K), weight (ndof, K)) or None
+        self.curv = curv
 
 
 def materialize_patches(wire: dict, block_global_dof: list[np.ndarray], dim: int, dofs=None) -> Patches:
@@ -286,7 +288,13 @@ def materialize_patches(wire: dict, block_global_dof: list[np.ndarray], dim: int
         etag, erec = etag[sel], erec[sel]
         if np.any(etag[interior] != 0):
             raise ValueError("a synthesized interior DOF cannot be constrained")
-    return Patches(dim, dof, poff, gc, gn, sg, role, Wp, wt, etag, erec)
+    curv = None
+    K = int(g.get("curv_k", 0) or 0)
+    if K > 0:
+        curv = (K, np.ascontiguousarray(np.asarray(g["curv_nodes"], np.int32).reshape(-1, K, 4)[sel]),
+                np.ascontiguousarray(np.asarray(g["curv_slot"], np.int32).reshape(-1, K)[sel]),
+                np.ascontiguousarray(np.asarray(g["curv_weight"], np.float64).reshape(-1, K)[sel]))
+    return Patches(dim, dof, poff, gc, gn, sg, role, Wp, wt, etag, erec, curv)
 
 
 def _entity_rows(g: dict, ndof: int):
@@ -365,14 +373,24 @@ def jacobi(dim, kind, delta, omega, n_sweeps, X, patches: Patches, es: dict, *, 
     pwt = _weights(patches.wt, dt)
     etag = patches.etag
     erec = None if patches.erec is None else np.ascontiguousarray(patches.erec, dt).reshape(-1)
+    ck, cn, cs, cw = _curv_args(patches, dt)
     tau_arr = None if tau is None else np.ascontiguousarray(tau, dt).reshape(-1)
     got = getattr(lib(), f"o_jacobi_{prec}")(
         dim, KIND[kind], float(delta), float(omega), int(n_sweeps), int(report_every),
         len(patches.dof), patches.dof, patches.poff, patches.gc, patches.gn.reshape(-1),
         np.ascontiguousarray(patches.s, dt).reshape(-1), patches.role, W, ws, _ptr(pwt), _ptr(etag), _ptr(erec),
-        n_nodes, Xw, _ptr(tau_arr), ns, egc, egn, es_, eW, ews, _ptr(ewt), work, en, md,
+        ck, _ptr(cn), _ptr(cs), _ptr(cw), n_nodes, Xw, _ptr(tau_arr), ns, egc, egn, es_, eW, ews, _ptr(ewt),
+        work, en, md,
     )
     return Xw.reshape(-1, dim), en[:got], md[:got]
+
+
+def _curv_args(patches: Patches, dt):
+    if patches.curv is None:
+        return 0, None, None, None
+    K, nodes, slot, wt = patches.curv
+    return (int(K), np.ascontiguousarray(nodes, np.int32).reshape(-1), np.ascontiguousarray(slot, np.int32).reshape(-1),
+            np.ascontiguousarray(wt, dt).reshape(-1))
 
 
 def patch_grads(dim, kind, X, patches: Patches, *, delta=0.0, tau=None, prec="f64"):
@@ -383,10 +401,11 @@ def patch_grads(dim, kind, X, patches: Patches, *, delta=0.0, tau=None, prec="f6
     W = np.ascontiguousarray(patches.W, dt).reshape(-1)
     ws = 0 if W.size == dim * dim else dim * dim
     pwt = _weights(patches.wt, dt)
+    ck, cn, cs, cw = _curv_args(patches, dt)
     tau_arr = None if tau is None else np.ascontiguousarray(tau, dt).reshape(-1)
     getattr(lib(), f"o_patch_grads_{prec}")(
         dim, KIND[kind], float(delta), len(patches.dof), patches.dof, patches.poff, patches.gc,
         patches.gn.reshape(-1), np.ascontiguousarray(patches.s, dt).reshape(-1), patches.role, W, ws,
-        _ptr(pwt), Xw, _ptr(tau_arr), out,
+        _ptr(pwt), ck, _ptr(cn), _ptr(cs), _ptr(cw), Xw, _ptr(tau_arr), out,
     )
     return out.reshape(-1, dim)

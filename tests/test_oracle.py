@@ -398,3 +398,46 @@ def test_constrained_step_matches_finite_differences(oracle, aspect):
             np.testing.assert_allclose(X1[dof], want, rtol=0, atol=1e-4 * np.abs(y))
             checked += 1
     assert checked >= 3
+
+
+def test_curvature_gradient_matches_finite_differences(oracle):
+    """The gradient pass with block-interface curvature windows (reference
+    src/curvature.hpp) equals central differences of the composed DOF energy:
+    its patch energy plus sum w r^2 over its windows, r = turn(p0, p1, p2) -
+    turn(p1, p2, p3) evaluated here with numpy."""
+    from egg_amd import grids
+    from tests.cases import Case, ogrid
+
+    g = ogrid(17, 9).grid
+    nodes, w = grids.interface_c2_windows(g, 0.7)
+    c = Case(g, curvature=(nodes, w))
+    P = c.patches(oracle)
+    K, cn, cs, cw = P.curv
+    G = oracle.patch_grads(2, "shape", c.X, P)
+
+    def turn(a, b, q):
+        e1, e2 = b - a, q - b
+        return np.arctan2(e1[0] * e2[1] - e1[1] * e2[0], e1 @ e2)
+
+    checked = 0
+    for d in np.flatnonzero((cs >= 0).any(1))[::7]:
+        a, b = P.poff[d], P.poff[d + 1]
+        dof = int(P.dof[d])
+        args = (P.gc[a:b], P.gn[:, a:b], P.s[:, a:b], P.role[a:b], P.W)
+
+        def energy(X):
+            e = oracle.patch_eval(2, "shape", 0.0, *args, X)[2]
+            for j in range(K):
+                if cs[d, j] >= 0:
+                    p = X[cn[d, j]]
+                    e += cw[d, j] * (turn(p[0], p[1], p[2]) - turn(p[1], p[2], p[3])) ** 2
+            return e
+
+        h = 1e-6
+        for k in range(2):
+            Xp, Xm = c.X.copy(), c.X.copy()
+            Xp[dof, k] += h
+            Xm[dof, k] -= h
+            assert (energy(Xp) - energy(Xm)) / (2 * h) == pytest.approx(G[dof, k], rel=1e-5, abs=1e-6)
+        checked += 1
+    assert checked >= 10
