@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <array>
+#include <cmath>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -203,8 +204,24 @@ struct Wire {
                 eg.curv_slot = keep(cs);
                 eg.curv_weight = keep(cw);
             }
-            if (g.contains("dir_nodes"))
-                throw py::value_error("wire: directional terms are not supported by this compute core");
+            if (g.contains("dir_k") && !g["dir_k"].is_none() && g["dir_k"].cast<int>() > 0) {
+                // per-DOF directional windows (reference src/bindings.cpp:393-409)
+                const int K = g["dir_k"].cast<int>();
+                IArr dn = get_i(g, "dir_nodes"), ds = get_i(g, "dir_slot"), dk = get_i(g, "dir_kind");
+                DArr dr = get_d(g, "dir_ref"), dw = get_d(g, "dir_weight");
+                const int64_t n = eg.ndof * K;
+                if ((int64_t)dn.size() != n * 4 || (int64_t)ds.size() != n || (int64_t)dk.size() != n ||
+                    (int64_t)dr.size() != n * d || (int64_t)dw.size() != n)
+                    throw py::value_error("validate_context: directional table sizes != ndof * dir_k (* 4, * D)");
+                eg.dir_k = K;
+                eg.dir_eps = g["dir_eps"].cast<double>();
+                eg.dir_nodes = keep(dn);
+                eg.dir_slot = keep(ds);
+                eg.dir_kind = keep(dk);
+                eg.dir_ref = keep(dr);
+                eg.dir_weight = keep(dw);
+                if (g.contains("dir_axis") && !g["dir_axis"].is_none()) eg.dir_axis = keep(get_d(g, "dir_axis"));
+            }
             groups.push_back(eg);
         }
         py::dict e = ctx["energy_stencil"].cast<py::dict>();
@@ -235,8 +252,22 @@ struct Wire {
                                       " != " + std::to_string(es.num_samples));
             es.weight = keep(wt);
         }
-        if (e.contains("dir_nodes"))
-            throw py::value_error("wire: energy-stencil directional terms are not supported");
+        if (e.contains("dir_num") && !e["dir_num"].is_none() && e["dir_num"].cast<int64_t>() > 0) {
+            // whole-sample directional table (reference src/bindings.cpp:471-480)
+            const int64_t n = e["dir_num"].cast<int64_t>();
+            IArr dn = get_i(e, "dir_nodes"), dk = get_i(e, "dir_kind");
+            DArr dr = get_d(e, "dir_ref"), dw = get_d(e, "dir_weight");
+            if ((int64_t)dn.size() != n * 4 || (int64_t)dk.size() != n || (int64_t)dr.size() != n * d ||
+                (int64_t)dw.size() != n)
+                throw py::value_error("validate_context: energy-stencil directional table sizes != dir_num (* 4, * D)");
+            es.dir_num = n;
+            es.dir_eps = e["dir_eps"].cast<double>();
+            es.dir_nodes = keep(dn);
+            es.dir_kind = keep(dk);
+            es.dir_ref = keep(dr);
+            es.dir_weight = keep(dw);
+            if (e.contains("dir_axis") && !e["dir_axis"].is_none()) es.dir_axis = keep(get_d(e, "dir_axis"));
+        }
 
         // structured tables
         py::list shl = structured["interior_shapes"].cast<py::list>();
@@ -672,6 +703,84 @@ PYBIND11_MODULE(cpp_core, m)
       .def("gradient", &Mlp::gradient, py::arg("rows"))
       .def("train", &Mlp::train, py::arg("epochs"), py::kw_only(), py::arg("batch") = 4096,
            py::arg("lr") = 1e-3, py::arg("seed") = 0);
+
+    m.def(
+      "directional_eval",
+      [](const py::array_t<double, py::array::c_style | py::array::forcecast>& X, const IArr& nodes,
+         const IArr& kind, const py::array_t<double, py::array::c_style | py::array::forcecast>& ref,
+         const DArr& weight, double eps) {
+          // host-side total and analytic gradient of a directional sample
+          // table (reference src/bindings.cpp:296-330, :1819-1870), in FP64
+          if (X.ndim() != 2 || (X.shape(1) != 2 && X.shape(1) != 3))
+              throw py::value_error("directional_eval: X must be (n_nodes, 2) or (n_nodes, 3)");
+          const int d = (int)X.shape(1);
+          const py::ssize_t n = weight.size(), nn = X.shape(0);
+          if (nodes.ndim() != 2 || nodes.shape(1) != 4 || nodes.shape(0) != n || kind.size() != n ||
+              ref.ndim() != 2 || ref.shape(0) != n || ref.shape(1) != d)
+              throw py::value_error("directional_eval: inconsistent sample-table shapes");
+          const double* x = X.data();
+          py::array_t<double> grad({nn, (py::ssize_t)d});
+          double* g = grad.mutable_data();
+          std::fill(g, g + nn * d, 0.0);
+          double e = 0.0;
+          for (py::ssize_t i = 0; i < n; ++i) {
+              const int k = kind.data()[i];
+              const int ar = k == 1 ? 3 : (k == 0 || k == 2) ? 2 : 0;
+              if (!ar) throw py::value_error("directional_eval: unsupported sample kind " + std::to_string(k));
+              const int* nd = nodes.data() + 4 * i;
+              for (int m = 0; m < ar; ++m)
+                  if (nd[m] < 0 || nd[m] >= nn) throw py::value_error("directional_eval: node id out of range");
+              const double* r = ref.data() + (py::ssize_t)d * i;
+              const double w = weight.data()[i];
+              auto P = [&](int m, int c) { return x[(py::ssize_t)nd[m] * d + c]; };
+              if (ar == 2) {  // parallel: u = p1 - p0; stem: u = p0 - p1
+                  const int hd = k == 0 ? 1 : 0;
+                  double u[3], ss = eps * eps, dot = 0.0;
+                  for (int c = 0; c < d; ++c) {
+                      u[c] = P(hd, c) - P(1 - hd, c);
+                      ss += u[c] * u[c];
+                      dot += u[c] * r[c];
+                  }
+                  const double sn = std::sqrt(ss), cc = dot / sn;
+                  e += w * cc * cc;
+                  for (int c = 0; c < d; ++c) {
+                      const double gu = 2.0 * w * cc * (r[c] / sn - cc * u[c] / ss);
+                      g[(py::ssize_t)nd[hd] * d + c] += gu;
+                      g[(py::ssize_t)nd[1 - hd] * d + c] -= gu;
+                  }
+              } else {  // line (a, c, b): r = q1/s1 + q2/s2, qk away from c
+                  double q[2][3], s[2], rv[3], qr[2] = {0.0, 0.0}, rr = 0.0;
+                  for (int t = 0; t < 2; ++t) {
+                      double ss = eps * eps;
+                      for (int c = 0; c < d; ++c) {
+                          q[t][c] = P(2 * t, c) - P(1, c);
+                          ss += q[t][c] * q[t][c];
+                      }
+                      s[t] = std::sqrt(ss);
+                  }
+                  for (int c = 0; c < d; ++c) {
+                      rv[c] = q[0][c] / s[0] + q[1][c] / s[1];
+                      rr += rv[c] * rv[c];
+                  }
+                  e += w * rr;
+                  for (int t = 0; t < 2; ++t)
+                      for (int c = 0; c < d; ++c) qr[t] += q[t][c] * rv[c];
+                  for (int c = 0; c < d; ++c) {
+                      double gs = 0.0;
+                      for (int t = 0; t < 2; ++t) {
+                          const double gt = 2.0 * w / s[t] * (rv[c] - q[t][c] * qr[t] / (s[t] * s[t]));
+                          g[(py::ssize_t)nd[2 * t] * d + c] += gt;
+                          gs += gt;
+                      }
+                      g[(py::ssize_t)nd[1] * d + c] -= gs;
+                  }
+              }
+          }
+          return py::make_tuple(e, grad);
+      },
+      py::arg("X"), py::arg("nodes"), py::arg("kind"), py::arg("ref"), py::arg("weight"), py::arg("eps") = 1e-12,
+      "Directional soft-energy total and analytic gradient over a sample table (host side, FP64): X (n_nodes, d), "
+      "nodes (n, 4) with -1 pads, kind (n,) 0 parallel / 1 line / 2 stem, ref (n, d), weight (n,).");
 
     m.def(
       "pcg_csr",

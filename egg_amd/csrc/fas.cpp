@@ -534,7 +534,7 @@ std::pair<std::vector<double>, std::vector<double>> Session::run_fas(int n_cycle
     const bool graphs_on = graph_chunk > 0 && !counting;
     if (graphs_on) {
         if (!d_ladder_parts.n) {
-            d_ladder_parts.alloc((size_t)2 * (kLadder + 1) * (kReduceBlocks / 2));
+            d_ladder_parts.alloc((size_t)2 * (kLadder + 1) * kReduceBlocks);  // cells + directional
             d_ladder_out.alloc(2 * (kLadder + 1));
             d_cyc_rep.alloc(2);
             d_alpha.alloc(1);
@@ -568,7 +568,8 @@ std::pair<std::vector<double>, std::vector<double>> Session::run_fas(int n_cycle
         double ls[4];
         auto ladder = [&](float a0, float a1) {
             if (F.structured_energy) {
-                launch_linesearch_pair(dim, kind, ra, F.corr.p, a0, a1, partials.p, d_ls.p, stream);
+                launch_linesearch_pair(dim, kind, ra, F.corr.p, a0, a1, partials.p, d_ls.p, stream,
+                                       F.es_dir.n ? &F.es_dir : nullptr);
             } else {
                 throw Internal("run_fas: the line search needs the structured cell energy stencil");
             }
@@ -772,7 +773,7 @@ void Session::fas_device_ladder(int kind)
     if (!F.structured_energy) throw Internal("run_fas: the line search needs the structured cell energy stencil");
     float* x = F.x[F.cur].p;
     launch_linesearch_ladder(dim, kind, fine_cells(x), F.corr.p, d_ladder_parts.p, (double)1e-6f, d_ladder_out.p,
-                             d_alpha.p, stream);
+                             d_alpha.p, stream, F.es_dir.n ? &F.es_dir : nullptr);
     launch_masked_axpy_dev(x, F.corr.p, d_alpha.p, F.d_free.p, F.lay.n_slots(), F.plane, dim, stream);
     halo(F, x);  // a no-op copy when nothing moved (the replicas already equal their owners)
 }

@@ -1,6 +1,7 @@
 // HIP kernels around the sweep: the fused halo / fan / share copy and the
 // FP64 energy + min-det reductions, plus the dimension dispatch of the
 // sweep launchers.
+#include "directional.hpp"
 #include "mg.hpp"
 #include "sweep_common.hpp"
 
@@ -422,7 +423,7 @@ __global__ __launch_bounds__(kReduceThreads) void pair_finish(const double* part
 }
 
 void launch_linesearch_ladder(int dim, int kind, const CellReduceArgs& a, const float* corr, double* partials,
-                              double tol, double* out, float* alpha, hipStream_t s)
+                              double tol, double* out, float* alpha, hipStream_t s, const DirView* dir)
 {
     const int nb = kReduceBlocks / 2;  // the pair kernel's grid: the same cell chunks
 #define EGGMI_LL(D, K) linesearch_ladder<D, K><<<nb, kReduceThreads, 0, s>>>(a, corr, partials)
@@ -438,11 +439,12 @@ void launch_linesearch_ladder(int dim, int kind, const CellReduceArgs& a, const 
             EGGMI_LL(3, kShape);
     }
 #undef EGGMI_LL
-    ladder_finish<<<1, kReduceThreads, 0, s>>>(partials, nb, tol, out, alpha);
+    const int nd = dir ? launch_dir_reduce(2, *dir, a.x, a.plane, corr, 0.0f, 0.0f, partials, nb, s) : 0;
+    ladder_finish<<<1, kReduceThreads, 0, s>>>(partials, nb + nd, tol, out, alpha);
 }
 
 void launch_linesearch_pair(int dim, int kind, const CellReduceArgs& a, const float* corr, float a0, float a1,
-                            double* partials, double* out, hipStream_t s)
+                            double* partials, double* out, hipStream_t s, const DirView* dir)
 {
     const int nb = kReduceBlocks / 2;  // 4 partials per block fit the 2*kReduceBlocks buffer
 #define EGGMI_LP(D, K) linesearch_pair<D, K><<<nb, kReduceThreads, 0, s>>>(a, corr, a0, a1, partials)
@@ -458,7 +460,8 @@ void launch_linesearch_pair(int dim, int kind, const CellReduceArgs& a, const fl
             EGGMI_LP(3, kShape);
     }
 #undef EGGMI_LP
-    pair_finish<<<1, kReduceThreads, 0, s>>>(partials, nb, out);
+    const int nd = dir ? launch_dir_reduce(1, *dir, a.x, a.plane, corr, a0, a1, partials, nb, s) : 0;
+    pair_finish<<<1, kReduceThreads, 0, s>>>(partials, nb + nd, out);
 }
 
 void launch_copy(float* x, long long plane, int dim, const int* src, const int* dst, long long n, hipStream_t s)
@@ -522,6 +525,58 @@ void launch_reduce_samples(int dim, int kind, const SampleReduceArgs& a, double*
 void launch_reduce_finish(const double* partials, int nparts, double* out_e, double* out_m, hipStream_t s)
 {
     reduce_finish<<<1, kReduceThreads, 0, s>>>(partials, nparts, out_e, out_m);
+}
+
+// Whole-sample directional energies at NA step lengths (reference
+// src/directional.hpp:279-345, added to the reported energy after the shape
+// total, src/sweep.hpp:1921, and to the FAS line-search energies,
+// src/fas.hpp:233). Positions are formed in FP32 as cell_trial forms them,
+// the energy evaluated and accumulated in FP64; the min det slot is +inf.
+// MODE 0: alpha 0; 1: a0, a1; 2: 0 then the ladder's alphas.
+template <int MODE>
+__global__ __launch_bounds__(kReduceThreads) void dir_reduce(const DirView v, const float* x, long long plane,
+                                                             const float* corr, float a0, float a1, double* partials)
+{
+    constexpr int NA = MODE == 0 ? 1 : MODE == 1 ? 2 : kLadder + 1;
+    double e[NA];
+#pragma unroll
+    for (int t = 0; t < NA; ++t) e[t] = 0.0;
+    for (int i = blockIdx.x * kReduceThreads + threadIdx.x; i < v.n; i += gridDim.x * kReduceThreads) {
+        const int kind = v.kind[i], ar = dir_arity(kind);
+        if (!ar) continue;
+#pragma unroll
+        for (int t = 0; t < NA; ++t) {
+            const float al = MODE == 0 ? 0.0f : MODE == 1 ? (t == 0 ? a0 : a1) : (t == 0 ? 0.0f : ladder_alpha(t));
+            double p[3][2] = {};
+            for (int m = 0; m < ar; ++m)
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    const long long q = c * plane + v.nodes[4 * i + m];
+                    p[m][c] = (double)(corr ? x[q] + al * corr[q] : x[q]);
+                }
+            e[t] += dir_energy<double>(kind, p, v.ref + 2 * i, (double)v.w[i], (double)v.eps);
+        }
+    }
+    for (int t = 0; t < NA; ++t) {
+        block_reduce_store(e[t], __builtin_huge_val(), partials + 2 * NA * blockIdx.x + 2 * t);
+        __syncthreads();
+    }
+}
+
+int launch_dir_reduce(int mode, const DirView& v, const float* x, long long plane, const float* corr, float a0,
+                      float a1, double* partials, int pbase, hipStream_t s)
+{
+    if (v.n <= 0) return 0;
+    int nb = (v.n + kReduceThreads - 1) / kReduceThreads;
+    if (nb > kReduceBlocks / 2) nb = kReduceBlocks / 2;
+    if (mode == 0)
+        dir_reduce<0><<<nb, kReduceThreads, 0, s>>>(v, x, plane, corr, a0, a1, partials + 2 * pbase);
+    else if (mode == 1)
+        dir_reduce<1><<<nb, kReduceThreads, 0, s>>>(v, x, plane, corr, a0, a1, partials + 4 * pbase);
+    else
+        dir_reduce<2><<<nb, kReduceThreads, 0, s>>>(v, x, plane, corr, a0, a1,
+                                                    partials + 2 * (kLadder + 1) * pbase);
+    return nb;
 }
 
 }  // namespace eggmi

@@ -56,6 +56,21 @@ struct StoredView {
 
 constexpr int kCounterSlots = 64, kCounterStride = 16;
 
+// Directional soft-energy samples on the store (2D): node slots (4 per
+// sample, the first arity of them used), kind (0 parallel, 1 line, 2 stem,
+// -1 none), frozen reference vector (2 per sample), weight. As a per-DOF
+// window table, n is the windows per DOF and slot the DOF's place (-1 pads);
+// as the energy reports' whole-sample table, n is the sample count.
+struct DirView {
+    int n = 0;
+    float eps = 0.0f;
+    const int* nodes = nullptr;
+    const int8_t* slot = nullptr;
+    const int8_t* kind = nullptr;
+    const float* ref = nullptr;
+    const float* w = nullptr;
+};
+
 struct SweepArgs {
     const float* xin = nullptr;
     float* xout = nullptr;
@@ -80,6 +95,8 @@ struct SweepArgs {
     const int* curv_nodes = nullptr;
     const int8_t* curv_slot = nullptr;
     const float* curv_w = nullptr;
+    // directional soft-energy windows of those DOFs (2D): dir.n per DOF
+    DirView dir;
     float* gout = nullptr;       // gradient-only launches: per-node gradient (plane layout)
     bool gsub = false;           // gradient-only: gout = gradient - gout, in place
     bool gpush = false;          // gradient-only: also write it into the DOF's replica slots
@@ -138,6 +155,14 @@ void launch_copy(float* x, long long plane, int dim, const int* src, const int* 
 void launch_reduce_cells(int dim, int kind, const CellReduceArgs& a, double* partials, hipStream_t s);
 void launch_reduce_samples(int dim, int kind, const SampleReduceArgs& a, double* partials, hipStream_t s);
 void launch_reduce_finish(const double* partials, int nparts, double* out_e, double* out_m, hipStream_t s);
+// The whole-sample directional energies in FP64 at x + alpha_t corr for
+// mode 0: alpha 0 (the energy reports; corr may be null), 1: a0, a1 (the
+// line-search pair), 2: the FAS ladder's slots; written as the partials of
+// blocks pbase, pbase + 1, ... in the layout of that mode's reduction (2,
+// 4 or 2 (kLadder + 1) doubles per block, min det +inf). Returns the block
+// count (at most kReduceBlocks / 2), 0 when the table is empty.
+int launch_dir_reduce(int mode, const DirView& v, const float* x, long long plane, const float* corr, float a0,
+                      float a1, double* partials, int pbase, hipStream_t s);
 
 // Tile shapes the sweep kernels are instantiated for (the autotuner picks one).
 struct TileShape {

@@ -40,10 +40,11 @@ void launch_masked_axpy(float* x, const float* c, float alpha, const uint8_t* fr
 constexpr int kLadder = 10;
 __host__ __device__ inline float ladder_alpha(int t) { return 1.0f / float(1 << (t - 1)); }
 // All trials (and alpha = 0) in one pass plus the pick on the device:
-// partials needs 2 (kLadder + 1) kReduceBlocks / 2 doubles, out 2 (kLadder + 1);
-// *alpha receives the accepted step (0: none).
+// partials needs 2 (kLadder + 1) kReduceBlocks doubles, out 2 (kLadder + 1);
+// *alpha receives the accepted step (0: none). `dir` (nullable): whole-sample
+// directional terms added to every slot's energy.
 void launch_linesearch_ladder(int dim, int kind, const CellReduceArgs& a, const float* corr, double* partials,
-                              double tol, double* out, float* alpha, hipStream_t s);
+                              double tol, double* out, float* alpha, hipStream_t s, const DirView* dir = nullptr);
 // a[i] = b[i] = src[i], i < n (a kernel: no copy-engine node in captured cycles)
 void launch_copy2(float* a, float* b, const float* src, long long n, hipStream_t s);
 // rep[*count] = src[0], rep[cap + *count] = src[1] (when *count < cap); ++*count.
@@ -51,7 +52,8 @@ void launch_append_report(const double* src, double* rep, int cap, int* count, h
 // x += (*alpha) c at free slots; nothing when *alpha == 0.
 void launch_masked_axpy_dev(float* x, const float* c, const float* alpha, const uint8_t* free, long long n,
                             long long plane, int dim, hipStream_t s);
+// partials: 4 kReduceBlocks doubles.
 void launch_linesearch_pair(int dim, int kind, const CellReduceArgs& a, const float* corr, float a0, float a1,
-                            double* partials, double* out, hipStream_t s);
+                            double* partials, double* out, hipStream_t s, const DirView* dir = nullptr);
 
 }  // namespace eggmi

@@ -1,5 +1,6 @@
 // Host side of the MI355X structured smoother (see session.hpp).
 #include "session.hpp"
+#include "directional.hpp"
 #include "mlp_host.hpp"
 
 #include <limits>
@@ -157,6 +158,11 @@ void Level::upload_tables()
     d_curv_nodes.upload(curv_nodes);
     d_curv_slot.upload(curv_slot);
     d_curv_w.upload(curv_w);
+    d_dir_nodes.upload(dir_nodes);
+    d_dir_slot.upload(dir_slot);
+    d_dir_kind.upload(dir_kind);
+    d_dir_ref.upload(dir_ref);
+    d_dir_w.upload(dir_w);
     st_lanes = 0;  // the stored layout follows the tile variant (make_tiles)
     // structured energy cells
     cell_off.assign(lay.shape.size() + 1, 0);
@@ -221,6 +227,8 @@ SweepArgs Level::sweep_args(const float* xin, float* xout, float omega, float de
     a.curv_nodes = d_curv_nodes.p;
     a.curv_slot = d_curv_slot.p;
     a.curv_w = d_curv_w.p;
+    if (!con_tag.empty() && dir_k > 0)
+        a.dir = DirView{dir_k, dir_eps, d_dir_nodes.p, d_dir_slot.p, d_dir_kind.p, d_dir_ref.p, d_dir_w.p};
     a.st.rounds = st_rounds;
     a.st.dof = d_st_dof.p;
     a.st.total = st_padded;
@@ -255,6 +263,9 @@ void check_index_range(const int32_t* v, long long n, long long N, const std::st
 
 // Mirrors the checks of the reference's validate_context (src/bindings.cpp:98-253)
 // for the fields this path consumes.
+void validate_dir(const int32_t* nodes, const int32_t* slot, const int32_t* kind, const double* ref,
+                  const double* wt, double eps, long long n, long long N, const std::string& grp);
+
 void validate(int dim, const eggmi_group* groups, int n_groups, const eggmi_stencil* es, long long N)
 {
     const int dd = dim * dim;
@@ -307,6 +318,13 @@ void validate(int dim, const eggmi_group* groups, int n_groups, const eggmi_sten
                 }
             }
         }
+        if (g.dir_k < 0) throw InvalidArgument("validate_context: " + grp + " negative dir_k");
+        if (g.dir_k > 0) {
+            if (dim != 2) throw InvalidArgument("validate_context: directional terms are 2D only in this build");
+            if (g.dir_k > 64) throw InvalidArgument("validate_context: more than 64 directional samples per DOF");
+            validate_dir(g.dir_nodes, g.dir_slot, g.dir_kind, g.dir_ref, g.dir_weight, g.dir_eps,
+                         (long long)g.ndof * g.dir_k, N, grp);
+        }
         if (g.entity_tag)
             for (long long d = 0; d < g.ndof; ++d) {
                 const int t = g.entity_tag[d];
@@ -333,6 +351,42 @@ void validate(int dim, const eggmi_group* groups, int n_groups, const eggmi_sten
         for (long long i = 0; i < es->num_samples; ++i)
             if (!std::isfinite(es->weight[i]))
                 throw InvalidArgument("validate_context: energy_stencil weight must be finite");
+    if (es->dir_num < 0) throw InvalidArgument("validate_context: energy_stencil negative dir_num");
+    if (es->dir_num > 0) {
+        if (dim != 2) throw InvalidArgument("validate_context: directional terms are 2D only in this build");
+        if (es->dir_num >= (1ll << 31)) throw InvalidArgument("validate_context: more than 2^31 directional samples");
+        validate_dir(es->dir_nodes, nullptr, es->dir_kind, es->dir_ref, es->dir_weight, es->dir_eps, es->dir_num, N,
+                     "energy_stencil");
+    }
+    int gk = -1;  // the groups' directional eps must agree (one per session)
+    for (int gi = 0; gi < n_groups; ++gi)
+        if (groups[gi].dir_k > 0) {
+            if (gk >= 0 && groups[gi].dir_eps != groups[gk].dir_eps)
+                throw InvalidArgument("validate_context: the groups' dir_eps differ");
+            gk = gi;
+        }
+}
+
+// A directional table (reference src/directional.hpp:37-48): the used node
+// slots of a sample of known kind in range, the DOF's slot (per-DOF tables)
+// in [-1, 3] and below the kind's arity, finite reference vector, weight and
+// eps. Samples of other kinds add nothing and are not checked further.
+void validate_dir(const int32_t* nodes, const int32_t* slot, const int32_t* kind, const double* ref,
+                  const double* wt, double eps, long long n, long long N, const std::string& grp)
+{
+    if (!std::isfinite(eps)) throw InvalidArgument("validate_context: " + grp + " dir_eps must be finite");
+    for (long long q = 0; q < n; ++q) {
+        const int sl = slot ? slot[q] : 0;
+        if (sl < -1 || sl > 3) throw InvalidArgument("validate_context: " + grp + " dir_slot outside [-1, 3]");
+        const int ar = dir_arity(kind[q]);
+        if (sl < 0 || ar == 0) continue;
+        if (sl >= ar)
+            throw InvalidArgument("validate_context: " + grp + " dir_slot beyond its kind's " + std::to_string(ar) +
+                                  " nodes");
+        check_index_range(nodes + 4 * q, ar, N, grp, "dir_nodes");
+        if (!std::isfinite(wt[q]) || !std::isfinite(ref[2 * q]) || !std::isfinite(ref[2 * q + 1]))
+            throw InvalidArgument("validate_context: " + grp + " dir_ref / dir_weight must be finite");
+    }
 }
 
 // A weight array that is absent or 1 everywhere leaves the unweighted sums.
@@ -562,6 +616,13 @@ void Session::create(int d, const eggmi_group* groups, int n_groups, const eggmi
     int kmax_curv = 0;  // curvature windows per special DOF (the widest group's)
     for (int gi = 0; gi < n_groups; ++gi) kmax_curv = std::max(kmax_curv, (int)groups[gi].curv_k);
     L.curv_k = kmax_curv;
+    int kmax_dir = 0;  // directional windows per special DOF (the widest group's)
+    for (int gi = 0; gi < n_groups; ++gi)
+        if (groups[gi].dir_k > 0) {
+            kmax_dir = std::max(kmax_dir, (int)groups[gi].dir_k);
+            L.dir_eps = (float)groups[gi].dir_eps;
+        }
+    L.dir_k = kmax_dir;
     bool wt_unit_all = true;  // per-occurrence energy weights all 1 (or absent)
     for (int gi = 0; gi < n_groups; ++gi)
         if (!unit_weights(groups[gi].weight, groups[gi].total_samples)) wt_unit_all = false;
@@ -585,7 +646,8 @@ void Session::create(int d, const eggmi_group* groups, int n_groups, const eggmi
             const bool con = g.entity_tag && g.entity_tag[di] != 0;
             bool curv = false;
             for (int j = 0; j < g.curv_k; ++j) curv = curv || g.curv_slot[di * g.curv_k + j] >= 0;
-            const bool special = con || curv;
+            for (int j = 0; j < g.dir_k; ++j) curv = curv || g.dir_slot[di * g.dir_k + j] >= 0;
+            const bool special = con || curv;  // constrained, or with curvature / directional windows
             if (special != (pass == 1)) {
                 off += g.P_of[di];
                 continue;
@@ -617,8 +679,23 @@ void Session::create(int d, const eggmi_group* groups, int n_groups, const eggmi
                         L.curv_nodes.push_back(live ? g2s[nid] : 0);
                     }
                 }
+                for (int j = 0; j < kmax_dir; ++j) {
+                    const long long q = di * g.dir_k + j;
+                    const int ar = j < g.dir_k ? dir_arity(g.dir_kind[q]) : 0;
+                    const bool live = ar > 0 && g.dir_slot[q] >= 0;
+                    L.dir_slot.push_back(live ? (int8_t)g.dir_slot[q] : (int8_t)-1);
+                    L.dir_kind.push_back(live ? (int8_t)g.dir_kind[q] : (int8_t)-1);
+                    L.dir_w.push_back(live ? (float)g.dir_weight[q] : 0.0f);
+                    for (int k = 0; k < 2; ++k) L.dir_ref.push_back(live ? (float)g.dir_ref[2 * q + k] : 0.0f);
+                    for (int m = 0; m < 4; ++m) {
+                        const int nid = live && m < ar ? g.dir_nodes[4 * q + m] : -1;
+                        if (nid >= 0 && g2s[nid] < 0)
+                            throw InvalidArgument("structured: a directional-sample node is in no block");
+                        L.dir_nodes.push_back(nid >= 0 ? g2s[nid] : 0);
+                    }
+                }
                 if (ib >= 0) {
-                    // a synthesized interior DOF with curvature windows takes the
+                    // a synthesized interior DOF with curvature or directional windows takes the
                     // stored path: its patch expanded here in the reference order
                     // (cell bits outer, corner bits inner, axis 0 most significant;
                     // src/structured_patch.hpp:89-130) with the uniform target
@@ -803,6 +880,21 @@ void Session::create(int d, const eggmi_group* groups, int n_groups, const eggmi
         }
     }
 
+    // ---- the energy reports' whole-sample directional table (owner slots) ----
+    for (long long i = 0; i < es->dir_num; ++i) {
+        const int ar = dir_arity(es->dir_kind[i]);
+        L.es_dir_kind.push_back(ar ? (int8_t)es->dir_kind[i] : (int8_t)-1);
+        L.es_dir_w.push_back(ar ? (float)es->dir_weight[i] : 0.0f);
+        for (int k = 0; k < 2; ++k) L.es_dir_ref.push_back(ar ? (float)es->dir_ref[2 * i + k] : 0.0f);
+        for (int m = 0; m < 4; ++m) {
+            const int nid = m < ar ? es->dir_nodes[4 * i + m] : -1;
+            if (nid >= 0 && g2s[nid] < 0)
+                throw InvalidArgument("structured: a directional-sample node references a global node absent from "
+                                      "every block's interior map");
+            L.es_dir_nodes.push_back(nid >= 0 ? g2s[nid] : 0);
+        }
+    }
+
     // ---- device residency (host validation and remap above touch no device) ----
     if (device == 3) {  // tables only: index maps for inspection, no device state
         host_only = true;
@@ -828,13 +920,21 @@ void Session::create(int d, const eggmi_group* groups, int n_groups, const eggmi
         L.d_es_W.upload(es_W_h);
         L.d_es_wt.upload(es_wt_h);
     }
+    if (es->dir_num > 0) {
+        L.d_es_dir_nodes.upload(L.es_dir_nodes);
+        L.d_es_dir_kind.upload(L.es_dir_kind);
+        L.d_es_dir_ref.upload(L.es_dir_ref);
+        L.d_es_dir_w.upload(L.es_dir_w);
+        L.es_dir = DirView{(int)es->dir_num, (float)es->dir_eps, L.d_es_dir_nodes.p, nullptr, L.d_es_dir_kind.p,
+                           L.d_es_dir_ref.p, L.d_es_dir_w.p};
+    }
     std::vector<float> xs((size_t)L.plane * d, 0.0f);
     for (long long s = 0; s < nslots; ++s)
         for (int k = 0; k < d; ++k) xs[(size_t)k * L.plane + s] = (float)Xp[(size_t)s * d + k];
     L.x[0].upload(xs);
     L.x[1].alloc(xs.size());
     L.cur = 0;
-    partials.alloc(2 * kReduceBlocks);
+    partials.alloc(4 * kReduceBlocks);  // reductions, then the directional table's blocks
 }
 
 void Session::halo(Level& L, float* x)
@@ -868,7 +968,9 @@ void Session::reduce(Level& L, const float* x, int kind, double delta, double* o
         a.delta = delta;
         launch_reduce_samples(dim, kind, a, partials.p, stream);
     }
-    launch_reduce_finish(partials.p, kReduceBlocks, out_e, out_m, stream);
+    // whole-sample directional terms join the reported energy (reference src/sweep.hpp:1921)
+    const int nd = launch_dir_reduce(0, L.es_dir, x, L.plane, nullptr, 0.0f, 0.0f, partials.p, kReduceBlocks, stream);
+    launch_reduce_finish(partials.p, kReduceBlocks + nd, out_e, out_m, stream);
 }
 
 void Session::need_device() const

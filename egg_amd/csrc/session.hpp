@@ -151,6 +151,26 @@ struct Level {
     DArr<int> d_curv_nodes;
     DArr<int8_t> d_curv_slot;
     DArr<float> d_curv_w;
+    // their directional windows (dir_k per special DOF): node slots (4 per
+    // window), the DOF's place (-1 = padding), kind (-1 = none), reference
+    // vector (2), weight; one eps
+    int dir_k = 0;
+    float dir_eps = 0.0f;
+    std::vector<int> dir_nodes;
+    std::vector<int8_t> dir_slot, dir_kind;
+    std::vector<float> dir_ref, dir_w;
+    DArr<int> d_dir_nodes;
+    DArr<int8_t> d_dir_slot, d_dir_kind;
+    DArr<float> d_dir_ref, d_dir_w;
+    // the energy reports' whole-sample directional table (same layout, no
+    // slot) and its device view (n = 0: none)
+    std::vector<int> es_dir_nodes;
+    std::vector<int8_t> es_dir_kind;
+    std::vector<float> es_dir_ref, es_dir_w;
+    DArr<int> d_es_dir_nodes;
+    DArr<int8_t> d_es_dir_kind;
+    DArr<float> d_es_dir_ref, d_es_dir_w;
+    DirView es_dir;
     float Wu[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
     DArr<int> d_st_dof, d_st_gc, d_st_gn;  // device copies in the padded lane layout (see StoredView)
     int st_rounds = 1;

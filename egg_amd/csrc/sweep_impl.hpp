@@ -23,6 +23,7 @@
 // occurrence order (floating-point summation order only).
 #pragma once
 
+#include "directional.hpp"
 #include "mlp.hpp"
 #include "sweep_common.hpp"
 
@@ -330,6 +331,19 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
         }
     }
     const bool curv = ENT && valid && etag == 0 && a.curv_k > 0;
+    // directional windows, on free and constrained special DOFs alike
+    // (reference src/sweep.hpp:1488-1510, :1746-1760), lane l taking l, l + G, ...
+    const bool dirw = ENT && valid && a.dir.n > 0;
+    // positions of window w's nodes, the DOF's own at `sub` when given
+    auto dir_load = [&](long long w, int sl, const float* sub, float (&p)[3][2]) {
+        const int ar = dir_arity(a.dir.kind[w]);
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+                p[m][i] = m >= ar ? 0.0f : (sub && m == sl) ? sub[i] : X[i * PL + a.dir.nodes[4 * w + m]];
+        return ar;
+    };
     Occ o0;
     load(0, o0);
     float g[D] = {}, h[kNH<D>] = {}, e = 0.0f, md = kInf;
@@ -409,6 +423,20 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
                 h[1] += hc * dsx * dsy;
                 h[2] += hc * dsy * dsy + stab;
                 e += wt * r * r;
+            }
+        if (dirw)
+            for (int j = l; j < a.dir.n; j += G) {
+                const long long w = (long long)ci * a.dir.n + j;
+                const int sl = a.dir.slot[w];
+                float p[3][2];
+                if (sl < 0 || !dir_load(w, sl, nullptr, p)) continue;
+                float gd[2] = {g[0], g[1]}, hd[3] = {h[0], h[1], h[2]};
+                dir_accumulate(a.dir.kind[w], p, a.dir.ref + 2 * w, a.dir.w[w], a.dir.eps, sl, gd, hd, e);
+                g[0] = gd[0];
+                g[1] = gd[1];
+                h[0] = hd[0];
+                h[1] = hd[1];
+                h[2] = hd[2];
             }
     }
 #pragma unroll
@@ -490,6 +518,14 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
                     const float r = curv_turn(p[0], p[1], p[2]) - curv_turn(p[1], p[2], p[3]);
                     et += a.curv_w[w] * r * r;
                 }
+            if (dirw)  // the directional windows' energy with the DOF at the trial
+                for (int j = l; j < a.dir.n; j += G) {
+                    const long long w = (long long)ci * a.dir.n + j;
+                    const int sl = a.dir.slot[w];
+                    float p[3][2];
+                    if (sl < 0 || !dir_load(w, sl, tr, p)) continue;
+                    et += dir_energy<float>(a.dir.kind[w], p, a.dir.ref + 2 * w, a.dir.w[w], a.dir.eps);
+                }
         }
         for (int r = 0; r < R; ++r) {
             Occ o = o0;
@@ -527,7 +563,7 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
 }
 
 // The special stored DOFs of a fine-level sweep or gradient pass (2D:
-// constrained ones and those with curvature windows), in a launch of their
+// constrained ones and those with curvature or directional windows), in a launch of their
 // own beside the sweep kernel: both read xin and write disjoint DOF slots.
 template <int KIND, int G, bool WT, bool GRAD>
 __global__ __launch_bounds__(256) void constrained_kernel(const SweepArgs a)

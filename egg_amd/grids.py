@@ -308,6 +308,32 @@ def interface_c2_windows(grid: Grid, weight: float = 1.0) -> tuple[np.ndarray, n
     return nodes, np.full(nodes.shape[0], float(weight))
 
 
+def wall_directional_samples(grid: Grid, weight: float = 1.0, eps: float = 1e-12):
+    """Directional samples on a 2D single block (the reference's three kinds,
+    egg/smoothing/directional.py:166-301, placed by hand): the first row of
+    nodes above the j = 0 wall as a chain of parallel segments with the
+    wall's normal (0, 1) as reference; every third node of that row a stem to
+    the next row up with the through axis (1, 0); the middle row as line
+    triples. Returns a ``DirectionalSamples``."""
+    from egg_amd.smoothing.directional import KIND_LINE, KIND_PARALLEL, KIND_STEM, DirectionalSamples
+
+    ids = np.asarray(grid.blocks[0])
+    n = ids.shape[0]
+    rows = []
+    for i in range(n - 1):
+        rows.append((int(ids[i, 1]), int(ids[i + 1, 1]), -1, KIND_PARALLEL, (0.0, 1.0)))
+    for i in range(2, n - 2, 3):
+        rows.append((int(ids[i, 2]), int(ids[i, 1]), -1, KIND_STEM, (1.0, 0.0)))
+    m = ids.shape[1] // 2
+    for i in range(1, n - 1, 2):
+        rows.append((int(ids[i - 1, m]), int(ids[i, m]), int(ids[i + 1, m]), KIND_LINE, (0.0, 0.0)))
+    nodes = np.full((len(rows), 4), -1, np.int32)
+    nodes[:, :3] = [r[:3] for r in rows]
+    k = np.asarray([r[3] for r in rows], np.int32)
+    ref = np.asarray([r[4] for r in rows], np.float64)
+    return DirectionalSamples(nodes, k, ref, np.zeros_like(ref), np.full(len(rows), float(weight)), eps)
+
+
 def local_edge_length(grid: Grid) -> np.ndarray:
     """Shortest incident block edge per node."""
     h = np.full(grid.n_nodes, np.inf)

@@ -17,12 +17,15 @@ consumes, with the field names and conventions of the reference's
   neighbour, -1 present but neither).
 
 Only the identity-family targets and plain shape samples are supported here
-(no geometry entities, interface, curvature or directional terms).
+(the segment, circle, ellipse and rail entities, curvature windows and
+directional terms are supported; no other geometry entities).
 """
 
 from __future__ import annotations
 
 import numpy as np
+
+from .directional import directional_dof_table, directional_energy_table
 
 
 def _bit(d: int, k: int) -> int:
@@ -128,7 +131,8 @@ def curvature_table(free_mask: np.ndarray, nodes: np.ndarray, weight: np.ndarray
 
 def build_flat_context(blocks: list[np.ndarray], free_mask: np.ndarray, d: int, *,
                        w_inv: np.ndarray | None = None, weight: np.ndarray | None = None,
-                       entities: dict | None = None, curvature: tuple | None = None) -> dict:
+                       entities: dict | None = None, curvature: tuple | None = None,
+                       directional=None) -> dict:
     """Assemble the flat wire for the moving DOFs ``free_mask`` of ``blocks``.
 
     ``w_inv`` is one uniform ``(d, d)`` inverse target (identity when
@@ -142,6 +146,9 @@ def build_flat_context(blocks: list[np.ndarray], free_mask: np.ndarray, d: int, 
     the group's ``entities`` field carries the per-type records.
     ``curvature`` = (nodes (W, 4), weight (W,)) adds block-interface C2
     curvature windows (2D; reference egg/smoothing/flat_context.py:293-304).
+    ``directional`` (a ``DirectionalSamples``) adds directional soft-energy
+    terms: per-DOF windows on the group, the whole-sample table on the energy
+    stencil (2D in the compute core; reference flat_context.py:305-320).
     """
     st = cell_stencil(blocks, d)
     ns = st["ns"]
@@ -212,7 +219,9 @@ def build_flat_context(blocks: list[np.ndarray], free_mask: np.ndarray, d: int, 
             if d != 2:
                 raise ValueError("curvature windows are 2D only")
             g.update(curvature_table(free_mask, *curvature))
+        g.update(directional_dof_table(free_mask, directional))
         groups.append(g)
     es = {"num_samples": int(ns)}
     es.update(fields(np.arange(ns)))
+    es.update(directional_energy_table(directional))
     return {"groups": groups, "energy_stencil": es}

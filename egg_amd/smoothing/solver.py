@@ -27,11 +27,11 @@ class SweepContext:
 
 
 def build_sweep_context(blocks: list, free_mask: np.ndarray, d: int, *, w_inv=None, weight=None,
-                        entities=None, curvature=None) -> SweepContext:
+                        entities=None, curvature=None, directional=None) -> SweepContext:
     """Context over ``blocks`` (per-block global-id arrays) with moving DOFs
-    ``free_mask``; ``w_inv`` / ``weight`` / ``entities`` / ``curvature`` as in
+    ``free_mask``; ``w_inv`` / ``weight`` / ``entities`` / ``curvature`` / ``directional`` as in
     :func:`build_flat_context`."""
     blocks = [np.asarray(b) for b in blocks]
     wire = build_flat_context(blocks, np.asarray(free_mask, bool), d, w_inv=w_inv, weight=weight,
-                              entities=entities, curvature=curvature)
+                              entities=entities, curvature=curvature, directional=directional)
     return SweepContext(d=d, wire=wire, block_global_dof=blocks, free_mask=np.asarray(free_mask, bool))

@@ -25,6 +25,8 @@
  *                 and min are accumulated in double (FP64 reductions)
  *   o_jacobi      src/sweep.hpp:1425-1650 (metric + interior update kernels),
  *                 :1938-2099 (jacobi_sweep_once / run_block_jacobi)
+ *   dir_*         src/directional.hpp (directional soft-energy terms), NumPy
+ *                 twin egg/smoothing/directional.py:304-369
  *
  * Pinning: golden vectors from tests/cpp/golden_metric.hpp,
  * golden_metric_3d.hpp and golden_patch.hpp (tests/golden/*.json), finite
@@ -432,14 +434,184 @@ void FN(o_solve)(int dim, const real* H, const real* g, real* x)
     }
 }
 
+/* ------------------------------------- directional soft-energy terms */
+
+/* Kinds (reference src/directional.hpp:9-27, NumPy twin
+ * egg/smoothing/directional.py:317-369): 0 parallel (nodes i, j), E = w c^2
+ * with c = (u.ref)/s, u = X[j] - X[i], s = sqrt(|u|^2 + eps^2); 2 stem (nodes
+ * n1, c), the same with u = X[n1] - X[c]; 1 line (nodes a, c, b),
+ * E = w |q1/s1 + q2/s2|^2 with qk = X[a|b] - X[c], sk = sqrt(|qk|^2 + eps^2).
+ * Any other kind contributes nothing. One table type serves the per-DOF
+ * windows (K per DOF, slot = the DOF's place, -1 pads) and the whole-sample
+ * table of the energy reports (num samples, slot unused). */
+#ifndef EGG_ORACLE_DIR_T
+#define EGG_ORACLE_DIR_T
+typedef struct {
+    int K;             /* per-DOF windows per DOF (0 = none) */
+    long long num;     /* whole-sample table length */
+    const int* nodes;  /* [.. * 4] global node ids, -1 pads */
+    const int* slot;   /* [..] the DOF's slot in the sample, -1 = pad */
+    const int* kind;   /* [..] */
+    const double* ref; /* [.. * dim] frozen reference vector */
+    const double* wt;  /* [..] weight */
+    double eps;
+} dir_t;
+static int dir_arity(int kind) { return kind == 1 ? 3 : (kind == 0 || kind == 2) ? 2 : 0; }
+#endif
+
+/* Energy of one sample at positions p[0..arity-1] (src/directional.hpp:95-124). */
+static real FN(dir_energy)(int dim, int kind, const real p[3][3], const double* ref, real w, real eps)
+{
+    if (kind == 0 || kind == 2) {
+        const int head = kind == 0 ? 1 : 0;
+        real ss = eps * eps, dot = 0;
+        for (int k = 0; k < dim; ++k) {
+            const real u = p[head][k] - p[1 - head][k];
+            ss += u * u;
+            dot += u * (real)ref[k];
+        }
+        const real c = dot / (real)sqrt(ss);
+        return w * c * c;
+    }
+    if (kind == 1) {
+        real s1 = eps * eps, s2 = eps * eps;
+        for (int k = 0; k < dim; ++k) {
+            const real a = p[0][k] - p[1][k], b = p[2][k] - p[1][k];
+            s1 += a * a;
+            s2 += b * b;
+        }
+        s1 = (real)sqrt(s1);
+        s2 = (real)sqrt(s2);
+        real e = 0;
+        for (int k = 0; k < dim; ++k) {
+            const real r = (p[0][k] - p[1][k]) / s1 + (p[2][k] - p[1][k]) / s2;
+            e += r * r;
+        }
+        return w * e;
+    }
+    return 0;
+}
+
+/* Gradient, Gauss-Newton Hessian (row-major dim x dim) and energy of one
+ * sample for the node at `slot`, added in, with the coupling stabiliser on
+ * the diagonal (src/directional.hpp:126-211): parallel / stem add
+ * 2w |dr|^2 I; line adds 2w |J_slot|_F (sum of the other two |J|_F) I. */
+static void FN(dir_accumulate)(int dim, int kind, const real p[3][3], const double* ref, real w, real eps, int slot,
+                               real* grad, real* hess, real* e0)
+{
+    if (kind == 0 || kind == 2) {
+        const int head = kind == 0 ? 1 : 0;
+        real u[3], ss = eps * eps, dot = 0;
+        for (int k = 0; k < dim; ++k) {
+            u[k] = p[head][k] - p[1 - head][k];
+            ss += u[k] * u[k];
+            dot += u[k] * (real)ref[k];
+        }
+        const real sn = (real)sqrt(ss), c = dot / sn;
+        real dr[3], dd = 0;
+        for (int k = 0; k < dim; ++k) {
+            dr[k] = (real)ref[k] / sn - c * u[k] / ss; /* dc/du */
+            dd += dr[k] * dr[k];
+        }
+        *e0 += w * c * c;
+        const real gc = 2 * w * c * (slot == head ? (real)1 : (real)-1), hc = 2 * w;
+        for (int i = 0; i < dim; ++i) {
+            grad[i] += gc * dr[i];
+            for (int j = 0; j < dim; ++j) hess[i * dim + j] += hc * dr[i] * dr[j];
+            hess[i * dim + i] += hc * dd;
+        }
+        return;
+    }
+    if (kind != 1) return;
+    /* unit vectors h_k = q_k / s_k and their Jacobians (I - q q^T / s^2) / s */
+    real q[2][3], h[2][3], J[2][9], sn[2];
+    for (int t = 0; t < 2; ++t) {
+        real ss = eps * eps;
+        for (int k = 0; k < dim; ++k) {
+            q[t][k] = p[2 * t][k] - p[1][k];
+            ss += q[t][k] * q[t][k];
+        }
+        sn[t] = (real)sqrt(ss);
+        for (int k = 0; k < dim; ++k) h[t][k] = q[t][k] / sn[t];
+        for (int i = 0; i < dim; ++i)
+            for (int j = 0; j < dim; ++j)
+                J[t][i * dim + j] = ((i == j) ? (real)1 / sn[t] : (real)0) - q[t][i] * q[t][j] / (ss * sn[t]);
+    }
+    real r[3], e = 0;
+    for (int k = 0; k < dim; ++k) {
+        r[k] = h[0][k] + h[1][k];
+        e += r[k] * r[k];
+    }
+    *e0 += w * e;
+    real Js[9], f[3] = {0, 0, 0}; /* |J_a|, |J_b|, |J_a + J_b| */
+    for (int k = 0; k < dim * dim; ++k) {
+        Js[k] = slot == 0 ? J[0][k] : slot == 2 ? J[1][k] : -(J[0][k] + J[1][k]);
+        f[0] += J[0][k] * J[0][k];
+        f[1] += J[1][k] * J[1][k];
+        f[2] += (J[0][k] + J[1][k]) * (J[0][k] + J[1][k]);
+    }
+    for (int t = 0; t < 3; ++t) f[t] = (real)sqrt(f[t]);
+    const real fs = slot == 0 ? f[0] : slot == 2 ? f[1] : f[2];
+    const real gc = 2 * w, stab = gc * fs * (f[0] + f[1] + f[2] - fs);
+    for (int i = 0; i < dim; ++i) {
+        real gi = 0;
+        for (int j = 0; j < dim; ++j) gi += Js[i * dim + j] * r[j];
+        grad[i] += gc * gi;
+        for (int j = 0; j < dim; ++j) {
+            real hij = 0;
+            for (int m = 0; m < dim; ++m) hij += Js[i * dim + m] * Js[j * dim + m];
+            hess[i * dim + j] += gc * hij;
+        }
+        hess[i * dim + i] += stab;
+    }
+}
+
+/* Sample positions of window j of per-DOF table row d (DOF at `trial` when given). */
+static int FN(dir_load)(int dim, const dir_t* dw, long long w, const real* X, int slot, const real* trial,
+                        real p[3][3])
+{
+    const int ar = dir_arity(dw->kind[w]);
+    for (int m = 0; m < ar; ++m)
+        for (int k = 0; k < dim; ++k)
+            p[m][k] = (trial && m == slot) ? trial[k] : X[(size_t)dw->nodes[4 * w + m] * dim + k];
+    return ar;
+}
+
+/* A DOF's windows into (grad, hess, e0) (src/directional.hpp:225-250). */
+static void FN(dir_dof_accumulate)(int dim, const dir_t* dw, int d, const real* X, real* grad, real* hess, real* e0)
+{
+    for (int j = 0; j < dw->K; ++j) {
+        const long long w = (long long)d * dw->K + j;
+        const int sl = dw->slot[w];
+        real p[3][3];
+        if (sl < 0 || !FN(dir_load)(dim, dw, w, X, sl, NULL, p)) continue;
+        FN(dir_accumulate)(dim, dw->kind[w], p, dw->ref + (size_t)dim * w, (real)dw->wt[w], (real)dw->eps, sl, grad,
+                           hess, e0);
+    }
+}
+
+/* A DOF's windows' energy with the DOF moved to `trial` (src/directional.hpp:252-277). */
+static real FN(dir_dof_trial)(int dim, const dir_t* dw, int d, const real* X, const real* trial)
+{
+    real e = 0;
+    for (int j = 0; j < dw->K; ++j) {
+        const long long w = (long long)d * dw->K + j;
+        const int sl = dw->slot[w];
+        real p[3][3];
+        if (sl < 0 || !FN(dir_load)(dim, dw, w, X, sl, trial, p)) continue;
+        e += FN(dir_energy)(dim, dw->kind[w], p, dw->ref + (size_t)dim * w, (real)dw->wt[w], (real)dw->eps);
+    }
+    return e;
+}
+
 /* ------------------------------------------------------------ reductions */
 
 /* Sum of mu and min det A over every energy-stencil sample
  * (reference src/sweep.hpp:1881-1922). mu and det are evaluated in double
  * from the REAL positions and accumulated in double. */
 void FN(o_energy_mindet)(int dim, int kind, double delta, long long ns, const int* gc, const int* gn,
-                         const real* s, const real* W, int w_stride, const real* wt, const real* X, double* E,
-                         double* md)
+                         const real* s, const real* W, int w_stride, const real* wt, const real* X,
+                         const dir_t* dirs, double* E, double* md)
 {
     double e = 0.0, m = INFINITY;
     for (long long i = 0; i < ns; ++i) {
@@ -468,6 +640,15 @@ void FN(o_energy_mindet)(int dim, int kind, double delta, long long ns, const in
         }
         e += (wt ? (double)wt[i] : 1.0) * val;
         if (det < m) m = det;
+    }
+    /* the whole-sample directional table, evaluated in double from the REAL
+       positions, joins the reported energy (src/sweep.hpp:1921) */
+    for (long long i = 0; dirs && i < dirs->num; ++i) {
+        const int ar = dir_arity(dirs->kind[i]);
+        double p[3][3];
+        for (int m2 = 0; m2 < ar; ++m2)
+            for (int k = 0; k < dim; ++k) p[m2][k] = (double)X[(size_t)dirs->nodes[4 * i + m2] * dim + k];
+        e += dir_energy_f64(dim, dirs->kind[i], p, dirs->ref + (size_t)dim * i, dirs->wt[i], dirs->eps);
     }
     *E = e;
     *md = m;
@@ -557,10 +738,12 @@ static void FN(ent_tangent)(int tag, const real* e, const real* p, real* t)
  * damped by omega on the step (no post-blend); trials pos + alpha delta are
  * projected onto the entity; the same accept rule as the free DOFs. */
 static void FN(update_constrained)(int kind, real delta, real omega, const FN(patch_t)* pv, const real* X,
-                                   int dof, int tag, const real* ent, real* out)
+                                   int dof, int tag, const real* ent, const dir_t* dw, int d, real* out)
 {
     real grad[3], hess[9], e0, md;
     FN(patch_eval_impl)(2, kind, delta, pv, X, grad, hess, &e0, &md);
+    if (dw) /* directional windows (src/sweep.hpp:1746-1760) */
+        FN(dir_dof_accumulate)(2, dw, d, X, grad, hess, &e0);
     const real pos[2] = {X[(size_t)dof * 2], X[(size_t)dof * 2 + 1]};
     real dl[2];
     const real ft = tag == ENT_RAIL ? FN(rail_foot)(ent, pos) : 0;
@@ -591,6 +774,7 @@ static void FN(update_constrained)(int kind, real delta, real omega, const FN(pa
         real trial[2], e_new, mdet;
         FN(ent_project)(tag, ent, raw, trial);
         FN(trial_impl)(2, kind, delta, pv, X, dof, trial, &e_new, &mdet);
+        if (dw) e_new += FN(dir_dof_trial)(2, dw, d, X, trial);
         const int ok = isfinite(e_new) && (e_new <= e0 + TOL_ENERGY) && (kind == KIND_UNTANGLE || mdet > 0);
         if (ok) {
             cur[0] = trial[0];
@@ -696,12 +880,14 @@ typedef struct {
 /* One DOF's block-Jacobi update: metric pass, Newton, <=10 halvings, omega
  * post-blend (reference src/sweep.hpp:1534-1650). X is the frozen snapshot. */
 static void FN(update_dof)(int dim, int kind, real delta, real omega, const FN(patch_t)* pv, const real* X, int dof,
-                       const real* tau, const FN(curv_t)* cv, real* out)
+                       const real* tau, const FN(curv_t)* cv, const dir_t* dw, int d, real* out)
 {
     real grad[3], hess[9], e0, md;
     FN(patch_eval_impl)(dim, kind, delta, pv, X, grad, hess, &e0, &md);
     if (cv) /* curvature windows on top of the shape metric (src/sweep.hpp:1461-1487) */
         FN(curv_accumulate)(cv->K, cv->nodes, cv->slot, cv->wt, X, grad, hess, &e0);
+    if (dw) /* directional windows (src/sweep.hpp:1488-1510) */
+        FN(dir_dof_accumulate)(dim, dw, d, X, grad, hess, &e0);
     real pos[3];
     for (int k = 0; k < dim; ++k) pos[k] = X[(size_t)dof * dim + k];
     if (tau) {
@@ -719,6 +905,7 @@ static void FN(update_dof)(int dim, int kind, real delta, real omega, const FN(p
         for (int k = 0; k < dim; ++k) trial[k] = pos[k] + alpha * dlt[k];
         FN(trial_impl)(dim, kind, delta, pv, X, dof, trial, &e_new, &mdet);
         if (cv) e_new += FN(curv_trial)(cv->K, cv->nodes, cv->slot, cv->wt, X, trial);
+        if (dw) e_new += FN(dir_dof_trial)(dim, dw, d, X, trial);
         if (tau)
             for (int k = 0; k < dim; ++k) e_new -= tau[(size_t)dof * dim + k] * trial[k];
         const int ok = isfinite(e_new) && (e_new <= e0 + TOL_ENERGY) && (kind == KIND_UNTANGLE || mdet > 0);
@@ -746,7 +933,7 @@ int FN(o_jacobi)(int dim, int kind, double delta, double omega, int n_sweeps, in
                  const int* dof, const long long* poff, const int* pgc, const int* pgn, const real* ps,
                  const int* prole, const real* W, int w_stride, const real* pwt, const int* etag,
                  const real* erec, int curv_k, const int* cnodes, const int* cslot, const real* cwt,
-                 int n_nodes, real* X, const real* tau, long long ns, const int* egc,
+                 const dir_t* dw, const dir_t* dirs, int n_nodes, real* X, const real* tau, long long ns, const int* egc,
                  const int* egn, const real* es, const real* eW, int ew_stride, const real* ewt, real* work,
                  double* energies, double* mindets)
 {
@@ -773,16 +960,16 @@ int FN(o_jacobi)(int dim, int kind, double delta, double omega, int n_sweeps, in
                               pwt ? pwt + o : NULL};
             if (etag && etag[d])  /* constrained boundary DOF (2D) */
                 FN(update_constrained)(kind, (real)delta, (real)omega, &pv, xin, dof[d], etag[d], erec + 4 * d,
-                                       xout + (size_t)dof[d] * dim);
+                                       dw && dw->K ? dw : NULL, d, xout + (size_t)dof[d] * dim);
             else {
                 FN(curv_t) cv = {curv_k, cnodes ? cnodes + (size_t)4 * curv_k * d : NULL,
                                  cslot ? cslot + (size_t)curv_k * d : NULL, cwt ? cwt + (size_t)curv_k * d : NULL};
                 FN(update_dof)(dim, kind, (real)delta, (real)omega, &pv, xin, dof[d], tau,
-                               curv_k > 0 ? &cv : NULL, xout + (size_t)dof[d] * dim);
+                               curv_k > 0 ? &cv : NULL, dw && dw->K ? dw : NULL, d, xout + (size_t)dof[d] * dim);
             }
         }
         if ((sw + 1) % k == 0 || sw == n_sweeps - 1) {
-            FN(o_energy_mindet)(dim, kind, delta, ns, egc, egn, es, eW, ew_stride, ewt, xout, &energies[nrep],
+            FN(o_energy_mindet)(dim, kind, delta, ns, egc, egn, es, eW, ew_stride, ewt, xout, dirs, &energies[nrep],
                                 &mindets[nrep]);
             ++nrep;
         }
@@ -800,7 +987,7 @@ int FN(o_jacobi)(int dim, int kind, double delta, double omega, int n_sweeps, in
 void FN(o_patch_grads)(int dim, int kind, double delta, int ndof, const int* dof, const long long* poff,
                        const int* pgc, const int* pgn, const real* ps, const int* prole, const real* W,
                        int w_stride, const real* pwt, int curv_k, const int* cnodes, const int* cslot,
-                       const real* cwt, const real* X, const real* tau, real* gout)
+                       const real* cwt, const dir_t* dw, const real* X, const real* tau, real* gout)
 {
     const long long total = poff[ndof];
 #pragma omp parallel for schedule(static)
@@ -821,6 +1008,7 @@ void FN(o_patch_grads)(int dim, int kind, double delta, int ndof, const int* dof
         if (curv_k > 0)
             FN(curv_accumulate)(curv_k, cnodes + (size_t)4 * curv_k * d, cslot + (size_t)curv_k * d,
                                 cwt + (size_t)curv_k * d, X, grad, hess, &e0);
+        if (dw && dw->K) FN(dir_dof_accumulate)(dim, dw, d, X, grad, hess, &e0);
         for (int k = 0; k < dim; ++k)
             gout[(size_t)dof[d] * dim + k] = grad[k] - (tau ? tau[(size_t)dof[d] * dim + k] : (real)0);
     }

@@ -62,19 +62,19 @@ def lib():
             f.restype = None
             f = getattr(L, "o_energy_mindet" + sfx)
             f.argtypes = [C.c_int, C.c_int, C.c_double, C.c_longlong, _ip, _ip, rp, rp, C.c_int, C.c_void_p, rp,
-                          _dp, _dp]
+                          C.c_void_p, _dp, _dp]
             f.restype = None
             f = getattr(L, "o_jacobi" + sfx)
             f.argtypes = [
                 C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
                 _ip, _lp, _ip, _ip, rp, _ip, rp, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
-                C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, rp, C.c_void_p,
+                C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, rp, C.c_void_p,
                 C.c_longlong, _ip, _ip, rp, rp, C.c_int, C.c_void_p, rp, _dp, _dp,
             ]
             f.restype = C.c_int
             f = getattr(L, "o_patch_grads" + sfx)
             f.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, _ip, _lp, _ip, _ip, rp, _ip, rp, C.c_int,
-                          C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, rp, C.c_void_p, rp]
+                          C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, rp, C.c_void_p, rp]
             f.restype = None
         L.o_set_threads.argtypes = [C.c_int]
         L.o_set_threads.restype = None
@@ -94,6 +94,35 @@ def _weights(w, dt):
 
 def _ptr(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class _DirT(C.Structure):
+    """The C oracle's directional table (dir_t)."""
+
+    _fields_ = [("K", C.c_int), ("num", C.c_longlong), ("nodes", C.c_void_p), ("slot", C.c_void_p),
+                ("kind", C.c_void_p), ("ref", C.c_void_p), ("wt", C.c_void_p), ("eps", C.c_double)]
+
+
+def _dir_table(t, dim, per_dof):
+    """A wire directional table (per-DOF "dir_k" windows, or the energy
+    stencil's whole-sample "dir_num" table) as a (_DirT, arrays) pair, or
+    (None, None) when absent. Node ids are global."""
+    key = "dir_k" if per_dof else "dir_num"
+    if t is None or int(t.get(key, 0) or 0) == 0:
+        return None, None
+    arrs = [np.ascontiguousarray(np.asarray(t["dir_nodes"], np.int32).reshape(-1)),
+            np.ascontiguousarray(np.asarray(t["dir_slot"], np.int32).reshape(-1)) if per_dof else None,
+            np.ascontiguousarray(np.asarray(t["dir_kind"], np.int32).reshape(-1)),
+            np.ascontiguousarray(np.asarray(t["dir_ref"], np.float64).reshape(-1)),
+            np.ascontiguousarray(np.asarray(t["dir_weight"], np.float64).reshape(-1))]
+    n = int(t[key])
+    st = _DirT(n if per_dof else 0, 0 if per_dof else n, *(_ptr(a) for a in arrs), float(t["dir_eps"]))
+    assert arrs[2].size == (len(arrs[4])) and arrs[3].size == dim * arrs[2].size
+    return st, arrs
+
+
+def _byref(st):
+    return None if st is None else C.cast(C.pointer(st), C.c_void_p)
 
 
 def metric(dim: int, kind: str | int, t, delta: float = 0.0, prec: str = "f64"):
@@ -157,11 +186,12 @@ def energy_mindet(dim, kind, delta, es, X, prec="f64"):
     in double (the stencil's optional "weight" scales mu, not det)."""
     dt = _dt(prec)
     ns, gc, gn, s, W, ws, wt = _stencil_arrays(es, dim, dt)
+    dts, keep = _dir_table(es, dim, False)
     E = np.zeros(1)
     m = np.zeros(1)
     getattr(lib(), f"o_energy_mindet_{prec}")(
         dim, KIND[kind], float(delta), ns, gc, gn, s, W, ws, _ptr(wt),
-        np.ascontiguousarray(X, dt).reshape(-1), E, m,
+        np.ascontiguousarray(X, dt).reshape(-1), _byref(dts), E, m,
     )
     return float(E[0]), float(m[0])
 
@@ -192,7 +222,7 @@ def interior_occurrences(dim: int, logical, idx):
 class Patches:
     """Per-DOF patches in global id space (CSR over DOFs)."""
 
-    def __init__(self, dim, dof, poff, gc, gn, s, role, W, wt=None, etag=None, erec=None, curv=None):
+    def __init__(self, dim, dof, poff, gc, gn, s, role, W, wt=None, etag=None, erec=None, curv=None, dirw=None):
         self.dim = dim
         self.dof = np.ascontiguousarray(dof, np.int32)
         self.poff = np.ascontiguousarray(poff, np.int64)
@@ -207,6 +237,9 @@ class Patches:
         self.erec = None if erec is None else np.ascontiguousarray(erec, np.float64).reshape(-1, 4)
         # per-DOF curvature windows (K, nodes (ndof, K, 4), slot (ndof, K), weight (ndof, K)) or None
         self.curv = curv
+        # per-DOF directional windows (a wire dict with dir_k, dir_eps, dir_nodes,
+        # dir_slot, dir_kind, dir_ref, dir_weight rows in this DOF order) or None
+        self.dirw = dirw
 
 
 def materialize_patches(wire: dict, block_global_dof: list[np.ndarray], dim: int, dofs=None) -> Patches:
@@ -294,7 +327,16 @@ def materialize_patches(wire: dict, block_global_dof: list[np.ndarray], dim: int
         curv = (K, np.ascontiguousarray(np.asarray(g["curv_nodes"], np.int32).reshape(-1, K, 4)[sel]),
                 np.ascontiguousarray(np.asarray(g["curv_slot"], np.int32).reshape(-1, K)[sel]),
                 np.ascontiguousarray(np.asarray(g["curv_weight"], np.float64).reshape(-1, K)[sel]))
-    return Patches(dim, dof, poff, gc, gn, sg, role, Wp, wt, etag, erec, curv)
+    dirw = None
+    K = int(g.get("dir_k", 0) or 0)
+    if K:
+        dirw = {"dir_k": K, "dir_eps": float(g["dir_eps"]),
+                "dir_nodes": np.asarray(g["dir_nodes"], np.int32).reshape(-1, K, 4)[sel],
+                "dir_slot": np.asarray(g["dir_slot"], np.int32).reshape(-1, K)[sel],
+                "dir_kind": np.asarray(g["dir_kind"], np.int32).reshape(-1, K)[sel],
+                "dir_ref": np.asarray(g["dir_ref"], np.float64).reshape(-1, K, dim)[sel],
+                "dir_weight": np.asarray(g["dir_weight"], np.float64).reshape(-1, K)[sel]}
+    return Patches(dim, dof, poff, gc, gn, sg, role, Wp, wt, etag, erec, curv, dirw)
 
 
 def _entity_rows(g: dict, ndof: int):
@@ -374,12 +416,14 @@ def jacobi(dim, kind, delta, omega, n_sweeps, X, patches: Patches, es: dict, *, 
     etag = patches.etag
     erec = None if patches.erec is None else np.ascontiguousarray(patches.erec, dt).reshape(-1)
     ck, cn, cs, cw = _curv_args(patches, dt)
+    dw, keep_w = _dir_table(patches.dirw, dim, True)
+    dts, keep_s = _dir_table(es, dim, False)
     tau_arr = None if tau is None else np.ascontiguousarray(tau, dt).reshape(-1)
     got = getattr(lib(), f"o_jacobi_{prec}")(
         dim, KIND[kind], float(delta), float(omega), int(n_sweeps), int(report_every),
         len(patches.dof), patches.dof, patches.poff, patches.gc, patches.gn.reshape(-1),
         np.ascontiguousarray(patches.s, dt).reshape(-1), patches.role, W, ws, _ptr(pwt), _ptr(etag), _ptr(erec),
-        ck, _ptr(cn), _ptr(cs), _ptr(cw), n_nodes, Xw, _ptr(tau_arr), ns, egc, egn, es_, eW, ews, _ptr(ewt),
+        ck, _ptr(cn), _ptr(cs), _ptr(cw), _byref(dw), _byref(dts), n_nodes, Xw, _ptr(tau_arr), ns, egc, egn, es_, eW, ews, _ptr(ewt),
         work, en, md,
     )
     return Xw.reshape(-1, dim), en[:got], md[:got]
@@ -402,10 +446,11 @@ def patch_grads(dim, kind, X, patches: Patches, *, delta=0.0, tau=None, prec="f6
     ws = 0 if W.size == dim * dim else dim * dim
     pwt = _weights(patches.wt, dt)
     ck, cn, cs, cw = _curv_args(patches, dt)
+    dw, keep = _dir_table(patches.dirw, dim, True)
     tau_arr = None if tau is None else np.ascontiguousarray(tau, dt).reshape(-1)
     getattr(lib(), f"o_patch_grads_{prec}")(
         dim, KIND[kind], float(delta), len(patches.dof), patches.dof, patches.poff, patches.gc,
         patches.gn.reshape(-1), np.ascontiguousarray(patches.s, dt).reshape(-1), patches.role, W, ws,
-        _ptr(pwt), ck, _ptr(cn), _ptr(cs), _ptr(cw), Xw, _ptr(tau_arr), out,
+        _ptr(pwt), ck, _ptr(cn), _ptr(cs), _ptr(cw), _byref(dw), Xw, _ptr(tau_arr), out,
     )
     return out.reshape(-1, dim)

@@ -9,10 +9,10 @@ from egg_amd.smoothing import build_structured_context_from_block_maps, build_sw
 
 
 class Case:
-    def __init__(self, grid, w_inv=None, weight=None, entities=None, curvature=None):
+    def __init__(self, grid, w_inv=None, weight=None, entities=None, curvature=None, directional=None):
         self.grid = grid
         self.ctx = build_sweep_context(grid.blocks, grid.free, grid.d, w_inv=w_inv, weight=weight,
-                                       entities=entities, curvature=curvature)
+                                       entities=entities, curvature=curvature, directional=directional)
         self.bsc = build_structured_context_from_block_maps(grid.d, grid.blocks)
 
     @property

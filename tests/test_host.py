@@ -274,3 +274,53 @@ def test_boundary_layer_target_profile_and_order():
                 rows = W[4 * (4 * i + j): 4 * (4 * i + j) + 4]
                 np.testing.assert_array_equal(rows, np.broadcast_to(np.diag([1.0, 1.0 / min(0.25 * 2**k, 1.0)]),
                                                                     (4, 2, 2)))
+
+
+def test_directional_eval_binding(oracle):
+    """cpp_core.directional_eval (host side; reference src/bindings.cpp:1819)
+    against the oracle's total and central differences of its own energy;
+    a reserved kind is refused as in the reference."""
+    from egg_amd import grids
+    from egg_amd._cpp import cpp_core
+    from tests.test_oracle import _dir_only_stencil
+
+    g = grids.single_block_2d(9, seed=4, amp=0.25)
+    s = grids.wall_directional_samples(g, weight=1.3, eps=1e-2)
+    e, gr = cpp_core.directional_eval(g.X, s.nodes, s.kind, s.ref, s.weight, s.eps)
+    E, _ = oracle.energy_mindet(2, "shape", 0.0, _dir_only_stencil(s), g.X)
+    assert e == pytest.approx(E, rel=1e-12)
+    h = 1e-6
+    for node in np.unique(s.nodes[s.nodes >= 0])[::3]:
+        for k in range(2):
+            Xp, Xm = g.X.copy(), g.X.copy()
+            Xp[node, k] += h
+            Xm[node, k] -= h
+            fd = (cpp_core.directional_eval(Xp, s.nodes, s.kind, s.ref, s.weight, s.eps)[0] -
+                  cpp_core.directional_eval(Xm, s.nodes, s.kind, s.ref, s.weight, s.eps)[0]) / (2 * h)
+            assert gr[node, k] == pytest.approx(fd, rel=1e-5, abs=1e-8)
+    kind = s.kind.copy()
+    kind[0] = 3
+    with pytest.raises(ValueError, match="unsupported sample kind"):
+        cpp_core.directional_eval(g.X, s.nodes, kind, s.ref, s.weight, s.eps)
+
+
+def test_directional_wire_validation():
+    """The session refuses a directional table it cannot run: 3D, a slot
+    beyond its kind's nodes, a node outside the mesh, a non-finite weight."""
+    from egg_amd import grids
+    from tests.cases import Case
+
+    g = grids.single_block_2d(9, seed=4, amp=0.2)
+    c = Case(g, directional=grids.wall_directional_samples(g))
+    info, _ = _tables(c)  # a valid wire passes; its windowed DOFs are the special ones
+    assert info["n_stored"] > 0
+    for field, i, val, msg in [("dir_slot", 0, 2, "beyond its kind"), ("dir_nodes", 1, 10**6, "dir_nodes"),
+                               ("dir_weight", 0, np.nan, "finite")]:
+        c2 = Case(g, directional=grids.wall_directional_samples(g))
+        grp = c2.ctx.wire["groups"][0]
+        live = np.flatnonzero(np.repeat(grp["dir_slot"] >= 0, 4 if field == "dir_nodes" else 1))
+        arr = np.array(grp[field], dtype=np.float64 if field == "dir_weight" else np.int32)
+        arr[live[i]] = val
+        grp[field] = arr
+        with pytest.raises(ValueError, match=msg):
+            _tables(c2)

@@ -441,3 +441,78 @@ def test_curvature_gradient_matches_finite_differences(oracle):
             assert (energy(Xp) - energy(Xm)) / (2 * h) == pytest.approx(G[dof, k], rel=1e-5, abs=1e-6)
         checked += 1
     assert checked >= 10
+
+
+def _dir_only_stencil(samples):
+    """An energy stencil with no shape samples and the whole-sample
+    directional table: energy_mindet then returns the directional total."""
+    from egg_amd.smoothing.directional import directional_energy_table
+
+    es = {"num_samples": 0, "gc": np.zeros(0, np.int32), "gn0": np.zeros(0, np.int32),
+          "gn1": np.zeros(0, np.int32), "s0": np.zeros(0), "s1": np.zeros(0), "W_inv": np.eye(2).reshape(1, 4)}
+    es.update(directional_energy_table(samples))
+    return es
+
+
+def _samples(rows, eps=1e-12):
+    from egg_amd.smoothing.directional import DirectionalSamples
+
+    nodes = np.full((len(rows), 4), -1, np.int32)
+    for i, r in enumerate(rows):
+        nodes[i, : len(r[0])] = r[0]
+    ref = np.asarray([r[2] for r in rows], np.float64)
+    return DirectionalSamples(nodes, np.asarray([r[1] for r in rows], np.int32), ref, np.zeros_like(ref),
+                              np.asarray([r[3] for r in rows], np.float64), eps)
+
+
+def test_directional_closed_forms(oracle):
+    """Directional energies (reference src/directional.hpp:95-124) at
+    configurations the definitions fix: a parallel segment perpendicular to
+    its normal costs 0 and one along it w |u|^2 / (|u|^2 + eps^2); a stem is
+    the same with the head at slot 0; a straight line triple costs 0 and a
+    folded one (both legs the same way) 4 w; a reserved kind nothing."""
+    X = np.array([[0.0, 0.0], [2.0, 0.0], [0.0, 3.0], [-2.0, 0.0], [1.0, 1.0]])
+    cases = [
+        ([0, 1], 0, (0.0, 1.0), 0.7, 0.0),             # u = (2, 0) _|_ n
+        ([0, 1], 0, (1.0, 0.0), 0.7, 0.7 * 4 / (4 + 0.25)),  # u || ref, eps = 0.5
+        ([2, 0], 2, (0.0, 1.0), 1.3, 1.3 * 9 / (9 + 0.25)),  # stem: u = p0 - p1 = (0, 3)
+        ([1, 0, 3], 1, (0.0, 0.0), 2.0, 0.0),          # legs (2, 0) and (-2, 0): opposed
+        ([1, 0, 1], 1, (0.0, 0.0), 0.5, 0.5 * 4 * 4 / (4 + 0.25)),  # folded: |2 q/s|^2
+        ([0, 1], 4, (1.0, 0.0), 9.0, 0.0),             # reserved kind
+    ]
+    for nodes, kind, ref, w, want in cases:
+        E, _ = oracle.energy_mindet(2, "shape", 0.0, _dir_only_stencil(_samples([(nodes, kind, ref, w)], 0.5)), X)
+        assert E == pytest.approx(want, rel=1e-12, abs=1e-15), (nodes, kind)
+    # the stem's head is slot 0: swapping the nodes of a parallel sample flips u, not c^2
+    E1, _ = oracle.energy_mindet(2, "shape", 0.0, _dir_only_stencil(_samples([([4, 0], 0, (0.6, 0.8), 1.0)])), X)
+    E2, _ = oracle.energy_mindet(2, "shape", 0.0, _dir_only_stencil(_samples([([0, 4], 2, (0.6, 0.8), 1.0)])), X)
+    assert E1 == pytest.approx(E2, rel=1e-14) and E1 == pytest.approx(1.4 ** 2 / 2, rel=1e-12)
+
+
+def test_directional_gradient_matches_finite_differences(oracle):
+    """The per-DOF gradient with directional windows (reference
+    src/directional.hpp:126-250) equals central differences of the total
+    energy (shape stencil plus the whole-sample directional table) in each
+    DOF's position, for all three kinds on a perturbed block; the host-side
+    directional_eval binding gives the same total and gradient."""
+    from egg_amd import grids
+    from tests.cases import Case
+
+    g = grids.single_block_2d(9, seed=3, amp=0.25)
+    smp = grids.wall_directional_samples(g, weight=0.8, eps=1e-3)
+    c = Case(g, directional=smp)
+    P = c.patches(oracle)
+    G = oracle.patch_grads(2, "shape", c.X, P)
+    es = c.ctx.energy_stencil
+    h = 1e-6
+    dofs = sorted({int(n) for n in smp.nodes[:, :3].reshape(-1) if n >= 0 and g.free[n]})
+    assert len(dofs) > 8
+    for dof in dofs:
+        for k in range(2):
+            Xp, Xm = c.X.copy(), c.X.copy()
+            Xp[dof, k] += h
+            Xm[dof, k] -= h
+            fd = (oracle.energy_mindet(2, "shape", 0.0, es, Xp)[0] - oracle.energy_mindet(2, "shape", 0.0, es, Xm)[0]) / (2 * h)
+            assert G[dof, k] == pytest.approx(fd, rel=1e-5, abs=1e-7), (dof, k)
+    E_dir, _ = oracle.energy_mindet(2, "shape", 0.0, _dir_only_stencil(smp), c.X)
+    assert E_dir > 0
