@@ -55,7 +55,7 @@ def test_directional_parity(oracle, variant):
     assert np.abs(X - Xo).max() / _extent(c) < POS_TOL
     np.testing.assert_allclose(e, eo, rtol=TRAJ_RTOL)
     X0, _, _ = oracle.jacobi(2, "barrier", 0.0, 0.8, 20, c.X, Case(g).patches(oracle), c.ctx.energy_stencil)
-    assert np.abs(X - X0).max() / _extent(c) > 1e-3  # the windows change the result
+    assert np.abs(X - X0).max() / _extent(c) > 1e-4  # the windows change the result
 
 
 def test_reported_energy_is_the_composed_objective(oracle):
@@ -76,21 +76,25 @@ def test_reported_energy_is_the_composed_objective(oracle):
 
 
 def test_sweeps_align_the_chain():
-    """With the terms, the first row above the wall turns parallel to it and
-    the stems upright: the directional total falls well below its value under
-    plain sweeps (reference tests/test_cpp_directional.py:136-160)."""
+    """After 100 sweeps from the perturbed block, the chain above the wall is
+    closer to parallel, the stems closer to upright and the line triples
+    straighter with the terms than under plain sweeps: the directional total
+    is below half of plain sweeps' (the shape metric alone also drifts
+    towards the uniform grid, which satisfies every term; reference
+    tests/test_cpp_directional.py:136-160)."""
     from egg_amd._cpp import cpp_core
 
-    c, g = _block(w=2.0, eps=1e-6)
+    g = grids.single_block_2d(17, seed=4, amp=0.25)
     smp = grids.wall_directional_samples(g, 2.0, 1e-6)
+    c = Case(g, directional=smp)
     a, b = _session(c), _session(Case(g))
-    ea, ma = a.run(200, omega=0.8)
-    b.run(200, omega=0.8)
+    ea, ma = a.run(100, omega=0.8)
+    b.run(100, omega=0.8)
 
     def ed(X):
         return cpp_core.directional_eval(X, smp.nodes, smp.kind, smp.ref, smp.weight, smp.eps)[0]
 
-    assert ma[-1] > 0 and ed(a.get_X()) < 0.5 * ed(b.get_X()) and ed(a.get_X()) < ed(c.X)
+    assert ma[-1] > 0 and ed(a.get_X()) < 0.5 * ed(b.get_X()) and ed(a.get_X()) < 1e-5 * ed(c.X)
 
 
 def test_directional_on_constrained_wall(oracle):
