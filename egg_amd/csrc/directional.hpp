@@ -1,4 +1,4 @@
-// Directional soft-energy samples (2D; reference src/directional.hpp,
+// Directional soft-energy samples (reference src/directional.hpp,
 // NumPy twin egg/smoothing/directional.py:304-369). Kinds:
 //   0 parallel (nodes i, j): E = w c^2, c = (u . ref) / s, u = p_j - p_i,
 //     s = sqrt(|u|^2 + eps^2): a chain segment kept parallel to a wall;
@@ -19,77 +19,133 @@ namespace eggmi {
 
 __host__ __device__ constexpr int dir_arity(int kind) { return kind == 1 ? 3 : (kind == 0 || kind == 2) ? 2 : 0; }
 
+// Packed index of H(i, j), i <= j, in the upper-triangle layout the Newton
+// solve reads ({h00, h01, h11} in 2D, {h00, h01, h02, h11, h12, h22} in 3D).
+template <int D> __host__ __device__ constexpr int dir_hidx(int i, int j) { return i * D - i * (i - 1) / 2 + (j - i); }
+
 // A sample's energy at positions p[0 .. arity-1] (src/directional.hpp:95-124);
 // T = float in the sweeps, double in the FP64 reductions.
-template <typename T>
-__device__ __forceinline__ T dir_energy(int kind, const T (&p)[3][2], const float* ref, T w, T eps)
+template <int D, typename T>
+__device__ __forceinline__ T dir_energy(int kind, const T (&p)[3][D], const float* ref, T w, T eps)
 {
     if (kind == 0 || kind == 2) {
         const int hd = kind == 0 ? 1 : 0;
-        const T ux = p[hd][0] - p[1 - hd][0], uy = p[hd][1] - p[1 - hd][1];
-        const T c = (ux * T(ref[0]) + uy * T(ref[1])) / sqrt(eps * eps + ux * ux + uy * uy);
+        T ss = eps * eps, dot = T(0);
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const T u = p[hd][k] - p[1 - hd][k];
+            ss += u * u;
+            dot += u * T(ref[k]);
+        }
+        const T c = dot / sqrt(ss);
         return w * c * c;
     }
     if (kind == 1) {
-        const T ax = p[0][0] - p[1][0], ay = p[0][1] - p[1][1], bx = p[2][0] - p[1][0], by = p[2][1] - p[1][1];
-        const T sa = sqrt(eps * eps + ax * ax + ay * ay), sb = sqrt(eps * eps + bx * bx + by * by);
-        const T rx = ax / sa + bx / sb, ry = ay / sa + by / sb;
-        return w * (rx * rx + ry * ry);
+        T sa = eps * eps, sb = eps * eps;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const T a = p[0][k] - p[1][k], b = p[2][k] - p[1][k];
+            sa += a * a;
+            sb += b * b;
+        }
+        sa = sqrt(sa);
+        sb = sqrt(sb);
+        T e = T(0);
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const T r = (p[0][k] - p[1][k]) / sa + (p[2][k] - p[1][k]) / sb;
+            e += r * r;
+        }
+        return w * e;
     }
     return T(0);
 }
 
-// Gradient, Hessian (h00, h01, h11) and energy of one sample for the node at
-// `slot`, added in (src/directional.hpp:126-211).
-__device__ __forceinline__ void dir_accumulate(int kind, const float (&p)[3][2], const float* ref, float w,
-                                               float eps, int slot, float (&g)[2], float (&h)[3], float& e)
+// Gradient, packed Hessian and energy of one sample for the node at `slot`,
+// added in (src/directional.hpp:126-211).
+template <int D>
+__device__ __forceinline__ void dir_accumulate(int kind, const float (&p)[3][D], const float* ref, float w,
+                                               float eps, int slot, float (&g)[D], float (&h)[D * (D + 1) / 2],
+                                               float& e)
 {
     if (kind == 0 || kind == 2) {
         const int hd = kind == 0 ? 1 : 0;
-        const float ux = p[hd][0] - p[1 - hd][0], uy = p[hd][1] - p[1 - hd][1];
-        const float ss = eps * eps + ux * ux + uy * uy, sn = sqrtf(ss);
-        const float c = (ux * ref[0] + uy * ref[1]) / sn;
-        const float drx = ref[0] / sn - c * ux / ss, dry = ref[1] / sn - c * uy / ss;  // dc/du
+        float u[D], ss = eps * eps, dot = 0.0f;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            u[k] = p[hd][k] - p[1 - hd][k];
+            ss += u[k] * u[k];
+            dot += u[k] * ref[k];
+        }
+        const float sn = sqrtf(ss), c = dot / sn;
+        float dr[D], dd = 0.0f;  // dc/du and the other endpoint's coupling |dr|^2
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            dr[k] = ref[k] / sn - c * u[k] / ss;
+            dd += dr[k] * dr[k];
+        }
         const float gc = 2.0f * w * c * (slot == hd ? 1.0f : -1.0f), hc = 2.0f * w;
-        const float dd = drx * drx + dry * dry;  // the other endpoint's coupling
-        g[0] += gc * drx;
-        g[1] += gc * dry;
-        h[0] += hc * drx * drx + hc * dd;
-        h[1] += hc * drx * dry;
-        h[2] += hc * dry * dry + hc * dd;
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            g[i] += gc * dr[i];
+#pragma unroll
+            for (int j = i; j < D; ++j) h[dir_hidx<D>(i, j)] += hc * dr[i] * dr[j] + (i == j ? hc * dd : 0.0f);
+        }
         e += w * c * c;
         return;
     }
     if (kind != 1) return;
     // unit legs hk = qk / sk and their Jacobians Jk = (I - qk qk^T / sk^2) / sk
-    float J[2][3], hx[2], hy[2];
+    float J[2][D][D], hv[2][D];
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
-        const float qx = p[2 * t][0] - p[1][0], qy = p[2 * t][1] - p[1][1];
-        const float ss = eps * eps + qx * qx + qy * qy, sn = sqrtf(ss), i3 = 1.0f / (ss * sn);
-        hx[t] = qx / sn;
-        hy[t] = qy / sn;
-        J[t][0] = 1.0f / sn - qx * qx * i3;
-        J[t][1] = -qx * qy * i3;
-        J[t][2] = 1.0f / sn - qy * qy * i3;
-    }
-    const float rx = hx[0] + hx[1], ry = hy[0] + hy[1];
-    float Js[3], f[3];
+        float q[D], ss = eps * eps;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) Js[k] = slot == 0 ? J[0][k] : slot == 2 ? J[1][k] : -(J[0][k] + J[1][k]);
-    // Frobenius norms of J_a, J_b, J_a + J_b (the off-diagonal counts twice)
-    auto frob = [](float a, float b, float c) { return sqrtf(a * a + 2.0f * b * b + c * c); };
-    f[0] = frob(J[0][0], J[0][1], J[0][2]);
-    f[1] = frob(J[1][0], J[1][1], J[1][2]);
-    f[2] = frob(J[0][0] + J[1][0], J[0][1] + J[1][1], J[0][2] + J[1][2]);
+        for (int k = 0; k < D; ++k) {
+            q[k] = p[2 * t][k] - p[1][k];
+            ss += q[k] * q[k];
+        }
+        const float sn = sqrtf(ss), i3 = 1.0f / (ss * sn);
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            hv[t][i] = q[i] / sn;
+#pragma unroll
+            for (int j = 0; j < D; ++j) J[t][i][j] = (i == j ? 1.0f / sn : 0.0f) - q[i] * q[j] * i3;
+        }
+    }
+    float r[D], Js[D][D], f[3] = {0.0f, 0.0f, 0.0f}, rr = 0.0f;  // f: |J_a|_F, |J_b|_F, |J_a + J_b|_F
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        r[i] = hv[0][i] + hv[1][i];
+        rr += r[i] * r[i];
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            const float a = J[0][i][j], b = J[1][i][j];
+            Js[i][j] = slot == 0 ? a : slot == 2 ? b : -(a + b);
+            f[0] += a * a;
+            f[1] += b * b;
+            f[2] += (a + b) * (a + b);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 3; ++t) f[t] = sqrtf(f[t]);
     const float fs = slot == 0 ? f[0] : slot == 2 ? f[1] : f[2];
     const float gc = 2.0f * w, stab = gc * fs * (f[0] + f[1] + f[2] - fs);
-    g[0] += gc * (Js[0] * rx + Js[1] * ry);
-    g[1] += gc * (Js[1] * rx + Js[2] * ry);
-    h[0] += gc * (Js[0] * Js[0] + Js[1] * Js[1]) + stab;
-    h[1] += gc * (Js[0] * Js[1] + Js[1] * Js[2]);
-    h[2] += gc * (Js[1] * Js[1] + Js[2] * Js[2]) + stab;
-    e += w * (rx * rx + ry * ry);
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        float gi = 0.0f;
+#pragma unroll
+        for (int j = 0; j < D; ++j) gi += Js[i][j] * r[j];
+        g[i] += gc * gi;
+#pragma unroll
+        for (int j = i; j < D; ++j) {
+            float hij = 0.0f;
+#pragma unroll
+            for (int m = 0; m < D; ++m) hij += Js[i][m] * Js[j][m];
+            h[dir_hidx<D>(i, j)] += gc * hij + (i == j ? stab : 0.0f);
+        }
+    }
+    e += w * rr;
 }
 
 }  // namespace eggmi

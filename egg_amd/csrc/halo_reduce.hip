@@ -439,7 +439,7 @@ void launch_linesearch_ladder(int dim, int kind, const CellReduceArgs& a, const 
             EGGMI_LL(3, kShape);
     }
 #undef EGGMI_LL
-    const int nd = dir ? launch_dir_reduce(2, *dir, a.x, a.plane, corr, 0.0f, 0.0f, partials, nb, s) : 0;
+    const int nd = dir ? launch_dir_reduce(dim, 2, *dir, a.x, a.plane, corr, 0.0f, 0.0f, partials, nb, s) : 0;
     ladder_finish<<<1, kReduceThreads, 0, s>>>(partials, nb + nd, tol, out, alpha);
 }
 
@@ -460,7 +460,7 @@ void launch_linesearch_pair(int dim, int kind, const CellReduceArgs& a, const fl
             EGGMI_LP(3, kShape);
     }
 #undef EGGMI_LP
-    const int nd = dir ? launch_dir_reduce(1, *dir, a.x, a.plane, corr, a0, a1, partials, nb, s) : 0;
+    const int nd = dir ? launch_dir_reduce(dim, 1, *dir, a.x, a.plane, corr, a0, a1, partials, nb, s) : 0;
     pair_finish<<<1, kReduceThreads, 0, s>>>(partials, nb + nd, out);
 }
 
@@ -533,7 +533,7 @@ void launch_reduce_finish(const double* partials, int nparts, double* out_e, dou
 // src/fas.hpp:233). Positions are formed in FP32 as cell_trial forms them,
 // the energy evaluated and accumulated in FP64; the min det slot is +inf.
 // MODE 0: alpha 0; 1: a0, a1; 2: 0 then the ladder's alphas.
-template <int MODE>
+template <int D, int MODE>
 __global__ __launch_bounds__(kReduceThreads) void dir_reduce(const DirView v, const float* x, long long plane,
                                                              const float* corr, float a0, float a1, double* partials)
 {
@@ -547,14 +547,14 @@ __global__ __launch_bounds__(kReduceThreads) void dir_reduce(const DirView v, co
 #pragma unroll
         for (int t = 0; t < NA; ++t) {
             const float al = MODE == 0 ? 0.0f : MODE == 1 ? (t == 0 ? a0 : a1) : (t == 0 ? 0.0f : ladder_alpha(t));
-            double p[3][2] = {};
+            double p[3][D] = {};
             for (int m = 0; m < ar; ++m)
 #pragma unroll
-                for (int c = 0; c < 2; ++c) {
+                for (int c = 0; c < D; ++c) {
                     const long long q = c * plane + v.nodes[4 * i + m];
                     p[m][c] = (double)(corr ? x[q] + al * corr[q] : x[q]);
                 }
-            e[t] += dir_energy<double>(kind, p, v.ref + 2 * i, (double)v.w[i], (double)v.eps);
+            e[t] += dir_energy<D, double>(kind, p, v.ref + D * i, (double)v.w[i], (double)v.eps);
         }
     }
     for (int t = 0; t < NA; ++t) {
@@ -563,19 +563,29 @@ __global__ __launch_bounds__(kReduceThreads) void dir_reduce(const DirView v, co
     }
 }
 
-int launch_dir_reduce(int mode, const DirView& v, const float* x, long long plane, const float* corr, float a0,
-                      float a1, double* partials, int pbase, hipStream_t s)
+template <int D>
+static void dir_reduce_d(int mode, int nb, const DirView& v, const float* x, long long plane, const float* corr,
+                         float a0, float a1, double* partials, int pbase, hipStream_t s)
+{
+    if (mode == 0)
+        dir_reduce<D, 0><<<nb, kReduceThreads, 0, s>>>(v, x, plane, corr, a0, a1, partials + 2 * pbase);
+    else if (mode == 1)
+        dir_reduce<D, 1><<<nb, kReduceThreads, 0, s>>>(v, x, plane, corr, a0, a1, partials + 4 * pbase);
+    else
+        dir_reduce<D, 2><<<nb, kReduceThreads, 0, s>>>(v, x, plane, corr, a0, a1,
+                                                       partials + 2 * (kLadder + 1) * pbase);
+}
+
+int launch_dir_reduce(int dim, int mode, const DirView& v, const float* x, long long plane, const float* corr,
+                      float a0, float a1, double* partials, int pbase, hipStream_t s)
 {
     if (v.n <= 0) return 0;
     int nb = (v.n + kReduceThreads - 1) / kReduceThreads;
     if (nb > kReduceBlocks / 2) nb = kReduceBlocks / 2;
-    if (mode == 0)
-        dir_reduce<0><<<nb, kReduceThreads, 0, s>>>(v, x, plane, corr, a0, a1, partials + 2 * pbase);
-    else if (mode == 1)
-        dir_reduce<1><<<nb, kReduceThreads, 0, s>>>(v, x, plane, corr, a0, a1, partials + 4 * pbase);
+    if (dim == 2)
+        dir_reduce_d<2>(mode, nb, v, x, plane, corr, a0, a1, partials, pbase, s);
     else
-        dir_reduce<2><<<nb, kReduceThreads, 0, s>>>(v, x, plane, corr, a0, a1,
-                                                    partials + 2 * (kLadder + 1) * pbase);
+        dir_reduce_d<3>(mode, nb, v, x, plane, corr, a0, a1, partials, pbase, s);
     return nb;
 }
 

@@ -56,9 +56,9 @@ struct StoredView {
 
 constexpr int kCounterSlots = 64, kCounterStride = 16;
 
-// Directional soft-energy samples on the store (2D): node slots (4 per
-// sample, the first arity of them used), kind (0 parallel, 1 line, 2 stem,
-// -1 none), frozen reference vector (2 per sample), weight. As a per-DOF
+// Directional soft-energy samples on the store: node slots (4 per sample,
+// the first arity of them used), kind (0 parallel, 1 line, 2 stem, -1
+// none), frozen reference vector (D per sample), weight. As a per-DOF
 // window table, n is the windows per DOF and slot the DOF's place (-1 pads);
 // as the energy reports' whole-sample table, n is the sample count.
 struct DirView {
@@ -95,7 +95,7 @@ struct SweepArgs {
     const int* curv_nodes = nullptr;
     const int8_t* curv_slot = nullptr;
     const float* curv_w = nullptr;
-    // directional soft-energy windows of those DOFs (2D): dir.n per DOF
+    // directional soft-energy windows of those DOFs: dir.n per DOF
     DirView dir;
     float* gout = nullptr;       // gradient-only launches: per-node gradient (plane layout)
     bool gsub = false;           // gradient-only: gout = gradient - gout, in place
@@ -161,8 +161,8 @@ void launch_reduce_finish(const double* partials, int nparts, double* out_e, dou
 // blocks pbase, pbase + 1, ... in the layout of that mode's reduction (2,
 // 4 or 2 (kLadder + 1) doubles per block, min det +inf). Returns the block
 // count (at most kReduceBlocks / 2), 0 when the table is empty.
-int launch_dir_reduce(int mode, const DirView& v, const float* x, long long plane, const float* corr, float a0,
-                      float a1, double* partials, int pbase, hipStream_t s);
+int launch_dir_reduce(int dim, int mode, const DirView& v, const float* x, long long plane, const float* corr,
+                      float a0, float a1, double* partials, int pbase, hipStream_t s);
 
 // Tile shapes the sweep kernels are instantiated for (the autotuner picks one).
 struct TileShape {

@@ -263,7 +263,7 @@ void check_index_range(const int32_t* v, long long n, long long N, const std::st
 
 // Mirrors the checks of the reference's validate_context (src/bindings.cpp:98-253)
 // for the fields this path consumes.
-void validate_dir(const int32_t* nodes, const int32_t* slot, const int32_t* kind, const double* ref,
+void validate_dir(int dim, const int32_t* nodes, const int32_t* slot, const int32_t* kind, const double* ref,
                   const double* wt, double eps, long long n, long long N, const std::string& grp);
 
 void validate(int dim, const eggmi_group* groups, int n_groups, const eggmi_stencil* es, long long N)
@@ -320,9 +320,8 @@ void validate(int dim, const eggmi_group* groups, int n_groups, const eggmi_sten
         }
         if (g.dir_k < 0) throw InvalidArgument("validate_context: " + grp + " negative dir_k");
         if (g.dir_k > 0) {
-            if (dim != 2) throw InvalidArgument("validate_context: directional terms are 2D only in this build");
             if (g.dir_k > 64) throw InvalidArgument("validate_context: more than 64 directional samples per DOF");
-            validate_dir(g.dir_nodes, g.dir_slot, g.dir_kind, g.dir_ref, g.dir_weight, g.dir_eps,
+            validate_dir(dim, g.dir_nodes, g.dir_slot, g.dir_kind, g.dir_ref, g.dir_weight, g.dir_eps,
                          (long long)g.ndof * g.dir_k, N, grp);
         }
         if (g.entity_tag)
@@ -353,9 +352,8 @@ void validate(int dim, const eggmi_group* groups, int n_groups, const eggmi_sten
                 throw InvalidArgument("validate_context: energy_stencil weight must be finite");
     if (es->dir_num < 0) throw InvalidArgument("validate_context: energy_stencil negative dir_num");
     if (es->dir_num > 0) {
-        if (dim != 2) throw InvalidArgument("validate_context: directional terms are 2D only in this build");
         if (es->dir_num >= (1ll << 31)) throw InvalidArgument("validate_context: more than 2^31 directional samples");
-        validate_dir(es->dir_nodes, nullptr, es->dir_kind, es->dir_ref, es->dir_weight, es->dir_eps, es->dir_num, N,
+        validate_dir(dim, es->dir_nodes, nullptr, es->dir_kind, es->dir_ref, es->dir_weight, es->dir_eps, es->dir_num, N,
                      "energy_stencil");
     }
     int gk = -1;  // the groups' directional eps must agree (one per session)
@@ -371,7 +369,7 @@ void validate(int dim, const eggmi_group* groups, int n_groups, const eggmi_sten
 // slots of a sample of known kind in range, the DOF's slot (per-DOF tables)
 // in [-1, 3] and below the kind's arity, finite reference vector, weight and
 // eps. Samples of other kinds add nothing and are not checked further.
-void validate_dir(const int32_t* nodes, const int32_t* slot, const int32_t* kind, const double* ref,
+void validate_dir(int dim, const int32_t* nodes, const int32_t* slot, const int32_t* kind, const double* ref,
                   const double* wt, double eps, long long n, long long N, const std::string& grp)
 {
     if (!std::isfinite(eps)) throw InvalidArgument("validate_context: " + grp + " dir_eps must be finite");
@@ -384,8 +382,9 @@ void validate_dir(const int32_t* nodes, const int32_t* slot, const int32_t* kind
             throw InvalidArgument("validate_context: " + grp + " dir_slot beyond its kind's " + std::to_string(ar) +
                                   " nodes");
         check_index_range(nodes + 4 * q, ar, N, grp, "dir_nodes");
-        if (!std::isfinite(wt[q]) || !std::isfinite(ref[2 * q]) || !std::isfinite(ref[2 * q + 1]))
-            throw InvalidArgument("validate_context: " + grp + " dir_ref / dir_weight must be finite");
+        bool ok = std::isfinite(wt[q]);
+        for (int k = 0; k < dim; ++k) ok = ok && std::isfinite(ref[(long long)dim * q + k]);
+        if (!ok) throw InvalidArgument("validate_context: " + grp + " dir_ref / dir_weight must be finite");
     }
 }
 
@@ -686,7 +685,7 @@ void Session::create(int d, const eggmi_group* groups, int n_groups, const eggmi
                     L.dir_slot.push_back(live ? (int8_t)g.dir_slot[q] : (int8_t)-1);
                     L.dir_kind.push_back(live ? (int8_t)g.dir_kind[q] : (int8_t)-1);
                     L.dir_w.push_back(live ? (float)g.dir_weight[q] : 0.0f);
-                    for (int k = 0; k < 2; ++k) L.dir_ref.push_back(live ? (float)g.dir_ref[2 * q + k] : 0.0f);
+                    for (int k = 0; k < d; ++k) L.dir_ref.push_back(live ? (float)g.dir_ref[(long long)d * q + k] : 0.0f);
                     for (int m = 0; m < 4; ++m) {
                         const int nid = live && m < ar ? g.dir_nodes[4 * q + m] : -1;
                         if (nid >= 0 && g2s[nid] < 0)
@@ -703,22 +702,28 @@ void Session::create(int d, const eggmi_group* groups, int n_groups, const eggmi
                     int lg[3];
                     for (int k = 0; k < d; ++k) lg[k] = g.interior_logical[di * d + k];
                     L.st_dof.push_back((int)dslot);
-                    for (int occ = 0; occ < 16; ++occ) {
-                        const int cell = occ >> 2, o = occ & 3;
-                        int corner[2], ob2[2];
-                        for (int k = 0; k < 2; ++k) {
-                            ob2[k] = (o >> (1 - k)) & 1;
-                            corner[k] = lg[k] - 1 + ((cell >> (1 - k)) & 1) + ob2[k];
+                    const int nc = 1 << d;
+                    auto is_dof = [&](const int* c) {
+                        for (int k = 0; k < d; ++k)
+                            if (c[k] != lg[k]) return false;
+                        return true;
+                    };
+                    for (int occ = 0; occ < nc * nc; ++occ) {
+                        const int cell = occ / nc, o = occ % nc;
+                        int corner[3] = {0, 0, 0}, ob2[3] = {0, 0, 0};
+                        for (int k = 0; k < d; ++k) {
+                            ob2[k] = (o >> (d - 1 - k)) & 1;
+                            corner[k] = lg[k] - 1 + ((cell >> (d - 1 - k)) & 1) + ob2[k];
                         }
-                        int role = (corner[0] == lg[0] && corner[1] == lg[1]) ? 0 : -1;
+                        int role = is_dof(corner) ? 0 : -1;
                         L.st_gc.push_back((int)lay.interior_slot(ib, corner));
                         uint8_t meta = 0;
-                        for (int k = 0; k < 2; ++k) {
-                            int nbr[2] = {corner[0], corner[1]};
+                        for (int k = 0; k < d; ++k) {
+                            int nbr[3] = {corner[0], corner[1], corner[2]};
                             nbr[k] += ob2[k] == 0 ? 1 : -1;
                             gn_rows.push_back((int)lay.interior_slot(ib, nbr));
                             if (ob2[k]) meta |= (uint8_t)(8 << k);
-                            if (role < 0 && nbr[0] == lg[0] && nbr[1] == lg[1]) role = k + 1;
+                            if (role < 0 && is_dof(nbr)) role = k + 1;
                         }
                         L.st_meta.push_back((uint8_t)(meta | (uint8_t)(role + 1)));
                         if (!w_uniform_all) throw Internal("structured: synthesized DOF under a per-sample target");
@@ -885,7 +890,7 @@ void Session::create(int d, const eggmi_group* groups, int n_groups, const eggmi
         const int ar = dir_arity(es->dir_kind[i]);
         L.es_dir_kind.push_back(ar ? (int8_t)es->dir_kind[i] : (int8_t)-1);
         L.es_dir_w.push_back(ar ? (float)es->dir_weight[i] : 0.0f);
-        for (int k = 0; k < 2; ++k) L.es_dir_ref.push_back(ar ? (float)es->dir_ref[2 * i + k] : 0.0f);
+        for (int k = 0; k < d; ++k) L.es_dir_ref.push_back(ar ? (float)es->dir_ref[(long long)d * i + k] : 0.0f);
         for (int m = 0; m < 4; ++m) {
             const int nid = m < ar ? es->dir_nodes[4 * i + m] : -1;
             if (nid >= 0 && g2s[nid] < 0)
@@ -969,7 +974,8 @@ void Session::reduce(Level& L, const float* x, int kind, double delta, double* o
         launch_reduce_samples(dim, kind, a, partials.p, stream);
     }
     // whole-sample directional terms join the reported energy (reference src/sweep.hpp:1921)
-    const int nd = launch_dir_reduce(0, L.es_dir, x, L.plane, nullptr, 0.0f, 0.0f, partials.p, kReduceBlocks, stream);
+    const int nd =
+        launch_dir_reduce(dim, 0, L.es_dir, x, L.plane, nullptr, 0.0f, 0.0f, partials.p, kReduceBlocks, stream);
     launch_reduce_finish(partials.p, kReduceBlocks + nd, out_e, out_m, stream);
 }
 

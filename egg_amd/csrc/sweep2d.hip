@@ -40,23 +40,14 @@ template <int KIND, bool TAU, bool GRAD, bool PROP = false> static void sweep2d_
     }
 }
 
-template <int KIND, int G, bool GRAD> static void constrained_g(const SweepArgs& a, hipStream_t s)
-{
-    const unsigned grid = (unsigned)((a.con_n + 256 / G - 1) / (256 / G));
-    if (a.st.wt)
-        constrained_kernel<KIND, G, true, GRAD><<<dim3(grid), dim3(256), 0, s>>>(a);
-    else
-        constrained_kernel<KIND, G, false, GRAD><<<dim3(grid), dim3(256), 0, s>>>(a);
-}
-
 // The special stored DOFs (constrained, or carrying curvature windows), laid
 // out for the variant's stored lane groups.
 template <int KIND, bool GRAD = false> static void launch_constrained(int v, const SweepArgs& a, hipStream_t s)
 {
     switch (tile_variant_stored_lanes(2, v)) {
-    case 4: constrained_g<KIND, 4, GRAD>(a, s); break;
-    case 8: constrained_g<KIND, 8, GRAD>(a, s); break;
-    default: constrained_g<KIND, 16, GRAD>(a, s); break;
+    case 4: constrained_g<2, KIND, 4, GRAD>(a, s); break;
+    case 8: constrained_g<2, KIND, 8, GRAD>(a, s); break;
+    default: constrained_g<2, KIND, 16, GRAD>(a, s); break;
     }
 }
 

@@ -28,7 +28,44 @@ template <int KIND, bool TAU, bool GRAD, bool PROP = false> static void sweep3d_
     }
 }
 
+// The special stored DOFs (those with directional windows), laid out for the
+// variant's stored lane groups.
+template <int KIND, bool GRAD = false> static void launch_special3d(int v, const SweepArgs& a, hipStream_t s)
+{
+    if (tile_variant_stored_lanes(3, v) == 16)
+        constrained_g<3, KIND, 16, GRAD>(a, s);
+    else
+        constrained_g<3, KIND, 64, GRAD>(a, s);
+}
+
+static void sweep3d_main(int kind, int v, const SweepArgs& a, hipStream_t s);
+static void grad3d_main(int kind, int v, const SweepArgs& a, hipStream_t s);
+
 void launch_sweep3d(int kind, int v, const SweepArgs& a, hipStream_t s)
+{
+    sweep3d_main(kind, v, a, s);
+    if (a.con_n > 0 && !a.tau) {
+        if (kind == kUntangle)
+            launch_special3d<kUntangle>(v, a, s);
+        else if (kind == kShapeSize)
+            launch_special3d<kShapeSize>(v, a, s);
+        else
+            launch_special3d<kShape>(v, a, s);
+    }
+}
+
+void launch_grad3d(int kind, int v, const SweepArgs& a, hipStream_t s)
+{
+    grad3d_main(kind, v, a, s);
+    if (a.con_n > 0 && !a.tau) {  // the fine level's special DOFs (FAS gradient)
+        if (kind == kShapeSize)
+            launch_special3d<kShapeSize, true>(v, a, s);
+        else
+            launch_special3d<kShape, true>(v, a, s);
+    }
+}
+
+static void sweep3d_main(int kind, int v, const SweepArgs& a, hipStream_t s)
 {
     if (a.tau) {
         if (kind == kShapeSize)
@@ -52,7 +89,7 @@ void launch_sweep3d(int kind, int v, const SweepArgs& a, hipStream_t s)
         sweep3d_v<kShape, false, false>(v, a, s);
 }
 
-void launch_grad3d(int kind, int v, const SweepArgs& a, hipStream_t s)
+static void grad3d_main(int kind, int v, const SweepArgs& a, hipStream_t s)
 {
     if (a.tau) {
         if (kind == kShapeSize)

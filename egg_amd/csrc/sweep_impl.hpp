@@ -247,7 +247,7 @@ __device__ __forceinline__ float curv_residual(const float (&p)[4][2], float (&d
 template <int D, int KIND, bool TAU, bool GRAD, int G, bool WT, bool ENT = false>
 __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
 {
-    static_assert(!ENT || (D == 2 && !TAU), "special DOFs: 2D, fine level");
+    static_assert(!ENT || !TAU, "special DOFs: fine level");
     static_assert(G >= 4 && G <= stored_lanes(D), "stored lane group size");
     const StoredView& st = a.st;
     const float* X = a.xin;
@@ -330,17 +330,17 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
             for (int k = 0; k < 4; ++k) erec[k] = a.con_rec[4 * ci + k];
         }
     }
-    const bool curv = ENT && valid && etag == 0 && a.curv_k > 0;
+    const bool curv = D == 2 && ENT && valid && etag == 0 && a.curv_k > 0;  // entities and curvature: 2D
     // directional windows, on free and constrained special DOFs alike
     // (reference src/sweep.hpp:1488-1510, :1746-1760), lane l taking l, l + G, ...
     const bool dirw = ENT && valid && a.dir.n > 0;
     // positions of window w's nodes, the DOF's own at `sub` when given
-    auto dir_load = [&](long long w, int sl, const float* sub, float (&p)[3][2]) {
+    auto dir_load = [&](long long w, int sl, const float* sub, float (&p)[3][D]) {
         const int ar = dir_arity(a.dir.kind[w]);
 #pragma unroll
         for (int m = 0; m < 3; ++m)
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+            for (int i = 0; i < D; ++i)
                 p[m][i] = m >= ar ? 0.0f : (sub && m == sl) ? sub[i] : X[i * PL + a.dir.nodes[4 * w + m]];
         return ar;
     };
@@ -392,6 +392,7 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
         }
     }
     if constexpr (ENT) {
+        if constexpr (D == 2)
         if (curv)  // curvature on top of the shape metric (reference src/sweep.hpp:1461-1487)
             for (int j = l; j < a.curv_k; j += G) {
                 const long long w = (long long)ci * a.curv_k + j;
@@ -428,15 +429,9 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
             for (int j = l; j < a.dir.n; j += G) {
                 const long long w = (long long)ci * a.dir.n + j;
                 const int sl = a.dir.slot[w];
-                float p[3][2];
+                float p[3][D];
                 if (sl < 0 || !dir_load(w, sl, nullptr, p)) continue;
-                float gd[2] = {g[0], g[1]}, hd[3] = {h[0], h[1], h[2]};
-                dir_accumulate(a.dir.kind[w], p, a.dir.ref + 2 * w, a.dir.w[w], a.dir.eps, sl, gd, hd, e);
-                g[0] = gd[0];
-                g[1] = gd[1];
-                h[0] = hd[0];
-                h[1] = hd[1];
-                h[2] = hd[2];
+                dir_accumulate<D>(a.dir.kind[w], p, a.dir.ref + D * w, a.dir.w[w], a.dir.eps, sl, g, h, e);
             }
     }
 #pragma unroll
@@ -465,7 +460,7 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
 #pragma unroll
     for (int k = 0; k < kNH<D>; ++k) h[k] = group_sum<G>(h[k]);
     float dl[D];
-    if (ENT && etag != 0) {
+    if (D == 2 && ENT && etag != 0) {
         const float ft = etag == 15 ? rail_foot(erec, pos) : 0.0f;
         if (etag == 15 && (ft < 0.0f || ft > 1.0f)) {
             // a rail beyond its ends releases the node: full-space step
@@ -499,12 +494,13 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
         float tr[D];
 #pragma unroll
         for (int k = 0; k < D; ++k) tr[k] = pos[k] + alpha * dl[k];
-        if (ENT && etag != 0) {
+        if (D == 2 && ENT && etag != 0) {
             const float raw[2] = {tr[0], tr[1]};
             ent_project(etag, erec, raw, tr);
         }
         float et = 0.0f, mt = kInf;
         if constexpr (ENT) {
+            if constexpr (D == 2)
             if (curv)  // the windows' energy with the DOF moved to the trial
                 for (int j = l; j < a.curv_k; j += G) {
                     const long long w = (long long)ci * a.curv_k + j;
@@ -522,9 +518,9 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
                 for (int j = l; j < a.dir.n; j += G) {
                     const long long w = (long long)ci * a.dir.n + j;
                     const int sl = a.dir.slot[w];
-                    float p[3][2];
+                    float p[3][D];
                     if (sl < 0 || !dir_load(w, sl, tr, p)) continue;
-                    et += dir_energy<float>(a.dir.kind[w], p, a.dir.ref + 2 * w, a.dir.w[w], a.dir.eps);
+                    et += dir_energy<D, float>(a.dir.kind[w], p, a.dir.ref + D * w, a.dir.w[w], a.dir.eps);
                 }
         }
         for (int r = 0; r < R; ++r) {
@@ -562,14 +558,24 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
     }
 }
 
-// The special stored DOFs of a fine-level sweep or gradient pass (2D:
-// constrained ones and those with curvature or directional windows), in a launch of their
-// own beside the sweep kernel: both read xin and write disjoint DOF slots.
-template <int KIND, int G, bool WT, bool GRAD>
+// The special stored DOFs of a fine-level sweep or gradient pass
+// (constrained ones and those with curvature windows, 2D, or directional
+// windows), in a launch of their own beside the sweep kernel: both read xin
+// and write disjoint DOF slots.
+template <int D, int KIND, int G, bool WT, bool GRAD>
 __global__ __launch_bounds__(256) void constrained_kernel(const SweepArgs a)
 {
     const int gid = int(blockIdx.x) * (256 / G) + int(threadIdx.x) / G;
-    stored_group<2, KIND, false, GRAD, G, WT, true>(a, a.st.n + gid, int(threadIdx.x) % G, gid < a.con_n);
+    stored_group<D, KIND, false, GRAD, G, WT, true>(a, a.st.n + gid, int(threadIdx.x) % G, gid < a.con_n);
+}
+
+template <int D, int KIND, int G, bool GRAD> static void constrained_g(const SweepArgs& a, hipStream_t s)
+{
+    const unsigned grid = (unsigned)((a.con_n + 256 / G - 1) / (256 / G));
+    if (a.st.wt)
+        constrained_kernel<D, KIND, G, true, GRAD><<<dim3(grid), dim3(256), 0, s>>>(a);
+    else
+        constrained_kernel<D, KIND, G, false, GRAD><<<dim3(grid), dim3(256), 0, s>>>(a);
 }
 
 // ------------------------------------------------------------------ tiles

@@ -309,15 +309,18 @@ def interface_c2_windows(grid: Grid, weight: float = 1.0) -> tuple[np.ndarray, n
 
 
 def wall_directional_samples(grid: Grid, weight: float = 1.0, eps: float = 1e-12):
-    """Directional samples on a 2D single block (the reference's three kinds,
+    """Directional samples on a single block (the reference's three kinds,
     egg/smoothing/directional.py:166-301, placed by hand): the first row of
     nodes above the j = 0 wall as a chain of parallel segments with the
     wall's normal (0, 1) as reference; every third node of that row a stem to
     the next row up with the through axis (1, 0); the middle row as line
-    triples. Returns a ``DirectionalSamples``."""
+    triples. A 3D block takes them in its middle k-plane (references with a
+    zero third component). Returns a ``DirectionalSamples``."""
     from egg_amd.smoothing.directional import KIND_LINE, KIND_PARALLEL, KIND_STEM, DirectionalSamples
 
     ids = np.asarray(grid.blocks[0])
+    if grid.d == 3:
+        ids = ids[:, :, ids.shape[2] // 2]
     n = ids.shape[0]
     rows = []
     for i in range(n - 1):
@@ -330,7 +333,7 @@ def wall_directional_samples(grid: Grid, weight: float = 1.0, eps: float = 1e-12
     nodes = np.full((len(rows), 4), -1, np.int32)
     nodes[:, :3] = [r[:3] for r in rows]
     k = np.asarray([r[3] for r in rows], np.int32)
-    ref = np.asarray([r[4] for r in rows], np.float64)
+    ref = np.asarray([r[4] + (0.0,) * (grid.d - 2) for r in rows], np.float64)
     return DirectionalSamples(nodes, k, ref, np.zeros_like(ref), np.full(len(rows), float(weight)), eps)
 
 

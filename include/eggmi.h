@@ -59,15 +59,15 @@ typedef struct {
     const int32_t* curv_nodes;        /* [ndof*curv_k*4] global node ids of each window */
     const int32_t* curv_slot;         /* [ndof*curv_k] the DOF's place (0..3) in it, -1 = pad */
     const double* curv_weight;        /* [ndof*curv_k] window weight */
-    int32_t dir_k;                    /* directional soft-energy samples per DOF (2D; 0 = none;
+    int32_t dir_k;                    /* directional soft-energy samples per DOF (0 = none;
                                          src/directional.hpp, src/sweep.hpp:1488-1510, :1746-1760,
                                          wire src/bindings.cpp:393-409) */
     double dir_eps;                   /* their epsilon (x / sqrt(|x|^2 + eps^2)); one per session */
     const int32_t* dir_nodes;         /* [ndof*dir_k*4] global node ids, -1 pads */
     const int32_t* dir_slot;          /* [ndof*dir_k] the DOF's slot in the sample, -1 = pad */
     const int32_t* dir_kind;          /* [ndof*dir_k] 0 parallel, 1 line, 2 stem; others add nothing */
-    const double* dir_ref;            /* [ndof*dir_k*2] frozen reference vector */
-    const double* dir_axis;           /* [ndof*dir_k*2] projector axis of the 3D-only kinds (unused) */
+    const double* dir_ref;            /* [ndof*dir_k*D] frozen reference vector */
+    const double* dir_axis;           /* [ndof*dir_k*D] projector axis of the 3D-only kinds (unused) */
     const double* dir_weight;         /* [ndof*dir_k] */
 } eggmi_group;
 
@@ -81,14 +81,14 @@ typedef struct {
     int64_t w_rows; /* 1 or num_samples */
     const double* weight; /* [num_samples] energy weight, NULL = 1 (src/sweep.hpp:122, :301-302) */
     /* whole-sample directional table added to every reported energy and to
-       the FAS line-search energies (2D; src/sweep.hpp:1921, src/fas.hpp:233,
+       the FAS line-search energies (src/sweep.hpp:1921, src/fas.hpp:233,
        wire src/bindings.cpp:471-480); 0 = none */
     int64_t dir_num;
     double dir_eps;
     const int32_t* dir_nodes;  /* [dir_num*4] global node ids, -1 pads */
     const int32_t* dir_kind;   /* [dir_num] */
-    const double* dir_ref;     /* [dir_num*2] */
-    const double* dir_axis;    /* [dir_num*2] (unused) */
+    const double* dir_ref;     /* [dir_num*D] */
+    const double* dir_axis;    /* [dir_num*D] (unused) */
     const double* dir_weight;  /* [dir_num] */
 } eggmi_stencil;
 

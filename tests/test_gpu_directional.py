@@ -1,4 +1,4 @@
-"""Directional soft-energy terms on the MI355X (reference
+"""Directional soft-energy terms (2D and 3D) on the MI355X (reference
 src/directional.hpp, src/sweep.hpp:1488-1510 and :1746-1760, reports
 src/sweep.hpp:1921, FAS line search src/fas.hpp:233) against the FP64 oracle
 on the same wire: each window's gradient, Gauss-Newton Hessian and
@@ -56,6 +56,25 @@ def test_directional_parity(oracle, variant):
     np.testing.assert_allclose(e, eo, rtol=TRAJ_RTOL)
     X0, _, _ = oracle.jacobi(2, "barrier", 0.0, 0.8, 20, c.X, Case(g).patches(oracle), c.ctx.energy_stencil)
     assert np.abs(X - X0).max() / _extent(c) > 1e-4  # the windows change the result
+
+
+@pytest.mark.parametrize("variant", [0, 4])
+def test_directional_parity_3d(oracle, variant):
+    """A 3D block with the samples in its middle plane: windowed interior
+    DOFs expanded to their 64 stored occurrences, 64 or 16 lanes per DOF."""
+    g = grids.single_block_3d(7, seed=2, amp=0.2)
+    c = Case(g, directional=grids.wall_directional_samples(g, 0.5, 1e-3))
+    s = _session(c)
+    s.core.set_tile_variant(variant)
+    e, m = s.run(12, omega=0.8, report_every=4)
+    Xo, eo, mo = oracle.jacobi(3, "barrier", 0.0, 0.8, 12, c.X, c.patches(oracle), c.ctx.energy_stencil,
+                               report_every=4)
+    X = s.get_X()
+    assert np.abs(X - Xo).max() / _extent(c) < POS_TOL
+    np.testing.assert_allclose(e, eo, rtol=TRAJ_RTOL)
+    f = _session(c)
+    ef, mf = f.run_fas(3, nu_pre=2, nu_post=2, omega=0.8, max_levels=2)
+    assert np.all(mf > 0) and ef[-1] < ef[0]
 
 
 def test_reported_energy_is_the_composed_objective(oracle):

@@ -305,8 +305,9 @@ def test_directional_eval_binding(oracle):
 
 
 def test_directional_wire_validation():
-    """The session refuses a directional table it cannot run: 3D, a slot
-    beyond its kind's nodes, a node outside the mesh, a non-finite weight."""
+    """The session takes a valid directional wire (2D and 3D) and refuses a
+    slot beyond its kind's nodes, a node outside the mesh, a non-finite
+    weight."""
     from egg_amd import grids
     from tests.cases import Case
 
@@ -314,6 +315,9 @@ def test_directional_wire_validation():
     c = Case(g, directional=grids.wall_directional_samples(g))
     info, _ = _tables(c)  # a valid wire passes; its windowed DOFs are the special ones
     assert info["n_stored"] > 0
+    g3 = grids.single_block_3d(7, seed=2, amp=0.2)
+    info3, _ = _tables(Case(g3, directional=grids.wall_directional_samples(g3)))
+    assert info3["n_stored"] > 0
     for field, i, val, msg in [("dir_slot", 0, 2, "beyond its kind"), ("dir_nodes", 1, 10**6, "dir_nodes"),
                                ("dir_weight", 0, np.nan, "finite")]:
         c2 = Case(g, directional=grids.wall_directional_samples(g))
