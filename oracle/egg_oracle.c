@@ -26,7 +26,10 @@
  *   o_jacobi      src/sweep.hpp:1425-1650 (metric + interior update kernels),
  *                 :1938-2099 (jacobi_sweep_once / run_block_jacobi)
  *   dir_*         src/directional.hpp (directional soft-energy terms), NumPy
- *                 twin egg/smoothing/directional.py:304-369
+ *                 twin egg/smoothing/directional.py:304-369; energies and
+ *                 gradients pinned (closed forms, finite differences), the
+ *                 Gauss-Newton block and stabiliser of dir_accumulate:
+ *                 parity unpinned (checked only through GPU parity)
  *
  * Pinning: golden vectors from tests/cpp/golden_metric.hpp,
  * golden_metric_3d.hpp and golden_patch.hpp (tests/golden/*.json), finite
