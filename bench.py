@@ -652,6 +652,106 @@ def bench_ogrid_bl(args):
     }
 
 
+def ogrid_wall_samples(g, weight=0.5, eps=1e-6):
+    """Directional samples on the O-grid's ring blocks (axis 1 index 0 on the
+    circle): a stem from each wall node to the next ring node with the wall
+    tangent as through axis, and the first ring row as parallel segments with
+    the radial direction at their midpoint as normal."""
+    from egg_amd.smoothing.directional import KIND_PARALLEL, KIND_STEM, DirectionalSamples
+
+    rows = []
+    for ids in g.blocks[1:]:
+        ids = np.asarray(ids)
+        for i in range(ids.shape[0]):
+            c0, n1 = int(ids[i, 0]), int(ids[i, 1])
+            t = np.array([-g.X[c0, 1], g.X[c0, 0]])
+            rows.append((n1, c0, KIND_STEM, t / np.linalg.norm(t)))
+            if i + 1 < ids.shape[0]:
+                a, b = int(ids[i, 1]), int(ids[i + 1, 1])
+                mid = 0.5 * (g.X[a] + g.X[b])
+                rows.append((a, b, KIND_PARALLEL, mid / np.linalg.norm(mid)))
+    nodes = np.full((len(rows), 4), -1, np.int32)
+    nodes[:, :2] = [r[:2] for r in rows]
+    ref = np.array([r[3] for r in rows])
+    return DirectionalSamples(nodes, np.array([r[2] for r in rows], np.int32), ref, np.zeros_like(ref),
+                              np.full(len(rows), float(weight)), eps)
+
+
+def bench_ogrid_terms(args):
+    """The config-2 O-grid (129/65, 2 % tangled) with the soft terms on top of
+    the shape metric: block-interface C2 curvature windows across every ring
+    and core interface, and directional stems and parallel segments at the
+    circle wall. The DOFs carrying windows run in the special stored-DOF
+    launch beside the sweep kernel, and the directional whole-sample table
+    joins every energy report. Untangle (plain), timed sweeps, CPU oracle
+    baseline and FP64 parity on the same sweeps."""
+    from egg_amd import grids
+    from egg_amd.smoothing import (CppStructuredSweepSession, build_structured_context_from_block_maps,
+                                   build_sweep_context, cpp_untangle)
+
+    t0 = time.perf_counter()
+    g = grids.tangle(grids.ogrid_2d(129, 65), frac=0.02, factor=1.5, seed=SEED)
+    curv = grids.interface_c2_windows(g, 0.5)
+    smp = ogrid_wall_samples(g)
+    ctx = build_sweep_context(g.blocks, g.free, 2, curvature=curv, directional=smp)
+    bsc = build_structured_context_from_block_maps(2, g.blocks)
+    ctx_iso, _, s_iso = _build(g)
+    Xu, md_u, outer, _ = cpp_untangle(ctx_iso, bsc, g.X, session=s_iso)
+    t_untangle = time.perf_counter() - t0
+    s = CppStructuredSweepSession(ctx, bsc, Xu)
+    info = s.core.info()
+    n_free = g.n_free
+    best, ms = s.core.autotune(20, 0.8)
+    s.core.set_graph_chunk(args.graph_chunk)
+    s.run(args.warmup, omega=0.8)
+    e, m, t_run, _, _ = _timed(s, args.steps, 0.8)
+    value = n_free * args.steps / t_run
+    # the same sweeps without the terms, same session shape and tile variant
+    s0 = s_iso
+    s0.set_X(Xu)
+    s0.core.set_tile_variant(int(best))
+    s0.core.set_graph_chunk(args.graph_chunk)
+    s0.run(args.warmup, omega=0.8)
+    _, _, t_plain, _, _ = _timed(s0, args.steps, 0.8)
+    log(f"terms: stored={info['n_stored']} synth={info['n_synth']} curvature windows={len(curv[0])} "
+        f"directional samples={smp.n}; {t_run / args.steps * 1e6:.2f} us/sweep vs {t_plain / args.steps * 1e6:.2f} "
+        f"plain -> {value:.4g} node-updates/s")
+    cpu = parity = None
+    if not args.no_cpu:
+        from oracle import oracle as O  # the checker: CPU baseline and parity reference
+
+        threads = _threads()
+        n_cpu = 50
+        patches = O.materialize_patches(ctx.wire, g.blocks, 2)
+        t0 = time.perf_counter()
+        Xo, eo, mo = O.jacobi(2, "barrier", 0.0, 0.8, n_cpu, Xu, patches, ctx.energy_stencil, threads=threads)
+        rate = n_free * n_cpu / (time.perf_counter() - t0)
+        cpu = {"value": rate, "unit": "node-updates/s", "cores": threads, "kind": "port",
+               "sample": f"{n_cpu} sweeps of the same grid and terms from the untangled state, OpenMP C oracle F64"}
+        s.set_X(Xu)
+        ep, mp = s.run(n_cpu, omega=0.8)
+        parity = _parity(s.get_X(), Xo, ep[-1], eo[-1], mp[-1], mo[-1], float((g.X.max(0) - g.X.min(0)).max()),
+                         sweeps=n_cpu)
+        log(f"cpu {cpu}\nparity {parity}")
+    return {
+        "metric": METRIC, "value": value, "unit": "node-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": t_run / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "2D 5-block O-grid 129/65, 2% tangled by 1.5h, with block-interface curvature "
+                               "windows (w 0.5) and directional wall stems and parallel segments (w 0.5), "
+                               "omega 0.8",
+                   "free_dofs": n_free, "special_dofs": int(info["n_stored"]), "curvature_windows": int(len(curv[0])),
+                   "directional_samples": int(smp.n), "seed": SEED},
+        "autotune": {"best_variant": int(best), "ms_per_20_sweeps": [float(v) for v in ms]},
+        "graph_chunk": args.graph_chunk, "time_to_untangled_s": t_untangle,
+        "plain_ms_per_step_same_variant": t_plain / args.steps * 1e3,
+        "final_energy": float(e[-1]), "final_min_det": float(m[-1]),
+        "parity": parity,
+        "roofline": None,  # two kernels per sweep; per-kernel times from the rocprofv3 summary
+        "cpu_baseline": cpu,
+    }
+
+
 def bench_mlp(args):
     """The learned proposal's kernels on their own: batched inference (BF16
     features from HBM) and training steps (forward + backward + Adam), 2D
@@ -700,7 +800,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline and parity leg")
     ap.add_argument("--no-converge", action="store_true", help="skip the time-to-converged runs")
     ap.add_argument("--cpu-sweeps", type=int, default=150)
-    ap.add_argument("--workload", default="ogrid", choices=("ogrid", "ogrid_bl", "batched", "cubed_sphere", "mlp"))
+    ap.add_argument("--workload", default="ogrid", choices=("ogrid", "ogrid_bl", "ogrid_terms", "batched", "cubed_sphere",
+                                                              "mlp"))
     ap.add_argument("--graph-chunk", type=int, default=GRAPH_CHUNK,
                     help="sweeps per captured hipGraph replay (even; 0 = single launches, e.g. under a tracer)")
     args = ap.parse_args()
@@ -709,7 +810,7 @@ def main() -> int:
               "(the path would shard by blocks, one halo exchange per sweep, but only the one-GPU path is built)",
               file=sys.stderr)
         return 2
-    fn = {"ogrid": bench_ogrid, "ogrid_bl": bench_ogrid_bl, "batched": bench_batched, "cubed_sphere": bench_cubed,
+    fn = {"ogrid": bench_ogrid, "ogrid_bl": bench_ogrid_bl, "ogrid_terms": bench_ogrid_terms, "batched": bench_batched, "cubed_sphere": bench_cubed,
           "mlp": bench_mlp}[args.workload]
     print(json.dumps(fn(args)), flush=True)
     return 0
