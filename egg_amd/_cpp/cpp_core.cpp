@@ -48,6 +48,13 @@ int device_id(const std::string& device)
     throw py::value_error("device must be 'auto', 'cpu', 'gpu' or 'tables'; got '" + device + "'");
 }
 
+int recurrence_id(const std::string& r)
+{
+    if (r == "standard") return 0;
+    if (r == "one_reduction") return 1;
+    throw py::value_error("recurrence must be 'standard' or 'one_reduction'; got '" + r + "'");
+}
+
 int require_dim(int d)
 {
     if (d != 2 && d != 3) throw py::value_error("dim must be 2 or 3; got " + std::to_string(d));
@@ -74,6 +81,21 @@ DArr get_d(const py::dict& d, const char* key)
 {
     if (!d.contains(key)) throw py::value_error(std::string("missing wire field '") + key + "'");
     return d[key].cast<DArr>();
+}
+
+// Per-sample energy weights: one value per sample, or a single value shared
+// by every sample (reference src/bindings.cpp:247,364-366 reads that one with
+// stride 0); the shared form is expanded here so the C ABI sees one per sample.
+DArr sample_weights(DArr wt, int64_t n, const char* where)
+{
+    if ((int64_t)wt.size() == n) return wt;
+    if (wt.size() == 1) {
+        DArr full((py::ssize_t)n);
+        std::fill(full.mutable_data(), full.mutable_data() + n, wt.data()[0]);
+        return full;
+    }
+    throw py::value_error(std::string("validate_context: ") + where + " weight size " + std::to_string(wt.size()) +
+                          " != " + std::to_string(n) + " (one per sample, or a single shared value)");
 }
 
 // Keeps the converted arrays alive while the C structs point into them.
@@ -148,13 +170,8 @@ struct Wire {
                 eg.interior_block = keep(IArr((py::ssize_t)neg.size(), neg.data()));
                 eg.interior_logical = keep(IArr((py::ssize_t)zero.size(), zero.data()));
             }
-            if (g.contains("weight") && !g["weight"].is_none()) {
-                DArr wt = get_d(g, "weight");
-                if ((int64_t)wt.size() != eg.total_samples)
-                    throw py::value_error("validate_context: group weight size " + std::to_string(wt.size()) +
-                                          " != " + std::to_string(eg.total_samples));
-                eg.weight = keep(wt);
-            }
+            if (g.contains("weight") && !g["weight"].is_none())
+                eg.weight = keep(sample_weights(get_d(g, "weight"), eg.total_samples, "group"));
             if (g.contains("entities") && !g["entities"].is_none() && py::len(g["entities"]) > 0) {
                 // the reference's per-type SoA sub-dicts, flattened to one tag and
                 // record per group-local DOF (every DOF covered exactly once)
@@ -245,13 +262,8 @@ struct Wire {
         if (eW.size() % dd != 0) throw py::value_error("validate_context: energy_stencil W_inv size");
         es.w_rows = (int64_t)eW.size() / dd;
         es.W_inv = keep(eW);
-        if (e.contains("weight") && !e["weight"].is_none()) {
-            DArr wt = get_d(e, "weight");
-            if ((int64_t)wt.size() != es.num_samples)
-                throw py::value_error("validate_context: energy_stencil weight size " + std::to_string(wt.size()) +
-                                      " != " + std::to_string(es.num_samples));
-            es.weight = keep(wt);
-        }
+        if (e.contains("weight") && !e["weight"].is_none())
+            es.weight = keep(sample_weights(get_d(e, "weight"), es.num_samples, "energy_stencil"));
         if (e.contains("dir_num") && !e["dir_num"].is_none() && e["dir_num"].cast<int64_t>() > 0) {
             // whole-sample directional table (reference src/bindings.cpp:471-480)
             const int64_t n = e["dir_num"].cast<int64_t>();
@@ -785,7 +797,7 @@ PYBIND11_MODULE(cpp_core, m)
     m.def(
       "pcg_csr",
       [](const py::array_t<int64_t, py::array::c_style | py::array::forcecast>& rowptr, const IArr& col,
-         const DArr& val, const DArr& b, double rtol, int max_iter) {
+         const DArr& val, const DArr& b, double rtol, int max_iter, const std::string& recurrence) {
           const int64_t n = (int64_t)b.size();
           if ((int64_t)rowptr.size() != n + 1) throw py::value_error("pcg: rowptr must have n+1 entries");
           if (col.size() != val.size()) throw py::value_error("pcg: col and val sizes differ");
@@ -794,14 +806,36 @@ PYBIND11_MODULE(cpp_core, m)
           int iters = 0, status = 0;
           double relres = 0.0, ms = 0.0;
           check(eggmi_pcg_csr(n, rowptr.data(), col.data(), val.data(), b.data(), x.mutable_data(), rtol, max_iter,
-                              &iters, &relres, &status, &ms));
+                              recurrence_id(recurrence), &iters, &relres, &status, &ms));
           last_pcg_ms() = ms;
           return py::make_tuple(x, iters, relres, status);
       },
       py::arg("rowptr"), py::arg("col"), py::arg("val"), py::arg("b"), py::kw_only(), py::arg("rtol") = 1e-10,
-      py::arg("max_iter") = 200,
-      "Jacobi-preconditioned CG (FP64, on the GPU) for a CSR operator; returns (x, iters, relres, status).");
-    m.def("last_pcg_ms", [] { return last_pcg_ms(); }, "Device time (ms) of the last pcg_csr solve, uploads excluded.");
+      py::arg("max_iter") = 200, py::arg("recurrence") = "standard",
+      "Jacobi-preconditioned CG (FP64, on the GPU) for a CSR operator; recurrence 'standard' (the reference's) "
+      "or 'one_reduction' (Chronopoulos-Gear); returns (x, iters, relres, status).");
+    m.def(
+      "pcg_lattice",
+      [](const IArr& shape, double diag, double off, const DArr& b, double rtol, int max_iter,
+         const std::string& recurrence) {
+          const int dim = (int)shape.size();
+          py::array_t<double> x((py::ssize_t)b.size());
+          int64_t n = 1;
+          for (int k = 0; k < dim; ++k) n *= shape.data()[k];
+          if (n != (int64_t)b.size()) throw py::value_error("pcg: b must hold prod(shape) entries");
+          int iters = 0, status = 0;
+          double relres = 0.0, ms = 0.0;
+          check(eggmi_pcg_lattice(dim, shape.data(), diag, off, b.data(), x.mutable_data(), rtol, max_iter,
+                                  recurrence_id(recurrence), &iters, &relres, &status, &ms));
+          last_pcg_ms() = ms;
+          return py::make_tuple(x, iters, relres, status);
+      },
+      py::arg("shape"), py::arg("diag"), py::arg("off"), py::arg("b"), py::kw_only(), py::arg("rtol") = 1e-10,
+      py::arg("max_iter") = 200, py::arg("recurrence") = "standard",
+      "Jacobi-preconditioned CG (FP64, on the GPU) for the matrix-free (2 dim + 1)-point lattice operator "
+      "(diag, off; Dirichlet closure); returns (x, iters, relres, status).");
+    m.def("last_pcg_ms", [] { return last_pcg_ms(); },
+          "Device time (ms) of the last pcg_csr / pcg_lattice solve, uploads excluded.");
 
     m.def(
       "cpp_structured_sweep",

@@ -479,17 +479,84 @@ eggmi_status eggmi_session_set_proposal(eggmi_session* s, eggmi_mlp* m, int mode
     });
 }
 
+}  // extern "C"
+
+namespace
+{
+// One solve on a stream of its own: the work area in one allocation, the
+// device time of the solve (uploads and downloads excluded) in *solve_ms.
+template <class Solve>
+void pcg_on_device(int64_t n, const double* b, double* x, double* solve_ms, Solve&& run)
+{
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw eggmi::NoDevice("pcg: no HIP device");
+    hipStream_t s;
+    EGGMI_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    try {
+        eggmi::DArr<double> d_b, d_x, d_work;
+        d_b.upload(std::vector<double>(b, b + n));
+        d_x.alloc(std::max<int64_t>(n, 1));
+        d_work.alloc(eggmi::pcg_work_doubles(n));
+        eggmi::PcgWork w;
+        eggmi::pcg_carve(d_work.p, n, w);
+        EGGMI_HIP(hipEventCreate(&e0));
+        EGGMI_HIP(hipEventCreate(&e1));
+        EGGMI_HIP(hipEventRecord(e0, s));
+        run(d_b.p, d_x.p, w, s);
+        EGGMI_HIP(hipEventRecord(e1, s));
+        EGGMI_HIP(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        EGGMI_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (solve_ms) *solve_ms = ms;
+        if (n > 0) EGGMI_HIP(hipMemcpy(x, d_x.p, n * sizeof(double), hipMemcpyDeviceToHost));
+    } catch (...) {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        (void)hipStreamDestroy(s);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    EGGMI_HIP(hipStreamDestroy(s));
+}
+
+eggmi::PcgRecurrence pcg_recurrence(int recurrence)
+{
+    if (recurrence != 0 && recurrence != 1)
+        throw eggmi::InvalidArgument("pcg: recurrence must be 0 (standard) or 1 (one reduction per iteration)");
+    return recurrence == 0 ? eggmi::PcgRecurrence::kStandard : eggmi::PcgRecurrence::kOneReduction;
+}
+
+void pcg_common_checks(int64_t n, const double* b, const double* x, double rtol, int max_iter)
+{
+    if (n < 0) throw eggmi::InvalidArgument("pcg: n must be >= 0");
+    need(b, "b");
+    need(x, "x");
+    if (max_iter < 0) throw eggmi::InvalidArgument("pcg: max_iter must be >= 0");
+    if (!(rtol >= 0.0)) throw eggmi::InvalidArgument("pcg: rtol must be >= 0");
+}
+
+void pcg_report(const eggmi::PcgResult& res, int* iters, double* relres, int* status)
+{
+    if (iters) *iters = res.iters;
+    if (relres) *relres = res.relres;
+    if (status) *status = res.status;
+}
+
+
+}  // namespace
+
+extern "C" {
+
 eggmi_status eggmi_pcg_csr(int64_t n, const int64_t* rowptr, const int32_t* col, const double* val,
-                           const double* b, double* x, double rtol, int max_iter, int* iters, double* relres,
-                           int* status, double* solve_ms)
+                           const double* b, double* x, double rtol, int max_iter, int recurrence, int* iters,
+                           double* relres, int* status, double* solve_ms)
 {
     return guarded([&] {
-        if (n < 0) throw eggmi::InvalidArgument("pcg: n must be >= 0");
+        pcg_common_checks(n, b, x, rtol, max_iter);
+        const eggmi::PcgRecurrence rec = pcg_recurrence(recurrence);
         need(rowptr, "rowptr");
-        need(b, "b");
-        need(x, "x");
-        if (max_iter < 0) throw eggmi::InvalidArgument("pcg: max_iter must be >= 0");
-        if (!(rtol >= 0.0)) throw eggmi::InvalidArgument("pcg: rtol must be >= 0");
         const int64_t nnz = n ? rowptr[n] : 0;
         if (rowptr[0] != 0 || nnz < 0) throw eggmi::InvalidArgument("pcg: rowptr must start at 0");
         for (int64_t i = 0; i < n; ++i)
@@ -506,45 +573,48 @@ eggmi_status eggmi_pcg_csr(int64_t n, const int64_t* rowptr, const int32_t* col,
         if (!has_diag_all) throw eggmi::InvalidArgument("pcg: every row needs a nonzero diagonal (Jacobi)");
         int ndev = 0;
         if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw eggmi::NoDevice("pcg: no HIP device");
-        hipStream_t s;
-        EGGMI_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         eggmi::DArr<long long> d_rp;
         eggmi::DArr<int> d_col;
-        eggmi::DArr<double> d_val, d_b, d_x, d_r, d_z, d_p, d_ap, d_dinv, d_part;
-        eggmi::DArr<char> d_state;
-        try {
-            d_rp.upload(std::vector<long long>(rowptr, rowptr + n + 1));
-            d_col.upload(std::vector<int>(col, col + nnz));
-            d_val.upload(std::vector<double>(val, val + nnz));
-            d_b.upload(std::vector<double>(b, b + n));
-            for (auto* a : {&d_x, &d_r, &d_z, &d_p, &d_ap, &d_dinv}) a->alloc(std::max<int64_t>(n, 1));
-            const int64_t grid = (n + 255) / 256;
-            d_part.alloc(2 * std::max<int64_t>(grid, 1));
-            d_state.alloc(eggmi::pcg_state_bytes());
-            eggmi::CsrView A {n, d_rp.p, d_col.p, d_val.p};
-            eggmi::PcgWork w {d_r.p, d_z.p, d_p.p, d_ap.p, d_dinv.p, d_part.p, d_state.p};
-            eggmi::PcgResult res {};
-            hipEvent_t e0, e1;
-            EGGMI_HIP(hipEventCreate(&e0));
-            EGGMI_HIP(hipEventCreate(&e1));
-            EGGMI_HIP(hipEventRecord(e0, s));
-            if (n > 0) res = eggmi::pcg_solve_device(A, d_b.p, d_x.p, rtol, max_iter, w, s);
-            EGGMI_HIP(hipEventRecord(e1, s));
-            EGGMI_HIP(hipEventSynchronize(e1));
-            float ms = 0.0f;
-            EGGMI_HIP(hipEventElapsedTime(&ms, e0, e1));
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
-            if (solve_ms) *solve_ms = ms;
-            if (n > 0) EGGMI_HIP(hipMemcpy(x, d_x.p, n * sizeof(double), hipMemcpyDeviceToHost));
-            if (iters) *iters = res.iters;
-            if (relres) *relres = res.relres;
-            if (status) *status = res.status;
-        } catch (...) {
-            (void)hipStreamDestroy(s);
-            throw;
+        eggmi::DArr<double> d_val;
+        d_rp.upload(std::vector<long long>(rowptr, rowptr + n + 1));
+        d_col.upload(std::vector<int>(col, col + nnz));
+        d_val.upload(std::vector<double>(val, val + nnz));
+        const eggmi::CsrView A {n, d_rp.p, d_col.p, d_val.p};
+        eggmi::PcgResult res;
+        pcg_on_device(n, b, x, solve_ms, [&](const double* db, double* dx, eggmi::PcgWork& w, hipStream_t s) {
+            res = eggmi::pcg_solve_csr(A, db, dx, rtol, max_iter, w, s, eggmi::PcgLaunch::kQueued, rec);
+        });
+        pcg_report(res, iters, relres, status);
+    });
+}
+
+eggmi_status eggmi_pcg_lattice(int dim, const int32_t* shape, double diag, double off, const double* b, double* x,
+                               double rtol, int max_iter, int recurrence, int* iters, double* relres, int* status,
+                               double* solve_ms)
+{
+    return guarded([&] {
+        const eggmi::PcgRecurrence rec = pcg_recurrence(recurrence);
+        if (dim < 1 || dim > 3) throw eggmi::InvalidArgument("pcg: lattice dim must be 1, 2 or 3");
+        need(shape, "shape");
+        eggmi::LatticeView A;
+        A.dim = dim;
+        A.n = 1;
+        for (int k = 0; k < dim; ++k) {
+            if (shape[k] < 1) throw eggmi::InvalidArgument("pcg: lattice shape entries must be >= 1");
+            A.shape[k] = shape[k];
+            A.n *= shape[k];
         }
-        EGGMI_HIP(hipStreamDestroy(s));
+        if (A.n >= (1ll << 31)) throw eggmi::InvalidArgument("pcg: lattice of 2^31 or more unknowns");
+        if (!(diag != 0.0) || !std::isfinite(diag) || !std::isfinite(off))
+            throw eggmi::InvalidArgument("pcg: lattice diag must be finite and nonzero (Jacobi), off finite");
+        A.diag = diag;
+        A.off = off;
+        pcg_common_checks(A.n, b, x, rtol, max_iter);
+        eggmi::PcgResult res;
+        pcg_on_device(A.n, b, x, solve_ms, [&](const double* db, double* dx, eggmi::PcgWork& w, hipStream_t s) {
+            res = eggmi::pcg_solve_lattice(A, db, dx, rtol, max_iter, w, s, eggmi::PcgLaunch::kQueued, rec);
+        });
+        pcg_report(res, iters, relres, status);
     });
 }
 

@@ -1,214 +1,648 @@
 // Jacobi-preconditioned conjugate gradients on the MI355X in FP64
-// (vectors and inner products), for a CSR operator: the solver skeleton of
-// the reference's control-net Gauss-Newton step (src/control_solve.hpp:
-// 761-818): x0 = 0, r = b, z = M^-1 r, p = z; per iteration Ap, alpha =
-// rz / pAp (stop on non-positive curvature), x += alpha p, r -= alpha Ap,
-// stop when |r|^2 <= rtol^2 |b|^2, beta = rz'/rz, p = z + beta p.
+// (vectors and inner products): the solver skeleton of the reference's
+// control-net Gauss-Newton step (src/control_solve.hpp:761-818). x0 = 0,
+// stop when |r|^2 <= rtol^2 |b|^2 (iterations counted with the converging
+// one), stop on non-positive curvature p.Ap <= 0 (not counted), nothing
+// checked before the first iteration. Two recurrences, the same iterates in
+// exact arithmetic:
 //
-// The scalars never leave the device: each reduction writes one FP64
-// partial per workgroup and a one-workgroup kernel folds them in workgroup
-// order (deterministic) and updates alpha / beta and the stop flag. A kernel
-// boundary, not an in-kernel device-scope fence, makes the partials of all
-// XCDs visible (a device-scope release per workgroup writes back its XCD's
-// L2 and costs far more than a launch). Five launches per iteration, no host
-// round trip; the host polls the flag every few iterations.
+// kStandard, the reference's: r = b, z = M^-1 r, p = z; per iteration Ap,
+//   alpha = rz / pAp, x += alpha p, r -= alpha Ap, check |r|, z = M^-1 r,
+//   beta = rz' / rz, p = z + beta p. Two launches per iteration:
+//   step_a(t): fold the r.r / r.z partials of iteration t-1, decide
+//     convergence, beta; p_t = z + beta p_(t-1) and A p_t = A z + beta A
+//     p_(t-1) (the operator's loads do not wait for beta, so they overlap
+//     the fold), the p.Ap partial of every workgroup;
+//   step_b(t): fold p.Ap, alpha, update x, r, z, the r.r / r.z partials.
+//
+// kOneReduction (Chronopoulos and Gear, 1989): u = M^-1 r and w = A u are
+//   carried as well, so the three inner products an iteration needs,
+//   (r, u), (w, u) and (r, r), are reduced together: gamma = (r, u),
+//   beta = gamma / gamma_prev, pAp = (w, u) - beta gamma / alpha_prev,
+//   alpha = gamma / pAp; p = u + beta p, s = w + beta s (= A p),
+//   x += alpha p, r -= alpha s, u -= alpha M^-1 s, w = A u. One launch per
+//   iteration: A u_(i+1) is formed from the iteration-i values at the
+//   neighbours, A u - alpha A M^-1 (w + beta s), so no kernel reads a vector
+//   entry another workgroup of the same kernel writes.
+//
+// Every workgroup folds the partials itself in the same fixed order, so all
+// of them take the same decisions, and the grid never synchronises inside a
+// kernel: a kernel boundary is the visibility point (a device-scope release
+// per workgroup writes back its XCD's L2, which costs more than a launch).
+// Scalars written by block 0 live in parity-indexed slots (iteration t uses
+// slot t & 1), so no kernel reads a slot that a block of the same kernel
+// writes; vectors read at neighbours alternate between two buffers.
 #include "pcg.hpp"
 
+#include <algorithm>
 #include <cmath>
 
 namespace eggmi
 {
 
-constexpr int kPcgThreads = 256;
+namespace
+{
 
 struct PcgState {
-    double b2, stop2, rz, rr, pap, alpha, beta;
-    int done;  // 1 converged, 2 non-positive curvature, 3 zero right-hand side
+    double rtol2, b2, stop2;
+    double rz[2];     // standard: r.z entering iteration t, slot t & 1
+    double rr[2];     // standard: r.r entering iteration t (b.b at t = 0)
+    double alpha[2];  // one-reduction: alpha_t, slot t & 1
+    double gamma[2];  // one-reduction: (r, u) entering iteration t
+    int t[2];         // iteration index of the last iteration kernel of each parity
+    int halt_a[2], halt_b[2];  // 1: the solve has ended (written by that step)
+    int max_iter;
+    int status;  // -1 while running, then the PcgResult status
     int iters;
+    double rr_end;
 };
 
+// 256-thread workgroup sums; every thread receives the same bits.
 __device__ __forceinline__ double wg_sum(double v)
 {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     __shared__ double red[kPcgThreads / 64];
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) red[w] = v;
+    __syncthreads();  // the previous call's readers are done with red
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
-    double t = 0.0;
-    if (threadIdx.x == 0)
-        for (int i = 0; i < kPcgThreads / 64; ++i) t += red[i];
-    __syncthreads();
-    return t;  // valid in thread 0
+    return (red[0] + red[1]) + (red[2] + red[3]);
 }
 
-// fixed-order fold of n partials by one workgroup (valid in thread 0)
-__device__ __forceinline__ double fold(const double* part, int n)
+template <int K> __device__ __forceinline__ void wg_sums(double (&v)[K])
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+        for (int k = 0; k < K; ++k) v[k] += __shfl_xor(v[k], o, 64);
+    __shared__ double red[K][kPcgThreads / 64];
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int k = 0; k < K; ++k) red[k][threadIdx.x >> 6] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = (red[k][0] + red[k][1]) + (red[k][2] + red[k][3]);
+}
+
+// this thread's share of a fixed-order fold of n partials; a workgroup sum
+// of it gives the same bits in every workgroup
+__device__ __forceinline__ double part_sum(const double* part, long long n)
 {
     double v = 0.0;
-    for (int i = threadIdx.x; i < n; i += kPcgThreads) v += part[i];
-    return wg_sum(v);
+    for (long long i = threadIdx.x; i < n; i += kPcgThreads) v += part[i];
+    return v;
 }
 
-__global__ __launch_bounds__(kPcgThreads) void pcg_init(const CsrView A, const double* __restrict__ dinv,
-                                                         const double* __restrict__ b, double* x, double* r,
-                                                         double* z, double* p, double* part)
+// partial k (0..2) of parity par: 6 arrays of np partials after the p.Ap ones
+__device__ __forceinline__ double* part_of(double* part, long long np, int par, int k)
 {
+    return part + np * (1 + 3 * par + k);
+}
+
+struct CsrOp {
+    CsrView A;
+    __device__ long long n() const { return A.n; }
+    __device__ double inv_diag(long long i) const
+    {
+        double d = 0.0;
+        for (long long q = A.rowptr[i]; q < A.rowptr[i + 1]; ++q)
+            if (A.col[q] == i) d += A.val[q];
+        return 1.0 / d;
+    }
+    // (A u)_i and (A v)_i in one walk over the row
+    template <bool TWO> __device__ void row(long long i, const double* u, const double* v, double& au, double& av) const
+    {
+        double su = 0.0, sv = 0.0;
+        for (long long q = A.rowptr[i]; q < A.rowptr[i + 1]; ++q) {
+            const int j = A.col[q];
+            const double a = A.val[q];
+            su += a * u[j];
+            if (TWO) sv += a * v[j];
+        }
+        au = su;
+        av = sv;
+    }
+    // (A u)_i, (A D w)_i and (A D s)_i, D = M^-1 = diag(dinv)
+    __device__ void row3(long long i, const double* u, const double* w, const double* s, const double* dinv,
+                         double& au, double& adw, double& ads) const
+    {
+        double su = 0.0, sw = 0.0, ss = 0.0;
+        for (long long q = A.rowptr[i]; q < A.rowptr[i + 1]; ++q) {
+            const int j = A.col[q];
+            const double a = A.val[q], ad = a * dinv[j];
+            su += a * u[j];
+            sw += ad * w[j];
+            ss += ad * s[j];
+        }
+        au = su;
+        adw = sw;
+        ads = ss;
+    }
+};
+
+// Unknowns are fewer than 2^31 (checked by the entry point), so the lattice
+// coordinates come from 32-bit divisions, once per row.
+struct LatticeOp {
+    LatticeView A;
+    __device__ long long n() const { return A.n; }
+    __device__ double inv_diag(long long) const { return 1.0 / A.diag; }
+    // the in-lattice neighbours of row i: their count and indices
+    __device__ int neighbours(long long i, int (&nb)[6]) const
+    {
+        unsigned rem = (unsigned)i, stride = 1;
+        int m = 0;
+        for (int k = A.dim - 1; k >= 0; --k) {
+            const unsigned e = (unsigned)A.shape[k], c = rem % e;
+            rem /= e;
+            if (c > 0) nb[m++] = (int)(i - stride);
+            if (c + 1 < e) nb[m++] = (int)(i + stride);
+            stride *= e;
+        }
+        return m;
+    }
+    template <bool TWO> __device__ void row(long long i, const double* u, const double* v, double& au, double& av) const
+    {
+        int nb[6];
+        const int m = neighbours(i, nb);
+        double su = 0.0, sv = 0.0;
+        for (int k = 0; k < m; ++k) {
+            su += u[nb[k]];
+            if (TWO) sv += v[nb[k]];
+        }
+        au = A.diag * u[i] + A.off * su;
+        av = TWO ? A.diag * v[i] + A.off * sv : 0.0;
+    }
+    // (A u)_i, (A D w)_i and (A D s)_i, D = diag^-1 I
+    __device__ void row3(long long i, const double* u, const double* w, const double* s, const double*, double& au,
+                         double& adw, double& ads) const
+    {
+        int nb[6];
+        const int m = neighbours(i, nb);
+        double su = 0.0, sw = 0.0, ss = 0.0;
+        for (int k = 0; k < m; ++k) {
+            su += u[nb[k]];
+            sw += w[nb[k]];
+            ss += s[nb[k]];
+        }
+        const double d = 1.0 / A.diag;
+        au = A.diag * u[i] + A.off * su;
+        adw = d * (A.diag * w[i] + A.off * sw);
+        ads = d * (A.diag * s[i] + A.off * ss);
+    }
+};
+
+__device__ __forceinline__ void finish(PcgState* st, int status, int iters, double rr)
+{
+    st->status = status;
+    st->iters = iters;
+    st->rr_end = rr;
+}
+
+__device__ __forceinline__ void reset_state(PcgState* st, double rtol2, int max_iter)
+{
+    st->rtol2 = rtol2;
+    st->t[1] = -1;
+    st->halt_a[1] = 0;
+    st->halt_b[1] = 0;
+    st->max_iter = max_iter;
+    st->status = -1;
+    st->iters = 0;
+    st->rr_end = 0.0;
+}
+
+// ------------------------------------------------------------ kStandard
+
+template <class Op>
+__global__ __launch_bounds__(kPcgThreads) void pcg_start(const Op op, const double* __restrict__ b, double* x,
+                                                         double* r, double* z, double* dinv, double* part,
+                                                         PcgState* st, double rtol2, int max_iter)
+{
+    const long long n = op.n(), np = gridDim.x;
     const long long i = (long long)blockIdx.x * kPcgThreads + threadIdx.x;
-    double rr = 0.0, rz = 0.0;
-    if (i < A.n) {
-        const double bi = b[i];
+    double v[2] = {0.0, 0.0};
+    if (i < n) {
+        const double bi = b[i], di = op.inv_diag(i);
+        dinv[i] = di;
         x[i] = 0.0;
         r[i] = bi;
-        const double zi = dinv[i] * bi;
+        const double zi = di * bi;
         z[i] = zi;
-        p[i] = zi;
-        rr = bi * bi;
-        rz = bi * zi;
+        v[0] = bi * bi;
+        v[1] = bi * zi;
     }
-    const double s_rr = wg_sum(rr), s_rz = wg_sum(rz);
+    wg_sums<2>(v);
     if (threadIdx.x == 0) {
-        part[blockIdx.x] = s_rr;
-        part[gridDim.x + blockIdx.x] = s_rz;
+        // the partials a step_b of parity 1 would leave: step_a(0) folds them
+        part_of(part, np, 1, 0)[blockIdx.x] = v[0];
+        part_of(part, np, 1, 1)[blockIdx.x] = v[1];
+        if (blockIdx.x == 0) reset_state(st, rtol2, max_iter);
     }
 }
 
-__global__ __launch_bounds__(kPcgThreads) void pcg_init_finish(const double* part, int nparts, PcgState* st,
-                                                                double rtol2)
+// Both steps issue every load they need (the state slots, the partials and
+// the vector entries) before anything waits on one, so the latencies of the
+// fold, the state and the vectors overlap; the decisions come after.
+template <class Op>
+__global__ __launch_bounds__(kPcgThreads) void pcg_step_a(const Op op, const double* __restrict__ z, double* p_new,
+                                                          const double* __restrict__ p_old, double* ap, double* part,
+                                                          PcgState* st, int par)
 {
-    const double b2 = fold(part, nparts);
-    const double rz0 = fold(part + nparts, nparts);
-    if (threadIdx.x == 0) {
-        st->b2 = b2;
-        st->rr = b2;
-        st->rz = rz0;
-        st->stop2 = rtol2 * b2;
-        st->done = (b2 == 0.0) ? 3 : 0;
-        st->iters = 0;
-    }
-}
-
-__global__ __launch_bounds__(kPcgThreads) void pcg_matvec(const CsrView A, const double* __restrict__ p, double* ap,
-                                                          double* part, const PcgState* st)
-{
-    if (st->done) return;
+    const int q = par ^ 1;
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    const long long n = op.n(), np = gridDim.x;
     const long long i = (long long)blockIdx.x * kPcgThreads + threadIdx.x;
-    double d = 0.0;
-    if (i < A.n) {
-        double acc = 0.0;
-        for (long long q = A.rowptr[i]; q < A.rowptr[i + 1]; ++q) acc += A.val[q] * p[A.col[q]];
-        ap[i] = acc;
-        d = p[i] * acc;
+    const long long ii = i < n ? i : n - 1;  // loads stay in range; stores are guarded
+    const int halted = st->halt_a[q] | st->halt_b[q];
+    const int t = st->t[q] + 1;
+    const double rz_old = st->rz[q], stop2 = st->stop2;
+    const int max_iter = st->max_iter;
+    double v[2] = {part_sum(part_of(part, np, q, 0), np), part_sum(part_of(part, np, q, 1), np)};
+    // A z and A p_(t-1) do not depend on beta (p_(t-1) is not yet written at t = 0: dropped below)
+    double az, aq;
+    op.template row<true>(ii, z, p_old, az, aq);
+    const double zi = z[ii];
+    double pi = p_old[ii];
+    if (halted) {
+        if (lead) st->halt_a[par] = 1;
+        return;
     }
-    const double s = wg_sum(d);
-    if (threadIdx.x == 0) part[blockIdx.x] = s;
+    wg_sums<2>(v);
+    const double rr = v[0], rz = v[1];
+    double beta = 0.0;
+    if (t == 0) {
+        aq = 0.0;
+        pi = 0.0;
+        if (rr == 0.0 || max_iter == 0) {  // b = 0, or no iteration allowed
+            if (lead) {
+                st->b2 = rr;
+                finish(st, rr == 0.0 ? 3 : 0, 0, rr);
+                st->halt_a[par] = 1;
+            }
+            return;
+        }
+    } else {
+        const bool conv = rr <= stop2;
+        if (conv || t >= max_iter) {
+            if (lead) {
+                finish(st, conv ? 1 : 0, t, rr);
+                st->halt_a[par] = 1;
+            }
+            return;
+        }
+        beta = rz / rz_old;
+    }
+    const double pn = __builtin_fma(beta, pi, zi);
+    const double apn = __builtin_fma(beta, aq, az);
+    if (i < n) {
+        p_new[i] = pn;
+        ap[i] = apn;
+    }
+    const double pap = wg_sum(i < n ? pn * apn : 0.0);
+    if (threadIdx.x == 0) part[blockIdx.x] = pap;
+    if (lead) {
+        st->t[par] = t;
+        st->rz[par] = rz;
+        st->rr[par] = rr;
+        if (t == 0) {
+            st->b2 = rr;
+            st->stop2 = st->rtol2 * rr;
+        }
+        st->halt_a[par] = 0;
+    }
 }
 
-__global__ __launch_bounds__(kPcgThreads) void pcg_alpha(const double* part, int nparts, PcgState* st)
+template <class Op>
+__global__ __launch_bounds__(kPcgThreads) void pcg_step_b(const Op op, const double* __restrict__ dinv, double* x,
+                                                          double* r, double* z, const double* __restrict__ p,
+                                                          const double* __restrict__ ap, double* part, PcgState* st,
+                                                          int par)
 {
-    if (st->done) return;
-    const double pap = fold(part, nparts);
-    if (threadIdx.x == 0) {
-        st->pap = pap;
-        if (!(pap > 0.0))
-            st->done = 2;
-        else
-            st->alpha = st->rz / pap;
-    }
-}
-
-__global__ __launch_bounds__(kPcgThreads) void pcg_update(const CsrView A, const double* __restrict__ dinv,
-                                                          double* x, double* r, double* z, const double* p,
-                                                          const double* ap, double* part, const PcgState* st)
-{
-    if (st->done) return;
-    const double a = st->alpha;
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    const long long n = op.n(), np = gridDim.x;
     const long long i = (long long)blockIdx.x * kPcgThreads + threadIdx.x;
-    double rr = 0.0, rz = 0.0;
-    if (i < A.n) {
-        x[i] += a * p[i];
-        const double ri = r[i] - a * ap[i];
-        r[i] = ri;
-        const double zi = dinv[i] * ri;
-        z[i] = zi;
-        rr = ri * ri;
-        rz = ri * zi;
+    const long long ii = i < n ? i : n - 1;
+    const int halted = st->halt_a[par];
+    const double rz_t = st->rz[par];
+    double pap = part_sum(part, np);
+    const double xi = x[ii], pi = p[ii], ri = r[ii], api = ap[ii], di = dinv[ii];
+    if (halted) {
+        if (lead) st->halt_b[par] = 1;
+        return;
     }
-    const double s_rr = wg_sum(rr), s_rz = wg_sum(rz);
+    pap = wg_sum(pap);
+    if (!(pap > 0.0)) {  // non-positive curvature: stop at the last iterate
+        if (lead) {
+            finish(st, 2, st->t[par], st->rr[par]);
+            st->halt_b[par] = 1;
+        }
+        return;
+    }
+    const double alpha = rz_t / pap;
+    double v[2] = {0.0, 0.0};
+    if (i < n) {
+        x[i] = xi + alpha * pi;
+        const double rn = ri - alpha * api;
+        r[i] = rn;
+        const double zn = di * rn;
+        z[i] = zn;
+        v[0] = rn * rn;
+        v[1] = rn * zn;
+    }
+    wg_sums<2>(v);
     if (threadIdx.x == 0) {
-        part[blockIdx.x] = s_rr;
-        part[gridDim.x + blockIdx.x] = s_rz;
+        part_of(part, np, par, 0)[blockIdx.x] = v[0];
+        part_of(part, np, par, 1)[blockIdx.x] = v[1];
+    }
+    if (lead) st->halt_b[par] = 0;
+}
+
+// ------------------------------------------------------------ kOneReduction
+
+template <class Op>
+__global__ __launch_bounds__(kPcgThreads) void cg1_diag(const Op op, double* dinv)
+{
+    const long long i = (long long)blockIdx.x * kPcgThreads + threadIdx.x;
+    if (i < op.n()) dinv[i] = op.inv_diag(i);
+}
+
+// x = 0, r = b, u = M^-1 b, w = A u, p = s = 0; the partials of iteration 0
+// in the parity-1 slots
+template <class Op>
+__global__ __launch_bounds__(kPcgThreads) void cg1_start(const Op op, const double* __restrict__ b, double* x,
+                                                         double* r, const double* __restrict__ dinv, double* p,
+                                                         double* u, double* w, double* s, double* part, PcgState* st,
+                                                         double rtol2, int max_iter)
+{
+    const long long n = op.n(), np = gridDim.x;
+    const long long i = (long long)blockIdx.x * kPcgThreads + threadIdx.x;
+    const long long ii = i < n ? i : n - 1;
+    double au, adw, ads;  // A D b: w_0 = A u_0 with u_0 = D b
+    op.row3(ii, b, b, b, dinv, au, adw, ads);
+    double v[3] = {0.0, 0.0, 0.0};
+    if (i < n) {
+        const double bi = b[i], ui = dinv[i] * bi;
+        x[i] = 0.0;
+        r[i] = bi;
+        p[i] = 0.0;
+        s[i] = 0.0;
+        u[i] = ui;
+        w[i] = adw;
+        v[0] = bi * ui;
+        v[1] = adw * ui;
+        v[2] = bi * bi;
+    }
+    wg_sums<3>(v);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) part_of(part, np, 1, k)[blockIdx.x] = v[k];
+        if (blockIdx.x == 0) reset_state(st, rtol2, max_iter);
     }
 }
 
-__global__ __launch_bounds__(kPcgThreads) void pcg_beta(const double* part, int nparts, PcgState* st)
+template <class Op>
+__global__ __launch_bounds__(kPcgThreads) void cg1_iter(const Op op, double* x, double* r,
+                                                        const double* __restrict__ dinv, double* p,
+                                                        const double* __restrict__ u_old,
+                                                        const double* __restrict__ w_old,
+                                                        const double* __restrict__ s_old, double* u_new, double* w_new,
+                                                        double* s_new, double* part, PcgState* st, int par)
 {
-    if (st->done) return;
-    const double rr_all = fold(part, nparts);
-    const double rz_new = fold(part + nparts, nparts);
-    if (threadIdx.x == 0) {
-        st->rr = rr_all;
-        st->iters += 1;
-        if (rr_all <= st->stop2) {
-            st->done = 1;
+    const int q = par ^ 1;
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    const long long n = op.n(), np = gridDim.x;
+    const long long i = (long long)blockIdx.x * kPcgThreads + threadIdx.x;
+    const long long ii = i < n ? i : n - 1;
+    const int halted = st->halt_a[q];
+    const int t = st->t[q] + 1;
+    const double alpha_prev = st->alpha[q], gamma_prev = st->gamma[q], stop2 = st->stop2;
+    const int max_iter = st->max_iter;
+    double v[3] = {part_sum(part_of(part, np, q, 0), np), part_sum(part_of(part, np, q, 1), np),
+                   part_sum(part_of(part, np, q, 2), np)};
+    // A u, A D w and A D s at the iteration-t values: A u_(t+1) = A u - alpha (A D w + beta A D s)
+    double au, adw, ads;
+    op.row3(ii, u_old, w_old, s_old, dinv, au, adw, ads);
+    const double xi = x[ii], ri = r[ii], pi = p[ii], ui = u_old[ii], wi = w_old[ii], si = s_old[ii], di = dinv[ii];
+    if (halted) {
+        if (lead) st->halt_a[par] = 1;
+        return;
+    }
+    wg_sums<3>(v);
+    const double gamma = v[0], delta = v[1], rr = v[2];
+    double beta = 0.0, pap = delta;
+    if (t == 0) {
+        if (rr == 0.0 || max_iter == 0) {  // b = 0, or no iteration allowed
+            if (lead) {
+                st->b2 = rr;
+                finish(st, rr == 0.0 ? 3 : 0, 0, rr);
+                st->halt_a[par] = 1;
+            }
+            return;
+        }
+    } else {
+        const bool conv = rr <= stop2;
+        if (conv || t >= max_iter) {
+            if (lead) {
+                finish(st, conv ? 1 : 0, t, rr);
+                st->halt_a[par] = 1;
+            }
+            return;
+        }
+        beta = gamma / gamma_prev;
+        pap = delta - beta * gamma / alpha_prev;
+    }
+    if (!(pap > 0.0)) {  // non-positive curvature: stop at the last iterate
+        if (lead) {
+            if (t == 0) st->b2 = rr;
+            finish(st, 2, t, rr);
+            st->halt_a[par] = 1;
+        }
+        return;
+    }
+    const double alpha = gamma / pap;
+    const double pn = __builtin_fma(beta, pi, ui);
+    const double sn = __builtin_fma(beta, si, wi);
+    const double un = __builtin_fma(-alpha, di * sn, ui);
+    const double rn = __builtin_fma(-alpha, sn, ri);
+    const double wn = __builtin_fma(-alpha, __builtin_fma(beta, ads, adw), au);
+    double o[3] = {0.0, 0.0, 0.0};
+    if (i < n) {
+        x[i] = __builtin_fma(alpha, pn, xi);
+        r[i] = rn;
+        p[i] = pn;
+        s_new[i] = sn;
+        u_new[i] = un;
+        w_new[i] = wn;
+        o[0] = rn * un;
+        o[1] = wn * un;
+        o[2] = rn * rn;
+    }
+    wg_sums<3>(o);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) part_of(part, np, par, k)[blockIdx.x] = o[k];
+    if (lead) {
+        st->t[par] = t;
+        st->alpha[par] = alpha;
+        st->gamma[par] = gamma;
+        if (t == 0) {
+            st->b2 = rr;
+            st->stop2 = st->rtol2 * rr;
+        }
+        st->halt_a[par] = 0;
+    }
+}
+
+// ------------------------------------------------------------ drivers
+
+template <class Op> struct Launcher {
+    const Op& op;
+    unsigned grid;
+    double* x;
+    PcgWork& w;
+    PcgRecurrence rec;
+    PcgState* st() const { return reinterpret_cast<PcgState*>(w.state); }
+    void start(const double* b, double rtol2, int max_iter, hipStream_t s) const
+    {
+        if (rec == PcgRecurrence::kStandard) {
+            pcg_start<Op><<<grid, kPcgThreads, 0, s>>>(op, b, x, w.r, w.u[0], w.dinv, w.part, st(), rtol2, max_iter);
         } else {
-            st->beta = rz_new / st->rz;
-            st->rz = rz_new;
+            cg1_diag<Op><<<grid, kPcgThreads, 0, s>>>(op, w.dinv);
+            cg1_start<Op><<<grid, kPcgThreads, 0, s>>>(op, b, x, w.r, w.dinv, w.p, w.u[1], w.w[1], w.s[1], w.part,
+                                                       st(), rtol2, max_iter);
         }
     }
-}
+    // one iteration of parity par; check_only: the test that follows the last one
+    void iteration(int par, hipStream_t s, bool check_only = false) const
+    {
+        const int q = par ^ 1;
+        if (rec == PcgRecurrence::kStandard) {
+            // z = u[0], Ap = w[0], p alternates over s[0] / s[1]
+            pcg_step_a<Op><<<grid, kPcgThreads, 0, s>>>(op, w.u[0], w.s[par], w.s[q], w.w[0], w.part, st(), par);
+            if (!check_only)
+                pcg_step_b<Op><<<grid, kPcgThreads, 0, s>>>(op, w.dinv, x, w.r, w.u[0], w.s[par], w.w[0], w.part,
+                                                            st(), par);
+        } else {
+            cg1_iter<Op><<<grid, kPcgThreads, 0, s>>>(op, x, w.r, w.dinv, w.p, w.u[q], w.w[q], w.s[q], w.u[par],
+                                                      w.w[par], w.s[par], w.part, st(), par);
+        }
+    }
+};
 
-__global__ __launch_bounds__(kPcgThreads) void pcg_direction(long long n, const double* z, double* p,
-                                                             const PcgState* st)
+template <class Op>
+PcgResult solve(const Op& op, long long n, const double* b, double* x, double rtol, int max_iter, PcgWork& w,
+                hipStream_t s, PcgLaunch mode, PcgRecurrence rec)
 {
-    if (st->done) return;
-    const double beta = st->beta;
-    const long long i = (long long)blockIdx.x * kPcgThreads + threadIdx.x;
-    if (i < n) p[i] = z[i] + beta * p[i];
-}
-
-__global__ __launch_bounds__(kPcgThreads) void jacobi_diag(const CsrView A, double* dinv)
-{
-    const long long i = (long long)blockIdx.x * kPcgThreads + threadIdx.x;
-    if (i >= A.n) return;
-    double d = 0.0;
-    for (long long q = A.rowptr[i]; q < A.rowptr[i + 1]; ++q)
-        if (A.col[q] == i) d += A.val[q];
-    dinv[i] = 1.0 / d;
-}
-
-PcgResult pcg_solve_device(const CsrView& A, const double* b, double* x, double rtol, int max_iter,
-                           PcgWork& w, hipStream_t s)
-{
-    const unsigned grid = (unsigned)((A.n + kPcgThreads - 1) / kPcgThreads);
-    const int np = (int)grid;
-    jacobi_diag<<<grid, kPcgThreads, 0, s>>>(A, w.dinv);
-    PcgState* st = reinterpret_cast<PcgState*>(w.state);
-    pcg_init<<<grid, kPcgThreads, 0, s>>>(A, w.dinv, b, x, w.r, w.z, w.p, w.part);
-    pcg_init_finish<<<1, kPcgThreads, 0, s>>>(w.part, np, st, rtol * rtol);
+    PcgResult res;
+    if (n <= 0) {
+        res.status = 3;
+        return res;
+    }
+    const Launcher<Op> L {op, (unsigned)pcg_parts(n), x, w, rec};
+    PcgState* st = L.st();
+    // iterations are launched kernel by kernel, kPcgChunk at a time
+    // (parities 0, 1, 0, ...); after each chunk the host reads the device
+    // status: synchronously when nothing may be allocated, else into pinned
+    // memory with one chunk of look-ahead (chunk k's status is read once
+    // chunk k + 1 is queued, so the device never idles on the poll; a solve
+    // that ended runs at most two chunks of iteration kernels that exit at
+    // once). Replaying the chunks as captured hipGraphs measured slower on
+    // the MI355X (33^3 lattice, 126 iterations: 1.27 against 1.09 ms).
+    int* pin = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    auto release = [&] {
+        for (auto e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (pin) (void)hipHostFree(pin);
+    };
+    bool halted = false;
+    try {
+        if (mode == PcgLaunch::kQueued) {
+            hip_check(hipHostMalloc((void**)&pin, 2 * sizeof(int), hipHostMallocDefault), "pcg pinned status");
+            for (auto& e : ev) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "pcg event");
+        }
+        L.start(b, rtol * rtol, max_iter, s);
+        int t = 0, k = 0, status = -1;
+        while (!halted && t < max_iter) {
+            const int m = std::min(kPcgChunk, max_iter - t);
+            for (int c = 0; c < m; ++c) L.iteration((t + c) & 1, s);
+            t += m;
+            if (mode == PcgLaunch::kNoAlloc) {
+                hip_check(hipMemcpyAsync(&status, &st->status, sizeof(int), hipMemcpyDeviceToHost, s), "pcg poll");
+                hip_check(hipStreamSynchronize(s), "pcg poll");
+                halted = status >= 0;
+                continue;
+            }
+            hip_check(hipMemcpyAsync(&pin[k & 1], &st->status, sizeof(int), hipMemcpyDeviceToHost, s), "pcg poll");
+            hip_check(hipEventRecord(ev[k & 1], s), "pcg poll");
+            if (k > 0) {
+                hip_check(hipEventSynchronize(ev[(k - 1) & 1]), "pcg poll");
+                halted = pin[(k - 1) & 1] >= 0;
+            }
+            ++k;
+        }
+        hip_check(hipStreamSynchronize(s), "pcg poll");
+        if (!halted && k > 0) halted = pin[(k - 1) & 1] >= 0;
+    } catch (...) {
+        release();
+        throw;
+    }
+    release();
+    // the check that follows the last iteration (or the b = 0 / max_iter = 0 exits)
+    if (!halted) L.iteration(max_iter & 1, s, true);
     PcgState h {};
-    constexpr int kPoll = 16;
-    for (int it = 0; it < max_iter; ++it) {
-        pcg_matvec<<<grid, kPcgThreads, 0, s>>>(A, w.p, w.ap, w.part, st);
-        pcg_alpha<<<1, kPcgThreads, 0, s>>>(w.part, np, st);
-        pcg_update<<<grid, kPcgThreads, 0, s>>>(A, w.dinv, x, w.r, w.z, w.p, w.ap, w.part, st);
-        pcg_beta<<<1, kPcgThreads, 0, s>>>(w.part, np, st);
-        pcg_direction<<<grid, kPcgThreads, 0, s>>>(A.n, w.z, w.p, st);
-        if ((it + 1) % kPoll == 0 || it + 1 == max_iter) {
-            hip_check(hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s), "pcg poll");
-            hip_check(hipStreamSynchronize(s), "pcg poll");
-            if (h.done) break;
-        }
-    }
     hip_check(hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s), "pcg result");
     hip_check(hipStreamSynchronize(s), "pcg result");
-    PcgResult res;
     res.iters = h.iters;
-    res.relres = h.b2 > 0.0 ? std::sqrt(h.rr / h.b2) : 0.0;
-    res.status = h.done;
+    res.status = h.status;
+    res.relres = h.b2 > 0.0 ? std::sqrt(h.rr_end / h.b2) : 0.0;
     return res;
 }
 
+constexpr size_t align256(size_t doubles) { return (doubles + 31) & ~size_t(31); }
+
+constexpr size_t work_vectors = 9;  // r, dinv, p, u[2], w[2], s[2]
+
+constexpr size_t work_doubles(long long n)
+{
+    const long long m = n > 1 ? n : 1, np = pcg_parts(n) > 1 ? pcg_parts(n) : 1;
+    return work_vectors * align256((size_t)m) + align256(7 * (size_t)np) + align256(sizeof(PcgState) / 8 + 1);
+}
+
+}  // namespace
+
 size_t pcg_state_bytes() { return sizeof(PcgState); }
+
+size_t pcg_work_doubles(long long n) { return work_doubles(n); }
+
+void pcg_carve(double* base, long long n, PcgWork& w)
+{
+    const size_t v = align256((size_t)std::max<long long>(n, 1));
+    double* const vec[work_vectors] = {base, base + v, base + 2 * v, base + 3 * v, base + 4 * v,
+                                       base + 5 * v, base + 6 * v, base + 7 * v, base + 8 * v};
+    w.r = vec[0];
+    w.dinv = vec[1];
+    w.p = vec[2];
+    w.u[0] = vec[3];
+    w.u[1] = vec[4];
+    w.w[0] = vec[5];
+    w.w[1] = vec[6];
+    w.s[0] = vec[7];
+    w.s[1] = vec[8];
+    w.part = base + work_vectors * v;
+    w.state = w.part + align256(7 * (size_t)std::max<long long>(pcg_parts(n), 1));
+}
+
+PcgResult pcg_solve_csr(const CsrView& A, const double* b, double* x, double rtol, int max_iter, PcgWork& w,
+                        hipStream_t s, PcgLaunch mode, PcgRecurrence rec)
+{
+    return solve(CsrOp {A}, A.n, b, x, rtol, max_iter, w, s, mode, rec);
+}
+
+PcgResult pcg_solve_lattice(const LatticeView& A, const double* b, double* x, double rtol, int max_iter,
+                            PcgWork& w, hipStream_t s, PcgLaunch mode, PcgRecurrence rec)
+{
+    return solve(LatticeOp {A}, A.n, b, x, rtol, max_iter, w, s, mode, rec);
+}
 
 }  // namespace eggmi

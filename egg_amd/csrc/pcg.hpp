@@ -1,5 +1,5 @@
-// Jacobi-preconditioned CG (FP64) on the device: operator view, work buffers
-// and the driver (pcg.hip).
+// Jacobi-preconditioned CG (FP64) on the device: operator views, the work
+// area and the drivers (pcg.hip).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -12,6 +12,7 @@ namespace eggmi
 
 void hip_check(hipError_t e, const char* what);
 
+// General sparse operator, CSR (rowptr[n + 1], col / val[nnz]).
 struct CsrView {
     long long n = 0;
     const long long* rowptr = nullptr;
@@ -19,10 +20,32 @@ struct CsrView {
     const double* val = nullptr;
 };
 
+// Matrix-free (2 dim + 1)-point lattice operator on shape[0] x .. x
+// shape[dim-1] unknowns (row-major, last axis fastest): `diag` on the
+// diagonal, `off` for every neighbour one step along an axis that lies in
+// the lattice (Dirichlet closure: neighbours outside add nothing).
+struct LatticeView {
+    int dim = 3;
+    int shape[3] = {1, 1, 1};
+    long long n = 0;
+    double diag = 0.0, off = 0.0;
+};
+
+// Device work area of one solve. Every vector holds n doubles; `part`
+// holds 7 * pcg_parts(n) doubles (the p.Ap partials, then three per
+// iteration parity); `state` pcg_state_bytes(). The standard recurrence
+// uses u[0] for z, w[0] for Ap and s[0] / s[1] for p.
 struct PcgWork {
-    double *r = nullptr, *z = nullptr, *p = nullptr, *ap = nullptr, *dinv = nullptr, *part = nullptr;
+    double *r = nullptr, *dinv = nullptr, *p = nullptr, *part = nullptr;
+    double* u[2] = {nullptr, nullptr};
+    double* w[2] = {nullptr, nullptr};
+    double* s[2] = {nullptr, nullptr};
     void* state = nullptr;
 };
+
+// kStandard: the reference's recurrence, two launches per iteration;
+// kOneReduction: Chronopoulos-Gear, one launch per iteration (pcg.hip).
+enum class PcgRecurrence { kStandard = 0, kOneReduction = 1 };
 
 struct PcgResult {
     int iters = 0;
@@ -30,8 +53,25 @@ struct PcgResult {
     int status = 0;  // 0 max_iter reached, 1 converged, 2 non-positive curvature, 3 b = 0
 };
 
+constexpr int kPcgThreads = 256;
+constexpr long long pcg_parts(long long n) { return (n + kPcgThreads - 1) / kPcgThreads; }
 size_t pcg_state_bytes();
-PcgResult pcg_solve_device(const CsrView& A, const double* b, double* x, double rtol, int max_iter, PcgWork& w,
-                           hipStream_t s);
+// doubles of one contiguous work area for n unknowns (vectors, partials and
+// the state, each 256-byte aligned)
+size_t pcg_work_doubles(long long n);
+// lay the work area out over `base` (pcg_work_doubles(n) doubles)
+void pcg_carve(double* base, long long n, PcgWork& w);
+
+// How the host follows the solve: kQueued reads the device status
+// asynchronously through pinned memory; kNoAlloc reads it synchronously and
+// allocates nothing (no pinned memory, no events).
+enum class PcgLaunch { kQueued, kNoAlloc };
+constexpr int kPcgChunk = 16;
+
+// x0 = 0; x receives the solution.
+PcgResult pcg_solve_csr(const CsrView& A, const double* b, double* x, double rtol, int max_iter, PcgWork& w,
+                        hipStream_t s, PcgLaunch mode, PcgRecurrence rec);
+PcgResult pcg_solve_lattice(const LatticeView& A, const double* b, double* x, double rtol, int max_iter,
+                            PcgWork& w, hipStream_t s, PcgLaunch mode, PcgRecurrence rec);
 
 }  // namespace eggmi

@@ -661,7 +661,25 @@ void Session::create(int d, const eggmi_group* groups, int n_groups, const eggmi
                 for (int k = 0; k < d; ++k) occ_slots.push_back(lookup(ob, g.gn[k][p], "gn"));
             }
             const long long dslot = lookup(ob, gid, "dof_idx");
-            int ib = g.interior_block[di];
+            const int ib = g.interior_block[di];
+            int lg[3] = {0, 0, 0};
+            if (ib >= 0) {
+                // a synthesized interior DOF, whichever path it takes below: it
+                // ships no samples, names its own block slot strictly inside the
+                // block, and its patch uses the uniform target with unit weights
+                if (P != 0) throw InvalidArgument("structured: interior DOF with stored samples");
+                if (ib >= nb) throw InvalidArgument("structured: interior_block out of range");
+                for (int k = 0; k < d; ++k) {
+                    lg[k] = g.interior_logical[di * d + k];
+                    if (lg[k] < 1 || lg[k] > shapes[ib][k] - 2)
+                        throw InvalidArgument("structured: interior_logical outside [1, n-2]");
+                }
+                if (lay.interior_slot(ib, lg) != dslot)
+                    throw InvalidArgument("structured: interior DOF does not live at its block slot");
+                if (!wt_unit_all)
+                    throw InvalidArgument("structured: a synthesized interior DOF needs unit sample weights "
+                                          "(ship its stored stencil instead)");
+            }
             if (special) {
                 if (con && ib >= 0)
                     throw InvalidArgument("structured: a synthesized interior DOF cannot be constrained");
@@ -697,10 +715,8 @@ void Session::create(int d, const eggmi_group* groups, int n_groups, const eggmi
                     // a synthesized interior DOF with curvature or directional windows takes the
                     // stored path: its patch expanded here in the reference order
                     // (cell bits outer, corner bits inner, axis 0 most significant;
-                    // src/structured_patch.hpp:89-130) with the uniform target
-                    if (P != 0) throw InvalidArgument("structured: interior DOF with stored samples");
-                    int lg[3];
-                    for (int k = 0; k < d; ++k) lg[k] = g.interior_logical[di * d + k];
+                    // src/structured_patch.hpp:89-130) with the synthesized patch's
+                    // identity target (src/structured_patch.hpp:175-213)
                     L.st_dof.push_back((int)dslot);
                     const int nc = 1 << d;
                     auto is_dof = [&](const int* c) {
@@ -726,8 +742,8 @@ void Session::create(int d, const eggmi_group* groups, int n_groups, const eggmi
                             if (role < 0 && is_dof(nbr)) role = k + 1;
                         }
                         L.st_meta.push_back((uint8_t)(meta | (uint8_t)(role + 1)));
-                        if (!w_uniform_all) throw Internal("structured: synthesized DOF under a per-sample target");
-                        if (!wt_unit_all) L.st_wt.push_back(1.0f);
+                        if (!w_uniform_all)
+                            for (int i = 0; i < dd; ++i) L.st_W.push_back(i % (d + 1) == 0 ? 1.0f : 0.0f);
                     }
                     L.st_off.push_back((int)L.st_gc.size());
                     off += P;
@@ -735,19 +751,6 @@ void Session::create(int d, const eggmi_group* groups, int n_groups, const eggmi
                 }
             }
             if (ib >= 0) {
-                if (P != 0) throw InvalidArgument("structured: interior DOF with stored samples");
-                if (!wt_unit_all)
-                    throw InvalidArgument("structured: a synthesized interior DOF needs unit sample weights "
-                                          "(ship its stored stencil instead)");
-                if (ib >= nb) throw InvalidArgument("structured: interior_block out of range");
-                int lg[3];
-                for (int k = 0; k < d; ++k) {
-                    lg[k] = g.interior_logical[di * d + k];
-                    if (lg[k] < 1 || lg[k] > shapes[ib][k] - 2)
-                        throw InvalidArgument("structured: interior_logical outside [1, n-2]");
-                }
-                if (lay.interior_slot(ib, lg) != dslot)
-                    throw InvalidArgument("structured: interior DOF does not live at its block slot");
                 synth_slots.push_back((int)dslot);
             } else {
                 L.st_dof.push_back((int)dslot);

@@ -2,7 +2,8 @@
 
 :func:`pcg_solve` runs the device solver behind ``eggmi_pcg_csr`` (the PCG
 skeleton of the reference's control-net step, src/control_solve.hpp:761-818)
-on a CSR operator; :func:`lattice_laplacian` builds the benchmark's SPD
+on a CSR operator, :func:`pcg_solve_lattice` the same solver matrix-free
+(``eggmi_pcg_lattice``); :func:`lattice_laplacian` builds the benchmark's SPD
 system: the graph Laplacian of an n^d node lattice with Dirichlet closure
 (every lattice edge to a node outside the lattice adds to the diagonal only),
 i.e. the 2d+1-point stencil 2d / -1.
@@ -38,10 +39,22 @@ def lattice_laplacian(n: int = 33, d: int = 3):
     return rowptr, c.astype(np.int32), v.astype(np.float64)
 
 
-def pcg_solve(rowptr, col, val, b, *, rtol: float = 1e-8, max_iter: int = 1000):
-    """Solve A x = b on the GPU; returns (x, iterations, relative residual, status)."""
+def pcg_solve_lattice(n: int, d: int, b, *, rtol: float = 1e-8, max_iter: int = 1000,
+                      recurrence: str = "standard"):
+    """Solve the Dirichlet lattice Laplacian system of :func:`lattice_laplacian`
+    matrix-free on the GPU; returns (x, iterations, relative residual, status)."""
+    from egg_amd._cpp import cpp_core
+
+    return cpp_core.pcg_lattice(np.full(d, n, np.int32), 2.0 * d, -1.0, np.ascontiguousarray(b, np.float64),
+                                rtol=rtol, max_iter=max_iter, recurrence=recurrence)
+
+
+def pcg_solve(rowptr, col, val, b, *, rtol: float = 1e-8, max_iter: int = 1000, recurrence: str = "standard"):
+    """Solve A x = b on the GPU; returns (x, iterations, relative residual, status).
+    ``recurrence``: "standard" (the reference's, two launches per iteration) or
+    "one_reduction" (Chronopoulos-Gear, one launch per iteration)."""
     from egg_amd._cpp import cpp_core
 
     return cpp_core.pcg_csr(np.ascontiguousarray(rowptr, np.int64), np.ascontiguousarray(col, np.int32),
                             np.ascontiguousarray(val, np.float64), np.ascontiguousarray(b, np.float64),
-                            rtol=rtol, max_iter=max_iter)
+                            rtol=rtol, max_iter=max_iter, recurrence=recurrence)

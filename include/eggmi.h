@@ -302,13 +302,28 @@ eggmi_status eggmi_session_set_proposal(eggmi_session* s, eggmi_mlp* m, int mode
  * (rowptr[n+1], col/val[nnz]; SPD expected): the PCG skeleton of the
  * reference's control-net Gauss-Newton step, pcg_solve
  * (src/control_solve.hpp:761-818): x0 = 0, stop when |r| <= rtol |b|,
- * stop on non-positive curvature. x receives the solution; *status is 1
+ * stop on non-positive curvature. recurrence 0: the reference's (two
+ * launches per iteration); 1: Chronopoulos-Gear, the same iterates in exact
+ * arithmetic with the three inner products of an iteration reduced together
+ * (one launch per iteration). x receives the solution; *status is 1
  * converged, 2 non-positive curvature, 3 zero right-hand side, 0 max_iter
  * reached; *solve_ms (may be null) the device time of the solve, uploads
  * excluded. */
 eggmi_status eggmi_pcg_csr(int64_t n, const int64_t* rowptr, const int32_t* col, const double* val,
-                           const double* b, double* x, double rtol, int max_iter, int* iters, double* relres,
-                           int* status, double* solve_ms);
+                           const double* b, double* x, double rtol, int max_iter, int recurrence, int* iters,
+                           double* relres, int* status, double* solve_ms);
+
+/* The same solver on the matrix-free (2 dim + 1)-point lattice operator of
+ * shape[0] x .. x shape[dim-1] unknowns (row-major, last axis fastest):
+ * `diag` on the diagonal, `off` for each neighbour one step along an axis
+ * inside the lattice (Dirichlet closure). diag = 2 dim, off = -1 is the
+ * lattice graph Laplacian with Dirichlet boundary. No counterpart in the
+ * reference's binding: it is the benchmark's spacing system without the
+ * CSR index traffic, for the PCG of src/control_solve.hpp:761-818. Outputs
+ * and recurrence as for eggmi_pcg_csr. */
+eggmi_status eggmi_pcg_lattice(int dim, const int32_t* shape, double diag, double off, const double* b, double* x,
+                               double rtol, int max_iter, int recurrence, int* iters, double* relres, int* status,
+                               double* solve_ms);
 
 const char* eggmi_last_error(void);
 

@@ -380,3 +380,30 @@ def test_graph_chunks_match_launches():
     np.testing.assert_array_equal(a.get_X(), b.get_X())
     with pytest.raises(ValueError):
         b.core.set_graph_chunk(7)
+
+
+@pytest.mark.gpu
+def test_raw_ctypes_session_flow(oracle):
+    """INTEGRATION's C-ABI flow through plain ctypes: eggmi_session_create ->
+    eggmi_run -> eggmi_get_X -> eggmi_session_destroy gives the pybind
+    session's bits and the oracle's positions on the O-grid."""
+    from egg_amd.smoothing import CppStructuredSweepSession, structured_arrays
+    from tests import capi_ctypes as CA
+    from tests.cases import ogrid
+
+    c = ogrid(33, 17, tangle_seed=None)
+    print("ctypes flow: O-grid 33/17, 40 sweeps", flush=True)
+    lib = CA.load()
+    s = CA.Session(lib, c.ctx.wire, structured_arrays(c.bsc), c.X, 2, device=1)
+    E, M = s.run(40, omega=0.8, report_every=10)
+    Xc = s.get_X()
+    s.destroy()
+    p = CppStructuredSweepSession(c.ctx, c.bsc, c.X, device="gpu")
+    e, m = p.run(40, omega=0.8, report_every=10)
+    np.testing.assert_array_equal(Xc, p.get_X())
+    np.testing.assert_array_equal(E, e)
+    np.testing.assert_array_equal(M, m)
+    Xo, eo, mo = oracle.jacobi(2, "barrier", 0.0, 0.8, 40, c.X, c.patches(oracle), c.ctx.energy_stencil,
+                               report_every=10)
+    assert np.abs(Xc - Xo).max() / 4.0 < 1e-5
+    assert abs(E[-1] - eo[-1]) / abs(eo[-1]) < 1e-5

@@ -328,3 +328,86 @@ def test_directional_wire_validation():
         grp[field] = arr
         with pytest.raises(ValueError, match=msg):
             _tables(c2)
+
+
+def test_windowed_interior_dof_checked_like_a_plain_one():
+    """A synthesized interior DOF that carries directional windows (it takes
+    the special stored path) is validated exactly like a plain interior DOF:
+    a logical index outside [1, n-2] or one naming another node is refused
+    before any slot is computed from it."""
+    from egg_amd import grids
+    from tests.cases import Case
+
+    g = grids.single_block_2d(17, seed=4, amp=0.2)
+    c = Case(g, directional=grids.wall_directional_samples(g))
+    grp = c.ctx.wire["groups"][0]
+    K = int(grp["dir_k"])
+    windowed = np.asarray(grp["dir_slot"]).reshape(-1, K).max(axis=1) >= 0
+    cand = np.flatnonzero(windowed & (np.asarray(grp["interior_block"]) >= 0))
+    assert cand.size, "the wall samples reach some synthesized interior DOF"
+    info, _ = _tables(c)
+    assert info["n_stored"] > 0
+    i = int(cand[0])
+    own = tuple(int(v) for v in np.asarray(grp["interior_logical"]).reshape(-1, 2)[i])
+    other = (1, 1) if own != (1, 1) else (2, 2)
+    for lg, msg in (((40, 40), r"outside \[1, n-2\]"), (other, "does not live at its block slot")):
+        c2 = Case(g, directional=grids.wall_directional_samples(g))
+        grp2 = c2.ctx.wire["groups"][0]
+        il = np.array(grp2["interior_logical"], np.int32).reshape(-1, 2)
+        il[i] = lg
+        grp2["interior_logical"] = il.reshape(-1)
+        with pytest.raises(ValueError, match=msg):
+            _tables(c2)
+
+
+def test_single_shared_weight_equals_full_array():
+    """A length-1 ``weight`` (group and energy stencil) is one weight shared
+    by every sample, as in the reference (src/bindings.cpp:247,364-366): it
+    gives the same tables and weighted flag as the full array of that value."""
+    from egg_amd import grids
+    from egg_amd.smoothing import structured_arrays
+    from tests.cases import Case
+
+    g = grids.single_block_2d(9, seed=2, amp=0.2)
+    ns = Case(g).ctx.energy_stencil["num_samples"]
+    full = Case(g, weight=np.full(ns, 1.7))
+    grp = dict(full.ctx.wire["groups"][0])
+    es = dict(full.ctx.energy_stencil)
+    one = {"groups": [dict(grp, weight=np.array([1.7]))], "energy_stencil": dict(es, weight=np.array([1.7]))}
+
+    def make(wire):
+        s = _core().CppStructuredSweepSession(wire, structured_arrays(full.bsc), full.X.ravel(), device="tables")
+        return s.info(), s.tables()
+
+    (ia, ta), (ib, tb) = make(full.ctx.wire), make(one)
+    assert ia == ib
+    for k in ta:
+        np.testing.assert_array_equal(np.asarray(ta[k]), np.asarray(tb[k]))
+    with pytest.raises(ValueError, match="weight size"):
+        make({"groups": [dict(grp, weight=np.array([1.7, 1.7]))], "energy_stencil": es})
+
+
+def test_raw_ctypes_session_tables_only():
+    """The C ABI driven by plain ctypes structs (no pybind module): a
+    tables-only session (device 3) builds the same index maps as the pybind
+    path, and validation errors come back as INVALID_ARGUMENT with the
+    thread's last-error message."""
+    import ctypes as C
+
+    from egg_amd.smoothing import structured_arrays
+    from tests import capi_ctypes as CA
+
+    lib = CA.load()
+    c = ogrid(17, 9)
+    s = CA.Session(lib, c.ctx.wire, structured_arrays(c.bsc), c.X, 2, device=3)
+    info = (C.c_int64 * 8)()
+    s.check(lib.eggmi_session_info(s.handle, C.byref(info)))
+    ref_info, _ = _tables(c)
+    assert [info[i] for i in range(5)] == [ref_info[k] for k in ("n_slots", "n_overflow", "n_halo", "n_fan",
+                                                                 "n_share")]
+    s.destroy()
+    bad = dict(c.ctx.wire["groups"][0])
+    bad["dof_idx"] = np.array(bad["dof_idx"]) + 10**6
+    with pytest.raises(ValueError, match="dof_idx"):
+        CA.Session(lib, {"groups": [bad], "energy_stencil": c.ctx.energy_stencil}, structured_arrays(c.bsc), c.X, 2,
+                   device=3)
