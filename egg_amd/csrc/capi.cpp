@@ -618,4 +618,31 @@ eggmi_status eggmi_pcg_lattice(int dim, const int32_t* shape, double diag, doubl
     });
 }
 
+eggmi_status eggmi_pcg_csr_noalloc(int64_t n, const int64_t* rowptr, const int32_t* col, const double* val,
+                                   const double* b, double* x, double rtol, int max_iter, int recurrence,
+                                   void* stream, int* iters, double* relres, int* status)
+{
+    return guarded([&] {
+        if (n < 0 || n > eggmi::kPcgNoAllocMaxN)
+            throw eggmi::InvalidArgument("pcg no-alloc: n = " + std::to_string(n) + " outside [0, " +
+                                         std::to_string(eggmi::kPcgNoAllocMaxN) + "], the work area's size");
+        if (max_iter < 0) throw eggmi::InvalidArgument("pcg: max_iter must be >= 0");
+        if (!(rtol >= 0.0)) throw eggmi::InvalidArgument("pcg: rtol must be >= 0");
+        const eggmi::PcgRecurrence rec = pcg_recurrence(recurrence);
+        if (n > 0) {
+            need(rowptr, "rowptr");
+            need(col, "col");
+            need(val, "val");
+            need(b, "b");
+            need(x, "x");
+        }
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw eggmi::NoDevice("pcg: no HIP device");
+        const eggmi::CsrView A {n, reinterpret_cast<const long long*>(rowptr), col, val};
+        const eggmi::PcgResult res =
+          eggmi::pcg_solve_csr_noalloc(A, b, x, rtol, max_iter, static_cast<hipStream_t>(stream), rec);
+        pcg_report(res, iters, relres, status);
+    });
+}
+
 }  // extern "C"

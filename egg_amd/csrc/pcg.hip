@@ -599,26 +599,14 @@ PcgResult solve(const Op& op, long long n, const double* b, double* x, double rt
     return res;
 }
 
-constexpr size_t align256(size_t doubles) { return (doubles + 31) & ~size_t(31); }
-
-constexpr size_t work_vectors = 9;  // r, dinv, p, u[2], w[2], s[2]
-
-constexpr size_t work_doubles(long long n)
-{
-    const long long m = n > 1 ? n : 1, np = pcg_parts(n) > 1 ? pcg_parts(n) : 1;
-    return work_vectors * align256((size_t)m) + align256(7 * (size_t)np) + align256(sizeof(PcgState) / 8 + 1);
-}
+static_assert(sizeof(PcgState) <= kPcgStateDoubles * sizeof(double), "PcgState outgrew its slot");
 
 }  // namespace
 
-size_t pcg_state_bytes() { return sizeof(PcgState); }
-
-size_t pcg_work_doubles(long long n) { return work_doubles(n); }
-
 void pcg_carve(double* base, long long n, PcgWork& w)
 {
-    const size_t v = align256((size_t)std::max<long long>(n, 1));
-    double* const vec[work_vectors] = {base, base + v, base + 2 * v, base + 3 * v, base + 4 * v,
+    const size_t v = pcg_align256((size_t)std::max<long long>(n, 1));
+    double* const vec[9] = {base, base + v, base + 2 * v, base + 3 * v, base + 4 * v,
                                        base + 5 * v, base + 6 * v, base + 7 * v, base + 8 * v};
     w.r = vec[0];
     w.dinv = vec[1];
@@ -629,8 +617,8 @@ void pcg_carve(double* base, long long n, PcgWork& w)
     w.w[1] = vec[6];
     w.s[0] = vec[7];
     w.s[1] = vec[8];
-    w.part = base + work_vectors * v;
-    w.state = w.part + align256(7 * (size_t)std::max<long long>(pcg_parts(n), 1));
+    w.part = base + 9 * v;
+    w.state = w.part + pcg_align256(7 * (size_t)std::max<long long>(pcg_parts(n), 1));
 }
 
 PcgResult pcg_solve_csr(const CsrView& A, const double* b, double* x, double rtol, int max_iter, PcgWork& w,

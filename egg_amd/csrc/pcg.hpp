@@ -33,7 +33,7 @@ struct LatticeView {
 
 // Device work area of one solve. Every vector holds n doubles; `part`
 // holds 7 * pcg_parts(n) doubles (the p.Ap partials, then three per
-// iteration parity); `state` pcg_state_bytes(). The standard recurrence
+// iteration parity); `state` kPcgStateDoubles. The standard recurrence
 // uses u[0] for z, w[0] for Ap and s[0] / s[1] for p.
 struct PcgWork {
     double *r = nullptr, *dinv = nullptr, *p = nullptr, *part = nullptr;
@@ -55,10 +55,15 @@ struct PcgResult {
 
 constexpr int kPcgThreads = 256;
 constexpr long long pcg_parts(long long n) { return (n + kPcgThreads - 1) / kPcgThreads; }
-size_t pcg_state_bytes();
-// doubles of one contiguous work area for n unknowns (vectors, partials and
-// the state, each 256-byte aligned)
-size_t pcg_work_doubles(long long n);
+constexpr size_t kPcgStateDoubles = 32;  // room for the solver state (checked in pcg.hip)
+constexpr size_t pcg_align256(size_t doubles) { return (doubles + 31) & ~size_t(31); }
+// doubles of one contiguous work area for n unknowns: 9 vectors, the
+// partials and the state, each 256-byte aligned
+constexpr size_t pcg_work_doubles(long long n)
+{
+    const long long m = n > 1 ? n : 1, np = pcg_parts(n) > 1 ? pcg_parts(n) : 1;
+    return 9 * pcg_align256((size_t)m) + pcg_align256(7 * (size_t)np) + kPcgStateDoubles;
+}
 // lay the work area out over `base` (pcg_work_doubles(n) doubles)
 void pcg_carve(double* base, long long n, PcgWork& w);
 
@@ -73,5 +78,12 @@ PcgResult pcg_solve_csr(const CsrView& A, const double* b, double* x, double rto
                         hipStream_t s, PcgLaunch mode, PcgRecurrence rec);
 PcgResult pcg_solve_lattice(const LatticeView& A, const double* b, double* x, double rtol, int max_iter,
                             PcgWork& w, hipStream_t s, PcgLaunch mode, PcgRecurrence rec);
+
+// The same CSR solve with its work area in a file-scope device array
+// (pcg_noalloc.hip) instead of an allocation: operands in device memory, at
+// most kPcgNoAllocMaxN unknowns, nothing allocated (kNoAlloc launching).
+constexpr long long kPcgNoAllocMaxN = 1ll << 18;
+PcgResult pcg_solve_csr_noalloc(const CsrView& A, const double* b, double* x, double rtol, int max_iter,
+                                hipStream_t s, PcgRecurrence rec);
 
 }  // namespace eggmi

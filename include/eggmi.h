@@ -325,6 +325,20 @@ eggmi_status eggmi_pcg_lattice(int dim, const int32_t* shape, double diag, doubl
                                double rtol, int max_iter, int recurrence, int* iters, double* relres, int* status,
                                double* solve_ms);
 
+/* eggmi_pcg_csr without any allocation. No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (tight solver loops,
+ * harnesses that forbid hipMalloc). Every pointer is DEVICE memory (rowptr
+ * int64[n+1], col int32[nnz], val, b, x double); the operator is not
+ * validated (eggmi_pcg_csr's checks need a host copy). The work area is a
+ * file-scope device array of the library (one per GPU, ~18 MiB, held until
+ * the process ends): n <= 262144 (EGGMI_INVALID_ARGUMENT beyond). All calls
+ * on one GPU share that array, so calls that overlap on one GPU race.
+ * stream: a hipStream_t, NULL for the default stream; the call returns when
+ * the solve is done. Same kernels, iterates and outputs as eggmi_pcg_csr. */
+eggmi_status eggmi_pcg_csr_noalloc(int64_t n, const int64_t* rowptr, const int32_t* col, const double* val,
+                                   const double* b, double* x, double rtol, int max_iter, int recurrence,
+                                   void* stream, int* iters, double* relres, int* status);
+
 const char* eggmi_last_error(void);
 
 /* Library liveness (reference cpp_core.ping, src/bindings.cpp:1544). */
