@@ -473,6 +473,25 @@ class Session
         return {to_array(e), to_array(m)};
     }
 
+    py::dict mg_level_tables(int level)
+    {
+        int64_t ns = 0, nh = 0, nsh = 0;
+        check(eggmi_mg_level_tables(s_, level, &ns, &nh, &nsh, nullptr, nullptr, nullptr, nullptr));
+        py::array_t<int32_t> src(nh + nsh), dst(nh + nsh);
+        py::array_t<uint8_t> fr(ns), fi(ns);
+        check(eggmi_mg_level_tables(s_, level, &ns, &nh, &nsh, src.mutable_data(), dst.mutable_data(),
+                                    fr.mutable_data(), fi.mutable_data()));
+        py::dict d;
+        d["n_slots"] = ns;
+        d["n_halo"] = nh;
+        d["n_share"] = nsh;
+        d["copy_src"] = src;
+        d["copy_dst"] = dst;
+        d["free"] = fr;
+        d["free_interior"] = fi;
+        return d;
+    }
+
     py::list mg_levels(int max_levels)
     {
         int32_t nl = 0, nb = 0;
@@ -665,6 +684,8 @@ PYBIND11_MODULE(cpp_core, m)
       .def("run_fas", &Session::run_fas, py::arg("n_cycles"), py::kw_only(), py::arg("nu_pre") = 2,
            py::arg("nu_post") = 2, py::arg("nu_coarse") = 32, py::arg("omega") = 1.0, py::arg("max_levels") = 32,
            py::arg("phase") = "barrier", "Run n_cycles FAS V-cycles; returns per-cycle (energies, mindets).")
+      .def("mg_level_tables", &Session::mg_level_tables, py::arg("level"),
+           "Copy tables (halo | share, node slots) and relaxed-DOF masks of FAS coarse level `level` (>= 1).")
       .def("mg_levels", &Session::mg_levels, py::arg("max_levels") = 32,
            "Coarse-hierarchy shape: one list per coarse level of per-block interior node-count tuples.")
       .def("get_X", &Session::get_X, "Download X gathered back to global node order, shape (N*dim,).")

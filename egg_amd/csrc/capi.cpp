@@ -151,6 +151,27 @@ eggmi_status eggmi_mg_levels(eggmi_session* s, int max_levels, int32_t* shapes_o
     });
 }
 
+eggmi_status eggmi_mg_level_tables(eggmi_session* s, int level, int64_t* n_slots, int64_t* n_halo,
+                                   int64_t* n_share, int32_t* copy_src, int32_t* copy_dst, uint8_t* free_mask,
+                                   uint8_t* free_interior)
+{
+    return guarded([&] {
+        need(s, "session");
+        s->s.build_hierarchy();
+        if (level < 1 || level > (int)s->s.coarse.size())
+            throw eggmi::InvalidArgument("mg level " + std::to_string(level) + " outside [1, " +
+                                         std::to_string(s->s.coarse.size()) + "]");
+        const eggmi::Level& L = *s->s.coarse[level - 1];
+        if (n_slots) *n_slots = L.lay.n_slots();
+        if (n_halo) *n_halo = L.n_halo;
+        if (n_share) *n_share = L.n_share;
+        if (copy_src) std::copy(L.copy_src.begin(), L.copy_src.end(), copy_src);
+        if (copy_dst) std::copy(L.copy_dst.begin(), L.copy_dst.end(), copy_dst);
+        if (free_mask) std::copy(L.free_mask.begin(), L.free_mask.end(), free_mask);
+        if (free_interior) std::copy(L.free_interior.begin(), L.free_interior.end(), free_interior);
+    });
+}
+
 eggmi_status eggmi_get_X(eggmi_session* s, double* X_out, int64_t n)
 {
     return guarded([&] {

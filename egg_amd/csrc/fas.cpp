@@ -1,6 +1,6 @@
-// FAS nonlinear multigrid for the structured smoother: hierarchy build
-// (reference src/mg_level.hpp:130-455), coarse interface tables
-// (src/mg_topology.hpp:121-304) and the V-cycle driver (src/fas.hpp:251-602),
+// FAS nonlinear multigrid for the structured smoother: the coarse levels
+// (reference src/mg_level.hpp:130-455 and src/mg_topology.hpp:121-304 fix
+// what a coarse level holds) and the V-cycle driver (src/fas.hpp:251-602),
 // over the HIP kernels in mg_kernels.hip / sweep*.hip.
 //
 // Every coarse level is a Level like the fine one: per-block padded layout,
@@ -13,190 +13,305 @@
 #include "session.hpp"
 
 #include <algorithm>
-#include <map>
-#include <unordered_map>
 #include <cmath>
-#include <unordered_set>
 
 namespace eggmi
 {
 
 namespace
 {
-constexpr int kMinCoarsenNodes = 5;  // reference src/mg_level.hpp:46
+constexpr int kMinCoarsenNodes = 5;  // an axis halves iff its node count is odd and >= 5 (src/mg_level.hpp:46,142)
 constexpr int kMaxHierarchyDepth = 32;
 
-struct Loc {
-    int b;
-    int p[3];
-};
+using Idx = std::array<int, 3>;
 
-Loc locate(const Layout& L, long long slot)
-{
-    int b = 0;
-    while (b + 1 < (int)L.base.size() && L.base[b + 1] <= slot) ++b;
-    long long rem = slot - L.base[b];
-    Loc r {b, {0, 0, 0}};
-    for (int k = 0; k < L.dim; ++k) {
-        r.p[k] = (int)(rem / L.stride[b][k]);
-        rem %= L.stride[b][k];
+// One coarsening step: the parent level's layout, relaxed-DOF masks and copy
+// tables in; the child's layout, masks and copy tables out. Every parent
+// lookup is a dense array over the parent's node slots.
+//
+// What a child holds (the reference's rules):
+// * shapes: per block and axis, halve an odd count n >= 5 to (n + 1) / 2,
+//   keep it otherwise; no level when nothing halves;
+// * interior DOFs: child node I relaxes iff its parent image I * factor is a
+//   synthesized interior DOF and every parent node of its full-weighting
+//   stencil (offsets -1, 0, 1 along the halved axes) relaxes, so the
+//   restriction reads only gradients a sweep wrote;
+// * ghost fills: a parent fill of the ghost behind face (axis, side) at
+//   tangential position t * factor becomes the child fill at t; its source,
+//   one step inward from the contact in the source block, becomes the child
+//   node one step inward from the same contact, at the source's tangential
+//   position divided by the source block's factors (dropped when it does not
+//   divide: a non-conforming remnant). The contact's normal axis in the
+//   source block is the one along which the source steps back onto the
+//   parent face node's own copy there (found through the parent share table;
+//   dropped when no axis or several do);
+// * share pairs: an owner -> copy pair survives iff both nodes are parent
+//   images of child nodes;
+// * interface DOFs: a surviving owner relaxes iff it lies on exactly one face
+//   of its block, strictly inside it, and every parent slot its restriction
+//   and patch read relaxes: inside the block a relaxed DOF or a copy of one,
+//   behind the face a ghost filled from a relaxed node on the parent level
+//   whose child image is filled too.
+struct Coarsening {
+    const Layout& pl;
+    const std::vector<uint8_t>& p_free;  // parent slots its sweeps relax
+    const std::vector<uint8_t>& p_int;   // parent synthesized interior DOFs
+    const HaloHost& ptab;
+    const int D;
+    std::vector<Idx> factor, cshape;
+    Layout cl;
+    std::vector<long long> fill_of;  // parent slot -> slot filling it (ghost fills), -1
+    std::vector<long long> rep;      // parent slot -> its share owner (itself when none)
+    HaloHost ctab;
+    std::vector<uint8_t> c_free, c_int;
+    std::vector<uint8_t> c_filled;  // child ghost slots with a fill
+
+    Coarsening(const Layout& parent, const std::vector<uint8_t>& free, const std::vector<uint8_t>& interior,
+               const HaloHost& tab)
+        : pl(parent), p_free(free), p_int(interior), ptab(tab), D(parent.dim)
+    {}
+
+    int block_of(const Layout& L, long long slot) const
+    {
+        return (int)(std::upper_bound(L.base.begin(), L.base.end(), slot) - L.base.begin()) - 1;
     }
-    return r;
-}
-
-// (axis, side) of a face ghost padded index, axis -1 when not a face ghost
-std::pair<int, int> ghost_face(const Layout& L, int b, const int* p)
-{
-    int axis = -1, side = 0;
-    for (int k = 0; k < L.dim; ++k) {
-        const bool lo = p[k] == 0, hi = p[k] == L.shape[b][k] + 1;
-        if (lo || hi) {
-            if (axis >= 0) return {-1, 0};
-            axis = k;
-            side = hi ? 1 : 0;
-        }
-    }
-    return {axis, side};
-}
-
-// Child ghost + share tables from the parent's (reference src/mg_topology.hpp:121-304).
-HaloHost coarsen_halo(const HaloHost& parent, const Layout& pl, const Layout& cl,
-                      const std::vector<std::array<int, 3>>& factor)
-{
-    HaloHost child;
-    const int D = pl.dim;
-    if (parent.sb.empty() && parent.share_src.empty()) return child;
-    std::unordered_map<long long, size_t> by_dst;
-    for (size_t e = 0; e < parent.sb.size(); ++e) by_dst.emplace(pl.padded_slot(parent.db[e], parent.dp[e].data()), e);
-
-    struct FaceInfo {
-        std::array<int, 3> first {};
-        std::array<bool, 3> same {};
-        bool seen = false, valid = true;
-        int normal = -1, side = 0;
-    };
-    std::map<std::array<int, 4>, FaceInfo> faces;  // (dst blk, axis, side, src blk)
-    for (size_t e = 0; e < parent.sb.size(); ++e) {
-        const int db = parent.db[e];
-        const auto [axis, side] = ghost_face(pl, db, parent.dp[e].data());
-        if (axis < 0) continue;
-        FaceInfo& fi = faces[{db, axis, side, parent.sb[e]}];
-        std::array<int, 3> logical = parent.sp[e];
-        for (int k = 0; k < D; ++k) logical[k] -= 1;
-        if (!fi.seen) {
-            fi.seen = true;
-            fi.first = logical;
-            fi.same = {true, true, true};
-        } else {
-            for (int k = 0; k < D; ++k)
-                if (logical[k] != fi.first[k]) fi.same[k] = false;
-        }
-    }
-    for (auto& [key, fi] : faces) {
-        const auto& sn = pl.shape[key[3]];
+    Idx padded_of(const Layout& L, int b, long long slot) const
+    {
+        Idx p {0, 0, 0};
+        long long r = slot - L.base[b];
         for (int k = 0; k < D; ++k) {
-            if (!fi.same[k]) continue;
-            const bool lo = fi.first[k] == 1, hi = fi.first[k] == sn[k] - 2;
-            if (!lo && !hi) continue;
-            if (fi.normal >= 0) {
-                fi.valid = false;
-                break;
-            }
-            fi.normal = k;
-            fi.side = hi ? 1 : 0;
+            p[k] = (int)(r / L.stride[b][k]);
+            r %= L.stride[b][k];
         }
-        if (fi.normal < 0) fi.valid = false;
+        return p;
     }
-    std::map<std::array<int, 3>, bool> face_set;
-    for (const auto& [key, fi] : faces) face_set[{key[0], key[1], key[2]}] = true;
 
-    for (const auto& fentry : face_set) {
-        const int db = fentry.first[0], axis = fentry.first[1], side = fentry.first[2];
-        const auto& cn = cl.shape[db];
-        const auto& fn = pl.shape[db];
-        long long count = 1;
-        int tshape[3] = {1, 1, 1};
-        for (int k = 0; k < D; ++k) {
-            tshape[k] = (k == axis) ? 1 : cn[k];
-            count *= tshape[k];
-        }
-        for (long long i = 0; i < count; ++i) {
-            int T[3] = {0, 0, 0};
-            long long rem = i;
-            for (int k = D - 1; k >= 0; --k) {
-                T[k] = (int)(rem % tshape[k]);
-                rem /= tshape[k];
-            }
-            int fine_ghost[3] = {0, 0, 0}, child_ghost[3] = {0, 0, 0};
+    bool shapes()
+    {
+        const size_t nb = pl.shape.size();
+        factor.assign(nb, {1, 1, 1});
+        cshape.assign(nb, {1, 1, 1});
+        bool any = false;
+        for (size_t b = 0; b < nb; ++b)
             for (int k = 0; k < D; ++k) {
+                const int n = pl.shape[b][k];
+                const bool halve = n % 2 == 1 && n >= kMinCoarsenNodes;
+                factor[b][k] = halve ? 2 : 1;
+                cshape[b][k] = halve ? (n + 1) / 2 : n;
+                any = any || halve;
+            }
+        if (!any) return false;
+        cl.build(D, cshape);
+        c_free.assign(cl.n_slots(), 0);
+        c_int.assign(cl.n_slots(), 0);
+        c_filled.assign(cl.n_slots(), 0);
+        fill_of.assign(pl.n_slots(), -1);
+        for (size_t e = 0; e < ptab.sb.size(); ++e)
+            fill_of[pl.padded_slot(ptab.db[e], ptab.dp[e].data())] = pl.padded_slot(ptab.sb[e], ptab.sp[e].data());
+        rep.resize(pl.n_slots());
+        for (long long s = 0; s < pl.n_slots(); ++s) rep[s] = s;
+        for (size_t p = 0; p < ptab.share_src.size(); ++p) rep[ptab.share_dst[p]] = ptab.share_src[p];
+        return true;
+    }
+
+    // calls f(I) for every child interior index of block b, row-major
+    template <class F> void each_node(int b, F&& f) const
+    {
+        const Idx& c = cshape[b];
+        Idx I {0, 0, 0};
+        const long long n = (long long)c[0] * (D > 1 ? c[1] : 1) * (D > 2 ? c[2] : 1);
+        for (long long i = 0; i < n; ++i) {
+            long long r = i;
+            for (int k = D - 1; k >= 0; --k) {
+                I[k] = (int)(r % c[k]);
+                r /= c[k];
+            }
+            f(I);
+        }
+    }
+
+    void interior_dofs()
+    {
+        for (int b = 0; b < (int)cshape.size(); ++b) {
+            // slot offsets of the full-weighting stencil: -1, 0, 1 along halved axes
+            std::vector<long long> stencil {0};
+            for (int k = 0; k < D; ++k) {
+                if (factor[b][k] == 1) continue;
+                std::vector<long long> next;
+                for (long long o : stencil)
+                    for (int s = -1; s <= 1; ++s) next.push_back(o + s * pl.stride[b][k]);
+                stencil.swap(next);
+            }
+            each_node(b, [&](const Idx& I) {
+                Idx F {0, 0, 0};
+                for (int k = 0; k < D; ++k) F[k] = I[k] * factor[b][k];
+                const long long img = pl.interior_slot(b, F.data());
+                if (!p_int[img]) return;
+                for (long long o : stencil)
+                    if (!p_free[img + o]) return;
+                const long long c = cl.interior_slot(b, I.data());
+                c_free[c] = 1;
+                c_int[c] = 1;
+            });
+        }
+    }
+
+    // the single face (axis, side) a padded index sits behind, axis -1 otherwise
+    std::pair<int, int> face_of(const Layout& L, int b, const Idx& p) const
+    {
+        int axis = -1, side = 0, hits = 0;
+        for (int k = 0; k < D; ++k)
+            if (p[k] == 0 || p[k] == L.shape[b][k] + 1) {
+                ++hits;
+                axis = k;
+                side = p[k] == 0 ? 0 : 1;
+            }
+        return hits == 1 ? std::make_pair(axis, side) : std::make_pair(-1, 0);
+    }
+
+    void ghost_tables()
+    {
+        for (size_t e = 0; e < ptab.sb.size(); ++e) {
+            const int db = ptab.db[e], sb = ptab.sb[e];
+            const Idx& dp = ptab.dp[e];
+            const auto [axis, side] = face_of(pl, db, dp);
+            if (axis < 0) continue;
+            // the child ghost whose parent image this is
+            Idx cg {0, 0, 0};
+            bool image = true;
+            for (int k = 0; k < D && image; ++k) {
                 if (k == axis) {
-                    fine_ghost[k] = side == 0 ? 0 : fn[k] + 1;
-                    child_ghost[k] = side == 0 ? 0 : cn[k] + 1;
+                    cg[k] = side == 0 ? 0 : cshape[db][k] + 1;
                 } else {
-                    fine_ghost[k] = T[k] * factor[db][k] + 1;
-                    child_ghost[k] = T[k] + 1;
+                    const int t = dp[k] - 1;
+                    image = t % factor[db][k] == 0;
+                    cg[k] = t / factor[db][k] + 1;
                 }
             }
-            const auto it = by_dst.find(pl.padded_slot(db, fine_ghost));
-            if (it == by_dst.end()) continue;
-            const size_t e = it->second;
-            const int sb = parent.sb[e];
-            const auto fit = faces.find({db, axis, side, sb});
-            if (fit == faces.end() || !fit->second.valid) continue;
-            const FaceInfo& fi = fit->second;
-            const auto& snc = cl.shape[sb];
-            const auto& snf = pl.shape[sb];
-            std::array<int, 3> q1 = parent.sp[e];
-            for (int k = 0; k < D; ++k) q1[k] -= 1;
-            if (q1[fi.normal] != (fi.side == 0 ? 1 : snf[fi.normal] - 2)) continue;
-            std::array<int, 3> c {0, 0, 0};
+            if (!image) continue;
+            // the contact's normal in the source block: the axis along which the
+            // source steps back onto a copy of the parent face node
+            Idx face = dp;
+            face[axis] = side == 0 ? 1 : pl.shape[db][axis];
+            const long long face_rep = rep[pl.padded_slot(db, face.data())];
+            Idx q = ptab.sp[e];  // padded
+            int normal = -1, nside = 0, found = 0;
+            for (int k = 0; k < D; ++k) {
+                const int n = pl.shape[sb][k];
+                for (int s = 0; s < 2; ++s) {
+                    const int lq = q[k] - 1;
+                    if (lq != (s == 0 ? 1 : n - 2)) continue;
+                    Idx tw = q;
+                    tw[k] = s == 0 ? 1 : n;  // the face-side node next to the source (padded)
+                    if (rep[pl.padded_slot(sb, tw.data())] != face_rep) continue;
+                    ++found;
+                    normal = k;
+                    nside = s;
+                }
+            }
+            if (found != 1) continue;
+            Idx cs {0, 0, 0};
+            bool conform = true;
+            for (int k = 0; k < D && conform; ++k) {
+                if (k == normal) {
+                    cs[k] = (nside == 0 ? 1 : cshape[sb][k] - 2) + 1;
+                } else {
+                    const int lq = q[k] - 1;
+                    conform = lq % factor[sb][k] == 0;
+                    cs[k] = lq / factor[sb][k] + 1;
+                }
+            }
+            if (!conform) continue;
+            const long long dst = cl.padded_slot(db, cg.data());
+            if (c_filled[dst]) continue;
+            c_filled[dst] = 1;
+            ctab.sb.push_back(sb);
+            ctab.sp.push_back(cs);
+            ctab.db.push_back(db);
+            ctab.dp.push_back(cg);
+        }
+    }
+
+    // child interior slot of a parent node slot, -1 when it is no parent image
+    long long child_of(long long slot) const
+    {
+        const int b = block_of(pl, slot);
+        const Idx p = padded_of(pl, b, slot);
+        Idx c {0, 0, 0};
+        for (int k = 0; k < D; ++k) {
+            const int l = p[k] - 1;
+            if (l < 0 || l >= pl.shape[b][k] || l % factor[b][k] != 0) return -1;
+            c[k] = l / factor[b][k];
+        }
+        return cl.interior_slot(b, c.data());
+    }
+
+    void share_pairs()
+    {
+        for (size_t p = 0; p < ptab.share_src.size(); ++p) {
+            const long long so = child_of(ptab.share_src[p]), co = child_of(ptab.share_dst[p]);
+            if (so < 0 || co < 0) continue;
+            ctab.share_src.push_back(so);
+            ctab.share_dst.push_back(co);
+        }
+    }
+
+    bool readable(long long slot) const { return p_free[slot] || p_free[rep[slot]]; }
+
+    void interface_dofs()
+    {
+        std::vector<uint8_t> tried(cl.n_slots(), 0);
+        for (long long node : ctab.share_src) {
+            if (tried[node]) continue;
+            tried[node] = 1;
+            const int b = block_of(cl, node);
+            const Idx cp = padded_of(cl, b, node);
+            Idx I {0, 0, 0};
+            int axis = -1, side = 0, faces = 0;
+            bool inside = true;
+            for (int k = 0; k < D; ++k) {
+                I[k] = cp[k] - 1;
+                if (I[k] < 0 || I[k] >= cshape[b][k]) inside = false;
+                if (I[k] == 0 || I[k] == cshape[b][k] - 1) {
+                    ++faces;
+                    axis = k;
+                    side = I[k] == 0 ? 0 : 1;
+                }
+            }
+            if (!inside || faces != 1) continue;
             bool ok = true;
-            for (int k = 0; k < D && ok; ++k) {
-                if (k == fi.normal) {
-                    c[k] = fi.side == 0 ? 1 : snc[k] - 2;
+            for (int k = 0; k < D && ok; ++k)
+                if (k != axis && (I[k] < 1 || I[k] > cshape[b][k] - 2)) ok = false;
+            // the 3^D parent neighbourhood of the image, and the child's
+            int o[3] = {-1, -1, -1};
+            const int n_off = D == 2 ? 9 : 27;
+            for (int e = 0; e < n_off && ok; ++e) {
+                int r = e;
+                for (int k = 0; k < D; ++k) {
+                    o[k] = r % 3 - 1;
+                    r /= 3;
+                }
+                Idx N {0, 0, 0}, C {0, 0, 0};
+                for (int k = 0; k < D; ++k) {
+                    N[k] = I[k] * factor[b][k] + o[k];
+                    C[k] = I[k] + o[k];
+                }
+                if (N[axis] >= 0 && N[axis] < pl.shape[b][axis]) {
+                    ok = readable(pl.interior_slot(b, N.data()));
                     continue;
                 }
-                const int f = factor[sb][k];
-                if (q1[k] % f != 0) {
-                    ok = false;
-                    break;
+                Idx gp {0, 0, 0}, cg {0, 0, 0};
+                for (int k = 0; k < D; ++k) {
+                    gp[k] = k == axis ? (side == 0 ? 0 : pl.shape[b][k] + 1) : N[k] + 1;
+                    cg[k] = k == axis ? (side == 0 ? 0 : cshape[b][k] + 1) : C[k] + 1;
                 }
-                c[k] = q1[k] / f;
+                const long long src = fill_of[pl.padded_slot(b, gp.data())];
+                ok = src >= 0 && p_free[src] && c_filled[cl.padded_slot(b, cg.data())];
             }
-            if (!ok) continue;
-            for (int k = 0; k < D; ++k) c[k] += 1;
-            child.sb.push_back(sb);
-            child.sp.push_back(c);
-            child.db.push_back(db);
-            child.dp.push_back({child_ghost[0], child_ghost[1], child_ghost[2]});
+            if (ok) c_free[node] = 1;  // relaxed, but not a synthesized interior DOF of the restriction
         }
     }
-    auto to_child = [&](long long slot, long long& out) {
-        const Loc l = locate(pl, slot);
-        int logical[3] = {0, 0, 0};
-        for (int k = 0; k < D; ++k) {
-            if (l.p[k] == 0) return false;
-            logical[k] = l.p[k] - 1;
-        }
-        int c[3] = {0, 0, 0};
-        for (int k = 0; k < D; ++k) {
-            const int f = factor[l.b][k];
-            if (logical[k] % f != 0) return false;
-            c[k] = logical[k] / f;
-            if (c[k] >= cl.shape[l.b][k]) return false;
-        }
-        out = cl.interior_slot(l.b, c);
-        return true;
-    };
-    for (size_t p = 0; p < parent.share_src.size(); ++p) {
-        long long so = 0, co = 0;
-        if (to_child(parent.share_src[p], so) && to_child(parent.share_dst[p], co)) {
-            child.share_src.push_back(so);
-            child.share_dst.push_back(co);
-        }
-    }
-    return child;
-}
+};
 
 }  // namespace
 
@@ -206,168 +321,47 @@ void Session::build_hierarchy()
 {
     if (built_levels >= 0) return;
     built_levels = 0;
-    const int D = dim;
     const Layout* pl = &fine.lay;
-    std::vector<uint8_t> p_free = fine_free, p_int = fine.mask;
-    HaloHost p_tab = fine_halo;
+    const std::vector<uint8_t>* p_free = &fine_free;
+    const std::vector<uint8_t>* p_int = &fine.mask;
+    const HaloHost* ptab = &fine_halo;
+    std::vector<HaloHost> tabs;  // every level's tables (the next step reads its parent's)
+    tabs.reserve(kMaxHierarchyDepth);
     for (int lvl = 1; lvl < kMaxHierarchyDepth; ++lvl) {
-        const size_t nb = pl->shape.size();
-        std::vector<std::array<int, 3>> factor(nb, {1, 1, 1}), cshape(nb, {1, 1, 1});
-        bool any = false;
-        for (size_t b = 0; b < nb; ++b)
-            for (int k = 0; k < D; ++k) {
-                const int n = pl->shape[b][k];
-                const bool halve = (n % 2 == 1) && n >= kMinCoarsenNodes;
-                factor[b][k] = halve ? 2 : 1;
-                cshape[b][k] = halve ? (n + 1) / 2 : n;
-                any = any || halve;
-            }
-        if (!any) break;
-        auto L = std::make_unique<Level>();
-        L->lay.build(D, cshape);
-        const long long ns = L->lay.n_slots();
-        std::vector<uint8_t> c_free(ns, 0), c_int(ns, 0);
-        L->mask.assign(ns, 0);
-        const int NO = D == 2 ? 9 : 27;
-        for (size_t b = 0; b < nb; ++b) {
-            const auto& cs = cshape[b];
-            const long long cnt = (long long)cs[0] * cs[1] * (D == 3 ? cs[2] : 1);
-            for (long long i = 0; i < cnt; ++i) {
-                int I[3] = {0, 0, 0};
-                long long rem = i;
-                for (int k = D - 1; k >= 0; --k) {
-                    I[k] = (int)(rem % cs[k]);
-                    rem /= cs[k];
-                }
-                int F[3] = {0, 0, 0};
-                for (int k = 0; k < D; ++k) F[k] = I[k] * factor[b][k];
-                if (!p_int[pl->interior_slot((int)b, F)]) continue;
-                bool ok = true;
-                for (int e = 0; e < NO && ok; ++e) {
-                    int r = e, P[3] = {F[0], F[1], F[2]};
-                    bool valid = true;
-                    for (int k = 0; k < D; ++k) {
-                        const int o = r % 3 - 1;
-                        r /= 3;
-                        if (factor[b][k] == 1) {
-                            if (o != 0) {
-                                valid = false;
-                                break;
-                            }
-                        } else {
-                            P[k] = F[k] + o;
-                        }
-                    }
-                    if (!valid) continue;
-                    ok = p_free[pl->interior_slot((int)b, P)] != 0;
-                }
-                if (!ok) continue;
-                const long long cid = L->lay.interior_slot((int)b, I);
-                c_free[cid] = 1;
-                c_int[cid] = 1;
-            }
-        }
-        HaloHost c_tab = coarsen_halo(p_tab, *pl, L->lay, factor);
-        if (!c_tab.share_src.empty()) {
-            // guarded interface DOFs (reference src/mg_level.hpp:258-392)
-            std::unordered_map<long long, size_t> parent_ghost;
-            for (size_t e = 0; e < p_tab.sb.size(); ++e) parent_ghost.emplace(pl->padded_slot(p_tab.db[e], p_tab.dp[e].data()), e);
-            std::unordered_map<long long, long long> copy_owner;
-            for (size_t p = 0; p < p_tab.share_src.size(); ++p) copy_owner.emplace(p_tab.share_dst[p], p_tab.share_src[p]);
-            auto read_ok = [&](long long nid) {
-                if (p_free[nid]) return true;
-                const auto it = copy_owner.find(nid);
-                return it != copy_owner.end() && p_free[it->second] != 0;
-            };
-            std::unordered_set<long long> child_ghost;
-            for (size_t e = 0; e < c_tab.sb.size(); ++e) child_ghost.insert(L->lay.padded_slot(c_tab.db[e], c_tab.dp[e].data()));
-            std::unordered_set<long long> seen;
-            for (long long node : c_tab.share_src) {
-                if (!seen.insert(node).second) continue;
-                const Loc l = locate(L->lay, node);
-                const int b = l.b;
-                const auto& cn = L->lay.shape[b];
-                const auto& fn = pl->shape[b];
-                int I[3] = {0, 0, 0};
-                bool interior = true;
-                for (int k = 0; k < D; ++k) {
-                    if (l.p[k] == 0) interior = false;
-                    I[k] = l.p[k] - 1;
-                }
-                if (!interior) continue;
-                int axis = -1, side = 0;
-                bool ok = true;
-                for (int k = 0; k < D && ok; ++k) {
-                    const bool lo = I[k] == 0, hi = I[k] == cn[k] - 1;
-                    if (lo || hi) {
-                        if (axis >= 0) ok = false;
-                        axis = k;
-                        side = hi ? 1 : 0;
-                    }
-                }
-                if (!ok || axis < 0) continue;
-                for (int k = 0; k < D && ok; ++k)
-                    if (k != axis && (I[k] < 1 || I[k] > cn[k] - 2)) ok = false;
-                if (!ok) continue;
-                for (int e = 0; e < NO && ok; ++e) {
-                    int r = e;
-                    long long N[3] = {0, 0, 0}, C[3] = {0, 0, 0};
-                    for (int k = 0; k < D; ++k) {
-                        const int o = r % 3 - 1;
-                        r /= 3;
-                        N[k] = (long long)I[k] * factor[b][k] + o;
-                        C[k] = I[k] + o;
-                    }
-                    if (N[axis] < 0 || N[axis] >= fn[axis]) {
-                        int gp[3] = {0, 0, 0}, cgp[3] = {0, 0, 0};
-                        for (int k = 0; k < D; ++k) {
-                            if (k == axis) {
-                                gp[k] = side == 0 ? 0 : fn[k] + 1;
-                                cgp[k] = side == 0 ? 0 : cn[k] + 1;
-                            } else {
-                                gp[k] = (int)N[k] + 1;
-                                cgp[k] = (int)C[k] + 1;
-                            }
-                        }
-                        const auto it = parent_ghost.find(pl->padded_slot(b, gp));
-                        ok = it != parent_ghost.end() &&
-                             p_free[pl->padded_slot(p_tab.sb[it->second], p_tab.sp[it->second].data())] != 0 &&
-                             child_ghost.count(L->lay.padded_slot(b, cgp)) != 0;
-                    } else {
-                        int P[3] = {(int)N[0], (int)N[1], (int)N[2]};
-                        ok = read_ok(pl->interior_slot(b, P));
-                    }
-                }
-                if (!ok) continue;
-                c_free[node] = 1;  // free, not free_interior
-            }
-        }
-        long long n_free = 0;
-        for (long long s = 0; s < ns; ++s) n_free += c_free[s];
+        Coarsening c(*pl, *p_free, *p_int, *ptab);
+        if (!c.shapes()) break;
+        c.interior_dofs();
+        c.ghost_tables();
+        c.share_pairs();
+        if (!c.ctab.share_src.empty()) c.interface_dofs();
+        const long long n_free = std::count(c.c_free.begin(), c.c_free.end(), (uint8_t)1);
         if (n_free == 0) break;
-        L->mask = c_free;  // every coarse DOF is synthesized
+        auto L = std::make_unique<Level>();
+        L->lay = c.cl;
+        L->mask = c.c_free;  // every coarse DOF is synthesized
         L->n_synth = n_free;
         L->st_off.assign(1, 0);
-        for (size_t e = 0; e < c_tab.sb.size(); ++e) {
-            L->copy_src.push_back((int)L->lay.padded_slot(c_tab.sb[e], c_tab.sp[e].data()));
-            L->copy_dst.push_back((int)L->lay.padded_slot(c_tab.db[e], c_tab.dp[e].data()));
+        for (size_t e = 0; e < c.ctab.sb.size(); ++e) {
+            L->copy_src.push_back((int)L->lay.padded_slot(c.ctab.sb[e], c.ctab.sp[e].data()));
+            L->copy_dst.push_back((int)L->lay.padded_slot(c.ctab.db[e], c.ctab.dp[e].data()));
         }
-        L->n_halo = (long long)c_tab.sb.size();
-        for (size_t p = 0; p < c_tab.share_src.size(); ++p) {
-            L->copy_src.push_back((int)c_tab.share_src[p]);
-            L->copy_dst.push_back((int)c_tab.share_dst[p]);
+        L->n_halo = (long long)c.ctab.sb.size();
+        for (size_t p = 0; p < c.ctab.share_src.size(); ++p) {
+            L->copy_src.push_back((int)c.ctab.share_src[p]);
+            L->copy_dst.push_back((int)c.ctab.share_dst[p]);
         }
-        L->n_share = (long long)c_tab.share_src.size();
-        L->factor = factor;
-        L->free_mask = c_free;
-        L->free_interior = c_int;
+        L->n_share = (long long)c.ctab.share_src.size();
+        L->factor = c.factor;
+        L->free_mask = c.c_free;
+        L->free_interior = c.c_int;
         L->build_push();
         coarse.push_back(std::move(L));
-        Level& C = *coarse.back();
+        tabs.push_back(std::move(c.ctab));
+        const Level& C = *coarse.back();
         pl = &C.lay;
-        p_free = C.free_mask;
-        p_int = C.free_interior;
-        p_tab = std::move(c_tab);
+        p_free = &C.free_mask;
+        p_int = &C.free_interior;
+        ptab = &tabs.back();
     }
     built_levels = (int)coarse.size();
 }

@@ -3,6 +3,7 @@
 from .cpp_backend import (
     BlockStructuredContext,
     CppStructuredSweepSession,
+    build_block_structured_context,
     build_structured_context_from_block_maps,
     cpp_structured_sweep,
     cpp_untangle,
@@ -15,6 +16,7 @@ __all__ = [
     "BlockStructuredContext",
     "CppStructuredSweepSession",
     "SweepContext",
+    "build_block_structured_context",
     "build_flat_context",
     "build_structured_context_from_block_maps",
     "build_sweep_context",

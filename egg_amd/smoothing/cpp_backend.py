@@ -1,24 +1,31 @@
 """Bridge to the MI355X compute core for the structured block-Jacobi sweep.
 
 Mirrors the reference's ``egg/smoothing/cpp_backend.py`` surface for the
-structured path: :func:`build_structured_context_from_block_maps`
-(reference :120-250), :func:`structured_arrays` (:679-691),
-:class:`CppStructuredSweepSession` (:456-677), :func:`cpp_structured_sweep`
-(:694-770) and :func:`cpp_untangle` (:773-848). The compute core is
+structured path: :class:`CppStructuredSweepSession` (:456-677),
+:func:`cpp_structured_sweep` (:694-770) and :func:`cpp_untangle` (:773-848);
+the halo-table builders (:56-455, :679-691) live in
+:mod:`egg_amd.smoothing.structured_tables` and are re-exported here. The compute core is
 ``egg_amd._cpp.cpp_core`` (HIP kernels behind the eggmi C ABI); there is no
 CPU fallback — importing the session without the built extension raises.
 """
 
 from __future__ import annotations
 
-from dataclasses import dataclass
 from typing import Any
 
 import numpy as np
 
+from .structured_tables import (
+    BlockStructuredContext,
+    build_block_structured_context,
+    build_structured_context_from_block_maps,
+    structured_arrays,
+)
+
 __all__ = [
     "BlockStructuredContext",
     "CppStructuredSweepSession",
+    "build_block_structured_context",
     "build_structured_context_from_block_maps",
     "cpp_structured_sweep",
     "cpp_untangle",
@@ -39,134 +46,6 @@ def _grid_mindet(X: np.ndarray, es: dict, d: int = 2) -> float:
     cols = [np.asarray(es[f"s{k}"])[:, None] * (X[np.asarray(es[f"gn{k}"])] - X[gc]) for k in range(d)]
     A = np.stack(cols, axis=2)  # (ns, d, d), A[:, :, k] = column k
     return float(np.linalg.det(A).min()) if len(gc) else float("inf")
-
-
-@dataclass
-class BlockStructuredContext:
-    """Halo-padded structured tables (reference cpp_backend.py:56-117)."""
-
-    d: int
-    interior_shapes: list
-    block_global_dof: list
-    halo_src_block: np.ndarray
-    halo_src_padded: np.ndarray
-    halo_dst_block: np.ndarray
-    halo_dst_padded: np.ndarray
-    sing_block: np.ndarray
-    sing_logical: np.ndarray
-
-    @property
-    def num_blocks(self) -> int:
-        return len(self.interior_shapes)
-
-    @property
-    def num_halo_entries(self) -> int:
-        return int(self.halo_dst_block.shape[0])
-
-
-def build_structured_context_from_block_maps(d: int, block_global_dof: list) -> BlockStructuredContext:
-    """Face-ghost halo table by shared-global-id twin matching.
-
-    For every face node of block ``bi`` that also lives in another block, the
-    ghost just outside that face is filled from the node one step behind the
-    twin, i.e. one step into the other block along the axis on which the twin
-    sits on a face and whose inward step leaves ``bi``. A node behind the
-    contact that is itself shared is sourced from its lowest (block, logical)
-    copy — the owner — so a halo source is never a broadcast destination.
-    Unresolvable or ambiguous contacts are left to the core's spare-ghost
-    mirrors. Entries are emitted block by block, axis by axis, low side then
-    high side, face nodes in row-major order (reference :120-250).
-    """
-    shapes = [tuple(int(s) for s in np.shape(b)) for b in block_global_dof]
-    arrays = [np.asarray(b) for b in block_global_dof]
-    flat = [np.ascontiguousarray(a.reshape(-1), dtype=np.int32) for a in arrays]
-    n_ids = (max(int(f.max()) for f in flat if f.size) + 1) if flat else 0
-    count = np.zeros(n_ids, np.int64)
-    for f in flat:
-        count += np.bincount(f, minlength=n_ids)
-    shared = count > 1
-    copies: dict[int, list] = {}
-    for b, a in enumerate(arrays):
-        for idx in zip(*np.nonzero(shared[a])):
-            loc = tuple(int(i) for i in idx)
-            copies.setdefault(int(a[loc]), []).append((b, loc))
-    members = [set(f.tolist()) for f in flat]
-
-    def source_of(dst: int, gid: int):
-        found = None
-        where = None
-        for b, loc in copies.get(gid, ()):
-            if b == dst:
-                continue
-            for ax in range(d):
-                if loc[ax] == 0:
-                    step = 1
-                elif loc[ax] == shapes[b][ax] - 1:
-                    step = -1
-                else:
-                    continue
-                inner = list(loc)
-                inner[ax] += step
-                g_in = int(arrays[b][tuple(inner)])
-                if g_in in members[dst]:
-                    continue  # a seam direction, not the contact normal
-                if found is not None and found != g_in:
-                    return None
-                found = g_in
-                where = (b, tuple(inner))
-        if found is None:
-            return None
-        if shared[found]:
-            return min(copies[found])
-        return where
-
-    sb, sp, db, dp = [], [], [], []
-    for b, a in enumerate(arrays):
-        shp = shapes[b]
-        for ax in range(d):
-            for side in (0, 1):
-                fixed = 0 if side == 0 else shp[ax] - 1
-                face = np.take(a, fixed, axis=ax)
-                for fidx in np.ndindex(face.shape):
-                    gid = int(face[fidx])
-                    if not shared[gid]:
-                        continue
-                    hit = source_of(b, gid)
-                    if hit is None:
-                        continue
-                    logical = list(fidx[:ax]) + [fixed] + list(fidx[ax:])
-                    ghost = [c + 1 for c in logical]
-                    ghost[ax] = 0 if side == 0 else shp[ax] + 1
-                    sb.append(hit[0])
-                    sp.append(tuple(c + 1 for c in hit[1]))
-                    db.append(b)
-                    dp.append(tuple(ghost))
-    rows = lambda v: np.asarray(v, np.int32).reshape(-1, d) if v else np.zeros((0, d), np.int32)  # noqa: E731
-    return BlockStructuredContext(
-        d=d,
-        interior_shapes=shapes,
-        block_global_dof=flat,
-        halo_src_block=np.asarray(sb, np.int32),
-        halo_src_padded=rows(sp),
-        halo_dst_block=np.asarray(db, np.int32),
-        halo_dst_padded=rows(dp),
-        sing_block=np.zeros((0,), np.int32),
-        sing_logical=np.zeros((0, d), np.int32),
-    )
-
-
-def structured_arrays(bsc: BlockStructuredContext) -> dict:
-    """The ``structured`` dict of the core binding (reference :679-691)."""
-    return {
-        "interior_shapes": [np.asarray(s, np.int32) for s in bsc.interior_shapes],
-        "block_global_dof": list(bsc.block_global_dof),
-        "halo_src_block": bsc.halo_src_block,
-        "halo_src_padded": bsc.halo_src_padded,
-        "halo_dst_block": bsc.halo_dst_block,
-        "halo_dst_padded": bsc.halo_dst_padded,
-        "sing_block": bsc.sing_block,
-        "sing_logical": bsc.sing_logical,
-    }
 
 
 def _as_bsc(grid: Any, d: int) -> BlockStructuredContext:

@@ -1,24 +1,18 @@
-"""Flat sweep wire for structured quad/hex blocks.
+"""The flat sweep wire ``{"groups", "energy_stencil"}`` of structured quad /
+hex blocks: the dictionary the compute core takes, with the keys and
+conventions of the reference's ``build_flat_context``
+(egg/smoothing/flat_context.py:136-346; unpacked by src/bindings.cpp:
+334-490).
 
-Builds the ``{"groups", "energy_stencil"}`` dictionary the compute core
-consumes, with the field names and conventions of the reference's
-``build_flat_context`` (reference egg/smoothing/flat_context.py:136-346):
-
-* every cell contributes ``2**d`` corner samples; sample id
-  ``2**d * cell + b`` over the blocks concatenated in order, cells row-major,
-  corner index ``b`` with axis 0 as the most significant bit;
-* a sample is a corner plus its ``d`` axis neighbours (neighbour ``k`` flips
-  bit ``k`` of ``b``) with sign ``+1`` when that bit is 0 and ``-1`` otherwise;
-* a DOF strictly inside a block (logical index in ``[1, n-2]`` on every
-  axis) under an isotropic uniform target ships no samples: its patch is
-  synthesized by the core from ``(interior_block, interior_logical)``;
-* every other moving DOF carries its incident samples in ascending
-  (cell, corner) order with its role (0 corner, ``1+k`` axis-``k``
-  neighbour, -1 present but neither).
-
-Only the identity-family targets and plain shape samples are supported here
-(the segment, circle, ellipse and rail entities, curvature windows and
-directional terms are supported; no other geometry entities).
+One group holds every moving DOF, in ascending node id. A DOF strictly
+inside a block (logical index in [1, n-2] on every axis) ships no samples
+when the target is one isotropic matrix and no sample carries a weight: the
+core synthesizes its patch from ``(interior_block, interior_logical)``.
+Every other moving DOF ships its incident samples (ascending sample id, see
+:mod:`egg_amd.smoothing.wire_tables` for the sample order and the roles),
+``P_of`` of them. Optional parts: per-sample targets ``W_inv`` and energy
+weights, boundary DOFs sliding on an entity (2D: segment, circle, ellipse,
+rail), block-interface curvature windows (2D) and directional terms.
 """
 
 from __future__ import annotations
@@ -26,202 +20,156 @@ from __future__ import annotations
 import numpy as np
 
 from .directional import directional_dof_table, directional_energy_table
+from .wire_tables import CellSamples, per_dof_slots
 
+__all__ = ["ENTITY_TYPES", "build_flat_context", "cell_stencil", "curvature_table"]
 
-def _bit(d: int, k: int) -> int:
-    return 1 << (d - 1 - k)
-
-
-def cell_stencil(blocks: list[np.ndarray], d: int) -> dict:
-    """Per-sample stencil (gc, gn[k], s[k]) and node->sample membership rows
-    (node, sample, role) for the cells of all blocks."""
-    nc = 1 << d
-    corner_cols = []
-    for ids in blocks:
-        ids = np.asarray(ids)
-        shp = ids.shape
-        if any(s < 2 for s in shp):
-            continue
-        cols = []
-        for b in range(nc):
-            sl = tuple(slice((b >> (d - 1 - k)) & 1, ((b >> (d - 1 - k)) & 1) + shp[k] - 1) for k in range(d))
-            cols.append(ids[sl].reshape(-1))
-        corner_cols.append(np.stack(cols, axis=1))
-    cells = np.concatenate(corner_cols) if corner_cols else np.zeros((0, nc), np.int64)
-    ncell = cells.shape[0]
-    b_idx = np.arange(nc)
-    gc = cells.reshape(-1)
-    gn = [cells[:, b_idx ^ _bit(d, k)].reshape(-1) for k in range(d)]
-    sign = np.array([[-1.0 if (b & _bit(d, k)) else 1.0 for k in range(d)] for b in range(nc)])
-    s = [np.tile(sign[:, k], ncell) for k in range(d)]
-    # role[a, b]: local corner a in sample b
-    role = np.full((nc, nc), -1, np.int32)
-    role[b_idx, b_idx] = 0
-    for k in range(d):
-        role[b_idx ^ _bit(d, k), b_idx] = 1 + k
-    sid = nc * np.arange(ncell)[:, None] + b_idx[None, :]
-    m_node = np.repeat(cells[:, :, None], nc, axis=2).reshape(-1)
-    m_sid = np.repeat(sid[:, None, :], nc, axis=1).reshape(-1)
-    m_role = np.tile(role.reshape(-1), ncell)
-    return {"gc": gc, "gn": gn, "s": s, "m_node": m_node, "m_sid": m_sid, "m_role": m_role,
-            "nc": nc, "ns": gc.shape[0], "ncell": ncell}
-
-
-# Entity types a constrained boundary DOF can slide on here, in the wire's
-# per-type SoA form (reference egg/geometry/entity_soa.py:109-155: tag,
-# record fields), 2D: segment (sx, sy, ex, ey), circle (cx, cy, r), ellipse
-# (cx, cy, rx, ry), rail (sx, sy, ex, ey; releases the node beyond its ends).
+# Entity kinds a boundary DOF can slide on here: (wire tag, record length),
+# the reference's per-type SoA records (egg/geometry/entity_soa.py:109-155):
+# segment (sx, sy, ex, ey), circle (cx, cy, r), ellipse (cx, cy, rx, ry),
+# rail (sx, sy, ex, ey; the node is released beyond its ends).
 ENTITY_TYPES = {"lineseg": (1, 4), "circle": (2, 3), "ellipse": (3, 4), "linerail": (15, 4)}
 
 
-def _entities_wire(dofs: np.ndarray, entities: dict | None) -> dict:
-    """The group's ``entities`` dict (reference egg/geometry/entity_soa.py:
-    598-680 group_entities_by_type): one sub-dict per present type with the
-    group-local DOF indices and packed records; free DOFs under "free"."""
+def cell_stencil(blocks: list, d: int) -> dict:
+    """The whole sample table of ``blocks``: gc, gn[k], s[k] per sample and
+    the node / sample / role incidence of every node (test and oracle use)."""
+    cs = CellSamples([np.asarray(b) for b in blocks], d)
+    f = cs.fields(np.arange(cs.n_samples))
+    node, sid, role = cs.incidence(np.ones(int(cs.cells.max()) + 1 if cs.cells.size else 0, bool))
+    return {"gc": f["gc"], "gn": [f[f"gn{k}"] for k in range(d)], "s": [f[f"s{k}"] for k in range(d)],
+            "m_node": node, "m_sid": sid, "m_role": role, "nc": cs.nc, "ns": cs.n_samples, "ncell": cs.n_cells}
+
+
+def _entity_groups(dofs: np.ndarray, entities: dict | None) -> dict:
+    """The group's ``entities`` field (reference egg/geometry/entity_soa.py:
+    598-680): one sub-dict per kind present, ordered by its first DOF, with
+    the group-local indices of its DOFs (ascending) and their records packed
+    back to back; DOFs not listed slide on nothing ("free", tag 0)."""
     if not entities:
         return {}
-    parts: dict[str, dict] = {}
-    for local, node in enumerate(dofs.tolist()):
-        kind, rec = entities.get(int(node), ("free", ()))
+    kind_of = np.full(dofs.size, "free", dtype=object)
+    rec_of: list = [()] * dofs.size
+    for node, (kind, rec) in entities.items():
+        i = int(np.searchsorted(dofs, node))
+        if i < dofs.size and dofs[i] == node:
+            kind_of[i] = kind
+            rec_of[i] = tuple(rec)
+    out = {}
+    kinds = sorted(set(kind_of.tolist()), key=lambda k: int(np.argmax(kind_of == k)))
+    for kind in kinds:
         if kind == "free":
-            tag, kf = 0, 0
+            tag, width = 0, 0
         elif kind in ENTITY_TYPES:
-            tag, kf = ENTITY_TYPES[kind]
+            tag, width = ENTITY_TYPES[kind]
         else:
             raise ValueError(f"entity type {kind!r} is not supported ({', '.join(ENTITY_TYPES)})")
-        if len(rec) != kf:
-            raise ValueError(f"a {kind} record holds {kf} values, got {len(rec)}")
-        p = parts.setdefault(kind, {"tag": tag, "kFields": kf, "dof_local": [], "records": []})
-        p["dof_local"].append(local)
-        p["records"].extend(float(v) for v in rec)
-    return {k: {"tag": v["tag"], "kFields": v["kFields"], "dof_local": np.asarray(v["dof_local"], np.int32),
-                "records": np.asarray(v["records"], np.float64)} for k, v in parts.items()}
+        local = np.flatnonzero(kind_of == kind)
+        recs = [rec_of[i] for i in local]
+        bad = [r for r in recs if len(r) != width]
+        if bad:
+            raise ValueError(f"a {kind} record holds {width} values, got {len(bad[0])}")
+        out[kind] = {"tag": tag, "kFields": width, "dof_local": local.astype(np.int32),
+                     "records": np.asarray(recs, np.float64).reshape(-1)}
+    return out
 
 
 def curvature_table(free_mask: np.ndarray, nodes: np.ndarray, weight: np.ndarray) -> dict:
-    """Per-DOF curvature window table (reference
-    egg/smoothing/interface_c2.py:230-279): every free node of a window
-    (four global node ids, a weight) lists it with its place in the window,
-    padded to a common width ``curv_k`` (place -1 = padding), in the group's
-    DOF order."""
-    free = np.asarray(free_mask, bool)
-    dofs = np.flatnonzero(free)
+    """Per-DOF curvature windows (reference egg/smoothing/interface_c2.py:
+    230-279): every window (four node ids, a weight) a moving DOF is one of
+    the nodes of, with the DOF's place in it, in window order, padded to the
+    widest DOF's count ``curv_k`` (place -1, nodes -1, weight 0)."""
     nodes = np.asarray(nodes, np.int64).reshape(-1, 4)
     weight = np.asarray(weight, np.float64).reshape(-1)
-    pos = np.full(free.shape[0], -1, np.int64)
-    pos[dofs] = np.arange(dofs.size)
-    lists = [[] for _ in range(dofs.size)]
-    for wi in range(nodes.shape[0]):
-        for slot in range(4):
-            n = int(nodes[wi, slot])
-            if free[n]:
-                lists[int(pos[n])].append((nodes[wi], slot, float(weight[wi])))
-    K = max((len(x) for x in lists), default=0)
+    K, win, place = per_dof_slots(free_mask, nodes, np.full(nodes.shape[0], 4))
     if K == 0:
         return {}
-    cn = np.full((dofs.size, K, 4), -1, np.int32)
-    cs = np.full((dofs.size, K), -1, np.int32)
-    cw = np.zeros((dofs.size, K))
-    for i, entry in enumerate(lists):
-        for j, (row, slot, w) in enumerate(entry):
-            cn[i, j], cs[i, j], cw[i, j] = row, slot, w
-    return {"curv_k": int(K), "curv_nodes": cn.reshape(-1), "curv_slot": cs.reshape(-1),
-            "curv_weight": cw.reshape(-1)}
+    pad = win < 0
+    rows = nodes[np.where(pad, 0, win)]
+    rows[pad] = -1
+    w = weight[np.where(pad, 0, win)]
+    w[pad] = 0.0
+    return {"curv_k": K, "curv_nodes": rows.astype(np.int32).reshape(-1),
+            "curv_slot": place.astype(np.int32).reshape(-1), "curv_weight": w.reshape(-1)}
 
 
-def build_flat_context(blocks: list[np.ndarray], free_mask: np.ndarray, d: int, *,
-                       w_inv: np.ndarray | None = None, weight: np.ndarray | None = None,
-                       entities: dict | None = None, curvature: tuple | None = None,
-                       directional=None) -> dict:
-    """Assemble the flat wire for the moving DOFs ``free_mask`` of ``blocks``.
+def _interior_nodes(blocks: list, n_nodes: int, d: int):
+    """Block and logical index of every node strictly inside a block whose
+    sides all have at least 3 nodes (-1 / 0 elsewhere)."""
+    where = np.full(n_nodes, -1, np.int32)
+    logical = np.zeros((n_nodes, d), np.int32)
+    for b, ids in enumerate(blocks):
+        ids = np.asarray(ids)
+        if min(ids.shape) < 3:
+            continue
+        inner = ids[(slice(1, -1),) * d].reshape(-1)
+        where[inner] = b
+        logical[inner] = np.indices([s - 2 for s in ids.shape]).reshape(d, -1).T + 1
+    return where, logical
 
-    ``w_inv`` is one uniform ``(d, d)`` inverse target (identity when
-    omitted) or ``(ns, d, d)`` per energy-stencil sample. ``weight`` is an
-    optional ``(ns,)`` per-sample energy weight (it scales a sample's mu,
-    gradient and Hessian, not its det); as in the reference
-    (egg/smoothing/flat_context.py:186-218) a weighted wire ships every DOF's
-    real stencil, so interior synthesis is off. ``entities`` maps a moving
-    node id to the entity it slides on, ``("lineseg", (sx, sy, ex, ey))`` or
-    ``("circle", (cx, cy, r))`` (2D): such a DOF ships its stored stencil and
-    the group's ``entities`` field carries the per-type records.
-    ``curvature`` = (nodes (W, 4), weight (W,)) adds block-interface C2
-    curvature windows (2D; reference egg/smoothing/flat_context.py:293-304).
-    ``directional`` (a ``DirectionalSamples``) adds directional soft-energy
-    terms: per-DOF windows on the group, the whole-sample table on the energy
-    stencil (2D in the compute core; reference flat_context.py:305-320).
-    """
-    st = cell_stencil(blocks, d)
-    ns = st["ns"]
-    N = int(free_mask.shape[0])
-    dd = d * d
-    w_inv = np.eye(d) if w_inv is None else np.asarray(w_inv, np.float64)
-    uniform = w_inv.shape == (d, d)
-    w0 = w_inv if uniform else (w_inv[0] if len(w_inv) else np.eye(d))
-    synth = bool(np.allclose(w0, w0[0, 0] * np.eye(d)) and (uniform or np.allclose(w_inv, w0)))
+
+def build_flat_context(blocks: list, free_mask: np.ndarray, d: int, *, w_inv: np.ndarray | None = None,
+                       weight: np.ndarray | None = None, entities: dict | None = None,
+                       curvature: tuple | None = None, directional=None) -> dict:
+    """Assemble the wire for the moving DOFs ``free_mask`` of ``blocks``.
+
+    ``w_inv``: one ``(d, d)`` inverse target (identity when omitted) or one
+    per sample ``(ns, d, d)``. ``weight``: optional ``(ns,)`` per-sample
+    energy weight (scales a sample's mu, gradient and Hessian, not its det);
+    a weighted wire ships every moving DOF's samples. ``entities``: 2D
+    ``{node: (kind, record)}`` with a kind of :data:`ENTITY_TYPES`; those
+    DOFs ship their samples. ``curvature``: ``(nodes (W, 4), weight (W,))``
+    block-interface windows (2D). ``directional``: a ``DirectionalSamples``."""
+    blocks = [np.asarray(b) for b in blocks]
+    cs = CellSamples(blocks, d)
+    ns = cs.n_samples
+    free = np.asarray(free_mask, bool)
+    W = np.eye(d) if w_inv is None else np.asarray(w_inv, np.float64)
+    uniform = W.shape == (d, d)
+    W0 = W if uniform else (W[0] if len(W) else np.eye(d))
+    synthesize = bool(np.allclose(W0, W0[0, 0] * np.eye(d)) and (uniform or np.allclose(W, W0)))
     if weight is not None:
         weight = np.asarray(weight, np.float64).reshape(-1)
         if weight.shape != (ns,):
             raise ValueError(f"weight must hold one value per energy-stencil sample ({ns}), got {weight.shape}")
-        synth = False
+        synthesize = False
+    if entities and d != 2:
+        raise ValueError("constrained boundary DOFs are 2D only here")
 
-    node_block = np.full(N, -1, np.int32)
-    node_logical = np.zeros((N, d), np.int32)
-    if synth:
-        for b, ids in enumerate(blocks):
-            ids = np.asarray(ids)
-            if any(s < 3 for s in ids.shape):
-                continue
-            inner = ids[tuple(slice(1, s - 1) for s in ids.shape)].reshape(-1)
-            grids = np.meshgrid(*[np.arange(1, s - 1, dtype=np.int32) for s in ids.shape], indexing="ij")
-            node_block[inner] = b
-            node_logical[inner] = np.stack([g.reshape(-1) for g in grids], axis=1)
+    where, logical = (_interior_nodes(blocks, free.size, d) if synthesize
+                      else (np.full(free.size, -1, np.int32), np.zeros((free.size, d), np.int32)))
     if entities:
-        if d != 2:
-            raise ValueError("constrained boundary DOFs are 2D only here")
-        node_block[np.asarray([k for k, v in entities.items() if v[0] != "free"], np.int64)] = -1
-    interior = node_block >= 0
+        sliding = [n for n, (kind, _) in entities.items() if kind != "free"]
+        where[np.asarray(sliding, np.int64)] = -1
 
-    m_node = st["m_node"]
-    keep = free_mask[m_node] & ~interior[m_node]
-    order = np.argsort(m_node[keep], kind="stable")
-    m_node = m_node[keep][order]
-    m_sid = st["m_sid"][keep][order]
-    m_role = st["m_role"][keep][order]
-
-    def fields(sid: np.ndarray) -> dict:
-        out = {"gc": st["gc"][sid].astype(np.int32)}
-        for k in range(d):
-            out[f"gn{k}"] = st["gn"][k][sid].astype(np.int32)
-            out[f"s{k}"] = st["s"][k][sid].astype(np.float64)
-        out["W_inv"] = (np.asarray(w_inv, np.float64).reshape(1, dd) if uniform
-                        else np.ascontiguousarray(w_inv[sid].reshape(-1, dd)))
+    def sample_part(sid: np.ndarray) -> dict:
+        out = cs.fields(sid)
+        out["W_inv"] = W.reshape(1, d * d) if uniform else np.ascontiguousarray(W[sid].reshape(-1, d * d))
         if weight is not None:
             out["weight"] = np.ascontiguousarray(weight[sid])
         return out
 
     groups = []
-    dofs = np.flatnonzero(free_mask)
+    dofs = np.flatnonzero(free)
     if dofs.size:
-        uniq, cnt = np.unique(m_node, return_counts=True)
-        P_of = np.zeros(dofs.size, np.int32)
-        P_of[np.searchsorted(dofs, uniq)] = cnt
-        g = {
+        node, sid, role = cs.incidence(free & (where < 0))
+        group = {
             "D": int(dofs.size),
-            "role": m_role.astype(np.int32),
+            "role": role.astype(np.int32),
             "dof_idx": dofs.astype(np.int32),
-            "P_of": P_of,
-            "interior_block": np.ascontiguousarray(node_block[dofs]),
-            "interior_logical": np.ascontiguousarray(node_logical[dofs]),
-            "entities": _entities_wire(dofs, entities),
+            "P_of": np.bincount(np.searchsorted(dofs, node), minlength=dofs.size).astype(np.int32),
+            "interior_block": np.ascontiguousarray(where[dofs]),
+            "interior_logical": np.ascontiguousarray(logical[dofs]),
+            "entities": _entity_groups(dofs, entities),
         }
-        g.update(fields(m_sid))
+        group.update(sample_part(sid))
         if curvature is not None:
             if d != 2:
                 raise ValueError("curvature windows are 2D only")
-            g.update(curvature_table(free_mask, *curvature))
-        g.update(directional_dof_table(free_mask, directional))
-        groups.append(g)
-    es = {"num_samples": int(ns)}
-    es.update(fields(np.arange(ns)))
-    es.update(directional_energy_table(directional))
-    return {"groups": groups, "energy_stencil": es}
+            group.update(curvature_table(free, *curvature))
+        group.update(directional_dof_table(free, directional))
+        groups.append(group)
+    stencil = {"num_samples": int(ns)}
+    stencil.update(sample_part(np.arange(ns)))
+    stencil.update(directional_energy_table(directional))
+    return {"groups": groups, "energy_stencil": stencil}

@@ -142,6 +142,18 @@ eggmi_status eggmi_mg_levels(eggmi_session* s, int max_levels, int32_t* shapes_o
                              int32_t* n_levels, int32_t* n_blocks);
 
 /* X gathered back to global node order (src/bindings.cpp:1359-1376). */
+/* Introspection of FAS coarse level `level` (1 = the first below the fine
+ * grid; no reference counterpart, the reference keeps MgLevel internal,
+ * src/mg_level.hpp:73-112): its slot count, its copy table (face-ghost
+ * fills first, n_halo of them, then n_share owner -> copy broadcasts; node
+ * slots of the level's padded layout) and its relaxed-DOF masks (free:
+ * every DOF its sweeps relax; free_interior: the block-interior ones).
+ * Call once with null arrays for the sizes. Available in tables-only
+ * sessions (device 3). */
+eggmi_status eggmi_mg_level_tables(eggmi_session* s, int level, int64_t* n_slots, int64_t* n_halo,
+                                   int64_t* n_share, int32_t* copy_src, int32_t* copy_dst, uint8_t* free_mask,
+                                   uint8_t* free_interior);
+
 eggmi_status eggmi_get_X(eggmi_session* s, double* X_out, int64_t n);
 
 /* Replace the resident positions (global order) — checkpoint resume. */
