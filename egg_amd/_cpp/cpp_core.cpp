@@ -492,6 +492,26 @@ class Session
         return d;
     }
 
+    py::array_t<double> mg_probe(int level, int op, const DArr& field)
+    {
+        int64_t ns = 0;
+        check(eggmi_mg_level_tables(s_, level, &ns, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr));
+        int64_t np_ = 0;
+        if (level == 1) {
+            eggmi_info info {};
+            check(eggmi_session_info(s_, &info));
+            np_ = info.n_slots;
+        } else {
+            check(eggmi_mg_level_tables(s_, level - 1, &np_, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr));
+        }
+        const int64_t n_in = (op == 2 ? ns : np_) * dim_, n_out = (op == 2 ? np_ : ns) * dim_ * (op == 3 ? 2 : 1);
+        if (op != 3 && (int64_t)field.size() != n_in)
+            throw py::value_error("mg_probe: field must hold dim * n_slots = " + std::to_string(n_in) + " values");
+        py::array_t<double> out(n_out);
+        check(eggmi_mg_probe(s_, level, op, op == 3 ? nullptr : field.data(), out.mutable_data()));
+        return out;
+    }
+
     py::list mg_levels(int max_levels)
     {
         int32_t nl = 0, nb = 0;
@@ -684,6 +704,9 @@ PYBIND11_MODULE(cpp_core, m)
       .def("run_fas", &Session::run_fas, py::arg("n_cycles"), py::kw_only(), py::arg("nu_pre") = 2,
            py::arg("nu_post") = 2, py::arg("nu_coarse") = 32, py::arg("omega") = 1.0, py::arg("max_levels") = 32,
            py::arg("phase") = "barrier", "Run n_cycles FAS V-cycles; returns per-cycle (energies, mindets).")
+      .def("mg_probe", &Session::mg_probe, py::arg("level"), py::arg("op"), py::arg("field"),
+           "FAS transfer operator `op` (0 inject, 1 full weighting, 2 prolongation, 3 tau check) between "
+           "coarse level `level` and its parent on a [dim][n_slots] field.")
       .def("mg_level_tables", &Session::mg_level_tables, py::arg("level"),
            "Copy tables (halo | share, node slots) and relaxed-DOF masks of FAS coarse level `level` (>= 1).")
       .def("mg_levels", &Session::mg_levels, py::arg("max_levels") = 32,
@@ -814,6 +837,54 @@ PYBIND11_MODULE(cpp_core, m)
       py::arg("X"), py::arg("nodes"), py::arg("kind"), py::arg("ref"), py::arg("weight"), py::arg("eps") = 1e-12,
       "Directional soft-energy total and analytic gradient over a sample table (host side, FP64): X (n_nodes, d), "
       "nodes (n, 4) with -1 pads, kind (n,) 0 parallel / 1 line / 2 stem, ref (n, d), weight (n,).");
+
+    // geometry entity calls (reference src/bindings.cpp:1872-1990) for the
+    // closed-form 2D kinds the compute core supports
+    auto geo_record = [](const DArr& params, int tag) {
+        const int need_n = tag == 2 ? 3 : (tag == 0 ? 0 : 4);
+        if ((int)params.size() < need_n)
+            throw py::value_error("geometry: tag " + std::to_string(tag) + " needs " + std::to_string(need_n) +
+                                  " parameters");
+        std::array<double, 4> rec {0.0, 0.0, 0.0, 0.0};
+        for (int k = 0; k < need_n; ++k) rec[k] = params.data()[k];
+        return rec;
+    };
+    m.def(
+      "geometry_project",
+      [geo_record](const DArr& p, int tag, const DArr& params) {
+          if (p.size() < 2) throw py::value_error("geometry_project: p must hold 2 coordinates");
+          const auto rec = geo_record(params, tag);
+          py::array_t<double> out(2);
+          check(eggmi_geometry_eval(tag, rec.data(), 1, p.data(), out.mutable_data(), nullptr));
+          return out;
+      },
+      py::arg("p"), py::arg("tag"), py::arg("params"), "Project point onto entity. Returns proj(2,).");
+    m.def(
+      "geometry_tangent",
+      [geo_record](const DArr& p, int tag, const DArr& params) {
+          if (p.size() < 2) throw py::value_error("geometry_tangent: p must hold 2 coordinates");
+          const auto rec = geo_record(params, tag);
+          py::array_t<double> out(2);
+          check(eggmi_geometry_eval(tag, rec.data(), 1, p.data(), nullptr, out.mutable_data()));
+          return out;
+      },
+      py::arg("p"), py::arg("tag"), py::arg("params"), "Tangent direction at point on entity. Returns tang(2,).");
+    m.def(
+      "geometry_project_batch",
+      [geo_record](const DArr& q, int tag, const DArr& records, const py::list& seg) {
+          if (q.ndim() != 2 || q.shape(1) != 2)
+              throw py::value_error("geometry_project_batch: q must be (n, 2) (the compute core's entities are 2D)");
+          if (tag == 0) throw py::value_error("geometry_project_batch: tag 0 (free) has no foot");
+          if (py::len(seg) != 0)
+              throw py::value_error("geometry_project_batch: segmented entity data is not supported here");
+          const auto rec = geo_record(records, tag);
+          const py::ssize_t n = q.shape(0);
+          py::array_t<double> feet({n, (py::ssize_t)2}), basis({n, (py::ssize_t)2, (py::ssize_t)1});
+          check(eggmi_geometry_eval(tag, rec.data(), n, q.data(), feet.mutable_data(), basis.mutable_data()));
+          return py::make_tuple(feet, basis);
+      },
+      py::arg("q"), py::arg("tag"), py::arg("records"), py::arg("seg"),
+      "Project (n, 2) points onto one entity: (feet (n, 2), tangent basis (n, 2, 1)).");
 
     m.def(
       "pcg_csr",

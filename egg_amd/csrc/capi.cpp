@@ -1,6 +1,7 @@
 // C ABI of the MI355X structured smoother (include/eggmi.h). Converts
 // exceptions to status codes and keeps the calling thread's last message.
 #include "eggmi.h"
+#include "entity.hpp"
 #include "mlp_host.hpp"
 #include "pcg.hpp"
 #include "session.hpp"
@@ -158,17 +159,60 @@ eggmi_status eggmi_mg_level_tables(eggmi_session* s, int level, int64_t* n_slots
     return guarded([&] {
         need(s, "session");
         s->s.build_hierarchy();
-        if (level < 1 || level > (int)s->s.coarse.size())
-            throw eggmi::InvalidArgument("mg level " + std::to_string(level) + " outside [1, " +
+        if (level < 0 || level > (int)s->s.coarse.size())
+            throw eggmi::InvalidArgument("mg level " + std::to_string(level) + " outside [0, " +
                                          std::to_string(s->s.coarse.size()) + "]");
-        const eggmi::Level& L = *s->s.coarse[level - 1];
+        const eggmi::Level& L = level == 0 ? s->s.fine : *s->s.coarse[level - 1];
         if (n_slots) *n_slots = L.lay.n_slots();
-        if (n_halo) *n_halo = L.n_halo;
+        if (n_halo) *n_halo = L.n_halo + L.n_fan;
         if (n_share) *n_share = L.n_share;
         if (copy_src) std::copy(L.copy_src.begin(), L.copy_src.end(), copy_src);
         if (copy_dst) std::copy(L.copy_dst.begin(), L.copy_dst.end(), copy_dst);
-        if (free_mask) std::copy(L.free_mask.begin(), L.free_mask.end(), free_mask);
-        if (free_interior) std::copy(L.free_interior.begin(), L.free_interior.end(), free_interior);
+        const std::vector<uint8_t>& fr = level == 0 ? s->s.fine_free : L.free_mask;
+        const std::vector<uint8_t>& fi = level == 0 ? s->s.fine.mask : L.free_interior;
+        if (free_mask) std::copy(fr.begin(), fr.end(), free_mask);
+        if (free_interior) std::copy(fi.begin(), fi.end(), free_interior);
+    });
+}
+
+eggmi_status eggmi_mg_probe(eggmi_session* s, int level, int op, const double* in, double* out)
+{
+    return guarded([&] {
+        need(s, "session");
+        need(out, "out");
+        if (op != 3) need(in, "in");
+        s->s.mg_probe(level, op, in, out);
+    });
+}
+
+eggmi_status eggmi_geometry_eval(int tag, const double* record, int64_t n, const double* q, double* proj,
+                                 double* tangent)
+{
+    return guarded([&] {
+        if (tag != 0 && tag != 1 && tag != 2 && tag != 3 && tag != 15)
+            throw eggmi::InvalidArgument("geometry: entity tag " + std::to_string(tag) +
+                                         " is not supported by this compute core (0 free, 1 lineseg, 2 circle, "
+                                         "3 ellipse, 15 linerail)");
+        if (n < 0) throw eggmi::InvalidArgument("geometry: n must be >= 0");
+        if (n == 0) return;
+        need(q, "q");
+        if (tag != 0) need(record, "record");
+        for (int64_t i = 0; i < n; ++i) {
+            const double* p = q + 2 * i;
+            double f[2] = {p[0], p[1]}, t[2] = {1.0, 0.0};
+            if (tag != 0) {
+                eggmi::ent_project(tag, record, p, f);
+                eggmi::ent_tangent(tag, record, p, t);
+            }
+            if (proj) {
+                proj[2 * i] = f[0];
+                proj[2 * i + 1] = f[1];
+            }
+            if (tangent) {
+                tangent[2 * i] = t[0];
+                tangent[2 * i + 1] = t[1];
+            }
+        }
     });
 }
 

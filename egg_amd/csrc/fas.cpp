@@ -422,6 +422,71 @@ TransferArgs Session::transfer(const Level& parent, const Level& child, bool pro
     return t;
 }
 
+// ------------------------------------------------------------------ probe
+
+// The transfer operators between coarse level `level` and its parent
+// applied to caller fields ([dim][n_slots] doubles of the level they live
+// on), for tests of their identities: 0 injection (parent -> child, every
+// interior child node), 1 full weighting of a ghost-inclusive parent field
+// (child relaxed DOFs, zero elsewhere), 2 prolongation of a child field
+// taken as the coarse change (parent relaxed nodes, zero at the other
+// interior nodes), 3 (level 1 only) the tau-shifted coarse gradient at the
+// injected fine state next to the full-weighted fine gradient, which equal
+// each other by construction of tau (two child fields).
+void Session::mg_probe(int level, int op, const double* in, double* out)
+{
+    need_device();
+    if (level < 1) throw InvalidArgument("mg probe: level must be >= 1");
+    if (op < 0 || op > 3) throw InvalidArgument("mg probe: op must be 0..3");
+    if (op == 3 && level != 1) throw InvalidArgument("mg probe: the tau check runs on level 1");
+    const size_t n = prepare_fas(level + 1);
+    if ((size_t)level > n)
+        throw InvalidArgument("mg probe: the hierarchy has " + std::to_string(coarse.size()) + " coarse levels");
+    Level& P = level == 1 ? fine : *coarse[level - 2];
+    Level& C = *coarse[level - 1];
+    auto to_device = [&](const Level& L, const double* src) {
+        std::vector<float> h((size_t)L.plane * dim, 0.0f);
+        const long long ns = L.lay.n_slots();
+        if (src)
+            for (int k = 0; k < dim; ++k)
+                for (long long q = 0; q < ns; ++q) h[(size_t)k * L.plane + q] = (float)src[(size_t)k * ns + q];
+        DArr<float> d;
+        d.upload(h);
+        return d;
+    };
+    auto to_host = [&](const Level& L, const float* dev, double* dst) {
+        std::vector<float> h((size_t)L.plane * dim);
+        EGGMI_HIP(hipStreamSynchronize(stream));
+        EGGMI_HIP(hipMemcpy(h.data(), dev, h.size() * sizeof(float), hipMemcpyDeviceToHost));
+        const long long ns = L.lay.n_slots();
+        for (int k = 0; k < dim; ++k)
+            for (long long q = 0; q < ns; ++q) dst[(size_t)k * ns + q] = h[(size_t)k * L.plane + q];
+    };
+    if (op == 0 || op == 1) {
+        DArr<float> f = to_device(P, in), c = to_device(C, nullptr);
+        const TransferArgs t = transfer(P, C, false);
+        if (op == 0)
+            launch_restrict(dim, t, f.p, c.p, nullptr, nullptr, C.d_free.p, stream);
+        else
+            launch_restrict(dim, t, nullptr, nullptr, f.p, c.p, C.d_free.p, stream);
+        to_host(C, c.p, out);
+    } else if (op == 2) {
+        DArr<float> c = to_device(C, in), zero = to_device(C, nullptr), f = to_device(P, nullptr);
+        launch_prolong(dim, transfer(P, C, true), c.p, zero.p, f.p, P.d_free.p, false, nullptr, nullptr, stream);
+        to_host(P, f.p, out);
+    } else {
+        float* x = P.x[P.cur].p;
+        grad(P, x, P.grad.p, nullptr, 0, false, true);
+        restrict_to(P, x, C, 0);
+        DArr<float> shifted = to_device(C, nullptr), rg = to_device(C, nullptr);
+        grad(C, C.rx.p, shifted.p, C.tau.p, 0);
+        launch_restrict(dim, transfer(P, C, false), nullptr, nullptr, P.grad.p, rg.p, C.d_free.p, stream);
+        to_host(C, shifted.p, out);
+        to_host(C, rg.p, out + (size_t)dim * C.lay.n_slots());
+    }
+    EGGMI_HIP(hipStreamSynchronize(stream));
+}
+
 // ------------------------------------------------------------------ driver
 
 void Session::smooth(Level& L, int sweeps, int kind, float omega, const float* tau)

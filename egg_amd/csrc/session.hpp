@@ -296,6 +296,8 @@ struct Session {
     std::vector<double> autotune_levels(int n_sweeps, double omega, int max_levels, int kind);
     // FAS internals
     void build_hierarchy();
+    // transfer operators of a level pair applied to caller fields (fas.cpp)
+    void mg_probe(int level, int op, const double* in, double* out);
     size_t prepare_fas(int max_levels);  // device state of the levels a cycle uses; their count
     void upload_level(size_t i);
     TransferArgs transfer(const Level& parent, const Level& child, bool prolong) const;

@@ -24,6 +24,7 @@
 #pragma once
 
 #include "directional.hpp"
+#include "entity.hpp"
 #include "mlp.hpp"
 #include "sweep_common.hpp"
 
@@ -151,63 +152,7 @@ __device__ __forceinline__ void count_update(const SweepArgs& a, int ntr, bool p
 
 // Stored-stencil DOF d handled by a group of G lanes (lane l owns
 // occurrences l, l+G, ... of the DOF's patch).
-// 2D entities a constrained boundary DOF slides on (reference
-// src/geometry.hpp:288-320 LineSeg, :322-357 LineRail, :398-431 Circle,
-// :432-464 Ellipse; FP32 tolerances of src/core.hpp: znorm 1e-7, tiny
-// 1e-20): tag 1 segment (sx, sy, ex, ey), 2 circle (cx, cy, r), 3 ellipse
-// (cx, cy, rx, ry), 15 rail (a segment that releases the node beyond its ends).
-__device__ __forceinline__ float rail_foot(const float* e, const float* p)
-{
-    const float abx = e[2] - e[0], aby = e[3] - e[1];
-    return ((p[0] - e[0]) * abx + (p[1] - e[1]) * aby) / fmaxf(abx * abx + aby * aby, 1e-20f);
-}
-__device__ __forceinline__ void ent_project(int tag, const float* e, const float* p, float* out)
-{
-    if (tag == 1 || tag == 15) {
-        const float abx = e[2] - e[0], aby = e[3] - e[1];
-        float t = rail_foot(e, p);
-        const bool released = tag == 15 && (t < 0.0f || t > 1.0f);
-        t = fminf(fmaxf(t, 0.0f), 1.0f);
-        out[0] = released ? p[0] : e[0] + t * abx;
-        out[1] = released ? p[1] : e[1] + t * aby;
-    } else if (tag == 3) {
-        const float sx = (p[0] - e[0]) / e[2], sy = (p[1] - e[1]) / e[3];
-        const float dist = sqrtf(sx * sx + sy * sy);
-        out[0] = e[0] + (dist < 1e-7f ? 0.70710678f : sx / dist) * e[2];
-        out[1] = e[1] + (dist < 1e-7f ? 0.70710678f : sy / dist) * e[3];
-    } else {
-        const float dx = p[0] - e[0], dy = p[1] - e[1];
-        const float dist = sqrtf(dx * dx + dy * dy);
-        out[0] = dist < 1e-7f ? e[0] + e[2] : e[0] + e[2] * dx / dist;
-        out[1] = dist < 1e-7f ? e[1] : e[1] + e[2] * dy / dist;
-    }
-}
-__device__ __forceinline__ void ent_tangent(int tag, const float* e, const float* p, float* t)
-{
-    if (tag == 3) {
-        // (-rx sin a, ry cos a) at the parametric angle a of the radially
-        // scaled point: sin a, cos a as its coordinates over its length
-        const float sx = (p[0] - e[0]) / e[2], sy = (p[1] - e[1]) / e[3];
-        const float dist = sqrtf(sx * sx + sy * sy);
-        const float sa = dist < 1e-7f ? 0.0f : sy / dist, ca = dist < 1e-7f ? 1.0f : sx / dist;
-        const float tx = -e[2] * sa, ty = e[3] * ca;
-        const float n = sqrtf(tx * tx + ty * ty);
-        t[0] = n < 1e-7f ? 1.0f : tx / n;
-        t[1] = n < 1e-7f ? 0.0f : ty / n;
-        return;
-    }
-    if (tag == 1 || tag == 15) {
-        const float abx = e[2] - e[0], aby = e[3] - e[1];
-        const float n = sqrtf(abx * abx + aby * aby);
-        t[0] = n < 1e-7f ? 1.0f : abx / n;
-        t[1] = n < 1e-7f ? 0.0f : aby / n;
-    } else {
-        const float dx = p[0] - e[0], dy = p[1] - e[1];
-        const float rn = sqrtf(dx * dx + dy * dy);
-        t[0] = rn < 1e-7f ? -0.0f : -dy / rn;
-        t[1] = rn < 1e-7f ? 1.0f : dx / rn;
-    }
-}
+// The entities a constrained boundary DOF slides on: entity.hpp.
 
 // Block-interface C2 curvature window p0..p3 (reference src/curvature.hpp):
 // residual r = turn(p0, p1, p2) - turn(p1, p2, p3) of the signed turning

@@ -148,11 +148,39 @@ eggmi_status eggmi_mg_levels(eggmi_session* s, int max_levels, int32_t* shapes_o
  * fills first, n_halo of them, then n_share owner -> copy broadcasts; node
  * slots of the level's padded layout) and its relaxed-DOF masks (free:
  * every DOF its sweeps relax; free_interior: the block-interior ones).
+ * Level 0 is the fine store: its copy table is halo | fan | share, with the
+ * fan mirrors counted in n_halo, and free_interior its synthesized DOFs.
  * Call once with null arrays for the sizes. Available in tables-only
  * sessions (device 3). */
 eggmi_status eggmi_mg_level_tables(eggmi_session* s, int level, int64_t* n_slots, int64_t* n_halo,
                                    int64_t* n_share, int32_t* copy_src, int32_t* copy_dst, uint8_t* free_mask,
                                    uint8_t* free_interior);
+
+/* The FAS transfer operators between coarse level `level` (>= 1) and its
+ * parent applied to caller fields, for tests of their identities (the
+ * reference tests its transfer kernels directly,
+ * tests/cpp/test_multigrid_device.cpp:76-220, 389-470). Fields are
+ * [dim][n_slots] doubles of the level they live on (n_slots from
+ * eggmi_mg_level_tables; level 0 = the fine store, eggmi_session_info).
+ * op 0: injection, in = parent field, out = child field (every interior
+ * node); op 1: (1/4, 1/2, 1/4) full weighting of a ghost-inclusive parent
+ * field, out = child field at its relaxed DOFs, zero elsewhere; op 2:
+ * multilinear prolongation of a child field (the coarse change), out =
+ * parent field at its relaxed nodes, zero at its other interior nodes;
+ * op 3 (level 1, in unused): out = the tau-shifted coarse gradient at the
+ * injected resident state, then the full-weighted fine gradient (two child
+ * fields; equal by construction of tau, src/fas.hpp:522-532). */
+eggmi_status eggmi_mg_probe(eggmi_session* s, int level, int op, const double* in, double* out);
+
+/* Projection of n 2D points q[n*2] onto one entity and the unit tangent
+ * there, in FP64 with the same closed forms the constrained sweep uses:
+ * tag 0 free (identity, tangent e_0), 1 segment, 2 circle, 3 ellipse
+ * (radial scaling), 15 rail (released beyond its ends), record as on the
+ * wire. proj / tangent (n*2 each) may be null. Replaces the host geometry
+ * calls of the binding for these kinds: geometry_project / geometry_tangent
+ * / geometry_project_batch (src/bindings.cpp:1872-1990). */
+eggmi_status eggmi_geometry_eval(int tag, const double* record, int64_t n, const double* q, double* proj,
+                                 double* tangent);
 
 eggmi_status eggmi_get_X(eggmi_session* s, double* X_out, int64_t n);
 

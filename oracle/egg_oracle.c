@@ -569,6 +569,20 @@ static void FN(dir_accumulate)(int dim, int kind, const real p[3][3], const doub
     }
 }
 
+/* One directional sample's gradient, Hessian and energy for its node
+ * `slot` (dir_accumulate, for tests); p holds arity x dim positions. */
+void FN(o_dir_eval)(int dim, int kind, const real* p, const double* ref, real w, real eps, int slot, real* grad,
+                    real* hess, real* e0)
+{
+    real q[3][3] = {{0}};
+    for (int m = 0; m < dir_arity(kind); ++m)
+        for (int k = 0; k < dim; ++k) q[m][k] = p[m * dim + k];
+    for (int i = 0; i < dim; ++i) grad[i] = 0;
+    for (int i = 0; i < dim * dim; ++i) hess[i] = 0;
+    *e0 = 0;
+    FN(dir_accumulate)(dim, kind, q, ref, w, eps, slot, grad, hess, e0);
+}
+
 /* Sample positions of window j of per-DOF table row d (DOF at `trial` when given). */
 static int FN(dir_load)(int dim, const dir_t* dw, long long w, const real* X, int slot, const real* trial,
                         real p[3][3])
@@ -735,6 +749,23 @@ static void FN(ent_tangent)(int tag, const real* e, const real* p, real* t)
     }
 }
 
+/* Projection of p onto entity (tag, record) and the unit tangent there, the
+ * pair the reference's geometry_project / geometry_tangent bindings return
+ * (src/bindings.cpp:1872-1943); tag 0 (free) leaves p and has tangent e_0
+ * (Free<2>, src/geometry.hpp). Pinned by tests/cpp/golden_geometry.hpp. */
+void FN(o_entity_eval)(int tag, const real* rec, const real* p, real* proj, real* tang)
+{
+    if (tag == 0) {
+        proj[0] = p[0];
+        proj[1] = p[1];
+        tang[0] = 1;
+        tang[1] = 0;
+        return;
+    }
+    FN(ent_project)(tag, rec, p, proj);
+    FN(ent_tangent)(tag, rec, p, tang);
+}
+
 /* One constrained DOF's update (reference src/sweep.hpp:1285-1403 and
  * :1848-1871): tangent-reduced Newton step delta = t * y with
  * y = solve1x1(t.H.t, t.g) (src/solve.hpp:83-90, src/patch.hpp:348-388),
@@ -850,6 +881,17 @@ static void FN(curv_accumulate)(int K, const int* nodes, const int* slot, const 
         hess[3] += (real)(hc * dr[s][1] * dr[s][1] + stab);
         *e0 += (real)(w * r * r);
     }
+}
+
+/* One curvature window's gradient, Hessian and energy for its node `slot`
+ * (the per-window body of curv_accumulate, for tests). */
+void FN(o_curv_eval)(const real* p8, real w, int slot, real* grad, real* hess, real* e0)
+{
+    int nodes[4] = {0, 1, 2, 3};
+    grad[0] = grad[1] = 0;
+    hess[0] = hess[1] = hess[2] = hess[3] = 0;
+    *e0 = 0;
+    FN(curv_accumulate)(1, nodes, &slot, &w, p8, grad, hess, e0);
 }
 
 /* The windows' energy with the DOF (node `slot` of each) moved to `trial`

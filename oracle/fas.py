@@ -44,12 +44,17 @@ def _interior_patches(dim, ids, dofs_logical):
     return O.Patches(dim, dof, poff, gc, gn, s, role, np.eye(dim).ravel())
 
 
-def build_levels(dim, ids, free, max_levels=32):
-    """Level 0 = the fine block; returns the list of levels."""
+def build_levels(dim, ids, free, max_levels=32, synth=None):
+    """Level 0 = the fine block; returns the list of levels. ``synth`` (bool
+    per node) overrides which fine nodes count as synthesized interior DOFs
+    (a DOF carrying curvature or directional windows ships a stored patch,
+    so its coarse image is no DOF)."""
     shape = ids.shape
     free_int = np.zeros(ids.size, bool)
     inner = ids[tuple(slice(1, n - 1) for n in shape)].reshape(-1)
     free_int[inner] = free[inner]
+    if synth is not None:
+        free_int &= np.asarray(synth, bool)
     levels = [_Level(ids, free.copy(), free_int)]
     for _ in range(1, max_levels):
         p = levels[-1]
@@ -128,12 +133,18 @@ def _stencil(dim, ids):
 
 
 def run_fas(dim, kind, ids, free, X, *, n_cycles, nu_pre=2, nu_post=2, nu_coarse=32, omega=1.0, max_levels=32,
-            prec="f64"):
-    """Returns (X_out, energies, mindets) per cycle."""
+            prec="f64", fine_patches=None, fine_stencil=None, synth=None):
+    """Returns (X_out, energies, mindets) per cycle. ``fine_patches`` /
+    ``fine_stencil`` replace the fine level's synthesized patches and plain
+    cell stencil (a wire with soft terms: its windows enter the fine sweeps,
+    the fine gradient that is restricted, and the line search); ``synth``
+    as in :func:`build_levels`."""
     tol = 1e-12 if prec == "f64" else 1e-6
-    levels = build_levels(dim, ids, free)
+    levels = build_levels(dim, ids, free, synth=synth)
+    if fine_patches is not None:
+        levels[0].patches = fine_patches
     n_coarse = min(len(levels) - 1, max_levels - 1) if max_levels > 1 else 0
-    es = _stencil(dim, ids)
+    es = _stencil(dim, ids) if fine_stencil is None else fine_stencil
     X = np.asarray(X, np.float64).copy()
     if n_coarse == 0:
         per = nu_pre + nu_post

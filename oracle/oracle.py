@@ -60,6 +60,15 @@ def lib():
             f = getattr(L, "o_solve" + sfx)
             f.argtypes = [C.c_int, rp, rp, rp]
             f.restype = None
+            f = getattr(L, "o_entity_eval" + sfx)
+            f.argtypes = [C.c_int, rp, rp, rp, rp]
+            f.restype = None
+            f = getattr(L, "o_curv_eval" + sfx)
+            f.argtypes = [rp, rt, C.c_int, rp, rp, rp]
+            f.restype = None
+            f = getattr(L, "o_dir_eval" + sfx)
+            f.argtypes = [C.c_int, C.c_int, rp, _dp, rt, rt, C.c_int, rp, rp, rp]
+            f.restype = None
             f = getattr(L, "o_energy_mindet" + sfx)
             f.argtypes = [C.c_int, C.c_int, C.c_double, C.c_longlong, _ip, _ip, rp, rp, C.c_int, C.c_void_p, rp,
                           C.c_void_p, _dp, _dp]
@@ -169,6 +178,39 @@ def solve(dim, H, g, prec="f64"):
         dim, np.ascontiguousarray(H, dt).reshape(-1), np.ascontiguousarray(g, dt), x
     )
     return x
+
+
+def entity_eval(tag: int, record, p, prec="f64"):
+    """Projection of the 2D point p onto entity (tag, record) and the unit
+    tangent there (restating the reference's geometry_project /
+    geometry_tangent for the closed-form kinds)."""
+    dt = _dt(prec)
+    rec = np.zeros(4, dt)
+    rec[: len(record)] = record
+    proj = np.zeros(2, dt)
+    tang = np.zeros(2, dt)
+    getattr(lib(), f"o_entity_eval_{prec}")(int(tag), rec, np.ascontiguousarray(p, dt), proj, tang)
+    return proj, tang
+
+
+def curv_eval(p4, w, slot, prec="f64"):
+    """Gradient, Hessian (2x2) and energy of one curvature window (four 2D
+    points) for its node `slot`."""
+    dt = _dt(prec)
+    g, H, e = np.zeros(2, dt), np.zeros(4, dt), np.zeros(1, dt)
+    getattr(lib(), f"o_curv_eval_{prec}")(np.ascontiguousarray(p4, dt).reshape(-1), float(w), int(slot), g, H, e)
+    return g, H.reshape(2, 2), float(e[0])
+
+
+def dir_eval(dim, kind, p, ref, w, eps, slot, prec="f64"):
+    """Gradient, Hessian (dim x dim) and energy of one directional sample
+    (positions p, arity x dim) for its node `slot`."""
+    dt = _dt(prec)
+    g, H, e = np.zeros(dim, dt), np.zeros(dim * dim, dt), np.zeros(1, dt)
+    getattr(lib(), f"o_dir_eval_{prec}")(int(dim), int(kind), np.ascontiguousarray(p, dt).reshape(-1),
+                                         np.ascontiguousarray(ref, np.float64), float(w), float(eps), int(slot),
+                                         g, H, e)
+    return g, H.reshape(dim, dim), float(e[0])
 
 
 def _stencil_arrays(es: dict, dim: int, dt):

@@ -1,7 +1,8 @@
 """Transcribe the reference's golden vectors into JSON fixtures.
 
 Reads the generated header files tests/cpp/golden_metric.hpp,
-tests/cpp/golden_metric_3d.hpp and tests/cpp/golden_patch.hpp of the
+tests/cpp/golden_metric_3d.hpp, tests/cpp/golden_patch.hpp and
+tests/cpp/golden_geometry.hpp of the
 reference checkout (numbers only: inputs and expected outputs of the
 reference's own C++ oracle bindings, produced there by
 tests/cpp/gen_golden.py) and writes them as data under tests/golden/.
@@ -47,18 +48,22 @@ def main() -> None:
     metric = _records(REF / "golden_metric.hpp", "kSamples")
     metric3d = _records(REF / "golden_metric_3d.hpp", "kSamples3D")
     patch = _records(REF / "golden_patch.hpp", "kPatchSamples")
-    assert len(metric) == 8 and len(metric3d) == 8 and len(patch) == 5, (
+    geometry = _records(REF / "golden_geometry.hpp", "kGeoSamples")
+    assert len(metric) == 8 and len(metric3d) == 8 and len(patch) == 5 and len(geometry) == 9, (
         len(metric),
         len(metric3d),
         len(patch),
+        len(geometry),
     )
+    for rec in geometry:
+        rec["tag"] = int(rec["tag"])
     for rec in patch:
         for k in ("n_nodes", "P", "tag"):
             rec[k] = int(rec[k])
         for k in ("gc", "gn0", "gn1", "role"):
             rec[k] = [int(v) for v in rec[k]]
     prov = {
-        "source": "bezmi/egg tests/cpp/golden_{metric,metric_3d,patch}.hpp",
+        "source": "bezmi/egg tests/cpp/golden_{metric,metric_3d,patch,geometry}.hpp",
         "generator": "reference tests/cpp/gen_golden.py (C++ oracle bindings)",
         "note": "numbers transcribed verbatim; patch RNG seed 20240601 upstream",
     }
@@ -67,7 +72,8 @@ def main() -> None:
         json.dumps({"provenance": prov, "samples": metric3d}, indent=1)
     )
     (OUT / "golden_patch.json").write_text(json.dumps({"provenance": prov, "samples": patch}, indent=1))
-    print("wrote", len(metric), len(metric3d), len(patch))
+    (OUT / "golden_geometry.json").write_text(json.dumps({"provenance": prov, "samples": geometry}, indent=1))
+    print("wrote", len(metric), len(metric3d), len(patch), len(geometry))
 
 
 if __name__ == "__main__":

@@ -170,3 +170,30 @@ def test_directional_graphs_and_fas():
     np.testing.assert_array_equal(e2, e1)
     np.testing.assert_array_equal(fb.get_X(), fa.get_X())
     assert np.all(m1 > 0) and e1[-1] < e1[0]
+
+
+def test_directional_fas_parity(oracle):
+    """FAS V-cycles with directional terms against the FAS oracle fed the
+    same wire: the windows enter the fine smoothing, the fine gradient the
+    restriction reads (the special-DOF gradient pass) and the line search
+    (the directional table in the device ladder); DOFs carrying windows are
+    no synthesized interior DOFs, so their coarse images do not relax."""
+    from oracle import fas
+
+    c, g = _block()
+    grp = c.ctx.wire["groups"][0]
+    synth = np.zeros(len(g.X), bool)
+    synth[np.asarray(grp["dof_idx"])[np.asarray(grp["interior_block"]) >= 0]] = True
+    K = int(grp["dir_k"])
+    windowed = np.asarray(grp["dir_slot"]).reshape(-1, K).max(axis=1) >= 0
+    synth[np.asarray(grp["dof_idx"])[windowed]] = False
+    print("directional FAS parity: 4 cycles, 3 levels", flush=True)
+    s = _session(c)
+    e, m = s.run_fas(4, nu_pre=2, nu_post=2, omega=0.8, max_levels=3)
+    Xo, eo, mo = fas.run_fas(2, "barrier", np.asarray(g.blocks[0]), g.free, c.X, n_cycles=4, nu_pre=2, nu_post=2,
+                             omega=0.8, max_levels=3, fine_patches=c.patches(oracle),
+                             fine_stencil=c.ctx.energy_stencil, synth=synth)
+    print("gpu", e, "\noracle", eo)
+    assert np.all(m > 0)
+    np.testing.assert_allclose(e, eo, rtol=1e-4)
+    assert np.abs(s.get_X() - Xo).max() / _extent(c) < POS_TOL

@@ -522,3 +522,142 @@ def _directional_fd(oracle, g):
             Xm[dof, k] -= h
             fd = (oracle.energy_mindet(d, "shape", 0.0, es, Xp)[0] - oracle.energy_mindet(d, "shape", 0.0, es, Xm)[0]) / (2 * h)
             assert G[dof, k] == pytest.approx(fd, rel=1e-5, abs=1e-7), (dof, k)
+
+
+def test_entity_projection_golden(oracle):
+    """The reference's golden geometry vectors (tests/cpp/golden_geometry.hpp:
+    free, segment with clamping at both ends, circle incl. its centre,
+    ellipse incl. its centre) against the oracle's projection and tangent
+    and the compute core's host geometry calls (cpp_core.geometry_project /
+    geometry_tangent / geometry_project_batch, the closed forms the
+    constrained sweep runs)."""
+    import json
+    from pathlib import Path
+
+    from egg_amd._cpp import cpp_core
+
+    samples = json.loads((Path(__file__).parent / "golden" / "golden_geometry.json").read_text())["samples"]
+    assert len(samples) == 9
+    for s in samples:
+        rec = s["params"][:4]
+        proj, tang = oracle.entity_eval(s["tag"], rec, s["p"])
+        np.testing.assert_allclose(proj, s["proj"], rtol=0, atol=1e-14)
+        np.testing.assert_allclose(tang, s["tang"], rtol=0, atol=1e-14)
+        np.testing.assert_allclose(cpp_core.geometry_project(np.asarray(s["p"]), s["tag"], np.asarray(rec)), s["proj"],
+                                   rtol=0, atol=1e-14)
+        np.testing.assert_allclose(cpp_core.geometry_tangent(np.asarray(s["p"]), s["tag"], np.asarray(rec)), s["tang"],
+                                   rtol=0, atol=1e-14)
+        if s["tag"]:
+            feet, basis = cpp_core.geometry_project_batch(np.asarray([s["p"]]), s["tag"], np.asarray(rec), [])
+            np.testing.assert_allclose(feet[0], s["proj"], rtol=0, atol=1e-14)
+            np.testing.assert_allclose(basis[0, :, 0], s["tang"], rtol=0, atol=1e-14)
+    with pytest.raises(ValueError, match="not supported"):
+        cpp_core.geometry_project(np.zeros(2), 7, np.zeros(12))
+
+
+def test_rail_projection_closed_form(oracle):
+    """A rail projects like its segment between the ends and releases the
+    point beyond them; its tangent is the segment's direction everywhere."""
+    rec = [0.0, 0.0, 2.0, 1.0]
+    inside, t = oracle.entity_eval(15, rec, [1.0, 1.5])
+    np.testing.assert_allclose(inside, [1.4, 0.7], atol=1e-15)
+    np.testing.assert_allclose(t, np.array([2.0, 1.0]) / np.sqrt(5.0), atol=1e-15)
+    for p in ([-1.0, -1.0], [5.0, 4.0]):
+        out, t = oracle.entity_eval(15, rec, p)
+        np.testing.assert_array_equal(out, p)
+
+
+def _fd_hessian(energy, x0, h=1e-5):
+    d = x0.size
+    H = np.zeros((d, d))
+    for i in range(d):
+        for j in range(d):
+            ei, ej = np.eye(d)[i] * h, np.eye(d)[j] * h
+            H[i, j] = (energy(x0 + ei + ej) - energy(x0 + ei - ej) - energy(x0 - ei + ej) + energy(x0 - ei - ej)) / (
+                4 * h * h)
+    return H
+
+
+def _turn(a, b, c):
+    e1, e2 = b - a, c - b
+    return np.arctan2(e1[0] * e2[1] - e1[1] * e2[0], e1 @ e2)
+
+
+@pytest.mark.parametrize("shape", ["line", "arc"])
+def test_curvature_hessian_pinned_at_zero_residual(oracle, shape):
+    """Where the window's residual vanishes (straight, or an arc of equal
+    turning angles), the Gauss-Newton block equals the exact Hessian of
+    w r^2 (finite differences of the energy); what remains is the diagonal
+    stabiliser, whose closed form on an evenly spaced straight window of
+    spacing L is 2 w |dr_s| sum_(m != s) |dr_m| = (14, 30, 30, 14) w / L^2
+    (|dr| = (1, 3, 3, 1) / L)."""
+    w, L = 1.7, 0.3
+    if shape == "line":
+        P = np.array([[0.1 + L * m, 0.2] for m in range(4)])
+    else:
+        ang = 0.2 * np.arange(4)
+        P = np.stack([np.cos(ang), np.sin(ang)], 1) * 2.0
+    for slot in range(4):
+        def energy(x):
+            Q = P.copy()
+            Q[slot] = x
+            return w * (_turn(Q[0], Q[1], Q[2]) - _turn(Q[1], Q[2], Q[3])) ** 2
+
+        g, H, e = oracle.curv_eval(P, w, slot)
+        assert abs(e) < 1e-20 and np.abs(g).max() < 1e-12
+        Hfd = _fd_hessian(energy, P[slot].copy())
+        stab = H - Hfd
+        assert abs(stab[0, 1]) < 1e-5 * np.abs(H).max() and abs(stab[0, 0] - stab[1, 1]) < 1e-5 * np.abs(H).max()
+        if shape == "line":
+            assert stab[0, 0] == pytest.approx((14, 30, 30, 14)[slot] * w / L**2, rel=1e-6)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_directional_hessians_pinned_at_zero_residual(oracle, dim):
+    """Parallel / stem samples with the segment perpendicular to the frozen
+    normal, and line samples with opposed legs, have zero residual: there
+    the Gauss-Newton block is the exact Hessian of the energy (finite
+    differences), the parallel / stem stabiliser is 2 w |dc/du|^2 I, which
+    is the trace of that Hessian, and on a symmetric line (legs of length
+    L) the stabiliser 2 w |J_s| (sum |J| - |J_s|) is (6, 8, 6) w / L^2 in
+    2D and (12, 16, 12) w / L^2 in 3D (|J| = sqrt(dim - 1) / L per leg,
+    twice that for the centre)."""
+    from egg_amd.smoothing.directional import KIND_LINE, KIND_PARALLEL, KIND_STEM
+
+    rng = np.random.default_rng(dim)
+    w, eps = 1.3, 1e-12
+    ref = rng.normal(size=dim)
+    ref /= np.linalg.norm(ref)
+    perp = rng.normal(size=dim)
+    perp -= (perp @ ref) * ref
+    perp *= 0.4 / np.linalg.norm(perp)
+    base = rng.normal(size=dim)
+    for kind in (KIND_PARALLEL, KIND_STEM):
+        P = np.stack([base, base + perp]) if kind == KIND_PARALLEL else np.stack([base + perp, base])
+        for slot in (0, 1):
+            def energy(x):
+                Q = P.copy()
+                Q[slot] = x
+                u = (Q[1] - Q[0]) if kind == KIND_PARALLEL else (Q[0] - Q[1])
+                return w * (u @ ref) ** 2 / (u @ u + eps**2)
+
+            g, H, e = oracle.dir_eval(dim, kind, P, ref, w, eps, slot)
+            assert abs(e) < 1e-20
+            Hfd = _fd_hessian(energy, P[slot].copy())
+            np.testing.assert_allclose(H, Hfd + np.trace(Hfd) * np.eye(dim), rtol=0, atol=1e-5 * np.abs(H).max())
+    L = 0.7
+    leg = rng.normal(size=dim)
+    leg *= L / np.linalg.norm(leg)
+    P = np.stack([base + leg, base, base - leg])
+    for slot, coeff in zip((0, 1, 2), (6, 8, 6) if dim == 2 else (12, 16, 12)):
+        def energy(x):
+            Q = P.copy()
+            Q[slot] = x
+            q1, q2 = Q[0] - Q[1], Q[2] - Q[1]
+            r = q1 / np.sqrt(q1 @ q1 + eps**2) + q2 / np.sqrt(q2 @ q2 + eps**2)
+            return w * (r @ r)
+
+        g, H, e = oracle.dir_eval(dim, KIND_LINE, P, ref, w, eps, slot)
+        assert abs(e) < 1e-20
+        Hfd = _fd_hessian(energy, P[slot].copy())
+        np.testing.assert_allclose(H - Hfd, coeff * w / L**2 * np.eye(dim), rtol=0, atol=1e-5 * np.abs(H).max())
