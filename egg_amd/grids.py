@@ -42,26 +42,36 @@ class Grid:
         return int(self.free.sum())
 
 
+def _det(c: list) -> np.ndarray:
+    """Closed-form det of the matrices with columns c[k] (arrays [..., d])."""
+    if len(c) == 2:
+        return c[0][..., 0] * c[1][..., 1] - c[1][..., 0] * c[0][..., 1]
+    return (c[0][..., 0] * (c[1][..., 1] * c[2][..., 2] - c[2][..., 1] * c[1][..., 2])
+            - c[1][..., 0] * (c[0][..., 1] * c[2][..., 2] - c[2][..., 1] * c[0][..., 2])
+            + c[2][..., 0] * (c[0][..., 1] * c[1][..., 2] - c[1][..., 1] * c[0][..., 2]))
+
+
 def corner_dets(blocks, X: np.ndarray, d: int) -> np.ndarray:
-    """det A of every (cell, corner) sample of the blocks."""
+    """det A of every (cell, corner) sample of the blocks, in the wire's
+    sample order: blocks in order, cells row-major, corner b with axis 0 the
+    most significant bit; column k of A is s_k (X[b ^ bit k] - X[b])."""
     out = []
+    nc = 1 << d
     for ids in blocks:
         ids = np.asarray(ids)
-        if any(s < 2 for s in ids.shape):
+        if min(ids.shape) < 2:
             continue
-        nc = 1 << d
-        corners = []
-        for b in range(nc):
-            sl = tuple(slice((b >> (d - 1 - k)) & 1, ((b >> (d - 1 - k)) & 1) + ids.shape[k] - 1) for k in range(d))
-            corners.append(ids[sl].reshape(-1))
-        C = np.stack(corners, 1)
+        P = X[ids]  # (n0, .., n_{d-1}, d)
+        corner = [P[tuple(slice((b >> (d - 1 - k)) & 1, ids.shape[k] - 1 + ((b >> (d - 1 - k)) & 1))
+                          for k in range(d))] for b in range(nc)]
+        dets = []
         for b in range(nc):
             cols = []
             for k in range(d):
                 bit = 1 << (d - 1 - k)
-                s = -1.0 if b & bit else 1.0
-                cols.append(s * (X[C[:, b ^ bit]] - X[C[:, b]]))
-            out.append(np.linalg.det(np.stack(cols, axis=2)))
+                cols.append((corner[b ^ bit] - corner[b]) * (-1.0 if b & bit else 1.0))
+            dets.append(_det(cols).reshape(-1))
+        out.append(np.stack(dets, 1).reshape(-1))
     return np.concatenate(out) if out else np.zeros(0)
 
 
@@ -431,13 +441,20 @@ def strip_2d(sides: list[int], ny: int, *, x0: float = 0.0, y0: float = 0.0, h: 
     sides[k] x ny nodes (axis 0 = x), consecutive blocks share a whole face.
     Uniform spacing h (default 1/(ny-1)); the outer boundary is fixed."""
     h = 1.0 / (ny - 1) if h is None else h
-    blocks, pts = [], []
+    # global node (column c, row j) has id c ny + j: the numbering of a merge
+    # that names nodes in order of first appearance (block 0 row-major, then
+    # each next block's columns past the shared one)
+    cols = 1 + sum(n - 1 for n in sides)
+    first = np.concatenate([[0], np.cumsum([n - 1 for n in sides])[:-1]]).astype(np.int64)
+    grid_ids = np.arange(cols * ny, dtype=np.int64).reshape(cols, ny)
+    ids = [grid_ids[c:c + n] for c, n in zip(first, sides)]
+    xcol = np.empty(cols)
     xs = x0
-    for n in sides:
-        i, j = np.meshgrid(np.arange(n), np.arange(ny), indexing="ij")
-        blocks.append(np.stack([xs + i * h, y0 + j * h], -1))
+    for c0, n in zip(first, sides):
+        xcol[c0:c0 + n] = xs + np.arange(n) * h
         xs += (n - 1) * h
-    ids, X = _merge_fast(blocks, 2)
+    X = np.stack(np.meshgrid(xcol, y0 + np.arange(ny) * h, indexing="ij"), -1).reshape(-1, 2)
+    ids = [b.astype(np.int32) for b in ids]
     x_lo, x_hi = x0, xs
     y_lo, y_hi = y0, y0 + (ny - 1) * h
     free = ~(np.isclose(X[:, 0], x_lo) | np.isclose(X[:, 0], x_hi) | np.isclose(X[:, 1], y_lo)

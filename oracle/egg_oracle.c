@@ -25,11 +25,19 @@
  *                 and min are accumulated in double (FP64 reductions)
  *   o_jacobi      src/sweep.hpp:1425-1650 (metric + interior update kernels),
  *                 :1938-2099 (jacobi_sweep_once / run_block_jacobi)
+ *   o_gauss_seidel  the reference's sequential NumPy node baseline,
+ *                 tests/smoothing/test_control_ref.py:95-135 (converged-state
+ *                 cross-check of the block-Jacobi path)
  *   dir_*         src/directional.hpp (directional soft-energy terms), NumPy
  *                 twin egg/smoothing/directional.py:304-369; energies and
- *                 gradients pinned (closed forms, finite differences), the
- *                 Gauss-Newton block and stabiliser of dir_accumulate:
- *                 parity unpinned (checked only through GPU parity)
+ *                 gradients pinned by closed forms and finite differences,
+ *                 the Gauss-Newton block by finite differences of the energy
+ *                 at zero residual (where it is the exact Hessian), the
+ *                 stabilisers by their closed forms on symmetric samples
+ *   curv_*        src/curvature.hpp (block-interface C2 windows); pinned the
+ *                 same way (straight and circular windows have zero residual)
+ *   o_entity_eval src/geometry.hpp:288-464 projections and tangents, pinned
+ *                 by tests/cpp/golden_geometry.hpp
  *
  * Pinning: golden vectors from tests/cpp/golden_metric.hpp,
  * golden_metric_3d.hpp and golden_patch.hpp (tests/golden/*.json), finite
@@ -1024,6 +1032,84 @@ int FN(o_jacobi)(int dim, int kind, double delta, double omega, int n_sweeps, in
     }
     if (xin != X) memcpy(X, xin, sizeof(real) * (size_t)n_nodes * dim);
     return nrep;
+}
+
+/*
+ * Sequential Gauss-Seidel restatement of the reference's NumPy node
+ * baseline (tests/smoothing/test_control_ref.py:95-135): the DOFs in order,
+ * each updated in place from the current X; Newton step -H^{-1} g of its
+ * patch (skipped when H is exactly singular, as np.linalg.solve refuses
+ * it), alpha = 1, 1/2, ... while alpha >= 1e-4, the first trial with a
+ * finite patch energy <= e0 + 1e-12 and patch min det > 0 taken; stop after
+ * a sweep that moved the nodes by less than 1e-12 in total. Independent of
+ * the block-Jacobi update (no omega, no frozen snapshot, another ladder and
+ * solve), it converges to the same minimiser. Returns the sweeps run.
+ */
+int FN(o_gauss_seidel)(int dim, int kind, double delta, int max_sweeps, int ndof, const int* dof,
+                       const long long* poff, const int* pgc, const int* pgn, const real* ps, const int* prole,
+                       const real* W, int w_stride, real* X)
+{
+    const long long total = poff[ndof];
+    int sw = 0;
+    while (sw < max_sweeps) {
+        ++sw;
+        double moved = 0;
+        for (int d = 0; d < ndof; ++d) {
+            const long long o = poff[d];
+            const int P = (int)(poff[d + 1] - o);
+            int gn_l[3 * 128];
+            real s_l[3 * 128];
+            for (int kk = 0; kk < dim; ++kk)
+                for (int p = 0; p < P; ++p) {
+                    gn_l[kk * P + p] = pgn[(size_t)kk * total + o + p];
+                    s_l[kk * P + p] = ps[(size_t)kk * total + o + p];
+                }
+            FN(patch_t) pv = {P, pgc + o, gn_l, s_l, prole + o, W + (size_t)w_stride * o, w_stride, NULL};
+            real g[3], H[9], e0, md;
+            FN(patch_eval_impl)(dim, kind, (real)delta, &pv, X, g, H, &e0, &md);
+            /* -H^{-1} g by the adjugate; an exactly singular H is skipped */
+            real adj[9], det;
+            if (dim == 2) {
+                det = H[0] * H[3] - H[1] * H[2];
+                adj[0] = H[3]; adj[1] = -H[1]; adj[2] = -H[2]; adj[3] = H[0];
+            } else {
+                adj[0] = H[4] * H[8] - H[5] * H[7];
+                adj[1] = H[2] * H[7] - H[1] * H[8];
+                adj[2] = H[1] * H[5] - H[2] * H[4];
+                adj[3] = H[5] * H[6] - H[3] * H[8];
+                adj[4] = H[0] * H[8] - H[2] * H[6];
+                adj[5] = H[2] * H[3] - H[0] * H[5];
+                adj[6] = H[3] * H[7] - H[4] * H[6];
+                adj[7] = H[1] * H[6] - H[0] * H[7];
+                adj[8] = H[0] * H[4] - H[1] * H[3];
+                det = H[0] * adj[0] + H[1] * adj[3] + H[2] * adj[6];
+            }
+            if (det == 0 || !isfinite(det)) continue;
+            real step[3];
+            for (int i = 0; i < dim; ++i) {
+                real acc = 0;
+                for (int j = 0; j < dim; ++j) acc += adj[i * dim + j] * g[j];
+                step[i] = -acc / det;
+            }
+            const int n = dof[d];
+            for (real alpha = 1; alpha >= (real)1e-4; alpha *= (real)0.5) {
+                real trial[3], e, m;
+                for (int i = 0; i < dim; ++i) trial[i] = X[(size_t)n * dim + i] + alpha * step[i];
+                FN(trial_impl)(dim, kind, (real)delta, &pv, X, n, trial, &e, &m);
+                if (isfinite(e) && e <= e0 + (real)1e-12 && m > 0) {
+                    double len = 0;
+                    for (int i = 0; i < dim; ++i) {
+                        X[(size_t)n * dim + i] = trial[i];
+                        len += (double)step[i] * step[i];
+                    }
+                    moved += alpha * sqrt(len);
+                    break;
+                }
+            }
+        }
+        if (moved < 1e-12) break;
+    }
+    return sw;
 }
 
 /* Per-DOF patch gradient (minus tau when given) into gout[dof*dim + k]

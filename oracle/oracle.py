@@ -81,6 +81,9 @@ def lib():
                 C.c_longlong, _ip, _ip, rp, rp, C.c_int, C.c_void_p, rp, _dp, _dp,
             ]
             f.restype = C.c_int
+            f = getattr(L, "o_gauss_seidel" + sfx)
+            f.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, _ip, _lp, _ip, _ip, rp, _ip, rp, C.c_int, rp]
+            f.restype = C.c_int
             f = getattr(L, "o_patch_grads" + sfx)
             f.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, _ip, _lp, _ip, _ip, rp, _ip, rp, C.c_int,
                           C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, rp, C.c_void_p, rp]
@@ -469,6 +472,20 @@ def jacobi(dim, kind, delta, omega, n_sweeps, X, patches: Patches, es: dict, *, 
         work, en, md,
     )
     return Xw.reshape(-1, dim), en[:got], md[:got]
+
+
+def gauss_seidel(dim, kind, X, patches: Patches, *, delta=0.0, max_sweeps=10000, prec="f64"):
+    """The reference's sequential node baseline (in-place Gauss-Seidel
+    Newton, tests/smoothing/test_control_ref.py:95-135) to convergence or
+    max_sweeps; returns (X_out, sweeps)."""
+    dt = _dt(prec)
+    Xw = np.ascontiguousarray(X, dt).reshape(-1).copy()
+    W = np.ascontiguousarray(patches.W, dt).reshape(-1)
+    ws = 0 if W.size == dim * dim else dim * dim
+    n = getattr(lib(), f"o_gauss_seidel_{prec}")(
+        dim, KIND[kind], float(delta), int(max_sweeps), len(patches.dof), patches.dof, patches.poff, patches.gc,
+        patches.gn.reshape(-1), np.ascontiguousarray(patches.s, dt).reshape(-1), patches.role, W, ws, Xw)
+    return Xw.reshape(-1, dim), n
 
 
 def _curv_args(patches: Patches, dt):
