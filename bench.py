@@ -155,6 +155,83 @@ def _converge(step, e_star, budget_s):
     return None, time.perf_counter() - t0, float(e[-1])
 
 
+def _cpu_rates(run, units, n_sweeps):
+    """CPU oracle timing in both precisions: per precision, one call of 5
+    sweeps to warm the threads and caches, then the median of 3 timed calls
+    of `n_sweeps` from the same start. run(prec, n) returns the oracle's
+    output; `units` = DOF updates per sweep. Returns (rates, FP64 output,
+    seconds per timed call)."""
+    rates, secs, out64 = {}, {}, None
+    for prec in ("f64", "f32"):
+        run(prec, min(5, n_sweeps))
+        t = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            out = run(prec, n_sweeps)
+            t.append(time.perf_counter() - t0)
+        if prec == "f64":
+            out64 = out
+        secs[prec] = t
+        rates[prec] = units * n_sweeps / float(np.median(t))
+    return rates, out64, secs
+
+
+def _cpu_entry(rates, threads, sample):
+    best = max(rates, key=rates.get)
+    return {"value": rates[best], "unit": "node-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{sample}, OpenMP C oracle {best.upper()} (f64 {rates['f64']:.4g}, f32 {rates['f32']:.4g} "
+                      "node-updates/s; 5 warm-up sweeps, median of 3 timed runs)"}
+
+
+def _error_triple(X, x_star, e_star, es, dim, extent):
+    """FP64 errors of a converged GPU state X against the FP64 plain-method
+    converged state: max-abs position error (against X*, when pinned),
+    relative energy error (E of X recomputed in FP64 on the CPU, against
+    E*) and min-det error (against min det of X*)."""
+    from oracle import oracle as O  # the checker
+
+    e, md = O.energy_mindet(dim, "barrier", 0.0, es, X)
+    out = {"rel_energy_err": abs(e - e_star) / abs(e_star), "energy_fp64": e, "min_det_fp64": md}
+    if x_star is not None:
+        es_, ms_ = O.energy_mindet(dim, "barrier", 0.0, es, x_star)
+        out.update(max_abs_pos_err=float(np.abs(X - x_star).max()),
+                   max_abs_pos_err_rel_extent=float(np.abs(X - x_star).max() / extent),
+                   min_det_err=abs(md - ms_))
+    return out
+
+
+def _config1(threads, n_sweeps=40, seed=1):
+    """The CPU reference's own case: one 129 x 129 block on the unit square,
+    interior perturbed by U(-0.3h, 0.3h), identity target shape metric,
+    40 FP64 sweeps at omega 0.8 on the host cores, beside the same 40 sweeps
+    on the GPU in FP32, with the FP64 error triple of the GPU result."""
+    from egg_amd import grids
+    from oracle import oracle as O  # the checker: CPU baseline and parity reference
+
+    g = grids.single_block_2d(129, seed=seed, amp=0.3)
+    ctx, bsc, s = _build(g)
+    patches = O.materialize_patches(ctx.wire, g.blocks, 2)
+    rates, (Xo, eo, mo), secs = _cpu_rates(
+        lambda prec, n: O.jacobi(2, "barrier", 0.0, 0.8, n, g.X, patches, ctx.energy_stencil, prec=prec,
+                                 threads=threads), g.n_free, n_sweeps)
+    s.run(n_sweeps, omega=0.8)
+    s.set_X(g.X)
+    t0 = time.perf_counter()
+    e, m = s.run(n_sweeps, omega=0.8)
+    t_gpu = time.perf_counter() - t0
+    Xg = s.get_X()
+    eg, mg = O.energy_mindet(2, "barrier", 0.0, ctx.energy_stencil, Xg)
+    return {"workload": f"one 129x129 block on [0,1]^2, interior U(+-0.3h), seed {seed}, {n_sweeps} sweeps, omega 0.8",
+            "free_dofs": g.n_free,
+            "cpu": _cpu_entry(rates, threads, f"all {n_sweeps} sweeps"),
+            "cpu_f64_s_per_run": secs["f64"],
+            "gpu_node_updates_per_s": g.n_free * n_sweeps / t_gpu, "gpu_s": t_gpu,
+            "parity": _parity(Xg, Xo, eg, eo[-1], mg, mo[-1], 1.0, sweeps=n_sweeps, energy_gpu_state=eg,
+                              energy_oracle=float(eo[-1]),
+                              energy_of_gpu_state="FP64 on the CPU (the energy nears 0 on this grid, so the "
+                                                  "relative error is of a small number)")}
+
+
 def _parity(Xg, Xo, eg, eo, mg, mo, extent, **extra):
     return {"max_abs_pos_err": float(np.abs(Xg - Xo).max()),
             "max_abs_pos_err_rel_extent": float(np.abs(Xg - Xo).max() / extent),
@@ -178,23 +255,16 @@ def _batched_sample(s, g, index, Xu, n_sweeps, target=300_000):
     ctx = build_sweep_context(sub.blocks, sub.free, 2)
     patches = O.materialize_patches(ctx.wire, sub.blocks, 2)
     threads = _threads()
-    rates, outs = {}, {}
-    for prec in ("f64", "f32"):
-        t0 = time.perf_counter()
-        outs[prec] = O.jacobi(2, "barrier", 0.0, 0.8, n_sweeps, sub.X, patches, ctx.energy_stencil, prec=prec,
-                              threads=threads)
-        rates[prec] = frees[gi] * n_sweeps / (time.perf_counter() - t0)
-    best = max(rates, key=rates.get)
-    cpu = {"value": rates[best], "unit": "node-updates/s", "cores": threads, "kind": "port",
-           "sample": f"{n_sweeps} sweeps of grid {gi} of the batch ({nb} blocks, {frees[gi]} free DOFs) from the "
-                     f"untangled state, OpenMP C oracle {best.upper()} (f64 {rates['f64']:.4g}, "
-                     f"f32 {rates['f32']:.4g} node-updates/s)"}
+    rates, (Xo, eo, mo), _ = _cpu_rates(
+        lambda prec, n: O.jacobi(2, "barrier", 0.0, 0.8, n, sub.X, patches, ctx.energy_stencil, prec=prec,
+                                 threads=threads), frees[gi], n_sweeps)
+    cpu = _cpu_entry(rates, threads, f"{n_sweeps} sweeps of grid {gi} of the batch ({nb} blocks, {frees[gi]} free "
+                                     "DOFs) from the untangled state")
     X0 = s.get_X()
     s.set_X(Xu)
     s.run(n_sweeps, omega=0.8)
     Xg = s.get_X()[n0:n0 + nn]
     s.set_X(X0)
-    Xo, eo, mo = outs["f64"]
     eg, mg = O.energy_mindet(2, "barrier", 0.0, ctx.energy_stencil, Xg)  # the GPU state, FP64 on the CPU
     extent = float((sub.X.max(0) - sub.X.min(0)).max())
     par = _parity(Xg, Xo, eg, eo[-1], mg, mo[-1], extent, sweeps=n_sweeps, grid=gi, free_dofs=frees[gi],
@@ -260,23 +330,17 @@ def _cubed_sample(s, g, ctx, Xu, every=40):
     # an empty energy stencil: the sweep's own report would sum the whole grid
     no_report = {"num_samples": 0, "gc": np.zeros(0, np.int32), "W_inv": np.eye(3),
                  **{f"gn{k}": np.zeros(0, np.int32) for k in range(3)}, **{f"s{k}": np.zeros(0) for k in range(3)}}
-    rates, outs = {}, {}
-    for prec in ("f64", "f32"):
-        t0 = time.perf_counter()
-        outs[prec] = O.jacobi(3, "barrier", 0.0, 0.8, 1, Xu, patches, no_report, prec=prec, threads=threads)
-        rates[prec] = len(sample) / (time.perf_counter() - t0)
-    best = max(rates, key=rates.get)
-    cpu = {"value": rates[best], "unit": "node-updates/s", "cores": threads, "kind": "port",
-           "sample": f"1 sweep of {len(sample)} DOFs (every {every}th) of the same grid from the untangled state, "
-                     f"OpenMP C oracle {best.upper()} (f64 {rates['f64']:.4g}, f32 {rates['f32']:.4g} "
-                     "node-updates/s)"}
+    rates, (Xo, _, _), _ = _cpu_rates(
+        lambda prec, n: O.jacobi(3, "barrier", 0.0, 0.8, n, Xu, patches, no_report, prec=prec, threads=threads),
+        len(sample), 1)
+    cpu = _cpu_entry(rates, threads, f"1 sweep of {len(sample)} DOFs (every {every}th) of the same grid from the "
+                                     "untangled state")
     X0 = s.get_X()
     s.set_X(Xu)
     eg_gpu, mg_gpu = s.run(1, omega=0.8)
     Xg = s.get_X()
     s.set_X(X0)
     dofs = np.asarray(ctx.wire["groups"][0]["dof_idx"])[sample]
-    Xo = outs["f64"][0]
     ec, mc = O.energy_mindet(3, "barrier", 0.0, ctx.energy_stencil, Xg)
     extent = float((g.X.max(0) - g.X.min(0)).max())
     par = _parity(Xg[dofs], Xo[dofs], float(eg_gpu[-1]), ec, float(mg_gpu[-1]), mc, extent, sweeps=1,
@@ -342,10 +406,18 @@ def bench_ogrid(args):
 
     # time to untangled + converged, from the same tangled start
     ttc = None
+    extent = float((g.X.max(0) - g.X.min(0)).max())
     fix = ROOT / "tests" / "golden" / "estar_ogrid129.json"
+    xfix = ROOT / "tests" / "golden" / "xstar_ogrid129.npy"
     if fix.exists() and not args.no_converge:
         e_star = json.loads(fix.read_text())["E_star"]
-        ttc = {"E_star": e_star, "rtol": CONV_RTOL, "untangle_s": t_untangle}
+        x_star = np.load(xfix, allow_pickle=False) if xfix.exists() else None
+        ttc = {"E_star": e_star, "rtol": CONV_RTOL, "untangle_s": t_untangle,
+               "errors": "each result's FP64 error triple: max-abs position vs the FP64 converged state X*, "
+                         "relative energy (FP64 recomputation on the CPU) vs E*, min det vs min det of X*"}
+
+        def errors(n):
+            return _error_triple(s.get_X(), x_star, e_star, ctx.energy_stencil, 2, extent) if n else None
 
         def plain_step(chunk=1000):
             e, m = s.run(chunk, omega=0.8, report_every=chunk)
@@ -353,7 +425,7 @@ def bench_ogrid(args):
 
         s.set_X(Xu)
         n, t, E = _converge(plain_step, e_star, 60.0)
-        ttc["plain"] = {"sweeps": n, "s": t + t_untangle if n else None, "E_end": E}
+        ttc["plain"] = {"sweeps": n, "s": t + t_untangle if n else None, "E_end": E, **(errors(n) or {})}
         log(f"plain: {n} sweeps, {t:.3f}s to converge")
 
         # FAS: V(4,4) on 3 levels (the batched workload's cycle), on every
@@ -368,7 +440,7 @@ def bench_ogrid(args):
                 return e, m, chunk
 
             n, t, E = _converge(fas_step, e_star, 60.0)
-            ttc[name] = {"cycles": n, "s": t + t_untangle if n else None, "E_end": E}
+            ttc[name] = {"cycles": n, "s": t + t_untangle if n else None, "E_end": E, **(errors(n) or {})}
             log(f"FAS {name}: {n} cycles, {t:.3f}s to converge")
 
         net, info = _train_proposal(make, (11, 12, 13, 14, 15))
@@ -376,7 +448,7 @@ def bench_ogrid(args):
         s.set_proposal(net, mode="predict_correct", scale=0.5)  # (selects a variant with a proposal kernel)
         n, t, E = _converge(plain_step, e_star, 60.0)
         ttc["mlp_proposal"] = {"sweeps": n, "s": t + t_untangle if n else None, "E_end": E,
-                               "mode": "predict_correct", "scale": 0.5, **info}
+                               "mode": "predict_correct", "scale": 0.5, **info, **(errors(n) or {})}
         log(f"MLP proposal: {n} sweeps, {t:.3f}s to converge ({info})")
         # the proposal in the fine-level smoothing of the fastest FAS cycle
         s.set_X(Xu)
@@ -384,38 +456,32 @@ def bench_ogrid(args):
                             e_star, 60.0)
         s.set_proposal(None)
         ttc["fas_v88_4_levels_coarse16_mlp_proposal"] = {"cycles": n, "s": t + t_untangle if n else None,
-                                                         "E_end": E, "mode": "predict_correct", "scale": 0.5}
+                                                         "E_end": E, "mode": "predict_correct", "scale": 0.5,
+                                                         **(errors(n) or {})}
         log(f"FAS V(8,8) + MLP proposal: {n} cycles, {t:.3f}s to converge")
         s.core.set_tile_variant(int(tune_best))
 
-    cpu = parity = None
+    cpu = parity = config1 = None
     if not args.no_cpu:
         from oracle import oracle as O  # the checker: CPU baseline and parity reference
 
         threads = _threads()
         n_cpu = args.cpu_sweeps
         patches = O.materialize_patches(ctx.wire, g.blocks, 2)
-        rates, outs = {}, {}
-        for prec in ("f64", "f32"):
-            t0 = time.perf_counter()
-            outs[prec] = O.jacobi(2, "barrier", 0.0, 0.8, n_cpu, Xu, patches, ctx.energy_stencil, prec=prec,
-                                  threads=threads)
-            rates[prec] = n_free * n_cpu / (time.perf_counter() - t0)
-        best = max(rates, key=rates.get)
-        cpu = {"value": rates[best], "unit": "node-updates/s", "cores": threads, "kind": "port",
-               "sample": f"{n_cpu} sweeps of the same O-grid from the untangled state, OpenMP C oracle "
-                         f"{best.upper()} (f64 {rates['f64']:.4g}, f32 {rates['f32']:.4g} node-updates/s)"}
+        rates, (Xo, eo, mo), _ = _cpu_rates(
+            lambda prec, n: O.jacobi(2, "barrier", 0.0, 0.8, n, Xu, patches, ctx.energy_stencil, prec=prec,
+                                     threads=threads), n_free, n_cpu)
+        cpu = _cpu_entry(rates, threads, f"{n_cpu} sweeps of the same O-grid from the untangled state")
         log(f"cpu {cpu}")
         s.set_X(Xu)
         ep, mp = s.run(n_cpu, omega=0.8)
         Xg = s.get_X()
-        Xo, eo, mo = outs["f64"]
-        extent = float((g.X.max(0) - g.X.min(0)).max())
-        parity = {"sweeps": n_cpu, "max_abs_pos_err": float(np.abs(Xg - Xo).max()),
-                  "max_abs_pos_err_rel_extent": float(np.abs(Xg - Xo).max() / extent),
-                  "rel_energy_err": float(abs(ep[-1] - eo[-1]) / abs(eo[-1])),
-                  "min_det_err": float(abs(mp[-1] - mo[-1])), "oracle": "FP64 CPU oracle (oracle/egg_oracle.c)"}
+        eg, mg = O.energy_mindet(2, "barrier", 0.0, ctx.energy_stencil, Xg)
+        parity = _parity(Xg, Xo, eg, eo[-1], mg, mo[-1], extent, sweeps=n_cpu,
+                         energy_of_gpu_state="FP64 on the CPU")
         log(f"parity {parity}")
+        config1 = _config1(threads)
+        log(f"config 1 {config1}")
 
     return {
         "metric": METRIC, "value": value, "unit": "node-updates/s", "n_gpus": 1, "steps": args.steps,
@@ -431,7 +497,7 @@ def bench_ogrid(args):
         "time_to_untangled_s": t_untangle, "untangle_outer": outer,
         "time_to_untangled_converged": ttc,
         "final_energy": float(e[-1]), "final_min_det": float(m[-1]),
-        "parity": parity, "roofline": roof, "cpu_baseline": cpu,
+        "parity": parity, "roofline": roof, "cpu_baseline": cpu, "config1_cpu_reference_case": config1,
     }
 
 
@@ -632,19 +698,12 @@ def bench_ogrid_bl(args):
         threads = _threads()
         n_cpu = 50
         patches = O.materialize_patches(ctx.wire, g.blocks, 2)
-        rates, outs = {}, {}
-        for prec in ("f64", "f32"):
-            t0 = time.perf_counter()
-            outs[prec] = O.jacobi(2, "barrier", 0.0, 0.8, n_cpu, Xu, patches, ctx.energy_stencil, prec=prec,
-                                  threads=threads)
-            rates[prec] = n_free * n_cpu / (time.perf_counter() - t0)
-        bestp = max(rates, key=rates.get)
-        cpu = {"value": rates[bestp], "unit": "node-updates/s", "cores": threads, "kind": "port",
-               "sample": f"{n_cpu} sweeps of the same grid and target from the untangled state, OpenMP C oracle "
-                         f"{bestp.upper()} (f64 {rates['f64']:.4g}, f32 {rates['f32']:.4g} node-updates/s)"}
+        rates, (Xo, eo, mo), _ = _cpu_rates(
+            lambda prec, n: O.jacobi(2, "barrier", 0.0, 0.8, n, Xu, patches, ctx.energy_stencil, prec=prec,
+                                     threads=threads), n_free, n_cpu)
+        cpu = _cpu_entry(rates, threads, f"{n_cpu} sweeps of the same grid and target from the untangled state")
         s.set_X(Xu)
         ep, mp = s.run(n_cpu, omega=0.8)
-        Xo, eo, mo = outs["f64"]
         parity = _parity(s.get_X(), Xo, ep[-1], eo[-1], mp[-1], mo[-1], float((g.X.max(0) - g.X.min(0)).max()),
                          sweeps=n_cpu)
         log(f"cpu {cpu}\nparity {parity}")
@@ -735,11 +794,15 @@ def bench_ogrid_terms(args):
         threads = _threads()
         n_cpu = 50
         patches = O.materialize_patches(ctx.wire, g.blocks, 2)
-        t0 = time.perf_counter()
-        Xo, eo, mo = O.jacobi(2, "barrier", 0.0, 0.8, n_cpu, Xu, patches, ctx.energy_stencil, threads=threads)
-        rate = n_free * n_cpu / (time.perf_counter() - t0)
-        cpu = {"value": rate, "unit": "node-updates/s", "cores": threads, "kind": "port",
-               "sample": f"{n_cpu} sweeps of the same grid and terms from the untangled state, OpenMP C oracle F64"}
+        O.jacobi(2, "barrier", 0.0, 0.8, 5, Xu, patches, ctx.energy_stencil, threads=threads)  # warm-up
+        t = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            Xo, eo, mo = O.jacobi(2, "barrier", 0.0, 0.8, n_cpu, Xu, patches, ctx.energy_stencil, threads=threads)
+            t.append(time.perf_counter() - t0)
+        cpu = {"value": n_free * n_cpu / float(np.median(t)), "unit": "node-updates/s", "cores": threads,
+               "kind": "port", "sample": f"{n_cpu} sweeps of the same grid and terms from the untangled state, "
+                                         "OpenMP C oracle F64 (5 warm-up sweeps, median of 3 timed runs)"}
         s.set_X(Xu)
         ep, mp = s.run(n_cpu, omega=0.8)
         parity = _parity(s.get_X(), Xo, ep[-1], eo[-1], mp[-1], mo[-1], float((g.X.max(0) - g.X.min(0)).max()),
