@@ -602,6 +602,13 @@ class Session
         return py::make_tuple(ms, n, kms);
     }
 
+    int64_t fas_cycle_kernels()
+    {
+        int64_t n = 0;
+        check(eggmi_fas_cycle_kernels(s_, &n));
+        return n;
+    }
+
     void set_kernel_timing(bool on) { check(eggmi_set_kernel_timing(s_, on ? 1 : 0)); }
     void set_counters(bool on) { check(eggmi_set_counters(s_, on ? 1 : 0)); }
     void set_graph_chunk(int chunk) { check(eggmi_set_graph_chunk(s_, chunk)); }
@@ -719,6 +726,7 @@ PYBIND11_MODULE(cpp_core, m)
       .def("halo_exchange", &Session::halo_exchange)
       .def("set_tile_variant", &Session::set_tile_variant, py::arg("variant"))
       .def("last_timing", &Session::last_timing)
+      .def("fas_cycle_kernels", &Session::fas_cycle_kernels)
       .def("set_kernel_timing", &Session::set_kernel_timing, py::arg("on"))
       .def("set_counters", &Session::set_counters, py::arg("on"))
       .def("set_graph_chunk", &Session::set_graph_chunk, py::arg("chunk"),

@@ -359,6 +359,15 @@ eggmi_status eggmi_last_timing(eggmi_session* s, double* ms, int64_t* sweep_laun
     });
 }
 
+eggmi_status eggmi_fas_cycle_kernels(eggmi_session* s, int64_t* kernels)
+{
+    return guarded([&] {
+        need(s, "session");
+        need(kernels, "kernels");
+        *kernels = s->s.last_fas_graph_kernels;
+    });
+}
+
 eggmi_status eggmi_set_kernel_timing(eggmi_session* s, int on)
 {
     return guarded([&] {

@@ -527,10 +527,15 @@ void Session::grad(Level& L, const float* x, float* out, const float* tau, int k
 void Session::restrict_to(Level& parent, const float* x_parent, Level& child, int kind)
 {
     const TransferArgs t = transfer(parent, child, false);
-    launch_restrict(dim, t, x_parent, child.x[0].p, parent.grad.p, child.tau.p, child.d_free.p, stream);
+    // one launch writes both Jacobi buffers and rx, replica slots included
+    RestrictPush push;
+    push.x1 = child.x[1].p;
+    push.rx = child.rx.p;
+    push.off = child.d_rp_off.p;
+    push.dst = child.d_rp_dst.p;
+    push.is_dst = child.d_is_dst.p;
+    launch_restrict(dim, t, x_parent, child.x[0].p, parent.grad.p, child.tau.p, child.d_free.p, stream, push);
     child.cur = 0;
-    halo(child, child.x[0].p);
-    launch_copy2(child.rx.p, child.x[1].p, child.x[0].p, (long long)child.plane * dim, stream);
     grad(child, child.rx.p, child.tau.p, nullptr, kind, true);  // tau = grad_c(Rx) - R grad at the DOFs
 }
 
@@ -589,6 +594,7 @@ std::pair<std::vector<double>, std::vector<double>> Session::run_fas(int n_cycle
     const int cap = (int)(d_fas_reports.n / 2);
     if (!d_ls.n) d_ls.alloc(4);
     last_launches = 0;
+    last_fas_graph_kernels = 0;
     counters_begin();
     const bool graphs_on = graph_chunk > 0 && !counting;
     if (graphs_on) {
@@ -616,6 +622,7 @@ std::pair<std::vector<double>, std::vector<double>> Session::run_fas(int n_cycle
             EGGMI_HIP(hipGraphLaunch(fas_graph(parity, kind, o, n_coarse), stream));
             const FasGraph& g = fas_graphs[parity];
             last_launches += g.launches;
+            last_fas_graph_kernels = g.kernels;
             if ((o.nu_pre + o.nu_post) % 2) F.cur = 1 - F.cur;
             continue;
         }
@@ -820,10 +827,10 @@ void Session::fas_head(int kind, const FasParams& o, size_t n_coarse)
     grad(F, x, F.grad.p, nullptr, kind, false, true);
     restrict_to(F, x, C0, kind);
     coarse_vcycle(0, n_coarse, kind, o);
-    // fine correction P(x_c - rx_c), zero where not free; its replicas
-    launch_prolong(dim, transfer(F, C0, true), C0.x[C0.cur].p, C0.rx.p, F.corr.p, F.d_free.p, false, nullptr,
-                   nullptr, stream);
-    halo(F, F.corr.p);
+    // fine correction P(x_c - rx_c), zero where not free, pushed into the
+    // replicas of the free slots (the others' replicas hold zero throughout)
+    launch_prolong(dim, transfer(F, C0, true), C0.x[C0.cur].p, C0.rx.p, F.corr.p, F.d_free.p, false,
+                   F.n_push ? F.d_cp_off.p : nullptr, F.n_push ? F.d_cp_dst.p : nullptr, stream, F.d_is_dst.p);
 }
 
 void Session::fas_device_ladder(int kind)
@@ -833,15 +840,16 @@ void Session::fas_device_ladder(int kind)
     float* x = F.x[F.cur].p;
     launch_linesearch_ladder(dim, kind, fine_cells(x), F.corr.p, d_ladder_parts.p, (double)1e-6f, d_ladder_out.p,
                              d_alpha.p, stream, F.es_dir.n ? &F.es_dir : nullptr);
-    launch_masked_axpy_dev(x, F.corr.p, d_alpha.p, F.d_free.p, F.lay.n_slots(), F.plane, dim, stream);
-    halo(F, x);  // a no-op copy when nothing moved (the replicas already equal their owners)
+    // the step and its replicas (the copies of fixed roots never change)
+    launch_masked_axpy_dev(x, F.corr.p, d_alpha.p, F.d_free.p, F.lay.n_slots(), F.plane, dim, stream,
+                           F.n_push ? F.d_cp_off.p : nullptr, F.n_push ? F.d_cp_dst.p : nullptr);
 }
 
-void Session::fas_tail(int kind, const FasParams& o, double* out_e, double* out_m)
+void Session::fas_tail(int kind, const FasParams& o, double* out_e, double* out_m, double* rep, int cap, int* count)
 {
     Level& F = fine;
     smooth(F, o.nu_post, kind, o.omega, nullptr);
-    reduce(F, F.x[F.cur].p, kind, 0.0, out_e, out_m);
+    reduce(F, F.x[F.cur].p, kind, 0.0, out_e, out_m, rep, cap, count);
 }
 
 hipGraphExec_t Session::fas_graph(int parity, int kind, const FasParams& o, size_t n_coarse)
@@ -868,8 +876,8 @@ hipGraphExec_t Session::fas_graph(int parity, int kind, const FasParams& o, size
     try {
         fas_head(kind, o, n_coarse);
         fas_device_ladder(kind);
-        fas_tail(kind, o, d_cyc_rep.p, d_cyc_rep.p + 1);
-        launch_append_report(d_cyc_rep.p, d_fas_reports.p, (int)(d_fas_reports.n / 2), d_rep_count.p, stream);
+        // the cycle's report is appended by the reduction's last launch
+        fas_tail(kind, o, d_cyc_rep.p, d_cyc_rep.p + 1, d_fas_reports.p, (int)(d_fas_reports.n / 2), d_rep_count.p);
     } catch (...) {
         (void)hipStreamEndCapture(stream, &graph);
         if (graph) (void)hipGraphDestroy(graph);
@@ -879,9 +887,19 @@ hipGraphExec_t Session::fas_graph(int parity, int kind, const FasParams& o, size
         throw;
     }
     EGGMI_HIP(hipStreamEndCapture(stream, &graph));
+    size_t n_nodes = 0;
+    EGGMI_HIP(hipGraphGetNodes(graph, nullptr, &n_nodes));
+    std::vector<hipGraphNode_t> nodes(n_nodes);
+    if (n_nodes) EGGMI_HIP(hipGraphGetNodes(graph, nodes.data(), &n_nodes));
+    long long n_kernels = 0;
+    for (auto nd : nodes) {
+        hipGraphNodeType t {};
+        if (hipGraphNodeGetType(nd, &t) == hipSuccess && t == hipGraphNodeTypeKernel) ++n_kernels;
+    }
     const hipError_t ie = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     g.launches = last_launches - launches0;
+    g.kernels = n_kernels;
     F.cur = cur0;
     for (size_t i = 0; i < coarse.size(); ++i) coarse[i]->cur = ccur[i];
     last_launches = launches0;

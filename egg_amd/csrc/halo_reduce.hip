@@ -234,7 +234,7 @@ __global__ __launch_bounds__(kReduceThreads) void reduce_samples(const SampleRed
 }
 
 __global__ __launch_bounds__(kReduceThreads) void reduce_finish(const double* partials, int nparts, double* out_e,
-                                                                double* out_m)
+                                                                double* out_m, double* rep, int cap, int* count)
 {
     double e = 0.0, m = __builtin_huge_val();
     for (int i = threadIdx.x; i < nparts; i += kReduceThreads) {
@@ -247,6 +247,14 @@ __global__ __launch_bounds__(kReduceThreads) void reduce_finish(const double* pa
     if (threadIdx.x == 0) {
         *out_e = tmp[0];
         *out_m = tmp[1];
+        if (rep) {
+            const int i = *count;
+            if (i < cap) {
+                rep[i] = tmp[0];
+                rep[cap + i] = tmp[1];
+            }
+            *count = i + 1;
+        }
     }
 }
 
@@ -522,9 +530,10 @@ void launch_reduce_samples(int dim, int kind, const SampleReduceArgs& a, double*
 #undef EGGMI_RS
 }
 
-void launch_reduce_finish(const double* partials, int nparts, double* out_e, double* out_m, hipStream_t s)
+void launch_reduce_finish(const double* partials, int nparts, double* out_e, double* out_m, hipStream_t s,
+                          double* rep, int cap, int* count)
 {
-    reduce_finish<<<1, kReduceThreads, 0, s>>>(partials, nparts, out_e, out_m);
+    reduce_finish<<<1, kReduceThreads, 0, s>>>(partials, nparts, out_e, out_m, rep, cap, count);
 }
 
 // Whole-sample directional energies at NA step lengths (reference

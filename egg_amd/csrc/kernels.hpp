@@ -154,7 +154,11 @@ void launch_grad(int dim, int kind, int tile_variant, const SweepArgs& a, hipStr
 void launch_copy(float* x, long long plane, int dim, const int* src, const int* dst, long long n, hipStream_t s);
 void launch_reduce_cells(int dim, int kind, const CellReduceArgs& a, double* partials, hipStream_t s);
 void launch_reduce_samples(int dim, int kind, const SampleReduceArgs& a, double* partials, hipStream_t s);
-void launch_reduce_finish(const double* partials, int nparts, double* out_e, double* out_m, hipStream_t s);
+// Folds the partials into *out_e, *out_m; with rep set, also appends them
+// as report *count (rep[*count] and rep[cap + *count], when *count < cap)
+// and increments *count.
+void launch_reduce_finish(const double* partials, int nparts, double* out_e, double* out_m, hipStream_t s,
+                          double* rep = nullptr, int cap = 0, int* count = nullptr);
 // The whole-sample directional energies in FP64 at x + alpha_t corr for
 // mode 0: alpha 0 (the energy reports; corr may be null), 1: a0, a1 (the
 // line-search pair), 2: the FAS ladder's slots; written as the partials of

@@ -52,10 +52,14 @@ template <int D> __device__ __forceinline__ long long padded_slot(const BlockDes
     return s;
 }
 
-// Threads enumerate the coarse interior nodes of every block.
+// Threads enumerate the coarse interior nodes of every block. With a push
+// table (p.off), the injected state goes to cx and the two extra coarse
+// buffers p.x1 and p.rx, at the node and at every replica slot it is the
+// root of; a replica slot's own thread writes nothing there.
 template <int D> __global__ __launch_bounds__(256) void restrict_kernel(const TransferArgs a, const float* fx,
                                                                        float* cx, const float* fgrad,
-                                                                       float* ctau, const uint8_t* cfree)
+                                                                       float* ctau, const uint8_t* cfree,
+                                                                       const RestrictPush p)
 {
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     if (i >= a.n) return;
@@ -72,7 +76,22 @@ template <int D> __global__ __launch_bounds__(256) void restrict_kernel(const Tr
     }
     const long long cs = padded_slot<D>(cb, Pc);
     const long long fs = padded_slot<D>(fb, Pf);
-    if (cx) {
+    if (cx && p.off) {
+        if (!p.is_dst[cs]) {
+            float v[D];
+#pragma unroll
+            for (int k = 0; k < D; ++k) v[k] = fx[k * a.fplane + fs];
+            for (int q = p.off[cs] - 1; q < p.off[cs + 1]; ++q) {
+                const long long d = q < p.off[cs] ? cs : (long long)p.dst[q];
+#pragma unroll
+                for (int k = 0; k < D; ++k) {
+                    cx[k * a.cplane + d] = v[k];
+                    p.x1[k * a.cplane + d] = v[k];
+                    p.rx[k * a.cplane + d] = v[k];
+                }
+            }
+        }
+    } else if (cx) {
 #pragma unroll
         for (int k = 0; k < D; ++k) cx[k * a.cplane + cs] = fx[k * a.fplane + fs];
     }
@@ -115,7 +134,7 @@ template <int D> __global__ __launch_bounds__(256) void restrict_kernel(const Tr
 template <int D, bool ADD>
 __global__ __launch_bounds__(256) void prolong_kernel(const TransferArgs a, const float* cx, const float* crx,
                                                       float* fout, const uint8_t* ffree, const int* cp_off,
-                                                      const int* cp_dst)
+                                                      const int* cp_dst, const uint8_t* is_dst)
 {
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     if (i >= a.n) return;
@@ -166,8 +185,17 @@ __global__ __launch_bounds__(256) void prolong_kernel(const TransferArgs a, cons
                 for (int k = 0; k < D; ++k) fout[k * a.fplane + d] = v[k];
             }
     } else {
+        // with a push table, a replica slot is its root's to write (a relaxed
+        // root pushes its value; the others' replicas keep their zero)
+        if (is_dst && is_dst[fs]) return;
 #pragma unroll
         for (int k = 0; k < D; ++k) fout[k * a.fplane + fs] = fr ? acc[k] : 0.0f;
+        if (fr && cp_off)
+            for (int q = cp_off[fs]; q < cp_off[fs + 1]; ++q) {
+                const long long d = cp_dst[q];
+#pragma unroll
+                for (int k = 0; k < D; ++k) fout[k * a.fplane + d] = acc[k];
+            }
     }
 }
 
@@ -179,14 +207,21 @@ __global__ __launch_bounds__(256) void masked_axpy_kernel(float* x, const float*
     for (int k = 0; k < dim; ++k) x[k * plane + i] = x[k * plane + i] + alpha * c[k * plane + i];
 }
 
+// x += (*alpha) c at free slots, the sums pushed into their replica slots
+// (push table cp_off / cp_dst, null = none)
 __global__ __launch_bounds__(256) void masked_axpy_dev_kernel(float* x, const float* c, const float* alpha,
                                                               const uint8_t* free, long long n, long long plane,
-                                                              int dim)
+                                                              int dim, const int* cp_off, const int* cp_dst)
 {
     const float al = *alpha;
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     if (al == 0.0f || i >= n || !free[i]) return;
-    for (int k = 0; k < dim; ++k) x[k * plane + i] = x[k * plane + i] + al * c[k * plane + i];
+    for (int k = 0; k < dim; ++k) {
+        const float v = x[k * plane + i] + al * c[k * plane + i];
+        x[k * plane + i] = v;
+        if (cp_off)
+            for (int q = cp_off[i]; q < cp_off[i + 1]; ++q) x[k * plane + cp_dst[q]] = v;
+    }
 }
 
 __global__ __launch_bounds__(256) void copy2_kernel(float* __restrict__ a, float* __restrict__ b,
@@ -199,45 +234,34 @@ __global__ __launch_bounds__(256) void copy2_kernel(float* __restrict__ a, float
     b[i] = v;
 }
 
-// Append one report (energy, min det) at the device counter (captured FAS
-// cycles cannot take the cycle index as an argument).
-__global__ void append_report_kernel(const double* src, double* rep, int cap, int* count)
-{
-    if (threadIdx.x != 0) return;
-    const int i = *count;
-    if (i < cap) {
-        rep[i] = src[0];
-        rep[cap + i] = src[1];
-    }
-    *count = i + 1;
-}
-
 static unsigned nblk(long long n) { return (unsigned)((n + 255) / 256); }
 
 void launch_restrict(int dim, const TransferArgs& a, const float* fx, float* cx, const float* fgrad, float* ctau,
-                     const uint8_t* cfree, hipStream_t s)
+                     const uint8_t* cfree, hipStream_t s, const RestrictPush& push)
 {
     if (a.n <= 0) return;
     if (dim == 2)
-        restrict_kernel<2><<<nblk(a.n), 256, 0, s>>>(a, fx, cx, fgrad, ctau, cfree);
+        restrict_kernel<2><<<nblk(a.n), 256, 0, s>>>(a, fx, cx, fgrad, ctau, cfree, push);
     else
-        restrict_kernel<3><<<nblk(a.n), 256, 0, s>>>(a, fx, cx, fgrad, ctau, cfree);
+        restrict_kernel<3><<<nblk(a.n), 256, 0, s>>>(a, fx, cx, fgrad, ctau, cfree, push);
 }
 
 void launch_prolong(int dim, const TransferArgs& a, const float* cx, const float* crx, float* fout,
-                    const uint8_t* ffree, bool add, const int* cp_off, const int* cp_dst, hipStream_t s)
+                    const uint8_t* ffree, bool add, const int* cp_off, const int* cp_dst, hipStream_t s,
+                    const uint8_t* is_dst)
 {
     if (a.n <= 0) return;
+    if (!add && !is_dst) cp_off = cp_dst = nullptr;  // a pushed correction needs the replica mask
     if (dim == 2) {
         if (add)
-            prolong_kernel<2, true><<<nblk(a.n), 256, 0, s>>>(a, cx, crx, fout, ffree, cp_off, cp_dst);
+            prolong_kernel<2, true><<<nblk(a.n), 256, 0, s>>>(a, cx, crx, fout, ffree, cp_off, cp_dst, nullptr);
         else
-            prolong_kernel<2, false><<<nblk(a.n), 256, 0, s>>>(a, cx, crx, fout, ffree, nullptr, nullptr);
+            prolong_kernel<2, false><<<nblk(a.n), 256, 0, s>>>(a, cx, crx, fout, ffree, cp_off, cp_dst, is_dst);
     } else {
         if (add)
-            prolong_kernel<3, true><<<nblk(a.n), 256, 0, s>>>(a, cx, crx, fout, ffree, cp_off, cp_dst);
+            prolong_kernel<3, true><<<nblk(a.n), 256, 0, s>>>(a, cx, crx, fout, ffree, cp_off, cp_dst, nullptr);
         else
-            prolong_kernel<3, false><<<nblk(a.n), 256, 0, s>>>(a, cx, crx, fout, ffree, nullptr, nullptr);
+            prolong_kernel<3, false><<<nblk(a.n), 256, 0, s>>>(a, cx, crx, fout, ffree, cp_off, cp_dst, is_dst);
     }
 }
 
@@ -246,15 +270,10 @@ void launch_copy2(float* a, float* b, const float* src, long long n, hipStream_t
     if (n > 0) copy2_kernel<<<nblk(n), 256, 0, s>>>(a, b, src, n);
 }
 
-void launch_append_report(const double* src, double* rep, int cap, int* count, hipStream_t s)
-{
-    append_report_kernel<<<1, 64, 0, s>>>(src, rep, cap, count);
-}
-
 void launch_masked_axpy_dev(float* x, const float* c, const float* alpha, const uint8_t* free, long long n,
-                            long long plane, int dim, hipStream_t s)
+                            long long plane, int dim, hipStream_t s, const int* cp_off, const int* cp_dst)
 {
-    if (n > 0) masked_axpy_dev_kernel<<<nblk(n), 256, 0, s>>>(x, c, alpha, free, n, plane, dim);
+    if (n > 0) masked_axpy_dev_kernel<<<nblk(n), 256, 0, s>>>(x, c, alpha, free, n, plane, dim, cp_off, cp_dst);
 }
 
 void launch_masked_axpy(float* x, const float* c, float alpha, const uint8_t* free, long long n, long long plane,

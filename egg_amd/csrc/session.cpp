@@ -113,21 +113,28 @@ void Level::build_push()
     for (size_t q = copy_src.size() - n_share; q < copy_src.size(); ++q) share_from[copy_dst[q]] = copy_src[q];
     std::vector<uint8_t> is_dof(mask.begin(), mask.end());
     for (int d : st_dof) is_dof[d] = 1;
-    std::vector<std::vector<int>> lists;
-    std::vector<int> count(ns + 1, 0);
-    std::vector<std::pair<int, int>> pushes;
+    std::vector<std::pair<int, int>> all;  // (root, destination) of every copy
+    is_dst.assign(ns, 0);
     for (size_t q = 0; q < copy_src.size(); ++q) {
         int r = copy_src[q];
         for (int guard = 0; share_from[r] >= 0 && guard < 8; ++guard) r = share_from[r];
-        if (is_dof[r]) pushes.emplace_back(r, copy_dst[q]);
+        all.emplace_back(r, copy_dst[q]);
+        is_dst[copy_dst[q]] = 1;
     }
-    for (auto& pr : pushes) ++count[pr.first + 1];
-    for (long long i = 0; i < ns; ++i) count[i + 1] += count[i];
-    cp_off.assign(count.begin(), count.end());
-    cp_dst.assign(pushes.size(), 0);
-    std::vector<int> fill(count.begin(), count.end() - 1);
-    for (auto& pr : pushes) cp_dst[fill[pr.first]++] = pr.second;
+    auto csr = [ns](const std::vector<std::pair<int, int>>& pairs, std::vector<int>& off, std::vector<int>& dst) {
+        off.assign(ns + 1, 0);
+        for (auto& pr : pairs) ++off[pr.first + 1];
+        for (long long i = 0; i < ns; ++i) off[i + 1] += off[i];
+        dst.assign(pairs.size(), 0);
+        std::vector<int> fill(off.begin(), off.end() - 1);
+        for (auto& pr : pairs) dst[fill[pr.first]++] = pr.second;
+    };
+    std::vector<std::pair<int, int>> pushes;
+    for (auto& pr : all)
+        if (is_dof[pr.first]) pushes.push_back(pr);
+    csr(pushes, cp_off, cp_dst);
     n_push = (long long)pushes.size();
+    csr(all, rp_off, rp_dst);
 }
 
 BlockDesc Level::block_desc(size_t b) const
@@ -151,6 +158,9 @@ void Level::upload_tables()
     d_copy_dst.upload(copy_dst);
     d_cp_off.upload(cp_off);
     d_cp_dst.upload(cp_dst);
+    d_rp_off.upload(rp_off);
+    d_rp_dst.upload(rp_dst);
+    d_is_dst.upload(is_dst);
     d_mask.upload(mask);
     d_st_dof.upload(st_dof);
     d_con_tag.upload(con_tag);
@@ -950,7 +960,8 @@ void Session::halo(Level& L, float* x)
     launch_copy(x, L.plane, dim, L.d_copy_src.p, L.d_copy_dst.p, (long long)L.copy_src.size(), stream);
 }
 
-void Session::reduce(Level& L, const float* x, int kind, double delta, double* out_e, double* out_m)
+void Session::reduce(Level& L, const float* x, int kind, double delta, double* out_e, double* out_m, double* rep,
+                     int cap, int* count)
 {
     if (L.structured_energy) {
         CellReduceArgs a;
@@ -979,7 +990,7 @@ void Session::reduce(Level& L, const float* x, int kind, double delta, double* o
     // whole-sample directional terms join the reported energy (reference src/sweep.hpp:1921)
     const int nd =
         launch_dir_reduce(dim, 0, L.es_dir, x, L.plane, nullptr, 0.0f, 0.0f, partials.p, kReduceBlocks, stream);
-    launch_reduce_finish(partials.p, kReduceBlocks + nd, out_e, out_m, stream);
+    launch_reduce_finish(partials.p, kReduceBlocks + nd, out_e, out_m, stream, rep, cap, count);
 }
 
 void Session::need_device() const

@@ -121,6 +121,13 @@ struct Level {
     std::vector<int> cp_off, cp_dst;
     DArr<int> d_cp_off, d_cp_dst;
     long long n_push = 0;
+    // every copy in push form (any root source, relaxed or not) and a mask
+    // of the copy destinations: a transfer kernel writes a node's value into
+    // its replicas itself, with no halo launch after it
+    std::vector<int> rp_off, rp_dst;
+    std::vector<uint8_t> is_dst;
+    DArr<int> d_rp_off, d_rp_dst;
+    DArr<uint8_t> d_is_dst;
     // synthesized interior DOFs
     std::vector<uint8_t> mask;  // [n_slots]
     DArr<uint8_t> d_mask;
@@ -222,6 +229,7 @@ struct Session {
     DArr<double> d_reports;
     double last_ms = 0.0;
     long long last_launches = 0;
+    long long last_fas_graph_kernels = 0;  // kernel nodes of the last replayed FAS cycle graph
     bool kernel_timing = false;     // bracket every sweep launch with events
     // hipGraph replay of sweep chunks (graph_chunk sweeps, even; 0 = off):
     // one captured graph per starting buffer, re-captured when a parameter
@@ -271,7 +279,10 @@ struct Session {
     void create(int dim, const eggmi_group* groups, int n_groups, const eggmi_stencil* es,
                 const eggmi_structured* st, const double* X, long long num_nodes, int device);
     void halo(Level& L, float* x);
-    void reduce(Level& L, const float* x, int kind, double delta, double* out_e, double* out_m);
+    // energy and min det of x into out_e / out_m (device); with rep set,
+    // also appended as report *count of the capacity-cap report buffer rep
+    void reduce(Level& L, const float* x, int kind, double delta, double* out_e, double* out_m,
+                double* rep = nullptr, int cap = 0, int* count = nullptr);
     std::pair<std::vector<double>, std::vector<double>> run_jacobi(int n_sweeps, int kind, double delta,
                                                                    double omega, int report_every);
     std::pair<std::vector<double>, std::vector<double>> run_fas(int n_cycles, int nu_pre, int nu_post,
@@ -313,7 +324,8 @@ struct Session {
     // smoothing and the report reduction
     void fas_head(int kind, const FasParams& o, size_t n_coarse);
     void fas_device_ladder(int kind);
-    void fas_tail(int kind, const FasParams& o, double* out_e, double* out_m);
+    void fas_tail(int kind, const FasParams& o, double* out_e, double* out_m, double* rep = nullptr, int cap = 0,
+                  int* count = nullptr);
     CellReduceArgs fine_cells(const float* x) const;
     // a whole cycle captured as a hipGraph, per parity of the fine Jacobi
     // buffer it starts from (the ladder decided on the device, the report
@@ -326,6 +338,7 @@ struct Session {
         const void* mlp = nullptr;
         const double* rep = nullptr;
         long long launches = 0;
+        long long kernels = 0;  // kernel nodes
     };
     FasGraph fas_graphs[2];
     hipGraphExec_t fas_graph(int parity, int kind, const FasParams& o, size_t n_coarse);

@@ -267,6 +267,11 @@ eggmi_status eggmi_autotune_levels(eggmi_session* s, int n_sweeps, double omega,
  * when kernel timing is on, the summed duration of those launches. */
 eggmi_status eggmi_last_timing(eggmi_session* s, double* ms, int64_t* sweep_launches, double* sweep_kernel_ms);
 
+/* Kernel nodes of the captured FAS cycle graph the last eggmi_run_fas call
+ * replayed (0 when it ran uncaptured). A diagnostic with no counterpart in
+ * the reference: the launch count of one V-cycle. */
+eggmi_status eggmi_fas_cycle_kernels(eggmi_session* s, int64_t* kernels);
+
 /* Bracket every sweep launch with HIP events (measurement only). */
 eggmi_status eggmi_set_kernel_timing(eggmi_session* s, int on);
 
