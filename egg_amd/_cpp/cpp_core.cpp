@@ -641,6 +641,7 @@ class Session
     {
         int nv = 0;
         check(eggmi_set_tile_variant(s_, -1, &nv));
+        nv *= 2;  // per-sweep launches, then one-workgroup smoothing
         constexpr int kCap = 64;  // more coarse levels than any hierarchy has
         std::vector<double> ms((size_t)kCap * nv);
         std::vector<int> best(kCap);
@@ -740,11 +741,13 @@ PYBIND11_MODULE(cpp_core, m)
       .def("autotune", &Session::autotune, py::arg("n_sweeps") = 20, py::arg("omega") = 0.8,
            "Time every tile variant on the current state (restored), keep the fastest; returns (best, ms).")
       .def("set_level_variants", &Session::set_level_variants, py::arg("variants"),
-           "Tile variant of each FAS coarse level (variants[i] for level i + 1; -1 = the session's).")
+           "Tile variant of each FAS coarse level (variants[i] for level i + 1; -1 = the session's); V + v "
+           "(V = the tile variant count) smooths the level in one single-workgroup launch on variant v's tiles.")
       .def("autotune_levels", &Session::autotune_levels, py::arg("n_sweeps") = 16, py::arg("omega") = 0.8,
            py::arg("max_levels") = 32, py::arg("phase") = "barrier",
-           "Time every tile variant on each FAS coarse level from the restricted resident state and keep each "
-           "level's fastest (the fine state is unchanged); returns (best per level, ms [level, variant]).")
+           "Time every tile variant on each FAS coarse level from the restricted resident state, as per-sweep "
+           "launches and (small levels) as one single-workgroup launch, and keep each level's fastest (the fine "
+           "state is unchanged); returns (best per level, ms [level, 2 V]; -1 = not timed).")
       .def("harvest", &Session::harvest, py::arg("mlp"), py::arg("n_snap"), py::kw_only(), py::arg("every") = 8,
            py::arg("horizon") = 8, py::arg("omega") = 0.8,
            "Append (features, displacement over `horizon` plain sweeps) pairs of every interior DOF to the "

@@ -339,7 +339,7 @@ eggmi_status eggmi_autotune_levels(eggmi_session* s, int n_sweeps, double omega,
         need(s, "session");
         need(n_levels, "n_levels");
         const auto t = s->s.autotune_levels(n_sweeps, omega, max_levels, phase_kind(phase));
-        const int nv = eggmi::num_tile_variants(s->s.dim);
+        const int nv = 2 * eggmi::num_tile_variants(s->s.dim);
         const int nl = (int)(t.size() / nv);
         if (ms)
             for (int64_t i = 0; i < capacity && i < (int64_t)t.size(); ++i) ms[i] = t[i];

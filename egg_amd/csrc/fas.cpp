@@ -20,7 +20,10 @@ namespace eggmi
 
 namespace
 {
-constexpr int kMinCoarsenNodes = 5;  // an axis halves iff its node count is odd and >= 5 (src/mg_level.hpp:46,142)
+constexpr int kMinCoarsenNodes = 5;
+// the level autotuner times one-workgroup smoothing on levels of at most
+// this many tiles (it walks them in turn)
+constexpr size_t kOneWgMaxTiles = 64;  // an axis halves iff its node count is odd and >= 5 (src/mg_level.hpp:46,142)
 constexpr int kMaxHierarchyDepth = 32;
 
 using Idx = std::array<int, 3>;
@@ -381,7 +384,7 @@ void Session::upload_level(size_t i)
     if (L.x[0].n) return;
     L.plane = (L.lay.n_slots() + 63) / 64 * 64;
     L.upload_tables();
-    L.make_tiles(coarse_variant(i));
+    apply_level_variant(i);
     const size_t nd = (size_t)L.plane * dim;
     for (auto* a : {&L.x[0], &L.x[1], &L.rx, &L.tau, &L.grad}) {
         a->alloc(nd);
@@ -495,6 +498,17 @@ void Session::smooth(Level& L, int sweeps, int kind, float omega, const float* t
     // smoothing sweeps as in plain runs; the other Jacobi buffer holds each
     // DOF's state before the last sweep (or correction), its displacement input
     const bool propose = proposal != nullptr && &L == &fine && tau == nullptr && kind != 1;
+    if (L.one_wg && tau && !propose) {
+        if (sweeps <= 0) return;
+        float* x0 = L.x[L.cur].p;
+        float* x1 = L.x[1 - L.cur].p;
+        SweepArgs a = L.sweep_args(x0, x1, omega, 0.0f, tau);
+        a.counters = counters_for(L);
+        launch_level_smooth(dim, kind, L.variant, a, x0, x1, sweeps, stream);
+        if (sweeps % 2) L.cur = 1 - L.cur;
+        ++last_launches;
+        return;
+    }
     for (int s = 0; s < sweeps; ++s) {
         float* xin = L.x[L.cur].p;
         float* xout = L.x[1 - L.cur].p;
@@ -724,8 +738,8 @@ std::vector<double> Session::autotune_levels(int n_sweeps, double omega, int max
     if (n_sweeps < 1) throw InvalidArgument("autotune_levels: n_sweeps must be >= 1");
     if (kind == 1) throw InvalidArgument("autotune_levels: FAS smooths the barrier phases only");
     const size_t n_coarse = prepare_fas(max_levels);
-    const int nv = num_tile_variants(dim);
-    std::vector<double> ms(n_coarse * nv, 0.0);
+    const int nt = num_tile_variants(dim), nv = 2 * nt;  // per-sweep launches, then one workgroup
+    std::vector<double> ms(n_coarse * nv, -1.0);
     if (n_coarse == 0) return ms;
     const bool count0 = counting;
     const long long launches0 = last_launches;
@@ -755,7 +769,12 @@ std::vector<double> Session::autotune_levels(int n_sweeps, double omega, int max
             for (int v = 0; v < nv; ++v) {
                 // timed as the captured graph a cycle replays them in: the
                 // launch gaps are the graph's, not the host's launch rate
-                L.make_tiles(v);
+                L.make_tiles(v % nt);
+                L.one_wg = v >= nt;
+                // one workgroup walks every tile in turn: only small levels
+                // of synthesized DOFs can gain from it (-1: not timed)
+                if (L.one_wg && (!L.st_dof.empty() || !L.con_tag.empty() || L.tiles.size() > kOneWgMaxTiles))
+                    continue;
                 reset();
                 EGGMI_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
                 try {
@@ -780,9 +799,10 @@ std::vector<double> Session::autotune_levels(int n_sweeps, double omega, int max
             }
             int best = 0;
             for (int v = 1; v < nv; ++v)
-                if (ms[i * nv + v] < ms[i * nv + best]) best = v;
+                if (ms[i * nv + v] >= 0.0 && ms[i * nv + v] < ms[i * nv + best]) best = v;
             picks[i] = best;
-            L.make_tiles(best);
+            L.make_tiles(best % nt);
+            L.one_wg = best >= nt;
             if (i + 1 < n_coarse) {
                 grad(L, L.x[L.cur].p, L.grad.p, L.tau.p, kind, false, true);
                 restrict_to(L, L.x[L.cur].p, *coarse[i + 1], kind);
@@ -793,7 +813,7 @@ std::vector<double> Session::autotune_levels(int n_sweeps, double omega, int max
         drop();
         counting = count0;
         last_launches = launches0;
-        for (size_t i = 0; i < n_coarse; ++i) coarse[i]->make_tiles(coarse_variant(i));
+        for (size_t i = 0; i < n_coarse; ++i) apply_level_variant(i);
         throw;
     }
     counting = count0;

@@ -5,6 +5,8 @@
 #include "mg.hpp"
 #include "sweep_common.hpp"
 
+#include <stdexcept>
+
 namespace eggmi
 {
 
@@ -12,6 +14,8 @@ void launch_sweep2d(int kind, int v, const SweepArgs& a, hipStream_t s);
 void launch_grad2d(int kind, int v, const SweepArgs& a, hipStream_t s);
 void launch_sweep3d(int kind, int v, const SweepArgs& a, hipStream_t s);
 void launch_grad3d(int kind, int v, const SweepArgs& a, hipStream_t s);
+void launch_level_smooth2d(int kind, int v, const SweepArgs& a, float* x0, float* x1, int n, hipStream_t s);
+void launch_level_smooth3d(int kind, int v, const SweepArgs& a, float* x0, float* x1, int n, hipStream_t s);
 
 // node tiles (tx = last axis); every node is worked on by 2^D lanes
 static constexpr TileShape kTiles2[] = {{32, 2, 1}, {16, 4, 1}, {64, 1, 1}, {32, 4, 1},
@@ -39,6 +43,17 @@ void launch_sweep(int dim, int kind, int v, const SweepArgs& a, hipStream_t s)
         launch_sweep2d(kind, v, a, s);
     else
         launch_sweep3d(kind, v, a, s);
+}
+
+void launch_level_smooth(int dim, int kind, int v, const SweepArgs& a, float* x0, float* x1, int n_sweeps,
+                         hipStream_t s)
+{
+    if (a.st.n > 0 || a.con_n > 0 || !a.tau)
+        throw std::invalid_argument("one-workgroup smoothing takes tau-shifted levels of synthesized DOFs only");
+    if (dim == 2)
+        launch_level_smooth2d(kind, v, a, x0, x1, n_sweeps, s);
+    else
+        launch_level_smooth3d(kind, v, a, x0, x1, n_sweeps, s);
 }
 
 void launch_grad(int dim, int kind, int v, const SweepArgs& a, hipStream_t s)

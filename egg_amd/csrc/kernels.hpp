@@ -151,6 +151,12 @@ void launch_sweep(int dim, int kind, int tile_variant, const SweepArgs& a, hipSt
 // Metric pass only: per-DOF gradient (minus tau when given) into a.gout
 // (a.gsub: a.gout = gradient - a.gout, in place).
 void launch_grad(int dim, int kind, int tile_variant, const SweepArgs& a, hipStream_t s);
+// n_sweeps tau-shifted sweeps of a FAS coarse level (synthesized DOFs only)
+// in one single-workgroup launch, over the variant's tile table: x0 holds
+// the state on entry (a.xin / a.xout are replaced), the result ends in x0
+// for an even n_sweeps and in x1 for an odd one.
+void launch_level_smooth(int dim, int kind, int tile_variant, const SweepArgs& a, float* x0, float* x1,
+                         int n_sweeps, hipStream_t s);
 void launch_copy(float* x, long long plane, int dim, const int* src, const int* dst, long long n, hipStream_t s);
 void launch_reduce_cells(int dim, int kind, const CellReduceArgs& a, double* partials, hipStream_t s);
 void launch_reduce_samples(int dim, int kind, const SampleReduceArgs& a, double* partials, hipStream_t s);

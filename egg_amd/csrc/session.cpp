@@ -1201,7 +1201,7 @@ void Session::set_tile_variant(int v)
     release_graphs();  // they hold the tile table being replaced
     fine.make_tiles(v);
     for (size_t i = 0; i < coarse.size(); ++i)
-        if (coarse[i]->x[0].n) coarse[i]->make_tiles(coarse_variant(i));
+        if (coarse[i]->x[0].n) apply_level_variant(i);
 }
 
 int Session::coarse_variant(size_t i) const
@@ -1209,15 +1209,22 @@ int Session::coarse_variant(size_t i) const
     return i < level_variant.size() && level_variant[i] >= 0 ? level_variant[i] : tile_variant;
 }
 
+void Session::apply_level_variant(size_t i)
+{
+    const int nv = num_tile_variants(dim), v = coarse_variant(i);
+    coarse[i]->one_wg = v >= nv;
+    coarse[i]->make_tiles(v % nv);
+}
+
 void Session::set_level_variants(const std::vector<int>& v)
 {
     for (int x : v)
-        if (x < -1 || x >= num_tile_variants(dim)) throw InvalidArgument("level tile variant out of range");
+        if (x < -1 || x >= 2 * num_tile_variants(dim)) throw InvalidArgument("level tile variant out of range");
     level_variant = v;
     if (host_only) return;
     release_graphs();
     for (size_t i = 0; i < coarse.size(); ++i)
-        if (coarse[i]->x[0].n) coarse[i]->make_tiles(coarse_variant(i));
+        if (coarse[i]->x[0].n) apply_level_variant(i);
 }
 
 std::vector<double> Session::autotune(int n_sweeps, double omega)

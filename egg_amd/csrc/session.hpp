@@ -135,6 +135,7 @@ struct Level {
     std::vector<TileDesc> tiles;
     DArr<TileDesc> d_tiles;
     int variant = 0;  // the tile variant the tile table is laid out for
+    bool one_wg = false;  // (FAS coarse level) each smoothing call is one single-workgroup launch
     std::vector<int> syn_slot, syn_blk;  // synthesized DOFs as a list (lazily, for harvesting)
     DArr<int> d_syn_slot, d_syn_blk;
     void ensure_synth_list();
@@ -262,7 +263,12 @@ struct Session {
     int built_levels = -1;
     // tile variant of each coarse level (-1 = the session's tile_variant)
     std::vector<int> level_variant;
+    // level variant of coarse level i: a tile variant v < V launches every
+    // sweep over the level's tiles; V + v smooths the whole level in one
+    // single-workgroup launch per smoothing call on variant v's tiles
+    // (V = num_tile_variants)
     int coarse_variant(size_t i) const;
+    void apply_level_variant(size_t i);
     std::vector<uint8_t> fine_free;  // relaxed fine DOF slots
     HaloHost fine_halo;
     DArr<double> d_ls;

@@ -53,6 +53,38 @@ template <int KIND, bool GRAD = false> static void launch_constrained(int v, con
 
 static void sweep2d_main(int kind, int v, const SweepArgs& a, hipStream_t s);
 
+// One workgroup smoothing a whole FAS coarse level (tau-shifted sweeps; the
+// tile variant's shape and lane layout)
+template <int KIND>
+static void level2d_v(int v, const SweepArgs& a, float* x0, float* x1, int n, hipStream_t s)
+{
+    switch (v) {
+    case 0: return launch_level_variant<2, KIND, 32, 2, 1>(a, x0, x1, n, s);
+    case 1: return launch_level_variant<2, KIND, 16, 4, 1>(a, x0, x1, n, s);
+    case 2: return launch_level_variant<2, KIND, 64, 1, 1>(a, x0, x1, n, s);
+    case 3: return launch_level_variant<2, KIND, 32, 4, 1>(a, x0, x1, n, s);
+    case 4: return launch_level_variant<2, KIND, 16, 1, 1, true, 1, true>(a, x0, x1, n, s);
+    case 5: return launch_level_variant<2, KIND, 8, 2, 1, true, 1, true>(a, x0, x1, n, s);
+    case 6: return launch_level_variant<2, KIND, 64, 1, 1, false>(a, x0, x1, n, s);
+    case 7: return launch_level_variant<2, KIND, 32, 2, 1, false>(a, x0, x1, n, s);
+    case 8: return launch_level_variant<2, KIND, 32, 4, 1, true, 2>(a, x0, x1, n, s);
+    case 9: return launch_level_variant<2, KIND, 64, 4, 1, true, 4>(a, x0, x1, n, s);
+    case 10: return launch_level_variant<2, KIND, 32, 4, 1, false, 2>(a, x0, x1, n, s);
+    case 11: return launch_level_variant<2, KIND, 64, 2, 1, true, 2>(a, x0, x1, n, s);
+    case 12:  // variants 12 and 13 differ from 0 only in their stored lanes, which coarse levels lack
+    case 13: return launch_level_variant<2, KIND, 32, 2, 1>(a, x0, x1, n, s);
+    default: throw std::invalid_argument("2D tile variant " + std::to_string(v) + " out of range");
+    }
+}
+
+void launch_level_smooth2d(int kind, int v, const SweepArgs& a, float* x0, float* x1, int n, hipStream_t s)
+{
+    if (kind == kShapeSize)
+        level2d_v<kShapeSize>(v, a, x0, x1, n, s);
+    else
+        level2d_v<kShape>(v, a, x0, x1, n, s);
+}
+
 void launch_sweep2d(int kind, int v, const SweepArgs& a, hipStream_t s)
 {
     sweep2d_main(kind, v, a, s);

@@ -39,6 +39,28 @@ template <int KIND, bool GRAD = false> static void launch_special3d(int v, const
 }
 
 static void sweep3d_main(int kind, int v, const SweepArgs& a, hipStream_t s);
+
+// One workgroup smoothing a whole FAS coarse level (tau-shifted sweeps)
+template <int KIND>
+static void level3d_v(int v, const SweepArgs& a, float* x0, float* x1, int n, hipStream_t s)
+{
+    switch (v) {
+    case 0: return launch_level_variant<3, KIND, 8, 4, 2>(a, x0, x1, n, s);
+    case 1: return launch_level_variant<3, KIND, 4, 4, 4>(a, x0, x1, n, s);
+    case 2:  // variant 4 differs from 2 only in its stored lanes, which coarse levels lack
+    case 4: return launch_level_variant<3, KIND, 8, 4, 2, false>(a, x0, x1, n, s);
+    case 3: return launch_level_variant<3, KIND, 8, 4, 2, true, 2>(a, x0, x1, n, s);
+    default: throw std::invalid_argument("3D tile variant " + std::to_string(v) + " out of range");
+    }
+}
+
+void launch_level_smooth3d(int kind, int v, const SweepArgs& a, float* x0, float* x1, int n, hipStream_t s)
+{
+    if (kind == kShapeSize)
+        level3d_v<kShapeSize>(v, a, x0, x1, n, s);
+    else
+        level3d_v<kShape>(v, a, x0, x1, n, s);
+}
 static void grad3d_main(int kind, int v, const SweepArgs& a, hipStream_t s);
 
 void launch_sweep3d(int kind, int v, const SweepArgs& a, hipStream_t s)

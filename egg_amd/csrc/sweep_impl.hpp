@@ -1097,4 +1097,38 @@ void launch_variant(const SweepArgs& a, hipStream_t s)
             <<<dim3(grid), dim3(NT), 0, s>>>(a);
 }
 
+// A whole FAS coarse level smoothed by one workgroup: n_sweeps tau-shifted
+// sweeps, each walking the level's tiles in turn. A workgroup barrier
+// follows every tile (the next tile restages the LDS tile), so the barrier
+// after a sweep's last tile also orders its writes, replicas included,
+// before the next sweep's reads: the part a kernel boundary plays between
+// one-sweep launches (all waves of a workgroup share one CU and its L1).
+// x0 holds the state on entry; sweep s reads x_(s mod 2) and writes the
+// other buffer. Per DOF the arithmetic is the one-sweep kernel's.
+template <int D, int KIND, int TX, int TY, int TZ, bool LDS, int CPL, bool SAMPLE>
+__global__ __launch_bounds__(SAMPLE ? 16 * TX * TY : ((1 << D) / CPL) * TX * TY * TZ) void level_smooth_kernel(
+    const SweepArgs a0, float* x0, float* x1, int n_sweeps)
+{
+    SweepArgs a = a0;
+    for (int s = 0; s < n_sweeps; ++s) {
+        a.xin = (s & 1) ? x1 : x0;
+        a.xout = (s & 1) ? x0 : x1;
+        for (int t = 0; t < a.n_tiles; ++t) {
+            if constexpr (SAMPLE)
+                sample_sweep<KIND, TX, TY, true, false>(a, t);
+            else
+                tile_sweep<D, KIND, TX, TY, TZ, true, false, false, LDS, CPL>(a, t);
+            __syncthreads();
+        }
+    }
+}
+
+template <int D, int KIND, int TX, int TY, int TZ, bool LDS = true, int CPL = 1, bool SAMPLE = false>
+void launch_level_variant(const SweepArgs& a, float* x0, float* x1, int n_sweeps, hipStream_t s)
+{
+    constexpr int NT = SAMPLE ? 16 * TX * TY : ((1 << D) / CPL) * TX * TY * TZ;
+    if (a.n_tiles == 0 || n_sweeps <= 0) return;
+    level_smooth_kernel<D, KIND, TX, TY, TZ, LDS, CPL, SAMPLE><<<dim3(1), dim3(NT), 0, s>>>(a, x0, x1, n_sweeps);
+}
+
 }  // namespace eggmi

@@ -245,7 +245,10 @@ eggmi_status eggmi_autotune(eggmi_session* s, int n_sweeps, double omega, double
  * variants[i] for coarse level i + 1 (level 0 is the fine grid, which keeps
  * the session's variant), -1 = the session's variant; levels beyond n
  * follow the session's variant, and n = 0 clears every choice. Coarse levels
- * never run the learned proposal, so every variant is allowed there. */
+ * never run the learned proposal, so every variant is allowed there. A
+ * value V + v (V = the tile variant count) smooths the level in one
+ * single-workgroup launch per smoothing call, walking variant v's tiles in
+ * turn with a workgroup barrier between them (same bits as v). */
 eggmi_status eggmi_set_level_variants(eggmi_session* s, const int* variants, int n);
 
 /* Level autotuner (no reference counterpart). Coarse levels have far fewer
@@ -255,10 +258,12 @@ eggmi_status eggmi_set_level_variants(eggmi_session* s, const int* variants, int
  * state restricted down the chain (as a cycle's first restriction, without
  * smoothing), time n_sweeps FAS-shifted sweeps of `phase` (barrier phases
  * only), replayed as one captured hipGraph as a cycle replays them, with
- * every tile variant and keep the fastest. The fine state is
- * not changed. ms[l * V + v] (< capacity; V = the variant count) receives
- * level l + 1's time with variant v, best[l] (< best_capacity) its pick,
- * *n_levels the number of levels tuned. */
+ * every tile variant and keep the fastest; levels of at most 64 tiles are
+ * also timed as one single-workgroup launch per variant (level variant
+ * V + v). The fine state is not changed. ms[l * 2V + v] (< capacity; V =
+ * the tile variant count) receives level l + 1's time with level variant
+ * v (-1: not timed), best[l] (< best_capacity) its pick, *n_levels the
+ * number of levels tuned. */
 eggmi_status eggmi_autotune_levels(eggmi_session* s, int n_sweeps, double omega, int max_levels, int phase,
                                    double* ms, int64_t capacity, int* best, int best_capacity, int* n_levels);
 

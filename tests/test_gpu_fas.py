@@ -252,12 +252,37 @@ def test_level_variants_fas_parity(oracle):
     np.testing.assert_allclose(s.get_X(), Xo, atol=1e-5)
 
 
+@pytest.mark.parametrize("graphs", [0, 256])
+@pytest.mark.parametrize("make", [lambda: ogrid(65, 33), _cube_case])
+def test_one_workgroup_levels_same_bits(make, graphs):
+    """A coarse level smoothed by one single-workgroup launch per smoothing
+    call (level variant V + v) walks variant v's tiles with the one-sweep
+    kernel's per-DOF arithmetic and Jacobi buffers, so its cycles give the
+    bits of the per-sweep launches of v, odd and even sweep counts alike."""
+    c = make()
+    nv = _session(c).core.set_tile_variant(-1)
+    for v in ([0, 2, 4] if c.grid.d == 2 else [0, 1, 3]):
+        a, b = _session(c), _session(c)
+        for s in (a, b):
+            s.core.set_graph_chunk(graphs)
+        a.core.set_level_variants([v] * 3)
+        b.core.set_level_variants([nv + v] * 3)
+        print(f"one-workgroup levels: dim {c.grid.d} variant {v} graphs {graphs}", flush=True)
+        for nu, nuc in ((2, 5), (3, 4)):
+            ea, ma = a.run_fas(2, nu_pre=nu, nu_post=nu, nu_coarse=nuc, omega=0.8, max_levels=4)
+            eb, mb = b.run_fas(2, nu_pre=nu, nu_post=nu, nu_coarse=nuc, omega=0.8, max_levels=4)
+            np.testing.assert_array_equal(eb, ea)
+            np.testing.assert_array_equal(mb, ma)
+            np.testing.assert_array_equal(b.get_X(), a.get_X())
+
+
 @pytest.mark.parametrize("make", [lambda: ogrid(65, 33), _cube_case])
 def test_autotune_levels(make):
-    """The level autotuner times every variant on each coarse level, leaves
-    the fine state as it was, and the cycles that follow give the bits of a
-    session told the same picks directly; out-of-range picks and the
-    untangle phase are refused."""
+    """The level autotuner times every variant on each coarse level, as
+    per-sweep launches and (small levels) as one single-workgroup launch,
+    leaves the fine state as it was, and the cycles that follow give the
+    bits of a session told the same picks directly; out-of-range picks and
+    the untangle phase are refused."""
     c = make()
     s, t = _session(c), _session(c)
     for x in (s, t):
@@ -266,8 +291,10 @@ def test_autotune_levels(make):
     X0 = s.get_X()
     best, ms = s.core.autotune_levels(4, 0.8, 4)
     n_lv = len(s.mg_levels())
-    assert ms.shape == (min(3, n_lv), nv) and np.all(ms > 0)
-    assert len(best) == ms.shape[0] and all(int(np.argmin(ms[i])) == best[i] for i in range(len(best)))
+    assert ms.shape == (min(3, n_lv), 2 * nv) and np.all(ms[:, :nv] > 0)
+    assert np.all((ms[:, nv:] > 0) | (ms[:, nv:] == -1)) and np.any(ms[-1, nv:] > 0)
+    timed = np.where(ms > 0, ms, np.inf)
+    assert len(best) == ms.shape[0] and all(int(np.argmin(timed[i])) == best[i] for i in range(len(best)))
     np.testing.assert_array_equal(s.get_X(), X0)
     t.core.set_level_variants(best)
     for _ in range(2):
@@ -276,6 +303,6 @@ def test_autotune_levels(make):
         np.testing.assert_array_equal(es, et)
         np.testing.assert_array_equal(s.get_X(), t.get_X())
     with pytest.raises(ValueError, match="level tile variant"):
-        s.core.set_level_variants([nv])
+        s.core.set_level_variants([2 * nv])
     with pytest.raises(ValueError, match="barrier"):
         s.core.autotune_levels(2, 0.8, 3, "untangle")

@@ -211,15 +211,16 @@ def test_capi_error_codes_without_device():
 
 
 def test_level_variants_validated_without_device():
-    """Per-level tile variants are range-checked (-1 = the session's)."""
+    """Per-level tile variants are range-checked (-1 = the session's; V + v
+    = one-workgroup smoothing on variant v's tiles)."""
     from egg_amd.smoothing import structured_arrays
 
     c = single(n=17)
     s = _core().CppStructuredSweepSession(c.ctx.wire, structured_arrays(c.bsc), c.X.ravel(), device="tables")
     nv = s.set_tile_variant(-1)
-    s.set_level_variants([-1, nv - 1, 0])
+    s.set_level_variants([-1, nv - 1, 0, nv, 2 * nv - 1])
     s.set_level_variants([])
-    for bad in ([nv], [-2], [0, nv + 3]):
+    for bad in ([2 * nv], [-2], [0, 2 * nv + 3]):
         with pytest.raises(ValueError, match="level tile variant"):
             s.set_level_variants(bad)
 
