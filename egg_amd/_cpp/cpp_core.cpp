@@ -659,7 +659,7 @@ class Session
         check(eggmi_session_harvest(s_, m.handle(), n_snap, every, horizon, omega, &added));
         return added;
     }
-    void set_proposal(const py::object& m, const std::string& mode, float scale)
+    void set_proposal(const py::object& m, const std::string& mode, float scale, int every)
     {
         int md;
         if (mode == "replace")
@@ -674,6 +674,7 @@ class Session
             return;
         }
         Mlp& net = m.cast<Mlp&>();
+        check(eggmi_session_set_proposal_every(s_, every));
         check(eggmi_session_set_proposal(s_, net.handle(), md, scale));
         proposal_ = m;  // the session borrows the network: keep it alive
     }
@@ -753,8 +754,9 @@ PYBIND11_MODULE(cpp_core, m)
            "Append (features, displacement over `horizon` plain sweeps) pairs of every interior DOF to the "
            "network's training set; returns the number of pairs added.")
       .def("set_proposal", &Session::set_proposal, py::arg("mlp"), py::kw_only(), py::arg("mode") = "replace",
-           py::arg("scale") = 1.0f,
-           "Use a trained network (or None) as the learned proposal of subsequent run() calls.");
+           py::arg("scale") = 1.0f, py::arg("every") = 1,
+           "Use a trained network (or None) as the learned proposal of subsequent run() calls, in sweeps 0, every, "
+           "2 every, ... of each call (FAS fine smoothing: every sweep, or with every > 1 each call's first).");
 
     py::class_<Mlp>(m, "ProposalMLP", "Learned Newton-step proposal network (BF16 matrix cores, FP32 master).")
       .def(py::init<int, uint64_t, int64_t>(), py::arg("dim") = 2, py::arg("seed") = 0,

@@ -545,11 +545,20 @@ eggmi_status eggmi_session_set_proposal(eggmi_session* s, eggmi_mlp* m, int mode
         if (m && m->m->dim != s->s.dim) throw eggmi::InvalidArgument("proposal network dimension differs from the session's");
         if (mode != 0 && mode != 1) throw eggmi::InvalidArgument("proposal mode must be 0 (replace) or 1 (predict-correct)");
         if (!(scale > 0.0f) || !std::isfinite(scale)) throw eggmi::InvalidArgument("proposal scale must be positive");
-        if (m && !eggmi::tile_variant_has_proposal(s->s.dim, s->s.tile_variant))
-            s->s.set_tile_variant(0);  // only the one-lane-per-cell LDS shapes have a proposal kernel
+        if (m && eggmi::tile_variant_proposal_partner(s->s.dim, s->s.tile_variant) < 0)
+            s->s.set_tile_variant(0);  // no proposal kernel runs on this variant's tile table
         s->s.proposal = m ? m->m.get() : nullptr;
         s->s.proposal_mode = mode;
         s->s.proposal_scale = scale;
+    });
+}
+
+eggmi_status eggmi_session_set_proposal_every(eggmi_session* s, int every)
+{
+    return guarded([&] {
+        need(s, "session");
+        if (every < 1) throw eggmi::InvalidArgument("proposal cadence must be >= 1");
+        s->s.proposal_every = every;
     });
 }
 

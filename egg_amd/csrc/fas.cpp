@@ -514,12 +514,13 @@ void Session::smooth(Level& L, int sweeps, int kind, float omega, const float* t
         float* xout = L.x[1 - L.cur].p;
         SweepArgs a = L.sweep_args(xin, xout, omega, 0.0f, tau);
         a.counters = counters_for(L);
-        if (propose) {
+        const bool here = propose && (proposal_every == 1 || s == 0);
+        if (here) {
             a.mlp = proposal->view();
             a.prop_mode = proposal_mode;
             a.prop_scale = proposal_scale;
         }
-        launch_sweep(dim, kind, L.variant, a, stream);
+        launch_sweep(dim, kind, here ? proposal_variant() : L.variant, a, stream);
         L.cur = 1 - L.cur;
         ++last_launches;
     }
@@ -879,7 +880,7 @@ hipGraphExec_t Session::fas_graph(int parity, int kind, const FasParams& o, size
     const void* mlp = propose ? (const void*)proposal->view().fwd : nullptr;
     if (g.exec && g.kind == kind && g.variant == tile_variant && g.nu_pre == o.nu_pre && g.nu_post == o.nu_post &&
         g.nu_coarse == o.nu_coarse && g.n_coarse == n_coarse && g.omega == o.omega && g.rep == d_fas_reports.p &&
-        g.mlp == mlp && (!propose || (g.mode == proposal_mode && g.scale == proposal_scale)))
+        g.mlp == mlp && (!propose || (g.mode == proposal_mode && g.scale == proposal_scale && g.every == proposal_every)))
         return g.exec;
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
     g = FasGraph {};
@@ -935,6 +936,7 @@ hipGraphExec_t Session::fas_graph(int parity, int kind, const FasParams& o, size
     g.mlp = mlp;
     g.mode = proposal_mode;
     g.scale = proposal_scale;
+    g.every = proposal_every;
     return g.exec;
 }
 

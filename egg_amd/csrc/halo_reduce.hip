@@ -30,6 +30,19 @@ static constexpr TileShape kTiles3[] = {{8, 4, 2}, {4, 4, 4}, {8, 4, 2} /* no LD
 
 int num_tile_variants(int dim) { return dim == 2 ? 14 : 5; }
 bool tile_variant_has_proposal(int dim, int v) { return dim == 2 ? v < 4 : v < 2; }
+int tile_variant_proposal_partner(int dim, int v)
+{
+    if (tile_variant_has_proposal(dim, v)) return v;
+    const TileShape t = tile_variant_shape(dim, v);
+    for (int p = 0; p < num_tile_variants(dim); ++p) {
+        if (!tile_variant_has_proposal(dim, p)) continue;
+        const TileShape q = tile_variant_shape(dim, p);
+        if (q.tx == t.tx && q.ty == t.ty && q.tz == t.tz &&
+            tile_variant_stored_lanes(dim, p) == tile_variant_stored_lanes(dim, v))
+            return p;
+    }
+    return -1;
+}
 TileShape tile_variant_shape(int dim, int v) { return dim == 2 ? kTiles2[v] : kTiles3[v]; }
 int tile_variant_stored_lanes(int dim, int v)
 {

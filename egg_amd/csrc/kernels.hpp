@@ -185,5 +185,8 @@ int tile_variant_stored_lanes(int dim, int v);
 // whether the variant has a learned-proposal kernel (the 2D one-lane-per-
 // sample variants do not)
 bool tile_variant_has_proposal(int dim, int v);
+// the variant whose proposal kernel runs on variant v's tile table (same
+// tile shape and stored lanes): v itself when it has one, else -1 if none
+int tile_variant_proposal_partner(int dim, int v);
 
 }  // namespace eggmi

@@ -1027,7 +1027,8 @@ std::pair<std::vector<double>, std::vector<double>> Session::run_jacobi(int n_sw
     int s = 0;
     while (s < n_sweeps) {
         const int next_rep = std::min(n_sweeps, (s / k + 1) * k);
-        if (graphs_on && next_rep - s >= graph_chunk) {
+        // a captured chunk repeats the cadence from its first sweep
+        if (graphs_on && next_rep - s >= graph_chunk && graph_chunk % proposal_every == 0) {
             // an even chunk ends in the buffer it started from
             EGGMI_HIP(hipGraphLaunch(sweep_graph(xin == L.x[0].p ? 0 : 1, kind, (float)omega, (float)delta, propose),
                                      stream));
@@ -1036,13 +1037,14 @@ std::pair<std::vector<double>, std::vector<double>> Session::run_jacobi(int n_sw
         } else {
             SweepArgs a = L.sweep_args(xin, xout, (float)omega, (float)delta, nullptr);
             a.counters = counters_for(L);
-            if (propose) {
+            const bool here = propose && proposes_at(s);
+            if (here) {
                 a.mlp = proposal->view();
                 a.prop_mode = proposal_mode;
                 a.prop_scale = proposal_scale;
             }
             if (kernel_timing) EGGMI_HIP(hipEventRecord(tev[2 * s], stream));
-            launch_sweep(dim, kind, tile_variant, a, stream);
+            launch_sweep(dim, kind, here ? proposal_variant() : tile_variant, a, stream);
             if (kernel_timing) EGGMI_HIP(hipEventRecord(tev[2 * s + 1], stream));
             std::swap(xin, xout);
             ++s;
@@ -1090,7 +1092,8 @@ hipGraphExec_t Session::sweep_graph(int parity, int kind, float omega, float del
     SweepGraph& g = graphs[parity];
     const void* mlp = propose ? (const void*)proposal->view().fwd : nullptr;
     if (g.exec && g.kind == kind && g.variant == tile_variant && g.chunk == graph_chunk && g.omega == omega &&
-        g.delta == delta && g.mlp == mlp && (!propose || (g.mode == proposal_mode && g.scale == proposal_scale)))
+        g.delta == delta && g.mlp == mlp &&
+        (!propose || (g.mode == proposal_mode && g.scale == proposal_scale && g.every == proposal_every)))
         return g.exec;
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
     g = SweepGraph {};
@@ -1102,12 +1105,13 @@ hipGraphExec_t Session::sweep_graph(int parity, int kind, float omega, float del
     try {
         for (int i = 0; i < graph_chunk; ++i) {
             SweepArgs a = L.sweep_args(i % 2 == 0 ? b0 : b1, i % 2 == 0 ? b1 : b0, omega, delta, nullptr);
-            if (propose) {
+            const bool here = propose && proposes_at(i);
+            if (here) {
                 a.mlp = proposal->view();
                 a.prop_mode = proposal_mode;
                 a.prop_scale = proposal_scale;
             }
-            launch_sweep(dim, kind, tile_variant, a, stream);
+            launch_sweep(dim, kind, here ? proposal_variant() : tile_variant, a, stream);
         }
     } catch (...) {
         (void)hipStreamEndCapture(stream, &graph);
@@ -1126,6 +1130,7 @@ hipGraphExec_t Session::sweep_graph(int parity, int kind, float omega, float del
     g.mlp = mlp;
     g.mode = proposal_mode;
     g.scale = proposal_scale;
+    g.every = proposal_every;
     return g.exec;
 }
 
@@ -1193,9 +1198,9 @@ std::pair<double, double> Session::energy(int kind, double delta)
 void Session::set_tile_variant(int v)
 {
     if (v < 0 || v >= num_tile_variants(dim)) throw InvalidArgument("tile variant out of range");
-    if (proposal && !tile_variant_has_proposal(dim, v))
-        throw InvalidArgument("tile variant " + std::to_string(v) + " has no learned-proposal kernel; "
-                              "unset the proposal first");
+    if (proposal && tile_variant_proposal_partner(dim, v) < 0)
+        throw InvalidArgument("tile variant " + std::to_string(v) + " has no learned-proposal kernel for its tile "
+                              "table; unset the proposal first");
     tile_variant = v;
     if (host_only) return;
     release_graphs();  // they hold the tile table being replaced
@@ -1248,7 +1253,7 @@ std::vector<double> Session::autotune(int n_sweeps, double omega)
     std::vector<double> ms(nv, 0.0);
     try {
         for (int v = 0; v < nv; ++v) {
-            if (prop0 && !tile_variant_has_proposal(dim, v)) {
+            if (prop0 && tile_variant_proposal_partner(dim, v) < 0) {
                 ms[v] = std::numeric_limits<double>::infinity();  // not usable with the proposal set
                 continue;
             }

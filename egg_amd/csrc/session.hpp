@@ -237,7 +237,7 @@ struct Session {
     // baked into it changes
     struct SweepGraph {
         hipGraphExec_t exec = nullptr;
-        int kind = -1, variant = -1, mode = 0, chunk = 0;
+        int kind = -1, variant = -1, mode = 0, chunk = 0, every = 1;
         float omega = 0.0f, delta = 0.0f, scale = 0.0f;
         const void* mlp = nullptr;
     };
@@ -277,6 +277,12 @@ struct Session {
     const Mlp* proposal = nullptr;  // learned proposal for run_jacobi (not owned)
     int proposal_mode = 0;
     float proposal_scale = 1.0f;
+    // cadence: plain runs propose in sweeps 0, k, 2k, ... of each run call;
+    // with k > 1 a FAS fine smoothing call proposes in its first sweep only
+    int proposal_every = 1;
+    bool proposes_at(int sweep) const { return sweep % proposal_every == 0; }
+    // the variant the proposal sweeps launch (same tile table as tile_variant)
+    int proposal_variant() const { return tile_variant_proposal_partner(dim, tile_variant); }
     bool pair_valid = false;        // the second Jacobi buffer holds the previous sweep's state
 
     Session() = default;
@@ -343,6 +349,7 @@ struct Session {
         float omega = 0.0f, scale = 0.0f;
         const void* mlp = nullptr;
         const double* rep = nullptr;
+        int every = 1;
         long long launches = 0;
         long long kernels = 0;  // kernel nodes
     };

@@ -93,13 +93,15 @@ class CppStructuredSweepSession:
     def set_X(self, X: np.ndarray) -> None:
         self._session.set_X(np.ascontiguousarray(X, np.float64).ravel())
 
-    def set_proposal(self, mlp, *, mode: str = "replace", scale: float = 1.0) -> None:
+    def set_proposal(self, mlp, *, mode: str = "replace", scale: float = 1.0, every: int = 1) -> None:
         """Use a trained ``cpp_core.ProposalMLP`` (or None) as the learned
         Newton-step proposal of subsequent :meth:`run` calls and of the
         fine-level smoothing sweeps of :meth:`run_fas`. ``mode`` "replace":
         an accepted proposal is the update; "predict_correct": the Newton
-        step continues from it."""
-        self._session.set_proposal(mlp, mode=mode, scale=scale)
+        step continues from it. ``every``: :meth:`run` proposes in sweeps 0,
+        every, 2 every, ... of each call; with every > 1 a FAS fine
+        smoothing call proposes in its first sweep only."""
+        self._session.set_proposal(mlp, mode=mode, scale=scale, every=every)
 
     def harvest(self, mlp, n_snap: int, *, every: int = 8, horizon: int = 8, omega: float = 0.8) -> int:
         """Append (features, displacement over ``horizon`` plain sweeps) pairs

@@ -347,6 +347,12 @@ eggmi_status eggmi_session_harvest(eggmi_session* s, eggmi_mlp* m, int n_snap, i
  * proposal. The session borrows m: keep it alive (or unset it) until the
  * session is destroyed. */
 eggmi_status eggmi_session_set_proposal(eggmi_session* s, eggmi_mlp* m, int mode, float scale);
+/* Proposal cadence (default 1): eggmi_run proposes in sweeps 0, every,
+ * 2 every, ... of each call (the other sweeps run the plain kernel of the
+ * session's tile variant); with every > 1, each fine-level smoothing call of
+ * eggmi_run_fas proposes in its first sweep only. No reference counterpart:
+ * the learned proposal is an addition of this build's. */
+eggmi_status eggmi_session_set_proposal_every(eggmi_session* s, int every);
 
 /* Jacobi-preconditioned conjugate gradients in FP64 on a CSR operator
  * (rowptr[n+1], col/val[nnz]; SPD expected): the PCG skeleton of the

@@ -47,6 +47,12 @@ class Structured(C.Structure):
                 ("n_sing", C.c_int64), ("sing_block", C.c_void_p), ("sing_logical", C.c_void_p)]
 
 
+class Info(C.Structure):
+    _fields_ = [("n_slots", C.c_int64), ("n_overflow", C.c_int64), ("n_halo", C.c_int64), ("n_fan", C.c_int64),
+                ("n_share", C.c_int64), ("n_tiles", C.c_int64), ("n_stored", C.c_int64), ("n_synth", C.c_int64),
+                ("structured_energy", C.c_int32)]
+
+
 def load():
     lib = C.CDLL(str(LIB_PATH))
     lib.eggmi_last_error.restype = C.c_char_p

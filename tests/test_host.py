@@ -206,6 +206,10 @@ def test_capi_error_codes_without_device():
     assert lib.eggmi_autotune_levels(None, 4, ctypes.c_double(0.8), 3, 0, None, ctypes.c_int64(0), None, 0,
                                      ctypes.byref(n)) == 1
     assert b"session" in lib.eggmi_last_error()
+    assert lib.eggmi_session_set_proposal_every(None, 2) == 1
+    k = ctypes.c_int64()
+    assert lib.eggmi_fas_cycle_kernels(None, ctypes.byref(k)) == 1
+    assert b"session" in lib.eggmi_last_error()
     assert lib.eggmi_mlp_destroy(None) == 0  # deleting nothing is fine
     assert lib.eggmi_session_destroy(None) == 0
 
@@ -401,11 +405,11 @@ def test_raw_ctypes_session_tables_only():
     lib = CA.load()
     c = ogrid(17, 9)
     s = CA.Session(lib, c.ctx.wire, structured_arrays(c.bsc), c.X, 2, device=3)
-    info = (C.c_int64 * 8)()
+    info = CA.Info()  # the whole struct: the library writes every field
     s.check(lib.eggmi_session_info(s.handle, C.byref(info)))
     ref_info, _ = _tables(c)
-    assert [info[i] for i in range(5)] == [ref_info[k] for k in ("n_slots", "n_overflow", "n_halo", "n_fan",
-                                                                 "n_share")]
+    keys = ("n_slots", "n_overflow", "n_halo", "n_fan", "n_share", "structured_energy")
+    assert [getattr(info, k) for k in keys] == [ref_info[k] for k in keys]
     s.destroy()
     bad = dict(c.ctx.wire["groups"][0])
     bad["dof_idx"] = np.array(bad["dof_idx"]) + 10**6
