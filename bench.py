@@ -349,6 +349,61 @@ def _cubed_sample(s, g, ctx, Xu, every=40):
     return cpu, par
 
 
+def _cubed_proposal(s, Xu, n_plain=400):
+    """The 3D learned proposal (85 -> 128 -> 128 -> 3) on the cubed sphere:
+    trained on the GPU on pairs harvested from plain runs on a held-out
+    tangle seed (6 snapshots 25 sweeps apart of its 1.75 M interior DOFs:
+    10.5 M pairs); then, from the untangled state, sweeps and time for the
+    plain method and for the proposal in every k-th sweep (replace mode) to
+    reach the energy the plain method has after `n_plain` sweeps (no FP64
+    converged energy is pinned for this grid)."""
+    from egg_amd import grids
+    from egg_amd._cpp import cpp_core
+    from egg_amd.smoothing import cpp_untangle
+
+    net = cpp_core.ProposalMLP(3, 0, 11_000_000)
+    t0 = time.perf_counter()
+    gh = grids.tangle(grids.cubed_sphere_3d(65), frac=0.01, factor=1.5, seed=11)
+    ctx_h, bsc_h, sh = _build(gh)
+    cpp_untangle(ctx_h, bsc_h, gh.X, session=sh)
+    sh.harvest(net, 6, every=25, horizon=2, omega=0.8)
+    del sh
+    t_h = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    hist = net.train(10, batch=4096, lr=1e-3, seed=1)
+    t_t = time.perf_counter() - t0
+    X0 = s.get_X()
+    s.set_X(Xu)
+    e, _ = s.run(n_plain, omega=0.8, report_every=n_plain)
+    target = float(e[-1])
+
+    def to_target(budget=30.0):
+        done, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < budget:
+            e, m = s.run(16, omega=0.8, report_every=16)
+            done += 16
+            if e[-1] <= target and m[-1] > 0:
+                return done, time.perf_counter() - t0
+        return None, time.perf_counter() - t0
+
+    runs = {}
+    for every in (None, 4, 8, 16):
+        s.set_X(Xu)
+        if every:
+            s.set_proposal(net, mode="replace", scale=1.0, every=every)
+        s.run(16, omega=0.8)  # warm the kernels
+        s.set_X(Xu)
+        n, t = to_target()
+        runs["plain" if every is None else f"every_{every}"] = {"sweeps": n, "s": t}
+        log(f"3D proposal {every}: {n} sweeps, {t:.3f}s to E {target:.8g}")
+        s.set_proposal(None)
+    s.set_X(X0)
+    return {"pairs": int(net.shape()["n_data"]), "harvest_s": t_h, "train_s": t_t,
+            "loss_first_last": [float(hist[0]), float(hist[-1])], "held_out_seed": 11,
+            "target": f"E after {n_plain} plain sweeps from the untangled state", "target_energy": target,
+            "mode": "replace", "scale": 1.0, "to_target": runs}
+
+
 def _train_proposal(make_grid, seeds, horizon=2, snaps=42, every=50, epochs=10):
     """Harvest (features, displacement over `horizon` plain sweeps) pairs of
     every interior DOF from plain runs on held-out tangle seeds (5 seeds x
@@ -450,15 +505,29 @@ def bench_ogrid(args):
         ttc["mlp_proposal"] = {"sweeps": n, "s": t + t_untangle if n else None, "E_end": E,
                                "mode": "predict_correct", "scale": 0.5, **info, **(errors(n) or {})}
         log(f"MLP proposal: {n} sweeps, {t:.3f}s to converge ({info})")
-        # the proposal in the fine-level smoothing of the fastest FAS cycle
-        s.set_X(Xu)
-        n, t, E = _converge(lambda: (*s.run_fas(5, nu_pre=8, nu_post=8, nu_coarse=16, omega=0.8, max_levels=4), 5),
-                            e_star, 60.0)
+        # the proposal in every 16th sweep only (replace mode), the plain
+        # kernel of the tuned variant in between
         s.set_proposal(None)
-        ttc["fas_v88_4_levels_coarse16_mlp_proposal"] = {"cycles": n, "s": t + t_untangle if n else None,
-                                                         "E_end": E, "mode": "predict_correct", "scale": 0.5,
-                                                         **(errors(n) or {})}
-        log(f"FAS V(8,8) + MLP proposal: {n} cycles, {t:.3f}s to converge")
+        s.core.set_tile_variant(int(tune_best))
+        s.set_X(Xu)
+        s.set_proposal(net, mode="replace", scale=1.0, every=16)
+        n, t, E = _converge(plain_step, e_star, 60.0)
+        ttc["mlp_proposal_every_16"] = {"sweeps": n, "s": t + t_untangle if n else None, "E_end": E,
+                                        "mode": "replace", "scale": 1.0, "every": 16, **(errors(n) or {})}
+        log(f"MLP proposal every 16th sweep: {n} sweeps, {t:.3f}s to converge")
+        # the proposal in the fine-level smoothing of the fastest FAS cycle:
+        # every sweep, then the first sweep of each smoothing call
+        for every, mode, scale in ((1, "predict_correct", 0.5), (2, "replace", 1.0)):
+            s.set_X(Xu)
+            s.set_proposal(net, mode=mode, scale=scale, every=every)
+            n, t, E = _converge(
+                lambda: (*s.run_fas(5, nu_pre=8, nu_post=8, nu_coarse=16, omega=0.8, max_levels=4), 5), e_star, 60.0)
+            key = "fas_v88_4_levels_coarse16_mlp_proposal" + ("" if every == 1 else "_first_sweep")
+            ttc[key] = {"cycles": n, "s": t + t_untangle if n else None, "E_end": E, "mode": mode, "scale": scale,
+                        "proposes_in": "every fine sweep" if every == 1 else "the first sweep of each fine smoothing",
+                        **(errors(n) or {})}
+            log(f"FAS V(8,8) + MLP proposal ({key}): {n} cycles, {t:.3f}s to converge")
+        s.set_proposal(None)
         s.core.set_tile_variant(int(tune_best))
 
     cpu = parity = config1 = None
@@ -618,11 +687,12 @@ def bench_cubed(args):
                 else:
                     x, iters, relres, status = pcg.pcg_solve_lattice(33, 3, b, rtol=1e-8, max_iter=1000,
                                                                      recurrence=rec)
-                ms = cpp_core.last_pcg_ms()
-                best_ms = ms if best_ms is None else min(best_ms, ms)
+                solve_ms = cpp_core.last_pcg_ms()
+                best_ms = solve_ms if best_ms is None else min(best_ms, solve_ms)
             pcg_runs[f"{op}_{rec}"] = {"iterations": int(iters), "relres": float(relres), "status": int(status),
                                       "solve_ms": best_ms, "us_per_iteration": best_ms * 1e3 / max(iters, 1)}
             log(f"PCG 33^3 {op} {rec}: {iters} iterations, relres {relres:.2e}, {best_ms:.3f} ms")
+    proposal = None if args.no_converge else _cubed_proposal(s, Xu)
     cpu = parity = None
     if not args.no_cpu:
         cpu, parity = _cubed_sample(s, g, ctx, Xu)
@@ -637,6 +707,7 @@ def bench_cubed(args):
         "fas_v44_3_levels": {"cycles": K, "ms_per_cycle": t_fas / K * 1e3, "updates_per_cycle_by_level": counts,
                              "energy_first_last": [float(ef[0]), float(ef[-1])]},
         "pcg_33cubed": pcg_runs,
+        "learned_proposal_3d": proposal,
         "autotune": {"best_variant": int(best), "ms_per_variant": [float(v) for v in ms],
                      "fas_level_variants": [int(v) for v in lv_best]},
         "time_to_untangled_s": t_untangle, "untangle_outer": outer, "roofline": roof, "cpu_baseline": cpu,
