@@ -429,12 +429,121 @@ class Session
             const py::object& control) :
         dim_(require_dim(dim))
     {
-        if (!control.is_none())
-            throw py::value_error("control-net smoothing (run_control) is not provided by this compute core");
         n_ = checked_num_nodes(X, dim_);
         Wire w(ctx, structured, dim_);
         check(eggmi_session_create(dim_, w.groups.data(), (int)w.groups.size(), &w.es, &w.st, X.data(),
                                    (int64_t)n_, device_id(device), &s_));
+        if (!control.is_none()) {
+            try {
+                attach_control(control.cast<py::dict>(), structured);
+            } catch (...) {
+                eggmi_session_destroy(s_);
+                s_ = nullptr;
+                throw;
+            }
+        }
+    }
+
+    // The control wire (reference egg/smoothing/control_backend.py:124-136)
+    // into eggmi_control; sliding walls are not supported.
+    void attach_control(const py::dict& w, const py::dict& structured)
+    {
+        for (const char* k : {"walls", "X0"})
+            if (w.contains(k) && !w[k].is_none())
+                throw py::value_error(std::string("control wire field '") + k +
+                                      "': sliding wall controls are not supported by this compute core");
+        eggmi_control c {};
+        c.block = w.contains("block") ? w["block"].cast<int>() : 0;
+        // the block's node shape, for the per-node array lengths
+        const py::list shapes = structured["interior_shapes"].cast<py::list>();
+        if (c.block < 0 || c.block >= (int)shapes.size())
+            throw py::value_error("control wire: block " + std::to_string(c.block) + " out of range");
+        IArr bshape = shapes[c.block].cast<IArr>();
+        if (bshape.size() != dim_) throw py::value_error("control wire: the block's shape must hold dim values");
+        IArr cs = get_i(w, "ctrl_shape");
+        if (cs.size() != dim_) throw py::value_error("control wire: ctrl_shape must hold dim values");
+        for (int k = 0; k < dim_; ++k) c.ctrl_shape[k] = cs.data()[k];
+        c.k_supp = w.contains("k_supp") ? w["k_supp"].cast<int>() : 4;
+        std::vector<IArr> offs;
+        std::vector<DArr> wts;
+        for (int k = 0; k < dim_; ++k) {
+            offs.push_back(get_i(w, ("off" + std::to_string(k)).c_str()));
+            wts.push_back(get_d(w, ("wt" + std::to_string(k)).c_str()));
+            if (offs.back().size() != bshape.data()[k])
+                throw py::value_error("control wire: off" + std::to_string(k) + " must hold one value per node of "
+                                      "the block's axis " + std::to_string(k));
+            if (wts.back().size() != offs.back().size() * c.k_supp)
+                throw py::value_error("control wire: wt" + std::to_string(k) + " must hold k_supp values per node");
+            c.off[k] = offs.back().data();
+            c.wt[k] = wts.back().data();
+        }
+        DArr C0 = get_d(w, "C0");
+        IArr fi = get_i(w, "free_idx");
+        int64_t nc = 1;
+        for (int k = 0; k < dim_; ++k) nc *= c.ctrl_shape[k];
+        if (C0.size() != nc * dim_) throw py::value_error("control wire: C0 must hold dim values per control");
+        c.C0 = C0.data();
+        c.n_free = fi.size();
+        c.free_idx = fi.data();
+        DArr b;
+        if (w.contains("b") && !w["b"].is_none()) {
+            b = get_d(w, "b");
+            int64_t nn = dim_;
+            for (int k = 0; k < dim_; ++k) nn *= bshape.data()[k];
+            if (b.size() != nn) throw py::value_error("control wire: b must hold dim values per node of the block");
+            c.b = b.data();
+        }
+        check(eggmi_session_set_control(s_, &c));
+        n_ctrl_ = nc;
+    }
+
+    void set_C(const DArr& C) { check(eggmi_control_set_C(s_, C.data(), (int64_t)C.size())); }
+
+    py::array_t<double> get_C()
+    {
+        if (n_ctrl_ == 0) throw py::value_error("get_C: the session has no control wire");
+        py::array_t<double> out(n_ctrl_ * dim_);
+        check(eggmi_control_get_C(s_, out.mutable_data(), (int64_t)out.size()));
+        return out;
+    }
+
+    py::dict run_control(int n_outer, const std::string& phase, double grad_tol, double alpha_min, double lm0,
+                         int pcg_max_iter, double pcg_rtol, bool pcg_forcing, bool model_db)
+    {
+        (void)model_db;  // the Boolean-sum linearization matters for sliding walls only
+        if (phase == "untangle")
+            throw py::value_error("run_control: the untangle phase stays nodal; run(phase=\"untangle\") first");
+        if (n_ctrl_ == 0) throw py::value_error("run_control: the session was built without a control wire");
+        eggmi_control_options o {};
+        o.n_outer = n_outer;
+        o.grad_tol = grad_tol;
+        o.alpha_min = alpha_min;
+        o.lm0 = lm0;
+        o.lm_tries = 8;
+        o.pcg_max_iter = pcg_max_iter;
+        o.pcg_rtol = pcg_rtol;
+        o.pcg_forcing = pcg_forcing ? 1 : 0;
+        const int64_t cap = std::max(n_outer, 0) + 1, pcap = cap * o.lm_tries;
+        std::vector<double> e(cap), m(cap), a(cap), pi(pcap), pr(pcap);
+        int64_t nr = 0, np = 0;
+        int32_t iters = 0, conv = 0;
+        check(eggmi_run_control(s_, &o, phase_id(phase), e.data(), m.data(), a.data(), cap, &nr, pi.data(), pr.data(),
+                                pcap, &np, &iters, &conv));
+        e.resize(nr);
+        m.resize(nr);
+        a.resize(std::max<int64_t>(nr - 1, 0));
+        pi.resize(np);
+        pr.resize(np);
+        py::dict d;
+        d["energies"] = to_array(e);
+        d["mindets"] = to_array(m);
+        d["alphas"] = to_array(a);
+        d["frame_jumps"] = to_array(std::vector<double>{});
+        d["iters"] = iters;
+        d["converged"] = conv != 0;
+        d["pcg_iters"] = to_array(pi);
+        d["pcg_relres"] = to_array(pr);
+        return d;
     }
     ~Session()
     {
@@ -682,6 +791,7 @@ class Session
   private:
     int dim_;
     size_t n_ = 0;
+    int64_t n_ctrl_ = 0;
     eggmi_session* s_ = nullptr;
     py::object proposal_ = py::none();
 };
@@ -728,6 +838,15 @@ PYBIND11_MODULE(cpp_core, m)
       .def("halo_exchange", &Session::halo_exchange)
       .def("set_tile_variant", &Session::set_tile_variant, py::arg("variant"))
       .def("last_timing", &Session::last_timing)
+      .def("set_C", &Session::set_C, py::arg("C"),
+           "Upload a full control lattice (n_ctrl * dim values) and re-evaluate the net into the state.")
+      .def("get_C", &Session::get_C, "Download the full control lattice (n_ctrl * dim values).")
+      .def("run_control", &Session::run_control, py::arg("n_outer"), py::kw_only(), py::arg("phase") = "barrier",
+           py::arg("grad_tol") = 1e-8, py::arg("alpha_min") = 1e-4, py::arg("lm0") = 0.0,
+           py::arg("pcg_max_iter") = 200, py::arg("pcg_rtol") = 1e-10, py::arg("pcg_forcing") = true,
+           py::arg("model_db") = true,
+           "Up to n_outer control-net reduced Gauss-Newton iterations on the resident net; returns {energies, "
+           "mindets, alphas, frame_jumps, iters, converged, pcg_iters, pcg_relres}.")
       .def("fas_cycle_kernels", &Session::fas_cycle_kernels)
       .def("set_kernel_timing", &Session::set_kernel_timing, py::arg("on"))
       .def("set_counters", &Session::set_counters, py::arg("on"))

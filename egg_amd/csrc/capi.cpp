@@ -553,6 +553,73 @@ eggmi_status eggmi_session_set_proposal(eggmi_session* s, eggmi_mlp* m, int mode
     });
 }
 
+eggmi_status eggmi_session_set_control(eggmi_session* s, const eggmi_control* c)
+{
+    return guarded([&] {
+        need(s, "session");
+        need(c, "control");
+        s->s.set_control(*c);
+    });
+}
+
+eggmi_status eggmi_control_set_C(eggmi_session* s, const double* C, int64_t n)
+{
+    return guarded([&] {
+        need(s, "session");
+        need(C, "C");
+        s->s.control_set_C(C, n);
+    });
+}
+
+eggmi_status eggmi_control_get_C(eggmi_session* s, double* C, int64_t n)
+{
+    return guarded([&] {
+        need(s, "session");
+        need(C, "C");
+        s->s.control_get_C(C, n);
+    });
+}
+
+eggmi_status eggmi_run_control(eggmi_session* s, const eggmi_control_options* o, int phase, double* energies,
+                               double* mindets, double* alphas, int64_t capacity, int64_t* n_reports,
+                               double* pcg_iters, double* pcg_relres, int64_t pcg_capacity, int64_t* n_pcg,
+                               int32_t* iters, int32_t* converged)
+{
+    return guarded([&] {
+        need(s, "session");
+        need(o, "options");
+        const int kind = phase_kind(phase);
+        if (o->n_outer < 0 || o->lm_tries < 1 || o->pcg_max_iter < 0)
+            throw eggmi::InvalidArgument("run_control: n_outer, pcg_max_iter must be >= 0 and lm_tries >= 1");
+        if (!(o->alpha_min > 0.0) || !(o->pcg_rtol >= 0.0) || !(o->grad_tol >= 0.0) || !(o->lm0 >= 0.0))
+            throw eggmi::InvalidArgument("run_control: alpha_min must be > 0; grad_tol, pcg_rtol, lm0 >= 0");
+        eggmi::ControlOptions op;
+        op.n_outer = o->n_outer;
+        op.grad_tol = o->grad_tol;
+        op.alpha_min = o->alpha_min;
+        op.lm0 = o->lm0;
+        op.lm_tries = o->lm_tries;
+        op.pcg_max_iter = o->pcg_max_iter;
+        op.pcg_rtol = o->pcg_rtol;
+        op.pcg_forcing = o->pcg_forcing != 0;
+        const eggmi::ControlReport r = s->s.run_control(op, kind);
+        const int64_t nr = (int64_t)r.energies.size(), np = (int64_t)r.pcg_iters.size();
+        for (int64_t i = 0; i < nr && i < capacity; ++i) {
+            if (energies) energies[i] = r.energies[i];
+            if (mindets) mindets[i] = r.mindets[i];
+            if (alphas && i + 1 < nr) alphas[i] = r.alphas[i];
+        }
+        for (int64_t i = 0; i < np && i < pcg_capacity; ++i) {
+            if (pcg_iters) pcg_iters[i] = r.pcg_iters[i];
+            if (pcg_relres) pcg_relres[i] = r.pcg_relres[i];
+        }
+        if (n_reports) *n_reports = nr;
+        if (n_pcg) *n_pcg = np;
+        if (iters) *iters = r.iters;
+        if (converged) *converged = r.converged ? 1 : 0;
+    });
+}
+
 eggmi_status eggmi_session_set_proposal_every(eggmi_session* s, int every)
 {
     return guarded([&] {

@@ -1,0 +1,91 @@
+// Control-net reduced Gauss-Newton smoother on one block (reference
+// src/control_solve.hpp:308-342, :533-676, kernels src/control_net.hpp:
+// 486-828): the block's nodes are X = M C + b, M the tensor product of
+// per-axis banded B-spline windows; each outer iteration pulls the fine
+// TMOP gradient back to the free controls (G = M_f^T g), solves
+// (A + lambda I) dC = -G with A = sum over samples of W_p^T H_p W_p (the
+// exact Hessian of E(C): the map is linear) by Jacobi-preconditioned
+// matrix-free PCG in FP64, and line-searches the node-space step M dC on the
+// global fine energy under the sweep's accept rule; a rejected step raises
+// the Levenberg damping.
+//
+// Device data: per-axis windows (off, wt) and their inverse (control ->
+// nodes with weights), the lattice C and offset b in FP64, the block's node
+// positions M C + b in FP64 (the loop's state: an FP32 store would floor
+// the reduced gradient near 1e-6, far above the loop's 1e-8 convergence
+// test; the session's FP32 store receives the rounded positions after each
+// accepted step), per-sample metric gradient and full symmetric Hessian
+// (rebuilt once per outer iteration), block-local FP64 node fields for the
+// gradient, the step M dC, the Hessian action and the contracted node
+// Hessians, and FP64 free-control vectors.
+#pragma once
+
+#include "kernels.hpp"
+
+#include <cstdint>
+#include <vector>
+
+namespace eggmi
+{
+
+// Block and map geometry the kernels share (by value).
+struct CtrlGeom {
+    int dim = 2;
+    int n[3] = {1, 1, 1};   // node shape (interior nodes of the block)
+    int c[3] = {1, 1, 1};   // control lattice shape
+    int ks = 4;             // support window width per axis
+    long long n_nodes = 0;  // prod(n)
+    long long n_ctrl = 0;   // prod(c)
+    long long n_cells = 0;  // prod(n - 1)
+    int n_free = 0;
+    BlockDesc bd {};        // the block in the fine store
+    long long plane = 0;
+    const int* off[3] = {};       // [n[k]] first control of node i's window
+    const double* wt[3] = {};     // [n[k] * ks]
+    const int* sup_off[3] = {};   // [c[k] + 1] control a -> its nodes
+    const int* sup_node[3] = {};
+    const double* sup_w[3] = {};
+    const int* free_idx = nullptr;  // [n_free] lattice index
+    const int* l2f = nullptr;       // [n_ctrl] lattice -> free slot, -1 = fixed
+};
+
+// Evaluation of the map over the block's nodes into block-local FP64 node
+// fields (dim per node, node order row-major) and, when given, into the FP32
+// store at the block's node slots:
+//   state: M C + b
+//   dir:   M v with v a free-control vector (fixed controls 0, no b)
+void launch_ctrl_eval_state(const CtrlGeom& g, const double* C, const double* b, double* out, float* out_plane,
+                            hipStream_t s);
+void launch_ctrl_eval_dir(const CtrlGeom& g, const double* v, double* out, float* out_plane, hipStream_t s);
+// Per sample of the block's cells (cell-major, corner bits axis 0 first):
+// dmu/dA (dim^2) and the symmetric d2mu/dA2 (upper triangle row-major,
+// dim^2 (dim^2 + 1) / 2) in FP64 at the block-local positions xb. kind:
+// 0 shape, 2 shape_size.
+void launch_ctrl_samples(const CtrlGeom& g, int kind, const double* xb, double* gs, double* hs, hipStream_t s);
+// The block's cell energy and min det at xb + a0 dir and xb + a1 dir (dir
+// null: at xb), FP64: out = {E(a0), md(a0), E(a1), md(a1)}; partials holds
+// 4 kCtrlParts doubles.
+constexpr int kCtrlParts = 256;
+void launch_ctrl_energy(const CtrlGeom& g, int kind, const double* xb, const double* dir, double a0, double a1,
+                        double* partials, double* out, hipStream_t s);
+// Per block node (block-local order), gathered over its incident samples:
+// mode 0: the energy gradient (dim) from gs; mode 1: the Hessian action
+// (dim) on the node field dir; mode 2: the contracted node Hessian (dim^2).
+void launch_ctrl_node(const CtrlGeom& g, int mode, const double* gs, const double* hs, const double* dir,
+                      double* out, hipStream_t s);
+// Per free control: mode 0 out (dim) = sum_nodes w * y; mode 1 out (dim^2) =
+// sum_nodes w^2 * y (dim^2 blocks: the Jacobi preconditioner).
+void launch_ctrl_pullback(const CtrlGeom& g, int mode, const double* y, double* out, hipStream_t s);
+// z = (P_f + lam I)^-1 r per free control (dim x dim blocks).
+void launch_ctrl_precond(const CtrlGeom& g, const double* P, double lam, const double* r, double* z, hipStream_t s);
+// One-workgroup FP64 reductions into *out: 0 x.y, 1 max |x|, 2 the trace sum
+// of n dim x dim blocks of x.
+void launch_ctrl_reduce(int op, const double* x, const double* y, long long n, int dim, double* out, hipStream_t s);
+// y = a x + y (xpay = false) or y = x + a y (xpay = true); n doubles.
+void launch_ctrl_axpy(long long n, double a, const double* x, double* y, bool xpay, hipStream_t s);
+// y = -x (n doubles)
+void launch_ctrl_neg(long long n, const double* x, double* y, hipStream_t s);
+// C[free_idx[f]] += a * v[f] (dim values each)
+void launch_ctrl_scatter_free(const CtrlGeom& g, double a, const double* v, double* C, hipStream_t s);
+
+}  // namespace eggmi

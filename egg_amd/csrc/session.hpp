@@ -4,6 +4,7 @@
 // ABI in capi.cpp (include/eggmi.h).
 #pragma once
 
+#include "control.hpp"
 #include "eggmi.h"
 #include "kernels.hpp"
 #include "mg.hpp"
@@ -217,6 +218,34 @@ struct Level {
 
 struct Mlp;
 
+// Control-net reduced Gauss-Newton state of one block (control_host.cpp).
+struct ControlOptions {
+    int n_outer = 30;
+    double grad_tol = 1e-8, alpha_min = 1e-4, lm0 = 0.0;
+    int lm_tries = 8, pcg_max_iter = 200;
+    double pcg_rtol = 1e-10;
+    bool pcg_forcing = true;
+};
+struct ControlReport {
+    std::vector<double> energies, mindets, alphas, pcg_iters, pcg_relres;
+    int iters = 0;
+    bool converged = false;
+};
+struct ControlNet {
+    CtrlGeom g;  // sizes, block geometry and the device tables below
+    int block = 0;
+    DArr<int> off[3], sup_off[3], sup_node[3], free_idx, l2f;
+    DArr<double> wt[3], sup_w[3], C, b;
+    // per sample: metric gradient and symmetric Hessian; block-local node
+    // fields (positions, step, gathers); free-control vectors; scalars
+    DArr<double> gs, hs, xb, ynode, dir, hnode, prec, G, dC, r, z, p, Ap, scal, eparts;
+    // the other blocks' cells (their energy and min det, constant during a run)
+    DArr<BlockDesc> oblocks;
+    DArr<long long> ocell_off;
+    int n_oblocks = 0;
+    long long n_ocells = 0;
+};
+
 struct Session {
     int dim = 2;
     long long M = 0;  // global node count
@@ -284,6 +313,13 @@ struct Session {
     // the variant the proposal sweeps launch (same tile table as tile_variant)
     int proposal_variant() const { return tile_variant_proposal_partner(dim, tile_variant); }
     bool pair_valid = false;        // the second Jacobi buffer holds the previous sweep's state
+    // control-net reduced Gauss-Newton (control_host.cpp)
+    std::unique_ptr<ControlNet> control;
+    void set_control(const eggmi_control& c);
+    void control_set_C(const double* C, long long n);
+    void control_get_C(double* C, long long n) const;
+    void control_eval_to_x();
+    ControlReport run_control(const ControlOptions& o, int kind);
 
     Session() = default;
     ~Session();

@@ -120,6 +120,29 @@ def single_block_2d(n: int = 129, *, seed: int = 0, amp: float = 0.3) -> Grid:
     raise RuntimeError("could not draw a valid perturbation")
 
 
+def wavy_block_2d(n: int = 13, *, wave: float = 0.35, perturb: float = 0.04, seed: int = 11) -> Grid:
+    """n x n nodes on [0, 4]^2 with the top edge raised to y = 4 + wave
+    sin(pi x / 4), the interior filled by transfinite interpolation of the
+    boundary and then moved by perturb * N(0, 1) per coordinate (the control-
+    net smoother's test block, reference tests/smoothing/test_control_ref.py:
+    41-86); the boundary is fixed."""
+    t = np.linspace(0.0, 1.0, n)
+    P = np.zeros((n, n, 2))
+    P[:, 0] = np.stack([4 * t, np.zeros(n)], -1)  # bottom
+    P[:, -1] = np.stack([4 * t, 4 + wave * np.sin(np.pi * t)], -1)  # top (x = 4 t)
+    P[0, :] = np.stack([np.zeros(n), 4 * t], -1)  # left
+    P[-1, :] = np.stack([np.full(n, 4.0), 4 * t], -1)  # right
+    u, v = t[:, None, None], t[None, :, None]
+    P[1:-1, 1:-1] = ((1 - u) * P[:1] + u * P[-1:] + (1 - v) * P[:, :1] + v * P[:, -1:]
+                     - ((1 - u) * (1 - v) * P[0, 0] + u * (1 - v) * P[-1, 0] + (1 - u) * v * P[0, -1]
+                        + u * v * P[-1, -1]))[1:-1, 1:-1]
+    P[1:-1, 1:-1] += perturb * np.random.default_rng(seed).standard_normal((n - 2, n - 2, 2))
+    ids = np.arange(n * n, dtype=np.int32).reshape(n, n)
+    free = np.zeros((n, n), bool)
+    free[1:-1, 1:-1] = True
+    return Grid(2, [ids], P.reshape(-1, 2), free.reshape(-1))
+
+
 def single_block_3d(n: int = 17, *, seed: int = 0, amp: float = 0.2) -> Grid:
     """n^3 nodes on [0, 1]^3, interior perturbed by U(-amp h, amp h)."""
     h = 1.0 / (n - 1)

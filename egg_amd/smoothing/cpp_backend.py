@@ -93,6 +93,34 @@ class CppStructuredSweepSession:
     def set_X(self, X: np.ndarray) -> None:
         self._session.set_X(np.ascontiguousarray(X, np.float64).ravel())
 
+    def run_control(self, n_outer: int, *, phase: str = "barrier", grad_tol: float = 1e-8, alpha_min: float = 1e-4,
+                    lm0: float = 0.0, pcg_max_iter: int = 200, pcg_rtol: float = 1e-10, pcg_forcing: bool = True,
+                    model_db: bool = True) -> dict:
+        """Up to ``n_outer`` control-net reduced Gauss-Newton iterations on
+        the resident net (the session needs ``control=``; reference
+        cpp_backend.py:591-641): the fine gradient pulled back to the free
+        controls, ``(A + lam I) dC = -G`` by Jacobi-preconditioned
+        matrix-free FP64 PCG with the exact per-sample Hessian action, a line
+        search of ``M dC`` on the global fine energy (accept: finite,
+        monotone to 1e-6, min det > 0), the damping raised on rejection.
+        ``pcg_forcing`` solves early steps only to max(pcg_rtol, min(0.5,
+        sqrt(max|G| / max|G|_0))). ``model_db`` concerns sliding wall
+        controls, which are not supported, and is ignored. Returns
+        ``{energies, mindets, alphas, frame_jumps, iters, converged,
+        pcg_iters, pcg_relres}``."""
+        return self._session.run_control(n_outer, phase=phase, grad_tol=grad_tol, alpha_min=alpha_min, lm0=lm0,
+                                         pcg_max_iter=pcg_max_iter, pcg_rtol=pcg_rtol, pcg_forcing=pcg_forcing,
+                                         model_db=model_db)
+
+    def get_C(self) -> np.ndarray:
+        """The full control lattice, shape ``(n_ctrl, d)``."""
+        return self._session.get_C().reshape(-1, self._d)
+
+    def set_C(self, C: np.ndarray) -> None:
+        """Upload a full control lattice ``(n_ctrl, d)`` and re-evaluate the
+        net (+ b) into the resident state."""
+        self._session.set_C(np.ascontiguousarray(C, np.float64).ravel())
+
     def set_proposal(self, mlp, *, mode: str = "replace", scale: float = 1.0, every: int = 1) -> None:
         """Use a trained ``cpp_core.ProposalMLP`` (or None) as the learned
         Newton-step proposal of subsequent :meth:`run` calls and of the

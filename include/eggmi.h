@@ -395,6 +395,70 @@ eggmi_status eggmi_pcg_csr_noalloc(int64_t n, const int64_t* rowptr, const int32
                                    const double* b, double* x, double rtol, int max_iter, int recurrence,
                                    void* stream, int* iters, double* relres, int* status);
 
+/* ---- control-net reduced Gauss-Newton (one block) */
+
+/* One block's control net (the reference's `control` wire,
+ * egg/smoothing/control_backend.py:124-136, unpacked at src/bindings.cpp
+ * into ControlNetHostT): the block's nodes are X = M C + b with M the tensor
+ * product of per-axis banded windows. Node i of axis k reads controls
+ * off[k][i] .. off[k][i] + k_supp - 1 with weights wt[k][i * k_supp ..].
+ * C0: the lattice (row-major over ctrl_shape, last axis fastest, dim values
+ * per control); free_idx: flat lattice indices of the free controls; b:
+ * the block's Boolean-sum offset in its node order (row-major over the
+ * block's node shape, dim values per node), NULL = 0. Sliding wall controls
+ * (the wire's walls / X0) have no counterpart here. */
+typedef struct {
+    int32_t block;
+    int32_t ctrl_shape[3];
+    int32_t k_supp;
+    const int32_t* off[3];
+    const double* wt[3];
+    const double* C0;
+    int64_t n_free;
+    const int32_t* free_idx;
+    const double* b;
+} eggmi_control;
+
+/* Attach a control net to a session and evaluate it into the resident
+ * state (the block's nodes become M C0 + b). Replaces the session
+ * constructor's `control=` argument (src/bindings.cpp:1153-1259,
+ * ControlSessionT ctor src/control_solve.hpp:352-480). */
+eggmi_status eggmi_session_set_control(eggmi_session* s, const eggmi_control* c);
+/* Upload a full lattice (n = n_ctrl * dim doubles) and re-evaluate the net
+ * into the resident state; download it. Replace set_C / get_C
+ * (src/control_solve.hpp:493-509). */
+eggmi_status eggmi_control_set_C(eggmi_session* s, const double* C, int64_t n);
+eggmi_status eggmi_control_get_C(eggmi_session* s, double* C, int64_t n);
+
+/* Options of the reduced Gauss-Newton loop (ControlOptions,
+ * src/control_solve.hpp:308-329). */
+typedef struct {
+    int32_t n_outer;       /* outer iterations at most */
+    double grad_tol;       /* converged when max |G| < grad_tol */
+    double alpha_min;      /* line-search floor */
+    double lm0;            /* initial Levenberg damping */
+    int32_t lm_tries;      /* damping ladder per outer iteration */
+    int32_t pcg_max_iter;
+    double pcg_rtol;       /* PCG relative-residual floor */
+    int32_t pcg_forcing;   /* 1: tolerance max(pcg_rtol, min(0.5, sqrt(max|G| / max|G|_0))) */
+} eggmi_control_options;
+
+/* Run the loop in place on the session's state (ControlSessionT::run_plain,
+ * src/control_solve.hpp:533-676): reduced gradient, Jacobi-preconditioned
+ * matrix-free FP64 PCG on (A + lam I) dC = -G, line search of M dC on the
+ * global fine energy (FP64 reductions; accept iff finite, E <= E0 + 1e-6 and
+ * min det > 0), damping x10 (floored at 1e-8 trace(P) / n) on rejection,
+ * lam / 3 after a full step. phase: barrier or shape_size (untangle is
+ * EGGMI_INVALID_ARGUMENT). energies / mindets receive the initial state's
+ * and every accepted iterate's (capacity >= n_outer + 1), alphas the
+ * accepted steps, pcg_iters / pcg_relres one entry per PCG solve (capacity
+ * pcg_capacity >= n_outer * lm_tries); *n_reports, *n_pcg, *iters and
+ * *converged the counts and flags. */
+eggmi_status eggmi_run_control(eggmi_session* s, const eggmi_control_options* o, int phase, double* energies,
+                               double* mindets, double* alphas, int64_t capacity, int64_t* n_reports,
+                               double* pcg_iters, double* pcg_relres, int64_t pcg_capacity, int64_t* n_pcg,
+                               int32_t* iters, int32_t* converged);
+
 const char* eggmi_last_error(void);
 
 /* Library liveness (reference cpp_core.ping, src/bindings.cpp:1544). */

@@ -207,6 +207,10 @@ def test_capi_error_codes_without_device():
                                      ctypes.byref(n)) == 1
     assert b"session" in lib.eggmi_last_error()
     assert lib.eggmi_session_set_proposal_every(None, 2) == 1
+    assert lib.eggmi_session_set_control(None, None) == 1
+    assert lib.eggmi_control_get_C(None, None, ctypes.c_int64(0)) == 1
+    assert lib.eggmi_run_control(None, None, 0, None, None, None, ctypes.c_int64(0), None, None, None,
+                                 ctypes.c_int64(0), None, None, None) == 1
     k = ctypes.c_int64()
     assert lib.eggmi_fas_cycle_kernels(None, ctypes.byref(k)) == 1
     assert b"session" in lib.eggmi_last_error()
