@@ -938,6 +938,90 @@ def bench_mlp(args):
             "roofline": None, "cpu_baseline": None}
 
 
+def _control_case(g, ctrl):
+    from egg_amd.smoothing import build_structured_context_from_block_maps, build_sweep_context
+    from egg_amd.smoothing import CppStructuredSweepSession
+    from egg_amd.smoothing.control import (boolean_sum_offset, boundary_ctrl_mask, build_control_wire, control_map,
+                                           fit_control_net)
+
+    shape = tuple(np.shape(g.blocks[0]))
+    ctx = build_sweep_context(g.blocks, g.free, g.d)
+    cm = control_map(shape, ctrl)
+    X0 = g.X.reshape(shape + (g.d,))
+    C0 = fit_control_net(cm, X0)
+    wire = build_control_wire(cm, C0, ~boundary_ctrl_mask(ctrl), boolean_sum_offset(cm, C0, X0))
+    s = CppStructuredSweepSession(ctx, build_structured_context_from_block_maps(g.d, g.blocks), g.X, control=wire)
+    return ctx, wire, s
+
+
+def bench_control(args):
+    """The control-net reduced Gauss-Newton smoother (run_control) on single
+    blocks whose optimum is not a uniform lattice: a 129 x 129 block under a
+    wavy top edge (interior perturbed by 0.005 N(0, 1), h = 0.031) with a
+    24 x 24 net, and a bulged 33^3 block with a 9^3 net; wall time to
+    convergence (max |G| < 1e-8) with the default inexact-Newton forcing.
+    CPU baseline and FP64 parity: the dense oracle loop on a 65 x 65 block
+    of the same generator with a 12 x 12 net, the device on the same
+    problem."""
+    from egg_amd import grids
+
+    runs = {}
+    g3 = grids.single_block_3d(33, seed=2, amp=0.15)
+    x, y, z = g3.X.T
+    g3.X = g3.X + np.stack([0 * x, 0 * y, 0.3 * np.sin(np.pi * x) * np.sin(np.pi * y) * z], -1)
+    for name, g, ctrl in (("2d_129_net24", grids.wavy_block_2d(129, perturb=0.005, seed=1), (24, 24)),
+                          ("3d_33_net9", g3, (9, 9, 9))):
+        ctx, wire, s = _control_case(g, ctrl)
+        s.run_control(1)  # warm the kernels
+        s.set_C(np.asarray(wire["C0"]).reshape(-1, g.d))
+        t0 = time.perf_counter()
+        r = s.run_control(30)
+        t = time.perf_counter() - t0
+        runs[name] = {"free_controls": int(len(wire["free_idx"])), "nodes": int(g.n_nodes), "s": t,
+                      "iters": int(r["iters"]), "converged": bool(r["converged"]),
+                      "pcg_iterations": [int(v) for v in r["pcg_iters"]],
+                      "energy_first_last": [float(r["energies"][0]), float(r["energies"][-1])],
+                      "final_min_det": float(r["mindets"][-1])}
+        log(f"control {name}: {runs[name]}")
+    cpu = parity = None
+    if not args.no_cpu:
+        from oracle import control as OC  # the checker: CPU baseline and parity reference
+
+        g = grids.wavy_block_2d(65, perturb=0.01, seed=1)
+        ctx, wire, s = _control_case(g, (12, 12))
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            Xo, Co, ro = OC.run_control(2, "barrier", g.X, ctx.energy_stencil, wire, g.blocks[0], n_outer=30)
+            times.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        rd = s.run_control(30, pcg_max_iter=1000, pcg_rtol=1e-12, pcg_forcing=False)
+        t_dev = time.perf_counter() - t0
+        cpu = {"value": float(np.median(times)), "unit": "s to converged", "cores": 1, "kind": "port",
+               "sample": f"65 x 65 block, 12 x 12 net ({len(wire['free_idx'])} free controls), dense FP64 oracle "
+                         f"loop ({ro['iters']} iterations), median of 3; the device on it: {t_dev:.4f} s"}
+        eo, ed = np.asarray(ro["energies"]), np.asarray(rd["energies"])
+        n = min(len(eo), len(ed))
+        parity = {"iters_oracle_device": [int(ro["iters"]), int(rd["iters"])],
+                  "max_rel_energy_err": float(np.max(np.abs(ed[:n] - eo[:n]) / np.abs(eo[:n]))),
+                  "energies_oracle_device": [[float(v) for v in eo], [float(v) for v in ed]],
+                  "max_abs_pos_err": float(np.abs(s.get_X() - Xo).max()),
+                  "max_abs_ctrl_err": float(np.abs(s.get_C() - Co).max()),
+                  "min_det_err": float(abs(rd["mindets"][-1] - ro["mindets"][-1])),
+                  "oracle": "dense FP64 reduced Gauss-Newton (oracle/control.py)"}
+        log(f"cpu {cpu}\nparity {parity}")
+    r2 = runs["2d_129_net24"]
+    return {
+        "metric": "control-net reduced Gauss-Newton: wall time to converged", "value": r2["s"], "unit": "s",
+        "n_gpus": 1, "steps": r2["iters"], "warmup": 1, "ms_per_step": r2["s"] / max(r2["iters"], 1) * 1e3,
+        "higher_is_better": False, "scaling": "strong", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "129x129 block on [0,4]^2 under a wavy top edge, interior perturbed (seed 1), "
+                               "24x24 cubic net, run_control(30) with inexact-Newton forcing; and a bulged 33^3 "
+                               "block, 9^3 net"},
+        "runs": runs, "cpu_baseline": cpu, "parity": parity, "roofline": None,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -947,7 +1031,7 @@ def main() -> int:
     ap.add_argument("--no-converge", action="store_true", help="skip the time-to-converged runs")
     ap.add_argument("--cpu-sweeps", type=int, default=150)
     ap.add_argument("--workload", default="ogrid", choices=("ogrid", "ogrid_bl", "ogrid_terms", "batched", "cubed_sphere",
-                                                              "mlp"))
+                                                              "mlp", "control"))
     ap.add_argument("--graph-chunk", type=int, default=GRAPH_CHUNK,
                     help="sweeps per captured hipGraph replay (even; 0 = single launches, e.g. under a tracer)")
     args = ap.parse_args()
@@ -957,7 +1041,7 @@ def main() -> int:
               file=sys.stderr)
         return 2
     fn = {"ogrid": bench_ogrid, "ogrid_bl": bench_ogrid_bl, "ogrid_terms": bench_ogrid_terms, "batched": bench_batched, "cubed_sphere": bench_cubed,
-          "mlp": bench_mlp}[args.workload]
+          "mlp": bench_mlp, "control": bench_control}[args.workload]
     print(json.dumps(fn(args)), flush=True)
     return 0
 

@@ -228,6 +228,30 @@ __global__ __launch_bounds__(kThreads) void ctrl_samples_kernel(const CtrlGeom g
         }
 }
 
+// z_p = H_p dA_p per sample, dA_p from the block-local direction field
+template <int D>
+__global__ __launch_bounds__(kThreads) void ctrl_sample_hv_kernel(const CtrlGeom g, const double* hs,
+                                                                  const double* dir, double* zs)
+{
+    constexpr int NC = 1 << D, V = kV<D>;
+    const long long p = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (p >= g.n_cells * NC) return;
+    const long long cell = p / NC;
+    const int b = int(p - cell * NC);
+    int I0[3];
+    cell_origin<D>(g, cell, I0);
+    double dA[V];
+    sample_A<D>(g, I0, b, dir, nullptr, 0.0, dA);
+    const double* hp = hs + p * kNHs<D>;
+#pragma unroll
+    for (int q = 0; q < V; ++q) {
+        double y = 0.0;
+#pragma unroll
+        for (int r = 0; r < V; ++r) y += hp[sym_index<D>(q, r)] * dA[r];
+        zs[p * V + q] = y;
+    }
+}
+
 // FP64 cell energy and min det of the block at two step lengths: each
 // workgroup walks a contiguous run of samples with stride kThreads, then a
 // fixed tree per workgroup and a one-workgroup fold over the partials
@@ -344,32 +368,6 @@ __global__ __launch_bounds__(kThreads) void ctrl_node_kernel(const CtrlGeom g, c
                 for (int i = 0; i < D; ++i)
 #pragma unroll
                     for (int k = 0; k < D; ++k) acc[i] += u[k] * gp[i * D + k];
-            } else if constexpr (MODE == 1) {
-                // dA of the sample from the direction field, then H dA
-                int Ic[3];
-                corner_of<D>(I0, b, Ic);
-                const long long nc = node_local<D>(g, Ic);
-                double dA[V];
-#pragma unroll
-                for (int k = 0; k < D; ++k) {
-                    int In[3];
-                    corner_of<D>(I0, b ^ (1 << (D - 1 - k)), In);
-                    const long long nn = node_local<D>(g, In);
-                    const double sk = ((b >> (D - 1 - k)) & 1) ? -1.0 : 1.0;
-#pragma unroll
-                    for (int i = 0; i < D; ++i) dA[i * D + k] = sk * (dir[nn * D + i] - dir[nc * D + i]);
-                }
-                const double* hp = hs + p * kNHs<D>;
-#pragma unroll
-                for (int i = 0; i < D; ++i)
-#pragma unroll
-                    for (int k = 0; k < D; ++k) {
-                        if (u[k] == 0.0) continue;
-                        double y = 0.0;
-#pragma unroll
-                        for (int r = 0; r < V; ++r) y += hp[sym_index<D>(i * D + k, r)] * dA[r];
-                        acc[i] += u[k] * y;
-                    }
             } else {
                 const double* hp = hs + p * kNHs<D>;
 #pragma unroll
@@ -393,41 +391,50 @@ __global__ __launch_bounds__(kThreads) void ctrl_node_kernel(const CtrlGeom g, c
 
 // ---------------------------------------------------------------- pullback
 
+// one workgroup per free control: its threads split the tensor product of
+// the per-axis support lists, then fold in a fixed tree
 template <int D, int MODE>
 __global__ __launch_bounds__(kThreads) void ctrl_pullback_kernel(const CtrlGeom g, const double* y, double* out)
 {
     constexpr int NO = MODE == 1 ? D * D : D;
-    const int f = blockIdx.x * kThreads + threadIdx.x;
-    if (f >= g.n_free) return;
+    const int f = blockIdx.x;
     long long rem = g.free_idx[f];
     int a[3] = {0, 0, 0};
     for (int k = D - 1; k >= 0; --k) {
         a[k] = int(rem % g.c[k]);
         rem /= g.c[k];
     }
-    double acc[NO] = {};
-    for (int q0 = g.sup_off[0][a[0]]; q0 < g.sup_off[0][a[0] + 1]; ++q0) {
-        const int i0 = g.sup_node[0][q0];
-        const double w0 = g.sup_w[0][q0];
-        for (int q1 = g.sup_off[1][a[1]]; q1 < g.sup_off[1][a[1] + 1]; ++q1) {
-            const int i1 = g.sup_node[1][q1];
-            const double w1 = w0 * g.sup_w[1][q1];
-            const int q2e = D == 3 ? g.sup_off[2][a[2] + 1] : 1;
-            for (int q2 = D == 3 ? g.sup_off[2][a[2]] : 0; q2 < q2e; ++q2) {
-                double w = w1;
-                long long n = (long long)i0 * g.n[1] + i1;
-                if constexpr (D == 3) {
-                    w *= g.sup_w[2][q2];
-                    n = n * g.n[2] + g.sup_node[2][q2];
-                }
-                const double ww = MODE == 1 ? w * w : w;
+    int lo[3] = {0, 0, 0}, len[3] = {1, 1, 1};
 #pragma unroll
-                for (int c = 0; c < NO; ++c) acc[c] += ww * y[n * NO + c];
-            }
-        }
+    for (int k = 0; k < D; ++k) {
+        lo[k] = g.sup_off[k][a[k]];
+        len[k] = g.sup_off[k][a[k] + 1] - lo[k];
     }
+    const int total = len[0] * len[1] * len[2];
+    double acc[NO] = {};
+    for (int t = threadIdx.x; t < total; t += kThreads) {
+        const int q2 = t % len[2], q1 = (t / len[2]) % len[1], q0 = t / (len[2] * len[1]);
+        double w = g.sup_w[0][lo[0] + q0] * g.sup_w[1][lo[1] + q1];
+        long long n = (long long)g.sup_node[0][lo[0] + q0] * g.n[1] + g.sup_node[1][lo[1] + q1];
+        if constexpr (D == 3) {
+            w *= g.sup_w[2][lo[2] + q2];
+            n = n * g.n[2] + g.sup_node[2][lo[2] + q2];
+        }
+        const double ww = MODE == 1 ? w * w : w;
 #pragma unroll
-    for (int c = 0; c < NO; ++c) out[(long long)f * NO + c] = acc[c];
+        for (int c = 0; c < NO; ++c) acc[c] += ww * y[n * NO + c];
+    }
+    __shared__ double sh[NO][kThreads];
+#pragma unroll
+    for (int c = 0; c < NO; ++c) sh[c][threadIdx.x] = acc[c];
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+#pragma unroll
+            for (int c = 0; c < NO; ++c) sh[c][threadIdx.x] += sh[c][threadIdx.x + w];
+        __syncthreads();
+    }
+    if ((int)threadIdx.x < NO) out[(long long)f * NO + threadIdx.x] = sh[threadIdx.x][0];
 }
 
 // ---------------------------------------------------------------- vectors
@@ -576,6 +583,16 @@ void launch_ctrl_samples(const CtrlGeom& g, int kind, const double* x, double* g
     }
 }
 
+void launch_ctrl_sample_hv(const CtrlGeom& g, const double* hs, const double* dir, double* zs, hipStream_t s)
+{
+    const long long n = g.n_cells * (1LL << g.dim);
+    if (n <= 0) return;
+    if (g.dim == 2)
+        ctrl_sample_hv_kernel<2><<<blocks_for(n), kThreads, 0, s>>>(g, hs, dir, zs);
+    else
+        ctrl_sample_hv_kernel<3><<<blocks_for(n), kThreads, 0, s>>>(g, hs, dir, zs);
+}
+
 void launch_ctrl_node(const CtrlGeom& g, int mode, const double* gs, const double* hs, const double* dir,
                       double* out, hipStream_t s)
 {
@@ -585,15 +602,11 @@ void launch_ctrl_node(const CtrlGeom& g, int mode, const double* gs, const doubl
     if (g.dim == 2) {
         if (mode == 0)
             EGGMI_CN(2, 0);
-        else if (mode == 1)
-            EGGMI_CN(2, 1);
         else
             EGGMI_CN(2, 2);
     } else {
         if (mode == 0)
             EGGMI_CN(3, 0);
-        else if (mode == 1)
-            EGGMI_CN(3, 1);
         else
             EGGMI_CN(3, 2);
     }
@@ -603,7 +616,7 @@ void launch_ctrl_node(const CtrlGeom& g, int mode, const double* gs, const doubl
 void launch_ctrl_pullback(const CtrlGeom& g, int mode, const double* y, double* out, hipStream_t s)
 {
     if (g.n_free <= 0) return;
-    const unsigned nb = blocks_for(g.n_free);
+    const unsigned nb = (unsigned)g.n_free;  // one workgroup per free control
     if (g.dim == 2) {
         if (mode == 0)
             ctrl_pullback_kernel<2, 0><<<nb, kThreads, 0, s>>>(g, y, out);

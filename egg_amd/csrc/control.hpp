@@ -68,9 +68,13 @@ void launch_ctrl_samples(const CtrlGeom& g, int kind, const double* xb, double* 
 constexpr int kCtrlParts = 256;
 void launch_ctrl_energy(const CtrlGeom& g, int kind, const double* xb, const double* dir, double a0, double a1,
                         double* partials, double* out, hipStream_t s);
+// Per sample: zs (dim^2) = H_p dA_p, dA_p the sample's A of the direction
+// field dir (block-local node order).
+void launch_ctrl_sample_hv(const CtrlGeom& g, const double* hs, const double* dir, double* zs, hipStream_t s);
 // Per block node (block-local order), gathered over its incident samples:
-// mode 0: the energy gradient (dim) from gs; mode 1: the Hessian action
-// (dim) on the node field dir; mode 2: the contracted node Hessian (dim^2).
+// mode 0: sum over samples of the node's column weights times gs (the
+// energy gradient for the metric gradients, the Hessian action for zs);
+// mode 2: the contracted node Hessian (dim^2) from hs.
 void launch_ctrl_node(const CtrlGeom& g, int mode, const double* gs, const double* hs, const double* dir,
                       double* out, hipStream_t s);
 // Per free control: mode 0 out (dim) = sum_nodes w * y; mode 1 out (dim^2) =

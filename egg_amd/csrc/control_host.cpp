@@ -95,6 +95,7 @@ void Session::set_control(const eggmi_control& c)
     const long long V = dim * dim, NH = V * (V + 1) / 2, ns = g.n_cells * (1LL << dim);
     const size_t nfd = (size_t)g.n_free * dim;
     net->gs.alloc(ns * V);
+    net->zs.alloc(ns * V);
     net->hs.alloc(ns * NH);
     net->xb.alloc(g.n_nodes * dim);
     net->ynode.alloc(g.n_nodes * dim);
@@ -185,7 +186,8 @@ ControlReport Session::run_control(const ControlOptions& o, int kind)
     // action gathered per node, pulled back to the free controls
     auto matvec = [&](double lam, const double* v, double* out) {
         launch_ctrl_eval_dir(g, v, c.dir.p, nullptr, stream);
-        launch_ctrl_node(g, 1, c.gs.p, c.hs.p, c.dir.p, c.ynode.p, stream);
+        launch_ctrl_sample_hv(g, c.hs.p, c.dir.p, c.zs.p, stream);
+        launch_ctrl_node(g, 0, c.zs.p, nullptr, nullptr, c.ynode.p, stream);
         launch_ctrl_pullback(g, 0, c.ynode.p, out, stream);
         if (lam > 0.0) launch_ctrl_axpy(nfd, lam, v, out, false, stream);
     };

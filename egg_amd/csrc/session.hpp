@@ -238,7 +238,7 @@ struct ControlNet {
     DArr<double> wt[3], sup_w[3], C, b;
     // per sample: metric gradient and symmetric Hessian; block-local node
     // fields (positions, step, gathers); free-control vectors; scalars
-    DArr<double> gs, hs, xb, ynode, dir, hnode, prec, G, dC, r, z, p, Ap, scal, eparts;
+    DArr<double> gs, zs, hs, xb, ynode, dir, hnode, prec, G, dC, r, z, p, Ap, scal, eparts;
     // the other blocks' cells (their energy and min det, constant during a run)
     DArr<BlockDesc> oblocks;
     DArr<long long> ocell_off;

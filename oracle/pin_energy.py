@@ -12,6 +12,9 @@ records how it was reached.
 
   python -m oracle.pin_energy --workload ogrid129 [--threads 8]
 
+The converged FP64 state is written next to it (xstar_<workload>.npy), the
+reference positions the benchmark's converged results are compared with.
+
 For the 129-node O-grid the plain method needs ~10^5 sweeps, hours of FP64
 CPU time from the untangled state; the pinned fixture was made from a warm
 start instead (same basin, the FP64 sweeps then settle the last digits):
@@ -55,6 +58,8 @@ def main():
     ap.add_argument("--start", default=None,
                     help=".npy of a near-converged state (global order) to start the FP64 sweeps from, "
                          "instead of untangling from the generated grid")
+    ap.add_argument("--x-only", action="store_true",
+                    help="write only the converged FP64 state (tests/golden/xstar_<workload>.npy), keep the E* fixture")
     ap.add_argument("--gpu-warm-start", default=None, metavar="NPY",
                     help="(on a GPU machine) write a warm start to NPY and stop: the device untangle, "
                          "300 FAS V(4,4) cycles and 20000 plain sweeps at omega 0.8; then rerun with --start NPY")
@@ -120,6 +125,10 @@ def main():
                 (f"the near-converged state {Path(args.start).name}" if args.start else
                  "the delta-continuation untangle of the generated grid")),
     }
+    np.save(ROOT / "tests" / "golden" / f"xstar_{args.workload}.npy", X)
+    print("wrote the converged state", f"xstar_{args.workload}.npy")
+    if args.x_only:
+        return
     dst = ROOT / "tests" / "golden" / f"estar_{args.workload}.json"
     dst.write_text(json.dumps(out, indent=1))
     print("wrote", dst)

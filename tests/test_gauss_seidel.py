@@ -61,3 +61,30 @@ def test_device_converges_to_the_gauss_seidel_minimiser(oracle):
     X = s.get_X()
     assert abs(e[-1] - eg) / eg < 2e-6 and m[-1] > 0
     assert np.abs(X - Xg).max() / 4.0 < 1e-5
+
+
+def test_pinned_converged_state_is_a_fixed_point(oracle):
+    """The FP64 converged state X* of the 129-node O-grid
+    (tests/golden/xstar_ogrid129.npy, oracle/pin_energy.py) carries the
+    pinned energy E* and is a fixed point of the FP64 sweep: one more
+    oracle sweep moves no node by more than 1e-9 of the extent and lowers
+    the energy by less than 1e-12 relative; every cell is valid."""
+    import json
+    from pathlib import Path
+
+    from egg_amd.smoothing import build_sweep_context
+
+    gold = Path(__file__).parent / "golden"
+    X = np.load(gold / "xstar_ogrid129.npy", allow_pickle=False)
+    e_star = json.loads((gold / "estar_ogrid129.json").read_text())["E_star"]
+    g = grids.tangle(grids.ogrid_2d(129, 65), frac=0.02, factor=1.5, seed=2026)
+    assert X.shape == g.X.shape
+    fixed = ~g.free
+    np.testing.assert_array_equal(X[fixed], g.X[fixed])  # the boundary never moved
+    ctx = build_sweep_context(g.blocks, g.free, 2)
+    e, md = oracle.energy_mindet(2, "barrier", 0.0, ctx.energy_stencil, X)
+    assert md > 0 and abs(e - e_star) <= 1e-9 * e_star
+    X1, e1, _ = oracle.jacobi(2, "barrier", 0.0, 0.8, 1, X, oracle.materialize_patches(ctx.wire, g.blocks, 2),
+                              ctx.energy_stencil)
+    assert np.abs(X1 - X).max() <= 1e-9 * 4.0
+    assert e - e1[-1] <= 1e-12 * e
