@@ -938,6 +938,50 @@ def bench_mlp(args):
             "roofline": None, "cpu_baseline": None}
 
 
+def bench_fas(args):
+    """FAS on the config-2 O-grid (129/65, 2 % tangled, seed 2026) alone:
+    per-cycle wall time and kernel count of V(8,8) on 4 levels (16 coarsest
+    sweeps) and V(4,4) on 3 levels from the untangled state, and the time to
+    |E - E*| <= 1e-6 E*, min det > 0 (E* pinned in
+    tests/golden/estar_ogrid129.json). Cycles replay as captured graphs unless
+    --graph-chunk 0 (single launches, e.g. under a kernel tracer)."""
+    from egg_amd import grids
+    from egg_amd.smoothing import cpp_untangle
+
+    g = grids.tangle(grids.ogrid_2d(129, 65), frac=0.02, factor=1.5, seed=SEED)
+    ctx, bsc, s = _build(g)
+    Xu, md_u, outer, _ = cpp_untangle(ctx, bsc, g.X, session=s)
+    s.core.autotune(20, 0.8)
+    lv_best, _ = s.core.autotune_levels(16, 0.8)
+    s.core.set_graph_chunk(args.graph_chunk)
+    e_star = json.loads((ROOT / "tests" / "golden" / "estar_ogrid129.json").read_text())["E_star"]
+    K = max(1, min(args.steps, 100))
+    out = {}
+    for name, nu, lev, nuc in (("v88_4_levels_coarse16", 8, 4, 16), ("v44_3_levels", 4, 3, 32)):
+        s.set_X(Xu)
+        s.run_fas(2, nu_pre=nu, nu_post=nu, nu_coarse=nuc, omega=0.8, max_levels=lev)  # warm-up (and capture)
+        s.set_X(Xu)
+        t0 = time.perf_counter()
+        s.run_fas(K, nu_pre=nu, nu_post=nu, nu_coarse=nuc, omega=0.8, max_levels=lev)
+        t = time.perf_counter() - t0
+        kernels = int(s.core.fas_cycle_kernels())
+        s.set_X(Xu)
+        n, tc, E = _converge(lambda: (*s.run_fas(5, nu_pre=nu, nu_post=nu, nu_coarse=nuc, omega=0.8,
+                                                  max_levels=lev), 5), e_star, 60.0)
+        out[name] = {"ms_per_cycle": t / K * 1e3, "kernels_per_cycle": kernels or None, "cycles_to_converged": n,
+                     "s_to_converged": tc if n else None, "E_end": E}
+        log(f"FAS {name}: {out[name]}")
+    r = out["v88_4_levels_coarse16"]
+    return {
+        "metric": "FAS V-cycle wall time (O-grid 129/65)", "value": r["ms_per_cycle"], "unit": "ms per V(8,8) cycle",
+        "n_gpus": 1, "steps": K, "warmup": 2, "ms_per_step": r["ms_per_cycle"], "higher_is_better": False,
+        "scaling": "strong", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "2D 5-block O-grid 129/65, 2% tangled by 1.5h, seed 2026, untangled; FAS cycles",
+                   "graph_chunk": args.graph_chunk, "fas_level_variants": [int(v) for v in lv_best]},
+        "cycles": out, "roofline": None, "cpu_baseline": None,
+    }
+
+
 def _control_case(g, ctrl):
     from egg_amd.smoothing import build_structured_context_from_block_maps, build_sweep_context
     from egg_amd.smoothing import CppStructuredSweepSession
@@ -1031,7 +1075,7 @@ def main() -> int:
     ap.add_argument("--no-converge", action="store_true", help="skip the time-to-converged runs")
     ap.add_argument("--cpu-sweeps", type=int, default=150)
     ap.add_argument("--workload", default="ogrid", choices=("ogrid", "ogrid_bl", "ogrid_terms", "batched", "cubed_sphere",
-                                                              "mlp", "control"))
+                                                              "mlp", "control", "fas"))
     ap.add_argument("--graph-chunk", type=int, default=GRAPH_CHUNK,
                     help="sweeps per captured hipGraph replay (even; 0 = single launches, e.g. under a tracer)")
     args = ap.parse_args()
@@ -1041,7 +1085,7 @@ def main() -> int:
               file=sys.stderr)
         return 2
     fn = {"ogrid": bench_ogrid, "ogrid_bl": bench_ogrid_bl, "ogrid_terms": bench_ogrid_terms, "batched": bench_batched, "cubed_sphere": bench_cubed,
-          "mlp": bench_mlp, "control": bench_control}[args.workload]
+          "mlp": bench_mlp, "control": bench_control, "fas": bench_fas}[args.workload]
     print(json.dumps(fn(args)), flush=True)
     return 0
 

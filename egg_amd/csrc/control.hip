@@ -212,8 +212,8 @@ __global__ __launch_bounds__(kThreads) void ctrl_samples_kernel(const CtrlGeom g
     cell_origin<D>(g, cell, I0);
     double t[V];
     sample_A<D>(g, I0, b, xb, nullptr, 0.0, t);
-    double* go = gs + p * V;
-    double* ho = hs + p * kNHs<D>;
+    // component-major (entry e of sample p at e * ns + p): coalesced
+    const long long ns = g.n_cells * NC;
     int e = 0;
     for (int q = 0; q < V; ++q)
         for (int r = q; r < V; ++r) {
@@ -223,8 +223,8 @@ __global__ __launch_bounds__(kThreads) void ctrl_samples_kernel(const CtrlGeom g
             T[q].a = 1.0;
             T[r].b = 1.0;
             const HD m = metric_value<D, KIND>(T);
-            ho[e++] = m.ab;
-            if (q == r) go[q] = m.a;
+            hs[(long long)(e++) * ns + p] = m.ab;
+            if (q == r) gs[(long long)q * ns + p] = m.a;
         }
 }
 
@@ -242,13 +242,16 @@ __global__ __launch_bounds__(kThreads) void ctrl_sample_hv_kernel(const CtrlGeom
     cell_origin<D>(g, cell, I0);
     double dA[V];
     sample_A<D>(g, I0, b, dir, nullptr, 0.0, dA);
-    const double* hp = hs + p * kNHs<D>;
+    const long long ns = g.n_cells * NC;
+    double H[kNHs<D>];
+#pragma unroll
+    for (int e = 0; e < kNHs<D>; ++e) H[e] = hs[(long long)e * ns + p];
 #pragma unroll
     for (int q = 0; q < V; ++q) {
         double y = 0.0;
 #pragma unroll
-        for (int r = 0; r < V; ++r) y += hp[sym_index<D>(q, r)] * dA[r];
-        zs[p * V + q] = y;
+        for (int r = 0; r < V; ++r) y += H[sym_index<D>(q, r)] * dA[r];
+        zs[(long long)q * ns + p] = y;
     }
 }
 
@@ -307,18 +310,24 @@ __global__ __launch_bounds__(kThreads) void ctrl_energy_kernel(const CtrlGeom g,
 
 __global__ __launch_bounds__(kThreads) void ctrl_energy_finish(const double* partials, int n, double* out)
 {
-    if (threadIdx.x != 0) return;
-    double e0 = 0.0, e1 = 0.0, m0 = __builtin_huge_val(), m1 = __builtin_huge_val();
-    for (int i = 0; i < n; ++i) {
-        e0 += partials[4 * i];
-        m0 = fmin(m0, partials[4 * i + 1]);
-        e1 += partials[4 * i + 2];
-        m1 = fmin(m1, partials[4 * i + 3]);
+    // n <= kCtrlParts = kThreads: one partial per thread, a fixed tree
+    __shared__ double sh[4][kThreads];
+    const bool in = (int)threadIdx.x < n;
+    sh[0][threadIdx.x] = in ? partials[4 * threadIdx.x] : 0.0;
+    sh[1][threadIdx.x] = in ? partials[4 * threadIdx.x + 1] : __builtin_huge_val();
+    sh[2][threadIdx.x] = in ? partials[4 * threadIdx.x + 2] : 0.0;
+    sh[3][threadIdx.x] = in ? partials[4 * threadIdx.x + 3] : __builtin_huge_val();
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + w];
+            sh[1][threadIdx.x] = fmin(sh[1][threadIdx.x], sh[1][threadIdx.x + w]);
+            sh[2][threadIdx.x] += sh[2][threadIdx.x + w];
+            sh[3][threadIdx.x] = fmin(sh[3][threadIdx.x], sh[3][threadIdx.x + w]);
+        }
+        __syncthreads();
     }
-    out[0] = e0;
-    out[1] = m0;
-    out[2] = e1;
-    out[3] = m1;
+    if (threadIdx.x < 4) out[threadIdx.x] = sh[threadIdx.x][0];
 }
 
 // ---------------------------------------------------------------- node gathers
@@ -332,6 +341,7 @@ __global__ __launch_bounds__(kThreads) void ctrl_node_kernel(const CtrlGeom g, c
     if (n >= g.n_nodes) return;
     int I[3];
     unravel_node<D>(g, n, I);
+    const long long ns = g.n_cells * NC;
     double acc[MODE == 2 ? V : D] = {};
     // incident cells: origin I - o for o in {0, 1}^D, the node at corner o
     for (int o = 0; o < NC; ++o) {
@@ -363,13 +373,12 @@ __global__ __launch_bounds__(kThreads) void ctrl_node_kernel(const CtrlGeom g, c
             }
             const long long p = cell * NC + b;
             if constexpr (MODE == 0) {
-                const double* gp = gs + p * V;
 #pragma unroll
                 for (int i = 0; i < D; ++i)
 #pragma unroll
-                    for (int k = 0; k < D; ++k) acc[i] += u[k] * gp[i * D + k];
+                    for (int k = 0; k < D; ++k) acc[i] += u[k] * gs[(long long)(i * D + k) * ns + p];
             } else {
-                const double* hp = hs + p * kNHs<D>;
+
 #pragma unroll
                 for (int i = 0; i < D; ++i)
 #pragma unroll
@@ -378,7 +387,8 @@ __global__ __launch_bounds__(kThreads) void ctrl_node_kernel(const CtrlGeom g, c
 #pragma unroll
                         for (int k = 0; k < D; ++k)
 #pragma unroll
-                            for (int l = 0; l < D; ++l) h += u[k] * u[l] * hp[sym_index<D>(i * D + k, j * D + l)];
+                            for (int l = 0; l < D; ++l)
+                                h += u[k] * u[l] * hs[(long long)sym_index<D>(i * D + k, j * D + l) * ns + p];
                         acc[i * D + j] += h;
                     }
             }
