@@ -36,6 +36,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <vector>
 
 namespace eggmi
 {
@@ -46,7 +47,7 @@ namespace
 struct PcgState {
     double rtol2, b2, stop2;
     double rz[2];     // standard: r.z entering iteration t, slot t & 1
-    double rr[2];     // standard: r.r entering iteration t (b.b at t = 0)
+    double rr[2];     // r.r entering iteration t (b.b at t = 0), slot t & 1
     double alpha[2];  // one-reduction: alpha_t, slot t & 1
     double gamma[2];  // one-reduction: (r, u) entering iteration t
     int t[2];         // iteration index of the last iteration kernel of each parity
@@ -57,11 +58,44 @@ struct PcgState {
     double rr_end;
 };
 
+// A DPP lane move of a double (both halves through the same control; every
+// source lane of the controls used here lies in the row, so `old` is never
+// taken).
+template <int CTRL> __device__ __forceinline__ double dpp_mov(double v)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+__device__ __forceinline__ double read_lane(double v, int lane)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, lane);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// 64-lane sum in register moves (no LDS round trips). Each DPP step adds to
+// every lane the value of a partner lane holding the same partial as the
+// other members of its group, and a + b == b + a exactly, so after the four
+// steps all 16 lanes of a row hold the same row sum; the four row sums are
+// then combined in one fixed order. Every lane, and every workgroup folding
+// the same inputs, receives the same bits.
+__device__ __forceinline__ double wave_sum(double v)
+{
+    v += dpp_mov<0xB1>(v);   // quad_perm [1,0,3,2]: lane ^ 1
+    v += dpp_mov<0x4E>(v);   // quad_perm [2,3,0,1]: lane ^ 2
+    v += dpp_mov<0x141>(v);  // row_half_mirror: the other quad of the eight
+    v += dpp_mov<0x140>(v);  // row_mirror: the other eight of the row
+    return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
+}
+
 // 256-thread workgroup sums; every thread receives the same bits.
 __device__ __forceinline__ double wg_sum(double v)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    v = wave_sum(v);
     __shared__ double red[kPcgThreads / 64];
     __syncthreads();  // the previous call's readers are done with red
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
@@ -72,9 +106,7 @@ __device__ __forceinline__ double wg_sum(double v)
 template <int K> __device__ __forceinline__ void wg_sums(double (&v)[K])
 {
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1)
-#pragma unroll
-        for (int k = 0; k < K; ++k) v[k] += __shfl_xor(v[k], o, 64);
+    for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
     __shared__ double red[K][kPcgThreads / 64];
     __syncthreads();
     if ((threadIdx.x & 63) == 0)
@@ -102,7 +134,9 @@ __device__ __forceinline__ double* part_of(double* part, long long np, int par, 
 
 struct CsrOp {
     CsrView A;
+    static constexpr bool kDiagTable = true;  // M^-1 varies by row: kept in dinv
     __device__ long long n() const { return A.n; }
+    __device__ double dinv_at(const double* dinv, long long i) const { return dinv[i]; }
     __device__ double inv_diag(long long i) const
     {
         double d = 0.0;
@@ -145,8 +179,10 @@ struct CsrOp {
 // coordinates come from 32-bit divisions, once per row.
 struct LatticeOp {
     LatticeView A;
+    static constexpr bool kDiagTable = false;  // M^-1 = 1 / diag on every row
     __device__ long long n() const { return A.n; }
     __device__ double inv_diag(long long) const { return 1.0 / A.diag; }
+    __device__ double dinv_at(const double*, long long) const { return 1.0 / A.diag; }
     // the in-lattice neighbours of row i: their count and indices
     __device__ int neighbours(long long i, int (&nb)[6]) const
     {
@@ -202,6 +238,7 @@ __device__ __forceinline__ void finish(PcgState* st, int status, int iters, doub
 __device__ __forceinline__ void reset_state(PcgState* st, double rtol2, int max_iter)
 {
     st->rtol2 = rtol2;
+    st->t[0] = -1;  // no iteration kernel of this solve has run yet
     st->t[1] = -1;
     st->halt_a[1] = 0;
     st->halt_b[1] = 0;
@@ -381,7 +418,7 @@ __global__ __launch_bounds__(kPcgThreads) void cg1_start(const Op op, const doub
     op.row3(ii, b, b, b, dinv, au, adw, ads);
     double v[3] = {0.0, 0.0, 0.0};
     if (i < n) {
-        const double bi = b[i], ui = dinv[i] * bi;
+        const double bi = b[i], ui = op.dinv_at(dinv, i) * bi;
         x[i] = 0.0;
         r[i] = bi;
         p[i] = 0.0;
@@ -422,7 +459,7 @@ __global__ __launch_bounds__(kPcgThreads) void cg1_iter(const Op op, double* x, 
     // A u, A D w and A D s at the iteration-t values: A u_(t+1) = A u - alpha (A D w + beta A D s)
     double au, adw, ads;
     op.row3(ii, u_old, w_old, s_old, dinv, au, adw, ads);
-    const double xi = x[ii], ri = r[ii], pi = p[ii], ui = u_old[ii], wi = w_old[ii], si = s_old[ii], di = dinv[ii];
+    const double xi = x[ii], ri = r[ii], pi = p[ii], ui = u_old[ii], wi = w_old[ii], si = s_old[ii], di = op.dinv_at(dinv, ii);
     if (halted) {
         if (lead) st->halt_a[par] = 1;
         return;
@@ -483,6 +520,7 @@ __global__ __launch_bounds__(kPcgThreads) void cg1_iter(const Op op, double* x, 
         for (int k = 0; k < 3; ++k) part_of(part, np, par, k)[blockIdx.x] = o[k];
     if (lead) {
         st->t[par] = t;
+        st->rr[par] = rr;  // r.r entering iteration t (the host sizes the last chunks from it)
         st->alpha[par] = alpha;
         st->gamma[par] = gamma;
         if (t == 0) {
@@ -507,7 +545,7 @@ template <class Op> struct Launcher {
         if (rec == PcgRecurrence::kStandard) {
             pcg_start<Op><<<grid, kPcgThreads, 0, s>>>(op, b, x, w.r, w.u[0], w.dinv, w.part, st(), rtol2, max_iter);
         } else {
-            cg1_diag<Op><<<grid, kPcgThreads, 0, s>>>(op, w.dinv);
+            if (Op::kDiagTable) cg1_diag<Op><<<grid, kPcgThreads, 0, s>>>(op, w.dinv);
             cg1_start<Op><<<grid, kPcgThreads, 0, s>>>(op, b, x, w.r, w.dinv, w.p, w.u[1], w.w[1], w.s[1], w.part,
                                                        st(), rtol2, max_iter);
         }
@@ -529,6 +567,39 @@ template <class Op> struct Launcher {
     }
 };
 
+// The queued driver's pinned read-back slots (two states) and poll events:
+// made once per device and host thread on first use and kept until the
+// process ends (a solve would otherwise pay a pinned allocation, two event
+// creations and their release, tens of microseconds, on every call). A
+// solve returns only once its stream has drained, so solves made one after
+// another on a thread share them safely.
+struct Poller {
+    PcgState* pin = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+Poller& poller()
+{
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "pcg device");
+    thread_local std::vector<Poller*> by_dev;
+    if ((int)by_dev.size() <= dev) by_dev.resize(dev + 1, nullptr);
+    if (!by_dev[dev]) {
+        Poller p;
+        try {
+            hip_check(hipHostMalloc((void**)&p.pin, 2 * sizeof(PcgState), hipHostMallocDefault), "pcg pinned status");
+            for (auto& e : p.ev) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "pcg event");
+        } catch (...) {
+            for (auto e : p.ev)
+                if (e) (void)hipEventDestroy(e);
+            if (p.pin) (void)hipHostFree(p.pin);
+            throw;
+        }
+        by_dev[dev] = new Poller(p);
+    }
+    return *by_dev[dev];
+}
+
 template <class Op>
 PcgResult solve(const Op& op, long long n, const double* b, double* x, double rtol, int max_iter, PcgWork& w,
                 hipStream_t s, PcgLaunch mode, PcgRecurrence rec)
@@ -542,29 +613,31 @@ PcgResult solve(const Op& op, long long n, const double* b, double* x, double rt
     PcgState* st = L.st();
     // iterations are launched kernel by kernel, kPcgChunk at a time
     // (parities 0, 1, 0, ...); after each chunk the host reads the device
-    // status: synchronously when nothing may be allocated, else into pinned
-    // memory with one chunk of look-ahead (chunk k's status is read once
-    // chunk k + 1 is queued, so the device never idles on the poll; a solve
-    // that ended runs at most two chunks of iteration kernels that exit at
-    // once). Replaying the chunks as captured hipGraphs measured slower on
-    // the MI355X (33^3 lattice, 126 iterations: 1.27 against 1.09 ms).
-    int* pin = nullptr;
-    hipEvent_t ev[2] = {nullptr, nullptr};
-    auto release = [&] {
-        for (auto e : ev)
-            if (e) (void)hipEventDestroy(e);
-        if (pin) (void)hipHostFree(pin);
-    };
+    // state: synchronously when nothing may be allocated, else into pinned
+    // memory with one chunk of look-ahead (chunk k's state is read once
+    // chunk k + 1 is queued, so the device never idles on the poll). Kernels
+    // launched after the solve ended exit at once but still cost a launch,
+    // so once two polls show the residual's decay rate, the next chunk is
+    // cut to the iterations the rate predicts are left (plus two), the
+    // queued ones counted. Replaying the chunks as captured hipGraphs
+    // measured slower on the MI355X (33^3 lattice, 126 iterations: 1.27
+    // against 1.09 ms).
+    PcgState* pin = nullptr;
+    hipEvent_t* ev = nullptr;
+    if (mode == PcgLaunch::kQueued) {
+        Poller& P = poller();
+        pin = P.pin;
+        ev = P.ev;
+    }
     bool halted = false;
-    try {
-        if (mode == PcgLaunch::kQueued) {
-            hip_check(hipHostMalloc((void**)&pin, 2 * sizeof(int), hipHostMallocDefault), "pcg pinned status");
-            for (auto& e : ev) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "pcg event");
-        }
+    PcgState h {};
+    {
         L.start(b, rtol * rtol, max_iter, s);
-        int t = 0, k = 0, status = -1;
+        int t = 0, k = 0, status = -1, m = kPcgChunk;
+        int t_a = -1;  // the last two polls' newest iteration and its r.r
+        double rr_a = 0.0;
         while (!halted && t < max_iter) {
-            const int m = std::min(kPcgChunk, max_iter - t);
+            m = std::min(m, max_iter - t);
             for (int c = 0; c < m; ++c) L.iteration((t + c) & 1, s);
             t += m;
             if (mode == PcgLaunch::kNoAlloc) {
@@ -573,26 +646,44 @@ PcgResult solve(const Op& op, long long n, const double* b, double* x, double rt
                 halted = status >= 0;
                 continue;
             }
-            hip_check(hipMemcpyAsync(&pin[k & 1], &st->status, sizeof(int), hipMemcpyDeviceToHost, s), "pcg poll");
+            hip_check(hipMemcpyAsync(&pin[k & 1], st, sizeof(PcgState), hipMemcpyDeviceToHost, s), "pcg poll");
             hip_check(hipEventRecord(ev[k & 1], s), "pcg poll");
+            m = kPcgChunk;
             if (k > 0) {
                 hip_check(hipEventSynchronize(ev[(k - 1) & 1]), "pcg poll");
-                halted = pin[(k - 1) & 1] >= 0;
+                h = pin[(k - 1) & 1];
+                halted = h.status >= 0;
+                const int tb = std::max(h.t[0], h.t[1]);
+                const double rr_b = tb >= 0 ? h.rr[tb & 1] : 0.0;
+                if (!halted && t_a >= 0 && tb > t_a && rr_b > 0.0 && rr_b < rr_a && h.stop2 > 0.0) {
+                    const double rate = std::log(rr_b / rr_a) / (double)(tb - t_a);  // < 0
+                    const double left = std::log(h.stop2 / rr_b) / rate;  // iterations after tb
+                    const int queued = t - (tb + 1);
+                    const int want = (int)std::ceil(std::max(left, 0.0)) + 2 - queued;
+                    m = std::max(1, std::min(kPcgChunk, want));
+                }
+                if (tb > t_a) {
+                    t_a = tb;
+                    rr_a = rr_b;
+                }
             }
             ++k;
         }
         hip_check(hipStreamSynchronize(s), "pcg poll");
-        if (!halted && k > 0) halted = pin[(k - 1) & 1] >= 0;
-    } catch (...) {
-        release();
-        throw;
+        if (!halted && k > 0) {
+            h = pin[(k - 1) & 1];
+            halted = h.status >= 0;
+        }
     }
-    release();
-    // the check that follows the last iteration (or the b = 0 / max_iter = 0 exits)
-    if (!halted) L.iteration(max_iter & 1, s, true);
-    PcgState h {};
-    hip_check(hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s), "pcg result");
-    hip_check(hipStreamSynchronize(s), "pcg result");
+    // A halted state no longer changes (later iteration kernels only mark
+    // their parity halted), so the last poll's copy is the result; else the
+    // check that follows the last iteration (or the b = 0 / max_iter = 0
+    // exits) runs and the state is read once more.
+    if (!halted || mode == PcgLaunch::kNoAlloc) {
+        if (!halted) L.iteration(max_iter & 1, s, true);
+        hip_check(hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s), "pcg result");
+        hip_check(hipStreamSynchronize(s), "pcg result");
+    }
     res.iters = h.iters;
     res.status = h.status;
     res.relres = h.b2 > 0.0 ? std::sqrt(h.rr_end / h.b2) : 0.0;

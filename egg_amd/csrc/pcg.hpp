@@ -67,9 +67,11 @@ constexpr size_t pcg_work_doubles(long long n)
 // lay the work area out over `base` (pcg_work_doubles(n) doubles)
 void pcg_carve(double* base, long long n, PcgWork& w);
 
-// How the host follows the solve: kQueued reads the device status
-// asynchronously through pinned memory; kNoAlloc reads it synchronously and
-// allocates nothing (no pinned memory, no events).
+// How the host follows the solve: kQueued reads the device state
+// asynchronously through pinned memory (two slots and two events per device
+// and host thread, made on first use and kept until the process ends);
+// kNoAlloc reads it synchronously and allocates nothing (no pinned memory,
+// no events).
 enum class PcgLaunch { kQueued, kNoAlloc };
 constexpr int kPcgChunk = 16;
 
