@@ -98,6 +98,135 @@ DArr sample_weights(DArr wt, int64_t n, const char* where)
                           " != " + std::to_string(n) + " (one per sample, or a single shared value)");
 }
 
+// ---- entities: the reference wire's per-kind records and CSR payloads
+// (egg/geometry/entity_soa.py:41-60 field table, :598-688 grouping;
+// src/geometry.hpp:1717-2520) as the compute core's self-describing records
+// (egg_amd/csrc/entity.hpp: tag, fields, payloads).
+
+// SoA fields per tag on the wire (kFields)
+constexpr int kWireFields[17] = {0, 4, 3, 4, 10, 9, 6, 8, 8, 10, 6, 2, 10, 8, 9, 4, 6};
+// positional parameters of a composite's segment record (kParamPad)
+constexpr int kParamPad = 12;
+constexpr int kEntBSplineTag = 10;
+
+struct Slice {
+    const double* p = nullptr;
+    long long n = 0;
+};
+
+Slice sub(const Slice& s, long long off, long long n, const char* what)
+{
+    if (off < 0 || n < 0 || off + n > s.n) throw py::value_error(std::string("wire: ") + what + " outside its payload");
+    return {s.p + off, n};
+}
+
+long long wire_count(double v, const char* what)
+{
+    if (!std::isfinite(v) || v < 0 || v != std::floor(v) || v > 1e9)
+        throw py::value_error(std::string("wire: ") + what + " must be a count");
+    return (long long)v;
+}
+
+// B-spline curve record: p n t0 t1 closed has_w | knots ctrl (w)
+void bspline_record(double p, double n, double t0, double t1, double closed, double has_w, const Slice& knots,
+                    const Slice& ctrl, const Slice& w, std::vector<double>& out)
+{
+    const long long pp = wire_count(p, "B-spline degree"), nn = wire_count(n, "B-spline control count");
+    if (knots.n != nn + pp + 1 || ctrl.n != 2 * nn || (has_w != 0.0 && w.n != nn))
+        throw py::value_error("wire: B-spline payload sizes do not match its degree and control count");
+    out.insert(out.end(), {double(kEntBSplineTag), p, n, t0, t1, closed, has_w});
+    out.insert(out.end(), knots.p, knots.p + knots.n);
+    out.insert(out.end(), ctrl.p, ctrl.p + ctrl.n);
+    if (has_w != 0.0) out.insert(out.end(), w.p, w.p + w.n);
+}
+
+// one segment of a composite from its positional record [tag, params]
+// (egg/geometry/entity_encoding.py:82-178); a B-spline's payload offsets
+// point into the composite's arena slice
+void segment_record(const double* rec, const Slice& arena, std::vector<double>& out)
+{
+    const double tv = rec[0];
+    const int t = (int)tv;
+    const double* q = rec + 1;
+    if (tv != t || !(t == 1 || t == 2 || t == 3 || t == 15 || (t >= 6 && t <= 10)))
+        throw py::value_error("wire: a composite segment must be a 2D curve (tag " + std::to_string(tv) + ")");
+    if (t == kEntBSplineTag) {
+        const long long nn = wire_count(q[1], "B-spline control count"), pp = wire_count(q[0], "B-spline degree");
+        const bool rat = q[7] != 0.0;
+        bspline_record(q[0], q[1], q[4], q[5], 0.0, q[7],
+                       sub(arena, wire_count(q[2], "knot offset"), nn + pp + 1, "B-spline knots"),
+                       sub(arena, wire_count(q[3], "control offset"), 2 * nn, "B-spline control net"),
+                       rat ? sub(arena, wire_count(q[6], "weight offset"), nn, "B-spline weights") : Slice {}, out);
+        return;
+    }
+    out.push_back(tv);
+    out.insert(out.end(), q, q + kWireFields[t]);
+}
+
+// one DOF's record from its kind's fields r[kWireFields[tag]] and payloads
+void wire_entity(int tag, const double* r, const std::vector<Slice>& seg, std::vector<double>& out)
+{
+    auto slot = [&](size_t i) { return i < seg.size() ? seg[i] : Slice {}; };
+    switch (tag) {
+    case 4:  // sphere: c(3) r ax(3) ay(3) -> + az = ax x ay
+    case 12: {  // cylinder: o(3) ax(3) ay(3) r -> o ax ay az r
+        const double* ax = tag == 4 ? r + 4 : r + 3;
+        const double* ay = ax + 3;
+        const double az[3] = {ax[1] * ay[2] - ax[2] * ay[1], ax[2] * ay[0] - ax[0] * ay[2],
+                              ax[0] * ay[1] - ax[1] * ay[0]};
+        out.push_back(tag);
+        if (tag == 4) {
+            out.insert(out.end(), r, r + 10);
+            out.insert(out.end(), az, az + 3);
+        } else {
+            out.insert(out.end(), r, r + 9);
+            out.insert(out.end(), az, az + 3);
+            out.push_back(r[9]);
+        }
+        return;
+    }
+    case 10:
+        bspline_record(r[0], r[1], r[2], r[3], r[4], r[5], slot(0), slot(1), slot(2), out);
+        return;
+    case 11: {  // composite: [n_segs, rec_off] + its arena slice
+        const Slice arena = slot(0);
+        const long long m = wire_count(r[0], "composite segment count"), ro = wire_count(r[1], "record offset");
+        if (m < 1) throw py::value_error("wire: a composite needs at least one segment");
+        const Slice recs = sub(arena, ro, m * (1 + kParamPad), "composite segment records");
+        const size_t base = out.size();
+        out.push_back(11);
+        out.push_back((double)m);
+        out.resize(out.size() + m, 0.0);
+        for (long long s = 0; s < m; ++s) {
+            out[base + 2 + s] = (double)(out.size() - base);
+            segment_record(recs.p + s * (1 + kParamPad), arena, out);
+        }
+        return;
+    }
+    case 14: {  // B-spline surface: pu pv nu nv - - - - has_w + 6 payloads
+        const long long pu = wire_count(r[0], "surface degree"), pv = wire_count(r[1], "surface degree");
+        const long long nu = wire_count(r[2], "surface control count"), nv = wire_count(r[3], "surface control count");
+        const bool rat = r[8] != 0.0;
+        const Slice ku = slot(0), kv = slot(1), ctrl = slot(2), w = slot(3), verts = slot(4), loops = slot(5);
+        if (ku.n != nu + pu + 1 || kv.n != nv + pv + 1 || ctrl.n != 3 * nu * nv || (rat && w.n != nu * nv))
+            throw py::value_error("wire: B-spline surface payload sizes do not match its degrees and net");
+        if (loops.n >= 2 && verts.n != 2 * wire_count(loops.p[loops.n - 1], "trim loop offset"))
+            throw py::value_error("wire: trim vertices do not match the loop table");
+        out.insert(out.end(), {14.0, r[0], r[1], r[2], r[3], r[8], (double)loops.n});
+        out.insert(out.end(), ku.p, ku.p + ku.n);
+        out.insert(out.end(), kv.p, kv.p + kv.n);
+        out.insert(out.end(), ctrl.p, ctrl.p + ctrl.n);
+        if (rat) out.insert(out.end(), w.p, w.p + w.n);
+        out.insert(out.end(), loops.p, loops.p + loops.n);
+        if (loops.n >= 2) out.insert(out.end(), verts.p, verts.p + verts.n);
+        return;
+    }
+    default:
+        out.push_back(tag);
+        out.insert(out.end(), r, r + kWireFields[tag]);
+    }
+}
+
 // Keeps the converted arrays alive while the C structs point into them.
 struct Wire {
     int dim = 2;
@@ -110,6 +239,8 @@ struct Wire {
     std::vector<const int32_t*> bgd_ptrs;
     std::vector<int32_t> shapes;
     IArr hsb, hsp, hdb, hdp, sb, sl;
+    std::vector<std::vector<int64_t>> ent_offs;
+    std::vector<std::vector<double>> ent_datas;
 
     const int32_t* keep(IArr a)
     {
@@ -173,41 +304,60 @@ struct Wire {
             if (g.contains("weight") && !g["weight"].is_none())
                 eg.weight = keep(sample_weights(get_d(g, "weight"), eg.total_samples, "group"));
             if (g.contains("entities") && !g["entities"].is_none() && py::len(g["entities"]) > 0) {
-                // the reference's per-type SoA sub-dicts, flattened to one tag and
-                // record per group-local DOF (every DOF covered exactly once)
+                // the per-kind partitions, one record per group-local DOF
+                // (every DOF covered exactly once)
                 std::vector<int32_t> tag(eg.ndof, -1);
-                std::vector<double> rec(eg.ndof * 4, 0.0);
+                std::vector<int64_t> eoff(eg.ndof, 0);
+                std::vector<double> edata;
                 for (auto item : g["entities"].cast<py::dict>()) {
                     py::dict ed = item.second.cast<py::dict>();
                     if (!ed.contains("tag"))
                         throw py::value_error("unpack_context: entities carries a group without a tag");
                     const int t = ed["tag"].cast<int>(), kf = ed["kFields"].cast<int>();
-                    if (!((t == 0 && kf == 0) || (t == 1 && kf == 4) || (t == 2 && kf == 3) ||
-                          (t == 3 && kf == 4) || (t == 15 && kf == 4)))
+                    if (t < 0 || t > 16 || kf != kWireFields[t])
                         throw py::value_error("wire: entity tag " + std::to_string(t) + " (" +
-                                              item.first.cast<std::string>() +
-                                              ") is not supported by this compute core (free, lineseg, circle, "
-                                              "ellipse, linerail)");
+                                              item.first.cast<std::string>() + ") with " + std::to_string(kf) +
+                                              " fields is not an entity kind of the wire");
                     IArr loc = get_i(ed, "dof_local");
-                    std::vector<double> r;
-                    if (kf) {
-                        DArr rr = get_d(ed, "records");
-                        if ((int64_t)rr.size() != (int64_t)loc.size() * kf)
-                            throw py::value_error("wire: entity records size != count * kFields");
-                        r.assign(rr.data(), rr.data() + rr.size());
-                    }
-                    for (py::ssize_t i = 0; i < loc.size(); ++i) {
+                    const py::ssize_t k = loc.size();
+                    DArr rr = kf ? get_d(ed, "records") : DArr(0);
+                    if ((int64_t)rr.size() != (int64_t)k * kf)
+                        throw py::value_error("wire: entity records size != count * kFields");
+                    std::vector<DArr> sdat;
+                    std::vector<py::array_t<int64_t, py::array::c_style | py::array::forcecast>> soff;
+                    if (ed.contains("seg") && !ed["seg"].is_none())
+                        for (auto sv : ed["seg"].cast<py::list>()) {
+                            py::dict sd = sv.cast<py::dict>();
+                            sdat.push_back(get_d(sd, "data"));
+                            soff.push_back(sd["off"].cast<py::array_t<int64_t, py::array::c_style | py::array::forcecast>>());
+                            if (soff.back().size() != k + 1)
+                                throw py::value_error("wire: entity seg offsets need count + 1 entries");
+                        }
+                    for (py::ssize_t i = 0; i < k; ++i) {
                         const int l = loc.data()[i];
                         if (l < 0 || l >= eg.ndof || tag[l] >= 0)
                             throw py::value_error("unpack_context: entity dof_local out of range or repeated");
                         tag[l] = t;
-                        for (int k = 0; k < kf; ++k) rec[(size_t)l * 4 + k] = r[(size_t)i * kf + k];
+                        if (t == 0) continue;
+                        std::vector<Slice> seg;
+                        for (size_t q = 0; q < sdat.size(); ++q) {
+                            const int64_t a = soff[q].data()[i], b = soff[q].data()[i + 1];
+                            if (a < 0 || b < a || b > (int64_t)sdat[q].size())
+                                throw py::value_error("wire: entity seg offsets outside their data");
+                            seg.push_back({sdat[q].data() + a, b - a});
+                        }
+                        eoff[l] = (int64_t)edata.size();
+                        wire_entity(t, rr.data() + (size_t)i * kf, seg, edata);
                     }
                 }
                 for (int32_t t : tag)
                     if (t < 0) throw py::value_error("unpack_context: entities do not cover every DOF of the group");
                 eg.entity_tag = keep(IArr((py::ssize_t)tag.size(), tag.data()));
-                eg.entity_rec = keep(DArr((py::ssize_t)rec.size(), rec.data()));
+                ent_offs.push_back(std::move(eoff));
+                ent_datas.push_back(std::move(edata));
+                eg.entity_off = ent_offs.back().data();
+                eg.entity_data = ent_datas.back().data();
+                eg.entity_len = (int64_t)ent_datas.back().size();
             }
             if (g.contains("curv_k") && !g["curv_k"].is_none() && g["curv_k"].cast<int>() > 0) {
                 const int K = g["curv_k"].cast<int>();
@@ -970,53 +1120,78 @@ PYBIND11_MODULE(cpp_core, m)
       "Directional soft-energy total and analytic gradient over a sample table (host side, FP64): X (n_nodes, d), "
       "nodes (n, 4) with -1 pads, kind (n,) 0 parallel / 1 line / 2 stem, ref (n, d), weight (n,).");
 
-    // geometry entity calls (reference src/bindings.cpp:1872-1990) for the
-    // closed-form 2D kinds the compute core supports
-    auto geo_record = [](const DArr& params, int tag) {
-        const int need_n = tag == 2 ? 3 : (tag == 0 ? 0 : 4);
-        if ((int)params.size() < need_n)
-            throw py::value_error("geometry: tag " + std::to_string(tag) + " needs " + std::to_string(need_n) +
+    // geometry entity calls (reference src/bindings.cpp:1872-1990): the
+    // single-point calls take the positional record (2D fixed-size kinds; 3D
+    // kinds and tag 0 are free there, as in the reference's 2D dispatch),
+    // the batch call one entity's SoA record and payloads
+    auto positional = [](const DArr& params, int tag) {
+        std::vector<double> b {double(tag)};
+        if (tag == 10 || tag == 11)
+            throw py::value_error("geometry: tag " + std::to_string(tag) +
+                                  " keeps its data in an arena; use geometry_project_batch");
+        if (tag < 0 || tag > 16) throw py::value_error("geometry: unknown entity tag " + std::to_string(tag));
+        const int nf = kWireFields[tag];
+        if ((int)params.size() < nf)
+            throw py::value_error("geometry: tag " + std::to_string(tag) + " needs " + std::to_string(nf) +
                                   " parameters");
-        std::array<double, 4> rec {0.0, 0.0, 0.0, 0.0};
-        for (int k = 0; k < need_n; ++k) rec[k] = params.data()[k];
-        return rec;
+        b.insert(b.end(), params.data(), params.data() + nf);
+        return b;
+    };
+    auto point_frame = [positional](const DArr& p, int tag, const DArr& params, bool foot) {
+        if (p.size() < 2) throw py::value_error("geometry: p must hold 2 coordinates");
+        const bool free2 = tag == 0 || tag == 4 || tag == 5 || tag == 12 || tag == 13 || tag == 14 || tag == 16;
+        const std::vector<double> b = free2 ? std::vector<double> {0.0} : positional(params, tag);
+        py::array_t<double> out(2);
+        const int K = free2 ? 2 : 1;
+        std::vector<double> f(2), B(2 * K);
+        check(eggmi_geometry_eval(2, b.data(), (int64_t)b.size(), 1, p.data(), f.data(), B.data()));
+        out.mutable_data()[0] = foot ? f[0] : B[0];
+        out.mutable_data()[1] = foot ? f[1] : B[K];
+        return out;
     };
     m.def(
-      "geometry_project",
-      [geo_record](const DArr& p, int tag, const DArr& params) {
-          if (p.size() < 2) throw py::value_error("geometry_project: p must hold 2 coordinates");
-          const auto rec = geo_record(params, tag);
-          py::array_t<double> out(2);
-          check(eggmi_geometry_eval(tag, rec.data(), 1, p.data(), out.mutable_data(), nullptr));
-          return out;
+      "geometry_project", [point_frame](const DArr& p, int tag, const DArr& params) {
+          return point_frame(p, tag, params, true);
       },
       py::arg("p"), py::arg("tag"), py::arg("params"), "Project point onto entity. Returns proj(2,).");
     m.def(
-      "geometry_tangent",
-      [geo_record](const DArr& p, int tag, const DArr& params) {
-          if (p.size() < 2) throw py::value_error("geometry_tangent: p must hold 2 coordinates");
-          const auto rec = geo_record(params, tag);
-          py::array_t<double> out(2);
-          check(eggmi_geometry_eval(tag, rec.data(), 1, p.data(), nullptr, out.mutable_data()));
-          return out;
+      "geometry_tangent", [point_frame](const DArr& p, int tag, const DArr& params) {
+          return point_frame(p, tag, params, false);
       },
       py::arg("p"), py::arg("tag"), py::arg("params"), "Tangent direction at point on entity. Returns tang(2,).");
     m.def(
       "geometry_project_batch",
-      [geo_record](const DArr& q, int tag, const DArr& records, const py::list& seg) {
-          if (q.ndim() != 2 || q.shape(1) != 2)
-              throw py::value_error("geometry_project_batch: q must be (n, 2) (the compute core's entities are 2D)");
+      [](const DArr& q, int tag, const DArr& records, const py::list& seg) {
+          if (q.ndim() != 2 || (q.shape(1) != 2 && q.shape(1) != 3))
+              throw py::value_error("geometry_project_batch: q must be (n, 2) or (n, 3)");
           if (tag == 0) throw py::value_error("geometry_project_batch: tag 0 (free) has no foot");
-          if (py::len(seg) != 0)
-              throw py::value_error("geometry_project_batch: segmented entity data is not supported here");
-          const auto rec = geo_record(records, tag);
+          if (tag < 0 || tag > 16) throw py::value_error("geometry_project_batch: unknown entity tag");
+          const int dim = (int)q.shape(1);
+          if ((int)records.size() < kWireFields[tag])
+              throw py::value_error("geometry_project_batch: record shorter than the kind's fields");
+          std::vector<DArr> keep;
+          std::vector<Slice> slices;
+          for (auto item : seg) {
+              py::dict sd = item.cast<py::dict>();
+              keep.push_back(get_d(sd, "data"));
+              auto off = sd["off"].cast<py::array_t<int64_t, py::array::c_style | py::array::forcecast>>();
+              if (off.size() < 2 || off.data()[0] < 0 || off.data()[1] < off.data()[0] ||
+                  off.data()[1] > (int64_t)keep.back().size())
+                  throw py::value_error("geometry_project_batch: seg offsets outside their data");
+              slices.push_back({keep.back().data() + off.data()[0], off.data()[1] - off.data()[0]});
+          }
+          std::vector<double> b;
+          wire_entity(tag, records.data(), slices, b);
           const py::ssize_t n = q.shape(0);
-          py::array_t<double> feet({n, (py::ssize_t)2}), basis({n, (py::ssize_t)2, (py::ssize_t)1});
-          check(eggmi_geometry_eval(tag, rec.data(), n, q.data(), feet.mutable_data(), basis.mutable_data()));
+          const int K = (tag == 4 || tag == 5 || tag == 12 || tag == 14) ? 2 : 1;
+          py::array_t<double> feet({n, (py::ssize_t)dim}), basis({n, (py::ssize_t)dim, (py::ssize_t)K});
+          check(eggmi_geometry_eval(dim, b.data(), (int64_t)b.size(), n, q.data(), feet.mutable_data(),
+                                    basis.mutable_data()));
           return py::make_tuple(feet, basis);
       },
       py::arg("q"), py::arg("tag"), py::arg("records"), py::arg("seg"),
-      "Project (n, 2) points onto one entity: (feet (n, 2), tangent basis (n, 2, 1)).");
+      "Project (n, d) points onto one entity given its SoA record and payloads: (feet (n, d), orthonormal tangent "
+      "basis (n, d, tdim)).");
 
     m.def(
       "pcg_csr",

@@ -185,33 +185,34 @@ eggmi_status eggmi_mg_probe(eggmi_session* s, int level, int op, const double* i
     });
 }
 
-eggmi_status eggmi_geometry_eval(int tag, const double* record, int64_t n, const double* q, double* proj,
-                                 double* tangent)
+eggmi_status eggmi_geometry_eval(int dim, const double* record, int64_t len, int64_t n, const double* q,
+                                 double* proj, double* basis)
 {
     return guarded([&] {
-        if (tag != 0 && tag != 1 && tag != 2 && tag != 3 && tag != 15)
-            throw eggmi::InvalidArgument("geometry: entity tag " + std::to_string(tag) +
-                                         " is not supported by this compute core (0 free, 1 lineseg, 2 circle, "
-                                         "3 ellipse, 15 linerail)");
+        if (dim != 2 && dim != 3) throw eggmi::InvalidArgument("geometry: dim must be 2 or 3");
         if (n < 0) throw eggmi::InvalidArgument("geometry: n must be >= 0");
+        need(record, "record");
+        if (len < 1) throw eggmi::InvalidArgument("geometry: empty record");
+        const int tag = record[0] == 0.0 ? 0 : (int)record[0];
+        if (tag != 0) eggmi::entity_blob_check(record, len, dim);
+        const int K = tag == 0 ? dim : eggmi::ent_tdim(tag);
         if (n == 0) return;
         need(q, "q");
-        if (tag != 0) need(record, "record");
         for (int64_t i = 0; i < n; ++i) {
-            const double* p = q + 2 * i;
-            double f[2] = {p[0], p[1]}, t[2] = {1.0, 0.0};
-            if (tag != 0) {
-                eggmi::ent_project(tag, record, p, f);
-                eggmi::ent_tangent(tag, record, p, t);
+            double f[3] = {}, B[3][3] = {};
+            if (tag == 0) {  // free: identity, the unit basis
+                for (int r = 0; r < dim; ++r) {
+                    f[r] = q[dim * i + r];
+                    B[r][r] = 1.0;
+                }
+            } else {
+                eggmi::ent_frame(tag, record, q + dim * i, f, B);
             }
-            if (proj) {
-                proj[2 * i] = f[0];
-                proj[2 * i + 1] = f[1];
-            }
-            if (tangent) {
-                tangent[2 * i] = t[0];
-                tangent[2 * i + 1] = t[1];
-            }
+            if (proj)
+                for (int r = 0; r < dim; ++r) proj[dim * i + r] = f[r];
+            if (basis)
+                for (int r = 0; r < dim; ++r)
+                    for (int c = 0; c < K; ++c) basis[(dim * i + r) * K + c] = B[c][r];
         }
     });
 }

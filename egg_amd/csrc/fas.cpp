@@ -627,6 +627,7 @@ std::pair<std::vector<double>, std::vector<double>> Session::run_fas(int n_cycle
     EGGMI_HIP(hipEventRecord(ev0, stream));
     // consistent halo on the resident state, both fine Jacobi buffers equal
     halo(F, F.x[F.cur].p);
+    F.next_run(stream);
     EGGMI_HIP(hipMemcpyAsync(F.x[1 - F.cur].p, F.x[F.cur].p, (size_t)F.plane * dim * sizeof(float),
                              hipMemcpyDeviceToDevice, stream));
     pair_valid = true;

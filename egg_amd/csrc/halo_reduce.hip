@@ -500,6 +500,13 @@ void launch_linesearch_pair(int dim, int kind, const CellReduceArgs& a, const fl
     pair_finish<<<1, kReduceThreads, 0, s>>>(partials, nb + nd, out);
 }
 
+__global__ void set_int(int* p, int v)
+{
+    if (threadIdx.x == 0) *p = v;
+}
+
+void launch_set_int(int* p, int v, hipStream_t s) { set_int<<<1, 64, 0, s>>>(p, v); }
+
 void launch_copy(float* x, long long plane, int dim, const int* src, const int* dst, long long n, hipStream_t s)
 {
     if (n <= 0) return;

@@ -83,12 +83,18 @@ struct SweepArgs {
     float omega = 1.0f;
     float delta = 0.0f;
     const float* tau = nullptr;  // FAS linear shift (plane layout), null = none
-    // constrained boundary DOFs (2D): stored DOFs st.n .. st.n + con_n - 1 of
-    // the stored arrays, each sliding on the entity con_tag[i] (1 segment
-    // sx sy ex ey, 2 circle cx cy r) with record con_rec[4 i ..]
+    // constrained boundary DOFs: stored DOFs st.n .. st.n + con_n - 1 of the
+    // stored arrays, each sliding on the entity con_tag[i] whose blob
+    // (entity.hpp) starts at con_blob + con_off[i]; con_seed[2 i ..] the
+    // foot parameter of its last projection (NaN: none yet), con_epoch[i]
+    // the run serial of its last update (*run_epoch: the current run's)
     int con_n = 0;
     const uint8_t* con_tag = nullptr;  // 0: a free DOF (there for its curvature windows)
-    const float* con_rec = nullptr;
+    const float* con_blob = nullptr;
+    const int* con_off = nullptr;
+    float* con_seed = nullptr;
+    int* con_epoch = nullptr;
+    const int* run_epoch = nullptr;
     // block-interface C2 curvature windows of those DOFs (2D): curv_k per DOF,
     // four node slots each, the DOF's place (-1 = padding) and the weight
     int curv_k = 0;
@@ -158,6 +164,8 @@ void launch_grad(int dim, int kind, int tile_variant, const SweepArgs& a, hipStr
 void launch_level_smooth(int dim, int kind, int tile_variant, const SweepArgs& a, float* x0, float* x1,
                          int n_sweeps, hipStream_t s);
 void launch_copy(float* x, long long plane, int dim, const int* src, const int* dst, long long n, hipStream_t s);
+// *p = v, stream-ordered (a kernel, so it may sit in captured work)
+void launch_set_int(int* p, int v, hipStream_t s);
 void launch_reduce_cells(int dim, int kind, const CellReduceArgs& a, double* partials, hipStream_t s);
 void launch_reduce_samples(int dim, int kind, const SampleReduceArgs& a, double* partials, hipStream_t s);
 // Folds the partials into *out_e, *out_m; with rep set, also appends them

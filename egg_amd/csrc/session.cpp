@@ -1,6 +1,7 @@
 // Host side of the MI355X structured smoother (see session.hpp).
 #include "session.hpp"
 #include "directional.hpp"
+#include "entity.hpp"
 #include "mlp_host.hpp"
 
 #include <limits>
@@ -164,7 +165,11 @@ void Level::upload_tables()
     d_mask.upload(mask);
     d_st_dof.upload(st_dof);
     d_con_tag.upload(con_tag);
-    d_con_rec.upload(con_rec);
+    d_con_off.upload(con_off);
+    d_con_blob.upload(con_blob);
+    d_con_epoch.upload(std::vector<int>(con_tag.size(), -1));
+    d_run_epoch.upload(std::vector<int>(1, run_epoch));
+    reset_seeds();
     d_curv_nodes.upload(curv_nodes);
     d_curv_slot.upload(curv_slot);
     d_curv_w.upload(curv_w);
@@ -219,6 +224,40 @@ void Level::upload_stored(int G)
     st_lanes = G;
 }
 
+void Level::reset_seeds()
+{
+    d_con_seed.upload(std::vector<float>(2 * con_tag.size(), std::numeric_limits<float>::quiet_NaN()));
+}
+
+Level::ConState Level::save_con_state() const
+{
+    ConState c;
+    c.run_epoch = run_epoch;
+    c.seed.resize(d_con_seed.n);
+    c.epoch.resize(d_con_epoch.n);
+    if (!c.seed.empty())
+        EGGMI_HIP(hipMemcpy(c.seed.data(), d_con_seed.p, c.seed.size() * sizeof(float), hipMemcpyDeviceToHost));
+    if (!c.epoch.empty())
+        EGGMI_HIP(hipMemcpy(c.epoch.data(), d_con_epoch.p, c.epoch.size() * sizeof(int), hipMemcpyDeviceToHost));
+    return c;
+}
+
+void Level::restore_con_state(const ConState& c)
+{
+    if (con_tag.empty()) return;
+    run_epoch = c.run_epoch;
+    d_con_seed.upload(c.seed);
+    d_con_epoch.upload(c.epoch);
+    d_run_epoch.upload(std::vector<int>(1, run_epoch));
+}
+
+void Level::next_run(hipStream_t s)
+{
+    if (con_tag.empty()) return;
+    ++run_epoch;
+    launch_set_int(d_run_epoch.p, run_epoch, s);
+}
+
 SweepArgs Level::sweep_args(const float* xin, float* xout, float omega, float delta, const float* tau) const
 {
     SweepArgs a;
@@ -232,7 +271,11 @@ SweepArgs Level::sweep_args(const float* xin, float* xout, float omega, float de
     a.st.n = (int)(st_dof.size() - con_tag.size());  // the constrained DOFs follow (con_*)
     a.con_n = (int)con_tag.size();
     a.con_tag = d_con_tag.p;
-    a.con_rec = d_con_rec.p;
+    a.con_blob = d_con_blob.p;
+    a.con_off = d_con_off.p;
+    a.con_seed = d_con_seed.p;
+    a.con_epoch = d_con_epoch.p;
+    a.run_epoch = d_run_epoch.p;
     a.curv_k = con_tag.empty() ? 0 : curv_k;
     a.curv_nodes = d_curv_nodes.p;
     a.curv_slot = d_curv_slot.p;
@@ -337,14 +380,18 @@ void validate(int dim, const eggmi_group* groups, int n_groups, const eggmi_sten
         if (g.entity_tag)
             for (long long d = 0; d < g.ndof; ++d) {
                 const int t = g.entity_tag[d];
-                if (t < 0 || (t > 3 && t != 15))
+                if (t == 0) continue;
+                if (t < 0 || t > kEntMaxTag)
                     throw InvalidArgument("validate_context: " + grp + " entity tag " + std::to_string(t) +
-                                          " is not supported (0 free, 1 segment, 2 circle, 3 ellipse, 15 rail)");
-                if (t != 0 && dim != 2)
-                    throw InvalidArgument("validate_context: constrained boundary DOFs are 2D only here");
-                for (int k = 0; k < 4; ++k)
-                    if (t != 0 && !std::isfinite(g.entity_rec[d * 4 + k]))
-                        throw InvalidArgument("validate_context: " + grp + " entity record must be finite");
+                                          " is not an entity kind");
+                if (!g.entity_off || !g.entity_data)
+                    throw InvalidArgument("validate_context: " + grp + " constrained DOFs need entity records");
+                const long long off = g.entity_off[d];
+                if (off < 0 || off >= g.entity_len)
+                    throw InvalidArgument("validate_context: " + grp + " entity_off outside entity_data");
+                if ((int)g.entity_data[off] != t)
+                    throw InvalidArgument("validate_context: " + grp + " entity record's tag differs from entity_tag");
+                entity_blob_check(g.entity_data + off, g.entity_len - off, dim);
             }
     }
     if (es->num_samples < 0) throw InvalidArgument("validate_context: energy_stencil negative size");
@@ -645,6 +692,21 @@ void Session::create(int d, const eggmi_group* groups, int n_groups, const eggmi
                 w_uniform_all = false;
         }
     }
+    // entity records in FP32, one copy per distinct record (the DOFs of one
+    // curve share it); returns its offset in the level's arena
+    std::unordered_map<std::string, int> ent_seen;
+    auto pack_entity = [&](const double* b, long long avail, int dim) -> int {
+        const long long len = entity_blob_check(b, avail, dim);
+        std::string key((const char*)b, (size_t)len * sizeof(double));
+        auto it = ent_seen.find(key);
+        if (it != ent_seen.end()) return it->second;
+        if ((long long)L.con_blob.size() + len > (long long)std::numeric_limits<int>::max())
+            throw InvalidArgument("structured: entity records exceed 2^31 values");
+        const int off = (int)L.con_blob.size();
+        for (long long i = 0; i < len; ++i) L.con_blob.push_back((float)b[i]);
+        ent_seen.emplace(std::move(key), off);
+        return off;
+    };
     // constrained boundary DOFs (an entity to slide on) are stored last, so
     // the sweep's stored view excludes them and a kernel of their own takes them
     for (int pass = 0; pass < 2; ++pass)
@@ -694,7 +756,9 @@ void Session::create(int d, const eggmi_group* groups, int n_groups, const eggmi
                 if (con && ib >= 0)
                     throw InvalidArgument("structured: a synthesized interior DOF cannot be constrained");
                 L.con_tag.push_back(con ? (uint8_t)g.entity_tag[di] : (uint8_t)0);
-                for (int k = 0; k < 4; ++k) L.con_rec.push_back(con ? (float)g.entity_rec[di * 4 + k] : 0.0f);
+                L.con_off.push_back(con ? pack_entity(g.entity_data + g.entity_off[di],
+                                                      g.entity_len - g.entity_off[di], d)
+                                        : 0);
                 for (int j = 0; j < kmax_curv; ++j) {
                     const bool live = j < g.curv_k && g.curv_slot[di * g.curv_k + j] >= 0;
                     L.curv_slot.push_back(live ? (int8_t)g.curv_slot[di * g.curv_k + j] : (int8_t)-1);
@@ -1012,6 +1076,7 @@ std::pair<std::vector<double>, std::vector<double>> Session::run_jacobi(int n_sw
     const size_t bytes = (size_t)L.plane * dim * sizeof(float);
     EGGMI_HIP(hipEventRecord(ev0, stream));
     halo(L, xin);
+    L.next_run(stream);
     // the write buffer must hold every slot no sweep writes (fixed nodes);
     // after a sweep it does, and it keeps the previous state of every DOF
     // (the learned proposal's previous-displacement input)
@@ -1247,6 +1312,8 @@ std::vector<double> Session::autotune(int n_sweeps, double omega)
     const bool pair0 = pair_valid;
     const bool count0 = counting;
     const Mlp* prop0 = proposal;
+    EGGMI_HIP(hipStreamSynchronize(stream));
+    const Level::ConState con0 = L.save_con_state();
     counting = false;
     proposal = nullptr;
     const int nv = num_tile_variants(dim);
@@ -1275,6 +1342,8 @@ std::vector<double> Session::autotune(int n_sweeps, double omega)
         proposal = prop0;
         throw;
     }
+    EGGMI_HIP(hipStreamSynchronize(stream));
+    L.restore_con_state(con0);
     int best = 0;
     for (int v = 1; v < nv; ++v)
         if (ms[v] < ms[best]) best = v;
@@ -1313,6 +1382,7 @@ void Session::set_X(const double* X)
     halo(L, L.x[L.cur].p);
     EGGMI_HIP(hipStreamSynchronize(stream));
     pair_valid = false;
+    L.reset_seeds();
 }
 
 std::vector<float> Session::packed() const

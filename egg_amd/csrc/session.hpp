@@ -145,12 +145,33 @@ struct Level {
     std::vector<uint8_t> st_meta;
     std::vector<float> st_W;  // empty = uniform
     std::vector<float> st_wt;  // per-occurrence energy weight; empty = 1
-    // the last con_tag.size() stored DOFs are constrained to an entity (2D):
-    // tag 1 segment, 2 circle, and a record of 4 floats each
+    // the last con_tag.size() stored DOFs are special: constrained to the
+    // entity con_tag (0: none, the DOF is there for its windows) whose FP32
+    // record (entity.hpp) starts at con_blob[con_off]; per DOF the foot
+    // parameter of its last projection (NaN: none, the iterative
+    // projections' warm start) and the run serial of its last update
+    // against the level's current one (a DOF's first update in a run is
+    // cold)
     std::vector<uint8_t> con_tag;
-    std::vector<float> con_rec;
+    std::vector<int> con_off;
+    std::vector<float> con_blob;
     DArr<uint8_t> d_con_tag;
-    DArr<float> d_con_rec;
+    DArr<int> d_con_off;
+    DArr<float> d_con_blob;
+    DArr<float> d_con_seed;
+    DArr<int> d_con_epoch;
+    DArr<int> d_run_epoch;
+    int run_epoch = 0;
+    void reset_seeds();              // positions replaced: no warm start
+    void next_run(hipStream_t s);    // a new run begins on stream s
+    // the special DOFs' seeds and run serials, kept across timing runs
+    struct ConState {
+        std::vector<float> seed;
+        std::vector<int> epoch;
+        int run_epoch = 0;
+    };
+    ConState save_con_state() const;
+    void restore_con_state(const ConState& c);
     // their curvature windows (curv_k per special DOF): owner slots of the
     // four nodes, the DOF's place in the window (-1 = padding), the weight
     int curv_k = 0;

@@ -109,6 +109,55 @@ template <int D> __device__ __forceinline__ void newton(const float* g, const fl
     }
 }
 
+// Tangent-reduced Newton step delta = B y from the orthonormal basis B[0..K-1]
+// (K = 1 or 2; src/patch.hpp:348-388): M = B^T H B, r = B^T g, y the
+// reference's solve of M y = -r (1x1: src/solve.hpp:83-90; 2x2 with the
+// same fallbacks as the node solve).
+template <int D>
+__device__ __forceinline__ void reduced_step(const float* g, const float* h, const float (*B)[3], int K, float* dl)
+{
+    float H[D][D];
+    if constexpr (D == 2) {
+        H[0][0] = h[0], H[0][1] = H[1][0] = h[1], H[1][1] = h[2];
+    } else {
+        H[0][0] = h[0], H[0][1] = H[1][0] = h[1], H[0][2] = H[2][0] = h[2];
+        H[1][1] = h[3], H[1][2] = H[2][1] = h[4], H[2][2] = h[5];
+    }
+    float M[2][2] = {}, r[2] = {};
+    for (int c = 0; c < K; ++c) {
+        float Hb[D];
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            float s = 0.0f;
+#pragma unroll
+            for (int j = 0; j < D; ++j) s += H[i][j] * B[c][j];
+            Hb[i] = s;
+        }
+        for (int e = 0; e < K; ++e) {
+            float s = 0.0f;
+#pragma unroll
+            for (int i = 0; i < D; ++i) s += B[e][i] * Hb[i];
+            M[c][e] = s;
+        }
+        float s = 0.0f;
+#pragma unroll
+        for (int i = 0; i < D; ++i) s += B[c][i] * g[i];
+        r[c] = s;
+    }
+    float y[2] = {0.0f, 0.0f};
+    if (K == 1) {
+        const float rn = fabsf(r[0]);
+        if (rn >= EntTol<float>::znorm) {
+            y[0] = -r[0] / M[0][0];
+            if (!isfinite(y[0])) y[0] = -0.1f * r[0] / rn;
+        }
+    } else {
+        solve2<float>(M[0][0], M[0][1], M[1][0], M[1][1], r[0], r[1], y[0], y[1]);
+    }
+#pragma unroll
+    for (int i = 0; i < D; ++i) dl[i] = B[0][i] * y[0] + (K == 2 ? B[1][i] * y[1] : 0.0f);
+}
+
 // Write v into the replica slots of a node (its ghost, fan and share
 // copies whose root is that node), when the level has a push table.
 template <int D>
@@ -263,16 +312,16 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
             }
     };
 
-    // special DOFs (ENT): tag != 0 slides on an entity; tag 0 is free and
-    // carries curvature windows, lane l taking windows l, l + G, ...
+    // special DOFs (ENT): tag != 0 slides on an entity (its blob at
+    // con_blob + con_off); tag 0 is free and carries curvature windows, lane
+    // l taking windows l, l + G, ...
     int etag = 0;
-    float erec[4] = {};
+    const float* eb = nullptr;
     const int ci = d - st.n;
     if constexpr (ENT) {
         if (valid) {
             etag = a.con_tag[ci];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) erec[k] = a.con_rec[4 * ci + k];
+            if (etag) eb = a.con_blob + a.con_off[ci];
         }
     }
     const bool curv = D == 2 && ENT && valid && etag == 0 && a.curv_k > 0;  // entities and curvature: 2D
@@ -405,28 +454,81 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
 #pragma unroll
     for (int k = 0; k < kNH<D>; ++k) h[k] = group_sum<G>(h[k]);
     float dl[D];
-    if (D == 2 && ENT && etag != 0) {
-        const float ft = etag == 15 ? rail_foot(erec, pos) : 0.0f;
-        if (etag == 15 && (ft < 0.0f || ft > 1.0f)) {
-            // a rail beyond its ends releases the node: full-space step
-            // (src/patch.hpp:400-409), still damped on the step
-            newton<D>(g, h, dl);
-            dl[0] *= a.omega;
-            dl[1] *= a.omega;
-        } else {
-            // y = solve1x1(t.H.t, t.g) (src/solve.hpp:83-90), delta = omega t y
-            float t[2];
-            ent_tangent(etag, erec, pos, t);
-            const float M = t[0] * (h[0] * t[0] + h[1] * t[1]) + t[1] * (h[1] * t[0] + h[2] * t[1]);
-            const float r = t[0] * g[0] + t[1] * g[1];
-            const float rn = fabsf(r);
-            float y = 0.0f;
-            if (rn >= 1e-7f) {
-                y = -r / M;
-                if (!isfinite(y)) y = -0.1f * r / rn;
+    // constrained DOF state: the foot parameter the last projection left
+    // (the warm start of the iterative kinds) in registers, identical in
+    // every lane of the group; pls: a warm sweep on a parametrization
+    // line-searches in its parameter from the foot qs along dq
+    float sd[2] = {0.0f, 0.0f}, qs[2] = {0.0f, 0.0f}, dq[2] = {0.0f, 0.0f};
+    bool pls = false;
+    if constexpr (ENT) {
+        if (etag != 0) {
+            sd[0] = a.con_seed[2 * ci];
+            sd[1] = a.con_seed[2 * ci + 1];
+            // the first update of a DOF in a run is cold (world-space line
+            // search, reprojected trials); later ones are warm
+            // (src/sweep.hpp:1938-1984)
+            const bool cold = a.con_epoch[ci] != *a.run_epoch;
+            const int K = ent_tdim(etag);
+            float B[2][3] = {};
+            bool release = false;
+            if (!cold && ent_is_param(etag)) {
+                // warm: project once to the foot, reduce the tangent step to
+                // the parameter, dq = (J^T J)^-1 J^T delta, damped by omega
+                // (src/sweep.hpp:1786-1820)
+                pls = true;
+                const float* f = eb + 1;
+                float J[2][3] = {};
+                par_invert(etag, f, pos, sd, std::isfinite(sd[0]), true, qs,
+                           etag == kEntBSplineSurf ? J : nullptr);
+                if (etag != kEntBSplineSurf) par_frame(etag, f, qs, J);
+                sd[0] = qs[0];
+                sd[1] = qs[1];
+#pragma unroll
+                for (int c = 0; c < 2; ++c)
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) B[c][k] = J[c][k];
+                ent_orthonormalize(B, K, D);
+                float del[D];
+                reduced_step<D>(g, h, B, K, del);
+                float JtJ[4] = {}, Jtd[2] = {};
+                for (int c = 0; c < K; ++c) {
+                    for (int e = 0; e < K; ++e) JtJ[c * 2 + e] = ent_dot(J[c], J[e], D);
+                    Jtd[c] = ent_dot(J[c], del, D);
+                }
+                if (K == 1) {
+                    dq[0] = fabsf(JtJ[0]) > EntTol<float>::tiny ? Jtd[0] / JtJ[0] : 0.0f;
+                } else {
+                    const float det = JtJ[0] * JtJ[3] - JtJ[1] * JtJ[2];
+                    if (fabsf(det) >= EntTol<float>::tiny) {
+                        const float inv = 1.0f / det;
+                        dq[0] = inv * (JtJ[3] * Jtd[0] - JtJ[1] * Jtd[1]);
+                        dq[1] = inv * (-JtJ[2] * Jtd[0] + JtJ[0] * Jtd[1]);
+                    }
+                }
+                dq[0] *= a.omega;
+                dq[1] *= a.omega;
+            } else if (ent_is_rail(etag)) {
+                // a rail beyond its ends releases the node: full-space step
+                // (src/patch.hpp:400-409), still damped on the step
+                const float ft = rail_foot(eb + 1, pos, D);
+                release = ft < 0.0f || ft > 1.0f;
+                if (!release) closed_tangent(etag, eb + 1, pos, B[0]);
+            } else if (ent_is_seeded(etag)) {
+                seeded_frame(etag, eb, pos, sd, false, (float*)nullptr, B);
+            } else {
+                float foot[3];
+                ent_frame(etag, eb, pos, foot, B);
             }
-            dl[0] = a.omega * (t[0] * y);
-            dl[1] = a.omega * (t[1] * y);
+            if (!pls) {
+                if (release)
+                    newton<D>(g, h, dl);
+                else
+                    reduced_step<D>(g, h, B, K, dl);
+#pragma unroll
+                for (int k = 0; k < D; ++k) dl[k] *= a.omega;
+            }
+        } else {
+            newton<D>(g, h, dl);
         }
     } else {
         newton<D>(g, h, dl);
@@ -439,9 +541,22 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
         float tr[D];
 #pragma unroll
         for (int k = 0; k < D; ++k) tr[k] = pos[k] + alpha * dl[k];
-        if (D == 2 && ENT && etag != 0) {
-            const float raw[2] = {tr[0], tr[1]};
-            ent_project(etag, erec, raw, tr);
+        if constexpr (ENT) {
+            if (etag != 0) {
+                if (pls) {  // the trial is the point at the clamped parameter
+                    float qt[2] = {qs[0] + alpha * dq[0], qs[1] + alpha * dq[1]};
+                    par_clamp(etag, eb + 1, qt);
+                    par_eval(etag, eb + 1, qt, tr);
+                } else {
+                    float raw[D];
+#pragma unroll
+                    for (int k = 0; k < D; ++k) raw[k] = tr[k];
+                    if (ent_is_seeded(etag))
+                        seeded_frame(etag, eb, raw, sd, false, tr, (float(*)[3]) nullptr);
+                    else
+                        ent_frame(etag, eb, raw, tr, (float(*)[3]) nullptr);
+                }
+            }
         }
         float et = 0.0f, mt = kInf;
         if constexpr (ENT) {
@@ -500,6 +615,13 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
     if (l == 0) {
         store_node<D>(a, dof, cur);
         count_update(a, ntr);
+        if constexpr (ENT) {
+            if (etag != 0) {
+                a.con_seed[2 * ci] = sd[0];
+                a.con_seed[2 * ci + 1] = sd[1];
+                a.con_epoch[ci] = *a.run_epoch;
+            }
+        }
     }
 }
 

@@ -438,6 +438,38 @@ def cubed_sphere_3d(n: int = 65, *, r: float = 1.0, half: float = 2.0) -> Grid:
     return Grid(3, ids, X, free)
 
 
+def spherical_shell_3d(n: int = 17, *, r: float = 1.0, half: float = 2.0, seed: int = 0, amp: float = 0.1,
+                       axis=(1.0, 0.37, 0.61)) -> tuple[Grid, dict]:
+    """The six shell blocks of ``cubed_sphere_3d(n)`` alone: the sphere of
+    radius r becomes a wall whose nodes slide on it (returned as
+    ``{node: ("sphere", (c, r, ax, ay))}`` for the wire's ``entities``, the
+    sphere's polar axis along ``axis`` so that no node sits on a pole of its
+    chart), the outer cube stays fixed. The sphere nodes start rotated by a
+    small seeded rotation's worth (up to ``amp`` of their spacing), the
+    others moved by up to ``amp`` of the local spacing."""
+    g = cubed_sphere_3d(n, r=r, half=half)
+    used = np.unique(np.concatenate([np.asarray(b).reshape(-1) for b in g.blocks[1:]]))
+    relabel = np.full(g.n_nodes, -1, np.int64)
+    relabel[used] = np.arange(used.size)
+    blocks = [relabel[np.asarray(b)].astype(np.int32) for b in g.blocks[1:]]
+    X = g.X[used].copy()
+    on = np.isclose(np.linalg.norm(X, axis=1), r)
+    free = ~(np.abs(X) >= half - 1e-9).any(axis=1)
+    rng = np.random.default_rng(seed)
+    h = local_edge_length(Grid(3, blocks, g.X[used], free))
+    step = amp * h[on, None] * rng.uniform(-1, 1, (int(on.sum()), 3))
+    P = X[on] + step
+    X[on] = r * P / np.linalg.norm(P, axis=1, keepdims=True)
+    inner = free & ~on
+    X[inner] += amp * h[inner, None] * rng.uniform(-1, 1, (int(inner.sum()), 3))
+    az = np.asarray(axis, float) / np.linalg.norm(axis)
+    ax = np.cross(az, (0.0, 0.0, 1.0))
+    ax /= np.linalg.norm(ax)
+    ay = np.cross(az, ax)
+    rec = (0.0, 0.0, 0.0, float(r), *ax, *ay)
+    return Grid(3, blocks, X, free), {int(i): ("sphere", rec) for i in np.flatnonzero(on)}
+
+
 def _merge_fast(block_coords: list, d: int, scale: float = 1e8):
     """Vectorized coincident-node merge (rounded coordinates)."""
     flat = [P.reshape(-1, d) for P in block_coords]

@@ -11,8 +11,9 @@ core synthesizes its patch from ``(interior_block, interior_logical)``.
 Every other moving DOF ships its incident samples (ascending sample id, see
 :mod:`egg_amd.smoothing.wire_tables` for the sample order and the roles),
 ``P_of`` of them. Optional parts: per-sample targets ``W_inv`` and energy
-weights, boundary DOFs sliding on an entity (2D: segment, circle, ellipse,
-rail), block-interface curvature windows (2D) and directional terms.
+weights, boundary DOFs sliding on an entity (:mod:`egg_amd.smoothing.entities`:
+2D curves, 3D curves and surfaces), block-interface curvature windows (2D)
+and directional terms.
 """
 
 from __future__ import annotations
@@ -20,16 +21,10 @@ from __future__ import annotations
 import numpy as np
 
 from .directional import directional_dof_table, directional_energy_table
+from .entities import ENTITY_TYPES, entity_groups
 from .wire_tables import CellSamples, per_dof_slots
 
 __all__ = ["ENTITY_TYPES", "build_flat_context", "cell_stencil", "curvature_table"]
-
-# Entity kinds a boundary DOF can slide on here: (wire tag, record length),
-# the reference's per-type SoA records (egg/geometry/entity_soa.py:109-155):
-# segment (sx, sy, ex, ey), circle (cx, cy, r), ellipse (cx, cy, rx, ry),
-# rail (sx, sy, ex, ey; the node is released beyond its ends).
-ENTITY_TYPES = {"lineseg": (1, 4), "circle": (2, 3), "ellipse": (3, 4), "linerail": (15, 4)}
-
 
 def cell_stencil(blocks: list, d: int) -> dict:
     """The whole sample table of ``blocks``: gc, gn[k], s[k] per sample and
@@ -39,39 +34,6 @@ def cell_stencil(blocks: list, d: int) -> dict:
     node, sid, role = cs.incidence(np.ones(int(cs.cells.max()) + 1 if cs.cells.size else 0, bool))
     return {"gc": f["gc"], "gn": [f[f"gn{k}"] for k in range(d)], "s": [f[f"s{k}"] for k in range(d)],
             "m_node": node, "m_sid": sid, "m_role": role, "nc": cs.nc, "ns": cs.n_samples, "ncell": cs.n_cells}
-
-
-def _entity_groups(dofs: np.ndarray, entities: dict | None) -> dict:
-    """The group's ``entities`` field (reference egg/geometry/entity_soa.py:
-    598-680): one sub-dict per kind present, ordered by its first DOF, with
-    the group-local indices of its DOFs (ascending) and their records packed
-    back to back; DOFs not listed slide on nothing ("free", tag 0)."""
-    if not entities:
-        return {}
-    kind_of = np.full(dofs.size, "free", dtype=object)
-    rec_of: list = [()] * dofs.size
-    for node, (kind, rec) in entities.items():
-        i = int(np.searchsorted(dofs, node))
-        if i < dofs.size and dofs[i] == node:
-            kind_of[i] = kind
-            rec_of[i] = tuple(rec)
-    out = {}
-    kinds = sorted(set(kind_of.tolist()), key=lambda k: int(np.argmax(kind_of == k)))
-    for kind in kinds:
-        if kind == "free":
-            tag, width = 0, 0
-        elif kind in ENTITY_TYPES:
-            tag, width = ENTITY_TYPES[kind]
-        else:
-            raise ValueError(f"entity type {kind!r} is not supported ({', '.join(ENTITY_TYPES)})")
-        local = np.flatnonzero(kind_of == kind)
-        recs = [rec_of[i] for i in local]
-        bad = [r for r in recs if len(r) != width]
-        if bad:
-            raise ValueError(f"a {kind} record holds {width} values, got {len(bad[0])}")
-        out[kind] = {"tag": tag, "kFields": width, "dof_local": local.astype(np.int32),
-                     "records": np.asarray(recs, np.float64).reshape(-1)}
-    return out
 
 
 def curvature_table(free_mask: np.ndarray, nodes: np.ndarray, weight: np.ndarray) -> dict:
@@ -116,9 +78,9 @@ def build_flat_context(blocks: list, free_mask: np.ndarray, d: int, *, w_inv: np
     ``w_inv``: one ``(d, d)`` inverse target (identity when omitted) or one
     per sample ``(ns, d, d)``. ``weight``: optional ``(ns,)`` per-sample
     energy weight (scales a sample's mu, gradient and Hessian, not its det);
-    a weighted wire ships every moving DOF's samples. ``entities``: 2D
-    ``{node: (kind, record)}`` with a kind of :data:`ENTITY_TYPES`; those
-    DOFs ship their samples. ``curvature``: ``(nodes (W, 4), weight (W,))``
+    a weighted wire ships every moving DOF's samples. ``entities``:
+    ``{node: (kind, spec)}`` with a kind of :data:`ENTITY_TYPES` (specs in
+    :mod:`egg_amd.smoothing.entities`); those DOFs ship their samples. ``curvature``: ``(nodes (W, 4), weight (W,))``
     block-interface windows (2D). ``directional``: a ``DirectionalSamples``."""
     blocks = [np.asarray(b) for b in blocks]
     cs = CellSamples(blocks, d)
@@ -133,8 +95,6 @@ def build_flat_context(blocks: list, free_mask: np.ndarray, d: int, *, w_inv: np
         if weight.shape != (ns,):
             raise ValueError(f"weight must hold one value per energy-stencil sample ({ns}), got {weight.shape}")
         synthesize = False
-    if entities and d != 2:
-        raise ValueError("constrained boundary DOFs are 2D only here")
 
     where, logical = (_interior_nodes(blocks, free.size, d) if synthesize
                       else (np.full(free.size, -1, np.int32), np.zeros((free.size, d), np.int32)))
@@ -160,7 +120,7 @@ def build_flat_context(blocks: list, free_mask: np.ndarray, d: int, *, w_inv: np
             "P_of": np.bincount(np.searchsorted(dofs, node), minlength=dofs.size).astype(np.int32),
             "interior_block": np.ascontiguousarray(where[dofs]),
             "interior_logical": np.ascontiguousarray(logical[dofs]),
-            "entities": _entity_groups(dofs, entities),
+            "entities": entity_groups(dofs, entities),
         }
         group.update(sample_part(sid))
         if curvature is not None:

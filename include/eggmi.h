@@ -48,12 +48,22 @@ typedef struct {
     const double* weight;             /* [total] per-occurrence energy weight (scales mu's value,
                                          gradient and Hessian, not det), NULL = 1
                                          (src/patch.hpp:45-77, :250-292) */
-    const int32_t* entity_tag;        /* [ndof] entity a boundary DOF slides on (2D): 0 free,
-                                         1 segment, 2 circle, 3 ellipse, 15 rail; NULL = all free
-                                         (the group's "entities", src/bindings.cpp:412-446) */
-    const double* entity_rec;         /* [ndof*4] its record: segment / rail (sx, sy, ex, ey),
-                                         circle (cx, cy, r, -), ellipse (cx, cy, rx, ry)
-                                         (src/geometry.hpp:288-464) */
+    const int32_t* entity_tag;        /* [ndof] entity kind a boundary DOF slides on, the
+                                         reference's tags (src/entity_soa.hpp:37-55): 0 free,
+                                         2D 1 segment, 2 circle, 3 ellipse, 6 circle arc,
+                                         7 ellipse arc, 8 / 9 quadratic / cubic Bezier,
+                                         10 B-spline (NURBS) curve, 11 composite path,
+                                         15 rail; 3D 4 sphere, 5 plane, 12 cylinder,
+                                         13 line, 14 B-spline (NURBS) surface, 16 rail;
+                                         NULL = all free (the group's "entities",
+                                         src/bindings.cpp:412-446) */
+    const int64_t* entity_off;        /* [ndof] start of the DOF's entity record in
+                                         entity_data (ignored for tag 0); DOFs on one entity
+                                         may share a record */
+    const double* entity_data;        /* [entity_len] the records: value 0 the tag, then
+                                         the kind's fields and payloads (egg_amd/csrc/entity.hpp
+                                         lists them; src/geometry.hpp:1717-2520) */
+    int64_t entity_len;
     int32_t curv_k;                   /* block-interface C2 curvature windows per DOF (2D; 0 =
                                          none; src/curvature.hpp, src/sweep.hpp:1461-1487) */
     const int32_t* curv_nodes;        /* [ndof*curv_k*4] global node ids of each window */
@@ -172,15 +182,17 @@ eggmi_status eggmi_mg_level_tables(eggmi_session* s, int level, int64_t* n_slots
  * fields; equal by construction of tau, src/fas.hpp:522-532). */
 eggmi_status eggmi_mg_probe(eggmi_session* s, int level, int op, const double* in, double* out);
 
-/* Projection of n 2D points q[n*2] onto one entity and the unit tangent
- * there, in FP64 with the same closed forms the constrained sweep uses:
- * tag 0 free (identity, tangent e_0), 1 segment, 2 circle, 3 ellipse
- * (radial scaling), 15 rail (released beyond its ends), record as on the
- * wire. proj / tangent (n*2 each) may be null. Replaces the host geometry
- * calls of the binding for these kinds: geometry_project / geometry_tangent
- * / geometry_project_batch (src/bindings.cpp:1872-1990). */
-eggmi_status eggmi_geometry_eval(int tag, const double* record, int64_t n, const double* q, double* proj,
-                                 double* tangent);
+/* Projection of n points q[n*dim] onto one entity and the orthonormal
+ * tangent basis there, in FP64 with the projections the constrained sweep
+ * uses (cold: no warm-start parameter): record[0..len-1] as in
+ * group.entity_data (record[0] = 0: free, identity and the unit basis), dim
+ * 2 or 3 the entity's. proj (n*dim) and basis (n*dim*k, column j of point i
+ * at [(i*dim + r)*k + j], k the tangent dimension: 1 for curves, 2 for
+ * surfaces, dim when free) may be null. Replaces the host geometry calls of
+ * the binding: geometry_project / geometry_tangent / geometry_project_batch
+ * (src/bindings.cpp:1872-1990, src/geometry.hpp:195-258, :2580-2612). */
+eggmi_status eggmi_geometry_eval(int dim, const double* record, int64_t len, int64_t n, const double* q,
+                                 double* proj, double* basis);
 
 eggmi_status eggmi_get_X(eggmi_session* s, double* X_out, int64_t n);
 

@@ -38,6 +38,9 @@
  *                 same way (straight and circular windows have zero residual)
  *   o_entity_eval src/geometry.hpp:288-464 projections and tangents, pinned
  *                 by tests/cpp/golden_geometry.hpp
+ *   o_entity_frame, update_constrained  every entity kind and the cold /
+ *                 warm constrained update (entity_oracle.c lists the
+ *                 reference code and the pins)
  *
  * Pinning: golden vectors from tests/cpp/golden_metric.hpp,
  * golden_metric_3d.hpp and golden_patch.hpp (tests/golden/*.json), finite
@@ -774,59 +777,133 @@ void FN(o_entity_eval)(int tag, const real* rec, const real* p, real* proj, real
     FN(ent_tangent)(tag, rec, p, tang);
 }
 
-/* One constrained DOF's update (reference src/sweep.hpp:1285-1403 and
- * :1848-1871): tangent-reduced Newton step delta = t * y with
- * y = solve1x1(t.H.t, t.g) (src/solve.hpp:83-90, src/patch.hpp:348-388),
- * damped by omega on the step (no post-blend); trials pos + alpha delta are
- * projected onto the entity; the same accept rule as the free DOFs. */
-static void FN(update_constrained)(int kind, real delta, real omega, const FN(patch_t)* pv, const real* X,
-                                   int dof, int tag, const real* ent, const dir_t* dw, int d, real* out)
+#include "entity_oracle.c"
+
+/* Tangent-reduced Newton step delta = B y from the orthonormal basis
+ * B[0..K-1] (reference src/patch.hpp:348-388): M = B^T H B, r = B^T g, y =
+ * solve1x1(M, r) (src/solve.hpp:83-90) or the 2x2 node solve. */
+static void FN(reduced_step)(int dim, const real* hess, const real* grad, real B[2][3], int K, real* dl)
+{
+    real M[4] = {0, 0, 0, 0}, r[2] = {0, 0}, y[2] = {0, 0};
+    for (int a = 0; a < K; ++a) {
+        real Hb[3];
+        for (int i = 0; i < dim; ++i) {
+            Hb[i] = 0;
+            for (int j = 0; j < dim; ++j) Hb[i] += hess[i * dim + j] * B[a][j];
+        }
+        for (int c = 0; c < K; ++c) {
+            M[a * K + c] = 0;
+            for (int i = 0; i < dim; ++i) M[a * K + c] += B[c][i] * Hb[i];
+        }
+        for (int i = 0; i < dim; ++i) r[a] += B[a][i] * grad[i];
+    }
+    if (K == 1) {
+        const real rn = fabs(r[0]);
+        if (rn >= TOL_ZNORM) {
+            y[0] = -r[0] / M[0];
+            if (!isfinite(y[0])) y[0] = (real)-0.1 * r[0] / rn;
+        }
+    } else {
+        FN(o_solve)(2, M, r, y);
+    }
+    for (int i = 0; i < dim; ++i) dl[i] = B[0][i] * y[0] + (K == 2 ? B[1][i] * y[1] : 0);
+}
+
+/* One constrained DOF's update. Cold (the first sweep of a run; reference
+ * src/sweep.hpp:1285-1403): tangent-reduced Newton step at the entity's
+ * frame of the DOF (a rail beyond its ends releases the node: full-space
+ * step, src/patch.hpp:400-409), damped by omega on the step, trials pos +
+ * alpha delta projected onto the entity (the iterative kinds from the
+ * DOF's seed, which each projection refreshes). Warm (later sweeps, the
+ * parametrized kinds; :1717-1844): project once to the foot q* (seeded),
+ * seed = q*, the step reduced to dq = (J^T J)^-1 J^T delta times omega, trials
+ * eval(trim clamp(q* + alpha dq)). The free DOFs' accept rule; no post-blend. */
+static void FN(update_constrained)(int dim, int kind, real delta, real omega, const FN(patch_t)* pv, const real* X,
+                                   int dof, int tag, const real* blob, real* seed, int cold, const dir_t* dw, int d,
+                                   real* out)
 {
     real grad[3], hess[9], e0, md;
-    FN(patch_eval_impl)(2, kind, delta, pv, X, grad, hess, &e0, &md);
+    FN(patch_eval_impl)(dim, kind, delta, pv, X, grad, hess, &e0, &md);
     if (dw) /* directional windows (src/sweep.hpp:1746-1760) */
-        FN(dir_dof_accumulate)(2, dw, d, X, grad, hess, &e0);
-    const real pos[2] = {X[(size_t)dof * 2], X[(size_t)dof * 2 + 1]};
-    real dl[2];
-    const real ft = tag == ENT_RAIL ? FN(rail_foot)(ent, pos) : 0;
-    if (tag == ENT_RAIL && (ft < 0 || ft > 1)) {
-        /* a rail whose foot lies beyond an end releases the node: full-space
-           step (src/patch.hpp:400-409) */
-        FN(o_solve)(2, hess, grad, dl);
-        dl[0] *= omega;
-        dl[1] *= omega;
-    } else {
-        real t[2];
-        FN(ent_tangent)(tag, ent, pos, t);
-        const real Ht0 = hess[0] * t[0] + hess[1] * t[1], Ht1 = hess[2] * t[0] + hess[3] * t[1];
-        const real M = t[0] * Ht0 + t[1] * Ht1;
-        const real r = t[0] * grad[0] + t[1] * grad[1];
-        const real rn = fabs(r);
-        real y = 0;
-        if (rn >= TOL_ZNORM) {
-            y = -r / M;
-            if (!isfinite(y)) y = (real)-0.1 * r / rn;
+        FN(dir_dof_accumulate)(dim, dw, d, X, grad, hess, &e0);
+    real pos[3] = {0, 0, 0};
+    for (int k = 0; k < dim; ++k) pos[k] = X[(size_t)dof * dim + k];
+    const int K = eo_tdim(tag);
+    real dl[3] = {0, 0, 0}, qs[2] = {0, 0}, dq[2] = {0, 0}, B[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    int pls = 0;
+    if (!cold && eo_param(tag)) {
+        pls = 1;
+        real J[2][3] = {{0, 0, 0}, {0, 0, 0}};
+        FN(eo_invert)(tag, blob + 1, pos, seed, isfinite(seed[0]), 1, qs, tag == 14 ? J : NULL);
+        if (tag != 14) FN(eo_frame)(tag, blob + 1, qs, J);
+        seed[0] = qs[0];
+        seed[1] = qs[1];
+        memcpy(B, J, sizeof(B));
+        FN(eo_gram_schmidt)(B, K, dim);
+        real del[3];
+        FN(reduced_step)(dim, hess, grad, B, K, del);
+        real JtJ[4] = {0, 0, 0, 0}, Jtd[2] = {0, 0};
+        for (int a = 0; a < K; ++a) {
+            for (int c = 0; c < K; ++c) JtJ[a * 2 + c] = FN(eo_dot)(J[a], J[c], dim);
+            Jtd[a] = FN(eo_dot)(J[a], del, dim);
         }
-        dl[0] = omega * (t[0] * y);
-        dl[1] = omega * (t[1] * y);
+        if (K == 1) {
+            dq[0] = fabs(JtJ[0]) > EO_TINY ? Jtd[0] / JtJ[0] : 0;
+        } else {
+            const real det = JtJ[0] * JtJ[3] - JtJ[1] * JtJ[2];
+            if (fabs(det) >= EO_TINY) {
+                dq[0] = (JtJ[3] * Jtd[0] - JtJ[1] * Jtd[1]) / det;
+                dq[1] = (-JtJ[2] * Jtd[0] + JtJ[0] * Jtd[1]) / det;
+            }
+        }
+        dq[0] *= omega;
+        dq[1] *= omega;
+    } else {
+        int release = 0;
+        if (tag == 15 || tag == 16) {
+            real ab2 = 0, s = 0;
+            const real* e = blob + 1;
+            for (int k = 0; k < dim; ++k)
+                ab2 += (e[dim + k] - e[k]) * (e[dim + k] - e[k]), s += (pos[k] - e[k]) * (e[dim + k] - e[k]);
+            const real ft = s / (ab2 > EO_TINY ? ab2 : EO_TINY);
+            release = ft < 0 || ft > 1;
+        }
+        if (release)
+            FN(o_solve)(dim, hess, grad, dl);
+        else {
+            if (eo_seeded(tag))
+                FN(eo_seeded_frame)(tag, blob, pos, seed, 0, NULL, B);
+            else
+                FN(eo_entity_frame)(tag, blob, pos, NULL, B);
+            FN(reduced_step)(dim, hess, grad, B, K, dl);
+        }
+        for (int k = 0; k < dim; ++k) dl[k] *= omega;
     }
-    real alpha = 1, cur[2] = {pos[0], pos[1]};
+    real alpha = 1, cur[3] = {pos[0], pos[1], pos[2]};
     for (int it = 0; it < MAX_BACKTRACKS; ++it) {
-        const real raw[2] = {pos[0] + alpha * dl[0], pos[1] + alpha * dl[1]};
-        real trial[2], e_new, mdet;
-        FN(ent_project)(tag, ent, raw, trial);
-        FN(trial_impl)(2, kind, delta, pv, X, dof, trial, &e_new, &mdet);
-        if (dw) e_new += FN(dir_dof_trial)(2, dw, d, X, trial);
+        real trial[3] = {0, 0, 0}, e_new, mdet;
+        if (pls) {
+            real qt[2] = {qs[0] + alpha * dq[0], qs[1] + alpha * dq[1]};
+            FN(eo_clamp_trim)(tag, blob + 1, qt);
+            FN(eo_eval)(tag, blob + 1, qt, trial);
+        } else {
+            real raw[3] = {0, 0, 0};
+            for (int k = 0; k < dim; ++k) raw[k] = pos[k] + alpha * dl[k];
+            if (eo_seeded(tag))
+                FN(eo_seeded_frame)(tag, blob, raw, seed, 0, trial, NULL);
+            else
+                FN(eo_entity_frame)(tag, blob, raw, trial, NULL);
+        }
+        FN(trial_impl)(dim, kind, delta, pv, X, dof, trial, &e_new, &mdet);
+        if (dw) e_new += FN(dir_dof_trial)(dim, dw, d, X, trial);
         const int ok = isfinite(e_new) && (e_new <= e0 + TOL_ENERGY) && (kind == KIND_UNTANGLE || mdet > 0);
         if (ok) {
-            cur[0] = trial[0];
-            cur[1] = trial[1];
+            for (int k = 0; k < dim; ++k) cur[k] = trial[k];
             break;
         }
         alpha *= (real)0.5;
     }
-    out[0] = cur[0];
-    out[1] = cur[1];
+    for (int k = 0; k < dim; ++k) out[k] = cur[k];
 }
 
 /* ------------------------------------------- block-interface C2 curvature */
@@ -985,7 +1062,7 @@ static void FN(update_dof)(int dim, int kind, real delta, real omega, const FN(p
 int FN(o_jacobi)(int dim, int kind, double delta, double omega, int n_sweeps, int report_every, int ndof,
                  const int* dof, const long long* poff, const int* pgc, const int* pgn, const real* ps,
                  const int* prole, const real* W, int w_stride, const real* pwt, const int* etag,
-                 const real* erec, int curv_k, const int* cnodes, const int* cslot, const real* cwt,
+                 const long long* eoff, const real* eblob, real* eseed, int curv_k, const int* cnodes, const int* cslot, const real* cwt,
                  const dir_t* dw, const dir_t* dirs, int n_nodes, real* X, const real* tau, long long ns, const int* egc,
                  const int* egn, const real* es, const real* eW, int ew_stride, const real* ewt, real* work,
                  double* energies, double* mindets)
@@ -1011,9 +1088,10 @@ int FN(o_jacobi)(int dim, int kind, double delta, double omega, int n_sweeps, in
                 }
             FN(patch_t) pv = {P, pgc + o, gn_l, s_l, prole + o, W + (size_t)w_stride * o, w_stride,
                               pwt ? pwt + o : NULL};
-            if (etag && etag[d])  /* constrained boundary DOF (2D) */
-                FN(update_constrained)(kind, (real)delta, (real)omega, &pv, xin, dof[d], etag[d], erec + 4 * d,
-                                       dw && dw->K ? dw : NULL, d, xout + (size_t)dof[d] * dim);
+            if (etag && etag[d]) /* constrained boundary DOF; the first sweep of a run is cold */
+                FN(update_constrained)(dim, kind, (real)delta, (real)omega, &pv, xin, dof[d], etag[d],
+                                       eblob + eoff[d], eseed + 2 * (size_t)d, sw == 0, dw && dw->K ? dw : NULL, d,
+                                       xout + (size_t)dof[d] * dim);
             else {
                 FN(curv_t) cv = {curv_k, cnodes ? cnodes + (size_t)4 * curv_k * d : NULL,
                                  cslot ? cslot + (size_t)curv_k * d : NULL, cwt ? cwt + (size_t)curv_k * d : NULL};

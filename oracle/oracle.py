@@ -63,6 +63,12 @@ def lib():
             f = getattr(L, "o_entity_eval" + sfx)
             f.argtypes = [C.c_int, rp, rp, rp, rp]
             f.restype = None
+            f = getattr(L, "o_entity_frame" + sfx)
+            f.argtypes = [rp, rp, rp, rp]
+            f.restype = None
+            f = getattr(L, "o_entity_param" + sfx)
+            f.argtypes = [rp, C.c_int, rp, C.c_void_p, rp]
+            f.restype = None
             f = getattr(L, "o_curv_eval" + sfx)
             f.argtypes = [rp, rt, C.c_int, rp, rp, rp]
             f.restype = None
@@ -76,8 +82,8 @@ def lib():
             f = getattr(L, "o_jacobi" + sfx)
             f.argtypes = [
                 C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
-                _ip, _lp, _ip, _ip, rp, _ip, rp, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
-                C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, rp, C.c_void_p,
+                _ip, _lp, _ip, _ip, rp, _ip, rp, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, rp, C.c_void_p,
                 C.c_longlong, _ip, _ip, rp, rp, C.c_int, C.c_void_p, rp, _dp, _dp,
             ]
             f.restype = C.c_int
@@ -183,6 +189,122 @@ def solve(dim, H, g, prec="f64"):
     return x
 
 
+# ---- entity records: the reference wire's per-kind SoA fields and CSR
+# payloads (egg/geometry/entity_soa.py:41-60, :598-688) as one record per
+# entity (value 0 the tag, then fields and payloads, as the compute core
+# lays them out; sphere and cylinder gain az = ax x ay, src/geometry.hpp:
+# 1585-1606; a composite's segments come from its positional arena records,
+# egg/geometry/entity_encoding.py:82-178).
+WIRE_FIELDS = {0: 0, 1: 4, 2: 3, 3: 4, 4: 10, 5: 9, 6: 6, 7: 8, 8: 8, 9: 10, 10: 6, 11: 2, 12: 10, 13: 8, 14: 9,
+               15: 4, 16: 6}
+
+
+def _bspline_rec(deg, n, t0, t1, closed, has_w, knots, ctrl, w):
+    deg, n = int(deg), int(n)
+    assert len(knots) == n + deg + 1 and len(ctrl) == 2 * n and (not has_w or len(w) == n)
+    out = [10.0, deg, n, t0, t1, closed, has_w, *knots, *ctrl]
+    return out + (list(w) if has_w else [])
+
+
+def entity_record(tag: int, rec, seg=()) -> np.ndarray:
+    """One entity's record from its wire fields ``rec`` and payload arrays
+    ``seg`` (one per CSR slot, this entity's slice)."""
+    r = [float(v) for v in np.asarray(rec, np.float64).reshape(-1)]
+    seg = [np.asarray(a, np.float64).reshape(-1) for a in seg]
+    if tag in (4, 12):
+        ax = np.array(r[4:7] if tag == 4 else r[3:6])
+        ay = np.array(r[7:10] if tag == 4 else r[6:9])
+        az = list(np.cross(ax, ay))
+        out = [float(tag)] + (r[:10] + az if tag == 4 else r[:9] + az + [r[9]])
+    elif tag == 10:
+        out = _bspline_rec(*r[:6], seg[0], seg[1], seg[2] if len(seg) > 2 else [])
+    elif tag == 11:
+        arena = seg[0]
+        m, ro = int(r[0]), int(r[1])
+        subs = []
+        for i in range(m):
+            sr = arena[ro + 13 * i: ro + 13 * (i + 1)]
+            st, q = int(sr[0]), sr[1:]
+            if st == 10:
+                n, deg = int(q[1]), int(q[0])
+                kn = arena[int(q[2]): int(q[2]) + n + deg + 1]
+                cp = arena[int(q[3]): int(q[3]) + 2 * n]
+                ww = arena[int(q[6]): int(q[6]) + n] if q[7] else []
+                subs.append(_bspline_rec(q[0], q[1], q[4], q[5], 0.0, q[7], kn, cp, ww))
+            else:
+                subs.append([float(st)] + list(q[:WIRE_FIELDS[st]]))
+        out = [11.0, float(m)] + [0.0] * m
+        for i, sb in enumerate(subs):
+            out[2 + i] = float(len(out))
+            out += sb
+    elif tag == 14:
+        ku, kv, cp, ww, verts, loops = seg
+        out = [14.0, r[0], r[1], r[2], r[3], r[8], float(len(loops)), *ku, *kv, *cp]
+        out += list(ww) if r[8] else []
+        out += list(loops) + (list(verts) if len(loops) >= 2 else [])
+    else:
+        out = [float(tag)] + r[:WIRE_FIELDS[tag]]
+    return np.asarray(out, np.float64)
+
+
+def wire_entities(g: dict, ndof: int):
+    """Per group-local DOF: entity tag (0 free), record offset and the
+    records, from a wire group's ``entities``; (None, None, None) when no
+    DOF is constrained."""
+    ents = g.get("entities") or {}
+    tag = np.zeros(ndof, np.int32)
+    off = np.zeros(ndof, np.int64)
+    blobs = []
+    size = 0
+    for part in ents.values():
+        t, kf = int(part["tag"]), int(part["kFields"])
+        loc = np.asarray(part["dof_local"], np.int64)
+        recs = np.asarray(part.get("records", []), np.float64).reshape(len(loc), kf) if kf else None
+        seg = part.get("seg") or []
+        for i, l in enumerate(loc):
+            tag[l] = t
+            if t == 0:
+                continue
+            sl = [np.asarray(sd["data"], np.float64)[int(sd["off"][i]): int(sd["off"][i + 1])] for sd in seg]
+            b = entity_record(t, recs[i], sl)
+            off[l] = size
+            blobs.append(b)
+            size += b.size
+    if not np.any(tag):
+        return None, None, None
+    return tag, off, np.concatenate(blobs)
+
+
+def entity_frame(record, p, prec="f64"):
+    """(foot, orthonormal tangent basis (dim, k)) of the point p on one
+    entity record."""
+    dt = _dt(prec)
+    rec = np.ascontiguousarray(record, dt)
+    tag = int(rec[0])
+    dim = 3 if tag in (4, 5, 12, 13, 14, 16) else 2
+    k = 2 if tag in (4, 5, 12, 14) else 1
+    pos = np.zeros(dim, dt)
+    basis = np.zeros(k * dim, dt)
+    getattr(lib(), f"o_entity_frame_{prec}")(rec, np.ascontiguousarray(p, dt), pos, basis)
+    return pos, basis.reshape(k, dim).T
+
+
+def entity_param(record, mode, x, seed=None, prec="f64"):
+    """The record's parametrization: mode 0 point at x, 1 raw frame at x
+    (dim, k), 2 foot parameter of the point x (cold; from ``seed`` for the
+    iterative kinds when given)."""
+    dt = _dt(prec)
+    rec = np.ascontiguousarray(record, dt)
+    tag = int(rec[0])
+    dim = 3 if tag in (4, 5, 12, 13, 14, 16) else 2
+    k = 2 if tag in (4, 5, 12, 14) else 1
+    out = np.zeros(dim if mode == 0 else (k * dim if mode == 1 else k), dt)
+    sd = None if seed is None else np.ascontiguousarray(np.resize(np.asarray(seed, dt), 2))
+    getattr(lib(), f"o_entity_param_{prec}")(rec, int(mode), np.ascontiguousarray(np.resize(np.asarray(x, dt), 3)),
+                                             _ptr(sd), out)
+    return out.reshape(k, dim).T if mode == 1 else out
+
+
 def entity_eval(tag: int, record, p, prec="f64"):
     """Projection of the 2D point p onto entity (tag, record) and the unit
     tangent there (restating the reference's geometry_project /
@@ -267,7 +389,8 @@ def interior_occurrences(dim: int, logical, idx):
 class Patches:
     """Per-DOF patches in global id space (CSR over DOFs)."""
 
-    def __init__(self, dim, dof, poff, gc, gn, s, role, W, wt=None, etag=None, erec=None, curv=None, dirw=None):
+    def __init__(self, dim, dof, poff, gc, gn, s, role, W, wt=None, etag=None, eoff=None, eblob=None, curv=None,
+                 dirw=None):
         self.dim = dim
         self.dof = np.ascontiguousarray(dof, np.int32)
         self.poff = np.ascontiguousarray(poff, np.int64)
@@ -277,9 +400,11 @@ class Patches:
         self.role = np.ascontiguousarray(role, np.int32)
         self.W = np.ascontiguousarray(W, np.float64)  # (dd,) uniform or (total, dd)
         self.wt = None if wt is None else np.ascontiguousarray(wt, np.float64)  # (total,) or None = unit
-        # per-DOF entity a constrained boundary DOF slides on (0 free, 1 segment, 2 circle) and its record
+        # per-DOF entity a constrained boundary DOF slides on (0 free) and
+        # the offset of its record in eblob (wire_entities)
         self.etag = None if etag is None else np.ascontiguousarray(etag, np.int32)
-        self.erec = None if erec is None else np.ascontiguousarray(erec, np.float64).reshape(-1, 4)
+        self.eoff = None if eoff is None else np.ascontiguousarray(eoff, np.int64)
+        self.eblob = None if eblob is None else np.ascontiguousarray(eblob, np.float64)
         # per-DOF curvature windows (K, nodes (ndof, K, 4), slot (ndof, K), weight (ndof, K)) or None
         self.curv = curv
         # per-DOF directional windows (a wire dict with dir_k, dir_eps, dir_nodes,
@@ -361,9 +486,9 @@ def materialize_patches(wire: dict, block_global_dof: list[np.ndarray], dim: int
     if wire_wt is not None:
         ww = np.asarray(wire_wt, np.float64).reshape(-1)
         wt = np.concatenate([ww[soff[d]:soff[d] + P_of[d]] for d in range(n)]) if n else ww[:0]
-    etag, erec = _entity_rows(g, len(P_all))
+    etag, eoff, eblob = wire_entities(g, len(P_all))
     if etag is not None:
-        etag, erec = etag[sel], erec[sel]
+        etag, eoff = etag[sel], eoff[sel]
         if np.any(etag[interior] != 0):
             raise ValueError("a synthesized interior DOF cannot be constrained")
     curv = None
@@ -381,23 +506,7 @@ def materialize_patches(wire: dict, block_global_dof: list[np.ndarray], dim: int
                 "dir_kind": np.asarray(g["dir_kind"], np.int32).reshape(-1, K)[sel],
                 "dir_ref": np.asarray(g["dir_ref"], np.float64).reshape(-1, K, dim)[sel],
                 "dir_weight": np.asarray(g["dir_weight"], np.float64).reshape(-1, K)[sel]}
-    return Patches(dim, dof, poff, gc, gn, sg, role, Wp, wt, etag, erec, curv, dirw)
-
-
-def _entity_rows(g: dict, ndof: int):
-    """Per group-local DOF entity tag (0 free) and record (4 values) from
-    the wire's per-type ``entities`` dicts, or (None, None) when none is
-    constrained."""
-    ents = g.get("entities") or {}
-    tag = np.zeros(ndof, np.int32)
-    rec = np.zeros((ndof, 4))
-    for part in ents.values():
-        loc = np.asarray(part["dof_local"], np.int64)
-        kf = int(part["kFields"])
-        tag[loc] = int(part["tag"])
-        if kf:
-            rec[loc, :kf] = np.asarray(part["records"], np.float64).reshape(-1, kf)
-    return (tag, rec) if np.any(tag) else (None, None)
+    return Patches(dim, dof, poff, gc, gn, sg, role, Wp, wt, etag, eoff, eblob, curv, dirw)
 
 
 def materialize_patches_loop(wire: dict, block_global_dof: list[np.ndarray], dim: int) -> Patches:
@@ -441,9 +550,11 @@ def materialize_patches_loop(wire: dict, block_global_dof: list[np.ndarray], dim
 
 
 def jacobi(dim, kind, delta, omega, n_sweeps, X, patches: Patches, es: dict, *, report_every=0,
-           prec="f64", tau=None, threads=None):
-    """Run n_sweeps of block-Jacobi in global id space. Returns
-    (X_out, energies, mindets)."""
+           prec="f64", tau=None, threads=None, seeds=None):
+    """Run n_sweeps of block-Jacobi in global id space (one run: its first
+    sweep is cold for the constrained DOFs). ``seeds`` (ndof, 2; NaN = none)
+    carries the constrained DOFs' foot parameters between runs and is
+    updated in place. Returns (X_out, energies, mindets)."""
     dt = _dt(prec)
     if threads is not None:
         lib().o_set_threads(int(threads))
@@ -459,7 +570,11 @@ def jacobi(dim, kind, delta, omega, n_sweeps, X, patches: Patches, es: dict, *, 
     ws = 0 if W.size == dim * dim else dim * dim
     pwt = _weights(patches.wt, dt)
     etag = patches.etag
-    erec = None if patches.erec is None else np.ascontiguousarray(patches.erec, dt).reshape(-1)
+    eoff = patches.eoff
+    eblob = None if patches.eblob is None else np.ascontiguousarray(patches.eblob, dt)
+    if seeds is None:
+        seeds = np.full((len(patches.dof), 2), np.nan)
+    sd = np.ascontiguousarray(seeds, dt).reshape(-1)
     ck, cn, cs, cw = _curv_args(patches, dt)
     dw, keep_w = _dir_table(patches.dirw, dim, True)
     dts, keep_s = _dir_table(es, dim, False)
@@ -467,10 +582,15 @@ def jacobi(dim, kind, delta, omega, n_sweeps, X, patches: Patches, es: dict, *, 
     got = getattr(lib(), f"o_jacobi_{prec}")(
         dim, KIND[kind], float(delta), float(omega), int(n_sweeps), int(report_every),
         len(patches.dof), patches.dof, patches.poff, patches.gc, patches.gn.reshape(-1),
-        np.ascontiguousarray(patches.s, dt).reshape(-1), patches.role, W, ws, _ptr(pwt), _ptr(etag), _ptr(erec),
+        np.ascontiguousarray(patches.s, dt).reshape(-1), patches.role, W, ws, _ptr(pwt), _ptr(etag), _ptr(eoff), _ptr(eblob),
+        _ptr(sd),
         ck, _ptr(cn), _ptr(cs), _ptr(cw), _byref(dw), _byref(dts), n_nodes, Xw, _ptr(tau_arr), ns, egc, egn, es_, eW, ews, _ptr(ewt),
         work, en, md,
     )
+    if isinstance(seeds, np.ndarray) and seeds.dtype == sd.dtype and seeds.flags.c_contiguous:
+        pass  # updated in place
+    elif isinstance(seeds, np.ndarray):
+        seeds[...] = sd.reshape(seeds.shape)
     return Xw.reshape(-1, dim), en[:got], md[:got]
 
 

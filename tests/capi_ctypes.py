@@ -23,7 +23,8 @@ class Group(C.Structure):
                 ("total_samples", C.c_int64), ("role", C.c_void_p), ("gc", C.c_void_p),
                 ("gn", C.c_void_p * 3), ("s", C.c_void_p * 3),
                 ("W_inv", C.c_void_p), ("w_rows", C.c_int64), ("weight", C.c_void_p),
-                ("entity_tag", C.c_void_p), ("entity_rec", C.c_void_p),
+                ("entity_tag", C.c_void_p), ("entity_off", C.c_void_p),
+                ("entity_data", C.c_void_p), ("entity_len", C.c_int64),
                 ("curv_k", C.c_int32), ("curv_nodes", C.c_void_p), ("curv_slot", C.c_void_p),
                 ("curv_weight", C.c_void_p),
                 ("dir_k", C.c_int32), ("dir_eps", C.c_double), ("dir_nodes", C.c_void_p),

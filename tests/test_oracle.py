@@ -551,8 +551,11 @@ def test_entity_projection_golden(oracle):
             feet, basis = cpp_core.geometry_project_batch(np.asarray([s["p"]]), s["tag"], np.asarray(rec), [])
             np.testing.assert_allclose(feet[0], s["proj"], rtol=0, atol=1e-14)
             np.testing.assert_allclose(basis[0, :, 0], s["tang"], rtol=0, atol=1e-14)
-    with pytest.raises(ValueError, match="not supported"):
-        cpp_core.geometry_project(np.zeros(2), 7, np.zeros(12))
+    # the arena kinds go through the batch call; unknown tags are refused
+    with pytest.raises(ValueError, match="arena"):
+        cpp_core.geometry_project(np.zeros(2), 10, np.zeros(12))
+    with pytest.raises(ValueError, match="unknown entity tag"):
+        cpp_core.geometry_project(np.zeros(2), 17, np.zeros(12))
 
 
 def test_rail_projection_closed_form(oracle):

@@ -116,7 +116,9 @@ def test_targets_weights_and_entities_disable_synthesis_where_they_must():
                                           if part["kFields"] else [], want[1])
             assert grp["interior_block"][i] == -1 or kind == "free"
     with pytest.raises(ValueError, match="not supported"):
-        build_flat_context(ann.blocks, ann.free, 2, entities={int(dofs[0]): ("bspline", (1.0,))})
+        build_flat_context(ann.blocks, ann.free, 2, entities={int(dofs[0]): ("torus", (1.0,))})
+    with pytest.raises(ValueError, match="holds 3 values"):
+        build_flat_context(ann.blocks, ann.free, 2, entities={int(dofs[0]): ("circle", (1.0,))})
 
 
 @pytest.mark.parametrize("dim", [2, 3])
