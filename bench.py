@@ -52,6 +52,34 @@ PEAK_HBM_GBS = 8000.0
 SEED = 2026
 CONV_RTOL = 1e-6
 GRAPH_CHUNK = 256  # sweeps per captured hipGraph (5-8 % per sweep on the small grids)
+# multi-sweep candidates (sweeps per launch, core tile rows, columns, kernel
+# variant): the fine level's plain sweeps (k = 1: the one-sweep kernels) and
+# the FAS coarse levels' smoothing calls; each tuner keeps the fastest
+MS_FINE_CANDS = [(1, 0, 0, 0)] + [(k, ty, tx, v) for k in (2, 3, 4) for ty, tx in ((16, 8), (8, 16), (16, 16))
+                                  for v in (5, 1, 2)]
+MS_LEVEL_CANDS = [(k, ty, tx, v) for k in (2, 4, 8) for ty, tx in ((8, 8), (16, 8), (16, 16)) for v in (5, 1, 0)]
+
+
+def _tune_multi(s, level_sweeps, max_levels=32, fine=True):
+    """Multi-sweep tuners: the fine level's plain sweeps (40 sweeps per
+    candidate, captured) and each FAS coarse level's smoothing call of
+    level_sweeps sweeps. Returns what each picked against its one-sweep time."""
+    out = {}
+    if fine:
+        ms = s.core.autotune_multi(40, 0.8, MS_FINE_CANDS)
+        b = int(np.argmin(ms))
+        out["fine"] = {"one_sweep_us_per_sweep": float(ms[0]) / 40 * 1e3,
+                       "best_multi": list(MS_FINE_CANDS[1 + int(np.argmin(ms[1:]))]),
+                       "best_multi_us_per_sweep": float(np.min(ms[1:])) / 40 * 1e3,
+                       "picked": list(MS_FINE_CANDS[b])}
+    lv = s.core.autotune_levels_multi(level_sweeps, 0.8, MS_LEVEL_CANDS, max_levels)
+    levels = []
+    for i in range(lv.shape[0]):
+        info = s.core.multi_sweep_info(i + 1)
+        levels.append({"one_sweep_us": float(lv[i, 0]) * 1e3, "best_us": float(np.min(np.where(lv[i] < 0, np.inf, lv[i]))) * 1e3,
+                       "picked": [info["k"], info["ty"], info["tx"], info["kernel"]] if info["k"] else "one-sweep"})
+    out["fas_levels"] = {"sweeps_per_call": level_sweeps, "levels": levels}
+    return out
 # HBM bytes per sweep launch by workload and tile variant, from two rocprofv3
 # --pmc runs of this file (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch,
 # median; profiles/sweep2d_ogrid129.md, batched64_fas.md,
@@ -108,7 +136,7 @@ def _build(g):
     return ctx, bsc, CppStructuredSweepSession(ctx, bsc, g.X)
 
 
-def _roofline(s, dim, n_free, steps, omega, k_avg, traffic=None):
+def _roofline(s, dim, n_free, steps, omega, k_avg, traffic=None, sweeps_per_launch=1.0):
     """Roofline of the sweep kernel: k_avg = its average launch time (HIP
     events on the session stream around the timed region, which holds only
     back-to-back sweep launches); flops per launch from the op-counted
@@ -123,13 +151,15 @@ def _roofline(s, dim, n_free, steps, omega, k_avg, traffic=None):
     s.set_X(X0)
     assert updates == n_free * steps, (updates, n_free * steps)
     tpu = trials / updates
-    flops = n_free * (F_METRIC[dim] + tpu * F_TRIAL[dim])
+    flops = n_free * sweeps_per_launch * (F_METRIC[dim] + tpu * F_TRIAL[dim])
     ach = flops / k_avg / 1e12
+    kernel = f"sweep_kernel<{dim}D, shape>" if sweeps_per_launch == 1 else \
+        f"multi_sweep_kernel<2D, shape> ({sweeps_per_launch:g} sweeps per launch)"
     return {"bound": "valu", "achieved": ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-            "frac": ach / PEAK_FP32_TFLOPS, "traffic": traffic, "kernel": f"sweep_kernel<{dim}D, shape>",
+            "frac": ach / PEAK_FP32_TFLOPS, "traffic": traffic, "kernel": kernel,
             "kernel_avg_us": k_avg * 1e6, "flops_per_launch": flops, "trials_per_update": tpu,
             "flops_per_update": f"{F_METRIC[dim]} + trials x {F_TRIAL[dim]}",
-            "algorithmic_bytes_per_launch": n_free * 4 * dim * 2}
+            "algorithmic_bytes_per_launch": n_free * sweeps_per_launch * 4 * dim * 2}
 
 
 def _timed(s, n, omega):
@@ -447,13 +477,16 @@ def bench_ogrid(args):
     log(f"autotune: variant {tune_best}, ms per 20 sweeps {np.round(tune_ms, 4).tolist()}")
     lv_best, lv_ms = s.core.autotune_levels(16, 0.8)  # FAS coarse levels, from the untangled state
     log(f"level autotune: variants {lv_best}")
+    multi = _tune_multi(s, 8)
+    log(f"multi-sweep tuners: {multi}")
     s.core.set_graph_chunk(args.graph_chunk)  # replay sweeps as captured hipGraph chunks
     s.run(args.warmup, omega=0.8)
     e, m, t_run, k_avg, X_start = _timed(s, args.steps, 0.8)
     value = n_free * args.steps / t_run
+    spl = args.steps / s.core.last_timing()[1]  # sweeps per launch
     X_end = s.get_X()
     s.set_X(X_start)
-    roof = _roofline(s, 2, n_free, args.steps, 0.8, k_avg, _traffic("ogrid", tune_best))
+    roof = _roofline(s, 2, n_free, args.steps, 0.8, k_avg, _traffic("ogrid", tune_best) if spl == 1 else None, spl)
     s.set_X(X_end)
     log(f"timed {args.steps} sweeps {t_run * 1e3:.2f} ms -> {value:.4g} node-updates/s  E={e[-1]:.9g}")
     log(f"sweep kernel {roof['kernel_avg_us']:.2f} us/launch, {roof['trials_per_update']:.3f} trials/update "
@@ -559,6 +592,7 @@ def bench_ogrid(args):
         "config": {"workload": "2D 5-block O-grid (129x129 disc + 4x 129x65 rings, r=0.5 in [-2,2]^2), "
                                "2% tangled by 1.5h, untangle then block-Jacobi TMOP sweeps, omega 0.8",
                    "free_dofs": n_free, "seed": SEED, "precision": "FP32 sweeps, FP64 reductions"},
+        "multi_sweep_tuners": multi,
         "autotune": {"best_variant": int(tune_best), "ms_per_20_sweeps": [float(v) for v in tune_ms],
                      "fas_level_variants": [int(v) for v in lv_best],
                      "fas_level_ms_per_16_sweeps": np.round(lv_ms, 4).tolist()},
@@ -953,6 +987,7 @@ def bench_fas(args):
     Xu, md_u, outer, _ = cpp_untangle(ctx, bsc, g.X, session=s)
     s.core.autotune(20, 0.8)
     lv_best, _ = s.core.autotune_levels(16, 0.8)
+    multi = None if args.no_multi else _tune_multi(s, 8, fine=False)
     s.core.set_graph_chunk(args.graph_chunk)
     e_star = json.loads((ROOT / "tests" / "golden" / "estar_ogrid129.json").read_text())["E_star"]
     K = max(1, min(args.steps, 100))
@@ -977,7 +1012,8 @@ def bench_fas(args):
         "n_gpus": 1, "steps": K, "warmup": 2, "ms_per_step": r["ms_per_cycle"], "higher_is_better": False,
         "scaling": "strong", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "2D 5-block O-grid 129/65, 2% tangled by 1.5h, seed 2026, untangled; FAS cycles",
-                   "graph_chunk": args.graph_chunk, "fas_level_variants": [int(v) for v in lv_best]},
+                   "graph_chunk": args.graph_chunk, "fas_level_variants": [int(v) for v in lv_best],
+                   "multi_sweep_tuners": multi},
         "cycles": out, "roofline": None, "cpu_baseline": None,
     }
 
@@ -1074,6 +1110,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline and parity leg")
     ap.add_argument("--no-converge", action="store_true", help="skip the time-to-converged runs")
     ap.add_argument("--cpu-sweeps", type=int, default=150)
+    ap.add_argument("--no-multi", action="store_true", help="(--workload fas) keep one launch per sweep on every level")
     ap.add_argument("--workload", default="ogrid", choices=("ogrid", "ogrid_bl", "ogrid_terms", "batched", "cubed_sphere",
                                                               "mlp", "control", "fas"))
     ap.add_argument("--graph-chunk", type=int, default=GRAPH_CHUNK,

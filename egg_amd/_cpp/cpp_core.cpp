@@ -896,6 +896,46 @@ class Session
     {
         check(eggmi_set_level_variants(s_, v.data(), (int)v.size()));
     }
+    int set_multi_sweep(int level, int k, int ty, int tx, int kernel)
+    {
+        int nk = 0;
+        check(eggmi_set_multi_sweep(s_, level, k, ty, tx, kernel, &nk));
+        return nk;
+    }
+    py::dict multi_sweep_info(int level, bool check_tables, unsigned seed)
+    {
+        int64_t info[8] = {};
+        double err = 0.0;
+        check(eggmi_multi_sweep_info(s_, level, info, check_tables ? &err : nullptr, seed));
+        py::dict d;
+        const char* names[8] = {"k", "ty", "tx", "kernel", "tiles", "lds_stride", "core_dofs", "updates"};
+        for (int i = 0; i < 8; ++i) d[names[i]] = info[i];
+        if (check_tables) d["check_err"] = err;
+        return d;
+    }
+    py::array_t<double> autotune_levels_multi(int n_sweeps, double omega, const std::vector<std::array<int, 4>>& cands,
+                                              int max_levels, const std::string& phase)
+    {
+        std::vector<int32_t> c;
+        for (auto& q : cands) c.insert(c.end(), q.begin(), q.end());
+        const size_t nc = cands.size() + 1;
+        std::vector<double> ms(64 * nc);
+        int nl = 0;
+        check(eggmi_autotune_levels_multi(s_, n_sweeps, omega, max_levels, phase_id(phase), c.data(),
+                                          (int)cands.size(), ms.data(), (int64_t)ms.size(), &nl));
+        ms.resize((size_t)nl * nc);
+        py::array_t<double> t = to_array(ms);
+        return t.attr("reshape")(nl, (int)nc);
+    }
+    py::array_t<double> autotune_multi(int n_sweeps, double omega, const std::vector<std::array<int, 4>>& cands,
+                                       double base_ms)
+    {
+        std::vector<int32_t> c;
+        for (auto& q : cands) c.insert(c.end(), q.begin(), q.end());
+        std::vector<double> ms(cands.size());
+        check(eggmi_autotune_multi(s_, n_sweeps, omega, c.data(), (int)cands.size(), base_ms, ms.data()));
+        return to_array(ms);
+    }
     py::tuple autotune_levels(int n_sweeps, double omega, int max_levels, const std::string& phase)
     {
         int nv = 0;
@@ -1010,6 +1050,24 @@ PYBIND11_MODULE(cpp_core, m)
            "(energy, min det) of the resident state, reduced on the device in FP64.")
       .def("autotune", &Session::autotune, py::arg("n_sweeps") = 20, py::arg("omega") = 0.8,
            "Time every tile variant on the current state (restored), keep the fastest; returns (best, ms).")
+      .def("set_multi_sweep", &Session::set_multi_sweep, py::arg("level"), py::arg("k"), py::arg("ty") = 16,
+           py::arg("tx") = 16, py::arg("kernel") = 0,
+           "k block-Jacobi sweeps per launch on a level (0 = fine, i = FAS coarse level i) over ty x tx core "
+           "tiles with LDS-staged dependency closures (the results of k launches up to a few float ulps); k <= 1 turns it off. Returns the "
+           "number of multi-sweep kernel variants.")
+      .def("multi_sweep_info", &Session::multi_sweep_info, py::arg("level") = 0, py::arg("check") = false,
+           py::arg("seed") = 1,
+           "A level's multi-sweep tables (k, tile, kernel, tiles, LDS stride, core DOFs, DOF updates per launch); "
+           "check=True adds check_err, the host-side schedule check of the tables (0 when right).")
+      .def("autotune_multi", &Session::autotune_multi, py::arg("n_sweeps"), py::arg("omega"), py::arg("cands"),
+           py::arg("base_ms") = std::numeric_limits<double>::infinity(),
+           "Time multi-sweep candidates (k, ty, tx, kernel) on the fine level from the current state (restored) "
+           "and keep the fastest if it beats base_ms; returns ms per candidate (inf: not buildable).")
+      .def("autotune_levels_multi", &Session::autotune_levels_multi, py::arg("n_sweeps"), py::arg("omega"),
+           py::arg("cands"), py::arg("max_levels") = 32, py::arg("phase") = "barrier",
+           "Per FAS coarse level, time one smoothing call with the level's one-sweep setting and per multi-sweep "
+           "candidate (k, ty, tx, kernel), keep the fastest; returns ms [level, 1 + candidates] (-1: not "
+           "buildable).")
       .def("set_level_variants", &Session::set_level_variants, py::arg("variants"),
            "Tile variant of each FAS coarse level (variants[i] for level i + 1; -1 = the session's); V + v "
            "(V = the tile variant count) smooths the level in one single-workgroup launch on variant v's tiles.")

@@ -333,6 +333,71 @@ eggmi_status eggmi_set_level_variants(eggmi_session* s, const int* variants, int
     });
 }
 
+eggmi_status eggmi_set_multi_sweep(eggmi_session* s, int level, int k, int ty, int tx, int kernel, int* n_kernels)
+{
+    return guarded([&] {
+        need(s, "session");
+        if (n_kernels) *n_kernels = eggmi::num_ms_variants();
+        s->s.set_multi_sweep(level, k, ty, tx, kernel);
+    });
+}
+
+eggmi_status eggmi_multi_sweep_info(eggmi_session* s, int level, int64_t* info, double* check_err, unsigned seed)
+{
+    return guarded([&] {
+        need(s, "session");
+        const eggmi::Level& L = s->s.level_ref(level);
+        if (info) {
+            const int64_t v[8] = {L.ms_sweeps,
+                                  L.ms.ty,
+                                  L.ms.tx,
+                                  L.ms_kernel,
+                                  (int64_t)L.ms.tiles.size(),
+                                  L.ms.lds_stride,
+                                  L.ms.core_dofs,
+                                  L.ms.updates};
+            std::copy(v, v + 8, info);
+        }
+        if (check_err) {  // every launch depth the tables serve (cadence remainders run fewer sweeps)
+            double e = L.multi_selfcheck(L.ms.k, seed);
+            for (int k = 1; k < L.ms.k; ++k) e = std::max(e, L.multi_selfcheck(k, seed + (unsigned)k));
+            *check_err = e;
+        }
+    });
+}
+
+eggmi_status eggmi_autotune_multi(eggmi_session* s, int n_sweeps, double omega, const int32_t* cand, int n_cands,
+                                  double base_ms, double* ms)
+{
+    return guarded([&] {
+        need(s, "session");
+        if (n_cands < 0) throw eggmi::InvalidArgument("n_cands must be >= 0");
+        if (n_cands > 0) need(cand, "cand");
+        std::vector<std::array<int, 4>> c(n_cands);
+        for (int i = 0; i < n_cands; ++i) c[i] = {cand[4 * i], cand[4 * i + 1], cand[4 * i + 2], cand[4 * i + 3]};
+        const auto t = s->s.autotune_multi(n_sweeps, omega, c, base_ms);
+        if (ms) std::copy(t.begin(), t.end(), ms);
+    });
+}
+
+eggmi_status eggmi_autotune_levels_multi(eggmi_session* s, int n_sweeps, double omega, int max_levels, int phase,
+                                         const int32_t* cand, int n_cands, double* ms, int64_t capacity,
+                                         int* n_levels)
+{
+    return guarded([&] {
+        need(s, "session");
+        need(n_levels, "n_levels");
+        if (n_cands < 0) throw eggmi::InvalidArgument("n_cands must be >= 0");
+        if (n_cands > 0) need(cand, "cand");
+        std::vector<std::array<int, 4>> c(n_cands);
+        for (int i = 0; i < n_cands; ++i) c[i] = {cand[4 * i], cand[4 * i + 1], cand[4 * i + 2], cand[4 * i + 3]};
+        const auto t = s->s.autotune_levels_multi(n_sweeps, omega, max_levels, phase_kind(phase), c);
+        if (ms)
+            for (int64_t i = 0; i < capacity && i < (int64_t)t.size(); ++i) ms[i] = t[i];
+        *n_levels = (int)(t.size() / (size_t)(n_cands + 1));
+    });
+}
+
 eggmi_status eggmi_autotune_levels(eggmi_session* s, int n_sweeps, double omega, int max_levels, int phase,
                                    double* ms, int64_t capacity, int* best, int best_capacity, int* n_levels)
 {

@@ -498,7 +498,7 @@ void Session::smooth(Level& L, int sweeps, int kind, float omega, const float* t
     // smoothing sweeps as in plain runs; the other Jacobi buffer holds each
     // DOF's state before the last sweep (or correction), its displacement input
     const bool propose = proposal != nullptr && &L == &fine && tau == nullptr && kind != 1;
-    if (L.one_wg && tau && !propose) {
+    if (L.one_wg && tau && !propose && !multi_k(L, propose)) {
         if (sweeps <= 0) return;
         float* x0 = L.x[L.cur].p;
         float* x1 = L.x[1 - L.cur].p;
@@ -507,6 +507,16 @@ void Session::smooth(Level& L, int sweeps, int kind, float omega, const float* t
         launch_level_smooth(dim, kind, L.variant, a, x0, x1, sweeps, stream);
         if (sweeps % 2) L.cur = 1 - L.cur;
         ++last_launches;
+        return;
+    }
+    if (const int mk = multi_k(L, propose)) {  // k sweeps per launch
+        for (int s = 0; s < sweeps;) {
+            const int kk = std::min(mk, sweeps - s);
+            launch_multi(L, L.x[L.cur].p, L.x[1 - L.cur].p, kk, kind, omega, 0.0f, tau);
+            L.cur = 1 - L.cur;
+            ++last_launches;
+            s += kk;
+        }
         return;
     }
     for (int s = 0; s < sweeps; ++s) {

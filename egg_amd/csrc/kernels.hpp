@@ -122,6 +122,64 @@ struct SweepArgs {
     float prop_scale = 1.0f;
 };
 
+// ------------------------------------------------------------------ multi-sweep
+// k block-Jacobi sweeps in one launch (2D levels without special DOFs). A
+// workgroup owns a core tile of DOFs and stages in LDS its dependency
+// closure: every node the core's k-th sweep depends on, k rings deep through
+// block interfaces, replicas and stored stencils, as rectangles of block
+// padded boxes (regions: synthesized DOFs use the 3x3 row addressing of the
+// one-sweep kernel) plus single extra nodes. Sub-sweep s recomputes every
+// DOF within distance k-1-s of the core from the LDS state of sub-sweep
+// s-1 (the outer rings redundantly, as the neighbouring tiles do for theirs),
+// ping-ponging two LDS buffers; the last sub-sweep writes the core to the
+// store and pushes its replicas. Per DOF the arithmetic is the one-sweep
+// kernel's device code (the compiler may fuse its multiply-adds differently
+// in the two kernels: results agree to a few float ulps).
+constexpr int kMsMaxK = 8;
+struct __align__(16) MsRegion {
+    long long base;  // store slot of the rectangle's first node
+    long long st0;   // slot stride between its rows (the block's padded row)
+    int h, w;        // rows, columns
+    int lds;         // LDS entry of the first node (row-major)
+    int pad;
+};
+struct __align__(16) MsTile {
+    int reg0, nreg;  // regions
+    int ext0, next;  // extra nodes (store slots), LDS entries n_lds - next .. n_lds - 1
+    int n_lds;
+    int syn0;               // first synthesized entry of the tile (entries sorted by distance)
+    int syn_cnt[kMsMaxK];   // synthesized entries within distance r of the core
+    int st0;                // first stored entry of the tile (sorted by distance)
+    int st_cnt[kMsMaxK];    // stored entries within distance r of the core
+    int push0, npush;       // LDS replica pairs (a ghost in a region <- its root)
+};
+struct MsArgs {
+    int k = 0;           // sweeps per launch (<= the depth the tables were built for)
+    int n_tiles = 0;
+    int lds_stride = 0;  // LDS floats per coordinate plane of one buffer (>= every tile's n_lds)
+    const MsTile* tiles = nullptr;
+    const MsRegion* regions = nullptr;
+    const int* ext = nullptr;       // extra nodes' store slots
+    // synthesized entries: x = LDS entry | (its region's row width << 16), y = store slot
+    const int2* syn = nullptr;
+    const int* st_slot = nullptr;   // store slot of each stored entry
+    StoredView st;                  // stored entries: dof, gc, gn are LDS entries
+    const int* push_src = nullptr;  // LDS entries
+    const int* push_dst = nullptr;
+};
+// dynamic LDS of a multi-sweep launch: two buffers of D planes
+inline size_t ms_lds_bytes(int dim, int stride) { return (size_t)stride * 2 * dim * sizeof(float); }
+constexpr size_t kMsMaxLdsBytes = 64 * 1024;
+// kernel variants: lanes per node (4 / CPL) and workgroup size
+struct MsVariant {
+    int cpl, threads;
+};
+int num_ms_variants();
+MsVariant ms_variant(int v);
+// k sweeps (m.k) in one launch; a.xin holds the state on entry, a.xout gets
+// the core DOFs (and their replicas) k sweeps later
+void launch_multi_sweep(int kind, int variant, const SweepArgs& a, const MsArgs& m, hipStream_t s);
+
 // Energy / min-det reduction over the structured cell stencil.
 struct CellReduceArgs {
     const float* x = nullptr;

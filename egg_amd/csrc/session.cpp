@@ -1083,6 +1083,7 @@ std::pair<std::vector<double>, std::vector<double>> Session::run_jacobi(int n_sw
     if (!pair_valid) EGGMI_HIP(hipMemcpyAsync(xout, xin, bytes, hipMemcpyDeviceToDevice, stream));
     pair_valid = true;
     const bool propose = proposal != nullptr && kind != 1;
+    const int mk = multi_k(L, propose);  // sweeps per multi-sweep launch (0: one per launch)
     int rep = 0;
     if (kernel_timing) ensure_timing_events(2 * n_sweeps);
     counters_begin();
@@ -1094,11 +1095,20 @@ std::pair<std::vector<double>, std::vector<double>> Session::run_jacobi(int n_sw
         const int next_rep = std::min(n_sweeps, (s / k + 1) * k);
         // a captured chunk repeats the cadence from its first sweep
         if (graphs_on && next_rep - s >= graph_chunk && graph_chunk % proposal_every == 0) {
-            // an even chunk ends in the buffer it started from
             EGGMI_HIP(hipGraphLaunch(sweep_graph(xin == L.x[0].p ? 0 : 1, kind, (float)omega, (float)delta, propose),
                                      stream));
             s += graph_chunk;
-            launches += graph_chunk;
+            const int nl = chunk_launches(mk);
+            launches += nl;
+            if (nl % 2) std::swap(xin, xout);  // an odd launch count ends in the other buffer
+        } else if (mk) {
+            const int kk = std::min(mk, next_rep - s);
+            if (kernel_timing) EGGMI_HIP(hipEventRecord(tev[2 * launches], stream));
+            launch_multi(L, xin, xout, kk, kind, (float)omega, (float)delta, nullptr);
+            if (kernel_timing) EGGMI_HIP(hipEventRecord(tev[2 * launches + 1], stream));
+            std::swap(xin, xout);
+            s += kk;
+            ++launches;
         } else {
             SweepArgs a = L.sweep_args(xin, xout, (float)omega, (float)delta, nullptr);
             a.counters = counters_for(L);
@@ -1108,9 +1118,9 @@ std::pair<std::vector<double>, std::vector<double>> Session::run_jacobi(int n_sw
                 a.prop_mode = proposal_mode;
                 a.prop_scale = proposal_scale;
             }
-            if (kernel_timing) EGGMI_HIP(hipEventRecord(tev[2 * s], stream));
+            if (kernel_timing) EGGMI_HIP(hipEventRecord(tev[2 * launches], stream));
             launch_sweep(dim, kind, here ? proposal_variant() : tile_variant, a, stream);
-            if (kernel_timing) EGGMI_HIP(hipEventRecord(tev[2 * s + 1], stream));
+            if (kernel_timing) EGGMI_HIP(hipEventRecord(tev[2 * launches + 1], stream));
             std::swap(xin, xout);
             ++s;
             ++launches;
@@ -1132,9 +1142,9 @@ std::pair<std::vector<double>, std::vector<double>> Session::run_jacobi(int n_sw
     last_launches = launches;
     last_kernel_ms = 0.0;
     if (kernel_timing)
-        for (int s = 0; s < n_sweeps; ++s) {
+        for (long long q = 0; q < launches; ++q) {
             float k = 0;
-            EGGMI_HIP(hipEventElapsedTime(&k, tev[2 * s], tev[2 * s + 1]));
+            EGGMI_HIP(hipEventElapsedTime(&k, tev[2 * q], tev[2 * q + 1]));
             last_kernel_ms += k;
         }
     return {std::vector<double>(hr.begin(), hr.begin() + nrep), std::vector<double>(hr.begin() + nrep, hr.end())};
@@ -1166,9 +1176,15 @@ hipGraphExec_t Session::sweep_graph(int parity, int kind, float omega, float del
     float* b0 = L.x[parity].p;
     float* b1 = L.x[1 - parity].p;
     hipGraph_t graph = nullptr;
+    const int mk = multi_k(L, propose);
     EGGMI_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     try {
-        for (int i = 0; i < graph_chunk; ++i) {
+        for (int i = 0, sw = 0; mk && sw < graph_chunk; ++i) {  // multi-sweep launches
+            const int kk = std::min(mk, graph_chunk - sw);
+            launch_multi(L, i % 2 == 0 ? b0 : b1, i % 2 == 0 ? b1 : b0, kk, kind, omega, delta, nullptr);
+            sw += kk;
+        }
+        for (int i = 0; !mk && i < graph_chunk; ++i) {
             SweepArgs a = L.sweep_args(i % 2 == 0 ? b0 : b1, i % 2 == 0 ? b1 : b0, omega, delta, nullptr);
             const bool here = propose && proposes_at(i);
             if (here) {
@@ -1318,6 +1334,8 @@ std::vector<double> Session::autotune(int n_sweeps, double omega)
     proposal = nullptr;
     const int nv = num_tile_variants(dim);
     std::vector<double> ms(nv, 0.0);
+    const int msk0 = L.ms_sweeps;  // one-sweep kernels while their tiles are timed
+    L.ms_sweeps = 0;
     try {
         for (int v = 0; v < nv; ++v) {
             if (prop0 && tile_variant_proposal_partner(dim, v) < 0) {
@@ -1340,6 +1358,7 @@ std::vector<double> Session::autotune(int n_sweeps, double omega)
     } catch (...) {
         counting = count0;
         proposal = prop0;
+        L.ms_sweeps = msk0;
         throw;
     }
     EGGMI_HIP(hipStreamSynchronize(stream));
@@ -1352,6 +1371,73 @@ std::vector<double> Session::autotune(int n_sweeps, double omega)
     EGGMI_HIP(hipStreamSynchronize(stream));
     counting = count0;
     proposal = prop0;
+    L.ms_sweeps = msk0;
+    return ms;
+}
+
+std::vector<double> Session::autotune_multi(int n_sweeps, double omega, const std::vector<std::array<int, 4>>& cands,
+                                            double base_ms)
+{
+    need_device();
+    if (n_sweeps < 1) throw InvalidArgument("autotune_multi: n_sweeps must be >= 1");
+    Level& L = fine;
+    const size_t n = (size_t)L.plane * dim;
+    DArr<float> save[2];
+    for (int b = 0; b < 2; ++b) {
+        save[b].alloc(n);
+        EGGMI_HIP(hipMemcpyAsync(save[b].p, L.x[b].p, n * sizeof(float), hipMemcpyDeviceToDevice, stream));
+    }
+    const int cur0 = L.cur;
+    const bool pair0 = pair_valid;
+    const bool count0 = counting;
+    EGGMI_HIP(hipStreamSynchronize(stream));
+    counting = false;
+    const std::array<int, 4> keep0 = {L.ms_sweeps, L.ms.ty, L.ms.tx, L.ms_kernel};
+    std::vector<double> ms(cands.size(), std::numeric_limits<double>::infinity());
+    auto restore = [&] {
+        for (int b = 0; b < 2; ++b)
+            EGGMI_HIP(hipMemcpyAsync(L.x[b].p, save[b].p, n * sizeof(float), hipMemcpyDeviceToDevice, stream));
+        L.cur = cur0;
+        pair_valid = pair0;
+    };
+    // each candidate is timed as one captured chunk of n_sweeps, replayed as
+    // the timed runs replay them (host launch gaps would favour fewer launches)
+    const int chunk0 = graph_chunk;
+    graph_chunk = n_sweeps;
+    try {
+        for (size_t c = 0; c < cands.size(); ++c) {
+            const auto& q = cands[c];
+            try {
+                set_multi_sweep(0, q[0], q[1], q[2], q[3]);
+            } catch (const InvalidArgument&) {
+                continue;  // not buildable (a closure outgrows the LDS): not timed
+            }
+            run_jacobi(n_sweeps, 0, 0.0, omega, n_sweeps);  // capture the chunk, warm the code
+            restore();
+            run_jacobi(n_sweeps, 0, 0.0, omega, n_sweeps);
+            ms[c] = last_ms;
+            restore();
+        }
+    } catch (...) {
+        counting = count0;
+        graph_chunk = chunk0;
+        restore();
+        set_multi_sweep(0, 0, 0, 0, 0);
+        throw;
+    }
+    EGGMI_HIP(hipStreamSynchronize(stream));
+    counting = count0;
+    graph_chunk = chunk0;
+    release_graphs();
+    size_t best = 0;
+    for (size_t c = 1; c < cands.size(); ++c)
+        if (ms[c] < ms[best]) best = c;
+    if (!cands.empty() && ms[best] < base_ms)
+        set_multi_sweep(0, cands[best][0], cands[best][1], cands[best][2], cands[best][3]);
+    else if (keep0[0] > 1)
+        set_multi_sweep(0, keep0[0], keep0[1], keep0[2], keep0[3]);
+    else
+        set_multi_sweep(0, 0, 0, 0, 0);
     return ms;
 }
 

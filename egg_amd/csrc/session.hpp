@@ -229,6 +229,36 @@ struct Level {
     long long n_coarse_nodes = 0, n_fine_nodes = 0;
     DArr<float> rx, tau, grad, corr;  // corr: the fine level's correction only
 
+    // multi-sweep tables (multisweep.cpp; 2D): k sweeps per launch over
+    // LDS-staged dependency closures of core tiles of ty x tx logical nodes
+    struct MsHost {
+        int k = 0, ty = 0, tx = 0, lds_stride = 0, rounds = 1, G = 16;
+        std::vector<MsTile> tiles;
+        std::vector<MsRegion> regions;
+        std::vector<int> ext, st_slot, st_lds, gc, gn, push_src, push_dst;
+        std::vector<int2> syn;  // synthesized entries (MsArgs::syn)
+        std::vector<uint8_t> meta;
+        std::vector<float> W;  // per-occurrence W_inv rows (empty: uniform)
+        long long n_st = 0, total = 0;
+        long long core_dofs = 0, updates = 0;  // DOFs, and DOF updates per launch over its k sub-sweeps
+    };
+    MsHost ms;  // ms.k == 0: none built
+    int ms_kernel = 0;  // kernel variant (ms_variant)
+    int ms_sweeps = 0;  // sweeps per launch in use (0: the one-sweep kernels)
+    DArr<MsTile> d_ms_tiles;
+    DArr<MsRegion> d_ms_regions;
+    DArr<int2> d_ms_syn;
+    DArr<int> d_ms_ext, d_ms_st_slot, d_ms_st_lds, d_ms_gc, d_ms_gn, d_ms_push_src, d_ms_push_dst;
+    DArr<uint8_t> d_ms_meta;
+    DArr<float> d_ms_W;
+    // builds the tables for depth k (throws InvalidArgument when the level
+    // is not eligible or a closure outgrows the LDS) and uploads them
+    void build_multi(int k, int ty, int tx, bool upload);
+    MsArgs multi_args(int k) const;
+    // max |difference| between k sweeps of a synthetic nonlinear update run
+    // through the tables' schedule and k plain synchronous sweeps of it
+    double multi_selfcheck(int k, unsigned seed) const;
+
     BlockDesc block_desc(size_t b) const;
     void upload_tables();
     void make_tiles(int v);  // also lays the stored DOFs out for the variant's lane groups
@@ -368,6 +398,31 @@ struct Session {
     // time n_sweeps plain sweeps per tile variant from the current state
     // (restored after each trial), keep the fastest; returns ms per variant
     std::vector<double> autotune(int n_sweeps, double omega);
+    // time n_sweeps plain sweeps per multi-sweep candidate (k, ty, tx,
+    // kernel) on the fine level from the current state (restored after each
+    // trial); keeps the fastest if it beats base_ms (else the previous
+    // setting); returns ms per candidate (+inf: not buildable)
+    std::vector<double> autotune_multi(int n_sweeps, double omega, const std::vector<std::array<int, 4>>& cands,
+                                       double base_ms);
+    // per FAS coarse level (from the fine state restricted down the chain):
+    // time one smoothing call of n_sweeps as the level's one-sweep setting
+    // and per multi-sweep candidate, keep the fastest; ms [level][1 + cand]
+    std::vector<double> autotune_levels_multi(int n_sweeps, double omega, int max_levels, int kind,
+                                              const std::vector<std::array<int, 4>>& cands);
+    // k sweeps per launch on level `level` (0 = fine, i >= 1 = FAS coarse
+    // level i, which prepare_fas must have built) over ty x tx core tiles
+    // with multi-sweep kernel variant `kernel`; k <= 1 returns the level to
+    // the one-sweep kernels (multisweep.cpp)
+    void set_multi_sweep(int level, int k, int ty, int tx, int kernel);
+    Level& level_ref(int level);
+    // k of the launches sweeping L (0 = one-sweep kernels); sweeps with the
+    // learned proposal keep the one-sweep kernels
+    static int multi_k(const Level& L, bool propose) { return L.ms_sweeps > 1 && !propose ? L.ms_sweeps : 0; }
+    // launches in a captured chunk of graph_chunk sweeps
+    int chunk_launches(int mk) const { return mk ? (graph_chunk + mk - 1) / mk : graph_chunk; }
+    // one multi-sweep launch of k sweeps on L: xin -> xout
+    void launch_multi(Level& L, const float* xin, float* xout, int k, int kind, float omega, float delta,
+                      const float* tau);
     // per-level tile variants of the FAS coarse levels (-1 = tile_variant)
     void set_level_variants(const std::vector<int>& v);
     // time n_sweeps FAS-shifted sweeps per tile variant on each of the first

@@ -238,14 +238,30 @@ __device__ __forceinline__ float curv_residual(const float (&p)[4][2], float (&d
 // boundary DOF (2D; stored index st.n + i slides on entity i): tangent-reduced
 // Newton step damped by omega, trials projected onto the entity, no
 // post-blend (reference src/sweep.hpp:1285-1403, :1848-1871).
-template <int D, int KIND, bool TAU, bool GRAD, int G, bool WT, bool ENT = false>
-__device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
+// Where a stored DOF's positions come from and where its result goes. The
+// one-sweep kernels read the snapshot `xin` and write the node (and its
+// replicas) into `xout`; the multi-sweep kernel substitutes its LDS staging.
+struct StoreIO {
+    __device__ const StoredView& st(const SweepArgs& a) const { return a.st; }
+    __device__ const float* X(const SweepArgs& a) const { return a.xin; }
+    __device__ long long PL(const SweepArgs& a) const { return a.plane; }
+    // store slot of stored DOF d (at position `dof` of X)
+    __device__ long long slot(int, int dof) const { return dof; }
+    template <int D> __device__ void finish(const SweepArgs& a, int, int dof, const float* cur, int ntr) const
+    {
+        store_node<D>(a, dof, cur);
+        count_update(a, ntr);
+    }
+};
+
+template <int D, int KIND, bool TAU, bool GRAD, int G, bool WT, bool ENT = false, class IO = StoreIO>
+__device__ void stored_group(const SweepArgs& a, int d, int l, bool valid, const IO& io = IO {})
 {
     static_assert(!ENT || !TAU, "special DOFs: fine level");
     static_assert(G >= 4 && G <= stored_lanes(D), "stored lane group size");
-    const StoredView& st = a.st;
-    const float* X = a.xin;
-    const long long PL = a.plane;
+    const StoredView& st = io.st(a);
+    const float* X = io.X(a);
+    const long long PL = io.PL(a);
     const int R = st.rounds;
     const int dof = valid ? st.dof[d] : 0;
     float pos[D];
@@ -436,7 +452,7 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
     if constexpr (TAU) {
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-            tau[k] = a.tau[k * PL + dof];
+            tau[k] = a.tau[k * a.plane + io.slot(d, dof)];
             g[k] -= tau[k];
             e0 -= tau[k] * pos[k];
         }
@@ -445,7 +461,7 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
         if (l == 0)
 #pragma unroll
             for (int k = 0; k < D; ++k) {
-                float& o = a.gout[k * PL + dof];
+                float& o = a.gout[k * a.plane + dof];
                 o = a.gsub ? g[k] - o : g[k];
             }
         if (l == 0 && a.gpush) push_replicas<D>(a, a.gout, dof, g);
@@ -613,8 +629,7 @@ __device__ void stored_group(const SweepArgs& a, int d, int l, bool valid)
 #pragma unroll
         for (int k = 0; k < D; ++k) cur[k] = pos[k] + a.omega * (cur[k] - pos[k]);
     if (l == 0) {
-        store_node<D>(a, dof, cur);
-        count_update(a, ntr);
+        io.template finish<D>(a, d, dof, cur, ntr);
         if constexpr (ENT) {
             if (etag != 0) {
                 a.con_seed[2 * ci] = sd[0];
@@ -646,6 +661,195 @@ template <int D, int KIND, int G, bool GRAD> static void constrained_g(const Swe
 }
 
 // ------------------------------------------------------------------ tiles
+
+// Matrix A of permuted sample OP of a synthesized DOF's cell whose corners
+// Pj are permuted so that corner 0 is the DOF (rhoj: the frame's axis sign
+// flips); the DOF moved to x0 when given.
+template <int D, class OPT>
+__device__ __forceinline__ void synth_A(const float (&Pj)[1 << D][D], const float (&rhoj)[D], OPT, const float* x0,
+                                       float* A)
+{
+    constexpr int op = OPT::value;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        const int nb = op ^ (1 << (D - 1 - k));
+        const float sk = (((op >> (D - 1 - k)) & 1) ? -1.0f : 1.0f) * rhoj[k];
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            const float pn = (nb == 0 && x0) ? x0[i] : Pj[nb][i];
+            const float pc = (op == 0 && x0) ? x0[i] : Pj[op][i];
+            A[i * D + k] = sk * (pn - pc);
+        }
+    }
+}
+
+// Newton update of one synthesized interior DOF by its lane group (NL =
+// 2^D / CPL lanes; lane gl holds cells gl * CPL .. of the patch, corners in
+// P, P[0][0] = the DOF): gradient, contracted Hessian and e0 reduced over the
+// group, Newton step, <= 10 halvings, omega post-blend. `node` is the DOF's
+// store slot (FAS shift tau, gradient output). Returns the trials evaluated
+// (10 = all rejected) with the result in cur; GRAD: writes the gradient
+// (lane 0) and returns -1.
+template <int D, int KIND, bool TAU, bool GRAD, int CPL>
+__device__ __forceinline__ int synth_solve(const SweepArgs& a, const float (&P)[CPL][1 << D][D],
+                                           const float (&rho)[CPL][D], int gl, long long node, float* cur)
+{
+    constexpr int NC = 1 << D;
+    constexpr int NL = NC / CPL;
+    const long long PL = a.plane;
+    const float delta = a.delta;
+    // sum over the node's cells in the order of the lane-group reduction
+    // ((c0 + c1) + (c2 + c3)) + ...: a lane folds its CPL cells the same way,
+    // then the group folds the lanes (the fold order of every CPL is the
+    // same; the compiler's multiply-add fusion inside a cell may differ)
+    auto cell_sum = [&](const float* x) -> float {
+        if constexpr (CPL == 1) return group_sum<NL>(x[0]);
+        else if constexpr (CPL == 2) return group_sum<NL>(x[0] + x[1]);
+        else if constexpr (CPL == 4) return group_sum<NL>((x[0] + x[1]) + (x[2] + x[3]));
+        else return group_sum<NL>(((x[0] + x[1]) + (x[2] + x[3])) + ((x[4] + x[5]) + (x[6] + x[7])));
+    };
+    // per cell: gradient and Hessian of the samples touching the DOF (em),
+    // value of the frozen ones (ec) and their min det (mc)
+    float gs[CPL][D] = {}, hs[CPL][kNH<D>] = {}, em[CPL], ec[CPL], mc = kInf;
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) {
+        em[j] = 0.0f;
+        ec[j] = 0.0f;
+        static_for<NC>([&](auto OP) {
+            constexpr int op = decltype(OP)::value;
+            constexpr int pc = __builtin_popcount(op);
+            float A[D * D];
+            synth_A<D>(P[j], rho[j], OP, nullptr, A);
+            const float det = detD<D>(A);
+            if constexpr (pc <= 1) {  // the DOF is the corner (op = 0) or an axis neighbour
+                float u[D];
+#pragma unroll
+                for (int c = 0; c < D; ++c) {
+                    const float sc = (((op >> (D - 1 - c)) & 1) ? -1.0f : 1.0f) * rho[j][c];
+                    if constexpr (op == 0)
+                        u[c] = -sc;
+                    else
+                        u[c] = (op == (1 << (D - 1 - c))) ? sc : 0.0f;
+                }
+                em[j] += sample_grad_hess<D, KIND>(A, u, delta, gs[j], hs[j]);
+            } else {
+                ec[j] += sample_value<D, KIND>(A, delta);
+                mc = fminf(mc, det);
+            }
+        });
+    }
+    float g[D], h[kNH<D>];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        float x[CPL];
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) x[j] = gs[j][k];
+        g[k] = cell_sum(x);
+    }
+    float e0;
+    {
+        float x[CPL];
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) x[j] = ec[j] + em[j];
+        e0 = cell_sum(x);
+    }
+    float pos[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) pos[k] = P[0][0][k];
+    float tau[D] = {};
+    if constexpr (TAU) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            tau[k] = a.tau[k * PL + node];
+            g[k] -= tau[k];
+            e0 -= tau[k] * pos[k];
+        }
+    }
+    if constexpr (GRAD) {
+        if (gl == 0)
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                float& o = a.gout[k * PL + node];
+                o = a.gsub ? g[k] - o : g[k];
+            }
+        if (gl == 0 && a.gpush) push_replicas<D>(a, a.gout, node, g);
+        return -1;
+    }
+#pragma unroll
+    for (int k = 0; k < kNH<D>; ++k) {
+        float x[CPL];
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) x[j] = hs[j][k];
+        h[k] = cell_sum(x);
+    }
+    float dl[D];
+    newton<D>(g, h, dl);
+    float alpha = 1.0f;
+    int ntr = 10;  // trials evaluated (10 = all rejected)
+#pragma unroll
+    for (int k = 0; k < D; ++k) cur[k] = pos[k];
+    for (int it = 0; it < 10; ++it) {
+        float tr[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) tr[k] = pos[k] + alpha * dl[k];
+        float ets[CPL], mt = mc;
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) {
+            float et = 0.0f;
+            static_for<NC>([&](auto OP) {
+                constexpr int op = decltype(OP)::value;
+                if constexpr (__builtin_popcount(op) <= 1) {
+                    float A[D * D];
+                    synth_A<D>(P[j], rho[j], OP, tr, A);
+                    et += sample_value<D, KIND>(A, delta);
+                    mt = fminf(mt, detD<D>(A));
+                }
+            });
+            ets[j] = ec[j] + et;
+        }
+        float et = cell_sum(ets);
+        mt = group_min<NL>(mt);
+        if constexpr (TAU) {
+#pragma unroll
+            for (int k = 0; k < D; ++k) et -= tau[k] * tr[k];
+        }
+        if (accept<KIND>(et, e0, mt)) {
+#pragma unroll
+            for (int k = 0; k < D; ++k) cur[k] = tr[k];
+            ntr = it + 1;
+            break;
+        }
+        alpha *= 0.5f;
+    }
+    if (a.omega != 1.0f)
+#pragma unroll
+        for (int k = 0; k < D; ++k) cur[k] = pos[k] + a.omega * (cur[k] - pos[k]);
+    return ntr;
+}
+
+// Corner permutation of cell cj of a synthesized DOF's patch: the frame's
+// axis sign flips rho and, per permuted corner op, its offset loc[k] in
+// {0, 1, 2} from the 3^D neighbourhood's low corner (the DOF at 1).
+template <int D, class F> __device__ __forceinline__ void synth_corners(int cj, float (&rho)[D], F&& corner)
+{
+    constexpr int NC = 1 << D;
+    int cb[3] = {0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < D; ++k) cb[k] = (cj >> (D - 1 - k)) & 1;
+#pragma unroll
+    for (int k = 0; k < D; ++k) rho[k] = cb[k] ? 1.0f : -1.0f;  // r_k = 1 - cb_k; rho = r_k ? -1 : 1
+#pragma unroll
+    for (int op = 0; op < NC; ++op) {
+        int loc[3] = {0, 0, 0};
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const int obit = (op >> (D - 1 - k)) & 1;
+            const int o = obit ^ (1 - cb[k]);  // actual corner bit
+            loc[k] = cb[k] + o;
+        }
+        corner(op, loc);
+    }
+}
 
 // Tile of TX (last axis) x TY x TZ nodes; 2^D lanes per node.
 // LDS = false: no staging; each lane reads its cell's corners straight from
@@ -721,22 +925,8 @@ __device__ void tile_sweep(const SweepArgs& a, int tile)
     float rho[CPL][D];  // axis sign flips of each cell's permuted frame
     float P[CPL][NC][D];
 #pragma unroll
-    for (int j = 0; j < CPL; ++j) {
-        const int cj = gl * CPL + j;
-        int cb[3] = {0, 0, 0};
-#pragma unroll
-        for (int k = 0; k < D; ++k) cb[k] = (cj >> (D - 1 - k)) & 1;
-#pragma unroll
-        for (int k = 0; k < D; ++k) rho[j][k] = cb[k] ? 1.0f : -1.0f;  // r_k = 1 - cb_k; rho = r_k ? -1 : 1
-#pragma unroll
-        for (int op = 0; op < NC; ++op) {
-            int loc[3] = {0, 0, 0};
-#pragma unroll
-            for (int k = 0; k < D; ++k) {
-                const int obit = (op >> (D - 1 - k)) & 1;
-                const int o = obit ^ (1 - cb[k]);  // actual corner bit
-                loc[k] = cb[k] + o;
-            }
+    for (int j = 0; j < CPL; ++j)
+        synth_corners<D>(gl * CPL + j, rho[j], [&](int op, const int* loc) {
             if constexpr (LDS) {
                 int li;
                 if constexpr (D == 2)
@@ -752,36 +942,9 @@ __device__ void tile_sweep(const SweepArgs& a, int tile)
 #pragma unroll
                 for (int k = 0; k < D; ++k) P[j][op][k] = a.xin[k * PL + g];
             }
-        }
-    }
+        });
 
     const float delta = a.delta;
-    // A of permuted sample op of the lane's cell j, with the DOF at `x0`
-    auto build = [&](int j, auto OP, const float* x0, float* A) {
-        constexpr int op = decltype(OP)::value;
-#pragma unroll
-        for (int k = 0; k < D; ++k) {
-            const int nb = op ^ (1 << (D - 1 - k));
-            const float sk = (((op >> (D - 1 - k)) & 1) ? -1.0f : 1.0f) * rho[j][k];
-#pragma unroll
-            for (int i = 0; i < D; ++i) {
-                const float pn = (nb == 0 && x0) ? x0[i] : P[j][nb][i];
-                const float pc = (op == 0 && x0) ? x0[i] : P[j][op][i];
-                A[i * D + k] = sk * (pn - pc);
-            }
-        }
-    };
-    // sum over the node's cells in the order of the lane-group reduction
-    // ((c0 + c1) + (c2 + c3)) + ...: a lane folds its CPL cells the same way,
-    // then the group folds the lanes (the fold order of every CPL is the
-    // same; the compiler's multiply-add fusion inside a cell may differ)
-    auto cell_sum = [&](const float* x) -> float {
-        if constexpr (CPL == 1) return group_sum<NL>(x[0]);
-        else if constexpr (CPL == 2) return group_sum<NL>(x[0] + x[1]);
-        else if constexpr (CPL == 4) return group_sum<NL>((x[0] + x[1]) + (x[2] + x[3]));
-        else return group_sum<NL>(((x[0] + x[1]) + (x[2] + x[3])) + ((x[4] + x[5]) + (x[6] + x[7])));
-    };
-
     if constexpr (PROP) {
         // ---- learned proposal: features of the node's one-ring (LDS), the
         // wave's matrix-core forward pass over its 16 (2D) / 8 (3D) nodes,
@@ -834,10 +997,10 @@ __device__ void tile_sweep(const SweepArgs& a, int tile)
         float e_now = 0.0f, e_try = 0.0f, m_try = kInf;
         static_for<NC>([&](auto OP) {
             float A[D * D];
-            build(0, OP, nullptr, A);
+            synth_A<D>(P[0], rho[0], OP, nullptr, A);
             e_now += sample_value<D, KIND>(A, delta);
             if constexpr (__builtin_popcount(decltype(OP)::value) <= 1) {
-                if (finite_p) build(0, OP, tr, A);
+                if (finite_p) synth_A<D>(P[0], rho[0], OP, tr, A);
             }
             e_try += sample_value<D, KIND>(A, delta);
             m_try = fminf(m_try, detD<D>(A));
@@ -866,122 +1029,9 @@ __device__ void tile_sweep(const SweepArgs& a, int tile)
         }
     }
 
-    // per cell: gradient and Hessian of the samples touching the DOF (em),
-    // value of the frozen ones (ec) and their min det (mc)
-    float gs[CPL][D] = {}, hs[CPL][kNH<D>] = {}, em[CPL], ec[CPL], mc = kInf;
-#pragma unroll
-    for (int j = 0; j < CPL; ++j) {
-        em[j] = 0.0f;
-        ec[j] = 0.0f;
-        static_for<NC>([&](auto OP) {
-            constexpr int op = decltype(OP)::value;
-            constexpr int pc = __builtin_popcount(op);
-            float A[D * D];
-            build(j, OP, nullptr, A);
-            const float det = detD<D>(A);
-            if constexpr (pc <= 1) {  // the DOF is the corner (op = 0) or an axis neighbour
-                float u[D];
-#pragma unroll
-                for (int c = 0; c < D; ++c) {
-                    const float sc = (((op >> (D - 1 - c)) & 1) ? -1.0f : 1.0f) * rho[j][c];
-                    if constexpr (op == 0)
-                        u[c] = -sc;
-                    else
-                        u[c] = (op == (1 << (D - 1 - c))) ? sc : 0.0f;
-                }
-                em[j] += sample_grad_hess<D, KIND>(A, u, delta, gs[j], hs[j]);
-            } else {
-                ec[j] += sample_value<D, KIND>(A, delta);
-                mc = fminf(mc, det);
-            }
-        });
-    }
-    float g[D], h[kNH<D>];
-#pragma unroll
-    for (int k = 0; k < D; ++k) {
-        float x[CPL];
-#pragma unroll
-        for (int j = 0; j < CPL; ++j) x[j] = gs[j][k];
-        g[k] = cell_sum(x);
-    }
-    float e0;
-    {
-        float x[CPL];
-#pragma unroll
-        for (int j = 0; j < CPL; ++j) x[j] = ec[j] + em[j];
-        e0 = cell_sum(x);
-    }
-    float pos[D];
-#pragma unroll
-    for (int k = 0; k < D; ++k) pos[k] = P[0][0][k];
-    float tau[D] = {};
-    if constexpr (TAU) {
-#pragma unroll
-        for (int k = 0; k < D; ++k) {
-            tau[k] = a.tau[k * PL + node];
-            g[k] -= tau[k];
-            e0 -= tau[k] * pos[k];
-        }
-    }
-    if constexpr (GRAD) {
-        if (gl == 0)
-#pragma unroll
-            for (int k = 0; k < D; ++k) {
-                float& o = a.gout[k * PL + node];
-                o = a.gsub ? g[k] - o : g[k];
-            }
-        if (gl == 0 && a.gpush) push_replicas<D>(a, a.gout, node, g);
-        return;
-    }
-#pragma unroll
-    for (int k = 0; k < kNH<D>; ++k) {
-        float x[CPL];
-#pragma unroll
-        for (int j = 0; j < CPL; ++j) x[j] = hs[j][k];
-        h[k] = cell_sum(x);
-    }
-    float dl[D];
-    newton<D>(g, h, dl);
-    float alpha = 1.0f, cur[D];
-    int ntr = 10;  // trials evaluated (10 = all rejected)
-#pragma unroll
-    for (int k = 0; k < D; ++k) cur[k] = pos[k];
-    for (int it = 0; it < 10; ++it) {
-        float tr[D];
-#pragma unroll
-        for (int k = 0; k < D; ++k) tr[k] = pos[k] + alpha * dl[k];
-        float ets[CPL], mt = mc;
-#pragma unroll
-        for (int j = 0; j < CPL; ++j) {
-            float et = 0.0f;
-            static_for<NC>([&](auto OP) {
-                constexpr int op = decltype(OP)::value;
-                if constexpr (__builtin_popcount(op) <= 1) {
-                    float A[D * D];
-                    build(j, OP, tr, A);
-                    et += sample_value<D, KIND>(A, delta);
-                    mt = fminf(mt, detD<D>(A));
-                }
-            });
-            ets[j] = ec[j] + et;
-        }
-        float et = cell_sum(ets);
-        mt = group_min<NL>(mt);
-        if constexpr (TAU) {
-#pragma unroll
-            for (int k = 0; k < D; ++k) et -= tau[k] * tr[k];
-        }
-        if (accept<KIND>(et, e0, mt)) {
-#pragma unroll
-            for (int k = 0; k < D; ++k) cur[k] = tr[k];
-            ntr = it + 1;
-            break;
-        }
-        alpha *= 0.5f;
-    }
-    if (a.omega != 1.0f)
-#pragma unroll
-        for (int k = 0; k < D; ++k) cur[k] = pos[k] + a.omega * (cur[k] - pos[k]);
+    float cur[D];
+    const int ntr = synth_solve<D, KIND, TAU, GRAD, CPL>(a, P, rho, gl, node, cur);
+    if constexpr (GRAD) return;
     if (gl == 0) {
         store_node<D>(a, node, cur);
         count_update(a, ntr);

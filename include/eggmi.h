@@ -279,6 +279,53 @@ eggmi_status eggmi_set_level_variants(eggmi_session* s, const int* variants, int
 eggmi_status eggmi_autotune_levels(eggmi_session* s, int n_sweeps, double omega, int max_levels, int phase,
                                    double* ms, int64_t capacity, int* best, int best_capacity, int* n_levels);
 
+/* Multi-sweep launches (no reference counterpart: the reference launches
+ * every sweep, src/sweep.hpp:1938-1984): level `level` (0 = the fine grid,
+ * i >= 1 = FAS coarse level i, which a run_fas must have prepared) runs k
+ * block-Jacobi sweeps per launch (k <= 8) over core tiles of ty x tx nodes
+ * with multi-sweep kernel variant `kernel` (< *n_kernels when n_kernels is
+ * non-null; eggmi_multi_sweep_info). Each workgroup stages its tile's
+ * dependency closure k rings deep in LDS and recomputes the outer rings with
+ * the one-sweep kernel's per-DOF device code, so the result equals k
+ * one-sweep launches up to a few float ulps (multiply-add fusion may
+ * differ between the two kernels). k <= 1 returns the
+ * level to one launch per sweep. 2D levels without special stored DOFs
+ * (constrained, curvature or directional windows) or per-sample energy
+ * weights, and closures within the LDS; INVALID_ARGUMENT otherwise. Runs
+ * with the learned proposal keep one launch per sweep. */
+eggmi_status eggmi_set_multi_sweep(eggmi_session* s, int level, int k, int ty, int tx, int kernel, int* n_kernels);
+
+/* A level's multi-sweep tables: info[0..7] = sweeps per launch in use (0:
+ * off), ty, tx, kernel variant, tiles, LDS nodes per buffer plane, core DOFs
+ * and DOF updates per launch over its k sub-sweeps (their ratio to k x core
+ * DOFs is the redundant recompute). With check_err non-null, also a
+ * host-side check of the tables: the largest difference between k sweeps
+ * of a synthetic nonlinear update of every DOF from its stencil (seeded
+ * state) run through the tables' schedule and k plain synchronous sweeps
+ * of it (0 when every closure, region, push and write is right). */
+eggmi_status eggmi_multi_sweep_info(eggmi_session* s, int level, int64_t* info, double* check_err, unsigned seed);
+
+/* Autotuner over multi-sweep candidates on the fine grid (no reference
+ * counterpart): cand[4 i .. 4 i + 3] = (k, ty, tx, kernel) for i < n_cands;
+ * times n_sweeps plain sweeps (omega) per candidate from the current state
+ * (restored after each trial) and keeps the fastest if it beats base_ms
+ * (else the previous setting stays). ms[i] receives each time (+inf: the
+ * candidate's closures outgrow the LDS). */
+eggmi_status eggmi_autotune_multi(eggmi_session* s, int n_sweeps, double omega, const int32_t* cand, int n_cands,
+                                  double base_ms, double* ms);
+
+/* The same per FAS coarse level (no reference counterpart): for each of the
+ * first max_levels - 1 coarse levels, from the resident fine state
+ * restricted down the chain, time one smoothing call of n_sweeps FAS-shifted
+ * sweeps of `phase` (captured as a cycle replays it) with the level's
+ * one-sweep setting and with each candidate, and keep the fastest.
+ * ms[l * (n_cands + 1) + 0] (< capacity) receives level l + 1's one-sweep
+ * time, [.. + 1 + i] candidate i's (-1: not buildable); *n_levels the
+ * levels tuned. The fine state is not changed. */
+eggmi_status eggmi_autotune_levels_multi(eggmi_session* s, int n_sweeps, double omega, int max_levels, int phase,
+                                         const int32_t* cand, int n_cands, double* ms, int64_t capacity,
+                                         int* n_levels);
+
 /* Device time of the last eggmi_run / eggmi_run_fas call in ms (HIP events
  * on the session stream), the number of sweep-kernel launches it made and,
  * when kernel timing is on, the summed duration of those launches. */
