@@ -479,6 +479,9 @@ def bench_ogrid(args):
     log(f"level autotune: variants {lv_best}")
     multi = _tune_multi(s, 8)
     log(f"multi-sweep tuners: {multi}")
+    if args.force_multi:  # time (and profile) a given fine-level multi-sweep setting instead of the tuner's
+        s.core.set_multi_sweep(0, *args.force_multi)
+        multi["forced_fine"] = list(args.force_multi)
     s.core.set_graph_chunk(args.graph_chunk)  # replay sweeps as captured hipGraph chunks
     s.run(args.warmup, omega=0.8)
     e, m, t_run, k_avg, X_start = _timed(s, args.steps, 0.8)
@@ -1111,6 +1114,9 @@ def main() -> int:
     ap.add_argument("--no-converge", action="store_true", help="skip the time-to-converged runs")
     ap.add_argument("--cpu-sweeps", type=int, default=150)
     ap.add_argument("--no-multi", action="store_true", help="(--workload fas) keep one launch per sweep on every level")
+    ap.add_argument("--force-multi", type=lambda v: tuple(int(x) for x in v.split(",")), default=None,
+                    metavar="K,TY,TX,KERNEL",
+                    help="(--workload ogrid) time the fine level's sweeps with this multi-sweep setting")
     ap.add_argument("--workload", default="ogrid", choices=("ogrid", "ogrid_bl", "ogrid_terms", "batched", "cubed_sphere",
                                                               "mlp", "control", "fas"))
     ap.add_argument("--graph-chunk", type=int, default=GRAPH_CHUNK,
