@@ -172,17 +172,58 @@ def _timed(s, n, omega):
     return e, m, wall, ms * 1e-3 / launches, X0
 
 
-def _converge(step, e_star, budget_s):
+def _converge(step, e_star, budget_s, rtol=CONV_RTOL):
     """Call step() (returns (energies, mindets, units)) until the last report
-    is within CONV_RTOL of e_star with min det > 0; returns (units, seconds)."""
+    is within rtol of e_star with min det > 0; returns (units, seconds)."""
     done = 0
     t0 = time.perf_counter()
     while time.perf_counter() - t0 < budget_s:
         e, m, u = step()
         done += u
-        if abs(e[-1] - e_star) <= CONV_RTOL * abs(e_star) and m[-1] > 0:
+        if abs(e[-1] - e_star) <= rtol * abs(e_star) and m[-1] > 0:
             return done, time.perf_counter() - t0, float(e[-1])
     return None, time.perf_counter() - t0, float(e[-1])
+
+
+def _ttc_fixture(s, Xu, t_untangle, name, budget_s=60.0, fas_chunk=10, plain_chunk=1000, dim=2, ctx=None,
+                 with_cpu=False):
+    """Time to untangled + converged on a large workload against its pinned
+    fixture (tests/golden/estar_<name>.json: the GPU iteration's fixed point,
+    its energy recomputed in FP64, with the band a run counts as converged
+    in), from the untangled state Xu: FAS V(4,4) on 3 levels (the
+    workload's cycle) and the plain sweep, each within budget_s. With
+    with_cpu, the FP64 energy / min det of each converged state are
+    recomputed on the CPU (the oracle) against E* and the fixture's min det."""
+    fix = ROOT / "tests" / "golden" / f"estar_{name}.json"
+    if not fix.exists():
+        return None
+    f = json.loads(fix.read_text())
+    e_star, rtol = f["E_star"], f.get("convergence_rtol", CONV_RTOL)
+    s.set_X(Xu)
+    out = {"E_star": e_star, "E_untangled": float(s.core.energy()[0]), "rtol": rtol, "untangle_s": t_untangle,
+           "E_star_is": f["how"], "E_star_bias_bound": f.get("bias_bound"), "note": f.get("convergence_note")}
+
+    def errors():
+        if not with_cpu:
+            return {}
+        from oracle import oracle as O  # the checker
+
+        E, md = O.energy_mindet(dim, "barrier", 0.0, ctx.energy_stencil, s.get_X().astype(np.float64))
+        return {"rel_energy_err": abs(float(E) - e_star) / abs(e_star), "energy_fp64": float(E),
+                "min_det_fp64": float(md), "min_det_err": abs(float(md) - f["min_det"])}
+
+    for key, step in (("fas_v44_3_levels", lambda: (*s.run_fas(fas_chunk, nu_pre=4, nu_post=4, omega=0.8,
+                                                                   max_levels=3), fas_chunk)),
+                      ("plain", lambda: (*s.run(plain_chunk, omega=0.8, report_every=plain_chunk), plain_chunk))):
+        s.set_X(Xu)
+        n, t, E = _converge(step, e_star, budget_s, rtol)
+        unit = "cycles" if key != "plain" else "sweeps"
+        out[key] = {unit: n, "s": t + t_untangle if n else None, "E_end": E,
+                    "within_budget_s": budget_s, **(errors() if n else {})}
+        log(f"{name} {key}: {n} {unit}, {t:.3f}s to converged (E {E:.10g}, E* {e_star:.10g})")
+    if out["fas_v44_3_levels"]["s"] and out["plain"]["s"]:
+        out["speedup_fas_vs_plain"] = out["plain"]["s"] / out["fas_v44_3_levels"]["s"]
+    return out
 
 
 def _cpu_rates(run, units, n_sweeps):
@@ -385,8 +426,8 @@ def _cubed_proposal(s, Xu, n_plain=400):
     tangle seed (6 snapshots 25 sweeps apart of its 1.75 M interior DOFs:
     10.5 M pairs); then, from the untangled state, sweeps and time for the
     plain method and for the proposal in every k-th sweep (replace mode) to
-    reach the energy the plain method has after `n_plain` sweeps (no FP64
-    converged energy is pinned for this grid)."""
+    reach the energy the plain method has after `n_plain` sweeps (the time
+    to the pinned converged energy is measured by _ttc_fixture)."""
     from egg_amd import grids
     from egg_amd._cpp import cpp_core
     from egg_amd.smoothing import cpp_untangle
@@ -652,10 +693,13 @@ def bench_batched(args):
     allv = sum(counts) * K / t_fas
     log(f"FAS V(4,4) x{K}: {t_fas * 1e3:.1f} ms, fine {fine:.4g} / all-level {allv:.4g} node-updates/s, "
         f"E {e[0]:.6g} -> {e[-1]:.6g}")
-    proposal = None
+    proposal = ttc = None
     if not args.no_converge:
         proposal = _batched_proposal(s, X_fas, K, e[-1], t_fas)
         log(f"learned proposal {proposal}")
+        s.set_proposal(None)
+        ttc = _ttc_fixture(s, Xu, t_untangle, "batched64", fas_chunk=10, plain_chunk=1000, ctx=ctx,
+                           with_cpu=not args.no_cpu)
     cpu = parity = None
     if not args.no_cpu:
         cpu, parity = _batched_sample(s, g, index, Xu, 50)
@@ -675,7 +719,7 @@ def bench_batched(args):
                      "fas_level_variants": [int(v) for v in lv_best]},
         "time_to_untangled_s": t_untangle, "untangle_outer": outer,
         "energy_first_last": [float(e[0]), float(e[-1])], "final_min_det": float(m[-1]),
-        "learned_proposal": proposal,
+        "learned_proposal": proposal, "time_to_untangled_converged": ttc,
         "roofline": roof, "cpu_baseline": cpu, "parity": parity,
     }
 
@@ -730,6 +774,8 @@ def bench_cubed(args):
                                       "solve_ms": best_ms, "us_per_iteration": best_ms * 1e3 / max(iters, 1)}
             log(f"PCG 33^3 {op} {rec}: {iters} iterations, relres {relres:.2e}, {best_ms:.3f} ms")
     proposal = None if args.no_converge else _cubed_proposal(s, Xu)
+    ttc = None if args.no_converge else _ttc_fixture(s, Xu, t_untangle, "cubed_sphere65", fas_chunk=10,
+                                                     plain_chunk=500, dim=3, ctx=ctx, with_cpu=not args.no_cpu)
     cpu = parity = None
     if not args.no_cpu:
         cpu, parity = _cubed_sample(s, g, ctx, Xu)
@@ -744,7 +790,7 @@ def bench_cubed(args):
         "fas_v44_3_levels": {"cycles": K, "ms_per_cycle": t_fas / K * 1e3, "updates_per_cycle_by_level": counts,
                              "energy_first_last": [float(ef[0]), float(ef[-1])]},
         "pcg_33cubed": pcg_runs,
-        "learned_proposal_3d": proposal,
+        "learned_proposal_3d": proposal, "time_to_untangled_converged": ttc,
         "autotune": {"best_variant": int(best), "ms_per_variant": [float(v) for v in ms],
                      "fas_level_variants": [int(v) for v in lv_best]},
         "time_to_untangled_s": t_untangle, "untangle_outer": outer, "roofline": roof, "cpu_baseline": cpu,

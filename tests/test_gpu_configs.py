@@ -7,12 +7,15 @@ the FP64 oracle and by properties that hold at any size:
   FP64 converged energy E* (tests/golden/estar_ogrid129.json).
 * 64 independent multi-block strips of 3-9 blocks, sides 33..513 (28.6 M
   DOFs), 2 % tangled: untangle, FAS V(4,4) on 3 levels keeps every cell
-  valid, lowers the energy, and reports the FP64 energy of the state it
-  returns; 50 sweeps of the batch match 50 oracle sweeps of one complete
-  grid of it (grids are independent).
+  valid, lowers the energy, reports the FP64 energy of the state it
+  returns and reaches the pinned energy floor (estar_batched64.json); 50
+  sweeps of the batch match 50 oracle sweeps of one complete grid of it
+  (grids are independent).
 * 3D cubed sphere, 7 x 65^3, 1 % tangled: untangle, FAS V(4,4) on 3
-  levels as above; one sweep matches the oracle on every 40th DOF (a Jacobi
-  sweep moves each DOF from frozen neighbours, so a sample sweep is exact).
+  levels as above, to 1e-6 of the pinned converged energy
+  (estar_cubed_sphere65.json); one sweep matches the oracle on every 40th
+  DOF (a Jacobi sweep moves each DOF from frozen neighbours, so a sample
+  sweep is exact).
 """
 
 from __future__ import annotations
@@ -89,6 +92,12 @@ def test_batched_64_grids(oracle):
     assert np.all(m > 0) and np.all(np.diff(e) < 0)
     ec, mc = oracle.energy_mindet(2, "barrier", 0.0, ctx.energy_stencil, s.get_X())
     assert abs(ec - e[-1]) <= 1e-9 * ec and mc == pytest.approx(m[-1], rel=1e-6)
+    # on to the pinned energy floor (tests/golden/estar_batched64.json: the
+    # strips' optimum has zero energy, E* is the floor FP32 positions set)
+    f = json.loads((Path(__file__).parent / "golden" / "estar_batched64.json").read_text())
+    ef, mf = s.run_fas(12, nu_pre=4, nu_post=4, omega=0.8, max_levels=3)
+    print(f"FAS V(4,4) x15: E {ef[-1]:.6g} vs the floor {f['E_star']:.6g}", flush=True)
+    assert np.all(mf > 0) and abs(ef[-1] - f["E_star"]) <= f["convergence_rtol"] * f["E_star"]
     # one complete grid of the batch (grids are independent): the one nearest 300 k DOFs
     frees = [int(g.free[n0:n0 + nn].sum()) for (_, _, n0, nn) in index]
     gi = int(np.argmin([abs(f - 300_000) for f in frees]))
@@ -133,3 +142,16 @@ def test_cubed_sphere_7x65(oracle):
     e, m = s.run_fas(2, nu_pre=4, nu_post=4, omega=0.8, max_levels=3)
     print(f"FAS V(4,4): E {e.tolist()}", flush=True)
     assert np.all(m > 0) and e[-1] < e[0] < e1[-1]
+    # FAS from the untangled state to the pinned converged energy (the GPU
+    # fixed point, tests/golden/estar_cubed_sphere65.json)
+    f = json.loads((Path(__file__).parent / "golden" / "estar_cubed_sphere65.json").read_text())
+    s.set_X(Xu)
+    cycles = 0
+    for _ in range(20):
+        ef, mf = s.run_fas(50, nu_pre=4, nu_post=4, omega=0.8, max_levels=3)
+        cycles += 50
+        assert np.all(mf > 0)
+        if abs(ef[-1] - f["E_star"]) <= f["convergence_rtol"] * f["E_star"]:
+            break
+    print(f"FAS V(4,4) to E*: {cycles} cycles, E {ef[-1]:.10g} vs {f['E_star']:.10g}", flush=True)
+    assert abs(ef[-1] - f["E_star"]) <= f["convergence_rtol"] * f["E_star"] and cycles <= 1000

@@ -664,3 +664,23 @@ def test_directional_hessians_pinned_at_zero_residual(oracle, dim):
         assert abs(e) < 1e-20
         Hfd = _fd_hessian(energy, P[slot].copy())
         np.testing.assert_allclose(H - Hfd, coeff * w / L**2 * np.eye(dim), rtol=0, atol=1e-5 * np.abs(H).max())
+
+
+def test_large_workload_fixtures():
+    """The pinned energies of the batched and cubed-sphere workloads are the
+    GPU iteration's fixed point (their FP64 fixed points are out of the CPU
+    oracle's reach); the same procedure lands within 1e-8 of the O-grid's FP64
+    E*, the bias bound each fixture records."""
+    import json
+    from pathlib import Path
+
+    gold = Path(__file__).parent / "golden"
+    e_ref = json.loads((gold / "estar_ogrid129.json").read_text())["E_star"]
+    for name in ("batched64", "cubed_sphere65"):
+        f = json.loads((gold / f"estar_{name}.json").read_text())
+        b = f["bias_bound"]
+        assert b["fp64_E_star"] == e_ref
+        assert abs(b["gpu_fixed_point_energy"] - e_ref) / e_ref == b["relative"] < 1e-8
+        assert f["E_star"] > 0 and f["min_det"] > 0 and 0 < f["convergence_rtol"] <= 0.1
+        # the FP64 recomputation on the CPU agrees with the device's FP64 reductions
+        assert abs(f["E_star"] - f["E_device_fp64_reduction"]) <= 1e-9 * f["E_star"]
