@@ -113,6 +113,82 @@ FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 def _traffic(workload, variant):
     kib = PMC_KIB.get((workload, int(variant)))
     return None if kib is None else (kib[0] * FETCH_SCALE + kib[1] * WRITE_SCALE) * 1024.0
+
+
+PMC_SWEEPS = 20  # sweeps per tile variant in a counter child run
+
+
+def _pmc_child(args):
+    """--pmc-child: the O-grid workload's plain sweeps for a rocprofv3 --pmc
+    run of the parent: untangle, then PMC_SWEEPS one-launch sweeps with every
+    tile variant in turn (the parent splits the shape-phase sweep dispatches
+    into blocks of PMC_SWEEPS in variant order); prints the byte count of the
+    store copy every first run makes (the counters' calibration)."""
+    from egg_amd import grids
+    from egg_amd.smoothing import cpp_untangle
+
+    g = grids.tangle(grids.ogrid_2d(129, 65), frac=0.02, factor=1.5, seed=SEED)
+    ctx, bsc, s = _build(g)
+    Xu, _, _, _ = cpp_untangle(ctx, bsc, g.X, session=s)
+    s.core.set_graph_chunk(0)
+    nv = len(s.core.autotune(1, 0.8)[1])  # the variant count (and every variant's code warmed)
+    for v in range(nv):
+        s.core.set_tile_variant(v)
+        s.set_X(Xu)
+        s.run(PMC_SWEEPS, omega=0.8)
+    plane = (int(s.core.info()["n_slots"]) + 63) // 64 * 64  # the store's plane stride (64-slot multiple)
+    print(json.dumps({"variants": nv, "store_bytes": plane * 2 * 4}), flush=True)
+
+
+def _live_traffic():
+    """HBM bytes per sweep launch of every tile variant on the O-grid,
+    measured in this run: two child runs of this file (--pmc-child) under
+    rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE (one counter per run, each
+    under a hard time limit), started before this process touches the GPU.
+    Each counter is calibrated by its reading for the device copy of the
+    store in the same run (N bytes, known). Returns ({variant: bytes},
+    info) or (None, reason)."""
+    import csv
+    import glob
+    import os
+    import shutil
+    import subprocess
+    import tempfile
+
+    if not shutil.which("rocprofv3"):
+        return None, "rocprofv3 not found"
+    reads = {}
+    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="eggmi_pmc_")
+        try:
+            r = subprocess.run(["timeout", "-s", "KILL", "150", "rocprofv3", "--pmc", ctr, "--output-format", "csv",
+                                "-d", d, "-o", "run", "--", sys.executable, str(ROOT / "bench.py"), "--pmc-child"],
+                               capture_output=True, text=True, cwd="/tmp", env={**os.environ, "TMPDIR": "/tmp"})
+            files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+            lines = [ln for ln in r.stdout.splitlines() if ln.startswith("{")]
+            if r.returncode != 0 or not files or not lines:
+                return None, f"rocprofv3 --pmc {ctr}: exit {r.returncode}, {len(files)} result files"
+            child = json.loads(lines[-1])
+            with open(files[0], newline="") as fh:
+                rows = [x for x in csv.DictReader(fh) if x.get("Counter_Name", ctr) == ctr]
+        finally:
+            shutil.rmtree(d, ignore_errors=True)
+        rows.sort(key=lambda x: int(x.get("Dispatch_Id", 0)))
+        sweeps = [float(x["Counter_Value"]) for x in rows if "sweep_kernel<2, 0," in x["Kernel_Name"]
+                  or "sweep_kernel_cpl<2, 0," in x["Kernel_Name"] or "sample_kernel<0," in x["Kernel_Name"]]
+        copies = [float(x["Counter_Value"]) for x in rows if "copyBuffer" in x["Kernel_Name"]]
+        nv = child["variants"]
+        tail = sweeps[-nv * PMC_SWEEPS:]
+        if len(tail) != nv * PMC_SWEEPS or not copies:
+            return None, f"rocprofv3 --pmc {ctr}: {len(sweeps)} sweep dispatches, {len(copies)} copies"
+        per_v = [float(np.median(tail[v * PMC_SWEEPS:(v + 1) * PMC_SWEEPS])) for v in range(nv)]
+        scale = child["store_bytes"] / (max(copies) * 1024.0)
+        reads[ctr] = (per_v, scale, max(copies))
+    fetch, fs, fcopy = reads["FETCH_SIZE"]
+    write, ws, wcopy = reads["WRITE_SIZE"]
+    out = {v: fetch[v] * 1024.0 * fs + write[v] * 1024.0 * ws for v in range(len(fetch))}
+    return out, {"fetch_kib": fetch, "write_kib": write, "fetch_scale": fs, "write_scale": ws,
+                 "copy_fetch_kib": fcopy, "copy_write_kib": wcopy}
 METRIC = "node-updates/s; time-to-untangled+converged, plain vs MLP-proposal, 1 GPU"
 
 
@@ -530,7 +606,16 @@ def bench_ogrid(args):
     spl = args.steps / s.core.last_timing()[1]  # sweeps per launch
     X_end = s.get_X()
     s.set_X(X_start)
-    roof = _roofline(s, 2, n_free, args.steps, 0.8, k_avg, _traffic("ogrid", tune_best) if spl == 1 else None, spl)
+    live, live_info = getattr(args, "live_traffic", None) or (None, "not run (--no-pmc)")
+    if spl == 1 and live:
+        traffic, source = live[int(tune_best)], {"source": "two rocprofv3 --pmc child runs of this bench "
+                                                 "(FETCH_SIZE, WRITE_SIZE), calibrated by the store copy",
+                                                 **live_info}
+    else:
+        traffic = _traffic("ogrid", tune_best) if spl == 1 else None
+        source = {"source": "committed PMC table (profiles/sweep2d_ogrid129.md)", "live": str(live_info)}
+    roof = _roofline(s, 2, n_free, args.steps, 0.8, k_avg, traffic, spl)
+    roof["traffic_source"] = source
     s.set_X(X_end)
     log(f"timed {args.steps} sweeps {t_run * 1e3:.2f} ms -> {value:.4g} node-updates/s  E={e[-1]:.9g}")
     log(f"sweep kernel {roof['kernel_avg_us']:.2f} us/launch, {roof['trials_per_update']:.3f} trials/update "
@@ -1159,6 +1244,10 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline and parity leg")
     ap.add_argument("--no-converge", action="store_true", help="skip the time-to-converged runs")
     ap.add_argument("--cpu-sweeps", type=int, default=150)
+    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--no-pmc", action="store_true",
+                    help="(--workload ogrid) roofline traffic from the committed PMC table instead of two live "
+                         "rocprofv3 --pmc child runs")
     ap.add_argument("--no-multi", action="store_true", help="(--workload fas) keep one launch per sweep on every level")
     ap.add_argument("--force-multi", type=lambda v: tuple(int(x) for x in v.split(",")), default=None,
                     metavar="K,TY,TX,KERNEL",
@@ -1168,11 +1257,20 @@ def main() -> int:
     ap.add_argument("--graph-chunk", type=int, default=GRAPH_CHUNK,
                     help="sweeps per captured hipGraph replay (even; 0 = single launches, e.g. under a tracer)")
     args = ap.parse_args()
+    if args.pmc_child:
+        _pmc_child(args)
+        return 0
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the path would shard by blocks, one halo exchange per sweep, but only the one-GPU path is built)",
               file=sys.stderr)
         return 2
+    if args.workload == "ogrid" and not args.no_pmc:  # before this process touches the GPU
+        try:
+            args.live_traffic = _live_traffic()
+        except Exception as exc:  # the table stands in, and the line says why
+            args.live_traffic = (None, f"{type(exc).__name__}: {exc}")
+        log(f"live traffic: {args.live_traffic}")
     fn = {"ogrid": bench_ogrid, "ogrid_bl": bench_ogrid_bl, "ogrid_terms": bench_ogrid_terms, "batched": bench_batched, "cubed_sphere": bench_cubed,
           "mlp": bench_mlp, "control": bench_control, "fas": bench_fas}[args.workload]
     print(json.dumps(fn(args)), flush=True)
