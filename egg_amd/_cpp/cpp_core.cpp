@@ -598,6 +598,10 @@ class Session
     // into eggmi_control; sliding walls are not supported.
     void attach_control(const py::dict& w, const py::dict& structured)
     {
+        if (w.contains("blocks")) {
+            attach_control_topo(w, structured);
+            return;
+        }
         for (const char* k : {"walls", "X0"})
             if (w.contains(k) && !w[k].is_none())
                 throw py::value_error(std::string("control wire field '") + k +
@@ -645,6 +649,126 @@ class Session
         }
         check(eggmi_session_set_control(s_, &c));
         n_ctrl_ = nc;
+        topo_ = false;
+    }
+
+    // The multi-block control wire (reference src/bindings.cpp:593-740, the
+    // "blocks" key): per-block nets over one stacked control vector, the
+    // reduction CSR (nq, r_off, r_col, r_coef) and optional penalty rows
+    // (p_off, p_col, p_coef, p_w); sliding-wall frames are not supported.
+    void attach_control_topo(const py::dict& w, const py::dict& structured)
+    {
+        for (const char* k : {"slide", "wall_faces", "seam_face", "arena"})
+            if (w.contains(k) && !w[k].is_none())
+                throw py::value_error(std::string("control topo wire field '") + k +
+                                      "': sliding-wall frames are not supported by this compute core");
+        const py::list shapes = structured["interior_shapes"].cast<py::list>();
+        const py::list blocks = w["blocks"].cast<py::list>();
+        std::vector<eggmi_control> nets;
+        std::vector<IArr> offs;
+        std::vector<DArr> wts, bs;
+        offs.reserve(blocks.size() * dim_);
+        wts.reserve(blocks.size() * dim_);
+        bs.reserve(blocks.size());
+        int64_t n_stack = 0;
+        for (const auto item : blocks) {
+            const py::dict bd = item.cast<py::dict>();
+            eggmi_control c {};
+            c.block = bd["block"].cast<int>();
+            if (c.block < 0 || c.block >= (int)shapes.size())
+                throw py::value_error("control topo: block " + std::to_string(c.block) + " out of range");
+            IArr bshape = shapes[c.block].cast<IArr>();
+            IArr cs = get_i(bd, "ctrl_shape");
+            if (cs.size() != dim_) throw py::value_error("control topo: ctrl_shape must have dim entries");
+            int64_t nc = 1, nn = dim_;
+            for (int k = 0; k < dim_; ++k) {
+                c.ctrl_shape[k] = cs.data()[k];
+                nc *= c.ctrl_shape[k];
+                nn *= bshape.data()[k];
+            }
+            c.k_supp = bd["k_supp"].cast<int>();
+            for (int k = 0; k < dim_; ++k) {
+                offs.push_back(get_i(bd, ("off" + std::to_string(k)).c_str()));
+                wts.push_back(get_d(bd, ("wt" + std::to_string(k)).c_str()));
+                if (offs.back().size() != bshape.data()[k])
+                    throw py::value_error("control topo: off" + std::to_string(k) +
+                                          " must hold one value per node of the block's axis " + std::to_string(k));
+                if (wts.back().size() != offs.back().size() * c.k_supp)
+                    throw py::value_error("control topo: wt" + std::to_string(k) + " size != off size * k_supp");
+                c.off[k] = offs.back().data();
+                c.wt[k] = wts.back().data();
+            }
+            bs.push_back(get_d(bd, "b"));
+            if (bs.back().size() != nn) throw py::value_error("control topo: b size != prod(node_shape) * dim");
+            c.b = bs.back().data();
+            n_stack += nc;
+            nets.push_back(c);
+        }
+        DArr C0 = get_d(w, "C0");
+        if (C0.size() != n_stack * dim_) throw py::value_error("control topo: C0 size != total controls * dim");
+        IArr r_off = get_i(w, "r_off"), r_col = get_i(w, "r_col");
+        DArr r_coef = get_d(w, "r_coef");
+        if (r_off.size() != n_stack * dim_ + 1 || r_col.size() != r_coef.size() ||
+            r_off.data()[r_off.size() - 1] != r_col.size())
+            throw py::value_error("control topo: malformed reduction CSR");
+        eggmi_control_topo t {};
+        t.n_nets = (int32_t)nets.size();
+        t.nets = nets.data();
+        t.C0 = C0.data();
+        t.nq = w["nq"].cast<int64_t>();
+        t.r_off = r_off.data();
+        t.r_col = r_col.data();
+        t.r_coef = r_coef.data();
+        IArr p_off, p_col;
+        DArr p_coef, p_w;
+        if (w.contains("p_off") && !w["p_off"].is_none()) {
+            p_off = get_i(w, "p_off");
+            p_col = get_i(w, "p_col");
+            p_coef = get_d(w, "p_coef");
+            p_w = get_d(w, "p_w");
+            if (p_col.size() != p_coef.size() ||
+                (p_off.size() == 0 ? p_col.size() != 0 : p_off.data()[p_off.size() - 1] != p_col.size()) ||
+                (p_off.size() > 0 && p_w.size() != p_off.size() - 1))
+                throw py::value_error("control topo: malformed penalty CSR");
+            t.n_pen = p_off.size() > 0 ? p_off.size() - 1 : 0;
+            t.p_off = p_off.data();
+            t.p_col = p_col.data();
+            t.p_coef = p_coef.data();
+            t.p_w = p_w.data();
+        }
+        check(eggmi_session_set_control_topo(s_, &t));
+        n_ctrl_ = n_stack;
+        topo_ = true;
+        std::vector<int64_t> nn;
+        for (const auto& b : bs) nn.push_back(b.size());
+        topo_b_sizes_ = nn;
+    }
+
+    void set_control_reduction(int64_t nq, const IArr& off, const IArr& col, const DArr& coef)
+    {
+        if (!topo_) throw py::value_error("set_control_reduction: session has no multi-block control wire");
+        if (off.size() != n_ctrl_ * dim_ + 1 || col.size() != coef.size() || off.data()[off.size() - 1] != col.size())
+            throw py::value_error("control topo: malformed reduction CSR");
+        check(eggmi_control_set_reduction(s_, nq, off.data(), col.data(), coef.data()));
+    }
+
+    void set_control_penalty(const IArr& off, const IArr& col, const DArr& coef, const DArr& w)
+    {
+        if (!topo_) throw py::value_error("set_control_penalty: session has no multi-block control wire");
+        if (col.size() != coef.size() || (off.size() == 0 ? col.size() != 0 : off.data()[off.size() - 1] != col.size()) ||
+            (off.size() > 0 && w.size() != off.size() - 1))
+            throw py::value_error("control topo: malformed penalty CSR");
+        const int64_t n_pen = off.size() > 0 ? off.size() - 1 : 0;
+        check(eggmi_control_set_penalty(s_, n_pen, off.data(), col.data(), coef.data(), w.data()));
+    }
+
+    void set_control_b(int block_index, const DArr& b)
+    {
+        if (!topo_) throw py::value_error("set_control_b: session has no multi-block control wire");
+        if (block_index < 0 || block_index >= (int)topo_b_sizes_.size())
+            throw py::value_error("control topo: set_b block index out of range");
+        if (b.size() != topo_b_sizes_[block_index]) throw py::value_error("control topo: set_b size mismatch");
+        check(eggmi_control_set_b(s_, block_index, b.data(), (int64_t)b.size()));
     }
 
     void set_C(const DArr& C) { check(eggmi_control_set_C(s_, C.data(), (int64_t)C.size())); }
@@ -982,6 +1106,8 @@ class Session
     int dim_;
     size_t n_ = 0;
     int64_t n_ctrl_ = 0;
+    bool topo_ = false;                   // a multi-block control wire is attached
+    std::vector<int64_t> topo_b_sizes_;   // its nets' b lengths
     eggmi_session* s_ = nullptr;
     py::object proposal_ = py::none();
 };
@@ -1031,6 +1157,12 @@ PYBIND11_MODULE(cpp_core, m)
       .def("set_C", &Session::set_C, py::arg("C"),
            "Upload a full control lattice (n_ctrl * dim values) and re-evaluate the net into the state.")
       .def("get_C", &Session::get_C, "Download the full control lattice (n_ctrl * dim values).")
+      .def("set_control_reduction", &Session::set_control_reduction, py::arg("nq"), py::arg("off"), py::arg("col"),
+           py::arg("coef"), "Replace the multi-block reduction CSR dC = R dq (same stacked-component rows).")
+      .def("set_control_penalty", &Session::set_control_penalty, py::arg("off"), py::arg("col"), py::arg("coef"),
+           py::arg("w"), "Replace the multi-block penalty rows (energy 1/2 sum w (P C)^2).")
+      .def("set_control_b", &Session::set_control_b, py::arg("block_index"), py::arg("b"),
+           "Replace one net's Boolean-sum offset and re-evaluate the nets into the state.")
       .def("run_control", &Session::run_control, py::arg("n_outer"), py::kw_only(), py::arg("phase") = "barrier",
            py::arg("grad_tol") = 1e-8, py::arg("alpha_min") = 1e-4, py::arg("lm0") = 0.0,
            py::arg("pcg_max_iter") = 200, py::arg("pcg_rtol") = 1e-10, py::arg("pcg_forcing") = true,

@@ -628,6 +628,41 @@ eggmi_status eggmi_session_set_control(eggmi_session* s, const eggmi_control* c)
     });
 }
 
+eggmi_status eggmi_session_set_control_topo(eggmi_session* s, const eggmi_control_topo* t)
+{
+    return guarded([&] {
+        need(s, "session");
+        need(t, "control topology");
+        s->s.set_control_topo(*t);
+    });
+}
+
+eggmi_status eggmi_control_set_reduction(eggmi_session* s, int64_t nq, const int32_t* r_off, const int32_t* r_col,
+                                         const double* r_coef)
+{
+    return guarded([&] {
+        need(s, "session");
+        s->s.control_topo_set_reduction(nq, r_off, r_col, r_coef);
+    });
+}
+
+eggmi_status eggmi_control_set_penalty(eggmi_session* s, int64_t n_pen, const int32_t* p_off, const int32_t* p_col,
+                                       const double* p_coef, const double* p_w)
+{
+    return guarded([&] {
+        need(s, "session");
+        s->s.control_topo_set_penalty(n_pen, p_off, p_col, p_coef, p_w);
+    });
+}
+
+eggmi_status eggmi_control_set_b(eggmi_session* s, int32_t net, const double* b, int64_t n)
+{
+    return guarded([&] {
+        need(s, "session");
+        s->s.control_topo_set_b(net, b, n);
+    });
+}
+
 eggmi_status eggmi_control_set_C(eggmi_session* s, const double* C, int64_t n)
 {
     return guarded([&] {

@@ -95,18 +95,17 @@ template <int D> __device__ __forceinline__ long long node_local(const CtrlGeom&
 
 // ---------------------------------------------------------------- map
 
-template <int D, bool DIR>
-__global__ __launch_bounds__(kThreads) void ctrl_eval_kernel(const CtrlGeom g, const double* src, const double* b,
-                                                             double* out_dir, float* out_plane)
+// M src at block-local node I (dim values into acc): FREE_MAP reads a
+// free-control vector through l2f (fixed controls 0), otherwise src is the
+// whole lattice
+template <int D, bool FREE_MAP>
+__device__ __forceinline__ void eval_at(const CtrlGeom& g, const int* I, const double* src, double* acc)
 {
-    const long long n = (long long)blockIdx.x * kThreads + threadIdx.x;
-    if (n >= g.n_nodes) return;
-    int I[3];
-    unravel_node<D>(g, n, I);
     int a0[3];
 #pragma unroll
     for (int k = 0; k < D; ++k) a0[k] = g.off[k][I[k]];
-    double acc[D] = {};
+#pragma unroll
+    for (int c = 0; c < D; ++c) acc[c] = 0.0;
     for (int t0 = 0; t0 < g.ks; ++t0) {
         const double w0 = g.wt[0][(long long)I[0] * g.ks + t0];
         if (w0 == 0.0) continue;
@@ -121,7 +120,7 @@ __global__ __launch_bounds__(kThreads) void ctrl_eval_kernel(const CtrlGeom g, c
                     a = a * g.c[2] + (a0[2] + t2);
                 }
                 if (w == 0.0) continue;
-                if constexpr (DIR) {
+                if constexpr (FREE_MAP) {
                     const int f = g.l2f[a];
                     if (f < 0) continue;
 #pragma unroll
@@ -133,6 +132,18 @@ __global__ __launch_bounds__(kThreads) void ctrl_eval_kernel(const CtrlGeom g, c
             }
         }
     }
+}
+
+template <int D, bool DIR>
+__global__ __launch_bounds__(kThreads) void ctrl_eval_kernel(const CtrlGeom g, const double* src, const double* b,
+                                                             double* out_dir, float* out_plane)
+{
+    const long long n = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (n >= g.n_nodes) return;
+    int I[3];
+    unravel_node<D>(g, n, I);
+    double acc[D];
+    eval_at<D, DIR>(g, I, src, acc);
     if constexpr (!DIR) {
 #pragma unroll
         for (int c = 0; c < D; ++c) acc[c] += b ? b[n * D + c] : 0.0;
@@ -199,33 +210,66 @@ __device__ __forceinline__ void sample_A(const CtrlGeom& g, const int* I0, int b
     }
 }
 
+// metric gradient and symmetric Hessian of block-local sample p (positions
+// xb of the block), stored component-major at index q of an ns-sample space
 template <int D, int KIND>
-__global__ __launch_bounds__(kThreads) void ctrl_samples_kernel(const CtrlGeom g, const double* xb, double* gs,
-                                                                double* hs)
+__device__ __forceinline__ void sample_derivs(const CtrlGeom& g, long long p, const double* xb, double* gs,
+                                              double* hs, long long ns, long long q)
 {
     constexpr int NC = 1 << D, V = kV<D>;
-    const long long p = (long long)blockIdx.x * kThreads + threadIdx.x;
-    if (p >= g.n_cells * NC) return;
     const long long cell = p / NC;
     const int b = int(p - cell * NC);
     int I0[3];
     cell_origin<D>(g, cell, I0);
     double t[V];
     sample_A<D>(g, I0, b, xb, nullptr, 0.0, t);
-    // component-major (entry e of sample p at e * ns + p): coalesced
-    const long long ns = g.n_cells * NC;
     int e = 0;
-    for (int q = 0; q < V; ++q)
-        for (int r = q; r < V; ++r) {
+    for (int i = 0; i < V; ++i)
+        for (int r = i; r < V; ++r) {
             HD T[V];
 #pragma unroll
             for (int j = 0; j < V; ++j) T[j] = HD(t[j]);
-            T[q].a = 1.0;
+            T[i].a = 1.0;
             T[r].b = 1.0;
             const HD m = metric_value<D, KIND>(T);
-            hs[(long long)(e++) * ns + p] = m.ab;
-            if (q == r) gs[(long long)q * ns + p] = m.a;
+            hs[(long long)(e++) * ns + q] = m.ab;
+            if (i == r) gs[(long long)i * ns + q] = m.a;
         }
+}
+
+template <int D, int KIND>
+__global__ __launch_bounds__(kThreads) void ctrl_samples_kernel(const CtrlGeom g, const double* xb, double* gs,
+                                                                double* hs)
+{
+    const long long p = (long long)blockIdx.x * kThreads + threadIdx.x;
+    const long long ns = g.n_cells * (1 << D);
+    // component-major (entry e of sample p at e * ns + p): coalesced
+    if (p < ns) sample_derivs<D, KIND>(g, p, xb, gs, hs, ns, p);
+}
+
+// z = H dA for block-local sample p (dA from the block-local field dir),
+// the Hessian read and z written at index q of an ns-sample space
+template <int D>
+__device__ __forceinline__ void sample_hv_at(const CtrlGeom& g, long long p, const double* hs, const double* dir,
+                                             double* zs, long long ns, long long q)
+{
+    constexpr int NC = 1 << D, V = kV<D>;
+    const long long cell = p / NC;
+    const int b = int(p - cell * NC);
+    int I0[3];
+    cell_origin<D>(g, cell, I0);
+    double dA[V];
+    sample_A<D>(g, I0, b, dir, nullptr, 0.0, dA);
+    double H[kNHs<D>];
+#pragma unroll
+    for (int e = 0; e < kNHs<D>; ++e) H[e] = hs[(long long)e * ns + q];
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+        double y = 0.0;
+#pragma unroll
+        for (int r = 0; r < V; ++r) y += H[sym_index<D>(i, r)] * dA[r];
+        zs[(long long)i * ns + q] = y;
+    }
 }
 
 // z_p = H_p dA_p per sample, dA_p from the block-local direction field
@@ -233,63 +277,36 @@ template <int D>
 __global__ __launch_bounds__(kThreads) void ctrl_sample_hv_kernel(const CtrlGeom g, const double* hs,
                                                                   const double* dir, double* zs)
 {
-    constexpr int NC = 1 << D, V = kV<D>;
     const long long p = (long long)blockIdx.x * kThreads + threadIdx.x;
-    if (p >= g.n_cells * NC) return;
+    const long long ns = g.n_cells * (1 << D);
+    if (p < ns) sample_hv_at<D>(g, p, hs, dir, zs, ns, p);
+}
+
+// energy and det of block-local sample p at xb + a dir
+template <int D, int KIND>
+__device__ __forceinline__ void sample_energy(const CtrlGeom& g, long long p, const double* xb, const double* dir,
+                                              double a, double& e, double& m)
+{
+    constexpr int NC = 1 << D, V = kV<D>;
     const long long cell = p / NC;
     const int b = int(p - cell * NC);
     int I0[3];
     cell_origin<D>(g, cell, I0);
-    double dA[V];
-    sample_A<D>(g, I0, b, dir, nullptr, 0.0, dA);
-    const long long ns = g.n_cells * NC;
-    double H[kNHs<D>];
-#pragma unroll
-    for (int e = 0; e < kNHs<D>; ++e) H[e] = hs[(long long)e * ns + p];
-#pragma unroll
-    for (int q = 0; q < V; ++q) {
-        double y = 0.0;
-#pragma unroll
-        for (int r = 0; r < V; ++r) y += H[sym_index<D>(q, r)] * dA[r];
-        zs[(long long)q * ns + p] = y;
+    double t[V];
+    sample_A<D>(g, I0, b, xb, dir, a, t);
+    if constexpr (D == 2) {
+        e += mu2_value<KIND, double>(t[0], t[1], t[2], t[3], 0.0);
+        m = fmin(m, t[0] * t[3] - t[1] * t[2]);
+    } else {
+        e += mu3_value<KIND, double>(t, 0.0);
+        m = fmin(m, det3(t));
     }
 }
 
-// FP64 cell energy and min det of the block at two step lengths: each
-// workgroup walks a contiguous run of samples with stride kThreads, then a
-// fixed tree per workgroup and a one-workgroup fold over the partials
-template <int D, int KIND>
-__global__ __launch_bounds__(kThreads) void ctrl_energy_kernel(const CtrlGeom g, const double* xb, const double* dir,
-                                                               double a0, double a1, double* partials)
+// fixed-tree fold of the four per-thread partials of a workgroup into
+// partials[4 * blockIdx.x ..]
+__device__ __forceinline__ void energy_fold(double e0, double m0, double e1, double m1, double* partials)
 {
-    constexpr int NC = 1 << D, V = kV<D>;
-    const long long ns = g.n_cells * NC;
-    const long long per = (ns + gridDim.x - 1) / gridDim.x;
-    const long long lo = (long long)blockIdx.x * per, hi = min(ns, lo + per);
-    double e0 = 0.0, e1 = 0.0, m0 = __builtin_huge_val(), m1 = __builtin_huge_val();
-    for (long long p = lo + threadIdx.x; p < hi; p += kThreads) {
-        const long long cell = p / NC;
-        const int b = int(p - cell * NC);
-        int I0[3];
-        cell_origin<D>(g, cell, I0);
-        double t[V];
-        sample_A<D>(g, I0, b, xb, dir, a0, t);
-        if constexpr (D == 2) {
-            e0 += mu2_value<KIND, double>(t[0], t[1], t[2], t[3], 0.0);
-            m0 = fmin(m0, t[0] * t[3] - t[1] * t[2]);
-        } else {
-            e0 += mu3_value<KIND, double>(t, 0.0);
-            m0 = fmin(m0, det3(t));
-        }
-        sample_A<D>(g, I0, b, xb, dir, a1, t);
-        if constexpr (D == 2) {
-            e1 += mu2_value<KIND, double>(t[0], t[1], t[2], t[3], 0.0);
-            m1 = fmin(m1, t[0] * t[3] - t[1] * t[2]);
-        } else {
-            e1 += mu3_value<KIND, double>(t, 0.0);
-            m1 = fmin(m1, det3(t));
-        }
-    }
     __shared__ double sh[4][kThreads];
     sh[0][threadIdx.x] = e0;
     sh[1][threadIdx.x] = m0;
@@ -306,6 +323,24 @@ __global__ __launch_bounds__(kThreads) void ctrl_energy_kernel(const CtrlGeom g,
         __syncthreads();
     }
     if (threadIdx.x < 4) partials[blockIdx.x * 4 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// FP64 cell energy and min det of the block at two step lengths: each
+// workgroup walks a contiguous run of samples with stride kThreads, then a
+// fixed tree per workgroup and a one-workgroup fold over the partials
+template <int D, int KIND>
+__global__ __launch_bounds__(kThreads) void ctrl_energy_kernel(const CtrlGeom g, const double* xb, const double* dir,
+                                                               double a0, double a1, double* partials)
+{
+    const long long ns = g.n_cells * (1 << D);
+    const long long per = (ns + gridDim.x - 1) / gridDim.x;
+    const long long lo = (long long)blockIdx.x * per, hi = min(ns, lo + per);
+    double e0 = 0.0, e1 = 0.0, m0 = __builtin_huge_val(), m1 = __builtin_huge_val();
+    for (long long p = lo + threadIdx.x; p < hi; p += kThreads) {
+        sample_energy<D, KIND>(g, p, xb, dir, a0, e0, m0);
+        sample_energy<D, KIND>(g, p, xb, dir, a1, e1, m1);
+    }
+    energy_fold(e0, m0, e1, m1, partials);
 }
 
 __global__ __launch_bounds__(kThreads) void ctrl_energy_finish(const double* partials, int n, double* out)
@@ -332,17 +367,15 @@ __global__ __launch_bounds__(kThreads) void ctrl_energy_finish(const double* par
 
 // ---------------------------------------------------------------- node gathers
 
+// add to acc what block-local node I receives from the block's incident
+// samples: mode 0 the column weights times gs (D values), mode 2 the
+// contracted Hessian from hs (D x D); the block's sample p sits at s0 + p
+// of an ns-sample component-major space
 template <int D, int MODE>
-__global__ __launch_bounds__(kThreads) void ctrl_node_kernel(const CtrlGeom g, const double* gs, const double* hs,
-                                                             const double* dir, double* out)
+__device__ __forceinline__ void node_acc(const CtrlGeom& g, const int* I, const double* gs, const double* hs,
+                                         long long ns, long long s0, double* acc)
 {
-    constexpr int NC = 1 << D, V = kV<D>;
-    const long long n = (long long)blockIdx.x * kThreads + threadIdx.x;
-    if (n >= g.n_nodes) return;
-    int I[3];
-    unravel_node<D>(g, n, I);
-    const long long ns = g.n_cells * NC;
-    double acc[MODE == 2 ? V : D] = {};
+    constexpr int NC = 1 << D;
     // incident cells: origin I - o for o in {0, 1}^D, the node at corner o
     for (int o = 0; o < NC; ++o) {
         int I0[3] = {0, 0, 0};
@@ -371,14 +404,13 @@ __global__ __launch_bounds__(kThreads) void ctrl_node_kernel(const CtrlGeom g, c
             } else {
                 continue;
             }
-            const long long p = cell * NC + b;
+            const long long p = s0 + cell * NC + b;
             if constexpr (MODE == 0) {
 #pragma unroll
                 for (int i = 0; i < D; ++i)
 #pragma unroll
                     for (int k = 0; k < D; ++k) acc[i] += u[k] * gs[(long long)(i * D + k) * ns + p];
             } else {
-
 #pragma unroll
                 for (int i = 0; i < D; ++i)
 #pragma unroll
@@ -394,21 +426,33 @@ __global__ __launch_bounds__(kThreads) void ctrl_node_kernel(const CtrlGeom g, c
             }
         }
     }
-    constexpr int NO = MODE == 2 ? V : D;
+}
+
+template <int D, int MODE>
+__global__ __launch_bounds__(kThreads) void ctrl_node_kernel(const CtrlGeom g, const double* gs, const double* hs,
+                                                             const double* dir, double* out)
+{
+    constexpr int NO = MODE == 2 ? D * D : D;
+    const long long n = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (n >= g.n_nodes) return;
+    int I[3];
+    unravel_node<D>(g, n, I);
+    double acc[NO] = {};
+    node_acc<D, MODE>(g, I, gs, hs, g.n_cells * (1 << D), 0, acc);
 #pragma unroll
     for (int c = 0; c < NO; ++c) out[n * NO + c] = acc[c];
 }
 
 // ---------------------------------------------------------------- pullback
 
-// one workgroup per free control: its threads split the tensor product of
-// the per-axis support lists, then fold in a fixed tree
+// one workgroup per control (lattice index a): its threads split the
+// tensor product of the per-axis support lists, then fold in a fixed tree
+// into out[0 .. NO)
 template <int D, int MODE>
-__global__ __launch_bounds__(kThreads) void ctrl_pullback_kernel(const CtrlGeom g, const double* y, double* out)
+__device__ __forceinline__ void pullback_wg(const CtrlGeom& g, long long a_flat, const double* y, double* out)
 {
     constexpr int NO = MODE == 1 ? D * D : D;
-    const int f = blockIdx.x;
-    long long rem = g.free_idx[f];
+    long long rem = a_flat;
     int a[3] = {0, 0, 0};
     for (int k = D - 1; k >= 0; --k) {
         a[k] = int(rem % g.c[k]);
@@ -444,7 +488,217 @@ __global__ __launch_bounds__(kThreads) void ctrl_pullback_kernel(const CtrlGeom 
             for (int c = 0; c < NO; ++c) sh[c][threadIdx.x] += sh[c][threadIdx.x + w];
         __syncthreads();
     }
-    if ((int)threadIdx.x < NO) out[(long long)f * NO + threadIdx.x] = sh[threadIdx.x][0];
+    if ((int)threadIdx.x < NO) out[threadIdx.x] = sh[threadIdx.x][0];
+}
+
+template <int D, int MODE>
+__global__ __launch_bounds__(kThreads) void ctrl_pullback_kernel(const CtrlGeom g, const double* y, double* out)
+{
+    constexpr int NO = MODE == 1 ? D * D : D;
+    const int f = blockIdx.x;
+    pullback_wg<D, MODE>(g, g.free_idx[f], y, out + (long long)f * NO);
+}
+
+// ---------------------------------------------------------------- several blocks
+
+// the segment s of prefix table pfx[0 .. n] with pfx[s] <= i < pfx[s + 1]
+__device__ __forceinline__ int find_seg(const long long* pfx, int n, long long i)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pfx[mid] <= i)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// positions of every block node from the FP32 store (its own slot)
+template <int D>
+__global__ __launch_bounds__(kThreads) void topo_load_kernel(const TopoView t, const float* x, double* out)
+{
+    const long long n = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (n >= t.n_all) return;
+    const TopoBlock& B = t.blk[find_seg(t.npfx, t.nb, n)];
+    int I[3];
+    unravel_node<D>(B.g, n - B.node0, I);
+    const long long s = node_slot<D>(B.g, I);
+#pragma unroll
+    for (int c = 0; c < D; ++c) out[n * D + c] = (double)x[c * B.g.plane + s];
+}
+
+// every block node takes its owner's net value: state M C + b (nodes whose
+// owner block has no net keep their value), direction M v (0 there); the
+// store receives the state at each node's own slot
+template <int D, bool DIR>
+__global__ __launch_bounds__(kThreads) void topo_eval_kernel(const TopoView t, const double* src, double* out,
+                                                             float* out_plane)
+{
+    const long long n = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (n >= t.n_all) return;
+    const long long o = t.owner[n];
+    const TopoBlock& Bo = t.blk[find_seg(t.npfx, t.nb, o)];
+    double acc[D];
+    if (Bo.net < 0) {
+        if constexpr (DIR) {
+#pragma unroll
+            for (int c = 0; c < D; ++c) acc[c] = 0.0;
+        } else {
+            return;
+        }
+    } else {
+        int I[3];
+        const long long lo = o - Bo.node0;
+        unravel_node<D>(Bo.g, lo, I);
+        eval_at<D, false>(Bo.g, I, src + Bo.ctrl0 * D, acc);
+        if constexpr (!DIR) {
+#pragma unroll
+            for (int c = 0; c < D; ++c) acc[c] += Bo.b ? Bo.b[lo * D + c] : 0.0;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < D; ++c) out[n * D + c] = acc[c];
+    if (out_plane) {
+        const TopoBlock& B = t.blk[find_seg(t.npfx, t.nb, n)];
+        int I[3];
+        unravel_node<D>(B.g, n - B.node0, I);
+        const long long s = node_slot<D>(B.g, I);
+#pragma unroll
+        for (int c = 0; c < D; ++c) out_plane[c * B.g.plane + s] = (float)acc[c];
+    }
+}
+
+template <int D, int KIND>
+__global__ __launch_bounds__(kThreads) void topo_samples_kernel(const TopoView t, const double* xb, double* gs,
+                                                                double* hs)
+{
+    const long long p = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (p >= t.ns_all) return;
+    const TopoBlock& B = t.blk[find_seg(t.spfx, t.nb, p)];
+    sample_derivs<D, KIND>(B.g, p - B.samp0, xb + B.node0 * D, gs, hs, t.ns_all, p);
+}
+
+template <int D>
+__global__ __launch_bounds__(kThreads) void topo_sample_hv_kernel(const TopoView t, const double* hs,
+                                                                  const double* dir, double* zs)
+{
+    const long long p = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (p >= t.ns_all) return;
+    const TopoBlock& B = t.blk[find_seg(t.spfx, t.nb, p)];
+    sample_hv_at<D>(B.g, p - B.samp0, hs, dir + B.node0 * D, zs, t.ns_all, p);
+}
+
+// per block node: an owner gathers the incident samples of itself and of
+// every copy it owns (their blocks' cells); a copy receives 0 (the adjoint
+// of the owner-to-copy broadcast)
+template <int D, int MODE>
+__global__ __launch_bounds__(kThreads) void topo_node_kernel(const TopoView t, const double* gs, const double* hs,
+                                                             double* out)
+{
+    constexpr int NO = MODE == 2 ? D * D : D;
+    const long long n = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (n >= t.n_all) return;
+    double acc[NO] = {};
+    if (t.owner[n] == n) {
+        for (int e = -1; e < t.grp_off[n + 1] - t.grp_off[n]; ++e) {
+            const long long m = e < 0 ? n : (long long)t.grp_mem[t.grp_off[n] + e];
+            const TopoBlock& B = t.blk[find_seg(t.npfx, t.nb, m)];
+            int I[3];
+            unravel_node<D>(B.g, m - B.node0, I);
+            node_acc<D, MODE>(B.g, I, gs, hs, t.ns_all, B.samp0, acc);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < NO; ++c) out[n * NO + c] = acc[c];
+}
+
+// one workgroup per stacked control
+template <int D, int MODE>
+__global__ __launch_bounds__(kThreads) void topo_pullback_kernel(const TopoView t, const double* y, double* out)
+{
+    constexpr int NO = MODE == 1 ? D * D : D;
+    const long long f = blockIdx.x;
+    const TopoBlock& B = t.blk[t.net_blk[find_seg(t.cpfx, t.n_nets, f)]];
+    pullback_wg<D, MODE>(B.g, f - B.ctrl0, y + B.node0 * NO, out + f * NO);
+}
+
+template <int D, int KIND>
+__global__ __launch_bounds__(kThreads) void topo_energy_kernel(const TopoView t, const double* xb, const double* dir,
+                                                               double a0, double a1, double* partials)
+{
+    const long long ns = t.ns_all;
+    const long long per = (ns + gridDim.x - 1) / gridDim.x;
+    const long long lo = (long long)blockIdx.x * per, hi = min(ns, lo + per);
+    double e0 = 0.0, e1 = 0.0, m0 = __builtin_huge_val(), m1 = __builtin_huge_val();
+    for (long long p = lo + threadIdx.x; p < hi; p += kThreads) {
+        const TopoBlock& B = t.blk[find_seg(t.spfx, t.nb, p)];
+        const double* xl = xb + B.node0 * D;
+        const double* dl = dir ? dir + B.node0 * D : nullptr;
+        sample_energy<D, KIND>(B.g, p - B.samp0, xl, dl, a0, e0, m0);
+        sample_energy<D, KIND>(B.g, p - B.samp0, xl, dl, a1, e1, m1);
+    }
+    energy_fold(e0, m0, e1, m1, partials);
+}
+
+// y = A x (+ lam z) over n_rows CSR rows
+__global__ __launch_bounds__(kThreads) void csr_apply_kernel(long long n_rows, const int* off, const int* col,
+                                                             const double* coef, const double* x, double lam,
+                                                             const double* z, double* y)
+{
+    const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n_rows) return;
+    double acc = 0.0;
+    for (int e = off[i]; e < off[i + 1]; ++e) acc += coef[e] * x[col[e]];
+    if (z) acc += lam * z[i];
+    y[i] = acc;
+}
+
+// y_i += sum over the transpose row i of coef * w_j * r_j
+__global__ __launch_bounds__(kThreads) void csrT_accum_kernel(long long n_rows, const int* off, const int* row,
+                                                              const double* coef, const double* w, const double* r,
+                                                              double* y)
+{
+    const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n_rows) return;
+    double acc = 0.0;
+    for (int e = off[i]; e < off[i + 1]; ++e) acc += coef[e] * w[row[e]] * r[row[e]];
+    y[i] += acc;
+}
+
+// diagonal of R^T P R for block-diagonal P (one D x D block per control):
+// the entries of column c (rows ascending) grouped by control, each group's
+// coefficient vector v adding v^T P_control v
+template <int D>
+__global__ __launch_bounds__(kThreads) void qdiag_build_kernel(long long nq, const int* g_off, const int* g_row,
+                                                               const double* g_coef, const double* P, double* diag)
+{
+    const long long c = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (c >= nq) return;
+    double d = 0.0;
+    int e = g_off[c];
+    const int end = g_off[c + 1];
+    while (e < end) {
+        const int ctrl = g_row[e] / D;
+        double v[D] = {};
+        for (; e < end && g_row[e] / D == ctrl; ++e) v[g_row[e] - ctrl * D] = g_coef[e];
+        const double* Pc = P + (long long)ctrl * D * D;
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+#pragma unroll
+            for (int j = 0; j < D; ++j) d += v[i] * Pc[i * D + j] * v[j];
+    }
+    diag[c] = d;
+}
+
+__global__ __launch_bounds__(kThreads) void qdiag_apply_kernel(long long nq, const double* diag, double lam,
+                                                               const double* r, double* z)
+{
+    const long long c = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (c >= nq) return;
+    const double d = diag[c] + lam;
+    z[c] = d > 0.0 ? r[c] / d : r[c];
 }
 
 // ---------------------------------------------------------------- vectors
@@ -673,6 +927,143 @@ void launch_ctrl_scatter_free(const CtrlGeom& g, double a, const double* v, doub
 {
     if (g.n_free > 0)
         ctrl_scatter_free_kernel<<<blocks_for(g.n_free), kThreads, 0, s>>>(g.n_free, g.dim, g.free_idx, a, v, C);
+}
+
+// ---------------------------------------------------------------- several blocks
+
+void launch_topo_load(const TopoView& t, const float* x, double* out, hipStream_t s)
+{
+    if (t.n_all <= 0) return;
+    if (t.dim == 2)
+        topo_load_kernel<2><<<blocks_for(t.n_all), kThreads, 0, s>>>(t, x, out);
+    else
+        topo_load_kernel<3><<<blocks_for(t.n_all), kThreads, 0, s>>>(t, x, out);
+}
+
+void launch_topo_eval(const TopoView& t, bool dir, const double* src, double* out, float* out_plane, hipStream_t s)
+{
+    if (t.n_all <= 0) return;
+    const unsigned nb = blocks_for(t.n_all);
+    if (t.dim == 2) {
+        if (dir)
+            topo_eval_kernel<2, true><<<nb, kThreads, 0, s>>>(t, src, out, out_plane);
+        else
+            topo_eval_kernel<2, false><<<nb, kThreads, 0, s>>>(t, src, out, out_plane);
+    } else {
+        if (dir)
+            topo_eval_kernel<3, true><<<nb, kThreads, 0, s>>>(t, src, out, out_plane);
+        else
+            topo_eval_kernel<3, false><<<nb, kThreads, 0, s>>>(t, src, out, out_plane);
+    }
+}
+
+void launch_topo_samples(const TopoView& t, int kind, const double* xb, double* gs, double* hs, hipStream_t s)
+{
+    if (t.ns_all <= 0) return;
+    const unsigned nb = blocks_for(t.ns_all);
+#define EGGMI_TS(D, K) topo_samples_kernel<D, K><<<nb, kThreads, 0, s>>>(t, xb, gs, hs)
+    if (t.dim == 2) {
+        if (kind == kShapeSize)
+            EGGMI_TS(2, kShapeSize);
+        else
+            EGGMI_TS(2, kShape);
+    } else {
+        if (kind == kShapeSize)
+            EGGMI_TS(3, kShapeSize);
+        else
+            EGGMI_TS(3, kShape);
+    }
+#undef EGGMI_TS
+}
+
+void launch_topo_sample_hv(const TopoView& t, const double* hs, const double* dir, double* zs, hipStream_t s)
+{
+    if (t.ns_all <= 0) return;
+    if (t.dim == 2)
+        topo_sample_hv_kernel<2><<<blocks_for(t.ns_all), kThreads, 0, s>>>(t, hs, dir, zs);
+    else
+        topo_sample_hv_kernel<3><<<blocks_for(t.ns_all), kThreads, 0, s>>>(t, hs, dir, zs);
+}
+
+void launch_topo_node(const TopoView& t, int mode, const double* gs, const double* hs, double* out, hipStream_t s)
+{
+    if (t.n_all <= 0) return;
+    const unsigned nb = blocks_for(t.n_all);
+    if (t.dim == 2) {
+        if (mode == 0)
+            topo_node_kernel<2, 0><<<nb, kThreads, 0, s>>>(t, gs, hs, out);
+        else
+            topo_node_kernel<2, 2><<<nb, kThreads, 0, s>>>(t, gs, hs, out);
+    } else {
+        if (mode == 0)
+            topo_node_kernel<3, 0><<<nb, kThreads, 0, s>>>(t, gs, hs, out);
+        else
+            topo_node_kernel<3, 2><<<nb, kThreads, 0, s>>>(t, gs, hs, out);
+    }
+}
+
+void launch_topo_pullback(const TopoView& t, int mode, const double* y, double* out, hipStream_t s)
+{
+    if (t.n_stack <= 0) return;
+    const unsigned nb = (unsigned)t.n_stack;  // one workgroup per stacked control
+    if (t.dim == 2) {
+        if (mode == 0)
+            topo_pullback_kernel<2, 0><<<nb, kThreads, 0, s>>>(t, y, out);
+        else
+            topo_pullback_kernel<2, 1><<<nb, kThreads, 0, s>>>(t, y, out);
+    } else {
+        if (mode == 0)
+            topo_pullback_kernel<3, 0><<<nb, kThreads, 0, s>>>(t, y, out);
+        else
+            topo_pullback_kernel<3, 1><<<nb, kThreads, 0, s>>>(t, y, out);
+    }
+}
+
+void launch_topo_energy(const TopoView& t, int kind, const double* xb, const double* dir, double a0, double a1,
+                        double* partials, double* out, hipStream_t s)
+{
+    const int nb = (int)std::max<long long>(1, std::min<long long>(kCtrlParts, (t.ns_all + kThreads - 1) / kThreads));
+#define EGGMI_TE(D, K) topo_energy_kernel<D, K><<<nb, kThreads, 0, s>>>(t, xb, dir, a0, a1, partials)
+    if (t.dim == 2) {
+        if (kind == kShapeSize)
+            EGGMI_TE(2, kShapeSize);
+        else
+            EGGMI_TE(2, kShape);
+    } else {
+        if (kind == kShapeSize)
+            EGGMI_TE(3, kShapeSize);
+        else
+            EGGMI_TE(3, kShape);
+    }
+#undef EGGMI_TE
+    ctrl_energy_finish<<<1, kThreads, 0, s>>>(partials, nb, out);
+}
+
+void launch_csr_apply(long long n_rows, const int* off, const int* col, const double* coef, const double* x,
+                      double lam, const double* z, double* y, hipStream_t s)
+{
+    if (n_rows > 0) csr_apply_kernel<<<blocks_for(n_rows), kThreads, 0, s>>>(n_rows, off, col, coef, x, lam, z, y);
+}
+
+void launch_csrT_accum(long long n_rows, const int* off, const int* row, const double* coef, const double* w,
+                       const double* r, double* y, hipStream_t s)
+{
+    if (n_rows > 0) csrT_accum_kernel<<<blocks_for(n_rows), kThreads, 0, s>>>(n_rows, off, row, coef, w, r, y);
+}
+
+void launch_qdiag_build(int dim, long long nq, const int* g_off, const int* g_row, const double* g_coef,
+                        const double* P, double* diag, hipStream_t s)
+{
+    if (nq <= 0) return;
+    if (dim == 2)
+        qdiag_build_kernel<2><<<blocks_for(nq), kThreads, 0, s>>>(nq, g_off, g_row, g_coef, P, diag);
+    else
+        qdiag_build_kernel<3><<<blocks_for(nq), kThreads, 0, s>>>(nq, g_off, g_row, g_coef, P, diag);
+}
+
+void launch_qdiag_apply(long long nq, const double* diag, double lam, const double* r, double* z, hipStream_t s)
+{
+    if (nq > 0) qdiag_apply_kernel<<<blocks_for(nq), kThreads, 0, s>>>(nq, diag, lam, r, z);
 }
 
 }  // namespace eggmi

@@ -49,6 +49,69 @@ struct CtrlGeom {
     const int* l2f = nullptr;       // [n_ctrl] lattice -> free slot, -1 = fixed
 };
 
+// Several blocks under one stacked control vector (reference
+// ControlTopoSessionT, src/control_solve.hpp:1585-2837): every block of the
+// session is a domain of cells; some carry a net. Block nodes are numbered
+// in one concatenated space (block by block, row-major within a block), so
+// are the cell samples and the nets' controls (the stacked vector).
+struct TopoBlock {
+    CtrlGeom g;             // block geometry; the net's tables when net >= 0 (all its controls free)
+    long long node0 = 0;    // first node in the concatenated node space
+    long long samp0 = 0;    // first sample in the concatenated sample space
+    long long ctrl0 = 0;    // first control of the net in the stacked vector
+    const double* b = nullptr;  // the net's offset (block node order), null = 0
+    int net = -1;
+};
+struct TopoView {
+    const TopoBlock* blk = nullptr;
+    int nb = 0;
+    const long long* npfx = nullptr;  // [nb + 1] node prefix over blocks
+    const long long* spfx = nullptr;  // [nb + 1] sample prefix over blocks
+    int n_nets = 0;
+    const int* net_blk = nullptr;     // [n_nets] block of each net
+    const long long* cpfx = nullptr;  // [n_nets + 1] stacked control prefix
+    // every node of a global point shared by several block nodes (block
+    // faces, periodic seams) names one owner, the node whose store slot is
+    // the point's root; owner[n] = n for owners and unshared nodes
+    const int* owner = nullptr;       // [n_all]
+    const int* grp_off = nullptr;     // [n_all + 1] the copies an owner owns
+    const int* grp_mem = nullptr;
+    long long n_all = 0, ns_all = 0, n_stack = 0;
+    int dim = 2;
+};
+
+// FP64 positions of every block node from the FP32 store (its own slot).
+void launch_topo_load(const TopoView& t, const float* x, double* out, hipStream_t s);
+// Every block node takes its owner's net value: state (src = stacked C;
+// nodes whose owner block has no net keep their value) or direction (src =
+// a stacked direction; 0 where the owner has no net); the store receives
+// the state at each node's own slot when out_plane is given.
+void launch_topo_eval(const TopoView& t, bool dir, const double* src, double* out, float* out_plane, hipStream_t s);
+// The per-sample, node and pullback stages of the single-block path over
+// the concatenated spaces: samples (gs, hs component-major over ns_all),
+// Hessian action, owner gathers (mode 0 with gs: D per node; mode 2 with
+// hs: D x D; copies receive 0) and the per-control pullback into the
+// stacked vector (mode 0: D, mode 1: the Jacobi blocks D x D).
+void launch_topo_samples(const TopoView& t, int kind, const double* xb, double* gs, double* hs, hipStream_t s);
+void launch_topo_sample_hv(const TopoView& t, const double* hs, const double* dir, double* zs, hipStream_t s);
+void launch_topo_node(const TopoView& t, int mode, const double* gs, const double* hs, double* out, hipStream_t s);
+void launch_topo_pullback(const TopoView& t, int mode, const double* y, double* out, hipStream_t s);
+// Energy and min det over every block's cells at xb + a0 dir and xb + a1 dir
+// (as launch_ctrl_energy).
+void launch_topo_energy(const TopoView& t, int kind, const double* xb, const double* dir, double a0, double a1,
+                        double* partials, double* out, hipStream_t s);
+// y = A x + lam z (z null: no term) over n_rows CSR rows.
+void launch_csr_apply(long long n_rows, const int* off, const int* col, const double* coef, const double* x,
+                      double lam, const double* z, double* y, hipStream_t s);
+// y_i += sum_e coef_e w_{row_e} r_{row_e} over transpose row i.
+void launch_csrT_accum(long long n_rows, const int* off, const int* row, const double* coef, const double* w,
+                       const double* r, double* y, hipStream_t s);
+// diag_c = sum over the controls in column c of R (transpose CSR, rows
+// ascending) of v^T P_control v; z = r / (diag + lam) (r where not > 0).
+void launch_qdiag_build(int dim, long long nq, const int* g_off, const int* g_row, const double* g_coef,
+                        const double* P, double* diag, hipStream_t s);
+void launch_qdiag_apply(long long nq, const double* diag, double lam, const double* r, double* z, hipStream_t s);
+
 // Evaluation of the map over the block's nodes into block-local FP64 node
 // fields (dim per node, node order row-major) and, when given, into the FP32
 // store at the block's node slots:

@@ -14,6 +14,41 @@
 namespace eggmi
 {
 
+void upload_net_axes(int dim, const CtrlGeom& g, const int32_t* const* off_in, const double* const* wt_in,
+                     DArr<int>* d_off, DArr<double>* d_wt, DArr<int>* d_soff, DArr<int>* d_snode, DArr<double>* d_sw)
+{
+    for (int k = 0; k < dim; ++k) {
+        if (!off_in[k] || !wt_in[k]) throw InvalidArgument("control: off / wt of every axis are required");
+        std::vector<int> off(off_in[k], off_in[k] + g.n[k]);
+        std::vector<double> wt(wt_in[k], wt_in[k] + (size_t)g.n[k] * g.ks);
+        std::vector<std::vector<std::pair<int, double>>> lists(g.c[k]);
+        for (int i = 0; i < g.n[k]; ++i) {
+            if (off[i] < 0 || off[i] + g.ks > g.c[k])
+                throw InvalidArgument("control: axis " + std::to_string(k) + " window of node " + std::to_string(i) +
+                                      " leaves the lattice");
+            for (int t = 0; t < g.ks; ++t) {
+                const double w = wt[(size_t)i * g.ks + t];
+                if (!std::isfinite(w)) throw InvalidArgument("control: non-finite basis weight");
+                if (w != 0.0) lists[off[i] + t].emplace_back(i, w);
+            }
+        }
+        std::vector<int> soff(g.c[k] + 1, 0), snode;
+        std::vector<double> sw;
+        for (int a = 0; a < g.c[k]; ++a) {
+            for (auto& [i, w] : lists[a]) {
+                snode.push_back(i);
+                sw.push_back(w);
+            }
+            soff[a + 1] = (int)snode.size();
+        }
+        d_off[k].upload(off);
+        d_wt[k].upload(wt);
+        d_soff[k].upload(soff);
+        d_snode[k].upload(snode);
+        d_sw[k].upload(sw);
+    }
+}
+
 void Session::set_control(const eggmi_control& c)
 {
     need_device();
@@ -41,37 +76,7 @@ void Session::set_control(const eggmi_control& c)
     g.bd = fine.block_desc((size_t)b);
     g.plane = fine.plane;
     net->block = b;
-    // per-axis windows and their inverse (control -> nodes, weights)
-    for (int k = 0; k < dim; ++k) {
-        if (!c.off[k] || !c.wt[k]) throw InvalidArgument("control: off / wt of every axis are required");
-        std::vector<int> off(c.off[k], c.off[k] + g.n[k]);
-        std::vector<double> wt(c.wt[k], c.wt[k] + (size_t)g.n[k] * g.ks);
-        std::vector<std::vector<std::pair<int, double>>> lists(g.c[k]);
-        for (int i = 0; i < g.n[k]; ++i) {
-            if (off[i] < 0 || off[i] + g.ks > g.c[k])
-                throw InvalidArgument("control: axis " + std::to_string(k) + " window of node " + std::to_string(i) +
-                                      " leaves the lattice");
-            for (int t = 0; t < g.ks; ++t) {
-                const double w = wt[(size_t)i * g.ks + t];
-                if (!std::isfinite(w)) throw InvalidArgument("control: non-finite basis weight");
-                if (w != 0.0) lists[off[i] + t].emplace_back(i, w);
-            }
-        }
-        std::vector<int> soff(g.c[k] + 1, 0), snode;
-        std::vector<double> sw;
-        for (int a = 0; a < g.c[k]; ++a) {
-            for (auto& [i, w] : lists[a]) {
-                snode.push_back(i);
-                sw.push_back(w);
-            }
-            soff[a + 1] = (int)snode.size();
-        }
-        net->off[k].upload(off);
-        net->wt[k].upload(wt);
-        net->sup_off[k].upload(soff);
-        net->sup_node[k].upload(snode);
-        net->sup_w[k].upload(sw);
-    }
+    upload_net_axes(dim, g, c.off, c.wt, net->off, net->wt, net->sup_off, net->sup_node, net->sup_w);
     std::vector<int> fidx(c.free_idx, c.free_idx + g.n_free), l2f(g.n_ctrl, -1);
     for (int f = 0; f < g.n_free; ++f) {
         if (fidx[f] < 0 || fidx[f] >= g.n_ctrl) throw InvalidArgument("control: free_idx out of range");
@@ -124,6 +129,7 @@ void Session::set_control(const eggmi_control& c)
             net->ocell_off.upload(off);
         }
     }
+    control_topo.reset();
     control = std::move(net);
     control_eval_to_x();
 }
@@ -140,6 +146,16 @@ void Session::control_eval_to_x()
 
 void Session::control_set_C(const double* C, long long n)
 {
+    if (control_topo) {
+        const long long want = control_topo->n_stack * dim;
+        if (n != want)
+            throw InvalidArgument("set_C: expected " + std::to_string(want) + " values, got " + std::to_string(n));
+        for (long long e = 0; e < n; ++e)
+            if (!std::isfinite(C[e])) throw InvalidArgument("set_C: C is not finite");
+        EGGMI_HIP(hipMemcpy(control_topo->C.p, C, n * sizeof(double), hipMemcpyHostToDevice));
+        control_topo_eval_to_x();
+        return;
+    }
     if (!control) throw InvalidArgument("set_C: the session has no control wire");
     if (n != control->g.n_ctrl * dim)
         throw InvalidArgument("set_C: expected " + std::to_string(control->g.n_ctrl * dim) + " values, got " +
@@ -150,6 +166,13 @@ void Session::control_set_C(const double* C, long long n)
 
 void Session::control_get_C(double* C, long long n) const
 {
+    if (control_topo) {
+        const long long want = control_topo->n_stack * dim;
+        if (n != want)
+            throw InvalidArgument("get_C: expected " + std::to_string(want) + " values, got " + std::to_string(n));
+        EGGMI_HIP(hipMemcpy(C, control_topo->C.p, n * sizeof(double), hipMemcpyDeviceToHost));
+        return;
+    }
     if (!control) throw InvalidArgument("get_C: the session has no control wire");
     if (n != control->g.n_ctrl * dim)
         throw InvalidArgument("get_C: expected " + std::to_string(control->g.n_ctrl * dim) + " values, got " +
@@ -160,6 +183,7 @@ void Session::control_get_C(double* C, long long n) const
 ControlReport Session::run_control(const ControlOptions& o, int kind)
 {
     need_device();
+    if (control_topo) return run_control_topo(o, kind);
     if (!control) throw InvalidArgument("run_control: the session has no control wire");
     if (kind == kUntangle) throw InvalidArgument("run_control: the untangle phase stays nodal (barrier phases only)");
     if (!fine.structured_energy)

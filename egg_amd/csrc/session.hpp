@@ -297,6 +297,41 @@ struct ControlNet {
     long long n_ocells = 0;
 };
 
+// A net's per-axis windows (checked to stay inside the lattice, finite)
+// and their inverse (control -> nodes with weights) onto the device; g
+// holds the node and lattice shapes and the window width.
+void upload_net_axes(int dim, const CtrlGeom& g, const int32_t* const* off, const double* const* wt, DArr<int>* d_off,
+                     DArr<double>* d_wt, DArr<int>* d_soff, DArr<int>* d_snode, DArr<double>* d_sw);
+
+// Several blocks' nets over one stacked control vector with a reduction
+// dC = R dq to the reduced DOFs and homogeneous penalty rows
+// (control_topo.cpp; reference ControlTopoSessionT,
+// src/control_solve.hpp:1585-2837).
+struct ControlTopo {
+    int n_nets = 0;
+    std::vector<int> net_block;
+    std::vector<long long> net_nodes;  // node count of each net's block
+    std::vector<TopoBlock> hblk;
+    std::vector<std::array<DArr<int>, 3>> off, sup_off, sup_node;
+    std::vector<std::array<DArr<double>, 3>> wt, sup_w;
+    std::vector<DArr<double>> b;
+    DArr<TopoBlock> blk;
+    DArr<long long> npfx, spfx, cpfx;
+    DArr<int> net_blk, owner, grp_off, grp_mem;
+    TopoView view;
+    long long n_all = 0, ns_all = 0, n_stack = 0, nq = 0, n_pen = 0;
+    // R (n_stack * dim rows, nq columns) and its transpose (rows ascending
+    // within a column); penalty rows P over the stacked components, their
+    // weights and transpose
+    DArr<int> r_off, r_col, g_off, g_row, p_off, p_col, pt_off, pt_row;
+    DArr<double> r_coef, g_coef, p_coef, p_w, pt_coef, r_pen;
+    std::vector<double> p_w_h;
+    // stacked lattice; node fields over every block node; per-sample
+    // derivatives; stacked vectors; reduced-space PCG vectors; scalars
+    DArr<double> C, xb, dir, ynode, hnode, gs, hs, zs, prec, vfull, wfull, dCfull;
+    DArr<double> qG, qdq, qr, qz, qp, qAp, qdiag, scal, eparts;
+};
+
 struct Session {
     int dim = 2;
     long long M = 0;  // global node count
@@ -371,6 +406,16 @@ struct Session {
     void control_get_C(double* C, long long n) const;
     void control_eval_to_x();
     ControlReport run_control(const ControlOptions& o, int kind);
+    // several blocks' nets (control_topo.cpp); set_C / get_C / run_control
+    // act on the stacked vector while one is attached
+    std::unique_ptr<ControlTopo> control_topo;
+    void set_control_topo(const eggmi_control_topo& t);
+    void control_topo_set_reduction(long long nq, const int* off, const int* col, const double* coef);
+    void control_topo_set_penalty(long long n_pen, const int* off, const int* col, const double* coef,
+                                  const double* w);
+    void control_topo_set_b(int net, const double* b, long long n);
+    void control_topo_eval_to_x();
+    ControlReport run_control_topo(const ControlOptions& o, int kind);
 
     Session() = default;
     ~Session();

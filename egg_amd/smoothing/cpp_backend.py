@@ -113,13 +113,31 @@ class CppStructuredSweepSession:
                                          model_db=model_db)
 
     def get_C(self) -> np.ndarray:
-        """The full control lattice, shape ``(n_ctrl, d)``."""
+        """The full control lattice, shape ``(n_ctrl, d)`` (the nets' stacked
+        lattices under a multi-block wire)."""
         return self._session.get_C().reshape(-1, self._d)
 
     def set_C(self, C: np.ndarray) -> None:
         """Upload a full control lattice ``(n_ctrl, d)`` and re-evaluate the
         net (+ b) into the resident state."""
         self._session.set_C(np.ascontiguousarray(C, np.float64).ravel())
+
+    def set_control_reduction(self, nq: int, off, col, coef) -> None:
+        """Replace the multi-block reduction dC = R dq (CSR over the stacked
+        components, ``nq`` columns; reference bindings.cpp:1475-1494)."""
+        self._session.set_control_reduction(int(nq), np.ascontiguousarray(off, np.int32),
+                                            np.ascontiguousarray(col, np.int32), np.ascontiguousarray(coef, np.float64))
+
+    def set_control_penalty(self, off, col, coef, w) -> None:
+        """Replace the multi-block penalty rows, energy 1/2 sum w (P C)^2
+        (reference bindings.cpp:1496-1516); empty ``off`` removes them."""
+        self._session.set_control_penalty(np.ascontiguousarray(off, np.int32), np.ascontiguousarray(col, np.int32),
+                                          np.ascontiguousarray(coef, np.float64), np.ascontiguousarray(w, np.float64))
+
+    def set_control_b(self, block_index: int, b) -> None:
+        """Replace net ``block_index``'s Boolean-sum offset (its block's node
+        order, d values per node) and re-evaluate the nets into the state."""
+        self._session.set_control_b(int(block_index), np.ascontiguousarray(b, np.float64).ravel())
 
     def set_proposal(self, mlp, *, mode: str = "replace", scale: float = 1.0, every: int = 1) -> None:
         """Use a trained ``cpp_core.ProposalMLP`` (or None) as the learned

@@ -489,6 +489,63 @@ eggmi_status eggmi_session_set_control(eggmi_session* s, const eggmi_control* c)
 eggmi_status eggmi_control_set_C(eggmi_session* s, const double* C, int64_t n);
 eggmi_status eggmi_control_get_C(eggmi_session* s, double* C, int64_t n);
 
+/* ---- control-net reduced Gauss-Newton over several blocks */
+
+/* Several blocks' nets over one stacked control vector (the reference's
+ * multi-block control wire, selected by its "blocks" key and unpacked at
+ * src/bindings.cpp:593-740 into ControlTopoHostT; session
+ * ControlTopoSessionT, src/control_solve.hpp:1585-2837). nets[i] gives
+ * net i's block, ctrl_shape, k_supp, off, wt and b with the meaning they have
+ * for one block; its C0, n_free and free_idx are not read (a net's freedom
+ * lives in R); at
+ * most one net per block. C0: the nets' lattices stacked in net order
+ * (n_stack * dim doubles, n_stack the total control count). The
+ * reduction: dC = R dq with R an (n_stack * dim) x nq CSR over the stacked
+ * components (row r = control * dim + component; r_off has n_stack * dim
+ * + 1 entries); rows without entries keep their component at C0. Optional
+ * homogeneous penalty rows P (n_pen x n_stack * dim CSR, p_off n_pen + 1
+ * entries, weights p_w): energy 1/2 sum_j p_w[j] (P_j C)^2, composed into
+ * the Gauss-Newton system and the line-search energy. Every block node
+ * takes the value of its global point's owner (the node at the point's
+ * root slot); blocks without a net keep their nodes, except where they
+ * copy a net block's owner. Sliding-wall frames (the wire's slide,
+ * wall_faces, seam_face) have no counterpart here. */
+typedef struct {
+    int32_t n_nets;
+    const eggmi_control* nets;
+    const double* C0;
+    int64_t nq;
+    const int32_t* r_off;
+    const int32_t* r_col;
+    const double* r_coef;
+    int64_t n_pen;
+    const int32_t* p_off;
+    const int32_t* p_col;
+    const double* p_coef;
+    const double* p_w;
+} eggmi_control_topo;
+
+/* Attach the nets (replacing any control net of the session) and evaluate
+ * them into the resident state with the owner-to-copy refresh
+ * (ControlTopoSessionT ctor + eval_to_x, src/control_solve.hpp:1590-2055,
+ * :2187-2191). eggmi_control_set_C / get_C / eggmi_run_control then act on
+ * the stacked vector (n_stack * dim doubles). */
+eggmi_status eggmi_session_set_control_topo(eggmi_session* s, const eggmi_control_topo* t);
+/* Replace the reduction (set_reduction, src/control_solve.hpp:2057-2117;
+ * the reference binding set_control_reduction, src/bindings.cpp:1475-1494):
+ * same row space, any column count nq. */
+eggmi_status eggmi_control_set_reduction(eggmi_session* s, int64_t nq, const int32_t* r_off, const int32_t* r_col,
+                                         const double* r_coef);
+/* Replace the penalty rows (set_penalty, src/control_solve.hpp:2120-2169;
+ * binding set_control_penalty, src/bindings.cpp:1496-1516); n_pen = 0
+ * removes them. */
+eggmi_status eggmi_control_set_penalty(eggmi_session* s, int64_t n_pen, const int32_t* p_off, const int32_t* p_col,
+                                       const double* p_coef, const double* p_w);
+/* Replace net `net`'s offset b (n = its block's node count * dim doubles)
+ * and re-evaluate the nets into the state (set_b, src/control_solve.hpp:
+ * 2172-2184; binding set_control_b, src/bindings.cpp:1518-1531). */
+eggmi_status eggmi_control_set_b(eggmi_session* s, int32_t net, const double* b, int64_t n);
+
 /* Options of the reduced Gauss-Newton loop (ControlOptions,
  * src/control_solve.hpp:308-329). */
 typedef struct {
@@ -503,7 +560,11 @@ typedef struct {
 } eggmi_control_options;
 
 /* Run the loop in place on the session's state (ControlSessionT::run_plain,
- * src/control_solve.hpp:533-676): reduced gradient, Jacobi-preconditioned
+ * src/control_solve.hpp:533-676; with several blocks' nets attached
+ * ControlTopoSessionT::run, :2209-2425, over the reduced DOFs q: gradient
+ * R^T (M^T g + P^T W P C), the same PCG on R^T (A + P^T W P) R + lam I
+ * preconditioned by the diagonal of R^T Pj R with Pj the per-control Jacobi
+ * blocks, the penalty energy added in the line search): reduced gradient, Jacobi-preconditioned
  * matrix-free FP64 PCG on (A + lam I) dC = -G, line search of M dC on the
  * global fine energy (FP64 reductions; accept iff finite, E <= E0 + 1e-6 and
  * min det > 0), damping x10 (floored at 1e-8 trace(P) / n) on rejection,
