@@ -1,7 +1,7 @@
 // Host driver of the control-net reduced Gauss-Newton smoother over several
 // blocks' nets (reference ControlTopoSessionT, src/control_solve.hpp:
 // 1585-2837: set-up :1590-2055, set_reduction / set_penalty / set_b
-// :2057-2184, run :2209-2425, matvec_q and pcg_solve_q :2544-2640). The
+// :2057-2184, run :2209-2425, matvec_q and pcg_solve_q :2529-2606). The
 // nets share one stacked control vector C whose step is dC = R dq over the
 // reduced DOFs q; homogeneous penalty rows 1/2 sum w (P C)^2 join the
 // Gauss-Newton system and the line-search energy. Every block of the session

@@ -161,7 +161,7 @@ def boolean_sum_offset(cmap: ControlMap, C: np.ndarray, X0: np.ndarray, keep: np
     block boundary; b depends on the boundary controls only. ``keep``
     (node-shaped bool): the boundary nodes whose mismatch is kept, the
     others' is taken as 0 (seam faces of a multi-block topology, whose
-    nodes stay pure spline, reference control_solve.hpp:2466-2476)."""
+    nodes stay pure spline, reference control_topology.py:1386-1420)."""
     X0 = np.asarray(X0, np.float64)
     delta = X0 - cmap.prolong(C)
     bnd = boundary_ctrl_mask(cmap.node_shape)
