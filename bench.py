@@ -1236,6 +1236,89 @@ def bench_control(args):
     }
 
 
+def bench_control_topo(args):
+    """The multi-block control-net smoother (run_control under a "blocks"
+    wire): the 2D O-grid of 129 x 129 + 4 x 129 x 65 nodes (untangled
+    generator state) with 24 x 24 and 24 x 12 cubic nets whose seam controls
+    are shared (build_control_topology), and the 7 x 33^3 cubed sphere with
+    9^3 nets; wall time to convergence (max |G| < 1e-8) with the default
+    inexact-Newton forcing. CPU baseline and FP64 parity: the dense oracle
+    loop (oracle/control.py run_control_topo) on the 33 x 33 + 4 x 33 x 17
+    O-grid with 8 x 8 / 8 x 5 nets, relaxed by 300 oracle Jacobi sweeps
+    (where the reduced Hessian is positive definite), the device on the
+    same problem with a tight PCG."""
+    from egg_amd import grids
+    from egg_amd.smoothing import (CppStructuredSweepSession, build_structured_context_from_block_maps,
+                                   build_sweep_context)
+    from egg_amd.smoothing.control import build_control_topology, topology_mismatch
+
+    def case(g, shapes):
+        ctx = build_sweep_context(g.blocks, g.free, g.d)
+        wire, _ = build_control_topology(g.blocks, g.X, shapes)
+        s = CppStructuredSweepSession(ctx, build_structured_context_from_block_maps(g.d, g.blocks), g.X,
+                                      control=wire)
+        return ctx, wire, s
+
+    runs = {}
+    for name, g, shapes in (("2d_ogrid129_net24", grids.ogrid_2d(129, 65), [(24, 24)] + [(24, 12)] * 4),
+                            ("3d_cubed33_net9", grids.cubed_sphere_3d(33), [(9, 9, 9)] * 7)):
+        ctx, wire, s = case(g, shapes)
+        s.run_control(1)  # warm the kernels
+        s.set_C(np.asarray(wire["C0"]).reshape(-1, g.d))
+        t0 = time.perf_counter()
+        r = s.run_control(30)
+        t = time.perf_counter() - t0
+        C = s.get_C().reshape(-1)
+        runs[name] = {"nets": len(wire["blocks"]), "stacked_controls": int(len(C) // g.d), "reduced_dofs": int(wire["nq"]),
+                      "nodes": int(g.n_nodes), "s": t, "iters": int(r["iters"]), "converged": bool(r["converged"]),
+                      "pcg_iterations": [int(v) for v in r["pcg_iters"]],
+                      "energy_first_last": [float(r["energies"][0]), float(r["energies"][-1])],
+                      "final_min_det": float(r["mindets"][-1]),
+                      "watertight_mismatch": topology_mismatch(wire, g.blocks, C)}
+        log(f"control_topo {name}: {runs[name]}")
+    cpu = parity = None
+    if not args.no_cpu:
+        from oracle import control as OC  # the checker: CPU baseline and parity reference
+        from oracle import oracle as O
+
+        g = grids.ogrid_2d(33, 17)
+        ctx = build_sweep_context(g.blocks, g.free, 2)
+        pats = O.materialize_patches(ctx.wire, g.blocks, 2)
+        X, _, _ = O.jacobi(2, "barrier", 0.0, 0.8, 300, g.X, pats, ctx.energy_stencil, report_every=300, threads=8)
+        g = grids.Grid(2, g.blocks, X.astype(np.float32).astype(np.float64), g.free)
+        ctx, wire, s = case(g, [(8, 8)] + [(8, 5)] * 4)
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            Xo, Co, ro = OC.run_control_topo(2, "barrier", g.X, ctx.energy_stencil, wire, g.blocks, n_outer=30)
+            times.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        rd = s.run_control(30, pcg_max_iter=2000, pcg_rtol=1e-12, pcg_forcing=False)
+        t_dev = time.perf_counter() - t0
+        cpu = {"value": float(np.median(times)), "unit": "s to converged", "cores": 1, "kind": "port",
+               "sample": f"33 x 33 + 4 x 33 x 17 O-grid, 8 x 8 / 8 x 5 nets ({wire['nq']} reduced DOFs), dense FP64 "
+                         f"oracle loop ({ro['iters']} iterations), median of 3; the device on it: {t_dev:.4f} s"}
+        eo, ed = np.asarray(ro["energies"]), np.asarray(rd["energies"])
+        n = min(len(eo), len(ed))
+        parity = {"iters_oracle_device": [int(ro["iters"]), int(rd["iters"])],
+                  "max_rel_energy_err": float(np.max(np.abs(ed[:n] - eo[:n]) / np.abs(eo[:n]))),
+                  "max_abs_pos_err": float(np.abs(s.get_X() - Xo).max()),
+                  "max_abs_ctrl_err": float(np.abs(s.get_C().reshape(-1) - Co).max()),
+                  "oracle": "dense FP64 multi-block reduced Gauss-Newton (oracle/control.py run_control_topo)"}
+        log(f"cpu {cpu}\nparity {parity}")
+    r2 = runs["2d_ogrid129_net24"]
+    return {
+        "metric": "multi-block control-net reduced Gauss-Newton: wall time to converged", "value": r2["s"],
+        "unit": "s", "n_gpus": 1, "steps": r2["iters"], "warmup": 1,
+        "ms_per_step": r2["s"] / max(r2["iters"], 1) * 1e3, "higher_is_better": False, "scaling": "strong",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "2D O-grid 129x129 + 4x129x65 (untangled generator state), 24x24 / 24x12 cubic "
+                               "nets with shared seam controls, run_control(30) with inexact-Newton forcing; and "
+                               "the 7x33^3 cubed sphere with 9^3 nets"},
+        "runs": runs, "cpu_baseline": cpu, "parity": parity, "roofline": None,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -1253,7 +1336,7 @@ def main() -> int:
                     metavar="K,TY,TX,KERNEL",
                     help="(--workload ogrid) time the fine level's sweeps with this multi-sweep setting")
     ap.add_argument("--workload", default="ogrid", choices=("ogrid", "ogrid_bl", "ogrid_terms", "batched", "cubed_sphere",
-                                                              "mlp", "control", "fas"))
+                                                              "mlp", "control", "control_topo", "fas"))
     ap.add_argument("--graph-chunk", type=int, default=GRAPH_CHUNK,
                     help="sweeps per captured hipGraph replay (even; 0 = single launches, e.g. under a tracer)")
     args = ap.parse_args()
@@ -1272,7 +1355,8 @@ def main() -> int:
             args.live_traffic = (None, f"{type(exc).__name__}: {exc}")
         log(f"live traffic: {args.live_traffic}")
     fn = {"ogrid": bench_ogrid, "ogrid_bl": bench_ogrid_bl, "ogrid_terms": bench_ogrid_terms, "batched": bench_batched, "cubed_sphere": bench_cubed,
-          "mlp": bench_mlp, "control": bench_control, "fas": bench_fas}[args.workload]
+          "mlp": bench_mlp, "control": bench_control, "control_topo": bench_control_topo,
+          "fas": bench_fas}[args.workload]
     print(json.dumps(fn(args)), flush=True)
     return 0
 
