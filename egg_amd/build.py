@@ -31,7 +31,7 @@ def build(jobs: int = 4, verbose: bool = False) -> None:
     import pybind11
 
     cmd = [
-        "g++", "-O2", "-shared", "-fPIC", "-std=c++17",
+        "g++", "-O2", "-shared", "-fPIC", "-std=c++23",
         f"-I{sysconfig.get_paths()['include']}", f"-I{pybind11.get_include()}", f"-I{ROOT / 'include'}",
         str(PKG / "_cpp" / "cpp_core.cpp"), "-o", str(module_path()),
         f"-L{LIB.parent}", "-leggmi", "-Wl,-rpath,$ORIGIN/../_lib",

@@ -45,7 +45,7 @@ long long check(const double* b, long long avail, int dim, bool in_composite)
 {
     if (avail < 1) bad("empty record");
     const double tv = b[0];
-    if (!std::isfinite(tv) || tv != std::floor(tv) || tv < 1 || tv > kEntMaxTag || tv == kEntFree)
+    if (!std::isfinite(tv) || tv != std::floor(tv) || tv < 1 || tv > (double)kEntMaxTag || tv == (double)kEntFree)
         bad("tag " + std::to_string(tv) + " is not an entity kind");
     const int tag = (int)tv;
     if (ent_dim(tag) != dim)
