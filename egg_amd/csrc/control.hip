@@ -537,7 +537,7 @@ __global__ __launch_bounds__(kThreads) void topo_eval_kernel(const TopoView t, c
                                                              float* out_plane)
 {
     const long long n = (long long)blockIdx.x * kThreads + threadIdx.x;
-    if (n >= t.n_all) return;
+    if (n >= t.n_all || (t.halt && *t.halt != 0.0)) return;
     const long long o = t.owner[n];
     const TopoBlock& Bo = t.blk[find_seg(t.npfx, t.nb, o)];
     double acc[D];
@@ -585,7 +585,7 @@ __global__ __launch_bounds__(kThreads) void topo_sample_hv_kernel(const TopoView
                                                                   const double* dir, double* zs)
 {
     const long long p = (long long)blockIdx.x * kThreads + threadIdx.x;
-    if (p >= t.ns_all) return;
+    if (p >= t.ns_all || (t.halt && *t.halt != 0.0)) return;
     const TopoBlock& B = t.blk[find_seg(t.spfx, t.nb, p)];
     sample_hv_at<D>(B.g, p - B.samp0, hs, dir + B.node0 * D, zs, t.ns_all, p);
 }
@@ -599,7 +599,7 @@ __global__ __launch_bounds__(kThreads) void topo_node_kernel(const TopoView t, c
 {
     constexpr int NO = MODE == 2 ? D * D : D;
     const long long n = (long long)blockIdx.x * kThreads + threadIdx.x;
-    if (n >= t.n_all) return;
+    if (n >= t.n_all || (t.halt && *t.halt != 0.0)) return;
     double acc[NO] = {};
     if (t.owner[n] == n) {
         for (int e = -1; e < t.grp_off[n + 1] - t.grp_off[n]; ++e) {
@@ -620,6 +620,7 @@ __global__ __launch_bounds__(kThreads) void topo_pullback_kernel(const TopoView 
 {
     constexpr int NO = MODE == 1 ? D * D : D;
     const long long f = blockIdx.x;
+    if (t.halt && *t.halt != 0.0) return;  // uniform over the workgroup
     const TopoBlock& B = t.blk[t.net_blk[find_seg(t.cpfx, t.n_nets, f)]];
     pullback_wg<D, MODE>(B.g, f - B.ctrl0, y + B.node0 * NO, out + f * NO);
 }
@@ -699,6 +700,117 @@ __global__ __launch_bounds__(kThreads) void qdiag_apply_kernel(long long nq, con
     if (c >= nq) return;
     const double d = diag[c] + lam;
     z[c] = d > 0.0 ? r[c] / d : r[c];
+}
+
+// ---------------------------------------------------------------- device-resident reduced PCG
+
+// The reduced-space PCG's recurrences with its scalars in device memory
+// (s: 0 r.z, 1 r.r, 2 p.Ap, 3 |b|^2, 4 stop threshold, 5 iterations, 6
+// state: 0 running, 1 converged, 2 negative curvature, 3 iteration cap),
+// one workgroup of 1024 threads each: strided partial sums and the same
+// fixed tree as ctrl_reduce_kernel, so the iterates are those of the
+// host-driven loop.
+__device__ __forceinline__ double wg1024_sum(double v, double* sh)
+{
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(1024) void qpcg_init_kernel(long long nq, const double* G, const double* diag, double lam,
+                                                         double rtol, double* x, double* r, double* z, double* p,
+                                                         double* s)
+{
+    __shared__ double sh[1024];
+    double rr = 0.0;
+    for (long long i = threadIdx.x; i < nq; i += 1024) {
+        x[i] = 0.0;
+        const double ri = -G[i];
+        r[i] = ri;
+        rr += ri * ri;
+    }
+    rr = wg1024_sum(rr, sh);
+    double rz = 0.0;
+    for (long long i = threadIdx.x; i < nq; i += 1024) {
+        const double d = diag[i] + lam, ri = r[i];
+        const double zi = d > 0.0 ? ri / d : ri;
+        z[i] = zi;
+        p[i] = zi;
+        rz += ri * zi;
+    }
+    rz = wg1024_sum(rz, sh);
+    if (threadIdx.x == 0) {
+        s[0] = rz;
+        s[1] = rr;
+        s[3] = rr;
+        s[4] = rtol * rtol * rr;
+        s[5] = 0.0;
+        s[6] = rr == 0.0 ? 1.0 : 0.0;
+    }
+}
+
+__global__ __launch_bounds__(1024) void qpcg_pap_kernel(long long nq, const double* p, const double* Ap, double* s)
+{
+    __shared__ double sh[1024];
+    if (s[6] != 0.0) return;
+    double acc = 0.0;
+    for (long long i = threadIdx.x; i < nq; i += 1024) acc += p[i] * Ap[i];
+    acc = wg1024_sum(acc, sh);
+    if (threadIdx.x == 0) s[2] = acc;
+}
+
+__global__ __launch_bounds__(1024) void qpcg_update_kernel(long long nq, const double* diag, double lam, int max_iter,
+                                                           double* x, double* r, double* z, double* p,
+                                                           const double* Ap, double* s)
+{
+    __shared__ double sh[1024];
+    if (s[6] != 0.0) return;
+    const double pap = s[2], rz_old = s[0], stop2 = s[4];
+    const int it = (int)s[5] + 1;
+    if (!(pap > 0.0)) {  // negative curvature: keep the last iterate
+        __syncthreads();
+        if (threadIdx.x == 0) s[6] = 2.0;
+        return;
+    }
+    const double a = rz_old / pap;
+    double rr = 0.0;
+    for (long long i = threadIdx.x; i < nq; i += 1024) {
+        x[i] = x[i] + a * p[i];
+        const double ri = r[i] + (-a) * Ap[i];
+        r[i] = ri;
+        rr += ri * ri;
+    }
+    rr = wg1024_sum(rr, sh);
+    if (rr <= stop2) {
+        if (threadIdx.x == 0) {
+            s[1] = rr;
+            s[5] = it;
+            s[6] = 1.0;
+        }
+        return;
+    }
+    double rz = 0.0;
+    for (long long i = threadIdx.x; i < nq; i += 1024) {
+        const double d = diag[i] + lam, ri = r[i];
+        const double zi = d > 0.0 ? ri / d : ri;
+        z[i] = zi;
+        rz += ri * zi;
+    }
+    rz = wg1024_sum(rz, sh);
+    const double beta = rz / rz_old;
+    for (long long i = threadIdx.x; i < nq; i += 1024) p[i] = z[i] + beta * p[i];
+    if (threadIdx.x == 0) {
+        s[0] = rz;
+        s[1] = rr;
+        s[5] = it;
+        if (it >= max_iter) s[6] = 3.0;
+    }
 }
 
 // ---------------------------------------------------------------- vectors
@@ -1064,6 +1176,23 @@ void launch_qdiag_build(int dim, long long nq, const int* g_off, const int* g_ro
 void launch_qdiag_apply(long long nq, const double* diag, double lam, const double* r, double* z, hipStream_t s)
 {
     if (nq > 0) qdiag_apply_kernel<<<blocks_for(nq), kThreads, 0, s>>>(nq, diag, lam, r, z);
+}
+
+void launch_qpcg_init(long long nq, const double* G, const double* diag, double lam, double rtol, double* x, double* r,
+                      double* z, double* p, double* s, hipStream_t st)
+{
+    qpcg_init_kernel<<<1, 1024, 0, st>>>(nq, G, diag, lam, rtol, x, r, z, p, s);
+}
+
+void launch_qpcg_pap(long long nq, const double* p, const double* Ap, double* s, hipStream_t st)
+{
+    qpcg_pap_kernel<<<1, 1024, 0, st>>>(nq, p, Ap, s);
+}
+
+void launch_qpcg_update(long long nq, const double* diag, double lam, int max_iter, double* x, double* r, double* z,
+                        double* p, const double* Ap, double* s, hipStream_t st)
+{
+    qpcg_update_kernel<<<1, 1024, 0, st>>>(nq, diag, lam, max_iter, x, r, z, p, Ap, s);
 }
 
 }  // namespace eggmi

@@ -336,10 +336,6 @@ ControlReport Session::run_control_topo(const ControlOptions& o, int kind)
         EGGMI_HIP(hipStreamSynchronize(stream));
         return v;
     };
-    auto dot = [&](const double* a, const double* b) {
-        launch_ctrl_reduce(0, a, b, nq, dim, c.scal.p, stream);
-        return scalar(0);
-    };
     // the penalty residuals P x into host memory
     std::vector<double> r0(c.n_pen), r1(c.n_pen);
     auto pen_residual = [&](const double* x, std::vector<double>& out) {
@@ -361,45 +357,55 @@ ControlReport Session::run_control_topo(const ControlOptions& o, int kind)
         launch_csr_apply(c.n_pen, c.p_off.p, c.p_col.p, c.p_coef.p, x, 0.0, nullptr, c.r_pen.p, stream);
         launch_csrT_accum(nsd, c.pt_off.p, c.pt_row.p, c.pt_coef.p, c.p_w.p, c.r_pen.p, c.wfull.p, stream);
     };
-    // (R^T (A + P^T W P) R + lam I) v -> out
+    // (R^T (A + P^T W P) R + lam I) v -> out, inside the PCG: its heavy
+    // kernels return at once once the PCG has stopped
+    TopoView tvh = tv;
+    tvh.halt = c.scal.p + 6;
     auto matvec = [&](double lam, const double* v, double* out) {
         launch_csr_apply(nsd, c.r_off.p, c.r_col.p, c.r_coef.p, v, 0.0, nullptr, c.vfull.p, stream);
-        launch_topo_eval(tv, true, c.vfull.p, c.dir.p, nullptr, stream);
-        launch_topo_sample_hv(tv, c.hs.p, c.dir.p, c.zs.p, stream);
-        launch_topo_node(tv, 0, c.zs.p, nullptr, c.ynode.p, stream);
-        launch_topo_pullback(tv, 0, c.ynode.p, c.wfull.p, stream);
+        launch_topo_eval(tvh, true, c.vfull.p, c.dir.p, nullptr, stream);
+        launch_topo_sample_hv(tvh, c.hs.p, c.dir.p, c.zs.p, stream);
+        launch_topo_node(tvh, 0, c.zs.p, nullptr, c.ynode.p, stream);
+        launch_topo_pullback(tvh, 0, c.ynode.p, c.wfull.p, stream);
         pen_accum(c.vfull.p);
         launch_csr_apply(nq, c.g_off.p, c.g_row.p, c.g_coef.p, c.wfull.p, lam, lam > 0.0 ? v : nullptr, out, stream);
     };
+    // PCG with device-resident scalars: the recurrences run in chunks of
+    // iterations with one 8-double read-back per chunk; the chunk length
+    // follows the residual's observed rate to the stop threshold (the
+    // iterations past the stop are no-ops on the device)
+    double ps[8];
+    auto read_state = [&]() {
+        EGGMI_HIP(hipMemcpyAsync(ps, c.scal.p, sizeof(ps), hipMemcpyDeviceToHost, stream));
+        EGGMI_HIP(hipStreamSynchronize(stream));
+    };
     auto pcg = [&](double lam, int max_iter, double rtol) -> std::pair<int, double> {
-        EGGMI_HIP(hipMemsetAsync(c.qdq.p, 0, nq * sizeof(double), stream));
-        launch_ctrl_neg(nq, c.qG.p, c.qr.p, stream);
-        const double b2 = dot(c.qr.p, c.qr.p);
-        if (b2 == 0.0) return {0, 0.0};
-        const double stop2 = rtol * rtol * b2;
-        double rr = b2;
-        launch_qdiag_apply(nq, c.qdiag.p, lam, c.qr.p, c.qz.p, stream);
-        EGGMI_HIP(hipMemcpyAsync(c.qp.p, c.qz.p, nq * sizeof(double), hipMemcpyDeviceToDevice, stream));
-        double rz = dot(c.qr.p, c.qz.p);
-        int it = 0;
-        for (; it < max_iter; ++it) {
-            matvec(lam, c.qp.p, c.qAp.p);
-            const double pap = dot(c.qp.p, c.qAp.p);
-            if (!(pap > 0.0)) break;  // negative curvature: keep the last iterate
-            const double a = rz / pap;
-            launch_ctrl_axpy(nq, a, c.qp.p, c.qdq.p, false, stream);
-            launch_ctrl_axpy(nq, -a, c.qAp.p, c.qr.p, false, stream);
-            rr = dot(c.qr.p, c.qr.p);
-            if (rr <= stop2) {
-                ++it;
-                break;
+        launch_qpcg_init(nq, c.qG.p, c.qdiag.p, lam, rtol, c.qdq.p, c.qr.p, c.qz.p, c.qp.p, c.scal.p, stream);
+        read_state();
+        if (ps[3] == 0.0) return {0, 0.0};
+        if (max_iter <= 0) return {0, 1.0};
+        double rr_prev = ps[3];
+        int chunk = 4, done_it = 0;
+        while (true) {
+            const int n = std::max(1, std::min(chunk, max_iter - done_it));
+            for (int i = 0; i < n; ++i) {
+                matvec(lam, c.qp.p, c.qAp.p);
+                launch_qpcg_pap(nq, c.qp.p, c.qAp.p, c.scal.p, stream);
+                launch_qpcg_update(nq, c.qdiag.p, lam, max_iter, c.qdq.p, c.qr.p, c.qz.p, c.qp.p, c.qAp.p, c.scal.p,
+                                   stream);
             }
-            launch_qdiag_apply(nq, c.qdiag.p, lam, c.qr.p, c.qz.p, stream);
-            const double rz_new = dot(c.qr.p, c.qz.p);
-            launch_ctrl_axpy(nq, rz_new / rz, c.qz.p, c.qp.p, true, stream);  // p = z + beta p
-            rz = rz_new;
+            read_state();
+            const int it = (int)ps[5];
+            if (ps[6] != 0.0 || it >= max_iter) break;
+            // iterations to the threshold at the chunk's mean rate, 2 to 32
+            const double rate = std::pow(ps[1] / rr_prev, 1.0 / std::max(1, it - done_it));
+            chunk = 8;
+            if (rate > 0.0 && rate < 1.0 && ps[1] > ps[4])
+                chunk = (int)std::clamp(std::ceil(std::log(ps[4] / ps[1]) / std::log(rate)), 2.0, 32.0);
+            rr_prev = ps[1];
+            done_it = it;
         }
-        return {it, std::sqrt(rr / b2)};
+        return {(int)ps[5], std::sqrt(ps[1] / ps[3])};
     };
 
     // start exactly on the nets' manifold

@@ -34,7 +34,27 @@ def wavy_pair(n=9, wave=0.3, seed=3):
     return c
 
 
+def periodic_ring(nt=25, nr=9):
+    """One block closing on itself: an annulus whose last angular row of
+    nodes is its first (a periodic seam inside one block)."""
+    ids = np.arange(nt * nr).reshape(nt, nr)
+    ids[-1, :] = ids[0, :]
+    used, ids = np.unique(ids, return_inverse=True)
+    ids = ids.reshape(nt, nr).astype(np.int32)
+    th = -2.0 * np.pi * np.arange(nt) / (nt - 1)
+    T, Rr = np.meshgrid(th, np.linspace(0.5, 1.5, nr), indexing="ij")
+    X = np.zeros((used.size, 2))
+    X[ids.reshape(-1)] = np.stack([Rr * np.cos(T), Rr * np.sin(T)], -1).reshape(-1, 2)
+    free = np.ones(used.size, bool)
+    free[ids[:, 0]] = False
+    free[ids[:, -1]] = False
+    rng = np.random.default_rng(8)
+    X[free] += rng.uniform(-0.02, 0.02, (int(free.sum()), 2))
+    return grids.Grid(2, [ids], X, free)
+
+
 CASES = {
+    "periodic_ring": (periodic_ring, [(9, 5)], {}),
     "lr_pair": (lambda: lr_pair(9).grid, [(5, 5)] * 2, {}),
     "lr_pair_c1": (lambda: lr_pair(9).grid, [(5, 5)] * 2, {"c1": True}),
     "lr_pair_pen": (lambda: lr_pair(9).grid, [(5, 5)] * 2, {"seam_weight": 10.0}),

@@ -17,7 +17,7 @@ from egg_amd import grids
 from egg_amd.smoothing import build_structured_context_from_block_maps, build_sweep_context
 from egg_amd.smoothing.control import build_control_topology, topology_mismatch
 from tests.cases import block_grid, ogrid
-from tests.test_control_topo import TEE, wavy_pair
+from tests.test_control_topo import TEE, periodic_ring, wavy_pair
 
 pytestmark = pytest.mark.gpu
 
@@ -111,6 +111,11 @@ def test_parity_blocks_without_nets(oracle):
     x, y = g.X.T
     g.X = g.X + np.stack([0.08 * np.sin(np.pi * y / 3.0) * x, 0.0 * y], -1)
     _parity(g, [(5, 6), (5, 5)], "barrier", oracle, relax=300, net_blocks=[1, 2])
+
+
+def test_parity_periodic_seam_in_one_block(oracle):
+    # the seam's copies and owners sit in the same block
+    _parity(periodic_ring(), [(9, 5)], "barrier", oracle, relax=300)
 
 
 def test_parity_3d(oracle):
