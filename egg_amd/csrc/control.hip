@@ -388,23 +388,15 @@ __device__ __forceinline__ void node_acc(const CtrlGeom& g, const int* I, const 
         if (!ok) continue;
         const long long cell = D == 2 ? (long long)I0[0] * (g.n[1] - 1) + I0[1]
                                       : ((long long)I0[0] * (g.n[1] - 1) + I0[1]) * (g.n[2] - 1) + I0[2];
-        for (int b = 0; b < NC; ++b) {
-            // the node's role in sample (cell, b): corner, axis-k neighbour or none
-            double u[D];
-            const int x = o ^ b;
-            if (x == 0) {
-#pragma unroll
-                for (int k = 0; k < D; ++k) u[k] = ((b >> (D - 1 - k)) & 1) ? 1.0 : -1.0;  // -s_k
-            } else if ((x & (x - 1)) == 0) {
-#pragma unroll
-                for (int k = 0; k < D; ++k) {
-                    const int bit = 1 << (D - 1 - k);
-                    u[k] = (x == bit) ? (((b & bit) ? -1.0 : 1.0)) : 0.0;  // s_k on axis k
-                }
-            } else {
-                continue;
-            }
+        // the node's role in sample (cell, b): the corner (b = o, weights
+        // u_k = -s_k on every axis) or the axis-k neighbour (b = o ^ bit k,
+        // the single weight u_k = s_k; s_k = +-1, so u_k^2 = 1)
+        {
+            const int b = o;
             const long long p = s0 + cell * NC + b;
+            double u[D];
+#pragma unroll
+            for (int k = 0; k < D; ++k) u[k] = ((b >> (D - 1 - k)) & 1) ? 1.0 : -1.0;
             if constexpr (MODE == 0) {
 #pragma unroll
                 for (int i = 0; i < D; ++i)
@@ -423,6 +415,22 @@ __device__ __forceinline__ void node_acc(const CtrlGeom& g, const int* I, const 
                                 h += u[k] * u[l] * hs[(long long)sym_index<D>(i * D + k, j * D + l) * ns + p];
                         acc[i * D + j] += h;
                     }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const int bit = 1 << (D - 1 - k);
+            const int b = o ^ bit;
+            const long long p = s0 + cell * NC + b;
+            if constexpr (MODE == 0) {
+                const double uk = (b & bit) ? -1.0 : 1.0;
+#pragma unroll
+                for (int i = 0; i < D; ++i) acc[i] += uk * gs[(long long)(i * D + k) * ns + p];
+            } else {
+#pragma unroll
+                for (int i = 0; i < D; ++i)
+#pragma unroll
+                    for (int j = 0; j < D; ++j) acc[i * D + j] += hs[(long long)sym_index<D>(i * D + k, j * D + k) * ns + p];
             }
         }
     }
