@@ -139,7 +139,7 @@ __global__ __launch_bounds__(kThreads) void ctrl_eval_kernel(const CtrlGeom g, c
                                                              double* out_dir, float* out_plane)
 {
     const long long n = (long long)blockIdx.x * kThreads + threadIdx.x;
-    if (n >= g.n_nodes) return;
+    if (n >= g.n_nodes || (g.halt && *g.halt != 0.0)) return;
     int I[3];
     unravel_node<D>(g, n, I);
     double acc[D];
@@ -279,7 +279,7 @@ __global__ __launch_bounds__(kThreads) void ctrl_sample_hv_kernel(const CtrlGeom
 {
     const long long p = (long long)blockIdx.x * kThreads + threadIdx.x;
     const long long ns = g.n_cells * (1 << D);
-    if (p < ns) sample_hv_at<D>(g, p, hs, dir, zs, ns, p);
+    if (p < ns && !(g.halt && *g.halt != 0.0)) sample_hv_at<D>(g, p, hs, dir, zs, ns, p);
 }
 
 // energy and det of block-local sample p at xb + a dir
@@ -442,7 +442,7 @@ __global__ __launch_bounds__(kThreads) void ctrl_node_kernel(const CtrlGeom g, c
 {
     constexpr int NO = MODE == 2 ? D * D : D;
     const long long n = (long long)blockIdx.x * kThreads + threadIdx.x;
-    if (n >= g.n_nodes) return;
+    if (n >= g.n_nodes || (g.halt && *g.halt != 0.0)) return;
     int I[3];
     unravel_node<D>(g, n, I);
     double acc[NO] = {};
@@ -504,6 +504,7 @@ __global__ __launch_bounds__(kThreads) void ctrl_pullback_kernel(const CtrlGeom 
 {
     constexpr int NO = MODE == 1 ? D * D : D;
     const int f = blockIdx.x;
+    if (g.halt && *g.halt != 0.0) return;  // uniform over the workgroup
     pullback_wg<D, MODE>(g, g.free_idx[f], y, out + (long long)f * NO);
 }
 
@@ -731,7 +732,62 @@ __device__ __forceinline__ double wg1024_sum(double v, double* sh)
     return r;
 }
 
-__global__ __launch_bounds__(1024) void qpcg_init_kernel(long long nq, const double* G, const double* diag, double lam,
+// z = M^-1 r by the workgroup, then its threads' partials of r.z in the
+// element order of ctrl_reduce_kernel: BLOCK false, M = diag + lam I (a
+// scalar per element); BLOCK true, M = one D x D block plus lam I per
+// control (Gaussian elimination as ctrl_precond_kernel)
+template <int D, bool BLOCK>
+__device__ __forceinline__ double wg_precond(long long nq, const double* P, double lam, const double* r, double* z)
+{
+    if constexpr (!BLOCK) {
+        double rz = 0.0;
+        for (long long i = threadIdx.x; i < nq; i += 1024) {
+            const double d = P[i] + lam, ri = r[i];
+            const double zi = d > 0.0 ? ri / d : ri;
+            z[i] = zi;
+            rz += ri * zi;
+        }
+        return rz;
+    } else {
+        const long long nf = nq / D;
+        for (long long f = threadIdx.x; f < nf; f += 1024) {
+            double A[D * D], rhs[D], x[D];
+#pragma unroll
+            for (int i = 0; i < D * D; ++i) A[i] = P[f * D * D + i];
+#pragma unroll
+            for (int i = 0; i < D; ++i) {
+                A[i * D + i] += lam;
+                rhs[i] = r[f * D + i];
+            }
+#pragma unroll
+            for (int i = 0; i < D; ++i) {
+#pragma unroll
+                for (int j = i + 1; j < D; ++j) {
+                    const double m = A[i * D + i] != 0.0 ? A[j * D + i] / A[i * D + i] : 0.0;
+#pragma unroll
+                    for (int k = i; k < D; ++k) A[j * D + k] -= m * A[i * D + k];
+                    rhs[j] -= m * rhs[i];
+                }
+            }
+#pragma unroll
+            for (int i = D - 1; i >= 0; --i) {
+                double t = rhs[i];
+#pragma unroll
+                for (int k = i + 1; k < D; ++k) t -= A[i * D + k] * x[k];
+                x[i] = A[i * D + i] != 0.0 ? t / A[i * D + i] : t;
+            }
+#pragma unroll
+            for (int i = 0; i < D; ++i) z[f * D + i] = x[i];
+        }
+        __syncthreads();  // z of other threads' controls, read below
+        double rz = 0.0;
+        for (long long i = threadIdx.x; i < nq; i += 1024) rz += r[i] * z[i];
+        return rz;
+    }
+}
+
+template <int D, bool BLOCK>
+__global__ __launch_bounds__(1024) void qpcg_init_kernel(long long nq, const double* G, const double* P, double lam,
                                                          double rtol, double* x, double* r, double* z, double* p,
                                                          double* s)
 {
@@ -743,16 +799,10 @@ __global__ __launch_bounds__(1024) void qpcg_init_kernel(long long nq, const dou
         r[i] = ri;
         rr += ri * ri;
     }
-    rr = wg1024_sum(rr, sh);
-    double rz = 0.0;
-    for (long long i = threadIdx.x; i < nq; i += 1024) {
-        const double d = diag[i] + lam, ri = r[i];
-        const double zi = d > 0.0 ? ri / d : ri;
-        z[i] = zi;
-        p[i] = zi;
-        rz += ri * zi;
-    }
+    rr = wg1024_sum(rr, sh);  // its barrier publishes r
+    double rz = wg_precond<D, BLOCK>(nq, P, lam, r, z);
     rz = wg1024_sum(rz, sh);
+    for (long long i = threadIdx.x; i < nq; i += 1024) p[i] = z[i];
     if (threadIdx.x == 0) {
         s[0] = rz;
         s[1] = rr;
@@ -773,7 +823,8 @@ __global__ __launch_bounds__(1024) void qpcg_pap_kernel(long long nq, const doub
     if (threadIdx.x == 0) s[2] = acc;
 }
 
-__global__ __launch_bounds__(1024) void qpcg_update_kernel(long long nq, const double* diag, double lam, int max_iter,
+template <int D, bool BLOCK>
+__global__ __launch_bounds__(1024) void qpcg_update_kernel(long long nq, const double* P, double lam, int max_iter,
                                                            double* x, double* r, double* z, double* p,
                                                            const double* Ap, double* s)
 {
@@ -794,7 +845,7 @@ __global__ __launch_bounds__(1024) void qpcg_update_kernel(long long nq, const d
         r[i] = ri;
         rr += ri * ri;
     }
-    rr = wg1024_sum(rr, sh);
+    rr = wg1024_sum(rr, sh);  // its barrier publishes r
     if (rr <= stop2) {
         if (threadIdx.x == 0) {
             s[1] = rr;
@@ -803,13 +854,7 @@ __global__ __launch_bounds__(1024) void qpcg_update_kernel(long long nq, const d
         }
         return;
     }
-    double rz = 0.0;
-    for (long long i = threadIdx.x; i < nq; i += 1024) {
-        const double d = diag[i] + lam, ri = r[i];
-        const double zi = d > 0.0 ? ri / d : ri;
-        z[i] = zi;
-        rz += ri * zi;
-    }
+    double rz = wg_precond<D, BLOCK>(nq, P, lam, r, z);
     rz = wg1024_sum(rz, sh);
     const double beta = rz / rz_old;
     for (long long i = threadIdx.x; i < nq; i += 1024) p[i] = z[i] + beta * p[i];
@@ -1186,10 +1231,15 @@ void launch_qdiag_apply(long long nq, const double* diag, double lam, const doub
     if (nq > 0) qdiag_apply_kernel<<<blocks_for(nq), kThreads, 0, s>>>(nq, diag, lam, r, z);
 }
 
-void launch_qpcg_init(long long nq, const double* G, const double* diag, double lam, double rtol, double* x, double* r,
-                      double* z, double* p, double* s, hipStream_t st)
+void launch_qpcg_init(int dim, bool block, long long nq, const double* G, const double* P, double lam, double rtol,
+                      double* x, double* r, double* z, double* p, double* s, hipStream_t st)
 {
-    qpcg_init_kernel<<<1, 1024, 0, st>>>(nq, G, diag, lam, rtol, x, r, z, p, s);
+    if (!block)
+        qpcg_init_kernel<2, false><<<1, 1024, 0, st>>>(nq, G, P, lam, rtol, x, r, z, p, s);
+    else if (dim == 2)
+        qpcg_init_kernel<2, true><<<1, 1024, 0, st>>>(nq, G, P, lam, rtol, x, r, z, p, s);
+    else
+        qpcg_init_kernel<3, true><<<1, 1024, 0, st>>>(nq, G, P, lam, rtol, x, r, z, p, s);
 }
 
 void launch_qpcg_pap(long long nq, const double* p, const double* Ap, double* s, hipStream_t st)
@@ -1197,10 +1247,15 @@ void launch_qpcg_pap(long long nq, const double* p, const double* Ap, double* s,
     qpcg_pap_kernel<<<1, 1024, 0, st>>>(nq, p, Ap, s);
 }
 
-void launch_qpcg_update(long long nq, const double* diag, double lam, int max_iter, double* x, double* r, double* z,
-                        double* p, const double* Ap, double* s, hipStream_t st)
+void launch_qpcg_update(int dim, bool block, long long nq, const double* P, double lam, int max_iter, double* x,
+                        double* r, double* z, double* p, const double* Ap, double* s, hipStream_t st)
 {
-    qpcg_update_kernel<<<1, 1024, 0, st>>>(nq, diag, lam, max_iter, x, r, z, p, Ap, s);
+    if (!block)
+        qpcg_update_kernel<2, false><<<1, 1024, 0, st>>>(nq, P, lam, max_iter, x, r, z, p, Ap, s);
+    else if (dim == 2)
+        qpcg_update_kernel<2, true><<<1, 1024, 0, st>>>(nq, P, lam, max_iter, x, r, z, p, Ap, s);
+    else
+        qpcg_update_kernel<3, true><<<1, 1024, 0, st>>>(nq, P, lam, max_iter, x, r, z, p, Ap, s);
 }
 
 }  // namespace eggmi

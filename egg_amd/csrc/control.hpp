@@ -47,6 +47,10 @@ struct CtrlGeom {
     const double* sup_w[3] = {};
     const int* free_idx = nullptr;  // [n_free] lattice index
     const int* l2f = nullptr;       // [n_ctrl] lattice -> free slot, -1 = fixed
+    // set: the direction evaluation, Hessian-action, gather and pullback
+    // kernels return at once while *halt != 0 (iterations a device-resident
+    // PCG queues past its stop)
+    const double* halt = nullptr;
 };
 
 // Several blocks under one stacked control vector (reference
@@ -116,17 +120,20 @@ void launch_qdiag_build(int dim, long long nq, const int* g_off, const int* g_ro
                         const double* P, double* diag, hipStream_t s);
 void launch_qdiag_apply(long long nq, const double* diag, double lam, const double* r, double* z, hipStream_t s);
 
-// The reduced PCG's recurrences with device-resident scalars (s, 8
-// doubles: 0 r.z, 1 r.r, 2 p.Ap, 3 |b|^2, 4 stop threshold, 5 iterations, 6
-// state 0 running / 1 converged / 2 negative curvature / 3 iteration cap),
-// one workgroup each. init: x = 0, r = -G, z = r / (diag + lam), p = z.
-// pap: p.Ap. update (skipped once the state is not running): the step,
-// the residual and its norm, the stop test, then z, r.z and p = z + beta p.
-void launch_qpcg_init(long long nq, const double* G, const double* diag, double lam, double rtol, double* x, double* r,
-                      double* z, double* p, double* s, hipStream_t st);
+// The PCG recurrences of the control loops with device-resident scalars
+// (s, 8 doubles: 0 r.z, 1 r.r, 2 p.Ap, 3 |b|^2, 4 stop threshold, 5
+// iterations, 6 state 0 running / 1 converged / 2 negative curvature / 3
+// iteration cap), one workgroup each. The preconditioner P: block false, a
+// diagonal (nq values, z = r / (P + lam) where positive); block true, one
+// dim x dim block per control (nq / dim of them, z = (P_f + lam I)^-1 r_f).
+// init: x = 0, r = -G, z, p = z. pap: p.Ap. update (skipped once the state
+// is not running): the step, the residual and its norm, the stop test,
+// then z, r.z and p = z + beta p.
+void launch_qpcg_init(int dim, bool block, long long nq, const double* G, const double* P, double lam, double rtol,
+                      double* x, double* r, double* z, double* p, double* s, hipStream_t st);
 void launch_qpcg_pap(long long nq, const double* p, const double* Ap, double* s, hipStream_t st);
-void launch_qpcg_update(long long nq, const double* diag, double lam, int max_iter, double* x, double* r, double* z,
-                        double* p, const double* Ap, double* s, hipStream_t st);
+void launch_qpcg_update(int dim, bool block, long long nq, const double* P, double lam, int max_iter, double* x,
+                        double* r, double* z, double* p, const double* Ap, double* s, hipStream_t st);
 
 // Evaluation of the map over the block's nodes into block-local FP64 node
 // fields (dim per node, node order row-major) and, when given, into the FP32

@@ -202,48 +202,53 @@ ControlReport Session::run_control(const ControlOptions& o, int kind)
         EGGMI_HIP(hipStreamSynchronize(stream));
         return v;
     };
-    auto dot = [&](const double* a, const double* b) {
-        launch_ctrl_reduce(0, a, b, nfd, dim, c.scal.p, stream);
-        return scalar(0);
-    };
     // (A + lam I) v -> out: direction field M v, the per-sample Hessian
-    // action gathered per node, pulled back to the free controls
+    // action gathered per node, pulled back to the free controls; inside
+    // the PCG, so the kernels return at once once it has stopped
+    CtrlGeom gh = g;
+    gh.halt = c.scal.p + 6;
     auto matvec = [&](double lam, const double* v, double* out) {
-        launch_ctrl_eval_dir(g, v, c.dir.p, nullptr, stream);
-        launch_ctrl_sample_hv(g, c.hs.p, c.dir.p, c.zs.p, stream);
-        launch_ctrl_node(g, 0, c.zs.p, nullptr, nullptr, c.ynode.p, stream);
-        launch_ctrl_pullback(g, 0, c.ynode.p, out, stream);
+        launch_ctrl_eval_dir(gh, v, c.dir.p, nullptr, stream);
+        launch_ctrl_sample_hv(gh, c.hs.p, c.dir.p, c.zs.p, stream);
+        launch_ctrl_node(gh, 0, c.zs.p, nullptr, nullptr, c.ynode.p, stream);
+        launch_ctrl_pullback(gh, 0, c.ynode.p, out, stream);
         if (lam > 0.0) launch_ctrl_axpy(nfd, lam, v, out, false, stream);
     };
+    // the reference's Jacobi-preconditioned CG (src/control_solve.hpp:
+    // 761-818) with its scalars on the device: chunks of iterations, one
+    // 8-double read-back each, the chunk sized from the residual's observed
+    // rate to the stop threshold
+    double ps[8];
+    auto read_state = [&]() {
+        EGGMI_HIP(hipMemcpyAsync(ps, c.scal.p, sizeof(ps), hipMemcpyDeviceToHost, stream));
+        EGGMI_HIP(hipStreamSynchronize(stream));
+    };
     auto pcg = [&](double lam, int max_iter, double rtol) -> std::pair<int, double> {
-        EGGMI_HIP(hipMemsetAsync(c.dC.p, 0, nfd * sizeof(double), stream));
-        launch_ctrl_neg(nfd, c.G.p, c.r.p, stream);
-        const double b2 = dot(c.r.p, c.r.p);
-        if (b2 == 0.0) return {0, 0.0};
-        const double stop2 = rtol * rtol * b2;
-        double rr = b2;
-        launch_ctrl_precond(g, c.prec.p, lam, c.r.p, c.z.p, stream);
-        EGGMI_HIP(hipMemcpyAsync(c.p.p, c.z.p, nfd * sizeof(double), hipMemcpyDeviceToDevice, stream));
-        double rz = dot(c.r.p, c.z.p);
-        int it = 0;
-        for (; it < max_iter; ++it) {
-            matvec(lam, c.p.p, c.Ap.p);
-            const double pap = dot(c.p.p, c.Ap.p);
-            if (!(pap > 0.0)) break;  // negative curvature: keep the last iterate
-            const double a = rz / pap;
-            launch_ctrl_axpy(nfd, a, c.p.p, c.dC.p, false, stream);
-            launch_ctrl_axpy(nfd, -a, c.Ap.p, c.r.p, false, stream);
-            rr = dot(c.r.p, c.r.p);
-            if (rr <= stop2) {
-                ++it;
-                break;
+        launch_qpcg_init(dim, true, nfd, c.G.p, c.prec.p, lam, rtol, c.dC.p, c.r.p, c.z.p, c.p.p, c.scal.p, stream);
+        read_state();
+        if (ps[3] == 0.0) return {0, 0.0};
+        if (max_iter <= 0) return {0, 1.0};
+        double rr_prev = ps[3];
+        int chunk = 4, done_it = 0;
+        while (true) {
+            const int n = std::max(1, std::min(chunk, max_iter - done_it));
+            for (int i = 0; i < n; ++i) {
+                matvec(lam, c.p.p, c.Ap.p);
+                launch_qpcg_pap(nfd, c.p.p, c.Ap.p, c.scal.p, stream);
+                launch_qpcg_update(dim, true, nfd, c.prec.p, lam, max_iter, c.dC.p, c.r.p, c.z.p, c.p.p, c.Ap.p,
+                                   c.scal.p, stream);
             }
-            launch_ctrl_precond(g, c.prec.p, lam, c.r.p, c.z.p, stream);
-            const double rz_new = dot(c.r.p, c.z.p);
-            launch_ctrl_axpy(nfd, rz_new / rz, c.z.p, c.p.p, true, stream);  // p = z + beta p
-            rz = rz_new;
+            read_state();
+            const int it = (int)ps[5];
+            if (ps[6] != 0.0 || it >= max_iter) break;
+            const double rate = std::pow(ps[1] / rr_prev, 1.0 / std::max(1, it - done_it));
+            chunk = 8;
+            if (rate > 0.0 && rate < 1.0 && ps[1] > ps[4])
+                chunk = (int)std::clamp(std::ceil(std::log(ps[4] / ps[1]) / std::log(rate)), 2.0, 32.0);
+            rr_prev = ps[1];
+            done_it = it;
         }
-        return {it, std::sqrt(rr / b2)};
+        return {(int)ps[5], std::sqrt(ps[1] / ps[3])};
     };
 
     // the net owns the block from here: start exactly on its manifold

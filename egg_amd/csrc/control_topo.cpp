@@ -380,7 +380,7 @@ ControlReport Session::run_control_topo(const ControlOptions& o, int kind)
         EGGMI_HIP(hipStreamSynchronize(stream));
     };
     auto pcg = [&](double lam, int max_iter, double rtol) -> std::pair<int, double> {
-        launch_qpcg_init(nq, c.qG.p, c.qdiag.p, lam, rtol, c.qdq.p, c.qr.p, c.qz.p, c.qp.p, c.scal.p, stream);
+        launch_qpcg_init(dim, false, nq, c.qG.p, c.qdiag.p, lam, rtol, c.qdq.p, c.qr.p, c.qz.p, c.qp.p, c.scal.p, stream);
         read_state();
         if (ps[3] == 0.0) return {0, 0.0};
         if (max_iter <= 0) return {0, 1.0};
@@ -391,7 +391,7 @@ ControlReport Session::run_control_topo(const ControlOptions& o, int kind)
             for (int i = 0; i < n; ++i) {
                 matvec(lam, c.qp.p, c.qAp.p);
                 launch_qpcg_pap(nq, c.qp.p, c.qAp.p, c.scal.p, stream);
-                launch_qpcg_update(nq, c.qdiag.p, lam, max_iter, c.qdq.p, c.qr.p, c.qz.p, c.qp.p, c.qAp.p, c.scal.p,
+                launch_qpcg_update(dim, false, nq, c.qdiag.p, lam, max_iter, c.qdq.p, c.qr.p, c.qz.p, c.qp.p, c.qAp.p, c.scal.p,
                                    stream);
             }
             read_state();
