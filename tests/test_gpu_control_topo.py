@@ -177,6 +177,20 @@ def test_state_calls_and_errors():
         _session(g, ctx, {**wire, "slide": [1]})
     with pytest.raises(ValueError, match="untangle"):
         s.run_control(1, phase="untangle")
+    # a reduction with another column count (the tangent-row topology's R and
+    # C0) replaces the first: the run is the oracle's on that wire
+    from oracle import control as OC2
+
+    w1, _ = build_control_topology(g.blocks, g.X, [(6, 6)] * 2, c1=True)
+    assert w1["nq"] != wire["nq"]
+    s.set_control_reduction(w1["nq"], w1["r_off"], w1["r_col"], w1["r_coef"])
+    s.set_C(np.asarray(w1["C0"]).reshape(-1, 2))
+    rq = s.run_control(30, pcg_max_iter=2000, pcg_rtol=1e-12, pcg_forcing=False)
+    _, _, ro = OC2.run_control_topo(2, "barrier", g.X, ctx.energy_stencil, w1, g.blocks, n_outer=30)
+    assert rq["iters"] == ro["iters"]
+    np.testing.assert_allclose(rq["energies"], ro["energies"], rtol=1e-9)
+    s.set_control_reduction(wire["nq"], wire["r_off"], wire["r_col"], wire["r_coef"])
+    s.set_C(C0)
     # the nodal smoother continues from the control state
     rd = s.run_control(30)
     e, m = s.run(10, omega=0.8)
