@@ -623,6 +623,104 @@ __global__ __launch_bounds__(kThreads) void topo_node_kernel(const TopoView t, c
     for (int c = 0; c < NO; ++c) out[n * NO + c] = acc[c];
 }
 
+// Per sample, its contributions to the cell's corner nodes, summed per
+// corner inside the cell's 2^D consecutive lanes: sample b gives its
+// corner node sum_k u_k z_(i,k) (u_k = -s_k) and its axis-k neighbour
+// (corner b ^ bit k) s_k z_(i,k); lane b adds its own term and what the
+// lanes b ^ bit k give their axis-k neighbour. z is the sample's Hessian
+// action H dA (HV) or its metric gradient. cc[(p) D + i]: corner b of the
+// cell of sample p (the sample index space, D values each).
+template <int D, bool HV>
+__global__ __launch_bounds__(kThreads) void topo_cell_kernel(const TopoView t, const double* gs, const double* hs,
+                                                             const double* dir, double* cc)
+{
+    constexpr int NC = 1 << D, V = kV<D>;
+    if (t.halt && *t.halt != 0.0) return;  // uniform over the launch
+    const long long p = (long long)blockIdx.x * kThreads + threadIdx.x;
+    // the sample count is a multiple of 2^D and so is kThreads: a cell's
+    // lanes are all in range or all out of it, and leave together
+    if (p >= t.ns_all) return;
+    double z[V];
+    if constexpr (HV) {
+        const TopoBlock& B = t.blk[find_seg(t.spfx, t.nb, p)];
+        const long long pl = p - B.samp0, cell = pl / NC;
+        const int b = int(pl - cell * NC);
+        int I0[3];
+        cell_origin<D>(B.g, cell, I0);
+        double dA[V];
+        sample_A<D>(B.g, I0, b, dir + B.node0 * D, nullptr, 0.0, dA);
+        double H[kNHs<D>];
+#pragma unroll
+        for (int e = 0; e < kNHs<D>; ++e) H[e] = hs[(long long)e * t.ns_all + p];
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+            double y = 0.0;
+#pragma unroll
+            for (int r = 0; r < V; ++r) y += H[sym_index<D>(i, r)] * dA[r];
+            z[i] = y;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < V; ++i) z[i] = gs[(long long)i * t.ns_all + p];
+    }
+    const int b = int(p & (NC - 1));  // samp0 of every block is a multiple of 2^D
+    double tot[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        double own = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) own += (((b >> (D - 1 - k)) & 1) ? 1.0 : -1.0) * z[i * D + k];
+        tot[i] = own;
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        const int bit = 1 << (D - 1 - k);
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            const double nb = ((b & bit) ? -1.0 : 1.0) * z[i * D + k];  // to corner b ^ bit
+            tot[i] += __shfl_xor(nb, bit);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < D; ++i) cc[p * D + i] = tot[i];
+}
+
+// per block node: an owner sums the corner entries of its incident cells
+// and of its copies' incident cells; a copy receives 0
+template <int D>
+__global__ __launch_bounds__(kThreads) void topo_node_cc_kernel(const TopoView t, const double* cc, double* out)
+{
+    constexpr int NC = 1 << D;
+    const long long n = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (n >= t.n_all || (t.halt && *t.halt != 0.0)) return;
+    double acc[D] = {};
+    if (t.owner[n] == n) {
+        for (int e = -1; e < t.grp_off[n + 1] - t.grp_off[n]; ++e) {
+            const long long m = e < 0 ? n : (long long)t.grp_mem[t.grp_off[n] + e];
+            const TopoBlock& B = t.blk[find_seg(t.npfx, t.nb, m)];
+            int I[3];
+            unravel_node<D>(B.g, m - B.node0, I);
+            for (int o = 0; o < NC; ++o) {
+                int I0[3] = {0, 0, 0};
+                bool ok = true;
+#pragma unroll
+                for (int k = 0; k < D; ++k) {
+                    I0[k] = I[k] - ((o >> (D - 1 - k)) & 1);
+                    ok = ok && I0[k] >= 0 && I0[k] <= B.g.n[k] - 2;
+                }
+                if (!ok) continue;
+                const long long cell = D == 2 ? (long long)I0[0] * (B.g.n[1] - 1) + I0[1]
+                                              : ((long long)I0[0] * (B.g.n[1] - 1) + I0[1]) * (B.g.n[2] - 1) + I0[2];
+                const long long q = B.samp0 + cell * NC + o;
+#pragma unroll
+                for (int i = 0; i < D; ++i) acc[i] += cc[q * D + i];
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < D; ++i) out[n * D + i] = acc[i];
+}
+
 // one workgroup per stacked control
 template <int D, int MODE>
 __global__ __launch_bounds__(kThreads) void topo_pullback_kernel(const TopoView t, const double* y, double* out)
@@ -1256,6 +1354,33 @@ void launch_qpcg_update(int dim, bool block, long long nq, const double* P, doub
         qpcg_update_kernel<2, true><<<1, 1024, 0, st>>>(nq, P, lam, max_iter, x, r, z, p, Ap, s);
     else
         qpcg_update_kernel<3, true><<<1, 1024, 0, st>>>(nq, P, lam, max_iter, x, r, z, p, Ap, s);
+}
+
+void launch_topo_cell(const TopoView& t, bool hv, const double* gs, const double* hs, const double* dir, double* cc,
+                      hipStream_t s)
+{
+    if (t.ns_all <= 0) return;
+    const unsigned nb = blocks_for(t.ns_all);
+    if (t.dim == 2) {
+        if (hv)
+            topo_cell_kernel<2, true><<<nb, kThreads, 0, s>>>(t, gs, hs, dir, cc);
+        else
+            topo_cell_kernel<2, false><<<nb, kThreads, 0, s>>>(t, gs, hs, dir, cc);
+    } else {
+        if (hv)
+            topo_cell_kernel<3, true><<<nb, kThreads, 0, s>>>(t, gs, hs, dir, cc);
+        else
+            topo_cell_kernel<3, false><<<nb, kThreads, 0, s>>>(t, gs, hs, dir, cc);
+    }
+}
+
+void launch_topo_node_cc(const TopoView& t, const double* cc, double* out, hipStream_t s)
+{
+    if (t.n_all <= 0) return;
+    if (t.dim == 2)
+        topo_node_cc_kernel<2><<<blocks_for(t.n_all), kThreads, 0, s>>>(t, cc, out);
+    else
+        topo_node_cc_kernel<3><<<blocks_for(t.n_all), kThreads, 0, s>>>(t, cc, out);
 }
 
 }  // namespace eggmi

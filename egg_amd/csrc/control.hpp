@@ -104,6 +104,15 @@ void launch_topo_samples(const TopoView& t, int kind, const double* xb, double* 
 void launch_topo_sample_hv(const TopoView& t, const double* hs, const double* dir, double* zs, hipStream_t s);
 void launch_topo_node(const TopoView& t, int mode, const double* gs, const double* hs, double* out, hipStream_t s);
 void launch_topo_pullback(const TopoView& t, int mode, const double* y, double* out, hipStream_t s);
+// The owner gather of the D-valued node field in two steps: per sample its
+// Hessian action (hv) or metric gradient summed into the cell's corners
+// inside the cell's lanes (cc: D values per cell corner, in the sample index
+// space), then per node the corners of its incident cells and its copies'
+// (copies receive 0). The same field as launch_topo_node mode 0 without the
+// per-sample intermediate.
+void launch_topo_cell(const TopoView& t, bool hv, const double* gs, const double* hs, const double* dir, double* cc,
+                      hipStream_t s);
+void launch_topo_node_cc(const TopoView& t, const double* cc, double* out, hipStream_t s);
 // Energy and min det over every block's cells at xb + a0 dir and xb + a1 dir
 // (as launch_ctrl_energy).
 void launch_topo_energy(const TopoView& t, int kind, const double* xb, const double* dir, double a0, double a1,

@@ -364,8 +364,8 @@ ControlReport Session::run_control_topo(const ControlOptions& o, int kind)
     auto matvec = [&](double lam, const double* v, double* out) {
         launch_csr_apply(nsd, c.r_off.p, c.r_col.p, c.r_coef.p, v, 0.0, nullptr, c.vfull.p, stream);
         launch_topo_eval(tvh, true, c.vfull.p, c.dir.p, nullptr, stream);
-        launch_topo_sample_hv(tvh, c.hs.p, c.dir.p, c.zs.p, stream);
-        launch_topo_node(tvh, 0, c.zs.p, nullptr, c.ynode.p, stream);
+        launch_topo_cell(tvh, true, nullptr, c.hs.p, c.dir.p, c.zs.p, stream);
+        launch_topo_node_cc(tvh, c.zs.p, c.ynode.p, stream);
         launch_topo_pullback(tvh, 0, c.ynode.p, c.wfull.p, stream);
         pen_accum(c.vfull.p);
         launch_csr_apply(nq, c.g_off.p, c.g_row.p, c.g_coef.p, c.wfull.p, lam, lam > 0.0 ? v : nullptr, out, stream);
@@ -424,7 +424,8 @@ ControlReport Session::run_control_topo(const ControlOptions& o, int kind)
     for (int outer = 0; outer < o.n_outer; ++outer) {
         // reduced gradient qG = R^T (M^T g + P^T W P C)
         launch_topo_samples(tv, kind, c.xb.p, c.gs.p, c.hs.p, stream);
-        launch_topo_node(tv, 0, c.gs.p, nullptr, c.ynode.p, stream);
+        launch_topo_cell(tv, false, c.gs.p, nullptr, nullptr, c.zs.p, stream);
+        launch_topo_node_cc(tv, c.zs.p, c.ynode.p, stream);
         launch_topo_pullback(tv, 0, c.ynode.p, c.wfull.p, stream);
         pen_accum(c.C.p);
         launch_csr_apply(nq, c.g_off.p, c.g_row.p, c.g_coef.p, c.wfull.p, 0.0, nullptr, c.qG.p, stream);
