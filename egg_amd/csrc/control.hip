@@ -247,41 +247,6 @@ __global__ __launch_bounds__(kThreads) void ctrl_samples_kernel(const CtrlGeom g
     if (p < ns) sample_derivs<D, KIND>(g, p, xb, gs, hs, ns, p);
 }
 
-// z = H dA for block-local sample p (dA from the block-local field dir),
-// the Hessian read and z written at index q of an ns-sample space
-template <int D>
-__device__ __forceinline__ void sample_hv_at(const CtrlGeom& g, long long p, const double* hs, const double* dir,
-                                             double* zs, long long ns, long long q)
-{
-    constexpr int NC = 1 << D, V = kV<D>;
-    const long long cell = p / NC;
-    const int b = int(p - cell * NC);
-    int I0[3];
-    cell_origin<D>(g, cell, I0);
-    double dA[V];
-    sample_A<D>(g, I0, b, dir, nullptr, 0.0, dA);
-    double H[kNHs<D>];
-#pragma unroll
-    for (int e = 0; e < kNHs<D>; ++e) H[e] = hs[(long long)e * ns + q];
-#pragma unroll
-    for (int i = 0; i < V; ++i) {
-        double y = 0.0;
-#pragma unroll
-        for (int r = 0; r < V; ++r) y += H[sym_index<D>(i, r)] * dA[r];
-        zs[(long long)i * ns + q] = y;
-    }
-}
-
-// z_p = H_p dA_p per sample, dA_p from the block-local direction field
-template <int D>
-__global__ __launch_bounds__(kThreads) void ctrl_sample_hv_kernel(const CtrlGeom g, const double* hs,
-                                                                  const double* dir, double* zs)
-{
-    const long long p = (long long)blockIdx.x * kThreads + threadIdx.x;
-    const long long ns = g.n_cells * (1 << D);
-    if (p < ns && !(g.halt && *g.halt != 0.0)) sample_hv_at<D>(g, p, hs, dir, zs, ns, p);
-}
-
 // energy and det of block-local sample p at xb + a dir
 template <int D, int KIND>
 __device__ __forceinline__ void sample_energy(const CtrlGeom& g, long long p, const double* xb, const double* dir,
@@ -589,16 +554,6 @@ __global__ __launch_bounds__(kThreads) void topo_samples_kernel(const TopoView t
     sample_derivs<D, KIND>(B.g, p - B.samp0, xb + B.node0 * D, gs, hs, t.ns_all, p);
 }
 
-template <int D>
-__global__ __launch_bounds__(kThreads) void topo_sample_hv_kernel(const TopoView t, const double* hs,
-                                                                  const double* dir, double* zs)
-{
-    const long long p = (long long)blockIdx.x * kThreads + threadIdx.x;
-    if (p >= t.ns_all || (t.halt && *t.halt != 0.0)) return;
-    const TopoBlock& B = t.blk[find_seg(t.spfx, t.nb, p)];
-    sample_hv_at<D>(B.g, p - B.samp0, hs, dir + B.node0 * D, zs, t.ns_all, p);
-}
-
 // per block node: an owner gathers the incident samples of itself and of
 // every copy it owns (their blocks' cells); a copy receives 0 (the adjoint
 // of the owner-to-copy broadcast)
@@ -628,30 +583,27 @@ __global__ __launch_bounds__(kThreads) void topo_node_kernel(const TopoView t, c
 // corner node sum_k u_k z_(i,k) (u_k = -s_k) and its axis-k neighbour
 // (corner b ^ bit k) s_k z_(i,k); lane b adds its own term and what the
 // lanes b ^ bit k give their axis-k neighbour. z is the sample's Hessian
-// action H dA (HV) or its metric gradient. cc[(p) D + i]: corner b of the
-// cell of sample p (the sample index space, D values each).
+// action H dA (HV, dA from the block-local field dir) or its metric
+// gradient; the sample is pl of block geometry g, at index p of the
+// ns-sample component-major arrays. cc[p D + i]: corner b of the sample's
+// cell. Every lane of a cell calls this together (the sample count and
+// kThreads are multiples of 2^D, and a block's first sample is too).
+static_assert(kThreads % 8 == 0, "a cell's lanes share a workgroup");
 template <int D, bool HV>
-__global__ __launch_bounds__(kThreads) void topo_cell_kernel(const TopoView t, const double* gs, const double* hs,
-                                                             const double* dir, double* cc)
+__device__ __forceinline__ void cell_corners(const CtrlGeom& g, long long pl, const double* gs, const double* hs,
+                                             const double* dir, long long ns, long long p, double* cc)
 {
     constexpr int NC = 1 << D, V = kV<D>;
-    if (t.halt && *t.halt != 0.0) return;  // uniform over the launch
-    const long long p = (long long)blockIdx.x * kThreads + threadIdx.x;
-    // the sample count is a multiple of 2^D and so is kThreads: a cell's
-    // lanes are all in range or all out of it, and leave together
-    if (p >= t.ns_all) return;
+    const int b = int(pl & (NC - 1));
     double z[V];
     if constexpr (HV) {
-        const TopoBlock& B = t.blk[find_seg(t.spfx, t.nb, p)];
-        const long long pl = p - B.samp0, cell = pl / NC;
-        const int b = int(pl - cell * NC);
         int I0[3];
-        cell_origin<D>(B.g, cell, I0);
+        cell_origin<D>(g, pl / NC, I0);
         double dA[V];
-        sample_A<D>(B.g, I0, b, dir + B.node0 * D, nullptr, 0.0, dA);
+        sample_A<D>(g, I0, b, dir, nullptr, 0.0, dA);
         double H[kNHs<D>];
 #pragma unroll
-        for (int e = 0; e < kNHs<D>; ++e) H[e] = hs[(long long)e * t.ns_all + p];
+        for (int e = 0; e < kNHs<D>; ++e) H[e] = hs[(long long)e * ns + p];
 #pragma unroll
         for (int i = 0; i < V; ++i) {
             double y = 0.0;
@@ -661,9 +613,8 @@ __global__ __launch_bounds__(kThreads) void topo_cell_kernel(const TopoView t, c
         }
     } else {
 #pragma unroll
-        for (int i = 0; i < V; ++i) z[i] = gs[(long long)i * t.ns_all + p];
+        for (int i = 0; i < V; ++i) z[i] = gs[(long long)i * ns + p];
     }
-    const int b = int(p & (NC - 1));  // samp0 of every block is a multiple of 2^D
     double tot[D];
 #pragma unroll
     for (int i = 0; i < D; ++i) {
@@ -685,12 +636,69 @@ __global__ __launch_bounds__(kThreads) void topo_cell_kernel(const TopoView t, c
     for (int i = 0; i < D; ++i) cc[p * D + i] = tot[i];
 }
 
+// add the corner entries of block-local node I's incident cells (the
+// block's samples from index s0 on) to acc
+template <int D>
+__device__ __forceinline__ void node_corners(const CtrlGeom& g, const int* I, const double* cc, long long s0,
+                                             double* acc)
+{
+    constexpr int NC = 1 << D;
+    for (int o = 0; o < NC; ++o) {
+        int I0[3] = {0, 0, 0};
+        bool ok = true;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            I0[k] = I[k] - ((o >> (D - 1 - k)) & 1);
+            ok = ok && I0[k] >= 0 && I0[k] <= g.n[k] - 2;
+        }
+        if (!ok) continue;
+        const long long cell = D == 2 ? (long long)I0[0] * (g.n[1] - 1) + I0[1]
+                                      : ((long long)I0[0] * (g.n[1] - 1) + I0[1]) * (g.n[2] - 1) + I0[2];
+        const long long q = s0 + cell * NC + o;
+#pragma unroll
+        for (int i = 0; i < D; ++i) acc[i] += cc[q * D + i];
+    }
+}
+
+template <int D, bool HV>
+__global__ __launch_bounds__(kThreads) void ctrl_cell_kernel(const CtrlGeom g, const double* gs, const double* hs,
+                                                             const double* dir, double* cc)
+{
+    if (g.halt && *g.halt != 0.0) return;  // uniform over the launch
+    const long long p = (long long)blockIdx.x * kThreads + threadIdx.x;
+    const long long ns = g.n_cells * (1 << D);
+    if (p < ns) cell_corners<D, HV>(g, p, gs, hs, dir, ns, p, cc);  // whole cells in or out
+}
+
+template <int D>
+__global__ __launch_bounds__(kThreads) void ctrl_node_cc_kernel(const CtrlGeom g, const double* cc, double* out)
+{
+    const long long n = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (n >= g.n_nodes || (g.halt && *g.halt != 0.0)) return;
+    int I[3];
+    unravel_node<D>(g, n, I);
+    double acc[D] = {};
+    node_corners<D>(g, I, cc, 0, acc);
+#pragma unroll
+    for (int i = 0; i < D; ++i) out[n * D + i] = acc[i];
+}
+
+template <int D, bool HV>
+__global__ __launch_bounds__(kThreads) void topo_cell_kernel(const TopoView t, const double* gs, const double* hs,
+                                                             const double* dir, double* cc)
+{
+    if (t.halt && *t.halt != 0.0) return;  // uniform over the launch
+    const long long p = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (p >= t.ns_all) return;  // whole cells in or out
+    const TopoBlock& B = t.blk[find_seg(t.spfx, t.nb, p)];
+    cell_corners<D, HV>(B.g, p - B.samp0, gs, hs, dir ? dir + B.node0 * D : nullptr, t.ns_all, p, cc);
+}
+
 // per block node: an owner sums the corner entries of its incident cells
 // and of its copies' incident cells; a copy receives 0
 template <int D>
 __global__ __launch_bounds__(kThreads) void topo_node_cc_kernel(const TopoView t, const double* cc, double* out)
 {
-    constexpr int NC = 1 << D;
     const long long n = (long long)blockIdx.x * kThreads + threadIdx.x;
     if (n >= t.n_all || (t.halt && *t.halt != 0.0)) return;
     double acc[D] = {};
@@ -700,21 +708,7 @@ __global__ __launch_bounds__(kThreads) void topo_node_cc_kernel(const TopoView t
             const TopoBlock& B = t.blk[find_seg(t.npfx, t.nb, m)];
             int I[3];
             unravel_node<D>(B.g, m - B.node0, I);
-            for (int o = 0; o < NC; ++o) {
-                int I0[3] = {0, 0, 0};
-                bool ok = true;
-#pragma unroll
-                for (int k = 0; k < D; ++k) {
-                    I0[k] = I[k] - ((o >> (D - 1 - k)) & 1);
-                    ok = ok && I0[k] >= 0 && I0[k] <= B.g.n[k] - 2;
-                }
-                if (!ok) continue;
-                const long long cell = D == 2 ? (long long)I0[0] * (B.g.n[1] - 1) + I0[1]
-                                              : ((long long)I0[0] * (B.g.n[1] - 1) + I0[1]) * (B.g.n[2] - 1) + I0[2];
-                const long long q = B.samp0 + cell * NC + o;
-#pragma unroll
-                for (int i = 0; i < D; ++i) acc[i] += cc[q * D + i];
-            }
+            node_corners<D>(B.g, I, cc, B.samp0, acc);
         }
     }
 #pragma unroll
@@ -1110,14 +1104,32 @@ void launch_ctrl_samples(const CtrlGeom& g, int kind, const double* x, double* g
     }
 }
 
-void launch_ctrl_sample_hv(const CtrlGeom& g, const double* hs, const double* dir, double* zs, hipStream_t s)
+void launch_ctrl_cell(const CtrlGeom& g, bool hv, const double* gs, const double* hs, const double* dir, double* cc,
+                      hipStream_t s)
 {
     const long long n = g.n_cells * (1LL << g.dim);
     if (n <= 0) return;
+    const unsigned nb = blocks_for(n);
+    if (g.dim == 2) {
+        if (hv)
+            ctrl_cell_kernel<2, true><<<nb, kThreads, 0, s>>>(g, gs, hs, dir, cc);
+        else
+            ctrl_cell_kernel<2, false><<<nb, kThreads, 0, s>>>(g, gs, hs, dir, cc);
+    } else {
+        if (hv)
+            ctrl_cell_kernel<3, true><<<nb, kThreads, 0, s>>>(g, gs, hs, dir, cc);
+        else
+            ctrl_cell_kernel<3, false><<<nb, kThreads, 0, s>>>(g, gs, hs, dir, cc);
+    }
+}
+
+void launch_ctrl_node_cc(const CtrlGeom& g, const double* cc, double* out, hipStream_t s)
+{
+    if (g.n_nodes <= 0) return;
     if (g.dim == 2)
-        ctrl_sample_hv_kernel<2><<<blocks_for(n), kThreads, 0, s>>>(g, hs, dir, zs);
+        ctrl_node_cc_kernel<2><<<blocks_for(g.n_nodes), kThreads, 0, s>>>(g, cc, out);
     else
-        ctrl_sample_hv_kernel<3><<<blocks_for(n), kThreads, 0, s>>>(g, hs, dir, zs);
+        ctrl_node_cc_kernel<3><<<blocks_for(g.n_nodes), kThreads, 0, s>>>(g, cc, out);
 }
 
 void launch_ctrl_node(const CtrlGeom& g, int mode, const double* gs, const double* hs, const double* dir,
@@ -1237,15 +1249,6 @@ void launch_topo_samples(const TopoView& t, int kind, const double* xb, double* 
             EGGMI_TS(3, kShape);
     }
 #undef EGGMI_TS
-}
-
-void launch_topo_sample_hv(const TopoView& t, const double* hs, const double* dir, double* zs, hipStream_t s)
-{
-    if (t.ns_all <= 0) return;
-    if (t.dim == 2)
-        topo_sample_hv_kernel<2><<<blocks_for(t.ns_all), kThreads, 0, s>>>(t, hs, dir, zs);
-    else
-        topo_sample_hv_kernel<3><<<blocks_for(t.ns_all), kThreads, 0, s>>>(t, hs, dir, zs);
 }
 
 void launch_topo_node(const TopoView& t, int mode, const double* gs, const double* hs, double* out, hipStream_t s)

@@ -97,11 +97,10 @@ void launch_topo_load(const TopoView& t, const float* x, double* out, hipStream_
 void launch_topo_eval(const TopoView& t, bool dir, const double* src, double* out, float* out_plane, hipStream_t s);
 // The per-sample, node and pullback stages of the single-block path over
 // the concatenated spaces: samples (gs, hs component-major over ns_all),
-// Hessian action, owner gathers (mode 0 with gs: D per node; mode 2 with
-// hs: D x D; copies receive 0) and the per-control pullback into the
-// stacked vector (mode 0: D, mode 1: the Jacobi blocks D x D).
+// owner gathers (mode 0 with gs: D per node; mode 2 with hs: D x D; copies
+// receive 0) and the per-control pullback into the stacked vector (mode 0:
+// D, mode 1: the Jacobi blocks D x D).
 void launch_topo_samples(const TopoView& t, int kind, const double* xb, double* gs, double* hs, hipStream_t s);
-void launch_topo_sample_hv(const TopoView& t, const double* hs, const double* dir, double* zs, hipStream_t s);
 void launch_topo_node(const TopoView& t, int mode, const double* gs, const double* hs, double* out, hipStream_t s);
 void launch_topo_pullback(const TopoView& t, int mode, const double* y, double* out, hipStream_t s);
 // The owner gather of the D-valued node field in two steps: per sample its
@@ -165,7 +164,15 @@ void launch_ctrl_energy(const CtrlGeom& g, int kind, const double* xb, const dou
                         double* partials, double* out, hipStream_t s);
 // Per sample: zs (dim^2) = H_p dA_p, dA_p the sample's A of the direction
 // field dir (block-local node order).
-void launch_ctrl_sample_hv(const CtrlGeom& g, const double* hs, const double* dir, double* zs, hipStream_t s);
+// Per sample: its Hessian action H_p dA_p (hv; dA_p from the block-local
+// direction field dir) or its metric gradient (gs), summed into the cell's
+// corners inside the cell's lanes (cc: D values per cell corner, in the
+// sample index space); then per block node the corner entries of its
+// incident cells: the node field of the energy gradient or of the Hessian
+// action, without a per-sample intermediate.
+void launch_ctrl_cell(const CtrlGeom& g, bool hv, const double* gs, const double* hs, const double* dir, double* cc,
+                      hipStream_t s);
+void launch_ctrl_node_cc(const CtrlGeom& g, const double* cc, double* out, hipStream_t s);
 // Per block node (block-local order), gathered over its incident samples:
 // mode 0: sum over samples of the node's column weights times gs (the
 // energy gradient for the metric gradients, the Hessian action for zs);

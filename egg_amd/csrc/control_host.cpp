@@ -209,8 +209,8 @@ ControlReport Session::run_control(const ControlOptions& o, int kind)
     gh.halt = c.scal.p + 6;
     auto matvec = [&](double lam, const double* v, double* out) {
         launch_ctrl_eval_dir(gh, v, c.dir.p, nullptr, stream);
-        launch_ctrl_sample_hv(gh, c.hs.p, c.dir.p, c.zs.p, stream);
-        launch_ctrl_node(gh, 0, c.zs.p, nullptr, nullptr, c.ynode.p, stream);
+        launch_ctrl_cell(gh, true, nullptr, c.hs.p, c.dir.p, c.zs.p, stream);
+        launch_ctrl_node_cc(gh, c.zs.p, c.ynode.p, stream);
         launch_ctrl_pullback(gh, 0, c.ynode.p, out, stream);
         if (lam > 0.0) launch_ctrl_axpy(nfd, lam, v, out, false, stream);
     };
@@ -275,7 +275,8 @@ ControlReport Session::run_control(const ControlOptions& o, int kind)
     for (int outer = 0; outer < o.n_outer; ++outer) {
         // reduced gradient G = M_f^T g from the per-sample derivatives
         launch_ctrl_samples(g, kind, c.xb.p, c.gs.p, c.hs.p, stream);
-        launch_ctrl_node(g, 0, c.gs.p, c.hs.p, nullptr, c.ynode.p, stream);
+        launch_ctrl_cell(g, false, c.gs.p, nullptr, nullptr, c.zs.p, stream);
+        launch_ctrl_node_cc(g, c.zs.p, c.ynode.p, stream);
         launch_ctrl_pullback(g, 0, c.ynode.p, c.G.p, stream);
         launch_ctrl_reduce(1, c.G.p, nullptr, nfd, dim, c.scal.p, stream);
         const double gmax = scalar(0);
