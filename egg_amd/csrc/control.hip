@@ -332,90 +332,6 @@ __global__ __launch_bounds__(kThreads) void ctrl_energy_finish(const double* par
 
 // ---------------------------------------------------------------- node gathers
 
-// add to acc what block-local node I receives from the block's incident
-// samples: mode 0 the column weights times gs (D values), mode 2 the
-// contracted Hessian from hs (D x D); the block's sample p sits at s0 + p
-// of an ns-sample component-major space
-template <int D, int MODE>
-__device__ __forceinline__ void node_acc(const CtrlGeom& g, const int* I, const double* gs, const double* hs,
-                                         long long ns, long long s0, double* acc)
-{
-    constexpr int NC = 1 << D;
-    // incident cells: origin I - o for o in {0, 1}^D, the node at corner o
-    for (int o = 0; o < NC; ++o) {
-        int I0[3] = {0, 0, 0};
-        bool ok = true;
-#pragma unroll
-        for (int k = 0; k < D; ++k) {
-            I0[k] = I[k] - ((o >> (D - 1 - k)) & 1);
-            ok = ok && I0[k] >= 0 && I0[k] <= g.n[k] - 2;
-        }
-        if (!ok) continue;
-        const long long cell = D == 2 ? (long long)I0[0] * (g.n[1] - 1) + I0[1]
-                                      : ((long long)I0[0] * (g.n[1] - 1) + I0[1]) * (g.n[2] - 1) + I0[2];
-        // the node's role in sample (cell, b): the corner (b = o, weights
-        // u_k = -s_k on every axis) or the axis-k neighbour (b = o ^ bit k,
-        // the single weight u_k = s_k; s_k = +-1, so u_k^2 = 1)
-        {
-            const int b = o;
-            const long long p = s0 + cell * NC + b;
-            double u[D];
-#pragma unroll
-            for (int k = 0; k < D; ++k) u[k] = ((b >> (D - 1 - k)) & 1) ? 1.0 : -1.0;
-            if constexpr (MODE == 0) {
-#pragma unroll
-                for (int i = 0; i < D; ++i)
-#pragma unroll
-                    for (int k = 0; k < D; ++k) acc[i] += u[k] * gs[(long long)(i * D + k) * ns + p];
-            } else {
-#pragma unroll
-                for (int i = 0; i < D; ++i)
-#pragma unroll
-                    for (int j = 0; j < D; ++j) {
-                        double h = 0.0;
-#pragma unroll
-                        for (int k = 0; k < D; ++k)
-#pragma unroll
-                            for (int l = 0; l < D; ++l)
-                                h += u[k] * u[l] * hs[(long long)sym_index<D>(i * D + k, j * D + l) * ns + p];
-                        acc[i * D + j] += h;
-                    }
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < D; ++k) {
-            const int bit = 1 << (D - 1 - k);
-            const int b = o ^ bit;
-            const long long p = s0 + cell * NC + b;
-            if constexpr (MODE == 0) {
-                const double uk = (b & bit) ? -1.0 : 1.0;
-#pragma unroll
-                for (int i = 0; i < D; ++i) acc[i] += uk * gs[(long long)(i * D + k) * ns + p];
-            } else {
-#pragma unroll
-                for (int i = 0; i < D; ++i)
-#pragma unroll
-                    for (int j = 0; j < D; ++j) acc[i * D + j] += hs[(long long)sym_index<D>(i * D + k, j * D + k) * ns + p];
-            }
-        }
-    }
-}
-
-template <int D, int MODE>
-__global__ __launch_bounds__(kThreads) void ctrl_node_kernel(const CtrlGeom g, const double* gs, const double* hs,
-                                                             const double* dir, double* out)
-{
-    constexpr int NO = MODE == 2 ? D * D : D;
-    const long long n = (long long)blockIdx.x * kThreads + threadIdx.x;
-    if (n >= g.n_nodes || (g.halt && *g.halt != 0.0)) return;
-    int I[3];
-    unravel_node<D>(g, n, I);
-    double acc[NO] = {};
-    node_acc<D, MODE>(g, I, gs, hs, g.n_cells * (1 << D), 0, acc);
-#pragma unroll
-    for (int c = 0; c < NO; ++c) out[n * NO + c] = acc[c];
-}
-
 // ---------------------------------------------------------------- pullback
 
 // one workgroup per control (lattice index a): its threads split the
@@ -554,30 +470,6 @@ __global__ __launch_bounds__(kThreads) void topo_samples_kernel(const TopoView t
     sample_derivs<D, KIND>(B.g, p - B.samp0, xb + B.node0 * D, gs, hs, t.ns_all, p);
 }
 
-// per block node: an owner gathers the incident samples of itself and of
-// every copy it owns (their blocks' cells); a copy receives 0 (the adjoint
-// of the owner-to-copy broadcast)
-template <int D, int MODE>
-__global__ __launch_bounds__(kThreads) void topo_node_kernel(const TopoView t, const double* gs, const double* hs,
-                                                             double* out)
-{
-    constexpr int NO = MODE == 2 ? D * D : D;
-    const long long n = (long long)blockIdx.x * kThreads + threadIdx.x;
-    if (n >= t.n_all || (t.halt && *t.halt != 0.0)) return;
-    double acc[NO] = {};
-    if (t.owner[n] == n) {
-        for (int e = -1; e < t.grp_off[n + 1] - t.grp_off[n]; ++e) {
-            const long long m = e < 0 ? n : (long long)t.grp_mem[t.grp_off[n] + e];
-            const TopoBlock& B = t.blk[find_seg(t.npfx, t.nb, m)];
-            int I[3];
-            unravel_node<D>(B.g, m - B.node0, I);
-            node_acc<D, MODE>(B.g, I, gs, hs, t.ns_all, B.samp0, acc);
-        }
-    }
-#pragma unroll
-    for (int c = 0; c < NO; ++c) out[n * NO + c] = acc[c];
-}
-
 // Per sample, its contributions to the cell's corner nodes, summed per
 // corner inside the cell's 2^D consecutive lanes: sample b gives its
 // corner node sum_k u_k z_(i,k) (u_k = -s_k) and its axis-k neighbour
@@ -636,9 +528,48 @@ __device__ __forceinline__ void cell_corners(const CtrlGeom& g, long long pl, co
     for (int i = 0; i < D; ++i) cc[p * D + i] = tot[i];
 }
 
+// The contracted node Hessians the same way: sample b gives its corner
+// sum_kl u_k u_l H_(ik,jl) and its axis-k neighbour H_(ik,jk) (u_k^2 = 1),
+// D x D values each; cc[p D^2 + i D + j]: corner b of the sample's cell.
+template <int D>
+__device__ __forceinline__ void cell_corner_hessians(long long pl, const double* hs, long long ns, long long p,
+                                                     double* cc)
+{
+    constexpr int NC = 1 << D, V = kV<D>;
+    const int b = int(pl & (NC - 1));
+    double H[kNHs<D>];
+#pragma unroll
+    for (int e = 0; e < kNHs<D>; ++e) H[e] = hs[(long long)e * ns + p];
+    double u[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) u[k] = ((b >> (D - 1 - k)) & 1) ? 1.0 : -1.0;
+    double tot[V];
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            double h = 0.0;
+#pragma unroll
+            for (int k = 0; k < D; ++k)
+#pragma unroll
+                for (int l = 0; l < D; ++l) h += u[k] * u[l] * H[sym_index<D>(i * D + k, j * D + l)];
+            tot[i * D + j] = h;
+        }
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        const int bit = 1 << (D - 1 - k);
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+#pragma unroll
+            for (int j = 0; j < D; ++j) tot[i * D + j] += __shfl_xor(H[sym_index<D>(i * D + k, j * D + k)], bit);
+    }
+#pragma unroll
+    for (int e = 0; e < V; ++e) cc[p * V + e] = tot[e];
+}
+
 // add the corner entries of block-local node I's incident cells (the
 // block's samples from index s0 on) to acc
-template <int D>
+template <int D, int NO = D>
 __device__ __forceinline__ void node_corners(const CtrlGeom& g, const int* I, const double* cc, long long s0,
                                              double* acc)
 {
@@ -656,7 +587,7 @@ __device__ __forceinline__ void node_corners(const CtrlGeom& g, const int* I, co
                                       : ((long long)I0[0] * (g.n[1] - 1) + I0[1]) * (g.n[2] - 1) + I0[2];
         const long long q = s0 + cell * NC + o;
 #pragma unroll
-        for (int i = 0; i < D; ++i) acc[i] += cc[q * D + i];
+        for (int i = 0; i < NO; ++i) acc[i] += cc[q * NO + i];
     }
 }
 
@@ -671,16 +602,24 @@ __global__ __launch_bounds__(kThreads) void ctrl_cell_kernel(const CtrlGeom g, c
 }
 
 template <int D>
+__global__ __launch_bounds__(kThreads) void ctrl_cell_hess_kernel(const CtrlGeom g, const double* hs, double* cc)
+{
+    const long long p = (long long)blockIdx.x * kThreads + threadIdx.x;
+    const long long ns = g.n_cells * (1 << D);
+    if (p < ns) cell_corner_hessians<D>(p, hs, ns, p, cc);  // whole cells in or out
+}
+
+template <int D, int NO>
 __global__ __launch_bounds__(kThreads) void ctrl_node_cc_kernel(const CtrlGeom g, const double* cc, double* out)
 {
     const long long n = (long long)blockIdx.x * kThreads + threadIdx.x;
     if (n >= g.n_nodes || (g.halt && *g.halt != 0.0)) return;
     int I[3];
     unravel_node<D>(g, n, I);
-    double acc[D] = {};
-    node_corners<D>(g, I, cc, 0, acc);
+    double acc[NO] = {};
+    node_corners<D, NO>(g, I, cc, 0, acc);
 #pragma unroll
-    for (int i = 0; i < D; ++i) out[n * D + i] = acc[i];
+    for (int i = 0; i < NO; ++i) out[n * NO + i] = acc[i];
 }
 
 template <int D, bool HV>
@@ -694,25 +633,34 @@ __global__ __launch_bounds__(kThreads) void topo_cell_kernel(const TopoView t, c
     cell_corners<D, HV>(B.g, p - B.samp0, gs, hs, dir ? dir + B.node0 * D : nullptr, t.ns_all, p, cc);
 }
 
-// per block node: an owner sums the corner entries of its incident cells
-// and of its copies' incident cells; a copy receives 0
 template <int D>
+__global__ __launch_bounds__(kThreads) void topo_cell_hess_kernel(const TopoView t, const double* hs, double* cc)
+{
+    const long long p = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (p >= t.ns_all) return;  // whole cells in or out
+    const TopoBlock& B = t.blk[find_seg(t.spfx, t.nb, p)];
+    cell_corner_hessians<D>(p - B.samp0, hs, t.ns_all, p, cc);
+}
+
+// per block node: an owner sums the corner entries (NO values each) of its
+// incident cells and of its copies' incident cells; a copy receives 0
+template <int D, int NO>
 __global__ __launch_bounds__(kThreads) void topo_node_cc_kernel(const TopoView t, const double* cc, double* out)
 {
     const long long n = (long long)blockIdx.x * kThreads + threadIdx.x;
     if (n >= t.n_all || (t.halt && *t.halt != 0.0)) return;
-    double acc[D] = {};
+    double acc[NO] = {};
     if (t.owner[n] == n) {
         for (int e = -1; e < t.grp_off[n + 1] - t.grp_off[n]; ++e) {
             const long long m = e < 0 ? n : (long long)t.grp_mem[t.grp_off[n] + e];
             const TopoBlock& B = t.blk[find_seg(t.npfx, t.nb, m)];
             int I[3];
             unravel_node<D>(B.g, m - B.node0, I);
-            node_corners<D>(B.g, I, cc, B.samp0, acc);
+            node_corners<D, NO>(B.g, I, cc, B.samp0, acc);
         }
     }
 #pragma unroll
-    for (int i = 0; i < D; ++i) out[n * D + i] = acc[i];
+    for (int i = 0; i < NO; ++i) out[n * NO + i] = acc[i];
 }
 
 // one workgroup per stacked control
@@ -1127,29 +1075,22 @@ void launch_ctrl_node_cc(const CtrlGeom& g, const double* cc, double* out, hipSt
 {
     if (g.n_nodes <= 0) return;
     if (g.dim == 2)
-        ctrl_node_cc_kernel<2><<<blocks_for(g.n_nodes), kThreads, 0, s>>>(g, cc, out);
+        ctrl_node_cc_kernel<2, 2><<<blocks_for(g.n_nodes), kThreads, 0, s>>>(g, cc, out);
     else
-        ctrl_node_cc_kernel<3><<<blocks_for(g.n_nodes), kThreads, 0, s>>>(g, cc, out);
+        ctrl_node_cc_kernel<3, 3><<<blocks_for(g.n_nodes), kThreads, 0, s>>>(g, cc, out);
 }
 
-void launch_ctrl_node(const CtrlGeom& g, int mode, const double* gs, const double* hs, const double* dir,
-                      double* out, hipStream_t s)
+void launch_ctrl_node_hessians(const CtrlGeom& g, const double* hs, double* cc, double* out, hipStream_t s)
 {
+    const long long ns = g.n_cells * (1LL << g.dim);
     if (g.n_nodes <= 0) return;
-    const unsigned nb = blocks_for(g.n_nodes);
-#define EGGMI_CN(D, M) ctrl_node_kernel<D, M><<<nb, kThreads, 0, s>>>(g, gs, hs, dir, out)
     if (g.dim == 2) {
-        if (mode == 0)
-            EGGMI_CN(2, 0);
-        else
-            EGGMI_CN(2, 2);
+        if (ns > 0) ctrl_cell_hess_kernel<2><<<blocks_for(ns), kThreads, 0, s>>>(g, hs, cc);
+        ctrl_node_cc_kernel<2, 4><<<blocks_for(g.n_nodes), kThreads, 0, s>>>(g, cc, out);
     } else {
-        if (mode == 0)
-            EGGMI_CN(3, 0);
-        else
-            EGGMI_CN(3, 2);
+        if (ns > 0) ctrl_cell_hess_kernel<3><<<blocks_for(ns), kThreads, 0, s>>>(g, hs, cc);
+        ctrl_node_cc_kernel<3, 9><<<blocks_for(g.n_nodes), kThreads, 0, s>>>(g, cc, out);
     }
-#undef EGGMI_CN
 }
 
 void launch_ctrl_pullback(const CtrlGeom& g, int mode, const double* y, double* out, hipStream_t s)
@@ -1249,23 +1190,6 @@ void launch_topo_samples(const TopoView& t, int kind, const double* xb, double* 
             EGGMI_TS(3, kShape);
     }
 #undef EGGMI_TS
-}
-
-void launch_topo_node(const TopoView& t, int mode, const double* gs, const double* hs, double* out, hipStream_t s)
-{
-    if (t.n_all <= 0) return;
-    const unsigned nb = blocks_for(t.n_all);
-    if (t.dim == 2) {
-        if (mode == 0)
-            topo_node_kernel<2, 0><<<nb, kThreads, 0, s>>>(t, gs, hs, out);
-        else
-            topo_node_kernel<2, 2><<<nb, kThreads, 0, s>>>(t, gs, hs, out);
-    } else {
-        if (mode == 0)
-            topo_node_kernel<3, 0><<<nb, kThreads, 0, s>>>(t, gs, hs, out);
-        else
-            topo_node_kernel<3, 2><<<nb, kThreads, 0, s>>>(t, gs, hs, out);
-    }
 }
 
 void launch_topo_pullback(const TopoView& t, int mode, const double* y, double* out, hipStream_t s)
@@ -1381,9 +1305,21 @@ void launch_topo_node_cc(const TopoView& t, const double* cc, double* out, hipSt
 {
     if (t.n_all <= 0) return;
     if (t.dim == 2)
-        topo_node_cc_kernel<2><<<blocks_for(t.n_all), kThreads, 0, s>>>(t, cc, out);
+        topo_node_cc_kernel<2, 2><<<blocks_for(t.n_all), kThreads, 0, s>>>(t, cc, out);
     else
-        topo_node_cc_kernel<3><<<blocks_for(t.n_all), kThreads, 0, s>>>(t, cc, out);
+        topo_node_cc_kernel<3, 3><<<blocks_for(t.n_all), kThreads, 0, s>>>(t, cc, out);
+}
+
+void launch_topo_node_hessians(const TopoView& t, const double* hs, double* cc, double* out, hipStream_t s)
+{
+    if (t.n_all <= 0) return;
+    if (t.dim == 2) {
+        if (t.ns_all > 0) topo_cell_hess_kernel<2><<<blocks_for(t.ns_all), kThreads, 0, s>>>(t, hs, cc);
+        topo_node_cc_kernel<2, 4><<<blocks_for(t.n_all), kThreads, 0, s>>>(t, cc, out);
+    } else {
+        if (t.ns_all > 0) topo_cell_hess_kernel<3><<<blocks_for(t.ns_all), kThreads, 0, s>>>(t, hs, cc);
+        topo_node_cc_kernel<3, 9><<<blocks_for(t.n_all), kThreads, 0, s>>>(t, cc, out);
+    }
 }
 
 }  // namespace eggmi

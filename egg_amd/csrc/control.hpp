@@ -95,23 +95,24 @@ void launch_topo_load(const TopoView& t, const float* x, double* out, hipStream_
 // a stacked direction; 0 where the owner has no net); the store receives
 // the state at each node's own slot when out_plane is given.
 void launch_topo_eval(const TopoView& t, bool dir, const double* src, double* out, float* out_plane, hipStream_t s);
-// The per-sample, node and pullback stages of the single-block path over
-// the concatenated spaces: samples (gs, hs component-major over ns_all),
-// owner gathers (mode 0 with gs: D per node; mode 2 with hs: D x D; copies
-// receive 0) and the per-control pullback into the stacked vector (mode 0:
-// D, mode 1: the Jacobi blocks D x D).
+// The per-sample and pullback stages of the single-block path over the
+// concatenated spaces: samples (gs, hs component-major over ns_all) and the
+// per-control pullback into the stacked vector (mode 0: D, mode 1: the
+// Jacobi blocks D x D).
 void launch_topo_samples(const TopoView& t, int kind, const double* xb, double* gs, double* hs, hipStream_t s);
-void launch_topo_node(const TopoView& t, int mode, const double* gs, const double* hs, double* out, hipStream_t s);
 void launch_topo_pullback(const TopoView& t, int mode, const double* y, double* out, hipStream_t s);
 // The owner gather of the D-valued node field in two steps: per sample its
 // Hessian action (hv) or metric gradient summed into the cell's corners
 // inside the cell's lanes (cc: D values per cell corner, in the sample index
 // space), then per node the corners of its incident cells and its copies'
-// (copies receive 0). The same field as launch_topo_node mode 0 without the
-// per-sample intermediate.
+// (copies receive 0).
 void launch_topo_cell(const TopoView& t, bool hv, const double* gs, const double* hs, const double* dir, double* cc,
                       hipStream_t s);
 void launch_topo_node_cc(const TopoView& t, const double* cc, double* out, hipStream_t s);
+// The owners' contracted node Hessians (sum over incident samples of
+// u^T H u, D x D; copies 0) through per-cell corner sums (cc: D x D values
+// per cell corner).
+void launch_topo_node_hessians(const TopoView& t, const double* hs, double* cc, double* out, hipStream_t s);
 // Energy and min det over every block's cells at xb + a0 dir and xb + a1 dir
 // (as launch_ctrl_energy).
 void launch_topo_energy(const TopoView& t, int kind, const double* xb, const double* dir, double a0, double a1,
@@ -173,12 +174,10 @@ void launch_ctrl_energy(const CtrlGeom& g, int kind, const double* xb, const dou
 void launch_ctrl_cell(const CtrlGeom& g, bool hv, const double* gs, const double* hs, const double* dir, double* cc,
                       hipStream_t s);
 void launch_ctrl_node_cc(const CtrlGeom& g, const double* cc, double* out, hipStream_t s);
-// Per block node (block-local order), gathered over its incident samples:
-// mode 0: sum over samples of the node's column weights times gs (the
-// energy gradient for the metric gradients, the Hessian action for zs);
-// mode 2: the contracted node Hessian (dim^2) from hs.
-void launch_ctrl_node(const CtrlGeom& g, int mode, const double* gs, const double* hs, const double* dir,
-                      double* out, hipStream_t s);
+// The contracted node Hessians (sum over incident samples of u^T H u with
+// u the node's column weights, D x D per node) through per-cell corner sums
+// (cc: D x D values per cell corner).
+void launch_ctrl_node_hessians(const CtrlGeom& g, const double* hs, double* cc, double* out, hipStream_t s);
 // Per free control: mode 0 out (dim) = sum_nodes w * y; mode 1 out (dim^2) =
 // sum_nodes w^2 * y (dim^2 blocks: the Jacobi preconditioner).
 void launch_ctrl_pullback(const CtrlGeom& g, int mode, const double* y, double* out, hipStream_t s);

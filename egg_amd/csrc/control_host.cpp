@@ -288,7 +288,7 @@ ControlReport Session::run_control(const ControlOptions& o, int kind)
         const double pcg_tol =
             o.pcg_forcing ? std::max(o.pcg_rtol, std::min(0.5, std::sqrt(gmax / gmax0))) : o.pcg_rtol;
         // Jacobi preconditioner from the contracted node Hessians; damping floor
-        launch_ctrl_node(g, 2, c.gs.p, c.hs.p, nullptr, c.hnode.p, stream);
+        launch_ctrl_node_hessians(g, c.hs.p, c.zs.p, c.hnode.p, stream);
         launch_ctrl_pullback(g, 1, c.hnode.p, c.prec.p, stream);
         launch_ctrl_reduce(2, c.prec.p, nullptr, g.n_free, dim, c.scal.p, stream);
         const double lam_floor = 1e-8 * scalar(0) / (double)std::max<long long>(nfd, 1);

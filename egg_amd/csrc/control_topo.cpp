@@ -440,7 +440,7 @@ ControlReport Session::run_control_topo(const ControlOptions& o, int kind)
             o.pcg_forcing ? std::max(o.pcg_rtol, std::min(0.5, std::sqrt(gmax / gmax0))) : o.pcg_rtol;
         // per-control Jacobi blocks from the owners' contracted Hessians, the
         // reduced diagonal through R; damping floor from their trace
-        launch_topo_node(tv, 2, nullptr, c.hs.p, c.hnode.p, stream);
+        launch_topo_node_hessians(tv, c.hs.p, c.zs.p, c.hnode.p, stream);
         launch_topo_pullback(tv, 1, c.hnode.p, c.prec.p, stream);
         launch_qdiag_build(dim, nq, c.g_off.p, c.g_row.p, c.g_coef.p, c.prec.p, c.qdiag.p, stream);
         launch_ctrl_reduce(2, c.prec.p, nullptr, c.n_stack, dim, c.scal.p, stream);
