@@ -646,14 +646,16 @@ def bench_ogrid(args):
         log(f"plain: {n} sweeps, {t:.3f}s to converge")
 
         # FAS: V(4,4) on 3 levels (the batched workload's cycle), on every
-        # level, and V(8,8) on 4 levels with 16 coarsest sweeps (the fastest
-        # found on this grid)
-        for name, nu, lev, nuc in (("fas_v44_3_levels", 4, 3, 32), ("fas_v44_all_levels", 4, 32, 32),
-                                   ("fas_v88_4_levels_coarse16", 8, 4, 16)):
+        # level, and V(8,8) on 4 levels with 16 coarsest sweeps at omega 0.8
+        # and at 0.9 (the fastest of a sweep over nu, levels, coarsest sweeps
+        # and omega on this grid)
+        for name, nu, lev, nuc, om in (("fas_v44_3_levels", 4, 3, 32, 0.8), ("fas_v44_all_levels", 4, 32, 32, 0.8),
+                                       ("fas_v88_4_levels_coarse16", 8, 4, 16, 0.8),
+                                       ("fas_v88_4_levels_coarse16_omega09", 8, 4, 16, 0.9)):
             s.set_X(Xu)
 
-            def fas_step(chunk=5, nu=nu, lev=lev, nuc=nuc):
-                e, m = s.run_fas(chunk, nu_pre=nu, nu_post=nu, nu_coarse=nuc, omega=0.8, max_levels=lev)
+            def fas_step(chunk=5, nu=nu, lev=lev, nuc=nuc, om=om):
+                e, m = s.run_fas(chunk, nu_pre=nu, nu_post=nu, nu_coarse=nuc, omega=om, max_levels=lev)
                 return e, m, chunk
 
             n, t, E = _converge(fas_step, e_star, 60.0)
