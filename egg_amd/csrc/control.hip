@@ -210,6 +210,73 @@ __device__ __forceinline__ void sample_A(const CtrlGeom& g, const int* I0, int b
     }
 }
 
+// sign of eps_(i k m), m the third index of a 3-permutation (i != k)
+__device__ __forceinline__ double lc_sign(int i, int k) { return ((k - i + 3) % 3 == 1) ? 1.0 : -1.0; }
+
+// d^2 det / dT_x dT_z of a 3 x 3 row-major matrix m (x = 3 i + j, z = 3 k + l):
+// eps_ikm' eps_jln' m_m'n', zero on a shared row or column
+__device__ __forceinline__ double d2det3(const double* m, int x, int z)
+{
+    const int i = x / 3, j = x % 3, k = z / 3, l = z % 3;
+    if (i == k || j == l) return 0.0;
+    return lc_sign(i, k) * lc_sign(j, l) * m[(3 - i - k) * 3 + (3 - j - l)];
+}
+
+// The 3D barrier-phase metrics' gradient and Hessian in closed form:
+// mu = s q / (9 D^2) - 1 (+ (D - 1)^2 for shape_size), s = |T|^2, q =
+// |cof T|^2, D = det T. With K = d^2 D (symmetric, K_xz = d2det3(T)),
+// grad D = cof T = c, grad q = 2 K c and hess q = 2 (K K + M), M_xz =
+// d2det3(c) (the third derivatives of det contracted with c); the product
+// and quotient rules give the rest. Stored as sample_derivs stores the
+// hyper-dual results (gs: 9 entries, hs: the 45-entry upper triangle).
+template <int KIND>
+__device__ __forceinline__ void mu3_derivs_closed(const double* t, double* gs, double* hs, long long ns, long long q)
+{
+    double c[9];
+    cof3(t, c);
+    double s = 0.0, qq = 0.0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        s += t[i] * t[i];
+        qq += c[i] * c[i];
+    }
+    const double D = t[0] * c[0] + t[1] * c[1] + t[2] * c[2];
+    double gq[9];
+#pragma unroll
+    for (int x = 0; x < 9; ++x) {
+        double acc = 0.0;
+#pragma unroll
+        for (int z = 0; z < 9; ++z) acc += d2det3(t, x, z) * c[z];
+        gq[x] = 2.0 * acc;
+    }
+    const double a = s * qq, iD = 1.0 / D, w = iD * iD * (1.0 / 9.0);
+    const double w1 = -(2.0 / 9.0) * iD * iD * iD;      // grad w = w1 c
+    const double w2 = (6.0 / 9.0) * iD * iD * iD * iD;  // hess w = w2 c c^T + w1 K
+    double ga[9];
+#pragma unroll
+    for (int x = 0; x < 9; ++x) {
+        ga[x] = qq * 2.0 * t[x] + s * gq[x];
+        double gx = w * ga[x] + a * w1 * c[x];
+        if constexpr (KIND == kShapeSize) gx += 2.0 * (D - 1.0) * c[x];
+        gs[(long long)x * ns + q] = gx;
+    }
+    int e = 0;
+#pragma unroll
+    for (int x = 0; x < 9; ++x)
+#pragma unroll
+        for (int y = x; y < 9; ++y) {
+            double kk = 0.0;
+#pragma unroll
+            for (int z = 0; z < 9; ++z) kk += d2det3(t, x, z) * d2det3(t, z, y);
+            const double Kxy = d2det3(t, x, y);
+            const double Hq = 2.0 * (kk + d2det3(c, x, y));
+            const double Ha = (x == y ? 2.0 * qq : 0.0) + s * Hq + 2.0 * t[x] * gq[y] + 2.0 * gq[x] * t[y];
+            double hv = w * Ha + w1 * (ga[x] * c[y] + c[x] * ga[y]) + a * (w2 * c[x] * c[y] + w1 * Kxy);
+            if constexpr (KIND == kShapeSize) hv += 2.0 * c[x] * c[y] + 2.0 * (D - 1.0) * Kxy;
+            hs[(long long)(e++) * ns + q] = hv;
+        }
+}
+
 // metric gradient and symmetric Hessian of block-local sample p (positions
 // xb of the block), stored component-major at index q of an ns-sample space
 template <int D, int KIND>
@@ -223,6 +290,12 @@ __device__ __forceinline__ void sample_derivs(const CtrlGeom& g, long long p, co
     cell_origin<D>(g, cell, I0);
     double t[V];
     sample_A<D>(g, I0, b, xb, nullptr, 0.0, t);
+    if constexpr (D == 3) {
+        // the closed form: 45 hyper-dual metric evaluations per sample were
+        // 20x slower, compute-bound, where this one streams its 54 outputs
+        mu3_derivs_closed<KIND>(t, gs, hs, ns, q);
+        return;
+    }
     int e = 0;
     for (int i = 0; i < V; ++i)
         for (int r = i; r < V; ++r) {
