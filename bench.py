@@ -282,7 +282,7 @@ def _ttc_fixture(s, Xu, t_untangle, name, budget_s=60.0, fas_chunk=10, plain_chu
     def errors():
         if not with_cpu:
             return {}
-        from oracle import oracle as O  # the checker
+        from oracle import oracle as O  # the oracle (CPU baseline and parity reference)
 
         E, md = O.energy_mindet(dim, "barrier", 0.0, ctx.energy_stencil, s.get_X().astype(np.float64))
         return {"rel_energy_err": abs(float(E) - e_star) / abs(e_star), "energy_fp64": float(E),
@@ -335,7 +335,7 @@ def _error_triple(X, x_star, e_star, es, dim, extent):
     converged state: max-abs position error (against X*, when pinned),
     relative energy error (E of X recomputed in FP64 on the CPU, against
     E*) and min-det error (against min det of X*)."""
-    from oracle import oracle as O  # the checker
+    from oracle import oracle as O  # the oracle (CPU baseline and parity reference)
 
     e, md = O.energy_mindet(dim, "barrier", 0.0, es, X)
     out = {"rel_energy_err": abs(e - e_star) / abs(e_star), "energy_fp64": e, "min_det_fp64": md}
@@ -353,7 +353,7 @@ def _config1(threads, n_sweeps=40, seed=1):
     40 FP64 sweeps at omega 0.8 on the host cores, beside the same 40 sweeps
     on the GPU in FP32, with the FP64 error triple of the GPU result."""
     from egg_amd import grids
-    from oracle import oracle as O  # the checker: CPU baseline and parity reference
+    from oracle import oracle as O  # the oracle: CPU baseline and parity reference
 
     g = grids.single_block_2d(129, seed=seed, amp=0.3)
     ctx, bsc, s = _build(g)
@@ -393,7 +393,7 @@ def _batched_sample(s, g, index, Xu, n_sweeps, target=300_000):
     grid's own sweeps. The GPU runs the whole batch from Xu."""
     from egg_amd import grids
     from egg_amd.smoothing import build_sweep_context
-    from oracle import oracle as O  # the checker: CPU baseline and parity reference
+    from oracle import oracle as O  # the oracle: CPU baseline and parity reference
 
     frees = [int(g.free[n0:n0 + nn].sum()) for (_, _, n0, nn) in index]
     gi = int(np.argmin([abs(f - target) for f in frees]))
@@ -468,7 +468,7 @@ def _cubed_sample(s, g, ctx, Xu, every=40):
     exactly as the full sweep does (frozen neighbours), so one GPU sweep of
     the whole grid is compared on the sample; the energy and min det of the
     GPU state are recomputed in FP64 on the CPU over the whole grid."""
-    from oracle import oracle as O  # the checker: CPU baseline and parity reference
+    from oracle import oracle as O  # the oracle: CPU baseline and parity reference
 
     nd = len(ctx.wire["groups"][0]["dof_idx"])
     sample = np.arange(0, nd, every)
@@ -696,7 +696,7 @@ def bench_ogrid(args):
 
     cpu = parity = config1 = None
     if not args.no_cpu:
-        from oracle import oracle as O  # the checker: CPU baseline and parity reference
+        from oracle import oracle as O  # the oracle: CPU baseline and parity reference
 
         threads = _threads()
         n_cpu = args.cpu_sweeps
@@ -934,7 +934,7 @@ def bench_ogrid_bl(args):
         f"{ach:.0f} GB/s algorithmic")
     cpu = parity = None
     if not args.no_cpu:
-        from oracle import oracle as O  # the checker: CPU baseline and parity reference
+        from oracle import oracle as O  # the oracle: CPU baseline and parity reference
 
         threads = _threads()
         n_cpu = 50
@@ -1030,7 +1030,7 @@ def bench_ogrid_terms(args):
         f"plain -> {value:.4g} node-updates/s")
     cpu = parity = None
     if not args.no_cpu:
-        from oracle import oracle as O  # the checker: CPU baseline and parity reference
+        from oracle import oracle as O  # the oracle: CPU baseline and parity reference
 
         threads = _threads()
         n_cpu = 50
@@ -1201,7 +1201,7 @@ def bench_control(args):
         log(f"control {name}: {runs[name]}")
     cpu = parity = None
     if not args.no_cpu:
-        from oracle import control as OC  # the checker: CPU baseline and parity reference
+        from oracle import control as OC  # the oracle: CPU baseline and parity reference
 
         g = grids.wavy_block_2d(65, perturb=0.01, seed=1)
         ctx, wire, s = _control_case(g, (12, 12))
@@ -1280,7 +1280,7 @@ def bench_control_topo(args):
         log(f"control_topo {name}: {runs[name]}")
     cpu = parity = None
     if not args.no_cpu:
-        from oracle import control as OC  # the checker: CPU baseline and parity reference
+        from oracle import control as OC  # the oracle: CPU baseline and parity reference
         from oracle import oracle as O
 
         g = grids.ogrid_2d(33, 17)

@@ -1,7 +1,7 @@
 /*
  * CPU oracle for the structured TMOP block-Jacobi smoother.
  *
- * TEST INFRASTRUCTURE ONLY. This file is the checker the HIP path is compared
+ * TEST INFRASTRUCTURE ONLY. This file is the reference the HIP path is compared
  * against; nothing in the shipped library links, loads or calls it. Only
  * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it.
  *

@@ -13,7 +13,7 @@ node units (the reference's real offsets divided by dim):
   order; fan mirrors sourced from the owner slot (g2s);
 * the device copy table is halo entries, then fan entries, then shares.
 
-Used by tests only, as the checker of the core's index maps.
+Used by tests only, as the reference the core's index maps are compared with.
 """
 
 from __future__ import annotations
