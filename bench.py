@@ -1346,7 +1346,7 @@ def main() -> int:
         _pmc_child(args)
         return 0
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: this library runs on one GPU; --gpus {args.gpus} is not supported "
               "(the path would shard by blocks, one halo exchange per sweep, but only the one-GPU path is built)",
               file=sys.stderr)
         return 2

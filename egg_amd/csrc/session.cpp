@@ -983,7 +983,7 @@ void Session::create(int d, const eggmi_group* groups, int n_groups, const eggmi
         return;
     }
     if (device == 2)
-        throw NoDevice("select_queue: no device available for device='cpu' (this build runs on the MI355X "
+        throw NoDevice("select_queue: no device available for device='cpu' (this library runs on the MI355X "
                        "GPU only). Use device='gpu' or 'auto'.");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)

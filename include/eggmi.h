@@ -410,7 +410,7 @@ eggmi_status eggmi_session_set_proposal(eggmi_session* s, eggmi_mlp* m, int mode
  * 2 every, ... of each call (the other sweeps run the plain kernel of the
  * session's tile variant); with every > 1, each fine-level smoothing call of
  * eggmi_run_fas proposes in its first sweep only. No reference counterpart:
- * the learned proposal is an addition of this build's. */
+ * the learned proposal is an addition of this library's. */
 eggmi_status eggmi_session_set_proposal_every(eggmi_session* s, int every);
 
 /* Jacobi-preconditioned conjugate gradients in FP64 on a CSR operator

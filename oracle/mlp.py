@@ -1,6 +1,6 @@
 """NumPy oracle of the learned proposal network (TEST INFRASTRUCTURE ONLY).
 
-No reference counterpart: the network is new in this build. Restates, in
+No reference counterpart: the network is new in this library. Restates, in
 plain NumPy, what egg_amd/csrc/mlp.hpp and mlp.hip compute:
 
 * ``bf16``: round-to-nearest-even of float32 to bfloat16 (kept as float32);
