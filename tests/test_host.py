@@ -420,3 +420,33 @@ def test_raw_ctypes_session_tables_only():
     with pytest.raises(ValueError, match="dof_idx"):
         CA.Session(lib, {"groups": [bad], "energy_stencil": c.ctx.energy_stencil}, structured_arrays(c.bsc), c.X, 2,
                    device=3)
+
+
+def test_raw_ctypes_control_topo_arguments():
+    """The multi-block control entry points through plain ctypes on a
+    tables-only session: missing arguments are INVALID_ARGUMENT, calls that
+    need a wire without one say so, and attaching nets needs device state."""
+    import ctypes as C
+
+    from egg_amd.smoothing import structured_arrays
+    from tests import capi_ctypes as CA
+
+    lib = CA.load()
+    c = ogrid(17, 9)
+    s = CA.Session(lib, c.ctx.wire, structured_arrays(c.bsc), c.X, 2, device=3)
+    try:
+        with pytest.raises(ValueError, match="control topology"):
+            s.check(lib.eggmi_session_set_control_topo(s.handle, None))
+        off = np.zeros(2, np.int32)
+        with pytest.raises(ValueError, match="no multi-block control wire"):
+            s.check(lib.eggmi_control_set_reduction(s.handle, C.c_int64(0), off.ctypes.data_as(C.c_void_p), None,
+                                                    None))
+        with pytest.raises(ValueError, match="no multi-block control wire"):
+            s.check(lib.eggmi_control_set_penalty(s.handle, C.c_int64(0), None, None, None, None))
+        with pytest.raises(ValueError, match="no multi-block control wire"):
+            s.check(lib.eggmi_control_set_b(s.handle, 0, None, C.c_int64(0)))
+        topo = (C.c_byte * 256)()  # a zeroed eggmi_control_topo (the struct is smaller)
+        with pytest.raises(ValueError, match="tables only"):
+            s.check(lib.eggmi_session_set_control_topo(s.handle, C.byref(topo)))
+    finally:
+        s.destroy()
